@@ -1,0 +1,87 @@
+"""ctypes mirror of include/icarion_gpu.h (structs and enums).
+
+Field order and types follow the header exactly; tests/test_abi.py checks the
+sizes against offsets computed by the C compiler.
+"""
+import ctypes as C
+
+ABI_VERSION = 1
+
+OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NO_DEVICE = range(6)
+
+INSTRUMENT_IMS, INSTRUMENT_TIMS, INSTRUMENT_QUADRUPOLE, INSTRUMENT_ORBITRAP = range(4)
+INTEGRATOR_RK4, INTEGRATOR_BORIS = range(2)
+COLLISION_NONE, COLLISION_HSS, COLLISION_EHSS, COLLISION_LANGEVIN, COLLISION_FRICTION = range(5)
+PRECISION_FP64, PRECISION_FP32_KAHAN = range(2)
+EHSS_GEOMETRY, EHSS_SURROGATE = range(2)
+
+_d = C.c_double
+_i32 = C.c_int32
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+
+
+class DomainDesc(C.Structure):
+    _fields_ = [
+        ("instrument", _i32), ("use_field_array", _i32), ("magnetic_enabled", _i32),
+        ("reserved0", _i32),
+        ("origin_m", _d * 3), ("length_m", _d), ("radius_m", _d),
+        ("radius_in_m", _d), ("radius_out_m", _d), ("radius_char_m", _d),
+        ("rot_g2l", _d * 9), ("rot_l2g", _d * 9),
+        ("dc_axial_V", _d), ("dc_quad_V", _d), ("dc_radial_V", _d),
+        ("rf_voltage_V", _d), ("rf_frequency_Hz", _d), ("field_array_scale", _d),
+        ("B_T", _d * 3),
+        ("temperature_K", _d), ("pressure_Pa", _d), ("gas_density_m3", _d),
+        ("gas_mass_kg", _d), ("gas_radius_m", _d), ("gas_polarizability_m3", _d),
+        ("gas_velocity_m_s", _d * 3), ("detector_z_m", _d),
+    ]
+
+
+class PhysicsDesc(C.Structure):
+    _fields_ = [
+        ("integrator", _i32), ("collision_model", _i32), ("ou_thermal_kicks", _i32),
+        ("precision", _i32), ("ehss_mode", _i32), ("reserved0", _i32), ("rng_seed", _u64),
+    ]
+
+
+class SpeciesDesc(C.Structure):
+    _fields_ = [
+        ("sigma_hss_m2", _d), ("sigma_ehss_m2", _d), ("ccs_m2", _d), ("K0_cm2_Vs", _d),
+        ("atom_offset", _u32), ("atom_count", _u32),
+    ]
+
+
+class IonSoA(C.Structure):
+    _fields_ = [
+        ("n", C.c_size_t),
+        ("x", C.POINTER(_d)), ("y", C.POINTER(_d)), ("z", C.POINTER(_d)),
+        ("vx", C.POINTER(_d)), ("vy", C.POINTER(_d)), ("vz", C.POINTER(_d)),
+        ("mass_kg", C.POINTER(_d)), ("charge_C", C.POINTER(_d)),
+        ("species", C.POINTER(_u32)), ("active", C.POINTER(C.c_uint8)),
+        ("born", C.POINTER(C.c_uint8)), ("birth_time_s", C.POINTER(_d)),
+        ("death_time_s", C.POINTER(_d)), ("arrival_time_s", C.POINTER(_d)),
+        ("n_collisions", C.POINTER(_u32)),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("ion_steps", _u64), ("collisions", _u64), ("alive", _u64), ("arrived", _u64),
+        ("t_end_s", _d), ("device_ms", _d),
+    ]
+
+
+class GpuDesc(C.Structure):
+    _fields_ = [("device", _i32), ("block_size", _u32), ("steps_per_launch", _u32),
+                ("reserved0", _u32)]
+
+
+class OracleField(C.Structure):
+    _fields_ = [
+        ("nx", _u32), ("ny", _u32), ("nz", _u32),
+        ("xs", C.POINTER(_d)), ("ys", C.POINTER(_d)), ("zs", C.POINTER(_d)),
+        ("Ex", C.POINTER(_d)), ("Ey", C.POINTER(_d)), ("Ez", C.POINTER(_d)),
+    ]
+
+
+IDENTITY9 = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)

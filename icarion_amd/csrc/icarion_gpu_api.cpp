@@ -1,0 +1,626 @@
+// icarion_gpu_api.cpp - C ABI (include/icarion_gpu.h) and host runtime of the
+// gfx950 batch ion-step library.
+//
+// The handle keeps the ion ensemble resident in HBM as SoA arrays and drives
+// the fused step kernels (step_kernels.hip) on one HIP stream.  Host-side work
+// per icarion_gpu_step(): build the per-step table (engine time accumulated as
+// t += dt exactly like the reference's t_i += dt, SimulationEngineLifecycle.cpp:172,
+// and the RF coefficient U_eff(t)/r0^2 of AnalyticalFieldModel.cpp:226-241),
+// upload it, and launch ceil(n_steps / steps_per_launch) kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/icarion_gpu.h"
+#include "step_params.hpp"
+
+namespace icarion {
+hipError_t launch_step(int integrator, int field, int coll, int precision, const StepParams& P,
+                       const IonPtrs& S, uint32_t block, hipStream_t stream);
+}
+
+using namespace icarion;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Failure{code, msg}; }
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) fail(ICARION_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return ICARION_OK;
+    } catch (const Failure& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return ICARION_ERR_INVALID;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return ICARION_ERR_INVALID;
+    }
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void resize(size_t count) {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+constexpr double kPiHost = 3.14159265358979323846;
+constexpr double kBoltzmannHost = 1.380649e-23;
+
+}  // namespace
+
+struct icarion_gpu {
+    int device = 0;
+    uint32_t block = 256;
+    uint32_t steps_per_launch = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    bool have_domain = false;
+    icarion_domain_desc dom{};
+    icarion_physics_desc phys{};
+    StepParams base{};
+
+    bool have_species = false;
+    uint32_t n_species = 0;
+    std::vector<icarion_species_desc> species_host;
+    DevBuf<SpeciesDev> species;
+    DevBuf<double4> atoms;
+    uint32_t n_atoms = 0;
+
+    bool have_field = false;
+    DevBuf<float4> fa32;
+    DevBuf<double4> fa64;
+
+    size_t n = 0;
+    bool have_state = false;
+    DevBuf<double> x, y, z, vx, vy, vz, mass, charge, birth, death, arrival;
+    DevBuf<uint32_t> spec, ncoll;
+    DevBuf<uint8_t> flags;
+    bool has_birth = false;
+
+    DevBuf<StepEntry> table;
+    DevBuf<unsigned long long> counters;
+    std::vector<StepEntry> table_host;
+};
+
+namespace {
+
+void set_device(const icarion_gpu* h) { hip_check(hipSetDevice(h->device), "hipSetDevice"); }
+
+bool is_identity(const double m[9]) {
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int i = 0; i < 9; ++i)
+        if (m[i] != I[i]) return false;
+    return true;
+}
+
+int field_kind(const icarion_domain_desc& d) {
+    if (d.use_field_array) return FIELD_ARRAY;
+    if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) return FIELD_ORBITRAP;
+    return FIELD_LINEAR;
+}
+
+int coll_kind(const icarion_physics_desc& p) {
+    switch (p.collision_model) {
+        case ICARION_COLLISION_HSS: return COLL_HSS;
+        case ICARION_COLLISION_EHSS: return COLL_EHSS;
+        case ICARION_COLLISION_LANGEVIN:
+        case ICARION_COLLISION_FRICTION: return COLL_DAMPING;
+        default: return COLL_NONE;
+    }
+}
+
+void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p) {
+    if (d.instrument < 0 || d.instrument > 3) fail(ICARION_ERR_INVALID, "unknown instrument");
+    if (p.integrator != ICARION_INTEGRATOR_RK4 && p.integrator != ICARION_INTEGRATOR_BORIS)
+        fail(ICARION_ERR_UNSUPPORTED, "integrator not covered by the device path (RK4, Boris)");
+    if (p.collision_model < 0 || p.collision_model > 4) fail(ICARION_ERR_INVALID, "unknown collision model");
+    if (p.precision != ICARION_PRECISION_FP64 && p.precision != ICARION_PRECISION_FP32_KAHAN)
+        fail(ICARION_ERR_INVALID, "unknown precision");
+    if (!(d.length_m > 0.0)) fail(ICARION_ERR_INVALID, "geometry.length_m must be > 0");
+    if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) {
+        if (!(d.radius_in_m > 0.0 && d.radius_out_m > d.radius_in_m && d.radius_char_m > d.radius_out_m))
+            fail(ICARION_ERR_INVALID, "Orbitrap needs 0 < radius_in < radius_out < radius_char");
+    } else if (!(d.radius_m > 0.0)) {
+        fail(ICARION_ERR_INVALID, "geometry.radius_m must be > 0");
+    }
+    if (p.collision_model != ICARION_COLLISION_NONE) {
+        if (!(d.temperature_K > 0.0) || !(d.gas_mass_kg > 0.0))
+            fail(ICARION_ERR_INVALID, "collisions need temperature_K > 0 and gas_mass_kg > 0");
+    }
+}
+
+StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_desc& p) {
+    StepParams P{};
+    P.instrument = d.instrument;
+    P.rot_identity = (is_identity(d.rot_g2l) && is_identity(d.rot_l2g) && d.origin_m[0] == 0.0 &&
+                      d.origin_m[1] == 0.0 && d.origin_m[2] == 0.0) ? 1 : 0;
+    if (d.instrument != ICARION_INSTRUMENT_ORBITRAP) {
+        // cylinder: contains() works in local coordinates; identity rotation
+        // with a non-zero origin still needs the shift
+        P.rot_identity = (is_identity(d.rot_g2l) && d.origin_m[0] == 0.0 && d.origin_m[1] == 0.0 &&
+                          d.origin_m[2] == 0.0) ? 1 : 0;
+    }
+    P.magnetic = d.magnetic_enabled;
+    P.damping_model = p.collision_model;
+    P.ou_kicks = p.ou_thermal_kicks;
+    P.detect = std::isfinite(d.detector_z_m) ? 1 : 0;
+    P.has_rf = (d.instrument != ICARION_INSTRUMENT_ORBITRAP && std::fabs(d.rf_voltage_V) > 1e-12) ? 1 : 0;
+    for (int i = 0; i < 3; ++i) {
+        P.origin[i] = d.origin_m[i];
+        P.B[i] = d.B_T[i];
+        P.u_gas[i] = d.gas_velocity_m_s[i];
+    }
+    P.L = d.length_m;
+    P.R = d.radius_m;
+    P.R_in = d.radius_in_m;
+    P.R_out = d.radius_out_m;
+    P.R_char = d.radius_char_m;
+    P.R_char_sq = d.radius_char_m * d.radius_char_m;
+    if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) {
+        P.ln_R_in = std::log(d.radius_in_m);
+        P.ln_R_out = std::log(d.radius_out_m);
+        const double r_in = d.radius_in_m, r_out = d.radius_out_m;
+        // AnalyticalFieldModel.cpp:201-202
+        P.orbi_k = 2.0 * d.dc_radial_V /
+                   (P.R_char_sq * std::log(r_out / r_in) - 0.5 * (r_out * r_out - r_in * r_in));
+    }
+    std::memcpy(P.g2l, d.rot_g2l, sizeof(P.g2l));
+    std::memcpy(P.l2g, d.rot_l2g, sizeof(P.l2g));
+    if (d.instrument == ICARION_INSTRUMENT_QUADRUPOLE) {
+        P.Ez_const = (std::fabs(d.dc_axial_V) > 1e-12) ? d.dc_axial_V / d.length_m : 0.0;
+    } else {
+        P.Ez_const = d.dc_axial_V / d.length_m;
+    }
+    P.fa_scale = d.field_array_scale;
+    P.T = d.temperature_K;
+    P.n = d.gas_density_m3;
+    P.m_gas = d.gas_mass_kg;
+    P.r_gas = d.gas_radius_m;
+    P.alpha_gas = d.gas_polarizability_m3;
+    if (d.temperature_K > 0.0 && d.gas_mass_kg > 0.0) {
+        P.sigma_neutral = std::sqrt(kBoltzmannHost * d.temperature_K / d.gas_mass_kg);
+        P.v_th = std::sqrt(8.0 * kBoltzmannHost * d.temperature_K / (kPiHost * d.gas_mass_kg));
+    }
+    P.det_z = d.detector_z_m;
+    P.seed = p.rng_seed;
+    return P;
+}
+
+// Per-step table: t_k by repeated addition and c = U_eff(t)/r0^2 at the
+// stage times the integrator evaluates (RK4: t, t + dt/2, t + dt; Boris: t).
+void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
+    const icarion_domain_desc& d = h->dom;
+    h->table_host.resize(n_steps);
+    const bool rf = h->base.has_rf;
+    const double r0_sq = d.radius_m * d.radius_m;
+    const double omega = 2.0 * kPiHost * d.rf_frequency_Hz;
+    const double half_dt = dt * 0.5;
+    auto coeff = [&](double t) {
+        const double U_eff = d.dc_quad_V + d.rf_voltage_V * std::cos(omega * t);
+        return U_eff / r0_sq;
+    };
+    double t = t0;
+    for (uint64_t k = 0; k < n_steps; ++k) {
+        StepEntry& e = h->table_host[k];
+        e.t = t;
+        if (rf) {
+            e.c0 = coeff(t);
+            if (h->phys.integrator == ICARION_INTEGRATOR_RK4) {
+                e.ch = coeff(t + half_dt);
+                e.c1 = coeff(t + dt);
+            } else {
+                e.ch = e.c1 = 0.0;
+            }
+        } else {
+            e.c0 = e.ch = e.c1 = 0.0;
+        }
+        t = t + dt;
+    }
+    h->table.resize(std::max<size_t>(n_steps, 1));
+    hip_check(hipMemcpyAsync(h->table.p, h->table_host.data(), n_steps * sizeof(StepEntry),
+                             hipMemcpyHostToDevice, h->stream),
+              "upload step table");
+}
+
+__global__ void zero_u64(unsigned long long* p, int n) {
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+IonPtrs ion_ptrs(icarion_gpu* h) {
+    IonPtrs S{};
+    S.x = h->x.p; S.y = h->y.p; S.z = h->z.p;
+    S.vx = h->vx.p; S.vy = h->vy.p; S.vz = h->vz.p;
+    S.mass = h->mass.p; S.charge = h->charge.p;
+    S.species = h->spec.p;
+    S.flags = h->flags.p;
+    S.birth = h->has_birth ? h->birth.p : nullptr;
+    S.death = h->death.p;
+    S.arrival = h->arrival.p;
+    S.ncoll = h->ncoll.p;
+    return S;
+}
+
+void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+             icarion_step_stats* stats) {
+    if (!h->have_domain) fail(ICARION_ERR_STATE, "icarion_gpu_set_domain not called");
+    if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
+    if (!h->have_state) fail(ICARION_ERR_STATE, "icarion_gpu_upload_state not called");
+    if (!(dt > 0.0) || !std::isfinite(dt)) fail(ICARION_ERR_INVALID, "dt must be finite and > 0");
+    if (!std::isfinite(t0)) fail(ICARION_ERR_INVALID, "t0 must be finite");
+    const int field = field_kind(h->dom);
+    if (field == FIELD_ARRAY && !h->have_field)
+        fail(ICARION_ERR_STATE, "domain uses a field array but none was uploaded");
+    const int coll = coll_kind(h->phys);
+    if (coll == COLL_EHSS && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
+        fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
+    set_device(h);
+    StepParams P = h->base;
+    P.dt = dt;
+    P.species = h->species.p;
+    P.atoms = h->atoms.p;
+    P.fa_f32 = h->fa32.p;
+    P.fa_f64 = (h->phys.precision == ICARION_PRECISION_FP64) ? h->fa64.p : nullptr;
+    P.n_ions = static_cast<uint32_t>(h->n);
+    P.counters = h->counters.p;
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, h->stream, h->counters.p, 4);
+    hip_check(hipGetLastError(), "zero counters");
+    double t_end = t0;
+    float ms = 0.0f;
+    if (n_steps > 0 && h->n > 0) {
+        build_table(h, t0, dt, n_steps);
+        t_end = h->table_host[n_steps - 1].t + dt;
+        const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
+        const IonPtrs S = ion_ptrs(h);
+        hip_check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+        for (uint64_t k0 = 0; k0 < n_steps; k0 += spl) {
+            const uint64_t nk = std::min<uint64_t>(spl, n_steps - k0);
+            P.table = h->table.p + k0;
+            P.step0 = step_offset + k0;
+            P.n_steps = static_cast<uint32_t>(nk);
+            const hipError_t e = launch_step(h->phys.integrator, field, coll, h->phys.precision, P,
+                                             S, h->block, h->stream);
+            if (e == hipErrorInvalidValue)
+                fail(ICARION_ERR_UNSUPPORTED, "no kernel for this integrator/field/collision/precision");
+            hip_check(e, "step kernel launch");
+        }
+        hip_check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    } else {
+        for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
+    }
+    if (stats) {
+        unsigned long long c[4] = {0, 0, 0, 0};
+        hip_check(hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, h->stream),
+                  "read counters");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        stats->ion_steps = c[0];
+        stats->collisions = c[1];
+        stats->t_end_s = t_end;
+        stats->device_ms = ms;
+        // alive / arrived from the flags and arrival arrays
+        std::vector<uint8_t> fl(h->n);
+        std::vector<double> ar(h->n);
+        if (h->n) {
+            hip_check(hipMemcpy(fl.data(), h->flags.p, h->n, hipMemcpyDeviceToHost), "read flags");
+            hip_check(hipMemcpy(ar.data(), h->arrival.p, h->n * sizeof(double), hipMemcpyDeviceToHost),
+                      "read arrival");
+        }
+        uint64_t alive = 0, arrived = 0;
+        for (size_t i = 0; i < h->n; ++i) {
+            if ((fl[i] & 3) == 3) ++alive;
+            if (!std::isnan(ar[i])) ++arrived;
+        }
+        stats->alive = alive;
+        stats->arrived = arrived;
+    } else {
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    }
+}
+
+template <class T>
+void copy_in(DevBuf<T>& b, const T* src, size_t n, hipStream_t s, const char* what) {
+    b.resize(n);
+    if (n) hip_check(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s), what);
+}
+
+template <class T>
+void copy_out(T* dst, const DevBuf<T>& b, size_t n, hipStream_t s, const char* what) {
+    if (dst && n) hip_check(hipMemcpyAsync(dst, b.p, n * sizeof(T), hipMemcpyDeviceToHost, s), what);
+}
+
+void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
+    if (!ions) fail(ICARION_ERR_INVALID, "ions is NULL");
+    if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species must precede upload_state");
+    const size_t n = ions->n;
+    if (n > 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, "at most 2^32-1 ions per handle");
+    if (n && (!ions->x || !ions->y || !ions->z || !ions->vx || !ions->vy || !ions->vz ||
+              !ions->mass_kg || !ions->charge_C || !ions->species || !ions->active))
+        fail(ICARION_ERR_INVALID, "x, y, z, vx, vy, vz, mass_kg, charge_C, species, active are required");
+    for (size_t i = 0; i < n; ++i) {
+        if (ions->species[i] >= h->n_species)
+            fail(ICARION_ERR_INVALID, "species index " + std::to_string(ions->species[i]) +
+                                          " of ion " + std::to_string(i) + " out of range");
+        if (!(ions->mass_kg[i] > 0.0))
+            fail(ICARION_ERR_INVALID, "mass_kg of ion " + std::to_string(i) + " must be > 0");
+    }
+    set_device(h);
+    hipStream_t s = h->stream;
+    copy_in(h->x, ions->x, n, s, "upload x");
+    copy_in(h->y, ions->y, n, s, "upload y");
+    copy_in(h->z, ions->z, n, s, "upload z");
+    copy_in(h->vx, ions->vx, n, s, "upload vx");
+    copy_in(h->vy, ions->vy, n, s, "upload vy");
+    copy_in(h->vz, ions->vz, n, s, "upload vz");
+    copy_in(h->mass, ions->mass_kg, n, s, "upload mass");
+    copy_in(h->charge, ions->charge_C, n, s, "upload charge");
+    copy_in(h->spec, ions->species, n, s, "upload species");
+    std::vector<uint8_t> fl(n);
+    for (size_t i = 0; i < n; ++i)
+        fl[i] = static_cast<uint8_t>((ions->active[i] ? 1 : 0) | ((ions->born ? ions->born[i] : 1) ? 2 : 0));
+    copy_in(h->flags, fl.data(), n, s, "upload flags");
+    h->has_birth = ions->birth_time_s != nullptr;
+    if (h->has_birth) copy_in(h->birth, ions->birth_time_s, n, s, "upload birth");
+    std::vector<double> tmp(n);
+    if (ions->death_time_s) {
+        copy_in(h->death, ions->death_time_s, n, s, "upload death");
+    } else {
+        std::fill(tmp.begin(), tmp.end(), -1.0);
+        copy_in(h->death, tmp.data(), n, s, "upload death");
+    }
+    if (ions->arrival_time_s) {
+        copy_in(h->arrival, ions->arrival_time_s, n, s, "upload arrival");
+    } else {
+        std::fill(tmp.begin(), tmp.end(), NAN);
+        copy_in(h->arrival, tmp.data(), n, s, "upload arrival");
+    }
+    std::vector<uint32_t> nc(n, 0);
+    copy_in(h->ncoll, ions->n_collisions ? ions->n_collisions : nc.data(), n, s, "upload ncoll");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    h->n = n;
+    h->have_state = true;
+}
+
+void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
+    if (!ions) fail(ICARION_ERR_INVALID, "ions is NULL");
+    if (!h->have_state) fail(ICARION_ERR_STATE, "no resident state");
+    if (ions->n != h->n) fail(ICARION_ERR_INVALID, "ions->n differs from the resident ensemble size");
+    set_device(h);
+    const size_t n = h->n;
+    hipStream_t s = h->stream;
+    copy_out(ions->x, h->x, n, s, "download x");
+    copy_out(ions->y, h->y, n, s, "download y");
+    copy_out(ions->z, h->z, n, s, "download z");
+    copy_out(ions->vx, h->vx, n, s, "download vx");
+    copy_out(ions->vy, h->vy, n, s, "download vy");
+    copy_out(ions->vz, h->vz, n, s, "download vz");
+    copy_out(ions->death_time_s, h->death, n, s, "download death");
+    copy_out(ions->arrival_time_s, h->arrival, n, s, "download arrival");
+    copy_out(ions->n_collisions, h->ncoll, n, s, "download ncoll");
+    std::vector<uint8_t> fl(n);
+    copy_out(fl.data(), h->flags, n, s, "download flags");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    for (size_t i = 0; i < n; ++i) {
+        if (ions->active) ions->active[i] = fl[i] & 1;
+        if (ions->born) ions->born[i] = (fl[i] >> 1) & 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int icarion_gpu_abi_version(void) { return ICARION_GPU_ABI_VERSION; }
+
+int icarion_gpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+size_t icarion_gpu_last_error(char* buf, size_t len) {
+    if (buf && len) {
+        const size_t k = std::min(len - 1, g_last_error.size());
+        std::memcpy(buf, g_last_error.data(), k);
+        buf[k] = '\0';
+    }
+    return g_last_error.size();
+}
+
+int icarion_gpu_create(const icarion_gpu_desc* desc, icarion_gpu** out) {
+    return guarded([&] {
+        if (!out) fail(ICARION_ERR_INVALID, "out is NULL");
+        *out = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            fail(ICARION_ERR_NO_DEVICE, "no HIP device visible");
+        auto h = new icarion_gpu();
+        try {
+            if (desc) {
+                h->device = desc->device;
+                if (desc->block_size) h->block = desc->block_size;
+                h->steps_per_launch = desc->steps_per_launch;
+            }
+            if (h->device < 0 || h->device >= count) fail(ICARION_ERR_INVALID, "device ordinal out of range");
+            if (h->block % 64 != 0 || h->block > 256 || h->block == 0)
+                fail(ICARION_ERR_INVALID, "block_size must be 64, 128, 192 or 256");
+            set_device(h);
+            hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
+            hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
+            h->counters.resize(4);
+        } catch (...) {
+            icarion_gpu_destroy(h);
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int icarion_gpu_destroy(icarion_gpu* h) {
+    if (!h) return ICARION_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    h->species.release(); h->atoms.release(); h->fa32.release(); h->fa64.release();
+    h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
+    h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
+    h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return ICARION_OK;
+}
+
+int icarion_gpu_set_domain(icarion_gpu* h, const icarion_domain_desc* domain,
+                           const icarion_physics_desc* physics) {
+    return guarded([&] {
+        if (!h || !domain || !physics) fail(ICARION_ERR_INVALID, "NULL argument");
+        validate_domain(*domain, *physics);
+        h->dom = *domain;
+        h->phys = *physics;
+        h->base = derive_params(*domain, *physics);
+        h->have_domain = true;
+    });
+}
+
+int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species, uint32_t n_species,
+                            const double* atoms_xyzr, uint32_t n_atoms) {
+    return guarded([&] {
+        if (!h || !species || n_species == 0) fail(ICARION_ERR_INVALID, "species table is empty");
+        if (n_atoms && !atoms_xyzr) fail(ICARION_ERR_INVALID, "atoms_xyzr is NULL");
+        std::vector<SpeciesDev> sd(n_species);
+        for (uint32_t s = 0; s < n_species; ++s) {
+            const icarion_species_desc& d = species[s];
+            if (static_cast<uint64_t>(d.atom_offset) + d.atom_count > n_atoms)
+                fail(ICARION_ERR_INVALID, "species " + std::to_string(s) + " atoms exceed the atom table");
+            sd[s].sigma_hss = d.sigma_hss_m2;
+            sd[s].sigma_ehss = d.sigma_ehss_m2;
+            sd[s].K0_cm2_Vs = d.K0_cm2_Vs;
+            sd[s].atom_offset = d.atom_offset;
+            sd[s].atom_count = d.atom_count;
+        }
+        set_device(h);
+        copy_in(h->species, sd.data(), n_species, h->stream, "upload species");
+        std::vector<double4> at(n_atoms);
+        for (uint32_t a = 0; a < n_atoms; ++a)
+            at[a] = make_double4(atoms_xyzr[4 * a], atoms_xyzr[4 * a + 1], atoms_xyzr[4 * a + 2],
+                                 atoms_xyzr[4 * a + 3]);
+        if (n_atoms) copy_in(h->atoms, at.data(), n_atoms, h->stream, "upload atoms");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        h->species_host.assign(species, species + n_species);
+        h->n_species = n_species;
+        h->n_atoms = n_atoms;
+        h->have_species = true;
+    });
+}
+
+int icarion_gpu_upload_field_array(icarion_gpu* h, uint32_t nx, uint32_t ny, uint32_t nz, double x0,
+                                   double dx, double y0, double dy, double z0, double dz,
+                                   const void* E, int32_t dtype) {
+    return guarded([&] {
+        if (!h || !E) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (nx < 2 || ny < 2 || nz < 2) fail(ICARION_ERR_INVALID, "field array needs >= 2 nodes per axis");
+        if (!(dx > 0.0 && dy > 0.0 && dz > 0.0)) fail(ICARION_ERR_INVALID, "grid spacing must be > 0");
+        if (dtype != 0 && dtype != 1) fail(ICARION_ERR_INVALID, "dtype must be 0 (float) or 1 (double)");
+        const size_t nn = static_cast<size_t>(nx) * ny * nz;
+        set_device(h);
+        std::vector<float4> f(nn);
+        std::vector<double4> g;
+        if (dtype == 0) {
+            const float* e = static_cast<const float*>(E);
+            for (size_t i = 0; i < nn; ++i) f[i] = make_float4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0f);
+            h->fa64.release();
+        } else {
+            const double* e = static_cast<const double*>(E);
+            g.resize(nn);
+            for (size_t i = 0; i < nn; ++i) {
+                g[i] = make_double4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0);
+                f[i] = make_float4((float)e[3 * i], (float)e[3 * i + 1], (float)e[3 * i + 2], 0.0f);
+            }
+            copy_in(h->fa64, g.data(), nn, h->stream, "upload field (double)");
+        }
+        copy_in(h->fa32, f.data(), nn, h->stream, "upload field (float)");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        StepParams& P = h->base;
+        P.nx = static_cast<int>(nx); P.ny = static_cast<int>(ny); P.nz = static_cast<int>(nz);
+        P.x0 = x0; P.y0 = y0; P.z0 = z0;
+        P.dx = dx; P.dy = dy; P.dz = dz;
+        P.inv_dx = 1.0 / dx; P.inv_dy = 1.0 / dy; P.inv_dz = 1.0 / dz;
+        P.x_last = x0 + (nx - 1) * dx;
+        P.y_last = y0 + (ny - 1) * dy;
+        P.z_last = z0 + (nz - 1) * dz;
+        h->have_field = true;
+    });
+}
+
+int icarion_gpu_upload_state(icarion_gpu* h, const icarion_ion_soa* ions) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_upload(h, ions);
+    });
+}
+
+int icarion_gpu_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                     icarion_step_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_step(h, t0, dt, n_steps, step_offset, stats);
+    });
+}
+
+int icarion_gpu_download_state(icarion_gpu* h, icarion_ion_soa* ions) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_download(h, ions);
+    });
+}
+
+int icarion_gpu_step_batch(icarion_gpu* h, icarion_ion_soa* ions, double t, double dt,
+                           uint64_t step_index, icarion_step_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_upload(h, ions);
+        do_step(h, t, dt, 1, step_index, stats);
+        do_download(h, ions);
+    });
+}
+
+}  // extern "C"

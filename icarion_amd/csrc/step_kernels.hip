@@ -1,0 +1,864 @@
+// step_kernels.hip - fused multi-step ion kernels for gfx950.
+//
+// One ion per lane.  A lane loads its ion once (coalesced SoA), runs
+// params.n_steps engine steps with the state in registers, and stores it
+// once; per step it does what the reference engine does to one ion:
+//   prepare  (SimulationEngineLifecycle.cpp:86-107): birth, domain lookup;
+//   push     (RK4Strategy.cpp:181-361 or BorisStrategy.cpp:18-98);
+//   collide  (HSS: HSSCollisionHandler.cpp:257-376, EHSS:
+//             EHSSCollisionHandler.cpp:222-455, OU kick: CollisionKernels.cpp:171-214);
+//   finalize (SimulationEngineLifecycle.cpp:137-192): boundary elimination with
+//             death at the step-start time, NaN kill.
+// Ions never interact, so running these phases back to back per ion gives
+// the reference's per-ion result.  Per-step wave-uniform values (engine time,
+// RF coefficient) come from a table read with scalar loads.  Counters are
+// reduced per wavefront before one atomic per wave.
+#include "device_common.hpp"
+#include "step_params.hpp"
+
+namespace icarion {
+
+// ======================= FP64 (plain method) ===============================
+
+using d3 = vec3<double>;
+
+__device__ __forceinline__ bool contains_f64(const StepParams& P, d3 p) {
+    d3 l = p;
+    if (!P.rot_identity) {
+        l = mat_mul(P.g2l, d3{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]});
+    }
+    if (P.instrument == 3) {
+        // OrbitrapGeometry::contains (OrbitrapGeometry.h:36-51).  The electrode
+        // radius r_e(z) is the root of f(r) = z^2 - (r^2 - R^2)/2 - Rm^2 ln(R/r),
+        // increasing in r below Rm, so r >= r_e(z) <=> f(r) >= 0 (no bisection).
+        const double z_abs = fabs(l.z);
+        if (z_abs > 0.5 * P.L + kGeomEpsilon) return false;
+        const double r2 = l.x * l.x + l.y * l.y;
+        const double r = sqrt(r2);
+        if (r > P.R_out + kGeomEpsilon) return false;
+        const double ln_r = 0.5 * log(r2);
+        const double z2 = z_abs * z_abs;
+        const double f_in = z2 - 0.5 * (r2 - P.R_in * P.R_in) - P.R_char_sq * (P.ln_R_in - ln_r);
+        const double f_out = z2 - 0.5 * (r2 - P.R_out * P.R_out) - P.R_char_sq * (P.ln_R_out - ln_r);
+        return f_in >= 0.0 && f_out <= 0.0;
+    }
+    if (l.z < -kGeomEpsilon || l.z >= P.L + kGeomEpsilon) return false;
+    const double r2 = l.x * l.x + l.y * l.y;
+    const double Re = P.R + kGeomEpsilon;
+    return r2 <= Re * Re;
+}
+
+// trilinear interpolation with clamping (fieldArrayLoader.cpp:103-162) on a
+// uniform grid: the axis search becomes an index computation.
+__device__ __forceinline__ void locate_f64(double v, double v0, double v_last, double inv_d,
+                                           double d, int n, int& i0, double& a) {
+    if (v <= v0) {
+        i0 = 0;
+        a = 0.0;
+    } else if (v >= v_last) {
+        i0 = n - 2;
+        a = 1.0;
+    } else {
+        int i = static_cast<int>((v - v0) * inv_d);
+        i = i > n - 2 ? n - 2 : i;
+        i0 = i;
+        a = (v - (v0 + static_cast<double>(i) * d)) * inv_d;
+    }
+}
+
+__device__ __forceinline__ d3 field_array_f64(const StepParams& P, d3 p) {
+    int ix, iy, iz;
+    double ax, ay, az;
+    locate_f64(p.x, P.x0, P.x_last, P.inv_dx, P.dx, P.nx, ix, ax);
+    locate_f64(p.y, P.y0, P.y_last, P.inv_dy, P.dy, P.ny, iy, ay);
+    locate_f64(p.z, P.z0, P.z_last, P.inv_dz, P.dz, P.nz, iz, az);
+    d3 r{0.0, 0.0, 0.0};
+    const size_t sx = 1, sy = static_cast<size_t>(P.nx), sz = static_cast<size_t>(P.nx) * P.ny;
+    const size_t base = ix + sy * iy + sz * iz;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int dx = c >> 2, dy = (c >> 1) & 1, dzz = c & 1;
+        const double w = (dx ? ax : 1.0 - ax) * (dy ? ay : 1.0 - ay) * (dzz ? az : 1.0 - az);
+        const size_t idx = base + dx * sx + dy * sy + dzz * sz;
+        double ex, ey, ez;
+        if (P.fa_f64) {
+            const double4 e = P.fa_f64[idx];
+            ex = e.x; ey = e.y; ez = e.z;
+        } else {
+            const float4 e = P.fa_f32[idx];
+            ex = e.x; ey = e.y; ez = e.z;
+        }
+        r.x += w * ex;
+        r.y += w * ey;
+        r.z += w * ez;
+    }
+    if (P.fa_scale != 1.0) r = r * P.fa_scale;
+    return r;
+}
+
+template <int FIELD>
+__device__ __forceinline__ d3 field_f64(const StepParams& P, d3 p, double c_rf) {
+    if constexpr (FIELD == FIELD_LINEAR) {
+        // AnalyticalFieldModel.cpp:136-161 (IMS) and :219-248 (quadrupole)
+        d3 E{0.0, 0.0, P.Ez_const};
+        if (P.has_rf) {
+            E.x = -2.0 * p.x * c_rf;
+            E.y = 2.0 * p.y * c_rf;
+        }
+        return E;
+    } else if constexpr (FIELD == FIELD_ORBITRAP) {
+        // AnalyticalFieldModel.cpp:188-217
+        d3 l = p;
+        if (!P.rot_identity)
+            l = mat_mul(P.g2l, d3{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]});
+        const double r_sq = fmax(1e-18, l.x * l.x + l.y * l.y);
+        const double C = 1.0 - P.R_char_sq / r_sq;
+        const double hk = 0.5 * P.orbi_k;
+        d3 E{hk * l.x * C, hk * l.y * C, -P.orbi_k * l.z};
+        if (!P.rot_identity) E = mat_mul(P.l2g, E);
+        return E;
+    } else {
+        return field_array_f64(P, p);
+    }
+}
+
+// DampingForce::calculate_gamma (DampingForce.cpp:105-277) with the per-ion
+// factor hoisted: Langevin gamma = lpre / max(1e-6, |v|); Friction constant.
+__device__ __forceinline__ double gamma_f64(const StepParams& P, double lpre, d3 v) {
+    if (P.damping_model == 3) return lpre / fmax(1e-6, sqrt(dot(v, v)));
+    return lpre;
+}
+
+template <int FIELD, int COLL>
+__device__ __forceinline__ d3 force_f64(const StepParams& P, d3 p, d3 v, double c_rf, double q,
+                                        double m, double lpre) {
+    d3 F = field_f64<FIELD>(P, p, c_rf) * q;
+    if constexpr (COLL == COLL_DAMPING) {
+        const double g = gamma_f64(P, lpre, v);
+        if (g > 0.0) {
+            const d3 rel{v.x - P.u_gas[0], v.y - P.u_gas[1], v.z - P.u_gas[2]};
+            F = F + rel * (-g * m);
+        }
+    }
+    return F;
+}
+
+__device__ __forceinline__ d3 iso_dir_f64(PhiloxStream& rng) {
+    const double u1 = rng.uniform_f64();
+    const double u2 = rng.uniform_f64();
+    const double cosT = 2.0 * u1 - 1.0;
+    const double sinT = sqrt(fmax(0.0, 1.0 - cosT * cosT));
+    const double phi = 2.0 * kPi * u2;
+    double s, c;
+    sincos(phi, &s, &c);
+    return d3{sinT * c, sinT * s, cosT};
+}
+
+__device__ __forceinline__ void ortho_basis_f64(d3 e, d3& t1, d3& t2) {
+    const d3 a = (fabs(e.x) < 0.9) ? d3{1, 0, 0} : d3{0, 1, 0};
+    d3 c1 = cross(e, a);
+    double n1 = sqrt(dot(c1, c1));
+    t1 = n1 > 0.0 ? d3{c1.x / n1, c1.y / n1, c1.z / n1} : d3{0, 0, 0};
+    d3 c2 = cross(e, t1);
+    double n2 = sqrt(dot(c2, c2));
+    t2 = n2 > 0.0 ? d3{c2.x / n2, c2.y / n2, c2.z / n2} : d3{0, 0, 0};
+}
+
+// CollisionKernels::ehss_collision (CollisionKernels.cpp:13-137, 241-381):
+// Shoemake orientation, first-contact ray test, specular reflection.  Rotated
+// atoms are recomputed per attempt instead of stored (same arithmetic).
+__device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn,
+                       uint32_t a0, uint32_t nat, PhiloxStream& rng) {
+    const double u1 = rng.uniform_f64();
+    const double u2 = rng.uniform_f64();
+    const double u3 = rng.uniform_f64();
+    if (nat == 0) return v_ion;
+    const double q1 = sqrt(1.0 - u1) * sin(2.0 * kPi * u2);
+    const double q2 = sqrt(1.0 - u1) * cos(2.0 * kPi * u2);
+    const double q3 = sqrt(u1) * sin(2.0 * kPi * u3);
+    const double q4 = sqrt(u1) * cos(2.0 * kPi * u3);
+    const double x = q1, y = q2, z = q3, w = q4;
+    double O[9];
+    O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
+    O[3] = 2 * (x * y + z * w);     O[4] = 1 - 2 * (x * x + z * z); O[5] = 2 * (y * z - x * w);
+    O[6] = 2 * (x * z - y * w);     O[7] = 2 * (y * z + x * w);     O[8] = 1 - 2 * (x * x + y * y);
+    d3 vrel = v_ion - v_n;
+    double vrel_mag = sqrt(dot(vrel, vrel));
+    if (vrel_mag <= kMinVelocityMag) {
+        vrel = iso_dir_f64(rng) * kMinVelocityMag;
+        vrel_mag = kMinVelocityMag;
+    }
+    const d3 ax{vrel.x / vrel_mag, vrel.y / vrel_mag, vrel.z / vrel_mag};
+    d3 t1, t2;
+    ortho_basis_f64(ax, t1, t2);
+    const double Bm[3][3] = {{t1.x, t2.x, ax.x}, {t1.y, t2.y, ax.y}, {t1.z, t2.z, ax.z}};
+    double Rm[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Rm[3 * i + j] = Bm[i][0] * O[j] + Bm[i][1] * O[3 + j] + Bm[i][2] * O[6 + j];
+    const double4* atoms = P.atoms + a0;
+    double b_max = 0.0;
+    for (uint32_t j = 0; j < nat; ++j) {
+        const double4 c = atoms[j];
+        const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
+        const double sra = dot(ax, ra);
+        const d3 rp = ra - ax * sra;
+        const double cur = sqrt(dot(rp, rp)) + c.w;
+        if (cur > b_max) b_max = cur;
+    }
+    b_max += P.r_gas;
+    for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
+        const double uA = rng.uniform_f64();
+        const double uB = rng.uniform_f64();
+        const double b = b_max * sqrt(uA);
+        double sphi, cphi;
+        sincos(2.0 * kPi * uB, &sphi, &cphi);
+        const d3 off = t1 * (b * cphi) + t2 * (b * sphi);
+        bool hit = false;
+        d3 nb{0, 0, 0};
+        double best_s = 0.0;
+        for (uint32_t j = 0; j < nat; ++j) {
+            const double4 c = atoms[j];
+            const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
+            const double Rsum = c.w + P.r_gas;
+            const double Rsum2 = Rsum * Rsum;
+            const d3 rel = off - ra;
+            const double sstar = dot(ax, rel);
+            const d3 dv = rel - ax * sstar;
+            const double dmin2 = dot(dv, dv);
+            if (dmin2 <= Rsum2) {
+                const double h = sqrt(fmax(0.0, Rsum2 - dmin2));
+                const double s_hit = sstar - h;
+                const d3 ph = rel - ax * s_hit;
+                const double phr2 = dot(ph, ph);
+                d3 nc;
+                if (phr2 <= kMinContactDistSq) {
+                    nc = ax * -1.0;
+                } else {
+                    nc = ph * (1.0 / sqrt(phr2));
+                    if (dot(ax, nc) > 0.0) nc = nc * -1.0;
+                }
+                if (!hit || s_hit < best_s) {
+                    hit = true;
+                    nb = nc;
+                    best_s = s_hit;
+                }
+            }
+        }
+        if (hit) {
+            const double vdn = dot(vrel, nb);
+            const d3 vr = vrel - nb * (2.0 * vdn);
+            const double inv_mt = 1.0 / (mi + mn);
+            const d3 Vcom = (v_ion * mi + v_n * mn) * inv_mt;
+            return Vcom + vr * (mn * inv_mt);
+        }
+        if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
+    }
+    return v_ion;
+}
+
+// One collision stage; returns true on a collision event.
+template <int COLL>
+__device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDev& sp, double m,
+                                            double lpre, d3& v, PhiloxStream& rng) {
+    if constexpr (COLL == COLL_HSS || COLL == COLL_EHSS) {
+        const double sigma = (COLL == COLL_HSS) ? sp.sigma_hss : sp.sigma_ehss;
+        if (sigma <= 0.0) return false;
+        // VelocitySampling::sample_neutral_velocity (VelocitySampling.cpp:9-35)
+        d3 vn;
+        {
+            double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
+            vn.x = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+            vn.y = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+            vn.z = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            vn = vn + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+        }
+        const d3 vrel = v - vn;
+        const double vrel_mag = sqrt(dot(vrel, vrel));
+        if (vrel_mag < kMinRelativeVelocity) return false;
+        const double Pc = 1.0 - exp(-P.n * sigma * vrel_mag * P.dt);
+        if (rng.uniform_f64() >= Pc) return false;
+        if constexpr (COLL == COLL_HSS) {
+            // CollisionKernels::hss_collision (CollisionKernels.cpp:139-169)
+            d3 vr = vrel;
+            double vm = vrel_mag;
+            if (vm <= kMinVelocityMag) {
+                vr = iso_dir_f64(rng) * kMinVelocityMag;
+                vm = kMinVelocityMag;
+            }
+            const d3 nd = iso_dir_f64(rng);
+            const double inv_mt = 1.0 / (m + P.m_gas);
+            const d3 Vcom = (v * m + vn * P.m_gas) * inv_mt;
+            v = Vcom + (nd * vm) * (P.m_gas * inv_mt);
+        } else {
+            v = ehss_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, rng);
+        }
+        return true;
+    } else if constexpr (COLL == COLL_DAMPING) {
+        if (P.ou_kicks) {
+            // ou_velocity_update, apply_damping = false (CollisionKernels.cpp:204-213)
+            const double g = gamma_f64(P, lpre, v);
+            if (g <= 0.0 || P.dt <= 0.0) return false;
+            const double kBT_over_m = kBoltzmann * P.T / m;
+            const double s = sqrt(2.0 * g * kBT_over_m * P.dt);
+            double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
+            double r = sqrt(-2.0 * log(u1));
+            double sn, cs;
+            sincos(2.0 * kPi * u2, &sn, &cs);
+            v.x += s * (r * cs);
+            v.y += s * (r * sn);
+            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+            r = sqrt(-2.0 * log(u1));
+            v.z += s * (r * cos(2.0 * kPi * u2));
+        }
+        return false;
+    } else {
+        return false;
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void add_counters(const StepParams& P, uint32_t pushed, uint32_t coll) {
+    const uint32_t sp = wave_sum_u32(pushed);
+    const uint32_t sc = wave_sum_u32(coll);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&P.counters[0], static_cast<unsigned long long>(sp));
+        atomicAdd(&P.counters[1], static_cast<unsigned long long>(sc));
+    }
+}
+
+template <int INTEG, int FIELD, int COLL>
+__global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const IonPtrs S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_range = i < P.n_ions;
+    uint32_t pushed = 0, ncoll_launch = 0;
+    if (in_range) {
+        d3 pos{S.x[i], S.y[i], S.z[i]};
+        d3 vel{S.vx[i], S.vy[i], S.vz[i]};
+        const double m = S.mass[i];
+        const double q = S.charge[i];
+        const SpeciesDev sp = P.species[S.species[i]];
+        uint8_t flags = S.flags[i];
+        bool active = flags & 1, born = (flags >> 1) & 1;
+        const double birth = S.birth ? S.birth[i] : -INFINITY;
+        double death = S.death[i];
+        double arrival = S.arrival[i];
+        uint32_t nc = S.ncoll[i];
+        const double inv_m = 1.0 / m;
+        // hoisted damping factor (DampingForce.cpp:150-177 / :269-271)
+        double lpre = 0.0;
+        if constexpr (COLL == COLL_DAMPING) {
+            if (P.damping_model == 3) {
+                if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
+                    const double mu = (m * P.m_gas) / (m + P.m_gas);
+                    const double cs1 = kPi * q * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
+                    lpre = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + m);
+                }
+            } else {
+                const double K0 = sp.K0_cm2_Vs;
+                if (K0 > 0.0 && m > 0.0 && P.n > 0.0) {
+                    const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
+                    lpre = q / ((K0t * (kLoschmidt / P.n)) * m);
+                }
+            }
+        }
+        bool checked = false;   // position known to be inside the domain
+        const double dt = P.dt;
+        const double half_dt = dt * 0.5;
+        for (uint32_t k = 0; k < P.n_steps; ++k) {
+            const StepEntry te = P.table[k];
+            const double t = te.t;
+            if (!born) {
+                if (t >= birth) {
+                    born = true;
+                    active = true;
+                    checked = false;
+                } else {
+                    continue;
+                }
+            }
+            if (!active) break;
+            if (!checked && !contains_f64(P, pos)) {
+                active = false;
+                death = t;
+                break;
+            }
+            const double z_before = pos.z;
+            if constexpr (INTEG == 0) {
+                // RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361)
+                const d3 p0 = pos, v0 = vel;
+                const d3 k1v = v0;
+                const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, q, m, lpre) * inv_m;
+                const d3 k2v = v0 + k1a * half_dt;
+                const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, q, m, lpre) * inv_m;
+                const d3 k3v = v0 + k2a * half_dt;
+                const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, q, m, lpre) * inv_m;
+                const d3 k4v = v0 + k3a * dt;
+                const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, q, m, lpre) * inv_m;
+                const double h6 = dt / 6.0;
+                pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
+                vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
+            } else {
+                // BorisStrategy::step (BorisStrategy.cpp:18-98)
+                const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, q, m, lpre) * inv_m;
+                const d3 vm = vel + a * half_dt;
+                d3 vp = vm;
+                if (P.magnetic) {
+                    const double qm = q * inv_m;
+                    const d3 tv = d3{P.B[0], P.B[1], P.B[2]} * (qm * dt * 0.5);
+                    const double s_f = 2.0 / (1.0 + dot(tv, tv));
+                    const d3 sv = tv * s_f;
+                    const d3 vprime = vm + cross(vm, tv);
+                    vp = vm + cross(vprime, sv);
+                }
+                pos = pos + vp * dt;
+                vel = vp + a * half_dt;
+            }
+            ++pushed;
+            if constexpr (COLL != COLL_NONE) {
+                PhiloxStream rng(P.seed, i, P.step0 + k);
+                if (collide_f64<COLL>(P, sp, m, lpre, vel, rng)) {
+                    ++nc;
+                    ++ncoll_launch;
+                }
+            }
+            if (P.detect && isnan(arrival)) {
+                if (z_before < P.det_z && pos.z >= P.det_z)
+                    arrival = t + dt * (P.det_z - z_before) / (pos.z - z_before);
+            }
+            if (!contains_f64(P, pos)) {
+                active = false;
+                death = t;
+                break;
+            }
+            checked = true;
+            if (!(isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z) && isfinite(vel.x) &&
+                  isfinite(vel.y) && isfinite(vel.z))) {
+                active = false;
+                death = t;
+                break;
+            }
+        }
+        S.x[i] = pos.x; S.y[i] = pos.y; S.z[i] = pos.z;
+        S.vx[i] = vel.x; S.vy[i] = vel.y; S.vz[i] = vel.z;
+        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+        S.death[i] = death;
+        S.arrival[i] = arrival;
+        S.ncoll[i] = nc;
+    }
+    add_counters(P, pushed, ncoll_launch);
+}
+
+// ======================= FP32 + Kahan (fast path) ===========================
+
+using f3 = vec3<float>;
+
+// compensated accumulation: (hi, lo) <- hi + lo + d  (Kahan / Fast2Sum)
+__device__ __forceinline__ void kahan_add(float& hi, float& lo, float d) {
+    const float y = d + lo;
+    const float t = hi + y;
+    lo = y - (t - hi);
+    hi = t;
+}
+
+// hardware transcendentals: v_log_f32 (log2), v_exp_f32 (exp2), v_sin/cos_f32
+// (argument in revolutions), v_sqrt_f32, v_rcp_f32
+__device__ __forceinline__ float fast_ln(float u) { return __builtin_amdgcn_logf(u) * 0.69314718056f; }
+__device__ __forceinline__ float fast_cos_rev(float r) { return __builtin_amdgcn_cosf(r); }
+__device__ __forceinline__ float fast_sin_rev(float r) { return __builtin_amdgcn_sinf(r); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+struct FastConst {
+    float Ez, orbi_hk, orbi_k, R_char_sq;
+    float ox, oy, oz;
+    float u_gas[3];
+    float sigma_neutral;
+    float n_dt;          // n * dt
+    float m_gas;
+    float L, half_L, R_eps_sq, R_in_sq, R_out, R_out_sq, ln_R_in, ln_R_out;
+    float g2l[9], l2g[9];
+};
+
+template <int FIELD>
+__device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C, f3 p, float c_rf) {
+    if constexpr (FIELD == FIELD_LINEAR) {
+        f3 E{0.0f, 0.0f, C.Ez};
+        if (P.has_rf) {
+            E.x = -2.0f * p.x * c_rf;
+            E.y = 2.0f * p.y * c_rf;
+        }
+        return E;
+    } else if constexpr (FIELD == FIELD_ORBITRAP) {
+        f3 l = p;
+        if (!P.rot_identity) {
+            const f3 s{p.x - C.ox, p.y - C.oy, p.z - C.oz};
+            l = f3{C.g2l[0] * s.x + C.g2l[1] * s.y + C.g2l[2] * s.z,
+                   C.g2l[3] * s.x + C.g2l[4] * s.y + C.g2l[5] * s.z,
+                   C.g2l[6] * s.x + C.g2l[7] * s.y + C.g2l[8] * s.z};
+        }
+        const float r_sq = fmaxf(1e-18f, l.x * l.x + l.y * l.y);
+        const float Cf = 1.0f - C.R_char_sq * __builtin_amdgcn_rcpf(r_sq);
+        f3 E{C.orbi_hk * l.x * Cf, C.orbi_hk * l.y * Cf, -C.orbi_k * l.z};
+        if (!P.rot_identity) {
+            E = f3{C.l2g[0] * E.x + C.l2g[1] * E.y + C.l2g[2] * E.z,
+                   C.l2g[3] * E.x + C.l2g[4] * E.y + C.l2g[5] * E.z,
+                   C.l2g[6] * E.x + C.l2g[7] * E.y + C.l2g[8] * E.z};
+        }
+        return E;
+    } else {
+        // trilinear on float4 nodes
+        auto loc = [](float v, float v0, float vl, float inv_d, int n, int& i0, float& a) {
+            if (v <= v0) { i0 = 0; a = 0.0f; }
+            else if (v >= vl) { i0 = n - 2; a = 1.0f; }
+            else {
+                const float f = (v - v0) * inv_d;
+                int i = static_cast<int>(f);
+                i = i > n - 2 ? n - 2 : i;
+                i0 = i;
+                a = f - static_cast<float>(i);
+            }
+        };
+        int ix, iy, iz;
+        float ax, ay, az;
+        loc(p.x, (float)P.x0, (float)P.x_last, (float)P.inv_dx, P.nx, ix, ax);
+        loc(p.y, (float)P.y0, (float)P.y_last, (float)P.inv_dy, P.ny, iy, ay);
+        loc(p.z, (float)P.z0, (float)P.z_last, (float)P.inv_dz, P.nz, iz, az);
+        const size_t sy = static_cast<size_t>(P.nx), sz = static_cast<size_t>(P.nx) * P.ny;
+        const size_t base = ix + sy * iy + sz * iz;
+        f3 r{0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int dx = c >> 2, dy = (c >> 1) & 1, dzz = c & 1;
+            const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dzz ? az : 1.0f - az);
+            const float4 e = P.fa_f32[base + dx + dy * sy + dzz * sz];
+            r.x += w * e.x;
+            r.y += w * e.y;
+            r.z += w * e.z;
+        }
+        const float s = (float)P.fa_scale;
+        if (s != 1.0f) r = r * s;
+        return r;
+    }
+}
+
+// Domain containment in FP32 (same predicates as contains_f64; the FP32
+// position is good to ~1e-7 relative, far finer than any domain feature).
+__device__ __forceinline__ bool contains_f32(const StepParams& P, const FastConst& C, float px,
+                                             float py, float pz) {
+    f3 l{px, py, pz};
+    if (!P.rot_identity) {
+        const f3 s{px - C.ox, py - C.oy, pz - C.oz};
+        l = f3{C.g2l[0] * s.x + C.g2l[1] * s.y + C.g2l[2] * s.z,
+               C.g2l[3] * s.x + C.g2l[4] * s.y + C.g2l[5] * s.z,
+               C.g2l[6] * s.x + C.g2l[7] * s.y + C.g2l[8] * s.z};
+    }
+    const float r2 = l.x * l.x + l.y * l.y;
+    if (P.instrument == 3) {
+        const float z_abs = fabsf(l.z);
+        if (z_abs > C.half_L) return false;
+        if (r2 > C.R_out_sq) return false;
+        const float ln_r = 0.5f * fast_ln(r2);
+        const float z2 = z_abs * z_abs;
+        const float f_in = z2 - 0.5f * (r2 - C.R_in_sq) - C.R_char_sq * (C.ln_R_in - ln_r);
+        const float f_out = z2 - 0.5f * (r2 - C.R_out * C.R_out) - C.R_char_sq * (C.ln_R_out - ln_r);
+        return f_in >= 0.0f && f_out <= 0.0f;
+    }
+    if (l.z < 0.0f || l.z >= C.L) return false;
+    return r2 <= C.R_eps_sq;
+}
+
+__device__ __forceinline__ f3 iso_dir_f32(PhiloxStream& rng) {
+    const float u1 = rng.uniform_f32();
+    const float u2 = rng.uniform_f32();
+    const float cosT = 2.0f * u1 - 1.0f;
+    const float sinT = fast_sqrt(fmaxf(0.0f, 1.0f - cosT * cosT));
+    return f3{sinT * fast_cos_rev(u2), sinT * fast_sin_rev(u2), cosT};
+}
+
+template <int COLL>
+__device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst& C,
+                                            const SpeciesDev& sp, float m, float inv_m_plus,
+                                            float lpre, float kbt2_dt_over_m, f3& v,
+                                            PhiloxStream& rng) {
+    if constexpr (COLL == COLL_HSS) {
+        const float sigma = (float)sp.sigma_hss;
+        if (!(sigma > 0.0f)) return false;
+        // three Maxwellian components from two Box-Muller pairs (both branches)
+        const float u1 = rng.uniform_f32(), u2 = rng.uniform_f32();
+        const float u3 = rng.uniform_f32(), u4 = rng.uniform_f32();
+        const float r1 = C.sigma_neutral * fast_sqrt(-2.0f * fast_ln(u1));
+        const float r2 = C.sigma_neutral * fast_sqrt(-2.0f * fast_ln(u3));
+        const f3 vn{r1 * fast_cos_rev(u2) + C.u_gas[0], r1 * fast_sin_rev(u2) + C.u_gas[1],
+                    r2 * fast_cos_rev(u4) + C.u_gas[2]};
+        const f3 vrel = v - vn;
+        const float vm2 = dot(vrel, vrel);
+        const float vm = fast_sqrt(vm2);
+        if (vm < (float)kMinRelativeVelocity) return false;
+        // P = 1 - exp(-x), evaluated as -expm1(-x) to keep small x accurate
+        const float x = C.n_dt * sigma * vm;
+        const float Pc = x < 1e-3f ? x * (1.0f - 0.5f * x * (1.0f - x * (1.0f / 3.0f)))
+                                   : 1.0f - __builtin_amdgcn_exp2f(-x * 1.44269504089f);
+        if (rng.uniform_f32() >= Pc) return false;
+        const f3 nd = iso_dir_f32(rng);
+        const f3 Vcom = (v * m + vn * C.m_gas) * inv_m_plus;
+        v = Vcom + nd * (vm * C.m_gas * inv_m_plus);
+        return true;
+    } else if constexpr (COLL == COLL_DAMPING) {
+        if (P.ou_kicks) {
+            const float g = (P.damping_model == 3) ? lpre / fmaxf(1e-6f, fast_sqrt(dot(v, v))) : lpre;
+            if (!(g > 0.0f)) return false;
+            const float s = fast_sqrt(g * kbt2_dt_over_m);
+            const float u1 = rng.uniform_f32(), u2 = rng.uniform_f32();
+            const float u3 = rng.uniform_f32(), u4 = rng.uniform_f32();
+            const float r1 = s * fast_sqrt(-2.0f * fast_ln(u1));
+            const float r2 = s * fast_sqrt(-2.0f * fast_ln(u3));
+            v.x += r1 * fast_cos_rev(u2);
+            v.y += r1 * fast_sin_rev(u2);
+            v.z += r2 * fast_cos_rev(u4);
+        }
+        return false;
+    } else {
+        return false;
+    }
+}
+
+template <int FIELD, int COLL>
+__device__ __forceinline__ f3 accel_f32(const StepParams& P, const FastConst& C, f3 p, f3 v,
+                                        float c_rf, float qm, float lpre) {
+    f3 a = field_f32<FIELD>(P, C, p, c_rf) * qm;
+    if constexpr (COLL == COLL_DAMPING) {
+        const float g = (P.damping_model == 3) ? lpre / fmaxf(1e-6f, fast_sqrt(dot(v, v))) : lpre;
+        if (g > 0.0f) a = a - f3{v.x - C.u_gas[0], v.y - C.u_gas[1], v.z - C.u_gas[2]} * g;
+    }
+    return a;
+}
+
+template <int INTEG, int FIELD, int COLL>
+__global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const IonPtrs S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in_range = i < P.n_ions;
+    uint32_t pushed = 0, ncoll_launch = 0;
+    if (in_range) {
+        FastConst C;
+        C.Ez = (float)P.Ez_const;
+        C.orbi_k = (float)P.orbi_k;
+        C.orbi_hk = (float)(0.5 * P.orbi_k);
+        C.R_char_sq = (float)P.R_char_sq;
+        C.ox = (float)P.origin[0]; C.oy = (float)P.origin[1]; C.oz = (float)P.origin[2];
+        C.u_gas[0] = (float)P.u_gas[0]; C.u_gas[1] = (float)P.u_gas[1]; C.u_gas[2] = (float)P.u_gas[2];
+        C.sigma_neutral = (float)P.sigma_neutral;
+        C.n_dt = (float)(P.n * P.dt);
+        C.m_gas = (float)P.m_gas;
+        C.L = (float)P.L;
+        C.half_L = (float)(0.5 * P.L);
+        C.R_eps_sq = (float)(P.R * P.R);
+        C.R_in_sq = (float)(P.R_in * P.R_in);
+        C.R_out = (float)P.R_out;
+        C.R_out_sq = (float)(P.R_out * P.R_out);
+        C.ln_R_in = (float)P.ln_R_in;
+        C.ln_R_out = (float)P.ln_R_out;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            C.g2l[c] = (float)P.g2l[c];
+            C.l2g[c] = (float)P.l2g[c];
+        }
+        // state: FP32 (hi, lo) pairs; hi + lo equals the FP64 value to ~2^-48
+        const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
+        float hi[6], lo[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            hi[c] = (float)X[c];
+            lo[c] = (float)(X[c] - (double)hi[c]);
+        }
+        const double md = S.mass[i];
+        const double qd = S.charge[i];
+        const SpeciesDev sp = P.species[S.species[i]];
+        const float m = (float)md;
+        const float qm = (float)(qd / md);
+        const float inv_m_plus = (float)(1.0 / (md + P.m_gas));
+        const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * P.dt);
+        uint8_t flags = S.flags[i];
+        bool active = flags & 1, born = (flags >> 1) & 1;
+        const double birth = S.birth ? S.birth[i] : -INFINITY;
+        double death = S.death[i];
+        double arrival = S.arrival[i];
+        uint32_t nc = S.ncoll[i];
+        // damping factor in units of 1/s (force per mass): Langevin lpre/|v|
+        float lpre = 0.0f;
+        if constexpr (COLL == COLL_DAMPING) {
+            double l = 0.0;
+            if (P.damping_model == 3) {
+                if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
+                    const double mu = (md * P.m_gas) / (md + P.m_gas);
+                    const double cs1 = kPi * qd * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
+                    l = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + md);
+                }
+            } else {
+                const double K0 = sp.K0_cm2_Vs;
+                if (K0 > 0.0 && md > 0.0 && P.n > 0.0) {
+                    const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
+                    l = qd / ((K0t * (kLoschmidt / P.n)) * md);
+                }
+            }
+            lpre = (float)l;
+        }
+        bool checked = false;
+        const float dt = (float)P.dt;
+        const float half_dt = 0.5f * dt;
+        const float h6 = dt * (1.0f / 6.0f);
+        for (uint32_t k = 0; k < P.n_steps; ++k) {
+            const StepEntry te = P.table[k];
+            const double t = te.t;
+            if (!born) {
+                if (t >= birth) {
+                    born = true;
+                    active = true;
+                    checked = false;
+                } else {
+                    continue;
+                }
+            }
+            if (!active) break;
+            if (!checked && !contains_f32(P, C, hi[0], hi[1], hi[2])) {
+                active = false;
+                death = t;
+                break;
+            }
+            const float z_before = hi[2];
+            const float z_before_lo = lo[2];
+            const f3 p0{hi[0], hi[1], hi[2]};
+            const f3 v0{hi[3], hi[4], hi[5]};
+            f3 dpos, dvel;
+            if constexpr (INTEG == 0) {
+                const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
+                const f3 k2v = v0 + k1a * half_dt;
+                const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, (float)te.ch, qm, lpre);
+                const f3 k3v = v0 + k2a * half_dt;
+                const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, (float)te.ch, qm, lpre);
+                const f3 k4v = v0 + k3a * dt;
+                const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, (float)te.c1, qm, lpre);
+                dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
+                dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
+            } else {
+                const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
+                const f3 vm = v0 + a * half_dt;
+                f3 rot{0.0f, 0.0f, 0.0f};
+                if (P.magnetic) {
+                    const f3 tv = f3{(float)P.B[0], (float)P.B[1], (float)P.B[2]} * (qm * half_dt);
+                    const float s_f = 2.0f / (1.0f + dot(tv, tv));
+                    const f3 vprime = vm + cross(vm, tv);
+                    rot = cross(vprime, tv * s_f);
+                }
+                // x += v+ dt, v = v+ + a dt/2 = v0 + a dt + rot (no cancellation)
+                dpos = (vm + rot) * dt;
+                dvel = a * dt + rot;
+            }
+            kahan_add(hi[0], lo[0], dpos.x);
+            kahan_add(hi[1], lo[1], dpos.y);
+            kahan_add(hi[2], lo[2], dpos.z);
+            kahan_add(hi[3], lo[3], dvel.x);
+            kahan_add(hi[4], lo[4], dvel.y);
+            kahan_add(hi[5], lo[5], dvel.z);
+            ++pushed;
+            if constexpr (COLL != COLL_NONE) {
+                PhiloxStream rng(P.seed, i, P.step0 + k);
+                f3 v{hi[3], hi[4], hi[5]};
+                const f3 vin = v;
+                const bool hit = collide_f32<COLL>(P, C, sp, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng);
+                if (COLL == COLL_HSS) {
+                    if (hit) {
+                        // a collision replaces the velocity: restart its compensation
+                        hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
+                        lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
+                        ++nc;
+                        ++ncoll_launch;
+                    }
+                } else {
+                    kahan_add(hi[3], lo[3], v.x - vin.x);
+                    kahan_add(hi[4], lo[4], v.y - vin.y);
+                    kahan_add(hi[5], lo[5], v.z - vin.z);
+                }
+            }
+            if (P.detect && isnan(arrival)) {
+                const double zb = (double)z_before + (double)z_before_lo;
+                const double za = (double)hi[2] + (double)lo[2];
+                if (zb < P.det_z && za >= P.det_z)
+                    arrival = t + P.dt * (P.det_z - zb) / (za - zb);
+            }
+            if (!contains_f32(P, C, hi[0], hi[1], hi[2])) {
+                active = false;
+                death = t;
+                break;
+            }
+            checked = true;
+            if (!(isfinite(hi[0]) && isfinite(hi[1]) && isfinite(hi[2]) && isfinite(hi[3]) &&
+                  isfinite(hi[4]) && isfinite(hi[5]))) {
+                active = false;
+                death = t;
+                break;
+            }
+        }
+        S.x[i] = (double)hi[0] + (double)lo[0];
+        S.y[i] = (double)hi[1] + (double)lo[1];
+        S.z[i] = (double)hi[2] + (double)lo[2];
+        S.vx[i] = (double)hi[3] + (double)lo[3];
+        S.vy[i] = (double)hi[4] + (double)lo[4];
+        S.vz[i] = (double)hi[5] + (double)lo[5];
+        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+        S.death[i] = death;
+        S.arrival[i] = arrival;
+        S.ncoll[i] = nc;
+    }
+    add_counters(P, pushed, ncoll_launch);
+}
+
+// ---- launch dispatch --------------------------------------------------------
+
+using KernelFn = void (*)(const StepParams, const IonPtrs);
+
+template <int I, int F, int Cc>
+KernelFn pick_prec(int precision) {
+    return precision == 1 ? step_f32_kernel<I, F, Cc> : step_f64_kernel<I, F, Cc>;
+}
+
+template <int I, int F>
+KernelFn pick_coll(int coll, int precision) {
+    switch (coll) {
+        case COLL_HSS: return pick_prec<I, F, COLL_HSS>(precision);
+        case COLL_EHSS: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS>;
+        case COLL_DAMPING: return pick_prec<I, F, COLL_DAMPING>(precision);
+        default: return pick_prec<I, F, COLL_NONE>(precision);
+    }
+}
+
+template <int I>
+KernelFn pick_field(int field, int coll, int precision) {
+    switch (field) {
+        case FIELD_ORBITRAP: return pick_coll<I, FIELD_ORBITRAP>(coll, precision);
+        case FIELD_ARRAY: return pick_coll<I, FIELD_ARRAY>(coll, precision);
+        default: return pick_coll<I, FIELD_LINEAR>(coll, precision);
+    }
+}
+
+// Returns hipSuccess, or hipErrorInvalidValue when the combination has no
+// kernel (the host maps that to ICARION_ERR_UNSUPPORTED).
+hipError_t launch_step(int integrator, int field, int coll, int precision, const StepParams& P,
+                       const IonPtrs& S, uint32_t block, hipStream_t stream) {
+    const KernelFn fn = integrator == 1 ? pick_field<1>(field, coll, precision)
+                                        : pick_field<0>(field, coll, precision);
+    if (!fn) return hipErrorInvalidValue;
+    const uint32_t grid = (P.n_ions + block - 1) / block;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, stream, P, S);
+    return hipGetLastError();
+}
+
+}  // namespace icarion

@@ -1,0 +1,108 @@
+// step_params.hpp - parameter block of the fused ion-step kernels, filled on
+// the host from the C-ABI descriptors and passed by value as the kernel
+// argument (so every field is a wave-uniform scalar-register value).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace icarion {
+
+// Field kinds the kernels are specialised for.
+enum FieldKind : int {
+    FIELD_LINEAR = 0,    // IMS / TIMS / quadrupole: (-2x c(t), 2y c(t), Ez)
+    FIELD_ORBITRAP = 1,  // hyperlogarithmic Orbitrap field in local coords
+    FIELD_ARRAY = 2      // trilinear field array
+};
+enum CollKind : int {
+    COLL_NONE = 0,
+    COLL_HSS = 1,
+    COLL_EHSS = 2,
+    COLL_DAMPING = 3     // Langevin or Friction damping force (+ OU kicks)
+};
+
+// Per-species record on the device.
+struct SpeciesDev {
+    double sigma_hss;
+    double sigma_ehss;
+    double K0_cm2_Vs;
+    uint32_t atom_offset;
+    uint32_t atom_count;
+};
+
+// Per-step uniform values: engine time and the RF coefficient
+// c = U_eff(t)/r0^2 at the step start, mid point and end.
+struct StepEntry {
+    double t;
+    double c0;
+    double ch;
+    double c1;
+};
+
+struct IonPtrs {
+    double *x, *y, *z, *vx, *vy, *vz;
+    const double *mass, *charge;
+    const uint32_t *species;
+    uint8_t *flags;          // bit 0 active, bit 1 born
+    const double *birth;     // may be null: born at the first step
+    double *death;
+    double *arrival;
+    uint32_t *ncoll;
+};
+
+struct StepParams {
+    // geometry (DomainConfig.geometry)
+    int instrument;
+    int rot_identity;        // rotation is the identity and origin is (0,0,0)
+    int magnetic;
+    int damping_model;       // 3 Langevin, 4 Friction (collision model code)
+    int ou_kicks;
+    int detect;              // arrival detection on
+    int has_rf;
+    int reserved;
+    double origin[3];
+    double L;
+    double R;
+    double R_in, R_out, R_char;
+    double ln_R_in, ln_R_out;
+    double g2l[9];
+    double l2g[9];
+    // fields
+    double Ez_const;         // IMS: axial_V/L; quadrupole: axial_V/L if set
+    double orbi_k;           // Orbitrap k
+    double R_char_sq;
+    double fa_scale;
+    double B[3];
+    // environment
+    double T;
+    double n;
+    double m_gas;
+    double r_gas;
+    double alpha_gas;
+    double u_gas[3];
+    double sigma_neutral;    // sqrt(kB T / m_gas)
+    double v_th;             // sqrt(8 kB T / (pi m_gas))
+    double det_z;
+    // field array (uniform node-centred grid, x fastest)
+    const float4 *fa_f32;
+    const double4 *fa_f64;
+    int nx, ny, nz, fa_pad;
+    double x0, y0, z0;
+    double dx, dy, dz;
+    double inv_dx, inv_dy, inv_dz;
+    double x_last, y_last, z_last;
+    // species / atoms
+    const SpeciesDev *species;
+    const double4 *atoms;    // x, y, z, r
+    // per-step table, entries [0, n_steps]
+    const StepEntry *table;
+    // run
+    double dt;
+    uint64_t step0;
+    uint64_t seed;
+    uint32_t n_steps;
+    uint32_t n_ions;
+    unsigned long long *counters;   // [0] ion-steps, [1] collisions
+};
+
+}  // namespace icarion

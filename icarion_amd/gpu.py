@@ -1,0 +1,171 @@
+"""ctypes binding of the C ABI (include/icarion_gpu.h) of libicarion_gpu.so.
+
+Loading fails loudly when the library is missing: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libicarion_gpu.so")
+
+_lib = None
+
+EXPORTED = (
+    "icarion_gpu_abi_version", "icarion_gpu_device_count", "icarion_gpu_last_error",
+    "icarion_gpu_create", "icarion_gpu_destroy", "icarion_gpu_set_domain",
+    "icarion_gpu_set_species", "icarion_gpu_upload_field_array", "icarion_gpu_upload_state",
+    "icarion_gpu_step", "icarion_gpu_download_state", "icarion_gpu_step_batch",
+)
+
+
+class IcarionError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"icarion_gpu error {code}: {msg}")
+        self.code = code
+
+
+def load(path=None):
+    """Load libicarion_gpu.so (built by icarion_amd.build); raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(
+            f"{p} not found: build it with `python -m icarion_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(p)
+    P = C.POINTER
+    lib.icarion_gpu_abi_version.restype = C.c_int
+    lib.icarion_gpu_device_count.restype = C.c_int
+    lib.icarion_gpu_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    lib.icarion_gpu_last_error.restype = C.c_size_t
+    lib.icarion_gpu_create.argtypes = [P(_abi.GpuDesc), P(C.c_void_p)]
+    lib.icarion_gpu_destroy.argtypes = [C.c_void_p]
+    lib.icarion_gpu_set_domain.argtypes = [C.c_void_p, P(_abi.DomainDesc), P(_abi.PhysicsDesc)]
+    lib.icarion_gpu_set_species.argtypes = [C.c_void_p, P(_abi.SpeciesDesc), C.c_uint32,
+                                            P(C.c_double), C.c_uint32]
+    lib.icarion_gpu_upload_field_array.argtypes = [
+        C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+        C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32]
+    lib.icarion_gpu_upload_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
+    lib.icarion_gpu_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
+                                     P(_abi.StepStats)]
+    lib.icarion_gpu_download_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
+    lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
+                                           C.c_uint64, P(_abi.StepStats)]
+    for name in EXPORTED:
+        if name not in ("icarion_gpu_abi_version", "icarion_gpu_device_count",
+                        "icarion_gpu_last_error"):
+            getattr(lib, name).restype = C.c_int
+    if lib.icarion_gpu_abi_version() != _abi.ABI_VERSION:
+        raise RuntimeError("libicarion_gpu.so ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def last_error(lib=None):
+    lib = lib or load()
+    buf = C.create_string_buffer(4096)
+    lib.icarion_gpu_last_error(buf, 4096)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc, lib):
+    if rc != _abi.OK:
+        raise IcarionError(rc, last_error(lib))
+
+
+def species_array(species_list):
+    arr = (_abi.SpeciesDesc * len(species_list))()
+    for i, s in enumerate(species_list):
+        arr[i] = s
+    return arr
+
+
+class Handle:
+    """One device-resident ion cloud in one domain (an icarion_gpu handle)."""
+
+    def __init__(self, device=0, block_size=0, steps_per_launch=0):
+        self.lib = load()
+        desc = _abi.GpuDesc(device=device, block_size=block_size,
+                            steps_per_launch=steps_per_launch)
+        h = C.c_void_p()
+        _check(self.lib.icarion_gpu_create(C.byref(desc), C.byref(h)), self.lib)
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.icarion_gpu_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_domain(self, domain, physics):
+        _check(self.lib.icarion_gpu_set_domain(self.h, C.byref(domain), C.byref(physics)),
+               self.lib)
+
+    def set_species(self, species_list, atoms_xyzr=None):
+        arr = species_array(species_list)
+        if atoms_xyzr is None or len(atoms_xyzr) == 0:
+            atoms = C.POINTER(C.c_double)()
+            n_atoms = 0
+        else:
+            a = np.ascontiguousarray(atoms_xyzr, dtype=np.float64).reshape(-1, 4)
+            self._keep.append(a)
+            atoms = a.ctypes.data_as(C.POINTER(C.c_double))
+            n_atoms = a.shape[0]
+        _check(self.lib.icarion_gpu_set_species(self.h, arr, len(species_list), atoms, n_atoms),
+               self.lib)
+
+    def upload_field_array(self, E, origin, spacing):
+        """E: array (nz, ny, nx, 3) or (N, 3) in x-fastest order, float32 or float64."""
+        E = np.ascontiguousarray(E)
+        nz, ny, nx = E.shape[:3]
+        dtype = 0 if E.dtype == np.float32 else 1
+        if dtype == 1:
+            E = E.astype(np.float64, copy=False)
+        rc = self.lib.icarion_gpu_upload_field_array(
+            self.h, nx, ny, nz, origin[0], spacing[0], origin[1], spacing[1], origin[2],
+            spacing[2], E.ctypes.data_as(C.c_void_p), dtype)
+        _check(rc, self.lib)
+
+    def upload_state(self, ens):
+        s = ens.soa()
+        _check(self.lib.icarion_gpu_upload_state(self.h, C.byref(s)), self.lib)
+
+    def step(self, t0, dt, n_steps, step_offset=0):
+        st = _abi.StepStats()
+        _check(self.lib.icarion_gpu_step(self.h, t0, dt, n_steps, step_offset, C.byref(st)),
+               self.lib)
+        return st
+
+    def download_state(self, ens):
+        s = ens.soa()
+        _check(self.lib.icarion_gpu_download_state(self.h, C.byref(s)), self.lib)
+        return ens
+
+    def step_batch(self, ens, t, dt, step_index):
+        s = ens.soa()
+        st = _abi.StepStats()
+        _check(self.lib.icarion_gpu_step_batch(self.h, C.byref(s), t, dt, step_index,
+                                               C.byref(st)), self.lib)
+        return st
+
+
+def device_count():
+    return load().icarion_gpu_device_count()

@@ -1,0 +1,257 @@
+/*
+ * icarion_gpu.h - C ABI of the MI355X (gfx950) batch ion-step library.
+ *
+ * One handle owns device-resident ion state for one ion cloud in one domain
+ * and advances it by whole engine steps:
+ *   prepare (birth, domain lookup, kill) -> push (RK4 or Boris) ->
+ *   collision (HSS / EHSS / OU thermal kick) -> finalize (boundary
+ *   elimination, NaN kill, t += dt).
+ * The reference engine step this replaces is
+ *   src/core/integrator/SimulationEngine.cpp:374-428 (process_timestep),
+ *   src/core/integrator/SimulationEngineLifecycle.cpp:72-195.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (ICARION_OK = 0).  No C++
+ *    exception crosses this boundary; the message of the last failure on the
+ *    calling thread is read with icarion_gpu_last_error().
+ *  - All pointers are plain host pointers unless a name says "device".
+ *  - SI units throughout (m, s, kg, C, V, K, Pa).
+ *  - A handle is not thread-safe: one host thread drives one handle.
+ *  - Nothing here allocates on the device except icarion_gpu_create(),
+ *    icarion_gpu_set_species(), icarion_gpu_upload_field_array() and
+ *    icarion_gpu_upload_state() (which (re)size the resident buffers).
+ */
+#ifndef ICARION_GPU_H
+#define ICARION_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICARION_GPU_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    ICARION_OK = 0,
+    ICARION_ERR_INVALID = 1,      /* bad argument or inconsistent descriptor  */
+    ICARION_ERR_HIP = 2,          /* a HIP runtime call failed                */
+    ICARION_ERR_UNSUPPORTED = 3,  /* configuration the device path does not
+                                     cover; the caller falls back to the CPU
+                                     path (reference step_batch returns false,
+                                     IIntegrationStrategy.h:106-132)          */
+    ICARION_ERR_STATE = 4,        /* call order violated (e.g. step before
+                                     upload)                                   */
+    ICARION_ERR_NO_DEVICE = 5     /* no HIP device visible                     */
+};
+
+/* ---- enumerations (names follow the reference config schema) ----------- */
+/* DomainConfig::instrument (src/core/config/types/InstrumentTypes.h) */
+enum icarion_instrument {
+    ICARION_INSTRUMENT_IMS = 0,
+    ICARION_INSTRUMENT_TIMS = 1,        /* analytic IMS field (AnalyticalFieldModel.cpp:72-73) */
+    ICARION_INSTRUMENT_QUADRUPOLE = 2,  /* "Quadrupole" / QuadrupoleRF */
+    ICARION_INSTRUMENT_ORBITRAP = 3
+};
+
+/* simulation.integrator */
+enum icarion_integrator {
+    ICARION_INTEGRATOR_RK4 = 0,    /* RK4Strategy.cpp:181-361 */
+    ICARION_INTEGRATOR_BORIS = 1   /* BorisStrategy.cpp:18-98 */
+};
+
+/* physics.collision_model */
+enum icarion_collision_model {
+    ICARION_COLLISION_NONE = 0,
+    ICARION_COLLISION_HSS = 1,       /* HSSCollisionHandler.cpp:257-376            */
+    ICARION_COLLISION_EHSS = 2,      /* EHSSCollisionHandler.cpp:131-455           */
+    ICARION_COLLISION_LANGEVIN = 3,  /* DampingForce(Langevin) in the push
+                                        (DampingForce.cpp:150-177) + optional OU
+                                        thermal kicks (CollisionKernels.cpp:171-214) */
+    ICARION_COLLISION_FRICTION = 4   /* DampingForce(Friction) (DampingForce.cpp:179-272) */
+};
+
+/* arithmetic of the push */
+enum icarion_precision {
+    ICARION_PRECISION_FP64 = 0,       /* plain method, FP64 throughout            */
+    ICARION_PRECISION_FP32_KAHAN = 1  /* FP32 state with Kahan-compensated
+                                         position; FP64 time / RF phase           */
+};
+
+/* how an EHSS collision picks the scattering normal */
+enum icarion_ehss_mode {
+    ICARION_EHSS_GEOMETRY = 0,   /* first-contact ray test over the atoms      */
+    ICARION_EHSS_SURROGATE = 1   /* MLP surrogate (icarion_gpu_set_surrogate)  */
+};
+
+/* ---- descriptors -------------------------------------------------------- */
+
+/* One simulation domain (DomainConfig: geometry + fields + env).  Values are
+ * the evaluated constants the reference loader produces
+ * (DomainConfigLoader.cpp:413-440 for E/N -> axial_V,
+ *  EnvironmentConfig.h:150-282 for n, gas mass, radius, polarizability). */
+typedef struct icarion_domain_desc {
+    int32_t instrument;           /* enum icarion_instrument                  */
+    int32_t use_field_array;      /* 1: E from the uploaded field array
+                                     (GridFieldProvider) instead of the
+                                     analytic model                           */
+    int32_t magnetic_enabled;     /* 1: uniform B_T used by Boris             */
+    int32_t reserved0;
+    double origin_m[3];
+    double length_m;
+    double radius_m;              /* cylinder radius / quadrupole r0          */
+    double radius_in_m;           /* Orbitrap inner electrode R1              */
+    double radius_out_m;          /* Orbitrap outer electrode R2              */
+    double radius_char_m;         /* Orbitrap characteristic radius R_m       */
+    double rot_g2l[9];            /* row-major global->local rotation          */
+    double rot_l2g[9];            /* row-major local->global rotation          */
+    /* analytic field inputs (an absent value is 0) */
+    double dc_axial_V;
+    double dc_quad_V;
+    double dc_radial_V;
+    double rf_voltage_V;
+    double rf_frequency_Hz;
+    double field_array_scale;     /* factor on the interpolated field (1.0 for
+                                     a single field_array_terms entry,
+                                     PhysicsSetup.cpp:168-171)                */
+    double B_T[3];
+    /* environment */
+    double temperature_K;
+    double pressure_Pa;
+    double gas_density_m3;        /* P/(kB T)                                 */
+    double gas_mass_kg;
+    double gas_radius_m;
+    double gas_polarizability_m3;
+    double gas_velocity_m_s[3];
+    /* arrival detection: global z of the detector plane; NaN disables */
+    double detector_z_m;
+} icarion_domain_desc;
+
+/* physics switches for one handle */
+typedef struct icarion_physics_desc {
+    int32_t integrator;           /* enum icarion_integrator                  */
+    int32_t collision_model;      /* enum icarion_collision_model             */
+    int32_t ou_thermal_kicks;     /* Langevin/Friction: add OU thermal kicks
+                                     with the ion's own gamma (apply_damping =
+                                     false, CollisionKernels.cpp:204-213)     */
+    int32_t precision;            /* enum icarion_precision                   */
+    int32_t ehss_mode;            /* enum icarion_ehss_mode                   */
+    int32_t reserved0;
+    uint64_t rng_seed;            /* simulation.rng_seed                      */
+} icarion_physics_desc;
+
+/* Per-species constants, resolved on the host for the domain's gas
+ * (species-DB priorities: HSSCollisionHandler.cpp:263-310,
+ *  EHSSCollisionHandler.cpp:462-547). */
+typedef struct icarion_species_desc {
+    double sigma_hss_m2;          /* HSS cross section for the domain gas     */
+    double sigma_ehss_m2;         /* EHSS sigma_eff for the domain gas        */
+    double ccs_m2;                /* ion CCS                                   */
+    double K0_cm2_Vs;             /* reduced mobility (Friction damping)       */
+    uint32_t atom_offset;         /* EHSS geometry: first atom in the table    */
+    uint32_t atom_count;          /* EHSS geometry: number of atoms            */
+} icarion_species_desc;
+
+/* Host SoA view of an ion ensemble (IonEnsemble hot/cold blocks,
+ * src/core/types/IonEnsemble.h:267-318).  Inputs are read by upload, outputs
+ * written by download; an output pointer may be NULL to skip it. */
+typedef struct icarion_ion_soa {
+    size_t n;
+    double *x, *y, *z;            /* position [m]                (in/out)     */
+    double *vx, *vy, *vz;         /* velocity [m/s]              (in/out)     */
+    double *mass_kg;              /* (in)                                      */
+    double *charge_C;             /* (in)                                      */
+    uint32_t *species;            /* index into the species table (in)         */
+    uint8_t *active;              /* (in/out)                                  */
+    uint8_t *born;                /* (in/out)                                  */
+    double *birth_time_s;         /* (in; NULL = born at upload time)          */
+    double *death_time_s;         /* (in/out; -1 while alive)                  */
+    double *arrival_time_s;       /* (out; first detector-plane crossing,
+                                     NaN if none)                              */
+    uint32_t *n_collisions;       /* (out; collision events so far)            */
+} icarion_ion_soa;
+
+/* Counters of one icarion_gpu_step() call. */
+typedef struct icarion_step_stats {
+    uint64_t ion_steps;           /* sum over steps of ions pushed             */
+    uint64_t collisions;          /* collision events                          */
+    uint64_t alive;               /* active and born after the call            */
+    uint64_t arrived;             /* ions with an arrival time after the call  */
+    double t_end_s;               /* engine time after the call                */
+    double device_ms;             /* device time of the stepping kernels       */
+} icarion_step_stats;
+
+/* handle creation options */
+typedef struct icarion_gpu_desc {
+    int32_t device;               /* HIP device ordinal                        */
+    uint32_t block_size;          /* threads per workgroup; 0 = tuned default  */
+    uint32_t steps_per_launch;    /* steps fused per kernel launch; 0 = default */
+    uint32_t reserved0;
+} icarion_gpu_desc;
+
+typedef struct icarion_gpu icarion_gpu;
+
+/* ---- entry points -------------------------------------------------------- */
+
+/* Library ABI version (ICARION_GPU_ABI_VERSION of the build). */
+int icarion_gpu_abi_version(void);
+
+/* Number of HIP devices visible; 0 if none. */
+int icarion_gpu_device_count(void);
+
+/* Message of the last failed call on this thread; returns its full length. */
+size_t icarion_gpu_last_error(char *buf, size_t len);
+
+/* Create / destroy a handle.  Replaces GPUIntegrationHelper / GPUCollisionHelper
+ * construction (src/core/integrator/strategies/GPUIntegrationStrategy.cpp:100-140). */
+int icarion_gpu_create(const icarion_gpu_desc *desc, icarion_gpu **out);
+int icarion_gpu_destroy(icarion_gpu *h);
+
+/* Domain + physics of the handle.  Mirrors what IntegrationStrategyFactory::create
+ * and CollisionHandlerFactory::create receive (IntegrationStrategyFactory.h:27-47,
+ * CollisionHandlerFactory.cpp:10-18). */
+int icarion_gpu_set_domain(icarion_gpu *h, const icarion_domain_desc *domain,
+                           const icarion_physics_desc *physics);
+
+/* Species table and EHSS atom table (x, y, z, r per atom, metres, COM-centred,
+ * geometryUtils.h:35-60).  atoms_xyzr may be NULL when n_atoms is 0. */
+int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
+                            uint32_t n_species, const double *atoms_xyzr,
+                            uint32_t n_atoms);
+
+/* Trilinear E field on a uniform node-centred grid, x fastest
+ * (index i + nx*(j + ny*k), fieldArrayLoader.cpp:9-11).  E holds 3*nx*ny*nz
+ * values interleaved (Ex, Ey, Ez) per node; dtype 0 = float, 1 = double.
+ * Axes: x_i = x0 + i*dx etc.  Replaces load_field_array + GridFieldProvider
+ * (fieldArrayLoader.cpp:13-162, GridFieldProvider.h:34-98). */
+int icarion_gpu_upload_field_array(icarion_gpu *h, uint32_t nx, uint32_t ny, uint32_t nz,
+                                   double x0, double dx, double y0, double dy,
+                                   double z0, double dz, const void *E, int32_t dtype);
+
+/* Copy host SoA state to the device (resident until the next upload). */
+int icarion_gpu_upload_state(icarion_gpu *h, const icarion_ion_soa *ions);
+
+/* Advance the resident ensemble by n_steps engine steps of dt starting at
+ * engine time t0.  step_offset numbers the first step for the per-ion RNG
+ * streams (Philox4x32-10 keyed by (rng_seed, ion index), counter = step).
+ * stats may be NULL.  The device path of IIntegrationStrategy::step_batch +
+ * ICollisionHandler::handle_batch + finalize_ions_after_integration, fused. */
+int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
+                     uint64_t step_offset, icarion_step_stats *stats);
+
+/* Copy the resident state back into host SoA arrays. */
+int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
+
+/* One engine step on a host-owned ensemble: upload, one step, download.
+ * The per-call shape of the reference batch hooks (IIntegrationStrategy.h:106-132,
+ * ICollisionHandler.h:109-134), crossing PCIe both ways. */
+int icarion_gpu_step_batch(icarion_gpu *h, icarion_ion_soa *ions, double t, double dt,
+                           uint64_t step_index, icarion_step_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICARION_GPU_H */

@@ -1,0 +1,159 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py - never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from icarion_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+RNG_PHILOX = 0
+RNG_MT19937_64 = 1
+
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so with the Makefile beside this file (g++)."""
+    args = ["make", "-C", HERE, "liboracle.so"]
+    if force:
+        args = ["make", "-C", HERE, "-B", "liboracle.so"]
+    subprocess.run(args, check=True, capture_output=True)
+    return LIB
+
+
+def build_ref():
+    """Compile the reference-kernel harness into _ref/ (needs /root/reference)."""
+    subprocess.run(["make", "-C", HERE, "ref"], check=True, capture_output=True)
+    return REF_HARNESS
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    P = C.POINTER
+    d = C.c_double
+    lib.oracle_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    lib.oracle_uniform_from_words.argtypes = [C.c_uint32, C.c_uint32]
+    lib.oracle_uniform_from_words.restype = d
+    lib.oracle_philox_uniforms.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, P(d)]
+    lib.oracle_field_E.argtypes = [P(_abi.DomainDesc), P(_abi.OracleField), P(d), d, P(d)]
+    lib.oracle_contains.argtypes = [P(_abi.DomainDesc), P(d)]
+    lib.oracle_contains.restype = C.c_int
+    lib.oracle_orbitrap_r_for_z.argtypes = [d, d, d]
+    lib.oracle_orbitrap_r_for_z.restype = d
+    lib.oracle_interpolate_field.argtypes = [P(_abi.OracleField), P(d), P(d)]
+    lib.oracle_damping_gamma.argtypes = [P(_abi.DomainDesc), C.c_int, P(_abi.SpeciesDesc), d, d,
+                                         P(d)]
+    lib.oracle_damping_gamma.restype = d
+    lib.oracle_mt_sample_neutral_velocity.argtypes = [C.c_uint64, d, d, P(d), P(d)]
+    lib.oracle_mt_hss_collision.argtypes = [C.c_uint64, P(d), P(d), d, d, P(d)]
+    lib.oracle_mt_ehss_collision.argtypes = [C.c_uint64, P(d), P(d), d, d, d, P(d), C.c_uint32,
+                                             P(d)]
+    lib.oracle_oapa_sigma.argtypes = [P(d), C.c_uint32, d]
+    lib.oracle_oapa_sigma.restype = d
+    lib.oracle_push_one.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), P(_abi.SpeciesDesc),
+                                    P(_abi.OracleField), d, d, P(d), P(d), d, d]
+    lib.oracle_run.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), P(_abi.SpeciesDesc),
+                               C.c_uint32, P(d), C.c_uint32, P(_abi.OracleField), P(_abi.IonSoA),
+                               d, d, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                               P(_abi.StepStats)]
+    lib.oracle_run.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def vec(*xs):
+    return (C.c_double * len(xs))(*xs)
+
+
+class Field:
+    """oracle_field built from a node-centred uniform grid E[nz, ny, nx, 3]."""
+
+    def __init__(self, E, origin, spacing):
+        E = np.asarray(E)
+        nz, ny, nx = E.shape[:3]
+        self.xs = origin[0] + spacing[0] * np.arange(nx, dtype=np.float64)
+        self.ys = origin[1] + spacing[1] * np.arange(ny, dtype=np.float64)
+        self.zs = origin[2] + spacing[2] * np.arange(nz, dtype=np.float64)
+        self.Ex = np.ascontiguousarray(E[..., 0], dtype=np.float64).ravel()
+        self.Ey = np.ascontiguousarray(E[..., 1], dtype=np.float64).ravel()
+        self.Ez = np.ascontiguousarray(E[..., 2], dtype=np.float64).ravel()
+        self.c = _abi.OracleField(nx, ny, nz, _dp(self.xs), _dp(self.ys), _dp(self.zs),
+                                  _dp(self.Ex), _dp(self.Ey), _dp(self.Ez))
+
+
+def species_array(species):
+    arr = (_abi.SpeciesDesc * len(species))()
+    for i, s in enumerate(species):
+        arr[i] = s
+    return arr
+
+
+def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=None, field=None,
+        rng_kind=RNG_PHILOX, threads=0):
+    """Run the oracle engine loop on ens in place; returns StepStats."""
+    lib = load()
+    sp = species_array(species)
+    if atoms is not None and len(atoms):
+        a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
+        ap, na = _dp(a), a.shape[0]
+    else:
+        a, ap, na = None, C.POINTER(C.c_double)(), 0
+    fptr = C.byref(field.c) if field is not None else C.POINTER(_abi.OracleField)()
+    soa = ens.soa()
+    st = _abi.StepStats()
+    rc = lib.oracle_run(C.byref(domain), C.byref(physics), sp, len(species), ap, na, fptr,
+                        C.byref(soa), t0, dt, n_steps, step_offset, rng_kind, threads,
+                        C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run failed ({rc})")
+    return st
+
+
+def field_E(domain, pos, t, field=None):
+    lib = load()
+    out = (C.c_double * 3)()
+    fptr = C.byref(field.c) if field is not None else C.POINTER(_abi.OracleField)()
+    lib.oracle_field_E(C.byref(domain), fptr, vec(*pos), t, out)
+    return np.array(out[:])
+
+
+def philox(ctr, key):
+    lib = load()
+    out = (C.c_uint32 * 4)()
+    lib.oracle_philox4x32_10((C.c_uint32 * 4)(*ctr), (C.c_uint32 * 2)(*key), out)
+    return list(out)
+
+
+def philox_uniforms(seed, ion, step, n):
+    lib = load()
+    out = (C.c_double * n)()
+    lib.oracle_philox_uniforms(seed, ion, step, n, out)
+    return np.array(out[:])
+
+
+def push_one(domain, physics, sp, mass, charge, pos, vel, t, dt, field=None):
+    lib = load()
+    p = vec(*pos)
+    v = vec(*vel)
+    fptr = C.byref(field.c) if field is not None else C.POINTER(_abi.OracleField)()
+    lib.oracle_push_one(C.byref(domain), C.byref(physics), C.byref(sp), fptr, mass, charge, p, v,
+                        t, dt)
+    return np.array(p[:]), np.array(v[:])

@@ -1,0 +1,928 @@
+// icarion_oracle.cpp - CPU FP64 restatement of the reference batch ion-step.
+//
+// TEST INFRASTRUCTURE ONLY: the parity checker and the host-CPU baseline.
+// It is never linked into, loaded by, or called from the product library.
+//
+// Each function below restates the reference function cited next to it,
+// keeping its operation order so that, driven by the same random numbers,
+// results agree to rounding.  Two random sources:
+//   * Philox4x32-10 streams keyed by (rng_seed, ion) with the step number in
+//     the counter - the streams the device kernels draw, so GPU and oracle
+//     trajectories can be compared ion by ion;
+//   * std::mt19937_64 + uniform_real_distribution / normal_distribution
+//     seeded rng_seed + i - the reference's PhysicsRng (CollisionTypes.cpp:8-20,
+//     RngUtils.cpp:51-53), so the oracle can be pinned bit for bit against the
+//     reference's own compiled kernels.
+//
+// Parity status of each part is listed in the file header of
+// tests/test_oracle_pins.py (what pins it and how).
+#include "icarion_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---- constants (src/utils/constants.h:13-65) ------------------------------
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kBoltzmann = 1.380649e-23;
+constexpr double kLoschmidt = 2.6867811e25;
+constexpr double kEpsilon0 = 8.854187817e-12;
+constexpr double kStpTemp = 273.15;
+// HSSCollisionHandler.cpp:20, EHSSCollisionHandler.cpp:20
+constexpr double kMinRelativeVelocity = 1e-10;
+// CollisionKernels.h:172-178
+constexpr double kMinVelocityMag = 1e-12;
+constexpr double kMinContactDistSq = 1e-24;
+constexpr int kEhssMaxAttempts = 256;
+constexpr int kBmaxExpansionAttempt = 64;
+constexpr double kBmaxExpansionFactor = 1.5;
+// CylindricalGeometry.h:245, OrbitrapGeometry.h:231
+constexpr double kGeomEpsilon = 1e-12;
+// AnalyticalFieldModel.cpp:12
+constexpr double kMinVoltage = 1e-12;
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalize(V3 a) {  // mathUtils.h:32-35
+    double n = norm(a);
+    return n > 0.0 ? a / n : V3{0, 0, 0};
+}
+inline V3 mat_mul(const double m[9], V3 v) {  // mathUtils.h:88-94 (row-major)
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z,
+            m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z};
+}
+inline bool finite3(V3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11, "Parallel random
+// numbers: as easy as 1, 2, 3"); round multipliers and Weyl key increments
+// are the published constants of the algorithm.
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u;
+constexpr uint32_t kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u;
+constexpr uint32_t kPhiloxW1 = 0xBB67AE85u;
+
+inline void philox(const uint32_t c_in[4], const uint32_t k_in[2], uint32_t out[4]) {
+    uint32_t c0 = c_in[0], c1 = c_in[1], c2 = c_in[2], c3 = c_in[3];
+    uint32_t k0 = k_in[0], k1 = k_in[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = static_cast<uint64_t>(kPhiloxM0) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(kPhiloxM1) * c2;
+        const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+        const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n1 = lo1;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        const uint32_t n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += kPhiloxW0;
+        k1 += kPhiloxW1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// 53-bit uniform on (0, 1]: k * 2^-53 + 2^-54 with k from the top 27 bits of
+// hi and the top 26 bits of lo.  Never 0, so log(u) is always finite.
+inline double uniform_from_words(uint32_t hi, uint32_t lo) {
+    const uint64_t k = (static_cast<uint64_t>(hi >> 5) << 26) | static_cast<uint64_t>(lo >> 6);
+    return static_cast<double>(k) * 0x1.0p-53 + 0x1.0p-54;
+}
+
+// Per-(ion, step) Philox stream: counter = (block, ion, step_lo, step_hi),
+// key = (seed_lo, seed_hi).  uniform01() consumes two words; normal() is
+// Box-Muller and caches the sine branch.
+class PhiloxStream {
+public:
+    PhiloxStream(uint64_t seed, uint32_t ion, uint64_t step) {
+        key_[0] = static_cast<uint32_t>(seed);
+        key_[1] = static_cast<uint32_t>(seed >> 32);
+        ctr_[0] = 0;
+        ctr_[1] = ion;
+        ctr_[2] = static_cast<uint32_t>(step);
+        ctr_[3] = static_cast<uint32_t>(step >> 32);
+        pos_ = 4;
+    }
+    double uniform01() {
+        if (pos_ >= 4) {
+            philox(ctr_, key_, buf_);
+            ctr_[0] += 1;
+            pos_ = 0;
+        }
+        const double u = uniform_from_words(buf_[pos_], buf_[pos_ + 1]);
+        pos_ += 2;
+        return u;
+    }
+    double normal() {
+        if (has_cached_) {
+            has_cached_ = false;
+            return cached_;
+        }
+        const double u1 = uniform01();
+        const double u2 = uniform01();
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        const double a = 2.0 * kPi * u2;
+        cached_ = r * std::sin(a);
+        has_cached_ = true;
+        return r * std::cos(a);
+    }
+
+private:
+    uint32_t key_[2];
+    uint32_t ctr_[4];
+    uint32_t buf_[4];
+    int pos_;
+    bool has_cached_ = false;
+    double cached_ = 0.0;
+};
+
+// PhysicsRng (CollisionTypes.cpp:8-20): mt19937_64 + uniform_real_distribution
+// + normal_distribution.
+class MtRng {
+public:
+    explicit MtRng(uint64_t seed) : eng_(seed), uni_(0.0, 1.0), norm_(0.0, 1.0) {}
+    double uniform01() { return uni_(eng_); }
+    double normal() { return norm_(eng_); }
+
+private:
+    std::mt19937_64 eng_;
+    std::uniform_real_distribution<double> uni_;
+    std::normal_distribution<double> norm_;
+};
+
+// ---- fields -------------------------------------------------------------
+
+// AnalyticalFieldModel::compute_ims_field (AnalyticalFieldModel.cpp:136-161).
+// Absent values evaluate to 0 (the has_value() guards).
+V3 ims_field(const icarion_domain_desc& d, V3 pos, double t) {
+    V3 E{0.0, 0.0, 0.0};
+    E.z = d.dc_axial_V / d.length_m;
+    if (std::fabs(d.rf_voltage_V) > kMinVoltage) {
+        const double r0_sq = d.radius_m * d.radius_m;
+        const double omega = 2.0 * kPi * d.rf_frequency_Hz;
+        const double U_eff = d.dc_quad_V + d.rf_voltage_V * std::cos(omega * t);
+        E.x = -2.0 * pos.x * U_eff / r0_sq;
+        E.y = 2.0 * pos.y * U_eff / r0_sq;
+    }
+    return E;
+}
+
+// AnalyticalFieldModel::compute_quadrupole_rf_field (AnalyticalFieldModel.cpp:219-248).
+// Global coordinates; rf.phase_rad is not used by the reference.
+V3 quadrupole_field(const icarion_domain_desc& d, V3 pos, double t) {
+    V3 E{0.0, 0.0, 0.0};
+    if (std::fabs(d.rf_voltage_V) > kMinVoltage) {
+        const double r0_sq = d.radius_m * d.radius_m;
+        const double omega = 2.0 * kPi * d.rf_frequency_Hz;
+        const double U_eff = d.dc_quad_V + d.rf_voltage_V * std::cos(omega * t);
+        E.x = -2.0 * pos.x * U_eff / r0_sq;
+        E.y = 2.0 * pos.y * U_eff / r0_sq;
+    }
+    if (std::fabs(d.dc_axial_V) > kMinVoltage) {
+        E.z = d.dc_axial_V / d.length_m;
+    }
+    return E;
+}
+
+// AnalyticalFieldModel::compute_orbitrap_field (AnalyticalFieldModel.cpp:188-217).
+V3 orbitrap_field(const icarion_domain_desc& d, V3 gpos) {
+    const V3 p = mat_mul(d.rot_g2l, gpos - V3{d.origin_m[0], d.origin_m[1], d.origin_m[2]});
+    const double r_sq = std::max(1e-18, p.x * p.x + p.y * p.y);
+    const double r_char = d.radius_char_m;
+    const double r_char_sq = r_char * r_char;
+    const double C = 1.0 - r_char_sq / r_sq;
+    const double voltage = d.dc_radial_V;
+    const double r_in = d.radius_in_m;
+    const double r_out = d.radius_out_m;
+    const double k = 2.0 * voltage / (r_char_sq * std::log(r_out / r_in) -
+                                      0.5 * (r_out * r_out - r_in * r_in));
+    const V3 E_local{0.5 * k * p.x * C, 0.5 * k * p.y * C, -k * p.z};
+    return mat_mul(d.rot_l2g, E_local);
+}
+
+// interpolate_field (fieldArrayLoader.cpp:103-162): linear axis search with
+// clamping outside the grid, x-fastest index i + nx*(j + ny*k).
+void locate(const double* arr, uint32_t n, double v, size_t& i0, size_t& i1, double& alpha) {
+    if (v <= arr[0]) {
+        i0 = 0; i1 = 1; alpha = 0.0;
+    } else if (v >= arr[n - 1]) {
+        i0 = n - 2; i1 = n - 1; alpha = 1.0;
+    } else {
+        for (size_t i = 0; i < n - 1; ++i) {
+            if (v >= arr[i] && v <= arr[i + 1]) {
+                i0 = i;
+                i1 = i + 1;
+                const double dv = arr[i + 1] - arr[i];
+                alpha = (dv > 0.0) ? (v - arr[i]) / dv : 0.0;
+                return;
+            }
+        }
+        i0 = n - 2; i1 = n - 1; alpha = 1.0;
+    }
+}
+
+V3 interpolate(const oracle_field& f, V3 pos) {
+    size_t ix0, ix1, iy0, iy1, iz0, iz1;
+    double ax, ay, az;
+    locate(f.xs, f.nx, pos.x, ix0, ix1, ax);
+    locate(f.ys, f.ny, pos.y, iy0, iy1, ay);
+    locate(f.zs, f.nz, pos.z, iz0, iz1, az);
+    V3 r{0.0, 0.0, 0.0};
+    for (size_t dx = 0; dx <= 1; ++dx) {
+        const double wx = (dx == 0 ? (1.0 - ax) : ax);
+        const size_t ix = (dx == 0 ? ix0 : ix1);
+        for (size_t dy = 0; dy <= 1; ++dy) {
+            const double wy = (dy == 0 ? (1.0 - ay) : ay);
+            const size_t iy = (dy == 0 ? iy0 : iy1);
+            for (size_t dz = 0; dz <= 1; ++dz) {
+                const double wz = (dz == 0 ? (1.0 - az) : az);
+                const size_t iz = (dz == 0 ? iz0 : iz1);
+                const double w = wx * wy * wz;
+                const size_t idx = ix + f.nx * (iy + f.ny * iz);
+                r.x += w * f.Ex[idx];
+                r.y += w * f.Ey[idx];
+                r.z += w * f.Ez[idx];
+            }
+        }
+    }
+    return r;
+}
+
+// ElectricFieldForce::compute_soa -> IFieldModel::E (ElectricFieldForce.cpp:66-83);
+// a field array replaces the analytic model (PhysicsSetup.cpp:107-195).
+V3 field_E(const icarion_domain_desc& d, const oracle_field* f, V3 pos, double t) {
+    if (d.use_field_array && f) {
+        V3 E = interpolate(*f, pos);
+        if (d.field_array_scale != 1.0) E = E * d.field_array_scale;
+        return E;
+    }
+    switch (d.instrument) {
+        case ICARION_INSTRUMENT_IMS:
+        case ICARION_INSTRUMENT_TIMS:
+            return ims_field(d, pos, t);
+        case ICARION_INSTRUMENT_QUADRUPOLE:
+            return quadrupole_field(d, pos, t);
+        case ICARION_INSTRUMENT_ORBITRAP:
+            return orbitrap_field(d, pos);
+        default:
+            return V3{0.0, 0.0, 0.0};
+    }
+}
+
+// ---- geometry -------------------------------------------------------------
+
+// OrbitrapGeometry::orbitrap_surface_residual (OrbitrapGeometry.h:173-178)
+double orbitrap_residual(double r, double z, double R, double R_m) {
+    const double term1 = z * z;
+    const double term2 = 0.5 * (r * r - R * R);
+    const double term3 = R_m * R_m * std::log(R / r);
+    return term1 - term2 - term3;
+}
+
+// OrbitrapGeometry::orbitrap_r_for_z (OrbitrapGeometry.h:181-219)
+double orbitrap_r_for_z(double z, double R, double R_m) {
+    const double z_abs = std::fabs(z);
+    const double eps = 1e-10;
+    const int max_iter = 80;
+    if (z_abs < eps) return R;
+    double r_lo = 0.1 * R;
+    double r_hi = R;
+    double f_lo = orbitrap_residual(r_lo, z_abs, R, R_m);
+    double f_hi = orbitrap_residual(r_hi, z_abs, R, R_m);
+    int expand_iter = 0;
+    while (f_lo * f_hi > 0.0 && expand_iter < 10) {
+        r_lo *= 0.5;
+        r_hi *= 1.5;
+        f_lo = orbitrap_residual(r_lo, z_abs, R, R_m);
+        f_hi = orbitrap_residual(r_hi, z_abs, R, R_m);
+        expand_iter++;
+    }
+    double r_mid = R;
+    for (int i = 0; i < max_iter; ++i) {
+        r_mid = 0.5 * (r_lo + r_hi);
+        const double f_mid = orbitrap_residual(r_mid, z_abs, R, R_m);
+        if (f_lo * f_mid <= 0.0) {
+            r_hi = r_mid;
+            f_hi = f_mid;
+        } else {
+            r_lo = r_mid;
+            f_lo = f_mid;
+        }
+        if (std::fabs(f_mid) < eps) break;
+    }
+    return r_mid;
+}
+
+// CylindricalGeometry::contains (CylindricalGeometry.h:35-40) and
+// OrbitrapGeometry::contains (OrbitrapGeometry.h:36-51); single domain, so
+// DomainManager::find_domain_index (DomainManager.cpp:66-109) reduces to it.
+bool contains(const icarion_domain_desc& d, V3 gpos) {
+    const V3 local = mat_mul(d.rot_g2l, gpos - V3{d.origin_m[0], d.origin_m[1], d.origin_m[2]});
+    if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) {
+        const double z_abs = std::fabs(local.z);
+        const double z_max = 0.5 * d.length_m;
+        if (z_abs > z_max + kGeomEpsilon) return false;
+        const double r2 = local.x * local.x + local.y * local.y;
+        const double r = std::sqrt(r2);
+        const double r_in_allowed = orbitrap_r_for_z(z_abs, d.radius_in_m, d.radius_char_m);
+        const double r_out_allowed = orbitrap_r_for_z(z_abs, d.radius_out_m, d.radius_char_m);
+        if (!(r_in_allowed > 0.0 && r_out_allowed > r_in_allowed)) return false;
+        return (r >= r_in_allowed - kGeomEpsilon) && (r <= r_out_allowed + kGeomEpsilon);
+    }
+    if (local.z < -kGeomEpsilon || local.z >= d.length_m + kGeomEpsilon) return false;
+    const double r2 = local.x * local.x + local.y * local.y;
+    return r2 <= (d.radius_m + kGeomEpsilon) * (d.radius_m + kGeomEpsilon);
+}
+
+// ---- damping (DampingForce.cpp:105-277) -----------------------------------
+double damping_gamma(const icarion_domain_desc& d, int model, const icarion_species_desc& sp,
+                     double m_ion, double q, V3 vel) {
+    const double gas_density = d.gas_density_m3;
+    const double temperature_K = d.temperature_K;
+    const double m_neutral = d.gas_mass_kg;
+    const double v_th = (temperature_K > 0.0 && m_neutral > 0.0)
+        ? std::sqrt(8.0 * kBoltzmann * temperature_K / (kPi * m_neutral))
+        : 0.0;
+    if (model == ICARION_COLLISION_LANGEVIN) {
+        // DampingForce.cpp:150-177
+        const double alpha = d.gas_polarizability_m3;
+        if (alpha <= 0.0 || gas_density <= 0.0 || v_th <= 0.0 || m_neutral <= 0.0) return 0.0;
+        const double v_mag = std::max(1e-6, norm(vel));
+        const double m_reduced = (m_ion * m_neutral) / (m_ion + m_neutral);
+        const double epsilon_factor = 4.0 * kPi * kEpsilon0;
+        const double cs = kPi * q * std::sqrt(alpha / (epsilon_factor * m_reduced)) / v_mag;
+        return gas_density * cs * v_th * m_neutral / (m_neutral + m_ion);
+    }
+    if (model == ICARION_COLLISION_FRICTION) {
+        // DampingForce.cpp:179-272, single-gas branch
+        const double K0_cm2_Vs = sp.K0_cm2_Vs;
+        if (K0_cm2_Vs <= 0.0 || m_ion <= 0.0 || gas_density <= 0.0) return 0.0;
+        const double K0_m2_Vs = K0_cm2_Vs * 1e-4;
+        const double N_ratio = kLoschmidt / gas_density;
+        const double teff_scale = std::sqrt(kStpTemp / std::max(temperature_K, 1.0));
+        const double K0_teff_m2_Vs = K0_m2_Vs * teff_scale;
+        const double ion_mobility = K0_teff_m2_Vs * N_ratio;
+        return q / (ion_mobility * m_ion);
+    }
+    return 0.0;
+}
+
+// ForceRegistry::compute_total_force_soa (ForceRegistry.cpp:42-88):
+// ElectricFieldForce (F = E q) + DampingForce (F = (v - u) * (-gamma m)).
+V3 total_force(const icarion_domain_desc& d, const oracle_field* f, int coll,
+               const icarion_species_desc& sp, double m, double q, V3 pos, V3 vel, double t) {
+    V3 F = field_E(d, f, pos, t) * q;
+    if (coll == ICARION_COLLISION_LANGEVIN || coll == ICARION_COLLISION_FRICTION) {
+        const double gamma = damping_gamma(d, coll, sp, m, q, vel);
+        if (gamma > 0.0) {
+            const V3 u{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+            F = F + (vel - u) * (-gamma * m);
+        }
+    }
+    return F;
+}
+
+// ---- pushes ------------------------------------------------------------
+
+// RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361):
+// stage times t, t+h/2, t+h/2, t+h; a = F * (1/m).
+void rk4_step(const icarion_domain_desc& d, const oracle_field* f, int coll,
+              const icarion_species_desc& sp, double m, double q, V3& pos, V3& vel,
+              double t, double dt) {
+    const double half_dt = dt * 0.5;
+    const double inv_m = 1.0 / m;
+    const V3 p0 = pos, v0 = vel;
+    const V3 k1v = v0;
+    const V3 k1a = total_force(d, f, coll, sp, m, q, p0, v0, t) * inv_m;
+    const V3 p2 = p0 + k1v * half_dt, v2 = v0 + k1a * half_dt;
+    const V3 k2v = v2;
+    const V3 k2a = total_force(d, f, coll, sp, m, q, p2, v2, t + half_dt) * inv_m;
+    const V3 p3 = p0 + k2v * half_dt, v3 = v0 + k2a * half_dt;
+    const V3 k3v = v3;
+    const V3 k3a = total_force(d, f, coll, sp, m, q, p3, v3, t + half_dt) * inv_m;
+    const V3 p4 = p0 + k3v * dt, v4 = v0 + k3a * dt;
+    const V3 k4v = v4;
+    const V3 k4a = total_force(d, f, coll, sp, m, q, p4, v4, t + dt) * inv_m;
+    pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * (dt / 6.0);
+    vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * (dt / 6.0);
+}
+
+// BorisStrategy::step (BorisStrategy.cpp:18-98): the half kicks use the total
+// registry force at the step start (damping included); x += v+ dt with the
+// pre-second-kick velocity.
+void boris_step(const icarion_domain_desc& d, const oracle_field* f, int coll,
+                const icarion_species_desc& sp, double m, double q, V3& pos, V3& vel,
+                double t, double dt) {
+    const double qm = q / m;
+    const V3 F = total_force(d, f, coll, sp, m, q, pos, vel, t);
+    const V3 a = F / m;
+    V3 B{0, 0, 0};
+    if (d.magnetic_enabled) B = V3{d.B_T[0], d.B_T[1], d.B_T[2]};
+    V3 v_minus = vel;
+    v_minus = v_minus + a * (dt * 0.5);
+    const V3 t_vec = B * (qm * dt * 0.5);
+    const double t_mag_sq = t_vec.x * t_vec.x + t_vec.y * t_vec.y + t_vec.z * t_vec.z;
+    const double s_factor = 2.0 / (1.0 + t_mag_sq);
+    const V3 s_vec = t_vec * s_factor;
+    const V3 vmxt{v_minus.y * t_vec.z - v_minus.z * t_vec.y,
+                  v_minus.z * t_vec.x - v_minus.x * t_vec.z,
+                  v_minus.x * t_vec.y - v_minus.y * t_vec.x};
+    const V3 v_prime = v_minus + vmxt;
+    const V3 vpxs{v_prime.y * s_vec.z - v_prime.z * s_vec.y,
+                  v_prime.z * s_vec.x - v_prime.x * s_vec.z,
+                  v_prime.x * s_vec.y - v_prime.y * s_vec.x};
+    const V3 v_plus = v_minus + vpxs;
+    pos = pos + v_plus * dt;
+    vel = v_plus + a * (dt * 0.5);
+}
+
+// ---- collision kernels (CollisionKernels.cpp, VelocitySampling.cpp,
+// CollisionGeometry.cpp) ------------------------------------------------------
+
+// VelocitySampling::box_muller_sample (VelocitySampling.cpp:56-66)
+template <class Rng>
+double box_muller(double sigma, Rng& rng) {
+    const double u1 = rng.uniform01();
+    const double u2 = rng.uniform01();
+    return sigma * std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
+}
+
+// VelocitySampling::sample_neutral_velocity (VelocitySampling.cpp:9-35)
+template <class Rng>
+V3 sample_neutral_velocity(double T_K, double mass_kg, V3 flow, Rng& rng) {
+    const double sigma = std::sqrt(kBoltzmann * T_K / mass_kg);
+    const double vx = box_muller(sigma, rng);
+    const double vy = box_muller(sigma, rng);
+    const double vz = box_muller(sigma, rng);
+    return V3{vx + flow.x, vy + flow.y, vz + flow.z};
+}
+
+// CollisionKernels::sample_isotropic_direction (CollisionKernels.cpp:383-397)
+template <class Rng>
+V3 sample_isotropic_direction(Rng& rng) {
+    const double u1 = rng.uniform01();
+    const double u2 = rng.uniform01();
+    const double cosT = 2.0 * u1 - 1.0;
+    const double sinT = std::sqrt(std::max(0.0, 1.0 - cosT * cosT));
+    const double phi = 2.0 * kPi * u2;
+    return V3{sinT * std::cos(phi), sinT * std::sin(phi), cosT};
+}
+
+// CollisionKernels::hss_collision (CollisionKernels.cpp:139-169)
+template <class Rng>
+V3 hss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass, Rng& rng) {
+    V3 vrel = v_ion - v_neutral;
+    double vrel_mag = norm(vrel);
+    if (vrel_mag <= kMinVelocityMag) {
+        vrel = sample_isotropic_direction(rng) * kMinVelocityMag;
+        vrel_mag = kMinVelocityMag;
+    }
+    const V3 new_dir = sample_isotropic_direction(rng);
+    const V3 vrel_scattered = new_dir * vrel_mag;
+    const double mt = ion_mass + neutral_mass;
+    const double inv_mt = 1.0 / mt;
+    const V3 Vcom = (v_ion * ion_mass + v_neutral * neutral_mass) * inv_mt;
+    return Vcom + vrel_scattered * (neutral_mass * inv_mt);
+}
+
+// CollisionGeometry::construct_orthonormal_basis (CollisionGeometry.cpp:11-28)
+void orthonormal_basis(V3 ehat, V3& t1, V3& t2) {
+    const V3 a = (std::fabs(ehat.x) < 0.9) ? V3{1, 0, 0} : V3{0, 1, 0};
+    t1 = normalize(cross(ehat, a));
+    t2 = normalize(cross(ehat, t1));
+}
+
+// CollisionGeometry::generate_random_rotation (CollisionGeometry.cpp:30-59),
+// Shoemake uniform quaternion; R row-major 3x3.
+template <class Rng>
+void random_rotation(Rng& rng, double R[3][3]) {
+    const double u1 = rng.uniform01();
+    const double u2 = rng.uniform01();
+    const double u3 = rng.uniform01();
+    const double q1 = std::sqrt(1.0 - u1) * std::sin(2.0 * kPi * u2);
+    const double q2 = std::sqrt(1.0 - u1) * std::cos(2.0 * kPi * u2);
+    const double q3 = std::sqrt(u1) * std::sin(2.0 * kPi * u3);
+    const double q4 = std::sqrt(u1) * std::cos(2.0 * kPi * u3);
+    const double x = q1, y = q2, z = q3, w = q4;
+    R[0][0] = 1 - 2 * (y * y + z * z);
+    R[0][1] = 2 * (x * y - z * w);
+    R[0][2] = 2 * (x * z + y * w);
+    R[1][0] = 2 * (x * y + z * w);
+    R[1][1] = 1 - 2 * (x * x + z * z);
+    R[1][2] = 2 * (y * z - x * w);
+    R[2][0] = 2 * (x * z - y * w);
+    R[2][1] = 2 * (y * z + x * w);
+    R[2][2] = 1 - 2 * (x * x + y * y);
+}
+
+// CollisionKernels::ehss_collision + ehss_collision_with_orientation +
+// rotate_and_analyze_molecule + sample_impact_geometry + detect_atom_collision
+// + compute_reflected_velocity + to_lab_frame (CollisionKernels.cpp:13-137,
+// 241-381).  sigma_eff_m2 = 0 and max_attempts = 256 as in the handler's call
+// (EHSSCollisionHandler.cpp:363-365).
+template <class Rng>
+V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
+                  double ion_radius, const double* atoms, uint32_t nat, Rng& rng,
+                  std::vector<V3>& rotated) {
+    double orient[3][3];
+    random_rotation(rng, orient);
+    if (nat == 0) return v_ion;
+    V3 vrel = v_ion - v_neutral;
+    double vrel_mag = norm(vrel);
+    if (vrel_mag <= kMinVelocityMag) {
+        vrel = sample_isotropic_direction(rng) * kMinVelocityMag;
+        vrel_mag = kMinVelocityMag;
+    }
+    const V3 axis = vrel / vrel_mag;
+    V3 t1, t2;
+    orthonormal_basis(axis, t1, t2);
+    const double basis[3][3] = {{t1.x, t2.x, axis.x}, {t1.y, t2.y, axis.y}, {t1.z, t2.z, axis.z}};
+    double Rm[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            Rm[i][j] = basis[i][0] * orient[0][j] + basis[i][1] * orient[1][j] +
+                       basis[i][2] * orient[2][j];
+    rotated.resize(nat);
+    double b_max = 0.0;
+    for (uint32_t j = 0; j < nat; ++j) {
+        const V3 c{atoms[4 * j], atoms[4 * j + 1], atoms[4 * j + 2]};
+        const V3 ra{Rm[0][0] * c.x + Rm[0][1] * c.y + Rm[0][2] * c.z,
+                    Rm[1][0] * c.x + Rm[1][1] * c.y + Rm[1][2] * c.z,
+                    Rm[2][0] * c.x + Rm[2][1] * c.y + Rm[2][2] * c.z};
+        rotated[j] = ra;
+        const double sra = dot(axis, ra);
+        const V3 ra_perp = ra - axis * sra;
+        const double cur = norm(ra_perp) + atoms[4 * j + 3];
+        if (cur > b_max) b_max = cur;
+    }
+    b_max += ion_radius;
+    for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
+        V3 s1, s2;
+        orthonormal_basis(axis, s1, s2);
+        const double uA = rng.uniform01();
+        const double uB = rng.uniform01();
+        const double b = b_max * std::sqrt(uA);
+        const double phi = 2.0 * kPi * uB;
+        const V3 offset = s1 * (b * std::cos(phi)) + s2 * (b * std::sin(phi));
+        bool hit = false;
+        V3 n_best{0, 0, 0};
+        double best_s = 0.0;
+        for (uint32_t j = 0; j < nat; ++j) {
+            const V3 ra = rotated[j];
+            const double Rsum = atoms[4 * j + 3] + ion_radius;
+            const double Rsum2 = Rsum * Rsum;
+            const V3 rel = offset - ra;
+            const double sstar = dot(axis, rel);
+            const V3 dvec = rel - axis * sstar;
+            const double dmin2 = dot(dvec, dvec);
+            if (dmin2 <= Rsum2) {
+                const double h = std::sqrt(std::max(0.0, Rsum2 - dmin2));
+                const double s_hit = sstar - h;
+                const V3 p_hit_rel = rel - axis * s_hit;
+                const double phr2 = dot(p_hit_rel, p_hit_rel);
+                V3 n_contact;
+                if (phr2 <= kMinContactDistSq) {
+                    n_contact = axis * -1.0;
+                } else {
+                    n_contact = p_hit_rel * (1.0 / std::sqrt(phr2));
+                    if (dot(axis, n_contact) > 0.0) n_contact = n_contact * -1.0;
+                }
+                if (!hit || s_hit < best_s) {
+                    hit = true;
+                    n_best = n_contact;
+                    best_s = s_hit;
+                }
+            }
+        }
+        if (hit) {
+            const double vdotn = dot(vrel, n_best);
+            const V3 vrel_reflected = vrel - n_best * (2.0 * vdotn);
+            const double mt = ion_mass + neutral_mass;
+            const double inv_mt = 1.0 / mt;
+            const V3 Vcom = (v_ion * ion_mass + v_neutral * neutral_mass) * inv_mt;
+            return Vcom + vrel_reflected * (neutral_mass * inv_mt);
+        }
+        if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
+    }
+    return v_ion;
+}
+
+// One collision stage of one ion.  HSS: HSSCollisionHandler::handle_collision
+// single-gas path (HSSCollisionHandler.cpp:257-376).  EHSS:
+// EHSSCollisionHandler::handle_collision single-gas online path
+// (EHSSCollisionHandler.cpp:222-374, 426-455).  Langevin / Friction with
+// ou_thermal_kicks: CollisionKernels::ou_velocity_update with apply_damping =
+// false (CollisionKernels.cpp:171-214) at the ion's own gamma.
+// Returns true when a collision event happened.
+template <class Rng>
+bool collide(const icarion_domain_desc& d, const icarion_physics_desc& ph,
+             const icarion_species_desc& sp, const double* atoms, double m_ion, double q,
+             V3& vel, double dt, Rng& rng, std::vector<V3>& scratch) {
+    const int model = ph.collision_model;
+    const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    if (model == ICARION_COLLISION_HSS || model == ICARION_COLLISION_EHSS) {
+        const double n = d.gas_density_m3;
+        const double T_K = d.temperature_K;
+        const double m_neutral = d.gas_mass_kg;
+        const double sigma = (model == ICARION_COLLISION_HSS) ? sp.sigma_hss_m2 : sp.sigma_ehss_m2;
+        if (sigma <= 0.0) return false;
+        const V3 v_neutral = sample_neutral_velocity(T_K, m_neutral, v_gas, rng);
+        const V3 v_rel = vel - v_neutral;
+        const double v_rel_mag = norm(v_rel);
+        if (v_rel_mag < kMinRelativeVelocity) return false;
+        const double P = 1.0 - std::exp(-n * sigma * v_rel_mag * dt);
+        if (rng.uniform01() >= P) return false;
+        if (model == ICARION_COLLISION_HSS) {
+            vel = hss_collision(vel, v_neutral, m_ion, m_neutral, rng);
+        } else {
+            vel = ehss_collision(vel, v_neutral, m_ion, m_neutral, d.gas_radius_m,
+                                 atoms + 4 * static_cast<size_t>(sp.atom_offset), sp.atom_count,
+                                 rng, scratch);
+        }
+        return true;
+    }
+    if ((model == ICARION_COLLISION_LANGEVIN || model == ICARION_COLLISION_FRICTION) &&
+        ph.ou_thermal_kicks) {
+        const double gamma = damping_gamma(d, model, sp, m_ion, q, vel);
+        if (gamma <= 0.0 || dt <= 0.0) return false;
+        const double kBT_over_m = kBoltzmann * d.temperature_K / m_ion;
+        const double sigma = std::sqrt(2.0 * gamma * kBT_over_m * dt);
+        vel.x += sigma * rng.normal();
+        vel.y += sigma * rng.normal();
+        vel.z += sigma * rng.normal();
+        return false;
+    }
+    return false;
+}
+
+// Per-ion engine step (SimulationEngine::process_timestep,
+// SimulationEngine.cpp:374-428 with prepare/finalize of
+// SimulationEngineLifecycle.cpp:72-195).  The reference runs each phase over
+// all ions before the next; ions do not interact, so running all phases of
+// one ion back to back gives the same per-ion result.
+struct IonCtx {
+    V3 pos, vel;
+    double m, q;
+    uint8_t active, born;
+    double birth, death, arrival;
+    uint32_t ncoll;
+};
+
+// rng_kind ORACLE_RNG_PHILOX: a fresh (seed, ion, step) stream per step;
+// ORACLE_RNG_MT19937_64: the ion's persistent PhysicsRng (*mt).
+uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
+                 const icarion_species_desc& sp, const double* atoms, const oracle_field* f,
+                 IonCtx& ion, const double* t_table, double dt, uint64_t n_steps,
+                 uint32_t ion_index, uint64_t step0, MtRng* mt, uint64_t& collisions,
+                 std::vector<V3>& scratch) {
+    uint64_t pushed = 0;
+    const bool detect = std::isfinite(d.detector_z_m);
+    for (uint64_t k = 0; k < n_steps; ++k) {
+        const double t = t_table[k];
+        // prepare_ions_for_integration (SimulationEngineLifecycle.cpp:86-107)
+        if (!ion.born && t >= ion.birth) {
+            ion.born = 1;
+            ion.active = 1;
+        }
+        if (!ion.active || !ion.born) continue;
+        if (!contains(d, ion.pos)) {
+            ion.active = 0;
+            ion.death = t;
+            continue;
+        }
+        // perform_integration
+        const double z_before = ion.pos.z;
+        if (ph.integrator == ICARION_INTEGRATOR_BORIS)
+            boris_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+        else
+            rk4_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+        ++pushed;
+        // perform_collisions
+        if (ph.collision_model != ICARION_COLLISION_NONE) {
+            bool hit;
+            if (mt) {
+                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, *mt, scratch);
+            } else {
+                PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
+                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch);
+            }
+            if (hit) {
+                ++ion.ncoll;
+                ++collisions;
+            }
+        }
+        // arrival: first crossing of the detector plane (linear in the step)
+        if (detect && std::isnan(ion.arrival)) {
+            const double z_after = ion.pos.z;
+            if (z_before < d.detector_z_m && z_after >= d.detector_z_m) {
+                ion.arrival = t + dt * (d.detector_z_m - z_before) / (z_after - z_before);
+            }
+        }
+        // finalize_ions_after_integration (SimulationEngineLifecycle.cpp:137-192)
+        if (!contains(d, ion.pos)) {
+            ion.active = 0;
+            ion.death = t;
+            continue;
+        }
+        if (!finite3(ion.pos) || !finite3(ion.vel)) {
+            ion.active = 0;
+            ion.death = t;
+        }
+    }
+    return pushed;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    philox(ctr, key, out);
+}
+
+double oracle_uniform_from_words(uint32_t hi, uint32_t lo) { return uniform_from_words(hi, lo); }
+
+void oracle_philox_uniforms(uint64_t seed, uint32_t ion, uint64_t step, uint32_t n, double* out) {
+    PhiloxStream s(seed, ion, step);
+    for (uint32_t i = 0; i < n; ++i) out[i] = s.uniform01();
+}
+
+void oracle_field_E(const icarion_domain_desc* domain, const oracle_field* field,
+                    const double pos[3], double t, double E_out[3]) {
+    const V3 E = field_E(*domain, field, V3{pos[0], pos[1], pos[2]}, t);
+    E_out[0] = E.x; E_out[1] = E.y; E_out[2] = E.z;
+}
+
+int oracle_contains(const icarion_domain_desc* domain, const double pos[3]) {
+    return contains(*domain, V3{pos[0], pos[1], pos[2]}) ? 1 : 0;
+}
+
+double oracle_orbitrap_r_for_z(double z, double R, double R_m) { return orbitrap_r_for_z(z, R, R_m); }
+
+void oracle_interpolate_field(const oracle_field* field, const double pos[3], double E_out[3]) {
+    const V3 E = interpolate(*field, V3{pos[0], pos[1], pos[2]});
+    E_out[0] = E.x; E_out[1] = E.y; E_out[2] = E.z;
+}
+
+double oracle_damping_gamma(const icarion_domain_desc* domain, int collision_model,
+                            const icarion_species_desc* sp, double mass_kg, double charge_C,
+                            const double vel[3]) {
+    return damping_gamma(*domain, collision_model, *sp, mass_kg, charge_C, V3{vel[0], vel[1], vel[2]});
+}
+
+void oracle_mt_sample_neutral_velocity(uint64_t seed, double T_K, double mass_kg,
+                                       const double flow[3], double out[3]) {
+    MtRng rng(seed);
+    const V3 v = sample_neutral_velocity(T_K, mass_kg, V3{flow[0], flow[1], flow[2]}, rng);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+void oracle_mt_hss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],
+                             double ion_mass, double neutral_mass, double out[3]) {
+    MtRng rng(seed);
+    const V3 v = hss_collision(V3{v_ion[0], v_ion[1], v_ion[2]},
+                               V3{v_neutral[0], v_neutral[1], v_neutral[2]}, ion_mass,
+                               neutral_mass, rng);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+void oracle_mt_ehss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],
+                              double ion_mass, double neutral_mass, double ion_radius,
+                              const double* atoms_xyzr, uint32_t n_atoms, double out[3]) {
+    MtRng rng(seed);
+    std::vector<V3> scratch;
+    const V3 v = ehss_collision(V3{v_ion[0], v_ion[1], v_ion[2]},
+                                V3{v_neutral[0], v_neutral[1], v_neutral[2]}, ion_mass,
+                                neutral_mass, ion_radius, atoms_xyzr, n_atoms, rng, scratch);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+double oracle_oapa_sigma(const double* atoms, uint32_t n_atoms, double neutral_radius) {
+    MtRng rng(12345);
+    double A_sum = 0.0;
+    double R[3][3];
+    const int n_orientations = 64;
+    for (int k = 0; k < n_orientations; ++k) {
+        random_rotation(rng, R);
+        double Rmax = 0.0;
+        for (uint32_t i = 0; i < n_atoms; ++i) {
+            const double cx = atoms[4 * i], cy = atoms[4 * i + 1], cz = atoms[4 * i + 2];
+            const V3 c{R[0][0] * cx + R[0][1] * cy + R[0][2] * cz,
+                       R[1][0] * cx + R[1][1] * cy + R[1][2] * cz,
+                       R[2][0] * cx + R[2][1] * cy + R[2][2] * cz};
+            const double r_eff = std::sqrt(c.x * c.x + c.y * c.y) + atoms[4 * i + 3] + neutral_radius;
+            Rmax = std::max(Rmax, r_eff);
+        }
+        A_sum += kPi * Rmax * Rmax;
+    }
+    return A_sum / static_cast<double>(n_orientations);
+}
+
+void oracle_push_one(const icarion_domain_desc* domain, const icarion_physics_desc* physics,
+                     const icarion_species_desc* sp, const oracle_field* field, double mass_kg,
+                     double charge_C, double pos[3], double vel[3], double t, double dt) {
+    V3 p{pos[0], pos[1], pos[2]}, v{vel[0], vel[1], vel[2]};
+    if (physics->integrator == ICARION_INTEGRATOR_BORIS)
+        boris_step(*domain, field, physics->collision_model, *sp, mass_kg, charge_C, p, v, t, dt);
+    else
+        rk4_step(*domain, field, physics->collision_model, *sp, mass_kg, charge_C, p, v, t, dt);
+    pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+    vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
+}
+
+int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* physics,
+               const icarion_species_desc* species, uint32_t n_species, const double* atoms,
+               uint32_t n_atoms, const oracle_field* field, icarion_ion_soa* ions, double t0,
+               double dt, uint64_t n_steps, uint64_t step_offset, int rng_kind, int n_threads,
+               icarion_step_stats* stats) {
+    (void)n_atoms;
+    if (!domain || !physics || !species || !ions || n_species == 0) return ICARION_ERR_INVALID;
+    const size_t n = ions->n;
+    // Engine time t_k: the reference advances every live ion by t_i += dt and
+    // takes the engine time as max_i t_i (SimulationEngineLifecycle.cpp:172,
+    // SimulationEngineUtils.cpp:49-62), i.e. the running sum t0 + dt + dt + ...
+    std::vector<double> t_table(n_steps + 1);
+    double t = t0;
+    for (uint64_t k = 0; k <= n_steps; ++k) {
+        t_table[k] = t;
+        t = t + dt;
+    }
+    uint64_t pushed_total = 0, coll_total = 0, alive = 0, arrived = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    #pragma omp parallel reduction(+ : pushed_total, coll_total, alive, arrived)
+    {
+        std::vector<V3> scratch;
+        #pragma omp for schedule(dynamic, 64)
+        for (long long ii = 0; ii < static_cast<long long>(n); ++ii) {
+            const size_t i = static_cast<size_t>(ii);
+            const uint32_t s = ions->species[i];
+            const icarion_species_desc& sp = species[s < n_species ? s : 0];
+            IonCtx ion;
+            ion.pos = V3{ions->x[i], ions->y[i], ions->z[i]};
+            ion.vel = V3{ions->vx[i], ions->vy[i], ions->vz[i]};
+            ion.m = ions->mass_kg[i];
+            ion.q = ions->charge_C[i];
+            ion.active = ions->active[i];
+            ion.born = ions->born ? ions->born[i] : 1;
+            ion.birth = ions->birth_time_s ? ions->birth_time_s[i] : t0;
+            ion.death = ions->death_time_s ? ions->death_time_s[i] : -1.0;
+            ion.arrival = ions->arrival_time_s ? ions->arrival_time_s[i] : NAN;
+            ion.ncoll = ions->n_collisions ? ions->n_collisions[i] : 0;
+            uint64_t coll = 0;
+            uint64_t pushed = 0;
+            if (rng_kind == ORACLE_RNG_MT19937_64) {
+                MtRng rng(physics->rng_seed + static_cast<uint64_t>(i));
+                pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
+                                 n_steps, static_cast<uint32_t>(i), step_offset, &rng, coll,
+                                 scratch);
+            } else {
+                pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
+                                 n_steps, static_cast<uint32_t>(i), step_offset, nullptr, coll,
+                                 scratch);
+            }
+            ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
+            ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;
+            ions->active[i] = ion.active;
+            if (ions->born) ions->born[i] = ion.born;
+            if (ions->death_time_s) ions->death_time_s[i] = ion.death;
+            if (ions->arrival_time_s) ions->arrival_time_s[i] = ion.arrival;
+            if (ions->n_collisions) ions->n_collisions[i] = ion.ncoll;
+            pushed_total += pushed;
+            coll_total += coll;
+            if (ion.active && ion.born) ++alive;
+            if (!std::isnan(ion.arrival)) ++arrived;
+        }
+    }
+    if (stats) {
+        stats->ion_steps = pushed_total;
+        stats->collisions = coll_total;
+        stats->alive = alive;
+        stats->arrived = arrived;
+        stats->t_end_s = t_table[n_steps];
+        stats->device_ms = 0.0;
+    }
+    return ICARION_OK;
+}
+
+}  // extern "C"

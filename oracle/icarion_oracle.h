@@ -1,0 +1,99 @@
+/*
+ * icarion_oracle.h - CPU FP64 restatement of the reference batch ion-step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker and the
+ * host-CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it; the product library (icarion_amd) never links,
+ * loads or calls it.
+ *
+ * Every function restates a reference function step for step; the
+ * citation next to each definition in icarion_oracle.cpp names the
+ * reference file:line it follows.  Descriptors are the product's own
+ * (include/icarion_gpu.h) so both sides read identical inputs.
+ */
+#ifndef ICARION_ORACLE_H
+#define ICARION_ORACLE_H
+
+#include "../include/icarion_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Field array in the reference's own layout (FieldArray,
+ * fieldArrayLoader.h:23-40): monotone axes, x-fastest FP64 components. */
+typedef struct oracle_field {
+    uint32_t nx, ny, nz;
+    const double *xs, *ys, *zs;
+    const double *Ex, *Ey, *Ez;
+} oracle_field;
+
+enum { ORACLE_RNG_PHILOX = 0, ORACLE_RNG_MT19937_64 = 1 };
+
+/* Philox4x32-10 block function (Salmon et al., SC'11). */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* The FP64 uniform the Philox stream hands out for words (hi, lo). */
+double oracle_uniform_from_words(uint32_t hi, uint32_t lo);
+
+/* First n FP64 uniforms of the (seed, ion, step) Philox stream. */
+void oracle_philox_uniforms(uint64_t seed, uint32_t ion, uint64_t step, uint32_t n, double *out);
+
+/* E(pos, t) of the domain (analytic model, or field array when
+ * domain->use_field_array and field != NULL). */
+void oracle_field_E(const icarion_domain_desc *domain, const oracle_field *field,
+                    const double pos[3], double t, double E_out[3]);
+
+/* Domain containment (CylindricalGeometry::contains / OrbitrapGeometry::contains). */
+int oracle_contains(const icarion_domain_desc *domain, const double pos[3]);
+
+/* Orbitrap electrode radius at axial position z by bisection
+ * (OrbitrapGeometry.h:181-219). */
+double oracle_orbitrap_r_for_z(double z, double R, double R_m);
+
+/* Trilinear interpolation with clamping (fieldArrayLoader.cpp:103-162). */
+void oracle_interpolate_field(const oracle_field *field, const double pos[3], double E_out[3]);
+
+/* Damping coefficient gamma of DampingForce for the collision model
+ * (Langevin / Friction), DampingForce.cpp:105-277. */
+double oracle_damping_gamma(const icarion_domain_desc *domain, int collision_model,
+                            const icarion_species_desc *sp, double mass_kg,
+                            double charge_C, const double vel[3]);
+
+/* ---- single kernels driven by std::mt19937_64 (PhysicsRng(seed)) ------
+ * These consume random numbers in exactly the reference order so they can be
+ * compared bit for bit with the reference's compiled kernels. */
+void oracle_mt_sample_neutral_velocity(uint64_t seed, double T_K, double mass_kg,
+                                       const double flow[3], double out[3]);
+void oracle_mt_hss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],
+                             double ion_mass, double neutral_mass, double out[3]);
+void oracle_mt_ehss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],
+                              double ion_mass, double neutral_mass, double ion_radius,
+                              const double *atoms_xyzr, uint32_t n_atoms, double out[3]);
+/* OAPA sigma_eff over 64 orientations from PhysicsRng(12345)
+ * (EHSSCollisionHandler.cpp:476-497). */
+double oracle_oapa_sigma(const double *atoms_xyzr, uint32_t n_atoms, double neutral_radius);
+
+/* One RK4 step / Boris step of a single ion (no collisions, no boundary),
+ * for unit tests of the push against closed forms. */
+void oracle_push_one(const icarion_domain_desc *domain, const icarion_physics_desc *physics,
+                     const icarion_species_desc *sp, const oracle_field *field,
+                     double mass_kg, double charge_C, double pos[3], double vel[3],
+                     double t, double dt);
+
+/* The whole engine loop: n_steps steps of dt from engine time t0 over the
+ * ensemble in place.  rng_kind selects Philox streams (identical to the
+ * device) or per-ion std::mt19937_64 seeded rng_seed + i (RngUtils.cpp:51-53).
+ * n_threads <= 0 uses the OpenMP default.  Returns 0 on success. */
+int oracle_run(const icarion_domain_desc *domain, const icarion_physics_desc *physics,
+               const icarion_species_desc *species, uint32_t n_species,
+               const double *atoms_xyzr, uint32_t n_atoms, const oracle_field *field,
+               icarion_ion_soa *ions, double t0, double dt, uint64_t n_steps,
+               uint64_t step_offset, int rng_kind, int n_threads,
+               icarion_step_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,75 @@
+"""The C-ABI library: it loads, exports every entry point include/icarion_gpu.h
+declares, and the ctypes structs match the C layout (no device needed)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from icarion_amd import _abi, gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "icarion_gpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t)\s+(icarion_gpu_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gpu.load()
+    names = declared_functions()
+    assert len(names) >= 12
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(gpu.EXPORTED)
+
+
+def test_abi_version():
+    assert gpu.load().icarion_gpu_abi_version() == _abi.ABI_VERSION
+
+
+def test_last_error_roundtrip_without_device():
+    lib = gpu.load()
+    h = C.c_void_p()
+    rc = lib.icarion_gpu_create(None, C.byref(h))
+    if rc == _abi.OK:  # a device is visible: nothing to check here
+        lib.icarion_gpu_destroy(h)
+        pytest.skip("device present")
+    assert rc == _abi.ERR_NO_DEVICE
+    assert "device" in gpu.last_error(lib)
+
+
+STRUCTS = {
+    "icarion_domain_desc": (_abi.DomainDesc, ["instrument", "origin_m", "length_m", "rot_g2l",
+                                              "dc_axial_V", "field_array_scale", "B_T",
+                                              "temperature_K", "gas_velocity_m_s",
+                                              "detector_z_m"]),
+    "icarion_physics_desc": (_abi.PhysicsDesc, ["integrator", "precision", "rng_seed"]),
+    "icarion_species_desc": (_abi.SpeciesDesc, ["sigma_hss_m2", "K0_cm2_Vs", "atom_offset",
+                                                "atom_count"]),
+    "icarion_ion_soa": (_abi.IonSoA, ["n", "x", "species", "active", "born", "n_collisions"]),
+    "icarion_step_stats": (_abi.StepStats, ["ion_steps", "alive", "t_end_s", "device_ms"]),
+    "icarion_gpu_desc": (_abi.GpuDesc, ["device", "block_size", "steps_per_launch"]),
+}
+
+
+def test_ctypes_layout_matches_c_compiler(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for cname, (_, fields) in STRUCTS.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in fields:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = dict(line.rsplit(" ", 1) for line in out.strip().splitlines())
+    for cname, (cls, fields) in STRUCTS.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for f in fields:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
