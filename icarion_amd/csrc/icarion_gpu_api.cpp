@@ -168,14 +168,8 @@ void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p
 StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_desc& p) {
     StepParams P{};
     P.instrument = d.instrument;
-    P.rot_identity = (is_identity(d.rot_g2l) && is_identity(d.rot_l2g) && d.origin_m[0] == 0.0 &&
-                      d.origin_m[1] == 0.0 && d.origin_m[2] == 0.0) ? 1 : 0;
-    if (d.instrument != ICARION_INSTRUMENT_ORBITRAP) {
-        // cylinder: contains() works in local coordinates; identity rotation
-        // with a non-zero origin still needs the shift
-        P.rot_identity = (is_identity(d.rot_g2l) && d.origin_m[0] == 0.0 && d.origin_m[1] == 0.0 &&
-                          d.origin_m[2] == 0.0) ? 1 : 0;
-    }
+    // the origin shift is always applied; rot_identity skips the 3x3 products
+    P.rot_identity = (is_identity(d.rot_g2l) && is_identity(d.rot_l2g)) ? 1 : 0;
     P.magnetic = d.magnetic_enabled;
     P.damping_model = p.collision_model;
     P.ou_kicks = p.ou_thermal_kicks;

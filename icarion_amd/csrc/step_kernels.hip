@@ -23,10 +23,8 @@ namespace icarion {
 using d3 = vec3<double>;
 
 __device__ __forceinline__ bool contains_f64(const StepParams& P, d3 p) {
-    d3 l = p;
-    if (!P.rot_identity) {
-        l = mat_mul(P.g2l, d3{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]});
-    }
+    d3 l{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]};
+    if (!P.rot_identity) l = mat_mul(P.g2l, l);
     if (P.instrument == 3) {
         // OrbitrapGeometry::contains (OrbitrapGeometry.h:36-51).  The electrode
         // radius r_e(z) is the root of f(r) = z^2 - (r^2 - R^2)/2 - Rm^2 ln(R/r),
@@ -60,7 +58,7 @@ __device__ __forceinline__ void locate_f64(double v, double v0, double v_last, d
         a = 1.0;
     } else {
         int i = static_cast<int>((v - v0) * inv_d);
-        i = i > n - 2 ? n - 2 : i;
+        i = i > n - 2 ? n - 2 : (i < 0 ? 0 : i);
         i0 = i;
         a = (v - (v0 + static_cast<double>(i) * d)) * inv_d;
     }
@@ -108,9 +106,8 @@ __device__ __forceinline__ d3 field_f64(const StepParams& P, d3 p, double c_rf) 
         return E;
     } else if constexpr (FIELD == FIELD_ORBITRAP) {
         // AnalyticalFieldModel.cpp:188-217
-        d3 l = p;
-        if (!P.rot_identity)
-            l = mat_mul(P.g2l, d3{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]});
+        d3 l{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]};
+        if (!P.rot_identity) l = mat_mul(P.g2l, l);
         const double r_sq = fmax(1e-18, l.x * l.x + l.y * l.y);
         const double C = 1.0 - P.R_char_sq / r_sq;
         const double hk = 0.5 * P.orbi_k;
@@ -498,9 +495,9 @@ __device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C,
         }
         return E;
     } else if constexpr (FIELD == FIELD_ORBITRAP) {
-        f3 l = p;
+        const f3 s{p.x - C.ox, p.y - C.oy, p.z - C.oz};
+        f3 l = s;
         if (!P.rot_identity) {
-            const f3 s{p.x - C.ox, p.y - C.oy, p.z - C.oz};
             l = f3{C.g2l[0] * s.x + C.g2l[1] * s.y + C.g2l[2] * s.z,
                    C.g2l[3] * s.x + C.g2l[4] * s.y + C.g2l[5] * s.z,
                    C.g2l[6] * s.x + C.g2l[7] * s.y + C.g2l[8] * s.z};
@@ -522,7 +519,7 @@ __device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C,
             else {
                 const float f = (v - v0) * inv_d;
                 int i = static_cast<int>(f);
-                i = i > n - 2 ? n - 2 : i;
+                i = i > n - 2 ? n - 2 : (i < 0 ? 0 : i);
                 i0 = i;
                 a = f - static_cast<float>(i);
             }
@@ -554,9 +551,9 @@ __device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C,
 // position is good to ~1e-7 relative, far finer than any domain feature).
 __device__ __forceinline__ bool contains_f32(const StepParams& P, const FastConst& C, float px,
                                              float py, float pz) {
-    f3 l{px, py, pz};
+    const f3 s{px - C.ox, py - C.oy, pz - C.oz};
+    f3 l = s;
     if (!P.rot_identity) {
-        const f3 s{px - C.ox, py - C.oy, pz - C.oz};
         l = f3{C.g2l[0] * s.x + C.g2l[1] * s.y + C.g2l[2] * s.z,
                C.g2l[3] * s.x + C.g2l[4] * s.y + C.g2l[5] * s.z,
                C.g2l[6] * s.x + C.g2l[7] * s.y + C.g2l[8] * s.z};

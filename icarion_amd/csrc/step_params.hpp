@@ -53,7 +53,7 @@ struct IonPtrs {
 struct StepParams {
     // geometry (DomainConfig.geometry)
     int instrument;
-    int rot_identity;        // rotation is the identity and origin is (0,0,0)
+    int rot_identity;        // global->local rotation is the identity (origin shift still applies)
     int magnetic;
     int damping_model;       // 3 Langevin, 4 Friction (collision model code)
     int ou_kicks;

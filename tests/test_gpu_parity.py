@@ -1,0 +1,319 @@
+"""Parity of the gfx950 kernels (through the C ABI) with the CPU oracle.
+
+FP64 plain path: identical Philox streams on both sides, so trajectories are
+compared ion by ion.  Tolerances:
+  * collisionless runs: relative 1e-9 over the whole run (the per-step bound of
+    the north-star criterion, held cumulatively);
+  * stochastic runs: >= 99 % of ions match to 1e-9 (a collision decision that
+    sits within rounding of its threshold can flip and send one ion on another
+    path), and the ensemble statistics match to 0.2 %.
+FP32 + Kahan fast path: against the FP64 oracle, relative 2e-5 on collisionless
+runs and statistical agreement (mobility 1 %, KS test) on stochastic runs.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from icarion_amd import _abi
+from icarion_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_launch=0,
+            block=0):
+    from icarion_amd import gpu
+    phys = w["physics"]
+    if precision is not None:
+        phys = W.physics(phys.integrator, phys.collision_model, phys.ou_thermal_kicks, precision,
+                         phys.rng_seed, phys.ehss_mode)
+    with gpu.Handle(steps_per_launch=steps_per_launch, block_size=block) as h:
+        h.set_domain(w["domain"], phys)
+        h.set_species(w["species"], w.get("atoms"))
+        if w.get("field") is not None:
+            E, origin, spacing = w["field"]
+            h.upload_field_array(E, origin, spacing)
+        h.upload_state(ens)
+        st = h.step(t0, w["dt"], n_steps, step_offset)
+        h.download_state(ens)
+    return st
+
+
+def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
+    field = None
+    if w.get("field") is not None:
+        field = oracle.Field(*w["field"])
+    return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
+                      step_offset=step_offset, atoms=w.get("atoms"), field=field)
+
+
+def rel_close(a, b, rtol, scale):
+    return np.abs(a - b) <= rtol * np.maximum(np.abs(b), scale)
+
+
+def compare_ions(g, o, rtol=1e-9, pos_scale=1e-6, vel_scale=1.0):
+    ok = np.ones(g.n, bool)
+    for f in ("x", "y", "z"):
+        ok &= rel_close(getattr(g, f), getattr(o, f), rtol, pos_scale)
+    for f in ("vx", "vy", "vz"):
+        ok &= rel_close(getattr(g, f), getattr(o, f), rtol, vel_scale)
+    ok &= g.active == o.active
+    return ok
+
+
+def small(w, n):
+    w = dict(w)
+    w["ensemble"] = w["ensemble"].subset(np.arange(n))
+    return w
+
+
+# ------------------------------------------------------------------ collisionless
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_quadrupole_collisionless_fp64(oracle, integrator):
+    w = W.quadrupole_workload(n=4096, U=20.0, seed=3)
+    w["physics"] = W.physics(integrator, _abi.COLLISION_NONE, seed=3)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    st = gpu_run(w, g, 3000, t0=1e-7)
+    oracle_run(oracle, w, o, 3000, t0=1e-7)
+    ok = compare_ions(g, o, rtol=1e-9)
+    assert ok.all(), np.nonzero(~ok)[0][:10]
+    np.testing.assert_array_equal(g.death_time_s, o.death_time_s)
+    assert st.ion_steps > 0 and 0 < st.alive < 4096
+
+
+def test_orbitrap_collisionless_fp64(oracle):
+    w = W.orbitrap_workload(n=2048, seed=5)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 5000)
+    oracle_run(oracle, w, o, 5000)
+    ok = compare_ions(g, o, rtol=1e-9)
+    assert ok.all()
+    assert g.alive.mean() > 0.9
+
+
+def test_ims_collisionless_single_step_bitwise_close(oracle):
+    w = W.ims_hss_workload(n=4096)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 1)
+    oracle_run(oracle, w, o, 1)
+    assert compare_ions(g, o, rtol=1e-14).all()
+
+
+# ------------------------------------------------------------------ stochastic FP64
+def test_ims_hss_fp64_ion_by_ion(oracle):
+    w = W.ims_hss_workload(n=4096, seed=123)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    z0 = w["ensemble"].z.copy()
+    sg = gpu_run(w, g, 3000)
+    so = oracle_run(oracle, w, o, 3000)
+    ok = compare_ions(g, o, rtol=1e-9)
+    assert ok.mean() >= 0.99, ok.mean()
+    assert (g.n_collisions == o.n_collisions).mean() >= 0.99
+    assert sg.ion_steps == so.ion_steps
+    assert sg.collisions == pytest.approx(so.collisions, rel=2e-3)
+    dzg = (g.z - z0)[g.alive].mean()
+    dzo = (o.z - z0)[o.alive].mean()
+    assert dzg == pytest.approx(dzo, rel=2e-3)
+
+
+def test_quadrupole_langevin_ou_fp64_ion_by_ion(oracle):
+    w = W.quadrupole_workload(n=4096, U=10.0, seed=9)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 3000)
+    oracle_run(oracle, w, o, 3000)
+    ok = compare_ions(g, o, rtol=1e-9)
+    assert ok.mean() >= 0.99, ok.mean()
+    assert (g.active == o.active).mean() >= 0.995
+
+
+def test_friction_fp64(oracle):
+    w = W.ims_hss_workload(n=2048)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_FRICTION, ou=1, seed=4)
+    w["species"] = [W.species(K0=2.1, ccs=100e-20)]
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 2000)
+    oracle_run(oracle, w, o, 2000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+
+
+def ehss_workload(n=1024, seed=31):
+    rng = np.random.default_rng(seed)
+    clusters = []
+    for nat in (1, 5, 23):
+        pos = rng.normal(0, 3e-10, (nat, 3))
+        pos -= pos.mean(0)
+        clusters.append(np.concatenate([pos, rng.uniform(1.5e-10, 2e-10, (nat, 1))], 1))
+    atoms = np.concatenate(clusters)
+    offs = np.cumsum([0] + [len(c) for c in clusters])[:-1]
+    sp = [W.species(sigma_ehss=s, atom_offset=int(a), atom_count=len(c))
+          for s, a, c in zip((40e-20, 90e-20, 200e-20), offs, clusters)]
+    w = W.ims_hss_workload(n=n, seed=seed)
+    w["domain"] = W.ims_domain(EN_Td=20.0, gas="He", pressure_Pa=101325.0)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS, seed=seed)
+    w["species"] = sp
+    w["atoms"] = atoms
+    w["ensemble"].species[:] = rng.integers(0, 3, n)
+    return w
+
+
+def test_ehss_fp64_ion_by_ion(oracle):
+    w = ehss_workload()
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 1000)
+    so = oracle_run(oracle, w, o, 1000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+
+
+def test_field_array_fp64(oracle):
+    w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 1500)
+    oracle_run(oracle, w, o, 1500)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+
+
+def test_birth_times_fp64(oracle):
+    w = W.ims_hss_workload(n=2048)
+    e = w["ensemble"]
+    rng = np.random.default_rng(1)
+    e.born[:] = 0
+    e.active[:] = 0
+    e.birth_time_s = rng.uniform(0, 1e-7, e.n)
+    g, o = e.copy(), e.copy()
+    gpu_run(w, g, 2000)
+    oracle_run(oracle, w, o, 2000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert g.born.all()
+
+
+def test_multi_call_equals_single_call():
+    """Resident stepping split over calls (and the per-step step_batch shape)
+    equals one call: same kernels, same RNG counters, same time accumulation."""
+    from icarion_amd import gpu
+    w = W.ims_hss_workload(n=1000)
+    a, b, c = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, a, 40)
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"])
+        h.upload_state(b)
+        t = 0.0
+        for k in range(4):
+            st = h.step(t, w["dt"], 10, 10 * k)
+            t = st.t_end_s
+        h.download_state(b)
+        t = 0.0
+        for k in range(40):
+            st = h.step_batch(c, t, w["dt"], k)
+            t = st.t_end_s
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+        np.testing.assert_array_equal(getattr(a, f), getattr(c, f))
+
+
+def test_launch_shape_invariance():
+    w = W.quadrupole_workload(n=3000, seed=2)
+    ref = w["ensemble"].copy()
+    gpu_run(w, ref, 500)
+    for spl, blk in ((7, 64), (128, 128), (500, 192)):
+        e = w["ensemble"].copy()
+        gpu_run(w, e, 500, steps_per_launch=spl, block=blk)
+        np.testing.assert_array_equal(e.x, ref.x)
+        np.testing.assert_array_equal(e.vz, ref.vz)
+
+
+# ------------------------------------------------------------------ FP32 + Kahan
+def test_fp32_kahan_collisionless_quadrupole(oracle):
+    w = W.quadrupole_workload(n=4096, U=0.0, seed=4)
+    w["physics"] = W.physics(_abi.INTEGRATOR_BORIS, _abi.COLLISION_NONE, seed=4)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 5000, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w, o, 5000)
+    both = g.alive & o.alive
+    assert (g.active == o.active).mean() > 0.999
+    ok = compare_ions(g.subset(np.nonzero(both)[0]), o.subset(np.nonzero(both)[0]), rtol=2e-5,
+                      pos_scale=1e-4, vel_scale=10.0)
+    assert ok.mean() > 0.999
+
+
+def test_fp32_kahan_orbitrap_phase(oracle):
+    w = W.orbitrap_workload(n=1024, seed=8)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 20000, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w, o, 20000)
+    both = g.alive & o.alive
+    dz = np.abs(g.z - o.z)[both]
+    assert np.median(dz) < 2e-8   # 20 us of axial motion, ~1e-5 of the amplitude
+
+
+def test_fp32_kahan_ims_hss_statistics(oracle):
+    w = W.ims_hss_workload(n=20000, seed=123)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    z0 = w["ensemble"].z.copy()
+    gpu_run(w, g, 5000, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w, o, 5000)
+    dzg = (g.z - z0)[g.alive]
+    dzo = (o.z - z0)[o.alive]
+    assert dzg.mean() == pytest.approx(dzo.mean(), rel=0.01)
+    assert dzg.std() == pytest.approx(dzo.std(), rel=0.03)
+    assert stats.ks_2samp(dzg, dzo).pvalue > 1e-3
+    assert g.n_collisions.mean() == pytest.approx(o.n_collisions.mean(), rel=0.005)
+
+
+def test_fp32_kahan_quadrupole_transmission(oracle):
+    w = W.quadrupole_workload(n=20000, U=0.0, seed=12)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 20000, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w, o, 20000)
+    bins = np.geomspace(100, 2000, 9)
+    m = w["ensemble"].mass_kg / 1.66053906660e-27
+    for lo, hi in zip(bins[:-1], bins[1:]):
+        sel = (m >= lo) & (m < hi)
+        tg, to = g.alive[sel].mean(), o.alive[sel].mean()
+        err = 4 * math.sqrt(max(to * (1 - to), 1e-4) / sel.sum()) + 0.01
+        assert abs(tg - to) < err, (lo, hi, tg, to)
+
+
+# ------------------------------------------------------------------ API errors
+def test_error_paths():
+    from icarion_amd import gpu
+    w = W.ims_hss_workload(n=16)
+    with gpu.Handle() as h:
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, 1e-10, 1)
+        assert ei.value.code == _abi.ERR_STATE
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"])
+        bad = w["ensemble"].copy()
+        bad.species[3] = 5
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.upload_state(bad)
+        assert ei.value.code == _abi.ERR_INVALID
+        ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS,
+                       precision=_abi.PRECISION_FP32_KAHAN)
+        h.set_domain(w["domain"], ph)
+        h.upload_state(w["ensemble"])
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, 1e-10, 1)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, -1.0, 1)
+        assert ei.value.code == _abi.ERR_INVALID
+
+
+def test_empty_ensemble():
+    from icarion_amd import gpu
+    from icarion_amd.ensemble import Ensemble
+    w = W.ims_hss_workload(n=16)
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"])
+        e = Ensemble(0)
+        h.upload_state(e)
+        st = h.step(0.0, 1e-10, 10)
+        assert st.ion_steps == 0 and st.alive == 0
+        assert st.t_end_s == pytest.approx(1e-9)
