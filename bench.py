@@ -7,7 +7,10 @@ Langevin damping + OU thermal kicks in He at 1 Pa / 300 K, Gaussian cloud
 
 A "step" is one engine time step of the whole cloud.  The ensemble is uploaded
 before the timed region (inputs resident in HBM); the timed region is exactly
---steps engine steps, bracketed by device synchronisation.  Both arithmetic
+--steps engine steps, bracketed by device synchronisation
+(icarion_gpu_synchronize: stream + device).  torch is not loaded: it bundles a
+second HIP/HSA runtime that cannot share the device with the library's in one
+process.  Both arithmetic
 paths run: the plain FP64 method and the FP32 + Kahan fast path (the headline
 value); their transmission per m/z bin is compared.  The CPU baseline is the
 oracle (same algorithm, FP64) on the host cores, timed on a bounded sample.
@@ -47,14 +50,25 @@ def parse():
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-sample-ions", type=int, default=100_000)
-    ap.add_argument("--cpu-sample-steps", type=int, default=1000)
+    ap.add_argument("--cpu-sample-steps", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter "
+                         "calibration for rocprofv3 --pmc); loads torch, not the library")
     return ap.parse_args()
 
 
-def sync():
+def copy_probe(mib):
+    """A plain copy kernel of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE of
+    this run calibrate the counters (expected mib MiB read and written)."""
     import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
     torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
 def run_path(w, precision, steps, warmup, spl):
@@ -70,10 +84,10 @@ def run_path(w, precision, steps, warmup, spl):
             h.upload_state(ens)
             h.step(0.0, w["dt"], warmup, 0)
         h.upload_state(ens)  # timed run starts from the initial state
-        sync()
+        h.synchronize()
         t0 = time.perf_counter()
         st = h.step(0.0, w["dt"], steps, 0)
-        sync()
+        h.synchronize()
         wall = time.perf_counter() - t0
         h.download_state(ens)
     launches = math.ceil(steps / (spl or 2000))
@@ -106,6 +120,9 @@ def cpu_baseline(w, n_ions, n_steps):
 
 def main():
     args = parse()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
               "multi-GPU sharding is not part of this build", file=sys.stderr)

@@ -607,6 +607,15 @@ int icarion_gpu_download_state(icarion_gpu* h, icarion_ion_soa* ions) {
     });
 }
 
+int icarion_gpu_synchronize(icarion_gpu* h) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        set_device(h);
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+}
+
 int icarion_gpu_step_batch(icarion_gpu* h, icarion_ion_soa* ions, double t, double dt,
                            uint64_t step_index, icarion_step_stats* stats) {
     return guarded([&] {

@@ -18,6 +18,7 @@ EXPORTED = (
     "icarion_gpu_create", "icarion_gpu_destroy", "icarion_gpu_set_domain",
     "icarion_gpu_set_species", "icarion_gpu_upload_field_array", "icarion_gpu_upload_state",
     "icarion_gpu_step", "icarion_gpu_download_state", "icarion_gpu_step_batch",
+    "icarion_gpu_synchronize",
 )
 
 
@@ -54,6 +55,7 @@ def load(path=None):
     lib.icarion_gpu_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
                                      P(_abi.StepStats)]
     lib.icarion_gpu_download_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
+    lib.icarion_gpu_synchronize.argtypes = [C.c_void_p]
     lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
                                            C.c_uint64, P(_abi.StepStats)]
     for name in EXPORTED:
@@ -158,6 +160,9 @@ class Handle:
         s = ens.soa()
         _check(self.lib.icarion_gpu_download_state(self.h, C.byref(s)), self.lib)
         return ens
+
+    def synchronize(self):
+        _check(self.lib.icarion_gpu_synchronize(self.h), self.lib)
 
     def step_batch(self, ens, t, dt, step_index):
         s = ens.soa()

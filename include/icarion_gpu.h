@@ -244,6 +244,9 @@ int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
 
+/* Block until all work queued on the handle's stream (and the device) is done. */
+int icarion_gpu_synchronize(icarion_gpu *h);
+
 /* One engine step on a host-owned ensemble: upload, one step, download.
  * The per-call shape of the reference batch hooks (IIntegrationStrategy.h:106-132,
  * ICollisionHandler.h:109-134), crossing PCIe both ways. */

@@ -1,0 +1,130 @@
+// icarion_gpu.hpp - C++17 host adaptor over the C ABI of icarion_gpu.h.
+//
+// BatchStepper follows the contract of the reference's batch hooks:
+//   IIntegrationStrategy::supports_batch / step_batch
+//     (src/core/integrator/strategies/IIntegrationStrategy.h:106-132)
+//   ICollisionHandler::supports_batch / handle_batch
+//     (src/core/physics/collisions/ICollisionHandler.h:109-134)
+// step_batch() returns false instead of throwing when the device path cannot
+// run (no device, unsupported configuration, HIP failure), so the caller falls
+// back to its CPU loop, as GPUIntegrationStrategy does
+// (GPUIntegrationStrategy.cpp:22-34, 277-279).  The fused device step covers
+// push + collision + boundary elimination of one engine step, so an engine that
+// uses it skips its own perform_collisions / finalize for that step.
+//
+// Resident mode (upload once, step many, download once) avoids the per-step
+// PCIe round trip of the reference's GPU helpers (GPUIntegrationHelper.cpp:188-228).
+#ifndef ICARION_GPU_HPP
+#define ICARION_GPU_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "icarion_gpu.h"
+
+namespace icarion {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+
+private:
+    int code_;
+};
+
+inline std::string last_error() {
+    char buf[1024];
+    icarion_gpu_last_error(buf, sizeof(buf));
+    return std::string(buf);
+}
+
+inline void check(int rc) {
+    if (rc != ICARION_OK) throw Error(rc, last_error());
+}
+
+class BatchStepper {
+public:
+    BatchStepper() = default;
+    explicit BatchStepper(const icarion_gpu_desc& desc) { check(icarion_gpu_create(&desc, &h_)); }
+    ~BatchStepper() { reset(); }
+    BatchStepper(const BatchStepper&) = delete;
+    BatchStepper& operator=(const BatchStepper&) = delete;
+    BatchStepper(BatchStepper&& o) noexcept : h_(std::exchange(o.h_, nullptr)), ready_(o.ready_) {}
+    BatchStepper& operator=(BatchStepper&& o) noexcept {
+        if (this != &o) {
+            reset();
+            h_ = std::exchange(o.h_, nullptr);
+            ready_ = o.ready_;
+        }
+        return *this;
+    }
+
+    // True when a device is visible and the handle exists
+    // (IIntegrationStrategy::supports_batch).
+    bool supports_batch() const { return h_ != nullptr && ready_; }
+
+    // Throwing configuration calls.
+    void configure(const icarion_domain_desc& domain, const icarion_physics_desc& physics,
+                   const std::vector<icarion_species_desc>& species,
+                   const std::vector<double>& atoms_xyzr = {}) {
+        check(icarion_gpu_set_domain(h_, &domain, &physics));
+        check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()),
+                                      atoms_xyzr.empty() ? nullptr : atoms_xyzr.data(),
+                                      static_cast<uint32_t>(atoms_xyzr.size() / 4)));
+        ready_ = true;
+    }
+    void upload_field_array(uint32_t nx, uint32_t ny, uint32_t nz, const double origin[3],
+                            const double spacing[3], const float* E_xyz) {
+        check(icarion_gpu_upload_field_array(h_, nx, ny, nz, origin[0], spacing[0], origin[1],
+                                             spacing[1], origin[2], spacing[2], E_xyz, 0));
+    }
+
+    // Resident mode.
+    void upload(const icarion_ion_soa& ions) { check(icarion_gpu_upload_state(h_, &ions)); }
+    icarion_step_stats advance(double t0, double dt, uint64_t n_steps, uint64_t step_offset) {
+        icarion_step_stats st{};
+        check(icarion_gpu_step(h_, t0, dt, n_steps, step_offset, &st));
+        return st;
+    }
+    void download(icarion_ion_soa& ions) { check(icarion_gpu_download_state(h_, &ions)); }
+
+    // One engine step on a host-owned ensemble.  false = "not handled, run the
+    // CPU path" (the reference's step_batch / handle_batch return value).
+    bool step_batch(icarion_ion_soa& ions, double t, double dt, uint64_t step_index,
+                    icarion_step_stats* stats = nullptr) noexcept {
+        if (!supports_batch()) return false;
+        icarion_step_stats st{};
+        const int rc = icarion_gpu_step_batch(h_, &ions, t, dt, step_index, &st);
+        if (rc != ICARION_OK) return false;
+        if (stats) *stats = st;
+        return true;
+    }
+
+    icarion_gpu* handle() const { return h_; }
+
+    // Factory that never throws: an empty stepper when no device is visible or
+    // creation fails (caller keeps using its CPU strategy).
+    static BatchStepper try_create(const icarion_gpu_desc& desc) {
+        BatchStepper s;
+        if (icarion_gpu_device_count() <= 0) return s;
+        if (icarion_gpu_create(&desc, &s.h_) != ICARION_OK) s.h_ = nullptr;
+        return s;
+    }
+
+private:
+    void reset() {
+        if (h_) icarion_gpu_destroy(h_);
+        h_ = nullptr;
+        ready_ = false;
+    }
+    icarion_gpu* h_ = nullptr;
+    bool ready_ = false;
+};
+
+}  // namespace icarion
+
+#endif  // ICARION_GPU_HPP

@@ -228,16 +228,26 @@ def test_launch_shape_invariance():
 
 # ------------------------------------------------------------------ FP32 + Kahan
 def test_fp32_kahan_collisionless_quadrupole(oracle):
+    """FP32 + Kahan vs the FP64 oracle, collisionless RF quadrupole (Boris), 5 RF
+    periods.  Error measured on the phase-space scale of the trap: position
+    against r0, velocity against r0 * Omega.  Ions within 5 % of the Mathieu
+    stability edge (q = 0.908), whose orbits amplify any perturbation, are held
+    to the survival check only."""
     w = W.quadrupole_workload(n=4096, U=0.0, seed=4)
     w["physics"] = W.physics(_abi.INTEGRATOR_BORIS, _abi.COLLISION_NONE, seed=4)
     g, o = w["ensemble"].copy(), w["ensemble"].copy()
     gpu_run(w, g, 5000, precision=_abi.PRECISION_FP32_KAHAN)
     oracle_run(oracle, w, o, 5000)
-    both = g.alive & o.alive
     assert (g.active == o.active).mean() > 0.999
-    ok = compare_ions(g.subset(np.nonzero(both)[0]), o.subset(np.nonzero(both)[0]), rtol=2e-5,
-                      pos_scale=1e-4, vel_scale=10.0)
-    assert ok.mean() > 0.999
+    r0, Om = 4e-3, 2 * math.pi * 1e6
+    qM = 4 * 1.602176634e-19 * 500.0 / (w["ensemble"].mass_kg * r0 * r0 * Om * Om)
+    sel = g.alive & o.alive & (np.abs(qM - 0.908) > 0.05 * 0.908)
+    dp = np.sqrt((g.x - o.x) ** 2 + (g.y - o.y) ** 2 + (g.z - o.z) ** 2)[sel] / r0
+    dv = np.sqrt((g.vx - o.vx) ** 2 + (g.vy - o.vy) ** 2 + (g.vz - o.vz) ** 2)[sel] / (r0 * Om)
+    q = [np.quantile(dp, x) for x in (0.5, 0.99, 1.0)] + [np.quantile(dv, x) for x in (0.5, 0.99, 1.0)]
+    msg = "pos err/r0 (median, p99, max) %.2e %.2e %.2e; vel err (median, p99, max) %.2e %.2e %.2e" % tuple(q)
+    print(msg)
+    assert q[0] < 1e-6 and q[1] < 1e-5 and q[3] < 1e-6 and q[4] < 1e-5, msg
 
 
 def test_fp32_kahan_orbitrap_phase(oracle):
