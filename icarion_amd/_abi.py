@@ -47,8 +47,17 @@ class PhysicsDesc(C.Structure):
 class SpeciesDesc(C.Structure):
     _fields_ = [
         ("sigma_hss_m2", _d), ("sigma_ehss_m2", _d), ("ccs_m2", _d), ("K0_cm2_Vs", _d),
-        ("atom_offset", _u32), ("atom_count", _u32),
+        ("atom_offset", _u32), ("atom_count", _u32), ("surrogate_features", C.c_float * 3),
+        ("reserved0", _u32),
     ]
+
+
+class TrainDesc(C.Structure):
+    _fields_ = [("epochs", _u32), ("batch", _u32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", _u64)]
+
+
+SURROGATE_NPARAMS = 17795
 
 
 class IonSoA(C.Structure):

@@ -21,9 +21,33 @@
 #include "../../include/icarion_gpu.h"
 #include "step_params.hpp"
 
+#include <numeric>
+#include <random>
+
+#include "surrogate.hpp"
+
 namespace icarion {
-hipError_t launch_step(int integrator, int field, int coll, int precision, const StepParams& P,
-                       const IonPtrs& S, uint32_t block, hipStream_t stream);
+hipError_t launch_step(int integrator, int field, int coll, int precision, int surrogate,
+                       const StepParams& P, const IonPtrs& S, uint32_t block, hipStream_t stream);
+hipError_t launch_synth_clusters(float4* atoms, const uint32_t* offsets, const uint32_t* counts,
+                                 uint32_t n, float r_min, float r_max, uint64_t seed, hipStream_t s);
+hipError_t launch_oapa(const float4* atoms, const uint32_t* offsets, const uint32_t* counts, uint32_t n,
+                       const double* rot64, double r_gas, double* sigma, hipStream_t s);
+hipError_t launch_features(const float4* atoms, const uint32_t* offsets, const uint32_t* counts,
+                           uint32_t n, double r_gas, const double* sigma, float* f, hipStream_t s);
+hipError_t launch_precompute(const float4* atoms, const uint32_t* offsets, const uint32_t* counts,
+                             uint32_t cluster_begin, uint32_t n_clusters, uint32_t n_orient,
+                             uint32_t n_sigma, uint32_t n_mu, uint32_t max_tries, uint64_t seed,
+                             double r_gas, double* sigma, double* mu, hipStream_t s);
+struct TrainBuffers {
+    float *X, *T;
+    float *Z1, *H1, *Z2, *H2, *Y, *dY, *dH2, *dH1;
+    float *params, *grad, *m, *v, *loss;
+    int B;
+};
+hipError_t mlp_fwd_bwd(const TrainBuffers& t, hipStream_t s);
+hipError_t adam_step(const TrainBuffers& t, float lr, float b1, float b2, float eps, int step, hipStream_t s);
+hipError_t mlp_infer(const uint4* blob, const float* X, float* Y, uint32_t n, hipStream_t s);
 }
 
 using namespace icarion;
@@ -99,8 +123,13 @@ struct icarion_gpu {
     uint32_t n_species = 0;
     std::vector<icarion_species_desc> species_host;
     DevBuf<SpeciesDev> species;
-    DevBuf<double4> atoms;
+    DevBuf<float4> atoms_f32;
+    DevBuf<uint32_t> atom_offsets, atom_counts;   // per species (EHSS tools)
     uint32_t n_atoms = 0;
+
+    bool have_surrogate = false;
+    std::vector<float> params;
+    DevBuf<uint4> mlp_blob;
 
     bool have_field = false;
     DevBuf<float4> fa32;
@@ -282,13 +311,19 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     if (field == FIELD_ARRAY && !h->have_field)
         fail(ICARION_ERR_STATE, "domain uses a field array but none was uploaded");
     const int coll = coll_kind(h->phys);
-    if (coll == COLL_EHSS && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
+    const bool surrogate = coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE;
+    if (surrogate && h->phys.precision != ICARION_PRECISION_FP32_KAHAN)
+        fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs in the FP32 + Kahan path");
+    if (surrogate && !h->have_surrogate)
+        fail(ICARION_ERR_STATE, "EHSS surrogate selected but no parameters set");
+    if (coll == COLL_EHSS && !surrogate && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     set_device(h);
     StepParams P = h->base;
     P.dt = dt;
     P.species = h->species.p;
-    P.atoms = h->atoms.p;
+    P.atoms = h->atoms_f32.p;
+    P.mlp_blob = h->mlp_blob.p;
     P.fa_f32 = h->fa32.p;
     P.fa_f64 = (h->phys.precision == ICARION_PRECISION_FP64) ? h->fa64.p : nullptr;
     P.n_ions = static_cast<uint32_t>(h->n);
@@ -308,8 +343,8 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
             P.table = h->table.p + k0;
             P.step0 = step_offset + k0;
             P.n_steps = static_cast<uint32_t>(nk);
-            const hipError_t e = launch_step(h->phys.integrator, field, coll, h->phys.precision, P,
-                                             S, h->block, h->stream);
+            const hipError_t e = launch_step(h->phys.integrator, field, coll, h->phys.precision,
+                                             surrogate ? 1 : 0, P, S, h->block, h->stream);
             if (e == hipErrorInvalidValue)
                 fail(ICARION_ERR_UNSUPPORTED, "no kernel for this integrator/field/collision/precision");
             hip_check(e, "step kernel launch");
@@ -439,6 +474,161 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
 
 }  // namespace
 
+namespace {
+
+// The 64 orientations of EHSSCollisionHandler::compute_effective_ccs: PhysicsRng(12345)
+// (std::mt19937_64 + uniform_real_distribution) through generate_random_rotation
+// (CollisionGeometry.cpp:30-59).
+std::vector<double> oapa_rotations() {
+    std::mt19937_64 eng(12345);
+    std::uniform_real_distribution<double> uni(0.0, 1.0);
+    std::vector<double> R(64 * 9);
+    for (int k = 0; k < 64; ++k) {
+        const double u1 = uni(eng), u2 = uni(eng), u3 = uni(eng);
+        const double x = std::sqrt(1.0 - u1) * std::sin(2.0 * kPiHost * u2);
+        const double y = std::sqrt(1.0 - u1) * std::cos(2.0 * kPiHost * u2);
+        const double z = std::sqrt(u1) * std::sin(2.0 * kPiHost * u3);
+        const double w = std::sqrt(u1) * std::cos(2.0 * kPiHost * u3);
+        double* m = &R[9 * k];
+        m[0] = 1 - 2 * (y * y + z * z); m[1] = 2 * (x * y - z * w);     m[2] = 2 * (x * z + y * w);
+        m[3] = 2 * (x * y + z * w);     m[4] = 1 - 2 * (x * x + z * z); m[5] = 2 * (y * z - x * w);
+        m[6] = 2 * (x * z - y * w);     m[7] = 2 * (y * z + x * w);     m[8] = 1 - 2 * (x * x + y * y);
+    }
+    return R;
+}
+
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// Pack fp32 parameters into the bf16 MFMA fragment blob of surrogate.hpp.
+std::vector<unsigned char> pack_blob(const std::vector<float>& p) {
+    std::vector<unsigned char> blob(kPackBytes, 0);
+    auto put = [&](size_t byte_off, int j, float v) {
+        const uint16_t b = bf16_rne(v);
+        std::memcpy(&blob[byte_off + 2 * j], &b, 2);
+    };
+    const float* W1 = p.data() + kOffW1;
+    const float* W2 = p.data() + kOffW2;
+    const float* W3 = p.data() + kOffW3;
+    for (int mb = 0; mb < 4; ++mb)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 4; ++j) {
+                const int row = 32 * mb + lane % 32, k = 4 * (lane / 32) + j;
+                put(kPackA1 + (mb * 64 + lane) * 8, j, k < kMlpIn ? W1[row * kMlpIn + k] : 0.f);
+            }
+    for (int mb = 0; mb < 4; ++mb)
+        for (int ks = 0; ks < 16; ++ks)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int j = 0; j < 4; ++j) {
+                    const int row = 32 * mb + lane % 32, k = 8 * ks + 4 * (lane / 32) + j;
+                    put(kPackA2 + ((mb * 16 + ks) * 64 + lane) * 8, j, W2[row * kMlpH + k]);
+                }
+    for (int ks = 0; ks < 16; ++ks)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 4; ++j) {
+                const int row = lane % 32, k = 8 * ks + 4 * (lane / 32) + j;
+                put(kPackA3 + (ks * 64 + lane) * 8, j, row < kMlpOut ? W3[row * kMlpH + k] : 0.f);
+            }
+    std::memcpy(&blob[kPackB1], p.data() + kOffB1, 128 * 4);
+    std::memcpy(&blob[kPackB2], p.data() + kOffB2, 128 * 4);
+    std::memcpy(&blob[kPackB3], p.data() + kOffB3, 3 * 4);
+    return blob;
+}
+
+void install_params(icarion_gpu* h, const float* params) {
+    h->params.assign(params, params + kMlpParams);
+    const std::vector<unsigned char> blob = pack_blob(h->params);
+    set_device(h);
+    copy_in(h->mlp_blob, reinterpret_cast<const uint4*>(blob.data()), kPackBytes / 16, h->stream, "upload surrogate");
+    hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    h->have_surrogate = true;
+}
+
+std::vector<float> init_params(uint64_t seed) {
+    // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases
+    std::mt19937_64 eng(seed);
+    std::vector<float> p(kMlpParams, 0.f);
+    auto fill = [&](int off, int n, int fan_in) {
+        std::uniform_real_distribution<float> u(-1.f / std::sqrt((float)fan_in), 1.f / std::sqrt((float)fan_in));
+        for (int i = 0; i < n; ++i) p[off + i] = u(eng);
+    };
+    fill(kOffW1, kMlpH * kMlpIn, kMlpIn);
+    fill(kOffW2, kMlpH * kMlpH, kMlpH);
+    fill(kOffW3, kMlpOut * kMlpH, kMlpH);
+    return p;
+}
+
+__global__ void gather_batch_kernel(const float* X, const float* T, const uint32_t* perm, const uint32_t* ctr,
+                                    float* Xb, float* Tb, int B) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= B) return;
+    const size_t src = perm[static_cast<size_t>(ctr[0]) * B + r];
+    for (int j = 0; j < kMlpIn; ++j) Xb[r * kMlpIn + j] = X[src * kMlpIn + j];
+    for (int j = 0; j < kMlpOut; ++j) Tb[r * kMlpOut + j] = T[src * kMlpOut + j];
+}
+
+__global__ void adam_graph_kernel(float* p, const float* g, float* m, float* v, int n, float lr, float b1,
+                                  float b2, float eps, const uint32_t* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float step = static_cast<float>(ctr[1] + 1);
+    const float c1 = 1.f / (1.f - powf(b1, step));
+    const float c2 = 1.f / (1.f - powf(b2, step));
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ void advance_counters_kernel(uint32_t* ctr) {
+    if (threadIdx.x == 0) {
+        ctr[0] += 1;
+        ctr[1] += 1;
+    }
+}
+
+__global__ void zero_f32_kernel(float* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+    const int i = threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+struct TrainDev {
+    DevBuf<float> Z1, H1, Z2, H2, Y, dY, dH2, dH1, params, grad, m, v, loss, Xb, Tb;
+    void alloc(int B) {
+        Z1.resize(static_cast<size_t>(B) * kMlpH); H1.resize(static_cast<size_t>(B) * kMlpH);
+        Z2.resize(static_cast<size_t>(B) * kMlpH); H2.resize(static_cast<size_t>(B) * kMlpH);
+        Y.resize(static_cast<size_t>(B) * kMlpOut); dY.resize(static_cast<size_t>(B) * kMlpOut);
+        dH2.resize(static_cast<size_t>(B) * kMlpH); dH1.resize(static_cast<size_t>(B) * kMlpH);
+        params.resize(kMlpParams); grad.resize(kMlpParams); m.resize(kMlpParams); v.resize(kMlpParams);
+        loss.resize(1);
+        Xb.resize(static_cast<size_t>(B) * kMlpIn); Tb.resize(static_cast<size_t>(B) * kMlpOut);
+    }
+    TrainBuffers view(float* X, float* T, int B) {
+        TrainBuffers t{};
+        t.X = X; t.T = T; t.Z1 = Z1.p; t.H1 = H1.p; t.Z2 = Z2.p; t.H2 = H2.p; t.Y = Y.p; t.dY = dY.p;
+        t.dH2 = dH2.p; t.dH1 = dH1.p; t.params = params.p; t.grad = grad.p; t.m = m.p; t.v = v.p;
+        t.loss = loss.p; t.B = B;
+        return t;
+    }
+    void release() {
+        for (DevBuf<float>* b : {&Z1, &H1, &Z2, &H2, &Y, &dY, &dH2, &dH1, &params, &grad, &m, &v, &loss, &Xb, &Tb})
+            b->release();
+    }
+};
+
+}  // namespace
+
 extern "C" {
 
 int icarion_gpu_abi_version(void) { return ICARION_GPU_ABI_VERSION; }
@@ -492,7 +682,8 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     if (!h) return ICARION_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    h->species.release(); h->atoms.release(); h->fa32.release(); h->fa64.release();
+    h->species.release(); h->atoms_f32.release(); h->atom_offsets.release(); h->atom_counts.release();
+    h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
     h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
@@ -516,11 +707,13 @@ int icarion_gpu_set_domain(icarion_gpu* h, const icarion_domain_desc* domain,
 }
 
 int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species, uint32_t n_species,
-                            const double* atoms_xyzr, uint32_t n_atoms) {
+                            const void* atoms_xyzr, uint32_t n_atoms, int32_t atoms_dtype) {
     return guarded([&] {
         if (!h || !species || n_species == 0) fail(ICARION_ERR_INVALID, "species table is empty");
         if (n_atoms && !atoms_xyzr) fail(ICARION_ERR_INVALID, "atoms_xyzr is NULL");
+        if (atoms_dtype != 0 && atoms_dtype != 1) fail(ICARION_ERR_INVALID, "atoms_dtype must be 0 or 1");
         std::vector<SpeciesDev> sd(n_species);
+        std::vector<uint32_t> offs(n_species), cnts(n_species);
         for (uint32_t s = 0; s < n_species; ++s) {
             const icarion_species_desc& d = species[s];
             if (static_cast<uint64_t>(d.atom_offset) + d.atom_count > n_atoms)
@@ -530,14 +723,29 @@ int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species,
             sd[s].K0_cm2_Vs = d.K0_cm2_Vs;
             sd[s].atom_offset = d.atom_offset;
             sd[s].atom_count = d.atom_count;
+            for (int f = 0; f < 3; ++f) sd[s].feat[f] = d.surrogate_features[f];
+            sd[s].pad = 0.f;
+            offs[s] = d.atom_offset;
+            cnts[s] = d.atom_count;
         }
         set_device(h);
         copy_in(h->species, sd.data(), n_species, h->stream, "upload species");
-        std::vector<double4> at(n_atoms);
-        for (uint32_t a = 0; a < n_atoms; ++a)
-            at[a] = make_double4(atoms_xyzr[4 * a], atoms_xyzr[4 * a + 1], atoms_xyzr[4 * a + 2],
-                                 atoms_xyzr[4 * a + 3]);
-        if (n_atoms) copy_in(h->atoms, at.data(), n_atoms, h->stream, "upload atoms");
+        copy_in(h->atom_offsets, offs.data(), n_species, h->stream, "upload atom offsets");
+        copy_in(h->atom_counts, cnts.data(), n_species, h->stream, "upload atom counts");
+        // The device holds one FP32 atom table; a double table is rounded to
+        // float here (the staging vector lives until the copy has finished).
+        h->atoms_f32.release();
+        std::vector<float4> at;
+        if (n_atoms && atoms_dtype == 1) {
+            const double* a = static_cast<const double*>(atoms_xyzr);
+            at.resize(n_atoms);
+            for (uint32_t i = 0; i < n_atoms; ++i)
+                at[i] = make_float4(static_cast<float>(a[4 * i]), static_cast<float>(a[4 * i + 1]),
+                                    static_cast<float>(a[4 * i + 2]), static_cast<float>(a[4 * i + 3]));
+            copy_in(h->atoms_f32, at.data(), n_atoms, h->stream, "upload atoms");
+        } else if (n_atoms) {
+            copy_in(h->atoms_f32, static_cast<const float4*>(atoms_xyzr), n_atoms, h->stream, "upload atoms");
+        }
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         h->species_host.assign(species, species + n_species);
         h->n_species = n_species;
@@ -606,6 +814,231 @@ int icarion_gpu_download_state(icarion_gpu* h, icarion_ion_soa* ions) {
         do_download(h, ions);
     });
 }
+
+
+int icarion_gpu_synth_clusters(icarion_gpu* h, uint32_t n_clusters, const uint32_t* counts, double r_min_m,
+                               double r_max_m, uint64_t seed, float* atoms_out, double* sigma_out,
+                               float* features_out) {
+    return guarded([&] {
+        if (!h || !counts || n_clusters == 0) fail(ICARION_ERR_INVALID, "no clusters requested");
+        if (!h->have_domain) fail(ICARION_ERR_STATE, "icarion_gpu_set_domain must precede synth_clusters");
+        if (!(r_min_m > 0.0 && r_max_m >= r_min_m)) fail(ICARION_ERR_INVALID, "atom radii must satisfy 0 < r_min <= r_max");
+        std::vector<uint32_t> offs(n_clusters);
+        uint64_t total = 0;
+        for (uint32_t c = 0; c < n_clusters; ++c) {
+            if (counts[c] == 0) fail(ICARION_ERR_INVALID, "a cluster needs at least one atom");
+            offs[c] = static_cast<uint32_t>(total);
+            total += counts[c];
+            if (total > 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, "at most 2^32-1 atoms in total");
+        }
+        set_device(h);
+        hipStream_t s = h->stream;
+        h->atoms_f32.resize(total);
+        copy_in(h->atom_offsets, offs.data(), n_clusters, s, "upload offsets");
+        copy_in(h->atom_counts, counts, n_clusters, s, "upload counts");
+        hip_check(launch_synth_clusters(h->atoms_f32.p, h->atom_offsets.p, h->atom_counts.p, n_clusters,
+                                        static_cast<float>(r_min_m), static_cast<float>(r_max_m), seed, s),
+                  "synth_clusters");
+        const std::vector<double> rot = oapa_rotations();
+        DevBuf<double> rot_d, sig_d;
+        DevBuf<float> feat_d;
+        copy_in(rot_d, rot.data(), rot.size(), s, "upload rotations");
+        sig_d.resize(n_clusters);
+        feat_d.resize(3ull * n_clusters);
+        const double r_gas = h->dom.gas_radius_m;
+        hip_check(launch_oapa(h->atoms_f32.p, h->atom_offsets.p, h->atom_counts.p, n_clusters, rot_d.p, r_gas,
+                              sig_d.p, s), "oapa");
+        hip_check(launch_features(h->atoms_f32.p, h->atom_offsets.p, h->atom_counts.p, n_clusters, r_gas,
+                                  sig_d.p, feat_d.p, s), "features");
+        std::vector<double> sig(n_clusters);
+        std::vector<float> feat(3ull * n_clusters);
+        copy_out(sig.data(), sig_d, n_clusters, s, "download sigma");
+        copy_out(feat.data(), feat_d, 3ull * n_clusters, s, "download features");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        std::vector<SpeciesDev> sd(n_clusters);
+        std::vector<icarion_species_desc> sh(n_clusters);
+        for (uint32_t c = 0; c < n_clusters; ++c) {
+            sd[c] = SpeciesDev{};
+            sd[c].sigma_ehss = sig[c];
+            sd[c].atom_offset = offs[c];
+            sd[c].atom_count = counts[c];
+            for (int f = 0; f < 3; ++f) sd[c].feat[f] = feat[3ull * c + f];
+            sh[c] = icarion_species_desc{};
+            sh[c].sigma_ehss_m2 = sig[c];
+            sh[c].atom_offset = offs[c];
+            sh[c].atom_count = counts[c];
+            for (int f = 0; f < 3; ++f) sh[c].surrogate_features[f] = feat[3ull * c + f];
+        }
+        copy_in(h->species, sd.data(), n_clusters, s, "upload species");
+        if (atoms_out) copy_out(reinterpret_cast<float4*>(atoms_out), h->atoms_f32, total, s, "download atoms");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (sigma_out) std::memcpy(sigma_out, sig.data(), sizeof(double) * n_clusters);
+        if (features_out) std::memcpy(features_out, feat.data(), sizeof(float) * 3 * n_clusters);
+        h->species_host = std::move(sh);
+        h->n_species = n_clusters;
+        h->n_atoms = static_cast<uint32_t>(total);
+        h->have_species = true;
+    });
+}
+
+int icarion_gpu_ehss_precompute(icarion_gpu* h, uint32_t cluster_begin, uint32_t n_clusters, uint32_t n_orient,
+                                uint32_t n_sigma, uint32_t n_mu, uint32_t max_tries, uint64_t seed,
+                                double* sigma_out, double* mu_out) {
+    return guarded([&] {
+        if (!h || !sigma_out || (n_mu && !mu_out)) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_domain || !h->have_species || !h->atoms_f32.p)
+            fail(ICARION_ERR_STATE, "needs a domain and a float atom table");
+        if (static_cast<uint64_t>(cluster_begin) + n_clusters > h->n_species)
+            fail(ICARION_ERR_INVALID, "cluster range exceeds the species table");
+        if (n_orient == 0) fail(ICARION_ERR_INVALID, "n_orient must be > 0");
+        set_device(h);
+        hipStream_t s = h->stream;
+        const uint64_t nt = static_cast<uint64_t>(n_clusters) * n_orient;
+        DevBuf<double> sig, mu;
+        sig.resize(nt);
+        mu.resize(std::max<uint64_t>(nt * n_mu, 1));
+        hip_check(launch_precompute(h->atoms_f32.p, h->atom_offsets.p, h->atom_counts.p, cluster_begin, n_clusters,
+                                    n_orient, n_sigma, n_mu, max_tries, seed, h->dom.gas_radius_m, sig.p, mu.p, s),
+                  "precompute");
+        copy_out(sigma_out, sig, nt, s, "download sigma");
+        if (n_mu) copy_out(mu_out, mu, nt * n_mu, s, "download mu");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    });
+}
+
+int icarion_gpu_surrogate_set_params(icarion_gpu* h, const float* params, uint32_t n_params) {
+    return guarded([&] {
+        if (!h || !params) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (n_params != static_cast<uint32_t>(kMlpParams)) fail(ICARION_ERR_INVALID, "wrong parameter count");
+        install_params(h, params);
+    });
+}
+
+int icarion_gpu_surrogate_get_params(icarion_gpu* h, float* params, uint32_t n_params) {
+    return guarded([&] {
+        if (!h || !params) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (n_params != static_cast<uint32_t>(kMlpParams)) fail(ICARION_ERR_INVALID, "wrong parameter count");
+        if (!h->have_surrogate) fail(ICARION_ERR_STATE, "no surrogate parameters");
+        std::memcpy(params, h->params.data(), sizeof(float) * kMlpParams);
+    });
+}
+
+int icarion_gpu_surrogate_train(icarion_gpu* h, const float* X, const float* T, size_t n,
+                                const icarion_train_desc* desc, float* loss_per_epoch) {
+    return guarded([&] {
+        if (!h || !X || !T || !desc) fail(ICARION_ERR_INVALID, "NULL argument");
+        const int B = static_cast<int>(desc->batch);
+        if (B <= 0 || static_cast<size_t>(B) > n) fail(ICARION_ERR_INVALID, "batch must be in [1, n]");
+        if (n > 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, "at most 2^32-1 samples");
+        set_device(h);
+        hipStream_t s = h->stream;
+        if (!h->have_surrogate) {
+            const std::vector<float> p0 = init_params(desc->seed);
+            h->params = p0;
+        }
+        DevBuf<float> Xd, Td;
+        copy_in(Xd, X, n * kMlpIn, s, "upload X");
+        copy_in(Td, T, n * kMlpOut, s, "upload T");
+        TrainDev td;
+        td.alloc(B);
+        copy_in(td.params, h->params.data(), kMlpParams, s, "upload params");
+        hipLaunchKernelGGL(zero_f32_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, td.m.p, kMlpParams);
+        hipLaunchKernelGGL(zero_f32_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, td.v.p, kMlpParams);
+        DevBuf<uint32_t> ctr, perm_d;
+        ctr.resize(2);
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctr.p, 2);
+        const size_t n_batches = n / static_cast<size_t>(B);
+        perm_d.resize(n_batches * B);
+        const TrainBuffers tb = td.view(td.Xb.p, td.Tb.p, B);
+        // one training step, captured once as a HIP graph and replayed: the batch
+        // offset and the Adam step count live in device counters
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipLaunchKernelGGL(gather_batch_kernel, dim3((B + 255) / 256), dim3(256), 0, s, Xd.p, Td.p, perm_d.p, ctr.p,
+                           td.Xb.p, td.Tb.p, B);
+        hipError_t ce = mlp_fwd_bwd(tb, s);
+        hipLaunchKernelGGL(adam_graph_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, td.params.p, td.grad.p,
+                           td.m.p, td.v.p, kMlpParams, desc->lr, desc->beta1, desc->beta2, desc->eps, ctr.p);
+        hipLaunchKernelGGL(advance_counters_kernel, dim3(1), dim3(64), 0, s, ctr.p);
+        const hipError_t ee = hipStreamEndCapture(s, &graph);
+        hip_check(ce, "capture training step");
+        hip_check(ee, "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        std::mt19937_64 eng(desc->seed ^ 0x5deece66dull);
+        std::vector<uint32_t> perm(n);
+        std::iota(perm.begin(), perm.end(), 0u);
+        try {
+            for (uint32_t ep = 0; ep < desc->epochs; ++ep) {
+                std::shuffle(perm.begin(), perm.end(), eng);
+                hip_check(hipMemcpyAsync(perm_d.p, perm.data(), sizeof(uint32_t) * n_batches * B,
+                                         hipMemcpyHostToDevice, s), "upload permutation");
+                hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, td.loss.p, 1);
+                hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctr.p, 1);  // batch index only
+                for (size_t b = 0; b < n_batches; ++b) hip_check(hipGraphLaunch(exec, s), "hipGraphLaunch");
+                float l = 0.f;
+                hip_check(hipMemcpyAsync(&l, td.loss.p, sizeof(float), hipMemcpyDeviceToHost, s), "read loss");
+                hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+                if (loss_per_epoch) loss_per_epoch[ep] = l / static_cast<float>(n_batches);
+            }
+        } catch (...) {
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            throw;
+        }
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        std::vector<float> p(kMlpParams);
+        copy_out(p.data(), td.params, kMlpParams, s, "download params");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        td.release();
+        install_params(h, p.data());
+    });
+}
+
+int icarion_gpu_surrogate_grad(icarion_gpu* h, const float* X, const float* T, size_t n, float* grad_out,
+                               float* loss_out) {
+    return guarded([&] {
+        if (!h || !X || !T || !grad_out || n == 0) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_surrogate) fail(ICARION_ERR_STATE, "no surrogate parameters");
+        if (n > (1u << 24)) fail(ICARION_ERR_INVALID, "at most 2^24 samples per gradient call");
+        set_device(h);
+        hipStream_t s = h->stream;
+        const int B = static_cast<int>(n);
+        TrainDev td;
+        td.alloc(B);
+        copy_in(td.params, h->params.data(), kMlpParams, s, "upload params");
+        copy_in(td.Xb, X, n * kMlpIn, s, "upload X");
+        copy_in(td.Tb, T, n * kMlpOut, s, "upload T");
+        hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, td.loss.p, 1);
+        hip_check(mlp_fwd_bwd(td.view(td.Xb.p, td.Tb.p, B), s), "fwd/bwd");
+        copy_out(grad_out, td.grad, kMlpParams, s, "download grad");
+        float l = 0.f;
+        hip_check(hipMemcpyAsync(&l, td.loss.p, sizeof(float), hipMemcpyDeviceToHost, s), "read loss");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (loss_out) *loss_out = l;
+        td.release();
+    });
+}
+
+int icarion_gpu_surrogate_infer(icarion_gpu* h, const float* X, size_t n, float* Y) {
+    return guarded([&] {
+        if (!h || !X || !Y) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_surrogate) fail(ICARION_ERR_STATE, "no surrogate parameters");
+        if (n == 0) return;
+        if (n > 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, "too many samples");
+        set_device(h);
+        hipStream_t s = h->stream;
+        DevBuf<float> Xd, Yd;
+        copy_in(Xd, X, n * kMlpIn, s, "upload X");
+        Yd.resize(n * kMlpOut);
+        hip_check(mlp_infer(h->mlp_blob.p, Xd.p, Yd.p, static_cast<uint32_t>(n), s), "mlp_infer");
+        copy_out(Y, Yd, n * kMlpOut, s, "download Y");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    });
+}
+
 
 int icarion_gpu_synchronize(icarion_gpu* h) {
     return guarded([&] {

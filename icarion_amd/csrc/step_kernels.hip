@@ -15,6 +15,7 @@
 // reduced per wavefront before one atomic per wave.
 #include "device_common.hpp"
 #include "step_params.hpp"
+#include "surrogate.hpp"
 
 namespace icarion {
 
@@ -195,10 +196,13 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             Rm[3 * i + j] = Bm[i][0] * O[j] + Bm[i][1] * O[3 + j] + Bm[i][2] * O[6 + j];
-    const double4* atoms = P.atoms + a0;
+    auto atom = [&](uint32_t j) -> double4 {
+        const float4 f = P.atoms[a0 + j];
+        return make_double4(f.x, f.y, f.z, f.w);
+    };
     double b_max = 0.0;
     for (uint32_t j = 0; j < nat; ++j) {
-        const double4 c = atoms[j];
+        const double4 c = atom(j);
         const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
         const double sra = dot(ax, ra);
         const d3 rp = ra - ax * sra;
@@ -217,7 +221,7 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
         d3 nb{0, 0, 0};
         double best_s = 0.0;
         for (uint32_t j = 0; j < nat; ++j) {
-            const double4 c = atoms[j];
+            const double4 c = atom(j);
             const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
             const double Rsum = c.w + P.r_gas;
             const double Rsum2 = Rsum * Rsum;
@@ -818,6 +822,211 @@ __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const
     add_counters(P, pushed, ncoll_launch);
 }
 
+// ======================= FP32 + EHSS surrogate ==============================
+//
+// EHSS with the scattering normal drawn by the surrogate MLP instead of the
+// geometry ray test.  The collision test is the reference's (neutral from the
+// Maxwellian, P = 1 - exp(-n sigma_eff |v_rel| dt), EHSSCollisionHandler.cpp:222-246);
+// a colliding ion draws (u, psi), the MLP maps (u, cos psi, sin psi, features)
+// to the contact normal in the collision frame, and the relative velocity is
+// reflected about it (offline-table branch, EHSSCollisionHandler.cpp:253-277).
+// Every lane stays in the step loop (dead lanes are masked, not exited) so the
+// wave-wide matrix instructions always see the whole wavefront.
+template <int INTEG, int FIELD>
+__global__ void __launch_bounds__(256) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
+    __shared__ uint4 mlp_lds[kPackBytes / 16];
+    mlp_stage(P.mlp_blob, mlp_lds);
+    const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < P.n_ions;
+    const uint32_t ii = valid ? i : 0;
+    FastConst C;
+    C.Ez = (float)P.Ez_const;
+    C.orbi_k = (float)P.orbi_k;
+    C.orbi_hk = (float)(0.5 * P.orbi_k);
+    C.R_char_sq = (float)P.R_char_sq;
+    C.ox = (float)P.origin[0]; C.oy = (float)P.origin[1]; C.oz = (float)P.origin[2];
+    C.u_gas[0] = (float)P.u_gas[0]; C.u_gas[1] = (float)P.u_gas[1]; C.u_gas[2] = (float)P.u_gas[2];
+    C.sigma_neutral = (float)P.sigma_neutral;
+    C.n_dt = (float)(P.n * P.dt);
+    C.m_gas = (float)P.m_gas;
+    C.L = (float)P.L;
+    C.half_L = (float)(0.5 * P.L);
+    C.R_eps_sq = (float)(P.R * P.R);
+    C.R_in_sq = (float)(P.R_in * P.R_in);
+    C.R_out = (float)P.R_out;
+    C.R_out_sq = (float)(P.R_out * P.R_out);
+    C.ln_R_in = (float)P.ln_R_in;
+    C.ln_R_out = (float)P.ln_R_out;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        C.g2l[c] = (float)P.g2l[c];
+        C.l2g[c] = (float)P.l2g[c];
+    }
+    const double X[6] = {S.x[ii], S.y[ii], S.z[ii], S.vx[ii], S.vy[ii], S.vz[ii]};
+    float hi[6], lo[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        hi[c] = (float)X[c];
+        lo[c] = (float)(X[c] - (double)hi[c]);
+    }
+    const double md = S.mass[ii];
+    const double qd = S.charge[ii];
+    const SpeciesDev sp = P.species[S.species[ii]];
+    const float sigma = (float)sp.sigma_ehss;
+    const float qm = (float)(qd / md);
+    const float m = (float)md;
+    const float inv_m_plus = (float)(1.0 / (md + P.m_gas));
+    uint8_t flags = S.flags[ii];
+    bool active = valid && (flags & 1), born = valid && ((flags >> 1) & 1);
+    const double birth = S.birth ? S.birth[ii] : -INFINITY;
+    double death = S.death[ii];
+    double arrival = S.arrival[ii];
+    uint32_t nc = S.ncoll[ii];
+    uint32_t pushed = 0, ncoll_launch = 0;
+    bool checked = false;
+    const float dt = (float)P.dt;
+    const float half_dt = 0.5f * dt;
+    const float h6 = dt * (1.0f / 6.0f);
+    for (uint32_t k = 0; k < P.n_steps; ++k) {
+        const StepEntry te = P.table[k];
+        const double t = te.t;
+        if (valid && !born && t >= birth) {
+            born = true;
+            active = true;
+            checked = false;
+        }
+        bool live = active && born;
+        if (live && !checked && !contains_f32(P, C, hi[0], hi[1], hi[2])) {
+            active = false;
+            death = t;
+            live = false;
+        }
+        bool coll = false;
+        float in[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        f3 vn{0.f, 0.f, 0.f}, vrel{0.f, 0.f, 0.f};
+        float vm = 0.f;
+        float z_before = hi[2], z_before_lo = lo[2];
+        if (live) {
+            const f3 p0{hi[0], hi[1], hi[2]};
+            const f3 v0{hi[3], hi[4], hi[5]};
+            f3 dpos, dvel;
+            if constexpr (INTEG == 0) {
+                const f3 k1a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, (float)te.c0, qm, 0.f);
+                const f3 k2v = v0 + k1a * half_dt;
+                const f3 k2a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + v0 * half_dt, k2v, (float)te.ch, qm, 0.f);
+                const f3 k3v = v0 + k2a * half_dt;
+                const f3 k3a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k2v * half_dt, k3v, (float)te.ch, qm, 0.f);
+                const f3 k4v = v0 + k3a * dt;
+                const f3 k4a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k3v * dt, k4v, (float)te.c1, qm, 0.f);
+                dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
+                dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
+            } else {
+                const f3 a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, (float)te.c0, qm, 0.f);
+                const f3 vmid = v0 + a * half_dt;
+                f3 rot{0.0f, 0.0f, 0.0f};
+                if (P.magnetic) {
+                    const f3 tv = f3{(float)P.B[0], (float)P.B[1], (float)P.B[2]} * (qm * half_dt);
+                    const float s_f = 2.0f / (1.0f + dot(tv, tv));
+                    const f3 vprime = vmid + cross(vmid, tv);
+                    rot = cross(vprime, tv * s_f);
+                }
+                dpos = (vmid + rot) * dt;
+                dvel = a * dt + rot;
+            }
+            kahan_add(hi[0], lo[0], dpos.x);
+            kahan_add(hi[1], lo[1], dpos.y);
+            kahan_add(hi[2], lo[2], dpos.z);
+            kahan_add(hi[3], lo[3], dvel.x);
+            kahan_add(hi[4], lo[4], dvel.y);
+            kahan_add(hi[5], lo[5], dvel.z);
+            ++pushed;
+            if (sigma > 0.0f) {
+                PhiloxStream rng(P.seed, i, P.step0 + k);
+                const float u1 = rng.uniform_f32(), u2 = rng.uniform_f32();
+                const float u3 = rng.uniform_f32(), u4 = rng.uniform_f32();
+                const float r1 = C.sigma_neutral * fast_sqrt(-2.0f * fast_ln(u1));
+                const float r2 = C.sigma_neutral * fast_sqrt(-2.0f * fast_ln(u3));
+                vn = f3{r1 * fast_cos_rev(u2) + C.u_gas[0], r1 * fast_sin_rev(u2) + C.u_gas[1],
+                        r2 * fast_cos_rev(u4) + C.u_gas[2]};
+                const f3 v{hi[3], hi[4], hi[5]};
+                vrel = v - vn;
+                vm = fast_sqrt(dot(vrel, vrel));
+                if (vm >= (float)kMinRelativeVelocity) {
+                    const float x = C.n_dt * sigma * vm;
+                    const float Pc = x < 1e-3f ? x * (1.0f - 0.5f * x * (1.0f - x * (1.0f / 3.0f)))
+                                               : 1.0f - __builtin_amdgcn_exp2f(-x * 1.44269504089f);
+                    if (rng.uniform_f32() < Pc) {
+                        coll = true;
+                        const float uq = rng.uniform_f32();
+                        const float psi = rng.uniform_f32();  // revolutions
+                        in[0] = uq;
+                        in[1] = fast_cos_rev(psi);
+                        in[2] = fast_sin_rev(psi);
+                        in[3] = sp.feat[0];
+                        in[4] = sp.feat[1];
+                        in[5] = sp.feat[2];
+                    }
+                }
+            }
+        }
+        if (__any(coll)) {
+            float out[3];
+            mlp_wave_eval(lds, in, out);
+            if (coll) {
+                const float nrm = fast_sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+                float mu = nrm > 0.f ? -out[2] / nrm : 0.f;
+                mu = fminf(1.f, fmaxf(0.f, mu));
+                const float st = fast_sqrt(fmaxf(0.f, 1.f - mu * mu));
+                const float inv = 1.0f / vm;
+                const f3 ax{vrel.x * inv, vrel.y * inv, vrel.z * inv};
+                const f3 a = fabsf(ax.x) < 0.9f ? f3{1.f, 0.f, 0.f} : f3{0.f, 1.f, 0.f};
+                f3 t1 = cross(ax, a);
+                t1 = t1 * (1.0f / fast_sqrt(dot(t1, t1)));
+                f3 t2 = cross(ax, t1);
+                t2 = t2 * (1.0f / fast_sqrt(dot(t2, t2)));
+                const f3 nvec = t1 * (st * in[1]) + t2 * (st * in[2]) + ax * (-mu);
+                const f3 vr = vrel - nvec * (2.0f * dot(vrel, nvec));
+                const f3 v{hi[3], hi[4], hi[5]};
+                const f3 Vcom = (v * m + vn * C.m_gas) * inv_m_plus;
+                const f3 vnew = Vcom + vr * (C.m_gas * inv_m_plus);
+                hi[3] = vnew.x; hi[4] = vnew.y; hi[5] = vnew.z;
+                lo[3] = 0.f; lo[4] = 0.f; lo[5] = 0.f;
+                ++nc;
+                ++ncoll_launch;
+            }
+        }
+        if (live) {
+            if (P.detect && isnan(arrival)) {
+                const double zb = (double)z_before + (double)z_before_lo;
+                const double za = (double)hi[2] + (double)lo[2];
+                if (zb < P.det_z && za >= P.det_z) arrival = t + P.dt * (P.det_z - zb) / (za - zb);
+            }
+            if (!contains_f32(P, C, hi[0], hi[1], hi[2]) ||
+                !(isfinite(hi[0]) && isfinite(hi[1]) && isfinite(hi[2]) && isfinite(hi[3]) &&
+                  isfinite(hi[4]) && isfinite(hi[5]))) {
+                active = false;
+                death = t;
+            } else {
+                checked = true;
+            }
+        }
+    }
+    if (valid) {
+        S.x[i] = (double)hi[0] + (double)lo[0];
+        S.y[i] = (double)hi[1] + (double)lo[1];
+        S.z[i] = (double)hi[2] + (double)lo[2];
+        S.vx[i] = (double)hi[3] + (double)lo[3];
+        S.vy[i] = (double)hi[4] + (double)lo[4];
+        S.vz[i] = (double)hi[5] + (double)lo[5];
+        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+        S.death[i] = death;
+        S.arrival[i] = arrival;
+        S.ncoll[i] = nc;
+    }
+    add_counters(P, pushed, ncoll_launch);
+}
+
 // ---- launch dispatch --------------------------------------------------------
 
 using KernelFn = void (*)(const StepParams, const IonPtrs);
@@ -846,12 +1055,26 @@ KernelFn pick_field(int field, int coll, int precision) {
     }
 }
 
+template <int I>
+KernelFn pick_surrogate(int field) {
+    switch (field) {
+        case FIELD_ORBITRAP: return step_surrogate_kernel<I, FIELD_ORBITRAP>;
+        case FIELD_ARRAY: return step_surrogate_kernel<I, FIELD_ARRAY>;
+        default: return step_surrogate_kernel<I, FIELD_LINEAR>;
+    }
+}
+
 // Returns hipSuccess, or hipErrorInvalidValue when the combination has no
 // kernel (the host maps that to ICARION_ERR_UNSUPPORTED).
-hipError_t launch_step(int integrator, int field, int coll, int precision, const StepParams& P,
-                       const IonPtrs& S, uint32_t block, hipStream_t stream) {
-    const KernelFn fn = integrator == 1 ? pick_field<1>(field, coll, precision)
-                                        : pick_field<0>(field, coll, precision);
+hipError_t launch_step(int integrator, int field, int coll, int precision, int surrogate,
+                       const StepParams& P, const IonPtrs& S, uint32_t block, hipStream_t stream) {
+    KernelFn fn;
+    if (surrogate) {
+        fn = integrator == 1 ? pick_surrogate<1>(field) : pick_surrogate<0>(field);
+    } else {
+        fn = integrator == 1 ? pick_field<1>(field, coll, precision)
+                             : pick_field<0>(field, coll, precision);
+    }
     if (!fn) return hipErrorInvalidValue;
     const uint32_t grid = (P.n_ions + block - 1) / block;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, stream, P, S);

@@ -28,6 +28,8 @@ struct SpeciesDev {
     double K0_cm2_Vs;
     uint32_t atom_offset;
     uint32_t atom_count;
+    float feat[3];           // surrogate cluster features
+    float pad;
 };
 
 // Per-step uniform values: engine time and the RF coefficient
@@ -93,7 +95,8 @@ struct StepParams {
     double x_last, y_last, z_last;
     // species / atoms
     const SpeciesDev *species;
-    const double4 *atoms;    // x, y, z, r
+    const float4 *atoms;     // x, y, z, r of every EHSS atom, FP32
+    const uint4 *mlp_blob;   // packed bf16 surrogate weights (surrogate.hpp)
     // per-step table, entries [0, n_steps]
     const StepEntry *table;
     // run

@@ -18,7 +18,9 @@ EXPORTED = (
     "icarion_gpu_create", "icarion_gpu_destroy", "icarion_gpu_set_domain",
     "icarion_gpu_set_species", "icarion_gpu_upload_field_array", "icarion_gpu_upload_state",
     "icarion_gpu_step", "icarion_gpu_download_state", "icarion_gpu_step_batch",
-    "icarion_gpu_synchronize",
+    "icarion_gpu_synchronize", "icarion_gpu_synth_clusters", "icarion_gpu_ehss_precompute",
+    "icarion_gpu_surrogate_set_params", "icarion_gpu_surrogate_get_params",
+    "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
 )
 
 
@@ -47,7 +49,19 @@ def load(path=None):
     lib.icarion_gpu_destroy.argtypes = [C.c_void_p]
     lib.icarion_gpu_set_domain.argtypes = [C.c_void_p, P(_abi.DomainDesc), P(_abi.PhysicsDesc)]
     lib.icarion_gpu_set_species.argtypes = [C.c_void_p, P(_abi.SpeciesDesc), C.c_uint32,
-                                            P(C.c_double), C.c_uint32]
+                                            C.c_void_p, C.c_uint32, C.c_int32]
+    f32p, f64p = P(C.c_float), P(C.c_double)
+    lib.icarion_gpu_synth_clusters.argtypes = [C.c_void_p, C.c_uint32, P(C.c_uint32), C.c_double,
+                                               C.c_double, C.c_uint64, f32p, f64p, f32p]
+    lib.icarion_gpu_ehss_precompute.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                                f64p, f64p]
+    lib.icarion_gpu_surrogate_set_params.argtypes = [C.c_void_p, f32p, C.c_uint32]
+    lib.icarion_gpu_surrogate_get_params.argtypes = [C.c_void_p, f32p, C.c_uint32]
+    lib.icarion_gpu_surrogate_train.argtypes = [C.c_void_p, f32p, f32p, C.c_size_t,
+                                                P(_abi.TrainDesc), f32p]
+    lib.icarion_gpu_surrogate_grad.argtypes = [C.c_void_p, f32p, f32p, C.c_size_t, f32p, f32p]
+    lib.icarion_gpu_surrogate_infer.argtypes = [C.c_void_p, f32p, C.c_size_t, f32p]
     lib.icarion_gpu_upload_field_array.argtypes = [
         C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
         C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_int32]
@@ -122,17 +136,90 @@ class Handle:
                self.lib)
 
     def set_species(self, species_list, atoms_xyzr=None):
+        """atoms_xyzr: (n_atoms, 4) float32 or float64 array (x, y, z, r in metres)."""
         arr = species_array(species_list)
+        dtype = 1
         if atoms_xyzr is None or len(atoms_xyzr) == 0:
-            atoms = C.POINTER(C.c_double)()
-            n_atoms = 0
+            atoms, n_atoms = None, 0
         else:
-            a = np.ascontiguousarray(atoms_xyzr, dtype=np.float64).reshape(-1, 4)
+            a = np.asarray(atoms_xyzr)
+            dtype = 0 if a.dtype == np.float32 else 1
+            a = np.ascontiguousarray(a, dtype=np.float32 if dtype == 0 else np.float64)
+            a = a.reshape(-1, 4)
             self._keep.append(a)
-            atoms = a.ctypes.data_as(C.POINTER(C.c_double))
-            n_atoms = a.shape[0]
-        _check(self.lib.icarion_gpu_set_species(self.h, arr, len(species_list), atoms, n_atoms),
+            atoms, n_atoms = a.ctypes.data_as(C.c_void_p), a.shape[0]
+        _check(self.lib.icarion_gpu_set_species(self.h, arr, len(species_list), atoms, n_atoms,
+                                                dtype), self.lib)
+
+    def synth_clusters(self, counts, r_min=1.5e-10, r_max=2.0e-10, seed=0, want_atoms=False):
+        """Random rigid clusters installed as the species table; returns
+        (atoms or None, sigma_oapa, features)."""
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        n = len(counts)
+        sig = np.zeros(n, np.float64)
+        feat = np.zeros((n, 3), np.float32)
+        atoms = np.zeros((int(counts.sum()), 4), np.float32) if want_atoms else None
+        f32p = C.POINTER(C.c_float)
+        rc = self.lib.icarion_gpu_synth_clusters(
+            self.h, n, counts.ctypes.data_as(C.POINTER(C.c_uint32)), r_min, r_max, seed,
+            atoms.ctypes.data_as(f32p) if atoms is not None else f32p(),
+            sig.ctypes.data_as(C.POINTER(C.c_double)), feat.ctypes.data_as(f32p))
+        _check(rc, self.lib)
+        return atoms, sig, feat
+
+    def ehss_precompute(self, cluster_begin, n_clusters, n_orient, n_sigma, n_mu, max_tries=10000,
+                        seed=0):
+        sig = np.zeros((n_clusters, n_orient), np.float64)
+        mu = np.zeros((n_clusters, n_orient, max(n_mu, 1)), np.float64)
+        d = C.POINTER(C.c_double)
+        rc = self.lib.icarion_gpu_ehss_precompute(self.h, cluster_begin, n_clusters, n_orient,
+                                                  n_sigma, n_mu, max_tries, seed,
+                                                  sig.ctypes.data_as(d), mu.ctypes.data_as(d))
+        _check(rc, self.lib)
+        return sig, mu[..., :n_mu]
+
+    def surrogate_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(self.lib.icarion_gpu_surrogate_set_params(
+            self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size), self.lib)
+
+    def surrogate_get_params(self):
+        p = np.zeros(_abi.SURROGATE_NPARAMS, np.float32)
+        _check(self.lib.icarion_gpu_surrogate_get_params(
+            self.h, p.ctypes.data_as(C.POINTER(C.c_float)), p.size), self.lib)
+        return p
+
+    def surrogate_train(self, X, T, epochs=10, batch=4096, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        d = _abi.TrainDesc(epochs=epochs, batch=batch, lr=lr, beta1=0.9, beta2=0.999, eps=1e-8,
+                           seed=seed)
+        losses = np.zeros(epochs, np.float32)
+        f32p = C.POINTER(C.c_float)
+        _check(self.lib.icarion_gpu_surrogate_train(self.h, X.ctypes.data_as(f32p),
+                                                    T.ctypes.data_as(f32p), X.shape[0],
+                                                    C.byref(d), losses.ctypes.data_as(f32p)),
                self.lib)
+        return losses
+
+    def surrogate_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(_abi.SURROGATE_NPARAMS, np.float32)
+        loss = C.c_float()
+        f32p = C.POINTER(C.c_float)
+        _check(self.lib.icarion_gpu_surrogate_grad(self.h, X.ctypes.data_as(f32p),
+                                                   T.ctypes.data_as(f32p), X.shape[0],
+                                                   g.ctypes.data_as(f32p), C.byref(loss)), self.lib)
+        return float(loss.value), g
+
+    def surrogate_infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 3), np.float32)
+        f32p = C.POINTER(C.c_float)
+        _check(self.lib.icarion_gpu_surrogate_infer(self.h, X.ctypes.data_as(f32p), X.shape[0],
+                                                    Y.ctypes.data_as(f32p)), self.lib)
+        return Y
 
     def upload_field_array(self, E, origin, spacing):
         """E: array (nz, ny, nx, 3) or (N, 3) in x-fastest order, float32 or float64."""

@@ -249,3 +249,28 @@ def field_array_workload(n=4_000_000, n_steps=20_000, grid=(256, 256, 256), seed
     ens.species[:] = sidx
     return dict(name="field_array_hss_rk4", domain=dom, physics=phys, species=sp, atoms=None,
                 ensemble=ens, dt=5e-11, n_steps=n_steps, field=(E, origin, spacing))
+
+
+def ehss_cluster_counts(n, seed=0, lo=20, hi=500):
+    """Atom counts of the EHSS clusters: uniform integers in [lo, hi]."""
+    return np.random.default_rng(seed).integers(lo, hi + 1, n).astype(np.uint32)
+
+
+def ehss_surrogate_workload(n=1_000_000, n_steps=1000, seed=21, pressure_Pa=101325.0,
+                            dt=2e-12, EN_Td=20.0, precision=_abi.PRECISION_FP32_KAHAN,
+                            ehss_mode=_abi.EHSS_SURROGATE):
+    """EHSS drift in He: n ions, each its own random rigid cluster of 20-500
+    atoms (radii 1.5-2.0 A, FP32 coordinates), He at 1 atm and 300 K, E/N = 20 Td,
+    RK4 dt = 2 ps.  The clusters are generated on the device
+    (icarion_gpu_synth_clusters), so the species table is filled by the handle;
+    this builder returns the domain, physics, the atom counts and the ensemble
+    (species[i] = i, mass = 12 Da per atom)."""
+    dom = ims_domain(origin_z=-5e-3, length=20e-3, radius=10e-3, EN_Td=EN_Td, gas="He",
+                     pressure_Pa=pressure_Pa, temperature_K=300.0)
+    phys = physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS, precision=precision, seed=seed,
+                   ehss_mode=ehss_mode)
+    counts = ehss_cluster_counts(n, seed)
+    ens = gaussian_cloud(n, counts.astype(np.float64) * 12.0, 0.5e-3, seed=seed)
+    ens.species[:] = np.arange(n, dtype=np.uint32)
+    return dict(name="ehss_he_20td", domain=dom, physics=phys, counts=counts, ensemble=ens, dt=dt,
+                n_steps=n_steps)

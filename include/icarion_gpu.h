@@ -152,7 +152,21 @@ typedef struct icarion_species_desc {
     double K0_cm2_Vs;             /* reduced mobility (Friction damping)       */
     uint32_t atom_offset;         /* EHSS geometry: first atom in the table    */
     uint32_t atom_count;          /* EHSS geometry: number of atoms            */
+    float surrogate_features[3];  /* cluster shape features of the EHSS
+                                     surrogate (icarion_gpu_synth_clusters)    */
+    uint32_t reserved0;
 } icarion_species_desc;
+
+/* Surrogate training options (Adam on mean 0.5*|y - t|^2). */
+typedef struct icarion_train_desc {
+    uint32_t epochs;
+    uint32_t batch;               /* samples per step                          */
+    float lr;
+    float beta1, beta2, eps;
+    uint64_t seed;                /* initialisation and shuffling              */
+} icarion_train_desc;
+
+#define ICARION_SURROGATE_NPARAMS 17795
 
 /* Host SoA view of an ion ensemble (IonEnsemble hot/cold blocks,
  * src/core/types/IonEnsemble.h:267-318).  Inputs are read by upload, outputs
@@ -216,10 +230,13 @@ int icarion_gpu_set_domain(icarion_gpu *h, const icarion_domain_desc *domain,
                            const icarion_physics_desc *physics);
 
 /* Species table and EHSS atom table (x, y, z, r per atom, metres, COM-centred,
- * geometryUtils.h:35-60).  atoms_xyzr may be NULL when n_atoms is 0. */
+ * geometryUtils.h:35-60).  atoms_dtype 0 = float, 1 = double.  The device
+ * keeps the table in float: a double table is rounded to float on upload, and
+ * the EHSS geometry arithmetic then runs in double on those values.
+ * atoms_xyzr may be NULL when n_atoms is 0. */
 int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
-                            uint32_t n_species, const double *atoms_xyzr,
-                            uint32_t n_atoms);
+                            uint32_t n_species, const void *atoms_xyzr,
+                            uint32_t n_atoms, int32_t atoms_dtype);
 
 /* Trilinear E field on a uniform node-centred grid, x fastest
  * (index i + nx*(j + ny*k), fieldArrayLoader.cpp:9-11).  E holds 3*nx*ny*nz
@@ -243,6 +260,52 @@ int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
 
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
+
+/* ---- EHSS geometry and the scattering surrogate ---------------------------- */
+
+/* Build n_clusters random rigid atom clusters on the device (cluster c has
+ * counts[c] atoms, radii uniform in [r_min_m, r_max_m], sequential attachment
+ * at 0.95 (r_i + r_j), no closer than 0.8 (r_i + r_j), centred) and install
+ * them as the species table (species c = cluster c) with sigma_ehss = the OAPA
+ * area of EHSSCollisionHandler::compute_effective_ccs (EHSSCollisionHandler.cpp:474-498,
+ * gas radius of the current domain) and the surrogate features.  Optional
+ * host outputs: atoms (4 floats per atom, cluster-major), sigma, features (3
+ * per cluster).  Needs icarion_gpu_set_domain first. */
+int icarion_gpu_synth_clusters(icarion_gpu *h, uint32_t n_clusters, const uint32_t *counts,
+                               double r_min_m, double r_max_m, uint64_t seed,
+                               float *atoms_out, double *sigma_out, float *features_out);
+
+/* Offline EHSS precompute (tools/ehss_offline_precompute.cpp:438-517) for
+ * species [cluster_begin, cluster_begin + n_clusters) of the current atom
+ * table (float atoms): per orientation sigma_eff (n_sigma rays) and n_mu
+ * first-contact mu = -n.axis samples conditional on a hit (at most max_tries
+ * rays; missing samples are NaN).  Outputs sigma_out[n_clusters*n_orient],
+ * mu_out[n_clusters*n_orient*n_mu]. */
+int icarion_gpu_ehss_precompute(icarion_gpu *h, uint32_t cluster_begin, uint32_t n_clusters,
+                                uint32_t n_orient, uint32_t n_sigma, uint32_t n_mu,
+                                uint32_t max_tries, uint64_t seed, double *sigma_out,
+                                double *mu_out);
+
+/* Surrogate parameters (fp32, ICARION_SURROGATE_NPARAMS, layout W1[128][6],
+ * b1, W2[128][128], b2, W3[3][128], b3).  set also packs the bf16 inference
+ * weights used by ICARION_EHSS_SURROGATE. */
+int icarion_gpu_surrogate_set_params(icarion_gpu *h, const float *params, uint32_t n_params);
+int icarion_gpu_surrogate_get_params(icarion_gpu *h, float *params, uint32_t n_params);
+
+/* Train the surrogate on the device: X[n][6] inputs, T[n][3] targets.
+ * Starts from the current parameters (or a seeded initialisation when none
+ * were set); loss_per_epoch (may be NULL) receives the mean training loss of
+ * each epoch.  Packs the inference weights at the end. */
+int icarion_gpu_surrogate_train(icarion_gpu *h, const float *X, const float *T, size_t n,
+                                const icarion_train_desc *desc, float *loss_per_epoch);
+
+/* Loss and gradient of the current parameters on (X, T) (one fp32
+ * forward + backward pass on the device; for gradient checks). */
+int icarion_gpu_surrogate_grad(icarion_gpu *h, const float *X, const float *T, size_t n,
+                               float *grad_out, float *loss_out);
+
+/* Batch inference with the packed bf16 weights on the matrix cores: Y[n][3]. */
+int icarion_gpu_surrogate_infer(icarion_gpu *h, const float *X, size_t n, float *Y);
 
 /* Block until all work queued on the handle's stream (and the device) is done. */
 int icarion_gpu_synchronize(icarion_gpu *h);

@@ -71,6 +71,15 @@ def load():
                                d, d, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
                                P(_abi.StepStats)]
     lib.oracle_run.restype = C.c_int
+    u32p, f32p = P(C.c_uint32), P(C.c_float)
+    lib.oracle_ehss_precompute.argtypes = [f32p, u32p, u32p, C.c_uint32, d, C.c_uint32,
+                                           C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, P(d),
+                                           P(d)]
+    lib.oracle_ehss_features.argtypes = [f32p, u32p, u32p, C.c_uint32, d, P(d), f32p]
+    lib.oracle_oapa_sigma_f32.argtypes = [f32p, u32p, u32p, C.c_uint32, d, P(d)]
+    lib.oracle_mlp_forward.argtypes = [f32p, f32p, C.c_size_t, f32p]
+    lib.oracle_mlp_loss_grad.argtypes = [f32p, f32p, f32p, C.c_size_t, P(d)]
+    lib.oracle_mlp_loss_grad.restype = d
     _lib = lib
     return lib
 
@@ -157,3 +166,69 @@ def push_one(domain, physics, sp, mass, charge, pos, vel, t, dt, field=None):
     lib.oracle_push_one(C.byref(domain), C.byref(physics), C.byref(sp), fptr, mass, charge, p, v,
                         t, dt)
     return np.array(p[:]), np.array(v[:])
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def cluster_offsets(counts):
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    offs = np.zeros(len(counts), np.uint32)
+    offs[1:] = np.cumsum(counts[:-1])
+    return offs, counts
+
+
+def ehss_precompute(atoms, counts, r_gas, n_orient, n_sigma, n_mu, max_tries=10000, seed=0):
+    lib = load()
+    atoms = np.ascontiguousarray(atoms, dtype=np.float32)
+    offs, counts = cluster_offsets(counts)
+    n = len(counts)
+    sig = np.zeros((n, n_orient))
+    mu = np.zeros((n, n_orient, max(n_mu, 1)))
+    lib.oracle_ehss_precompute(_fp(atoms), _up(offs), _up(counts), n, r_gas, n_orient, n_sigma,
+                               n_mu, max_tries, seed, _dp(sig), _dp(mu))
+    return sig, mu[..., :n_mu]
+
+
+def oapa_sigma(atoms, counts, r_gas):
+    lib = load()
+    atoms = np.ascontiguousarray(atoms, dtype=np.float32)
+    offs, counts = cluster_offsets(counts)
+    sig = np.zeros(len(counts))
+    lib.oracle_oapa_sigma_f32(_fp(atoms), _up(offs), _up(counts), len(counts), r_gas, _dp(sig))
+    return sig
+
+
+def ehss_features(atoms, counts, r_gas, sigma):
+    lib = load()
+    atoms = np.ascontiguousarray(atoms, dtype=np.float32)
+    offs, counts = cluster_offsets(counts)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    f = np.zeros((len(counts), 3), np.float32)
+    lib.oracle_ehss_features(_fp(atoms), _up(offs), _up(counts), len(counts), r_gas, _dp(sigma),
+                             _fp(f))
+    return f
+
+
+def mlp_forward(params, X):
+    lib = load()
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.zeros((X.shape[0], 3), np.float32)
+    lib.oracle_mlp_forward(_fp(params), _fp(X), X.shape[0], _fp(Y))
+    return Y
+
+
+def mlp_loss_grad(params, X, T):
+    lib = load()
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.zeros(params.size)
+    loss = lib.oracle_mlp_loss_grad(_fp(params), _fp(X), _fp(T), X.shape[0], _dp(g))
+    return loss, g

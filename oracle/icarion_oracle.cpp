@@ -926,3 +926,242 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
 }
 
 }  // extern "C"
+
+// ---- EHSS offline precompute and surrogate reference ------------------------
+namespace {
+
+constexpr int kH = 128;
+constexpr int kIn = 6;
+constexpr int kOut = 3;
+constexpr size_t kOffW1 = 0, kOffB1 = kOffW1 + kH * kIn, kOffW2 = kOffB1 + kH,
+                 kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kOut * kH,
+                 kNParams = kOffB3 + kOut;
+static_assert(kNParams == ORACLE_MLP_NPARAMS, "parameter count");
+
+// quaternion_to_rotation + sample_random_quaternion (tools/ehss_offline_precompute.cpp:115-156)
+template <class Rng>
+void precompute_rotation(Rng& rng, double R[3][3]) {
+    const double u1 = rng.uniform01();
+    const double u2 = rng.uniform01();
+    const double u3 = rng.uniform01();
+    double x = std::sqrt(1.0 - u1) * std::sin(2.0 * kPi * u2);
+    double y = std::sqrt(1.0 - u1) * std::cos(2.0 * kPi * u2);
+    double z = std::sqrt(u1) * std::sin(2.0 * kPi * u3);
+    double w = std::sqrt(u1) * std::cos(2.0 * kPi * u3);
+    const double nrm = std::sqrt(w * w + x * x + y * y + z * z);
+    w /= nrm; x /= nrm; y /= nrm; z /= nrm;
+    R[0][0] = 1.0 - 2.0 * (y * y + z * z);
+    R[0][1] = 2.0 * (x * y - z * w);
+    R[0][2] = 2.0 * (x * z + y * w);
+    R[1][0] = 2.0 * (x * y + z * w);
+    R[1][1] = 1.0 - 2.0 * (x * x + z * z);
+    R[1][2] = 2.0 * (y * z - x * w);
+    R[2][0] = 2.0 * (x * z - y * w);
+    R[2][1] = 2.0 * (y * z + x * w);
+    R[2][2] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+// detect_atom_collision with collision axis +z (tools/ehss_offline_precompute.cpp:162-213)
+bool precompute_contact(const std::vector<V3>& ra, const std::vector<double>& rad, double ox,
+                        double oy, double r_gas, double& mu) {
+    bool hit = false;
+    double best_s = 0.0;
+    double nz_best = 0.0;
+    for (size_t j = 0; j < ra.size(); ++j) {
+        const double Rsum = rad[j] + r_gas;
+        const double Rsum2 = Rsum * Rsum;
+        const double rx = ox - ra[j].x, ry = oy - ra[j].y, rz = 0.0 - ra[j].z;
+        const double sstar = rz;
+        const double dmin2 = rx * rx + ry * ry;
+        if (dmin2 <= Rsum2) {
+            const double h = std::sqrt(std::max(0.0, Rsum2 - dmin2));
+            const double s_hit = sstar - h;
+            const double pz = rz - s_hit;
+            const double phr2 = rx * rx + ry * ry + pz * pz;
+            double nz;
+            if (phr2 <= kMinContactDistSq) {
+                nz = -1.0;
+            } else {
+                const double inv = 1.0 / std::sqrt(phr2);
+                nz = pz * inv;
+                if (nz > 0.0) nz = -nz;
+            }
+            if (!hit || s_hit < best_s) {
+                hit = true;
+                best_s = s_hit;
+                nz_best = nz;
+            }
+        }
+    }
+    if (hit) mu = std::min(1.0, std::max(-1.0, -nz_best));
+    return hit;
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_ehss_precompute(const float* atoms, const uint32_t* offsets, const uint32_t* counts,
+                            uint32_t n_clusters, double r_gas, uint32_t n_orient,
+                            uint32_t n_sigma, uint32_t n_mu, uint32_t max_tries, uint64_t seed,
+                            double* sigma_out, double* mu_out) {
+    const uint64_t key = seed ^ ORACLE_PRECOMPUTE_KEY;
+    #pragma omp parallel for schedule(dynamic, 4) collapse(2)
+    for (long long c = 0; c < static_cast<long long>(n_clusters); ++c) {
+        for (long long k = 0; k < static_cast<long long>(n_orient); ++k) {
+            PhiloxStream rng(key, static_cast<uint32_t>(c), static_cast<uint64_t>(k));
+            double R[3][3];
+            precompute_rotation(rng, R);
+            const uint32_t a0 = offsets[c], na = counts[c];
+            std::vector<V3> ra(na);
+            std::vector<double> rad(na);
+            double b_base = 0.0;
+            for (uint32_t j = 0; j < na; ++j) {
+                const double cx = atoms[4 * (a0 + j)], cy = atoms[4 * (a0 + j) + 1],
+                             cz = atoms[4 * (a0 + j) + 2];
+                rad[j] = atoms[4 * (a0 + j) + 3];
+                ra[j] = V3{R[0][0] * cx + R[0][1] * cy + R[0][2] * cz,
+                           R[1][0] * cx + R[1][1] * cy + R[1][2] * cz,
+                           R[2][0] * cx + R[2][1] * cy + R[2][2] * cz};
+                b_base = std::max(b_base, std::sqrt(ra[j].x * ra[j].x + ra[j].y * ra[j].y) + rad[j]);
+            }
+            const double b_max = b_base + r_gas;
+            const double area = kPi * b_max * b_max;
+            uint32_t hits = 0;
+            double mu;
+            for (uint32_t s = 0; s < n_sigma; ++s) {
+                const double u1 = rng.uniform01();
+                const double u2 = rng.uniform01();
+                const double r = b_max * std::sqrt(u1);
+                const double phi = 2.0 * kPi * u2;
+                if (precompute_contact(ra, rad, r * std::cos(phi), r * std::sin(phi), r_gas, mu)) ++hits;
+            }
+            sigma_out[c * n_orient + k] =
+                n_sigma ? (static_cast<double>(hits) / static_cast<double>(n_sigma)) * area : 0.0;
+            uint32_t got = 0, tries = 0;
+            double* row = mu_out + (static_cast<size_t>(c) * n_orient + k) * n_mu;
+            while (got < n_mu && tries < max_tries) {
+                ++tries;
+                const double u1 = rng.uniform01();
+                const double u2 = rng.uniform01();
+                const double r = b_max * std::sqrt(u1);
+                const double phi = 2.0 * kPi * u2;
+                if (precompute_contact(ra, rad, r * std::cos(phi), r * std::sin(phi), r_gas, mu)) row[got++] = mu;
+            }
+            for (; got < n_mu; ++got) row[got] = NAN;
+        }
+    }
+}
+
+void oracle_oapa_sigma_f32(const float* atoms, const uint32_t* offsets, const uint32_t* counts,
+                           uint32_t n_clusters, double r_gas, double* sigma_out) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (long long c = 0; c < static_cast<long long>(n_clusters); ++c) {
+        std::vector<double> a(4 * static_cast<size_t>(counts[c]));
+        for (size_t i = 0; i < a.size(); ++i) a[i] = atoms[4 * static_cast<size_t>(offsets[c]) + i];
+        sigma_out[c] = oracle_oapa_sigma(a.data(), counts[c], r_gas);
+    }
+}
+
+void oracle_ehss_features(const float* atoms, const uint32_t* offsets, const uint32_t* counts,
+                          uint32_t n_clusters, double r_gas, const double* sigma_oapa,
+                          float* f) {
+    for (uint32_t c = 0; c < n_clusters; ++c) {
+        const uint32_t a0 = offsets[c], na = counts[c];
+        double mx = 0, my = 0, mz = 0, mr = 0;
+        for (uint32_t j = 0; j < na; ++j) {
+            mx += atoms[4 * (a0 + j)];
+            my += atoms[4 * (a0 + j) + 1];
+            mz += atoms[4 * (a0 + j) + 2];
+            mr += atoms[4 * (a0 + j) + 3];
+        }
+        mx /= na; my /= na; mz /= na; mr /= na;
+        double rg2 = 0;
+        for (uint32_t j = 0; j < na; ++j) {
+            const double dx = atoms[4 * (a0 + j)] - mx, dy = atoms[4 * (a0 + j) + 1] - my,
+                         dz = atoms[4 * (a0 + j) + 2] - mz;
+            rg2 += dx * dx + dy * dy + dz * dz;
+        }
+        const double b_eff = std::sqrt(sigma_oapa[c] / kPi);
+        f[3 * c] = static_cast<float>(std::log(static_cast<double>(na)) / std::log(500.0));
+        f[3 * c + 1] = static_cast<float>(std::sqrt(rg2 / na) / b_eff);
+        f[3 * c + 2] = static_cast<float>((mr + r_gas) / b_eff);
+    }
+}
+
+void oracle_mlp_forward(const float* p, const float* X, size_t n, float* Y) {
+    #pragma omp parallel for schedule(static)
+    for (long long s = 0; s < static_cast<long long>(n); ++s) {
+        double h1[kH], h2[kH];
+        for (int o = 0; o < kH; ++o) {
+            double a = p[kOffB1 + o];
+            for (int i = 0; i < kIn; ++i) a += static_cast<double>(p[kOffW1 + o * kIn + i]) * X[s * kIn + i];
+            h1[o] = a > 0.0 ? a : 0.0;
+        }
+        for (int o = 0; o < kH; ++o) {
+            double a = p[kOffB2 + o];
+            for (int i = 0; i < kH; ++i) a += static_cast<double>(p[kOffW2 + o * kH + i]) * h1[i];
+            h2[o] = a > 0.0 ? a : 0.0;
+        }
+        for (int o = 0; o < kOut; ++o) {
+            double a = p[kOffB3 + o];
+            for (int i = 0; i < kH; ++i) a += static_cast<double>(p[kOffW3 + o * kH + i]) * h2[i];
+            Y[s * kOut + o] = static_cast<float>(a);
+        }
+    }
+}
+
+double oracle_mlp_loss_grad(const float* p, const float* X, const float* T, size_t n, double* g) {
+    for (size_t i = 0; i < kNParams; ++i) g[i] = 0.0;
+    double loss = 0.0;
+    const double inv_n = 1.0 / static_cast<double>(n);
+    for (size_t s = 0; s < n; ++s) {
+        double z1[kH], h1[kH], z2[kH], h2[kH], y[kOut], dy[kOut], dh2[kH], dh1[kH];
+        for (int o = 0; o < kH; ++o) {
+            double a = p[kOffB1 + o];
+            for (int i = 0; i < kIn; ++i) a += static_cast<double>(p[kOffW1 + o * kIn + i]) * X[s * kIn + i];
+            z1[o] = a;
+            h1[o] = a > 0.0 ? a : 0.0;
+        }
+        for (int o = 0; o < kH; ++o) {
+            double a = p[kOffB2 + o];
+            for (int i = 0; i < kH; ++i) a += static_cast<double>(p[kOffW2 + o * kH + i]) * h1[i];
+            z2[o] = a;
+            h2[o] = a > 0.0 ? a : 0.0;
+        }
+        for (int o = 0; o < kOut; ++o) {
+            double a = p[kOffB3 + o];
+            for (int i = 0; i < kH; ++i) a += static_cast<double>(p[kOffW3 + o * kH + i]) * h2[i];
+            y[o] = a;
+            const double d = a - T[s * kOut + o];
+            loss += 0.5 * d * d * inv_n;
+            dy[o] = d * inv_n;
+        }
+        for (int i = 0; i < kH; ++i) dh2[i] = 0.0;
+        for (int o = 0; o < kOut; ++o) {
+            g[kOffB3 + o] += dy[o];
+            for (int i = 0; i < kH; ++i) {
+                g[kOffW3 + o * kH + i] += dy[o] * h2[i];
+                dh2[i] += dy[o] * p[kOffW3 + o * kH + i];
+            }
+        }
+        for (int i = 0; i < kH; ++i) dh1[i] = 0.0;
+        for (int o = 0; o < kH; ++o) {
+            const double dz = z2[o] > 0.0 ? dh2[o] : 0.0;
+            g[kOffB2 + o] += dz;
+            for (int i = 0; i < kH; ++i) {
+                g[kOffW2 + o * kH + i] += dz * h1[i];
+                dh1[i] += dz * p[kOffW2 + o * kH + i];
+            }
+        }
+        for (int o = 0; o < kH; ++o) {
+            const double dz = z1[o] > 0.0 ? dh1[o] : 0.0;
+            g[kOffB1 + o] += dz;
+            for (int i = 0; i < kIn; ++i) g[kOffW1 + o * kIn + i] += dz * X[s * kIn + i];
+        }
+        (void)y;
+    }
+    return loss;
+}
+
+}  // extern "C"

@@ -92,6 +92,42 @@ int oracle_run(const icarion_domain_desc *domain, const icarion_physics_desc *ph
                uint64_t step_offset, int rng_kind, int n_threads,
                icarion_step_stats *stats);
 
+/* ---- EHSS offline precompute (tools/ehss_offline_precompute.cpp:438-517) ----
+ * Per (cluster c, orientation k): Shoemake orientation, b_max = max(r_xy + r) +
+ * r_gas, sigma_eff = hits/n_sigma * pi b_max^2, then n_mu first-contact samples
+ * mu = -n_z conditional on a hit (at most max_tries rays).  Atoms are float
+ * (x, y, z, r) in metres; cluster c uses atoms [offsets[c], offsets[c] + counts[c]).
+ * Random numbers: the Philox stream (seed ^ ORACLE_PRECOMPUTE_KEY, cluster,
+ * orientation) the device kernel draws.  mu rows of clusters that ran out of
+ * tries are filled with NaN. */
+#define ORACLE_PRECOMPUTE_KEY 0x6a09e667f3bcc909ull
+void oracle_ehss_precompute(const float *atoms_xyzr, const uint32_t *offsets,
+                            const uint32_t *counts, uint32_t n_clusters, double r_gas,
+                            uint32_t n_orient, uint32_t n_sigma, uint32_t n_mu,
+                            uint32_t max_tries, uint64_t seed, double *sigma_out,
+                            double *mu_out);
+
+/* Cluster shape features of the surrogate input:
+ * f1 = ln(n_atoms)/ln(500), f2 = R_g / sqrt(sigma_oapa/pi),
+ * f3 = (mean atom radius + r_gas) / sqrt(sigma_oapa/pi). */
+void oracle_ehss_features(const float *atoms_xyzr, const uint32_t *offsets,
+                          const uint32_t *counts, uint32_t n_clusters, double r_gas,
+                          const double *sigma_oapa, float *features_out);
+
+/* OAPA of many clusters given as float atoms (oracle_oapa_sigma per cluster). */
+void oracle_oapa_sigma_f32(const float *atoms_xyzr, const uint32_t *offsets,
+                           const uint32_t *counts, uint32_t n_clusters, double r_gas,
+                           double *sigma_out);
+
+/* ---- surrogate MLP 6 -> 128 -> 128 -> 3 (ReLU), parameters in the layout
+ * W1[128][6], b1[128], W2[128][128], b2[128], W3[3][128], b3[3] ---------- */
+#define ORACLE_MLP_NPARAMS 17795
+void oracle_mlp_forward(const float *params, const float *X, size_t n, float *Y);
+/* Mean over the batch of 0.5*|y - t|^2 and its gradient (grad has
+ * ORACLE_MLP_NPARAMS entries, overwritten). */
+double oracle_mlp_loss_grad(const float *params, const float *X, const float *T, size_t n,
+                            double *grad);
+
 #ifdef __cplusplus
 }
 #endif

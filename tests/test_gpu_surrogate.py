@@ -1,0 +1,161 @@
+"""EHSS geometry tools and the scattering surrogate on the device, against the
+oracle, plus the gain test: the trained network reproduces the plain EHSS
+mobility within 0.5 % while running several times faster, and an untrained
+network does not."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from icarion_amd import _abi
+from icarion_amd import surrogate as S
+from icarion_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+R_HE = 1.3e-10
+
+
+def he_handle(**kw):
+    from icarion_amd import gpu
+    h = gpu.Handle(**kw)
+    h.set_domain(W.ims_domain(EN_Td=20.0, gas="He"), W.physics(_abi.INTEGRATOR_RK4,
+                                                              _abi.COLLISION_EHSS))
+    return h
+
+
+def test_synth_clusters_oapa_and_features(oracle):
+    counts = W.ehss_cluster_counts(96, seed=1, lo=20, hi=120)
+    with he_handle() as h:
+        atoms, sig, feat = h.synth_clusters(counts, seed=4, want_atoms=True)
+    ref = oracle.oapa_sigma(atoms, counts, R_HE)
+    np.testing.assert_allclose(sig, ref, rtol=1e-12)
+    np.testing.assert_allclose(feat, oracle.ehss_features(atoms, counts, R_HE, ref), rtol=1e-5)
+    offs, _ = oracle.cluster_offsets(counts)
+    for c in range(len(counts)):
+        a = atoms[offs[c]:offs[c] + counts[c]].astype(np.float64)
+        assert np.all((a[:, 3] >= 1.5e-10 * (1 - 1e-6)) & (a[:, 3] <= 2.0e-10 * (1 + 1e-6)))
+        np.testing.assert_allclose(a[:, :3].mean(0), 0.0, atol=1e-14)
+        d = np.linalg.norm(a[:, None, :3] - a[None, :, :3], axis=-1)
+        lim = 0.8 * (a[:, None, 3] + a[None, :, 3])
+        iu = np.triu_indices(len(a), 1)
+        assert (d[iu] >= lim[iu] * (1 - 1e-5)).mean() > 0.99
+
+
+def test_precompute_matches_oracle(oracle):
+    counts = W.ehss_cluster_counts(12, seed=2, lo=20, hi=80)
+    with he_handle() as h:
+        atoms, _, _ = h.synth_clusters(counts, seed=7, want_atoms=True)
+        sg, mg = h.ehss_precompute(0, len(counts), 24, 64, 6, seed=11)
+    so, mo = oracle.ehss_precompute(atoms, counts, R_HE, 24, 64, 6, seed=11)
+    np.testing.assert_allclose(sg, so, rtol=1e-12)
+    assert np.isclose(mg, mo, rtol=1e-10, atol=1e-12).mean() > 0.995
+
+
+def test_surrogate_gradient_matches_oracle(oracle):
+    rng = np.random.default_rng(3)
+    p = (0.1 * rng.standard_normal(_abi.SURROGATE_NPARAMS)).astype(np.float32)
+    X = rng.uniform(-1, 1, (2000, 6)).astype(np.float32)
+    T = rng.uniform(-1, 1, (2000, 3)).astype(np.float32)
+    with he_handle() as h:
+        h.surrogate_set_params(p)
+        loss, g = h.surrogate_grad(X, T)
+    lref, gref = oracle.mlp_loss_grad(p, X, T)
+    assert loss == pytest.approx(lref, rel=1e-5)
+    for a, b in ((0, 768), (768, 896), (896, 17280), (17280, 17408), (17408, 17792),
+                 (17792, 17795)):
+        scale = np.abs(gref[a:b]).max()
+        np.testing.assert_allclose(g[a:b], gref[a:b], atol=2e-4 * scale)
+
+
+def test_surrogate_mfma_inference_matches_oracle(oracle):
+    rng = np.random.default_rng(5)
+    p = (0.15 * rng.standard_normal(_abi.SURROGATE_NPARAMS)).astype(np.float32)
+    X = rng.uniform(-1, 1, (1000, 6)).astype(np.float32)  # not a multiple of 64 on purpose
+    with he_handle() as h:
+        h.surrogate_set_params(p)
+        Y = h.surrogate_infer(X)
+        np.testing.assert_array_equal(h.surrogate_get_params(), p)
+    ref = oracle.mlp_forward(p, X)
+    rms = np.sqrt((ref ** 2).mean())
+    # bf16 weights and activations, fp32 accumulation
+    np.testing.assert_allclose(Y, ref, atol=0.03 * rms)
+    assert np.corrcoef(Y.ravel(), ref.ravel())[0, 1] > 0.999
+
+
+def test_surrogate_training_learns_mu_distribution():
+    counts = W.ehss_cluster_counts(256, seed=9, lo=20, hi=300)
+    with he_handle() as h:
+        _, _, feat = h.synth_clusters(counts, seed=9)
+        losses, X, T = S.train_on_device(h, 256, samples_per_cluster=400, features=feat,
+                                         epochs=30, batch=4096, lr=2e-3, seed=1)
+        assert losses[-1] < 0.2 * losses[0]
+        Y = h.surrogate_infer(X)
+    mu_t = -T[:, 2].reshape(256, 400)
+    mu_p = S.mu_from_output(Y).reshape(256, 400)
+    # the momentum-transfer moment <mu^2> per cluster and overall
+    m2_t, m2_p = (mu_t ** 2).mean(1), (mu_p ** 2).mean(1)
+    assert abs(m2_p.mean() / m2_t.mean() - 1) < 0.005
+    assert np.abs(m2_p / m2_t - 1).mean() < 0.03
+
+
+def run_ehss(w, counts, params, mode, precision, n_steps, ensemble, seed_clusters):
+    from icarion_amd import gpu
+    ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS, precision=precision, seed=77,
+                   ehss_mode=mode)
+    e = ensemble.copy()
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], ph)
+        h.synth_clusters(counts, seed=seed_clusters)
+        if params is not None:
+            h.surrogate_set_params(params)
+        h.upload_state(e)
+        h.synchronize()
+        t0 = time.perf_counter()
+        st = h.step(0.0, w["dt"], n_steps)
+        el = time.perf_counter() - t0
+        h.download_state(e)
+    return e, st, el
+
+
+def test_surrogate_gain_over_plain_ehss():
+    n, n_steps = 32768, 50_000
+    w = W.ehss_surrogate_workload(n=n, n_steps=n_steps, seed=21)
+    counts = w["counts"]
+    with he_handle() as h:
+        _, _, feat = h.synth_clusters(counts, seed=21)
+        S.train_on_device(h, 2048, samples_per_cluster=1000, features=feat, epochs=20,
+                          batch=8192, lr=2e-3, seed=2)
+        trained = h.surrogate_get_params()
+    rng = np.random.default_rng(0)
+    untrained = (0.05 * rng.standard_normal(_abi.SURROGATE_NPARAMS)).astype(np.float32)
+    z0 = w["ensemble"].z.copy()
+    T = n_steps * w["dt"]
+    E = w["domain"].dc_axial_V / w["domain"].length_m
+
+    def K(e):
+        a = e.alive
+        return (e.z[a] - z0[a]).mean() / (T * E)
+
+    e_plain, s_plain, t_plain = run_ehss(w, counts, None, _abi.EHSS_GEOMETRY, _abi.PRECISION_FP64,
+                                         n_steps, w["ensemble"], 21)
+    e_sur, s_sur, t_sur = run_ehss(w, counts, trained, _abi.EHSS_SURROGATE,
+                                   _abi.PRECISION_FP32_KAHAN, n_steps, w["ensemble"], 21)
+    e_unt, _, _ = run_ehss(w, counts, untrained, _abi.EHSS_SURROGATE, _abi.PRECISION_FP32_KAHAN,
+                           n_steps, w["ensemble"], 21)
+    k_plain, k_sur, k_unt = K(e_plain), K(e_sur), K(e_unt)
+    # statistical error of one run's K from the spread of displacements
+    a = e_plain.alive
+    se = (e_plain.z[a] - z0[a]).std() / math.sqrt(a.sum()) / (T * E)
+    err_sur = abs(k_sur / k_plain - 1)
+    err_unt = abs(k_unt / k_plain - 1)
+    rate_plain = s_plain.ion_steps / t_plain
+    rate_sur = s_sur.ion_steps / t_sur
+    print(f"K plain {k_plain:.5e}  surrogate {k_sur:.5e} (err {err_sur:.4%})  untrained "
+          f"{k_unt:.5e} (err {err_unt:.4%})  se {se / k_plain:.4%}; ion-steps/s plain "
+          f"{rate_plain:.3e} surrogate {rate_sur:.3e} (x{rate_sur / rate_plain:.1f})")
+    assert err_sur < 0.005 + 3 * math.sqrt(2) * se / k_plain
+    assert err_unt > 2 * err_sur
+    assert rate_sur > 3 * rate_plain
+    assert s_sur.collisions == pytest.approx(s_plain.collisions, rel=0.01)

@@ -1,0 +1,99 @@
+"""Pins of the oracle's EHSS precompute, cluster features and surrogate MLP.
+
+  precompute (tools/ehss_offline_precompute.cpp semantics):
+      single atom: every ray inside b_max = r_a + r_g hits, so sigma_eff =
+      pi (r_a + r_g)^2 exactly, and mu of a sphere hit uniformly over its
+      projected disk has P(mu <= x) = x^2 (KS test); two atoms far apart: the
+      orientation average of sigma_eff approaches twice one atom's area.
+  OAPA of one atom = pi (r_a + r_g)^2 (float-atom variant).
+  MLP: gradient = central finite differences of the loss; closed-form
+      outputs of hand-built networks (bias-only, single path relu(relu(x)))."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+R_HE = 1.3e-10
+
+
+def test_precompute_single_atom(oracle):
+    atoms = np.array([[0, 0, 0, 1.7e-10]], np.float32)
+    sig, mu = oracle.ehss_precompute(atoms, [1], R_HE, n_orient=16, n_sigma=200, n_mu=400, seed=3)
+    R = float(np.float32(1.7e-10)) + R_HE
+    np.testing.assert_allclose(sig, math.pi * R * R, rtol=1e-12)
+    m = mu.ravel()
+    assert np.all((m >= 0) & (m <= 1))
+    assert stats.kstest(m, lambda x: np.clip(x, 0, 1) ** 2).pvalue > 1e-3
+
+
+def test_precompute_two_separated_atoms(oracle):
+    r = 1.5e-10
+    atoms = np.array([[0, 0, -5e-9, r], [0, 0, 5e-9, r]], np.float32)
+    sig, mu = oracle.ehss_precompute(atoms, [2], R_HE, n_orient=2000, n_sigma=200, n_mu=4, seed=5)
+    one = math.pi * (r + R_HE) ** 2
+    # projected discs overlap only for orientations within ~(2R/10nm) of the
+    # axis; the average stays within a few percent of two full discs
+    assert sig.mean() == pytest.approx(2 * one, rel=0.05)
+    assert np.isfinite(mu).all()
+
+
+def test_oapa_f32_single_atom(oracle):
+    atoms = np.array([[0, 0, 0, 1.8e-10]], np.float32)
+    s = oracle.oapa_sigma(atoms, [1], R_HE)
+    R = float(np.float32(1.8e-10)) + R_HE
+    assert s[0] == pytest.approx(math.pi * R * R, rel=1e-14)
+
+
+def test_features_of_a_known_cluster(oracle):
+    atoms = np.array([[1e-10, 0, 0, 1.6e-10], [-1e-10, 0, 0, 1.8e-10]], np.float32)
+    sig = np.array([1e-18])
+    f = oracle.ehss_features(atoms, [2], R_HE, sig)
+    b = math.sqrt(1e-18 / math.pi)
+    assert f[0, 0] == pytest.approx(math.log(2) / math.log(500), rel=1e-6)
+    assert f[0, 1] == pytest.approx(1e-10 / b, rel=1e-6)
+    assert f[0, 2] == pytest.approx((1.7e-10 + R_HE) / b, rel=1e-6)
+
+
+def _random_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return (scale * rng.standard_normal(17795)).astype(np.float32)
+
+
+def test_mlp_gradient_matches_finite_differences(oracle):
+    p = _random_params(1)
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, (7, 6)).astype(np.float32)
+    T = rng.uniform(-1, 1, (7, 3)).astype(np.float32)
+    loss, g = oracle.mlp_loss_grad(p, X, T)
+    idx = np.concatenate([rng.choice(17795, 60, replace=False),
+                          np.arange(0, 10), np.arange(17795 - 3, 17795)])
+    checked = 0
+    for i in idx:
+        h = 1e-2 * max(abs(p[i]), 0.05)
+        pp, pm = p.copy(), p.copy()
+        pp[i] = np.float32(p[i] + h)
+        pm[i] = np.float32(p[i] - h)
+        lp, _ = oracle.mlp_loss_grad(pp, X, T)
+        lm, _ = oracle.mlp_loss_grad(pm, X, T)
+        fd = (lp - lm) / (float(pp[i]) - float(pm[i]))
+        # skip entries whose finite difference straddles a ReLU kink
+        if abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(g[i]):
+            checked += 1
+    assert checked >= int(0.9 * len(idx))
+    assert loss > 0
+
+
+def test_mlp_forward_closed_forms(oracle):
+    p = np.zeros(17795, np.float32)
+    p[-3:] = [0.25, -0.5, 2.0]  # b3
+    X = np.random.default_rng(0).uniform(-1, 1, (5, 6)).astype(np.float32)
+    np.testing.assert_allclose(oracle.mlp_forward(p, X), np.tile([0.25, -0.5, 2.0], (5, 1)))
+    p = np.zeros(17795, np.float32)
+    W1, W2, W3 = 0, 128 * 6 + 128, 128 * 6 + 128 + 128 * 128 + 128
+    p[W1 + 0] = 1.0          # h1[0] = relu(x0)
+    p[W2 + 0] = 1.0          # h2[0] = relu(h1[0])
+    p[W3 + 128 * 2 + 0] = -3.0  # y2 = -3 h2[0]
+    Y = oracle.mlp_forward(p, X)
+    np.testing.assert_allclose(Y[:, 2], -3.0 * np.maximum(X[:, 0], 0), rtol=1e-6)
+    np.testing.assert_allclose(Y[:, :2], 0.0)
