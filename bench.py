@@ -1,19 +1,33 @@
 """Benchmark of the batch ion-step path on one MI355X.
 
-Workload (one GPU, the quadrupole configuration): 1M ions, masses log-uniform
-over 100-2000 Da, z = +1, RF quadrupole r0 = 4 mm, V = 500 V at 1 MHz, U = 0,
-Langevin damping + OU thermal kicks in He at 1 Pa / 300 K, Gaussian cloud
-(sigma 0.2 mm), Maxwellian 300 K velocities, Boris push, dt = 1e-9 s.
+Headline workload (the EHSS surrogate configuration): 1M ions in He at 1 atm /
+300 K and E/N = 20 Td, each ion its own random rigid cluster of 20-500 atoms
+(radii 1.5-2.0 A, FP32 coordinates, mass 12 Da per atom), RK4 with dt = 2 ps.
+Two arithmetic paths run over the same clusters and initial cloud:
+
+  * plain: the EHSS geometry method (random orientation, first-contact ray test
+    over the atoms) in FP64;
+  * fast (the headline value): FP32 + Kahan push with the scattering normal
+    drawn by the MLP surrogate (6->128->128->3, BF16 weights, FP32
+    accumulation) - trained before the timed region on the GPU, by the
+    library's own kernels, on 10^7 first-contact samples from the EHSS
+    precompute of 10^4 of the clusters.
 
 A "step" is one engine time step of the whole cloud.  The ensemble is uploaded
 before the timed region (inputs resident in HBM); the timed region is exactly
 --steps engine steps, bracketed by device synchronisation
-(icarion_gpu_synchronize: stream + device).  torch is not loaded: it bundles a
-second HIP/HSA runtime that cannot share the device with the library's in one
-process.  Both arithmetic
-paths run: the plain FP64 method and the FP32 + Kahan fast path (the headline
-value); their transmission per m/z bin is compared.  The CPU baseline is the
-oracle (same algorithm, FP64) on the host cores, timed on a bounded sample.
+(icarion_gpu_synchronize: stream + device).  The plain path is timed on
+--plain-steps steps of the same cloud (its rate per ion-step does not depend on
+the step count).  The mobility check runs both paths over a long drift of a
+smaller cloud and reports K of each, their relative difference and the
+statistical error of one K.  The CPU baseline is the oracle's plain EHSS (FP64,
+OpenMP) on the host cores, timed on a bounded sample of the same workload.
+
+torch is not loaded: it bundles a second HIP/HSA runtime that cannot share the
+device with the library's in one process.
+
+--workload quadrupole runs the RF-quadrupole configuration instead (1M ions,
+Boris, Langevin + OU in He at 1 Pa, FP64 vs FP32 + Kahan, transmission per m/z).
 
 Prints one JSON line.
 """
@@ -31,31 +45,60 @@ sys.path.insert(0, ROOT)
 
 MI355X_PEAK_FP64_TFLOPS = 78.6
 MI355X_PEAK_FP32_TFLOPS = 157.3
+MI355X_PEAK_BF16_TFLOPS = 2500.0
 
 # Algorithmic FLOPs per ion-step (FMA = 2, add/mul/div = 1; sqrt, log, exp,
 # sin/cos and the integer Philox rounds not counted), counted from
-# icarion_amd/csrc/step_kernels.hip for the Boris + quadrupole RF field +
-# Langevin damping + OU kick kernels: force 26, Boris update 18, OU kick 25,
-# uniforms 8, cylinder test 9 (FP64); the FP32 path adds 36 for the Kahan
-# compensation of six components and the kick.
+# icarion_amd/csrc/step_kernels.hip.
+#   quadrupole (Boris + quadrupole RF field + Langevin damping + OU kick):
+#     force 26, Boris update 18, OU kick 25, uniforms 8, cylinder test 9 (FP64);
+#     the FP32 path adds 36 for the Kahan compensation of six components.
+#   EHSS surrogate kernel, per ion-step: RK4 in a uniform field 4 stages x
+#     (3 velocity + 3 position) FMAs + combination 24 + Kahan 6 x 4 = 84;
+#     neutral draw 10, |v_rel| 5, probability 6, cylinder test 5 (twice) = 115.
+#     Per collision: the MLP 2 (8*128 + 128*128 + 128*3) + 2*128 + 2*128 + 3 =
+#     36 099 (the first layer's K is padded to 8 in the matrix cores), frame +
+#     reflection + centre of mass 60.
 FLOPS_PER_ION_STEP = {"fp64": 86, "fp32": 122}
+EHSS_FLOPS_STEP = 115
+EHSS_FLOPS_COLLISION = 2 * (8 * 128 + 128 * 128 + 128 * 3) + 2 * 128 + 2 * 128 + 3 + 60
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100_000)
+    ap.add_argument("--workload", choices=("ehss", "quadrupole"), default="ehss")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed engine steps (default: ehss 4000, quadrupole 100000)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--ions", type=int, default=1_000_000)
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--cpu-sample-ions", type=int, default=100_000)
-    ap.add_argument("--cpu-sample-steps", type=int, default=20000)
+    ap.add_argument("--plain-steps", type=int, default=100,
+                    help="ehss: engine steps of the timed plain FP64 run")
+    ap.add_argument("--train-clusters", type=int, default=10_000)
+    ap.add_argument("--train-samples", type=int, default=1000,
+                    help="first-contact samples per training cluster")
+    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--params-out", default=None,
+                    help="ehss: save the trained surrogate parameters (.npy)")
+    ap.add_argument("--params-in", default=None,
+                    help="ehss: load surrogate parameters saved by --params-out instead of "
+                         "training (for profiler runs)")
+    ap.add_argument("--k-ions", type=int, default=65536)
+    ap.add_argument("--k-steps", type=int, default=25_000)
+    ap.add_argument("--no-k-check", action="store_true")
+    ap.add_argument("--cpu-sample-ions", type=int, default=None)
+    ap.add_argument("--cpu-sample-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter "
                          "calibration for rocprofv3 --pmc); loads torch, not the library")
     return ap.parse_args()
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def copy_probe(mib):
@@ -71,7 +114,203 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
-def run_path(w, precision, steps, warmup, spl):
+def timed_steps(h, ens, dt, steps, warmup):
+    """Warm up, re-upload the initial state, then time exactly `steps` steps."""
+    if warmup:
+        h.upload_state(ens)
+        h.step(0.0, dt, warmup, 0)
+    h.upload_state(ens)
+    h.synchronize()
+    t0 = time.perf_counter()
+    st = h.step(0.0, dt, steps, 0)
+    h.synchronize()
+    return st, time.perf_counter() - t0
+
+
+# ---------------------------------------------------------------- EHSS surrogate
+def ehss_handle(w, precision, mode, spl=0, params=None, n_clusters=None):
+    """A handle holding the workload's clusters [0, n_clusters) (all by default);
+    cluster c depends only on (seed, c, counts[c])."""
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS, precision=precision,
+                   seed=w["physics"].rng_seed, ehss_mode=mode)
+    h = gpu.Handle(device=0, steps_per_launch=spl)
+    h.set_domain(w["domain"], ph)
+    _, sig, feat = h.synth_clusters(w["counts"][:n_clusters], seed=w["cluster_seed"])
+    if params is not None:
+        h.surrogate_set_params(params)
+    return h, sig, feat
+
+
+def train_surrogate(w, args):
+    from icarion_amd import _abi
+    from icarion_amd import surrogate as S
+    n_tr = min(args.train_clusters, len(w["counts"]))
+    h, _, feat = ehss_handle(w, _abi.PRECISION_FP32_KAHAN, _abi.EHSS_SURROGATE, n_clusters=n_tr)
+    with h:
+        t0 = time.perf_counter()
+        losses, X, _ = S.train_on_device(h, n_tr, samples_per_cluster=args.train_samples,
+                                         features=feat, epochs=args.train_epochs, batch=8192,
+                                         lr=2e-3, seed=2)
+        params = h.surrogate_get_params()
+    return params, dict(clusters=n_tr, samples=int(X.shape[0]), epochs=args.train_epochs,
+                        batch=8192, loss_first=float(losses[0]), loss_last=float(losses[-1]),
+                        seconds=time.perf_counter() - t0)
+
+
+def k_of(ens, z0, T, E):
+    a = ens.alive
+    dz = ens.z[a] - z0[a]
+    return dz.mean() / (T * E), dz.std() / math.sqrt(a.sum()) / (T * E)
+
+
+def mobility_check(args, params):
+    """K of the plain FP64 method and of FP32 + surrogate over the same long drift."""
+    from icarion_amd import _abi
+    from icarion_amd import workloads as W
+    w = W.ehss_surrogate_workload(n=args.k_ions, n_steps=args.k_steps, seed=21)
+    w["cluster_seed"] = 21
+    z0 = w["ensemble"].z.copy()
+    T = args.k_steps * w["dt"]
+    E = w["domain"].dc_axial_V / w["domain"].length_m
+    out = {}
+    for name, prec, mode, p in (("plain", _abi.PRECISION_FP64, _abi.EHSS_GEOMETRY, None),
+                                ("surrogate", _abi.PRECISION_FP32_KAHAN, _abi.EHSS_SURROGATE,
+                                 params)):
+        e = w["ensemble"].copy()
+        h, _, _ = ehss_handle(w, prec, mode, params=p)
+        with h:
+            h.upload_state(e)
+            h.synchronize()
+            t0 = time.perf_counter()
+            st = h.step(0.0, w["dt"], args.k_steps, 0)
+            el = time.perf_counter() - t0
+            h.download_state(e)
+        out[name] = (k_of(e, z0, T, E), st, el)
+        log(f"mobility check {name}: K {out[name][0][0]:.5e} in {el:.1f} s")
+    (kp, se), sp, tp = out["plain"]
+    (ks, _), ss, ts = out["surrogate"]
+    return {"ions": args.k_ions, "steps": args.k_steps, "drift_time_s": T,
+            "K_plain_m2_Vs": kp, "K_surrogate_m2_Vs": ks, "rel_error": abs(ks / kp - 1),
+            "stat_error_one_K": se / kp,
+            "collisions_plain": sp.collisions, "collisions_surrogate": ss.collisions,
+            "rate_plain": sp.ion_steps / tp, "rate_surrogate": ss.ion_steps / ts}
+
+
+def cpu_baseline_ehss(w, n_ions, n_steps):
+    """The oracle's plain EHSS (FP64) on the host cores over the first n_ions
+    ions of the workload (their clusters regenerated on the device)."""
+    from oracle import binding
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    counts = w["counts"][:n_ions]
+    with gpu.Handle(device=0) as h:
+        h.set_domain(w["domain"], W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS))
+        atoms, sig, _ = h.synth_clusters(counts, seed=w["cluster_seed"], want_atoms=True)
+    offs = np.zeros(n_ions, np.int64)
+    offs[1:] = np.cumsum(counts[:-1])
+    species = [W.species(sigma_ehss=float(s), atom_offset=int(o), atom_count=int(c))
+               for s, o, c in zip(sig, offs, counts)]
+    ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS, seed=w["physics"].rng_seed)
+    e = w["ensemble"].subset(np.arange(n_ions))
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    st = binding.run(w["domain"], ph, species, e, 0.0, w["dt"], n_steps, atoms=atoms,
+                     threads=threads)
+    el = time.perf_counter() - t0
+    return dict(value=st.ion_steps / el, unit="ion-steps/s", cores=threads, kind="port",
+                sample=f"{n_ions} ions x {n_steps} steps of the same workload, plain EHSS "
+                       f"(FP64 oracle, OpenMP over {threads} threads, {el:.1f} s)")
+
+
+def bench_ehss(args):
+    from icarion_amd import _abi
+    from icarion_amd import workloads as W
+    steps = args.steps if args.steps is not None else 4000
+    w = W.ehss_surrogate_workload(n=args.ions, n_steps=steps, seed=21)
+    w["cluster_seed"] = 21
+    if args.params_in:
+        params = np.load(args.params_in)
+        train = {"loaded_from": os.path.basename(args.params_in)}
+        log(f"{w['name']} {args.ions} ions x {steps} steps; surrogate loaded")
+    else:
+        log(f"{w['name']} {args.ions} ions x {steps} steps; training the surrogate")
+        params, train = train_surrogate(w, args)
+        log(f"trained in {train['seconds']:.1f} s (loss {train['loss_first']:.4f} -> "
+            f"{train['loss_last']:.4f})")
+    if args.params_out:
+        np.save(args.params_out, params)
+
+    h, _, _ = ehss_handle(w, _abi.PRECISION_FP32_KAHAN, _abi.EHSS_SURROGATE,
+                          args.steps_per_launch, params)
+    with h:
+        fs, fwall = timed_steps(h, w["ensemble"].copy(), w["dt"], steps, args.warmup)
+    spl = args.steps_per_launch or 2000
+    launches = math.ceil(steps / spl)
+    value = fs.ion_steps / fwall
+    log(f"fp32+surrogate {fwall:.3f} s, {value:.3e} ion-steps/s")
+    avg_launch_s = fs.device_ms * 1e-3 / launches
+    # the MLP's bf16 matrix-core work is >99 % of the algorithmic FLOPs, so the
+    # kernel is priced against the dense bf16 peak
+    flops = EHSS_FLOPS_STEP * fs.ion_steps + EHSS_FLOPS_COLLISION * fs.collisions
+    achieved = flops / launches / avg_launch_s / 1e12
+    res = {
+        "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
+        "value": value,
+        "unit": "ion-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": fwall * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (Kahan-compensated state; bf16 MLP weights, f32 accumulation; "
+                 "f64 time)",
+        "data": "synthetic",
+        "config": {
+            "workload": "ehss_surrogate_he_20td",
+            "ions": args.ions, "clusters": "one per ion, 20-500 atoms, radii 1.5-2.0 A",
+            "gas": "He 101325 Pa 300 K", "E_over_N_Td": 20.0, "dt_s": w["dt"],
+            "integrator": "rk4", "engine_steps": steps,
+            "mlp": "6-128-128-3 bf16/f32", "training": train,
+        },
+        "roofline": {
+            "bound": "mfma", "achieved": achieved, "peak": MI355X_PEAK_BF16_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK_BF16_TFLOPS, "traffic": None,
+            "kernel": "step_surrogate_kernel<RK4, linear field>",
+            "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
+            "flops_per_ion_step": EHSS_FLOPS_STEP,
+            "flops_per_collision": EHSS_FLOPS_COLLISION,
+            "collisions_per_ion_step": fs.collisions / max(fs.ion_steps, 1),
+        },
+        "alive_fraction": fs.alive / args.ions,
+    }
+    if not args.skip_fp64:
+        h, _, _ = ehss_handle(w, _abi.PRECISION_FP64, _abi.EHSS_GEOMETRY, args.steps_per_launch)
+        with h:
+            ps, pwall = timed_steps(h, w["ensemble"].copy(), w["dt"], args.plain_steps,
+                                    min(args.warmup, args.plain_steps))
+        pv = ps.ion_steps / pwall
+        log(f"fp64 plain EHSS {pwall:.3f} s, {pv:.3e} ion-steps/s")
+        res["fp64_plain"] = {"value": pv, "unit": "ion-steps/s", "steps": args.plain_steps,
+                             "wall_s": pwall,
+                             "collisions_per_ion_step": ps.collisions / max(ps.ion_steps, 1)}
+        res["surrogate_over_plain"] = value / pv
+    if not args.no_k_check:
+        res["mobility_check"] = mc = mobility_check(args, params)
+        log(f"K rel. error {mc['rel_error']:.4%} (one-K statistical error "
+            f"{mc['stat_error_one_K']:.3%})")
+    if not args.no_cpu:
+        n = args.cpu_sample_ions or 8192
+        k = args.cpu_sample_steps or 16000
+        res["cpu_baseline"] = cpu_baseline_ehss(w, min(n, args.ions), k)
+    return res
+
+
+# ---------------------------------------------------------------- quadrupole
+def run_quadrupole_path(w, precision, steps, warmup, spl):
     from icarion_amd import gpu
     from icarion_amd import workloads as W
     ph = W.physics(w["physics"].integrator, w["physics"].collision_model,
@@ -80,15 +319,7 @@ def run_path(w, precision, steps, warmup, spl):
     with gpu.Handle(device=0, steps_per_launch=spl) as h:
         h.set_domain(w["domain"], ph)
         h.set_species(w["species"])
-        if warmup:
-            h.upload_state(ens)
-            h.step(0.0, w["dt"], warmup, 0)
-        h.upload_state(ens)  # timed run starts from the initial state
-        h.synchronize()
-        t0 = time.perf_counter()
-        st = h.step(0.0, w["dt"], steps, 0)
-        h.synchronize()
-        wall = time.perf_counter() - t0
+        st, wall = timed_steps(h, ens, w["dt"], steps, warmup)
         h.download_state(ens)
     launches = math.ceil(steps / (spl or 2000))
     return dict(ens=ens, stats=st, wall=wall, launches=launches)
@@ -103,66 +334,43 @@ def transmission(ens, masses_amu):
     return bins, out
 
 
-def cpu_baseline(w, n_ions, n_steps):
+def cpu_baseline_quadrupole(w, n_ions, n_steps):
     from oracle import binding
     threads = len(os.sched_getaffinity(0))
-    sub = dict(w)
     e = w["ensemble"].subset(np.arange(n_ions))
     t0 = time.perf_counter()
     st = binding.run(w["domain"], w["physics"], w["species"], e, 0.0, w["dt"], n_steps,
                      threads=threads)
     el = time.perf_counter() - t0
-    del sub
     return dict(value=st.ion_steps / el, unit="ion-steps/s", cores=threads, kind="port",
                 sample=f"{n_ions} ions x {n_steps} steps of the same workload "
                        f"(FP64 oracle, OpenMP over {threads} threads, {el:.1f} s)")
 
 
-def main():
-    args = parse()
-    if args.copy_probe_mib:
-        copy_probe(args.copy_probe_mib)
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
-              "multi-GPU sharding is not part of this build", file=sys.stderr)
-        sys.exit(2)
-    from icarion_amd import _abi, gpu
+def bench_quadrupole(args):
+    from icarion_amd import _abi
     from icarion_amd import workloads as W
-    if gpu.device_count() < 1:
-        print("no HIP device visible", file=sys.stderr)
-        sys.exit(1)
-    w = W.quadrupole_workload(n=args.ions, n_steps=args.steps)
+    steps = args.steps if args.steps is not None else 100_000
+    w = W.quadrupole_workload(n=args.ions, n_steps=steps)
     masses = w["ensemble"].mass_kg / 1.66053906660e-27
-    print(f"bench: {w['name']} {args.ions} ions x {args.steps} steps", file=sys.stderr,
-          flush=True)
-    fast = run_path(w, _abi.PRECISION_FP32_KAHAN, args.steps, args.warmup, args.steps_per_launch)
-    print(f"bench: fp32+kahan {fast['wall']:.3f} s", file=sys.stderr, flush=True)
-    plain = None
-    if not args.skip_fp64:
-        plain = run_path(w, _abi.PRECISION_FP64, args.steps, args.warmup, args.steps_per_launch)
-        print(f"bench: fp64 {plain['wall']:.3f} s", file=sys.stderr, flush=True)
-
+    log(f"{w['name']} {args.ions} ions x {steps} steps")
+    fast = run_quadrupole_path(w, _abi.PRECISION_FP32_KAHAN, steps, args.warmup,
+                               args.steps_per_launch)
+    log(f"fp32+kahan {fast['wall']:.3f} s")
     fs = fast["stats"]
     value = fs.ion_steps / fast["wall"]
     avg_launch_s = fs.device_ms * 1e-3 / fast["launches"]
     achieved = FLOPS_PER_ION_STEP["fp32"] * (fs.ion_steps / fast["launches"]) / avg_launch_s / 1e12
     res = {
         "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
-        "value": value,
-        "unit": "ion-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": fast["wall"] * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": fast["wall"] * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 (Kahan-compensated state; f64 time and RF phase)",
         "data": "synthetic",
         "config": {
             "workload": "rf_quadrupole_boris_langevin_ou",
-            "ions": args.ions, "dt_s": w["dt"], "engine_steps": args.steps,
+            "ions": args.ions, "dt_s": w["dt"], "engine_steps": steps,
             "masses_amu": "log-uniform 100-2000", "r0_m": 4e-3, "V_rf": 500.0,
             "f_rf_Hz": 1e6, "U_dc": 0.0, "gas": "He 1 Pa 300 K",
         },
@@ -175,7 +383,10 @@ def main():
         },
         "alive_fraction": fs.alive / args.ions,
     }
-    if plain is not None:
+    if not args.skip_fp64:
+        plain = run_quadrupole_path(w, _abi.PRECISION_FP64, steps, args.warmup,
+                                    args.steps_per_launch)
+        log(f"fp64 {plain['wall']:.3f} s")
         ps = plain["stats"]
         pv = ps.ion_steps / plain["wall"]
         p_avg = ps.device_ms * 1e-3 / plain["launches"]
@@ -191,8 +402,26 @@ def main():
         res["transmission_fp64"] = tp
         res["transmission_max_abs_diff"] = float(np.max(np.abs(np.array(tf) - np.array(tp))))
     if not args.no_cpu:
-        n_cpu = min(args.cpu_sample_ions, args.ions)
-        res["cpu_baseline"] = cpu_baseline(w, n_cpu, min(args.cpu_sample_steps, args.steps))
+        n_cpu = min(args.cpu_sample_ions or 100_000, args.ions)
+        res["cpu_baseline"] = cpu_baseline_quadrupole(w, n_cpu,
+                                                      min(args.cpu_sample_steps or 20000, steps))
+    return res
+
+
+def main():
+    args = parse()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
+              "multi-GPU sharding is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    from icarion_amd import gpu
+    if gpu.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    res = bench_ehss(args) if args.workload == "ehss" else bench_quadrupole(args)
     print(json.dumps(res), flush=True)
 
 

@@ -328,6 +328,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     P.fa_f64 = (h->phys.precision == ICARION_PRECISION_FP64) ? h->fa64.p : nullptr;
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
+    fill_fast_const(P);
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, h->stream, h->counters.p, 4);
     hip_check(hipGetLastError(), "zero counters");
     double t_end = t0;
@@ -497,51 +498,10 @@ std::vector<double> oapa_rotations() {
     return R;
 }
 
-uint16_t bf16_rne(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
-}
-
-// Pack fp32 parameters into the bf16 MFMA fragment blob of surrogate.hpp.
-std::vector<unsigned char> pack_blob(const std::vector<float>& p) {
-    std::vector<unsigned char> blob(kPackBytes, 0);
-    auto put = [&](size_t byte_off, int j, float v) {
-        const uint16_t b = bf16_rne(v);
-        std::memcpy(&blob[byte_off + 2 * j], &b, 2);
-    };
-    const float* W1 = p.data() + kOffW1;
-    const float* W2 = p.data() + kOffW2;
-    const float* W3 = p.data() + kOffW3;
-    for (int mb = 0; mb < 4; ++mb)
-        for (int lane = 0; lane < 64; ++lane)
-            for (int j = 0; j < 4; ++j) {
-                const int row = 32 * mb + lane % 32, k = 4 * (lane / 32) + j;
-                put(kPackA1 + (mb * 64 + lane) * 8, j, k < kMlpIn ? W1[row * kMlpIn + k] : 0.f);
-            }
-    for (int mb = 0; mb < 4; ++mb)
-        for (int ks = 0; ks < 16; ++ks)
-            for (int lane = 0; lane < 64; ++lane)
-                for (int j = 0; j < 4; ++j) {
-                    const int row = 32 * mb + lane % 32, k = 8 * ks + 4 * (lane / 32) + j;
-                    put(kPackA2 + ((mb * 16 + ks) * 64 + lane) * 8, j, W2[row * kMlpH + k]);
-                }
-    for (int ks = 0; ks < 16; ++ks)
-        for (int lane = 0; lane < 64; ++lane)
-            for (int j = 0; j < 4; ++j) {
-                const int row = lane % 32, k = 8 * ks + 4 * (lane / 32) + j;
-                put(kPackA3 + (ks * 64 + lane) * 8, j, row < kMlpOut ? W3[row * kMlpH + k] : 0.f);
-            }
-    std::memcpy(&blob[kPackB1], p.data() + kOffB1, 128 * 4);
-    std::memcpy(&blob[kPackB2], p.data() + kOffB2, 128 * 4);
-    std::memcpy(&blob[kPackB3], p.data() + kOffB3, 3 * 4);
-    return blob;
-}
-
 void install_params(icarion_gpu* h, const float* params) {
     h->params.assign(params, params + kMlpParams);
-    const std::vector<unsigned char> blob = pack_blob(h->params);
+    std::vector<unsigned char> blob(kPackBytes);
+    mlp_pack_blob(h->params.data(), blob.data());
     set_device(h);
     copy_in(h->mlp_blob, reinterpret_cast<const uint4*>(blob.data()), kPackBytes / 16, h->stream, "upload surrogate");
     hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");

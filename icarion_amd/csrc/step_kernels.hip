@@ -478,16 +478,6 @@ __device__ __forceinline__ float fast_cos_rev(float r) { return __builtin_amdgcn
 __device__ __forceinline__ float fast_sin_rev(float r) { return __builtin_amdgcn_sinf(r); }
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-struct FastConst {
-    float Ez, orbi_hk, orbi_k, R_char_sq;
-    float ox, oy, oz;
-    float u_gas[3];
-    float sigma_neutral;
-    float n_dt;          // n * dt
-    float m_gas;
-    float L, half_L, R_eps_sq, R_in_sq, R_out, R_out_sq, ln_R_in, ln_R_out;
-    float g2l[9], l2g[9];
-};
 
 template <int FIELD>
 __device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C, f3 p, float c_rf) {
@@ -649,29 +639,7 @@ __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const
     const bool in_range = i < P.n_ions;
     uint32_t pushed = 0, ncoll_launch = 0;
     if (in_range) {
-        FastConst C;
-        C.Ez = (float)P.Ez_const;
-        C.orbi_k = (float)P.orbi_k;
-        C.orbi_hk = (float)(0.5 * P.orbi_k);
-        C.R_char_sq = (float)P.R_char_sq;
-        C.ox = (float)P.origin[0]; C.oy = (float)P.origin[1]; C.oz = (float)P.origin[2];
-        C.u_gas[0] = (float)P.u_gas[0]; C.u_gas[1] = (float)P.u_gas[1]; C.u_gas[2] = (float)P.u_gas[2];
-        C.sigma_neutral = (float)P.sigma_neutral;
-        C.n_dt = (float)(P.n * P.dt);
-        C.m_gas = (float)P.m_gas;
-        C.L = (float)P.L;
-        C.half_L = (float)(0.5 * P.L);
-        C.R_eps_sq = (float)(P.R * P.R);
-        C.R_in_sq = (float)(P.R_in * P.R_in);
-        C.R_out = (float)P.R_out;
-        C.R_out_sq = (float)(P.R_out * P.R_out);
-        C.ln_R_in = (float)P.ln_R_in;
-        C.ln_R_out = (float)P.ln_R_out;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            C.g2l[c] = (float)P.g2l[c];
-            C.l2g[c] = (float)P.l2g[c];
-        }
+        const FastConst& C = P.fc;
         // state: FP32 (hi, lo) pairs; hi + lo equals the FP64 value to ~2^-48
         const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
         float hi[6], lo[6];
@@ -833,36 +801,14 @@ __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const
 // Every lane stays in the step loop (dead lanes are masked, not exited) so the
 // wave-wide matrix instructions always see the whole wavefront.
 template <int INTEG, int FIELD>
-__global__ void __launch_bounds__(256) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
+__global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
     __shared__ uint4 mlp_lds[kPackBytes / 16];
     mlp_stage(P.mlp_blob, mlp_lds);
     const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < P.n_ions;
     const uint32_t ii = valid ? i : 0;
-    FastConst C;
-    C.Ez = (float)P.Ez_const;
-    C.orbi_k = (float)P.orbi_k;
-    C.orbi_hk = (float)(0.5 * P.orbi_k);
-    C.R_char_sq = (float)P.R_char_sq;
-    C.ox = (float)P.origin[0]; C.oy = (float)P.origin[1]; C.oz = (float)P.origin[2];
-    C.u_gas[0] = (float)P.u_gas[0]; C.u_gas[1] = (float)P.u_gas[1]; C.u_gas[2] = (float)P.u_gas[2];
-    C.sigma_neutral = (float)P.sigma_neutral;
-    C.n_dt = (float)(P.n * P.dt);
-    C.m_gas = (float)P.m_gas;
-    C.L = (float)P.L;
-    C.half_L = (float)(0.5 * P.L);
-    C.R_eps_sq = (float)(P.R * P.R);
-    C.R_in_sq = (float)(P.R_in * P.R_in);
-    C.R_out = (float)P.R_out;
-    C.R_out_sq = (float)(P.R_out * P.R_out);
-    C.ln_R_in = (float)P.ln_R_in;
-    C.ln_R_out = (float)P.ln_R_out;
-#pragma unroll
-    for (int c = 0; c < 9; ++c) {
-        C.g2l[c] = (float)P.g2l[c];
-        C.l2g[c] = (float)P.l2g[c];
-    }
+    const FastConst& C = P.fc;
     const double X[6] = {S.x[ii], S.y[ii], S.z[ii], S.vx[ii], S.vy[ii], S.vz[ii]};
     float hi[6], lo[6];
 #pragma unroll

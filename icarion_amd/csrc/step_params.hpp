@@ -52,6 +52,19 @@ struct IonPtrs {
     uint32_t *ncoll;
 };
 
+// FP32 copies of the run constants used by the fast path, rounded once on the
+// host (kernel arguments: wave-uniform, so they stay in scalar registers).
+struct FastConst {
+    float Ez, orbi_hk, orbi_k, R_char_sq;
+    float ox, oy, oz;
+    float u_gas[3];
+    float sigma_neutral;
+    float n_dt;          // n * dt
+    float m_gas;
+    float L, half_L, R_eps_sq, R_in_sq, R_out, R_out_sq, ln_R_in, ln_R_out;
+    float g2l[9], l2g[9];
+};
+
 struct StepParams {
     // geometry (DomainConfig.geometry)
     int instrument;
@@ -106,6 +119,33 @@ struct StepParams {
     uint32_t n_steps;
     uint32_t n_ions;
     unsigned long long *counters;   // [0] ion-steps, [1] collisions
+    FastConst fc;
 };
+
+// Fill P.fc from the FP64 fields (after P.dt is set).
+inline void fill_fast_const(StepParams& P) {
+    FastConst& C = P.fc;
+    C.Ez = (float)P.Ez_const;
+    C.orbi_k = (float)P.orbi_k;
+    C.orbi_hk = (float)(0.5 * P.orbi_k);
+    C.R_char_sq = (float)P.R_char_sq;
+    C.ox = (float)P.origin[0]; C.oy = (float)P.origin[1]; C.oz = (float)P.origin[2];
+    C.u_gas[0] = (float)P.u_gas[0]; C.u_gas[1] = (float)P.u_gas[1]; C.u_gas[2] = (float)P.u_gas[2];
+    C.sigma_neutral = (float)P.sigma_neutral;
+    C.n_dt = (float)(P.n * P.dt);
+    C.m_gas = (float)P.m_gas;
+    C.L = (float)P.L;
+    C.half_L = (float)(0.5 * P.L);
+    C.R_eps_sq = (float)(P.R * P.R);
+    C.R_in_sq = (float)(P.R_in * P.R_in);
+    C.R_out = (float)P.R_out;
+    C.R_out_sq = (float)(P.R_out * P.R_out);
+    C.ln_R_in = (float)P.ln_R_in;
+    C.ln_R_out = (float)P.ln_R_out;
+    for (int c = 0; c < 9; ++c) {
+        C.g2l[c] = (float)P.g2l[c];
+        C.l2g[c] = (float)P.l2g[c];
+    }
+}
 
 }  // namespace icarion

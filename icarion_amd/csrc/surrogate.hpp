@@ -1,6 +1,6 @@
 // surrogate.hpp - the EHSS scattering surrogate: an MLP 6 -> 128 -> 128 -> 3
-// (ReLU), evaluated one wavefront (64 ions) at a time on the matrix cores with
-// v_mfma_f32_32x32x8_bf16 (bf16 weights and activations, fp32 accumulation).
+// (ReLU), evaluated one wavefront (64 ions) at a time on the matrix cores
+// (bf16 weights and activations, fp32 accumulation).
 //
 // Inputs per ion: (u, cos psi, sin psi, f1, f2, f3) - a uniform quantile level,
 // the azimuth of the contact normal about the relative velocity, and three
@@ -10,16 +10,26 @@
 // (EHSSCollisionHandler.cpp:253-277).
 //
 // Layout: the network is computed transposed, H^T = W X^T, so the ions are the
-// N dimension of the MFMA.  For 32x32x8 (A: 32x8, B: 8x32, D: 32x32):
-//   A lane l: row l%32, k = 4*(l/32) + j   (j = 0..3)
-//   B lane l: col l%32, k = 4*(l/32) + j
-//   D lane l: col l%32, row = 8*(v/4) + 4*(l/32) + v%4  (v = 0..15)
-// so the D fragment of one layer, for k-step s = v/4 of the next, is exactly
-// that layer's B fragment: activations chain through registers, no shuffles.
+// N dimension of the matrix instructions (two 32-ion blocks per wavefront).
+// Every 32x32 output block has the accumulator layout
+//   D lane l: col l%32, row = 8*(v/4) + 4*(l/32) + v%4  (v = 0..15).
+// Layer 1 uses v_mfma_f32_32x32x8_bf16 (K = 8: the six inputs, a constant 1
+// that carries the bias, a zero):
+//   A lane l: row l%32, k = 4*(l/32) + j;  B lane l: col l%32, k = 4*(l/32) + j.
+// Layers 2 and 3 use v_mfma_f32_32x32x16_bf16 (K = 16 per instruction): an
+// operand slot (half h = l/32, element j = 0..7) holds the same k for A and B,
+// and the sum over k does not depend on which hidden unit a slot carries.  The
+// B operand of K-chunk c = 2*mb + q is taken straight from accumulator
+// registers v = 8q + j of the previous layer's row block mb, so slot (h, j)
+// carries hidden unit 32*mb + 16*q + 8*(j/4) + 4*h + j%4; the host packs the
+// weight columns in that order.  Activations chain through registers, no
+// shuffles.  Layer 3 is consumed block by block inside layer 2's loop, so
+// layer 2's activations never exist all at once.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 namespace icarion {
 
@@ -35,33 +45,94 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
 constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
 
-// Packed inference blob (bytes): bf16 A fragments in lane order, then biases.
-//   A1: [4 mb][1 ks][64 lanes] x 4 bf16   ( 2048 B)
-//   A2: [4 mb][16 ks][64 lanes] x 4 bf16  (32768 B)
-//   A3: [1 mb][16 ks][64 lanes] x 4 bf16  ( 8192 B)
-//   b1[128], b2[128], b3[32] float         ( 1152 B)
+// Packed inference blob (bytes): bf16 A fragments in lane order, then a bias.
+//   A1: [4 mb][64 lanes] x 4 bf16         ( 2048 B)  column 6 = b1
+//   A2: [4 mb][9 kc][64 lanes] x 8 bf16   (36864 B)  chunk 8 = b2 in slot (0, 0)
+//   A3: [8 kc][64 lanes] x 8 bf16         ( 8192 B)  rows 3..31 zero
+//   b3[32] float                          (  128 B)
+// Chunk 8 of layer 2 multiplies a constant operand that is 1 in slot (0, 0)
+// and 0 elsewhere, so the matrix cores add b2.
 constexpr int kPackA1 = 0;
-constexpr int kPackA2 = kPackA1 + 4 * 1 * 64 * 8;
-constexpr int kPackA3 = kPackA2 + 4 * 16 * 64 * 8;
-constexpr int kPackB1 = kPackA3 + 1 * 16 * 64 * 8;
-constexpr int kPackB2 = kPackB1 + 128 * 4;
-constexpr int kPackB3 = kPackB2 + 128 * 4;
+constexpr int kPackA2 = kPackA1 + 4 * 64 * 8;
+constexpr int kPackA3 = kPackA2 + 4 * 9 * 64 * 16;
+constexpr int kPackB3 = kPackA3 + 8 * 64 * 16;
 constexpr int kPackBytes = kPackB3 + 32 * 4;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32v4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ short f2bf(float f) {
-    // round to nearest even (NaN not expected here)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return static_cast<short>(u >> 16);
+// Hidden unit carried by slot (half h, element j) of K-chunk kc (layers 2, 3).
+__host__ __device__ constexpr int mlp_chunk_unit(int kc, int h, int j) {
+    return 16 * kc + 8 * (j / 4) + 4 * h + j % 4;
 }
 
-__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
-    bf16x4 r;
-    r[0] = f2bf(a); r[1] = f2bf(b); r[2] = f2bf(c); r[3] = f2bf(d);
-    return r;
+// float -> bf16 bits, round to nearest even (host packing; the device converts
+// with v_cvt_pk_bf16_f32, which rounds the same way).
+inline uint16_t mlp_bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// Pack fp32 parameters (natural layout above) into the blob.
+inline void mlp_pack_blob(const float* p, unsigned char* blob) {
+    memset(blob, 0, kPackBytes);
+    auto put = [&](int byte_off, int j, float v) {
+        const uint16_t b = mlp_bf16_bits(v);
+        memcpy(blob + byte_off + 2 * j, &b, 2);
+    };
+    const float* W1 = p + kOffW1;
+    const float* b1 = p + kOffB1;
+    const float* W2 = p + kOffW2;
+    const float* W3 = p + kOffW3;
+    for (int mb = 0; mb < 4; ++mb)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 4; ++j) {
+                const int row = 32 * mb + lane % 32, k = 4 * (lane / 32) + j;
+                const float v = k < kMlpIn ? W1[row * kMlpIn + k] : (k == kMlpIn ? b1[row] : 0.f);
+                put(kPackA1 + (mb * 64 + lane) * 8, j, v);
+            }
+    const float* b2 = p + kOffB2;
+    for (int mb = 0; mb < 4; ++mb)
+        for (int kc = 0; kc < 9; ++kc)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int j = 0; j < 8; ++j) {
+                    const int row = 32 * mb + lane % 32;
+                    const float v = kc < 8 ? W2[row * kMlpH + mlp_chunk_unit(kc, lane / 32, j)]
+                                           : (lane / 32 == 0 && j == 0 ? b2[row] : 0.f);
+                    put(kPackA2 + ((mb * 9 + kc) * 64 + lane) * 16, j, v);
+                }
+    for (int kc = 0; kc < 8; ++kc)
+        for (int lane = 0; lane < 64; ++lane)
+            for (int j = 0; j < 8; ++j) {
+                const int row = lane % 32, k = mlp_chunk_unit(kc, lane / 32, j);
+                put(kPackA3 + (kc * 64 + lane) * 16, j, row < kMlpOut ? W3[row * kMlpH + k] : 0.f);
+            }
+    memcpy(blob + kPackB3, p + kOffB3, kMlpOut * 4);
+}
+
+// Two floats -> a bf16 pair (round to nearest even), then ReLU on the pair as
+// two signed 16-bit integers: a negative bf16 has its sign bit set, so the max
+// with 0 gives +0.  Rounding keeps the sign, so this is bf16(relu(x)).
+__device__ __forceinline__ uint32_t relu_pk(float a, float b) {
+    const f32x2 v = {a, b};
+    const bf16x2 r = __builtin_convertvector(v, bf16x2);
+    const s16x2 m = __builtin_elementwise_max(__builtin_bit_cast(s16x2, r), (s16x2){0, 0});
+    return __builtin_bit_cast(uint32_t, m);
+}
+
+// bf16 ReLU of accumulator registers c[base .. base+7] as one B operand.
+__device__ __forceinline__ bf16x8 relu_pack8(const f32x16& c, int base) {
+    const u32v4 u = {relu_pk(c[base], c[base + 1]), relu_pk(c[base + 2], c[base + 3]),
+                     relu_pk(c[base + 4], c[base + 5]), relu_pk(c[base + 6], c[base + 7])};
+    return __builtin_bit_cast(bf16x8, u);
 }
 
 // Copy the packed blob from global memory into LDS (all threads of the block).
@@ -70,83 +141,82 @@ __device__ __forceinline__ void mlp_stage(const uint4* __restrict__ blob, uint4*
     __syncthreads();
 }
 
-// Evaluate the MLP for the 64 ions of this wavefront; in[] are this lane's
-// inputs (in[6], in[7] = 0), out[] receives this lane's outputs.  Must be
-// called by all 64 lanes together (matrix instructions are wave-wide).
-__device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const float in[8], float out[3]) {
+// Evaluate the MLP for the 64 ions of this wavefront; in[0..5] are this lane's
+// inputs, out[] receives this lane's outputs.  Must be called by all 64 lanes
+// together (matrix instructions are wave-wide).
+__device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const float in[6], float out[3]) {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;  // 0: lanes 0-31, 1: lanes 32-63
     const bf16x4* A1 = reinterpret_cast<const bf16x4*>(lds + kPackA1);
-    const bf16x4* A2 = reinterpret_cast<const bf16x4*>(lds + kPackA2);
-    const bf16x4* A3 = reinterpret_cast<const bf16x4*>(lds + kPackA3);
-    const float* b1 = reinterpret_cast<const float*>(lds + kPackB1);
-    const float* b2 = reinterpret_cast<const float*>(lds + kPackB2);
+    const bf16x8* A2 = reinterpret_cast<const bf16x8*>(lds + kPackA2);
+    const bf16x8* A3 = reinterpret_cast<const bf16x8*>(lds + kPackA3);
     const float* b3 = reinterpret_cast<const float*>(lds + kPackB3);
 
-    // B operand of layer 1 for the two 32-ion blocks: lane l needs features
-    // 4*half..4*half+3 of ion 32*nb + l%32.  Own ion: nb == half.
-    float partner[8];
+    // B operand of layer 1 for the two 32-ion blocks: lane l needs slots
+    // 4*half..4*half+3 of ion 32*nb + l%32, slots = (x0..x5, 1, 0).
+    float partner[6];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) partner[j] = __shfl_xor(in[j], 32, 64);
-    bf16x4 X[2];
-    {
-        const float* own = in;
-        const float* oth = partner;
-        // nb = 0: lanes < 32 own ion (features 0-3), lanes >= 32 partner's ion (4-7)
-        X[0] = half == 0 ? pack4(own[0], own[1], own[2], own[3]) : pack4(oth[4], oth[5], oth[6], oth[7]);
-        // nb = 1: lanes < 32 partner's ion (0-3), lanes >= 32 own ion (4-7)
-        X[1] = half == 0 ? pack4(oth[0], oth[1], oth[2], oth[3]) : pack4(own[4], own[5], own[6], own[7]);
-    }
-    float res[2][3];
+    for (int j = 0; j < 6; ++j) partner[j] = __shfl_xor(in[j], 32, 64);
+    auto pack_in = [](float a, float b, float c, float d) {
+        bf16v4 r;
+        r[0] = (__bf16)a; r[1] = (__bf16)b; r[2] = (__bf16)c; r[3] = (__bf16)d;
+        return __builtin_bit_cast(bf16x4, r);
+    };
+    // nb = 0: lanes < 32 own ion (slots 0-3), lanes >= 32 partner's ion (4-7);
+    // nb = 1: lanes < 32 partner's ion (0-3), lanes >= 32 own ion (4-7)
+    const bf16x4 X0 = half == 0 ? pack_in(in[0], in[1], in[2], in[3])
+                                : pack_in(partner[4], partner[5], 1.0f, 0.0f);
+    const bf16x4 X1 = half == 0 ? pack_in(partner[0], partner[1], partner[2], partner[3])
+                                : pack_in(in[4], in[5], 1.0f, 0.0f);
+    // Both 32-ion blocks go through each weight fragment together, so every
+    // A fragment is read from LDS once per wavefront.
+    // layer 1 (bias in slot 6): chunk kc = 2*mb + q holds rows 32*mb + 16*q + ...
+    bf16x8 H0[8], H1[8];
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-        // layer 1: 4 row blocks of 32 hidden units
-        bf16x4 B2[16];
+    for (int mb = 0; mb < 4; ++mb) {
+        const bf16x4 a = A1[mb * 64 + lane];
+        f32x16 c0 = {}, c1 = {};
+        c0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, X0, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, X1, c1, 0, 0, 0);
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            f32x16 acc = {};
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A1[mb * 64 + lane], X[nb], acc, 0, 0, 0);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int row = 32 * mb + 8 * s + 4 * half;
-                const float4 bb = *reinterpret_cast<const float4*>(b1 + row);
-                B2[4 * mb + s] = pack4(fmaxf(acc[4 * s] + bb.x, 0.f), fmaxf(acc[4 * s + 1] + bb.y, 0.f),
-                                       fmaxf(acc[4 * s + 2] + bb.z, 0.f), fmaxf(acc[4 * s + 3] + bb.w, 0.f));
-            }
+        for (int q = 0; q < 2; ++q) {
+            H0[2 * mb + q] = relu_pack8(c0, 8 * q);
+            H1[2 * mb + q] = relu_pack8(c1, 8 * q);
         }
-        // layer 2
-        bf16x4 B3[16];
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            f32x16 acc = {};
-#pragma unroll
-            for (int ks = 0; ks < 16; ++ks)
-                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A2[(mb * 16 + ks) * 64 + lane], B2[ks], acc, 0, 0, 0);
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int row = 32 * mb + 8 * s + 4 * half;
-                const float4 bb = *reinterpret_cast<const float4*>(b2 + row);
-                B3[4 * mb + s] = pack4(fmaxf(acc[4 * s] + bb.x, 0.f), fmaxf(acc[4 * s + 1] + bb.y, 0.f),
-                                       fmaxf(acc[4 * s + 2] + bb.z, 0.f), fmaxf(acc[4 * s + 3] + bb.w, 0.f));
-            }
-        }
-        // layer 3: rows 0..2 of one 32-row block
-        f32x16 acc = {};
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks)
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A3[ks * 64 + lane], B3[ks], acc, 0, 0, 0);
-        // lanes < 32 hold rows 0..3 (v = 0..3) for ion 32*nb + lane
-        res[nb][0] = acc[0] + b3[0];
-        res[nb][1] = acc[1] + b3[1];
-        res[nb][2] = acc[2] + b3[2];
     }
+    // the constant operand of layer 2's bias chunk: 1 in slot (0, 0)
+    const bf16x8 one = __builtin_bit_cast(bf16x8, (u32v4){half == 0 ? 0x3F80u : 0u, 0u, 0u, 0u});
+    // layers 2 and 3, one 32-unit block of layer 2 at a time
+    f32x16 y0 = {}, y1 = {};
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        const bf16x8* A2m = A2 + mb * 9 * 64 + lane;
+        f32x16 c0 = {}, c1 = {};
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2m[8 * 64], one, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2m[8 * 64], one, c1, 0, 0, 0);
+#pragma unroll
+        for (int kc = 0; kc < 8; ++kc) {
+            const bf16x8 a = A2m[kc * 64];
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, H0[kc], c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, H1[kc], c1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const bf16x8 a3 = A3[(2 * mb + q) * 64 + lane];
+            y0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a3, relu_pack8(c0, 8 * q), y0, 0, 0, 0);
+            y1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a3, relu_pack8(c1, 8 * q), y1, 0, 0, 0);
+        }
+    }
+    // lanes < 32 hold output rows 0..2 (v = 0..2) of ion 32*nb + lane in y{nb}
+    const float res0[3] = {y0[0] + b3[0], y0[1] + b3[1], y0[2] + b3[2]};
+    const float res1[3] = {y1[0] + b3[0], y1[1] + b3[1], y1[2] + b3[2]};
     // ion `lane`: lanes < 32 read their nb = 0 result; lanes >= 32 take the
     // nb = 1 result computed in lane - 32
     float o1[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) o1[c] = __shfl_xor(res[1][c], 32, 64);
+    for (int c = 0; c < 3; ++c) o1[c] = __shfl_xor(res1[c], 32, 64);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[c] = half == 0 ? res[0][c] : o1[c];
+    for (int c = 0; c < 3; ++c) out[c] = half == 0 ? res0[c] : o1[c];
 }
 
 }  // namespace icarion
