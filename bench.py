@@ -61,6 +61,10 @@ MI355X_PEAK_BF16_TFLOPS = 2500.0
 #     reflection + centre of mass 60.
 FLOPS_PER_ION_STEP = {"fp64": 86, "fp32": 122}
 EHSS_FLOPS_STEP = 115
+# HBM bytes per ion per launch of step_surrogate_kernel, measured with rocprofv3
+# FETCH_SIZE / WRITE_SIZE (calibrated on a 1 GiB copy) at 1M ions x 2000 steps:
+# 139.0 MB read + 70.0 MB written (state round trip + each ion's species row).
+EHSS_TRAFFIC_BYTES_PER_ION_LAUNCH = 209.0
 EHSS_FLOPS_COLLISION = 2 * (8 * 128 + 128 * 128 + 128 * 3) + 2 * 128 + 2 * 128 + 3 + 60
 
 
@@ -278,7 +282,10 @@ def bench_ehss(args):
         },
         "roofline": {
             "bound": "mfma", "achieved": achieved, "peak": MI355X_PEAK_BF16_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK_BF16_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK_BF16_TFLOPS,
+            "traffic": EHSS_TRAFFIC_BYTES_PER_ION_LAUNCH * args.ions,
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated "
+                              "(profiles/step_surrogate_kernel_ehss_he_20td_1M.md)",
             "kernel": "step_surrogate_kernel<RK4, linear field>",
             "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
             "flops_per_ion_step": EHSS_FLOPS_STEP,
