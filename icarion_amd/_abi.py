@@ -5,12 +5,12 @@ sizes against offsets computed by the C compiler.
 """
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NO_DEVICE = range(6)
 
 INSTRUMENT_IMS, INSTRUMENT_TIMS, INSTRUMENT_QUADRUPOLE, INSTRUMENT_ORBITRAP = range(4)
-INTEGRATOR_RK4, INTEGRATOR_BORIS = range(2)
+INTEGRATOR_RK4, INTEGRATOR_BORIS, INTEGRATOR_RK45 = range(3)
 COLLISION_NONE, COLLISION_HSS, COLLISION_EHSS, COLLISION_LANGEVIN, COLLISION_FRICTION = range(5)
 PRECISION_FP64, PRECISION_FP32_KAHAN = range(2)
 EHSS_GEOMETRY, EHSS_SURROGATE = range(2)
@@ -70,14 +70,33 @@ class IonSoA(C.Structure):
         ("born", C.POINTER(C.c_uint8)), ("birth_time_s", C.POINTER(_d)),
         ("death_time_s", C.POINTER(_d)), ("arrival_time_s", C.POINTER(_d)),
         ("n_collisions", C.POINTER(_u32)),
+        ("time_s", C.POINTER(_d)), ("dt_s", C.POINTER(_d)),
     ]
 
 
 class StepStats(C.Structure):
     _fields_ = [
         ("ion_steps", _u64), ("collisions", _u64), ("alive", _u64), ("arrived", _u64),
-        ("t_end_s", _d), ("device_ms", _d),
+        ("rejected", _u64), ("t_end_s", _d), ("device_ms", _d),
     ]
+
+
+class Rk45Desc(C.Structure):
+    _fields_ = [
+        ("atol", _d), ("rtol", _d), ("safety_factor", _d), ("min_step_factor", _d),
+        ("max_step_factor", _d), ("max_step_increase", _d), ("max_step_decrease", _d),
+        ("absolute_min_step_s", _d), ("accept_at_dt_min", _i32), ("reserved0", _i32),
+    ]
+
+
+def rk45_defaults(**kw):
+    """RK45Strategy::AdaptiveConfig defaults (RK45Strategy.h:54-65)."""
+    d = Rk45Desc(atol=1e-8, rtol=1e-6, safety_factor=0.9, min_step_factor=1e-6,
+                 max_step_factor=1e3, max_step_increase=5.0, max_step_decrease=0.1,
+                 absolute_min_step_s=0.0, accept_at_dt_min=1)
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
 
 
 class GpuDesc(C.Structure):

@@ -29,6 +29,9 @@
 namespace icarion {
 hipError_t launch_step(int integrator, int field, int coll, int precision, int surrogate,
                        const StepParams& P, const IonPtrs& S, uint32_t block, hipStream_t stream);
+hipError_t launch_rk45_step(int field, int coll, const StepParams& P, const IonPtrs& S, const Rk45Ptrs& R,
+                            uint32_t block, hipStream_t stream);
+hipError_t launch_rk45_init(const Rk45Ptrs& R, uint32_t n, double t0, double dt, hipStream_t stream);
 hipError_t launch_synth_clusters(float4* atoms, const uint32_t* offsets, const uint32_t* counts,
                                  uint32_t n, float r_min, float r_max, uint64_t seed, hipStream_t s);
 hipError_t launch_oapa(const float4* atoms, const uint32_t* offsets, const uint32_t* counts, uint32_t n,
@@ -102,6 +105,17 @@ struct DevBuf {
     }
 };
 
+template <class T>
+void copy_in(DevBuf<T>& b, const T* src, size_t n, hipStream_t s, const char* what) {
+    b.resize(n);
+    if (n) hip_check(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s), what);
+}
+
+template <class T>
+void copy_out(T* dst, const DevBuf<T>& b, size_t n, hipStream_t s, const char* what) {
+    if (dst && n) hip_check(hipMemcpyAsync(dst, b.p, n * sizeof(T), hipMemcpyDeviceToHost, s), what);
+}
+
 constexpr double kPiHost = 3.14159265358979323846;
 constexpr double kBoltzmannHost = 1.380649e-23;
 
@@ -145,6 +159,12 @@ struct icarion_gpu {
     DevBuf<StepEntry> table;
     DevBuf<unsigned long long> counters;
     std::vector<StepEntry> table_host;
+
+    // RK45: per-ion time, dt and step-control state; the engine record
+    Rk45Cfg rk45{1e-8, 1e-6, 0.9, 1e-6, 1e3, 5.0, 0.1, 0.0, 1, 0};
+    DevBuf<double> r_t, r_dt, r_err, r_k1x, r_k1y, r_k1z;
+    DevBuf<uint8_t> r_fsal;
+    DevBuf<Rk45Engine> r_eng;
 };
 
 namespace {
@@ -176,8 +196,9 @@ int coll_kind(const icarion_physics_desc& p) {
 
 void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p) {
     if (d.instrument < 0 || d.instrument > 3) fail(ICARION_ERR_INVALID, "unknown instrument");
-    if (p.integrator != ICARION_INTEGRATOR_RK4 && p.integrator != ICARION_INTEGRATOR_BORIS)
-        fail(ICARION_ERR_UNSUPPORTED, "integrator not covered by the device path (RK4, Boris)");
+    if (p.integrator != ICARION_INTEGRATOR_RK4 && p.integrator != ICARION_INTEGRATOR_BORIS &&
+        p.integrator != ICARION_INTEGRATOR_RK45)
+        fail(ICARION_ERR_UNSUPPORTED, "integrator not covered by the device path (RK4, Boris, RK45)");
     if (p.collision_model < 0 || p.collision_model > 4) fail(ICARION_ERR_INVALID, "unknown collision model");
     if (p.precision != ICARION_PRECISION_FP64 && p.precision != ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_INVALID, "unknown precision");
@@ -204,6 +225,10 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
     P.ou_kicks = p.ou_thermal_kicks;
     P.detect = std::isfinite(d.detector_z_m) ? 1 : 0;
     P.has_rf = (d.instrument != ICARION_INSTRUMENT_ORBITRAP && std::fabs(d.rf_voltage_V) > 1e-12) ? 1 : 0;
+    P.rf_U = d.dc_quad_V;
+    P.rf_V = d.rf_voltage_V;
+    P.rf_omega = 2.0 * kPiHost * d.rf_frequency_Hz;
+    P.r0_sq = d.radius_m * d.radius_m;
     for (int i = 0; i < 3; ++i) {
         P.origin[i] = d.origin_m[i];
         P.B[i] = d.B_T[i];
@@ -300,8 +325,125 @@ IonPtrs ion_ptrs(icarion_gpu* h) {
     return S;
 }
 
+// The RK45 engine loop: one launch pair (step kernel + engine-time advance) per
+// engine step, replayed from a captured HIP graph of up to 32 steps.
+void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                  icarion_step_stats* stats) {
+    const int field = field_kind(h->dom);
+    const int coll = coll_kind(h->phys);
+    if (h->phys.precision != ICARION_PRECISION_FP64)
+        fail(ICARION_ERR_UNSUPPORTED, "RK45 runs in FP64");
+    if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
+        fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with the fixed-step FP32 path");
+    set_device(h);
+    hipStream_t s = h->stream;
+    StepParams P = h->base;
+    P.dt = dt;
+    P.species = h->species.p;
+    P.atoms = h->atoms_f32.p;
+    P.fa_f32 = h->fa32.p;
+    P.fa_f64 = h->fa64.p;
+    P.n_ions = static_cast<uint32_t>(h->n);
+    P.counters = h->counters.p;
+    P.rk45 = h->rk45;
+    fill_fast_const(P);
+    const IonPtrs S = ion_ptrs(h);
+    h->r_eng.resize(1);
+    Rk45Ptrs R{h->r_t.p, h->r_dt.p, h->r_err.p, h->r_k1x.p, h->r_k1y.p, h->r_k1z.p, h->r_fsal.p, h->r_eng.p};
+    Rk45Engine e0{t0, dt, step_offset, 0ull, ~0ull, 0ull, 0ull, 0ull};
+    hip_check(hipMemcpyAsync(h->r_eng.p, &e0, sizeof(e0), hipMemcpyHostToDevice, s), "upload engine record");
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, h->counters.p, 4);
+    hip_check(hipGetLastError(), "zero counters");
+    float ms = 0.0f;
+    if (h->n > 0) {
+        hip_check(launch_rk45_init(R, static_cast<uint32_t>(h->n), t0, dt, s), "rk45 init");
+        hip_check(hipEventRecord(h->ev0, s), "hipEventRecord");
+        const uint64_t G = std::min<uint64_t>(n_steps, 32);
+        uint64_t done = 0;
+        if (n_steps >= 2 * G && G > 1) {
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipError_t ce = hipSuccess;
+            for (uint64_t k = 0; k < G && ce == hipSuccess; ++k)
+                ce = launch_rk45_step(field, coll, P, S, R, h->block, s);
+            const hipError_t ee = hipStreamEndCapture(s, &graph);
+            if (ce == hipErrorInvalidValue) {
+                if (graph) (void)hipGraphDestroy(graph);
+                fail(ICARION_ERR_UNSUPPORTED, "no RK45 kernel for this field/collision");
+            }
+            hip_check(ce, "capture RK45 steps");
+            hip_check(ee, "hipStreamEndCapture");
+            const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (ie != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                hip_check(ie, "hipGraphInstantiate");
+            }
+            hipError_t le = hipSuccess;
+            for (; done + G <= n_steps && le == hipSuccess; done += G) le = hipGraphLaunch(exec, s);
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            hip_check(le, "hipGraphLaunch");
+        }
+        for (; done < n_steps; ++done) {
+            const hipError_t e = launch_rk45_step(field, coll, P, S, R, h->block, s);
+            if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no RK45 kernel for this field/collision");
+            hip_check(e, "RK45 step launch");
+        }
+        hip_check(hipEventRecord(h->ev1, s), "hipEventRecord");
+        hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    } else {
+        // no ions: the engine time still advances by the fallback dt
+        Rk45Engine e_empty = e0;
+        for (uint64_t k = 0; k < n_steps; ++k) e_empty.T = e_empty.T + dt;
+        e_empty.step = step_offset + n_steps;
+        hip_check(hipMemcpyAsync(h->r_eng.p, &e_empty, sizeof(e_empty), hipMemcpyHostToDevice, s),
+                  "upload engine record");
+    }
+    Rk45Engine e1{};
+    unsigned long long c[4] = {0, 0, 0, 0};
+    hip_check(hipMemcpyAsync(&e1, h->r_eng.p, sizeof(e1), hipMemcpyDeviceToHost, s), "read engine record");
+    hip_check(hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, s), "read counters");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (e1.failed)
+        fail(ICARION_ERR_INVALID, "RK45: " + std::to_string(e1.failed) +
+                                      " ion steps found no accepted step in 10 attempts");
+    if (stats) {
+        stats->ion_steps = c[0];
+        stats->collisions = c[1];
+        stats->rejected = e1.rejected;
+        stats->t_end_s = e1.T;
+        stats->device_ms = ms;
+        std::vector<uint8_t> fl(h->n);
+        std::vector<double> ar(h->n);
+        if (h->n) {
+            copy_out(fl.data(), h->flags, h->n, s, "read flags");
+            copy_out(ar.data(), h->arrival, h->n, s, "read arrival");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        }
+        uint64_t alive = 0, arrived = 0;
+        for (size_t i = 0; i < h->n; ++i) {
+            if ((fl[i] & 3) == 3) ++alive;
+            if (!std::isnan(ar[i])) ++arrived;
+        }
+        stats->alive = alive;
+        stats->arrived = arrived;
+    }
+}
+
 void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
              icarion_step_stats* stats) {
+    if (h->have_domain && h->phys.integrator == ICARION_INTEGRATOR_RK45) {
+        if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
+        if (!h->have_state) fail(ICARION_ERR_STATE, "icarion_gpu_upload_state not called");
+        if (!(dt > 0.0) || !std::isfinite(dt)) fail(ICARION_ERR_INVALID, "dt must be finite and > 0");
+        if (!std::isfinite(t0)) fail(ICARION_ERR_INVALID, "t0 must be finite");
+        if (field_kind(h->dom) == FIELD_ARRAY && !h->have_field)
+            fail(ICARION_ERR_STATE, "domain uses a field array but none was uploaded");
+        do_step_rk45(h, t0, dt, n_steps, step_offset, stats);
+        return;
+    }
     if (!h->have_domain) fail(ICARION_ERR_STATE, "icarion_gpu_set_domain not called");
     if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
     if (!h->have_state) fail(ICARION_ERR_STATE, "icarion_gpu_upload_state not called");
@@ -363,6 +505,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         stats->ion_steps = c[0];
         stats->collisions = c[1];
+        stats->rejected = 0;
         stats->t_end_s = t_end;
         stats->device_ms = ms;
         // alive / arrived from the flags and arrival arrays
@@ -385,16 +528,6 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     }
 }
 
-template <class T>
-void copy_in(DevBuf<T>& b, const T* src, size_t n, hipStream_t s, const char* what) {
-    b.resize(n);
-    if (n) hip_check(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s), what);
-}
-
-template <class T>
-void copy_out(T* dst, const DevBuf<T>& b, size_t n, hipStream_t s, const char* what) {
-    if (dst && n) hip_check(hipMemcpyAsync(dst, b.p, n * sizeof(T), hipMemcpyDeviceToHost, s), what);
-}
 
 void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
     if (!ions) fail(ICARION_ERR_INVALID, "ions is NULL");
@@ -428,21 +561,25 @@ void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
     copy_in(h->flags, fl.data(), n, s, "upload flags");
     h->has_birth = ions->birth_time_s != nullptr;
     if (h->has_birth) copy_in(h->birth, ions->birth_time_s, n, s, "upload birth");
-    std::vector<double> tmp(n);
-    if (ions->death_time_s) {
-        copy_in(h->death, ions->death_time_s, n, s, "upload death");
-    } else {
-        std::fill(tmp.begin(), tmp.end(), -1.0);
-        copy_in(h->death, tmp.data(), n, s, "upload death");
-    }
-    if (ions->arrival_time_s) {
-        copy_in(h->arrival, ions->arrival_time_s, n, s, "upload arrival");
-    } else {
-        std::fill(tmp.begin(), tmp.end(), NAN);
-        copy_in(h->arrival, tmp.data(), n, s, "upload arrival");
-    }
+    // host staging buffers stay alive until the stream synchronisation below
+    std::vector<double> dead(ions->death_time_s ? 0 : n, -1.0);
+    std::vector<double> arr(ions->arrival_time_s ? 0 : n, NAN);
+    copy_in(h->death, ions->death_time_s ? ions->death_time_s : dead.data(), n, s, "upload death");
+    copy_in(h->arrival, ions->arrival_time_s ? ions->arrival_time_s : arr.data(), n, s, "upload arrival");
     std::vector<uint32_t> nc(n, 0);
     copy_in(h->ncoll, ions->n_collisions ? ions->n_collisions : nc.data(), n, s, "upload ncoll");
+    // RK45 state: ion time / dt from the caller (NaN / <= 0 = set by the first
+    // step), fresh step-control memory (RK45Strategy.h:192-198)
+    std::vector<double> tt(ions->time_s ? 0 : n, NAN), dd(ions->dt_s ? 0 : n, 0.0);
+    std::vector<double> ones(n, 1.0), zeros(n, 0.0);
+    std::vector<uint8_t> nofsal(n, 0);
+    copy_in(h->r_t, ions->time_s ? ions->time_s : tt.data(), n, s, "upload time");
+    copy_in(h->r_dt, ions->dt_s ? ions->dt_s : dd.data(), n, s, "upload dt");
+    copy_in(h->r_err, ones.data(), n, s, "upload rk45 error");
+    copy_in(h->r_k1x, zeros.data(), n, s, "upload rk45 k1");
+    copy_in(h->r_k1y, zeros.data(), n, s, "upload rk45 k1");
+    copy_in(h->r_k1z, zeros.data(), n, s, "upload rk45 k1");
+    copy_in(h->r_fsal, nofsal.data(), n, s, "upload rk45 fsal");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     h->n = n;
     h->have_state = true;
@@ -464,6 +601,8 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
     copy_out(ions->death_time_s, h->death, n, s, "download death");
     copy_out(ions->arrival_time_s, h->arrival, n, s, "download arrival");
     copy_out(ions->n_collisions, h->ncoll, n, s, "download ncoll");
+    copy_out(ions->time_s, h->r_t, n, s, "download time");
+    copy_out(ions->dt_s, h->r_dt, n, s, "download dt");
     std::vector<uint8_t> fl(n);
     copy_out(fl.data(), h->flags, n, s, "download flags");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -644,6 +783,8 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->species.release(); h->atoms_f32.release(); h->atom_offsets.release(); h->atom_counts.release();
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
+    h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
+    h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
     h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
@@ -663,6 +804,24 @@ int icarion_gpu_set_domain(icarion_gpu* h, const icarion_domain_desc* domain,
         h->phys = *physics;
         h->base = derive_params(*domain, *physics);
         h->have_domain = true;
+    });
+}
+
+int icarion_gpu_set_rk45(icarion_gpu* h, const icarion_rk45_desc* d) {
+    return guarded([&] {
+        if (!h || !d) fail(ICARION_ERR_INVALID, "NULL argument");
+        // RK45Strategy(const AdaptiveConfig&) (RK45Strategy.cpp:179-195)
+        if (!(d->atol > 0.0) || !(d->rtol > 0.0)) fail(ICARION_ERR_INVALID, "RK45 tolerances must be positive");
+        if (!(d->safety_factor > 0.0 && d->safety_factor < 1.0))
+            fail(ICARION_ERR_INVALID, "RK45 safety factor must be in (0, 1)");
+        if (!(d->absolute_min_step_s >= 0.0))
+            fail(ICARION_ERR_INVALID, "RK45 absolute_min_step_s must be non-negative");
+        if (!(d->min_step_factor > 0.0 && d->max_step_factor > 0.0 && d->max_step_increase > 0.0 &&
+              d->max_step_decrease > 0.0))
+            fail(ICARION_ERR_INVALID, "RK45 step factors must be positive");
+        h->rk45 = Rk45Cfg{d->atol, d->rtol, d->safety_factor, d->min_step_factor, d->max_step_factor,
+                          d->max_step_increase, d->max_step_decrease, d->absolute_min_step_s,
+                          d->accept_at_dt_min ? 1 : 0, 0};
     });
 }
 

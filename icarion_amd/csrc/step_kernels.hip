@@ -13,329 +13,10 @@
 // the reference's per-ion result.  Per-step wave-uniform values (engine time,
 // RF coefficient) come from a table read with scalar loads.  Counters are
 // reduced per wavefront before one atomic per wave.
-#include "device_common.hpp"
-#include "step_params.hpp"
+#include "step_device.hpp"
 #include "surrogate.hpp"
 
 namespace icarion {
-
-// ======================= FP64 (plain method) ===============================
-
-using d3 = vec3<double>;
-
-__device__ __forceinline__ bool contains_f64(const StepParams& P, d3 p) {
-    d3 l{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]};
-    if (!P.rot_identity) l = mat_mul(P.g2l, l);
-    if (P.instrument == 3) {
-        // OrbitrapGeometry::contains (OrbitrapGeometry.h:36-51).  The electrode
-        // radius r_e(z) is the root of f(r) = z^2 - (r^2 - R^2)/2 - Rm^2 ln(R/r),
-        // increasing in r below Rm, so r >= r_e(z) <=> f(r) >= 0 (no bisection).
-        const double z_abs = fabs(l.z);
-        if (z_abs > 0.5 * P.L + kGeomEpsilon) return false;
-        const double r2 = l.x * l.x + l.y * l.y;
-        const double r = sqrt(r2);
-        if (r > P.R_out + kGeomEpsilon) return false;
-        const double ln_r = 0.5 * log(r2);
-        const double z2 = z_abs * z_abs;
-        const double f_in = z2 - 0.5 * (r2 - P.R_in * P.R_in) - P.R_char_sq * (P.ln_R_in - ln_r);
-        const double f_out = z2 - 0.5 * (r2 - P.R_out * P.R_out) - P.R_char_sq * (P.ln_R_out - ln_r);
-        return f_in >= 0.0 && f_out <= 0.0;
-    }
-    if (l.z < -kGeomEpsilon || l.z >= P.L + kGeomEpsilon) return false;
-    const double r2 = l.x * l.x + l.y * l.y;
-    const double Re = P.R + kGeomEpsilon;
-    return r2 <= Re * Re;
-}
-
-// trilinear interpolation with clamping (fieldArrayLoader.cpp:103-162) on a
-// uniform grid: the axis search becomes an index computation.
-__device__ __forceinline__ void locate_f64(double v, double v0, double v_last, double inv_d,
-                                           double d, int n, int& i0, double& a) {
-    if (v <= v0) {
-        i0 = 0;
-        a = 0.0;
-    } else if (v >= v_last) {
-        i0 = n - 2;
-        a = 1.0;
-    } else {
-        int i = static_cast<int>((v - v0) * inv_d);
-        i = i > n - 2 ? n - 2 : (i < 0 ? 0 : i);
-        i0 = i;
-        a = (v - (v0 + static_cast<double>(i) * d)) * inv_d;
-    }
-}
-
-__device__ __forceinline__ d3 field_array_f64(const StepParams& P, d3 p) {
-    int ix, iy, iz;
-    double ax, ay, az;
-    locate_f64(p.x, P.x0, P.x_last, P.inv_dx, P.dx, P.nx, ix, ax);
-    locate_f64(p.y, P.y0, P.y_last, P.inv_dy, P.dy, P.ny, iy, ay);
-    locate_f64(p.z, P.z0, P.z_last, P.inv_dz, P.dz, P.nz, iz, az);
-    d3 r{0.0, 0.0, 0.0};
-    const size_t sx = 1, sy = static_cast<size_t>(P.nx), sz = static_cast<size_t>(P.nx) * P.ny;
-    const size_t base = ix + sy * iy + sz * iz;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int dx = c >> 2, dy = (c >> 1) & 1, dzz = c & 1;
-        const double w = (dx ? ax : 1.0 - ax) * (dy ? ay : 1.0 - ay) * (dzz ? az : 1.0 - az);
-        const size_t idx = base + dx * sx + dy * sy + dzz * sz;
-        double ex, ey, ez;
-        if (P.fa_f64) {
-            const double4 e = P.fa_f64[idx];
-            ex = e.x; ey = e.y; ez = e.z;
-        } else {
-            const float4 e = P.fa_f32[idx];
-            ex = e.x; ey = e.y; ez = e.z;
-        }
-        r.x += w * ex;
-        r.y += w * ey;
-        r.z += w * ez;
-    }
-    if (P.fa_scale != 1.0) r = r * P.fa_scale;
-    return r;
-}
-
-template <int FIELD>
-__device__ __forceinline__ d3 field_f64(const StepParams& P, d3 p, double c_rf) {
-    if constexpr (FIELD == FIELD_LINEAR) {
-        // AnalyticalFieldModel.cpp:136-161 (IMS) and :219-248 (quadrupole)
-        d3 E{0.0, 0.0, P.Ez_const};
-        if (P.has_rf) {
-            E.x = -2.0 * p.x * c_rf;
-            E.y = 2.0 * p.y * c_rf;
-        }
-        return E;
-    } else if constexpr (FIELD == FIELD_ORBITRAP) {
-        // AnalyticalFieldModel.cpp:188-217
-        d3 l{p.x - P.origin[0], p.y - P.origin[1], p.z - P.origin[2]};
-        if (!P.rot_identity) l = mat_mul(P.g2l, l);
-        const double r_sq = fmax(1e-18, l.x * l.x + l.y * l.y);
-        const double C = 1.0 - P.R_char_sq / r_sq;
-        const double hk = 0.5 * P.orbi_k;
-        d3 E{hk * l.x * C, hk * l.y * C, -P.orbi_k * l.z};
-        if (!P.rot_identity) E = mat_mul(P.l2g, E);
-        return E;
-    } else {
-        return field_array_f64(P, p);
-    }
-}
-
-// DampingForce::calculate_gamma (DampingForce.cpp:105-277) with the per-ion
-// factor hoisted: Langevin gamma = lpre / max(1e-6, |v|); Friction constant.
-__device__ __forceinline__ double gamma_f64(const StepParams& P, double lpre, d3 v) {
-    if (P.damping_model == 3) return lpre / fmax(1e-6, sqrt(dot(v, v)));
-    return lpre;
-}
-
-template <int FIELD, int COLL>
-__device__ __forceinline__ d3 force_f64(const StepParams& P, d3 p, d3 v, double c_rf, double q,
-                                        double m, double lpre) {
-    d3 F = field_f64<FIELD>(P, p, c_rf) * q;
-    if constexpr (COLL == COLL_DAMPING) {
-        const double g = gamma_f64(P, lpre, v);
-        if (g > 0.0) {
-            const d3 rel{v.x - P.u_gas[0], v.y - P.u_gas[1], v.z - P.u_gas[2]};
-            F = F + rel * (-g * m);
-        }
-    }
-    return F;
-}
-
-__device__ __forceinline__ d3 iso_dir_f64(PhiloxStream& rng) {
-    const double u1 = rng.uniform_f64();
-    const double u2 = rng.uniform_f64();
-    const double cosT = 2.0 * u1 - 1.0;
-    const double sinT = sqrt(fmax(0.0, 1.0 - cosT * cosT));
-    const double phi = 2.0 * kPi * u2;
-    double s, c;
-    sincos(phi, &s, &c);
-    return d3{sinT * c, sinT * s, cosT};
-}
-
-__device__ __forceinline__ void ortho_basis_f64(d3 e, d3& t1, d3& t2) {
-    const d3 a = (fabs(e.x) < 0.9) ? d3{1, 0, 0} : d3{0, 1, 0};
-    d3 c1 = cross(e, a);
-    double n1 = sqrt(dot(c1, c1));
-    t1 = n1 > 0.0 ? d3{c1.x / n1, c1.y / n1, c1.z / n1} : d3{0, 0, 0};
-    d3 c2 = cross(e, t1);
-    double n2 = sqrt(dot(c2, c2));
-    t2 = n2 > 0.0 ? d3{c2.x / n2, c2.y / n2, c2.z / n2} : d3{0, 0, 0};
-}
-
-// CollisionKernels::ehss_collision (CollisionKernels.cpp:13-137, 241-381):
-// Shoemake orientation, first-contact ray test, specular reflection.  Rotated
-// atoms are recomputed per attempt instead of stored (same arithmetic).
-__device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn,
-                       uint32_t a0, uint32_t nat, PhiloxStream& rng) {
-    const double u1 = rng.uniform_f64();
-    const double u2 = rng.uniform_f64();
-    const double u3 = rng.uniform_f64();
-    if (nat == 0) return v_ion;
-    const double q1 = sqrt(1.0 - u1) * sin(2.0 * kPi * u2);
-    const double q2 = sqrt(1.0 - u1) * cos(2.0 * kPi * u2);
-    const double q3 = sqrt(u1) * sin(2.0 * kPi * u3);
-    const double q4 = sqrt(u1) * cos(2.0 * kPi * u3);
-    const double x = q1, y = q2, z = q3, w = q4;
-    double O[9];
-    O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
-    O[3] = 2 * (x * y + z * w);     O[4] = 1 - 2 * (x * x + z * z); O[5] = 2 * (y * z - x * w);
-    O[6] = 2 * (x * z - y * w);     O[7] = 2 * (y * z + x * w);     O[8] = 1 - 2 * (x * x + y * y);
-    d3 vrel = v_ion - v_n;
-    double vrel_mag = sqrt(dot(vrel, vrel));
-    if (vrel_mag <= kMinVelocityMag) {
-        vrel = iso_dir_f64(rng) * kMinVelocityMag;
-        vrel_mag = kMinVelocityMag;
-    }
-    const d3 ax{vrel.x / vrel_mag, vrel.y / vrel_mag, vrel.z / vrel_mag};
-    d3 t1, t2;
-    ortho_basis_f64(ax, t1, t2);
-    const double Bm[3][3] = {{t1.x, t2.x, ax.x}, {t1.y, t2.y, ax.y}, {t1.z, t2.z, ax.z}};
-    double Rm[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            Rm[3 * i + j] = Bm[i][0] * O[j] + Bm[i][1] * O[3 + j] + Bm[i][2] * O[6 + j];
-    auto atom = [&](uint32_t j) -> double4 {
-        const float4 f = P.atoms[a0 + j];
-        return make_double4(f.x, f.y, f.z, f.w);
-    };
-    double b_max = 0.0;
-    for (uint32_t j = 0; j < nat; ++j) {
-        const double4 c = atom(j);
-        const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
-        const double sra = dot(ax, ra);
-        const d3 rp = ra - ax * sra;
-        const double cur = sqrt(dot(rp, rp)) + c.w;
-        if (cur > b_max) b_max = cur;
-    }
-    b_max += P.r_gas;
-    for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
-        const double uA = rng.uniform_f64();
-        const double uB = rng.uniform_f64();
-        const double b = b_max * sqrt(uA);
-        double sphi, cphi;
-        sincos(2.0 * kPi * uB, &sphi, &cphi);
-        const d3 off = t1 * (b * cphi) + t2 * (b * sphi);
-        bool hit = false;
-        d3 nb{0, 0, 0};
-        double best_s = 0.0;
-        for (uint32_t j = 0; j < nat; ++j) {
-            const double4 c = atom(j);
-            const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
-            const double Rsum = c.w + P.r_gas;
-            const double Rsum2 = Rsum * Rsum;
-            const d3 rel = off - ra;
-            const double sstar = dot(ax, rel);
-            const d3 dv = rel - ax * sstar;
-            const double dmin2 = dot(dv, dv);
-            if (dmin2 <= Rsum2) {
-                const double h = sqrt(fmax(0.0, Rsum2 - dmin2));
-                const double s_hit = sstar - h;
-                const d3 ph = rel - ax * s_hit;
-                const double phr2 = dot(ph, ph);
-                d3 nc;
-                if (phr2 <= kMinContactDistSq) {
-                    nc = ax * -1.0;
-                } else {
-                    nc = ph * (1.0 / sqrt(phr2));
-                    if (dot(ax, nc) > 0.0) nc = nc * -1.0;
-                }
-                if (!hit || s_hit < best_s) {
-                    hit = true;
-                    nb = nc;
-                    best_s = s_hit;
-                }
-            }
-        }
-        if (hit) {
-            const double vdn = dot(vrel, nb);
-            const d3 vr = vrel - nb * (2.0 * vdn);
-            const double inv_mt = 1.0 / (mi + mn);
-            const d3 Vcom = (v_ion * mi + v_n * mn) * inv_mt;
-            return Vcom + vr * (mn * inv_mt);
-        }
-        if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
-    }
-    return v_ion;
-}
-
-// One collision stage; returns true on a collision event.
-template <int COLL>
-__device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDev& sp, double m,
-                                            double lpre, d3& v, PhiloxStream& rng) {
-    if constexpr (COLL == COLL_HSS || COLL == COLL_EHSS) {
-        const double sigma = (COLL == COLL_HSS) ? sp.sigma_hss : sp.sigma_ehss;
-        if (sigma <= 0.0) return false;
-        // VelocitySampling::sample_neutral_velocity (VelocitySampling.cpp:9-35)
-        d3 vn;
-        {
-            double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
-            vn.x = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
-            vn.y = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
-            vn.z = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-            vn = vn + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
-        }
-        const d3 vrel = v - vn;
-        const double vrel_mag = sqrt(dot(vrel, vrel));
-        if (vrel_mag < kMinRelativeVelocity) return false;
-        const double Pc = 1.0 - exp(-P.n * sigma * vrel_mag * P.dt);
-        if (rng.uniform_f64() >= Pc) return false;
-        if constexpr (COLL == COLL_HSS) {
-            // CollisionKernels::hss_collision (CollisionKernels.cpp:139-169)
-            d3 vr = vrel;
-            double vm = vrel_mag;
-            if (vm <= kMinVelocityMag) {
-                vr = iso_dir_f64(rng) * kMinVelocityMag;
-                vm = kMinVelocityMag;
-            }
-            const d3 nd = iso_dir_f64(rng);
-            const double inv_mt = 1.0 / (m + P.m_gas);
-            const d3 Vcom = (v * m + vn * P.m_gas) * inv_mt;
-            v = Vcom + (nd * vm) * (P.m_gas * inv_mt);
-        } else {
-            v = ehss_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, rng);
-        }
-        return true;
-    } else if constexpr (COLL == COLL_DAMPING) {
-        if (P.ou_kicks) {
-            // ou_velocity_update, apply_damping = false (CollisionKernels.cpp:204-213)
-            const double g = gamma_f64(P, lpre, v);
-            if (g <= 0.0 || P.dt <= 0.0) return false;
-            const double kBT_over_m = kBoltzmann * P.T / m;
-            const double s = sqrt(2.0 * g * kBT_over_m * P.dt);
-            double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
-            double r = sqrt(-2.0 * log(u1));
-            double sn, cs;
-            sincos(2.0 * kPi * u2, &sn, &cs);
-            v.x += s * (r * cs);
-            v.y += s * (r * sn);
-            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
-            r = sqrt(-2.0 * log(u1));
-            v.z += s * (r * cos(2.0 * kPi * u2));
-        }
-        return false;
-    } else {
-        return false;
-    }
-}
-
-__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ void add_counters(const StepParams& P, uint32_t pushed, uint32_t coll) {
-    const uint32_t sp = wave_sum_u32(pushed);
-    const uint32_t sc = wave_sum_u32(coll);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&P.counters[0], static_cast<unsigned long long>(sp));
-        atomicAdd(&P.counters[1], static_cast<unsigned long long>(sc));
-    }
-}
 
 template <int INTEG, int FIELD, int COLL>
 __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const IonPtrs S) {
@@ -427,7 +108,7 @@ __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const
             ++pushed;
             if constexpr (COLL != COLL_NONE) {
                 PhiloxStream rng(P.seed, i, P.step0 + k);
-                if (collide_f64<COLL>(P, sp, m, lpre, vel, rng)) {
+                if (collide_f64<COLL>(P, sp, m, lpre, vel, rng, dt)) {
                     ++nc;
                     ++ncoll_launch;
                 }

@@ -52,6 +52,32 @@ struct IonPtrs {
     uint32_t *ncoll;
 };
 
+// RK45 adaptive step control (RK45Strategy::AdaptiveConfig, RK45Strategy.h:54-65).
+struct Rk45Cfg {
+    double atol, rtol, safety, min_factor, max_factor, max_inc, max_dec, abs_min;
+    int accept_at_dt_min, pad;
+};
+
+// Per-ion RK45 state resident on the device (RK45Strategy::RK45State,
+// RK45Strategy.h:192-198, and the engine's dt_per_ion_) plus the engine record
+// that carries the shared engine time from one step launch to the next.
+struct Rk45Engine {
+    double T;                       // engine time of the step about to run
+    double dt_fallback;             // config dt (SimulationEngineUtils.cpp:82)
+    unsigned long long step;        // absolute step number (RNG counter)
+    unsigned long long tmax;        // ordered bits: max ion time after the step
+    unsigned long long bmin;        // ordered bits: min birth > T of unborn ions
+    unsigned long long dtmax;       // ordered bits: max accepted dt
+    unsigned long long rejected;    // rejected trial steps
+    unsigned long long failed;      // ions with no accepted trial
+};
+
+struct Rk45Ptrs {
+    double *t, *dt, *last_err, *k1x, *k1y, *k1z;
+    uint8_t *fsal;
+    Rk45Engine *eng;
+};
+
 // FP32 copies of the run constants used by the fast path, rounded once on the
 // host (kernel arguments: wave-uniform, so they stay in scalar registers).
 struct FastConst {
@@ -88,6 +114,7 @@ struct StepParams {
     double R_char_sq;
     double fa_scale;
     double B[3];
+    double rf_U, rf_V, rf_omega, r0_sq;   // U_eff(t) = U + V cos(omega t) (RK45 evaluates it)
     // environment
     double T;
     double n;
@@ -120,6 +147,7 @@ struct StepParams {
     uint32_t n_ions;
     unsigned long long *counters;   // [0] ion-steps, [1] collisions
     FastConst fc;
+    Rk45Cfg rk45;
 };
 
 // Fill P.fc from the FP64 fields (after P.dt is set).

@@ -14,7 +14,7 @@ def _ptr(a, ctype):
 
 class Ensemble:
     FIELDS_F64 = ("x", "y", "z", "vx", "vy", "vz", "mass_kg", "charge_C", "death_time_s",
-                  "arrival_time_s")
+                  "arrival_time_s", "time_s", "dt_s")
 
     def __init__(self, n):
         self.n = int(n)
@@ -27,6 +27,7 @@ class Ensemble:
         self.n_collisions = np.zeros(self.n, np.uint32)
         self.death_time_s[:] = -1.0
         self.arrival_time_s[:] = np.nan
+        self.time_s[:] = np.nan   # ion time t_i; NaN = the t0 of the first step
 
     def copy(self):
         e = Ensemble(self.n)
@@ -49,8 +50,7 @@ class Ensemble:
         d = C.c_double
         s = _abi.IonSoA()
         s.n = self.n
-        for f in ("x", "y", "z", "vx", "vy", "vz", "mass_kg", "charge_C", "death_time_s",
-                  "arrival_time_s"):
+        for f in self.FIELDS_F64:
             setattr(s, f, _ptr(getattr(self, f), d))
         s.species = _ptr(self.species, C.c_uint32)
         s.active = _ptr(self.active, C.c_uint8)

@@ -21,6 +21,7 @@ EXPORTED = (
     "icarion_gpu_synchronize", "icarion_gpu_synth_clusters", "icarion_gpu_ehss_precompute",
     "icarion_gpu_surrogate_set_params", "icarion_gpu_surrogate_get_params",
     "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
+    "icarion_gpu_set_rk45",
 )
 
 
@@ -48,6 +49,7 @@ def load(path=None):
     lib.icarion_gpu_create.argtypes = [P(_abi.GpuDesc), P(C.c_void_p)]
     lib.icarion_gpu_destroy.argtypes = [C.c_void_p]
     lib.icarion_gpu_set_domain.argtypes = [C.c_void_p, P(_abi.DomainDesc), P(_abi.PhysicsDesc)]
+    lib.icarion_gpu_set_rk45.argtypes = [C.c_void_p, P(_abi.Rk45Desc)]
     lib.icarion_gpu_set_species.argtypes = [C.c_void_p, P(_abi.SpeciesDesc), C.c_uint32,
                                             C.c_void_p, C.c_uint32, C.c_int32]
     f32p, f64p = P(C.c_float), P(C.c_double)
@@ -134,6 +136,10 @@ class Handle:
     def set_domain(self, domain, physics):
         _check(self.lib.icarion_gpu_set_domain(self.h, C.byref(domain), C.byref(physics)),
                self.lib)
+
+    def set_rk45(self, desc):
+        """RK45 step control (an _abi.Rk45Desc, e.g. _abi.rk45_defaults(atol=...))."""
+        _check(self.lib.icarion_gpu_set_rk45(self.h, C.byref(desc)), self.lib)
 
     def set_species(self, species_list, atoms_xyzr=None):
         """atoms_xyzr: (n_atoms, 4) float32 or float64 array (x, y, z, r in metres)."""

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define ICARION_GPU_ABI_VERSION 1
+#define ICARION_GPU_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -59,7 +59,10 @@ enum icarion_instrument {
 /* simulation.integrator */
 enum icarion_integrator {
     ICARION_INTEGRATOR_RK4 = 0,    /* RK4Strategy.cpp:181-361 */
-    ICARION_INTEGRATOR_BORIS = 1   /* BorisStrategy.cpp:18-98 */
+    ICARION_INTEGRATOR_BORIS = 1,  /* BorisStrategy.cpp:18-98 */
+    ICARION_INTEGRATOR_RK45 = 2    /* RK45Strategy::step (RK45Strategy.cpp:354-520):
+                                      Dormand-Prince 5(4), per-ion adaptive dt,
+                                      engine time = max of the ion times */
 };
 
 /* physics.collision_model */
@@ -142,6 +145,22 @@ typedef struct icarion_physics_desc {
     uint64_t rng_seed;            /* simulation.rng_seed                      */
 } icarion_physics_desc;
 
+/* Adaptive step control of ICARION_INTEGRATOR_RK45
+ * (RK45Strategy::AdaptiveConfig, RK45Strategy.h:54-65; defaults in brackets,
+ * used until icarion_gpu_set_rk45 is called). */
+typedef struct icarion_rk45_desc {
+    double atol;                  /* absolute tolerance [m, m/s]     (1e-8)   */
+    double rtol;                  /* relative tolerance              (1e-6)   */
+    double safety_factor;         /*                                 (0.9)    */
+    double min_step_factor;       /* dt_min = dt_ion * factor        (1e-6)   */
+    double max_step_factor;       /* dt_max = dt_ion * factor        (1e3)    */
+    double max_step_increase;     /* growth clamp per step           (5.0)    */
+    double max_step_decrease;     /* shrink clamp per step           (0.1)    */
+    double absolute_min_step_s;   /* dt floor, 0 = none              (0.0)    */
+    int32_t accept_at_dt_min;     /* accept at dt_min even if err > 1 (1)     */
+    int32_t reserved0;
+} icarion_rk45_desc;
+
 /* Per-species constants, resolved on the host for the domain's gas
  * (species-DB priorities: HSSCollisionHandler.cpp:263-310,
  *  EHSSCollisionHandler.cpp:462-547). */
@@ -185,6 +204,13 @@ typedef struct icarion_ion_soa {
     double *arrival_time_s;       /* (out; first detector-plane crossing,
                                      NaN if none)                              */
     uint32_t *n_collisions;       /* (out; collision events so far)            */
+    double *time_s;               /* (in/out) ion time t_i (IonEnsemble output
+                                     block); RK45 advances it by each ion's
+                                     accepted dt.  NULL or NaN on upload = the
+                                     t0 of the first step                      */
+    double *dt_s;                 /* (in/out) RK45 per-ion dt (the engine's
+                                     dt_per_ion_); NULL or <= 0 on upload = the
+                                     dt of the first step                      */
 } icarion_ion_soa;
 
 /* Counters of one icarion_gpu_step() call. */
@@ -193,6 +219,7 @@ typedef struct icarion_step_stats {
     uint64_t collisions;          /* collision events                          */
     uint64_t alive;               /* active and born after the call            */
     uint64_t arrived;             /* ions with an arrival time after the call  */
+    uint64_t rejected;            /* RK45: rejected trial steps                */
     double t_end_s;               /* engine time after the call                */
     double device_ms;             /* device time of the stepping kernels       */
 } icarion_step_stats;
@@ -260,6 +287,10 @@ int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
 
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
+
+/* Step control of ICARION_INTEGRATOR_RK45 (RK45Strategy(const AdaptiveConfig&),
+ * RK45Strategy.cpp:179-195: positive tolerances, safety factor in (0, 1)). */
+int icarion_gpu_set_rk45(icarion_gpu *h, const icarion_rk45_desc *desc);
 
 /* ---- EHSS geometry and the scattering surrogate ---------------------------- */
 

@@ -80,6 +80,20 @@ def load():
     lib.oracle_mlp_forward.argtypes = [f32p, f32p, C.c_size_t, f32p]
     lib.oracle_mlp_loss_grad.argtypes = [f32p, f32p, f32p, C.c_size_t, P(d)]
     lib.oracle_mlp_loss_grad.restype = d
+    R = P(_abi.Rk45Desc)
+    lib.oracle_rk45_error.argtypes = [R, P(d), P(d), P(d)]
+    lib.oracle_rk45_error.restype = d
+    lib.oracle_rk45_new_step.argtypes = [R, d, d, d, d, d]
+    lib.oracle_rk45_new_step.restype = d
+    lib.oracle_rk45_push_one.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), R,
+                                         P(_abi.SpeciesDesc), P(_abi.OracleField), d, d, P(d),
+                                         P(d), d, d, C.c_void_p, P(d), P(C.c_int)]
+    lib.oracle_rk45_push_one.restype = d
+    lib.oracle_run_rk45.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), R,
+                                    P(_abi.SpeciesDesc), C.c_uint32, P(d), C.c_uint32,
+                                    P(_abi.OracleField), P(_abi.IonSoA), C.c_void_p, d, d,
+                                    C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats)]
+    lib.oracle_run_rk45.restype = C.c_int
     _lib = lib
     return lib
 
@@ -232,3 +246,67 @@ def mlp_loss_grad(params, X, T):
     g = np.zeros(params.size)
     loss = lib.oracle_mlp_loss_grad(_fp(params), _fp(X), _fp(T), X.shape[0], _dp(g))
     return loss, g
+
+
+# oracle_rk45_ion: t, dt, last_error, k1a[3], fsal, reserved
+RK45_ION = np.dtype([("t", "f8"), ("dt", "f8"), ("last_error", "f8"), ("k1a", "f8", 3),
+                     ("fsal", "i4"), ("reserved", "i4")])
+
+
+def rk45_state(ens):
+    """Fresh per-ion RK45 state from an ensemble (time_s NaN and dt_s <= 0 are
+    filled in by the run from t0 and dt)."""
+    st = np.zeros(ens.n, RK45_ION)
+    st["t"] = ens.time_s
+    st["dt"] = ens.dt_s
+    st["last_error"] = 1.0
+    return st
+
+
+def rk45_error(cfg, y4, y5, y0):
+    lib = load()
+    return lib.oracle_rk45_error(C.byref(cfg), vec(*y4), vec(*y5), vec(*y0))
+
+
+def rk45_new_step(cfg, dt, err, dt_min, dt_max, prev_err):
+    return load().oracle_rk45_new_step(C.byref(cfg), dt, err, dt_min, dt_max, prev_err)
+
+
+def rk45_push_one(domain, physics, cfg, sp, mass, charge, pos, vel, t, dt, state=None,
+                  field=None):
+    """One RK45Strategy::step; returns (pos, vel, dt_used, dt_next, rejected, state)."""
+    lib = load()
+    st = np.zeros(1, RK45_ION) if state is None else np.array(state, RK45_ION).reshape(1)
+    if state is None:
+        st["last_error"] = 1.0
+    p, v = vec(*pos), vec(*vel)
+    dn = C.c_double()
+    rej = C.c_int()
+    fptr = C.byref(field.c) if field is not None else C.POINTER(_abi.OracleField)()
+    used = lib.oracle_rk45_push_one(C.byref(domain), C.byref(physics), C.byref(cfg),
+                                    C.byref(sp), fptr, mass, charge, p, v, t, dt,
+                                    st.ctypes.data, C.byref(dn), C.byref(rej))
+    return np.array(p[:]), np.array(v[:]), used, dn.value, rej.value, st[0]
+
+
+def run_rk45(domain, physics, cfg, species, ens, t0, dt, n_steps, step_offset=0, atoms=None,
+             field=None, state=None, threads=0):
+    """The RK45 engine loop on ens in place; returns (StepStats, state).  Pass
+    the returned state to the next call to continue a run."""
+    lib = load()
+    sp = species_array(species)
+    if atoms is not None and len(atoms):
+        a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
+        ap, na = _dp(a), a.shape[0]
+    else:
+        a, ap, na = None, C.POINTER(C.c_double)(), 0
+    fptr = C.byref(field.c) if field is not None else C.POINTER(_abi.OracleField)()
+    st = rk45_state(ens) if state is None else state
+    soa = ens.soa()
+    stats = _abi.StepStats()
+    rc = lib.oracle_run_rk45(C.byref(domain), C.byref(physics), C.byref(cfg), sp, len(species),
+                             ap, na, fptr, C.byref(soa), st.ctypes.data, t0, dt, n_steps,
+                             step_offset, threads, C.byref(stats))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_rk45 failed ({rc})")
+    return stats, st

@@ -453,6 +453,120 @@ void boris_step(const icarion_domain_desc& d, const oracle_field* f, int coll,
     vel = v_plus + a * (dt * 0.5);
 }
 
+// ---- RK45 (RK45Strategy.cpp:101-520) ---------------------------------------
+// Dormand-Prince 5(4) tableau (Hairer, Norsett, Wanner 1993; the reference's
+// RK45Strategy.cpp:101-157).  The reference calls the propagated solution (the
+// a7 row, FSAL) "y4" and the embedded solution "y5"; the names are kept.
+namespace dp {
+constexpr double c2 = 1.0 / 5.0, c3 = 3.0 / 10.0, c4 = 4.0 / 5.0, c5 = 8.0 / 9.0, c6 = 1.0,
+                 c7 = 1.0;
+constexpr double a21 = 1.0 / 5.0;
+constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+constexpr double a41 = 44.0 / 45.0, a42 = -56.0 / 15.0, a43 = 32.0 / 9.0;
+constexpr double a51 = 19372.0 / 6561.0, a52 = -25360.0 / 2187.0, a53 = 64448.0 / 6561.0,
+                 a54 = -212.0 / 729.0;
+constexpr double a61 = 9017.0 / 3168.0, a62 = -355.0 / 33.0, a63 = 46732.0 / 5247.0,
+                 a64 = 49.0 / 176.0, a65 = -5103.0 / 18656.0;
+constexpr double a71 = 35.0 / 384.0, a72 = 0.0, a73 = 500.0 / 1113.0, a74 = 125.0 / 192.0,
+                 a75 = -2187.0 / 6784.0, a76 = 11.0 / 84.0;
+constexpr double b51 = 5179.0 / 57600.0, b52 = 0.0, b53 = 7571.0 / 16695.0, b54 = 393.0 / 640.0,
+                 b55 = -92097.0 / 339200.0, b56 = 187.0 / 2100.0, b57 = 1.0 / 40.0;
+// PI controller (RK45Strategy.cpp:159-171)
+constexpr double kBeta = 0.04;
+constexpr double kAlpha = 0.2 - kBeta * 0.75;
+constexpr double kRejectExp = 0.2;
+constexpr int kMaxReject = 10;
+constexpr double kMinError = 1e-10;
+constexpr double kDtMinTol = 1.001;
+}  // namespace dp
+
+// RK45Strategy::estimate_error (RK45Strategy.cpp:272-313)
+double rk45_error(const icarion_rk45_desc& c, const double y4[6], const double y5[6],
+                  const double y0[6]) {
+    double m = 0.0;
+    for (int i = 0; i < 6; ++i) {
+        const double e = std::fabs(y5[i] - y4[i]);
+        const double sc = c.atol + c.rtol * std::fabs(y0[i]);
+        m = std::max(m, e / sc);
+    }
+    return std::max(m, dp::kMinError);
+}
+
+// RK45Strategy::compute_new_step (RK45Strategy.cpp:315-351)
+double rk45_new_step(const icarion_rk45_desc& c, double dt, double err, double dt_min,
+                     double dt_max, double prev_err) {
+    double f;
+    if (err > 1.0) {
+        f = c.safety_factor * std::pow(1.0 / err, dp::kRejectExp);
+    } else {
+        f = c.safety_factor * std::pow(prev_err / err, dp::kBeta) * std::pow(1.0 / err, dp::kAlpha);
+    }
+    f = std::max(f, c.max_step_decrease);
+    f = std::min(f, c.max_step_increase);
+    double dn = dt * f;
+    dn = std::max(dn, dt_min);
+    dn = std::min(dn, dt_max);
+    return dn;
+}
+
+// RK45Strategy::step (RK45Strategy.cpp:354-520) for one ion from time t with
+// trial dt.  Returns the accepted dt (-1 when no trial was accepted).
+double rk45_step(const icarion_domain_desc& d, const oracle_field* f, int coll,
+                 const icarion_rk45_desc& cfg, const icarion_species_desc& sp, double m, double q,
+                 V3& pos, V3& vel, double t, double dt, oracle_rk45_ion& st, double& dt_next,
+                 int& rejected) {
+    const double inv_m = 1.0 / m;
+    auto acc = [&](V3 p, V3 v, double tt) { return total_force(d, f, coll, sp, m, q, p, v, tt) * inv_m; };
+    const double dt_min = std::max(dt * cfg.min_step_factor, cfg.absolute_min_step_s);
+    const double dt_max = dt * cfg.max_step_factor;
+    double h = dt;
+    rejected = 0;
+    const V3 p0 = pos, v0 = vel;
+    for (int attempt = 0; attempt < dp::kMaxReject; ++attempt) {
+        using namespace dp;
+        const V3 k1v = v0;
+        const V3 k1a = st.fsal ? V3{st.k1a[0], st.k1a[1], st.k1a[2]} : acc(p0, v0, t);
+        const V3 p2 = p0 + k1v * (h * a21), v2 = v0 + k1a * (h * a21);
+        const V3 k2v = v2, k2a = acc(p2, v2, t + c2 * h);
+        const V3 p3 = p0 + (k1v * a31 + k2v * a32) * h, v3 = v0 + (k1a * a31 + k2a * a32) * h;
+        const V3 k3v = v3, k3a = acc(p3, v3, t + c3 * h);
+        const V3 p4 = p0 + (k1v * a41 + k2v * a42 + k3v * a43) * h;
+        const V3 v4 = v0 + (k1a * a41 + k2a * a42 + k3a * a43) * h;
+        const V3 k4v = v4, k4a = acc(p4, v4, t + c4 * h);
+        const V3 p5 = p0 + (k1v * a51 + k2v * a52 + k3v * a53 + k4v * a54) * h;
+        const V3 v5 = v0 + (k1a * a51 + k2a * a52 + k3a * a53 + k4a * a54) * h;
+        const V3 k5v = v5, k5a = acc(p5, v5, t + c5 * h);
+        const V3 p6 = p0 + (k1v * a61 + k2v * a62 + k3v * a63 + k4v * a64 + k5v * a65) * h;
+        const V3 v6 = v0 + (k1a * a61 + k2a * a62 + k3a * a63 + k4a * a64 + k5a * a65) * h;
+        const V3 k6v = v6, k6a = acc(p6, v6, t + c6 * h);
+        const V3 p7 = p0 + (k1v * a71 + k2v * a72 + k3v * a73 + k4v * a74 + k5v * a75 + k6v * a76) * h;
+        const V3 v7 = v0 + (k1a * a71 + k2a * a72 + k3a * a73 + k4a * a74 + k5a * a75 + k6a * a76) * h;
+        const V3 k7v = v7, k7a = acc(p7, v7, t + c7 * h);
+        const V3 p5s = p0 + (k1v * b51 + k2v * b52 + k3v * b53 + k4v * b54 + k5v * b55 +
+                             k6v * b56 + k7v * b57) * h;
+        const V3 v5s = v0 + (k1a * b51 + k2a * b52 + k3a * b53 + k4a * b54 + k5a * b55 +
+                             k6a * b56 + k7a * b57) * h;
+        const double y4[6] = {p7.x, p7.y, p7.z, v7.x, v7.y, v7.z};
+        const double y5[6] = {p5s.x, p5s.y, p5s.z, v5s.x, v5s.y, v5s.z};
+        const double y0[6] = {p0.x, p0.y, p0.z, v0.x, v0.y, v0.z};
+        const double err = rk45_error(cfg, y4, y5, y0);
+        const bool force_accept = cfg.accept_at_dt_min && (h <= dt_min * kDtMinTol);
+        if (err <= 1.0 || force_accept) {
+            pos = p7;
+            vel = v7;
+            st.k1a[0] = k7a.x; st.k1a[1] = k7a.y; st.k1a[2] = k7a.z;
+            st.fsal = 1;
+            dt_next = rk45_new_step(cfg, h, err, dt_min, dt_max, st.last_error);
+            st.last_error = err;
+            return h;
+        }
+        h = rk45_new_step(cfg, h, err, dt_min, dt_max, st.last_error);
+        ++rejected;
+        st.fsal = 0;
+    }
+    return -1.0;
+}
+
 // ---- collision kernels (CollisionKernels.cpp, VelocitySampling.cpp,
 // CollisionGeometry.cpp) ------------------------------------------------------
 
@@ -920,6 +1034,172 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
         stats->alive = alive;
         stats->arrived = arrived;
         stats->t_end_s = t_table[n_steps];
+        stats->device_ms = 0.0;
+    }
+    return ICARION_OK;
+}
+
+double oracle_rk45_error(const icarion_rk45_desc* cfg, const double y4[6], const double y5[6],
+                         const double y0[6]) {
+    return rk45_error(*cfg, y4, y5, y0);
+}
+
+double oracle_rk45_new_step(const icarion_rk45_desc* cfg, double dt, double err, double dt_min,
+                            double dt_max, double prev_err) {
+    return rk45_new_step(*cfg, dt, err, dt_min, dt_max, prev_err);
+}
+
+double oracle_rk45_push_one(const icarion_domain_desc* domain, const icarion_physics_desc* physics,
+                            const icarion_rk45_desc* cfg, const icarion_species_desc* sp,
+                            const oracle_field* field, double mass_kg, double charge_C,
+                            double pos[3], double vel[3], double t, double dt,
+                            oracle_rk45_ion* st, double* dt_next, int* rejected) {
+    V3 p{pos[0], pos[1], pos[2]}, v{vel[0], vel[1], vel[2]};
+    double dn = 0.0;
+    int rej = 0;
+    const double used = rk45_step(*domain, field, physics->collision_model, *cfg, *sp, mass_kg,
+                                  charge_C, p, v, t, dt, *st, dn, rej);
+    pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+    vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
+    if (dt_next) *dt_next = dn;
+    if (rejected) *rejected = rej;
+    return used;
+}
+
+int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_desc* physics,
+                    const icarion_rk45_desc* cfg, const icarion_species_desc* species,
+                    uint32_t n_species, const double* atoms, uint32_t n_atoms,
+                    const oracle_field* field, icarion_ion_soa* ions, oracle_rk45_ion* rk45,
+                    double t0, double dt, uint64_t n_steps, uint64_t step_offset, int n_threads,
+                    icarion_step_stats* stats) {
+    (void)n_atoms;
+    if (!domain || !physics || !cfg || !species || !ions || !rk45 || n_species == 0)
+        return ICARION_ERR_INVALID;
+    const icarion_domain_desc& d = *domain;
+    const icarion_physics_desc& ph = *physics;
+    const size_t n = ions->n;
+    const bool detect = std::isfinite(d.detector_z_m);
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    for (size_t i = 0; i < n; ++i) {
+        if (std::isnan(rk45[i].t)) rk45[i].t = t0;
+        if (!(rk45[i].dt > 0.0)) rk45[i].dt = dt;
+    }
+    double T = t0;
+    uint64_t pushed_total = 0, coll_total = 0, rejected_total = 0;
+    bool failed = false;
+    for (uint64_t k = 0; k < n_steps; ++k) {
+        double max_dt_used = 0.0;
+        #pragma omp parallel reduction(+ : pushed_total, coll_total, rejected_total) \
+            reduction(max : max_dt_used) reduction(|| : failed)
+        {
+            std::vector<V3> scratch;
+            #pragma omp for schedule(dynamic, 64)
+            for (long long ii = 0; ii < static_cast<long long>(n); ++ii) {
+                const size_t i = static_cast<size_t>(ii);
+                const uint32_t s = ions->species[i];
+                const icarion_species_desc& sp = species[s < n_species ? s : 0];
+                oracle_rk45_ion& st = rk45[i];
+                uint8_t born = ions->born ? ions->born[i] : 1;
+                uint8_t active = ions->active[i];
+                const double birth = ions->birth_time_s ? ions->birth_time_s[i] : t0;
+                // prepare_ions_for_integration (SimulationEngineLifecycle.cpp:86-107)
+                if (!born && T >= birth) {
+                    born = 1;
+                    active = 1;
+                }
+                if (ions->born) ions->born[i] = born;
+                if (!active || !born) {
+                    ions->active[i] = active;
+                    continue;
+                }
+                V3 pos{ions->x[i], ions->y[i], ions->z[i]};
+                V3 vel{ions->vx[i], ions->vy[i], ions->vz[i]};
+                if (!contains(d, pos)) {
+                    ions->active[i] = 0;
+                    if (ions->death_time_s) ions->death_time_s[i] = T;
+                    continue;
+                }
+                const double z_before = pos.z;
+                double dt_next = 0.0;
+                int rej = 0;
+                const double used = rk45_step(d, field, ph.collision_model, *cfg, sp,
+                                              ions->mass_kg[i], ions->charge_C[i], pos, vel, T,
+                                              st.dt, st, dt_next, rej);
+                rejected_total += static_cast<uint64_t>(rej);
+                if (used < 0.0) {
+                    failed = true;
+                    continue;
+                }
+                ++pushed_total;
+                max_dt_used = std::max(max_dt_used, used);
+                // perform_collisions with the accepted dt
+                if (ph.collision_model != ICARION_COLLISION_NONE) {
+                    PhiloxStream rng(ph.rng_seed, static_cast<uint32_t>(i), step_offset + k);
+                    if (collide(d, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, used,
+                                rng, scratch)) {
+                        if (ions->n_collisions) ++ions->n_collisions[i];
+                        ++coll_total;
+                    }
+                }
+                if (detect && ions->arrival_time_s && std::isnan(ions->arrival_time_s[i])) {
+                    if (z_before < d.detector_z_m && pos.z >= d.detector_z_m)
+                        ions->arrival_time_s[i] =
+                            T + used * (d.detector_z_m - z_before) / (pos.z - z_before);
+                }
+                // finalize_ions_after_integration (SimulationEngineLifecycle.cpp:137-192)
+                bool alive = contains(d, pos);
+                if (alive) {
+                    st.t = st.t + used;
+                    st.dt = dt_next > 0.0 ? dt_next : (used > 0.0 ? used : dt);
+                    if (!finite3(pos) || !finite3(vel)) alive = false;
+                }
+                if (!alive) {
+                    active = 0;
+                    if (ions->death_time_s) ions->death_time_s[i] = T;
+                }
+                ions->active[i] = active;
+                ions->x[i] = pos.x; ions->y[i] = pos.y; ions->z[i] = pos.z;
+                ions->vx[i] = vel.x; ions->vy[i] = vel.y; ions->vz[i] = vel.z;
+            }
+        }
+        if (failed) return ICARION_ERR_INVALID;
+        // next_engine_time_after_step (SimulationEngineUtils.cpp:49-83), with
+        // total_time_s unbounded
+        double Tn = T;
+        for (size_t i = 0; i < n; ++i) Tn = std::max(Tn, rk45[i].t);
+        if (!(Tn > T)) {
+            double next_birth = INFINITY;
+            for (size_t i = 0; i < n; ++i) {
+                if (ions->born && ions->born[i]) continue;
+                const double bt = ions->birth_time_s ? ions->birth_time_s[i] : t0;
+                if (bt > T) next_birth = std::min(next_birth, bt);
+            }
+            if (std::isfinite(next_birth)) Tn = next_birth;
+            else if (max_dt_used > 0.0) Tn = T + max_dt_used;
+            else Tn = T + dt;
+        }
+        T = Tn;
+    }
+    if (ions->time_s)
+        for (size_t i = 0; i < n; ++i) ions->time_s[i] = rk45[i].t;
+    if (ions->dt_s)
+        for (size_t i = 0; i < n; ++i) ions->dt_s[i] = rk45[i].dt;
+    if (stats) {
+        uint64_t alive = 0, arrived = 0;
+        for (size_t i = 0; i < n; ++i) {
+            if (ions->active[i] && (!ions->born || ions->born[i])) ++alive;
+            if (ions->arrival_time_s && !std::isnan(ions->arrival_time_s[i])) ++arrived;
+        }
+        stats->ion_steps = pushed_total;
+        stats->collisions = coll_total;
+        stats->alive = alive;
+        stats->arrived = arrived;
+        stats->rejected = rejected_total;
+        stats->t_end_s = T;
         stats->device_ms = 0.0;
     }
     return ICARION_OK;

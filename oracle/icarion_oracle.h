@@ -92,6 +92,52 @@ int oracle_run(const icarion_domain_desc *domain, const icarion_physics_desc *ph
                uint64_t step_offset, int rng_kind, int n_threads,
                icarion_step_stats *stats);
 
+/* ---- RK45 (RK45Strategy.cpp:101-520) -------------------------------------
+ * Per-ion adaptive state carried between engine steps (RK45Strategy::RK45State,
+ * RK45Strategy.h:192-198 + the engine's dt_per_ion_). */
+typedef struct oracle_rk45_ion {
+    double t;            /* ion time t_i                                   */
+    double dt;           /* next trial dt (dt_per_ion_)                     */
+    double last_error;   /* PI memory (1.0 initially)                       */
+    double k1a[3];       /* FSAL acceleration of the last accepted step     */
+    int32_t fsal;        /* k1a valid                                       */
+    int32_t reserved;
+} oracle_rk45_ion;
+
+/* err = max over x, y, z, vx, vy, vz of |y5 - y4| / (atol + rtol |y0|),
+ * floored at 1e-10 (RK45Strategy::estimate_error, :272-313). */
+double oracle_rk45_error(const icarion_rk45_desc *cfg, const double y4[6], const double y5[6],
+                         const double y0[6]);
+
+/* PI step-size controller (RK45Strategy::compute_new_step, :315-351). */
+double oracle_rk45_new_step(const icarion_rk45_desc *cfg, double dt, double err, double dt_min,
+                            double dt_max, double prev_err);
+
+/* One RK45Strategy::step of one ion from engine time t with trial dt (no
+ * collisions, no boundary): pos/vel updated in place, st updated (FSAL,
+ * last_error); returns the accepted dt, *dt_next the suggestion, *rejected the
+ * rejected trials; -1 if no trial was accepted (the reference throws). */
+double oracle_rk45_push_one(const icarion_domain_desc *domain, const icarion_physics_desc *physics,
+                            const icarion_rk45_desc *cfg, const icarion_species_desc *sp,
+                            const oracle_field *field, double mass_kg, double charge_C,
+                            double pos[3], double vel[3], double t, double dt,
+                            oracle_rk45_ion *st, double *dt_next, int *rejected);
+
+/* The engine loop with RK45: n_steps engine steps from engine time t0.  Each
+ * step integrates every live ion from the engine time with its own dt_i, uses
+ * the accepted dt for its collision and its time t_i, and then takes the
+ * engine time as max_i t_i (SimulationEngine.cpp:374-428,
+ * SimulationEngineIntegration.cpp:110-146, SimulationEngineLifecycle.cpp:172,
+ * SimulationEngineUtils.cpp:49-83).  rk45[] holds the per-ion state in/out
+ * (its t, dt replace ions->time_s / dt_s).  Random numbers: the Philox streams
+ * of the device.  stats->t_end_s = the engine time after the last step. */
+int oracle_run_rk45(const icarion_domain_desc *domain, const icarion_physics_desc *physics,
+                    const icarion_rk45_desc *cfg, const icarion_species_desc *species,
+                    uint32_t n_species, const double *atoms_xyzr, uint32_t n_atoms,
+                    const oracle_field *field, icarion_ion_soa *ions, oracle_rk45_ion *rk45,
+                    double t0, double dt, uint64_t n_steps, uint64_t step_offset, int n_threads,
+                    icarion_step_stats *stats);
+
 /* ---- EHSS offline precompute (tools/ehss_offline_precompute.cpp:438-517) ----
  * Per (cluster c, orientation k): Shoemake orientation, b_max = max(r_xy + r) +
  * r_gas, sigma_eff = hits/n_sigma * pi b_max^2, then n_mu first-contact samples

@@ -1,0 +1,239 @@
+"""RK45 engine loop on the gfx950 kernels (one launch per engine step, engine
+time = max of the ion times) against the oracle's restatement of
+RK45Strategy::step inside SimulationEngine::process_timestep.
+
+Identical Philox streams on both sides, so ions are compared one by one:
+positions, velocities, the per-ion time t_i and the next trial dt, plus the
+engine time, the ion-step count and the rejected trials.
+
+Why three kinds of test.  The step-size controller divides |y5 - y4| - a
+difference of two nearly equal states - by atol + rtol |y| and raises it to
+fractional powers, so where a step is smooth the next dt follows the
+states' last bits: in the oracle alone, a one-ulp change of one velocity
+moves some ions' dt by 0.4 % after two steps and 18 % after twelve, and the
+shared engine clock (max of the ion times) carries that into every ion's RF
+phase.  The device's pow, cos and the reference's libm differ in the last
+bit now and then, so adaptive multi-step runs are compared as distributions
+(KS tests, counts, drift), while
+  * constant-dt chains (clamps of 1, every trial accepted) are compared ion
+    by ion to 1e-9, like the fixed-step paths, and
+  * single adaptive steps (rejections included) ion by ion to 1e-8 for the
+    state and 1e-5 for the next dt (the retried step size and the next dt are
+    powers of that roundoff-sensitive estimate).
+The kernel keeps floating-point contraction off so that its arithmetic rounds
+like the reference's around the accept/reject decisions."""
+import numpy as np
+import pytest
+
+from icarion_amd import _abi
+from icarion_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rk45_physics(w, collision=None, ou=None):
+    p = w["physics"]
+    return W.physics(_abi.INTEGRATOR_RK45, p.collision_model if collision is None else collision,
+                     p.ou_thermal_kicks if ou is None else ou, _abi.PRECISION_FP64, p.rng_seed)
+
+
+def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
+    from icarion_amd import gpu
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], ph)
+        if cfg is not None:
+            h.set_rk45(cfg)
+        h.set_species(w["species"], w.get("atoms"))
+        if w.get("field") is not None:
+            E, origin, spacing = w["field"]
+            h.upload_field_array(E, origin, spacing)
+        h.upload_state(ens)
+        st = h.step(t0, dt, n_steps, step_offset)
+        h.download_state(ens)
+    return st
+
+
+def oracle_rk45(oracle, w, ph, ens, n_steps, dt, t0=0.0, cfg=None):
+    cfg = cfg or _abi.rk45_defaults()
+    field = oracle.Field(*w["field"]) if w.get("field") is not None else None
+    st, _ = oracle.run_rk45(w["domain"], ph, cfg, w["species"], ens, t0, dt, n_steps,
+                            atoms=w.get("atoms"), field=field)
+    return st
+
+
+def close(a, b, rtol, scale):
+    return np.abs(a - b) <= rtol * np.maximum(np.abs(b), scale)
+
+
+def match(g, o, rtol=1e-6, dt_rtol=1e-5):
+    ok = np.ones(g.n, bool)
+    for f in ("x", "y", "z"):
+        ok &= close(getattr(g, f), getattr(o, f), rtol, 1e-6)
+    for f in ("vx", "vy", "vz"):
+        ok &= close(getattr(g, f), getattr(o, f), rtol, 1.0)
+    ok &= close(g.time_s, o.time_s, rtol, 1e-12)
+    ok &= close(g.dt_s, o.dt_s, dt_rtol, 1e-18)
+    ok &= g.active == o.active
+    return ok
+
+
+def report(tag, g, o, sg, so):
+    """One line per test: counts and the worst per-field relative deviation."""
+    worst = {}
+    for f in ("x", "vx", "z", "vz", "time_s", "dt_s"):
+        a, b = getattr(g, f), getattr(o, f)
+        fin = np.isfinite(a) & np.isfinite(b)
+        d = np.abs(a[fin] - b[fin]) / np.maximum(np.abs(b[fin]), 1e-300)
+        worst[f] = float(np.median(d)) if d.size else 0.0, float(d.max()) if d.size else 0.0
+    print(f"{tag}: ion-steps {sg.ion_steps}/{so.ion_steps} rejected {sg.rejected}/{so.rejected} "
+          f"collisions {sg.collisions}/{so.collisions} T {sg.t_end_s!r}/{so.t_end_s!r} "
+          f"match {match(g, o).mean():.4f} median/max rel dev "
+          + " ".join(f"{k} {v[0]:.1e}/{v[1]:.1e}" for k, v in worst.items()))
+
+
+def case_workload(case):
+    if case == "quadrupole_rf":
+        w = W.quadrupole_workload(n=2048, U=10.0, seed=5)
+        return w, rk45_physics(w, collision=_abi.COLLISION_NONE, ou=0), 1e-9
+    if case == "orbitrap":
+        w = W.orbitrap_workload(n=1000)
+        return w, rk45_physics(w, collision=_abi.COLLISION_NONE, ou=0), 1e-9
+    if case == "langevin_ou":
+        w = W.quadrupole_workload(n=2048, U=10.0, seed=13)
+        return w, rk45_physics(w), 1e-9
+    w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
+    return w, rk45_physics(w), w["dt"]
+
+
+CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_rk45_constant_dt_chain_ion_by_ion(oracle, case):
+    # Growth and shrink clamps of 1 and tolerances no step exceeds: every
+    # trial is accepted and dt stays put, so the run is the Dormand-Prince
+    # chain itself (FSAL reuse, stage times, RF phase at the engine clock,
+    # engine-time updates, collisions with the accepted dt), compared to the
+    # 1e-9 per-ion bar of the fixed-step paths.
+    w, ph, dt = case_workload(case)
+    cfg = _abi.rk45_defaults(atol=1.0, rtol=1.0, max_step_increase=1.0, max_step_decrease=1.0)
+    n = 100
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_rk45(w, ph, g, n, dt, cfg=cfg)
+    so = oracle_rk45(oracle, w, ph, o, n, dt, cfg=cfg)
+    report(case, g, o, sg, so)
+    assert sg.rejected == 0 and so.rejected == 0
+    assert sg.ion_steps == pytest.approx(so.ion_steps, rel=1e-3)
+    assert sg.t_end_s == pytest.approx(so.t_end_s, rel=1e-12)
+    need = 0.99 if case == "field_array_hss" else 0.999
+    assert match(g, o, rtol=1e-9, dt_rtol=1e-12).mean() >= need
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_rk45_adaptive_single_step_ion_by_ion(oracle, case):
+    # One adaptive step from the same states: the accepted state after any
+    # rejections, the ion time and the controller's next dt
+    w, ph, dt = case_workload(case)
+    cfg = (_abi.rk45_defaults(atol=1e-7, rtol=1e-5) if case == "field_array_hss"
+           else _abi.rk45_defaults())
+    dt = dt * 20.0   # large enough that some ions are rejected
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_rk45(w, ph, g, 1, dt, cfg=cfg)
+    so = oracle_rk45(oracle, w, ph, o, 1, dt, cfg=cfg)
+    report(case, g, o, sg, so)
+    assert sg.ion_steps == so.ion_steps
+    assert sg.rejected == so.rejected
+    assert sg.t_end_s == so.t_end_s
+    assert match(g, o, rtol=1e-8, dt_rtol=1e-5).mean() >= 0.99
+
+
+def test_rk45_ims_hss_ion_by_ion(oracle):
+    # In the uniform IMS field the error estimate sits at its floor, so every
+    # accepted step grows dt five-fold (the reference's behaviour): within a
+    # few steps the cloud is accelerated out of the tube.  Six steps cover it.
+    w = W.ims_hss_workload(n=8192, seed=11)
+    ph = rk45_physics(w)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_rk45(w, ph, g, 6, w["dt"])
+    so = oracle_rk45(oracle, w, ph, o, 6, w["dt"])
+    report("ims_hss", g, o, sg, so)
+    assert match(g, o, rtol=1e-9, dt_rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    assert sg.ion_steps == so.ion_steps
+    assert sg.t_end_s == so.t_end_s
+
+
+def ks_p(a, b):
+    from scipy import stats
+    return stats.ks_2samp(a, b).pvalue
+
+
+def test_rk45_rf_quadrupole_long_run_statistics(oracle):
+    # 200 adaptive steps through the RF: trajectories are compared as
+    # distributions (see the module docstring)
+    w = W.quadrupole_workload(n=4096, U=10.0, seed=5)
+    ph = rk45_physics(w, collision=_abi.COLLISION_NONE, ou=0)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_rk45(w, ph, g, 200, 1e-9)          # graph path (>= 64 steps)
+    so = oracle_rk45(oracle, w, ph, o, 200, 1e-9)
+    report("quadrupole_rf_long", g, o, sg, so)
+    assert sg.ion_steps == pytest.approx(so.ion_steps, rel=1e-3)
+    assert sg.rejected == pytest.approx(so.rejected, rel=2e-3)
+    # the engine time is the max over 4096 chaotic dt sequences
+    assert sg.t_end_s == pytest.approx(so.t_end_s, rel=1e-2)
+    assert abs(g.alive.mean() - o.alive.mean()) < 0.01
+    a = g.alive & o.alive
+    assert ks_p(np.hypot(g.x, g.y)[a], np.hypot(o.x, o.y)[a]) > 1e-3
+    assert ks_p(g.dt_s[a], o.dt_s[a]) > 1e-3
+
+
+def test_rk45_field_array_hss_long_run_statistics(oracle):
+    w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
+    ph = rk45_physics(w)
+    cfg = _abi.rk45_defaults(atol=1e-7, rtol=1e-5)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    z0 = w["ensemble"].z
+    sg = gpu_rk45(w, ph, g, 200, w["dt"], cfg=cfg)
+    so = oracle_rk45(oracle, w, ph, o, 200, w["dt"], cfg=cfg)
+    report("field_array_hss_long", g, o, sg, so)
+    assert sg.ion_steps == pytest.approx(so.ion_steps, rel=0.02)
+    assert sg.collisions == pytest.approx(so.collisions, rel=0.02)
+    assert abs(g.alive.mean() - o.alive.mean()) < 0.03
+    a_g, a_o = g.alive, o.alive
+    assert (g.z - z0)[a_g].mean() == pytest.approx((o.z - z0)[a_o].mean(), rel=0.05)
+    assert ks_p(g.z[a_g], o.z[a_o]) > 1e-3
+
+
+def test_rk45_split_calls_continue_the_run():
+    from icarion_amd import gpu
+    w = W.quadrupole_workload(n=1024, U=10.0, seed=17)
+    ph = rk45_physics(w, collision=_abi.COLLISION_NONE, ou=0)
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    sa = gpu_rk45(w, ph, a, 100, 1e-9)
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], ph)
+        h.set_species(w["species"])
+        h.upload_state(b)
+        s1 = h.step(0.0, 1e-9, 60, 0)
+        s2 = h.step(s1.t_end_s, 1e-9, 40, 60)
+        h.download_state(b)
+    assert s2.t_end_s == sa.t_end_s
+    assert s1.ion_steps + s2.ion_steps == sa.ion_steps
+    for f in ("x", "y", "z", "vx", "vy", "vz", "time_s", "dt_s"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+
+
+def test_rk45_rejects_fp32_and_bad_control():
+    from icarion_amd import gpu
+    w = W.quadrupole_workload(n=16)
+    ph = W.physics(_abi.INTEGRATOR_RK45, precision=_abi.PRECISION_FP32_KAHAN)
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], ph)
+        h.set_species(w["species"])
+        h.upload_state(w["ensemble"])
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, 1e-9, 1)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_rk45(_abi.rk45_defaults(safety_factor=1.5))
+        assert ei.value.code == _abi.ERR_INVALID
