@@ -99,6 +99,24 @@ def rk45_defaults(**kw):
     return d
 
 
+class CloudDesc(C.Structure):
+    _fields_ = [
+        ("domain", DomainDesc), ("physics", PhysicsDesc),
+        ("species", C.POINTER(SpeciesDesc)), ("n_species", _u32), ("reserved0", _u32),
+        ("ions", C.POINTER(IonSoA)), ("t0_s", _d), ("dt_s", _d), ("n_steps", _u64),
+        ("step_offset", _u64),
+    ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("block_size", _u32), ("groups_per_cu", _u32)]
+
+
+class CloudStats(C.Structure):
+    _fields_ = [("ion_steps", _u64), ("collisions", _u64), ("alive", _u64), ("arrived", _u64),
+                ("t_end_s", _d), ("done_ms", _d)]
+
+
 class GpuDesc(C.Structure):
     _fields_ = [("device", _i32), ("block_size", _u32), ("steps_per_launch", _u32),
                 ("reserved0", _u32)]

@@ -32,6 +32,9 @@ hipError_t launch_step(int integrator, int field, int coll, int precision, int s
 hipError_t launch_rk45_step(int field, int coll, const StepParams& P, const IonPtrs& S, const Rk45Ptrs& R,
                             uint32_t block, hipStream_t stream);
 hipError_t launch_rk45_init(const Rk45Ptrs& R, uint32_t n, double t0, double dt, hipStream_t stream);
+hipError_t launch_batch(const BatchParams& B64, const BatchParams& B32, uint32_t n_clouds,
+                        unsigned long long* counters, uint32_t block64, uint32_t grid64, uint32_t block32,
+                        uint32_t grid32, hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
 hipError_t launch_synth_clusters(float4* atoms, const uint32_t* offsets, const uint32_t* counts,
                                  uint32_t n, float r_min, float r_max, uint64_t seed, hipStream_t s);
 hipError_t launch_oapa(const float4* atoms, const uint32_t* offsets, const uint32_t* counts, uint32_t n,
@@ -160,6 +163,36 @@ struct icarion_gpu {
     DevBuf<unsigned long long> counters;
     std::vector<StepEntry> table_host;
 
+    // batched clouds (independent of the single-cloud state above)
+    struct BatchCloud {
+        size_t n = 0, off = 0;
+        int precision = 0;
+        uint32_t kind = 0;
+        uint64_t n_steps = 0;
+        double t_end = 0.0;
+        bool has_birth = false;
+    };
+    uint32_t b_n = 0;
+    size_t b_total = 0;
+    std::vector<BatchCloud> b_clouds;
+    std::vector<StepParams> b_params;
+    DevBuf<StepParams> b_params_d;
+    DevBuf<IonPtrs> b_ions_d;
+    DevBuf<uint32_t> b_kinds_d, b_queue;
+    DevBuf<BatchTile> b_tiles[2];
+    uint32_t b_n_tiles[2] = {0, 0}, b_tiled_block[2] = {0, 0};
+    DevBuf<unsigned long long> b_done, b_start, b_counters;
+    DevBuf<double> b_x, b_y, b_z, b_vx, b_vy, b_vz, b_mass, b_charge, b_birth, b_death, b_arrival;
+    DevBuf<uint32_t> b_spec, b_ncoll;
+    DevBuf<uint8_t> b_flags;
+    DevBuf<SpeciesDev> b_species;
+    DevBuf<StepEntry> b_tables;
+    icarion_batch_tuning b_tuned[2] = {{0, 0}, {0, 0}};
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int n_cu = 0;
+    int clock_khz = 0;
+
     // RK45: per-ion time, dt and step-control state; the engine record
     Rk45Cfg rk45{1e-8, 1e-6, 0.9, 1e-6, 1e3, 5.0, 0.1, 0.0, 1, 0};
     DevBuf<double> r_t, r_dt, r_err, r_k1x, r_k1y, r_k1z;
@@ -272,10 +305,13 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
 
 // Per-step table: t_k by repeated addition and c = U_eff(t)/r0^2 at the
 // stage times the integrator evaluates (RK4: t, t + dt/2, t + dt; Boris: t).
-void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
-    const icarion_domain_desc& d = h->dom;
-    h->table_host.resize(n_steps);
-    const bool rf = h->base.has_rf;
+// Per-step table of a run: t_k by repeated addition from t0 and, with RF,
+// c = U_eff(t)/r0^2 at the stage times the integrator evaluates (RK4: t,
+// t + dt/2, t + dt; Boris: t).  Appends n_steps entries to out.
+void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, bool rf, int integrator,
+                  double t0, double dt, uint64_t n_steps) {
+    const size_t base = out.size();
+    out.resize(base + n_steps);
     const double r0_sq = d.radius_m * d.radius_m;
     const double omega = 2.0 * kPiHost * d.rf_frequency_Hz;
     const double half_dt = dt * 0.5;
@@ -285,11 +321,11 @@ void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
     };
     double t = t0;
     for (uint64_t k = 0; k < n_steps; ++k) {
-        StepEntry& e = h->table_host[k];
+        StepEntry& e = out[base + k];
         e.t = t;
         if (rf) {
             e.c0 = coeff(t);
-            if (h->phys.integrator == ICARION_INTEGRATOR_RK4) {
+            if (integrator == ICARION_INTEGRATOR_RK4) {
                 e.ch = coeff(t + half_dt);
                 e.c1 = coeff(t + dt);
             } else {
@@ -300,6 +336,11 @@ void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
         }
         t = t + dt;
     }
+}
+
+void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
+    h->table_host.clear();
+    append_table(h->table_host, h->dom, h->base.has_rf != 0, h->phys.integrator, t0, dt, n_steps);
     h->table.resize(std::max<size_t>(n_steps, 1));
     hip_check(hipMemcpyAsync(h->table.p, h->table_host.data(), n_steps * sizeof(StepEntry),
                              hipMemcpyHostToDevice, h->stream),
@@ -612,6 +653,383 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
     }
 }
 
+// ---- batched clouds --------------------------------------------------------
+
+uint32_t batch_kind(const icarion_physics_desc& p, int field, int coll) {
+    const uint32_t c = coll == COLL_HSS ? 1u : (coll == COLL_DAMPING ? 2u : 0u);
+    return static_cast<uint32_t>(p.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
+           2u * (field == FIELD_ORBITRAP ? 1u : 0u) + 4u * c;
+}
+
+void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t n_clouds) {
+    if (!clouds || n_clouds == 0) fail(ICARION_ERR_INVALID, "no clouds");
+    std::vector<icarion_gpu::BatchCloud> bc(n_clouds);
+    std::vector<StepParams> params(n_clouds);
+    std::vector<uint32_t> kinds(n_clouds);
+    std::vector<SpeciesDev> species;
+    std::vector<uint32_t> species_off(n_clouds);
+    std::vector<StepEntry> tables;
+    std::vector<size_t> table_off(n_clouds);
+    // identical step tables are shared: (t0, dt, n, RF inputs, stage layout)
+    struct Key {
+        double t0, dt, U, V, f, r0;
+        uint64_t n;
+        int rf, rk4;
+        bool operator==(const Key& o) const {
+            return std::memcmp(this, &o, sizeof(Key)) == 0;
+        }
+    };
+    std::vector<std::pair<Key, size_t>> seen;
+    size_t total = 0;
+    for (uint32_t c = 0; c < n_clouds; ++c) {
+        const icarion_cloud_desc& cd = clouds[c];
+        const std::string tag = "cloud " + std::to_string(c) + ": ";
+        validate_domain(cd.domain, cd.physics);
+        if (cd.physics.integrator == ICARION_INTEGRATOR_RK45)
+            fail(ICARION_ERR_UNSUPPORTED, tag + "RK45 is not covered by the batch");
+        const int coll = coll_kind(cd.physics);
+        const int field = field_kind(cd.domain);
+        if (coll == COLL_EHSS) fail(ICARION_ERR_UNSUPPORTED, tag + "EHSS is not covered by the batch");
+        if (field == FIELD_ARRAY) fail(ICARION_ERR_UNSUPPORTED, tag + "field arrays are not covered by the batch");
+        if (!(cd.dt_s > 0.0) || !std::isfinite(cd.dt_s) || !std::isfinite(cd.t0_s))
+            fail(ICARION_ERR_INVALID, tag + "t0 must be finite and dt finite and > 0");
+        if (cd.n_steps >= 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, tag + "n_steps must be < 2^32");
+        if (!cd.species || cd.n_species == 0) fail(ICARION_ERR_INVALID, tag + "species table is empty");
+        const icarion_ion_soa* io = cd.ions;
+        if (!io) fail(ICARION_ERR_INVALID, tag + "ions is NULL");
+        const size_t n = io->n;
+        if (n && (!io->x || !io->y || !io->z || !io->vx || !io->vy || !io->vz || !io->mass_kg ||
+                  !io->charge_C || !io->species || !io->active))
+            fail(ICARION_ERR_INVALID, tag + "x, y, z, vx, vy, vz, mass_kg, charge_C, species, active are required");
+        for (size_t i = 0; i < n; ++i) {
+            if (io->species[i] >= cd.n_species) fail(ICARION_ERR_INVALID, tag + "species index out of range");
+            if (!(io->mass_kg[i] > 0.0)) fail(ICARION_ERR_INVALID, tag + "mass_kg must be > 0");
+        }
+        bc[c].n = n;
+        bc[c].off = total;
+        bc[c].precision = cd.physics.precision;
+        bc[c].kind = batch_kind(cd.physics, field, coll);
+        bc[c].n_steps = cd.n_steps;
+        bc[c].has_birth = io->birth_time_s != nullptr;
+        total += n;
+        species_off[c] = static_cast<uint32_t>(species.size());
+        for (uint32_t k = 0; k < cd.n_species; ++k) {
+            const icarion_species_desc& d = cd.species[k];
+            SpeciesDev sd{};
+            sd.sigma_hss = d.sigma_hss_m2;
+            sd.sigma_ehss = d.sigma_ehss_m2;
+            sd.K0_cm2_Vs = d.K0_cm2_Vs;
+            species.push_back(sd);
+        }
+        StepParams P = derive_params(cd.domain, cd.physics);
+        const Key key{cd.t0_s, cd.dt_s, cd.domain.dc_quad_V, cd.domain.rf_voltage_V, cd.domain.rf_frequency_Hz,
+                      cd.domain.radius_m, cd.n_steps, P.has_rf, cd.physics.integrator == ICARION_INTEGRATOR_RK4};
+        size_t off = SIZE_MAX;
+        for (const auto& kv : seen)
+            if (kv.first == key) off = kv.second;
+        if (off == SIZE_MAX) {
+            off = tables.size();
+            append_table(tables, cd.domain, P.has_rf != 0, cd.physics.integrator, cd.t0_s, cd.dt_s, cd.n_steps);
+            seen.push_back({key, off});
+        }
+        table_off[c] = off;
+        bc[c].t_end = cd.n_steps ? tables[off + cd.n_steps - 1].t + cd.dt_s : cd.t0_s;
+        P.dt = cd.dt_s;
+        P.step0 = cd.step_offset;
+        P.n_steps = static_cast<uint32_t>(cd.n_steps);
+        P.n_ions = static_cast<uint32_t>(n);
+        fill_fast_const(P);
+        params[c] = P;
+        kinds[c] = bc[c].kind;
+    }
+    if (total > 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, "at most 2^32-1 ions per batch");
+    set_device(h);
+    hipStream_t s = h->stream;
+    // concatenated SoA; host staging lives until the synchronisation below
+    std::vector<double> hx(total), hy(total), hz(total), hvx(total), hvy(total), hvz(total), hm(total),
+        hq(total), hb(total, -INFINITY), hd(total, -1.0), ha(total, NAN);
+    std::vector<uint32_t> hs(total), hn(total, 0);
+    std::vector<uint8_t> hf(total);
+    for (uint32_t c = 0; c < n_clouds; ++c) {
+        const icarion_ion_soa* io = clouds[c].ions;
+        const size_t o = bc[c].off;
+        for (size_t i = 0; i < bc[c].n; ++i) {
+            hx[o + i] = io->x[i]; hy[o + i] = io->y[i]; hz[o + i] = io->z[i];
+            hvx[o + i] = io->vx[i]; hvy[o + i] = io->vy[i]; hvz[o + i] = io->vz[i];
+            hm[o + i] = io->mass_kg[i]; hq[o + i] = io->charge_C[i];
+            hs[o + i] = io->species[i];
+            hf[o + i] = static_cast<uint8_t>((io->active[i] ? 1 : 0) | ((io->born ? io->born[i] : 1) ? 2 : 0));
+            if (io->birth_time_s) hb[o + i] = io->birth_time_s[i];
+            if (io->death_time_s) hd[o + i] = io->death_time_s[i];
+            if (io->arrival_time_s) ha[o + i] = io->arrival_time_s[i];
+            if (io->n_collisions) hn[o + i] = io->n_collisions[i];
+        }
+    }
+    const size_t nt = std::max<size_t>(total, 1);
+    h->b_x.resize(nt); h->b_y.resize(nt); h->b_z.resize(nt);
+    copy_in(h->b_x, hx.data(), total, s, "batch x"); copy_in(h->b_y, hy.data(), total, s, "batch y");
+    copy_in(h->b_z, hz.data(), total, s, "batch z"); copy_in(h->b_vx, hvx.data(), total, s, "batch vx");
+    copy_in(h->b_vy, hvy.data(), total, s, "batch vy"); copy_in(h->b_vz, hvz.data(), total, s, "batch vz");
+    copy_in(h->b_mass, hm.data(), total, s, "batch mass"); copy_in(h->b_charge, hq.data(), total, s, "batch charge");
+    copy_in(h->b_birth, hb.data(), total, s, "batch birth"); copy_in(h->b_death, hd.data(), total, s, "batch death");
+    copy_in(h->b_arrival, ha.data(), total, s, "batch arrival"); copy_in(h->b_spec, hs.data(), total, s, "batch species");
+    copy_in(h->b_ncoll, hn.data(), total, s, "batch ncoll"); copy_in(h->b_flags, hf.data(), total, s, "batch flags");
+    copy_in(h->b_species, species.data(), species.size(), s, "batch species table");
+    copy_in(h->b_tables, tables.data(), std::max<size_t>(tables.size(), 1), s, "batch step tables");
+    h->b_counters.resize(4ull * n_clouds);
+    h->b_done.resize(n_clouds);
+    h->b_start.resize(1);
+    h->b_queue.resize(2);
+    std::vector<IonPtrs> ions(n_clouds);
+    for (uint32_t c = 0; c < n_clouds; ++c) {
+        const size_t o = bc[c].off;
+        IonPtrs& S = ions[c];
+        S.x = h->b_x.p + o; S.y = h->b_y.p + o; S.z = h->b_z.p + o;
+        S.vx = h->b_vx.p + o; S.vy = h->b_vy.p + o; S.vz = h->b_vz.p + o;
+        S.mass = h->b_mass.p + o; S.charge = h->b_charge.p + o; S.species = h->b_spec.p + o;
+        S.flags = h->b_flags.p + o;
+        S.birth = bc[c].has_birth ? h->b_birth.p + o : nullptr;
+        S.death = h->b_death.p + o; S.arrival = h->b_arrival.p + o; S.ncoll = h->b_ncoll.p + o;
+        StepParams& P = params[c];
+        P.species = h->b_species.p + species_off[c];
+        P.table = h->b_tables.p + table_off[c];
+        P.counters = h->b_counters.p + 4ull * c;
+        P.atoms = nullptr;
+        P.fa_f32 = nullptr;
+        P.fa_f64 = nullptr;
+        P.mlp_blob = nullptr;
+    }
+    copy_in(h->b_params_d, params.data(), n_clouds, s, "batch params");
+    copy_in(h->b_ions_d, ions.data(), n_clouds, s, "batch ion pointers");
+    copy_in(h->b_kinds_d, kinds.data(), n_clouds, s, "batch kinds");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    h->b_clouds = std::move(bc);
+    h->b_params = std::move(params);
+    h->b_n = n_clouds;
+    h->b_total = total;
+    h->b_tiled_block[0] = h->b_tiled_block[1] = 0;
+}
+
+// Tiles of one precision class, longest clouds first (step count x ions per
+// step; FP64 bodies are costed like FP32 ones - the classes never share a
+// queue).
+void batch_build_tiles(icarion_gpu* h, int cls, uint32_t block) {
+    if (h->b_tiled_block[cls] == block) return;
+    std::vector<uint32_t> order;
+    for (uint32_t c = 0; c < h->b_n; ++c)
+        if ((h->b_clouds[c].precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0) == cls && h->b_clouds[c].n)
+            order.push_back(c);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return h->b_clouds[a].n_steps > h->b_clouds[b].n_steps;
+    });
+    std::vector<BatchTile> tiles;
+    for (uint32_t c : order)
+        for (size_t f = 0; f < h->b_clouds[c].n; f += block)
+            tiles.push_back(BatchTile{c, static_cast<uint32_t>(f)});
+    h->b_n_tiles[cls] = static_cast<uint32_t>(tiles.size());
+    if (!tiles.empty()) {
+        copy_in(h->b_tiles[cls], tiles.data(), tiles.size(), h->stream, "batch tiles");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    }
+    h->b_tiled_block[cls] = block;
+}
+
+icarion_batch_tuning batch_shape(const icarion_gpu* h, int cls, const icarion_batch_tuning* req) {
+    icarion_batch_tuning t = req ? *req : h->b_tuned[cls];
+    if (t.block_size == 0) t.block_size = 256;
+    if (t.groups_per_cu == 0) t.groups_per_cu = 4;
+    if (t.block_size != 64 && t.block_size != 128 && t.block_size != 256)
+        fail(ICARION_ERR_INVALID, "batch block_size must be 64, 128 or 256");
+    if (t.groups_per_cu > 64) fail(ICARION_ERR_INVALID, "batch groups_per_cu must be <= 64");
+    return t;
+}
+
+// One batch launch captured as a HIP graph (reset + persistent kernels of the
+// classes in `mask`); returns device milliseconds.
+float batch_launch(icarion_gpu* h, icarion_batch_tuning t64, icarion_batch_tuning t32, int mask) {
+    batch_build_tiles(h, 0, t64.block_size);
+    batch_build_tiles(h, 1, t32.block_size);
+    BatchParams B[2];
+    for (int cls = 0; cls < 2; ++cls) {
+        B[cls].params = h->b_params_d.p;
+        B[cls].ions = h->b_ions_d.p;
+        B[cls].kinds = h->b_kinds_d.p;
+        B[cls].tiles = h->b_tiles[cls].p;
+        B[cls].n_tiles = (mask >> cls) & 1 ? h->b_n_tiles[cls] : 0;
+        B[cls].pad = 0;
+        B[cls].queue = h->b_queue.p + cls;
+        B[cls].done = h->b_done.p;
+        B[cls].start = h->b_start.p;
+    }
+    const uint32_t n_cu = static_cast<uint32_t>(std::max(h->n_cu, 1));
+    hipStream_t s = h->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    const hipError_t le = launch_batch(B[0], B[1], h->b_n, h->b_counters.p, t64.block_size,
+                                       t64.groups_per_cu * n_cu, t32.block_size, t32.groups_per_cu * n_cu, s,
+                                       h->stream2, h->ev_fork, h->ev_join);
+    const hipError_t ee = hipStreamEndCapture(s, &graph);
+    if (le != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        hip_check(le, "capture batch launch");
+        hip_check(ee, "hipStreamEndCapture");
+    }
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        hip_check(ie, "hipGraphInstantiate");
+    }
+    hipError_t e = hipEventRecord(h->ev0, s);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, s);
+    if (e == hipSuccess) e = hipEventRecord(h->ev1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    hip_check(e, "batch graph launch");
+    return ms;
+}
+
+void do_batch_run(icarion_gpu* h, const icarion_batch_tuning* r64, const icarion_batch_tuning* r32,
+                  icarion_cloud_stats* per_cloud, double* device_ms) {
+    if (h->b_n == 0) fail(ICARION_ERR_STATE, "icarion_gpu_batch_upload not called");
+    set_device(h);
+    const icarion_batch_tuning t64 = batch_shape(h, 0, r64), t32 = batch_shape(h, 1, r32);
+    const float ms = batch_launch(h, t64, t32, 3);
+    if (device_ms) *device_ms = ms;
+    if (!per_cloud) return;
+    const uint32_t n = h->b_n;
+    std::vector<unsigned long long> cnt(4ull * n), done(n), start(1);
+    std::vector<uint8_t> fl(h->b_total);
+    std::vector<double> ar(h->b_total);
+    hipStream_t s = h->stream;
+    copy_out(cnt.data(), h->b_counters, cnt.size(), s, "batch counters");
+    copy_out(done.data(), h->b_done, n, s, "batch completion clocks");
+    copy_out(start.data(), h->b_start, 1, s, "batch start clock");
+    copy_out(fl.data(), h->b_flags, h->b_total, s, "batch flags");
+    copy_out(ar.data(), h->b_arrival, h->b_total, s, "batch arrival");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const double tick_ms = h->clock_khz > 0 ? 1.0 / static_cast<double>(h->clock_khz) : 0.0;
+    for (uint32_t c = 0; c < n; ++c) {
+        const auto& bc = h->b_clouds[c];
+        icarion_cloud_stats& st = per_cloud[c];
+        st.ion_steps = cnt[4ull * c];
+        st.collisions = cnt[4ull * c + 1];
+        uint64_t alive = 0, arrived = 0;
+        for (size_t i = bc.off; i < bc.off + bc.n; ++i) {
+            if ((fl[i] & 3) == 3) ++alive;
+            if (!std::isnan(ar[i])) ++arrived;
+        }
+        st.alive = alive;
+        st.arrived = arrived;
+        st.t_end_s = bc.t_end;
+        st.done_ms = (done[c] && start[0] != ~0ull && done[c] >= start[0])
+                         ? static_cast<double>(done[c] - start[0]) * tick_ms : 0.0;
+    }
+}
+
+void do_batch_autotune(icarion_gpu* h, uint32_t trial_steps, icarion_batch_tuning* best64,
+                       icarion_batch_tuning* best32, double* rate64, double* rate32) {
+    if (h->b_n == 0) fail(ICARION_ERR_STATE, "icarion_gpu_batch_upload not called");
+    if (trial_steps == 0) fail(ICARION_ERR_INVALID, "trial_steps must be > 0");
+    set_device(h);
+    hipStream_t s = h->stream;
+    const size_t n = h->b_total;
+    // snapshot of the mutable state, restored before every trial and at the end
+    DevBuf<double> sx, sy, sz, svx, svy, svz, sd, sa;
+    DevBuf<uint32_t> snc;
+    DevBuf<uint8_t> sfl;
+    auto d2d = [&](auto& dst, const auto& src) {
+        if (n) hip_check(hipMemcpyAsync(dst.p, src.p, n * sizeof(*src.p), hipMemcpyDeviceToDevice, s), "batch snapshot");
+    };
+    sx.resize(n); sy.resize(n); sz.resize(n); svx.resize(n); svy.resize(n); svz.resize(n); sd.resize(n);
+    sa.resize(n); snc.resize(n); sfl.resize(n);
+    d2d(sx, h->b_x); d2d(sy, h->b_y); d2d(sz, h->b_z); d2d(svx, h->b_vx); d2d(svy, h->b_vy); d2d(svz, h->b_vz);
+    d2d(sd, h->b_death); d2d(sa, h->b_arrival); d2d(snc, h->b_ncoll); d2d(sfl, h->b_flags);
+    auto restore = [&]() {
+        d2d(h->b_x, sx); d2d(h->b_y, sy); d2d(h->b_z, sz); d2d(h->b_vx, svx); d2d(h->b_vy, svy);
+        d2d(h->b_vz, svz); d2d(h->b_death, sd); d2d(h->b_arrival, sa); d2d(h->b_ncoll, snc);
+        d2d(h->b_flags, sfl);
+    };
+    // trial parameters: every cloud cut to at most trial_steps steps
+    std::vector<StepParams> trial = h->b_params;
+    for (auto& P : trial) P.n_steps = std::min<uint32_t>(P.n_steps, trial_steps);
+    copy_in(h->b_params_d, trial.data(), trial.size(), s, "batch trial params");
+    static const uint32_t kBlocks[] = {64, 128, 256};
+    static const uint32_t kGroups[] = {1, 2, 4, 8, 16};
+    icarion_batch_tuning best[2] = {{256, 4}, {256, 4}};
+    double best_rate[2] = {0.0, 0.0};
+    for (int cls = 0; cls < 2; ++cls) {
+        bool present = false;
+        for (const auto& c : h->b_clouds) present |= (c.precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0) == cls;
+        if (!present) continue;
+        for (uint32_t b : kBlocks) {
+            for (uint32_t g : kGroups) {
+                if (static_cast<uint64_t>(b) * g > 2048) continue;   // resident-thread limit per CU
+                restore();
+                const icarion_batch_tuning t{b, g};
+                const float ms = batch_launch(h, t, t, 1 << cls);
+                std::vector<unsigned long long> cnt(4ull * h->b_n);
+                copy_out(cnt.data(), h->b_counters, cnt.size(), s, "batch counters");
+                hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+                uint64_t steps = 0;
+                for (uint32_t c = 0; c < h->b_n; ++c) steps += cnt[4ull * c];
+                const double rate = ms > 0.0f ? static_cast<double>(steps) / (ms * 1e-3) : 0.0;
+                if (rate > best_rate[cls]) {
+                    best_rate[cls] = rate;
+                    best[cls] = t;
+                }
+            }
+        }
+        h->b_tuned[cls] = best[cls];
+    }
+    restore();
+    copy_in(h->b_params_d, h->b_params.data(), h->b_params.size(), s, "batch params");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (best64) *best64 = best[0];
+    if (best32) *best32 = best[1];
+    if (rate64) *rate64 = best_rate[0];
+    if (rate32) *rate32 = best_rate[1];
+}
+
+void do_batch_download(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t n_clouds) {
+    if (h->b_n == 0) fail(ICARION_ERR_STATE, "no resident batch");
+    if (!clouds || n_clouds != h->b_n) fail(ICARION_ERR_INVALID, "n_clouds differs from the resident batch");
+    set_device(h);
+    hipStream_t s = h->stream;
+    const size_t n = h->b_total;
+    std::vector<double> hx(n), hy(n), hz(n), hvx(n), hvy(n), hvz(n), hd(n), ha(n);
+    std::vector<uint32_t> hn(n);
+    std::vector<uint8_t> hf(n);
+    copy_out(hx.data(), h->b_x, n, s, "batch x"); copy_out(hy.data(), h->b_y, n, s, "batch y");
+    copy_out(hz.data(), h->b_z, n, s, "batch z"); copy_out(hvx.data(), h->b_vx, n, s, "batch vx");
+    copy_out(hvy.data(), h->b_vy, n, s, "batch vy"); copy_out(hvz.data(), h->b_vz, n, s, "batch vz");
+    copy_out(hd.data(), h->b_death, n, s, "batch death"); copy_out(ha.data(), h->b_arrival, n, s, "batch arrival");
+    copy_out(hn.data(), h->b_ncoll, n, s, "batch ncoll"); copy_out(hf.data(), h->b_flags, n, s, "batch flags");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    for (uint32_t c = 0; c < n_clouds; ++c) {
+        icarion_ion_soa* io = clouds[c].ions;
+        const auto& bc = h->b_clouds[c];
+        if (!io || io->n != bc.n) fail(ICARION_ERR_INVALID, "cloud " + std::to_string(c) + ": ions->n differs");
+        const size_t o = bc.off;
+        for (size_t i = 0; i < bc.n; ++i) {
+            if (io->x) io->x[i] = hx[o + i];
+            if (io->y) io->y[i] = hy[o + i];
+            if (io->z) io->z[i] = hz[o + i];
+            if (io->vx) io->vx[i] = hvx[o + i];
+            if (io->vy) io->vy[i] = hvy[o + i];
+            if (io->vz) io->vz[i] = hvz[o + i];
+            if (io->death_time_s) io->death_time_s[i] = hd[o + i];
+            if (io->arrival_time_s) io->arrival_time_s[i] = ha[o + i];
+            if (io->n_collisions) io->n_collisions[i] = hn[o + i];
+            if (io->active) io->active[i] = hf[o + i] & 1;
+            if (io->born) io->born[i] = (hf[o + i] >> 1) & 1;
+        }
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -769,6 +1187,13 @@ int icarion_gpu_create(const icarion_gpu_desc* desc, icarion_gpu** out) {
             hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
             hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
             h->counters.resize(4);
+            hip_check(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device),
+                      "hipDeviceGetAttribute");
+            hip_check(hipDeviceGetAttribute(&h->clock_khz, hipDeviceAttributeWallClockRate, h->device),
+                      "hipDeviceGetAttribute");
         } catch (...) {
             icarion_gpu_destroy(h);
             throw;
@@ -781,10 +1206,20 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     if (!h) return ICARION_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     h->species.release(); h->atoms_f32.release(); h->atom_offsets.release(); h->atom_counts.release();
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
     h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
+    h->b_params_d.release(); h->b_ions_d.release(); h->b_kinds_d.release(); h->b_queue.release();
+    h->b_tiles[0].release(); h->b_tiles[1].release(); h->b_done.release(); h->b_start.release();
+    h->b_counters.release(); h->b_x.release(); h->b_y.release(); h->b_z.release(); h->b_vx.release();
+    h->b_vy.release(); h->b_vz.release(); h->b_mass.release(); h->b_charge.release(); h->b_birth.release();
+    h->b_death.release(); h->b_arrival.release(); h->b_spec.release(); h->b_ncoll.release();
+    h->b_flags.release(); h->b_species.release(); h->b_tables.release();
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
@@ -1165,6 +1600,36 @@ int icarion_gpu_synchronize(icarion_gpu* h) {
         set_device(h);
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+}
+
+int icarion_gpu_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t n_clouds) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_batch_upload(h, clouds, n_clouds);
+    });
+}
+
+int icarion_gpu_batch_run(icarion_gpu* h, const icarion_batch_tuning* fp64, const icarion_batch_tuning* fp32,
+                          icarion_cloud_stats* per_cloud, double* device_ms) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_batch_run(h, fp64, fp32, per_cloud, device_ms);
+    });
+}
+
+int icarion_gpu_batch_autotune(icarion_gpu* h, uint32_t trial_steps, icarion_batch_tuning* best64,
+                               icarion_batch_tuning* best32, double* rate64, double* rate32) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_batch_autotune(h, trial_steps, best64, best32, rate64, rate32);
+    });
+}
+
+int icarion_gpu_batch_download(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t n_clouds) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        do_batch_download(h, clouds, n_clouds);
     });
 }
 

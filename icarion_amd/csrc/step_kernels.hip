@@ -19,124 +19,127 @@
 namespace icarion {
 
 template <int INTEG, int FIELD, int COLL>
+__device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, const uint32_t i,
+        uint32_t& pushed, uint32_t& ncoll_launch) {
+    d3 pos{S.x[i], S.y[i], S.z[i]};
+    d3 vel{S.vx[i], S.vy[i], S.vz[i]};
+    const double m = S.mass[i];
+    const double q = S.charge[i];
+    const SpeciesDev sp = P.species[S.species[i]];
+    uint8_t flags = S.flags[i];
+    bool active = flags & 1, born = (flags >> 1) & 1;
+    const double birth = S.birth ? S.birth[i] : -INFINITY;
+    double death = S.death[i];
+    double arrival = S.arrival[i];
+    uint32_t nc = S.ncoll[i];
+    const double inv_m = 1.0 / m;
+    // hoisted damping factor (DampingForce.cpp:150-177 / :269-271)
+    double lpre = 0.0;
+    if constexpr (COLL == COLL_DAMPING) {
+        if (P.damping_model == 3) {
+            if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
+                const double mu = (m * P.m_gas) / (m + P.m_gas);
+                const double cs1 = kPi * q * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
+                lpre = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + m);
+            }
+        } else {
+            const double K0 = sp.K0_cm2_Vs;
+            if (K0 > 0.0 && m > 0.0 && P.n > 0.0) {
+                const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
+                lpre = q / ((K0t * (kLoschmidt / P.n)) * m);
+            }
+        }
+    }
+    bool checked = false;   // position known to be inside the domain
+    const double dt = P.dt;
+    const double half_dt = dt * 0.5;
+    for (uint32_t k = 0; k < P.n_steps; ++k) {
+        const StepEntry te = P.table[k];
+        const double t = te.t;
+        if (!born) {
+            if (t >= birth) {
+                born = true;
+                active = true;
+                checked = false;
+            } else {
+                continue;
+            }
+        }
+        if (!active) break;
+        if (!checked && !contains_f64(P, pos)) {
+            active = false;
+            death = t;
+            break;
+        }
+        const double z_before = pos.z;
+        if constexpr (INTEG == 0) {
+            // RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361)
+            const d3 p0 = pos, v0 = vel;
+            const d3 k1v = v0;
+            const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, q, m, lpre) * inv_m;
+            const d3 k2v = v0 + k1a * half_dt;
+            const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, q, m, lpre) * inv_m;
+            const d3 k3v = v0 + k2a * half_dt;
+            const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, q, m, lpre) * inv_m;
+            const d3 k4v = v0 + k3a * dt;
+            const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, q, m, lpre) * inv_m;
+            const double h6 = dt / 6.0;
+            pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
+            vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
+        } else {
+            // BorisStrategy::step (BorisStrategy.cpp:18-98)
+            const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, q, m, lpre) * inv_m;
+            const d3 vm = vel + a * half_dt;
+            d3 vp = vm;
+            if (P.magnetic) {
+                const double qm = q * inv_m;
+                const d3 tv = d3{P.B[0], P.B[1], P.B[2]} * (qm * dt * 0.5);
+                const double s_f = 2.0 / (1.0 + dot(tv, tv));
+                const d3 sv = tv * s_f;
+                const d3 vprime = vm + cross(vm, tv);
+                vp = vm + cross(vprime, sv);
+            }
+            pos = pos + vp * dt;
+            vel = vp + a * half_dt;
+        }
+        ++pushed;
+        if constexpr (COLL != COLL_NONE) {
+            PhiloxStream rng(P.seed, i, P.step0 + k);
+            if (collide_f64<COLL>(P, sp, m, lpre, vel, rng, dt)) {
+                ++nc;
+                ++ncoll_launch;
+            }
+        }
+        if (P.detect && isnan(arrival)) {
+            if (z_before < P.det_z && pos.z >= P.det_z)
+                arrival = t + dt * (P.det_z - z_before) / (pos.z - z_before);
+        }
+        if (!contains_f64(P, pos)) {
+            active = false;
+            death = t;
+            break;
+        }
+        checked = true;
+        if (!(isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z) && isfinite(vel.x) &&
+              isfinite(vel.y) && isfinite(vel.z))) {
+            active = false;
+            death = t;
+            break;
+        }
+    }
+    S.x[i] = pos.x; S.y[i] = pos.y; S.z[i] = pos.z;
+    S.vx[i] = vel.x; S.vy[i] = vel.y; S.vz[i] = vel.z;
+    S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+    S.death[i] = death;
+    S.arrival[i] = arrival;
+    S.ncoll[i] = nc;
+}
+
+template <int INTEG, int FIELD, int COLL>
 __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_range = i < P.n_ions;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (in_range) {
-        d3 pos{S.x[i], S.y[i], S.z[i]};
-        d3 vel{S.vx[i], S.vy[i], S.vz[i]};
-        const double m = S.mass[i];
-        const double q = S.charge[i];
-        const SpeciesDev sp = P.species[S.species[i]];
-        uint8_t flags = S.flags[i];
-        bool active = flags & 1, born = (flags >> 1) & 1;
-        const double birth = S.birth ? S.birth[i] : -INFINITY;
-        double death = S.death[i];
-        double arrival = S.arrival[i];
-        uint32_t nc = S.ncoll[i];
-        const double inv_m = 1.0 / m;
-        // hoisted damping factor (DampingForce.cpp:150-177 / :269-271)
-        double lpre = 0.0;
-        if constexpr (COLL == COLL_DAMPING) {
-            if (P.damping_model == 3) {
-                if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
-                    const double mu = (m * P.m_gas) / (m + P.m_gas);
-                    const double cs1 = kPi * q * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
-                    lpre = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + m);
-                }
-            } else {
-                const double K0 = sp.K0_cm2_Vs;
-                if (K0 > 0.0 && m > 0.0 && P.n > 0.0) {
-                    const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
-                    lpre = q / ((K0t * (kLoschmidt / P.n)) * m);
-                }
-            }
-        }
-        bool checked = false;   // position known to be inside the domain
-        const double dt = P.dt;
-        const double half_dt = dt * 0.5;
-        for (uint32_t k = 0; k < P.n_steps; ++k) {
-            const StepEntry te = P.table[k];
-            const double t = te.t;
-            if (!born) {
-                if (t >= birth) {
-                    born = true;
-                    active = true;
-                    checked = false;
-                } else {
-                    continue;
-                }
-            }
-            if (!active) break;
-            if (!checked && !contains_f64(P, pos)) {
-                active = false;
-                death = t;
-                break;
-            }
-            const double z_before = pos.z;
-            if constexpr (INTEG == 0) {
-                // RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361)
-                const d3 p0 = pos, v0 = vel;
-                const d3 k1v = v0;
-                const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, q, m, lpre) * inv_m;
-                const d3 k2v = v0 + k1a * half_dt;
-                const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, q, m, lpre) * inv_m;
-                const d3 k3v = v0 + k2a * half_dt;
-                const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, q, m, lpre) * inv_m;
-                const d3 k4v = v0 + k3a * dt;
-                const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, q, m, lpre) * inv_m;
-                const double h6 = dt / 6.0;
-                pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
-                vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
-            } else {
-                // BorisStrategy::step (BorisStrategy.cpp:18-98)
-                const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, q, m, lpre) * inv_m;
-                const d3 vm = vel + a * half_dt;
-                d3 vp = vm;
-                if (P.magnetic) {
-                    const double qm = q * inv_m;
-                    const d3 tv = d3{P.B[0], P.B[1], P.B[2]} * (qm * dt * 0.5);
-                    const double s_f = 2.0 / (1.0 + dot(tv, tv));
-                    const d3 sv = tv * s_f;
-                    const d3 vprime = vm + cross(vm, tv);
-                    vp = vm + cross(vprime, sv);
-                }
-                pos = pos + vp * dt;
-                vel = vp + a * half_dt;
-            }
-            ++pushed;
-            if constexpr (COLL != COLL_NONE) {
-                PhiloxStream rng(P.seed, i, P.step0 + k);
-                if (collide_f64<COLL>(P, sp, m, lpre, vel, rng, dt)) {
-                    ++nc;
-                    ++ncoll_launch;
-                }
-            }
-            if (P.detect && isnan(arrival)) {
-                if (z_before < P.det_z && pos.z >= P.det_z)
-                    arrival = t + dt * (P.det_z - z_before) / (pos.z - z_before);
-            }
-            if (!contains_f64(P, pos)) {
-                active = false;
-                death = t;
-                break;
-            }
-            checked = true;
-            if (!(isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z) && isfinite(vel.x) &&
-                  isfinite(vel.y) && isfinite(vel.z))) {
-                active = false;
-                death = t;
-                break;
-            }
-        }
-        S.x[i] = pos.x; S.y[i] = pos.y; S.z[i] = pos.z;
-        S.vx[i] = vel.x; S.vy[i] = vel.y; S.vz[i] = vel.z;
-        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
-        S.death[i] = death;
-        S.arrival[i] = arrival;
-        S.ncoll[i] = nc;
-    }
+    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL>(P, S, i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -315,159 +318,162 @@ __device__ __forceinline__ f3 accel_f32(const StepParams& P, const FastConst& C,
 }
 
 template <int INTEG, int FIELD, int COLL>
+__device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, const uint32_t i,
+        uint32_t& pushed, uint32_t& ncoll_launch) {
+    const FastConst& C = P.fc;
+    // state: FP32 (hi, lo) pairs; hi + lo equals the FP64 value to ~2^-48
+    const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
+    float hi[6], lo[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        hi[c] = (float)X[c];
+        lo[c] = (float)(X[c] - (double)hi[c]);
+    }
+    const double md = S.mass[i];
+    const double qd = S.charge[i];
+    const SpeciesDev sp = P.species[S.species[i]];
+    const float m = (float)md;
+    const float qm = (float)(qd / md);
+    const float inv_m_plus = (float)(1.0 / (md + P.m_gas));
+    const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * P.dt);
+    uint8_t flags = S.flags[i];
+    bool active = flags & 1, born = (flags >> 1) & 1;
+    const double birth = S.birth ? S.birth[i] : -INFINITY;
+    double death = S.death[i];
+    double arrival = S.arrival[i];
+    uint32_t nc = S.ncoll[i];
+    // damping factor in units of 1/s (force per mass): Langevin lpre/|v|
+    float lpre = 0.0f;
+    if constexpr (COLL == COLL_DAMPING) {
+        double l = 0.0;
+        if (P.damping_model == 3) {
+            if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
+                const double mu = (md * P.m_gas) / (md + P.m_gas);
+                const double cs1 = kPi * qd * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
+                l = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + md);
+            }
+        } else {
+            const double K0 = sp.K0_cm2_Vs;
+            if (K0 > 0.0 && md > 0.0 && P.n > 0.0) {
+                const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
+                l = qd / ((K0t * (kLoschmidt / P.n)) * md);
+            }
+        }
+        lpre = (float)l;
+    }
+    bool checked = false;
+    const float dt = (float)P.dt;
+    const float half_dt = 0.5f * dt;
+    const float h6 = dt * (1.0f / 6.0f);
+    for (uint32_t k = 0; k < P.n_steps; ++k) {
+        const StepEntry te = P.table[k];
+        const double t = te.t;
+        if (!born) {
+            if (t >= birth) {
+                born = true;
+                active = true;
+                checked = false;
+            } else {
+                continue;
+            }
+        }
+        if (!active) break;
+        if (!checked && !contains_f32(P, C, hi[0], hi[1], hi[2])) {
+            active = false;
+            death = t;
+            break;
+        }
+        const float z_before = hi[2];
+        const float z_before_lo = lo[2];
+        const f3 p0{hi[0], hi[1], hi[2]};
+        const f3 v0{hi[3], hi[4], hi[5]};
+        f3 dpos, dvel;
+        if constexpr (INTEG == 0) {
+            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
+            const f3 k2v = v0 + k1a * half_dt;
+            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, (float)te.ch, qm, lpre);
+            const f3 k3v = v0 + k2a * half_dt;
+            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, (float)te.ch, qm, lpre);
+            const f3 k4v = v0 + k3a * dt;
+            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, (float)te.c1, qm, lpre);
+            dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
+            dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
+        } else {
+            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
+            const f3 vm = v0 + a * half_dt;
+            f3 rot{0.0f, 0.0f, 0.0f};
+            if (P.magnetic) {
+                const f3 tv = f3{(float)P.B[0], (float)P.B[1], (float)P.B[2]} * (qm * half_dt);
+                const float s_f = 2.0f / (1.0f + dot(tv, tv));
+                const f3 vprime = vm + cross(vm, tv);
+                rot = cross(vprime, tv * s_f);
+            }
+            // x += v+ dt, v = v+ + a dt/2 = v0 + a dt + rot (no cancellation)
+            dpos = (vm + rot) * dt;
+            dvel = a * dt + rot;
+        }
+        kahan_add(hi[0], lo[0], dpos.x);
+        kahan_add(hi[1], lo[1], dpos.y);
+        kahan_add(hi[2], lo[2], dpos.z);
+        kahan_add(hi[3], lo[3], dvel.x);
+        kahan_add(hi[4], lo[4], dvel.y);
+        kahan_add(hi[5], lo[5], dvel.z);
+        ++pushed;
+        if constexpr (COLL != COLL_NONE) {
+            PhiloxStream rng(P.seed, i, P.step0 + k);
+            f3 v{hi[3], hi[4], hi[5]};
+            const f3 vin = v;
+            const bool hit = collide_f32<COLL>(P, C, sp, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng);
+            if (COLL == COLL_HSS) {
+                if (hit) {
+                    // a collision replaces the velocity: restart its compensation
+                    hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
+                    lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
+                    ++nc;
+                    ++ncoll_launch;
+                }
+            } else {
+                kahan_add(hi[3], lo[3], v.x - vin.x);
+                kahan_add(hi[4], lo[4], v.y - vin.y);
+                kahan_add(hi[5], lo[5], v.z - vin.z);
+            }
+        }
+        if (P.detect && isnan(arrival)) {
+            const double zb = (double)z_before + (double)z_before_lo;
+            const double za = (double)hi[2] + (double)lo[2];
+            if (zb < P.det_z && za >= P.det_z)
+                arrival = t + P.dt * (P.det_z - zb) / (za - zb);
+        }
+        if (!contains_f32(P, C, hi[0], hi[1], hi[2])) {
+            active = false;
+            death = t;
+            break;
+        }
+        checked = true;
+        if (!(isfinite(hi[0]) && isfinite(hi[1]) && isfinite(hi[2]) && isfinite(hi[3]) &&
+              isfinite(hi[4]) && isfinite(hi[5]))) {
+            active = false;
+            death = t;
+            break;
+        }
+    }
+    S.x[i] = (double)hi[0] + (double)lo[0];
+    S.y[i] = (double)hi[1] + (double)lo[1];
+    S.z[i] = (double)hi[2] + (double)lo[2];
+    S.vx[i] = (double)hi[3] + (double)lo[3];
+    S.vy[i] = (double)hi[4] + (double)lo[4];
+    S.vz[i] = (double)hi[5] + (double)lo[5];
+    S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+    S.death[i] = death;
+    S.arrival[i] = arrival;
+    S.ncoll[i] = nc;
+}
+
+template <int INTEG, int FIELD, int COLL>
 __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in_range = i < P.n_ions;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (in_range) {
-        const FastConst& C = P.fc;
-        // state: FP32 (hi, lo) pairs; hi + lo equals the FP64 value to ~2^-48
-        const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
-        float hi[6], lo[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            hi[c] = (float)X[c];
-            lo[c] = (float)(X[c] - (double)hi[c]);
-        }
-        const double md = S.mass[i];
-        const double qd = S.charge[i];
-        const SpeciesDev sp = P.species[S.species[i]];
-        const float m = (float)md;
-        const float qm = (float)(qd / md);
-        const float inv_m_plus = (float)(1.0 / (md + P.m_gas));
-        const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * P.dt);
-        uint8_t flags = S.flags[i];
-        bool active = flags & 1, born = (flags >> 1) & 1;
-        const double birth = S.birth ? S.birth[i] : -INFINITY;
-        double death = S.death[i];
-        double arrival = S.arrival[i];
-        uint32_t nc = S.ncoll[i];
-        // damping factor in units of 1/s (force per mass): Langevin lpre/|v|
-        float lpre = 0.0f;
-        if constexpr (COLL == COLL_DAMPING) {
-            double l = 0.0;
-            if (P.damping_model == 3) {
-                if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
-                    const double mu = (md * P.m_gas) / (md + P.m_gas);
-                    const double cs1 = kPi * qd * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
-                    l = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + md);
-                }
-            } else {
-                const double K0 = sp.K0_cm2_Vs;
-                if (K0 > 0.0 && md > 0.0 && P.n > 0.0) {
-                    const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
-                    l = qd / ((K0t * (kLoschmidt / P.n)) * md);
-                }
-            }
-            lpre = (float)l;
-        }
-        bool checked = false;
-        const float dt = (float)P.dt;
-        const float half_dt = 0.5f * dt;
-        const float h6 = dt * (1.0f / 6.0f);
-        for (uint32_t k = 0; k < P.n_steps; ++k) {
-            const StepEntry te = P.table[k];
-            const double t = te.t;
-            if (!born) {
-                if (t >= birth) {
-                    born = true;
-                    active = true;
-                    checked = false;
-                } else {
-                    continue;
-                }
-            }
-            if (!active) break;
-            if (!checked && !contains_f32(P, C, hi[0], hi[1], hi[2])) {
-                active = false;
-                death = t;
-                break;
-            }
-            const float z_before = hi[2];
-            const float z_before_lo = lo[2];
-            const f3 p0{hi[0], hi[1], hi[2]};
-            const f3 v0{hi[3], hi[4], hi[5]};
-            f3 dpos, dvel;
-            if constexpr (INTEG == 0) {
-                const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
-                const f3 k2v = v0 + k1a * half_dt;
-                const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, (float)te.ch, qm, lpre);
-                const f3 k3v = v0 + k2a * half_dt;
-                const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, (float)te.ch, qm, lpre);
-                const f3 k4v = v0 + k3a * dt;
-                const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, (float)te.c1, qm, lpre);
-                dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
-                dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
-            } else {
-                const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
-                const f3 vm = v0 + a * half_dt;
-                f3 rot{0.0f, 0.0f, 0.0f};
-                if (P.magnetic) {
-                    const f3 tv = f3{(float)P.B[0], (float)P.B[1], (float)P.B[2]} * (qm * half_dt);
-                    const float s_f = 2.0f / (1.0f + dot(tv, tv));
-                    const f3 vprime = vm + cross(vm, tv);
-                    rot = cross(vprime, tv * s_f);
-                }
-                // x += v+ dt, v = v+ + a dt/2 = v0 + a dt + rot (no cancellation)
-                dpos = (vm + rot) * dt;
-                dvel = a * dt + rot;
-            }
-            kahan_add(hi[0], lo[0], dpos.x);
-            kahan_add(hi[1], lo[1], dpos.y);
-            kahan_add(hi[2], lo[2], dpos.z);
-            kahan_add(hi[3], lo[3], dvel.x);
-            kahan_add(hi[4], lo[4], dvel.y);
-            kahan_add(hi[5], lo[5], dvel.z);
-            ++pushed;
-            if constexpr (COLL != COLL_NONE) {
-                PhiloxStream rng(P.seed, i, P.step0 + k);
-                f3 v{hi[3], hi[4], hi[5]};
-                const f3 vin = v;
-                const bool hit = collide_f32<COLL>(P, C, sp, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng);
-                if (COLL == COLL_HSS) {
-                    if (hit) {
-                        // a collision replaces the velocity: restart its compensation
-                        hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
-                        lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
-                        ++nc;
-                        ++ncoll_launch;
-                    }
-                } else {
-                    kahan_add(hi[3], lo[3], v.x - vin.x);
-                    kahan_add(hi[4], lo[4], v.y - vin.y);
-                    kahan_add(hi[5], lo[5], v.z - vin.z);
-                }
-            }
-            if (P.detect && isnan(arrival)) {
-                const double zb = (double)z_before + (double)z_before_lo;
-                const double za = (double)hi[2] + (double)lo[2];
-                if (zb < P.det_z && za >= P.det_z)
-                    arrival = t + P.dt * (P.det_z - zb) / (za - zb);
-            }
-            if (!contains_f32(P, C, hi[0], hi[1], hi[2])) {
-                active = false;
-                death = t;
-                break;
-            }
-            checked = true;
-            if (!(isfinite(hi[0]) && isfinite(hi[1]) && isfinite(hi[2]) && isfinite(hi[3]) &&
-                  isfinite(hi[4]) && isfinite(hi[5]))) {
-                active = false;
-                death = t;
-                break;
-            }
-        }
-        S.x[i] = (double)hi[0] + (double)lo[0];
-        S.y[i] = (double)hi[1] + (double)lo[1];
-        S.z[i] = (double)hi[2] + (double)lo[2];
-        S.vx[i] = (double)hi[3] + (double)lo[3];
-        S.vy[i] = (double)hi[4] + (double)lo[4];
-        S.vz[i] = (double)hi[5] + (double)lo[5];
-        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
-        S.death[i] = death;
-        S.arrival[i] = arrival;
-        S.ncoll[i] = nc;
-    }
+    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL>(P, S, i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -652,6 +658,129 @@ __global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams
         S.ncoll[i] = nc;
     }
     add_counters(P, pushed, ncoll_launch);
+}
+
+// ======================= batched clouds (persistent) ========================
+//
+// A fixed pool of workgroups pulls tiles from a queue until it is empty; the
+// host orders the tiles longest first, so the long clouds start first and the
+// short ones fill in at the end.  Every workgroup leaves the loop once the
+// queue index passes the tile count.  Each tile runs the same per-ion body as
+// the single-cloud kernels, so a cloud's result does not depend on the batch.
+
+template <int PREC, int I, int F, int Cc>
+__device__ __forceinline__ void batch_ion(const StepParams& P, const IonPtrs& S, uint32_t i,
+                                          uint32_t& pushed, uint32_t& nc) {
+    if constexpr (PREC == 1) {
+        f32_ion<I, F, Cc>(P, S, i, pushed, nc);
+    } else {
+        f64_ion<I, F, Cc>(P, S, i, pushed, nc);
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& P, const IonPtrs& S,
+                                               uint32_t i, uint32_t& pushed, uint32_t& nc) {
+    switch (kind) {
+        case 0: batch_ion<PREC, 0, FIELD_LINEAR, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 1: batch_ion<PREC, 1, FIELD_LINEAR, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 2: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 3: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 4: batch_ion<PREC, 0, FIELD_LINEAR, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 5: batch_ion<PREC, 1, FIELD_LINEAR, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 6: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 7: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 8: batch_ion<PREC, 0, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 9: batch_ion<PREC, 1, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 10: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 11: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        default: break;
+    }
+}
+
+// The per-cloud tables come in as separate read-only, non-aliased arguments so
+// that the wave-uniform parameter loads stay scalar.
+template <int PREC>
+__global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict__ params,
+                                                    const IonPtrs* __restrict__ ions,
+                                                    const uint32_t* __restrict__ kinds,
+                                                    const BatchTile* __restrict__ tiles, uint32_t n_tiles,
+                                                    uint32_t* queue, unsigned long long* done,
+                                                    unsigned long long* start) {
+    // Queue and clock updates are made by wave 0 under a wave-uniform branch
+    // (all its lanes, one combined increment): a lane-divergent region on each
+    // side of the loop's back edge could be merged by the compiler into a
+    // divergent loop in which lanes 1..63 pass the barriers again before lane 0
+    // has taken the next tile.
+    __shared__ uint32_t s_tile;
+    const bool wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x) == 0;
+    const uint32_t lane = threadIdx.x & 63;
+    if (wave0) atomicMin(start, static_cast<unsigned long long>(__builtin_amdgcn_s_memrealtime()));
+    uint32_t prev = 0xFFFFFFFFu;
+    for (;;) {
+        if (wave0) {
+            if (prev != 0xFFFFFFFFu)
+                atomicMax(&done[prev], static_cast<unsigned long long>(__builtin_amdgcn_s_memrealtime()));
+            const uint32_t got = atomicAdd(queue, lane == 0 ? 1u : 0u);
+            s_tile = __builtin_amdgcn_readfirstlane(got);
+        }
+        __syncthreads();
+        const uint32_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (tile >= n_tiles) break;
+        const uint32_t c = __builtin_amdgcn_readfirstlane(tiles[tile].cloud);
+        const uint32_t first = __builtin_amdgcn_readfirstlane(tiles[tile].first);
+        const StepParams& P = params[c];
+        const IonPtrs S = ions[c];
+        const uint32_t kind = __builtin_amdgcn_readfirstlane(kinds[c]);
+        const uint32_t i = first + threadIdx.x;
+        uint32_t pushed = 0, nc = 0;
+        if (i < P.n_ions) batch_dispatch<PREC>(kind, P, S, i, pushed, nc);
+        add_counters(P, pushed, nc);
+        __syncthreads();   // the whole tile is done before its cloud's clock is stamped
+        prev = c;
+    }
+}
+
+__global__ void batch_reset_kernel(uint32_t* queue, unsigned long long* done, unsigned long long* start,
+                                   unsigned long long* counters, uint32_t n_clouds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2) queue[i] = 0u;
+    if (i == 0) *start = ~0ull;
+    if (i < n_clouds) {
+        done[i] = 0ull;
+        counters[4 * i + 0] = 0ull; counters[4 * i + 1] = 0ull;
+        counters[4 * i + 2] = 0ull; counters[4 * i + 3] = 0ull;
+    }
+}
+
+// Reset the queue/clock/counter records, then the persistent kernel of each
+// precision class present (B64 / B32 with n_tiles == 0 are skipped); the two
+// classes run on two streams so that they overlap.
+hipError_t launch_batch(const BatchParams& B64, const BatchParams& B32, uint32_t n_clouds,
+                        unsigned long long* counters, uint32_t block64, uint32_t grid64, uint32_t block32,
+                        uint32_t grid32, hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((n_clouds + 255) / 256 + 1), dim3(256), 0, s, B64.queue,
+                       B64.done, B64.start, counters, n_clouds);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const bool both = B64.n_tiles && B32.n_tiles;
+    if (both) {
+        if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess) return e;
+    }
+    if (B64.n_tiles)
+        hipLaunchKernelGGL(batch_kernel<0>, dim3(grid64), dim3(block64), 0, s, B64.params, B64.ions, B64.kinds,
+                           B64.tiles, B64.n_tiles, B64.queue, B64.done, B64.start);
+    if (B32.n_tiles)
+        hipLaunchKernelGGL(batch_kernel<1>, dim3(grid32), dim3(block32), 0, both ? s2 : s, B32.params, B32.ions,
+                           B32.kinds, B32.tiles, B32.n_tiles, B32.queue, B32.done, B32.start);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (both) {
+        if ((e = hipEventRecord(join, s2)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // ---- launch dispatch --------------------------------------------------------

@@ -176,4 +176,28 @@ inline void fill_fast_const(StepParams& P) {
     }
 }
 
+// ---- batched clouds ----------------------------------------------------------
+// One tile = up to blockDim ions of one cloud, advanced through all of that
+// cloud's steps by one workgroup of the persistent batch kernel.
+struct BatchTile {
+    uint32_t cloud;
+    uint32_t first;      // first ion of the tile within its cloud
+};
+
+// kind of a cloud's kernel body: integrator + 2 * field (0 linear, 1 Orbitrap)
+// + 4 * collision (0 none, 1 HSS, 2 damping)
+constexpr uint32_t kBatchKinds = 12;
+
+struct BatchParams {
+    const StepParams *params;       // per cloud (n_steps = the cloud's steps)
+    const IonPtrs *ions;            // per cloud
+    const uint32_t *kinds;          // per cloud
+    const BatchTile *tiles;         // in the order the queue hands them out
+    uint32_t n_tiles;
+    uint32_t pad;
+    uint32_t *queue;                // next tile to hand out
+    unsigned long long *done;       // per cloud: clock at the end of its last tile
+    unsigned long long *start;      // clock at the first workgroup's start
+};
+
 }  // namespace icarion

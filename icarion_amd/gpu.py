@@ -21,7 +21,8 @@ EXPORTED = (
     "icarion_gpu_synchronize", "icarion_gpu_synth_clusters", "icarion_gpu_ehss_precompute",
     "icarion_gpu_surrogate_set_params", "icarion_gpu_surrogate_get_params",
     "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
-    "icarion_gpu_set_rk45",
+    "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
+    "icarion_gpu_batch_autotune", "icarion_gpu_batch_download",
 )
 
 
@@ -50,6 +51,12 @@ def load(path=None):
     lib.icarion_gpu_destroy.argtypes = [C.c_void_p]
     lib.icarion_gpu_set_domain.argtypes = [C.c_void_p, P(_abi.DomainDesc), P(_abi.PhysicsDesc)]
     lib.icarion_gpu_set_rk45.argtypes = [C.c_void_p, P(_abi.Rk45Desc)]
+    lib.icarion_gpu_batch_upload.argtypes = [C.c_void_p, P(_abi.CloudDesc), C.c_uint32]
+    lib.icarion_gpu_batch_run.argtypes = [C.c_void_p, P(_abi.BatchTuning), P(_abi.BatchTuning),
+                                          P(_abi.CloudStats), P(C.c_double)]
+    lib.icarion_gpu_batch_autotune.argtypes = [C.c_void_p, C.c_uint32, P(_abi.BatchTuning),
+                                               P(_abi.BatchTuning), P(C.c_double), P(C.c_double)]
+    lib.icarion_gpu_batch_download.argtypes = [C.c_void_p, P(_abi.CloudDesc), C.c_uint32]
     lib.icarion_gpu_set_species.argtypes = [C.c_void_p, P(_abi.SpeciesDesc), C.c_uint32,
                                             C.c_void_p, C.c_uint32, C.c_int32]
     f32p, f64p = P(C.c_float), P(C.c_double)
@@ -115,6 +122,7 @@ class Handle:
         _check(self.lib.icarion_gpu_create(C.byref(desc), C.byref(h)), self.lib)
         self.h = h
         self._keep = []
+        self._batch_n = 0
 
     def close(self):
         if self.h:
@@ -140,6 +148,56 @@ class Handle:
     def set_rk45(self, desc):
         """RK45 step control (an _abi.Rk45Desc, e.g. _abi.rk45_defaults(atol=...))."""
         _check(self.lib.icarion_gpu_set_rk45(self.h, C.byref(desc)), self.lib)
+
+    # ---- batched clouds ------------------------------------------------------
+    def _cloud_array(self, clouds):
+        """clouds: list of dicts with domain, physics, species (list), ensemble,
+        t0, dt, n_steps and optional step_offset."""
+        arr = (_abi.CloudDesc * len(clouds))()
+        keep = []
+        for k, c in enumerate(clouds):
+            sp = species_array(c["species"])
+            soa = c["ensemble"].soa()
+            keep += [sp, soa, c["ensemble"]]
+            arr[k].domain = c["domain"]
+            arr[k].physics = c["physics"]
+            arr[k].species = C.cast(sp, C.POINTER(_abi.SpeciesDesc))
+            arr[k].n_species = len(c["species"])
+            arr[k].ions = C.pointer(soa)
+            arr[k].t0_s = c.get("t0", 0.0)
+            arr[k].dt_s = c["dt"]
+            arr[k].n_steps = c["n_steps"]
+            arr[k].step_offset = c.get("step_offset", 0)
+        return arr, keep
+
+    def batch_upload(self, clouds):
+        arr, keep = self._cloud_array(clouds)
+        _check(self.lib.icarion_gpu_batch_upload(self.h, arr, len(clouds)), self.lib)
+        self._batch_n = len(clouds)
+
+    def batch_run(self, fp64=None, fp32=None):
+        """Returns (list of CloudStats, device_ms).  fp64 / fp32: (block_size,
+        groups_per_cu) launch shapes or None for the tuned ones."""
+        def tun(t):
+            return C.byref(_abi.BatchTuning(*t)) if t is not None else C.POINTER(_abi.BatchTuning)()
+        stats = (_abi.CloudStats * self._batch_n)()
+        ms = C.c_double()
+        _check(self.lib.icarion_gpu_batch_run(self.h, tun(fp64), tun(fp32), stats, C.byref(ms)),
+               self.lib)
+        return list(stats), ms.value
+
+    def batch_autotune(self, trial_steps=200):
+        """Returns ((block, groups) fp64, (block, groups) fp32, rate64, rate32)."""
+        b64, b32 = _abi.BatchTuning(), _abi.BatchTuning()
+        r64, r32 = C.c_double(), C.c_double()
+        _check(self.lib.icarion_gpu_batch_autotune(self.h, trial_steps, C.byref(b64), C.byref(b32),
+                                                   C.byref(r64), C.byref(r32)), self.lib)
+        return ((b64.block_size, b64.groups_per_cu), (b32.block_size, b32.groups_per_cu),
+                r64.value, r32.value)
+
+    def batch_download(self, clouds):
+        arr, keep = self._cloud_array(clouds)
+        _check(self.lib.icarion_gpu_batch_download(self.h, arr, len(clouds)), self.lib)
 
     def set_species(self, species_list, atoms_xyzr=None):
         """atoms_xyzr: (n_atoms, 4) float32 or float64 array (x, y, z, r in metres)."""

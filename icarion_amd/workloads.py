@@ -274,3 +274,33 @@ def ehss_surrogate_workload(n=1_000_000, n_steps=1000, seed=21, pressure_Pa=1013
     ens.species[:] = np.arange(n, dtype=np.uint32)
     return dict(name="ehss_he_20td", domain=dom, physics=phys, counts=counts, ensemble=ens, dt=dt,
                 n_steps=n_steps)
+
+
+def batched_workload(n_clouds=256, size_lo=1_000, size_hi=200_000, seed=5,
+                     precision=_abi.PRECISION_FP32_KAHAN, steps=None):
+    """Batched mixed instruments: n_clouds independent clouds with sizes
+    log-uniform over [size_lo, size_hi], the kinds cycling IMS (uniform E, HSS,
+    RK4, dt 1e-10 s), RF quadrupole (Boris, Langevin + OU, dt 1e-9 s) and
+    Orbitrap (collisionless RK4, dt 1e-9 s).  steps: engine steps per kind
+    {"ims": .., "quadrupole": .., "orbitrap": ..}; default 1e5 / 1e5 / 1e6.
+    Every cloud gets its own RNG seed.  Returns a list of cloud dicts for
+    gpu.Handle.batch_upload."""
+    steps = steps or {"ims": 100_000, "quadrupole": 100_000, "orbitrap": 1_000_000}
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(size_lo), math.log(size_hi), n_clouds)).astype(int)
+    kinds = ("ims", "quadrupole", "orbitrap")
+    clouds = []
+    for c in range(n_clouds):
+        kind = kinds[c % 3]
+        n = int(sizes[c])
+        s = seed * 1000 + c
+        if kind == "ims":
+            w = ims_hss_workload(n=n, seed=s, precision=precision)
+        elif kind == "quadrupole":
+            w = quadrupole_workload(n=n, seed=s, precision=precision)
+        else:
+            w = orbitrap_workload(n=n, seed=s, precision=precision)
+        clouds.append(dict(kind=kind, domain=w["domain"], physics=w["physics"],
+                           species=w["species"], ensemble=w["ensemble"], t0=0.0, dt=w["dt"],
+                           n_steps=int(steps[kind]), step_offset=0))
+    return clouds

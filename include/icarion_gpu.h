@@ -341,6 +341,70 @@ int icarion_gpu_surrogate_infer(icarion_gpu *h, const float *X, size_t n, float 
 /* Block until all work queued on the handle's stream (and the device) is done. */
 int icarion_gpu_synchronize(icarion_gpu *h);
 
+/* ---- batched clouds -------------------------------------------------------
+ * Many independent clouds (each its own domain, physics, species and ions)
+ * advanced in one persistent launch: a fixed pool of workgroups takes tiles
+ * (up to block_size ions of one cloud, all of that cloud's steps) from a
+ * queue ordered longest first.  Per cloud the result is that of
+ * icarion_gpu_step on a handle of its own (same kernels, same random
+ * streams).  Covered: RK4 / Boris; IMS, TIMS, quadrupole and Orbitrap
+ * analytic fields; no collisions, HSS, Langevin / Friction (+ OU kicks);
+ * FP64 or FP32 + Kahan per cloud.  Not covered (ICARION_ERR_UNSUPPORTED):
+ * field arrays, EHSS, RK45.  The batch lives in the handle beside (and
+ * independent of) its single-cloud state. */
+typedef struct icarion_cloud_desc {
+    icarion_domain_desc domain;
+    icarion_physics_desc physics;
+    const icarion_species_desc *species;
+    uint32_t n_species;
+    uint32_t reserved0;
+    icarion_ion_soa *ions;        /* host SoA: read by upload, written by
+                                     download                                  */
+    double t0_s;                  /* engine time of the first step             */
+    double dt_s;
+    uint64_t n_steps;             /* < 2^32                                    */
+    uint64_t step_offset;         /* first RNG step number                     */
+} icarion_cloud_desc;
+
+/* Launch shape of the persistent batch kernel (per precision class). */
+typedef struct icarion_batch_tuning {
+    uint32_t block_size;          /* ions per tile = threads per workgroup:
+                                     64, 128 or 256 (0 = tuned / default)      */
+    uint32_t groups_per_cu;       /* persistent workgroups per compute unit
+                                     (0 = tuned / default)                     */
+} icarion_batch_tuning;
+
+typedef struct icarion_cloud_stats {
+    uint64_t ion_steps;
+    uint64_t collisions;
+    uint64_t alive;
+    uint64_t arrived;
+    double t_end_s;               /* engine time after the cloud's steps       */
+    double done_ms;               /* when the cloud's last tile finished,
+                                     after the first workgroup started         */
+} icarion_cloud_stats;
+
+/* Copy n_clouds clouds to the device (replaces any previous batch). */
+int icarion_gpu_batch_upload(icarion_gpu *h, const icarion_cloud_desc *clouds, uint32_t n_clouds);
+
+/* Run every cloud of the batch through all its steps in one graph launch.
+ * fp64 / fp32: launch shapes of the two precision classes (NULL = the
+ * tuned shapes, else 256 ions x 4 workgroups per CU).  per_cloud (n_clouds
+ * entries) and device_ms may be NULL. */
+int icarion_gpu_batch_run(icarion_gpu *h, const icarion_batch_tuning *fp64,
+                          const icarion_batch_tuning *fp32, icarion_cloud_stats *per_cloud,
+                          double *device_ms);
+
+/* Time each candidate launch shape on the resident batch with every cloud cut
+ * to at most trial_steps steps (the batch state is restored after each trial)
+ * and keep the fastest per precision class for icarion_gpu_batch_run.  best64 /
+ * best32 and the rates (ion-steps/s) may be NULL. */
+int icarion_gpu_batch_autotune(icarion_gpu *h, uint32_t trial_steps, icarion_batch_tuning *best64,
+                               icarion_batch_tuning *best32, double *rate64, double *rate32);
+
+/* Copy the batch state back into the clouds' host SoA arrays. */
+int icarion_gpu_batch_download(icarion_gpu *h, const icarion_cloud_desc *clouds, uint32_t n_clouds);
+
 /* One engine step on a host-owned ensemble: upload, one step, download.
  * The per-call shape of the reference batch hooks (IIntegrationStrategy.h:106-132,
  * ICollisionHandler.h:109-134), crossing PCIe both ways. */

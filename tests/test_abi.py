@@ -50,9 +50,17 @@ STRUCTS = {
     "icarion_physics_desc": (_abi.PhysicsDesc, ["integrator", "precision", "rng_seed"]),
     "icarion_species_desc": (_abi.SpeciesDesc, ["sigma_hss_m2", "K0_cm2_Vs", "atom_offset",
                                                 "atom_count"]),
-    "icarion_ion_soa": (_abi.IonSoA, ["n", "x", "species", "active", "born", "n_collisions"]),
-    "icarion_step_stats": (_abi.StepStats, ["ion_steps", "alive", "t_end_s", "device_ms"]),
+    "icarion_ion_soa": (_abi.IonSoA, ["n", "x", "species", "active", "born", "n_collisions",
+                                      "time_s", "dt_s"]),
+    "icarion_step_stats": (_abi.StepStats, ["ion_steps", "alive", "rejected", "t_end_s",
+                                            "device_ms"]),
     "icarion_gpu_desc": (_abi.GpuDesc, ["device", "block_size", "steps_per_launch"]),
+    "icarion_rk45_desc": (_abi.Rk45Desc, ["atol", "absolute_min_step_s", "accept_at_dt_min"]),
+    "icarion_train_desc": (_abi.TrainDesc, ["epochs", "lr", "eps", "seed"]),
+    "icarion_cloud_desc": (_abi.CloudDesc, ["domain", "physics", "species", "n_species", "ions",
+                                            "t0_s", "n_steps", "step_offset"]),
+    "icarion_batch_tuning": (_abi.BatchTuning, ["block_size", "groups_per_cu"]),
+    "icarion_cloud_stats": (_abi.CloudStats, ["ion_steps", "arrived", "t_end_s", "done_ms"]),
 }
 
 

@@ -1,0 +1,152 @@
+"""Batched clouds in one persistent launch (icarion_gpu_batch_*): each cloud's
+result must be the one a handle of its own gives (same kernel bodies, same
+random streams) - to 1e-9 relative, the per-step bar of the single-cloud
+parity tests, since the batch kernel is a separate function the compiler
+contracts on its own (differences seen: ~1e-13) - and bit for bit the same
+whatever the tile / launch shape; the autotuner only picks among launch
+shapes.  Single-cloud runs are pinned against the oracle
+(test_gpu_parity.py), which carries that parity over to the batch."""
+import numpy as np
+import pytest
+
+from icarion_amd import _abi
+from icarion_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "death_time_s", "arrival_time_s", "n_collisions",
+          "active", "born")
+
+
+def mixed_clouds(seed=3):
+    """Twelve small clouds over every batch kind and both precisions."""
+    rng = np.random.default_rng(seed)
+    clouds = []
+    for k in range(12):
+        n = int(rng.integers(100, 3000))
+        prec = _abi.PRECISION_FP64 if k % 2 == 0 else _abi.PRECISION_FP32_KAHAN
+        kind = ("ims", "quadrupole", "orbitrap", "quadrupole_rk4_friction")[k % 4]
+        if kind == "ims":
+            w = W.ims_hss_workload(n=n, seed=100 + k, precision=prec)
+            steps = int(rng.integers(50, 400))
+        elif kind == "quadrupole":
+            w = W.quadrupole_workload(n=n, U=10.0, seed=100 + k, precision=prec)
+            steps = int(rng.integers(50, 400))
+        elif kind == "orbitrap":
+            w = W.orbitrap_workload(n=n, seed=100 + k, precision=prec)
+            steps = int(rng.integers(200, 800))
+        else:
+            w = W.quadrupole_workload(n=n, U=5.0, seed=100 + k, precision=prec)
+            w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_FRICTION, ou=1,
+                                     precision=prec, seed=100 + k)
+            w["species"] = [W.species(K0=2.0)]
+            steps = int(rng.integers(50, 400))
+        clouds.append(dict(kind=kind, domain=w["domain"], physics=w["physics"],
+                           species=w["species"], ensemble=w["ensemble"].copy(),
+                           t0=1e-7 * k, dt=w["dt"], n_steps=steps, step_offset=7 * k))
+    return clouds
+
+
+def copy_clouds(clouds):
+    out = []
+    for c in clouds:
+        d = dict(c)
+        d["ensemble"] = c["ensemble"].copy()
+        out.append(d)
+    return out
+
+
+def single_runs(clouds):
+    from icarion_amd import gpu
+    stats = []
+    for c in clouds:
+        with gpu.Handle() as h:
+            h.set_domain(c["domain"], c["physics"])
+            h.set_species(c["species"])
+            h.upload_state(c["ensemble"])
+            stats.append(h.step(c["t0"], c["dt"], c["n_steps"], c["step_offset"]))
+            h.download_state(c["ensemble"])
+    return stats
+
+
+def batch_runs(clouds, fp64=None, fp32=None, autotune=0):
+    from icarion_amd import gpu
+    with gpu.Handle() as h:
+        h.batch_upload(clouds)
+        tuned = h.batch_autotune(autotune) if autotune else None
+        stats, ms = h.batch_run(fp64, fp32)
+        h.batch_download(clouds)
+    return stats, ms, tuned
+
+
+def assert_same(a, b):
+    for ca, cb in zip(a, b):
+        for f in FIELDS:
+            np.testing.assert_array_equal(getattr(ca["ensemble"], f), getattr(cb["ensemble"], f),
+                                          err_msg=f"{ca['kind']} {f}")
+
+
+def close_fraction(a, b, rtol=1e-9):
+    """Fraction of ions whose state agrees to rtol (positions on a 1 um scale,
+    velocities on 1 m/s) with identical discrete state."""
+    ok = np.ones(a.n, bool)
+    for f, scale in (("x", 1e-6), ("y", 1e-6), ("z", 1e-6), ("vx", 1.0), ("vy", 1.0),
+                     ("vz", 1.0)):
+        u, v = getattr(a, f), getattr(b, f)
+        ok &= np.abs(u - v) <= rtol * np.maximum(np.abs(v), scale)
+    for f in ("active", "born", "n_collisions"):
+        ok &= getattr(a, f) == getattr(b, f)
+    return ok.mean()
+
+
+def test_batch_equals_one_handle_per_cloud():
+    ref = mixed_clouds()
+    bat = copy_clouds(ref)
+    s_ref = single_runs(ref)
+    s_bat, ms, _ = batch_runs(bat)
+    for ca, cb, a, b in zip(bat, ref, s_bat, s_ref):
+        frac = close_fraction(ca["ensemble"], cb["ensemble"])
+        stochastic = ca["physics"].collision_model != _abi.COLLISION_NONE
+        assert frac >= (0.995 if stochastic else 1.0), (ca["kind"], frac)
+        assert a.ion_steps == pytest.approx(b.ion_steps, rel=1e-3)
+        assert a.collisions == pytest.approx(b.collisions, rel=5e-3, abs=2)
+        assert a.t_end_s == b.t_end_s
+        assert 0.0 < a.done_ms <= ms * 1.05 + 0.05
+
+
+@pytest.mark.parametrize("shape", [(64, 1), (128, 8), (256, 16)])
+def test_batch_result_does_not_depend_on_the_launch_shape(shape):
+    base = mixed_clouds(seed=9)
+    other = copy_clouds(base)
+    batch_runs(base, fp64=(256, 4), fp32=(256, 4))
+    batch_runs(other, fp64=shape, fp32=shape)
+    assert_same(other, base)
+
+
+def test_batch_autotune_picks_a_candidate_and_keeps_results():
+    base = mixed_clouds(seed=11)
+    tuned = copy_clouds(base)
+    batch_runs(base)
+    _, _, (t64, t32, r64, r32) = batch_runs(tuned, autotune=100)
+    for b, g in (t64, t32):
+        assert b in (64, 128, 256) and g in (1, 2, 4, 8, 16) and b * g <= 2048
+    assert r64 > 0 and r32 > 0
+    assert_same(tuned, base)
+
+
+def test_batch_rejects_what_it_does_not_cover():
+    from icarion_amd import gpu
+    w = W.ims_hss_workload(n=64)
+    bad = [dict(domain=w["domain"], physics=W.physics(_abi.INTEGRATOR_RK45), species=w["species"],
+                ensemble=w["ensemble"], dt=1e-10, n_steps=10)]
+    with gpu.Handle() as h:
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_upload(bad)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        bad[0]["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS)
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_upload(bad)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_run()
+        assert ei.value.code == _abi.ERR_STATE
