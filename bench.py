@@ -71,7 +71,8 @@ EHSS_FLOPS_COLLISION = 2 * (8 * 128 + 128 * 128 + 128 * 3) + 2 * 128 + 2 * 128 +
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ehss", "quadrupole"), default="ehss")
+    ap.add_argument("--workload", choices=("ehss", "quadrupole", "batched", "field_array", "ims"),
+                    default="ehss")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed engine steps (default: ehss 4000, quadrupole 100000)")
     ap.add_argument("--warmup", type=int, default=200)
@@ -95,6 +96,12 @@ def parse():
     ap.add_argument("--cpu-sample-ions", type=int, default=None)
     ap.add_argument("--cpu-sample-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--clouds", type=int, default=256, help="batched: number of clouds")
+    ap.add_argument("--steps-scale", type=float, default=1.0,
+                    help="batched: factor on every cloud's step count (1 = the full workload)")
+    ap.add_argument("--autotune-steps", type=int, default=300,
+                    help="batched: steps per cloud in each autotuner trial (0 = no tuning)")
+    ap.add_argument("--fp64", action="store_true", help="batched: FP64 clouds instead of FP32")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter "
                          "calibration for rocprofv3 --pmc); loads torch, not the library")
@@ -415,6 +422,203 @@ def bench_quadrupole(args):
     return res
 
 
+# ---------------------------------------------------------------- batched clouds
+# Algorithmic FLOPs per ion-step of the FP32 + Kahan bodies, counted from
+# step_kernels.hip as above: IMS RK4 + HSS 84 push + 31 collision test + ~20 x
+# the collision fraction; quadrupole Boris + Langevin + OU 122; Orbitrap RK4 4 x
+# (13 field + 3 accel + 9 stage) + 30 combination + 24 Kahan = 154.
+BATCH_FLOPS = {"ims": 130, "quadrupole": 122, "orbitrap": 154}
+
+
+def bench_batched(args):
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    prec = _abi.PRECISION_FP64 if args.fp64 else _abi.PRECISION_FP32_KAHAN
+    steps = {k: max(1, int(round(v * args.steps_scale)))
+             for k, v in {"ims": 100_000, "quadrupole": 100_000, "orbitrap": 1_000_000}.items()}
+    t0 = time.perf_counter()
+    clouds = W.batched_workload(n_clouds=args.clouds, precision=prec, steps=steps)
+    n_ions = sum(c["ensemble"].n for c in clouds)
+    log(f"batched: {args.clouds} clouds, {n_ions} ions, steps {steps} "
+        f"(built in {time.perf_counter() - t0:.1f} s)")
+    with gpu.Handle(device=0) as h:
+        h.batch_upload(clouds)
+        tuning = None
+        if args.autotune_steps:
+            t1 = time.perf_counter()
+            t64, t32, r64, r32 = h.batch_autotune(args.autotune_steps)
+            tuning = {"fp64": t64, "fp32": t32, "trial_steps": args.autotune_steps,
+                      "trial_rate": r64 if args.fp64 else r32,
+                      "seconds": time.perf_counter() - t1}
+            log(f"autotuned {t64 if args.fp64 else t32} in {tuning['seconds']:.1f} s")
+        h.synchronize()
+        t1 = time.perf_counter()
+        stats, dev_ms = h.batch_run()
+        h.synchronize()
+        wall = time.perf_counter() - t1
+    total = sum(st.ion_steps for st in stats)
+    value = total / wall
+    log(f"batched: {total:.3e} ion-steps in {wall:.2f} s -> {value:.3e} ion-steps/s")
+    done = np.array([st.done_ms for st in stats])
+    kinds = [c["kind"] for c in clouds]
+    flops = sum(BATCH_FLOPS[k] * st.ion_steps for k, st in zip(kinds, stats))
+    peak = MI355X_PEAK_FP64_TFLOPS if args.fp64 else MI355X_PEAK_FP32_TFLOPS
+    achieved = flops / (dev_ms * 1e-3) / 1e12
+    per_kind = {}
+    for k in ("ims", "quadrupole", "orbitrap"):
+        sel = [i for i, kk in enumerate(kinds) if kk == k]
+        per_kind[k] = {"clouds": len(sel), "ions": int(sum(clouds[i]["ensemble"].n for i in sel)),
+                       "steps": steps[k],
+                       "ion_steps": int(sum(stats[i].ion_steps for i in sel)),
+                       "alive_fraction": float(sum(stats[i].alive for i in sel)
+                                               / max(1, sum(clouds[i]["ensemble"].n for i in sel))),
+                       "done_ms_max": float(done[sel].max()) if sel else 0.0,
+                       "done_ms_median": float(np.median(done[sel])) if sel else 0.0}
+    return {
+        "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": max(steps.values()),
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.fp64 else "f32 (Kahan-compensated state; f64 time and RF phase)",
+        "data": "synthetic",
+        "config": {"workload": "batched_mixed_instruments", "clouds": args.clouds,
+                   "ions": n_ions, "sizes": "log-uniform 1e3-2e5", "steps_per_kind": steps,
+                   "launch": "one graph: reset + persistent tile-queue kernel",
+                   "tuning": tuning},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": "batch_kernel<%s>" % ("FP64" if args.fp64 else "FP32"),
+                     "device_ms": dev_ms, "flops_per_ion_step": BATCH_FLOPS},
+        "per_kind": per_kind,
+        "completion_ms": {"min": float(done.min()), "median": float(np.median(done)),
+                          "max": float(done.max())},
+    }
+
+
+# ---------------------------------------------------------------- field array (C3)
+def bench_field_array(args):
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    steps = args.steps if args.steps is not None else 20_000
+    t0 = time.perf_counter()
+    w = W.field_array_workload(n=args.ions if args.ions != 1_000_000 else 4_000_000,
+                               n_steps=steps)
+    n = w["ensemble"].n
+    log(f"field array: {n} ions x {steps} steps, grid built in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    for name, prec in (("fp32", _abi.PRECISION_FP32_KAHAN), ("fp64", _abi.PRECISION_FP64)):
+        if name == "fp64" and args.skip_fp64:
+            continue
+        ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, precision=prec,
+                       seed=w["physics"].rng_seed)
+        e = w["ensemble"].copy()
+        with gpu.Handle(device=0, steps_per_launch=args.steps_per_launch) as h:
+            h.set_domain(w["domain"], ph)
+            h.set_species(w["species"])
+            E, origin, spacing = w["field"]
+            h.upload_field_array(E, origin, spacing)
+            st, wall = timed_steps(h, e, w["dt"], steps, min(args.warmup, steps))
+            h.download_state(e)
+        out[name] = dict(stats=st, wall=wall, ens=e)
+        log(f"{name}: {wall:.2f} s, {st.ion_steps / wall:.3e} ion-steps/s")
+    fs = out["fp32"]["stats"]
+    value = fs.ion_steps / out["fp32"]["wall"]
+    spl = args.steps_per_launch or 2000
+    launches = math.ceil(steps / spl)
+    avg = fs.device_ms * 1e-3 / launches
+    # trilinear gathers: 4 stages x 8 corners x 16 B (float4) per ion-step
+    gbytes = 4 * 8 * 16 * fs.ion_steps / launches / avg / 1e9
+    res = {
+        "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": min(args.warmup, steps), "ms_per_step": out["fp32"]["wall"] * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (Kahan-compensated state; f64 time)", "data": "synthetic",
+        "config": {"workload": "field_array_hss_rk4", "ions": n, "grid": [256, 256, 256],
+                   "extent_mm": [10, 10, 50], "gas": "N2 300 Pa 300 K", "dt_s": w["dt"]},
+        "roofline": {"bound": "hbm", "achieved": gbytes, "peak": 8000.0, "unit": "GB/s",
+                     "frac": gbytes / 8000.0, "traffic": None,
+                     "kernel": "step_f32_kernel<RK4, field array, HSS>",
+                     "note": "achieved = algorithmic gather bytes (4 stages x 8 corners x 16 B "
+                             "per ion-step); most gathers hit L2 / MALL"},
+        "alive_fraction": fs.alive / n,
+    }
+    if "fp64" in out:
+        ps = out["fp64"]["stats"]
+        pv = ps.ion_steps / out["fp64"]["wall"]
+        g, o = out["fp32"]["ens"], out["fp64"]["ens"]
+        z0 = w["ensemble"].z
+        res["fp64_plain"] = {"value": pv, "unit": "ion-steps/s"}
+        res["fast_over_plain"] = value / pv
+        res["fp32_vs_fp64"] = {
+            "alive": [float(g.alive.mean()), float(o.alive.mean())],
+            "mean_dz_m": [float((g.z - z0)[g.alive].mean()), float((o.z - z0)[o.alive].mean())]}
+    return res
+
+
+# ---------------------------------------------------------------- IMS (C1)
+def bench_ims(args):
+    """C1 at full size on the GPU (FP64 and FP32) and on the host (the oracle,
+    all cores): mobility K, arrival-time mean and width at the detector plane
+    and survival, compared."""
+    from oracle import binding
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    steps = args.steps if args.steps is not None else 100_000
+    n = args.ions if args.ions != 1_000_000 else 10_000
+    w = W.ims_hss_workload(n=n, n_steps=steps)
+    z0 = w["ensemble"].z.copy()
+    E = w["domain"].dc_axial_V / w["domain"].length_m
+    T = steps * w["dt"]
+
+    def summary(e, st, wall):
+        a = e.alive
+        arr = e.arrival_time_s[np.isfinite(e.arrival_time_s)]
+        return {"ion_steps_per_s": st.ion_steps / wall, "K_m2_Vs": float((e.z - z0)[a].mean() / (T * E)),
+                "arrival_mean_s": float(arr.mean()) if arr.size else None,
+                "arrival_std_s": float(arr.std()) if arr.size else None,
+                "arrived_fraction": float(arr.size / e.n), "survival": float(a.mean()),
+                "collisions_per_ion_step": st.collisions / max(st.ion_steps, 1)}
+    out = {}
+    for name, prec in (("fp32", _abi.PRECISION_FP32_KAHAN), ("fp64", _abi.PRECISION_FP64)):
+        ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, precision=prec,
+                       seed=w["physics"].rng_seed)
+        e = w["ensemble"].copy()
+        with gpu.Handle(device=0) as h:
+            h.set_domain(w["domain"], ph)
+            h.set_species(w["species"])
+            st, wall = timed_steps(h, e, w["dt"], steps, 0)
+            h.download_state(e)
+        out[name] = summary(e, st, wall)
+        log(f"ims {name}: {out[name]}")
+    e = w["ensemble"].copy()
+    threads = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    st = binding.run(w["domain"], w["physics"], w["species"], e, 0.0, w["dt"], steps,
+                     threads=threads)
+    wall = time.perf_counter() - t0
+    out["oracle"] = summary(e, st, wall)
+    log(f"ims oracle: {out['oracle']}")
+    f, o = out["fp32"], out["oracle"]
+    return {
+        "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
+        "value": f["ion_steps_per_s"], "unit": "ion-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (Kahan-compensated state; f64 time)",
+        "data": "synthetic",
+        "config": {"workload": "ims_hss_rk4_10td", "ions": n, "dt_s": w["dt"],
+                   "gas": "N2 101325 Pa 300 K", "E_over_N_Td": 10.0},
+        "results": out,
+        "fp32_vs_oracle": {"K_rel": abs(f["K_m2_Vs"] / o["K_m2_Vs"] - 1),
+                           "arrival_mean_rel": (abs(f["arrival_mean_s"] / o["arrival_mean_s"] - 1)
+                                                if o["arrival_mean_s"] else None),
+                           "survival_abs": abs(f["survival"] - o["survival"])},
+        "cpu_baseline": {"value": o["ion_steps_per_s"], "unit": "ion-steps/s", "cores": threads,
+                         "kind": "port", "sample": f"the whole workload ({n} ions x {steps} "
+                                                    f"steps), FP64 oracle, {wall:.1f} s"},
+    }
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -428,7 +632,8 @@ def main():
     if gpu.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    res = bench_ehss(args) if args.workload == "ehss" else bench_quadrupole(args)
+    res = {"ehss": bench_ehss, "quadrupole": bench_quadrupole, "batched": bench_batched,
+           "field_array": bench_field_array, "ims": bench_ims}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 
