@@ -74,9 +74,11 @@ public:
         check(icarion_gpu_set_domain(h_, &domain, &physics));
         check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()),
                                       atoms_xyzr.empty() ? nullptr : atoms_xyzr.data(),
-                                      static_cast<uint32_t>(atoms_xyzr.size() / 4)));
+                                      static_cast<uint32_t>(atoms_xyzr.size() / 4), 1));
         ready_ = true;
     }
+    // Step control of ICARION_INTEGRATOR_RK45 (RK45Strategy(const AdaptiveConfig&)).
+    void set_rk45(const icarion_rk45_desc& cfg) { check(icarion_gpu_set_rk45(h_, &cfg)); }
     void upload_field_array(uint32_t nx, uint32_t ny, uint32_t nz, const double origin[3],
                             const double spacing[3], const float* E_xyz) {
         check(icarion_gpu_upload_field_array(h_, nx, ny, nz, origin[0], spacing[0], origin[1],
@@ -123,6 +125,46 @@ private:
     }
     icarion_gpu* h_ = nullptr;
     bool ready_ = false;
+};
+
+// Many independent single-domain clouds in one persistent launch
+// (icarion_gpu_batch_*).  Each cloud's result is that of a BatchStepper of its
+// own run in resident mode with the same descriptors.  The clouds' host SoA
+// arrays must stay valid from upload() to download().
+class CloudBatch {
+public:
+    explicit CloudBatch(const icarion_gpu_desc& desc) { check(icarion_gpu_create(&desc, &h_)); }
+    ~CloudBatch() {
+        if (h_) icarion_gpu_destroy(h_);
+    }
+    CloudBatch(const CloudBatch&) = delete;
+    CloudBatch& operator=(const CloudBatch&) = delete;
+
+    void upload(std::vector<icarion_cloud_desc> clouds) {
+        check(icarion_gpu_batch_upload(h_, clouds.data(), static_cast<uint32_t>(clouds.size())));
+        clouds_ = std::move(clouds);
+    }
+    // Picks the launch shape per precision class on the resident batch (its
+    // state is restored); returns the FP64 and FP32 choices.
+    std::pair<icarion_batch_tuning, icarion_batch_tuning> autotune(uint32_t trial_steps = 300) {
+        icarion_batch_tuning b64{}, b32{};
+        check(icarion_gpu_batch_autotune(h_, trial_steps, &b64, &b32, nullptr, nullptr));
+        return {b64, b32};
+    }
+    // One graph launch over every cloud; per-cloud counters and completion times.
+    std::vector<icarion_cloud_stats> run(double* device_ms = nullptr) {
+        std::vector<icarion_cloud_stats> st(clouds_.size());
+        check(icarion_gpu_batch_run(h_, nullptr, nullptr, st.data(), device_ms));
+        return st;
+    }
+    void download() {
+        check(icarion_gpu_batch_download(h_, clouds_.data(), static_cast<uint32_t>(clouds_.size())));
+    }
+    icarion_gpu* handle() const { return h_; }
+
+private:
+    icarion_gpu* h_ = nullptr;
+    std::vector<icarion_cloud_desc> clouds_;
 };
 
 }  // namespace icarion

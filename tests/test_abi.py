@@ -81,3 +81,36 @@ def test_ctypes_layout_matches_c_compiler(tmp_path):
         assert int(got[cname]) == C.sizeof(cls), cname
         for f in fields:
             assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, f"{cname}.{f}"
+
+
+def test_cpp_adaptor_compiles_and_links(tmp_path):
+    """include/icarion_gpu.hpp compiles as C++17 and links against the built
+    library; without a device the never-throwing factory yields an empty
+    stepper whose step_batch() reports "not handled"."""
+    lib = gpu.load()
+    so = lib._name
+    src = tmp_path / "adaptor.cpp"
+    src.write_text(
+        '#include "icarion_gpu.hpp"\n'
+        "#include <cstdio>\n"
+        "int main() {\n"
+        "    icarion::BatchStepper s = icarion::BatchStepper::try_create(icarion_gpu_desc{0, 0, 0, 0});\n"
+        "    icarion_ion_soa soa{};\n"
+        "    const bool handled = s.step_batch(soa, 0.0, 1e-9, 0);\n"
+        "    int code = -1;\n"
+        "    try { icarion::CloudBatch b(icarion_gpu_desc{0, 0, 0, 0}); code = 0; }\n"
+        "    catch (const icarion::Error& e) { code = e.code(); }\n"
+        '    std::printf("%d %d %d %d\\n", icarion_gpu_abi_version(), icarion_gpu_device_count(),\n'
+        "                handled ? 1 : 0, code);\n"
+        "    return 0;\n"
+        "}\n")
+    exe = tmp_path / "adaptor"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), so, f"-Wl,-rpath,{os.path.dirname(so)}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    version, devices, handled, code = map(int, out)
+    assert version == _abi.ABI_VERSION
+    if devices > 0:
+        pytest.skip("device present")
+    assert handled == 0
+    assert code == _abi.ERR_NO_DEVICE
