@@ -71,7 +71,8 @@ EHSS_FLOPS_COLLISION = 2 * (8 * 128 + 128 * 128 + 128 * 3) + 2 * 128 + 2 * 128 +
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ehss", "quadrupole", "batched", "field_array", "ims"),
+    ap.add_argument("--workload", choices=("ehss", "quadrupole", "batched", "batched_ehss",
+                                           "field_array", "ims"),
                     default="ehss")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed engine steps (default: ehss 4000, quadrupole 100000)")
@@ -209,6 +210,42 @@ def mobility_check(args, params):
             "rate_plain": sp.ion_steps / tp, "rate_surrogate": ss.ion_steps / ts}
 
 
+def angle_law_check(w, params, n_clusters=64, n_samples=20000):
+    """Scattering-angle law of the surrogate against plain EHSS on clusters
+    outside the training set (device clusters, device EHSS precompute, matrix-
+    core inference): per-cluster two-sample KS distance of mu = cos(contact
+    normal, axis), next to that of two independent EHSS samples and of the
+    hard-sphere law."""
+    from scipy import stats
+    from icarion_amd import _abi, gpu
+    from icarion_amd import surrogate as S
+    from icarion_amd import workloads as W
+    seed = 9999
+    counts = W.ehss_cluster_counts(n_clusters, seed=seed)
+    rng = np.random.default_rng(seed)
+    with gpu.Handle(device=0) as h:
+        h.set_domain(w["domain"], W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS))
+        _, _, feat = h.synth_clusters(counts, seed=seed)
+        mu1 = h.ehss_precompute(0, n_clusters, n_samples, 0, 1, seed=31)[1][..., 0]
+        mu2 = h.ehss_precompute(0, n_clusters, n_samples, 0, 1, seed=32)[1][..., 0]
+        h.surrogate_set_params(params)
+        X = np.zeros((n_clusters, n_samples, 6), np.float32)
+        psi = rng.uniform(0.0, 1.0, X.shape[:2])
+        X[..., 0] = rng.uniform(0.0, 1.0, X.shape[:2])
+        X[..., 1], X[..., 2] = np.cos(2 * np.pi * psi), np.sin(2 * np.pi * psi)
+        X[..., 3:] = feat[:, None, :]
+        mu_s = S.mu_from_output(h.surrogate_infer(X.reshape(-1, 6))).reshape(mu1.shape)
+    sur = [stats.ks_2samp(mu_s[c], mu1[c]) for c in range(n_clusters)]
+    d_null = [stats.ks_2samp(mu2[c], mu1[c]).statistic for c in range(n_clusters)]
+    d_hs = [stats.ks_2samp(np.sqrt(rng.uniform(0.0, 1.0, n_samples)), mu1[c]).statistic
+            for c in range(n_clusters)]
+    return {"held_out_clusters": n_clusters, "samples_per_cluster": n_samples,
+            "ks_D_surrogate_vs_ehss": float(np.mean([r.statistic for r in sur])),
+            "ks_D_ehss_vs_ehss": float(np.mean(d_null)),
+            "ks_D_hard_sphere_vs_ehss": float(np.mean(d_hs)),
+            "clusters_with_p_below_0.01": int(sum(r.pvalue < 0.01 for r in sur))}
+
+
 def cpu_baseline_ehss(w, n_ions, n_steps):
     """The oracle's plain EHSS (FP64) on the host cores over the first n_ions
     ions of the workload (their clusters regenerated on the device)."""
@@ -316,6 +353,10 @@ def bench_ehss(args):
         res["mobility_check"] = mc = mobility_check(args, params)
         log(f"K rel. error {mc['rel_error']:.4%} (one-K statistical error "
             f"{mc['stat_error_one_K']:.3%})")
+    if not args.no_k_check:
+        res["angle_law_check"] = al = angle_law_check(w, params)
+        log(f"angle law KS D: surrogate {al['ks_D_surrogate_vs_ehss']:.4f}, two EHSS samples "
+            f"{al['ks_D_ehss_vs_ehss']:.4f}, hard sphere {al['ks_D_hard_sphere_vs_ehss']:.4f}")
     if not args.no_cpu:
         n = args.cpu_sample_ions or 8192
         k = args.cpu_sample_steps or 16000
@@ -495,6 +536,83 @@ def bench_batched(args):
     }
 
 
+def bench_batched_ehss(args):
+    """Many EHSS clouds (sizes log-uniform 1e3-2e5, each with its own table of 16
+    random clusters) through the batch's FP32 + surrogate class in one graph
+    launch, against the plain FP64 EHSS rate on the 1M-ion EHSS workload."""
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    steps = args.steps if args.steps is not None else 4000
+    params = np.load(args.params_in or os.path.join(ROOT, "icarion_amd", "data",
+                                                    "surrogate_he_20td_clusters.npy"))
+    n_pool = 4096
+    pool = W.ehss_surrogate_workload(n=n_pool, seed=21)
+    pool["cluster_seed"] = 21
+    h, sig, feat = ehss_handle(pool, _abi.PRECISION_FP32_KAHAN, _abi.EHSS_SURROGATE)
+    h.close()
+    rng = np.random.default_rng(5)
+    sizes = np.exp(rng.uniform(math.log(1e3), math.log(2e5), args.clouds)).astype(int)
+    clouds = []
+    for c, n in enumerate(sizes):
+        idx = rng.choice(n_pool, 16, replace=False)
+        integ = _abi.INTEGRATOR_RK4 if c % 2 == 0 else _abi.INTEGRATOR_BORIS
+        w = W.ehss_surrogate_cloud(int(n), sig[idx], feat[idx], pool["counts"][idx],
+                                   seed=5000 + c, integrator=integ, n_steps=steps)
+        clouds.append(dict(kind="ehss_surrogate", domain=w["domain"], physics=w["physics"],
+                           species=w["species"], ensemble=w["ensemble"], t0=0.0, dt=w["dt"],
+                           n_steps=steps, step_offset=0))
+    n_ions = int(sizes.sum())
+    log(f"batched EHSS: {args.clouds} clouds, {n_ions} ions x {steps} steps")
+    with gpu.Handle(device=0) as h:
+        h.surrogate_set_params(params)
+        h.batch_upload(clouds)
+        h.synchronize()
+        t1 = time.perf_counter()
+        stats, dev_ms = h.batch_run()
+        h.synchronize()
+        wall = time.perf_counter() - t1
+    total = sum(st.ion_steps for st in stats)
+    coll = sum(st.collisions for st in stats)
+    value = total / wall
+    log(f"batched EHSS: {total:.3e} ion-steps in {wall:.2f} s -> {value:.3e} ion-steps/s")
+    achieved = (EHSS_FLOPS_STEP * total + EHSS_FLOPS_COLLISION * coll) / (dev_ms * 1e-3) / 1e12
+    done = np.array([st.done_ms for st in stats])
+    res = {
+        "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (Kahan-compensated state; bf16 MLP weights, f32 accumulation; f64 time)",
+        "data": "synthetic",
+        "config": {"workload": "batched_ehss_surrogate_he_20td", "clouds": args.clouds,
+                   "ions": n_ions, "sizes": "log-uniform 1e3-2e5",
+                   "clusters_per_cloud": 16, "integrators": "RK4 / Boris alternating",
+                   "launch": "one graph: reset + persistent surrogate tile-queue kernel",
+                   "weights": "committed (icarion_amd/data/surrogate_he_20td_clusters.npy)"
+                   if not args.params_in else os.path.basename(args.params_in)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK_BF16_TFLOPS,
+                     "traffic": None, "kernel": "batch_surrogate_kernel", "device_ms": dev_ms,
+                     "collisions_per_ion_step": coll / max(total, 1)},
+        "completion_ms": {"min": float(done.min()), "median": float(np.median(done)),
+                          "max": float(done.max())},
+        "alive_fraction": float(sum(st.alive for st in stats) / n_ions),
+    }
+    if not args.skip_fp64:
+        w = W.ehss_surrogate_workload(n=args.ions, n_steps=args.plain_steps, seed=21)
+        w["cluster_seed"] = 21
+        h, _, _ = ehss_handle(w, _abi.PRECISION_FP64, _abi.EHSS_GEOMETRY, args.steps_per_launch)
+        with h:
+            ps, pwall = timed_steps(h, w["ensemble"].copy(), w["dt"], args.plain_steps,
+                                    min(args.warmup, args.plain_steps))
+        pv = ps.ion_steps / pwall
+        log(f"fp64 plain EHSS (1M-ion workload) {pv:.3e} ion-steps/s")
+        res["fp64_plain"] = {"value": pv, "unit": "ion-steps/s", "steps": args.plain_steps,
+                             "workload": "ehss_he_20td, %d ions" % args.ions}
+        res["surrogate_batched_over_plain"] = value / pv
+    return res
+
+
 # ---------------------------------------------------------------- field array (C3)
 def bench_field_array(args):
     from icarion_amd import _abi, gpu
@@ -633,7 +751,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
     res = {"ehss": bench_ehss, "quadrupole": bench_quadrupole, "batched": bench_batched,
-           "field_array": bench_field_array, "ims": bench_ims}[args.workload](args)
+           "batched_ehss": bench_batched_ehss, "field_array": bench_field_array,
+           "ims": bench_ims}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -32,9 +32,9 @@ hipError_t launch_step(int integrator, int field, int coll, int precision, int s
 hipError_t launch_rk45_step(int field, int coll, const StepParams& P, const IonPtrs& S, const Rk45Ptrs& R,
                             uint32_t block, hipStream_t stream);
 hipError_t launch_rk45_init(const Rk45Ptrs& R, uint32_t n, double t0, double dt, hipStream_t stream);
-hipError_t launch_batch(const BatchParams& B64, const BatchParams& B32, uint32_t n_clouds,
-                        unsigned long long* counters, uint32_t block64, uint32_t grid64, uint32_t block32,
-                        uint32_t grid32, hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join);
+hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
+                        const uint4* blob, uint32_t n_clouds, unsigned long long* counters, hipStream_t s,
+                        const hipStream_t* side, hipEvent_t fork, const hipEvent_t* join);
 hipError_t launch_synth_clusters(float4* atoms, const uint32_t* offsets, const uint32_t* counts,
                                  uint32_t n, float r_min, float r_max, uint64_t seed, hipStream_t s);
 hipError_t launch_oapa(const float4* atoms, const uint32_t* offsets, const uint32_t* counts, uint32_t n,
@@ -167,6 +167,7 @@ struct icarion_gpu {
     struct BatchCloud {
         size_t n = 0, off = 0;
         int precision = 0;
+        int cls = 0;                 // precision class (kBatchClasses)
         uint32_t kind = 0;
         uint64_t n_steps = 0;
         double t_end = 0.0;
@@ -179,8 +180,8 @@ struct icarion_gpu {
     DevBuf<StepParams> b_params_d;
     DevBuf<IonPtrs> b_ions_d;
     DevBuf<uint32_t> b_kinds_d, b_queue;
-    DevBuf<BatchTile> b_tiles[2];
-    uint32_t b_n_tiles[2] = {0, 0}, b_tiled_block[2] = {0, 0};
+    DevBuf<BatchTile> b_tiles[kBatchClasses];
+    uint32_t b_n_tiles[kBatchClasses] = {0, 0, 0}, b_tiled_block[kBatchClasses] = {0, 0, 0};
     DevBuf<unsigned long long> b_done, b_start, b_counters;
     DevBuf<double> b_x, b_y, b_z, b_vx, b_vy, b_vz, b_mass, b_charge, b_birth, b_death, b_arrival;
     DevBuf<uint32_t> b_spec, b_ncoll;
@@ -188,8 +189,8 @@ struct icarion_gpu {
     DevBuf<SpeciesDev> b_species;
     DevBuf<StepEntry> b_tables;
     icarion_batch_tuning b_tuned[2] = {{0, 0}, {0, 0}};
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t side[kBatchClasses - 1] = {nullptr, nullptr};   // streams of the other classes
+    hipEvent_t ev_fork = nullptr, ev_join[kBatchClasses - 1] = {nullptr, nullptr};
     int n_cu = 0;
     int clock_khz = 0;
 
@@ -656,6 +657,7 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
 // ---- batched clouds --------------------------------------------------------
 
 uint32_t batch_kind(const icarion_physics_desc& p, int field, int coll) {
+    // the surrogate class has its own kernel: integrator + 2 * Orbitrap
     const uint32_t c = coll == COLL_HSS ? 1u : (coll == COLL_DAMPING ? 2u : 0u);
     return static_cast<uint32_t>(p.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
            2u * (field == FIELD_ORBITRAP ? 1u : 0u) + 4u * c;
@@ -689,7 +691,11 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
             fail(ICARION_ERR_UNSUPPORTED, tag + "RK45 is not covered by the batch");
         const int coll = coll_kind(cd.physics);
         const int field = field_kind(cd.domain);
-        if (coll == COLL_EHSS) fail(ICARION_ERR_UNSUPPORTED, tag + "EHSS is not covered by the batch");
+        const bool surrogate = coll == COLL_EHSS && cd.physics.ehss_mode == ICARION_EHSS_SURROGATE;
+        if (coll == COLL_EHSS && !surrogate)
+            fail(ICARION_ERR_UNSUPPORTED, tag + "EHSS geometry collisions are not covered by the batch");
+        if (surrogate && cd.physics.precision != ICARION_PRECISION_FP32_KAHAN)
+            fail(ICARION_ERR_UNSUPPORTED, tag + "the EHSS surrogate runs in the FP32 + Kahan path");
         if (field == FIELD_ARRAY) fail(ICARION_ERR_UNSUPPORTED, tag + "field arrays are not covered by the batch");
         if (!(cd.dt_s > 0.0) || !std::isfinite(cd.dt_s) || !std::isfinite(cd.t0_s))
             fail(ICARION_ERR_INVALID, tag + "t0 must be finite and dt finite and > 0");
@@ -708,7 +714,10 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         bc[c].n = n;
         bc[c].off = total;
         bc[c].precision = cd.physics.precision;
-        bc[c].kind = batch_kind(cd.physics, field, coll);
+        bc[c].cls = surrogate ? 2 : (cd.physics.precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0);
+        bc[c].kind = surrogate ? static_cast<uint32_t>(cd.physics.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
+                                     2u * (field == FIELD_ORBITRAP ? 1u : 0u)
+                               : batch_kind(cd.physics, field, coll);
         bc[c].n_steps = cd.n_steps;
         bc[c].has_birth = io->birth_time_s != nullptr;
         total += n;
@@ -719,6 +728,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
             sd.sigma_hss = d.sigma_hss_m2;
             sd.sigma_ehss = d.sigma_ehss_m2;
             sd.K0_cm2_Vs = d.K0_cm2_Vs;
+            for (int f = 0; f < 3; ++f) sd.feat[f] = d.surrogate_features[f];
             species.push_back(sd);
         }
         StepParams P = derive_params(cd.domain, cd.physics);
@@ -779,7 +789,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     h->b_counters.resize(4ull * n_clouds);
     h->b_done.resize(n_clouds);
     h->b_start.resize(1);
-    h->b_queue.resize(2);
+    h->b_queue.resize(kBatchClasses);
     std::vector<IonPtrs> ions(n_clouds);
     for (uint32_t c = 0; c < n_clouds; ++c) {
         const size_t o = bc[c].off;
@@ -807,7 +817,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     h->b_params = std::move(params);
     h->b_n = n_clouds;
     h->b_total = total;
-    h->b_tiled_block[0] = h->b_tiled_block[1] = 0;
+    for (int k = 0; k < kBatchClasses; ++k) h->b_tiled_block[k] = 0;
 }
 
 // Tiles of one precision class, longest clouds first (step count x ions per
@@ -817,7 +827,7 @@ void batch_build_tiles(icarion_gpu* h, int cls, uint32_t block) {
     if (h->b_tiled_block[cls] == block) return;
     std::vector<uint32_t> order;
     for (uint32_t c = 0; c < h->b_n; ++c)
-        if ((h->b_clouds[c].precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0) == cls && h->b_clouds[c].n)
+        if (h->b_clouds[c].cls == cls && h->b_clouds[c].n)
             order.push_back(c);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
         return h->b_clouds[a].n_steps > h->b_clouds[b].n_steps;
@@ -847,10 +857,20 @@ icarion_batch_tuning batch_shape(const icarion_gpu* h, int cls, const icarion_ba
 // One batch launch captured as a HIP graph (reset + persistent kernels of the
 // classes in `mask`); returns device milliseconds.
 float batch_launch(icarion_gpu* h, icarion_batch_tuning t64, icarion_batch_tuning t32, int mask) {
-    batch_build_tiles(h, 0, t64.block_size);
-    batch_build_tiles(h, 1, t32.block_size);
-    BatchParams B[2];
-    for (int cls = 0; cls < 2; ++cls) {
+    // the surrogate class has one shape: 256 ions per tile and two workgroups
+    // per CU, the occupancy its registers (2 waves per SIMD) and 47 KB of LDS
+    // weights allow
+    const icarion_batch_tuning shape[kBatchClasses] = {t64, t32, {256, 2}};
+    for (int cls = 0; cls < kBatchClasses; ++cls) batch_build_tiles(h, cls, shape[cls].block_size);
+    const bool need_blob = ((mask >> 2) & 1) && h->b_n_tiles[2];
+    if (need_blob && !h->have_surrogate)
+        fail(ICARION_ERR_STATE, "the batch holds EHSS surrogate clouds but no surrogate parameters are set");
+    BatchParams B[kBatchClasses];
+    uint32_t block[kBatchClasses], grid[kBatchClasses];
+    const uint32_t n_cu = static_cast<uint32_t>(std::max(h->n_cu, 1));
+    for (int cls = 0; cls < kBatchClasses; ++cls) {
+        block[cls] = shape[cls].block_size;
+        grid[cls] = shape[cls].groups_per_cu * n_cu;
         B[cls].params = h->b_params_d.p;
         B[cls].ions = h->b_ions_d.p;
         B[cls].kinds = h->b_kinds_d.p;
@@ -861,14 +881,12 @@ float batch_launch(icarion_gpu* h, icarion_batch_tuning t64, icarion_batch_tunin
         B[cls].done = h->b_done.p;
         B[cls].start = h->b_start.p;
     }
-    const uint32_t n_cu = static_cast<uint32_t>(std::max(h->n_cu, 1));
     hipStream_t s = h->stream;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    const hipError_t le = launch_batch(B[0], B[1], h->b_n, h->b_counters.p, t64.block_size,
-                                       t64.groups_per_cu * n_cu, t32.block_size, t32.groups_per_cu * n_cu, s,
-                                       h->stream2, h->ev_fork, h->ev_join);
+    const hipError_t le = launch_batch(B, block, grid, h->mlp_blob.p, h->b_n, h->b_counters.p, s, h->side,
+                                       h->ev_fork, h->ev_join);
     const hipError_t ee = hipStreamEndCapture(s, &graph);
     if (le != hipSuccess || ee != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -897,7 +915,7 @@ void do_batch_run(icarion_gpu* h, const icarion_batch_tuning* r64, const icarion
     if (h->b_n == 0) fail(ICARION_ERR_STATE, "icarion_gpu_batch_upload not called");
     set_device(h);
     const icarion_batch_tuning t64 = batch_shape(h, 0, r64), t32 = batch_shape(h, 1, r32);
-    const float ms = batch_launch(h, t64, t32, 3);
+    const float ms = batch_launch(h, t64, t32, (1 << kBatchClasses) - 1);
     if (device_ms) *device_ms = ms;
     if (!per_cloud) return;
     const uint32_t n = h->b_n;
@@ -963,7 +981,7 @@ void do_batch_autotune(icarion_gpu* h, uint32_t trial_steps, icarion_batch_tunin
     double best_rate[2] = {0.0, 0.0};
     for (int cls = 0; cls < 2; ++cls) {
         bool present = false;
-        for (const auto& c : h->b_clouds) present |= (c.precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0) == cls;
+        for (const auto& c : h->b_clouds) present |= c.cls == cls;
         if (!present) continue;
         for (uint32_t b : kBlocks) {
             for (uint32_t g : kGroups) {
@@ -1187,9 +1205,11 @@ int icarion_gpu_create(const icarion_gpu_desc* desc, icarion_gpu** out) {
             hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
             hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
             h->counters.resize(4);
-            hip_check(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking), "hipStreamCreate");
             hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
-            hip_check(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming), "hipEventCreate");
+            for (int k = 0; k < kBatchClasses - 1; ++k) {
+                hip_check(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking), "hipStreamCreate");
+                hip_check(hipEventCreateWithFlags(&h->ev_join[k], hipEventDisableTiming), "hipEventCreate");
+            }
             hip_check(hipDeviceGetAttribute(&h->n_cu, hipDeviceAttributeMultiprocessorCount, h->device),
                       "hipDeviceGetAttribute");
             hip_check(hipDeviceGetAttribute(&h->clock_khz, hipDeviceAttributeWallClockRate, h->device),
@@ -1206,20 +1226,24 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     if (!h) return ICARION_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
+    for (hipStream_t st : h->side)
+        if (st) (void)hipStreamSynchronize(st);
     h->species.release(); h->atoms_f32.release(); h->atom_offsets.release(); h->atom_counts.release();
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
     h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
     h->b_params_d.release(); h->b_ions_d.release(); h->b_kinds_d.release(); h->b_queue.release();
-    h->b_tiles[0].release(); h->b_tiles[1].release(); h->b_done.release(); h->b_start.release();
+    for (auto& t : h->b_tiles) t.release();
+    h->b_done.release(); h->b_start.release();
     h->b_counters.release(); h->b_x.release(); h->b_y.release(); h->b_z.release(); h->b_vx.release();
     h->b_vy.release(); h->b_vz.release(); h->b_mass.release(); h->b_charge.release(); h->b_birth.release();
     h->b_death.release(); h->b_arrival.release(); h->b_spec.release(); h->b_ncoll.release();
     h->b_flags.release(); h->b_species.release(); h->b_tables.release();
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
-    if (h->ev_join) (void)hipEventDestroy(h->ev_join);
-    if (h->stream2) (void)hipStreamDestroy(h->stream2);
+    for (hipEvent_t ev : h->ev_join)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t st : h->side)
+        if (st) (void)hipStreamDestroy(st);
     h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();

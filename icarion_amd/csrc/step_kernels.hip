@@ -487,12 +487,13 @@ __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const
 // reflected about it (offline-table branch, EHSSCollisionHandler.cpp:253-277).
 // Every lane stays in the step loop (dead lanes are masked, not exited) so the
 // wave-wide matrix instructions always see the whole wavefront.
+// One ion per lane over P.n_steps steps; every lane of the wavefront must call
+// it (lanes with i >= P.n_ions are carried along masked); `lds` holds the
+// packed MLP weights.
 template <int INTEG, int FIELD>
-__global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
-    __shared__ uint4 mlp_lds[kPackBytes / 16];
-    mlp_stage(P.mlp_blob, mlp_lds);
-    const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, uint32_t i,
+                                         const unsigned char* lds, uint32_t& pushed,
+                                         uint32_t& ncoll_launch) {
     const bool valid = i < P.n_ions;
     const uint32_t ii = valid ? i : 0;
     const FastConst& C = P.fc;
@@ -516,7 +517,6 @@ __global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams
     double death = S.death[ii];
     double arrival = S.arrival[ii];
     uint32_t nc = S.ncoll[ii];
-    uint32_t pushed = 0, ncoll_launch = 0;
     bool checked = false;
     const float dt = (float)P.dt;
     const float half_dt = 0.5f * dt;
@@ -657,7 +657,16 @@ __global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams
         S.arrival[i] = arrival;
         S.ncoll[i] = nc;
     }
-    add_counters(P, pushed, ncoll_launch);
+}
+
+template <int INTEG, int FIELD>
+__global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
+    __shared__ uint4 mlp_lds[kPackBytes / 16];
+    mlp_stage(P.mlp_blob, mlp_lds);
+    const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
+    uint32_t pushed = 0, nc = 0;
+    surr_ion<INTEG, FIELD>(P, S, blockIdx.x * blockDim.x + threadIdx.x, lds, pushed, nc);
+    add_counters(P, pushed, nc);
 }
 
 // ======================= batched clouds (persistent) ========================
@@ -698,20 +707,17 @@ __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& 
     }
 }
 
-// The per-cloud tables come in as separate read-only, non-aliased arguments so
-// that the wave-uniform parameter loads stay scalar.
-template <int PREC>
-__global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict__ params,
-                                                    const IonPtrs* __restrict__ ions,
-                                                    const uint32_t* __restrict__ kinds,
-                                                    const BatchTile* __restrict__ tiles, uint32_t n_tiles,
-                                                    uint32_t* queue, unsigned long long* done,
-                                                    unsigned long long* start) {
-    // Queue and clock updates are made by wave 0 under a wave-uniform branch
-    // (all its lanes, one combined increment): a lane-divergent region on each
-    // side of the loop's back edge could be merged by the compiler into a
-    // divergent loop in which lanes 1..63 pass the barriers again before lane 0
-    // has taken the next tile.
+// Persistent tile loop shared by the batch kernels: take tiles from the queue
+// until it is empty, run `body(cloud, first ion)` on each, stamp the cloud's
+// completion clock after the tile.  Queue and clock updates are made by wave 0
+// under a wave-uniform branch (all its lanes, one combined increment): a
+// lane-divergent region on each side of the loop's back edge could be merged
+// by the compiler into a divergent loop in which lanes 1..63 pass the barriers
+// again before lane 0 has taken the next tile.
+template <typename Body>
+__device__ __forceinline__ void batch_queue_loop(const BatchTile* __restrict__ tiles, uint32_t n_tiles,
+                                                 uint32_t* queue, unsigned long long* done,
+                                                 unsigned long long* start, Body&& body) {
     __shared__ uint32_t s_tile;
     const bool wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x) == 0;
     const uint32_t lane = threadIdx.x & 63;
@@ -730,6 +736,22 @@ __global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict
         if (tile >= n_tiles) break;
         const uint32_t c = __builtin_amdgcn_readfirstlane(tiles[tile].cloud);
         const uint32_t first = __builtin_amdgcn_readfirstlane(tiles[tile].first);
+        body(c, first);
+        __syncthreads();   // the whole tile is done before its cloud's clock is stamped
+        prev = c;
+    }
+}
+
+// The per-cloud tables come in as separate read-only, non-aliased arguments so
+// that the wave-uniform parameter loads stay scalar.
+template <int PREC>
+__global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict__ params,
+                                                    const IonPtrs* __restrict__ ions,
+                                                    const uint32_t* __restrict__ kinds,
+                                                    const BatchTile* __restrict__ tiles, uint32_t n_tiles,
+                                                    uint32_t* queue, unsigned long long* done,
+                                                    unsigned long long* start) {
+    batch_queue_loop(tiles, n_tiles, queue, done, start, [&](uint32_t c, uint32_t first) {
         const StepParams& P = params[c];
         const IonPtrs S = ions[c];
         const uint32_t kind = __builtin_amdgcn_readfirstlane(kinds[c]);
@@ -737,15 +759,49 @@ __global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict
         uint32_t pushed = 0, nc = 0;
         if (i < P.n_ions) batch_dispatch<PREC>(kind, P, S, i, pushed, nc);
         add_counters(P, pushed, nc);
-        __syncthreads();   // the whole tile is done before its cloud's clock is stamped
-        prev = c;
-    }
+    });
+}
+
+// FP32 + EHSS surrogate clouds: the packed MLP weights are staged into LDS once
+// per persistent workgroup and serve every tile it takes.  kind = integrator
+// (0 RK4, 1 Boris) + 2 * (Orbitrap field).  A wavefront whose 64 ions all lie
+// beyond the cloud's end skips the tile (the MLP is evaluated per wavefront,
+// so the other wavefronts do not need it).
+__global__ void __launch_bounds__(256, 2) batch_surrogate_kernel(const StepParams* __restrict__ params,
+                                                                 const IonPtrs* __restrict__ ions,
+                                                                 const uint32_t* __restrict__ kinds,
+                                                                 const BatchTile* __restrict__ tiles,
+                                                                 uint32_t n_tiles, uint32_t* queue,
+                                                                 unsigned long long* done,
+                                                                 unsigned long long* start,
+                                                                 const uint4* __restrict__ blob) {
+    __shared__ uint4 mlp_lds[kPackBytes / 16];
+    mlp_stage(blob, mlp_lds);
+    const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
+    batch_queue_loop(tiles, n_tiles, queue, done, start, [&](uint32_t c, uint32_t first) {
+        const StepParams& P = params[c];
+        const IonPtrs S = ions[c];
+        const uint32_t kind = __builtin_amdgcn_readfirstlane(kinds[c]);
+        const uint32_t wave_first = first + __builtin_amdgcn_readfirstlane(threadIdx.x & ~63u);
+        uint32_t pushed = 0, nc = 0;
+        if (wave_first < P.n_ions) {
+            const uint32_t i = first + threadIdx.x;
+            switch (kind) {
+                case 0: surr_ion<0, FIELD_LINEAR>(P, S, i, lds, pushed, nc); break;
+                case 1: surr_ion<1, FIELD_LINEAR>(P, S, i, lds, pushed, nc); break;
+                case 2: surr_ion<0, FIELD_ORBITRAP>(P, S, i, lds, pushed, nc); break;
+                case 3: surr_ion<1, FIELD_ORBITRAP>(P, S, i, lds, pushed, nc); break;
+                default: break;
+            }
+        }
+        add_counters(P, pushed, nc);
+    });
 }
 
 __global__ void batch_reset_kernel(uint32_t* queue, unsigned long long* done, unsigned long long* start,
                                    unsigned long long* counters, uint32_t n_clouds) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 2) queue[i] = 0u;
+    if (i < kBatchClasses) queue[i] = 0u;
     if (i == 0) *start = ~0ull;
     if (i < n_clouds) {
         done[i] = 0ull;
@@ -755,30 +811,39 @@ __global__ void batch_reset_kernel(uint32_t* queue, unsigned long long* done, un
 }
 
 // Reset the queue/clock/counter records, then the persistent kernel of each
-// precision class present (B64 / B32 with n_tiles == 0 are skipped); the two
-// classes run on two streams so that they overlap.
-hipError_t launch_batch(const BatchParams& B64, const BatchParams& B32, uint32_t n_clouds,
-                        unsigned long long* counters, uint32_t block64, uint32_t grid64, uint32_t block32,
-                        uint32_t grid32, hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3((n_clouds + 255) / 256 + 1), dim3(256), 0, s, B64.queue,
-                       B64.done, B64.start, counters, n_clouds);
+// class present (FP64, FP32, FP32 + surrogate; a class with n_tiles == 0 is
+// skipped).  The first class present runs on `s`, the others on the side
+// streams, forked from and joined back into `s`, so that the classes overlap.
+hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
+                        const uint4* blob, uint32_t n_clouds, unsigned long long* counters, hipStream_t s,
+                        const hipStream_t* side, hipEvent_t fork, const hipEvent_t* join) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((n_clouds + 255) / 256 + 1), dim3(256), 0, s, B[0].queue,
+                       B[0].done, B[0].start, counters, n_clouds);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const bool both = B64.n_tiles && B32.n_tiles;
-    if (both) {
-        if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess) return e;
+    int present[kBatchClasses], n_present = 0;
+    for (int c = 0; c < kBatchClasses; ++c)
+        if (B[c].n_tiles) present[n_present++] = c;
+    if (n_present > 1 && (e = hipEventRecord(fork, s)) != hipSuccess) return e;
+    for (int k = 0; k < n_present; ++k) {
+        const int c = present[k];
+        hipStream_t st = k == 0 ? s : side[k - 1];
+        if (k > 0 && (e = hipStreamWaitEvent(st, fork, 0)) != hipSuccess) return e;
+        const BatchParams& b = B[c];
+        if (c == 0)
+            hipLaunchKernelGGL(batch_kernel<0>, dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions, b.kinds,
+                               b.tiles, b.n_tiles, b.queue, b.done, b.start);
+        else if (c == 1)
+            hipLaunchKernelGGL(batch_kernel<1>, dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions, b.kinds,
+                               b.tiles, b.n_tiles, b.queue, b.done, b.start);
+        else
+            hipLaunchKernelGGL(batch_surrogate_kernel, dim3(grid[c]), dim3(256), 0, st, b.params, b.ions,
+                               b.kinds, b.tiles, b.n_tiles, b.queue, b.done, b.start, blob);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (B64.n_tiles)
-        hipLaunchKernelGGL(batch_kernel<0>, dim3(grid64), dim3(block64), 0, s, B64.params, B64.ions, B64.kinds,
-                           B64.tiles, B64.n_tiles, B64.queue, B64.done, B64.start);
-    if (B32.n_tiles)
-        hipLaunchKernelGGL(batch_kernel<1>, dim3(grid32), dim3(block32), 0, both ? s2 : s, B32.params, B32.ions,
-                           B32.kinds, B32.tiles, B32.n_tiles, B32.queue, B32.done, B32.start);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (both) {
-        if ((e = hipEventRecord(join, s2)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
+    for (int k = 1; k < n_present; ++k) {
+        if ((e = hipEventRecord(join[k - 1], side[k - 1])) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s, join[k - 1], 0)) != hipSuccess) return e;
     }
     return hipSuccess;
 }

@@ -187,6 +187,9 @@ struct BatchTile {
 // kind of a cloud's kernel body: integrator + 2 * field (0 linear, 1 Orbitrap)
 // + 4 * collision (0 none, 1 HSS, 2 damping)
 constexpr uint32_t kBatchKinds = 12;
+// Precision classes of the batch, one persistent kernel each:
+// 0 FP64, 1 FP32 + Kahan, 2 FP32 + Kahan with the EHSS surrogate.
+constexpr int kBatchClasses = 3;
 
 struct BatchParams {
     const StepParams *params;       // per cloud (n_steps = the cloud's steps)

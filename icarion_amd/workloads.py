@@ -103,7 +103,8 @@ def physics(integrator=_abi.INTEGRATOR_RK4, collision=_abi.COLLISION_NONE, ou=0,
     return p
 
 
-def species(sigma_hss=0.0, sigma_ehss=0.0, ccs=0.0, K0=0.0, atom_offset=0, atom_count=0):
+def species(sigma_hss=0.0, sigma_ehss=0.0, ccs=0.0, K0=0.0, atom_offset=0, atom_count=0,
+            features=(0.0, 0.0, 0.0)):
     s = _abi.SpeciesDesc()
     s.sigma_hss_m2 = sigma_hss
     s.sigma_ehss_m2 = sigma_ehss
@@ -111,6 +112,8 @@ def species(sigma_hss=0.0, sigma_ehss=0.0, ccs=0.0, K0=0.0, atom_offset=0, atom_
     s.K0_cm2_Vs = K0
     s.atom_offset = atom_offset
     s.atom_count = atom_count
+    for f in range(3):
+        s.surrogate_features[f] = float(features[f])
     return s
 
 
@@ -274,6 +277,26 @@ def ehss_surrogate_workload(n=1_000_000, n_steps=1000, seed=21, pressure_Pa=1013
     ens.species[:] = np.arange(n, dtype=np.uint32)
     return dict(name="ehss_he_20td", domain=dom, physics=phys, counts=counts, ensemble=ens, dt=dt,
                 n_steps=n_steps)
+
+
+def ehss_surrogate_cloud(n, sigma, features, counts, seed=0, integrator=_abi.INTEGRATOR_RK4,
+                         dt=2e-12, n_steps=1000):
+    """One EHSS-surrogate cloud for the batch: the He 20 Td drift of
+    ehss_surrogate_workload with a species table of given clusters (sigma_ehss =
+    their OAPA areas, their surrogate features, mass 12 Da per atom); each ion
+    takes a random cluster.  sigma, features, counts: per cluster, e.g. from
+    gpu.Handle.synth_clusters on a handle of the same domain."""
+    dom = ims_domain(origin_z=-5e-3, length=20e-3, radius=10e-3, EN_Td=20.0, gas="He",
+                     pressure_Pa=101325.0, temperature_K=300.0)
+    phys = physics(integrator, _abi.COLLISION_EHSS, precision=_abi.PRECISION_FP32_KAHAN, seed=seed,
+                   ehss_mode=_abi.EHSS_SURROGATE)
+    sp = [species(sigma_ehss=float(sigma[k]), features=features[k]) for k in range(len(sigma))]
+    rng = np.random.default_rng(seed)
+    which = rng.integers(0, len(sigma), n).astype(np.uint32)
+    ens = gaussian_cloud(n, np.asarray(counts, np.float64)[which] * 12.0, 0.5e-3, seed=seed)
+    ens.species[:] = which
+    return dict(name="ehss_surrogate_cloud", domain=dom, physics=phys, species=sp, ensemble=ens,
+                dt=dt, n_steps=n_steps)
 
 
 def batched_workload(n_clouds=256, size_lo=1_000, size_hi=200_000, seed=5,

@@ -349,8 +349,10 @@ int icarion_gpu_synchronize(icarion_gpu *h);
  * icarion_gpu_step on a handle of its own (same kernels, same random
  * streams).  Covered: RK4 / Boris; IMS, TIMS, quadrupole and Orbitrap
  * analytic fields; no collisions, HSS, Langevin / Friction (+ OU kicks);
- * FP64 or FP32 + Kahan per cloud.  Not covered (ICARION_ERR_UNSUPPORTED):
- * field arrays, EHSS, RK45.  The batch lives in the handle beside (and
+ * FP64 or FP32 + Kahan per cloud; EHSS with the surrogate (FP32 + Kahan,
+ * the handle's surrogate parameters, species sigma_ehss and features; atoms
+ * are not needed).  Not covered (ICARION_ERR_UNSUPPORTED): field arrays, EHSS
+ * geometry collisions, RK45.  The batch lives in the handle beside (and
  * independent of) its single-cloud state. */
 typedef struct icarion_cloud_desc {
     icarion_domain_desc domain;
@@ -388,9 +390,12 @@ typedef struct icarion_cloud_stats {
 int icarion_gpu_batch_upload(icarion_gpu *h, const icarion_cloud_desc *clouds, uint32_t n_clouds);
 
 /* Run every cloud of the batch through all its steps in one graph launch.
- * fp64 / fp32: launch shapes of the two precision classes (NULL = the
- * tuned shapes, else 256 ions x 4 workgroups per CU).  per_cloud (n_clouds
- * entries) and device_ms may be NULL. */
+ * fp64 / fp32: launch shapes of the FP64 and FP32 + Kahan classes (NULL = the
+ * tuned shapes, else 256 ions x 4 workgroups per CU); EHSS-surrogate clouds
+ * run in a third class at 256 ions x 2 workgroups per CU (the occupancy its
+ * registers and LDS weights allow) and need icarion_gpu_surrogate_set_params
+ * (or _train) on this handle first (else ICARION_ERR_STATE).  per_cloud
+ * (n_clouds entries) and device_ms may be NULL. */
 int icarion_gpu_batch_run(icarion_gpu *h, const icarion_batch_tuning *fp64,
                           const icarion_batch_tuning *fp32, icarion_cloud_stats *per_cloud,
                           double *device_ms);

@@ -6,6 +6,8 @@ contracts on its own (differences seen: ~1e-13) - and bit for bit the same
 whatever the tile / launch shape; the autotuner only picks among launch
 shapes.  Single-cloud runs are pinned against the oracle
 (test_gpu_parity.py), which carries that parity over to the batch."""
+import os
+
 import numpy as np
 import pytest
 
@@ -56,22 +58,26 @@ def copy_clouds(clouds):
     return out
 
 
-def single_runs(clouds):
+def single_runs(clouds, params=None):
     from icarion_amd import gpu
     stats = []
     for c in clouds:
         with gpu.Handle() as h:
             h.set_domain(c["domain"], c["physics"])
             h.set_species(c["species"])
+            if c["physics"].collision_model == _abi.COLLISION_EHSS:
+                h.surrogate_set_params(params)
             h.upload_state(c["ensemble"])
             stats.append(h.step(c["t0"], c["dt"], c["n_steps"], c["step_offset"]))
             h.download_state(c["ensemble"])
     return stats
 
 
-def batch_runs(clouds, fp64=None, fp32=None, autotune=0):
+def batch_runs(clouds, fp64=None, fp32=None, autotune=0, params=None):
     from icarion_amd import gpu
     with gpu.Handle() as h:
+        if params is not None:
+            h.surrogate_set_params(params)
         h.batch_upload(clouds)
         tuned = h.batch_autotune(autotune) if autotune else None
         stats, ms = h.batch_run(fp64, fp32)
@@ -150,3 +156,71 @@ def test_batch_rejects_what_it_does_not_cover():
         with pytest.raises(gpu.IcarionError) as ei:
             h.batch_run()
         assert ei.value.code == _abi.ERR_STATE
+
+
+SURROGATE_WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                 "icarion_amd", "data", "surrogate_he_20td_clusters.npy")
+
+
+def surrogate_clouds(seed=4):
+    """Three EHSS-surrogate clouds (RK4 and Boris; 16 random clusters from the
+    device generator) next to one ordinary cloud of each precision, so that all
+    three classes of the batch run in one launch."""
+    from icarion_amd import gpu
+    rng = np.random.default_rng(seed)
+    counts = W.ehss_cluster_counts(16, seed)
+    w0 = W.ehss_surrogate_workload(n=16, seed=seed)
+    with gpu.Handle() as h:
+        h.set_domain(w0["domain"], w0["physics"])
+        _, sigma, feat = h.synth_clusters(counts, seed=seed)
+    clouds = []
+    for k, integ in enumerate((_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS, _abi.INTEGRATOR_RK4)):
+        n = int(rng.integers(100, 2000))
+        w = W.ehss_surrogate_cloud(n, sigma, feat, counts, seed=50 + k, integrator=integ)
+        clouds.append(dict(kind=f"ehss_surrogate_{k}", domain=w["domain"], physics=w["physics"],
+                           species=w["species"], ensemble=w["ensemble"].copy(), t0=1e-9 * k,
+                           dt=w["dt"], n_steps=int(rng.integers(200, 600)), step_offset=11 * k))
+    return clouds + mixed_clouds(seed)[:2]
+
+
+def test_batch_surrogate_clouds_equal_one_handle_per_cloud():
+    params = np.load(SURROGATE_WEIGHTS)
+    ref = surrogate_clouds()
+    bat = copy_clouds(ref)
+    s_ref = single_runs(ref, params)
+    s_bat, ms, _ = batch_runs(bat, params=params)
+    for ca, cb, a, b in zip(bat, ref, s_bat, s_ref):
+        frac = close_fraction(ca["ensemble"], cb["ensemble"])
+        assert frac >= 0.995, (ca["kind"], frac)
+        assert a.ion_steps == pytest.approx(b.ion_steps, rel=1e-3)
+        assert a.collisions == pytest.approx(b.collisions, rel=5e-3, abs=2)
+        assert a.t_end_s == b.t_end_s
+        assert 0.0 < a.done_ms <= ms * 1.05 + 0.05
+    # the surrogate clouds collide (about 0.6 events per ion-step at 1 atm He)
+    for c, a in zip(bat[:3], s_bat[:3]):
+        assert a.collisions > 0.3 * a.ion_steps, c["kind"]
+
+
+def test_batch_surrogate_results_do_not_depend_on_the_other_classes_shapes():
+    params = np.load(SURROGATE_WEIGHTS)
+    base = surrogate_clouds(seed=6)
+    other = copy_clouds(base)
+    batch_runs(base, fp64=(256, 4), fp32=(256, 4), params=params)
+    batch_runs(other, fp64=(64, 2), fp32=(128, 16), params=params)
+    assert_same(other, base)
+
+
+def test_batch_surrogate_needs_parameters():
+    from icarion_amd import gpu
+    clouds = surrogate_clouds(seed=7)[:1]
+    with gpu.Handle() as h:
+        h.batch_upload(clouds)
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_run()
+        assert ei.value.code == _abi.ERR_STATE
+        bad = dict(clouds[0])
+        bad["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS,
+                                   precision=_abi.PRECISION_FP64, ehss_mode=_abi.EHSS_SURROGATE)
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_upload([bad])
+        assert ei.value.code == _abi.ERR_UNSUPPORTED

@@ -159,3 +159,41 @@ def test_surrogate_gain_over_plain_ehss():
     assert err_unt > 2 * err_sur
     assert rate_sur > 3 * rate_plain
     assert s_sur.collisions == pytest.approx(s_plain.collisions, rel=0.01)
+
+
+def test_surrogate_angle_law_ks_on_held_out_clusters():
+    """Scattering-angle law of the surrogate against plain EHSS, all on the
+    device: the shipped weights, matrix-core inference, and the device EHSS
+    precompute on device-generated clusters outside the training set.  Per
+    cluster, the KS distance between surrogate mu and EHSS mu (20 000 each) is
+    that of two independent EHSS samples; the hard-sphere law is rejected."""
+    import os
+    from scipy import stats
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    params = np.load(os.path.join(root, "icarion_amd", "data", "surrogate_he_20td_clusters.npy"))
+    n_cl, N = 32, 20000
+    counts = W.ehss_cluster_counts(n_cl, seed=4321)
+    rng = np.random.default_rng(5)
+    with he_handle() as h:
+        _, _, feat = h.synth_clusters(counts, seed=4321)
+        mu1 = h.ehss_precompute(0, n_cl, N, 0, 1, seed=11)[1][..., 0]
+        mu2 = h.ehss_precompute(0, n_cl, N, 0, 1, seed=12)[1][..., 0]
+        h.surrogate_set_params(params)
+        X = np.zeros((n_cl, N, 6), np.float32)
+        psi = rng.uniform(0.0, 1.0, (n_cl, N))
+        X[..., 0] = rng.uniform(0.0, 1.0, (n_cl, N))
+        X[..., 1], X[..., 2] = np.cos(2 * np.pi * psi), np.sin(2 * np.pi * psi)
+        X[..., 3:] = feat[:, None, :]
+        mu_s = S.mu_from_output(h.surrogate_infer(X.reshape(-1, 6))).reshape(n_cl, N)
+    assert np.isfinite(mu1).all() and np.isfinite(mu2).all()
+    d_sur = np.array([stats.ks_2samp(mu_s[c], mu1[c]).statistic for c in range(n_cl)])
+    p_sur = np.array([stats.ks_2samp(mu_s[c], mu1[c]).pvalue for c in range(n_cl)])
+    d_null = np.array([stats.ks_2samp(mu2[c], mu1[c]).statistic for c in range(n_cl)])
+    d_hs = np.array([stats.ks_2samp(np.sqrt(rng.uniform(0.0, 1.0, N)), mu1[c]).statistic
+                     for c in range(n_cl)])
+    print(f"KS D surrogate {d_sur.mean():.4f} (max {d_sur.max():.4f}), two EHSS samples "
+          f"{d_null.mean():.4f}, hard sphere {d_hs.mean():.4f}; p < 0.01 in "
+          f"{(p_sur < 0.01).sum()} of {n_cl}")
+    assert d_sur.mean() < 1.25 * d_null.mean()
+    assert (p_sur < 0.01).mean() <= 3 / n_cl
+    assert d_hs.mean() > 3 * d_sur.mean()

@@ -97,3 +97,59 @@ def test_mlp_forward_closed_forms(oracle):
     Y = oracle.mlp_forward(p, X)
     np.testing.assert_allclose(Y[:, 2], -3.0 * np.maximum(X[:, 0], 0), rtol=1e-6)
     np.testing.assert_allclose(Y[:, :2], 0.0)
+
+
+def random_cluster(n, rng, r_lo=1.5e-10, r_hi=2.0e-10):
+    """A rigid cluster by sequential attachment (each new atom touches a random
+    earlier one at 0.95 (r_i + r_j), no closer than 0.8 (r_i + r_k) to any),
+    centred; (n, 4) float32 x, y, z, r."""
+    r = rng.uniform(r_lo, r_hi, n)
+    pos = np.zeros((n, 3))
+    k = 1
+    while k < n:
+        j = rng.integers(0, k)
+        d = rng.standard_normal(3)
+        p = pos[j] + 0.95 * (r[j] + r[k]) * d / np.linalg.norm(d)
+        if np.all(np.linalg.norm(pos[:k] - p, axis=1) >= 0.8 * (r[:k] + r[k])):
+            pos[k] = p
+            k += 1
+    pos -= pos.mean(0)
+    return np.concatenate([pos, r[:, None]], 1).astype(np.float32)
+
+
+def test_committed_surrogate_reproduces_the_ehss_angle_law(oracle):
+    """The shipped weights (icarion_amd/data, trained on the device) on clusters
+    they never saw: per cluster, the KS distance between the surrogate's mu =
+    cos(contact normal, axis) and the oracle's EHSS first-contact mu (20 000
+    each) is that of two independent EHSS samples, while the hard-sphere law
+    (mu^2 uniform, what EHSS reduces to for one atom) is rejected."""
+    import os
+    from icarion_amd import surrogate as S
+    from icarion_amd import workloads as W
+    from oracle import binding as O
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    params = np.load(os.path.join(root, "icarion_amd", "data", "surrogate_he_20td_clusters.npy"))
+    rng = np.random.default_rng(1234)
+    counts = W.ehss_cluster_counts(16, 1234)
+    atoms = np.concatenate([random_cluster(int(c), rng) for c in counts])
+    r_gas = W.ims_domain(EN_Td=20.0, gas="He").gas_radius_m
+    feat = O.ehss_features(atoms, counts, r_gas, O.oapa_sigma(atoms, counts, r_gas))
+    N = 20000
+    mu1 = O.ehss_precompute(atoms, counts, r_gas, N, 0, 1, seed=1)[1].reshape(len(counts), N)
+    mu2 = O.ehss_precompute(atoms, counts, r_gas, N, 0, 1, seed=2)[1].reshape(len(counts), N)
+    d_sur, d_null, d_hs, p_sur = [], [], [], []
+    for c in range(len(counts)):
+        X = np.zeros((N, 6), np.float32)
+        psi = rng.uniform(0.0, 1.0, N)
+        X[:, 0] = rng.uniform(0.0, 1.0, N)
+        X[:, 1], X[:, 2] = np.cos(2 * np.pi * psi), np.sin(2 * np.pi * psi)
+        X[:, 3:] = feat[c]
+        a = stats.ks_2samp(S.mu_from_output(O.mlp_forward(params, X)), mu1[c])
+        d_sur.append(a.statistic)
+        p_sur.append(a.pvalue)
+        d_null.append(stats.ks_2samp(mu2[c], mu1[c]).statistic)
+        d_hs.append(stats.ks_2samp(np.sqrt(rng.uniform(0.0, 1.0, N)), mu1[c]).statistic)
+    d_sur, d_null, d_hs, p_sur = map(np.asarray, (d_sur, d_null, d_hs, p_sur))
+    assert d_sur.mean() < 1.25 * d_null.mean()
+    assert (p_sur < 0.01).mean() <= 2 / len(counts)
+    assert d_hs.mean() > 3 * d_sur.mean()
