@@ -5,7 +5,7 @@ sizes against offsets computed by the C compiler.
 """
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NO_DEVICE = range(6)
 
@@ -14,6 +14,9 @@ INTEGRATOR_RK4, INTEGRATOR_BORIS, INTEGRATOR_RK45 = range(3)
 COLLISION_NONE, COLLISION_HSS, COLLISION_EHSS, COLLISION_LANGEVIN, COLLISION_FRICTION = range(5)
 PRECISION_FP64, PRECISION_FP32_KAHAN = range(2)
 EHSS_GEOMETRY, EHSS_SURROGATE = range(2)
+(FIELD_SCALE_CONSTANT, FIELD_SCALE_DC_AXIAL, FIELD_SCALE_DC_QUAD, FIELD_SCALE_DC_RADIAL,
+ FIELD_SCALE_RF) = range(5)
+MAX_FIELD_TERMS = 4
 
 _d = C.c_double
 _i32 = C.c_int32
@@ -122,11 +125,26 @@ class GpuDesc(C.Structure):
                 ("reserved0", _u32)]
 
 
+class OracleFieldTerm(C.Structure):
+    _fields_ = [
+        ("scale_kind", _i32), ("reserved0", _i32), ("constant_scale", _d), ("frequency_Hz", _d),
+        ("phase_rad", _d), ("Ex", C.POINTER(_d)), ("Ey", C.POINTER(_d)), ("Ez", C.POINTER(_d)),
+    ]
+
+
 class OracleField(C.Structure):
     _fields_ = [
         ("nx", _u32), ("ny", _u32), ("nz", _u32),
         ("xs", C.POINTER(_d)), ("ys", C.POINTER(_d)), ("zs", C.POINTER(_d)),
         ("Ex", C.POINTER(_d)), ("Ey", C.POINTER(_d)), ("Ez", C.POINTER(_d)),
+        ("n_terms", _u32), ("reserved0", _u32), ("terms", C.POINTER(OracleFieldTerm)),
+    ]
+
+
+class FieldTermDesc(C.Structure):
+    _fields_ = [
+        ("E", C.c_void_p), ("dtype", _i32), ("scale_kind", _i32), ("constant_scale", _d),
+        ("frequency_Hz", _d), ("phase_rad", _d),
     ]
 
 

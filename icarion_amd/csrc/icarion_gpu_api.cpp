@@ -151,6 +151,19 @@ struct icarion_gpu {
     bool have_field = false;
     DevBuf<float4> fa32;
     DevBuf<double4> fa64;
+    // the field-array source, kept apart from the domain (set_domain may follow
+    // the upload); applied to the kernel parameters at every launch
+    struct FieldSrc {
+        int nx = 0, ny = 0, nz = 0, n_terms = 0;
+        bool nonuniform = false;
+        double x0 = 0, y0 = 0, z0 = 0, dx = 1, dy = 1, dz = 1;
+        double x_last = 0, y_last = 0, z_last = 0;
+        int kind[kMaxFieldTerms] = {0, 0, 0, 0};
+        double cst[kMaxFieldTerms] = {0, 0, 0, 0}, freq[kMaxFieldTerms] = {0, 0, 0, 0},
+               phase[kMaxFieldTerms] = {0, 0, 0, 0};
+    } fsrc;
+    DevBuf<double> ax64[3];
+    DevBuf<float> ax32[3];
 
     size_t n = 0;
     bool have_state = false;
@@ -216,6 +229,14 @@ int field_kind(const icarion_domain_desc& d) {
     if (d.use_field_array) return FIELD_ARRAY;
     if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) return FIELD_ORBITRAP;
     return FIELD_LINEAR;
+}
+
+// The kernel variant of a handle's field: the general field-array path for
+// non-uniform axes or several terms.
+int launch_field_kind(const icarion_gpu* h) {
+    const int f = field_kind(h->dom);
+    if (f == FIELD_ARRAY && (h->fsrc.nonuniform || h->fsrc.n_terms > 1)) return FIELD_ARRAY_X;
+    return f;
 }
 
 int coll_kind(const icarion_physics_desc& p) {
@@ -309,8 +330,10 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
 // Per-step table of a run: t_k by repeated addition from t0 and, with RF,
 // c = U_eff(t)/r0^2 at the stage times the integrator evaluates (RK4: t,
 // t + dt/2, t + dt; Boris: t).  Appends n_steps entries to out.
+// stage_times: a field array of several time-scaled terms, whose kernels take
+// the stage times t, t + dt/2, t + dt in place of the RF coefficients.
 void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, bool rf, int integrator,
-                  double t0, double dt, uint64_t n_steps) {
+                  double t0, double dt, uint64_t n_steps, bool stage_times = false) {
     const size_t base = out.size();
     out.resize(base + n_steps);
     const double r0_sq = d.radius_m * d.radius_m;
@@ -324,7 +347,11 @@ void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, boo
     for (uint64_t k = 0; k < n_steps; ++k) {
         StepEntry& e = out[base + k];
         e.t = t;
-        if (rf) {
+        if (stage_times) {
+            e.c0 = t;
+            e.ch = t + half_dt;
+            e.c1 = t + dt;
+        } else if (rf) {
             e.c0 = coeff(t);
             if (integrator == ICARION_INTEGRATOR_RK4) {
                 e.ch = coeff(t + half_dt);
@@ -339,9 +366,50 @@ void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, boo
     }
 }
 
+// The field-array grid and terms into the launch parameters; each term's
+// scale kind resolved against the current domain
+// (CompositeFieldProvider::compute_scale_factor, CompositeFieldProvider.h:117-147).
+void apply_field(const icarion_gpu* h, StepParams& P) {
+    const auto& f = h->fsrc;
+    P.nx = f.nx; P.ny = f.ny; P.nz = f.nz;
+    P.n_terms = f.n_terms;
+    P.x0 = f.x0; P.y0 = f.y0; P.z0 = f.z0;
+    P.dx = f.dx; P.dy = f.dy; P.dz = f.dz;
+    P.inv_dx = 1.0 / f.dx; P.inv_dy = 1.0 / f.dy; P.inv_dz = 1.0 / f.dz;
+    P.x_last = f.x_last; P.y_last = f.y_last; P.z_last = f.z_last;
+    P.ax_x = f.nonuniform ? h->ax64[0].p : nullptr;
+    P.ax_y = f.nonuniform ? h->ax64[1].p : nullptr;
+    P.ax_z = f.nonuniform ? h->ax64[2].p : nullptr;
+    P.axf_x = f.nonuniform ? h->ax32[0].p : nullptr;
+    P.axf_y = f.nonuniform ? h->ax32[1].p : nullptr;
+    P.axf_z = f.nonuniform ? h->ax32[2].p : nullptr;
+    const icarion_domain_desc& d = h->dom;
+    for (int j = 0; j < kMaxFieldTerms; ++j) {
+        P.term_rf[j] = 0;
+        P.term_a[j] = 0.0;
+        P.term_w[j] = 0.0;
+        P.term_phi[j] = 0.0;
+        if (j >= f.n_terms) continue;
+        switch (f.kind[j]) {
+            case ICARION_FIELD_SCALE_CONSTANT: P.term_a[j] = f.cst[j]; break;
+            case ICARION_FIELD_SCALE_DC_AXIAL: P.term_a[j] = d.dc_axial_V; break;
+            case ICARION_FIELD_SCALE_DC_QUAD: P.term_a[j] = d.dc_quad_V; break;
+            case ICARION_FIELD_SCALE_DC_RADIAL: P.term_a[j] = d.dc_radial_V; break;
+            default: {
+                const double fr = f.freq[j] > 0.0 ? f.freq[j] : d.rf_frequency_Hz;
+                P.term_rf[j] = 1;
+                P.term_a[j] = d.rf_voltage_V;
+                P.term_w[j] = 2.0 * kPiHost * fr;
+                P.term_phi[j] = f.phase[j];
+            }
+        }
+    }
+}
+
 void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
     h->table_host.clear();
-    append_table(h->table_host, h->dom, h->base.has_rf != 0, h->phys.integrator, t0, dt, n_steps);
+    const bool stage_times = field_kind(h->dom) == FIELD_ARRAY && h->fsrc.n_terms >= 2;
+    append_table(h->table_host, h->dom, h->base.has_rf != 0, h->phys.integrator, t0, dt, n_steps, stage_times);
     h->table.resize(std::max<size_t>(n_steps, 1));
     hip_check(hipMemcpyAsync(h->table.p, h->table_host.data(), n_steps * sizeof(StepEntry),
                              hipMemcpyHostToDevice, h->stream),
@@ -371,7 +439,7 @@ IonPtrs ion_ptrs(icarion_gpu* h) {
 // engine step, replayed from a captured HIP graph of up to 32 steps.
 void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
                   icarion_step_stats* stats) {
-    const int field = field_kind(h->dom);
+    const int field = launch_field_kind(h);
     const int coll = coll_kind(h->phys);
     if (h->phys.precision != ICARION_PRECISION_FP64)
         fail(ICARION_ERR_UNSUPPORTED, "RK45 runs in FP64");
@@ -385,6 +453,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     P.atoms = h->atoms_f32.p;
     P.fa_f32 = h->fa32.p;
     P.fa_f64 = h->fa64.p;
+    apply_field(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
     P.rk45 = h->rk45;
@@ -510,6 +579,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     P.mlp_blob = h->mlp_blob.p;
     P.fa_f32 = h->fa32.p;
     P.fa_f64 = (h->phys.precision == ICARION_PRECISION_FP64) ? h->fa64.p : nullptr;
+    apply_field(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
     fill_fast_const(P);
@@ -528,7 +598,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
             P.table = h->table.p + k0;
             P.step0 = step_offset + k0;
             P.n_steps = static_cast<uint32_t>(nk);
-            const hipError_t e = launch_step(h->phys.integrator, field, coll, h->phys.precision,
+            const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), coll, h->phys.precision,
                                              surrogate ? 1 : 0, P, S, h->block, h->stream);
             if (e == hipErrorInvalidValue)
                 fail(ICARION_ERR_UNSUPPORTED, "no kernel for this integrator/field/collision/precision");
@@ -1230,6 +1300,8 @@ int icarion_gpu_destroy(icarion_gpu* h) {
         if (st) (void)hipStreamSynchronize(st);
     h->species.release(); h->atoms_f32.release(); h->atom_offsets.release(); h->atom_counts.release();
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
+    for (auto& a : h->ax64) a.release();
+    for (auto& a : h->ax32) a.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
     h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
     h->b_params_d.release(); h->b_ions_d.release(); h->b_kinds_d.release(); h->b_queue.release();
@@ -1332,6 +1404,48 @@ int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species,
     });
 }
 
+namespace {
+
+// Node-major device copies of the terms (element node * n_terms + term);
+// double copies too when any term arrives in double.
+void upload_field_data(icarion_gpu* h, size_t nn, const icarion_field_term_desc* terms, uint32_t nt) {
+    bool any_double = false;
+    for (uint32_t j = 0; j < nt; ++j) any_double |= terms[j].dtype == 1;
+    std::vector<float4> f(nn * nt);
+    std::vector<double4> g(any_double ? nn * nt : 0);
+    for (uint32_t j = 0; j < nt; ++j) {
+        if (terms[j].dtype == 0) {
+            const float* e = static_cast<const float*>(terms[j].E);
+            for (size_t i = 0; i < nn; ++i) {
+                f[i * nt + j] = make_float4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0f);
+                if (any_double) g[i * nt + j] = make_double4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0);
+            }
+        } else {
+            const double* e = static_cast<const double*>(terms[j].E);
+            for (size_t i = 0; i < nn; ++i) {
+                f[i * nt + j] = make_float4((float)e[3 * i], (float)e[3 * i + 1], (float)e[3 * i + 2], 0.0f);
+                g[i * nt + j] = make_double4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0);
+            }
+        }
+    }
+    if (any_double) copy_in(h->fa64, g.data(), g.size(), h->stream, "upload field (double)");
+    else h->fa64.release();
+    copy_in(h->fa32, f.data(), f.size(), h->stream, "upload field (float)");
+    hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+// An axis whose nodes are x0 + i * d to within 1e-12 of its span.
+bool uniform_axis(const double* a, uint32_t n, double& d) {
+    d = (a[n - 1] - a[0]) / static_cast<double>(n - 1);
+    if (!(d > 0.0)) return false;
+    const double tol = 1e-12 * (a[n - 1] - a[0]);
+    for (uint32_t i = 0; i < n; ++i)
+        if (std::fabs(a[i] - (a[0] + static_cast<double>(i) * d)) > tol) return false;
+    return true;
+}
+
+}  // namespace
+
 int icarion_gpu_upload_field_array(icarion_gpu* h, uint32_t nx, uint32_t ny, uint32_t nz, double x0,
                                    double dx, double y0, double dy, double z0, double dz,
                                    const void* E, int32_t dtype) {
@@ -1340,33 +1454,80 @@ int icarion_gpu_upload_field_array(icarion_gpu* h, uint32_t nx, uint32_t ny, uin
         if (nx < 2 || ny < 2 || nz < 2) fail(ICARION_ERR_INVALID, "field array needs >= 2 nodes per axis");
         if (!(dx > 0.0 && dy > 0.0 && dz > 0.0)) fail(ICARION_ERR_INVALID, "grid spacing must be > 0");
         if (dtype != 0 && dtype != 1) fail(ICARION_ERR_INVALID, "dtype must be 0 (float) or 1 (double)");
-        const size_t nn = static_cast<size_t>(nx) * ny * nz;
         set_device(h);
-        std::vector<float4> f(nn);
-        std::vector<double4> g;
-        if (dtype == 0) {
-            const float* e = static_cast<const float*>(E);
-            for (size_t i = 0; i < nn; ++i) f[i] = make_float4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0f);
-            h->fa64.release();
-        } else {
-            const double* e = static_cast<const double*>(E);
-            g.resize(nn);
-            for (size_t i = 0; i < nn; ++i) {
-                g[i] = make_double4(e[3 * i], e[3 * i + 1], e[3 * i + 2], 0.0);
-                f[i] = make_float4((float)e[3 * i], (float)e[3 * i + 1], (float)e[3 * i + 2], 0.0f);
-            }
-            copy_in(h->fa64, g.data(), nn, h->stream, "upload field (double)");
+        icarion_field_term_desc t{};
+        t.E = E;
+        t.dtype = dtype;
+        upload_field_data(h, static_cast<size_t>(nx) * ny * nz, &t, 1);
+        for (auto& a : h->ax64) a.release();
+        for (auto& a : h->ax32) a.release();
+        icarion_gpu::FieldSrc f;
+        f.nx = static_cast<int>(nx); f.ny = static_cast<int>(ny); f.nz = static_cast<int>(nz);
+        f.n_terms = 1;
+        f.x0 = x0; f.y0 = y0; f.z0 = z0;
+        f.dx = dx; f.dy = dy; f.dz = dz;
+        f.x_last = x0 + (nx - 1) * dx;
+        f.y_last = y0 + (ny - 1) * dy;
+        f.z_last = z0 + (nz - 1) * dz;
+        h->fsrc = f;
+        h->have_field = true;
+    });
+}
+
+int icarion_gpu_upload_field_terms(icarion_gpu* h, uint32_t nx, uint32_t ny, uint32_t nz,
+                                   const double* xs, const double* ys, const double* zs,
+                                   const icarion_field_term_desc* terms, uint32_t n_terms) {
+    return guarded([&] {
+        if (!h || !xs || !ys || !zs || !terms) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (nx < 2 || ny < 2 || nz < 2) fail(ICARION_ERR_INVALID, "field array needs >= 2 nodes per axis");
+        if (n_terms == 0) fail(ICARION_ERR_INVALID, "no field terms");
+        if (n_terms > static_cast<uint32_t>(kMaxFieldTerms))
+            fail(ICARION_ERR_UNSUPPORTED, "at most " + std::to_string(kMaxFieldTerms) + " field terms");
+        const double* axes[3] = {xs, ys, zs};
+        const uint32_t dims[3] = {nx, ny, nz};
+        for (int k = 0; k < 3; ++k) {
+            for (uint32_t i = 0; i < dims[k]; ++i)
+                if (!std::isfinite(axes[k][i])) fail(ICARION_ERR_INVALID, "axis values must be finite");
+            for (uint32_t i = 0; i + 1 < dims[k]; ++i)
+                if (axes[k][i + 1] < axes[k][i]) fail(ICARION_ERR_INVALID, "axes must be non-decreasing");
+            if (!(axes[k][dims[k] - 1] > axes[k][0])) fail(ICARION_ERR_INVALID, "an axis has zero extent");
         }
-        copy_in(h->fa32, f.data(), nn, h->stream, "upload field (float)");
-        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
-        StepParams& P = h->base;
-        P.nx = static_cast<int>(nx); P.ny = static_cast<int>(ny); P.nz = static_cast<int>(nz);
-        P.x0 = x0; P.y0 = y0; P.z0 = z0;
-        P.dx = dx; P.dy = dy; P.dz = dz;
-        P.inv_dx = 1.0 / dx; P.inv_dy = 1.0 / dy; P.inv_dz = 1.0 / dz;
-        P.x_last = x0 + (nx - 1) * dx;
-        P.y_last = y0 + (ny - 1) * dy;
-        P.z_last = z0 + (nz - 1) * dz;
+        for (uint32_t j = 0; j < n_terms; ++j) {
+            if (!terms[j].E) fail(ICARION_ERR_INVALID, "field term without data");
+            if (terms[j].dtype != 0 && terms[j].dtype != 1)
+                fail(ICARION_ERR_INVALID, "dtype must be 0 (float) or 1 (double)");
+            if (terms[j].scale_kind < ICARION_FIELD_SCALE_CONSTANT || terms[j].scale_kind > ICARION_FIELD_SCALE_RF)
+                fail(ICARION_ERR_INVALID, "unknown field scale kind");
+        }
+        set_device(h);
+        upload_field_data(h, static_cast<size_t>(nx) * ny * nz, terms, n_terms);
+        icarion_gpu::FieldSrc f;
+        f.nx = static_cast<int>(nx); f.ny = static_cast<int>(ny); f.nz = static_cast<int>(nz);
+        f.n_terms = static_cast<int>(n_terms);
+        double d[3];
+        bool uniform = true;
+        for (int k = 0; k < 3; ++k) uniform &= uniform_axis(axes[k], dims[k], d[k]);
+        f.nonuniform = !uniform;
+        f.x0 = xs[0]; f.y0 = ys[0]; f.z0 = zs[0];
+        f.dx = uniform ? d[0] : 1.0; f.dy = uniform ? d[1] : 1.0; f.dz = uniform ? d[2] : 1.0;
+        f.x_last = xs[nx - 1]; f.y_last = ys[ny - 1]; f.z_last = zs[nz - 1];
+        for (auto& a : h->ax64) a.release();
+        for (auto& a : h->ax32) a.release();
+        if (!uniform) {
+            for (int k = 0; k < 3; ++k) {
+                std::vector<float> af(axes[k], axes[k] + dims[k]);
+                copy_in(h->ax64[k], axes[k], dims[k], h->stream, "upload axis");
+                copy_in(h->ax32[k], af.data(), dims[k], h->stream, "upload axis (float)");
+                hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+            }
+        }
+        for (uint32_t j = 0; j < n_terms; ++j) {
+            f.kind[j] = terms[j].scale_kind;
+            f.cst[j] = terms[j].constant_scale;
+            f.freq[j] = terms[j].frequency_Hz;
+            f.phase[j] = terms[j].phase_rad;
+        }
+        h->fsrc = f;
         h->have_field = true;
     });
 }

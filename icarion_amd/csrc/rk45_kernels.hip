@@ -128,8 +128,10 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                         }
                     }
                 }
+                // analytic fields take the RF coefficient, field arrays the stage time
                 auto acc = [&](d3 p, d3 v, double tt) {
-                    return force_f64<FIELD, COLL>(P, p, v, rf_coeff(P, tt), q, m, lpre) * inv_m;
+                    const double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
+                    return force_f64<FIELD, COLL>(P, p, v, c, q, m, lpre) * inv_m;
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);
@@ -289,6 +291,7 @@ hipError_t launch_rk45_step(int field, int coll, const StepParams& P, const IonP
                             uint32_t block, hipStream_t stream) {
     Rk45Fn fn = field == FIELD_ORBITRAP ? pick_rk45_coll<FIELD_ORBITRAP>(coll)
                 : field == FIELD_ARRAY  ? pick_rk45_coll<FIELD_ARRAY>(coll)
+                : field == FIELD_ARRAY_X ? pick_rk45_coll<FIELD_ARRAY_X>(coll)
                                         : pick_rk45_coll<FIELD_LINEAR>(coll);
     if (!fn) return hipErrorInvalidValue;
     const uint32_t grid = (P.n_ions + block - 1) / block;

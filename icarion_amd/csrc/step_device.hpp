@@ -55,31 +55,88 @@ __device__ __forceinline__ void locate_f64(double v, double v0, double v_last, d
     }
 }
 
-__device__ __forceinline__ d3 field_array_f64(const StepParams& P, d3 p) {
+// interpolate_field's axis search (fieldArrayLoader.cpp:115-137) on a
+// non-uniform axis: clamp below / above, else the first interval i with
+// ax[i] <= v <= ax[i+1] - for a non-decreasing axis that is the first i with
+// ax[i+1] >= v, found by bisection; NaN takes the search's fallback.
+__device__ __forceinline__ void locate_axis_f64(const double* __restrict__ ax, int n, double v, int& i0,
+                                                double& a) {
+    if (v <= ax[0]) {
+        i0 = 0;
+        a = 0.0;
+    } else if (v >= ax[n - 1] || v != v) {
+        i0 = n - 2;
+        a = 1.0;
+    } else {
+        int first = 0, len = n - 1;
+        while (len > 0) {
+            const int half = len >> 1;
+            if (ax[first + 1 + half] < v) {
+                first += half + 1;
+                len -= half + 1;
+            } else {
+                len = half;
+            }
+        }
+        i0 = first;
+        const double dv = ax[first + 1] - ax[first];
+        a = (dv > 0.0) ? (v - ax[first]) / dv : 0.0;
+    }
+}
+
+// Time factor of field-array term j (CompositeFieldProvider::compute_scale_factor,
+// CompositeFieldProvider.h:117-147; the host resolves each kind to a, w, phi).
+__device__ __forceinline__ double term_scale_f64(const StepParams& P, int j, double t) {
+    return P.term_rf[j] ? P.term_a[j] * cos(P.term_w[j] * t + P.term_phi[j]) : P.term_a[j];
+}
+
+// GridFieldProvider (one term) or CompositeFieldProvider::get_E
+// (CompositeFieldProvider.h:80-97: per term, interpolate, then add scale * E_i
+// in term order); t is the stage time.  GEN = false: uniform grid, one term.
+template <bool GEN>
+__device__ __forceinline__ d3 field_array_f64(const StepParams& P, d3 p, double t) {
     int ix, iy, iz;
     double ax, ay, az;
-    locate_f64(p.x, P.x0, P.x_last, P.inv_dx, P.dx, P.nx, ix, ax);
-    locate_f64(p.y, P.y0, P.y_last, P.inv_dy, P.dy, P.ny, iy, ay);
-    locate_f64(p.z, P.z0, P.z_last, P.inv_dz, P.dz, P.nz, iz, az);
-    d3 r{0.0, 0.0, 0.0};
+    if (GEN && P.ax_x) {
+        locate_axis_f64(P.ax_x, P.nx, p.x, ix, ax);
+        locate_axis_f64(P.ax_y, P.ny, p.y, iy, ay);
+        locate_axis_f64(P.ax_z, P.nz, p.z, iz, az);
+    } else {
+        locate_f64(p.x, P.x0, P.x_last, P.inv_dx, P.dx, P.nx, ix, ax);
+        locate_f64(p.y, P.y0, P.y_last, P.inv_dy, P.dy, P.ny, iy, ay);
+        locate_f64(p.z, P.z0, P.z_last, P.inv_dz, P.dz, P.nz, iz, az);
+    }
     const size_t sx = 1, sy = static_cast<size_t>(P.nx), sz = static_cast<size_t>(P.nx) * P.ny;
     const size_t base = ix + sy * iy + sz * iz;
+    const int nt = (GEN && P.n_terms > 1) ? P.n_terms : 1;
+    d3 r{0.0, 0.0, 0.0};
+    for (int j = 0; j < nt; ++j) {
+        d3 e{0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int dx = c >> 2, dy = (c >> 1) & 1, dzz = c & 1;
-        const double w = (dx ? ax : 1.0 - ax) * (dy ? ay : 1.0 - ay) * (dzz ? az : 1.0 - az);
-        const size_t idx = base + dx * sx + dy * sy + dzz * sz;
-        double ex, ey, ez;
-        if (P.fa_f64) {
-            const double4 e = P.fa_f64[idx];
-            ex = e.x; ey = e.y; ez = e.z;
-        } else {
-            const float4 e = P.fa_f32[idx];
-            ex = e.x; ey = e.y; ez = e.z;
+        for (int c = 0; c < 8; ++c) {
+            const int dx = c >> 2, dy = (c >> 1) & 1, dzz = c & 1;
+            const double w = (dx ? ax : 1.0 - ax) * (dy ? ay : 1.0 - ay) * (dzz ? az : 1.0 - az);
+            const size_t idx = (base + dx * sx + dy * sy + dzz * sz) * nt + j;
+            double ex, ey, ez;
+            if (P.fa_f64) {
+                const double4 v = P.fa_f64[idx];
+                ex = v.x; ey = v.y; ez = v.z;
+            } else {
+                const float4 v = P.fa_f32[idx];
+                ex = v.x; ey = v.y; ez = v.z;
+            }
+            e.x += w * ex;
+            e.y += w * ey;
+            e.z += w * ez;
         }
-        r.x += w * ex;
-        r.y += w * ey;
-        r.z += w * ez;
+        if (nt == 1) {
+            r = e;
+        } else {
+            const double s = term_scale_f64(P, j, t);
+            r.x += s * e.x;
+            r.y += s * e.y;
+            r.z += s * e.z;
+        }
     }
     if (P.fa_scale != 1.0) r = r * P.fa_scale;
     return r;
@@ -106,7 +163,8 @@ __device__ __forceinline__ d3 field_f64(const StepParams& P, d3 p, double c_rf) 
         if (!P.rot_identity) E = mat_mul(P.l2g, E);
         return E;
     } else {
-        return field_array_f64(P, p);
+        // c_rf carries the stage time here
+        return field_array_f64<FIELD == FIELD_ARRAY_X>(P, p, c_rf);
     }
 }
 

@@ -163,9 +163,12 @@ __device__ __forceinline__ float fast_sin_rev(float r) { return __builtin_amdgcn
 __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 
+// c: the stage's RF coefficient (analytic fields) or the stage time (field
+// arrays), in FP64 as the step table holds it.
 template <int FIELD>
-__device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C, f3 p, float c_rf) {
+__device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C, f3 p, double c) {
     if constexpr (FIELD == FIELD_LINEAR) {
+        const float c_rf = (float)c;
         f3 E{0.0f, 0.0f, C.Ez};
         if (P.has_rf) {
             E.x = -2.0f * p.x * c_rf;
@@ -190,7 +193,8 @@ __device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C,
         }
         return E;
     } else {
-        // trilinear on float4 nodes
+        // trilinear on float4 nodes; the axis search of the FP64 path on
+        // non-uniform axes
         auto loc = [](float v, float v0, float vl, float inv_d, int n, int& i0, float& a) {
             if (v <= v0) { i0 = 0; a = 0.0f; }
             else if (v >= vl) { i0 = n - 2; a = 1.0f; }
@@ -202,22 +206,63 @@ __device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C,
                 a = f - static_cast<float>(i);
             }
         };
+        auto loc_axis = [](const float* __restrict__ ax, int n, float v, int& i0, float& a) {
+            if (v <= ax[0]) { i0 = 0; a = 0.0f; }
+            else if (v >= ax[n - 1] || v != v) { i0 = n - 2; a = 1.0f; }
+            else {
+                int first = 0, len = n - 1;
+                while (len > 0) {
+                    const int half = len >> 1;
+                    if (ax[first + 1 + half] < v) { first += half + 1; len -= half + 1; }
+                    else len = half;
+                }
+                i0 = first;
+                const float dv = ax[first + 1] - ax[first];
+                a = dv > 0.0f ? (v - ax[first]) / dv : 0.0f;
+            }
+        };
+        constexpr bool GEN = FIELD == FIELD_ARRAY_X;
         int ix, iy, iz;
         float ax, ay, az;
-        loc(p.x, (float)P.x0, (float)P.x_last, (float)P.inv_dx, P.nx, ix, ax);
-        loc(p.y, (float)P.y0, (float)P.y_last, (float)P.inv_dy, P.ny, iy, ay);
-        loc(p.z, (float)P.z0, (float)P.z_last, (float)P.inv_dz, P.nz, iz, az);
+        if (GEN && P.axf_x) {
+            loc_axis(P.axf_x, P.nx, p.x, ix, ax);
+            loc_axis(P.axf_y, P.ny, p.y, iy, ay);
+            loc_axis(P.axf_z, P.nz, p.z, iz, az);
+        } else {
+            loc(p.x, (float)P.x0, (float)P.x_last, (float)P.inv_dx, P.nx, ix, ax);
+            loc(p.y, (float)P.y0, (float)P.y_last, (float)P.inv_dy, P.ny, iy, ay);
+            loc(p.z, (float)P.z0, (float)P.z_last, (float)P.inv_dz, P.nz, iz, az);
+        }
         const size_t sy = static_cast<size_t>(P.nx), sz = static_cast<size_t>(P.nx) * P.ny;
         const size_t base = ix + sy * iy + sz * iz;
         f3 r{0.0f, 0.0f, 0.0f};
+        if (!GEN || P.n_terms <= 1) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int dx = c >> 2, dy = (c >> 1) & 1, dzz = c & 1;
-            const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dzz ? az : 1.0f - az);
-            const float4 e = P.fa_f32[base + dx + dy * sy + dzz * sz];
-            r.x += w * e.x;
-            r.y += w * e.y;
-            r.z += w * e.z;
+            for (int c8 = 0; c8 < 8; ++c8) {
+                const int dx = c8 >> 2, dy = (c8 >> 1) & 1, dzz = c8 & 1;
+                const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dzz ? az : 1.0f - az);
+                const float4 e = P.fa_f32[base + dx + dy * sy + dzz * sz];
+                r.x += w * e.x;
+                r.y += w * e.y;
+                r.z += w * e.z;
+            }
+        } else {
+            // the time factors in FP64 (phase never in FP32), applied in FP32
+            const int nt = P.n_terms;
+            for (int j = 0; j < nt; ++j) {
+                const float sj = (float)term_scale_f64(P, j, c);
+                f3 e{0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int c8 = 0; c8 < 8; ++c8) {
+                    const int dx = c8 >> 2, dy = (c8 >> 1) & 1, dzz = c8 & 1;
+                    const float w = (dx ? ax : 1.0f - ax) * (dy ? ay : 1.0f - ay) * (dzz ? az : 1.0f - az);
+                    const float4 v4 = P.fa_f32[(base + dx + dy * sy + dzz * sz) * nt + j];
+                    e.x += w * v4.x;
+                    e.y += w * v4.y;
+                    e.z += w * v4.z;
+                }
+                r = r + e * sj;
+            }
         }
         const float s = (float)P.fa_scale;
         if (s != 1.0f) r = r * s;
@@ -308,8 +353,8 @@ __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst
 
 template <int FIELD, int COLL>
 __device__ __forceinline__ f3 accel_f32(const StepParams& P, const FastConst& C, f3 p, f3 v,
-                                        float c_rf, float qm, float lpre) {
-    f3 a = field_f32<FIELD>(P, C, p, c_rf) * qm;
+                                        double c, float qm, float lpre) {
+    f3 a = field_f32<FIELD>(P, C, p, c) * qm;
     if constexpr (COLL == COLL_DAMPING) {
         const float g = (P.damping_model == 3) ? lpre / fmaxf(1e-6f, fast_sqrt(dot(v, v))) : lpre;
         if (g > 0.0f) a = a - f3{v.x - C.u_gas[0], v.y - C.u_gas[1], v.z - C.u_gas[2]} * g;
@@ -389,17 +434,17 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
         const f3 v0{hi[3], hi[4], hi[5]};
         f3 dpos, dvel;
         if constexpr (INTEG == 0) {
-            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
+            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, qm, lpre);
             const f3 k2v = v0 + k1a * half_dt;
-            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, (float)te.ch, qm, lpre);
+            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, te.ch, qm, lpre);
             const f3 k3v = v0 + k2a * half_dt;
-            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, (float)te.ch, qm, lpre);
+            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, te.ch, qm, lpre);
             const f3 k4v = v0 + k3a * dt;
-            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, (float)te.c1, qm, lpre);
+            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, te.c1, qm, lpre);
             dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
             dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
         } else {
-            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, (float)te.c0, qm, lpre);
+            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, qm, lpre);
             const f3 vm = v0 + a * half_dt;
             f3 rot{0.0f, 0.0f, 0.0f};
             if (P.magnetic) {
@@ -545,17 +590,17 @@ __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, 
             const f3 v0{hi[3], hi[4], hi[5]};
             f3 dpos, dvel;
             if constexpr (INTEG == 0) {
-                const f3 k1a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, (float)te.c0, qm, 0.f);
+                const f3 k1a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, te.c0, qm, 0.f);
                 const f3 k2v = v0 + k1a * half_dt;
-                const f3 k2a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + v0 * half_dt, k2v, (float)te.ch, qm, 0.f);
+                const f3 k2a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + v0 * half_dt, k2v, te.ch, qm, 0.f);
                 const f3 k3v = v0 + k2a * half_dt;
-                const f3 k3a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k2v * half_dt, k3v, (float)te.ch, qm, 0.f);
+                const f3 k3a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k2v * half_dt, k3v, te.ch, qm, 0.f);
                 const f3 k4v = v0 + k3a * dt;
-                const f3 k4a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k3v * dt, k4v, (float)te.c1, qm, 0.f);
+                const f3 k4a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k3v * dt, k4v, te.c1, qm, 0.f);
                 dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
                 dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
             } else {
-                const f3 a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, (float)te.c0, qm, 0.f);
+                const f3 a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, te.c0, qm, 0.f);
                 const f3 vmid = v0 + a * half_dt;
                 f3 rot{0.0f, 0.0f, 0.0f};
                 if (P.magnetic) {
@@ -872,6 +917,7 @@ KernelFn pick_field(int field, int coll, int precision) {
     switch (field) {
         case FIELD_ORBITRAP: return pick_coll<I, FIELD_ORBITRAP>(coll, precision);
         case FIELD_ARRAY: return pick_coll<I, FIELD_ARRAY>(coll, precision);
+        case FIELD_ARRAY_X: return pick_coll<I, FIELD_ARRAY_X>(coll, precision);
         default: return pick_coll<I, FIELD_LINEAR>(coll, precision);
     }
 }
@@ -881,6 +927,7 @@ KernelFn pick_surrogate(int field) {
     switch (field) {
         case FIELD_ORBITRAP: return step_surrogate_kernel<I, FIELD_ORBITRAP>;
         case FIELD_ARRAY: return step_surrogate_kernel<I, FIELD_ARRAY>;
+        case FIELD_ARRAY_X: return step_surrogate_kernel<I, FIELD_ARRAY_X>;
         default: return step_surrogate_kernel<I, FIELD_LINEAR>;
     }
 }

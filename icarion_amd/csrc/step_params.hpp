@@ -12,7 +12,9 @@ namespace icarion {
 enum FieldKind : int {
     FIELD_LINEAR = 0,    // IMS / TIMS / quadrupole: (-2x c(t), 2y c(t), Ez)
     FIELD_ORBITRAP = 1,  // hyperlogarithmic Orbitrap field in local coords
-    FIELD_ARRAY = 2      // trilinear field array
+    FIELD_ARRAY = 2,     // trilinear field array: uniform grid, one term
+    FIELD_ARRAY_X = 3    // trilinear field array: non-uniform axes and/or
+                         // several time-scaled terms
 };
 enum CollKind : int {
     COLL_NONE = 0,
@@ -32,8 +34,12 @@ struct SpeciesDev {
     float pad;
 };
 
-// Per-step uniform values: engine time and the RF coefficient
-// c = U_eff(t)/r0^2 at the step start, mid point and end.
+// Field-array terms a domain may superpose (CompositeFieldProvider).
+constexpr int kMaxFieldTerms = 4;
+
+// Per-step uniform values at the step start, mid point and end: for analytic
+// fields the RF coefficient c = U_eff(t)/r0^2; for a field array of several
+// time-scaled terms the stage time itself (the kernel evaluates the scales).
 struct StepEntry {
     double t;
     double c0;
@@ -125,10 +131,16 @@ struct StepParams {
     double sigma_neutral;    // sqrt(kB T / m_gas)
     double v_th;             // sqrt(8 kB T / (pi m_gas))
     double det_z;
-    // field array (uniform node-centred grid, x fastest)
+    // field array (rectilinear node-centred grid, x fastest; with several
+    // terms the node's terms are adjacent: element (node * n_terms + term))
     const float4 *fa_f32;
     const double4 *fa_f64;
-    int nx, ny, nz, fa_pad;
+    int nx, ny, nz, n_terms;
+    const double *ax_x, *ax_y, *ax_z;     // non-uniform axes (NULL: uniform grid)
+    const float *axf_x, *axf_y, *axf_z;   // the same axes in FP32
+    // per term: scale = a (constant kinds) or a cos(w t + phi) (RF kind)
+    int term_rf[kMaxFieldTerms];
+    double term_a[kMaxFieldTerms], term_w[kMaxFieldTerms], term_phi[kMaxFieldTerms];
     double x0, y0, z0;
     double dx, dy, dz;
     double inv_dx, inv_dy, inv_dz;

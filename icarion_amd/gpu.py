@@ -22,7 +22,7 @@ EXPORTED = (
     "icarion_gpu_surrogate_set_params", "icarion_gpu_surrogate_get_params",
     "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
     "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
-    "icarion_gpu_batch_autotune", "icarion_gpu_batch_download",
+    "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
 )
 
 
@@ -295,6 +295,28 @@ class Handle:
         rc = self.lib.icarion_gpu_upload_field_array(
             self.h, nx, ny, nz, origin[0], spacing[0], origin[1], spacing[1], origin[2],
             spacing[2], E.ctypes.data_as(C.c_void_p), dtype)
+        _check(rc, self.lib)
+
+    def upload_field_terms(self, axes, terms):
+        """axes: (xs, ys, zs) node coordinates (non-decreasing, possibly
+        non-uniform); terms: [(E[nz, ny, nx, 3], scale_kind, constant_scale,
+        frequency_Hz, phase_rad), ...] (E float32 or float64)."""
+        xs, ys, zs = (np.ascontiguousarray(a, dtype=np.float64) for a in axes)
+        arr = (_abi.FieldTermDesc * len(terms))()
+        keep = []
+        for i, (E, kind, const, freq, phase) in enumerate(terms):
+            E = np.ascontiguousarray(E)
+            assert E.shape[:3] == (len(zs), len(ys), len(xs))
+            dtype = 0 if E.dtype == np.float32 else 1
+            if dtype == 1:
+                E = E.astype(np.float64, copy=False)
+            keep.append(E)
+            arr[i] = _abi.FieldTermDesc(E.ctypes.data_as(C.c_void_p), dtype, kind, const, freq,
+                                        phase)
+        d = C.POINTER(C.c_double)
+        rc = self.lib.icarion_gpu_upload_field_terms(
+            self.h, len(xs), len(ys), len(zs), xs.ctypes.data_as(d), ys.ctypes.data_as(d),
+            zs.ctypes.data_as(d), arr, len(terms))
         _check(rc, self.lib)
 
     def upload_state(self, ens):

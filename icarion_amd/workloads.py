@@ -254,6 +254,69 @@ def field_array_workload(n=4_000_000, n_steps=20_000, grid=(256, 256, 256), seed
                 ensemble=ens, dt=5e-11, n_steps=n_steps, field=(E, origin, spacing))
 
 
+def stretched_axis(lo, hi, n, amp=0.12):
+    """A non-uniform monotone axis: nodes lo + (hi - lo) s(u), u uniform in [0, 1],
+    s(u) = u + amp sin(2 pi u) / (2 pi) (|amp| < 1 keeps s increasing)."""
+    u = np.linspace(0.0, 1.0, n)
+    return lo + (hi - lo) * (u + amp * np.sin(2 * np.pi * u) / (2 * np.pi))
+
+
+def gaussian_field_on_axes(xs, ys, zs, n_gauss=8, amp=(0.2, 1.0), seed=0):
+    """E = -grad phi at the nodes of a rectilinear grid, phi = sum_g A_g
+    exp(-|r - c_g|^2 / (2 s_g^2)) with A_g in U[amp] (random sign), s_g a fifth to
+    a half of the smallest extent and centres uniform in the box; returns
+    E[nz, ny, nx, 3] (float64)."""
+    rng = np.random.default_rng(seed)
+    lo = np.array([xs[0], ys[0], zs[0]])
+    ext = np.array([xs[-1] - xs[0], ys[-1] - ys[0], zs[-1] - zs[0]])
+    A = rng.uniform(*amp, n_gauss) * rng.choice([-1.0, 1.0], n_gauss)
+    s = rng.uniform(0.2, 0.5, n_gauss) * ext.min()
+    c = lo + ext * rng.uniform(0.0, 1.0, (n_gauss, 3))
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    E = np.zeros(Z.shape + (3,))
+    for g in range(n_gauss):
+        dx, dy, dz = X - c[g, 0], Y - c[g, 1], Z - c[g, 2]
+        f = A[g] * np.exp(-(dx * dx + dy * dy + dz * dz) / (2 * s[g] ** 2)) / s[g] ** 2
+        E[..., 0] += f * dx
+        E[..., 1] += f * dy
+        E[..., 2] += f * dz
+    return E
+
+
+def field_terms_workload(n=4096, grid=(20, 16, 36), seed=3, precision=_abi.PRECISION_FP64,
+                         collision=_abi.COLLISION_NONE, integrator=_abi.INTEGRATOR_RK4,
+                         uniform=False, n_terms=3, dt=5e-11):
+    """Field-array IMS with a composite source (CompositeFieldProvider): unit-voltage
+    Gaussian-sum fields on a (by default non-uniform) grid over 10 x 10 x 50 mm,
+    scaled by the domain's DC axial voltage (200 V), an RF term (150 V at
+    2 MHz, phase 0.3 rad) and a constant (-50), N2 at 300 Pa / 300 K; ions of
+    150/400/900 Da near the axis."""
+    nx, ny, nz = grid
+    if uniform:
+        xs, ys, zs = (np.linspace(-5e-3, 5e-3, nx), np.linspace(-5e-3, 5e-3, ny),
+                      np.linspace(0.0, 50e-3, nz))
+    else:
+        xs, ys, zs = (stretched_axis(-5e-3, 5e-3, nx), stretched_axis(-5e-3, 5e-3, ny, -0.2),
+                      stretched_axis(0.0, 50e-3, nz, 0.3))
+    kinds = [(_abi.FIELD_SCALE_DC_AXIAL, 0.0, 0.0, 0.0), (_abi.FIELD_SCALE_RF, 0.0, 2e6, 0.3),
+             (_abi.FIELD_SCALE_CONSTANT, -50.0, 0.0, 0.0)][:n_terms]
+    terms = [(gaussian_field_on_axes(xs, ys, zs, amp=(1.0, 5.0), seed=seed * 10 + k),) + kinds[k]
+             for k in range(n_terms)]
+    dom = ims_domain(origin_z=0.0, length=50e-3, radius=5e-3, axial_V=200.0, gas="N2",
+                     pressure_Pa=300.0)
+    dom.use_field_array = 1
+    dom.rf_voltage_V, dom.rf_frequency_Hz = 150.0, 1e6
+    phys = physics(integrator, collision, precision=precision, seed=seed)
+    sp = [species(sigma_hss=c * 1e-20, ccs=c * 1e-20) for c in (120.0, 200.0, 320.0)]
+    rng = np.random.default_rng(seed + 5)
+    sidx = rng.integers(0, 3, n).astype(np.uint32)
+    ens = gaussian_cloud(n, np.array([150.0, 400.0, 900.0])[sidx], (1e-3, 1e-3, 2e-3),
+                         center=(0.0, 0.0, 25e-3), seed=seed)
+    ens.species[:] = sidx
+    return dict(name="field_terms", domain=dom, physics=phys, species=sp, atoms=None,
+                ensemble=ens, dt=dt, n_steps=1000, axes=(xs, ys, zs), terms=terms)
+
+
 def ehss_cluster_counts(n, seed=0, lo=20, hi=500):
     """Atom counts of the EHSS clusters: uniform integers in [lo, hi]."""
     return np.random.default_rng(seed).integers(lo, hi + 1, n).astype(np.uint32)

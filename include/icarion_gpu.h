@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define ICARION_GPU_ABI_VERSION 2
+#define ICARION_GPU_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -81,6 +81,18 @@ enum icarion_precision {
     ICARION_PRECISION_FP64 = 0,       /* plain method, FP64 throughout            */
     ICARION_PRECISION_FP32_KAHAN = 1  /* FP32 state with Kahan-compensated
                                          position; FP64 time / RF phase           */
+};
+
+/* time dependence of one field-array term
+ * (FieldsConfig::FieldArrayTerm::ScaleKind; CompositeFieldProvider.h:117-147) */
+enum icarion_field_scale {
+    ICARION_FIELD_SCALE_CONSTANT = 0,   /* constant_scale                      */
+    ICARION_FIELD_SCALE_DC_AXIAL = 1,   /* the domain's dc_axial_V             */
+    ICARION_FIELD_SCALE_DC_QUAD = 2,    /* the domain's dc_quad_V              */
+    ICARION_FIELD_SCALE_DC_RADIAL = 3,  /* the domain's dc_radial_V            */
+    ICARION_FIELD_SCALE_RF = 4          /* rf_voltage_V * cos(2 pi f t + phase),
+                                           f = frequency_Hz, or the domain's
+                                           rf_frequency_Hz when that is 0     */
 };
 
 /* how an EHSS collision picks the scattering normal */
@@ -273,6 +285,36 @@ int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
 int icarion_gpu_upload_field_array(icarion_gpu *h, uint32_t nx, uint32_t ny, uint32_t nz,
                                    double x0, double dx, double y0, double dy,
                                    double z0, double dz, const void *E, int32_t dtype);
+
+/* One entry of a domain's field_array_terms: a field array on the shared
+ * grid of icarion_gpu_upload_field_terms and its scaling. */
+typedef struct icarion_field_term_desc {
+    const void *E;                /* 3*nx*ny*nz values, (Ex, Ey, Ez) per node,
+                                     x fastest                                */
+    int32_t dtype;                /* 0 = float, 1 = double                     */
+    int32_t scale_kind;           /* enum icarion_field_scale                  */
+    double constant_scale;        /* ICARION_FIELD_SCALE_CONSTANT              */
+    double frequency_Hz;          /* ICARION_FIELD_SCALE_RF (0 = domain's)     */
+    double phase_rad;             /* ICARION_FIELD_SCALE_RF                    */
+} icarion_field_term_desc;
+
+#define ICARION_MAX_FIELD_TERMS 4
+
+/* The field-array source of a domain with use_field_array = 1, on a
+ * rectilinear node-centred grid whose axes may be non-uniform (xs[nx], ys[ny],
+ * zs[nz], non-decreasing).  Interpolation is trilinear with clamping outside
+ * the grid, the node of each axis found as by the reference's axis search
+ * (interpolate_field, fieldArrayLoader.cpp:103-162).  One term is used
+ * unscaled (GridFieldProvider, PhysicsSetup.cpp:165-171); 2 to
+ * ICARION_MAX_FIELD_TERMS terms are superposed as sum_i scale_i(t) E_i(r)
+ * (CompositeFieldProvider::get_E, CompositeFieldProvider.h:80-97), with the
+ * voltages taken from the domain set by icarion_gpu_set_domain at step time.
+ * The terms share the grid (unit-voltage solutions of one mesh); the domain's
+ * field_array_scale multiplies the result.  Replaces load_field_array +
+ * GridFieldProvider / CompositeFieldProvider (PhysicsSetup.cpp:107-176). */
+int icarion_gpu_upload_field_terms(icarion_gpu *h, uint32_t nx, uint32_t ny, uint32_t nz,
+                                   const double *xs, const double *ys, const double *zs,
+                                   const icarion_field_term_desc *terms, uint32_t n_terms);
 
 /* Copy host SoA state to the device (resident until the next upload). */
 int icarion_gpu_upload_state(icarion_gpu *h, const icarion_ion_soa *ions);

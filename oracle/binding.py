@@ -55,6 +55,8 @@ def load():
     lib.oracle_orbitrap_r_for_z.argtypes = [d, d, d]
     lib.oracle_orbitrap_r_for_z.restype = d
     lib.oracle_interpolate_field.argtypes = [P(_abi.OracleField), P(d), P(d)]
+    lib.oracle_field_term_scale.argtypes = [P(_abi.DomainDesc), P(_abi.OracleFieldTerm), d]
+    lib.oracle_field_term_scale.restype = d
     lib.oracle_damping_gamma.argtypes = [P(_abi.DomainDesc), C.c_int, P(_abi.SpeciesDesc), d, d,
                                          P(d)]
     lib.oracle_damping_gamma.restype = d
@@ -119,7 +121,39 @@ class Field:
         self.Ey = np.ascontiguousarray(E[..., 1], dtype=np.float64).ravel()
         self.Ez = np.ascontiguousarray(E[..., 2], dtype=np.float64).ravel()
         self.c = _abi.OracleField(nx, ny, nz, _dp(self.xs), _dp(self.ys), _dp(self.zs),
-                                  _dp(self.Ex), _dp(self.Ey), _dp(self.Ez))
+                                  _dp(self.Ex), _dp(self.Ey), _dp(self.Ez), 0, 0,
+                                  C.POINTER(_abi.OracleFieldTerm)())
+
+
+class FieldTerms:
+    """oracle_field on a rectilinear grid (axes xs, ys, zs, possibly
+    non-uniform) with terms [(E[nz, ny, nx, 3], scale_kind, constant_scale,
+    frequency_Hz, phase_rad), ...]."""
+
+    def __init__(self, axes, terms):
+        self.xs, self.ys, self.zs = (np.ascontiguousarray(a, dtype=np.float64) for a in axes)
+        nx, ny, nz = len(self.xs), len(self.ys), len(self.zs)
+        self.arrays = []
+        self.terms = (_abi.OracleFieldTerm * len(terms))()
+        for i, (E, kind, const, freq, phase) in enumerate(terms):
+            E = np.asarray(E)
+            assert E.shape[:3] == (nz, ny, nx)
+            comp = [np.ascontiguousarray(E[..., c], dtype=np.float64).ravel() for c in range(3)]
+            self.arrays.append(comp)
+            self.terms[i] = _abi.OracleFieldTerm(kind, 0, const, freq, phase, _dp(comp[0]),
+                                                 _dp(comp[1]), _dp(comp[2]))
+        first = self.arrays[0]
+        self.c = _abi.OracleField(nx, ny, nz, _dp(self.xs), _dp(self.ys), _dp(self.zs),
+                                  _dp(first[0]), _dp(first[1]), _dp(first[2]), len(terms), 0,
+                                  C.cast(self.terms, C.POINTER(_abi.OracleFieldTerm)))
+
+
+def field_term_scale(domain, term, t):
+    """scale_i(t) of one term (kind, constant, frequency, phase) for the domain."""
+    lib = load()
+    kind, const, freq, phase = term
+    tt = _abi.OracleFieldTerm(kind, 0, const, freq, phase)
+    return lib.oracle_field_term_scale(C.byref(domain), C.byref(tt), t)
 
 
 def species_array(species):

@@ -238,7 +238,8 @@ void locate(const double* arr, uint32_t n, double v, size_t& i0, size_t& i1, dou
     }
 }
 
-V3 interpolate(const oracle_field& f, V3 pos) {
+V3 interpolate_arrays(const oracle_field& f, const double* Ex, const double* Ey, const double* Ez,
+                      V3 pos) {
     size_t ix0, ix1, iy0, iy1, iz0, iz1;
     double ax, ay, az;
     locate(f.xs, f.nx, pos.x, ix0, ix1, ax);
@@ -256,20 +257,56 @@ V3 interpolate(const oracle_field& f, V3 pos) {
                 const size_t iz = (dz == 0 ? iz0 : iz1);
                 const double w = wx * wy * wz;
                 const size_t idx = ix + f.nx * (iy + f.ny * iz);
-                r.x += w * f.Ex[idx];
-                r.y += w * f.Ey[idx];
-                r.z += w * f.Ez[idx];
+                r.x += w * Ex[idx];
+                r.y += w * Ey[idx];
+                r.z += w * Ez[idx];
             }
         }
     }
     return r;
 }
 
+// CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147);
+// ValueOrWaveform values are the domain's constants (eval_voltage, :150-172).
+double term_scale(const icarion_domain_desc& d, const oracle_field_term& term, double t) {
+    switch (term.scale_kind) {
+        case ICARION_FIELD_SCALE_CONSTANT: return term.constant_scale;
+        case ICARION_FIELD_SCALE_DC_AXIAL: return d.dc_axial_V;
+        case ICARION_FIELD_SCALE_DC_QUAD: return d.dc_quad_V;
+        case ICARION_FIELD_SCALE_DC_RADIAL: return d.dc_radial_V;
+        case ICARION_FIELD_SCALE_RF: {
+            const double V = d.rf_voltage_V;
+            const double f = term.frequency_Hz > 0.0 ? term.frequency_Hz : d.rf_frequency_Hz;
+            const double omega = 2.0 * kPi * f;
+            return V * std::cos(omega * t + term.phase_rad);
+        }
+        default: return 0.0;
+    }
+}
+
+// The field source of a domain with field arrays: GridFieldProvider for one
+// array, CompositeFieldProvider::get_E (CompositeFieldProvider.h:80-97) for
+// several (per term: interpolate, then add scale * E_i in term order).
+V3 interpolate(const oracle_field& f, V3 pos, const icarion_domain_desc* d = nullptr, double t = 0.0) {
+    if (f.n_terms == 0) return interpolate_arrays(f, f.Ex, f.Ey, f.Ez, pos);
+    if (f.n_terms == 1) return interpolate_arrays(f, f.terms[0].Ex, f.terms[0].Ey, f.terms[0].Ez, pos);
+    V3 E{0.0, 0.0, 0.0};
+    for (uint32_t i = 0; i < f.n_terms; ++i) {
+        const oracle_field_term& term = f.terms[i];
+        const double scale = d ? term_scale(*d, term, t) : 1.0;
+        const V3 Ei = interpolate_arrays(f, term.Ex, term.Ey, term.Ez, pos);
+        E.x += scale * Ei.x;
+        E.y += scale * Ei.y;
+        E.z += scale * Ei.z;
+    }
+    return E;
+}
+
 // ElectricFieldForce::compute_soa -> IFieldModel::E (ElectricFieldForce.cpp:66-83);
 // a field array replaces the analytic model (PhysicsSetup.cpp:107-195).
 V3 field_E(const icarion_domain_desc& d, const oracle_field* f, V3 pos, double t) {
     if (d.use_field_array && f) {
-        V3 E = interpolate(*f, pos);
+        V3 E = interpolate(*f, pos, &d, t);
         if (d.field_array_scale != 1.0) E = E * d.field_array_scale;
         return E;
     }
@@ -889,6 +926,11 @@ int oracle_contains(const icarion_domain_desc* domain, const double pos[3]) {
 }
 
 double oracle_orbitrap_r_for_z(double z, double R, double R_m) { return orbitrap_r_for_z(z, R, R_m); }
+
+double oracle_field_term_scale(const icarion_domain_desc* domain, const oracle_field_term* term,
+                               double t) {
+    return term_scale(*domain, *term, t);
+}
 
 void oracle_interpolate_field(const oracle_field* field, const double pos[3], double E_out[3]) {
     const V3 E = interpolate(*field, V3{pos[0], pos[1], pos[2]});

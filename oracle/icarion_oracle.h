@@ -20,13 +20,36 @@
 extern "C" {
 #endif
 
+/* One term of a composite field (CompositeFieldProvider::FieldTerm,
+ * CompositeFieldProvider.h:39-46): its components on the field's grid and
+ * its time dependence (enum icarion_field_scale). */
+typedef struct oracle_field_term {
+    int32_t scale_kind;
+    int32_t reserved0;
+    double constant_scale;
+    double frequency_Hz;
+    double phase_rad;
+    const double *Ex, *Ey, *Ez;
+} oracle_field_term;
+
 /* Field array in the reference's own layout (FieldArray,
- * fieldArrayLoader.h:23-40): monotone axes, x-fastest FP64 components. */
+ * fieldArrayLoader.h:23-40): monotone axes, x-fastest FP64 components.
+ * n_terms == 0: the single array (Ex, Ey, Ez), unscaled (GridFieldProvider);
+ * n_terms == 1: that term's arrays, unscaled (PhysicsSetup.cpp:165-171);
+ * n_terms >= 2: sum_i scale_i(t) E_i (CompositeFieldProvider). */
 typedef struct oracle_field {
     uint32_t nx, ny, nz;
     const double *xs, *ys, *zs;
     const double *Ex, *Ey, *Ez;
+    uint32_t n_terms;
+    uint32_t reserved0;
+    const oracle_field_term *terms;
 } oracle_field;
+
+/* CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147)
+ * with the domain's constant voltages. */
+double oracle_field_term_scale(const icarion_domain_desc *domain, const oracle_field_term *term,
+                               double t);
 
 enum { ORACLE_RNG_PHILOX = 0, ORACLE_RNG_MT19937_64 = 1 };
 

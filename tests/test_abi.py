@@ -61,6 +61,8 @@ STRUCTS = {
                                             "t0_s", "n_steps", "step_offset"]),
     "icarion_batch_tuning": (_abi.BatchTuning, ["block_size", "groups_per_cu"]),
     "icarion_cloud_stats": (_abi.CloudStats, ["ion_steps", "arrived", "t_end_s", "done_ms"]),
+    "icarion_field_term_desc": (_abi.FieldTermDesc, ["E", "dtype", "scale_kind", "constant_scale",
+                                                     "phase_rad"]),
 }
 
 

@@ -35,6 +35,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
         if w.get("field") is not None:
             E, origin, spacing = w["field"]
             h.upload_field_array(E, origin, spacing)
+        if w.get("terms") is not None:
+            h.upload_field_terms(w["axes"], w["terms"])
         h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -45,6 +47,8 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
     field = None
     if w.get("field") is not None:
         field = oracle.Field(*w["field"])
+    if w.get("terms") is not None:
+        field = oracle.FieldTerms(w["axes"], w["terms"])
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
                       step_offset=step_offset, atoms=w.get("atoms"), field=field)
 
@@ -187,6 +191,75 @@ def test_field_array_fp64(oracle):
     gpu_run(w, g, 1500)
     oracle_run(oracle, w, o, 1500)
     assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_field_terms_nonuniform_collisionless_fp64(oracle, integrator):
+    """Composite field (DC + RF + constant terms) on non-uniform axes: ion by
+    ion against the oracle's axis search and CompositeFieldProvider sum."""
+    w = W.field_terms_workload(n=2048, seed=3, integrator=integrator)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 1500, t0=2e-8)
+    oracle_run(oracle, w, o, 1500, t0=2e-8)
+    assert compare_ions(g, o, rtol=1e-9).all()
+    # the ions moved through the grid: several cells along z
+    assert np.abs(g.z - w["ensemble"].z).max() > 1e-4
+
+
+def test_field_terms_hss_fp64(oracle):
+    w = W.field_terms_workload(n=2048, seed=4, collision=_abi.COLLISION_HSS)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 1500)
+    so = oracle_run(oracle, w, o, 1500)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+
+
+def test_field_terms_fp32_against_fp64_oracle(oracle):
+    w = W.field_terms_workload(n=2048, seed=5, n_terms=2)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 1500, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w, o, 1500)
+    assert compare_ions(g, o, rtol=2e-5, pos_scale=1e-3, vel_scale=10.0).mean() >= 0.99
+
+
+def test_field_terms_single_uniform_term_equals_field_array():
+    """One term on uniform axes is the plain field array (unscaled, whatever
+    its kind): bit for bit the same trajectories as icarion_gpu_upload_field_array."""
+    w = W.field_terms_workload(n=1024, seed=6, uniform=True, n_terms=1,
+                               collision=_abi.COLLISION_HSS)
+    w["terms"] = [(w["terms"][0][0], _abi.FIELD_SCALE_RF, 0.0, 3e6, 1.0)]
+    xs, ys, zs = w["axes"]
+    plain = dict(w)
+    plain.pop("terms")
+    plain["field"] = (w["terms"][0][0], (xs[0], ys[0], zs[0]),
+                      ((xs[-1] - xs[0]) / (len(xs) - 1), (ys[-1] - ys[0]) / (len(ys) - 1),
+                       (zs[-1] - zs[0]) / (len(zs) - 1)))
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, a, 800)
+    gpu_run(plain, b, 800)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+
+
+def test_field_terms_reject_bad_input():
+    from icarion_amd import gpu
+    w = W.field_terms_workload(n=16, seed=7)
+    xs, ys, zs = w["axes"]
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        bad = xs.copy()
+        bad[3], bad[4] = bad[4], bad[3]
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.upload_field_terms((bad, ys, zs), w["terms"])
+        assert ei.value.code == _abi.ERR_INVALID
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.upload_field_terms((xs, ys, zs), w["terms"] * 2)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        bad_kind = [(w["terms"][0][0], 9, 0.0, 0.0, 0.0)]
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.upload_field_terms((xs, ys, zs), bad_kind)
+        assert ei.value.code == _abi.ERR_INVALID
 
 
 def test_birth_times_fp64(oracle):

@@ -47,6 +47,8 @@ def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
         if w.get("field") is not None:
             E, origin, spacing = w["field"]
             h.upload_field_array(E, origin, spacing)
+        if w.get("terms") is not None:
+            h.upload_field_terms(w["axes"], w["terms"])
         h.upload_state(ens)
         st = h.step(t0, dt, n_steps, step_offset)
         h.download_state(ens)
@@ -56,6 +58,8 @@ def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
 def oracle_rk45(oracle, w, ph, ens, n_steps, dt, t0=0.0, cfg=None):
     cfg = cfg or _abi.rk45_defaults()
     field = oracle.Field(*w["field"]) if w.get("field") is not None else None
+    if w.get("terms") is not None:
+        field = oracle.FieldTerms(w["axes"], w["terms"])
     st, _ = oracle.run_rk45(w["domain"], ph, cfg, w["species"], ens, t0, dt, n_steps,
                             atoms=w.get("atoms"), field=field)
     return st
@@ -101,11 +105,14 @@ def case_workload(case):
     if case == "langevin_ou":
         w = W.quadrupole_workload(n=2048, U=10.0, seed=13)
         return w, rk45_physics(w), 1e-9
+    if case == "field_terms":
+        w = W.field_terms_workload(n=2048, seed=8)
+        return w, rk45_physics(w), w["dt"]
     w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
     return w, rk45_physics(w), w["dt"]
 
 
-CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss"]
+CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss", "field_terms"]
 
 
 @pytest.mark.parametrize("case", CASES)

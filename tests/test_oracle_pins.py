@@ -225,6 +225,57 @@ def test_trilinear_exact_and_clamped(oracle):
         assert out[1] == pytest.approx(2 * ref, rel=1e-10, abs=1e-12)
 
 
+def test_composite_field_terms(oracle):
+    """CompositeFieldProvider semantics on a non-uniform grid: the field is
+    sum_i scale_i(t) E_i(r), each E_i the scipy trilinear interpolant of its
+    array, with scale_i the closed forms of the scale kinds (constant; the
+    domain's DC voltages; V cos(2 pi f t + phase), f falling back to the
+    domain's RF frequency); a single term is used unscaled whatever its kind."""
+    rng = np.random.default_rng(8)
+    xs = np.sort(rng.uniform(-1e-3, 1e-3, 9))
+    ys = np.sort(rng.uniform(-1e-3, 1e-3, 6))
+    zs = np.sort(rng.uniform(0.0, 5e-3, 11))
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    arrays = [np.stack([np.sin(900 * X + k) * 1e3, np.cos(700 * Y - k) * 2e3,
+                        (Z * 1e3 + k) * 1e2], -1) for k in range(4)]
+    d = W.ims_domain(EN_Td=5.0)
+    d.use_field_array = 1
+    d.dc_axial_V, d.dc_quad_V, d.dc_radial_V = 120.0, -35.0, 7.5
+    d.rf_voltage_V, d.rf_frequency_Hz = 250.0, 1.3e6
+    kinds = [(_abi.FIELD_SCALE_DC_AXIAL, 0.0, 0.0, 0.0), (_abi.FIELD_SCALE_RF, 0.0, 2.1e6, 0.4),
+             (_abi.FIELD_SCALE_CONSTANT, -3.25, 0.0, 0.0), (_abi.FIELD_SCALE_RF, 0.0, 0.0, -1.1)]
+
+    def scale(kind, const, f, ph, t):
+        if kind == _abi.FIELD_SCALE_CONSTANT:
+            return const
+        if kind == _abi.FIELD_SCALE_RF:
+            f = f if f > 0 else d.rf_frequency_Hz
+            return d.rf_voltage_V * math.cos(2 * math.pi * f * t + ph)
+        return {_abi.FIELD_SCALE_DC_AXIAL: d.dc_axial_V, _abi.FIELD_SCALE_DC_QUAD: d.dc_quad_V,
+                _abi.FIELD_SCALE_DC_RADIAL: d.dc_radial_V}[kind]
+
+    for kind in (_abi.FIELD_SCALE_DC_QUAD, _abi.FIELD_SCALE_DC_RADIAL):
+        assert oracle.field_term_scale(d, (kind, 0.0, 0.0, 0.0), 1e-6) == scale(kind, 0, 0, 0, 0)
+    f = oracle.FieldTerms((xs, ys, zs), [(a,) + k for a, k in zip(arrays, kinds)])
+    rgis = [[interpolate.RegularGridInterpolator((zs, ys, xs), a[..., c]) for c in range(3)]
+            for a in arrays]
+    for _ in range(30):
+        p = [rng.uniform(xs[0], xs[-1]), rng.uniform(ys[0], ys[-1]), rng.uniform(zs[0], zs[-1])]
+        t = rng.uniform(0.0, 3e-6)
+        got = oracle.field_E(d, p, t, field=f)
+        ref = np.zeros(3)
+        for k, r in zip(kinds, rgis):
+            s_k = scale(*k, t)
+            assert oracle.field_term_scale(d, k, t) == pytest.approx(s_k, rel=1e-12, abs=1e-12)
+            ref += s_k * np.array([r[c]([p[2], p[1], p[0]])[0] for c in range(3)])
+        np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    # one term: GridFieldProvider, no scaling even for an RF term
+    one = oracle.FieldTerms((xs, ys, zs), [(arrays[1],) + kinds[1]])
+    p = [0.5 * (xs[0] + xs[-1]), 0.5 * (ys[0] + ys[-1]), 0.5 * (zs[0] + zs[-1])]
+    ref = np.array([rgis[1][c]([p[2], p[1], p[0]])[0] for c in range(3)])
+    np.testing.assert_allclose(oracle.field_E(d, p, 7.7e-7, field=one), ref, rtol=1e-10)
+
+
 # ---------------------------------------------------------------- pushes
 def _push_many(oracle, d, ph, m, q, p, v, t0, dt, n):
     sp = W.species()
