@@ -125,6 +125,18 @@ class GpuDesc(C.Structure):
                 ("reserved0", _u32)]
 
 
+class GasComponent(C.Structure):
+    _fields_ = [("density_m3", _d), ("mass_kg", _d), ("participates", _i32), ("reserved0", _i32)]
+
+
+MAX_GAS_COMPONENTS = 4
+
+
+class OracleGasMixture(C.Structure):
+    _fields_ = [("n_components", _u32), ("n_species", _u32),
+                ("components", C.POINTER(GasComponent)), ("sigma_m2", C.POINTER(_d))]
+
+
 class OracleFieldTerm(C.Structure):
     _fields_ = [
         ("scale_kind", _i32), ("reserved0", _i32), ("constant_scale", _d), ("frequency_Hz", _d),

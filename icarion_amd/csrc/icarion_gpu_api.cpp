@@ -164,6 +164,11 @@ struct icarion_gpu {
     } fsrc;
     DevBuf<double> ax64[3];
     DevBuf<float> ax32[3];
+    // HSS gas mixture: the participating components with n > 0 (in order) and
+    // their per-species cross sections, as the kernels read them
+    std::vector<icarion_gas_component> mix;
+    uint32_t mix_species = 0;
+    DevBuf<double> mix_sigma;
 
     size_t n = 0;
     bool have_state = false;
@@ -171,6 +176,7 @@ struct icarion_gpu {
     DevBuf<uint32_t> spec, ncoll;
     DevBuf<uint8_t> flags;
     bool has_birth = false;
+    uint32_t species_used = 0;   // 1 + the largest species index of the resident ions
 
     DevBuf<StepEntry> table;
     DevBuf<unsigned long long> counters;
@@ -247,6 +253,13 @@ int coll_kind(const icarion_physics_desc& p) {
         case ICARION_COLLISION_FRICTION: return COLL_DAMPING;
         default: return COLL_NONE;
     }
+}
+
+// The kernel variant of a handle's collision stage: HSS over a gas mixture
+// has its own.
+int launch_coll_kind(const icarion_gpu* h) {
+    const int c = coll_kind(h->phys);
+    return (c == COLL_HSS && !h->mix.empty()) ? COLL_HSS_MIX : c;
 }
 
 void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p) {
@@ -406,6 +419,25 @@ void apply_field(const icarion_gpu* h, StepParams& P) {
     }
 }
 
+// The gas mixture into the launch parameters (neutral widths at the domain's
+// temperature); an error when the species table changed size since.
+void apply_mixture(const icarion_gpu* h, StepParams& P) {
+    // a species table replaced after the upload must still cover the ions
+    if (h->species_used > h->n_species)
+        fail(ICARION_ERR_STATE, "the species table no longer covers the resident ions' species indices");
+    P.n_mix = static_cast<int>(h->mix.size());
+    P.mix_sigma = h->mix.empty() ? nullptr : h->mix_sigma.p;
+    if (!h->mix.empty() && h->mix_species != h->n_species)
+        fail(ICARION_ERR_STATE, "the gas mixture's species count differs from the species table");
+    for (int c = 0; c < kMaxGasComponents; ++c) {
+        const bool on = c < P.n_mix;
+        P.mix_n[c] = on ? h->mix[c].density_m3 : 0.0;
+        P.mix_m[c] = on ? h->mix[c].mass_kg : 0.0;
+        // VelocitySampling::sample_neutral_velocity's width (VelocitySampling.cpp:17)
+        P.mix_sig[c] = on ? std::sqrt(kBoltzmannHost * h->dom.temperature_K / h->mix[c].mass_kg) : 0.0;
+    }
+}
+
 void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
     h->table_host.clear();
     const bool stage_times = field_kind(h->dom) == FIELD_ARRAY && h->fsrc.n_terms >= 2;
@@ -440,7 +472,7 @@ IonPtrs ion_ptrs(icarion_gpu* h) {
 void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
                   icarion_step_stats* stats) {
     const int field = launch_field_kind(h);
-    const int coll = coll_kind(h->phys);
+    const int coll = launch_coll_kind(h);
     if (h->phys.precision != ICARION_PRECISION_FP64)
         fail(ICARION_ERR_UNSUPPORTED, "RK45 runs in FP64");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -454,6 +486,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     P.fa_f32 = h->fa32.p;
     P.fa_f64 = h->fa64.p;
     apply_field(h, P);
+    apply_mixture(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
     P.rk45 = h->rk45;
@@ -580,6 +613,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     P.fa_f32 = h->fa32.p;
     P.fa_f64 = (h->phys.precision == ICARION_PRECISION_FP64) ? h->fa64.p : nullptr;
     apply_field(h, P);
+    apply_mixture(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
     fill_fast_const(P);
@@ -598,7 +632,8 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
             P.table = h->table.p + k0;
             P.step0 = step_offset + k0;
             P.n_steps = static_cast<uint32_t>(nk);
-            const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), coll, h->phys.precision,
+            const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), launch_coll_kind(h),
+                                             h->phys.precision,
                                              surrogate ? 1 : 0, P, S, h->block, h->stream);
             if (e == hipErrorInvalidValue)
                 fail(ICARION_ERR_UNSUPPORTED, "no kernel for this integrator/field/collision/precision");
@@ -649,13 +684,16 @@ void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
     if (n && (!ions->x || !ions->y || !ions->z || !ions->vx || !ions->vy || !ions->vz ||
               !ions->mass_kg || !ions->charge_C || !ions->species || !ions->active))
         fail(ICARION_ERR_INVALID, "x, y, z, vx, vy, vz, mass_kg, charge_C, species, active are required");
+    uint32_t used = 0;
     for (size_t i = 0; i < n; ++i) {
         if (ions->species[i] >= h->n_species)
             fail(ICARION_ERR_INVALID, "species index " + std::to_string(ions->species[i]) +
                                           " of ion " + std::to_string(i) + " out of range");
         if (!(ions->mass_kg[i] > 0.0))
             fail(ICARION_ERR_INVALID, "mass_kg of ion " + std::to_string(i) + " must be > 0");
+        used = std::max(used, ions->species[i] + 1);
     }
+    h->species_used = used;
     set_device(h);
     hipStream_t s = h->stream;
     copy_in(h->x, ions->x, n, s, "upload x");
@@ -1302,6 +1340,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
     for (auto& a : h->ax64) a.release();
     for (auto& a : h->ax32) a.release();
+    h->mix_sigma.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
     h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
     h->b_params_d.release(); h->b_ions_d.release(); h->b_kinds_d.release(); h->b_queue.release();
@@ -1471,6 +1510,55 @@ int icarion_gpu_upload_field_array(icarion_gpu* h, uint32_t nx, uint32_t ny, uin
         f.z_last = z0 + (nz - 1) * dz;
         h->fsrc = f;
         h->have_field = true;
+    });
+}
+
+int icarion_gpu_set_gas_mixture(icarion_gpu* h, const icarion_gas_component* components,
+                                uint32_t n_components, const double* sigma_m2, uint32_t n_species) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (n_components == 0) {
+            h->mix.clear();
+            h->mix_species = 0;
+            h->mix_sigma.release();
+            return;
+        }
+        if (!components || !sigma_m2) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (n_components > static_cast<uint32_t>(kMaxGasComponents))
+            fail(ICARION_ERR_UNSUPPORTED, "at most " + std::to_string(kMaxGasComponents) + " gas components");
+        if (!h->have_species || n_species != h->n_species)
+            fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
+        // the components the handler's loop reaches (HSSCollisionHandler.cpp:128-134)
+        std::vector<uint32_t> keep;
+        for (uint32_t c = 0; c < n_components; ++c) {
+            const icarion_gas_component& g = components[c];
+            if (!std::isfinite(g.density_m3) || !std::isfinite(g.mass_kg))
+                fail(ICARION_ERR_INVALID, "gas component values must be finite");
+            if (!g.participates || g.density_m3 <= 0.0) continue;
+            if (!(g.mass_kg > 0.0)) fail(ICARION_ERR_INVALID, "gas component mass must be > 0");
+            for (uint32_t sp = 0; sp < n_species; ++sp)
+                if (!(sigma_m2[static_cast<size_t>(sp) * n_components + c] > 0.0))
+                    fail(ICARION_ERR_INVALID, "missing cross section for species " + std::to_string(sp) +
+                                                  " and gas component " + std::to_string(c));
+            keep.push_back(c);
+        }
+        std::vector<double> sig(static_cast<size_t>(n_species) * std::max<size_t>(keep.size(), 1), 0.0);
+        for (uint32_t sp = 0; sp < n_species; ++sp)
+            for (size_t j = 0; j < keep.size(); ++j)
+                sig[sp * keep.size() + j] = sigma_m2[static_cast<size_t>(sp) * n_components + keep[j]];
+        set_device(h);
+        std::vector<icarion_gas_component> mix;
+        for (uint32_t c : keep) mix.push_back(components[c]);
+        if (mix.empty()) {
+            // no component can collide: the handler finds k_total = 0 and
+            // returns; one zero-density component gives the kernels the same
+            mix.push_back(icarion_gas_component{0.0, components[0].mass_kg > 0.0 ? components[0].mass_kg : 1.0,
+                                                1, 0});
+        }
+        copy_in(h->mix_sigma, sig.data(), sig.size(), h->stream, "upload mixture cross sections");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        h->mix = std::move(mix);
+        h->mix_species = n_species;
     });
 }
 

@@ -110,7 +110,8 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
             } else {
                 const double m = S.mass[i];
                 const double q = S.charge[i];
-                const SpeciesDev sp = P.species[S.species[i]];
+                const uint32_t sidx = S.species[i];
+                const SpeciesDev sp = P.species[sidx];
                 const double inv_m = 1.0 / m;
                 double lpre = 0.0;
                 if constexpr (COLL == COLL_DAMPING) {
@@ -191,7 +192,7 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                     const double z_before = p0.z;
                     if constexpr (COLL != COLL_NONE) {
                         PhiloxStream rng(P.seed, i, kstep);
-                        if (collide_f64<COLL>(P, sp, m, lpre, vel, rng, used)) {
+                        if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, used)) {
                             ++nc;
                             ++ncoll_launch;
                         }
@@ -280,6 +281,7 @@ Rk45Fn pick_rk45_coll(int coll) {
     switch (coll) {
         case COLL_NONE: return rk45_step_kernel<FIELD, COLL_NONE>;
         case COLL_HSS: return rk45_step_kernel<FIELD, COLL_HSS>;
+        case COLL_HSS_MIX: return rk45_step_kernel<FIELD, COLL_HSS_MIX>;
         case COLL_EHSS: return rk45_step_kernel<FIELD, COLL_EHSS>;
         case COLL_DAMPING: return rk45_step_kernel<FIELD, COLL_DAMPING>;
         default: return nullptr;

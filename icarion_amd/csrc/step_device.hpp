@@ -310,8 +310,80 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
 
 // One collision stage; returns true on a collision event.
 template <int COLL>
-__device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDev& sp, double m,
-                                            double lpre, d3& v, PhiloxStream& rng, double dt) {
+__device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDev& sp, uint32_t sidx,
+                                            double m, double lpre, d3& v, PhiloxStream& rng, double dt) {
+    if constexpr (COLL == COLL_HSS_MIX) {
+        {
+            // HSSCollisionHandler::handle_collision mixture path
+            // (HSSCollisionHandler.cpp:105-249): a neutral per component, rates
+            // from the actual relative speeds, one test on the total rate, the
+            // partner drawn with weight k_c, HSS with the neutral drawn for it
+            const double* sg = P.mix_sigma + static_cast<size_t>(sidx) * P.n_mix;
+            d3 vn[kMaxGasComponents];
+            double k[kMaxGasComponents];
+            double k_total = 0.0;
+#pragma unroll
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                k[c] = -1.0;
+                vn[c] = d3{0.0, 0.0, 0.0};
+                if (c < P.n_mix) {
+                    const double sn = P.mix_sig[c];
+                    double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
+                    vn[c].x = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                    u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+                    vn[c].y = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                    u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+                    vn[c].z = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                    vn[c] = vn[c] + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+                    const d3 vr = v - vn[c];
+                    const double vm = sqrt(dot(vr, vr));
+                    if (vm >= kMinRelativeVelocity) {
+                        k[c] = P.mix_n[c] * sg[c] * vm;
+                        k_total += k[c];
+                    }
+                }
+            }
+            if (!(k_total > 0.0)) return false;
+            double Pt = 1.0;
+            if (k_total * dt <= 50.0) Pt = 1.0 - exp(-k_total * dt);
+            if (rng.uniform_f64() >= Pt) return false;
+            const double r = rng.uniform_f64() * k_total;
+            // the partner: the first component whose running sum exceeds r,
+            // else the first one with a rate
+            int sel = -1, first = -1;
+            double cum = 0.0;
+#pragma unroll
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                if (k[c] >= 0.0) {
+                    if (first < 0) first = c;
+                    cum += k[c];
+                    if (sel < 0 && r < cum) sel = c;
+                }
+            }
+            if (sel < 0) sel = first;
+            d3 vs{0.0, 0.0, 0.0};
+            double ms = 0.0;
+#pragma unroll
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                if (c == sel) {
+                    vs = vn[c];
+                    ms = P.mix_m[c];
+                }
+            }
+            // CollisionKernels::hss_collision (CollisionKernels.cpp:139-169)
+            d3 vr = v - vs;
+            double vm = sqrt(dot(vr, vr));
+            if (vm <= kMinVelocityMag) {
+                vr = iso_dir_f64(rng) * kMinVelocityMag;
+                vm = kMinVelocityMag;
+            }
+            const d3 nd = iso_dir_f64(rng);
+            const double inv_mt = 1.0 / (m + ms);
+            const d3 Vcom = (v * m + vs * ms) * inv_mt;
+            v = Vcom + (nd * vm) * (ms * inv_mt);
+            return true;
+        }
+    }
     if constexpr (COLL == COLL_HSS || COLL == COLL_EHSS) {
         const double sigma = (COLL == COLL_HSS) ? sp.sigma_hss : sp.sigma_ehss;
         if (sigma <= 0.0) return false;

@@ -25,7 +25,8 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
     d3 vel{S.vx[i], S.vy[i], S.vz[i]};
     const double m = S.mass[i];
     const double q = S.charge[i];
-    const SpeciesDev sp = P.species[S.species[i]];
+    const uint32_t sidx = S.species[i];
+    const SpeciesDev sp = P.species[sidx];
     uint8_t flags = S.flags[i];
     bool active = flags & 1, born = (flags >> 1) & 1;
     const double birth = S.birth ? S.birth[i] : -INFINITY;
@@ -105,7 +106,7 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
         ++pushed;
         if constexpr (COLL != COLL_NONE) {
             PhiloxStream rng(P.seed, i, P.step0 + k);
-            if (collide_f64<COLL>(P, sp, m, lpre, vel, rng, dt)) {
+            if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, dt)) {
                 ++nc;
                 ++ncoll_launch;
             }
@@ -306,10 +307,72 @@ __device__ __forceinline__ f3 iso_dir_f32(PhiloxStream& rng) {
 
 template <int COLL>
 __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst& C,
-                                            const SpeciesDev& sp, float m, float inv_m_plus,
-                                            float lpre, float kbt2_dt_over_m, f3& v,
-                                            PhiloxStream& rng) {
-    if constexpr (COLL == COLL_HSS) {
+                                            const SpeciesDev& sp, uint32_t sidx, float m,
+                                            float inv_m_plus, float lpre, float kbt2_dt_over_m, f3& v,
+                                            PhiloxStream& rng, float dt) {
+    if constexpr (COLL == COLL_HSS_MIX) {
+        {
+            // the mixture path of collide_f64 in FP32 (rates, test and partner
+            // choice as there; neutrals from two Box-Muller pairs)
+            const double* sg = P.mix_sigma + static_cast<size_t>(sidx) * P.n_mix;
+            f3 vn[kMaxGasComponents];
+            float k[kMaxGasComponents];
+            float k_total = 0.0f;
+#pragma unroll
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                k[c] = -1.0f;
+                vn[c] = f3{0.0f, 0.0f, 0.0f};
+                if (c < P.n_mix) {
+                    const float sn = (float)P.mix_sig[c];
+                    const float u1 = rng.uniform_f32(), u2 = rng.uniform_f32();
+                    const float u3 = rng.uniform_f32(), u4 = rng.uniform_f32();
+                    const float r1 = sn * fast_sqrt(-2.0f * fast_ln(u1));
+                    const float r2 = sn * fast_sqrt(-2.0f * fast_ln(u3));
+                    vn[c] = f3{r1 * fast_cos_rev(u2) + C.u_gas[0], r1 * fast_sin_rev(u2) + C.u_gas[1],
+                               r2 * fast_cos_rev(u4) + C.u_gas[2]};
+                    const f3 vr = v - vn[c];
+                    const float vm = fast_sqrt(dot(vr, vr));
+                    if (vm >= (float)kMinRelativeVelocity) {
+                        k[c] = (float)(P.mix_n[c] * sg[c]) * vm;
+                        k_total += k[c];
+                    }
+                }
+            }
+            if (!(k_total > 0.0f)) return false;
+            const float x = k_total * dt;
+            const float Pc = x < 1e-3f ? x * (1.0f - 0.5f * x * (1.0f - x * (1.0f / 3.0f)))
+                                       : (x <= 50.0f ? 1.0f - __builtin_amdgcn_exp2f(-x * 1.44269504089f) : 1.0f);
+            if (rng.uniform_f32() >= Pc) return false;
+            const float r = rng.uniform_f32() * k_total;
+            int sel = -1, first = -1;
+            float cum = 0.0f;
+#pragma unroll
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                if (k[c] >= 0.0f) {
+                    if (first < 0) first = c;
+                    cum += k[c];
+                    if (sel < 0 && r < cum) sel = c;
+                }
+            }
+            if (sel < 0) sel = first;
+            f3 vs{0.0f, 0.0f, 0.0f};
+            float ms = 0.0f;
+#pragma unroll
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                if (c == sel) {
+                    vs = vn[c];
+                    ms = (float)P.mix_m[c];
+                }
+            }
+            const f3 vr = v - vs;
+            const float vm = fast_sqrt(dot(vr, vr));
+            const f3 nd = iso_dir_f32(rng);
+            const float inv_mt = 1.0f / (m + ms);
+            const f3 Vcom = (v * m + vs * ms) * inv_mt;
+            v = Vcom + nd * (vm * ms * inv_mt);
+            return true;
+        }
+    } else if constexpr (COLL == COLL_HSS) {
         const float sigma = (float)sp.sigma_hss;
         if (!(sigma > 0.0f)) return false;
         // three Maxwellian components from two Box-Muller pairs (both branches)
@@ -376,7 +439,8 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
     }
     const double md = S.mass[i];
     const double qd = S.charge[i];
-    const SpeciesDev sp = P.species[S.species[i]];
+    const uint32_t sidx = S.species[i];
+    const SpeciesDev sp = P.species[sidx];
     const float m = (float)md;
     const float qm = (float)(qd / md);
     const float inv_m_plus = (float)(1.0 / (md + P.m_gas));
@@ -468,8 +532,8 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
             PhiloxStream rng(P.seed, i, P.step0 + k);
             f3 v{hi[3], hi[4], hi[5]};
             const f3 vin = v;
-            const bool hit = collide_f32<COLL>(P, C, sp, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng);
-            if (COLL == COLL_HSS) {
+            const bool hit = collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng, dt);
+            if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
                 if (hit) {
                     // a collision replaces the velocity: restart its compensation
                     hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
@@ -906,6 +970,7 @@ template <int I, int F>
 KernelFn pick_coll(int coll, int precision) {
     switch (coll) {
         case COLL_HSS: return pick_prec<I, F, COLL_HSS>(precision);
+        case COLL_HSS_MIX: return pick_prec<I, F, COLL_HSS_MIX>(precision);
         case COLL_EHSS: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS>;
         case COLL_DAMPING: return pick_prec<I, F, COLL_DAMPING>(precision);
         default: return pick_prec<I, F, COLL_NONE>(precision);

@@ -20,7 +20,8 @@ enum CollKind : int {
     COLL_NONE = 0,
     COLL_HSS = 1,
     COLL_EHSS = 2,
-    COLL_DAMPING = 3     // Langevin or Friction damping force (+ OU kicks)
+    COLL_DAMPING = 3,    // Langevin or Friction damping force (+ OU kicks)
+    COLL_HSS_MIX = 4     // HSS over a gas mixture
 };
 
 // Per-species record on the device.
@@ -36,6 +37,8 @@ struct SpeciesDev {
 
 // Field-array terms a domain may superpose (CompositeFieldProvider).
 constexpr int kMaxFieldTerms = 4;
+// Components of an HSS gas mixture.
+constexpr int kMaxGasComponents = 4;
 
 // Per-step uniform values at the step start, mid point and end: for analytic
 // fields the RF coefficient c = U_eff(t)/r0^2; for a field array of several
@@ -141,6 +144,13 @@ struct StepParams {
     // per term: scale = a (constant kinds) or a cos(w t + phi) (RF kind)
     int term_rf[kMaxFieldTerms];
     double term_a[kMaxFieldTerms], term_w[kMaxFieldTerms], term_phi[kMaxFieldTerms];
+    // HSS gas mixture (n_mix = 0: the single gas above): the participating
+    // components with n > 0, in order; sigma of species s with component c at
+    // mix_sigma[s * n_mix + c]
+    int n_mix, mix_pad;
+    double mix_n[kMaxGasComponents], mix_m[kMaxGasComponents];
+    double mix_sig[kMaxGasComponents];   // sqrt(kB T / m_c)
+    const double *mix_sigma;
     double x0, y0, z0;
     double dx, dy, dz;
     double inv_dx, inv_dy, inv_dz;

@@ -23,6 +23,7 @@ EXPORTED = (
     "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
     "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
+    "icarion_gpu_set_gas_mixture",
 )
 
 
@@ -296,6 +297,19 @@ class Handle:
             self.h, nx, ny, nz, origin[0], spacing[0], origin[1], spacing[1], origin[2],
             spacing[2], E.ctypes.data_as(C.c_void_p), dtype)
         _check(rc, self.lib)
+
+    def set_gas_mixture(self, components, sigma):
+        """components: [(density_m3, mass_kg, participates), ...];
+        sigma: [n_species, n_components] cross sections (m^2).  An empty list
+        returns to the single-gas path."""
+        arr = (_abi.GasComponent * max(len(components), 1))()
+        for i, (n, m, part) in enumerate(components):
+            arr[i] = _abi.GasComponent(n, m, int(part), 0)
+        sig = np.ascontiguousarray(sigma if len(components) else np.zeros((0, 0)), dtype=np.float64)
+        n_species = sig.shape[0] if len(components) else 0
+        _check(self.lib.icarion_gpu_set_gas_mixture(
+            self.h, arr, len(components), sig.ctypes.data_as(C.POINTER(C.c_double)), n_species),
+            self.lib)
 
     def upload_field_terms(self, axes, terms):
         """axes: (xs, ys, zs) node coordinates (non-decreasing, possibly

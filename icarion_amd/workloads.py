@@ -154,6 +154,29 @@ def ims_hss_workload(n=10_000, n_steps=100_000, seed=123, precision=_abi.PRECISI
                 ensemble=ens, dt=1e-10, n_steps=n_steps)
 
 
+def ims_mixture_workload(n=4096, seed=31, precision=_abi.PRECISION_FP64,
+                         integrator=_abi.INTEGRATOR_RK4):
+    """The IMS drift of ims_hss_workload in a gas mixture at 1 atm / 300 K:
+    N2 78 %, He 20 %, Ar 2 % and a trace of CO2 that does not take part in
+    collisions; two ion species (100 Da / 100 A^2 in N2, 250 Da / 180 A^2) with
+    per-gas cross sections.  Returns the workload dict with "mixture":
+    (components [(n_i, m_i, participates)], sigma[n_species, n_components])."""
+    dom = ims_domain(detector_z=0.5e-3)
+    n_tot = dom.gas_density_m3
+    comps = [(0.78 * n_tot, 28.0134 * AMU_TO_KG, 1), (0.20 * n_tot, 4.0026 * AMU_TO_KG, 1),
+             (0.02 * n_tot, 39.948 * AMU_TO_KG, 1), (4e-4 * n_tot, 44.01 * AMU_TO_KG, 0)]
+    sigma = np.array([[100e-20, 62e-20, 105e-20, 120e-20],
+                      [180e-20, 120e-20, 190e-20, 210e-20]])
+    phys = physics(integrator, _abi.COLLISION_HSS, precision=precision, seed=seed)
+    sp = [species(sigma_hss=100e-20, ccs=100e-20), species(sigma_hss=180e-20, ccs=180e-20)]
+    rng = np.random.default_rng(seed + 1)
+    sidx = rng.integers(0, 2, n).astype(np.uint32)
+    ens = gaussian_cloud(n, np.array([100.0, 250.0])[sidx], 0.5e-3, seed=seed)
+    ens.species[:] = sidx
+    return dict(name="ims_hss_mixture", domain=dom, physics=phys, species=sp, atoms=None,
+                ensemble=ens, dt=1e-10, n_steps=2000, mixture=(comps, sigma))
+
+
 def quadrupole_workload(n=1_000_000, n_steps=100_000, U=0.0, seed=123,
                         precision=_abi.PRECISION_FP64, ou=1):
     """RF quadrupole, Boris, Langevin: 1M ions, masses log-uniform 100-2000 Da,

@@ -188,6 +188,16 @@ typedef struct icarion_species_desc {
     uint32_t reserved0;
 } icarion_species_desc;
 
+/* One component of a gas mixture (EnvironmentConfig gas_mixture entry). */
+typedef struct icarion_gas_component {
+    double density_m3;            /* n_i                                       */
+    double mass_kg;               /* neutral mass                              */
+    int32_t participates;         /* participates_in_collisions                */
+    int32_t reserved0;
+} icarion_gas_component;
+
+#define ICARION_MAX_GAS_COMPONENTS 4
+
 /* Surrogate training options (Adam on mean 0.5*|y - t|^2). */
 typedef struct icarion_train_desc {
     uint32_t epochs;
@@ -276,6 +286,21 @@ int icarion_gpu_set_domain(icarion_gpu *h, const icarion_domain_desc *domain,
 int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
                             uint32_t n_species, const void *atoms_xyzr,
                             uint32_t n_atoms, int32_t atoms_dtype);
+
+/* HSS over a gas mixture: HSSCollisionHandler's mixture path
+ * (HSSCollisionHandler.cpp:105-249) - per participating component with
+ * n_i > 0 a Maxwellian neutral at the domain temperature and flow, rate
+ * k_i = n_i sigma_i |v - v_i| (skipped below the minimum relative speed), one
+ * collision test on k_total, the partner drawn with weight k_i, an HSS
+ * collision with that neutral.  sigma_m2[s * n_components + c] is species s's
+ * cross section with component c, resolved by the caller with the handler's
+ * priority (species-DB CCS_HSS for that gas > derived from the reference
+ * gas > the component's cross_section_m2 > the ion CCS); it must be > 0 for
+ * every component that participates with n_i > 0 (the reference throws
+ * otherwise).  n_species must equal that of icarion_gpu_set_species.
+ * n_components = 0 returns to the single-gas path.  Affects HSS only. */
+int icarion_gpu_set_gas_mixture(icarion_gpu *h, const icarion_gas_component *components,
+                                uint32_t n_components, const double *sigma_m2, uint32_t n_species);
 
 /* Trilinear E field on a uniform node-centred grid, x fastest
  * (index i + nx*(j + ny*k), fieldArrayLoader.cpp:9-11).  E holds 3*nx*ny*nz

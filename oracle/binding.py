@@ -71,8 +71,11 @@ def load():
     lib.oracle_run.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), P(_abi.SpeciesDesc),
                                C.c_uint32, P(d), C.c_uint32, P(_abi.OracleField), P(_abi.IonSoA),
                                d, d, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
-                               P(_abi.StepStats)]
+                               P(_abi.StepStats), P(_abi.OracleGasMixture)]
     lib.oracle_run.restype = C.c_int
+    lib.oracle_mt_hss_mixture.argtypes = [C.c_uint64, P(_abi.DomainDesc),
+                                          P(_abi.OracleGasMixture), P(d), d, P(d), d, P(d)]
+    lib.oracle_mt_hss_mixture.restype = C.c_int
     u32p, f32p = P(C.c_uint32), P(C.c_float)
     lib.oracle_ehss_precompute.argtypes = [f32p, u32p, u32p, C.c_uint32, d, C.c_uint32,
                                            C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, P(d),
@@ -94,7 +97,8 @@ def load():
     lib.oracle_run_rk45.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), R,
                                     P(_abi.SpeciesDesc), C.c_uint32, P(d), C.c_uint32,
                                     P(_abi.OracleField), P(_abi.IonSoA), C.c_void_p, d, d,
-                                    C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats)]
+                                    C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats),
+                                    P(_abi.OracleGasMixture)]
     lib.oracle_run_rk45.restype = C.c_int
     _lib = lib
     return lib
@@ -163,8 +167,39 @@ def species_array(species):
     return arr
 
 
+class GasMixture:
+    """oracle_gas_mixture: components [(density_m3, mass_kg, participates), ...]
+    and sigma[n_species, n_components] (m^2)."""
+
+    def __init__(self, components, sigma):
+        self.comps = (_abi.GasComponent * max(len(components), 1))()
+        for i, (n, m, part) in enumerate(components):
+            self.comps[i] = _abi.GasComponent(n, m, int(part), 0)
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        n_species = self.sigma.shape[0] if len(components) else 0
+        self.c = _abi.OracleGasMixture(len(components), n_species,
+                                       C.cast(self.comps, C.POINTER(_abi.GasComponent)),
+                                       _dp(self.sigma))
+
+
+def _mix_ptr(mixture):
+    return C.byref(mixture.c) if mixture is not None else C.POINTER(_abi.OracleGasMixture)()
+
+
+def mt_hss_mixture(seed, domain, mixture, sigma_row, mass, vel, dt):
+    """One mixture collision stage on a fresh PhysicsRng(seed): returns
+    (selected component or -1, velocity after, selected neutral's velocity)."""
+    lib = load()
+    v = vec(*vel)
+    vn = (C.c_double * 3)()
+    row = np.ascontiguousarray(sigma_row, dtype=np.float64)
+    sel = lib.oracle_mt_hss_mixture(seed, C.byref(domain), C.byref(mixture.c), _dp(row), mass, v,
+                                    dt, vn)
+    return sel, np.array(v[:]), np.array(vn[:])
+
+
 def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=None, field=None,
-        rng_kind=RNG_PHILOX, threads=0):
+        rng_kind=RNG_PHILOX, threads=0, mixture=None):
     """Run the oracle engine loop on ens in place; returns StepStats."""
     lib = load()
     sp = species_array(species)
@@ -178,7 +213,7 @@ def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=Non
     st = _abi.StepStats()
     rc = lib.oracle_run(C.byref(domain), C.byref(physics), sp, len(species), ap, na, fptr,
                         C.byref(soa), t0, dt, n_steps, step_offset, rng_kind, threads,
-                        C.byref(st))
+                        C.byref(st), _mix_ptr(mixture))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed ({rc})")
     return st
@@ -324,7 +359,7 @@ def rk45_push_one(domain, physics, cfg, sp, mass, charge, pos, vel, t, dt, state
 
 
 def run_rk45(domain, physics, cfg, species, ens, t0, dt, n_steps, step_offset=0, atoms=None,
-             field=None, state=None, threads=0):
+             field=None, state=None, threads=0, mixture=None):
     """The RK45 engine loop on ens in place; returns (StepStats, state).  Pass
     the returned state to the next call to continue a run."""
     lib = load()
@@ -340,7 +375,7 @@ def run_rk45(domain, physics, cfg, species, ens, t0, dt, n_steps, step_offset=0,
     stats = _abi.StepStats()
     rc = lib.oracle_run_rk45(C.byref(domain), C.byref(physics), C.byref(cfg), sp, len(species),
                              ap, na, fptr, C.byref(soa), st.ctypes.data, t0, dt, n_steps,
-                             step_offset, threads, C.byref(stats))
+                             step_offset, threads, C.byref(stats), _mix_ptr(mixture))
     if rc != 0:
         raise RuntimeError(f"oracle_run_rk45 failed ({rc})")
     return stats, st

@@ -775,8 +775,60 @@ V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
     return v_ion;
 }
 
+// HSSCollisionHandler::handle_collision mixture path (HSSCollisionHandler.cpp:105-249):
+// a neutral per participating component with n_i > 0, rates from the actual
+// relative speeds, one test on the total rate, the partner drawn with weight
+// k_i, then hss_collision with the neutral sampled for it.  Returns the
+// selected component index, or -1.
+template <class Rng>
+int collide_mixture(const icarion_domain_desc& d, const oracle_gas_mixture& mix,
+                    const double* sigma_row, double m_ion, V3& vel, double dt, Rng& rng,
+                    V3* vn_out = nullptr) {
+    struct Data {
+        uint32_t comp;
+        double k;
+        V3 vn;
+    };
+    Data data[ICARION_MAX_GAS_COMPONENTS];
+    int nd = 0;
+    double k_total = 0.0;
+    const V3 flow{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    for (uint32_t i = 0; i < mix.n_components && i < ICARION_MAX_GAS_COMPONENTS; ++i) {
+        const icarion_gas_component& c = mix.components[i];
+        if (!c.participates) continue;
+        const double n_i = c.density_m3;
+        if (n_i <= 0.0) continue;
+        const double sigma_i = sigma_row[i];
+        const V3 v_neutral = sample_neutral_velocity(d.temperature_K, c.mass_kg, flow, rng);
+        const V3 v_rel = vel - v_neutral;
+        const double v_rel_mag = norm(v_rel);
+        if (v_rel_mag < kMinRelativeVelocity) continue;
+        const double k_i = n_i * sigma_i * v_rel_mag;
+        data[nd++] = Data{i, k_i, v_neutral};
+        k_total += k_i;
+    }
+    if (k_total <= 0.0 || nd == 0) return -1;
+    double P_total = 1.0;
+    if (k_total * dt <= 50.0) P_total = 1.0 - std::exp(-k_total * dt);
+    if (rng.uniform01() >= P_total) return -1;
+    const double r = rng.uniform01() * k_total;
+    int sel = 0;
+    double cum = 0.0;
+    for (int j = 0; j < nd; ++j) {
+        cum += data[j].k;
+        if (r < cum) {
+            sel = j;
+            break;
+        }
+    }
+    if (vn_out) *vn_out = data[sel].vn;
+    vel = hss_collision(vel, data[sel].vn, m_ion, mix.components[data[sel].comp].mass_kg, rng);
+    return static_cast<int>(data[sel].comp);
+}
+
 // One collision stage of one ion.  HSS: HSSCollisionHandler::handle_collision
-// single-gas path (HSSCollisionHandler.cpp:257-376).  EHSS:
+// single-gas path (HSSCollisionHandler.cpp:257-376), or the mixture path when
+// a mixture is given.  EHSS:
 // EHSSCollisionHandler::handle_collision single-gas online path
 // (EHSSCollisionHandler.cpp:222-374, 426-455).  Langevin / Friction with
 // ou_thermal_kicks: CollisionKernels::ou_velocity_update with apply_damping =
@@ -785,9 +837,12 @@ V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
 template <class Rng>
 bool collide(const icarion_domain_desc& d, const icarion_physics_desc& ph,
              const icarion_species_desc& sp, const double* atoms, double m_ion, double q,
-             V3& vel, double dt, Rng& rng, std::vector<V3>& scratch) {
+             V3& vel, double dt, Rng& rng, std::vector<V3>& scratch,
+             const oracle_gas_mixture* mix = nullptr, const double* mix_sigma_row = nullptr) {
     const int model = ph.collision_model;
     const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    if (model == ICARION_COLLISION_HSS && mix && mix->n_components > 0)
+        return collide_mixture(d, *mix, mix_sigma_row, m_ion, vel, dt, rng) >= 0;
     if (model == ICARION_COLLISION_HSS || model == ICARION_COLLISION_EHSS) {
         const double n = d.gas_density_m3;
         const double T_K = d.temperature_K;
@@ -842,7 +897,8 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
                  const icarion_species_desc& sp, const double* atoms, const oracle_field* f,
                  IonCtx& ion, const double* t_table, double dt, uint64_t n_steps,
                  uint32_t ion_index, uint64_t step0, MtRng* mt, uint64_t& collisions,
-                 std::vector<V3>& scratch) {
+                 std::vector<V3>& scratch, const oracle_gas_mixture* mix,
+                 const double* mix_row) {
     uint64_t pushed = 0;
     const bool detect = std::isfinite(d.detector_z_m);
     for (uint64_t k = 0; k < n_steps; ++k) {
@@ -869,10 +925,10 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
         if (ph.collision_model != ICARION_COLLISION_NONE) {
             bool hit;
             if (mt) {
-                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, *mt, scratch);
+                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, *mt, scratch, mix, mix_row);
             } else {
                 PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
-                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch);
+                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, mix, mix_row);
             }
             if (hit) {
                 ++ion.ncoll;
@@ -959,6 +1015,20 @@ void oracle_mt_hss_collision(uint64_t seed, const double v_ion[3], const double 
     out[0] = v.x; out[1] = v.y; out[2] = v.z;
 }
 
+int oracle_mt_hss_mixture(uint64_t seed, const icarion_domain_desc* domain,
+                          const oracle_gas_mixture* mixture, const double* sigma_row, double mass_kg,
+                          double vel[3], double dt, double vn_out[3]) {
+    MtRng rng(seed);
+    V3 v{vel[0], vel[1], vel[2]};
+    V3 vn{0.0, 0.0, 0.0};
+    const int sel = collide_mixture(*domain, *mixture, sigma_row, mass_kg, v, dt, rng, &vn);
+    vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
+    if (vn_out) {
+        vn_out[0] = vn.x; vn_out[1] = vn.y; vn_out[2] = vn.z;
+    }
+    return sel;
+}
+
 void oracle_mt_ehss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],
                               double ion_mass, double neutral_mass, double ion_radius,
                               const double* atoms_xyzr, uint32_t n_atoms, double out[3]) {
@@ -1007,9 +1077,10 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
                const icarion_species_desc* species, uint32_t n_species, const double* atoms,
                uint32_t n_atoms, const oracle_field* field, icarion_ion_soa* ions, double t0,
                double dt, uint64_t n_steps, uint64_t step_offset, int rng_kind, int n_threads,
-               icarion_step_stats* stats) {
+               icarion_step_stats* stats, const oracle_gas_mixture* mixture) {
     (void)n_atoms;
     if (!domain || !physics || !species || !ions || n_species == 0) return ICARION_ERR_INVALID;
+    if (mixture && mixture->n_components > 0 && mixture->n_species != n_species) return ICARION_ERR_INVALID;
     const size_t n = ions->n;
     // Engine time t_k: the reference advances every live ion by t_i += dt and
     // takes the engine time as max_i t_i (SimulationEngineLifecycle.cpp:172,
@@ -1047,15 +1118,19 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
             ion.ncoll = ions->n_collisions ? ions->n_collisions[i] : 0;
             uint64_t coll = 0;
             uint64_t pushed = 0;
+            const double* mix_row = (mixture && mixture->n_components > 0)
+                                        ? mixture->sigma_m2 + static_cast<size_t>(s < n_species ? s : 0) *
+                                                                  mixture->n_components
+                                        : nullptr;
             if (rng_kind == ORACLE_RNG_MT19937_64) {
                 MtRng rng(physics->rng_seed + static_cast<uint64_t>(i));
                 pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
                                  n_steps, static_cast<uint32_t>(i), step_offset, &rng, coll,
-                                 scratch);
+                                 scratch, mixture, mix_row);
             } else {
                 pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
                                  n_steps, static_cast<uint32_t>(i), step_offset, nullptr, coll,
-                                 scratch);
+                                 scratch, mixture, mix_row);
             }
             ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
             ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;
@@ -1113,10 +1188,11 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                     uint32_t n_species, const double* atoms, uint32_t n_atoms,
                     const oracle_field* field, icarion_ion_soa* ions, oracle_rk45_ion* rk45,
                     double t0, double dt, uint64_t n_steps, uint64_t step_offset, int n_threads,
-                    icarion_step_stats* stats) {
+                    icarion_step_stats* stats, const oracle_gas_mixture* mixture) {
     (void)n_atoms;
     if (!domain || !physics || !cfg || !species || !ions || !rk45 || n_species == 0)
         return ICARION_ERR_INVALID;
+    if (mixture && mixture->n_components > 0 && mixture->n_species != n_species) return ICARION_ERR_INVALID;
     const icarion_domain_desc& d = *domain;
     const icarion_physics_desc& ph = *physics;
     const size_t n = ions->n;
@@ -1181,8 +1257,12 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                 // perform_collisions with the accepted dt
                 if (ph.collision_model != ICARION_COLLISION_NONE) {
                     PhiloxStream rng(ph.rng_seed, static_cast<uint32_t>(i), step_offset + k);
+                    const double* mix_row =
+                        (mixture && mixture->n_components > 0)
+                            ? mixture->sigma_m2 + static_cast<size_t>(s < n_species ? s : 0) * mixture->n_components
+                            : nullptr;
                     if (collide(d, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, used,
-                                rng, scratch)) {
+                                rng, scratch, mixture, mix_row)) {
                         if (ions->n_collisions) ++ions->n_collisions[i];
                         ++coll_total;
                     }

@@ -51,6 +51,15 @@ typedef struct oracle_field {
 double oracle_field_term_scale(const icarion_domain_desc *domain, const oracle_field_term *term,
                                double t);
 
+/* Gas mixture of the HSS mixture path: components and the per-species
+ * cross sections sigma_m2[s * n_components + c]. */
+typedef struct oracle_gas_mixture {
+    uint32_t n_components;
+    uint32_t n_species;
+    const icarion_gas_component *components;
+    const double *sigma_m2;
+} oracle_gas_mixture;
+
 enum { ORACLE_RNG_PHILOX = 0, ORACLE_RNG_MT19937_64 = 1 };
 
 /* Philox4x32-10 block function (Salmon et al., SC'11). */
@@ -113,7 +122,7 @@ int oracle_run(const icarion_domain_desc *domain, const icarion_physics_desc *ph
                const double *atoms_xyzr, uint32_t n_atoms, const oracle_field *field,
                icarion_ion_soa *ions, double t0, double dt, uint64_t n_steps,
                uint64_t step_offset, int rng_kind, int n_threads,
-               icarion_step_stats *stats);
+               icarion_step_stats *stats, const oracle_gas_mixture *mixture);
 
 /* ---- RK45 (RK45Strategy.cpp:101-520) -------------------------------------
  * Per-ion adaptive state carried between engine steps (RK45Strategy::RK45State,
@@ -159,7 +168,16 @@ int oracle_run_rk45(const icarion_domain_desc *domain, const icarion_physics_des
                     uint32_t n_species, const double *atoms_xyzr, uint32_t n_atoms,
                     const oracle_field *field, icarion_ion_soa *ions, oracle_rk45_ion *rk45,
                     double t0, double dt, uint64_t n_steps, uint64_t step_offset, int n_threads,
-                    icarion_step_stats *stats);
+                    icarion_step_stats *stats, const oracle_gas_mixture *mixture);
+
+/* One mixture collision stage (HSSCollisionHandler.cpp:105-249) of an ion of
+ * mass m with velocity vel (in/out) on a PhysicsRng seeded with seed
+ * (mt19937_64).  sigma_row: the ion's cross sections per component.  Returns
+ * the selected component (or -1 without a collision); vn_out (may be NULL)
+ * receives the selected neutral's velocity. */
+int oracle_mt_hss_mixture(uint64_t seed, const icarion_domain_desc *domain,
+                          const oracle_gas_mixture *mixture, const double *sigma_row, double mass_kg,
+                          double vel[3], double dt, double vn_out[3]);
 
 /* ---- EHSS offline precompute (tools/ehss_offline_precompute.cpp:438-517) ----
  * Per (cluster c, orientation k): Shoemake orientation, b_max = max(r_xy + r) +

@@ -63,6 +63,7 @@ STRUCTS = {
     "icarion_cloud_stats": (_abi.CloudStats, ["ion_steps", "arrived", "t_end_s", "done_ms"]),
     "icarion_field_term_desc": (_abi.FieldTermDesc, ["E", "dtype", "scale_kind", "constant_scale",
                                                      "phase_rad"]),
+    "icarion_gas_component": (_abi.GasComponent, ["density_m3", "mass_kg", "participates"]),
 }
 
 

@@ -37,6 +37,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
             h.upload_field_array(E, origin, spacing)
         if w.get("terms") is not None:
             h.upload_field_terms(w["axes"], w["terms"])
+        if w.get("mixture") is not None:
+            h.set_gas_mixture(*w["mixture"])
         h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -49,8 +51,9 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
         field = oracle.Field(*w["field"])
     if w.get("terms") is not None:
         field = oracle.FieldTerms(w["axes"], w["terms"])
+    mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
-                      step_offset=step_offset, atoms=w.get("atoms"), field=field)
+                      step_offset=step_offset, atoms=w.get("atoms"), field=field, mixture=mixture)
 
 
 def rel_close(a, b, rtol, scale):
@@ -260,6 +263,63 @@ def test_field_terms_reject_bad_input():
         with pytest.raises(gpu.IcarionError) as ei:
             h.upload_field_terms((xs, ys, zs), bad_kind)
         assert ei.value.code == _abi.ERR_INVALID
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_hss_mixture_fp64_ion_by_ion(oracle, integrator):
+    """HSS over a four-component gas mixture (one component outside the
+    collisions), two ion species with per-gas cross sections."""
+    w = W.ims_mixture_workload(n=2048, seed=31, integrator=integrator)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 2000)
+    so = oracle_run(oracle, w, o, 2000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    assert so.collisions > 0.3 * so.ion_steps
+
+
+def test_hss_mixture_fp32_statistics(oracle):
+    w = W.ims_mixture_workload(n=8192, seed=32)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    z0 = w["ensemble"].z.copy()
+    sg = gpu_run(w, g, 4000, precision=_abi.PRECISION_FP32_KAHAN)
+    so = oracle_run(oracle, w, o, 4000)
+    assert sg.collisions == pytest.approx(so.collisions, rel=5e-3)
+    for s in (0, 1):
+        a = w["ensemble"].species == s
+        dz_g, dz_o = (g.z - z0)[a], (o.z - z0)[a]
+        se = math.sqrt(dz_g.var() / a.sum() + dz_o.var() / a.sum())
+        assert abs(dz_g.mean() - dz_o.mean()) < 4 * se + 0.01 * abs(dz_o.mean())
+
+
+def test_hss_mixture_controls():
+    """Mixture set-up errors, and the empty mixture (no component taking part)
+    has no collisions; clearing it returns to the single gas."""
+    from icarion_amd import gpu
+    w = W.ims_mixture_workload(n=512, seed=33)
+    comps, sigma = w["mixture"]
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"])
+        bad = sigma.copy()
+        bad[1, 2] = 0.0
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_gas_mixture(comps, bad)
+        assert ei.value.code == _abi.ERR_INVALID
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_gas_mixture(comps + comps[:1], np.concatenate([sigma, sigma[:, :1]], 1))
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_gas_mixture(comps, sigma[:1])
+        assert ei.value.code == _abi.ERR_INVALID
+        h.set_gas_mixture([(c[0], c[1], 0) for c in comps], sigma)
+        h.upload_state(w["ensemble"].copy())
+        st = h.step(0.0, w["dt"], 200)
+        assert st.collisions == 0 and st.ion_steps == 200 * 512
+        h.set_gas_mixture([], None)
+        h.upload_state(w["ensemble"].copy())
+        st = h.step(0.0, w["dt"], 200)
+        assert st.collisions > 0
 
 
 def test_birth_times_fp64(oracle):

@@ -49,6 +49,8 @@ def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
             h.upload_field_array(E, origin, spacing)
         if w.get("terms") is not None:
             h.upload_field_terms(w["axes"], w["terms"])
+        if w.get("mixture") is not None:
+            h.set_gas_mixture(*w["mixture"])
         h.upload_state(ens)
         st = h.step(t0, dt, n_steps, step_offset)
         h.download_state(ens)
@@ -60,8 +62,9 @@ def oracle_rk45(oracle, w, ph, ens, n_steps, dt, t0=0.0, cfg=None):
     field = oracle.Field(*w["field"]) if w.get("field") is not None else None
     if w.get("terms") is not None:
         field = oracle.FieldTerms(w["axes"], w["terms"])
+    mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
     st, _ = oracle.run_rk45(w["domain"], ph, cfg, w["species"], ens, t0, dt, n_steps,
-                            atoms=w.get("atoms"), field=field)
+                            atoms=w.get("atoms"), field=field, mixture=mixture)
     return st
 
 
@@ -108,11 +111,14 @@ def case_workload(case):
     if case == "field_terms":
         w = W.field_terms_workload(n=2048, seed=8)
         return w, rk45_physics(w), w["dt"]
+    if case == "hss_mixture":
+        w = W.ims_mixture_workload(n=2048, seed=34)
+        return w, rk45_physics(w), w["dt"]
     w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
     return w, rk45_physics(w), w["dt"]
 
 
-CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss", "field_terms"]
+CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss", "field_terms", "hss_mixture"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -132,7 +138,7 @@ def test_rk45_constant_dt_chain_ion_by_ion(oracle, case):
     assert sg.rejected == 0 and so.rejected == 0
     assert sg.ion_steps == pytest.approx(so.ion_steps, rel=1e-3)
     assert sg.t_end_s == pytest.approx(so.t_end_s, rel=1e-12)
-    need = 0.99 if case == "field_array_hss" else 0.999
+    need = 0.99 if case in ("field_array_hss", "hss_mixture") else 0.999
     assert match(g, o, rtol=1e-9, dt_rtol=1e-12).mean() >= need
 
 

@@ -531,6 +531,57 @@ def test_hss_thermalisation_and_isotropy(oracle):
     np.testing.assert_allclose(frac, [1 / 3] * 3, atol=0.02)
 
 
+def test_hss_mixture_rates_partner_choice_and_kinematics(oracle):
+    """HSSCollisionHandler's mixture path (HSSCollisionHandler.cpp:105-249) in
+    the oracle, against an independent Monte-Carlo evaluation of its law: for a
+    fast ion, the collision probability is E[1 - exp(-dt sum_i n_i sigma_i
+    |v - v_i|)] over independent Maxwellian neutrals, the partner is i with
+    probability E[P k_i / k_total] / E[P]; components that do not take part or
+    have no density never collide; the scattered velocity is an HSS collision
+    with the selected partner's mass and sampled velocity."""
+    from icarion_amd.constants import AMU_TO_KG, BOLTZMANN
+    d = W.ims_domain(EN_Td=10.0)
+    n_tot = d.gas_density_m3
+    masses = np.array([28.0, 4.0, 40.0, 44.0, 20.0]) * AMU_TO_KG
+    dens = np.array([0.7, 0.25, 0.05, 0.3, 0.0]) * n_tot
+    part = [1, 1, 1, 0, 1]
+    sig_row = np.array([100e-20, 60e-20, 110e-20, 500e-20, 500e-20])
+    mix = oracle.GasMixture([(n, m, p) for n, m, p in zip(dens, masses, part)], sig_row[None, :])
+    m_ion = 150.0 * AMU_TO_KG
+    v0 = np.array([3000.0, 400.0, -200.0])
+    dt = 5e-12
+    n = 20000
+    counts = np.zeros(5, int)
+    for seed in range(n):
+        sel, v1, vn = oracle.mt_hss_mixture(seed + 1, d, mix, sig_row, m_ion, v0, dt)
+        if sel < 0:
+            np.testing.assert_array_equal(v1, v0)
+            continue
+        counts[sel] += 1
+        ms = masses[sel]
+        Vcom = (m_ion * v0 + ms * vn) / (m_ion + ms)
+        assert np.linalg.norm(v1 - Vcom) == pytest.approx(
+            np.linalg.norm(v0 - vn) * ms / (m_ion + ms), rel=1e-10)
+    assert counts[3] == 0 and counts[4] == 0
+    # the law, by Monte Carlo in numpy
+    rng = np.random.default_rng(0)
+    N = 2_000_000
+    k = np.zeros((N, 3))
+    for c in range(3):
+        s = math.sqrt(BOLTZMANN * d.temperature_K / masses[c])
+        vn = rng.normal(0.0, s, (N, 3))
+        k[:, c] = dens[c] * sig_row[c] * np.linalg.norm(v0 - vn, axis=1)
+    kt = k.sum(1)
+    P = 1.0 - np.exp(-kt * dt)
+    p_coll = P.mean()
+    p_sel = (P[:, None] * k / kt[:, None]).mean(0) / p_coll
+    frac = counts.sum() / n
+    assert abs(frac - p_coll) < 4 * math.sqrt(p_coll * (1 - p_coll) / n) + 1e-3
+    for c in range(3):
+        f = counts[c] / counts.sum()
+        assert abs(f - p_sel[c]) < 4 * math.sqrt(p_sel[c] * (1 - p_sel[c]) / counts.sum()) + 1e-3
+
+
 def test_langevin_gamma_matches_gioumousis_stevenson(oracle):
     import ctypes as C
     lib = oracle.load()
