@@ -137,6 +137,16 @@ class OracleGasMixture(C.Structure):
                 ("components", C.POINTER(GasComponent)), ("sigma_m2", C.POINTER(_d))]
 
 
+class OracleEhssOffline(C.Structure):
+    _fields_ = [("n_species", _u32), ("reserved0", _u32), ("n_orient", C.POINTER(_u32)),
+                ("mu_stride", C.POINTER(_u32)), ("sigma_eff", C.POINTER(_d)), ("mu", C.POINTER(_d))]
+
+
+class OracleCollisionTables(C.Structure):
+    _fields_ = [("mixture", C.POINTER(OracleGasMixture)),
+                ("ehss_offline", C.POINTER(OracleEhssOffline))]
+
+
 class OracleFieldTerm(C.Structure):
     _fields_ = [
         ("scale_kind", _i32), ("reserved0", _i32), ("constant_scale", _d), ("frequency_Hz", _d),

@@ -169,6 +169,11 @@ struct icarion_gpu {
     std::vector<icarion_gas_component> mix;
     uint32_t mix_species = 0;
     DevBuf<double> mix_sigma;
+    // offline EHSS sample sets (per species {first orientation, orientations,
+    // mu per orientation, first mu}; none when off_species == 0)
+    uint32_t off_species = 0;
+    DevBuf<uint4> off_tab;
+    DevBuf<double> off_sigma, off_mu;
 
     size_t n = 0;
     bool have_state = false;
@@ -259,7 +264,9 @@ int coll_kind(const icarion_physics_desc& p) {
 // has its own.
 int launch_coll_kind(const icarion_gpu* h) {
     const int c = coll_kind(h->phys);
-    return (c == COLL_HSS && !h->mix.empty()) ? COLL_HSS_MIX : c;
+    if (c == COLL_HSS && !h->mix.empty()) return COLL_HSS_MIX;
+    if (c == COLL_EHSS && h->off_species && h->phys.ehss_mode == ICARION_EHSS_GEOMETRY) return COLL_EHSS_TAB;
+    return c;
 }
 
 void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p) {
@@ -429,6 +436,16 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
     P.mix_sigma = h->mix.empty() ? nullptr : h->mix_sigma.p;
     if (!h->mix.empty() && h->mix_species != h->n_species)
         fail(ICARION_ERR_STATE, "the gas mixture's species count differs from the species table");
+    P.ehss_off = nullptr;
+    P.ehss_off_sigma = nullptr;
+    P.ehss_off_mu = nullptr;
+    if (h->off_species) {
+        if (h->off_species != h->n_species)
+            fail(ICARION_ERR_STATE, "the offline EHSS sets' species count differs from the species table");
+        P.ehss_off = h->off_tab.p;
+        P.ehss_off_sigma = h->off_sigma.p;
+        P.ehss_off_mu = h->off_mu.p;
+    }
     for (int c = 0; c < kMaxGasComponents; ++c) {
         const bool on = c < P.n_mix;
         P.mix_n[c] = on ? h->mix[c].density_m3 : 0.0;
@@ -1341,6 +1358,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     for (auto& a : h->ax64) a.release();
     for (auto& a : h->ax32) a.release();
     h->mix_sigma.release();
+    h->off_tab.release(); h->off_sigma.release(); h->off_mu.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
     h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
     h->b_params_d.release(); h->b_ions_d.release(); h->b_kinds_d.release(); h->b_queue.release();
@@ -1559,6 +1577,47 @@ int icarion_gpu_set_gas_mixture(icarion_gpu* h, const icarion_gas_component* com
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         h->mix = std::move(mix);
         h->mix_species = n_species;
+    });
+}
+
+int icarion_gpu_set_ehss_offline(icarion_gpu* h, const uint32_t* n_orient, const uint32_t* mu_stride,
+                                 const double* sigma_eff, const double* mu, uint32_t n_species) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (n_species == 0) {
+            h->off_species = 0;
+            h->off_tab.release();
+            h->off_sigma.release();
+            h->off_mu.release();
+            return;
+        }
+        if (!n_orient || !mu_stride) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_species || n_species != h->n_species)
+            fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
+        std::vector<uint4> tab(n_species);
+        uint64_t n_sig = 0, n_mu = 0;
+        for (uint32_t sp = 0; sp < n_species; ++sp) {
+            if (n_orient[sp] > 0 && mu_stride[sp] == 0)
+                fail(ICARION_ERR_INVALID, "species " + std::to_string(sp) + ": mu_stride must be > 0");
+            tab[sp] = make_uint4(static_cast<uint32_t>(n_sig), n_orient[sp], n_orient[sp] ? mu_stride[sp] : 0u,
+                                 static_cast<uint32_t>(n_mu));
+            n_sig += n_orient[sp];
+            n_mu += static_cast<uint64_t>(n_orient[sp]) * (n_orient[sp] ? mu_stride[sp] : 0u);
+            if (n_sig > 0xFFFFFFFFull || n_mu > 0xFFFFFFFFull)
+                fail(ICARION_ERR_UNSUPPORTED, "offline EHSS sets above 2^32 entries");
+        }
+        if (n_sig == 0) fail(ICARION_ERR_INVALID, "no species has an offline set");
+        if (!sigma_eff || !mu) fail(ICARION_ERR_INVALID, "NULL argument");
+        for (uint64_t i = 0; i < n_sig; ++i)
+            if (!std::isfinite(sigma_eff[i])) fail(ICARION_ERR_INVALID, "sigma_eff values must be finite");
+        for (uint64_t i = 0; i < n_mu; ++i)
+            if (!std::isfinite(mu[i])) fail(ICARION_ERR_INVALID, "mu values must be finite");
+        set_device(h);
+        copy_in(h->off_tab, tab.data(), tab.size(), h->stream, "upload offline EHSS index");
+        copy_in(h->off_sigma, sigma_eff, static_cast<size_t>(n_sig), h->stream, "upload offline sigma_eff");
+        copy_in(h->off_mu, mu, static_cast<size_t>(n_mu), h->stream, "upload offline mu");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        h->off_species = n_species;
     });
 }
 

@@ -283,6 +283,7 @@ Rk45Fn pick_rk45_coll(int coll) {
         case COLL_HSS: return rk45_step_kernel<FIELD, COLL_HSS>;
         case COLL_HSS_MIX: return rk45_step_kernel<FIELD, COLL_HSS_MIX>;
         case COLL_EHSS: return rk45_step_kernel<FIELD, COLL_EHSS>;
+        case COLL_EHSS_TAB: return rk45_step_kernel<FIELD, COLL_EHSS_TAB>;
         case COLL_DAMPING: return rk45_step_kernel<FIELD, COLL_DAMPING>;
         default: return nullptr;
     }

@@ -384,8 +384,22 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
             return true;
         }
     }
-    if constexpr (COLL == COLL_HSS || COLL == COLL_EHSS) {
-        const double sigma = (COLL == COLL_HSS) ? sp.sigma_hss : sp.sigma_ehss;
+    if constexpr (COLL == COLL_HSS || COLL == COLL_EHSS || COLL == COLL_EHSS_TAB) {
+        double sigma = (COLL == COLL_HSS) ? sp.sigma_hss : sp.sigma_ehss;
+        // EHSS offline sample set of the species (EHSSCollisionHandler.cpp:151-172):
+        // the orientation is drawn first and its sigma_eff enters the test
+        uint4 off = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t oi = 0;
+        if constexpr (COLL == COLL_EHSS_TAB) {
+            if (P.ehss_off) {
+                off = P.ehss_off[sidx];
+                if (off.y > 0u) {
+                    oi = static_cast<uint32_t>(rng.uniform_f64() * static_cast<double>(off.y));
+                    if (oi >= off.y) oi = off.y - 1u;
+                    sigma = P.ehss_off_sigma[off.x + oi];
+                }
+            }
+        }
         if (sigma <= 0.0) return false;
         // VelocitySampling::sample_neutral_velocity (VelocitySampling.cpp:9-35)
         d3 vn;
@@ -415,6 +429,23 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
             const double inv_mt = 1.0 / (m + P.m_gas);
             const d3 Vcom = (v * m + vn * P.m_gas) * inv_mt;
             v = Vcom + (nd * vm) * (P.m_gas * inv_mt);
+        } else if (COLL == COLL_EHSS_TAB && off.y > 0u) {
+            // offline branch of apply_collision (EHSSCollisionHandler.cpp:253-277)
+            uint32_t mi = static_cast<uint32_t>(rng.uniform_f64() * static_cast<double>(off.z));
+            if (mi >= off.z) mi = off.z - 1u;
+            double mu = P.ehss_off_mu[static_cast<size_t>(off.w) + static_cast<size_t>(oi) * off.z + mi];
+            mu = fmin(1.0, fmax(-1.0, mu));
+            const double sin_theta = sqrt(fmax(0.0, 1.0 - mu * mu));
+            const double phi = 2.0 * kPi * rng.uniform_f64();
+            const d3 ax{vrel.x / vrel_mag, vrel.y / vrel_mag, vrel.z / vrel_mag};
+            d3 t1, t2;
+            ortho_basis_f64(ax, t1, t2);
+            const d3 nc = t1 * (sin_theta * cos(phi)) + t2 * (sin_theta * sin(phi)) + ax * (-mu);
+            const double vdn = vrel.x * nc.x + vrel.y * nc.y + vrel.z * nc.z;
+            const d3 vr = vrel - nc * (2.0 * vdn);
+            const double inv_mt = 1.0 / (m + P.m_gas);
+            const d3 Vcom = (v * m + vn * P.m_gas) * inv_mt;
+            v = Vcom + vr * (P.m_gas * inv_mt);
         } else {
             v = ehss_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, rng);
         }

@@ -21,7 +21,8 @@ enum CollKind : int {
     COLL_HSS = 1,
     COLL_EHSS = 2,
     COLL_DAMPING = 3,    // Langevin or Friction damping force (+ OU kicks)
-    COLL_HSS_MIX = 4     // HSS over a gas mixture
+    COLL_HSS_MIX = 4,    // HSS over a gas mixture
+    COLL_EHSS_TAB = 5    // EHSS with offline sample sets for some species
 };
 
 // Per-species record on the device.
@@ -151,6 +152,11 @@ struct StepParams {
     double mix_n[kMaxGasComponents], mix_m[kMaxGasComponents];
     double mix_sig[kMaxGasComponents];   // sqrt(kB T / m_c)
     const double *mix_sigma;
+    // offline EHSS sample sets per species (NULL: none): {first orientation,
+    // orientations (0: the online geometry), mu per orientation, first mu}
+    const uint4 *ehss_off;
+    const double *ehss_off_sigma;   // sigma_eff per orientation
+    const double *ehss_off_mu;      // mu samples, orientation-major
     double x0, y0, z0;
     double dx, dy, dz;
     double inv_dx, inv_dy, inv_dz;

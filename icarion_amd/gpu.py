@@ -23,7 +23,7 @@ EXPORTED = (
     "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
     "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
-    "icarion_gpu_set_gas_mixture",
+    "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline",
 )
 
 
@@ -310,6 +310,28 @@ class Handle:
         _check(self.lib.icarion_gpu_set_gas_mixture(
             self.h, arr, len(components), sig.ctypes.data_as(C.POINTER(C.c_double)), n_species),
             self.lib)
+
+    def set_ehss_offline(self, sets):
+        """sets[s]: None (species s keeps the online geometry) or
+        (sigma_eff[n_orient], mu[n_orient, mu_stride]); [] clears."""
+        n = len(sets)
+        n_orient = np.zeros(max(n, 1), np.uint32)
+        stride = np.zeros(max(n, 1), np.uint32)
+        sig, mus = [], []
+        for k, t in enumerate(sets):
+            if t is None:
+                continue
+            sg, mu = np.asarray(t[0], np.float64), np.asarray(t[1], np.float64)
+            n_orient[k], stride[k] = sg.shape[0], mu.shape[1]
+            sig.append(sg.ravel())
+            mus.append(mu.ravel())
+        sg = np.ascontiguousarray(np.concatenate(sig) if sig else np.zeros(1))
+        mu = np.ascontiguousarray(np.concatenate(mus) if mus else np.zeros(1))
+        u32 = C.POINTER(C.c_uint32)
+        d = C.POINTER(C.c_double)
+        _check(self.lib.icarion_gpu_set_ehss_offline(
+            self.h, n_orient.ctypes.data_as(u32), stride.ctypes.data_as(u32),
+            sg.ctypes.data_as(d), mu.ctypes.data_as(d), n), self.lib)
 
     def upload_field_terms(self, axes, terms):
         """axes: (xs, ys, zs) node coordinates (non-decreasing, possibly

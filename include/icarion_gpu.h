@@ -302,6 +302,19 @@ int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
 int icarion_gpu_set_gas_mixture(icarion_gpu *h, const icarion_gas_component *components,
                                 uint32_t n_components, const double *sigma_m2, uint32_t n_species);
 
+/* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
+ * what tools/ehss_offline_precompute.cpp and icarion_gpu_ehss_precompute
+ * produce).  Species s gets n_orient[s] orientations (0: it keeps the online
+ * geometry method), each with a sigma_eff and mu_stride[s] first-contact mu;
+ * sigma_eff and mu hold the sets of all species concatenated in species order
+ * (mu orientation-major).  An EHSS collision of such a species draws an
+ * orientation first, tests with its sigma_eff and scatters with one of its mu
+ * at a uniform azimuth (EHSSCollisionHandler.cpp:151-172, 222-299).  Used by
+ * the FP64 EHSS path (ICARION_EHSS_GEOMETRY, fixed-step and RK45); n_species
+ * must equal that of icarion_gpu_set_species; n_species = 0 clears. */
+int icarion_gpu_set_ehss_offline(icarion_gpu *h, const uint32_t *n_orient, const uint32_t *mu_stride,
+                                 const double *sigma_eff, const double *mu, uint32_t n_species);
+
 /* Trilinear E field on a uniform node-centred grid, x fastest
  * (index i + nx*(j + ny*k), fieldArrayLoader.cpp:9-11).  E holds 3*nx*ny*nz
  * values interleaved (Ex, Ey, Ez) per node; dtype 0 = float, 1 = double.

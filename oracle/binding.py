@@ -71,8 +71,11 @@ def load():
     lib.oracle_run.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), P(_abi.SpeciesDesc),
                                C.c_uint32, P(d), C.c_uint32, P(_abi.OracleField), P(_abi.IonSoA),
                                d, d, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
-                               P(_abi.StepStats), P(_abi.OracleGasMixture)]
+                               P(_abi.StepStats), P(_abi.OracleCollisionTables)]
     lib.oracle_run.restype = C.c_int
+    lib.oracle_mt_ehss_offline.argtypes = [C.c_uint64, P(_abi.DomainDesc), C.c_uint32, C.c_uint32,
+                                           P(d), P(d), d, P(d), d, P(C.c_uint32), P(d), P(d)]
+    lib.oracle_mt_ehss_offline.restype = C.c_int
     lib.oracle_mt_hss_mixture.argtypes = [C.c_uint64, P(_abi.DomainDesc),
                                           P(_abi.OracleGasMixture), P(d), d, P(d), d, P(d)]
     lib.oracle_mt_hss_mixture.restype = C.c_int
@@ -98,7 +101,7 @@ def load():
                                     P(_abi.SpeciesDesc), C.c_uint32, P(d), C.c_uint32,
                                     P(_abi.OracleField), P(_abi.IonSoA), C.c_void_p, d, d,
                                     C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats),
-                                    P(_abi.OracleGasMixture)]
+                                    P(_abi.OracleCollisionTables)]
     lib.oracle_run_rk45.restype = C.c_int
     _lib = lib
     return lib
@@ -182,8 +185,53 @@ class GasMixture:
                                        _dp(self.sigma))
 
 
-def _mix_ptr(mixture):
-    return C.byref(mixture.c) if mixture is not None else C.POINTER(_abi.OracleGasMixture)()
+class EhssOffline:
+    """oracle_ehss_offline from per-species tables: sets[s] = None (online
+    geometry) or (sigma_eff[n_orient], mu[n_orient, mu_stride])."""
+
+    def __init__(self, sets):
+        n = len(sets)
+        self.n_orient = np.zeros(n, np.uint32)
+        self.stride = np.zeros(n, np.uint32)
+        sig, mus = [], []
+        for s_, t in enumerate(sets):
+            if t is None:
+                continue
+            sg, mu = np.asarray(t[0], np.float64), np.asarray(t[1], np.float64)
+            self.n_orient[s_], self.stride[s_] = sg.shape[0], mu.shape[1]
+            sig.append(sg.ravel())
+            mus.append(mu.ravel())
+        self.sigma = np.ascontiguousarray(np.concatenate(sig) if sig else np.zeros(1))
+        self.mu = np.ascontiguousarray(np.concatenate(mus) if mus else np.zeros(1))
+        u32 = C.POINTER(C.c_uint32)
+        self.c = _abi.OracleEhssOffline(n, 0, self.n_orient.ctypes.data_as(u32),
+                                        self.stride.ctypes.data_as(u32), _dp(self.sigma),
+                                        _dp(self.mu))
+
+
+def _tables_ptr(mixture=None, ehss_offline=None):
+    if mixture is None and ehss_offline is None:
+        return C.POINTER(_abi.OracleCollisionTables)()
+    t = _abi.OracleCollisionTables(
+        C.pointer(mixture.c) if mixture is not None else C.POINTER(_abi.OracleGasMixture)(),
+        C.pointer(ehss_offline.c) if ehss_offline is not None
+        else C.POINTER(_abi.OracleEhssOffline)())
+    return C.pointer(t)
+
+
+def mt_ehss_offline(seed, domain, sigma_eff, mu, mass, vel, dt):
+    """One offline-table EHSS collision stage on a fresh PhysicsRng(seed):
+    returns (collided, orientation drawn, mu used, velocity after, neutral)."""
+    lib = load()
+    sg = np.ascontiguousarray(sigma_eff, dtype=np.float64)
+    m = np.ascontiguousarray(mu, dtype=np.float64)
+    v = vec(*vel)
+    o = C.c_uint32()
+    mu_used = C.c_double()
+    vn = (C.c_double * 3)()
+    hit = lib.oracle_mt_ehss_offline(seed, C.byref(domain), sg.shape[0], m.shape[1], _dp(sg),
+                                     _dp(m), mass, v, dt, C.byref(o), C.byref(mu_used), vn)
+    return bool(hit), o.value, mu_used.value, np.array(v[:]), np.array(vn[:])
 
 
 def mt_hss_mixture(seed, domain, mixture, sigma_row, mass, vel, dt):
@@ -199,7 +247,7 @@ def mt_hss_mixture(seed, domain, mixture, sigma_row, mass, vel, dt):
 
 
 def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=None, field=None,
-        rng_kind=RNG_PHILOX, threads=0, mixture=None):
+        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None):
     """Run the oracle engine loop on ens in place; returns StepStats."""
     lib = load()
     sp = species_array(species)
@@ -213,7 +261,7 @@ def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=Non
     st = _abi.StepStats()
     rc = lib.oracle_run(C.byref(domain), C.byref(physics), sp, len(species), ap, na, fptr,
                         C.byref(soa), t0, dt, n_steps, step_offset, rng_kind, threads,
-                        C.byref(st), _mix_ptr(mixture))
+                        C.byref(st), _tables_ptr(mixture, ehss_offline))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed ({rc})")
     return st
@@ -359,7 +407,7 @@ def rk45_push_one(domain, physics, cfg, sp, mass, charge, pos, vel, t, dt, state
 
 
 def run_rk45(domain, physics, cfg, species, ens, t0, dt, n_steps, step_offset=0, atoms=None,
-             field=None, state=None, threads=0, mixture=None):
+             field=None, state=None, threads=0, mixture=None, ehss_offline=None):
     """The RK45 engine loop on ens in place; returns (StepStats, state).  Pass
     the returned state to the next call to continue a run."""
     lib = load()
@@ -375,7 +423,8 @@ def run_rk45(domain, physics, cfg, species, ens, t0, dt, n_steps, step_offset=0,
     stats = _abi.StepStats()
     rc = lib.oracle_run_rk45(C.byref(domain), C.byref(physics), C.byref(cfg), sp, len(species),
                              ap, na, fptr, C.byref(soa), st.ctypes.data, t0, dt, n_steps,
-                             step_offset, threads, C.byref(stats), _mix_ptr(mixture))
+                             step_offset, threads, C.byref(stats),
+                             _tables_ptr(mixture, ehss_offline))
     if rc != 0:
         raise RuntimeError(f"oracle_run_rk45 failed ({rc})")
     return stats, st

@@ -775,6 +775,59 @@ V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
     return v_ion;
 }
 
+// The collision tables one ion sees: its row of the mixture cross sections
+// and its species' offline EHSS sample set.
+struct IonTables {
+    const oracle_gas_mixture* mix = nullptr;
+    const double* mix_row = nullptr;
+    uint32_t off_n = 0, off_stride = 0;
+    const double* off_sigma = nullptr;
+    const double* off_mu = nullptr;
+};
+
+// EHSSCollisionHandler::handle_collision with an offline sample set, single gas
+// (EHSSCollisionHandler.cpp:151-172, 222-299, 426-455).  Returns true on a
+// collision; orient / mu_used report the draws.
+template <class Rng>
+bool ehss_offline_collide(const icarion_domain_desc& d, uint32_t n_orient, uint32_t stride,
+                          const double* sig, const double* mus, double m_ion, V3& vel, double dt,
+                          Rng& rng, uint32_t* orient = nullptr, double* mu_used = nullptr,
+                          V3* vn_out = nullptr) {
+    size_t idx = static_cast<size_t>(rng.uniform01() * static_cast<double>(n_orient));
+    if (idx >= n_orient) idx = n_orient - 1;
+    if (orient) *orient = static_cast<uint32_t>(idx);
+    const double sigma_eff = sig[idx];
+    if (sigma_eff <= 0.0) return false;
+    const double m_neutral = d.gas_mass_kg;
+    const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    const V3 v_neutral = sample_neutral_velocity(d.temperature_K, m_neutral, v_gas, rng);
+    if (vn_out) *vn_out = v_neutral;
+    const V3 v_rel = vel - v_neutral;
+    const double v_rel_mag = norm(v_rel);
+    if (v_rel_mag < kMinRelativeVelocity) return false;
+    const double P = 1.0 - std::exp(-d.gas_density_m3 * sigma_eff * v_rel_mag * dt);
+    if (rng.uniform01() >= P) return false;
+    size_t mu_idx = static_cast<size_t>(rng.uniform01() * static_cast<double>(stride));
+    if (mu_idx >= stride) mu_idx = stride - 1;
+    double mu = mus[idx * stride + mu_idx];
+    mu = std::min(1.0, std::max(-1.0, mu));
+    if (mu_used) *mu_used = mu;
+    const double sin_theta = std::sqrt(std::max(0.0, 1.0 - mu * mu));
+    const double phi = 2.0 * kPi * rng.uniform01();
+    const V3 axis{v_rel.x / v_rel_mag, v_rel.y / v_rel_mag, v_rel.z / v_rel_mag};
+    V3 t1, t2;
+    orthonormal_basis(axis, t1, t2);
+    const V3 n_contact = t1 * (sin_theta * std::cos(phi)) + t2 * (sin_theta * std::sin(phi)) +
+                         axis * (-mu);
+    const double vdotn = v_rel.x * n_contact.x + v_rel.y * n_contact.y + v_rel.z * n_contact.z;
+    const V3 v_rel_reflected = v_rel - n_contact * (2.0 * vdotn);
+    const double mt = m_ion + m_neutral;
+    const double inv_mt = 1.0 / mt;
+    const V3 Vcom = (vel * m_ion + v_neutral * m_neutral) * inv_mt;
+    vel = Vcom + v_rel_reflected * (m_neutral * inv_mt);
+    return true;
+}
+
 // HSSCollisionHandler::handle_collision mixture path (HSSCollisionHandler.cpp:105-249):
 // a neutral per participating component with n_i > 0, rates from the actual
 // relative speeds, one test on the total rate, the partner drawn with weight
@@ -838,11 +891,14 @@ template <class Rng>
 bool collide(const icarion_domain_desc& d, const icarion_physics_desc& ph,
              const icarion_species_desc& sp, const double* atoms, double m_ion, double q,
              V3& vel, double dt, Rng& rng, std::vector<V3>& scratch,
-             const oracle_gas_mixture* mix = nullptr, const double* mix_sigma_row = nullptr) {
+             const IonTables& tb = IonTables{}) {
     const int model = ph.collision_model;
     const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
-    if (model == ICARION_COLLISION_HSS && mix && mix->n_components > 0)
-        return collide_mixture(d, *mix, mix_sigma_row, m_ion, vel, dt, rng) >= 0;
+    if (model == ICARION_COLLISION_HSS && tb.mix && tb.mix->n_components > 0)
+        return collide_mixture(d, *tb.mix, tb.mix_row, m_ion, vel, dt, rng) >= 0;
+    if (model == ICARION_COLLISION_EHSS && tb.off_n > 0)
+        return ehss_offline_collide(d, tb.off_n, tb.off_stride, tb.off_sigma, tb.off_mu, m_ion, vel,
+                                    dt, rng);
     if (model == ICARION_COLLISION_HSS || model == ICARION_COLLISION_EHSS) {
         const double n = d.gas_density_m3;
         const double T_K = d.temperature_K;
@@ -897,8 +953,7 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
                  const icarion_species_desc& sp, const double* atoms, const oracle_field* f,
                  IonCtx& ion, const double* t_table, double dt, uint64_t n_steps,
                  uint32_t ion_index, uint64_t step0, MtRng* mt, uint64_t& collisions,
-                 std::vector<V3>& scratch, const oracle_gas_mixture* mix,
-                 const double* mix_row) {
+                 std::vector<V3>& scratch, const IonTables& tb) {
     uint64_t pushed = 0;
     const bool detect = std::isfinite(d.detector_z_m);
     for (uint64_t k = 0; k < n_steps; ++k) {
@@ -925,10 +980,10 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
         if (ph.collision_model != ICARION_COLLISION_NONE) {
             bool hit;
             if (mt) {
-                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, *mt, scratch, mix, mix_row);
+                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, *mt, scratch, tb);
             } else {
                 PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
-                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, mix, mix_row);
+                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb);
             }
             if (hit) {
                 ++ion.ncoll;
@@ -955,6 +1010,48 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
     }
     return pushed;
 }
+
+// Per-species views of a run's collision tables.
+struct RunTables {
+    const oracle_collision_tables* t = nullptr;
+    std::vector<size_t> off_base, mu_base;
+    bool ok(uint32_t n_species) const {
+        if (!t) return true;
+        if (t->mixture && t->mixture->n_components > 0 && t->mixture->n_species != n_species) return false;
+        if (t->ehss_offline && t->ehss_offline->n_species != n_species) return false;
+        return true;
+    }
+    explicit RunTables(const oracle_collision_tables* tables) : t(tables) {
+        if (t && t->ehss_offline) {
+            const oracle_ehss_offline& o = *t->ehss_offline;
+            off_base.resize(o.n_species);
+            mu_base.resize(o.n_species);
+            size_t a = 0, b = 0;
+            for (uint32_t s = 0; s < o.n_species; ++s) {
+                off_base[s] = a;
+                mu_base[s] = b;
+                a += o.n_orient[s];
+                b += static_cast<size_t>(o.n_orient[s]) * o.mu_stride[s];
+            }
+        }
+    }
+    IonTables for_species(uint32_t s) const {
+        IonTables tb;
+        if (!t) return tb;
+        if (t->mixture && t->mixture->n_components > 0) {
+            tb.mix = t->mixture;
+            tb.mix_row = t->mixture->sigma_m2 + static_cast<size_t>(s) * t->mixture->n_components;
+        }
+        if (t->ehss_offline && s < t->ehss_offline->n_species && t->ehss_offline->n_orient[s] > 0) {
+            const oracle_ehss_offline& o = *t->ehss_offline;
+            tb.off_n = o.n_orient[s];
+            tb.off_stride = o.mu_stride[s];
+            tb.off_sigma = o.sigma_eff + off_base[s];
+            tb.off_mu = o.mu + mu_base[s];
+        }
+        return tb;
+    }
+};
 
 }  // namespace
 
@@ -1029,6 +1126,26 @@ int oracle_mt_hss_mixture(uint64_t seed, const icarion_domain_desc* domain,
     return sel;
 }
 
+int oracle_mt_ehss_offline(uint64_t seed, const icarion_domain_desc* domain, uint32_t n_orient,
+                           uint32_t mu_stride, const double* sigma_eff, const double* mu,
+                           double mass_kg, double vel[3], double dt, uint32_t* orient_out,
+                           double* mu_out, double vn_out[3]) {
+    MtRng rng(seed);
+    V3 v{vel[0], vel[1], vel[2]};
+    V3 vn{0.0, 0.0, 0.0};
+    uint32_t o = 0;
+    double m = NAN;
+    const bool hit = ehss_offline_collide(*domain, n_orient, mu_stride, sigma_eff, mu, mass_kg, v, dt,
+                                          rng, &o, &m, &vn);
+    vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
+    if (orient_out) *orient_out = o;
+    if (mu_out) *mu_out = m;
+    if (vn_out) {
+        vn_out[0] = vn.x; vn_out[1] = vn.y; vn_out[2] = vn.z;
+    }
+    return hit ? 1 : 0;
+}
+
 void oracle_mt_ehss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],
                               double ion_mass, double neutral_mass, double ion_radius,
                               const double* atoms_xyzr, uint32_t n_atoms, double out[3]) {
@@ -1077,10 +1194,11 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
                const icarion_species_desc* species, uint32_t n_species, const double* atoms,
                uint32_t n_atoms, const oracle_field* field, icarion_ion_soa* ions, double t0,
                double dt, uint64_t n_steps, uint64_t step_offset, int rng_kind, int n_threads,
-               icarion_step_stats* stats, const oracle_gas_mixture* mixture) {
+               icarion_step_stats* stats, const oracle_collision_tables* tables) {
     (void)n_atoms;
     if (!domain || !physics || !species || !ions || n_species == 0) return ICARION_ERR_INVALID;
-    if (mixture && mixture->n_components > 0 && mixture->n_species != n_species) return ICARION_ERR_INVALID;
+    const RunTables rt(tables);
+    if (!rt.ok(n_species)) return ICARION_ERR_INVALID;
     const size_t n = ions->n;
     // Engine time t_k: the reference advances every live ion by t_i += dt and
     // takes the engine time as max_i t_i (SimulationEngineLifecycle.cpp:172,
@@ -1118,19 +1236,16 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
             ion.ncoll = ions->n_collisions ? ions->n_collisions[i] : 0;
             uint64_t coll = 0;
             uint64_t pushed = 0;
-            const double* mix_row = (mixture && mixture->n_components > 0)
-                                        ? mixture->sigma_m2 + static_cast<size_t>(s < n_species ? s : 0) *
-                                                                  mixture->n_components
-                                        : nullptr;
+            const IonTables tb = rt.for_species(s < n_species ? s : 0);
             if (rng_kind == ORACLE_RNG_MT19937_64) {
                 MtRng rng(physics->rng_seed + static_cast<uint64_t>(i));
                 pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
                                  n_steps, static_cast<uint32_t>(i), step_offset, &rng, coll,
-                                 scratch, mixture, mix_row);
+                                 scratch, tb);
             } else {
                 pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
                                  n_steps, static_cast<uint32_t>(i), step_offset, nullptr, coll,
-                                 scratch, mixture, mix_row);
+                                 scratch, tb);
             }
             ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
             ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;
@@ -1188,11 +1303,12 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                     uint32_t n_species, const double* atoms, uint32_t n_atoms,
                     const oracle_field* field, icarion_ion_soa* ions, oracle_rk45_ion* rk45,
                     double t0, double dt, uint64_t n_steps, uint64_t step_offset, int n_threads,
-                    icarion_step_stats* stats, const oracle_gas_mixture* mixture) {
+                    icarion_step_stats* stats, const oracle_collision_tables* tables) {
     (void)n_atoms;
     if (!domain || !physics || !cfg || !species || !ions || !rk45 || n_species == 0)
         return ICARION_ERR_INVALID;
-    if (mixture && mixture->n_components > 0 && mixture->n_species != n_species) return ICARION_ERR_INVALID;
+    const RunTables rt(tables);
+    if (!rt.ok(n_species)) return ICARION_ERR_INVALID;
     const icarion_domain_desc& d = *domain;
     const icarion_physics_desc& ph = *physics;
     const size_t n = ions->n;
@@ -1257,12 +1373,9 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                 // perform_collisions with the accepted dt
                 if (ph.collision_model != ICARION_COLLISION_NONE) {
                     PhiloxStream rng(ph.rng_seed, static_cast<uint32_t>(i), step_offset + k);
-                    const double* mix_row =
-                        (mixture && mixture->n_components > 0)
-                            ? mixture->sigma_m2 + static_cast<size_t>(s < n_species ? s : 0) * mixture->n_components
-                            : nullptr;
+                    const IonTables tb = rt.for_species(s < n_species ? s : 0);
                     if (collide(d, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, used,
-                                rng, scratch, mixture, mix_row)) {
+                                rng, scratch, tb)) {
                         if (ions->n_collisions) ++ions->n_collisions[i];
                         ++coll_total;
                     }

@@ -60,6 +60,25 @@ typedef struct oracle_gas_mixture {
     const double *sigma_m2;
 } oracle_gas_mixture;
 
+/* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30)
+ * per species: species s has n_orient[s] orientations (0: the online geometry
+ * method), each with sigma_eff and mu_stride[s] first-contact mu; the tables
+ * of all species concatenated in species order. */
+typedef struct oracle_ehss_offline {
+    uint32_t n_species;
+    uint32_t reserved0;
+    const uint32_t *n_orient;
+    const uint32_t *mu_stride;
+    const double *sigma_eff;
+    const double *mu;
+} oracle_ehss_offline;
+
+/* Optional collision tables of a run (either pointer may be NULL). */
+typedef struct oracle_collision_tables {
+    const oracle_gas_mixture *mixture;
+    const oracle_ehss_offline *ehss_offline;
+} oracle_collision_tables;
+
 enum { ORACLE_RNG_PHILOX = 0, ORACLE_RNG_MT19937_64 = 1 };
 
 /* Philox4x32-10 block function (Salmon et al., SC'11). */
@@ -122,7 +141,7 @@ int oracle_run(const icarion_domain_desc *domain, const icarion_physics_desc *ph
                const double *atoms_xyzr, uint32_t n_atoms, const oracle_field *field,
                icarion_ion_soa *ions, double t0, double dt, uint64_t n_steps,
                uint64_t step_offset, int rng_kind, int n_threads,
-               icarion_step_stats *stats, const oracle_gas_mixture *mixture);
+               icarion_step_stats *stats, const oracle_collision_tables *tables);
 
 /* ---- RK45 (RK45Strategy.cpp:101-520) -------------------------------------
  * Per-ion adaptive state carried between engine steps (RK45Strategy::RK45State,
@@ -168,7 +187,7 @@ int oracle_run_rk45(const icarion_domain_desc *domain, const icarion_physics_des
                     uint32_t n_species, const double *atoms_xyzr, uint32_t n_atoms,
                     const oracle_field *field, icarion_ion_soa *ions, oracle_rk45_ion *rk45,
                     double t0, double dt, uint64_t n_steps, uint64_t step_offset, int n_threads,
-                    icarion_step_stats *stats, const oracle_gas_mixture *mixture);
+                    icarion_step_stats *stats, const oracle_collision_tables *tables);
 
 /* One mixture collision stage (HSSCollisionHandler.cpp:105-249) of an ion of
  * mass m with velocity vel (in/out) on a PhysicsRng seeded with seed
@@ -178,6 +197,17 @@ int oracle_run_rk45(const icarion_domain_desc *domain, const icarion_physics_des
 int oracle_mt_hss_mixture(uint64_t seed, const icarion_domain_desc *domain,
                           const oracle_gas_mixture *mixture, const double *sigma_row, double mass_kg,
                           double vel[3], double dt, double vn_out[3]);
+
+/* One EHSS collision stage with an offline sample set (EHSSCollisionHandler.cpp:
+ * 151-172, 222-299: orientation drawn first, sigma_eff of that orientation in
+ * the collision test, mu from its samples, contact normal at a uniform
+ * azimuth, specular reflection) on a PhysicsRng seeded with seed.  Returns 1
+ * on a collision; orient_out / mu_out (may be NULL) receive the orientation
+ * index drawn and, on a collision, the mu used; vn_out the neutral. */
+int oracle_mt_ehss_offline(uint64_t seed, const icarion_domain_desc *domain, uint32_t n_orient,
+                           uint32_t mu_stride, const double *sigma_eff, const double *mu,
+                           double mass_kg, double vel[3], double dt, uint32_t *orient_out,
+                           double *mu_out, double vn_out[3]);
 
 /* ---- EHSS offline precompute (tools/ehss_offline_precompute.cpp:438-517) ----
  * Per (cluster c, orientation k): Shoemake orientation, b_max = max(r_xy + r) +

@@ -39,6 +39,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
             h.upload_field_terms(w["axes"], w["terms"])
         if w.get("mixture") is not None:
             h.set_gas_mixture(*w["mixture"])
+        if w.get("ehss_offline") is not None:
+            h.set_ehss_offline(w["ehss_offline"])
         h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -52,8 +54,10 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
     if w.get("terms") is not None:
         field = oracle.FieldTerms(w["axes"], w["terms"])
     mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
+    off = oracle.EhssOffline(w["ehss_offline"]) if w.get("ehss_offline") is not None else None
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
-                      step_offset=step_offset, atoms=w.get("atoms"), field=field, mixture=mixture)
+                      step_offset=step_offset, atoms=w.get("atoms"), field=field, mixture=mixture,
+                      ehss_offline=off)
 
 
 def rel_close(a, b, rtol, scale):
@@ -186,6 +190,50 @@ def test_ehss_fp64_ion_by_ion(oracle):
     so = oracle_run(oracle, w, o, 1000)
     assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
     assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_ehss_offline_sets_fp64_ion_by_ion(oracle, integrator):
+    """Offline EHSS sample sets for two of three species (orientation-resolved
+    sigma_eff and mu samples), the online geometry for the third."""
+    w = ehss_workload(n=2048, seed=35)
+    w["physics"] = W.physics(integrator, _abi.COLLISION_EHSS, seed=35)
+    rng = np.random.default_rng(35)
+    w["ehss_offline"] = [
+        (rng.uniform(30e-20, 60e-20, 16), np.sqrt(rng.uniform(0.0, 1.0, (16, 50)))),
+        None,
+        (rng.uniform(150e-20, 260e-20, 40), rng.uniform(-0.2, 1.0, (40, 7))),
+    ]
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 1000)
+    so = oracle_run(oracle, w, o, 1000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    # the sets change the result: without them every species uses the geometry
+    plain = dict(w)
+    plain.pop("ehss_offline")
+    o2 = w["ensemble"].copy()
+    so2 = oracle_run(oracle, plain, o2, 1000)
+    assert compare_ions(g, o2, rtol=1e-9).mean() < 0.9
+
+
+def test_ehss_offline_controls():
+    from icarion_amd import gpu
+    w = ehss_workload(n=64, seed=36)
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"], w["atoms"])
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_ehss_offline([None, None])
+        assert ei.value.code == _abi.ERR_INVALID
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_ehss_offline([None, None, None])
+        assert ei.value.code == _abi.ERR_INVALID
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_ehss_offline([(np.array([1e-18, np.nan]), np.ones((2, 3))), None, None])
+        assert ei.value.code == _abi.ERR_INVALID
+        h.set_ehss_offline([(np.array([1e-18]), np.ones((1, 3))), None, None])
+        h.set_ehss_offline([])
 
 
 def test_field_array_fp64(oracle):

@@ -153,3 +153,52 @@ def test_committed_surrogate_reproduces_the_ehss_angle_law(oracle):
     assert d_sur.mean() < 1.25 * d_null.mean()
     assert (p_sur < 0.01).mean() <= 2 / len(counts)
     assert d_hs.mean() > 3 * d_sur.mean()
+
+
+def test_ehss_offline_table_path(oracle):
+    """The offline-sample-set branch of EHSSCollisionHandler (EHSSCollisionHandler.cpp:
+    151-172, 222-299) in the oracle.  A table of the hard-sphere law (mu^2
+    uniform, what one atom gives) scatters isotropically in the centre of mass
+    (cos chi uniform, KS) and conserves |v_rel|; the orientation is drawn
+    uniformly and its own sigma_eff enters the collision test (two orientations
+    of different sigma: the collision fraction is the mean of their
+    probabilities, by Monte Carlo over the Maxwellian); the mu used is one of
+    the drawn orientation's samples."""
+    import math
+    from icarion_amd import workloads as W
+    from icarion_amd.constants import AMU_TO_KG, BOLTZMANN
+    d = W.ims_domain(EN_Td=20.0, gas="He")
+    rng = np.random.default_rng(3)
+    n_or, stride = 8, 400
+    mu_tab = np.sqrt(rng.uniform(0.0, 1.0, (n_or, stride)))
+    sig = np.full(n_or, 80e-20)
+    m_ion = 300.0 * AMU_TO_KG
+    v0 = np.array([2500.0, -300.0, 700.0])
+    dt = 2e-12
+    cos_chi, orients = [], []
+    for seed in range(6000):
+        hit, o, mu, v1, vn = oracle.mt_ehss_offline(seed + 1, d, sig, mu_tab, m_ion, v0, dt)
+        orients.append(o)
+        if not hit:
+            continue
+        assert np.any(np.abs(mu_tab[o] - mu) == 0.0)
+        mn = d.gas_mass_kg
+        Vcom = (m_ion * v0 + mn * vn) / (m_ion + mn)
+        vr0 = v0 - vn
+        vr1 = (v1 - Vcom) * (m_ion + mn) / mn
+        assert np.linalg.norm(vr1) == pytest.approx(np.linalg.norm(vr0), rel=1e-9)
+        cos_chi.append(vr0 @ vr1 / (vr0 @ vr0))
+    assert stats.kstest(cos_chi, "uniform", args=(-1.0, 2.0)).pvalue > 1e-3
+    counts = np.bincount(orients, minlength=n_or)
+    assert stats.chisquare(counts).pvalue > 1e-3
+    # sigma of the drawn orientation: orientations with 40 and 160 A^2
+    sig2 = np.array([40e-20, 160e-20])
+    n_hit = 0
+    N = 20000
+    for seed in range(N):
+        n_hit += oracle.mt_ehss_offline(seed + 1, d, sig2, mu_tab[:2], m_ion, v0, dt)[0]
+    s_n = math.sqrt(BOLTZMANN * d.temperature_K / d.gas_mass_kg)
+    vn = rng.normal(0.0, s_n, (400000, 3))
+    vrel = np.linalg.norm(v0 - vn, axis=1)
+    p = np.mean([np.mean(1 - np.exp(-d.gas_density_m3 * s * vrel * dt)) for s in sig2])
+    assert abs(n_hit / N - p) < 4 * math.sqrt(p * (1 - p) / N) + 1e-3
