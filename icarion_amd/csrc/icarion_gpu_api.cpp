@@ -29,8 +29,8 @@
 namespace icarion {
 hipError_t launch_step(int integrator, int field, int coll, int precision, int surrogate,
                        const StepParams& P, const IonPtrs& S, uint32_t block, hipStream_t stream);
-hipError_t launch_rk45_step(int field, int coll, const StepParams& P, const IonPtrs& S, const Rk45Ptrs& R,
-                            uint32_t block, hipStream_t stream);
+hipError_t launch_rk45_step(int field, int coll, int precision, const StepParams& P, const IonPtrs& S,
+                            const Rk45Ptrs& R, uint32_t block, hipStream_t stream);
 hipError_t launch_rk45_init(const Rk45Ptrs& R, uint32_t n, double t0, double dt, hipStream_t stream);
 hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
                         const uint4* blob, uint32_t n_clouds, unsigned long long* counters, hipStream_t s,
@@ -490,8 +490,8 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
                   icarion_step_stats* stats) {
     const int field = launch_field_kind(h);
     const int coll = launch_coll_kind(h);
-    if (h->phys.precision != ICARION_PRECISION_FP64)
-        fail(ICARION_ERR_UNSUPPORTED, "RK45 runs in FP64");
+    if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
+        fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
         fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with the fixed-step FP32 path");
     set_device(h);
@@ -527,7 +527,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
             hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             hipError_t ce = hipSuccess;
             for (uint64_t k = 0; k < G && ce == hipSuccess; ++k)
-                ce = launch_rk45_step(field, coll, P, S, R, h->block, s);
+                ce = launch_rk45_step(field, coll, h->phys.precision, P, S, R, h->block, s);
             const hipError_t ee = hipStreamEndCapture(s, &graph);
             if (ce == hipErrorInvalidValue) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -547,7 +547,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
             hip_check(le, "hipGraphLaunch");
         }
         for (; done < n_steps; ++done) {
-            const hipError_t e = launch_rk45_step(field, coll, P, S, R, h->block, s);
+            const hipError_t e = launch_rk45_step(field, coll, h->phys.precision, P, S, R, h->block, s);
             if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no RK45 kernel for this field/collision");
             hip_check(e, "RK45 step launch");
         }

@@ -1,5 +1,6 @@
 // rk45_kernels.hip - the RK45 engine step (RK45Strategy::step,
-// RK45Strategy.cpp:354-520) on gfx950, FP64.
+// RK45Strategy.cpp:354-520) on gfx950: FP64, and FP32 + Kahan stages with the
+// error estimate, step control and time in FP64.
 //
 // Floating-point contraction is off in this file: the step-size controller
 // accepts or rejects a trial on err <= 1, and separate multiply and add
@@ -9,6 +10,7 @@
 #pragma clang fp contract(off)
 
 #include "step_device.hpp"
+#include "step_device_f32.hpp"
 
 namespace icarion {
 
@@ -245,6 +247,233 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
     }
 }
 
+// The same engine step with FP32 stages: positions and velocities are FP32
+// (hi, lo) pairs advanced by compensated sums, forces and stage combinations in
+// FP32; the error estimate y5 - y4 = h sum_j (b5_j - a7_j) k_j is formed
+// directly from the stages and accumulated in FP64 (so it does not suffer the
+// cancellation of a difference of two FP32 solutions), and the controller,
+// per-ion time and dt, stage times and RF phase stay FP64.
+template <int FIELD, int COLL>
+__global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, const IonPtrs S,
+                                                            const Rk45Ptrs R) {
+    using namespace dp;
+    // error weights of the embedded pair (b5 - a7; b5_7 for the FSAL stage)
+    constexpr double e1 = b51 - a71, e3 = b53 - a73, e4 = b54 - a74, e5 = b55 - a75, e6 = b56 - a76,
+                     e7 = b57;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const FastConst& C = P.fc;
+    Rk45Engine* E = R.eng;
+    const double T = E->T;
+    const unsigned long long kstep = E->step;
+    uint32_t pushed = 0, ncoll_launch = 0, rejected = 0, failed = 0;
+    double tmax = -INFINITY, bmin = INFINITY, dtmax = 0.0;
+    if (i < P.n_ions) {
+        const double birth = S.birth ? S.birth[i] : -INFINITY;
+        uint8_t flags = S.flags[i];
+        bool active = flags & 1, born = (flags >> 1) & 1;
+        double t_i = R.t[i], dt_i = R.dt[i];
+        double death = S.death[i];
+        double arrival = S.arrival[i];
+        uint32_t nc = S.ncoll[i];
+        if (!born && T >= birth) {
+            born = true;
+            active = true;
+        }
+        if (!born && birth > T) bmin = birth;
+        if (active && born) {
+            const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
+            float hi[6], lo[6];
+#pragma unroll
+            for (int c6 = 0; c6 < 6; ++c6) {
+                hi[c6] = (float)X[c6];
+                lo[c6] = (float)(X[c6] - (double)hi[c6]);
+            }
+            if (!contains_f32(P, C, hi[0], hi[1], hi[2])) {
+                active = false;
+                death = T;
+            } else {
+                const double md = S.mass[i];
+                const double qd = S.charge[i];
+                const uint32_t sidx = S.species[i];
+                const SpeciesDev sp = P.species[sidx];
+                const float m = (float)md;
+                const float qm = (float)(qd / md);
+                const float inv_m_plus = (float)(1.0 / (md + P.m_gas));
+                float lpre = 0.0f;
+                if constexpr (COLL == COLL_DAMPING) {
+                    double l = 0.0;
+                    if (P.damping_model == 3) {
+                        if (P.alpha_gas > 0.0 && P.n > 0.0 && P.v_th > 0.0 && P.m_gas > 0.0) {
+                            const double mu = (md * P.m_gas) / (md + P.m_gas);
+                            const double cs1 = kPi * qd * sqrt(P.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
+                            l = P.n * cs1 * P.v_th * P.m_gas / (P.m_gas + md);
+                        }
+                    } else {
+                        const double K0 = sp.K0_cm2_Vs;
+                        if (K0 > 0.0 && md > 0.0 && P.n > 0.0) {
+                            const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(P.T, 1.0));
+                            l = qd / ((K0t * (kLoschmidt / P.n)) * md);
+                        }
+                    }
+                    lpre = (float)l;
+                }
+                auto acc = [&](f3 p, f3 v, double tt) {
+                    const double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
+                    return accel_f32<FIELD, COLL>(P, C, p, v, c, qm, lpre);
+                };
+                const Rk45Cfg& c = P.rk45;
+                const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);
+                const double dt_max = dt_i * c.max_factor;
+                double last_err = R.last_err[i];
+                bool fsal = R.fsal[i] != 0;
+                f3 k1a_saved{(float)R.k1x[i], (float)R.k1y[i], (float)R.k1z[i]};
+                const f3 p0{hi[0], hi[1], hi[2]}, v0{hi[3], hi[4], hi[5]};
+                const double y0[6] = {X[0], X[1], X[2], X[3], X[4], X[5]};
+                double h = dt_i, used = -1.0, dt_next = 0.0;
+                f3 dp7{0.f, 0.f, 0.f}, dv7{0.f, 0.f, 0.f}, k7a_keep{0.f, 0.f, 0.f};
+                for (int attempt = 0; attempt < kMaxReject; ++attempt) {
+                    const float hf = (float)h;
+                    const f3 k1v = v0;
+                    const f3 k1a = fsal ? k1a_saved : acc(p0, v0, T);
+                    const f3 k2v = v0 + k1a * (hf * (float)a21);
+                    const f3 k2a = acc(p0 + k1v * (hf * (float)a21), k2v, T + c2 * h);
+                    const f3 k3v = v0 + (k1a * (float)a31 + k2a * (float)a32) * hf;
+                    const f3 k3a = acc(p0 + (k1v * (float)a31 + k2v * (float)a32) * hf, k3v, T + c3 * h);
+                    const f3 k4v = v0 + (k1a * (float)a41 + k2a * (float)a42 + k3a * (float)a43) * hf;
+                    const f3 k4a = acc(p0 + (k1v * (float)a41 + k2v * (float)a42 + k3v * (float)a43) * hf, k4v,
+                                       T + c4 * h);
+                    const f3 k5v = v0 + (k1a * (float)a51 + k2a * (float)a52 + k3a * (float)a53 +
+                                         k4a * (float)a54) * hf;
+                    const f3 k5a = acc(p0 + (k1v * (float)a51 + k2v * (float)a52 + k3v * (float)a53 +
+                                             k4v * (float)a54) * hf,
+                                       k5v, T + c5 * h);
+                    const f3 k6v = v0 + (k1a * (float)a61 + k2a * (float)a62 + k3a * (float)a63 +
+                                         k4a * (float)a64 + k5a * (float)a65) * hf;
+                    const f3 k6a = acc(p0 + (k1v * (float)a61 + k2v * (float)a62 + k3v * (float)a63 +
+                                             k4v * (float)a64 + k5v * (float)a65) * hf,
+                                       k6v, T + h);
+                    const f3 dp = (k1v * (float)a71 + k3v * (float)a73 + k4v * (float)a74 + k5v * (float)a75 +
+                                   k6v * (float)a76) * hf;
+                    const f3 dv = (k1a * (float)a71 + k3a * (float)a73 + k4a * (float)a74 + k5a * (float)a75 +
+                                   k6a * (float)a76) * hf;
+                    const f3 v7 = v0 + dv;
+                    const f3 k7a = acc(p0 + dp, v7, T + h);
+                    // RK45Strategy::estimate_error (RK45Strategy.cpp:272-313) on
+                    // y5 - y4 formed from the stages in FP64
+                    const double ep[3] = {
+                        h * (e1 * k1v.x + e3 * k3v.x + e4 * k4v.x + e5 * k5v.x + e6 * k6v.x + e7 * v7.x),
+                        h * (e1 * k1v.y + e3 * k3v.y + e4 * k4v.y + e5 * k5v.y + e6 * k6v.y + e7 * v7.y),
+                        h * (e1 * k1v.z + e3 * k3v.z + e4 * k4v.z + e5 * k5v.z + e6 * k6v.z + e7 * v7.z)};
+                    const double ev[3] = {
+                        h * (e1 * k1a.x + e3 * k3a.x + e4 * k4a.x + e5 * k5a.x + e6 * k6a.x + e7 * k7a.x),
+                        h * (e1 * k1a.y + e3 * k3a.y + e4 * k4a.y + e5 * k5a.y + e6 * k6a.y + e7 * k7a.y),
+                        h * (e1 * k1a.z + e3 * k3a.z + e4 * k4a.z + e5 * k5a.z + e6 * k6a.z + e7 * k7a.z)};
+                    double err = 0.0;
+#pragma unroll
+                    for (int c3i = 0; c3i < 3; ++c3i) {
+                        err = fmax(err, fabs(ep[c3i]) / (c.atol + c.rtol * fabs(y0[c3i])));
+                        err = fmax(err, fabs(ev[c3i]) / (c.atol + c.rtol * fabs(y0[3 + c3i])));
+                    }
+                    err = fmax(err, kMinError);
+                    if (err <= 1.0 || (c.accept_at_dt_min && h <= dt_min * kDtMinTol)) {
+                        dp7 = dp;
+                        dv7 = dv;
+                        k7a_keep = k7a;
+                        fsal = true;
+                        dt_next = rk45_new_step(c, h, err, dt_min, dt_max, last_err);
+                        last_err = err;
+                        used = h;
+                        break;
+                    }
+                    h = rk45_new_step(c, h, err, dt_min, dt_max, last_err);
+                    ++rejected;
+                    fsal = false;
+                }
+                if (used < 0.0) {
+                    failed = 1;
+                } else {
+                    pushed = 1;
+                    dtmax = used;
+                    k1a_saved = k7a_keep;
+                    const float z_before = hi[2], z_before_lo = lo[2];
+                    kahan_add(hi[0], lo[0], dp7.x);
+                    kahan_add(hi[1], lo[1], dp7.y);
+                    kahan_add(hi[2], lo[2], dp7.z);
+                    kahan_add(hi[3], lo[3], dv7.x);
+                    kahan_add(hi[4], lo[4], dv7.y);
+                    kahan_add(hi[5], lo[5], dv7.z);
+                    if constexpr (COLL != COLL_NONE) {
+                        PhiloxStream rng(P.seed, i, kstep);
+                        f3 v{hi[3], hi[4], hi[5]};
+                        const f3 vin = v;
+                        const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * used);
+                        const bool hit = collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m,
+                                                           v, rng, (float)used, (float)(P.n * used));
+                        if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
+                            if (hit) {
+                                hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
+                                lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
+                                ++nc;
+                                ++ncoll_launch;
+                            }
+                        } else {
+                            kahan_add(hi[3], lo[3], v.x - vin.x);
+                            kahan_add(hi[4], lo[4], v.y - vin.y);
+                            kahan_add(hi[5], lo[5], v.z - vin.z);
+                        }
+                    }
+                    if (P.detect && isnan(arrival)) {
+                        const double zb = (double)z_before + (double)z_before_lo;
+                        const double za = (double)hi[2] + (double)lo[2];
+                        if (zb < P.det_z && za >= P.det_z) arrival = T + used * (P.det_z - zb) / (za - zb);
+                    }
+                    if (!contains_f32(P, C, hi[0], hi[1], hi[2])) {
+                        active = false;
+                        death = T;
+                    } else {
+                        t_i = t_i + used;
+                        dt_i = dt_next > 0.0 ? dt_next : (used > 0.0 ? used : E->dt_fallback);
+                        if (!(isfinite(hi[0]) && isfinite(hi[1]) && isfinite(hi[2]) && isfinite(hi[3]) &&
+                              isfinite(hi[4]) && isfinite(hi[5]))) {
+                            active = false;
+                            death = T;
+                        }
+                    }
+                    S.x[i] = (double)hi[0] + (double)lo[0];
+                    S.y[i] = (double)hi[1] + (double)lo[1];
+                    S.z[i] = (double)hi[2] + (double)lo[2];
+                    S.vx[i] = (double)hi[3] + (double)lo[3];
+                    S.vy[i] = (double)hi[4] + (double)lo[4];
+                    S.vz[i] = (double)hi[5] + (double)lo[5];
+                }
+                R.last_err[i] = last_err;
+                R.fsal[i] = fsal ? 1 : 0;
+                R.k1x[i] = k1a_saved.x; R.k1y[i] = k1a_saved.y; R.k1z[i] = k1a_saved.z;
+            }
+        }
+        tmax = t_i;
+        R.t[i] = t_i;
+        R.dt[i] = dt_i;
+        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+        S.death[i] = death;
+        S.arrival[i] = arrival;
+        S.ncoll[i] = nc;
+    }
+    add_counters(P, pushed, ncoll_launch);
+    tmax = wave_max_f64(tmax);
+    bmin = wave_min_f64(bmin);
+    dtmax = wave_max_f64(dtmax);
+    const uint32_t rej = wave_sum_u32(rejected);
+    const uint32_t fl = wave_sum_u32(failed);
+    if ((threadIdx.x & 63) == 0) {
+        if (tmax > -INFINITY) atomicMax(&E->tmax, ord_bits(tmax));
+        if (bmin < INFINITY) atomicMin(&E->bmin, ord_bits(bmin));
+        if (dtmax > 0.0) atomicMax(&E->dtmax, ord_bits(dtmax));
+        if (rej) atomicAdd(&E->rejected, static_cast<unsigned long long>(rej));
+        if (fl) atomicAdd(&E->failed, static_cast<unsigned long long>(fl));
+    }
+}
+
 // next_engine_time_after_step (SimulationEngineUtils.cpp:49-83), total_time_s
 // unbounded; then clears the reductions for the next step.
 __global__ void rk45_advance_kernel(Rk45Engine* E) {
@@ -277,7 +506,16 @@ __global__ void rk45_init_kernel(Rk45Ptrs R, uint32_t n, double t0, double dt) {
 using Rk45Fn = void (*)(const StepParams, const IonPtrs, const Rk45Ptrs);
 
 template <int FIELD>
-Rk45Fn pick_rk45_coll(int coll) {
+Rk45Fn pick_rk45_coll(int coll, int precision) {
+    if (precision == 1) {
+        switch (coll) {   // FP32 stages: no EHSS geometry (as the fixed-step FP32 path)
+            case COLL_NONE: return rk45_step_f32_kernel<FIELD, COLL_NONE>;
+            case COLL_HSS: return rk45_step_f32_kernel<FIELD, COLL_HSS>;
+            case COLL_HSS_MIX: return rk45_step_f32_kernel<FIELD, COLL_HSS_MIX>;
+            case COLL_DAMPING: return rk45_step_f32_kernel<FIELD, COLL_DAMPING>;
+            default: return nullptr;
+        }
+    }
     switch (coll) {
         case COLL_NONE: return rk45_step_kernel<FIELD, COLL_NONE>;
         case COLL_HSS: return rk45_step_kernel<FIELD, COLL_HSS>;
@@ -290,12 +528,12 @@ Rk45Fn pick_rk45_coll(int coll) {
 }
 
 // One RK45 engine step (step kernel + engine-time advance) on the stream.
-hipError_t launch_rk45_step(int field, int coll, const StepParams& P, const IonPtrs& S, const Rk45Ptrs& R,
-                            uint32_t block, hipStream_t stream) {
-    Rk45Fn fn = field == FIELD_ORBITRAP ? pick_rk45_coll<FIELD_ORBITRAP>(coll)
-                : field == FIELD_ARRAY  ? pick_rk45_coll<FIELD_ARRAY>(coll)
-                : field == FIELD_ARRAY_X ? pick_rk45_coll<FIELD_ARRAY_X>(coll)
-                                        : pick_rk45_coll<FIELD_LINEAR>(coll);
+hipError_t launch_rk45_step(int field, int coll, int precision, const StepParams& P, const IonPtrs& S,
+                            const Rk45Ptrs& R, uint32_t block, hipStream_t stream) {
+    Rk45Fn fn = field == FIELD_ORBITRAP ? pick_rk45_coll<FIELD_ORBITRAP>(coll, precision)
+                : field == FIELD_ARRAY  ? pick_rk45_coll<FIELD_ARRAY>(coll, precision)
+                : field == FIELD_ARRAY_X ? pick_rk45_coll<FIELD_ARRAY_X>(coll, precision)
+                                        : pick_rk45_coll<FIELD_LINEAR>(coll, precision);
     if (!fn) return hipErrorInvalidValue;
     const uint32_t grid = (P.n_ions + block - 1) / block;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, stream, P, S, R);

@@ -426,8 +426,10 @@ int icarion_gpu_synchronize(icarion_gpu *h);
  * advanced in one persistent launch: a fixed pool of workgroups takes tiles
  * (up to block_size ions of one cloud, all of that cloud's steps) from a
  * queue ordered longest first.  Per cloud the result is that of
- * icarion_gpu_step on a handle of its own (same kernels, same random
- * streams).  Covered: RK4 / Boris; IMS, TIMS, quadrupole and Orbitrap
+ * icarion_gpu_step on a handle of its own (the same per-ion code and random
+ * streams; the compiler may fuse multiply-adds differently in the batch
+ * kernel, so FP32 clouds can differ from a single-cloud run by FP32
+ * rounding, which collisions amplify into statistical agreement).  Covered: RK4 / Boris; IMS, TIMS, quadrupole and Orbitrap
  * analytic fields; no collisions, HSS, Langevin / Friction (+ OU kicks);
  * FP64 or FP32 + Kahan per cloud; EHSS with the surrogate (FP32 + Kahan,
  * the handle's surrogate parameters, species sigma_ehss and features; atoms
