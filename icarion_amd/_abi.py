@@ -17,6 +17,11 @@ EHSS_GEOMETRY, EHSS_SURROGATE = range(2)
 (FIELD_SCALE_CONSTANT, FIELD_SCALE_DC_AXIAL, FIELD_SCALE_DC_QUAD, FIELD_SCALE_DC_RADIAL,
  FIELD_SCALE_RF) = range(5)
 MAX_FIELD_TERMS = 4
+(WAVEFORM_CONSTANT, WAVEFORM_LINEAR, WAVEFORM_QUADRATIC, WAVEFORM_EXPONENTIAL, WAVEFORM_SINUSOIDAL,
+ WAVEFORM_PWM, WAVEFORM_PULSED, WAVEFORM_ARBITRARY) = range(8)
+INTERP_LINEAR, INTERP_STEP, INTERP_CUBIC = range(3)
+SLOT_DC_AXIAL, SLOT_DC_QUAD, SLOT_DC_RADIAL, SLOT_RF_VOLTAGE, SLOT_RF_FREQUENCY = range(5)
+N_VOLTAGE_SLOTS = 5
 
 _d = C.c_double
 _i32 = C.c_int32
@@ -147,6 +152,29 @@ class OracleCollisionTables(C.Structure):
                 ("ehss_offline", C.POINTER(OracleEhssOffline))]
 
 
+class WaveformDesc(C.Structure):
+    _fields_ = [("type", _i32), ("interp", _i32), ("clamp", _i32), ("reserved0", _i32),
+                ("p", _d * 8), ("times_s", C.POINTER(_d)), ("values", C.POINTER(_d)),
+                ("n_points", _u32), ("reserved1", _u32)]
+
+
+def waveform(type_, p=(), clamp=1, interp=0, points=()):
+    """An icarion_waveform_desc; the returned object keeps its arrays alive."""
+    import numpy as np
+    w = WaveformDesc()
+    w.type, w.clamp, w.interp = type_, int(clamp), interp
+    for i, x in enumerate(p):
+        w.p[i] = float(x)
+    pts = np.asarray(list(points), dtype=np.float64).reshape(-1, 2)
+    w._ts = np.ascontiguousarray(pts[:, 0])
+    w._vs = np.ascontiguousarray(pts[:, 1])
+    if len(pts):
+        w.times_s = w._ts.ctypes.data_as(C.POINTER(_d))
+        w.values = w._vs.ctypes.data_as(C.POINTER(_d))
+    w.n_points = len(pts)
+    return w
+
+
 class OracleFieldTerm(C.Structure):
     _fields_ = [
         ("scale_kind", _i32), ("reserved0", _i32), ("constant_scale", _d), ("frequency_Hz", _d),
@@ -160,6 +188,7 @@ class OracleField(C.Structure):
         ("xs", C.POINTER(_d)), ("ys", C.POINTER(_d)), ("zs", C.POINTER(_d)),
         ("Ex", C.POINTER(_d)), ("Ey", C.POINTER(_d)), ("Ez", C.POINTER(_d)),
         ("n_terms", _u32), ("reserved0", _u32), ("terms", C.POINTER(OracleFieldTerm)),
+        ("waveforms", C.POINTER(WaveformDesc) * 5),
     ]
 
 

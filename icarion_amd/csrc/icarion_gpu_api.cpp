@@ -22,9 +22,11 @@
 #include "step_params.hpp"
 
 #include <numeric>
+#include <memory>
 #include <random>
 
 #include "surrogate.hpp"
+#include "waveform.hpp"
 
 namespace icarion {
 hipError_t launch_step(int integrator, int field, int coll, int precision, int surrogate,
@@ -166,6 +168,9 @@ struct icarion_gpu {
     DevBuf<float> ax32[3];
     // HSS gas mixture: the participating components with n > 0 (in order) and
     // their per-species cross sections, as the kernels read them
+    // voltage waveforms of the analytic fields (per slot; none when !has_waves)
+    std::unique_ptr<HostWaveform> wave[ICARION_N_VOLTAGE_SLOTS];
+    bool has_waves = false;
     std::vector<icarion_gas_component> mix;
     uint32_t mix_species = 0;
     DevBuf<double> mix_sigma;
@@ -353,24 +358,57 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
 // stage_times: a field array of several time-scaled terms, whose kernels take
 // the stage times t, t + dt/2, t + dt in place of the RF coefficients.
 void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, bool rf, int integrator,
-                  double t0, double dt, uint64_t n_steps, bool stage_times = false) {
+                  double t0, double dt, uint64_t n_steps, bool stage_times = false,
+                  const HostWaveform* const* wf = nullptr) {
     const size_t base = out.size();
     out.resize(base + n_steps);
     const double r0_sq = d.radius_m * d.radius_m;
     const double omega = 2.0 * kPiHost * d.rf_frequency_Hz;
     const double half_dt = dt * 0.5;
+    // E_z of the IMS field, or of the quadrupole's axial DC where set
+    // (AnalyticalFieldModel.cpp:136-161, :243-245)
+    auto ez_of = [&](double axial) {
+        if (d.instrument == ICARION_INSTRUMENT_QUADRUPOLE) return std::fabs(axial) > 1e-12 ? axial / d.length_m : 0.0;
+        return axial / d.length_m;
+    };
+    const double ez_const = ez_of(d.dc_axial_V);
     auto coeff = [&](double t) {
         const double U_eff = d.dc_quad_V + d.rf_voltage_V * std::cos(omega * t);
         return U_eff / r0_sq;
     };
+    // with waveforms every voltage is evaluated at the stage time, and the RF
+    // term is present where |V(t)| > 1e-12 (AnalyticalFieldModel.cpp:150-159, 236-242)
+    auto slot = [&](int k, double constant, double t) { return (wf && wf[k]) ? wf[k]->eval(t) : constant; };
+    auto coeff_wf = [&](double t) {
+        const double V = slot(ICARION_SLOT_RF_VOLTAGE, d.rf_voltage_V, t);
+        if (!(std::fabs(V) > 1e-12)) return 0.0;
+        const double f = slot(ICARION_SLOT_RF_FREQUENCY, d.rf_frequency_Hz, t);
+        const double U = slot(ICARION_SLOT_DC_QUAD, d.dc_quad_V, t);
+        const double om = 2.0 * kPiHost * f;
+        return (U + V * std::cos(om * t)) / r0_sq;
+    };
+    auto ez_wf = [&](double t) { return ez_of(slot(ICARION_SLOT_DC_AXIAL, d.dc_axial_V, t)); };
     double t = t0;
     for (uint64_t k = 0; k < n_steps; ++k) {
         StepEntry& e = out[base + k];
         e.t = t;
+        e.pad = 0.0;
+        e.e0 = e.eh = e.e1 = ez_const;
         if (stage_times) {
             e.c0 = t;
             e.ch = t + half_dt;
             e.c1 = t + dt;
+        } else if (wf) {
+            e.c0 = rf ? coeff_wf(t) : 0.0;
+            e.e0 = ez_wf(t);
+            if (integrator == ICARION_INTEGRATOR_RK4) {
+                e.ch = rf ? coeff_wf(t + half_dt) : 0.0;
+                e.c1 = rf ? coeff_wf(t + dt) : 0.0;
+                e.eh = ez_wf(t + half_dt);
+                e.e1 = ez_wf(t + dt);
+            } else {
+                e.ch = e.c1 = 0.0;
+            }
         } else if (rf) {
             e.c0 = coeff(t);
             if (integrator == ICARION_INTEGRATOR_RK4) {
@@ -458,7 +496,12 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
 void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
     h->table_host.clear();
     const bool stage_times = field_kind(h->dom) == FIELD_ARRAY && h->fsrc.n_terms >= 2;
-    append_table(h->table_host, h->dom, h->base.has_rf != 0, h->phys.integrator, t0, dt, n_steps, stage_times);
+    const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
+    for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) wf[k] = h->wave[k].get();
+    const bool rf = h->base.has_rf != 0 || (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE] &&
+                                            h->dom.instrument != ICARION_INSTRUMENT_ORBITRAP);
+    append_table(h->table_host, h->dom, rf, h->phys.integrator, t0, dt, n_steps, stage_times,
+                 h->has_waves ? wf : nullptr);
     h->table.resize(std::max<size_t>(n_steps, 1));
     hip_check(hipMemcpyAsync(h->table.p, h->table_host.data(), n_steps * sizeof(StepEntry),
                              hipMemcpyHostToDevice, h->stream),
@@ -490,6 +533,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
                   icarion_step_stats* stats) {
     const int field = launch_field_kind(h);
     const int coll = launch_coll_kind(h);
+    if (h->has_waves) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with constant voltages");
     if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -621,8 +665,11 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
         fail(ICARION_ERR_STATE, "EHSS surrogate selected but no parameters set");
     if (coll == COLL_EHSS && !surrogate && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
+    if (h->has_waves && field != FIELD_LINEAR)
+        fail(ICARION_ERR_UNSUPPORTED, "voltage waveforms apply to the analytic IMS / quadrupole fields");
     set_device(h);
     StepParams P = h->base;
+    if (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
     P.dt = dt;
     P.species = h->species.p;
     P.atoms = h->atoms_f32.p;
@@ -1577,6 +1624,43 @@ int icarion_gpu_set_gas_mixture(icarion_gpu* h, const icarion_gas_component* com
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         h->mix = std::move(mix);
         h->mix_species = n_species;
+    });
+}
+
+int icarion_gpu_set_waveforms(icarion_gpu* h, const icarion_waveform_desc* const* slots) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        std::unique_ptr<HostWaveform> w[ICARION_N_VOLTAGE_SLOTS];
+        bool any = false;
+        for (int k = 0; slots && k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+            const icarion_waveform_desc* d = slots[k];
+            if (!d) continue;
+            if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
+                fail(ICARION_ERR_INVALID, "unknown waveform type");
+            for (double v : d->p)
+                if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
+            auto hw = std::make_unique<HostWaveform>();
+            hw->d = *d;
+            if (d->type == ICARION_WAVEFORM_ARBITRARY) {
+                // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
+                if (!d->times_s || !d->values || d->n_points == 0)
+                    fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
+                if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
+                    fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
+                hw->ts.assign(d->times_s, d->times_s + d->n_points);
+                hw->vs.assign(d->values, d->values + d->n_points);
+                for (uint32_t i = 0; i < d->n_points; ++i)
+                    if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
+                        (i && !(hw->ts[i] > hw->ts[i - 1])))
+                        fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
+            }
+            hw->d.times_s = nullptr;
+            hw->d.values = nullptr;
+            w[k] = std::move(hw);
+            any = true;
+        }
+        for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) h->wave[k] = std::move(w[k]);
+        h->has_waves = any;
     });
 }
 

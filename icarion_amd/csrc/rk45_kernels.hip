@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                 // analytic fields take the RF coefficient, field arrays the stage time
                 auto acc = [&](d3 p, d3 v, double tt) {
                     const double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
-                    return force_f64<FIELD, COLL>(P, p, v, c, q, m, lpre) * inv_m;
+                    return force_f64<FIELD, COLL>(P, p, v, c, P.Ez_const, q, m, lpre) * inv_m;
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);
@@ -319,7 +319,7 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
                 }
                 auto acc = [&](f3 p, f3 v, double tt) {
                     const double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
-                    return accel_f32<FIELD, COLL>(P, C, p, v, c, qm, lpre);
+                    return accel_f32<FIELD, COLL>(P, C, p, v, c, P.Ez_const, qm, lpre);
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);

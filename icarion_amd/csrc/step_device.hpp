@@ -142,11 +142,13 @@ __device__ __forceinline__ d3 field_array_f64(const StepParams& P, d3 p, double 
     return r;
 }
 
+// c_rf: the stage's RF coefficient (analytic fields) or stage time (field
+// arrays); ez: the stage's axial field (analytic IMS / quadrupole fields).
 template <int FIELD>
-__device__ __forceinline__ d3 field_f64(const StepParams& P, d3 p, double c_rf) {
+__device__ __forceinline__ d3 field_f64(const StepParams& P, d3 p, double c_rf, double ez) {
     if constexpr (FIELD == FIELD_LINEAR) {
         // AnalyticalFieldModel.cpp:136-161 (IMS) and :219-248 (quadrupole)
-        d3 E{0.0, 0.0, P.Ez_const};
+        d3 E{0.0, 0.0, ez};
         if (P.has_rf) {
             E.x = -2.0 * p.x * c_rf;
             E.y = 2.0 * p.y * c_rf;
@@ -176,9 +178,9 @@ __device__ __forceinline__ double gamma_f64(const StepParams& P, double lpre, d3
 }
 
 template <int FIELD, int COLL>
-__device__ __forceinline__ d3 force_f64(const StepParams& P, d3 p, d3 v, double c_rf, double q,
-                                        double m, double lpre) {
-    d3 F = field_f64<FIELD>(P, p, c_rf) * q;
+__device__ __forceinline__ d3 force_f64(const StepParams& P, d3 p, d3 v, double c_rf, double ez,
+                                        double q, double m, double lpre) {
+    d3 F = field_f64<FIELD>(P, p, c_rf, ez) * q;
     if constexpr (COLL == COLL_DAMPING) {
         const double g = gamma_f64(P, lpre, v);
         if (g > 0.0) {

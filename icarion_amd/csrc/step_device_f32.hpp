@@ -29,10 +29,11 @@ __device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sq
 // c: the stage's RF coefficient (analytic fields) or the stage time (field
 // arrays), in FP64 as the step table holds it.
 template <int FIELD>
-__device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C, f3 p, double c) {
+__device__ __forceinline__ f3 field_f32(const StepParams& P, const FastConst& C, f3 p, double c,
+                                        double ez) {
     if constexpr (FIELD == FIELD_LINEAR) {
         const float c_rf = (float)c;
-        f3 E{0.0f, 0.0f, C.Ez};
+        f3 E{0.0f, 0.0f, (float)ez};
         if (P.has_rf) {
             E.x = -2.0f * p.x * c_rf;
             E.y = 2.0f * p.y * c_rf;
@@ -278,8 +279,8 @@ __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst
 
 template <int FIELD, int COLL>
 __device__ __forceinline__ f3 accel_f32(const StepParams& P, const FastConst& C, f3 p, f3 v,
-                                        double c, float qm, float lpre) {
-    f3 a = field_f32<FIELD>(P, C, p, c) * qm;
+                                        double c, double ez, float qm, float lpre) {
+    f3 a = field_f32<FIELD>(P, C, p, c, ez) * qm;
     if constexpr (COLL == COLL_DAMPING) {
         const float g = (P.damping_model == 3) ? lpre / fmaxf(1e-6f, fast_sqrt(dot(v, v))) : lpre;
         if (g > 0.0f) a = a - f3{v.x - C.u_gas[0], v.y - C.u_gas[1], v.z - C.u_gas[2]} * g;

@@ -78,19 +78,19 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
             // RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361)
             const d3 p0 = pos, v0 = vel;
             const d3 k1v = v0;
-            const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, q, m, lpre) * inv_m;
+            const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, te.e0, q, m, lpre) * inv_m;
             const d3 k2v = v0 + k1a * half_dt;
-            const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, q, m, lpre) * inv_m;
+            const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lpre) * inv_m;
             const d3 k3v = v0 + k2a * half_dt;
-            const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, q, m, lpre) * inv_m;
+            const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lpre) * inv_m;
             const d3 k4v = v0 + k3a * dt;
-            const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, q, m, lpre) * inv_m;
+            const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lpre) * inv_m;
             const double h6 = dt / 6.0;
             pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
             vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
         } else {
             // BorisStrategy::step (BorisStrategy.cpp:18-98)
-            const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, q, m, lpre) * inv_m;
+            const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, te.e0, q, m, lpre) * inv_m;
             const d3 vm = vel + a * half_dt;
             d3 vp = vm;
             if (P.magnetic) {
@@ -222,17 +222,17 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
         const f3 v0{hi[3], hi[4], hi[5]};
         f3 dpos, dvel;
         if constexpr (INTEG == 0) {
-            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, qm, lpre);
+            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lpre);
             const f3 k2v = v0 + k1a * half_dt;
-            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, te.ch, qm, lpre);
+            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lpre);
             const f3 k3v = v0 + k2a * half_dt;
-            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, te.ch, qm, lpre);
+            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lpre);
             const f3 k4v = v0 + k3a * dt;
-            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, te.c1, qm, lpre);
+            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lpre);
             dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
             dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
         } else {
-            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, qm, lpre);
+            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lpre);
             const f3 vm = v0 + a * half_dt;
             f3 rot{0.0f, 0.0f, 0.0f};
             if (P.magnetic) {
@@ -379,17 +379,17 @@ __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, 
             const f3 v0{hi[3], hi[4], hi[5]};
             f3 dpos, dvel;
             if constexpr (INTEG == 0) {
-                const f3 k1a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, te.c0, qm, 0.f);
+                const f3 k1a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, te.c0, te.e0, qm, 0.f);
                 const f3 k2v = v0 + k1a * half_dt;
-                const f3 k2a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + v0 * half_dt, k2v, te.ch, qm, 0.f);
+                const f3 k2a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, 0.f);
                 const f3 k3v = v0 + k2a * half_dt;
-                const f3 k3a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k2v * half_dt, k3v, te.ch, qm, 0.f);
+                const f3 k3a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, 0.f);
                 const f3 k4v = v0 + k3a * dt;
-                const f3 k4a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k3v * dt, k4v, te.c1, qm, 0.f);
+                const f3 k4a = accel_f32<FIELD, COLL_NONE>(P, C, p0 + k3v * dt, k4v, te.c1, te.e1, qm, 0.f);
                 dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
                 dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
             } else {
-                const f3 a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, te.c0, qm, 0.f);
+                const f3 a = accel_f32<FIELD, COLL_NONE>(P, C, p0, v0, te.c0, te.e0, qm, 0.f);
                 const f3 vmid = v0 + a * half_dt;
                 f3 rot{0.0f, 0.0f, 0.0f};
                 if (P.magnetic) {

@@ -42,13 +42,18 @@ constexpr int kMaxFieldTerms = 4;
 constexpr int kMaxGasComponents = 4;
 
 // Per-step uniform values at the step start, mid point and end: for analytic
-// fields the RF coefficient c = U_eff(t)/r0^2; for a field array of several
-// time-scaled terms the stage time itself (the kernel evaluates the scales).
+// fields the RF coefficient c = U_eff(t)/r0^2 and the axial field E_z(t); for a
+// field array of several time-scaled terms c is the stage time itself (the
+// kernel evaluates the scales).
 struct StepEntry {
     double t;
     double c0;
     double ch;
     double c1;
+    double e0;
+    double eh;
+    double e1;
+    double pad;
 };
 
 struct IonPtrs {

@@ -23,7 +23,7 @@ EXPORTED = (
     "icarion_gpu_surrogate_train", "icarion_gpu_surrogate_grad", "icarion_gpu_surrogate_infer",
     "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
-    "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline",
+    "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
 )
 
 
@@ -310,6 +310,17 @@ class Handle:
         _check(self.lib.icarion_gpu_set_gas_mixture(
             self.h, arr, len(components), sig.ctypes.data_as(C.POINTER(C.c_double)), n_species),
             self.lib)
+
+    def set_waveforms(self, slots):
+        """slots: {icarion_voltage_slot: _abi.waveform(...)}; {} or None clears."""
+        if not slots:
+            _check(self.lib.icarion_gpu_set_waveforms(self.h, None), self.lib)
+            return
+        arr = (C.POINTER(_abi.WaveformDesc) * _abi.N_VOLTAGE_SLOTS)()
+        for k, w in slots.items():
+            arr[k] = C.pointer(w)
+        self._keep.append(dict(slots))
+        _check(self.lib.icarion_gpu_set_waveforms(self.h, arr), self.lib)
 
     def set_ehss_offline(self, sets):
         """sets[s]: None (species s keeps the online geometry) or

@@ -95,6 +95,44 @@ enum icarion_field_scale {
                                            rf_frequency_Hz when that is 0     */
 };
 
+/* Waveform of a time-varying voltage (ValueOrWaveform; WaveformConfig.h:19-200).
+ * Parameters p[] by type:
+ *   CONSTANT     p0 value
+ *   LINEAR       p0 start value, p1 end value, p2 start time, p3 end time; clamp
+ *   QUADRATIC    p0 a, p1 b, p2 c (a + b t + c t^2), p3 start time, p4 end time
+ *   EXPONENTIAL  p0 offset, p1 amplitude, p2 rate [1/s], p3 start, p4 end; clamp
+ *   SINUSOIDAL   p0 offset, p1 amplitude, p2 frequency [Hz], p3 phase [rad]
+ *   PWM          p0 low, p1 high, p2 frequency, p3 duty cycle, p4 phase, p5 start,
+ *                p6 end; clamp
+ *   PULSED       p0 low, p1 high, p2 pulse start, p3 pulse width
+ *   ARBITRARY    times_s / values (n_points, times increasing), interp */
+enum icarion_waveform_type {
+    ICARION_WAVEFORM_CONSTANT = 0,
+    ICARION_WAVEFORM_LINEAR = 1,
+    ICARION_WAVEFORM_QUADRATIC = 2,
+    ICARION_WAVEFORM_EXPONENTIAL = 3,
+    ICARION_WAVEFORM_SINUSOIDAL = 4,
+    ICARION_WAVEFORM_PWM = 5,
+    ICARION_WAVEFORM_PULSED = 6,
+    ICARION_WAVEFORM_ARBITRARY = 7
+};
+
+enum icarion_waveform_interp {
+    ICARION_INTERP_LINEAR = 0,
+    ICARION_INTERP_STEP = 1,
+    ICARION_INTERP_CUBIC = 2     /* Catmull-Rom Hermite (WaveformConfig.cpp:54-85) */
+};
+
+/* The voltages of the analytic fields that may follow a waveform. */
+enum icarion_voltage_slot {
+    ICARION_SLOT_DC_AXIAL = 0,
+    ICARION_SLOT_DC_QUAD = 1,
+    ICARION_SLOT_DC_RADIAL = 2,
+    ICARION_SLOT_RF_VOLTAGE = 3,
+    ICARION_SLOT_RF_FREQUENCY = 4,
+    ICARION_N_VOLTAGE_SLOTS = 5
+};
+
 /* how an EHSS collision picks the scattering normal */
 enum icarion_ehss_mode {
     ICARION_EHSS_GEOMETRY = 0,   /* first-contact ray test over the atoms      */
@@ -197,6 +235,19 @@ typedef struct icarion_gas_component {
 } icarion_gas_component;
 
 #define ICARION_MAX_GAS_COMPONENTS 4
+
+typedef struct icarion_waveform_desc {
+    int32_t type;                 /* enum icarion_waveform_type                */
+    int32_t interp;               /* ARBITRARY: enum icarion_waveform_interp   */
+    int32_t clamp;                /* LINEAR / EXPONENTIAL / PWM: hold the end
+                                     value after the end time                  */
+    int32_t reserved0;
+    double p[8];                  /* parameters, see icarion_waveform_type     */
+    const double *times_s;        /* ARBITRARY                                 */
+    const double *values;         /* ARBITRARY                                 */
+    uint32_t n_points;            /* ARBITRARY                                 */
+    uint32_t reserved1;
+} icarion_waveform_desc;
 
 /* Surrogate training options (Adam on mean 0.5*|y - t|^2). */
 typedef struct icarion_train_desc {
@@ -301,6 +352,17 @@ int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
  * n_components = 0 returns to the single-gas path.  Affects HSS only. */
 int icarion_gpu_set_gas_mixture(icarion_gpu *h, const icarion_gas_component *components,
                                 uint32_t n_components, const double *sigma_m2, uint32_t n_species);
+
+/* Time-varying voltages of the analytic IMS / TIMS and quadrupole fields:
+ * slots[k] (ICARION_N_VOLTAGE_SLOTS entries, NULL = the domain's constant)
+ * replaces the value of slot k, evaluated at each stage time as
+ * AnalyticalFieldModel::compute_ims_field / compute_quadrupole_rf_field do
+ * (AnalyticalFieldModel.cpp:130-248: E_z = axial(t)/L, U_eff = quad(t) +
+ * V(t) cos(2 pi f(t) t) where |V(t)| > 1e-12).  The host evaluates them into
+ * the FP64 step table.  slots = NULL clears.  Fixed-step integrators (RK4,
+ * Boris) with analytic fields; RK45, field arrays, the Orbitrap and the batch
+ * keep constant voltages (ICARION_ERR_UNSUPPORTED when combined). */
+int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const *slots);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
  * what tools/ehss_offline_precompute.cpp and icarion_gpu_ehss_precompute

@@ -55,6 +55,8 @@ def load():
     lib.oracle_orbitrap_r_for_z.argtypes = [d, d, d]
     lib.oracle_orbitrap_r_for_z.restype = d
     lib.oracle_interpolate_field.argtypes = [P(_abi.OracleField), P(d), P(d)]
+    lib.oracle_waveform_value.argtypes = [P(_abi.WaveformDesc), d]
+    lib.oracle_waveform_value.restype = d
     lib.oracle_field_term_scale.argtypes = [P(_abi.DomainDesc), P(_abi.OracleFieldTerm), d]
     lib.oracle_field_term_scale.restype = d
     lib.oracle_damping_gamma.argtypes = [P(_abi.DomainDesc), C.c_int, P(_abi.SpeciesDesc), d, d,
@@ -153,6 +155,21 @@ class FieldTerms:
         self.c = _abi.OracleField(nx, ny, nz, _dp(self.xs), _dp(self.ys), _dp(self.zs),
                                   _dp(first[0]), _dp(first[1]), _dp(first[2]), len(terms), 0,
                                   C.cast(self.terms, C.POINTER(_abi.OracleFieldTerm)))
+
+
+class Waveforms:
+    """An oracle_field carrying only voltage waveforms (for an analytic-field
+    domain): slots {slot: WaveformDesc}."""
+
+    def __init__(self, slots):
+        self.slots = dict(slots)
+        self.c = _abi.OracleField()
+        for k, w in self.slots.items():
+            self.c.waveforms[k] = C.pointer(w)
+
+
+def waveform_value(w, t):
+    return load().oracle_waveform_value(C.byref(w), t)
 
 
 def field_term_scale(domain, term, t):

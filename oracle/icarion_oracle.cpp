@@ -169,15 +169,99 @@ private:
 
 // ---- fields -------------------------------------------------------------
 
+// The waveform types of WaveformConfig.h:33-200 and ArbitraryWaveform::evaluate
+// (WaveformConfig.cpp:10-91).
+double waveform_value(const icarion_waveform_desc& w, double t) {
+    const double* p = w.p;
+    switch (w.type) {
+        case ICARION_WAVEFORM_CONSTANT: return p[0];
+        case ICARION_WAVEFORM_LINEAR: {
+            const double start_value = p[0], end_value = p[1], start_time = p[2], end_time = p[3];
+            if (t < start_time) return start_value;
+            if (t >= end_time) return w.clamp ? end_value : start_value;
+            const double duration = end_time - start_time;
+            const double t_norm = (t - start_time) / duration;
+            return start_value + (end_value - start_value) * t_norm;
+        }
+        case ICARION_WAVEFORM_QUADRATIC: {
+            const double a = p[0], b = p[1], c = p[2];
+            if (t < p[3] || t > p[4]) return a;
+            return a + b * t + c * t * t;
+        }
+        case ICARION_WAVEFORM_EXPONENTIAL: {
+            const double offset = p[0], amplitude = p[1], rate = p[2], start = p[3], end = p[4];
+            auto value_at = [&](double rel_t) { return offset + amplitude * std::exp(rate * rel_t); };
+            if (t < start) return value_at(0.0);
+            if (t >= end) return w.clamp ? value_at(end - start) : value_at(0.0);
+            return value_at(t - start);
+        }
+        case ICARION_WAVEFORM_SINUSOIDAL: {
+            const double omega = 2.0 * kPi * p[2];
+            return p[0] + p[1] * std::sin(omega * t + p[3]);
+        }
+        case ICARION_WAVEFORM_PWM: {
+            const double low = p[0], high = p[1], f = p[2], duty = p[3], ph = p[4];
+            if (f <= 0.0) return low;
+            auto eval_at = [&](double t_rel) {
+                const double period = 1.0 / f;
+                const double phase_shift = ph / (2.0 * kPi * f);
+                double phase = std::fmod(t_rel + phase_shift, period);
+                if (phase < 0.0) phase += period;
+                return (phase < duty * period) ? high : low;
+            };
+            if (t < p[5]) return low;
+            if (t >= p[6]) return w.clamp ? eval_at(p[6] - p[5]) : low;
+            return eval_at(t - p[5]);
+        }
+        case ICARION_WAVEFORM_PULSED:
+            return (t >= p[2] && t < p[2] + p[3]) ? p[1] : p[0];
+        default: {
+            const double* ts = w.times_s;
+            const double* vs = w.values;
+            const size_t n = w.n_points;
+            if (t <= ts[0]) return vs[0];
+            if (t >= ts[n - 1]) return vs[n - 1];
+            const size_t idx = static_cast<size_t>(std::lower_bound(ts, ts + n, t) - ts);
+            if (idx == 0) return vs[0];
+            const size_t i0 = idx - 1, i1 = idx;
+            const double t0 = ts[i0], t1 = ts[i1], v0 = vs[i0], v1 = vs[i1];
+            if (w.interp == ICARION_INTERP_STEP) return v0;
+            if (w.interp == ICARION_INTERP_LINEAR) {
+                const double alpha = (t - t0) / (t1 - t0);
+                return v0 + alpha * (v1 - v0);
+            }
+            const double alpha = (t - t0) / (t1 - t0);
+            const double m0 = (i0 == 0) ? (v1 - v0) / (t1 - t0) : (v1 - vs[i0 - 1]) / (t1 - ts[i0 - 1]);
+            const double m1 = (i1 == n - 1) ? (v1 - v0) / (t1 - t0) : (vs[i1 + 1] - v0) / (ts[i1 + 1] - t0);
+            // Hermite basis in the reference's operation order
+            const double h00 = 2 * alpha * alpha * alpha - 3 * alpha * alpha + 1;
+            const double h10 = alpha * alpha * alpha - 2 * alpha * alpha + alpha;
+            const double h01 = -2 * alpha * alpha * alpha + 3 * alpha * alpha;
+            const double h11 = alpha * alpha * alpha - alpha * alpha;
+            const double dt = t1 - t0;
+            return h00 * v0 + h10 * dt * m0 + h01 * v1 + h11 * dt * m1;
+        }
+    }
+}
+
+// eval_value of a voltage slot: its waveform when one is set, else the constant.
+double slot_value(const oracle_field* f, int slot, double constant, double t) {
+    return (f && f->waveforms[slot]) ? waveform_value(*f->waveforms[slot], t) : constant;
+}
+
 // AnalyticalFieldModel::compute_ims_field (AnalyticalFieldModel.cpp:136-161).
 // Absent values evaluate to 0 (the has_value() guards).
-V3 ims_field(const icarion_domain_desc& d, V3 pos, double t) {
+V3 ims_field(const icarion_domain_desc& d, V3 pos, double t, const oracle_field* f) {
     V3 E{0.0, 0.0, 0.0};
-    E.z = d.dc_axial_V / d.length_m;
-    if (std::fabs(d.rf_voltage_V) > kMinVoltage) {
+    const double dc_axial = slot_value(f, ICARION_SLOT_DC_AXIAL, d.dc_axial_V, t);
+    const double dc_quad = slot_value(f, ICARION_SLOT_DC_QUAD, d.dc_quad_V, t);
+    const double rf_voltage = slot_value(f, ICARION_SLOT_RF_VOLTAGE, d.rf_voltage_V, t);
+    const double rf_freq = slot_value(f, ICARION_SLOT_RF_FREQUENCY, d.rf_frequency_Hz, t);
+    E.z = dc_axial / d.length_m;
+    if (std::fabs(rf_voltage) > kMinVoltage) {
         const double r0_sq = d.radius_m * d.radius_m;
-        const double omega = 2.0 * kPi * d.rf_frequency_Hz;
-        const double U_eff = d.dc_quad_V + d.rf_voltage_V * std::cos(omega * t);
+        const double omega = 2.0 * kPi * rf_freq;
+        const double U_eff = dc_quad + rf_voltage * std::cos(omega * t);
         E.x = -2.0 * pos.x * U_eff / r0_sq;
         E.y = 2.0 * pos.y * U_eff / r0_sq;
     }
@@ -186,17 +270,21 @@ V3 ims_field(const icarion_domain_desc& d, V3 pos, double t) {
 
 // AnalyticalFieldModel::compute_quadrupole_rf_field (AnalyticalFieldModel.cpp:219-248).
 // Global coordinates; rf.phase_rad is not used by the reference.
-V3 quadrupole_field(const icarion_domain_desc& d, V3 pos, double t) {
+V3 quadrupole_field(const icarion_domain_desc& d, V3 pos, double t, const oracle_field* f) {
     V3 E{0.0, 0.0, 0.0};
-    if (std::fabs(d.rf_voltage_V) > kMinVoltage) {
+    const double rf_voltage = slot_value(f, ICARION_SLOT_RF_VOLTAGE, d.rf_voltage_V, t);
+    const double rf_freq = slot_value(f, ICARION_SLOT_RF_FREQUENCY, d.rf_frequency_Hz, t);
+    const double dc_quad = slot_value(f, ICARION_SLOT_DC_QUAD, d.dc_quad_V, t);
+    const double dc_axial = slot_value(f, ICARION_SLOT_DC_AXIAL, d.dc_axial_V, t);
+    if (std::fabs(rf_voltage) > kMinVoltage) {
         const double r0_sq = d.radius_m * d.radius_m;
-        const double omega = 2.0 * kPi * d.rf_frequency_Hz;
-        const double U_eff = d.dc_quad_V + d.rf_voltage_V * std::cos(omega * t);
+        const double omega = 2.0 * kPi * rf_freq;
+        const double U_eff = dc_quad + rf_voltage * std::cos(omega * t);
         E.x = -2.0 * pos.x * U_eff / r0_sq;
         E.y = 2.0 * pos.y * U_eff / r0_sq;
     }
-    if (std::fabs(d.dc_axial_V) > kMinVoltage) {
-        E.z = d.dc_axial_V / d.length_m;
+    if (std::fabs(dc_axial) > kMinVoltage) {
+        E.z = dc_axial / d.length_m;
     }
     return E;
 }
@@ -313,9 +401,9 @@ V3 field_E(const icarion_domain_desc& d, const oracle_field* f, V3 pos, double t
     switch (d.instrument) {
         case ICARION_INSTRUMENT_IMS:
         case ICARION_INSTRUMENT_TIMS:
-            return ims_field(d, pos, t);
+            return ims_field(d, pos, t, f);
         case ICARION_INSTRUMENT_QUADRUPOLE:
-            return quadrupole_field(d, pos, t);
+            return quadrupole_field(d, pos, t, f);
         case ICARION_INSTRUMENT_ORBITRAP:
             return orbitrap_field(d, pos);
         default:
@@ -1079,6 +1167,8 @@ int oracle_contains(const icarion_domain_desc* domain, const double pos[3]) {
 }
 
 double oracle_orbitrap_r_for_z(double z, double R, double R_m) { return orbitrap_r_for_z(z, R, R_m); }
+
+double oracle_waveform_value(const icarion_waveform_desc* w, double t) { return waveform_value(*w, t); }
 
 double oracle_field_term_scale(const icarion_domain_desc* domain, const oracle_field_term* term,
                                double t) {

@@ -44,7 +44,15 @@ typedef struct oracle_field {
     uint32_t n_terms;
     uint32_t reserved0;
     const oracle_field_term *terms;
+    /* voltage waveforms of the analytic fields per icarion_voltage_slot (NULL:
+     * the domain's constant); the grid members are unused when the domain has
+     * no field array */
+    const icarion_waveform_desc *waveforms[ICARION_N_VOLTAGE_SLOTS];
 } oracle_field;
+
+/* ValueOrWaveform::evaluate of one waveform (WaveformConfig.h:33-200,
+ * WaveformConfig.cpp:10-91). */
+double oracle_waveform_value(const icarion_waveform_desc *w, double t);
 
 /* CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147)
  * with the domain's constant voltages. */

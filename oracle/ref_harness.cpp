@@ -10,6 +10,7 @@
 //   RK4Strategy::step_batch                     (RK4Strategy.cpp:172-361)
 //   BorisStrategy::step                         (BorisStrategy.cpp:18-98)
 //   PhysicsRng                                  (CollisionTypes.cpp:8-20)
+//   Waveform / ArbitraryWaveform::evaluate      (WaveformConfig.h, WaveformConfig.cpp)
 // The only harness-side physics is the test force below (the same pattern as
 // the reference's own test forces, tests/integrator/test_boris_strategy.cpp:28-49):
 // the analytic quadrupole field of AnalyticalFieldModel.cpp:219-248, written
@@ -25,6 +26,7 @@
 #include <vector>
 
 #include "core/config/types/DomainConfig.h"
+#include "core/config/types/WaveformConfig.h"
 #include "core/integrator/strategies/BorisStrategy.h"
 #include "core/integrator/strategies/RK4Strategy.h"
 #include "core/physics/collisions/core/CollisionKernels.h"
@@ -128,6 +130,45 @@ void run_push() {
 }
 
 // "neutral": seed T m fx fy fz
+// waveform: type clamp interp p0..p7 n_points (t v)... n_t t...
+// prints the waveform value at each t
+void run_waveform() {
+    int type, clamp, interp;
+    double p[8];
+    std::cin >> type >> clamp >> interp;
+    for (double& x : p) std::cin >> x;
+    size_t n_points;
+    std::cin >> n_points;
+    config::ArbitraryWaveform arb;
+    for (size_t i = 0; i < n_points; ++i) {
+        double t, v;
+        std::cin >> t >> v;
+        arb.times_s.push_back(t);
+        arb.values.push_back(v);
+    }
+    arb.interp = interp == 1 ? config::ArbitraryWaveform::Interpolation::Step
+               : interp == 2 ? config::ArbitraryWaveform::Interpolation::Cubic
+                             : config::ArbitraryWaveform::Interpolation::Linear;
+    config::Waveform w;
+    switch (type) {
+        case 0: w.data = config::ConstantWaveform{p[0]}; break;
+        case 1: w.data = config::LinearWaveform{p[0], p[1], p[2], p[3], clamp != 0}; break;
+        case 2: w.data = config::QuadraticWaveform{p[0], p[1], p[2], p[3], p[4]}; break;
+        case 3: w.data = config::ExponentialWaveform{p[0], p[1], p[2], p[3], p[4], clamp != 0}; break;
+        case 4: w.data = config::SinusoidalWaveform{p[0], p[1], p[2], p[3]}; break;
+        case 5: w.data = config::PWMWaveform{p[0], p[1], p[2], p[3], p[4], p[5], p[6], clamp != 0}; break;
+        case 6: w.data = config::PulsedWaveform{p[0], p[1], p[2], p[3]}; break;
+        default: w.data = arb; break;
+    }
+    size_t n_t;
+    std::cin >> n_t;
+    for (size_t i = 0; i < n_t; ++i) {
+        double t;
+        std::cin >> t;
+        put(w.evaluate(t));
+    }
+}
+
 void run_neutral() {
     unsigned long long seed;
     double T, m;
@@ -177,6 +218,7 @@ int main() {
         else if (mode == "neutral") run_neutral();
         else if (mode == "hss") run_hss();
         else if (mode == "ehss") run_ehss();
+        else if (mode == "waveform") run_waveform();
         else {
             std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
             return 2;

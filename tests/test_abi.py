@@ -64,6 +64,8 @@ STRUCTS = {
     "icarion_field_term_desc": (_abi.FieldTermDesc, ["E", "dtype", "scale_kind", "constant_scale",
                                                      "phase_rad"]),
     "icarion_gas_component": (_abi.GasComponent, ["density_m3", "mass_kg", "participates"]),
+    "icarion_waveform_desc": (_abi.WaveformDesc, ["type", "clamp", "p", "times_s", "values",
+                                                  "n_points"]),
 }
 
 

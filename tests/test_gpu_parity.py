@@ -41,6 +41,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
             h.set_gas_mixture(*w["mixture"])
         if w.get("ehss_offline") is not None:
             h.set_ehss_offline(w["ehss_offline"])
+        if w.get("waveforms") is not None:
+            h.set_waveforms(w["waveforms"])
         h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -53,6 +55,8 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
         field = oracle.Field(*w["field"])
     if w.get("terms") is not None:
         field = oracle.FieldTerms(w["axes"], w["terms"])
+    if w.get("waveforms") is not None:
+        field = oracle.Waveforms(w["waveforms"])
     mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
     off = oracle.EhssOffline(w["ehss_offline"]) if w.get("ehss_offline") is not None else None
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
@@ -368,6 +372,86 @@ def test_hss_mixture_controls():
         h.upload_state(w["ensemble"].copy())
         st = h.step(0.0, w["dt"], 200)
         assert st.collisions > 0
+
+
+def quadrupole_waveforms():
+    """RF amplitude modulated (sinusoidal), RF frequency ramped (linear), a DC
+    quad pulse and an axial DC ramp (arbitrary, cubic)."""
+    return {
+        _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (450.0, 120.0, 2e5, 0.3)),
+        _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_LINEAR, (1.0e6, 1.3e6, 0.5e-6, 2.5e-6)),
+        _abi.SLOT_DC_QUAD: _abi.waveform(_abi.WAVEFORM_PULSED, (0.0, 35.0, 1.0e-6, 0.7e-6)),
+        _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_ARBITRARY, (), 1, _abi.INTERP_CUBIC,
+                                          [(0.0, 0.0), (0.8e-6, 4.0), (1.7e-6, -3.0), (3e-6, 6.0)]),
+    }
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_waveforms_quadrupole_fp64_ion_by_ion(oracle, integrator):
+    """Time-varying voltages (four waveform types) evaluated by the host into the
+    step table, against the oracle evaluating them at every force call."""
+    w = W.quadrupole_workload(n=2048, U=10.0, seed=41)
+    w["physics"] = W.physics(integrator, _abi.COLLISION_NONE, seed=41)
+    w["waveforms"] = quadrupole_waveforms()
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 3000)
+    oracle_run(oracle, w, o, 3000)
+    assert compare_ions(g, o, rtol=1e-9).all()
+    # the waveforms matter: constant voltages give other trajectories
+    plain = dict(w)
+    plain.pop("waveforms")
+    o2 = w["ensemble"].copy()
+    oracle_run(oracle, plain, o2, 3000)
+    assert compare_ions(g, o2, rtol=1e-6).mean() < 0.5
+
+
+def test_waveforms_tims_ramp_hss(oracle):
+    """A TIMS-like linear ramp of the axial voltage with HSS collisions (FP64
+    ion by ion) and the FP32 path against the FP64 oracle."""
+    w = W.ims_hss_workload(n=2048, seed=42)
+    w["domain"].instrument = _abi.INSTRUMENT_TIMS
+    w["waveforms"] = {_abi.SLOT_DC_AXIAL: _abi.waveform(
+        _abi.WAVEFORM_LINEAR, (w["domain"].dc_axial_V, 0.2 * w["domain"].dc_axial_V, 0.0, 2e-7))}
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 2000)
+    so = oracle_run(oracle, w, o, 2000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    w2 = W.quadrupole_workload(n=2048, U=10.0, seed=43)
+    w2["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, seed=43)
+    w2["waveforms"] = quadrupole_waveforms()
+    g2, o2 = w2["ensemble"].copy(), w2["ensemble"].copy()
+    gpu_run(w2, g2, 3000, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w2, o2, 3000)
+    assert compare_ions(g2, o2, rtol=2e-5, pos_scale=1e-3, vel_scale=10.0).mean() >= 0.99
+
+
+def test_waveforms_constant_equals_constant_and_limits():
+    from icarion_amd import gpu
+    w = W.quadrupole_workload(n=512, U=10.0, seed=44)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, seed=44)
+    d = w["domain"]
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, a, 500)
+    wc = dict(w)
+    wc["waveforms"] = {_abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_CONSTANT, (d.rf_voltage_V,)),
+                       _abi.SLOT_DC_QUAD: _abi.waveform(_abi.WAVEFORM_CONSTANT, (d.dc_quad_V,))}
+    gpu_run(wc, b, 500)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    with gpu.Handle() as h:
+        h.set_domain(d, W.physics(_abi.INTEGRATOR_RK45, seed=44))
+        h.set_species(w["species"])
+        h.set_waveforms(quadrupole_waveforms())
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, 1e-9, 5)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_waveforms({_abi.SLOT_DC_AXIAL: _abi.waveform(
+                _abi.WAVEFORM_ARBITRARY, (), 1, 0, [(1e-6, 1.0), (0.5e-6, 2.0)])})
+        assert ei.value.code == _abi.ERR_INVALID
+        h.set_waveforms(None)
 
 
 def test_birth_times_fp64(oracle):

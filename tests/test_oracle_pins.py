@@ -276,6 +276,22 @@ def test_composite_field_terms(oracle):
     np.testing.assert_allclose(oracle.field_E(d, p, 7.7e-7, field=one), ref, rtol=1e-10)
 
 
+def test_golden_waveforms_bitexact(oracle):
+    """Every ValueOrWaveform type of the reference (WaveformConfig.h,
+    WaveformConfig.cpp, compiled unmodified: tests/golden/make_ref_waveforms.py),
+    bit for bit at times before, inside, on the edges of and after its window."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_waveforms.json")
+    with open(path) as f:
+        cases = json.load(f)["cases"]
+    assert len(cases) == 13
+    for c in cases:
+        w = _abi.waveform(c["type"], c["p"], c["clamp"], c["interp"], c["points"])
+        got = [oracle.waveform_value(w, t) for t in c["t"]]
+        assert got == c["value"], (c["type"], c["clamp"], c["interp"])
+
+
 # ---------------------------------------------------------------- pushes
 def _push_many(oracle, d, ph, m, q, p, v, t0, dt, n):
     sp = W.species()
