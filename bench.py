@@ -72,7 +72,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=("ehss", "quadrupole", "batched", "batched_ehss",
-                                           "field_array", "ims"),
+                                           "field_array", "ims", "snapshots"),
                     default="ehss")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed engine steps (default: ehss 4000, quadrupole 100000)")
@@ -103,6 +103,10 @@ def parse():
     ap.add_argument("--autotune-steps", type=int, default=300,
                     help="batched: steps per cloud in each autotuner trial (0 = no tuning)")
     ap.add_argument("--fp64", action="store_true", help="batched: FP64 clouds instead of FP32")
+    ap.add_argument("--quad-u", type=float, default=0.0,
+                    help="quadrupole / snapshots: DC voltage U (V)")
+    ap.add_argument("--write-every", type=int, default=100,
+                    help="snapshots: write interval in steps (the reference's default 100)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter "
                          "calibration for rocprofv3 --pmc); loads torch, not the library")
@@ -737,6 +741,119 @@ def bench_ims(args):
     }
 
 
+def bench_snapshots(args):
+    """Stepping with trajectory snapshots every --write-every steps (the engine's
+    sampling of OutputManager) on the quadrupole cloud, FP32 + Kahan.  Four runs
+    of the same steps from the same state: no snapshots; snapshots through
+    icarion_gpu_step_snapshots (packed on the device, copied out on a second stream
+    while stepping goes on), consumed by a copy into a host buffer (the value);
+    the same with a consumer that keeps nothing (the device and PCIe share); and
+    the resident-mode loop without overlap (step to the snapshot, download the
+    state, copy it into the buffer)."""
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    steps = args.steps if args.steps is not None else 20_000
+    w = W.quadrupole_workload(n=args.ions, n_steps=steps, U=args.quad_u)
+    ph = W.physics(w["physics"].integrator, w["physics"].collision_model,
+                   w["physics"].ou_thermal_kicks, _abi.PRECISION_FP32_KAHAN, w["physics"].rng_seed)
+    dt, n = w["dt"], args.ions
+    interval = args.write_every * dt
+    ens = w["ensemble"].copy()
+    log(f"snapshots: {n} ions x {steps} steps, every {args.write_every} steps")
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 3))
+    spec = np.empty(n, np.uint32)
+    flags = np.empty(n, np.uint8)
+    snap_steps = []
+
+    def consume(snap):
+        np.copyto(pos, snap["positions"])
+        np.copyto(vel, snap["velocities"])
+        np.copyto(spec, snap["species"])
+        np.copyto(flags, snap["flags"])
+        snap_steps.append(snap["step"])
+
+    with gpu.Handle(device=0) as h:
+        h.set_domain(w["domain"], ph)
+        h.set_species(w["species"])
+        h.upload_state(ens)
+        h.step(0.0, dt, args.warmup, 0)
+        # no snapshots
+        h.upload_state(ens)
+        h.synchronize()
+        t0 = time.perf_counter()
+        st_plain = h.step(0.0, dt, steps, 0)
+        h.synchronize()
+        wall_plain = time.perf_counter() - t0
+        log(f"no snapshots {wall_plain:.3f} s")
+        # snapshots, overlapped
+        h.upload_state(ens)
+        h.synchronize()
+        t0 = time.perf_counter()
+        _, _, st_snap = h.step_snapshots(0.0, dt, steps, interval, on_snapshot=consume, copy=False)
+        h.synchronize()
+        wall_snap = time.perf_counter() - t0
+        log(f"snapshots {wall_snap:.3f} s ({len(snap_steps)} snapshots)")
+        # snapshots handed to a consumer that keeps nothing: the device + PCIe share
+        h.upload_state(ens)
+        h.synchronize()
+        t0 = time.perf_counter()
+        _, _, st_noop = h.step_snapshots(0.0, dt, steps, interval, on_snapshot=lambda s: None,
+                                         copy=False)
+        h.synchronize()
+        wall_noop = time.perf_counter() - t0
+        log(f"snapshots, no-op consumer {wall_noop:.3f} s")
+        # resident-mode loop: step to each snapshot, download, consume
+        e = ens.copy()
+        h.upload_state(ens)
+        h.synchronize()
+        t0 = time.perf_counter()
+        t, prev = 0.0, 0
+        for k in snap_steps:
+            st = h.step(t, dt, k + 1 - prev, prev)
+            h.download_state(e)
+            np.copyto(pos[:, 0], e.x)
+            np.copyto(pos[:, 1], e.y)
+            np.copyto(pos[:, 2], e.z)
+            np.copyto(vel[:, 0], e.vx)
+            np.copyto(vel[:, 1], e.vy)
+            np.copyto(vel[:, 2], e.vz)
+            t, prev = st.t_end_s, k + 1
+        if prev < steps:
+            h.step(t, dt, steps - prev, prev)
+        h.synchronize()
+        wall_loop = time.perf_counter() - t0
+        log(f"download loop {wall_loop:.3f} s")
+    value = st_snap.ion_steps / wall_snap
+    launches_plain = math.ceil(steps / 2000)
+    avg_launch_s = st_plain.device_ms * 1e-3 / launches_plain
+    achieved = FLOPS_PER_ION_STEP["fp32"] * (st_plain.ion_steps / launches_plain) / avg_launch_s / 1e12
+    snap_bytes = 53 * n
+    return {
+        "metric": "ion-steps/s on 1 MI355X with trajectory snapshots",
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": wall_snap * 1e3 / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (Kahan-compensated state; f64 time and RF phase)", "data": "synthetic",
+        "config": {"workload": "rf_quadrupole_boris_langevin_ou_snapshots", "ions": n, "U_dc": args.quad_u,
+                   "dt_s": dt, "engine_steps": steps, "write_every_steps": args.write_every,
+                   "snapshots": len(snap_steps), "snapshot_bytes": snap_bytes},
+        "no_snapshots": {"value": st_plain.ion_steps / wall_plain, "wall_s": wall_plain},
+        "noop_consumer": {"value": st_noop.ion_steps / wall_noop, "wall_s": wall_noop,
+                          "overhead": wall_noop / wall_plain - 1.0},
+        "download_loop": {"value": st_plain.ion_steps / wall_loop, "wall_s": wall_loop},
+        "snapshot_overhead": wall_snap / wall_plain - 1.0,
+        "download_loop_overhead": wall_loop / wall_plain - 1.0,
+        "snapshot_rate_GBps": len(snap_steps) * snap_bytes / wall_snap / 1e9,
+        "alive_fraction_end": st_snap.alive / n,
+        "roofline": {
+            "bound": "mfma", "achieved": achieved, "peak": MI355X_PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK_FP32_TFLOPS, "traffic": None,
+            "kernel": "step_f32_kernel<Boris, linear field, damping> (run without snapshots)",
+        },
+    }
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -752,7 +869,7 @@ def main():
         sys.exit(1)
     res = {"ehss": bench_ehss, "quadrupole": bench_quadrupole, "batched": bench_batched,
            "batched_ehss": bench_batched_ehss, "field_array": bench_field_array,
-           "ims": bench_ims}[args.workload](args)
+           "ims": bench_ims, "snapshots": bench_snapshots}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

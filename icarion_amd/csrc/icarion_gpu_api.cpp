@@ -34,6 +34,10 @@ hipError_t launch_step(int integrator, int field, int coll, int precision, int s
 hipError_t launch_rk45_step(int field, int coll, int precision, const StepParams& P, const IonPtrs& S,
                             const Rk45Ptrs& R, uint32_t block, hipStream_t stream);
 hipError_t launch_rk45_init(const Rk45Ptrs& R, uint32_t n, double t0, double dt, hipStream_t stream);
+hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* pos, double* vel, uint32_t* spec,
+                                uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count, uint32_t* host_count,
+                                uint32_t* next_count, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t* p, hipStream_t s);
 hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
                         const uint4* blob, uint32_t n_clouds, unsigned long long* counters, hipStream_t s,
                         const hipStream_t* side, hipEvent_t fork, const hipEvent_t* join);
@@ -222,6 +226,25 @@ struct icarion_gpu {
     hipEvent_t ev_fork = nullptr, ev_join[kBatchClasses - 1] = {nullptr, nullptr};
     int n_cu = 0;
     int clock_khz = 0;
+
+    // trajectory snapshots: kSnapSlots device staging slots, each holding one packed
+    // snapshot in full (positions [n][3], velocities [n][3], species [n], flags
+    // [n]: 53 B per ion) and the compacted records of the ions that changed
+    // since the previous snapshot (56 B each) with their count; the flags at
+    // the last snapshot; and the pinned host copy of the current snapshot that
+    // the callback reads, updated in full or by the changed records
+    struct SnapRing {
+        size_t cap = 0;                         // ions per slot
+        DevBuf<uint8_t> full[3], rec[3];        // kSnapSlots
+        DevBuf<uint32_t> count;                 // [kSnapSlots]
+        DevBuf<uint8_t> prev;
+        void* host_full = nullptr;              // hipHostMalloc, 53 B per ion
+        void* host_rec = nullptr;               // hipHostMalloc, 56 B per ion
+        uint32_t* host_count = nullptr;         // hipHostMalloc, [kSnapSlots]
+        uint32_t* host_count_dev = nullptr;     // its device address
+        hipEvent_t packed[3] = {nullptr, nullptr, nullptr};
+        hipEvent_t copied = nullptr;
+    } snap;
 
     // RK45: per-ion time, dt and step-control state; the engine record
     Rk45Cfg rk45{1e-8, 1e-6, 0.9, 1e-6, 1e3, 5.0, 0.1, 0.0, 1, 0};
@@ -509,8 +532,30 @@ void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
 }
 
 __global__ void zero_u64(unsigned long long* p, int n) {
-    const int i = threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+constexpr int kCounterWords = kCounterStripes * kCounterStride;
+
+// Clear the striped launch counters of a single cloud.
+void zero_counters(icarion_gpu* h, hipStream_t s) {
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, s, h->counters.p, kCounterWords);
+    hip_check(hipGetLastError(), "zero counters");
+}
+
+// Ion-steps and collisions summed over the stripes (stream-ordered read).
+void read_counters(icarion_gpu* h, hipStream_t s, uint64_t* ion_steps, uint64_t* collisions) {
+    std::vector<unsigned long long> c(kCounterWords);
+    hip_check(hipMemcpyAsync(c.data(), h->counters.p, c.size() * sizeof(c[0]), hipMemcpyDeviceToHost, s),
+              "read counters");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    uint64_t a = 0, b = 0;
+    for (int k = 0; k < kCounterStripes; ++k) {
+        a += c[size_t(k) * kCounterStride];
+        b += c[size_t(k) * kCounterStride + 1];
+    }
+    *ion_steps = a;
+    *collisions = b;
 }
 
 IonPtrs ion_ptrs(icarion_gpu* h) {
@@ -550,6 +595,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     apply_mixture(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
+    P.counter_stripes = kCounterStripes;
     P.rk45 = h->rk45;
     fill_fast_const(P);
     const IonPtrs S = ion_ptrs(h);
@@ -557,8 +603,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     Rk45Ptrs R{h->r_t.p, h->r_dt.p, h->r_err.p, h->r_k1x.p, h->r_k1y.p, h->r_k1z.p, h->r_fsal.p, h->r_eng.p};
     Rk45Engine e0{t0, dt, step_offset, 0ull, ~0ull, 0ull, 0ull, 0ull};
     hip_check(hipMemcpyAsync(h->r_eng.p, &e0, sizeof(e0), hipMemcpyHostToDevice, s), "upload engine record");
-    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, s, h->counters.p, 4);
-    hip_check(hipGetLastError(), "zero counters");
+    zero_counters(h, s);
     float ms = 0.0f;
     if (h->n > 0) {
         hip_check(launch_rk45_init(R, static_cast<uint32_t>(h->n), t0, dt, s), "rk45 init");
@@ -607,10 +652,9 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
                   "upload engine record");
     }
     Rk45Engine e1{};
-    unsigned long long c[4] = {0, 0, 0, 0};
+    uint64_t c[2] = {0, 0};
     hip_check(hipMemcpyAsync(&e1, h->r_eng.p, sizeof(e1), hipMemcpyDeviceToHost, s), "read engine record");
-    hip_check(hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, s), "read counters");
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    read_counters(h, s, &c[0], &c[1]);
     if (e1.failed)
         fail(ICARION_ERR_INVALID, "RK45: " + std::to_string(e1.failed) +
                                       " ion steps found no accepted step in 10 attempts");
@@ -637,9 +681,78 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     }
 }
 
+// Trajectory sampling of one icarion_gpu_step_snapshots call.
+struct SnapshotReq {
+    double interval;
+    bool first;
+    double* next;
+    icarion_snapshot_fn fn;
+    void* user;
+};
+
+// One planned snapshot: after local step k, at engine time t; next = the next
+// write time once it is recorded.
+struct SnapshotAt {
+    uint64_t k;
+    double t, next;
+};
+
+// The engine's sampling rule (SimulationEngine.cpp:275-278): after its first
+// step, and after every step whose end time reaches the next write time, which
+// then becomes that time + the interval (OutputManager.cpp:230, 246-253).  The
+// step end times are the table's start times + dt, the engine's accumulation.
+std::vector<SnapshotAt> plan_snapshots(const std::vector<StepEntry>& table, double dt, uint64_t n_steps,
+                                       const SnapshotReq& r) {
+    std::vector<SnapshotAt> out;
+    double next = *r.next;
+    for (uint64_t k = 0; k < n_steps; ++k) {
+        const double t = table[k].t + dt;
+        if ((k == 0 && r.first) || t >= next) {
+            next = t + r.interval;
+            out.push_back({k, t, next});
+        }
+    }
+    return out;
+}
+
+// Snapshots in flight: snapshot q is brought to the host once snapshot q + 2 is
+// queued, so two write intervals of steps keep the device busy meanwhile.
+constexpr int kSnapSlots = 3;
+constexpr size_t kSnapFullBytes = 53;     // per ion: 6 doubles, species, flags
+constexpr size_t kSnapRecordBytes = 56;   // per changed ion: 6 doubles, index, flags
+
+void ensure_snap_ring(icarion_gpu* h) {
+    auto& R = h->snap;
+    if (!R.copied) hip_check(hipEventCreateWithFlags(&R.copied, hipEventDisableTiming), "hipEventCreate");
+    for (int j = 0; j < kSnapSlots; ++j)
+        if (!R.packed[j]) hip_check(hipEventCreateWithFlags(&R.packed[j], hipEventDisableTiming), "hipEventCreate");
+    if (!R.host_count) {
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&R.host_count), kSnapSlots * sizeof(uint32_t),
+                                hipHostMallocMapped | hipHostMallocCoherent),
+                  "hipHostMalloc snapshot count");
+        hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&R.host_count_dev), R.host_count, 0),
+                  "hipHostGetDevicePointer");
+        R.count.resize(kSnapSlots);
+    }
+    if (R.cap >= h->n && R.host_full) return;
+    R.cap = 0;
+    if (R.host_full) (void)hipHostFree(R.host_full);
+    if (R.host_rec) (void)hipHostFree(R.host_rec);
+    R.host_full = R.host_rec = nullptr;
+    for (int j = 0; j < kSnapSlots; ++j) {
+        R.full[j].resize(kSnapFullBytes * h->n);
+        R.rec[j].resize(kSnapRecordBytes * h->n);
+    }
+    R.prev.resize(h->n);
+    hip_check(hipHostMalloc(&R.host_full, kSnapFullBytes * h->n, hipHostMallocDefault), "hipHostMalloc snapshot");
+    hip_check(hipHostMalloc(&R.host_rec, kSnapRecordBytes * h->n, hipHostMallocDefault), "hipHostMalloc snapshot");
+    R.cap = h->n;
+}
+
 void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
-             icarion_step_stats* stats) {
+             icarion_step_stats* stats, const SnapshotReq* snap = nullptr) {
     if (h->have_domain && h->phys.integrator == ICARION_INTEGRATOR_RK45) {
+        if (snap) fail(ICARION_ERR_UNSUPPORTED, "snapshots are taken on the fixed-step integrators");
         if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
         if (!h->have_state) fail(ICARION_ERR_STATE, "icarion_gpu_upload_state not called");
         if (!(dt > 0.0) || !std::isfinite(dt)) fail(ICARION_ERR_INVALID, "dt must be finite and > 0");
@@ -680,40 +793,162 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     apply_mixture(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
+    P.counter_stripes = kCounterStripes;
     fill_fast_const(P);
-    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(64), 0, h->stream, h->counters.p, 4);
-    hip_check(hipGetLastError(), "zero counters");
+    zero_counters(h, h->stream);
     double t_end = t0;
     float ms = 0.0f;
+    if (snap && n_steps > 0 && h->n == 0) {
+        // no ions: the schedule still runs, with empty snapshots
+        std::vector<StepEntry> tab;
+        append_table(tab, h->dom, false, h->phys.integrator, t0, dt, n_steps);
+        const std::vector<SnapshotAt> plan = plan_snapshots(tab, dt, n_steps, *snap);
+        for (size_t q = 0; q < plan.size(); ++q) {
+            *snap->next = plan[q].next;
+            if (snap->fn(snap->user, q, step_offset + plan[q].k, plan[q].t, 0, nullptr, nullptr, nullptr,
+                         nullptr) != 0)
+                fail(ICARION_ERR_STATE, "snapshot callback ended the run after step " +
+                                            std::to_string(step_offset + plan[q].k));
+        }
+    }
     if (n_steps > 0 && h->n > 0) {
         build_table(h, t0, dt, n_steps);
         t_end = h->table_host[n_steps - 1].t + dt;
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
         const IonPtrs S = ion_ptrs(h);
+        std::vector<SnapshotAt> plan;
+        if (snap) {
+            plan = plan_snapshots(h->table_host, dt, n_steps, *snap);
+            ensure_snap_ring(h);
+        }
+        auto& R = h->snap;
+        const size_t n = h->n;
+        size_t q_issued = 0, q_done = 0;
+        bool stopped = false;
+        uint64_t stop_step = 0;
+        uint8_t* const hf = static_cast<uint8_t*>(R.host_full);
+        double* const hpos = reinterpret_cast<double*>(hf);
+        double* const hvel = hpos + 3 * n;
+        uint32_t* const hspec = reinterpret_cast<uint32_t*>(hf + 48 * n);
+        uint8_t* const hflags = hf + 52 * n;
+        // Snapshot q_done moves to the host in two stages, each started as soon
+        // as the one before has finished, so the calling thread keeps queueing
+        // steps meanwhile: once its pack (and count) is done, the copy starts
+        // on a side stream - of the changed ions' records when they are few,
+        // else of the whole snapshot; once the copy is done, the records are
+        // scattered over the previous snapshot and fn is called.  block = wait
+        // (spinning) until snapshot q_done is delivered.
+        bool copying = false, compact = false;
+        size_t n_rec = 0;
+        auto ready = [](hipEvent_t e, bool block) {
+            for (;;) {
+                const hipError_t r = hipEventQuery(e);
+                if (r == hipSuccess) return true;
+                if (r != hipErrorNotReady) hip_check(r, "hipEventQuery");
+                if (!block) return false;
+            }
+        };
+        auto advance = [&](bool block) {
+            while (q_done < q_issued) {
+                const int j = static_cast<int>(q_done % kSnapSlots);
+                if (!copying) {
+                    if (!ready(R.packed[j], block)) return;
+                    n_rec = R.host_count[j];
+                    if (n_rec > n) fail(ICARION_ERR_HIP, "snapshot record count exceeds the ion count");
+                    compact = q_done > 0 && n_rec * 8 <= n;
+                    if (!compact)
+                        hip_check(hipMemcpyAsync(hf, R.full[j].p, kSnapFullBytes * n, hipMemcpyDeviceToHost,
+                                                 h->side[0]),
+                                  "snapshot copy");
+                    else if (n_rec)
+                        hip_check(hipMemcpyAsync(R.host_rec, R.rec[j].p, kSnapRecordBytes * n_rec,
+                                                 hipMemcpyDeviceToHost, h->side[0]),
+                                  "snapshot record copy");
+                    hip_check(hipEventRecord(R.copied, h->side[0]), "hipEventRecord");
+                    copying = true;
+                }
+                if (!ready(R.copied, block)) return;
+                copying = false;
+                if (compact) {
+                    const uint8_t* r = static_cast<const uint8_t*>(R.host_rec);
+                    for (size_t k = 0; k < n_rec; ++k, r += kSnapRecordBytes) {
+                        double pv[6];
+                        uint32_t idx, fl;
+                        std::memcpy(pv, r, sizeof(pv));
+                        std::memcpy(&idx, r + 48, sizeof(idx));
+                        std::memcpy(&fl, r + 52, sizeof(fl));
+                        if (idx >= n) fail(ICARION_ERR_HIP, "snapshot record index out of range");
+                        std::memcpy(hpos + 3 * size_t(idx), pv, 3 * sizeof(double));
+                        std::memcpy(hvel + 3 * size_t(idx), pv + 3, 3 * sizeof(double));
+                        hflags[idx] = static_cast<uint8_t>(fl);
+                    }
+                }
+                const SnapshotAt& a = plan[q_done];
+                *snap->next = a.next;
+                ++q_done;
+                if (snap->fn(snap->user, q_done - 1, step_offset + a.k, a.t, n, hpos, hvel, hspec, hflags) != 0) {
+                    stopped = true;
+                    stop_step = step_offset + a.k;
+                    return;
+                }
+                block = false;
+            }
+        };
+        // Pack snapshot q_issued on the step stream, its count published to
+        // pinned memory behind it; first free its slot if every slot is in flight.
+        auto issue = [&]() {
+            if (q_issued - q_done == kSnapSlots) advance(true);
+            if (stopped) return;
+            const int j = static_cast<int>(q_issued % kSnapSlots);
+            uint8_t* d = R.full[j].p;
+            double* pos = reinterpret_cast<double*>(d);
+            if (q_issued == 0) hip_check(launch_zero_u32(R.count.p, h->stream), "snapshot count clear");
+            hip_check(launch_pack_snapshot(S, static_cast<uint32_t>(n), q_issued == 0 ? 1 : 0, pos, pos + 3 * n,
+                                           reinterpret_cast<uint32_t*>(d + 48 * n), d + 52 * n, R.prev.p,
+                                           R.rec[j].p, R.count.p + j, R.host_count_dev + j,
+                                           R.count.p + (j + 1) % kSnapSlots, h->stream),
+                      "snapshot pack launch");
+            hip_check(hipEventRecord(R.packed[j], h->stream), "hipEventRecord");
+            ++q_issued;
+            advance(false);
+        };
         hip_check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
-        for (uint64_t k0 = 0; k0 < n_steps; k0 += spl) {
-            const uint64_t nk = std::min<uint64_t>(spl, n_steps - k0);
+        size_t q_next = 0;
+        uint64_t k0 = 0;
+        while (k0 < n_steps && !stopped) {
+            uint64_t k1 = std::min<uint64_t>(k0 + spl, n_steps);
+            if (q_next < plan.size()) k1 = std::min<uint64_t>(k1, plan[q_next].k + 1);
             P.table = h->table.p + k0;
             P.step0 = step_offset + k0;
-            P.n_steps = static_cast<uint32_t>(nk);
+            P.n_steps = static_cast<uint32_t>(k1 - k0);
             const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), launch_coll_kind(h),
                                              h->phys.precision,
                                              surrogate ? 1 : 0, P, S, h->block, h->stream);
             if (e == hipErrorInvalidValue)
                 fail(ICARION_ERR_UNSUPPORTED, "no kernel for this integrator/field/collision/precision");
             hip_check(e, "step kernel launch");
+            if (q_next < plan.size() && plan[q_next].k + 1 == k1) {
+                issue();
+                ++q_next;
+            } else if (q_done < q_issued) {
+                advance(false);
+            }
+            k0 = k1;
         }
         hip_check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+        while (q_done < q_issued && !stopped) advance(true);
+        if (copying) (void)hipStreamSynchronize(h->side[0]);
         hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
         hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+        if (stopped)
+            fail(ICARION_ERR_STATE, "snapshot callback ended the run after step " + std::to_string(stop_step) +
+                                        "; steps through " + std::to_string(step_offset + k0 - 1) + " ran");
     } else {
         for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
     }
     if (stats) {
-        unsigned long long c[4] = {0, 0, 0, 0};
-        hip_check(hipMemcpyAsync(c, h->counters.p, sizeof(c), hipMemcpyDeviceToHost, h->stream),
-                  "read counters");
-        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        uint64_t c[2] = {0, 0};
+        read_counters(h, h->stream, &c[0], &c[1]);
         stats->ion_steps = c[0];
         stats->collisions = c[1];
         stats->rejected = 0;
@@ -1376,7 +1611,7 @@ int icarion_gpu_create(const icarion_gpu_desc* desc, icarion_gpu** out) {
             hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
             hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
             hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
-            h->counters.resize(4);
+            h->counters.resize(kCounterWords);
             hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
             for (int k = 0; k < kBatchClasses - 1; ++k) {
                 hip_check(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking), "hipStreamCreate");
@@ -1423,6 +1658,17 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
+    for (int j = 0; j < 3; ++j) {
+        h->snap.full[j].release();
+        h->snap.rec[j].release();
+        if (h->snap.packed[j]) (void)hipEventDestroy(h->snap.packed[j]);
+    }
+    if (h->snap.copied) (void)hipEventDestroy(h->snap.copied);
+    h->snap.count.release();
+    h->snap.prev.release();
+    if (h->snap.host_full) (void)hipHostFree(h->snap.host_full);
+    if (h->snap.host_rec) (void)hipHostFree(h->snap.host_rec);
+    if (h->snap.host_count) (void)hipHostFree(h->snap.host_count);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1775,6 +2021,20 @@ int icarion_gpu_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uin
     return guarded([&] {
         if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
         do_step(h, t0, dt, n_steps, step_offset, stats);
+    });
+}
+
+int icarion_gpu_step_snapshots(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                               double write_interval_s, int write_first, double* next_write_s,
+                               icarion_snapshot_fn fn, void* user, icarion_step_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "handle is NULL");
+        if (!fn || !next_write_s) fail(ICARION_ERR_INVALID, "fn and next_write_s are required");
+        if (!(write_interval_s > 0.0) || !std::isfinite(write_interval_s))
+            fail(ICARION_ERR_INVALID, "write_interval_s must be finite and > 0");
+        if (std::isnan(*next_write_s)) fail(ICARION_ERR_INVALID, "*next_write_s is NaN");
+        const SnapshotReq r{write_interval_s, write_first != 0, next_write_s, fn, user};
+        do_step(h, t0, dt, n_steps, step_offset, stats, &r);
     });
 }
 

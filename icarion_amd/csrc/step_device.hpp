@@ -484,9 +484,13 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 __device__ __forceinline__ void add_counters(const StepParams& P, uint32_t pushed, uint32_t coll) {
     const uint32_t sp = wave_sum_u32(pushed);
     const uint32_t sc = wave_sum_u32(coll);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&P.counters[0], static_cast<unsigned long long>(sp));
-        atomicAdd(&P.counters[1], static_cast<unsigned long long>(sc));
+    if ((threadIdx.x & 63) == 0 && (sp | sc)) {
+        uint32_t s = 0;
+        if (P.counter_stripes > 1)
+            s = ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % static_cast<uint32_t>(P.counter_stripes)) *
+                kCounterStride;
+        if (sp) atomicAdd(&P.counters[s], static_cast<unsigned long long>(sp));
+        if (sc) atomicAdd(&P.counters[s + 1], static_cast<unsigned long long>(sc));
     }
 }
 

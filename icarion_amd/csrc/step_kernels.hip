@@ -687,6 +687,116 @@ hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint3
     return hipSuccess;
 }
 
+// ======================= trajectory snapshots ================================
+//
+// One pass over the SoA state per snapshot:
+//   * the full snapshot in OutputManager's layout (positions and velocities
+//     interleaved [n][3] in FP64, species, flags) into a device staging slot;
+//   * the records of the ions that may have changed since the previous
+//     snapshot: those active now, or whose flags changed (died or were born
+//     since).  Unborn and dead ions do not move, so the others hold the values
+//     the previous snapshot gave them.  Records are compacted per workgroup
+//     (wave ballots + one atomic per workgroup); their order across workgroups
+//     is arbitrary and each carries its ion index.
+// prev[] keeps every ion's flags at the last snapshot; full = 1 selects every ion.
+struct SnapRecord {
+    double p[3];
+    double v[3];
+    uint32_t idx;
+    uint32_t flags;
+};
+static_assert(sizeof(SnapRecord) == 56, "snapshot record layout");
+
+// 1024 threads per workgroup: the records of its 16 waves take one atomic.  No
+// thread leaves before the barriers (lanes past n carry sel = false).
+constexpr int kPackThreads = 1024;
+
+__global__ void __launch_bounds__(kPackThreads) pack_snapshot_kernel(const IonPtrs S, uint32_t n, int full,
+                                                                     double* __restrict__ pos, double* __restrict__ vel,
+                                                                     uint32_t* __restrict__ spec,
+                                                                     uint8_t* __restrict__ flags,
+                                                                     uint8_t* __restrict__ prev,
+                                                                     SnapRecord* __restrict__ rec,
+                                                                     uint32_t* __restrict__ count) {
+    __shared__ uint32_t wave_base[kPackThreads / 64];
+    __shared__ uint32_t block_base;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < n;
+    bool sel = false;
+    double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
+    uint8_t f = 0;
+    if (in) {
+        x = S.x[i]; y = S.y[i]; z = S.z[i];
+        vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
+        f = S.flags[i];
+        const size_t o = 3ull * i;
+        pos[o] = x; pos[o + 1] = y; pos[o + 2] = z;
+        vel[o] = vx; vel[o + 1] = vy; vel[o + 2] = vz;
+        spec[i] = S.species[i];
+        flags[i] = f;
+        sel = full || (f & 1) || f != prev[i];
+        prev[i] = f;
+    }
+    const unsigned long long m = __ballot(sel);
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const int wave = static_cast<int>(threadIdx.x >> 6);
+    if (lane == 0) wave_base[wave] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        const int n_waves = static_cast<int>(blockDim.x >> 6);
+        for (int w = 0; w < n_waves; ++w) {
+            const uint32_t c = wave_base[w];
+            wave_base[w] = total;
+            total += c;
+        }
+        block_base = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    if (sel) {
+        const uint32_t k = block_base + wave_base[wave] +
+                           static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+        SnapRecord r;
+        r.p[0] = x; r.p[1] = y; r.p[2] = z;
+        r.v[0] = vx; r.v[1] = vy; r.v[2] = vz;
+        r.idx = i;
+        r.flags = f;
+        rec[k] = r;
+    }
+}
+
+// After a pack: its record count to pinned host memory (a vector store, read by
+// the host once the stream has passed this kernel), and the counter of the
+// next pack cleared.
+__global__ void publish_count_kernel(const uint32_t* count, uint32_t* host_count, uint32_t* next_count) {
+    if (threadIdx.x == 0) {
+        host_count[0] = count[0];
+        next_count[0] = 0u;
+        __threadfence_system();
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p) {
+    if (threadIdx.x == 0) p[0] = 0u;
+}
+
+hipError_t launch_zero_u32(uint32_t* p, hipStream_t s) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+// count must be zero on entry (launch_zero_u32 before the first pack; each
+// publish clears next_count for the pack after it).
+hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* pos, double* vel, uint32_t* spec,
+                                uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count, uint32_t* host_count,
+                                uint32_t* next_count, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pack_snapshot_kernel, dim3((n + kPackThreads - 1) / kPackThreads), dim3(kPackThreads), 0, s,
+                       S, n, full, pos, vel, spec, flags, prev, static_cast<SnapRecord*>(rec), count);
+    hipLaunchKernelGGL(publish_count_kernel, dim3(1), dim3(64), 0, s, count, host_count, next_count);
+    return hipGetLastError();
+}
+
 // ---- launch dispatch --------------------------------------------------------
 
 using KernelFn = void (*)(const StepParams, const IonPtrs);

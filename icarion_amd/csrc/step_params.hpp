@@ -36,6 +36,13 @@ struct SpeciesDev {
     float pad;
 };
 
+// Launch counters of a single cloud are striped over cache lines so that the
+// waves' atomics (which on a multi-XCD device are served beyond the XCDs' L2)
+// do not serialise on one address: stripe s holds {ion-steps, collisions} at
+// counters[s * kCounterStride].
+constexpr int kCounterStripes = 64;
+constexpr int kCounterStride = 16;   // 128 B
+
 // Field-array terms a domain may superpose (CompositeFieldProvider).
 constexpr int kMaxFieldTerms = 4;
 // Components of an HSS gas mixture.
@@ -115,7 +122,7 @@ struct StepParams {
     int ou_kicks;
     int detect;              // arrival detection on
     int has_rf;
-    int reserved;
+    int counter_stripes;     // 0: counters = {ion-steps, collisions}; k: k stripes of kCounterStride
     double origin[3];
     double L;
     double R;

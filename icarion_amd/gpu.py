@@ -24,6 +24,7 @@ EXPORTED = (
     "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
+    "icarion_gpu_step_snapshots",
 )
 
 
@@ -78,6 +79,9 @@ def load(path=None):
     lib.icarion_gpu_upload_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
     lib.icarion_gpu_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
                                      P(_abi.StepStats)]
+    lib.icarion_gpu_step_snapshots.argtypes = [
+        C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64, C.c_double, C.c_int,
+        P(C.c_double), _abi.SnapshotFn, C.c_void_p, P(_abi.StepStats)]
     lib.icarion_gpu_download_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
     lib.icarion_gpu_synchronize.argtypes = [C.c_void_p]
     lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
@@ -375,6 +379,49 @@ class Handle:
         _check(self.lib.icarion_gpu_step(self.h, t0, dt, n_steps, step_offset, C.byref(st)),
                self.lib)
         return st
+
+    def step_snapshots(self, t0, dt, n_steps, write_interval, step_offset=0, write_first=True,
+                       next_write=None, on_snapshot=None, stop_after=None, copy=True):
+        """icarion_gpu_step with trajectory snapshots.
+
+        Snapshots follow SimulationEngine::run's sampling (SimulationEngine.cpp:275-278,
+        OutputManager.cpp:230, 246-253); next_write defaults to write_interval, the value
+        OutputManager::initialize sets (OutputManager.cpp:161).  Each snapshot is passed to
+        on_snapshot(dict) when given, else collected; stop_after=q asks the library to stop
+        after snapshot q.  copy=False hands on_snapshot views of the library's pinned
+        buffers, valid during the call only.  Returns (snapshots, next_write, stats)."""
+        nxt = C.c_double(write_interval if next_write is None else next_write)
+        out = []
+
+        def cb(_user, index, step, t, n, pos, vel, spec, flags):
+            if n:
+                snap = {
+                    "index": index, "step": step, "t": t,
+                    "positions": np.ctypeslib.as_array(pos, shape=(n, 3)),
+                    "velocities": np.ctypeslib.as_array(vel, shape=(n, 3)),
+                    "species": np.ctypeslib.as_array(spec, shape=(n,)),
+                    "flags": np.ctypeslib.as_array(flags, shape=(n,)),
+                }
+                if copy or on_snapshot is None:
+                    for k in ("positions", "velocities", "species", "flags"):
+                        snap[k] = snap[k].copy()
+            else:
+                snap = {"index": index, "step": step, "t": t,
+                        "positions": np.zeros((0, 3)), "velocities": np.zeros((0, 3)),
+                        "species": np.zeros(0, np.uint32), "flags": np.zeros(0, np.uint8)}
+            if on_snapshot is not None:
+                on_snapshot(snap)
+            else:
+                out.append(snap)
+            return 1 if (stop_after is not None and index >= stop_after) else 0
+
+        fn = _abi.SnapshotFn(cb)
+        st = _abi.StepStats()
+        _check(self.lib.icarion_gpu_step_snapshots(self.h, t0, dt, n_steps, step_offset,
+                                                   write_interval, 1 if write_first else 0,
+                                                   C.byref(nxt), fn, None, C.byref(st)),
+               self.lib)
+        return out, nxt.value, st
 
     def download_state(self, ens):
         s = ens.soa()

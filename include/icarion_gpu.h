@@ -430,6 +430,35 @@ int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
 
+/* One trajectory snapshot, in the layout OutputManager::log_step buffers and
+ * HDF5Writer writes (OutputManager.cpp:197-227, hdf5Writer.cpp:158-299):
+ * positions and velocities interleaved per ion [n][3] (FP64), the ion's
+ * species index and flags (bit 0 active, bit 1 born).  The arrays are valid
+ * during the call only.  Return 0 to continue, nonzero to stop the run. */
+typedef int (*icarion_snapshot_fn)(void *user, uint64_t index, uint64_t step, double t_s, size_t n,
+                                   const double *positions, const double *velocities,
+                                   const uint32_t *species, const uint8_t *flags);
+
+/* icarion_gpu_step with the trajectory sampling of SimulationEngine::run /
+ * OutputManager (SimulationEngine.cpp:268-283, OutputManager.cpp:246-253):
+ * after the first step when write_first is set, and after every step whose end
+ * time t reaches *next_write_s, which then becomes t + write_interval_s.
+ * Launches are cut at the snapshot steps.  Each snapshot is packed on the
+ * device, and it reaches pinned host memory while the steps that follow it run.
+ * After the first snapshot of a call, only the ions that may have changed
+ * since the previous snapshot cross PCIe (active ones, and ones that died or
+ * were born since), when they are few; fn always sees every ion.  fn is
+ * called on the calling thread, in snapshot order, one snapshot behind the
+ * steps queued, and it must not call into the same handle.
+ * *next_write_s is updated so that a following call continues the schedule.
+ * Fixed-step integrators (RK4, Boris); RK45 returns ICARION_ERR_UNSUPPORTED.
+ * With no ions, fn is called with n = 0 and NULL arrays.  A nonzero return of
+ * fn ends the run after the steps already queued (ICARION_ERR_STATE). */
+int icarion_gpu_step_snapshots(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
+                               uint64_t step_offset, double write_interval_s, int write_first,
+                               double *next_write_s, icarion_snapshot_fn fn, void *user,
+                               icarion_step_stats *stats);
+
 /* Step control of ICARION_INTEGRATOR_RK45 (RK45Strategy(const AdaptiveConfig&),
  * RK45Strategy.cpp:179-195: positive tolerances, safety factor in (0, 1)). */
 int icarion_gpu_set_rk45(icarion_gpu *h, const icarion_rk45_desc *desc);

@@ -93,6 +93,17 @@ public:
         return st;
     }
     void download(icarion_ion_soa& ions) { check(icarion_gpu_download_state(h_, &ions)); }
+    // Resident stepping with OutputManager's trajectory sampling: fn receives each
+    // snapshot; next_write_s carries the schedule from one call to the next
+    // (initially the write interval, as OutputManager::initialize sets it).
+    icarion_step_stats advance_sampled(double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                                       double write_interval_s, bool write_first, double& next_write_s,
+                                       icarion_snapshot_fn fn, void* user) {
+        icarion_step_stats st{};
+        check(icarion_gpu_step_snapshots(h_, t0, dt, n_steps, step_offset, write_interval_s,
+                                         write_first ? 1 : 0, &next_write_s, fn, user, &st));
+        return st;
+    }
 
     // One engine step on a host-owned ensemble.  false = "not handled, run the
     // CPU path" (the reference's step_batch / handle_batch return value).
