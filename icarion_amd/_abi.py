@@ -20,6 +20,7 @@ MAX_FIELD_TERMS = 4
 (WAVEFORM_CONSTANT, WAVEFORM_LINEAR, WAVEFORM_QUADRATIC, WAVEFORM_EXPONENTIAL, WAVEFORM_SINUSOIDAL,
  WAVEFORM_PWM, WAVEFORM_PULSED, WAVEFORM_ARBITRARY) = range(8)
 INTERP_LINEAR, INTERP_STEP, INTERP_CUBIC = range(3)
+WAVEFORM_LIBRARY, WAVEFORM_INLINE = range(2)
 SLOT_DC_AXIAL, SLOT_DC_QUAD, SLOT_DC_RADIAL, SLOT_RF_VOLTAGE, SLOT_RF_FREQUENCY = range(5)
 N_VOLTAGE_SLOTS = 5
 
@@ -160,16 +161,16 @@ class OracleCollisionTables(C.Structure):
 
 
 class WaveformDesc(C.Structure):
-    _fields_ = [("type", _i32), ("interp", _i32), ("clamp", _i32), ("reserved0", _i32),
+    _fields_ = [("type", _i32), ("interp", _i32), ("clamp", _i32), ("source", _i32),
                 ("p", _d * 8), ("times_s", C.POINTER(_d)), ("values", C.POINTER(_d)),
                 ("n_points", _u32), ("reserved1", _u32)]
 
 
-def waveform(type_, p=(), clamp=1, interp=0, points=()):
+def waveform(type_, p=(), clamp=1, interp=0, points=(), source=WAVEFORM_LIBRARY):
     """An icarion_waveform_desc; the returned object keeps its arrays alive."""
     import numpy as np
     w = WaveformDesc()
-    w.type, w.clamp, w.interp = type_, int(clamp), interp
+    w.type, w.clamp, w.interp, w.source = type_, int(clamp), interp, source
     for i, x in enumerate(p):
         w.p[i] = float(x)
     pts = np.asarray(list(points), dtype=np.float64).reshape(-1, 2)

@@ -195,6 +195,10 @@ struct icarion_gpu {
     DevBuf<StepEntry> table;
     DevBuf<unsigned long long> counters;
     std::vector<StepEntry> table_host;
+    // composite field-array term scales at every stage time (voltage waveforms)
+    DevBuf<double> term_scales;
+    std::vector<double> term_scales_host;
+    bool use_term_scales = false;
 
     // batched clouds (independent of the single-cloud state above)
     struct BatchCloud {
@@ -516,6 +520,35 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
     }
 }
 
+// CompositeFieldProvider::eval_voltage (CompositeFieldProvider.h:150-171): the
+// domain's constant, else a library waveform at t; an inline waveform gives 0.
+double composite_voltage(const icarion_gpu* h, int slot, double constant, double t) {
+    const HostWaveform* w = h->wave[slot].get();
+    if (!w) return constant;
+    if (w->d.source == ICARION_WAVEFORM_INLINE) return 0.0;
+    return w->eval(t);
+}
+
+// CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147).
+double composite_scale(const icarion_gpu* h, int j, double t) {
+    const auto& f = h->fsrc;
+    const icarion_domain_desc& d = h->dom;
+    switch (f.kind[j]) {
+        case ICARION_FIELD_SCALE_CONSTANT: return f.cst[j];
+        case ICARION_FIELD_SCALE_DC_AXIAL: return composite_voltage(h, ICARION_SLOT_DC_AXIAL, d.dc_axial_V, t);
+        case ICARION_FIELD_SCALE_DC_QUAD: return composite_voltage(h, ICARION_SLOT_DC_QUAD, d.dc_quad_V, t);
+        case ICARION_FIELD_SCALE_DC_RADIAL: return composite_voltage(h, ICARION_SLOT_DC_RADIAL, d.dc_radial_V, t);
+        default: {
+            const double V = composite_voltage(h, ICARION_SLOT_RF_VOLTAGE, d.rf_voltage_V, t);
+            const double fr = f.freq[j] > 0.0
+                                  ? f.freq[j]
+                                  : composite_voltage(h, ICARION_SLOT_RF_FREQUENCY, d.rf_frequency_Hz, t);
+            const double omega = 2.0 * kPiHost * fr;
+            return V * std::cos(omega * t + f.phase[j]);
+        }
+    }
+}
+
 void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
     h->table_host.clear();
     const bool stage_times = field_kind(h->dom) == FIELD_ARRAY && h->fsrc.n_terms >= 2;
@@ -525,6 +558,28 @@ void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
                                             h->dom.instrument != ICARION_INSTRUMENT_ORBITRAP);
     append_table(h->table_host, h->dom, rf, h->phys.integrator, t0, dt, n_steps, stage_times,
                  h->has_waves ? wf : nullptr);
+    // with waveforms, the composite's scales at every stage time go into a
+    // table of their own, and the stage values become its row numbers
+    h->use_term_scales = stage_times && h->has_waves;
+    if (h->use_term_scales) {
+        if (3 * n_steps > (1ull << 52)) fail(ICARION_ERR_INVALID, "too many steps for one call");
+        h->term_scales_host.assign(3 * n_steps * kMaxFieldTerms, 0.0);
+        const double half_dt = dt * 0.5;
+        for (uint64_t k = 0; k < n_steps; ++k) {
+            StepEntry& e = h->table_host[k];
+            const double ts[3] = {e.t, e.t + half_dt, e.t + dt};
+            for (int st = 0; st < 3; ++st)
+                for (int j = 0; j < h->fsrc.n_terms; ++j)
+                    h->term_scales_host[(3 * k + st) * kMaxFieldTerms + j] = composite_scale(h, j, ts[st]);
+            e.c0 = static_cast<double>(3 * k);
+            e.ch = static_cast<double>(3 * k + 1);
+            e.c1 = static_cast<double>(3 * k + 2);
+        }
+        h->term_scales.resize(h->term_scales_host.size());
+        hip_check(hipMemcpyAsync(h->term_scales.p, h->term_scales_host.data(),
+                                 h->term_scales_host.size() * sizeof(double), hipMemcpyHostToDevice, h->stream),
+                  "upload term scales");
+    }
     h->table.resize(std::max<size_t>(n_steps, 1));
     hip_check(hipMemcpyAsync(h->table.p, h->table_host.data(), n_steps * sizeof(StepEntry),
                              hipMemcpyHostToDevice, h->stream),
@@ -778,8 +833,8 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
         fail(ICARION_ERR_STATE, "EHSS surrogate selected but no parameters set");
     if (coll == COLL_EHSS && !surrogate && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
-    if (h->has_waves && field != FIELD_LINEAR)
-        fail(ICARION_ERR_UNSUPPORTED, "voltage waveforms apply to the analytic IMS / quadrupole fields");
+    if (h->has_waves && field == FIELD_ORBITRAP)
+        fail(ICARION_ERR_UNSUPPORTED, "the Orbitrap field runs with constant voltages");
     set_device(h);
     StepParams P = h->base;
     if (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
@@ -813,6 +868,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     }
     if (n_steps > 0 && h->n > 0) {
         build_table(h, t0, dt, n_steps);
+        P.term_scales = h->use_term_scales ? h->term_scales.p : nullptr;
         t_end = h->table_host[n_steps - 1].t + dt;
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
         const IonPtrs S = ion_ptrs(h);
@@ -1658,6 +1714,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->x.release(); h->y.release(); h->z.release(); h->vx.release(); h->vy.release(); h->vz.release();
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
+    h->term_scales.release();
     for (int j = 0; j < 3; ++j) {
         h->snap.full[j].release();
         h->snap.rec[j].release();
@@ -1883,6 +1940,8 @@ int icarion_gpu_set_waveforms(icarion_gpu* h, const icarion_waveform_desc* const
             if (!d) continue;
             if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
                 fail(ICARION_ERR_INVALID, "unknown waveform type");
+            if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
+                fail(ICARION_ERR_INVALID, "unknown waveform source");
             for (double v : d->p)
                 if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
             auto hw = std::make_unique<HostWaveform>();

@@ -86,7 +86,10 @@ __device__ __forceinline__ void locate_axis_f64(const double* __restrict__ ax, i
 
 // Time factor of field-array term j (CompositeFieldProvider::compute_scale_factor,
 // CompositeFieldProvider.h:117-147; the host resolves each kind to a, w, phi).
+// Scale of field-array term j at a stage: t is the stage time, or with voltage
+// waveforms the row of the host's scale table (wave-uniform either way).
 __device__ __forceinline__ double term_scale_f64(const StepParams& P, int j, double t) {
+    if (P.term_scales) return P.term_scales[static_cast<size_t>(t) * kMaxFieldTerms + j];
     return P.term_rf[j] ? P.term_a[j] * cos(P.term_w[j] * t + P.term_phi[j]) : P.term_a[j];
 }
 

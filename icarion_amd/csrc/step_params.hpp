@@ -157,6 +157,10 @@ struct StepParams {
     // per term: scale = a (constant kinds) or a cos(w t + phi) (RF kind)
     int term_rf[kMaxFieldTerms];
     double term_a[kMaxFieldTerms], term_w[kMaxFieldTerms], term_phi[kMaxFieldTerms];
+    // with voltage waveforms: every term's scale at every stage time, row r =
+    // 3 * step + stage (0 start, 1 mid point, 2 end), kMaxFieldTerms per row;
+    // the step table's stage values are then row numbers (NULL: closed form)
+    const double *term_scales;
     // HSS gas mixture (n_mix = 0: the single gas above): the participating
     // components with n > 0, in order; sigma of species s with component c at
     // mix_sigma[s * n_mix + c]

@@ -236,12 +236,21 @@ typedef struct icarion_gas_component {
 
 #define ICARION_MAX_GAS_COMPONENTS 4
 
+/* Where a domain's ValueOrWaveform takes its waveform from (WaveformConfig.h:
+ * 242-277).  The analytic fields evaluate both alike.  The time scales of a
+ * composite field array (two or more terms) resolve only constants and library
+ * references, and take an inline waveform as 0 (CompositeFieldProvider.h:150-171). */
+enum icarion_waveform_source {
+    ICARION_WAVEFORM_LIBRARY = 0,   /* waveform_ref into the domain's waveform_library */
+    ICARION_WAVEFORM_INLINE = 1     /* an inline waveform                              */
+};
+
 typedef struct icarion_waveform_desc {
     int32_t type;                 /* enum icarion_waveform_type                */
     int32_t interp;               /* ARBITRARY: enum icarion_waveform_interp   */
     int32_t clamp;                /* LINEAR / EXPONENTIAL / PWM: hold the end
                                      value after the end time                  */
-    int32_t reserved0;
+    int32_t source;               /* enum icarion_waveform_source              */
     double p[8];                  /* parameters, see icarion_waveform_type     */
     const double *times_s;        /* ARBITRARY                                 */
     const double *values;         /* ARBITRARY                                 */
@@ -353,15 +362,21 @@ int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
 int icarion_gpu_set_gas_mixture(icarion_gpu *h, const icarion_gas_component *components,
                                 uint32_t n_components, const double *sigma_m2, uint32_t n_species);
 
-/* Time-varying voltages of the analytic IMS / TIMS and quadrupole fields:
- * slots[k] (ICARION_N_VOLTAGE_SLOTS entries, NULL = the domain's constant)
- * replaces the value of slot k, evaluated at each stage time as
- * AnalyticalFieldModel::compute_ims_field / compute_quadrupole_rf_field do
- * (AnalyticalFieldModel.cpp:130-248: E_z = axial(t)/L, U_eff = quad(t) +
- * V(t) cos(2 pi f(t) t) where |V(t)| > 1e-12).  The host evaluates them into
- * the FP64 step table.  slots = NULL clears.  Fixed-step integrators (RK4,
- * Boris) with analytic fields; RK45, field arrays, the Orbitrap and the batch
- * keep constant voltages (ICARION_ERR_UNSUPPORTED when combined). */
+/* Time-varying voltages of a domain: slots[k] (ICARION_N_VOLTAGE_SLOTS
+ * entries, NULL = the domain's constant) replaces the value of slot k,
+ * evaluated at each stage time.
+ *   - Analytic IMS / TIMS and quadrupole fields, as
+ *     AnalyticalFieldModel::compute_ims_field / compute_quadrupole_rf_field do
+ *     (AnalyticalFieldModel.cpp:130-248: E_z = axial(t)/L, U_eff = quad(t) +
+ *     V(t) cos(2 pi f(t) t) where |V(t)| > 1e-12).
+ *   - The term scales of a composite field array (two or more terms), as
+ *     CompositeFieldProvider::compute_scale_factor does
+ *     (CompositeFieldProvider.h:117-171: DC kinds V(t); RF V(t) cos(2 pi f t +
+ *     phi) with f the term's own frequency, else f(t); an inline waveform
+ *     counts as 0 there).  One field array is not scaled (GridFieldProvider).
+ * The host evaluates them into FP64 tables per step.  slots = NULL clears.
+ * Fixed-step integrators (RK4, Boris); RK45, the Orbitrap and the batch keep
+ * constant voltages (ICARION_ERR_UNSUPPORTED when combined). */
 int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const *slots);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;

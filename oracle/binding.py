@@ -57,7 +57,8 @@ def load():
     lib.oracle_interpolate_field.argtypes = [P(_abi.OracleField), P(d), P(d)]
     lib.oracle_waveform_value.argtypes = [P(_abi.WaveformDesc), d]
     lib.oracle_waveform_value.restype = d
-    lib.oracle_field_term_scale.argtypes = [P(_abi.DomainDesc), P(_abi.OracleFieldTerm), d]
+    lib.oracle_field_term_scale.argtypes = [P(_abi.DomainDesc), P(_abi.OracleFieldTerm),
+                                            P(_abi.OracleField), d]
     lib.oracle_field_term_scale.restype = d
     lib.oracle_damping_gamma.argtypes = [P(_abi.DomainDesc), C.c_int, P(_abi.SpeciesDesc), d, d,
                                          P(d)]
@@ -137,9 +138,10 @@ class Field:
 class FieldTerms:
     """oracle_field on a rectilinear grid (axes xs, ys, zs, possibly
     non-uniform) with terms [(E[nz, ny, nx, 3], scale_kind, constant_scale,
-    frequency_Hz, phase_rad), ...]."""
+    frequency_Hz, phase_rad), ...] and optional voltage waveforms
+    {slot: WaveformDesc}."""
 
-    def __init__(self, axes, terms):
+    def __init__(self, axes, terms, waveforms=None):
         self.xs, self.ys, self.zs = (np.ascontiguousarray(a, dtype=np.float64) for a in axes)
         nx, ny, nz = len(self.xs), len(self.ys), len(self.zs)
         self.arrays = []
@@ -155,6 +157,9 @@ class FieldTerms:
         self.c = _abi.OracleField(nx, ny, nz, _dp(self.xs), _dp(self.ys), _dp(self.zs),
                                   _dp(first[0]), _dp(first[1]), _dp(first[2]), len(terms), 0,
                                   C.cast(self.terms, C.POINTER(_abi.OracleFieldTerm)))
+        self.slots = dict(waveforms or {})
+        for k, w in self.slots.items():
+            self.c.waveforms[k] = C.pointer(w)
 
 
 class Waveforms:
@@ -172,12 +177,14 @@ def waveform_value(w, t):
     return load().oracle_waveform_value(C.byref(w), t)
 
 
-def field_term_scale(domain, term, t):
-    """scale_i(t) of one term (kind, constant, frequency, phase) for the domain."""
+def field_term_scale(domain, term, t, field=None):
+    """scale_i(t) of one term (kind, constant, frequency, phase) for the domain;
+    field (FieldTerms or Waveforms) supplies voltage waveforms."""
     lib = load()
     kind, const, freq, phase = term
     tt = _abi.OracleFieldTerm(kind, 0, const, freq, phase)
-    return lib.oracle_field_term_scale(C.byref(domain), C.byref(tt), t)
+    fp = C.byref(field.c) if field is not None else C.POINTER(_abi.OracleField)()
+    return lib.oracle_field_term_scale(C.byref(domain), C.byref(tt), fp, t)
 
 
 def species_array(species):

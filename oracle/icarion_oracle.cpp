@@ -356,16 +356,30 @@ V3 interpolate_arrays(const oracle_field& f, const double* Ex, const double* Ey,
 
 // CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147);
 // ValueOrWaveform values are the domain's constants (eval_voltage, :150-172).
-double term_scale(const icarion_domain_desc& d, const oracle_field_term& term, double t) {
+// CompositeFieldProvider::eval_voltage (CompositeFieldProvider.h:150-171): the
+// constant, else a library waveform evaluated at t; an inline waveform is not
+// looked up there and gives 0.
+double composite_voltage(const oracle_field* f, int slot, double constant, double t) {
+    if (!f || !f->waveforms[slot]) return constant;
+    const icarion_waveform_desc& w = *f->waveforms[slot];
+    if (w.source == ICARION_WAVEFORM_INLINE) return 0.0;
+    return waveform_value(w, t);
+}
+
+// CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147).
+double term_scale(const icarion_domain_desc& d, const oracle_field_term& term, double t,
+                  const oracle_field* f = nullptr) {
     switch (term.scale_kind) {
         case ICARION_FIELD_SCALE_CONSTANT: return term.constant_scale;
-        case ICARION_FIELD_SCALE_DC_AXIAL: return d.dc_axial_V;
-        case ICARION_FIELD_SCALE_DC_QUAD: return d.dc_quad_V;
-        case ICARION_FIELD_SCALE_DC_RADIAL: return d.dc_radial_V;
+        case ICARION_FIELD_SCALE_DC_AXIAL: return composite_voltage(f, ICARION_SLOT_DC_AXIAL, d.dc_axial_V, t);
+        case ICARION_FIELD_SCALE_DC_QUAD: return composite_voltage(f, ICARION_SLOT_DC_QUAD, d.dc_quad_V, t);
+        case ICARION_FIELD_SCALE_DC_RADIAL: return composite_voltage(f, ICARION_SLOT_DC_RADIAL, d.dc_radial_V, t);
         case ICARION_FIELD_SCALE_RF: {
-            const double V = d.rf_voltage_V;
-            const double f = term.frequency_Hz > 0.0 ? term.frequency_Hz : d.rf_frequency_Hz;
-            const double omega = 2.0 * kPi * f;
+            const double V = composite_voltage(f, ICARION_SLOT_RF_VOLTAGE, d.rf_voltage_V, t);
+            const double fr = term.frequency_Hz > 0.0
+                                  ? term.frequency_Hz
+                                  : composite_voltage(f, ICARION_SLOT_RF_FREQUENCY, d.rf_frequency_Hz, t);
+            const double omega = 2.0 * kPi * fr;
             return V * std::cos(omega * t + term.phase_rad);
         }
         default: return 0.0;
@@ -381,7 +395,7 @@ V3 interpolate(const oracle_field& f, V3 pos, const icarion_domain_desc* d = nul
     V3 E{0.0, 0.0, 0.0};
     for (uint32_t i = 0; i < f.n_terms; ++i) {
         const oracle_field_term& term = f.terms[i];
-        const double scale = d ? term_scale(*d, term, t) : 1.0;
+        const double scale = d ? term_scale(*d, term, t, &f) : 1.0;
         const V3 Ei = interpolate_arrays(f, term.Ex, term.Ey, term.Ez, pos);
         E.x += scale * Ei.x;
         E.y += scale * Ei.y;
@@ -1171,8 +1185,8 @@ double oracle_orbitrap_r_for_z(double z, double R, double R_m) { return orbitrap
 double oracle_waveform_value(const icarion_waveform_desc* w, double t) { return waveform_value(*w, t); }
 
 double oracle_field_term_scale(const icarion_domain_desc* domain, const oracle_field_term* term,
-                               double t) {
-    return term_scale(*domain, *term, t);
+                               const oracle_field* field, double t) {
+    return term_scale(*domain, *term, t, field);
 }
 
 void oracle_interpolate_field(const oracle_field* field, const double pos[3], double E_out[3]) {

@@ -54,10 +54,11 @@ typedef struct oracle_field {
  * WaveformConfig.cpp:10-91). */
 double oracle_waveform_value(const icarion_waveform_desc *w, double t);
 
-/* CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-147)
- * with the domain's constant voltages. */
+/* CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-171):
+ * the domain's constant voltages, or the waveforms of field (may be NULL) by
+ * CompositeFieldProvider::eval_voltage's rule (inline waveforms give 0). */
 double oracle_field_term_scale(const icarion_domain_desc *domain, const oracle_field_term *term,
-                               double t);
+                               const oracle_field *field, double t);
 
 /* Gas mixture of the HSS mixture path: components and the per-species
  * cross sections sigma_m2[s * n_components + c]. */

@@ -54,8 +54,8 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
     if w.get("field") is not None:
         field = oracle.Field(*w["field"])
     if w.get("terms") is not None:
-        field = oracle.FieldTerms(w["axes"], w["terms"])
-    if w.get("waveforms") is not None:
+        field = oracle.FieldTerms(w["axes"], w["terms"], w.get("waveforms"))
+    elif w.get("waveforms") is not None and w.get("field") is None:
         field = oracle.Waveforms(w["waveforms"])
     mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
     off = oracle.EhssOffline(w["ehss_offline"]) if w.get("ehss_offline") is not None else None
@@ -315,6 +315,73 @@ def test_field_terms_reject_bad_input():
         with pytest.raises(gpu.IcarionError) as ei:
             h.upload_field_terms((xs, ys, zs), bad_kind)
         assert ei.value.code == _abi.ERR_INVALID
+
+
+def composite_waveforms(source=_abi.WAVEFORM_LIBRARY):
+    """An axial DC ramp, an amplitude-modulated RF voltage and a ramped RF
+    frequency for the composite field-array terms."""
+    return {
+        _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (200.0, 40.0, 0.0, 6e-8),
+                                          source=source),
+        _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (150.0, 60.0, 3e7, 0.2)),
+        _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_LINEAR, (1e6, 3e6, 1e-8, 5e-8)),
+    }
+
+
+def field_terms_waveform_workload(n, seed, **kw):
+    w = W.field_terms_workload(n=n, seed=seed, **kw)
+    E1 = w["terms"][1][0]
+    w["terms"][1] = (E1, _abi.FIELD_SCALE_RF, 0.0, 0.0, 0.3)   # f from the frequency waveform
+    w["waveforms"] = composite_waveforms()
+    return w
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_field_terms_waveforms_fp64(oracle, integrator):
+    """Composite field-array terms scaled by voltage waveforms (the host's scale
+    table at every stage time) against the oracle evaluating
+    CompositeFieldProvider's scales at every force call, ion by ion."""
+    w = field_terms_waveform_workload(2048, 13, integrator=integrator)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 1500, t0=2e-9)
+    oracle_run(oracle, w, o, 1500, t0=2e-9)
+    assert compare_ions(g, o, rtol=1e-9).all()
+    plain = dict(w)
+    plain.pop("waveforms")
+    c = w["ensemble"].copy()
+    gpu_run(plain, c, 1500, t0=2e-9)
+    assert compare_ions(g, c, rtol=1e-6).mean() < 0.5
+
+
+def test_field_terms_waveforms_hss_and_fp32(oracle):
+    w = field_terms_waveform_workload(2048, 14, collision=_abi.COLLISION_HSS)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 1500)
+    so = oracle_run(oracle, w, o, 1500)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    w2 = field_terms_waveform_workload(2048, 15)
+    g2, o2 = w2["ensemble"].copy(), w2["ensemble"].copy()
+    gpu_run(w2, g2, 1500, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w2, o2, 1500)
+    assert compare_ions(g2, o2, rtol=2e-5, pos_scale=1e-3, vel_scale=10.0).mean() >= 0.99
+
+
+def test_field_terms_inline_waveform_scales_to_zero():
+    """An inline waveform on a composite's DC scale counts as 0
+    (CompositeFieldProvider::eval_voltage looks up library references only):
+    bit for bit the run whose DC term has the constant scale 0."""
+    w = field_terms_waveform_workload(1024, 16)
+    w["waveforms"] = composite_waveforms(source=_abi.WAVEFORM_INLINE)
+    z = dict(w)
+    z["terms"] = list(w["terms"])
+    z["terms"][0] = (w["terms"][0][0], _abi.FIELD_SCALE_CONSTANT, 0.0, 0.0, 0.0)
+    z["waveforms"] = {k: v for k, v in composite_waveforms().items() if k != _abi.SLOT_DC_AXIAL}
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, a, 600)
+    gpu_run(z, b, 600)
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
 
 
 @pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])

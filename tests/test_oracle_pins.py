@@ -276,6 +276,49 @@ def test_composite_field_terms(oracle):
     np.testing.assert_allclose(oracle.field_E(d, p, 7.7e-7, field=one), ref, rtol=1e-10)
 
 
+def test_composite_field_terms_waveforms(oracle):
+    """CompositeFieldProvider::compute_scale_factor with voltage waveforms
+    (CompositeFieldProvider.h:117-171): a DC kind scales by its slot's waveform
+    at t, the RF kind by V(t) cos(2 pi f t + phase) with the term's own f, else
+    the frequency waveform's f(t); slots without a waveform keep the domain's
+    constant; an inline waveform counts as 0.  The waveform values come from
+    oracle.waveform_value, which is pinned bit for bit to the reference's own
+    WaveformConfig.cpp (test_golden_waveforms_bitexact)."""
+    d = W.ims_domain(EN_Td=5.0)
+    d.use_field_array = 1
+    d.dc_axial_V, d.dc_quad_V, d.dc_radial_V = 120.0, -35.0, 7.5
+    d.rf_voltage_V, d.rf_frequency_Hz = 250.0, 1.3e6
+    wf = {_abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (200.0, 40.0, 0.0, 6e-8)),
+          _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (150.0, 60.0, 3e7, 0.2)),
+          _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_LINEAR, (1e6, 3e6, 1e-8, 5e-8))}
+    inline = dict(wf)
+    inline[_abi.SLOT_DC_AXIAL] = _abi.waveform(_abi.WAVEFORM_LINEAR, (200.0, 40.0, 0.0, 6e-8),
+                                               source=_abi.WAVEFORM_INLINE)
+    E = np.zeros((2, 2, 2, 3))
+    axes = (np.array([0.0, 1.0]),) * 3
+    f = oracle.FieldTerms(axes, [(E, _abi.FIELD_SCALE_CONSTANT, 1.0, 0.0, 0.0)] * 2, wf)
+    fi = oracle.FieldTerms(axes, [(E, _abi.FIELD_SCALE_CONSTANT, 1.0, 0.0, 0.0)] * 2, inline)
+    V = lambda slot, t: oracle.waveform_value(wf[slot], t)
+    for t in np.linspace(0.0, 9e-8, 37):
+        dc = (_abi.FIELD_SCALE_DC_AXIAL, 0.0, 0.0, 0.0)
+        assert oracle.field_term_scale(d, dc, t, f) == V(_abi.SLOT_DC_AXIAL, t)
+        assert oracle.field_term_scale(d, dc, t, fi) == 0.0
+        assert oracle.field_term_scale(d, dc, t) == d.dc_axial_V
+        # slots without a waveform: the domain's constants
+        assert oracle.field_term_scale(d, (_abi.FIELD_SCALE_DC_QUAD, 0, 0, 0), t, f) == d.dc_quad_V
+        assert oracle.field_term_scale(d, (_abi.FIELD_SCALE_DC_RADIAL, 0, 0, 0), t, f) == d.dc_radial_V
+        assert oracle.field_term_scale(d, (_abi.FIELD_SCALE_CONSTANT, -3.5, 0, 0), t, f) == -3.5
+        own = oracle.field_term_scale(d, (_abi.FIELD_SCALE_RF, 0.0, 2.1e6, 0.4), t, f)
+        assert own == pytest.approx(V(_abi.SLOT_RF_VOLTAGE, t) * math.cos(2 * math.pi * 2.1e6 * t + 0.4),
+                                    rel=1e-13, abs=1e-12)
+        wave_f = oracle.field_term_scale(d, (_abi.FIELD_SCALE_RF, 0.0, 0.0, -1.1), t, f)
+        ref = V(_abi.SLOT_RF_VOLTAGE, t) * math.cos(
+            2 * math.pi * V(_abi.SLOT_RF_FREQUENCY, t) * t - 1.1)
+        assert wave_f == pytest.approx(ref, rel=1e-13, abs=1e-12)
+    # the ramp is live: its value at the end differs from the start
+    assert V(_abi.SLOT_DC_AXIAL, 6e-8) == pytest.approx(40.0)
+
+
 def test_golden_waveforms_bitexact(oracle):
     """Every ValueOrWaveform type of the reference (WaveformConfig.h,
     WaveformConfig.cpp, compiled unmodified: tests/golden/make_ref_waveforms.py),
