@@ -72,7 +72,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=("ehss", "quadrupole", "batched", "batched_ehss",
-                                           "field_array", "ims", "snapshots"),
+                                           "field_array", "ims", "snapshots", "domains"),
                     default="ehss")
     ap.add_argument("--steps", type=int, default=None,
                     help="timed engine steps (default: ehss 4000, quadrupole 100000)")
@@ -854,6 +854,53 @@ def bench_snapshots(args):
     }
 
 
+def bench_domains(args):
+    """The three-domain instrument (IMS region -> RF quadrupole -> IMS region,
+    HSS in each domain's gas, RK4 dt 1 ns) with 1M ions in FP64: ion-steps/s of
+    the multi-domain kernel (a domain lookup per ion and step, the domain's
+    field and gas per lane) and the oracle's multi-domain loop on the host."""
+    from icarion_amd import _abi, gpu
+    from icarion_amd import workloads as W
+    steps = args.steps if args.steps is not None else 10_000
+    w = W.multi_domain_workload(n=args.ions, collision=_abi.COLLISION_HSS)
+    log(f"domains: {args.ions} ions x {steps} steps, 3 domains")
+    ens = w["ensemble"].copy()
+    with gpu.Handle(device=0, steps_per_launch=args.steps_per_launch) as h:
+        h.set_domains(w["domains"], w["physics"], w["apertures"])
+        h.set_species(w["species"])
+        st, wall = timed_steps(h, ens, w["dt"], steps, args.warmup)
+        h.download_state(ens)
+        dom = h.download_domains(ens.n)
+    value = st.ion_steps / wall
+    launches = math.ceil(steps / (args.steps_per_launch or 2000))
+    res = {
+        "metric": "ion-steps/s on 1 MI355X, three-domain instrument (FP64)",
+        "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "three_domains_ims_quadrupole_ims_hss", "ions": args.ions,
+                   "dt_s": w["dt"], "engine_steps": steps},
+        "collisions": st.collisions, "alive_fraction": st.alive / args.ions,
+        "alive_by_domain": np.bincount(dom[(ens.active == 1) & (ens.born == 1)], minlength=3).tolist(),
+        "avg_launch_ms": st.device_ms / launches,
+    }
+    if not args.no_cpu:
+        from oracle import binding
+        threads = len(os.sched_getaffinity(0))
+        n_cpu = min(args.cpu_sample_ions or 200_000, args.ions)
+        k = min(args.cpu_sample_steps or 5000, steps)
+        e = w["ensemble"].subset(np.arange(n_cpu))
+        di = np.full(n_cpu, -1, np.int32)
+        t0 = time.perf_counter()
+        so = binding.run_multi(w["domains"], w["physics"], w["species"], e, di, 0.0, w["dt"], k,
+                               end_apertures=w["apertures"], threads=threads)
+        el = time.perf_counter() - t0
+        res["cpu_baseline"] = dict(value=so.ion_steps / el, unit="ion-steps/s", cores=threads, kind="port",
+                                   sample=f"{n_cpu} ions x {k} steps of the same workload (FP64 oracle, "
+                                          f"OpenMP over {threads} threads, {el:.1f} s)")
+    return res
+
+
 def main():
     args = parse()
     if args.copy_probe_mib:
@@ -869,7 +916,7 @@ def main():
         sys.exit(1)
     res = {"ehss": bench_ehss, "quadrupole": bench_quadrupole, "batched": bench_batched,
            "batched_ehss": bench_batched_ehss, "field_array": bench_field_array,
-           "ims": bench_ims, "snapshots": bench_snapshots}[args.workload](args)
+           "ims": bench_ims, "snapshots": bench_snapshots, "domains": bench_domains}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -38,6 +38,8 @@ hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* 
                                 uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count, uint32_t* host_count,
                                 uint32_t* next_count, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, hipStream_t s);
+hipError_t launch_multi_step(int integrator, int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
+                             hipStream_t s);
 hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
                         const uint4* blob, uint32_t n_clouds, unsigned long long* counters, hipStream_t s,
                         const hipStream_t* side, hipEvent_t fork, const hipEvent_t* join);
@@ -183,6 +185,20 @@ struct icarion_gpu {
     uint32_t off_species = 0;
     DevBuf<uint4> off_tab;
     DevBuf<double> off_sigma, off_mu;
+
+    // several domains (icarion_gpu_set_domains); n_dom = 0: the single domain
+    // above.  Per domain: the descriptor, end aperture^2; the z-sorted spans
+    // of the cylindrical domains; each ion's domain index
+    uint32_t n_dom = 0;
+    std::vector<icarion_domain_desc> doms;
+    std::vector<double> dom_ap_sq;
+    std::vector<DomainSpan> dom_spans;
+    bool dom_has_orbitrap = false;
+    DevBuf<StepParams> dom_params_d;
+    DevBuf<StepEntry> dom_tables;
+    DevBuf<DomainSpan> dom_spans_d;
+    DevBuf<double> dom_ap_d;
+    DevBuf<int32_t> ion_dom;
 
     size_t n = 0;
     bool have_state = false;
@@ -624,6 +640,7 @@ IonPtrs ion_ptrs(icarion_gpu* h) {
     S.death = h->death.p;
     S.arrival = h->arrival.p;
     S.ncoll = h->ncoll.p;
+    S.domain = h->ion_dom.p;
     return S;
 }
 
@@ -804,8 +821,16 @@ void ensure_snap_ring(icarion_gpu* h) {
     R.cap = h->n;
 }
 
+void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                   icarion_step_stats* stats);
+
 void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
              icarion_step_stats* stats, const SnapshotReq* snap = nullptr) {
+    if (h->have_domain && h->n_dom >= 2) {
+        if (snap) fail(ICARION_ERR_UNSUPPORTED, "snapshots are taken on single-domain handles");
+        do_step_multi(h, t0, dt, n_steps, step_offset, stats);
+        return;
+    }
     if (h->have_domain && h->phys.integrator == ICARION_INTEGRATOR_RK45) {
         if (snap) fail(ICARION_ERR_UNSUPPORTED, "snapshots are taken on the fixed-step integrators");
         if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
@@ -1031,6 +1056,118 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
 }
 
 
+// Alive / arrived counts and the counters into stats (stream-ordered reads).
+void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats) {
+    if (!stats) {
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        return;
+    }
+    uint64_t c[2] = {0, 0};
+    read_counters(h, h->stream, &c[0], &c[1]);
+    stats->ion_steps = c[0];
+    stats->collisions = c[1];
+    stats->rejected = 0;
+    stats->t_end_s = t_end;
+    stats->device_ms = ms;
+    std::vector<uint8_t> fl(h->n);
+    std::vector<double> ar(h->n);
+    if (h->n) {
+        hip_check(hipMemcpy(fl.data(), h->flags.p, h->n, hipMemcpyDeviceToHost), "read flags");
+        hip_check(hipMemcpy(ar.data(), h->arrival.p, h->n * sizeof(double), hipMemcpyDeviceToHost), "read arrival");
+    }
+    uint64_t alive = 0, arrived = 0;
+    for (size_t i = 0; i < h->n; ++i) {
+        if ((fl[i] & 3) == 3) ++alive;
+        if (!std::isnan(ar[i])) ++arrived;
+    }
+    stats->alive = alive;
+    stats->arrived = arrived;
+}
+
+// The engine loop over several domains: per-domain parameter records and
+// coefficient tables, the span list, then one multi-domain launch per
+// steps_per_launch steps.
+void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                   icarion_step_stats* stats) {
+    if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
+    if (!h->have_state) fail(ICARION_ERR_STATE, "icarion_gpu_upload_state not called");
+    if (!(dt > 0.0) || !std::isfinite(dt)) fail(ICARION_ERR_INVALID, "dt must be finite and > 0");
+    if (!std::isfinite(t0)) fail(ICARION_ERR_INVALID, "t0 must be finite");
+    if (h->phys.integrator != ICARION_INTEGRATOR_RK4 && h->phys.integrator != ICARION_INTEGRATOR_BORIS)
+        fail(ICARION_ERR_UNSUPPORTED, "several domains run with RK4 or Boris");
+    if (h->phys.precision != ICARION_PRECISION_FP64) fail(ICARION_ERR_UNSUPPORTED, "several domains run in FP64");
+    const int coll = coll_kind(h->phys);
+    if (coll == COLL_EHSS) fail(ICARION_ERR_UNSUPPORTED, "several domains: EHSS is not covered");
+    if (!h->mix.empty()) fail(ICARION_ERR_UNSUPPORTED, "several domains: gas mixtures are not covered");
+    if (h->has_waves) fail(ICARION_ERR_UNSUPPORTED, "several domains run with constant voltages");
+    for (const auto& d : h->doms)
+        if (d.use_field_array) fail(ICARION_ERR_UNSUPPORTED, "several domains run with analytic fields");
+    if (h->species_used > h->n_species)
+        fail(ICARION_ERR_STATE, "the species table no longer covers the resident ions' species indices");
+    set_device(h);
+    const uint32_t nd = h->n_dom;
+    zero_counters(h, h->stream);
+    double t_end = t0;
+    float ms = 0.0f;
+    if (n_steps > 0 && h->n > 0) {
+        std::vector<StepParams> Ps(nd);
+        std::vector<StepEntry> tabs;
+        tabs.reserve(size_t(nd) * n_steps);
+        for (uint32_t d = 0; d < nd; ++d) {
+            StepParams& P = Ps[d];
+            P = derive_params(h->doms[d], h->phys);
+            P.dt = dt;
+            P.species = h->species.p;
+            P.n_ions = static_cast<uint32_t>(h->n);
+            append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps);
+        }
+        t_end = tabs[n_steps - 1].t + dt;
+        h->dom_tables.resize(tabs.size());
+        copy_in(h->dom_tables, tabs.data(), tabs.size(), h->stream, "domain tables");
+        for (uint32_t d = 0; d < nd; ++d) Ps[d].table = h->dom_tables.p + size_t(d) * n_steps;
+        h->dom_params_d.resize(nd);
+        copy_in(h->dom_params_d, Ps.data(), nd, h->stream, "domain parameters");
+        h->dom_ap_d.resize(nd);
+        copy_in(h->dom_ap_d, h->dom_ap_sq.data(), nd, h->stream, "domain apertures");
+        h->dom_spans_d.resize(std::max<size_t>(h->dom_spans.size(), 1));
+        if (!h->dom_spans.empty())
+            copy_in(h->dom_spans_d, h->dom_spans.data(), h->dom_spans.size(), h->stream, "domain spans");
+        MultiParams M{};
+        M.dom = h->dom_params_d.p;
+        M.spans = h->dom_spans_d.p;
+        M.t_table = h->dom_tables.p;
+        M.ap_sq = h->dom_ap_d.p;
+        M.species = h->species.p;
+        M.counters = h->counters.p;
+        M.n_dom = static_cast<int>(nd);
+        M.n_spans = static_cast<int>(h->dom_spans.size());
+        M.has_orbitrap = h->dom_has_orbitrap ? 1 : 0;
+        M.counter_stripes = kCounterStripes;
+        M.handoff_tol = 1e-9;   // shared_boundary_handoff_domain's default (DomainManager.h:194-197)
+        M.dt = dt;
+        M.seed = h->phys.rng_seed;
+        M.n_ions = static_cast<uint32_t>(h->n);
+        const IonPtrs S = ion_ptrs(h);
+        const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
+        const int integ = h->phys.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0;
+        hip_check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+        for (uint64_t k0 = 0; k0 < n_steps; k0 += spl) {
+            M.k0 = static_cast<uint32_t>(k0);
+            M.step0 = step_offset + k0;
+            M.n_steps = static_cast<uint32_t>(std::min<uint64_t>(spl, n_steps - k0));
+            const hipError_t e = launch_multi_step(integ, coll, M, S, h->block, h->stream);
+            if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no multi-domain kernel for this case");
+            hip_check(e, "multi-domain step launch");
+        }
+        hip_check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    } else {
+        for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
+    }
+    finish_stats(h, t_end, ms, stats);
+}
+
 void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
     if (!ions) fail(ICARION_ERR_INVALID, "ions is NULL");
     if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species must precede upload_state");
@@ -1085,6 +1222,9 @@ void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
     copy_in(h->r_k1y, zeros.data(), n, s, "upload rk45 k1");
     copy_in(h->r_k1z, zeros.data(), n, s, "upload rk45 k1");
     copy_in(h->r_fsal, nofsal.data(), n, s, "upload rk45 fsal");
+    // domain index before the first step: none (set at each step start)
+    std::vector<int32_t> no_dom(n, -1);
+    copy_in(h->ion_dom, no_dom.data(), n, s, "upload domain index");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     h->n = n;
     h->have_state = true;
@@ -1715,6 +1855,8 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->mass.release(); h->charge.release(); h->birth.release(); h->death.release(); h->arrival.release();
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
     h->term_scales.release();
+    h->dom_params_d.release(); h->dom_tables.release(); h->dom_spans_d.release(); h->dom_ap_d.release();
+    h->ion_dom.release();
     for (int j = 0; j < 3; ++j) {
         h->snap.full[j].release();
         h->snap.rec[j].release();
@@ -1742,6 +1884,65 @@ int icarion_gpu_set_domain(icarion_gpu* h, const icarion_domain_desc* domain,
         h->phys = *physics;
         h->base = derive_params(*domain, *physics);
         h->have_domain = true;
+        h->n_dom = 0;
+    });
+}
+
+int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, const double* end_aperture_m,
+                            uint32_t n_domains, const icarion_physics_desc* physics) {
+    return guarded([&] {
+        if (!h || !domains || !physics) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (n_domains == 0) fail(ICARION_ERR_INVALID, "n_domains must be >= 1");
+        if (n_domains > ICARION_MAX_DOMAINS) fail(ICARION_ERR_UNSUPPORTED, "at most ICARION_MAX_DOMAINS domains");
+        for (uint32_t k = 0; k < n_domains; ++k) {
+            validate_domain(domains[k], *physics);
+            if (end_aperture_m && !(end_aperture_m[k] >= 0.0))
+                fail(ICARION_ERR_INVALID, "end_aperture_m must be >= 0");
+        }
+        h->dom = domains[0];
+        h->phys = *physics;
+        h->base = derive_params(domains[0], *physics);
+        h->have_domain = true;
+        h->n_dom = 0;
+        h->doms.clear();
+        h->dom_ap_sq.clear();
+        h->dom_spans.clear();
+        h->dom_has_orbitrap = false;
+        if (n_domains == 1) return;
+        h->doms.assign(domains, domains + n_domains);
+        // DomainManager's constructor (DomainManager.cpp:26-63): a span per
+        // cylindrical domain, z range and radius widened by 1e-12 m, with the
+        // index of its domain, sorted by z_min (std::sort, as there)
+        constexpr double kDomainEps = 1e-12;   // DOMAIN_BOUNDARY_EPSILON (DomainManager.h:42)
+        for (uint32_t k = 0; k < n_domains; ++k) {
+            const icarion_domain_desc& d = domains[k];
+            const double ap = (end_aperture_m && end_aperture_m[k] > 0.0) ? end_aperture_m[k] : d.radius_m;
+            h->dom_ap_sq.push_back(ap * ap);
+            if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) {
+                h->dom_has_orbitrap = true;
+                continue;
+            }
+            const double z0 = d.origin_m[2], z1 = z0 + d.length_m;
+            DomainSpan sp{};
+            sp.z_min = std::min(z0, z1) - kDomainEps;
+            sp.z_max = std::max(z0, z1) + kDomainEps;
+            sp.radius = d.radius_m + kDomainEps;
+            sp.idx = static_cast<int>(k);
+            h->dom_spans.push_back(sp);
+        }
+        std::sort(h->dom_spans.begin(), h->dom_spans.end(),
+                  [](const DomainSpan& a, const DomainSpan& b) { return a.z_min < b.z_min; });
+        h->n_dom = n_domains;
+    });
+}
+
+int icarion_gpu_download_domains(icarion_gpu* h, int32_t* domain_index) {
+    return guarded([&] {
+        if (!h || (!domain_index && h->n)) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_state) fail(ICARION_ERR_STATE, "no resident state");
+        set_device(h);
+        if (h->n) copy_out(domain_index, h->ion_dom, h->n, h->stream, "download domain index");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     });
 }
 

@@ -1,0 +1,259 @@
+// multi_kernels.hip - the fused ion step over several domains (FP64).
+//
+// A multi-domain instrument: every domain keeps its own geometry, analytic
+// field and gas, read per lane from a parameter record in global memory (a
+// wave may hold ions in several domains).  Per step and ion, the engine's
+// domain logic (SimulationEngineLifecycle.cpp:72-195) around the same push and
+// collision bodies as the single-domain kernels:
+//   prepare:  birth; the domain = DomainManager::find_domain_index of the
+//             position (DomainManager.cpp:66-109); none -> death at t;
+//   push:     RK4 (RK4Strategy.cpp:181-361) or Boris (BorisStrategy.cpp:18-98)
+//             in that domain's field, the domain fixed for the whole step;
+//   collide:  with that domain's gas (SimulationEngineCollisionRuntime.cpp:
+//             35-60 groups the ions by their step domain);
+//   finalize: find_domain_index of the new position; none -> the next domain
+//             when the ion left through the end face
+//             (forward_axial_bridge_domain, DomainManager.cpp:163-174), else
+//             death at t; found -> shared_boundary_handoff_domain
+//             (DomainManager.cpp:176-198) may move the index on; NaN kill.
+// An ion kept alive by the bridge is looked up again at the next step start.
+#include "step_device.hpp"
+
+namespace icarion {
+
+// DomainManager::find_domain_index: the spans (cylindrical domains sorted by
+// z_min) from std::lower_bound(z_max < z) on while z >= z_min, with the
+// radial prefilter on the global x, y; then the linear scan in domain order.
+// With an Orbitrap domain only the linear scan.
+__device__ __forceinline__ int find_domain(const MultiParams& M, d3 p) {
+    if (!M.has_orbitrap && M.n_spans > 0) {
+        int first = 0, count = M.n_spans;
+        while (count > 0) {
+            const int step = count / 2;
+            const int it = first + step;
+            if (M.spans[it].z_max < p.z) {
+                first = it + 1;
+                count -= step + 1;
+            } else {
+                count = step;
+            }
+        }
+        for (int s = first; s < M.n_spans; ++s) {
+            const DomainSpan sp = M.spans[s];
+            if (p.z < sp.z_min) break;
+            if (p.x * p.x + p.y * p.y > sp.radius * sp.radius) continue;
+            if (sp.idx >= 0 && sp.idx < M.n_dom && contains_f64(M.dom[sp.idx], p)) return sp.idx;
+        }
+    }
+    for (int d = 0; d < M.n_dom; ++d)
+        if (contains_f64(M.dom[d], p)) return d;
+    return -1;
+}
+
+// forward_axial_bridge_domain: the next domain when z reached this domain's end.
+__device__ __forceinline__ int bridge_domain(const MultiParams& M, int cur, d3 p) {
+    if (cur < 0 || cur + 1 >= M.n_dom) return -1;
+    const StepParams& D = M.dom[cur];
+    return (p.z >= D.origin[2] + D.L) ? cur + 1 : -1;
+}
+
+// shared_boundary_handoff_domain: still in the current domain but within the
+// tolerance of its end face and inside its end aperture -> the next domain.
+__device__ __forceinline__ int handoff_domain(const MultiParams& M, int cur, int found, d3 p) {
+    if (cur < 0 || found != cur || cur + 1 >= M.n_dom) return -1;
+    const StepParams& D = M.dom[cur];
+    const double local_z = p.z - D.origin[2];
+    if (local_z < D.L - M.handoff_tol) return -1;
+    const double dx = p.x - D.origin[0];
+    const double dy = p.y - D.origin[1];
+    return (dx * dx + dy * dy <= M.ap_sq[cur]) ? cur + 1 : -1;
+}
+
+// The hoisted damping factor of the single-domain kernels for one domain's gas
+// (DampingForce.cpp:150-177 / :269-271).
+__device__ __forceinline__ double damping_pre_f64(const StepParams& D, const SpeciesDev& sp, double m, double q) {
+    double lpre = 0.0;
+    if (D.damping_model == 3) {
+        if (D.alpha_gas > 0.0 && D.n > 0.0 && D.v_th > 0.0 && D.m_gas > 0.0) {
+            const double mu = (m * D.m_gas) / (m + D.m_gas);
+            const double cs1 = kPi * q * sqrt(D.alpha_gas / (4.0 * kPi * kEpsilon0 * mu));
+            lpre = D.n * cs1 * D.v_th * D.m_gas / (D.m_gas + m);
+        }
+    } else {
+        const double K0 = sp.K0_cm2_Vs;
+        if (K0 > 0.0 && m > 0.0 && D.n > 0.0) {
+            const double K0t = K0 * 1e-4 * sqrt(kStpTemp / fmax(D.T, 1.0));
+            lpre = q / ((K0t * (kLoschmidt / D.n)) * m);
+        }
+    }
+    return lpre;
+}
+
+template <int COLL>
+__device__ __forceinline__ d3 domain_force(const StepParams& D, d3 p, d3 v, double c, double ez, double q, double m,
+                                           double lpre) {
+    if (D.instrument == 3) return force_f64<FIELD_ORBITRAP, COLL>(D, p, v, c, ez, q, m, lpre);
+    return force_f64<FIELD_LINEAR, COLL>(D, p, v, c, ez, q, m, lpre);
+}
+
+template <int INTEG, int COLL>
+__device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S, const uint32_t i, uint32_t& pushed,
+                                          uint32_t& ncoll_launch) {
+    d3 pos{S.x[i], S.y[i], S.z[i]};
+    d3 vel{S.vx[i], S.vy[i], S.vz[i]};
+    const double m = S.mass[i];
+    const double q = S.charge[i];
+    const uint32_t sidx = S.species[i];
+    const SpeciesDev sp = M.species[sidx];
+    uint8_t flags = S.flags[i];
+    bool active = flags & 1, born = (flags >> 1) & 1;
+    const double birth = S.birth ? S.birth[i] : -INFINITY;
+    double death = S.death[i];
+    double arrival = S.arrival[i];
+    uint32_t nc = S.ncoll[i];
+    int cur = S.domain[i];
+    const double inv_m = 1.0 / m;
+    double lpre = 0.0;
+    int lpre_dom = -1;
+    int found = -1;
+    bool checked = false;   // found = the domain of the current position
+    const double dt = M.dt;
+    const double half_dt = dt * 0.5;
+    for (uint32_t k = 0; k < M.n_steps; ++k) {
+        const double t = M.t_table[M.k0 + k].t;
+        if (!born) {
+            if (t >= birth) {
+                born = true;
+                active = true;
+                checked = false;
+            } else {
+                continue;
+            }
+        }
+        if (!active) break;
+        const int dom = checked ? found : find_domain(M, pos);
+        if (dom < 0) {
+            active = false;
+            death = t;
+            break;
+        }
+        cur = dom;
+        const StepParams& D = M.dom[dom];
+        if constexpr (COLL == COLL_DAMPING) {
+            if (dom != lpre_dom) {
+                lpre = damping_pre_f64(D, sp, m, q);
+                lpre_dom = dom;
+            }
+        }
+        const StepEntry te = D.table[M.k0 + k];
+        const double z_before = pos.z;
+        if constexpr (INTEG == 0) {
+            const d3 p0 = pos, v0 = vel;
+            const d3 k1v = v0;
+            const d3 k1a = domain_force<COLL>(D, p0, v0, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 k2v = v0 + k1a * half_dt;
+            const d3 k2a = domain_force<COLL>(D, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k3v = v0 + k2a * half_dt;
+            const d3 k3a = domain_force<COLL>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k4v = v0 + k3a * dt;
+            const d3 k4a = domain_force<COLL>(D, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lpre) * inv_m;
+            const double h6 = dt / 6.0;
+            pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
+            vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
+        } else {
+            const d3 a = domain_force<COLL>(D, pos, vel, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 vm = vel + a * half_dt;
+            d3 vp = vm;
+            if (D.magnetic) {
+                const double qm = q * inv_m;
+                const d3 tv = d3{D.B[0], D.B[1], D.B[2]} * (qm * dt * 0.5);
+                const double s_f = 2.0 / (1.0 + dot(tv, tv));
+                const d3 sv = tv * s_f;
+                const d3 vprime = vm + cross(vm, tv);
+                vp = vm + cross(vprime, sv);
+            }
+            pos = pos + vp * dt;
+            vel = vp + a * half_dt;
+        }
+        ++pushed;
+        if constexpr (COLL != COLL_NONE) {
+            PhiloxStream rng(M.seed, i, M.step0 + k);
+            if (collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dt)) {
+                ++nc;
+                ++ncoll_launch;
+            }
+        }
+        if (D.detect && isnan(arrival)) {
+            if (z_before < D.det_z && pos.z >= D.det_z)
+                arrival = t + dt * (D.det_z - z_before) / (pos.z - z_before);
+        }
+        found = find_domain(M, pos);
+        if (found < 0) {
+            const int b = bridge_domain(M, cur, pos);
+            if (b < 0) {
+                active = false;
+                death = t;
+                break;
+            }
+            cur = b;
+            checked = false;
+        } else {
+            const int h = handoff_domain(M, cur, found, pos);
+            cur = h >= 0 ? h : found;
+            checked = true;
+        }
+        if (!(isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z) && isfinite(vel.x) &&
+              isfinite(vel.y) && isfinite(vel.z))) {
+            active = false;
+            death = t;
+            break;
+        }
+    }
+    S.x[i] = pos.x; S.y[i] = pos.y; S.z[i] = pos.z;
+    S.vx[i] = vel.x; S.vy[i] = vel.y; S.vz[i] = vel.z;
+    S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+    S.death[i] = death;
+    S.arrival[i] = arrival;
+    S.ncoll[i] = nc;
+    S.domain[i] = cur;
+}
+
+template <int INTEG, int COLL>
+__global__ void __launch_bounds__(256) step_multi_f64_kernel(const MultiParams M, const IonPtrs S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t pushed = 0, ncoll_launch = 0;
+    if (i < M.n_ions) multi_ion<INTEG, COLL>(M, S, i, pushed, ncoll_launch);
+    add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
+}
+
+template <int INTEG>
+static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
+                                     hipStream_t s) {
+    const dim3 grid((M.n_ions + block - 1) / block);
+    switch (coll) {
+        case COLL_NONE:
+            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_NONE>), grid, dim3(block), 0, s, M, S);
+            break;
+        case COLL_HSS:
+            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS>), grid, dim3(block), 0, s, M, S);
+            break;
+        case COLL_DAMPING:
+            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// integrator 0 RK4, 1 Boris; hipErrorInvalidValue for a collision kind with no
+// multi-domain kernel.
+hipError_t launch_multi_step(int integrator, int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
+                             hipStream_t s) {
+    if (M.n_ions == 0) return hipSuccess;
+    if (integrator == 0) return launch_multi_integ<0>(coll, M, S, block, s);
+    if (integrator == 1) return launch_multi_integ<1>(coll, M, S, block, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace icarion

@@ -484,17 +484,22 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
-__device__ __forceinline__ void add_counters(const StepParams& P, uint32_t pushed, uint32_t coll) {
+__device__ __forceinline__ void add_counters_to(unsigned long long* counters, int stripes, uint32_t pushed,
+                                                uint32_t coll) {
     const uint32_t sp = wave_sum_u32(pushed);
     const uint32_t sc = wave_sum_u32(coll);
     if ((threadIdx.x & 63) == 0 && (sp | sc)) {
         uint32_t s = 0;
-        if (P.counter_stripes > 1)
-            s = ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % static_cast<uint32_t>(P.counter_stripes)) *
+        if (stripes > 1)
+            s = ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % static_cast<uint32_t>(stripes)) *
                 kCounterStride;
-        if (sp) atomicAdd(&P.counters[s], static_cast<unsigned long long>(sp));
-        if (sc) atomicAdd(&P.counters[s + 1], static_cast<unsigned long long>(sc));
+        if (sp) atomicAdd(&counters[s], static_cast<unsigned long long>(sp));
+        if (sc) atomicAdd(&counters[s + 1], static_cast<unsigned long long>(sc));
     }
+}
+
+__device__ __forceinline__ void add_counters(const StepParams& P, uint32_t pushed, uint32_t coll) {
+    add_counters_to(P.counters, P.counter_stripes, pushed, coll);
 }
 
 }  // namespace icarion

@@ -72,6 +72,7 @@ struct IonPtrs {
     double *death;
     double *arrival;
     uint32_t *ncoll;
+    int32_t *domain;         // several domains: each ion's domain index
 };
 
 // RK45 adaptive step control (RK45Strategy::AdaptiveConfig, RK45Strategy.h:54-65).
@@ -219,6 +220,30 @@ inline void fill_fast_const(StepParams& P) {
         C.l2g[c] = (float)P.l2g[c];
     }
 }
+
+// ---- several domains ---------------------------------------------------------
+constexpr int kMaxDomains = 8;
+
+// One cylindrical domain's entry in DomainManager's z-sorted span list
+// (DomainManager.cpp:43-63): z range and radius widened by 1e-12 m.
+struct DomainSpan {
+    double z_min, z_max, radius;
+    int idx, pad;
+};
+
+struct MultiParams {
+    const StepParams *dom;          // per domain; table = its coefficients, k0-based
+    const DomainSpan *spans;        // sorted by z_min (n_spans; none with an Orbitrap)
+    const StepEntry *t_table;       // engine times (any domain's table)
+    const double *ap_sq;            // per domain: end aperture^2
+    const SpeciesDev *species;
+    unsigned long long *counters;
+    int n_dom, n_spans, has_orbitrap, counter_stripes;
+    double handoff_tol;             // shared_boundary_handoff_domain's tolerance
+    double dt;
+    uint64_t step0, seed;
+    uint32_t k0, n_steps, n_ions, pad;
+};
 
 // ---- batched clouds ----------------------------------------------------------
 // One tile = up to blockDim ions of one cloud, advanced through all of that

@@ -24,7 +24,7 @@ EXPORTED = (
     "icarion_gpu_set_rk45", "icarion_gpu_batch_upload", "icarion_gpu_batch_run",
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
-    "icarion_gpu_step_snapshots",
+    "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
 )
 
 
@@ -83,6 +83,9 @@ def load(path=None):
         C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64, C.c_double, C.c_int,
         P(C.c_double), _abi.SnapshotFn, C.c_void_p, P(_abi.StepStats)]
     lib.icarion_gpu_download_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
+    lib.icarion_gpu_set_domains.argtypes = [C.c_void_p, P(_abi.DomainDesc), P(C.c_double), C.c_uint32,
+                                            P(_abi.PhysicsDesc)]
+    lib.icarion_gpu_download_domains.argtypes = [C.c_void_p, P(C.c_int32)]
     lib.icarion_gpu_synchronize.argtypes = [C.c_void_p]
     lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
                                            C.c_uint64, P(_abi.StepStats)]
@@ -149,6 +152,24 @@ class Handle:
     def set_domain(self, domain, physics):
         _check(self.lib.icarion_gpu_set_domain(self.h, C.byref(domain), C.byref(physics)),
                self.lib)
+
+    def set_domains(self, domains, physics, end_apertures=None):
+        """Several domains (a list of DomainDesc) with optional end apertures (m)."""
+        arr = (_abi.DomainDesc * len(domains))(*domains)
+        ap = None
+        if end_apertures is not None:
+            ap = np.ascontiguousarray(end_apertures, dtype=np.float64)
+            assert len(ap) == len(domains)
+        app = ap.ctypes.data_as(C.POINTER(C.c_double)) if ap is not None else C.POINTER(C.c_double)()
+        _check(self.lib.icarion_gpu_set_domains(self.h, arr, app, len(domains), C.byref(physics)),
+               self.lib)
+
+    def download_domains(self, n):
+        """Every resident ion's domain index (n = the resident count)."""
+        out = np.empty(n, dtype=np.int32)
+        _check(self.lib.icarion_gpu_download_domains(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_int32))), self.lib)
+        return out
 
     def set_rk45(self, desc):
         """RK45 step control (an _abi.Rk45Desc, e.g. _abi.rk45_defaults(atol=...))."""

@@ -340,6 +340,29 @@ def field_terms_workload(n=4096, grid=(20, 16, 36), seed=3, precision=_abi.PRECI
                 ensemble=ens, dt=dt, n_steps=1000, axes=(xs, ys, zs), terms=terms)
 
 
+def multi_domain_workload(n=4096, seed=51, integrator=_abi.INTEGRATOR_RK4, collision=_abi.COLLISION_NONE,
+                          dt=1e-9, n_steps=10_000):
+    """A three-domain instrument along z: an IMS region (0-5 mm, radius 3 mm,
+    20 V axial, N2 at 10 Pa) with a 1.5 mm exit aperture, an RF quadrupole
+    (5-15 mm, r0 2 mm, 200 V at 3 MHz, He at 1 Pa) and a second IMS region
+    (15-20 mm, radius 3 mm, 10 V, N2 at 10 Pa).  Ions of 100-500 Da start in
+    the first region (Gaussian around z = 1 mm) moving along +z at 1.5 km/s on
+    top of 300 K thermal velocities, so that they cross both boundaries, and
+    many are lost where the quadrupole's radius is smaller than the drift
+    region's."""
+    d0 = ims_domain(origin_z=0.0, length=5e-3, radius=3e-3, axial_V=20.0, gas="N2", pressure_Pa=10.0)
+    d1 = quadrupole_domain(r0=2e-3, V=200.0, f=3e6, length=10e-3, origin_z=5e-3, gas="He", pressure_Pa=1.0)
+    d2 = ims_domain(origin_z=15e-3, length=5e-3, radius=3e-3, axial_V=10.0, gas="N2", pressure_Pa=10.0)
+    rng = np.random.default_rng(seed + 3)
+    masses = rng.uniform(100.0, 500.0, n)
+    ens = gaussian_cloud(n, masses, (0.8e-3, 0.8e-3, 0.5e-3), center=(0.0, 0.0, 1e-3), seed=seed)
+    ens.vz += 1500.0
+    sp = [species(sigma_hss=150e-20, ccs=150e-20, K0=2.0)]
+    return dict(name="multi_domain", domains=[d0, d1, d2], apertures=[1.5e-3, 0.0, 0.0],
+                physics=physics(integrator, collision, seed=seed), species=sp, atoms=None,
+                ensemble=ens, dt=dt, n_steps=n_steps)
+
+
 def ehss_cluster_counts(n, seed=0, lo=20, hi=500):
     """Atom counts of the EHSS clusters: uniform integers in [lo, hi]."""
     return np.random.default_rng(seed).integers(lo, hi + 1, n).astype(np.uint32)

@@ -445,6 +445,33 @@ int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
 
+#define ICARION_MAX_DOMAINS 8
+
+/* Several domains on one handle: a multi-domain instrument as DomainManager
+ * holds it (DomainManager.cpp:16-229) with the engine's per-step domain logic
+ * (SimulationEngineLifecycle.cpp:72-195):
+ *   - at each step start an ion takes the first domain that contains it, by
+ *     find_domain_index's z-sorted span search and then its linear scan, and
+ *     dies if none does;
+ *   - it moves in that domain's field and collides with that domain's gas;
+ *   - after the push, an ion in no domain lives on only when it left its
+ *     domain through the end face into the next one
+ *     (forward_axial_bridge_domain); its domain index follows
+ *     shared_boundary_handoff_domain, with end aperture end_aperture_m[k]
+ *     (0 or NULL: the radius).
+ * Replaces any domain set before; n_domains = 1 is icarion_gpu_set_domain.
+ * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
+ * voltages, RK4 or Boris, FP64, no / HSS / Langevin / Friction collisions, one
+ * species table for all domains.  Other combinations return
+ * ICARION_ERR_UNSUPPORTED when stepped. */
+int icarion_gpu_set_domains(icarion_gpu *h, const icarion_domain_desc *domains,
+                            const double *end_aperture_m, uint32_t n_domains,
+                            const icarion_physics_desc *physics);
+
+/* Every resident ion's domain index after the last step (the ensemble's
+ * domain_index; -1 before the ion's first step).  n = the resident count. */
+int icarion_gpu_download_domains(icarion_gpu *h, int32_t *domain_index);
+
 /* One trajectory snapshot, in the layout OutputManager::log_step buffers and
  * HDF5Writer writes (OutputManager.cpp:197-227, hdf5Writer.cpp:158-299):
  * positions and velocities interleaved per ion [n][3] (FP64), the ion's

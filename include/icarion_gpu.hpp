@@ -77,6 +77,17 @@ public:
                                       static_cast<uint32_t>(atoms_xyzr.size() / 4), 1));
         ready_ = true;
     }
+    // Several domains (icarion_gpu_set_domains): one descriptor and end aperture
+    // per DomainConfig, in the config's order.
+    void configure_domains(const std::vector<icarion_domain_desc>& domains, const std::vector<double>& end_aperture_m,
+                           const icarion_physics_desc& physics, const std::vector<icarion_species_desc>& species) {
+        if (!end_aperture_m.empty() && end_aperture_m.size() != domains.size())
+            throw Error(ICARION_ERR_INVALID, "one end aperture per domain");
+        check(icarion_gpu_set_domains(h_, domains.data(), end_aperture_m.empty() ? nullptr : end_aperture_m.data(),
+                                      static_cast<uint32_t>(domains.size()), &physics));
+        check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()), nullptr, 0, 1));
+        ready_ = true;
+    }
     // Step control of ICARION_INTEGRATOR_RK45 (RK45Strategy(const AdaptiveConfig&)).
     void set_rk45(const icarion_rk45_desc& cfg) { check(icarion_gpu_set_rk45(h_, &cfg)); }
     void upload_field_array(uint32_t nx, uint32_t ny, uint32_t nz, const double origin[3],

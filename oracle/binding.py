@@ -76,6 +76,12 @@ def load():
                                d, d, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
                                P(_abi.StepStats), P(_abi.OracleCollisionTables)]
     lib.oracle_run.restype = C.c_int
+    lib.oracle_find_domain.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(d)]
+    lib.oracle_find_domain.restype = C.c_int
+    lib.oracle_run_multi.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(_abi.PhysicsDesc),
+                                     P(_abi.SpeciesDesc), C.c_uint32, P(_abi.IonSoA), P(C.c_int32),
+                                     d, d, C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats)]
+    lib.oracle_run_multi.restype = C.c_int
     lib.oracle_mt_ehss_offline.argtypes = [C.c_uint64, P(_abi.DomainDesc), C.c_uint32, C.c_uint32,
                                            P(d), P(d), d, P(d), d, P(C.c_uint32), P(d), P(d)]
     lib.oracle_mt_ehss_offline.restype = C.c_int
@@ -288,6 +294,37 @@ def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=Non
                         C.byref(st), _tables_ptr(mixture, ehss_offline))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed ({rc})")
+    return st
+
+
+def _domains(domains, end_apertures):
+    arr = (_abi.DomainDesc * len(domains))(*domains)
+    ap = None if end_apertures is None else np.ascontiguousarray(end_apertures, dtype=np.float64)
+    app = _dp(ap) if ap is not None else C.POINTER(C.c_double)()
+    return arr, ap, app
+
+
+def find_domain(domains, pos, end_apertures=None):
+    """DomainManager::find_domain_index of pos over the domains (-1: none)."""
+    arr, ap, app = _domains(domains, end_apertures)
+    return load().oracle_find_domain(arr, app, len(domains), vec(*pos))
+
+
+def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
+              end_apertures=None, threads=0):
+    """The engine loop over several domains on ens and domain_index (int32,
+    in/out) in place; returns StepStats."""
+    lib = load()
+    arr, ap, app = _domains(domains, end_apertures)
+    sp = species_array(species)
+    soa = ens.soa()
+    st = _abi.StepStats()
+    assert domain_index.dtype == np.int32 and domain_index.flags.c_contiguous
+    rc = lib.oracle_run_multi(arr, app, len(domains), C.byref(physics), sp, len(species), C.byref(soa),
+                              domain_index.ctypes.data_as(C.POINTER(C.c_int32)), t0, dt, n_steps,
+                              step_offset, threads, C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_multi failed ({rc})")
     return st
 
 

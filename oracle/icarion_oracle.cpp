@@ -1113,6 +1113,133 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
     return pushed;
 }
 
+// DomainManager of a multi-domain instrument (DomainManager.cpp:16-109,
+// 163-198): spans of the cylindrical domains sorted by z_min, the lookup, the
+// forward axial bridge and the shared-boundary handoff.
+struct Domains {
+    struct Span {
+        double z_min, z_max, radius;
+        int idx;
+    };
+    const icarion_domain_desc* d;
+    uint32_t n;
+    std::vector<double> ap_sq;
+    std::vector<Span> spans;
+    bool has_orbitrap = false;
+
+    Domains(const icarion_domain_desc* doms, const double* end_aperture_m, uint32_t count) : d(doms), n(count) {
+        const double eps = 1e-12;   // DOMAIN_BOUNDARY_EPSILON (DomainManager.h:42)
+        for (uint32_t i = 0; i < n; ++i) {
+            const double ap = (end_aperture_m && end_aperture_m[i] > 0.0) ? end_aperture_m[i] : d[i].radius_m;
+            ap_sq.push_back(ap * ap);
+            if (d[i].instrument == ICARION_INSTRUMENT_ORBITRAP) {
+                has_orbitrap = true;
+                continue;
+            }
+            const double z0 = d[i].origin_m[2], z1 = z0 + d[i].length_m;
+            spans.push_back(Span{std::min(z0, z1) - eps, std::max(z0, z1) + eps, d[i].radius_m + eps,
+                                 static_cast<int>(i)});
+        }
+        std::sort(spans.begin(), spans.end(), [](const Span& a, const Span& b) { return a.z_min < b.z_min; });
+    }
+    int linear(V3 p) const {
+        for (uint32_t i = 0; i < n; ++i)
+            if (contains(d[i], p)) return static_cast<int>(i);
+        return -1;
+    }
+    // find_domain_index (DomainManager.cpp:66-109)
+    int find(V3 p) const {
+        if (has_orbitrap || spans.empty()) return linear(p);
+        auto it = std::lower_bound(spans.begin(), spans.end(), p.z,
+                                   [](const Span& s, double z) { return s.z_max < z; });
+        for (; it != spans.end(); ++it) {
+            if (p.z < it->z_min) break;
+            if ((p.x * p.x + p.y * p.y) > it->radius * it->radius) continue;
+            const int idx = it->idx;
+            if (idx >= 0 && idx < static_cast<int>(n) && contains(d[idx], p)) return idx;
+        }
+        return linear(p);
+    }
+    // forward_axial_bridge_domain (DomainManager.cpp:163-174)
+    int bridge(int cur, V3 p) const {
+        if (cur < 0 || static_cast<uint32_t>(cur + 1) >= n) return -1;
+        const double cur_end = d[cur].origin_m[2] + d[cur].length_m;
+        return (p.z >= cur_end) ? cur + 1 : -1;
+    }
+    // shared_boundary_handoff_domain (DomainManager.cpp:176-198), tolerance 1e-9 m
+    int handoff(int cur, int found, V3 p) const {
+        const double tol = 1e-9;
+        if (cur < 0 || found != cur || static_cast<uint32_t>(cur + 1) >= n) return -1;
+        const double local_z = p.z - d[cur].origin_m[2];
+        if (local_z < d[cur].length_m - tol) return -1;
+        const double dx = p.x - d[cur].origin_m[0];
+        const double dy = p.y - d[cur].origin_m[1];
+        return (dx * dx + dy * dy <= ap_sq[cur]) ? cur + 1 : -1;
+    }
+};
+
+// One ion through the engine loop over several domains (SimulationEngine.cpp:
+// 374-428; prepare / finalize of SimulationEngineLifecycle.cpp:72-195): the
+// domain found at the step start carries the push and the collision; after
+// the push the lookup, the bridge and the handoff decide life and the index.
+uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const icarion_species_desc& sp,
+                       IonCtx& ion, int& dom_idx, const double* t_table, double dt, uint64_t n_steps,
+                       uint32_t ion_index, uint64_t step0, uint64_t& collisions, std::vector<V3>& scratch) {
+    uint64_t pushed = 0;
+    const IonTables tb;
+    for (uint64_t k = 0; k < n_steps; ++k) {
+        const double t = t_table[k];
+        if (!ion.born && t >= ion.birth) {
+            ion.born = 1;
+            ion.active = 1;
+        }
+        if (!ion.active || !ion.born) continue;
+        const int dom = D.find(ion.pos);
+        if (dom < 0) {
+            ion.active = 0;
+            ion.death = t;
+            continue;
+        }
+        dom_idx = dom;
+        const icarion_domain_desc& d = D.d[dom];
+        const double z_before = ion.pos.z;
+        if (ph.integrator == ICARION_INTEGRATOR_BORIS)
+            boris_step(d, nullptr, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+        else
+            rk4_step(d, nullptr, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+        ++pushed;
+        if (ph.collision_model != ICARION_COLLISION_NONE) {
+            PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
+            if (collide(d, ph, sp, nullptr, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
+                ++ion.ncoll;
+                ++collisions;
+            }
+        }
+        if (std::isfinite(d.detector_z_m) && std::isnan(ion.arrival)) {
+            if (z_before < d.detector_z_m && ion.pos.z >= d.detector_z_m)
+                ion.arrival = t + dt * (d.detector_z_m - z_before) / (ion.pos.z - z_before);
+        }
+        const int found = D.find(ion.pos);
+        if (found < 0) {
+            const int b = D.bridge(dom_idx, ion.pos);
+            if (b < 0) {
+                ion.active = 0;
+                ion.death = t;
+                continue;
+            }
+            dom_idx = b;
+        } else {
+            const int h = D.handoff(dom_idx, found, ion.pos);
+            dom_idx = h >= 0 ? h : found;
+        }
+        if (!finite3(ion.pos) || !finite3(ion.vel)) {
+            ion.active = 0;
+            ion.death = t;
+        }
+    }
+    return pushed;
+}
+
 // Per-species views of a run's collision tables.
 struct RunTables {
     const oracle_collision_tables* t = nullptr;
@@ -1292,6 +1419,81 @@ void oracle_push_one(const icarion_domain_desc* domain, const icarion_physics_de
         rk4_step(*domain, field, physics->collision_model, *sp, mass_kg, charge_C, p, v, t, dt);
     pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
     vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
+}
+
+int oracle_find_domain(const icarion_domain_desc* domains, const double* end_aperture_m, uint32_t n_domains,
+                       const double pos[3]) {
+    const Domains D(domains, end_aperture_m, n_domains);
+    return D.find(V3{pos[0], pos[1], pos[2]});
+}
+
+int oracle_run_multi(const icarion_domain_desc* domains, const double* end_aperture_m, uint32_t n_domains,
+                     const icarion_physics_desc* physics, const icarion_species_desc* species, uint32_t n_species,
+                     icarion_ion_soa* ions, int32_t* domain_index, double t0, double dt, uint64_t n_steps,
+                     uint64_t step_offset, int n_threads, icarion_step_stats* stats) {
+    if (!domains || n_domains == 0 || !physics || !species || !ions || !domain_index || n_species == 0)
+        return ICARION_ERR_INVALID;
+    const Domains D(domains, end_aperture_m, n_domains);
+    const size_t n = ions->n;
+    std::vector<double> t_table(n_steps + 1);
+    double t = t0;
+    for (uint64_t k = 0; k <= n_steps; ++k) {
+        t_table[k] = t;
+        t = t + dt;
+    }
+    uint64_t pushed_total = 0, coll_total = 0, alive = 0, arrived = 0;
+#ifdef _OPENMP
+    if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+    (void)n_threads;
+#endif
+    #pragma omp parallel reduction(+ : pushed_total, coll_total, alive, arrived)
+    {
+        std::vector<V3> scratch;
+        #pragma omp for schedule(dynamic, 64)
+        for (long long ii = 0; ii < static_cast<long long>(n); ++ii) {
+            const size_t i = static_cast<size_t>(ii);
+            const uint32_t s = ions->species[i];
+            const icarion_species_desc& sp = species[s < n_species ? s : 0];
+            IonCtx ion;
+            ion.pos = V3{ions->x[i], ions->y[i], ions->z[i]};
+            ion.vel = V3{ions->vx[i], ions->vy[i], ions->vz[i]};
+            ion.m = ions->mass_kg[i];
+            ion.q = ions->charge_C[i];
+            ion.active = ions->active[i];
+            ion.born = ions->born ? ions->born[i] : 1;
+            ion.birth = ions->birth_time_s ? ions->birth_time_s[i] : t0;
+            ion.death = ions->death_time_s ? ions->death_time_s[i] : -1.0;
+            ion.arrival = ions->arrival_time_s ? ions->arrival_time_s[i] : NAN;
+            ion.ncoll = ions->n_collisions ? ions->n_collisions[i] : 0;
+            int dom = domain_index[i];
+            uint64_t coll = 0;
+            const uint64_t pushed = run_ion_multi(D, *physics, sp, ion, dom, t_table.data(), dt, n_steps,
+                                                  static_cast<uint32_t>(i), step_offset, coll, scratch);
+            ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
+            ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;
+            ions->active[i] = ion.active;
+            if (ions->born) ions->born[i] = ion.born;
+            if (ions->death_time_s) ions->death_time_s[i] = ion.death;
+            if (ions->arrival_time_s) ions->arrival_time_s[i] = ion.arrival;
+            if (ions->n_collisions) ions->n_collisions[i] = ion.ncoll;
+            domain_index[i] = dom;
+            pushed_total += pushed;
+            coll_total += coll;
+            if (ion.active && ion.born) ++alive;
+            if (!std::isnan(ion.arrival)) ++arrived;
+        }
+    }
+    if (stats) {
+        stats->ion_steps = pushed_total;
+        stats->collisions = coll_total;
+        stats->alive = alive;
+        stats->arrived = arrived;
+        stats->t_end_s = t_table[n_steps];
+        stats->rejected = 0;
+        stats->device_ms = 0.0;
+    }
+    return ICARION_OK;
 }
 
 int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* physics,

@@ -54,6 +54,19 @@ typedef struct oracle_field {
  * WaveformConfig.cpp:10-91). */
 double oracle_waveform_value(const icarion_waveform_desc *w, double t);
 
+/* DomainManager::find_domain_index (DomainManager.cpp:66-109) over n_domains
+ * domains with their end apertures (NULL: the radii). */
+int oracle_find_domain(const icarion_domain_desc *domains, const double *end_aperture_m, uint32_t n_domains,
+                       const double pos[3]);
+
+/* The engine loop over several domains (analytic fields, constant voltages,
+ * RK4 / Boris, Philox streams); domain_index[n] in/out is each ion's domain
+ * index (-1 before its first step). */
+int oracle_run_multi(const icarion_domain_desc *domains, const double *end_aperture_m, uint32_t n_domains,
+                     const icarion_physics_desc *physics, const icarion_species_desc *species, uint32_t n_species,
+                     icarion_ion_soa *ions, int32_t *domain_index, double t0, double dt, uint64_t n_steps,
+                     uint64_t step_offset, int n_threads, icarion_step_stats *stats);
+
 /* CompositeFieldProvider::compute_scale_factor (CompositeFieldProvider.h:117-171):
  * the domain's constant voltages, or the waveforms of field (may be NULL) by
  * CompositeFieldProvider::eval_voltage's rule (inline waveforms give 0). */

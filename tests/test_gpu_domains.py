@@ -1,0 +1,162 @@
+"""Several domains on the device (icarion_gpu_set_domains) against the oracle's
+multi-domain engine loop.
+
+Identical Philox streams on both sides, so ions are compared one by one:
+  * collisionless runs: every ion to 1e-9 relative, the same domain index,
+    liveness and death time;
+  * stochastic runs (HSS, Langevin + OU): >= 99 % of ions to 1e-9 (a
+    collision decision within rounding of its threshold can flip), the
+    collision count within 0.3 %.
+The hand-derived bridge and handoff cases of the oracle's own tests
+(tests/test_oracle_domains.py) run on the device and must agree exactly.
+"""
+import numpy as np
+import pytest
+
+from icarion_amd import _abi
+from icarion_amd import workloads as W
+from test_gpu_parity import compare_ions
+from test_oracle_domains import _gap_domains, _one_ion
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None, apertures=None,
+              steps_per_launch=0, h=None):
+    from icarion_amd import gpu
+    own = h is None
+    if own:
+        h = gpu.Handle(steps_per_launch=steps_per_launch)
+    try:
+        if own:
+            h.set_domains(domains or w["domains"], physics or w["physics"],
+                          apertures if apertures is not None else w.get("apertures"))
+            h.set_species(w["species"])
+            h.upload_state(ens)
+        st = h.step(t0, w["dt"], n_steps, step_offset)
+        h.download_state(ens)
+        dom = h.download_domains(ens.n)
+    finally:
+        if own:
+            h.close()
+    return st, dom
+
+
+def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None,
+                 apertures=None, dom=None):
+    di = np.full(ens.n, -1, np.int32) if dom is None else dom
+    st = oracle.run_multi(domains or w["domains"], physics or w["physics"], w["species"], ens, di,
+                          t0, w["dt"], n_steps, step_offset=step_offset,
+                          end_apertures=apertures if apertures is not None else w.get("apertures"))
+    return st, di
+
+
+@pytest.mark.parametrize("integrator,collision", [
+    (_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE),
+    (_abi.INTEGRATOR_BORIS, _abi.COLLISION_NONE),
+    (_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS),
+    (_abi.INTEGRATOR_BORIS, _abi.COLLISION_LANGEVIN),
+])
+def test_three_domains_ion_by_ion(oracle, integrator, collision):
+    w = W.multi_domain_workload(n=2048, seed=52, integrator=integrator, collision=collision)
+    if collision == _abi.COLLISION_LANGEVIN:
+        w["physics"].ou_thermal_kicks = 1
+    print(f"domains integ={integrator} coll={collision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9)
+    ok &= dg == do
+    ok &= g.death_time_s == o.death_time_s
+    if collision == _abi.COLLISION_NONE:
+        assert ok.all(), np.flatnonzero(~ok)[:10]
+        assert sg.ion_steps == so.ion_steps
+    else:
+        assert ok.mean() >= 0.99
+        if collision == _abi.COLLISION_HSS:
+            assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    assert sg.alive == pytest.approx(so.alive, rel=0.02, abs=5)
+    # the run crosses every domain: live ions sit in several, dead ones died in several
+    assert len(set(do[o.active == 1].tolist())) >= 2 or len(set(do.tolist())) >= 3
+
+
+def test_bridge_and_handoff_cases_match_the_oracle(oracle):
+    w = {"species": [W.species()], "dt": 1e-9}
+    ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE)
+    # bridge into a gap: alive one step with the next index, dead at the next start
+    for n_steps in (1, 2):
+        g, o = _one_ion(4.999e-3, vz=5e5), _one_ion(4.999e-3, vz=5e5)
+        _, dg = gpu_multi(w, g, n_steps, physics=ph, domains=_gap_domains(), apertures=[2e-3, 0.0])
+        _, do = oracle_multi(oracle, w, o, n_steps, physics=ph, domains=_gap_domains(),
+                             apertures=[2e-3, 0.0])
+        assert (g.active[0], dg[0], g.death_time_s[0]) == (o.active[0], do[0], o.death_time_s[0])
+        assert g.z[0] == o.z[0]
+    # handoff within 1e-9 m of the end face, inside / outside the aperture
+    d0 = W.ims_domain(origin_z=0.0, length=5e-3, radius=3e-3, axial_V=0.0, gas="N2", pressure_Pa=10.0)
+    d1 = W.ims_domain(origin_z=5e-3, length=5e-3, radius=1e-3, axial_V=0.0, gas="N2", pressure_Pa=10.0)
+    phb = W.physics(_abi.INTEGRATOR_BORIS, _abi.COLLISION_NONE)
+    for r, ap, expect in ((0.5e-3, 1.5e-3, 1), (2.0e-3, 1.5e-3, 0), (2.0e-3, 0.0, 1)):
+        g = _one_ion(5e-3 - 0.5e-9, r=r)
+        _, dg = gpu_multi(w, g, 3, physics=phb, domains=[d0, d1], apertures=[ap, 0.0])
+        assert g.active[0] == 1 and dg[0] == expect, (r, ap)
+
+
+def test_split_calls_and_launch_shape():
+    w = W.multi_domain_workload(n=1024, seed=53, collision=_abi.COLLISION_HSS)
+    a, b, c = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+    _, da = gpu_multi(w, a, 3000)
+    from icarion_amd import gpu
+    with gpu.Handle() as h:
+        h.set_domains(w["domains"], w["physics"], w["apertures"])
+        h.set_species(w["species"])
+        h.upload_state(b)
+        st = h.step(0.0, w["dt"], 1300, 0)
+        st = h.step(st.t_end_s, w["dt"], 1700, 1300)
+        h.download_state(b)
+        db = h.download_domains(b.n)
+    _, dc = gpu_multi(w, c, 3000, steps_per_launch=97)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+        np.testing.assert_array_equal(getattr(a, f), getattr(c, f), err_msg=f)
+    np.testing.assert_array_equal(da, db)
+    np.testing.assert_array_equal(da, dc)
+
+
+def test_one_domain_equals_set_domain():
+    from test_gpu_parity import gpu_run
+    w = W.multi_domain_workload(n=1024, seed=54, collision=_abi.COLLISION_HSS)
+    single = dict(w, domain=w["domains"][0])
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(single, a, 2000)
+    gpu_multi(w, b, 2000, domains=[w["domains"][0]], apertures=[0.0])
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+
+
+def test_unsupported_combinations():
+    from icarion_amd import gpu
+    w = W.multi_domain_workload(n=64, seed=55)
+    p = w["physics"]
+    cases = [
+        W.physics(p.integrator, p.collision_model, precision=_abi.PRECISION_FP32_KAHAN),
+        W.physics(_abi.INTEGRATOR_RK45, p.collision_model),
+        W.physics(p.integrator, _abi.COLLISION_EHSS),
+    ]
+    for ph in cases:
+        with gpu.Handle() as h:
+            h.set_domains(w["domains"], ph, w["apertures"])
+            h.set_species(w["species"])
+            h.upload_state(w["ensemble"].copy())
+            with pytest.raises(gpu.IcarionError) as ei:
+                h.step(0.0, w["dt"], 5)
+            assert ei.value.code == _abi.ERR_UNSUPPORTED
+    with gpu.Handle() as h:
+        h.set_domains(w["domains"], p, w["apertures"])
+        h.set_species(w["species"])
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step_snapshots(0.0, w["dt"], 10, 2 * w["dt"])
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_domains(w["domains"] * 3, p)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
