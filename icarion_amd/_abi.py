@@ -155,9 +155,15 @@ class OracleEhssOffline(C.Structure):
                 ("mu_stride", C.POINTER(_u32)), ("sigma_eff", C.POINTER(_d)), ("mu", C.POINTER(_d))]
 
 
+class OracleEhssOrientations(C.Structure):
+    _fields_ = [("n_species", _u32), ("reserved0", _u32), ("n_orient", C.POINTER(_u32)),
+                ("quat_wxyz", C.POINTER(_d)), ("area_m2", C.POINTER(_d))]
+
+
 class OracleCollisionTables(C.Structure):
     _fields_ = [("mixture", C.POINTER(OracleGasMixture)),
-                ("ehss_offline", C.POINTER(OracleEhssOffline))]
+                ("ehss_offline", C.POINTER(OracleEhssOffline)),
+                ("ehss_orient", C.POINTER(OracleEhssOrientations))]
 
 
 class WaveformDesc(C.Structure):

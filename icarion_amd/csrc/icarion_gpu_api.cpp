@@ -185,6 +185,11 @@ struct icarion_gpu {
     uint32_t off_species = 0;
     DevBuf<uint4> off_tab;
     DevBuf<double> off_sigma, off_mu;
+    // EHSS orientation samples (per species {first orientation, orientations};
+    // each orientation's rotation and area; none when ori_species == 0)
+    uint32_t ori_species = 0;
+    DevBuf<uint2> ori_tab;
+    DevBuf<double> ori_R, ori_area;
 
     // several domains (icarion_gpu_set_domains); n_dom = 0: the single domain
     // above.  Per domain: the descriptor, end aperture^2; the z-sorted spans
@@ -313,7 +318,8 @@ int coll_kind(const icarion_physics_desc& p) {
 int launch_coll_kind(const icarion_gpu* h) {
     const int c = coll_kind(h->phys);
     if (c == COLL_HSS && !h->mix.empty()) return COLL_HSS_MIX;
-    if (c == COLL_EHSS && h->off_species && h->phys.ehss_mode == ICARION_EHSS_GEOMETRY) return COLL_EHSS_TAB;
+    if (c == COLL_EHSS && (h->off_species || h->ori_species) && h->phys.ehss_mode == ICARION_EHSS_GEOMETRY)
+        return COLL_EHSS_TAB;
     return c;
 }
 
@@ -526,6 +532,13 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
         P.ehss_off = h->off_tab.p;
         P.ehss_off_sigma = h->off_sigma.p;
         P.ehss_off_mu = h->off_mu.p;
+    }
+    if (h->ori_species) {
+        if (h->ori_species != h->n_species)
+            fail(ICARION_ERR_STATE, "the EHSS orientation samples' species count differs from the species table");
+        P.ehss_ori = h->ori_tab.p;
+        P.ehss_ori_R = h->ori_R.p;
+        P.ehss_ori_area = h->ori_area.p;
     }
     for (int c = 0; c < kMaxGasComponents; ++c) {
         const bool on = c < P.n_mix;
@@ -1837,6 +1850,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     for (auto& a : h->ax32) a.release();
     h->mix_sigma.release();
     h->off_tab.release(); h->off_sigma.release(); h->off_mu.release();
+    h->ori_tab.release(); h->ori_R.release(); h->ori_area.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
     h->r_k1z.release(); h->r_fsal.release(); h->r_eng.release();
     h->b_params_d.release(); h->b_ions_d.release(); h->b_kinds_d.release(); h->b_queue.release();
@@ -2208,6 +2222,70 @@ int icarion_gpu_set_ehss_offline(icarion_gpu* h, const uint32_t* n_orient, const
         copy_in(h->off_mu, mu, static_cast<size_t>(n_mu), h->stream, "upload offline mu");
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         h->off_species = n_species;
+    });
+}
+
+// CollisionGeometry::quaternion_to_rotation (CollisionGeometry.cpp:73-104):
+// normalise, then the rotation matrix, row-major.
+void quaternion_rotation(const double q[4], double R[9]) {
+    double w = q[0], x = q[1], y = q[2], z = q[3];
+    const double norm = std::sqrt(w * w + x * x + y * y + z * z);
+    if (norm <= 0.0) {
+        const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        std::memcpy(R, I, sizeof(I));
+        return;
+    }
+    w /= norm;
+    x /= norm;
+    y /= norm;
+    z /= norm;
+    R[0] = 1.0 - 2.0 * (y * y + z * z);
+    R[1] = 2.0 * (x * y - z * w);
+    R[2] = 2.0 * (x * z + y * w);
+    R[3] = 2.0 * (x * y + z * w);
+    R[4] = 1.0 - 2.0 * (x * x + z * z);
+    R[5] = 2.0 * (y * z - x * w);
+    R[6] = 2.0 * (x * z - y * w);
+    R[7] = 2.0 * (y * z + x * w);
+    R[8] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+int icarion_gpu_set_ehss_orientations(icarion_gpu* h, const uint32_t* n_orient, const double* quat_wxyz,
+                                      const double* area_m2, uint32_t n_species) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (n_species == 0) {
+            h->ori_species = 0;
+            h->ori_tab.release();
+            h->ori_R.release();
+            h->ori_area.release();
+            return;
+        }
+        if (!n_orient) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_species || n_species != h->n_species)
+            fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
+        std::vector<uint2> tab(n_species);
+        uint64_t total = 0;
+        for (uint32_t sp = 0; sp < n_species; ++sp) {
+            tab[sp] = make_uint2(static_cast<uint32_t>(total), n_orient[sp]);
+            total += n_orient[sp];
+            if (total > 0xFFFFFFFFull / 9) fail(ICARION_ERR_UNSUPPORTED, "too many EHSS orientation samples");
+        }
+        if (total == 0) fail(ICARION_ERR_INVALID, "no species has orientation samples");
+        if (!quat_wxyz || !area_m2) fail(ICARION_ERR_INVALID, "NULL argument");
+        std::vector<double> R(9 * total);
+        for (uint64_t i = 0; i < total; ++i) {
+            for (int c = 0; c < 4; ++c)
+                if (!std::isfinite(quat_wxyz[4 * i + c])) fail(ICARION_ERR_INVALID, "quaternions must be finite");
+            if (!std::isfinite(area_m2[i])) fail(ICARION_ERR_INVALID, "areas must be finite");
+            quaternion_rotation(quat_wxyz + 4 * i, R.data() + 9 * i);
+        }
+        set_device(h);
+        copy_in(h->ori_tab, tab.data(), tab.size(), h->stream, "upload EHSS orientation index");
+        copy_in(h->ori_R, R.data(), R.size(), h->stream, "upload EHSS orientations");
+        copy_in(h->ori_area, area_m2, static_cast<size_t>(total), h->stream, "upload EHSS orientation areas");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        h->ori_species = n_species;
     });
 }
 

@@ -215,24 +215,15 @@ __device__ __forceinline__ void ortho_basis_f64(d3 e, d3& t1, d3& t2) {
     t2 = n2 > 0.0 ? d3{c2.x / n2, c2.y / n2, c2.z / n2} : d3{0, 0, 0};
 }
 
-// CollisionKernels::ehss_collision (CollisionKernels.cpp:13-137, 241-381):
-// Shoemake orientation, first-contact ray test, specular reflection.  Rotated
-// atoms are recomputed per attempt instead of stored (same arithmetic).
-__device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn,
-                       uint32_t a0, uint32_t nat, PhiloxStream& rng) {
-    const double u1 = rng.uniform_f64();
-    const double u2 = rng.uniform_f64();
-    const double u3 = rng.uniform_f64();
-    if (nat == 0) return v_ion;
-    const double q1 = sqrt(1.0 - u1) * sin(2.0 * kPi * u2);
-    const double q2 = sqrt(1.0 - u1) * cos(2.0 * kPi * u2);
-    const double q3 = sqrt(u1) * sin(2.0 * kPi * u3);
-    const double q4 = sqrt(u1) * cos(2.0 * kPi * u3);
-    const double x = q1, y = q2, z = q3, w = q4;
-    double O[9];
-    O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
-    O[3] = 2 * (x * y + z * w);     O[4] = 1 - 2 * (x * x + z * z); O[5] = 2 * (y * z - x * w);
-    O[6] = 2 * (x * z - y * w);     O[7] = 2 * (y * z + x * w);     O[8] = 1 - 2 * (x * x + y * y);
+// CollisionKernels::ehss_collision_with_orientation (CollisionKernels.cpp:44-137,
+// 241-281, 283-381): the molecule in orientation O (row-major 3x3, in the
+// collision-axis frame), the impact range widened to sqrt(sigma_eff / pi)
+// when that is larger, first-contact ray test over up to max_attempts
+// impact parameters, specular reflection.  Rotated atoms are recomputed per
+// attempt instead of stored (same arithmetic).
+template <class Rot>
+__device__ d3 ehss_core_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, uint32_t a0, uint32_t nat,
+                            const Rot& O, double sigma_eff, int max_attempts, PhiloxStream& rng) {
     d3 vrel = v_ion - v_n;
     double vrel_mag = sqrt(dot(vrel, vrel));
     if (vrel_mag <= kMinVelocityMag) {
@@ -263,7 +254,11 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
         if (cur > b_max) b_max = cur;
     }
     b_max += P.r_gas;
-    for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
+    if (sigma_eff > 0.0) {
+        const double b_sigma = sqrt(sigma_eff / kPi);
+        if (b_sigma > b_max) b_max = b_sigma;
+    }
+    for (int attempt = 0; attempt < max_attempts; ++attempt) {
         const double uA = rng.uniform_f64();
         const double uB = rng.uniform_f64();
         const double b = b_max * sqrt(uA);
@@ -311,6 +306,27 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
         if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
     }
     return v_ion;
+}
+
+// CollisionKernels::ehss_collision (CollisionKernels.cpp:13-42): a Shoemake
+// orientation (CollisionGeometry::generate_random_rotation), then the core
+// with sigma_eff = 0 and 256 attempts.
+__device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn,
+                       uint32_t a0, uint32_t nat, PhiloxStream& rng) {
+    const double u1 = rng.uniform_f64();
+    const double u2 = rng.uniform_f64();
+    const double u3 = rng.uniform_f64();
+    if (nat == 0) return v_ion;
+    const double q1 = sqrt(1.0 - u1) * sin(2.0 * kPi * u2);
+    const double q2 = sqrt(1.0 - u1) * cos(2.0 * kPi * u2);
+    const double q3 = sqrt(u1) * sin(2.0 * kPi * u3);
+    const double q4 = sqrt(u1) * cos(2.0 * kPi * u3);
+    const double x = q1, y = q2, z = q3, w = q4;
+    double O[9];
+    O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
+    O[3] = 2 * (x * y + z * w);     O[4] = 1 - 2 * (x * x + z * z); O[5] = 2 * (y * z - x * w);
+    O[6] = 2 * (x * z - y * w);     O[7] = 2 * (y * z + x * w);     O[8] = 1 - 2 * (x * x + y * y);
+    return ehss_core_f64(P, v_ion, v_n, mi, mn, a0, nat, O, 0.0, kEhssMaxAttempts, rng);
 }
 
 // One collision stage; returns true on a collision event.
@@ -394,6 +410,7 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
         // EHSS offline sample set of the species (EHSSCollisionHandler.cpp:151-172):
         // the orientation is drawn first and its sigma_eff enters the test
         uint4 off = make_uint4(0u, 0u, 0u, 0u);
+        uint2 ori = make_uint2(0u, 0u);
         uint32_t oi = 0;
         if constexpr (COLL == COLL_EHSS_TAB) {
             if (P.ehss_off) {
@@ -402,6 +419,16 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
                     oi = static_cast<uint32_t>(rng.uniform_f64() * static_cast<double>(off.y));
                     if (oi >= off.y) oi = off.y - 1u;
                     sigma = P.ehss_off_sigma[off.x + oi];
+                }
+            }
+            // else the species' orientation samples (EHSSCollisionHandler.cpp:
+            // 187-217): an orientation drawn first, its area for the gas as sigma_eff
+            if (off.y == 0u && P.ehss_ori) {
+                ori = P.ehss_ori[sidx];
+                if (ori.y > 0u) {
+                    oi = static_cast<uint32_t>(rng.uniform_f64() * static_cast<double>(ori.y));
+                    if (oi >= ori.y) oi = ori.y - 1u;
+                    sigma = P.ehss_ori_area[ori.x + oi];
                 }
             }
         }
@@ -451,6 +478,12 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
             const double inv_mt = 1.0 / (m + P.m_gas);
             const d3 Vcom = (v * m + vn * P.m_gas) * inv_mt;
             v = Vcom + vr * (P.m_gas * inv_mt);
+        } else if (COLL == COLL_EHSS_TAB && ori.y > 0u) {
+            // the stored orientation, sigma_eff and the forced-hit budget
+            // (EHSSCollisionHandler.cpp:286-293, CollisionKernels.cpp:110)
+            const double* O = P.ehss_ori_R + 9ull * (ori.x + oi);
+            v = ehss_core_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, O, sigma, kEhssMaxAttempts * 16,
+                              rng);
         } else {
             v = ehss_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, rng);
         }

@@ -174,6 +174,11 @@ struct StepParams {
     const uint4 *ehss_off;
     const double *ehss_off_sigma;   // sigma_eff per orientation
     const double *ehss_off_mu;      // mu samples, orientation-major
+    // EHSS orientation samples per species (NULL: none): {first orientation,
+    // orientations}; each orientation's rotation (row-major 3x3) and area
+    const uint2 *ehss_ori;
+    const double *ehss_ori_R;
+    const double *ehss_ori_area;
     double x0, y0, z0;
     double dx, dy, dz;
     double inv_dx, inv_dy, inv_dz;

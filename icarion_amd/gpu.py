@@ -25,6 +25,7 @@ EXPORTED = (
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
+    "icarion_gpu_set_ehss_orientations",
 )
 
 
@@ -86,6 +87,8 @@ def load(path=None):
     lib.icarion_gpu_set_domains.argtypes = [C.c_void_p, P(_abi.DomainDesc), P(C.c_double), C.c_uint32,
                                             P(_abi.PhysicsDesc)]
     lib.icarion_gpu_download_domains.argtypes = [C.c_void_p, P(C.c_int32)]
+    lib.icarion_gpu_set_ehss_orientations.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_double),
+                                                      P(C.c_double), C.c_uint32]
     lib.icarion_gpu_synchronize.argtypes = [C.c_void_p]
     lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
                                            C.c_uint64, P(_abi.StepStats)]
@@ -368,6 +371,26 @@ class Handle:
         _check(self.lib.icarion_gpu_set_ehss_offline(
             self.h, n_orient.ctypes.data_as(u32), stride.ctypes.data_as(u32),
             sg.ctypes.data_as(d), mu.ctypes.data_as(d), n), self.lib)
+
+    def set_ehss_orientations(self, sets):
+        """sets[s]: None (no orientation samples) or (quat_wxyz[n, 4], area_m2[n]);
+        [] clears."""
+        n = len(sets)
+        n_orient = np.zeros(max(n, 1), np.uint32)
+        qs, ars = [], []
+        for k, t in enumerate(sets):
+            if t is None:
+                continue
+            q, a = np.asarray(t[0], np.float64).reshape(-1, 4), np.asarray(t[1], np.float64).ravel()
+            n_orient[k] = q.shape[0]
+            qs.append(q.ravel())
+            ars.append(a)
+        q = np.ascontiguousarray(np.concatenate(qs) if qs else np.zeros(4))
+        a = np.ascontiguousarray(np.concatenate(ars) if ars else np.zeros(1))
+        d = C.POINTER(C.c_double)
+        _check(self.lib.icarion_gpu_set_ehss_orientations(
+            self.h, n_orient.ctypes.data_as(C.POINTER(C.c_uint32)), q.ctypes.data_as(d),
+            a.ctypes.data_as(d), n), self.lib)
 
     def upload_field_terms(self, axes, terms):
         """axes: (xs, ys, zs) node coordinates (non-decreasing, possibly

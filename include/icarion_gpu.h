@@ -362,6 +362,22 @@ int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
 int icarion_gpu_set_gas_mixture(icarion_gpu *h, const icarion_gas_component *components,
                                 uint32_t n_components, const double *sigma_m2, uint32_t n_species);
 
+/* EHSS orientation samples per species (EHSSOrientationSamples,
+ * EHSSSamples.h:14-17: orientations_quat and the areas_by_gas_m2 entry of the
+ * domain's gas), as EHSSCollisionHandler::handle_collision uses them
+ * (EHSSCollisionHandler.cpp:187-217, 286-293).  A species with n_orient[s] > 0
+ * and no offline sample set draws one of its orientations uniformly, takes its
+ * area as sigma_eff in the collision test and, on a collision, runs the
+ * first-contact test with the molecule in that orientation
+ * (CollisionKernels::ehss_collision_with_orientation with that sigma_eff and
+ * the forced-hit budget of 16 x 256 impact parameters).  quat_wxyz: 4 values
+ * per orientation (w, x, y, z; normalised on use as quaternion_to_rotation
+ * does), area_m2: one per orientation, both species-major.  n_species must
+ * equal that of icarion_gpu_set_species; n_species = 0 clears.  EHSS geometry
+ * method, FP64. */
+int icarion_gpu_set_ehss_orientations(icarion_gpu *h, const uint32_t *n_orient, const double *quat_wxyz,
+                                      const double *area_m2, uint32_t n_species);
+
 /* Time-varying voltages of a domain: slots[k] (ICARION_N_VOLTAGE_SLOTS
  * entries, NULL = the domain's constant) replaces the value of slot k,
  * evaluated at each stage time.

@@ -67,6 +67,9 @@ def load():
     lib.oracle_mt_hss_collision.argtypes = [C.c_uint64, P(d), P(d), d, d, P(d)]
     lib.oracle_mt_ehss_collision.argtypes = [C.c_uint64, P(d), P(d), d, d, d, P(d), C.c_uint32,
                                              P(d)]
+    lib.oracle_mt_ehss_oriented.argtypes = [C.c_uint64, P(d), P(d), d, d, d, P(d), C.c_uint32,
+                                            P(d), d, P(d)]
+    lib.oracle_quaternion_to_rotation.argtypes = [P(d), P(d)]
     lib.oracle_oapa_sigma.argtypes = [P(d), C.c_uint32, d]
     lib.oracle_oapa_sigma.restype = d
     lib.oracle_push_one.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), P(_abi.SpeciesDesc),
@@ -239,14 +242,53 @@ class EhssOffline:
                                         _dp(self.mu))
 
 
-def _tables_ptr(mixture=None, ehss_offline=None):
-    if mixture is None and ehss_offline is None:
+class EhssOrientations:
+    """oracle_ehss_orientations from per-species samples: sets[s] = None (no
+    samples) or (quat_wxyz[n_orient, 4], area_m2[n_orient])."""
+
+    def __init__(self, sets):
+        n = len(sets)
+        self.n_orient = np.zeros(n, np.uint32)
+        qs, ars = [], []
+        for s_, t in enumerate(sets):
+            if t is None:
+                continue
+            q, a = np.asarray(t[0], np.float64).reshape(-1, 4), np.asarray(t[1], np.float64).ravel()
+            assert q.shape[0] == a.shape[0]
+            self.n_orient[s_] = q.shape[0]
+            qs.append(q.ravel())
+            ars.append(a)
+        self.quat = np.ascontiguousarray(np.concatenate(qs) if qs else np.zeros(4))
+        self.area = np.ascontiguousarray(np.concatenate(ars) if ars else np.zeros(1))
+        self.c = _abi.OracleEhssOrientations(n, 0, self.n_orient.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                             _dp(self.quat), _dp(self.area))
+
+
+def _tables_ptr(mixture=None, ehss_offline=None, ehss_orient=None):
+    if mixture is None and ehss_offline is None and ehss_orient is None:
         return C.POINTER(_abi.OracleCollisionTables)()
     t = _abi.OracleCollisionTables(
         C.pointer(mixture.c) if mixture is not None else C.POINTER(_abi.OracleGasMixture)(),
         C.pointer(ehss_offline.c) if ehss_offline is not None
-        else C.POINTER(_abi.OracleEhssOffline)())
+        else C.POINTER(_abi.OracleEhssOffline)(),
+        C.pointer(ehss_orient.c) if ehss_orient is not None
+        else C.POINTER(_abi.OracleEhssOrientations)())
     return C.pointer(t)
+
+
+def mt_ehss_oriented(seed, vi, vn, mi, mn, rion, atoms, quat, sigma):
+    """CollisionKernels::ehss_collision_with_orientation on PhysicsRng(seed)."""
+    a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
+    out = (C.c_double * 3)()
+    load().oracle_mt_ehss_oriented(seed, vec(*vi), vec(*vn), mi, mn, rion, _dp(a), a.shape[0],
+                                   vec(*quat), sigma, out)
+    return list(out)
+
+
+def quaternion_to_rotation(quat):
+    out = (C.c_double * 9)()
+    load().oracle_quaternion_to_rotation(vec(*quat), out)
+    return list(out)
 
 
 def mt_ehss_offline(seed, domain, sigma_eff, mu, mass, vel, dt):
@@ -277,7 +319,7 @@ def mt_hss_mixture(seed, domain, mixture, sigma_row, mass, vel, dt):
 
 
 def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=None, field=None,
-        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None):
+        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None, ehss_orient=None):
     """Run the oracle engine loop on ens in place; returns StepStats."""
     lib = load()
     sp = species_array(species)
@@ -291,7 +333,7 @@ def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=Non
     st = _abi.StepStats()
     rc = lib.oracle_run(C.byref(domain), C.byref(physics), sp, len(species), ap, na, fptr,
                         C.byref(soa), t0, dt, n_steps, step_offset, rng_kind, threads,
-                        C.byref(st), _tables_ptr(mixture, ehss_offline))
+                        C.byref(st), _tables_ptr(mixture, ehss_offline, ehss_orient))
     if rc != 0:
         raise RuntimeError(f"oracle_run failed ({rc})")
     return st

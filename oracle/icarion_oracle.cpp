@@ -785,17 +785,15 @@ void random_rotation(Rng& rng, double R[3][3]) {
     R[2][2] = 1 - 2 * (x * x + y * y);
 }
 
-// CollisionKernels::ehss_collision + ehss_collision_with_orientation +
-// rotate_and_analyze_molecule + sample_impact_geometry + detect_atom_collision
-// + compute_reflected_velocity + to_lab_frame (CollisionKernels.cpp:13-137,
-// 241-381).  sigma_eff_m2 = 0 and max_attempts = 256 as in the handler's call
-// (EHSSCollisionHandler.cpp:363-365).
+// CollisionKernels::ehss_collision_with_orientation + rotate_and_analyze_molecule
+// + sample_impact_geometry + detect_atom_collision + compute_reflected_velocity
+// + to_lab_frame (CollisionKernels.cpp:44-137, 241-381): the molecule in
+// orientation orient (collision-axis frame), the impact range widened to
+// sqrt(sigma_eff / pi) when larger, up to max_attempts impact parameters.
 template <class Rng>
-V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
-                  double ion_radius, const double* atoms, uint32_t nat, Rng& rng,
-                  std::vector<V3>& rotated) {
-    double orient[3][3];
-    random_rotation(rng, orient);
+V3 ehss_collision_oriented(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
+                           double ion_radius, const double* atoms, uint32_t nat, const double orient[3][3],
+                           double sigma_eff, int max_attempts, Rng& rng, std::vector<V3>& rotated) {
     if (nat == 0) return v_ion;
     V3 vrel = v_ion - v_neutral;
     double vrel_mag = norm(vrel);
@@ -826,7 +824,11 @@ V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
         if (cur > b_max) b_max = cur;
     }
     b_max += ion_radius;
-    for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
+    if (sigma_eff > 0.0) {
+        const double b_sigma = std::sqrt(sigma_eff / kPi);
+        if (b_sigma > b_max) b_max = b_sigma;
+    }
+    for (int attempt = 0; attempt < max_attempts; ++attempt) {
         V3 s1, s2;
         orthonormal_basis(axis, s1, s2);
         const double uA = rng.uniform01();
@@ -877,6 +879,43 @@ V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
     return v_ion;
 }
 
+// CollisionKernels::ehss_collision (CollisionKernels.cpp:13-42): a random
+// orientation first, then the oriented kernel with sigma_eff_m2 = 0 and
+// max_attempts = 256 as in the handler's online call (EHSSCollisionHandler.cpp:294-296).
+template <class Rng>
+V3 ehss_collision(V3 v_ion, V3 v_neutral, double ion_mass, double neutral_mass,
+                  double ion_radius, const double* atoms, uint32_t nat, Rng& rng,
+                  std::vector<V3>& rotated) {
+    double orient[3][3];
+    random_rotation(rng, orient);
+    return ehss_collision_oriented(v_ion, v_neutral, ion_mass, neutral_mass, ion_radius, atoms, nat, orient,
+                                   0.0, kEhssMaxAttempts, rng, rotated);
+}
+
+// CollisionGeometry::quaternion_to_rotation (CollisionGeometry.cpp:73-104).
+void quaternion_to_rotation(const double q[4], double R[3][3]) {
+    double w = q[0], x = q[1], y = q[2], z = q[3];
+    const double nrm = std::sqrt(w * w + x * x + y * y + z * z);
+    if (nrm <= 0.0) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) R[i][j] = (i == j) ? 1.0 : 0.0;
+        return;
+    }
+    w /= nrm;
+    x /= nrm;
+    y /= nrm;
+    z /= nrm;
+    R[0][0] = 1.0 - 2.0 * (y * y + z * z);
+    R[0][1] = 2.0 * (x * y - z * w);
+    R[0][2] = 2.0 * (x * z + y * w);
+    R[1][0] = 2.0 * (x * y + z * w);
+    R[1][1] = 1.0 - 2.0 * (x * x + z * z);
+    R[1][2] = 2.0 * (y * z - x * w);
+    R[2][0] = 2.0 * (x * z - y * w);
+    R[2][1] = 2.0 * (y * z + x * w);
+    R[2][2] = 1.0 - 2.0 * (x * x + y * y);
+}
+
 // The collision tables one ion sees: its row of the mixture cross sections
 // and its species' offline EHSS sample set.
 struct IonTables {
@@ -885,7 +924,38 @@ struct IonTables {
     uint32_t off_n = 0, off_stride = 0;
     const double* off_sigma = nullptr;
     const double* off_mu = nullptr;
+    uint32_t ori_n = 0;                 // orientation samples of the species
+    const double* ori_quat = nullptr;   // 4 per orientation (w, x, y, z)
+    const double* ori_area = nullptr;   // the area for the domain's gas
 };
+
+// EHSSCollisionHandler::handle_collision with orientation samples, single gas
+// (EHSSCollisionHandler.cpp:187-217, 219-313, 426-455): the orientation drawn
+// first, its area as sigma_eff in the test; on a collision the oriented
+// first-contact kernel with that sigma_eff, 256 attempts and force_hit (x16).
+template <class Rng>
+bool ehss_orientation_collide(const icarion_domain_desc& d, uint32_t n_orient, const double* quat,
+                              const double* area, const double* atoms, uint32_t nat, double m_ion, V3& vel,
+                              double dt, Rng& rng, std::vector<V3>& scratch, uint32_t* orient = nullptr) {
+    size_t idx = static_cast<size_t>(rng.uniform01() * static_cast<double>(n_orient));
+    if (idx >= n_orient) idx = n_orient - 1;
+    if (orient) *orient = static_cast<uint32_t>(idx);
+    const double sigma_eff = area[idx];
+    if (sigma_eff <= 0.0) return false;
+    const double m_neutral = d.gas_mass_kg;
+    const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    const V3 v_neutral = sample_neutral_velocity(d.temperature_K, m_neutral, v_gas, rng);
+    const V3 v_rel = vel - v_neutral;
+    const double v_rel_mag = norm(v_rel);
+    if (v_rel_mag < kMinRelativeVelocity) return false;
+    const double P = 1.0 - std::exp(-d.gas_density_m3 * sigma_eff * v_rel_mag * dt);
+    if (rng.uniform01() >= P) return false;
+    double R[3][3];
+    quaternion_to_rotation(quat + 4 * idx, R);
+    vel = ehss_collision_oriented(vel, v_neutral, m_ion, m_neutral, d.gas_radius_m, atoms, nat, R, sigma_eff,
+                                  kEhssMaxAttempts * 16, rng, scratch);
+    return true;
+}
 
 // EHSSCollisionHandler::handle_collision with an offline sample set, single gas
 // (EHSSCollisionHandler.cpp:151-172, 222-299, 426-455).  Returns true on a
@@ -1001,6 +1071,10 @@ bool collide(const icarion_domain_desc& d, const icarion_physics_desc& ph,
     if (model == ICARION_COLLISION_EHSS && tb.off_n > 0)
         return ehss_offline_collide(d, tb.off_n, tb.off_stride, tb.off_sigma, tb.off_mu, m_ion, vel,
                                     dt, rng);
+    if (model == ICARION_COLLISION_EHSS && tb.ori_n > 0)
+        return ehss_orientation_collide(d, tb.ori_n, tb.ori_quat, tb.ori_area,
+                                        atoms + 4 * static_cast<size_t>(sp.atom_offset), sp.atom_count, m_ion,
+                                        vel, dt, rng, scratch);
     if (model == ICARION_COLLISION_HSS || model == ICARION_COLLISION_EHSS) {
         const double n = d.gas_density_m3;
         const double T_K = d.temperature_K;
@@ -1243,14 +1317,24 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
 // Per-species views of a run's collision tables.
 struct RunTables {
     const oracle_collision_tables* t = nullptr;
-    std::vector<size_t> off_base, mu_base;
+    std::vector<size_t> off_base, mu_base, ori_base;
     bool ok(uint32_t n_species) const {
         if (!t) return true;
         if (t->mixture && t->mixture->n_components > 0 && t->mixture->n_species != n_species) return false;
         if (t->ehss_offline && t->ehss_offline->n_species != n_species) return false;
+        if (t->ehss_orient && t->ehss_orient->n_species != n_species) return false;
         return true;
     }
     explicit RunTables(const oracle_collision_tables* tables) : t(tables) {
+        if (t && t->ehss_orient) {
+            const oracle_ehss_orientations& o = *t->ehss_orient;
+            ori_base.resize(o.n_species);
+            size_t a = 0;
+            for (uint32_t s = 0; s < o.n_species; ++s) {
+                ori_base[s] = a;
+                a += o.n_orient[s];
+            }
+        }
         if (t && t->ehss_offline) {
             const oracle_ehss_offline& o = *t->ehss_offline;
             off_base.resize(o.n_species);
@@ -1277,6 +1361,12 @@ struct RunTables {
             tb.off_stride = o.mu_stride[s];
             tb.off_sigma = o.sigma_eff + off_base[s];
             tb.off_mu = o.mu + mu_base[s];
+        }
+        if (t->ehss_orient && s < t->ehss_orient->n_species && t->ehss_orient->n_orient[s] > 0) {
+            const oracle_ehss_orientations& o = *t->ehss_orient;
+            tb.ori_n = o.n_orient[s];
+            tb.ori_quat = o.quat_wxyz + 4 * ori_base[s];
+            tb.ori_area = o.area_m2 + ori_base[s];
         }
         return tb;
     }
@@ -1375,6 +1465,27 @@ int oracle_mt_ehss_offline(uint64_t seed, const icarion_domain_desc* domain, uin
         vn_out[0] = vn.x; vn_out[1] = vn.y; vn_out[2] = vn.z;
     }
     return hit ? 1 : 0;
+}
+
+void oracle_mt_ehss_oriented(uint64_t seed, const double v_ion[3], const double v_neutral[3], double ion_mass,
+                             double neutral_mass, double ion_radius, const double* atoms_xyzr, uint32_t n_atoms,
+                             const double quat_wxyz[4], double sigma_eff, double out[3]) {
+    MtRng rng(seed);
+    std::vector<V3> scratch;
+    double R[3][3];
+    quaternion_to_rotation(quat_wxyz, R);
+    const V3 v = ehss_collision_oriented(V3{v_ion[0], v_ion[1], v_ion[2]},
+                                         V3{v_neutral[0], v_neutral[1], v_neutral[2]}, ion_mass, neutral_mass,
+                                         ion_radius, atoms_xyzr, n_atoms, R, sigma_eff, kEhssMaxAttempts * 16, rng,
+                                         scratch);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+void oracle_quaternion_to_rotation(const double quat_wxyz[4], double R[9]) {
+    double M[3][3];
+    quaternion_to_rotation(quat_wxyz, M);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[3 * i + j] = M[i][j];
 }
 
 void oracle_mt_ehss_collision(uint64_t seed, const double v_ion[3], const double v_neutral[3],

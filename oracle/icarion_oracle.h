@@ -95,11 +95,32 @@ typedef struct oracle_ehss_offline {
     const double *mu;
 } oracle_ehss_offline;
 
-/* Optional collision tables of a run (either pointer may be NULL). */
+/* EHSS orientation samples (EHSSOrientationSamples, EHSSSamples.h:14-17) per
+ * species: n_orient[s] orientations as quaternions (w, x, y, z) and their
+ * areas for the domain's gas; the tables of all species concatenated. */
+typedef struct oracle_ehss_orientations {
+    uint32_t n_species;
+    uint32_t reserved0;
+    const uint32_t *n_orient;
+    const double *quat_wxyz;
+    const double *area_m2;
+} oracle_ehss_orientations;
+
+/* Optional collision tables of a run (any pointer may be NULL). */
 typedef struct oracle_collision_tables {
     const oracle_gas_mixture *mixture;
     const oracle_ehss_offline *ehss_offline;
+    const oracle_ehss_orientations *ehss_orient;
 } oracle_collision_tables;
+
+/* CollisionKernels::ehss_collision_with_orientation (CollisionKernels.cpp:44-137)
+ * with the rotation of quaternion_to_rotation (CollisionGeometry.cpp:73-104),
+ * sigma_eff and the forced-hit budget, on one MT19937-64 stream seeded with seed
+ * (the reference's PhysicsRng). */
+void oracle_mt_ehss_oriented(uint64_t seed, const double v_ion[3], const double v_neutral[3], double ion_mass,
+                             double neutral_mass, double ion_radius, const double *atoms_xyzr, uint32_t n_atoms,
+                             const double quat_wxyz[4], double sigma_eff, double out[3]);
+void oracle_quaternion_to_rotation(const double quat_wxyz[4], double R[9]);
 
 enum { ORACLE_RNG_PHILOX = 0, ORACLE_RNG_MT19937_64 = 1 };
 

@@ -18,6 +18,7 @@
 //
 // Protocol: stdin holds whitespace-separated numbers, the first token naming
 // the mode; stdout gets the results with 17 significant digits.
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <iostream>
@@ -29,6 +30,7 @@
 #include "core/config/types/WaveformConfig.h"
 #include "core/integrator/strategies/BorisStrategy.h"
 #include "core/integrator/strategies/RK4Strategy.h"
+#include "core/physics/collisions/core/CollisionGeometry.h"
 #include "core/physics/collisions/core/CollisionKernels.h"
 #include "core/physics/collisions/core/VelocitySampling.h"
 #include "core/physics/forces/ForceContext.h"
@@ -209,6 +211,44 @@ void run_ehss() {
     put3(physics::collision_core::CollisionKernels::ehss_collision(vi, vn, mi, mn, rion, c, r, rng));
 }
 
+// ehss_orient: CollisionKernels::ehss_collision_with_orientation with the
+// rotation of CollisionGeometry::quaternion_to_rotation, sigma_eff and the
+// forced-hit budget, as EHSSCollisionHandler calls it for orientation samples
+void run_ehss_orient() {
+    unsigned long long seed;
+    std::cin >> seed;
+    Vec3 vi = read3();
+    Vec3 vn = read3();
+    double mi, mn, rion;
+    size_t nat;
+    std::cin >> mi >> mn >> rion >> nat;
+    std::vector<Vec3> c(nat);
+    std::vector<double> r(nat);
+    for (size_t j = 0; j < nat; ++j) {
+        c[j] = read3();
+        std::cin >> r[j];
+    }
+    std::array<double, 4> q;
+    std::cin >> q[0] >> q[1] >> q[2] >> q[3];
+    double sigma;
+    std::cin >> sigma;
+    double R[3][3];
+    physics::collision_core::CollisionGeometry::quaternion_to_rotation(q, R);
+    physics::PhysicsRng rng(seed);
+    put3(physics::collision_core::CollisionKernels::ehss_collision_with_orientation(
+        vi, vn, mi, mn, rion, c, r, R, rng, 256, sigma, true));
+}
+
+// quat: CollisionGeometry::quaternion_to_rotation, row-major
+void run_quat() {
+    std::array<double, 4> q;
+    std::cin >> q[0] >> q[1] >> q[2] >> q[3];
+    double R[3][3];
+    physics::collision_core::CollisionGeometry::quaternion_to_rotation(q, R);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) std::printf("%.17g%c", R[i][j], (i == 2 && j == 2) ? '\n' : ' ');
+}
+
 }  // namespace
 
 int main() {
@@ -219,6 +259,8 @@ int main() {
         else if (mode == "hss") run_hss();
         else if (mode == "ehss") run_ehss();
         else if (mode == "waveform") run_waveform();
+        else if (mode == "ehss_orient") run_ehss_orient();
+        else if (mode == "quat") run_quat();
         else {
             std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
             return 2;

@@ -41,6 +41,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
             h.set_gas_mixture(*w["mixture"])
         if w.get("ehss_offline") is not None:
             h.set_ehss_offline(w["ehss_offline"])
+        if w.get("ehss_orient") is not None:
+            h.set_ehss_orientations(w["ehss_orient"])
         if w.get("waveforms") is not None:
             h.set_waveforms(w["waveforms"])
         h.upload_state(ens)
@@ -59,9 +61,10 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
         field = oracle.Waveforms(w["waveforms"])
     mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
     off = oracle.EhssOffline(w["ehss_offline"]) if w.get("ehss_offline") is not None else None
+    ori = oracle.EhssOrientations(w["ehss_orient"]) if w.get("ehss_orient") is not None else None
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
                       step_offset=step_offset, atoms=w.get("atoms"), field=field, mixture=mixture,
-                      ehss_offline=off)
+                      ehss_offline=off, ehss_orient=ori)
 
 
 def rel_close(a, b, rtol, scale):
@@ -219,6 +222,51 @@ def test_ehss_offline_sets_fp64_ion_by_ion(oracle, integrator):
     o2 = w["ensemble"].copy()
     so2 = oracle_run(oracle, plain, o2, 1000)
     assert compare_ions(g, o2, rtol=1e-9).mean() < 0.9
+
+
+def orientation_sets(rng, n_species, with_samples):
+    """Per species: None, or random (unnormalised) quaternions with areas."""
+    sets = []
+    for s_ in range(n_species):
+        if not with_samples[s_]:
+            sets.append(None)
+            continue
+        n = int(rng.integers(3, 40))
+        sets.append((rng.normal(size=(n, 4)), rng.uniform(40e-20, 250e-20, n)))
+    return sets
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_ehss_orientation_samples_fp64_ion_by_ion(oracle, integrator):
+    """EHSS orientation samples (stored quaternions + areas) for two of three
+    species, the online geometry for the third; an offline set for one of the
+    two takes precedence over its orientation samples."""
+    w = ehss_workload(n=2048, seed=37)
+    w["physics"] = W.physics(integrator, _abi.COLLISION_EHSS, seed=37)
+    rng = np.random.default_rng(37)
+    w["ehss_orient"] = orientation_sets(rng, 3, (True, False, True))
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 1000)
+    so = oracle_run(oracle, w, o, 1000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    plain = dict(w)
+    plain.pop("ehss_orient")
+    o2 = w["ensemble"].copy()
+    oracle_run(oracle, plain, o2, 1000)
+    assert compare_ions(g, o2, rtol=1e-9).mean() < 0.9
+    # precedence: with an offline set for species 0 as well, species 0 follows
+    # the offline set exactly as without orientation samples
+    w3 = dict(w)
+    w3["ehss_offline"] = [(rng.uniform(30e-20, 60e-20, 16), np.sqrt(rng.uniform(0.0, 1.0, (16, 50)))),
+                          None, None]
+    w4 = dict(w3)
+    w4["ehss_orient"] = [None, None, w["ehss_orient"][2]]
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w3, a, 600)
+    gpu_run(w4, b, 600)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
 
 
 def test_ehss_offline_controls():

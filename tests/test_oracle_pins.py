@@ -470,6 +470,38 @@ def test_golden_ehss_kernel_bitexact(oracle, golden):
         assert list(out) == c["out"], c["cluster"]
 
 
+def _orient_golden():
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_ehss_orient.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_golden_quaternion_rotation_bitexact(oracle):
+    """CollisionGeometry::quaternion_to_rotation of the reference (compiled in
+    oracle/_ref; tests/golden/make_ref_ehss_orient.py): normalisation, the zero
+    quaternion's identity, every element bit for bit."""
+    g = _orient_golden()
+    assert len(g["quat"]) == 10
+    for c in g["quat"]:
+        assert oracle.quaternion_to_rotation(c["q"]) == c["R"]
+
+
+def test_golden_ehss_oriented_kernel_bitexact(oracle):
+    """CollisionKernels::ehss_collision_with_orientation of the reference with a
+    stored orientation, sigma_eff (0, inside and outside the geometric impact
+    range) and the forced-hit budget, on PhysicsRng(seed): the oracle's oriented
+    kernel bit for bit (24 cases over reference molecules and synthetic
+    clusters)."""
+    g = _orient_golden()
+    assert len(g["ehss_orient"]) == 24
+    for c in g["ehss_orient"]:
+        atoms = np.asarray(g["clusters"][c["cluster"]], dtype=np.float64)
+        out = oracle.mt_ehss_oriented(c["seed"], c["vi"], c["vn"], c["mi"], c["mn"], c["rion"], atoms,
+                                      c["q"], c["sigma"])
+        assert out == c["out"], (c["cluster"], c["sigma"])
+
+
 def test_golden_push_rk4_and_boris(oracle, golden):
     for c in golden["push"]:
         d = W.quadrupole_domain(r0=c["r0"], V=c["V"], f=c["f"], U=c["U"], axial_V=c["axial_V"],
@@ -507,6 +539,29 @@ def test_live_reference_harness_ehss_random_clusters(oracle):
         lib.oracle_mt_ehss_collision(seed, oracle.vec(*vi), oracle.vec(*vn), mi, mn, rion,
                                      oracle._dp(atoms), nat, out)
         assert list(out) == ref
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(
+    os.path.abspath(__file__))), "oracle", "_ref", "ref_harness")), reason="oracle/_ref not built")
+def test_live_reference_harness_ehss_oriented(oracle):
+    """Fresh random orientations, sigma_eff and clusters straight through the
+    reference's ehss_collision_with_orientation."""
+    import subprocess
+    rng = np.random.default_rng()
+    for _ in range(20):
+        nat = int(rng.integers(1, 40))
+        atoms = np.concatenate([rng.normal(0, 4e-10, (nat, 3)), rng.uniform(1.5e-10, 2e-10, (nat, 1))], 1)
+        seed = int(rng.integers(1, 2**62))
+        vi, vn = rng.normal(0, 300, 3), rng.normal(0, 900, 3)
+        mi, mn, rion = 300 * AMU_TO_KG, GASES["He"][0], 1.3e-10
+        q = rng.normal(size=4)
+        sigma = float(rng.choice([0.0, 1e-19, 1e-17]))
+        text = (f"ehss_orient {seed} " + " ".join(repr(float(x)) for x in (*vi, *vn, mi, mn, rion))
+                + f" {nat} " + " ".join(repr(float(x)) for x in atoms.ravel())
+                + " " + " ".join(repr(float(x)) for x in (*q, sigma)) + "\n")
+        res = subprocess.run([oracle.REF_HARNESS], input=text, capture_output=True, text=True, check=True)
+        ref = [float(x) for x in res.stdout.split()]
+        assert oracle.mt_ehss_oriented(seed, vi, vn, mi, mn, rion, atoms, q, sigma) == ref
 
 
 # ---------------------------------------------------------------- collision physics
