@@ -63,8 +63,8 @@ FLOPS_PER_ION_STEP = {"fp64": 86, "fp32": 122}
 EHSS_FLOPS_STEP = 115
 # HBM bytes per ion per launch of step_surrogate_kernel, measured with rocprofv3
 # FETCH_SIZE / WRITE_SIZE (calibrated on a 1 GiB copy) at 1M ions x 2000 steps:
-# 139.0 MB read + 70.0 MB written (state round trip + each ion's species row).
-EHSS_TRAFFIC_BYTES_PER_ION_LAUNCH = 209.0
+# 141.1 MB read + 70.0 MB written (state round trip + each ion's species row).
+EHSS_TRAFFIC_BYTES_PER_ION_LAUNCH = 211.1
 EHSS_FLOPS_COLLISION = 2 * (8 * 128 + 128 * 128 + 128 * 3) + 2 * 128 + 2 * 128 + 3 + 60
 
 
@@ -828,7 +828,7 @@ def bench_snapshots(args):
     launches_plain = math.ceil(steps / 2000)
     avg_launch_s = st_plain.device_ms * 1e-3 / launches_plain
     achieved = FLOPS_PER_ION_STEP["fp32"] * (st_plain.ion_steps / launches_plain) / avg_launch_s / 1e12
-    snap_bytes = 53 * n
+    snap_bytes = 57 * n
     return {
         "metric": "ion-steps/s on 1 MI355X with trajectory snapshots",
         "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": steps,

@@ -92,9 +92,10 @@ class StepStats(C.Structure):
 
 # int (*icarion_snapshot_fn)(void*, uint64_t index, uint64_t step, double t_s, size_t n,
 #                           const double* pos, const double* vel, const uint32_t* species,
-#                           const uint8_t* flags)
+#                           const int32_t* domain_index, const uint8_t* flags)
 SnapshotFn = C.CFUNCTYPE(C.c_int, C.c_void_p, _u64, _u64, _d, C.c_size_t, C.POINTER(_d),
-                         C.POINTER(_d), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8))
+                         C.POINTER(_d), C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
+                         C.POINTER(C.c_uint8))
 
 
 class Rk45Desc(C.Structure):

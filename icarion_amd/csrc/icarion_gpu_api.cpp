@@ -35,8 +35,8 @@ hipError_t launch_rk45_step(int field, int coll, int precision, const StepParams
                             const Rk45Ptrs& R, uint32_t block, hipStream_t stream);
 hipError_t launch_rk45_init(const Rk45Ptrs& R, uint32_t n, double t0, double dt, hipStream_t stream);
 hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* pos, double* vel, uint32_t* spec,
-                                uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count, uint32_t* host_count,
-                                uint32_t* next_count, hipStream_t s);
+                                int32_t* dom, uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count,
+                                uint32_t* host_count, uint32_t* next_count, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, hipStream_t s);
 hipError_t launch_multi_step(int integrator, int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
                              hipStream_t s);
@@ -253,8 +253,8 @@ struct icarion_gpu {
     int clock_khz = 0;
 
     // trajectory snapshots: kSnapSlots device staging slots, each holding one packed
-    // snapshot in full (positions [n][3], velocities [n][3], species [n], flags
-    // [n]: 53 B per ion) and the compacted records of the ions that changed
+    // snapshot in full (positions [n][3], velocities [n][3], species [n], domain
+    // index [n], flags [n]: 57 B per ion) and the compacted records of the ions that changed
     // since the previous snapshot (56 B each) with their count; the flags at
     // the last snapshot; and the pinned host copy of the current snapshot that
     // the callback reads, updated in full or by the changed records
@@ -263,7 +263,7 @@ struct icarion_gpu {
         DevBuf<uint8_t> full[3], rec[3];        // kSnapSlots
         DevBuf<uint32_t> count;                 // [kSnapSlots]
         DevBuf<uint8_t> prev;
-        void* host_full = nullptr;              // hipHostMalloc, 53 B per ion
+        void* host_full = nullptr;              // hipHostMalloc, 57 B per ion
         void* host_rec = nullptr;               // hipHostMalloc, 56 B per ion
         uint32_t* host_count = nullptr;         // hipHostMalloc, [kSnapSlots]
         uint32_t* host_count_dev = nullptr;     // its device address
@@ -803,8 +803,8 @@ std::vector<SnapshotAt> plan_snapshots(const std::vector<StepEntry>& table, doub
 // Snapshots in flight: snapshot q is brought to the host once snapshot q + 2 is
 // queued, so two write intervals of steps keep the device busy meanwhile.
 constexpr int kSnapSlots = 3;
-constexpr size_t kSnapFullBytes = 53;     // per ion: 6 doubles, species, flags
-constexpr size_t kSnapRecordBytes = 56;   // per changed ion: 6 doubles, index, flags
+constexpr size_t kSnapFullBytes = 57;     // per ion: 6 doubles, species, domain, flags
+constexpr size_t kSnapRecordBytes = 56;   // per changed ion: 6 doubles, index, flags, domain
 
 void ensure_snap_ring(icarion_gpu* h) {
     auto& R = h->snap;
@@ -834,14 +834,158 @@ void ensure_snap_ring(icarion_gpu* h) {
     R.cap = h->n;
 }
 
+// The launch loop of one stepping call: launches of at most spl steps, cut at
+// the snapshot steps of plan (empty: no snapshots).  After each snapshot step
+// the state is packed on the step stream and moved to the host while the next
+// steps run.  launch(k0, nk) queues steps k0 .. k0 + nk - 1 of the call.
+// Returns the device time from the first launch to the last (ms).
+template <class Launch>
+float run_launches(icarion_gpu* h, const IonPtrs& S, uint64_t n_steps, uint64_t step_offset, uint64_t spl,
+                   const std::vector<SnapshotAt>& plan, const SnapshotReq* snap, Launch&& launch) {
+    auto& R = h->snap;
+    const size_t n = h->n;
+    size_t q_issued = 0, q_done = 0;
+    bool stopped = false;
+    uint64_t stop_step = 0;
+    // the pinned snapshot the callback sees (none when the call takes no snapshots)
+    uint8_t* const hf = plan.empty() ? nullptr : static_cast<uint8_t*>(R.host_full);
+    double* const hpos = hf ? reinterpret_cast<double*>(hf) : nullptr;
+    double* const hvel = hf ? hpos + 3 * n : nullptr;
+    uint32_t* const hspec = hf ? reinterpret_cast<uint32_t*>(hf + 48 * n) : nullptr;
+    int32_t* const hdom = hf ? reinterpret_cast<int32_t*>(hf + 52 * n) : nullptr;
+    uint8_t* const hflags = hf ? hf + 56 * n : nullptr;
+    // Snapshot q_done moves to the host in two stages, each started as soon
+    // as the one before has finished, so the calling thread keeps queueing
+    // steps meanwhile: once its pack (and count) is done, the copy starts
+    // on a side stream - of the changed ions' records when they are few,
+    // else of the whole snapshot; once the copy is done, the records are
+    // scattered over the previous snapshot and fn is called.  block = wait
+    // (spinning) until snapshot q_done is delivered.
+    bool copying = false, compact = false;
+    size_t n_rec = 0;
+    auto ready = [](hipEvent_t e, bool block) {
+        for (;;) {
+            const hipError_t r = hipEventQuery(e);
+            if (r == hipSuccess) return true;
+            if (r != hipErrorNotReady) hip_check(r, "hipEventQuery");
+            if (!block) return false;
+        }
+    };
+    auto advance = [&](bool block) {
+        while (q_done < q_issued) {
+            const int j = static_cast<int>(q_done % kSnapSlots);
+            if (!copying) {
+                if (!ready(R.packed[j], block)) return;
+                n_rec = R.host_count[j];
+                if (n_rec > n) fail(ICARION_ERR_HIP, "snapshot record count exceeds the ion count");
+                compact = q_done > 0 && n_rec * 8 <= n;
+                if (!compact)
+                    hip_check(hipMemcpyAsync(hf, R.full[j].p, kSnapFullBytes * n, hipMemcpyDeviceToHost,
+                                             h->side[0]),
+                              "snapshot copy");
+                else if (n_rec)
+                    hip_check(hipMemcpyAsync(R.host_rec, R.rec[j].p, kSnapRecordBytes * n_rec,
+                                             hipMemcpyDeviceToHost, h->side[0]),
+                              "snapshot record copy");
+                hip_check(hipEventRecord(R.copied, h->side[0]), "hipEventRecord");
+                copying = true;
+            }
+            if (!ready(R.copied, block)) return;
+            copying = false;
+            if (compact) {
+                const uint8_t* r = static_cast<const uint8_t*>(R.host_rec);
+                for (size_t k = 0; k < n_rec; ++k, r += kSnapRecordBytes) {
+                    double pv[6];
+                    uint32_t idx;
+                    std::memcpy(pv, r, sizeof(pv));
+                    std::memcpy(&idx, r + 48, sizeof(idx));
+                    if (idx >= n) fail(ICARION_ERR_HIP, "snapshot record index out of range");
+                    std::memcpy(hpos + 3 * size_t(idx), pv, 3 * sizeof(double));
+                    std::memcpy(hvel + 3 * size_t(idx), pv + 3, 3 * sizeof(double));
+                    hflags[idx] = r[52];
+                    hdom[idx] = static_cast<int8_t>(r[53]);
+                }
+            }
+            const SnapshotAt& a = plan[q_done];
+            *snap->next = a.next;
+            ++q_done;
+            if (snap->fn(snap->user, q_done - 1, step_offset + a.k, a.t, n, hpos, hvel, hspec, hdom, hflags) != 0) {
+                stopped = true;
+                stop_step = step_offset + a.k;
+                return;
+            }
+            block = false;
+        }
+    };
+    // Pack snapshot q_issued on the step stream, its count published to
+    // pinned memory behind it; first free its slot if every slot is in flight.
+    auto issue = [&]() {
+        if (q_issued - q_done == kSnapSlots) advance(true);
+        if (stopped) return;
+        const int j = static_cast<int>(q_issued % kSnapSlots);
+        uint8_t* d = R.full[j].p;
+        double* pos = reinterpret_cast<double*>(d);
+        if (q_issued == 0) hip_check(launch_zero_u32(R.count.p, h->stream), "snapshot count clear");
+        hip_check(launch_pack_snapshot(S, static_cast<uint32_t>(n), q_issued == 0 ? 1 : 0, pos, pos + 3 * n,
+                                       reinterpret_cast<uint32_t*>(d + 48 * n),
+                                       reinterpret_cast<int32_t*>(d + 52 * n), d + 56 * n, R.prev.p,
+                                       R.rec[j].p, R.count.p + j, R.host_count_dev + j,
+                                       R.count.p + (j + 1) % kSnapSlots, h->stream),
+                  "snapshot pack launch");
+        hip_check(hipEventRecord(R.packed[j], h->stream), "hipEventRecord");
+        ++q_issued;
+        advance(false);
+    };
+    hip_check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+    size_t q_next = 0;
+    uint64_t k0 = 0;
+    while (k0 < n_steps && !stopped) {
+        uint64_t k1 = std::min<uint64_t>(k0 + spl, n_steps);
+        if (q_next < plan.size()) k1 = std::min<uint64_t>(k1, plan[q_next].k + 1);
+        launch(k0, k1 - k0);
+        if (q_next < plan.size() && plan[q_next].k + 1 == k1) {
+            issue();
+            ++q_next;
+        } else if (q_done < q_issued) {
+            advance(false);
+        }
+        k0 = k1;
+    }
+    hip_check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+    while (q_done < q_issued && !stopped) advance(true);
+    if (copying) (void)hipStreamSynchronize(h->side[0]);
+    float ms = 0.0f;
+    hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+    if (stopped)
+        fail(ICARION_ERR_STATE, "snapshot callback ended the run after step " + std::to_string(stop_step) +
+                                    "; steps through " + std::to_string(step_offset + k0 - 1) + " ran");
+    return ms;
+}
+
 void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
-                   icarion_step_stats* stats);
+                   icarion_step_stats* stats, const SnapshotReq* snap);
+
+// A snapshot schedule over a call with no resident ions: the callback still
+// sees every due snapshot, with n = 0 and no arrays.
+void deliver_empty_snapshots(const icarion_domain_desc& d, int integrator, double t0, double dt, uint64_t n_steps,
+                             uint64_t step_offset, const SnapshotReq& snap) {
+    std::vector<StepEntry> tab;
+    append_table(tab, d, false, integrator, t0, dt, n_steps);
+    const std::vector<SnapshotAt> plan = plan_snapshots(tab, dt, n_steps, snap);
+    for (size_t q = 0; q < plan.size(); ++q) {
+        *snap.next = plan[q].next;
+        if (snap.fn(snap.user, q, step_offset + plan[q].k, plan[q].t, 0, nullptr, nullptr, nullptr, nullptr,
+                    nullptr) != 0)
+            fail(ICARION_ERR_STATE, "snapshot callback ended the run after step " +
+                                        std::to_string(step_offset + plan[q].k));
+    }
+}
 
 void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
              icarion_step_stats* stats, const SnapshotReq* snap = nullptr) {
     if (h->have_domain && h->n_dom >= 2) {
-        if (snap) fail(ICARION_ERR_UNSUPPORTED, "snapshots are taken on single-domain handles");
-        do_step_multi(h, t0, dt, n_steps, step_offset, stats);
+        do_step_multi(h, t0, dt, n_steps, step_offset, stats, snap);
         return;
     }
     if (h->have_domain && h->phys.integrator == ICARION_INTEGRATOR_RK45) {
@@ -891,19 +1035,8 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     zero_counters(h, h->stream);
     double t_end = t0;
     float ms = 0.0f;
-    if (snap && n_steps > 0 && h->n == 0) {
-        // no ions: the schedule still runs, with empty snapshots
-        std::vector<StepEntry> tab;
-        append_table(tab, h->dom, false, h->phys.integrator, t0, dt, n_steps);
-        const std::vector<SnapshotAt> plan = plan_snapshots(tab, dt, n_steps, *snap);
-        for (size_t q = 0; q < plan.size(); ++q) {
-            *snap->next = plan[q].next;
-            if (snap->fn(snap->user, q, step_offset + plan[q].k, plan[q].t, 0, nullptr, nullptr, nullptr,
-                         nullptr) != 0)
-                fail(ICARION_ERR_STATE, "snapshot callback ended the run after step " +
-                                            std::to_string(step_offset + plan[q].k));
-        }
-    }
+    if (snap && n_steps > 0 && h->n == 0)
+        deliver_empty_snapshots(h->dom, h->phys.integrator, t0, dt, n_steps, step_offset, *snap);
     if (n_steps > 0 && h->n > 0) {
         build_table(h, t0, dt, n_steps);
         P.term_scales = h->use_term_scales ? h->term_scales.p : nullptr;
@@ -915,128 +1048,16 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
             plan = plan_snapshots(h->table_host, dt, n_steps, *snap);
             ensure_snap_ring(h);
         }
-        auto& R = h->snap;
-        const size_t n = h->n;
-        size_t q_issued = 0, q_done = 0;
-        bool stopped = false;
-        uint64_t stop_step = 0;
-        uint8_t* const hf = static_cast<uint8_t*>(R.host_full);
-        double* const hpos = reinterpret_cast<double*>(hf);
-        double* const hvel = hpos + 3 * n;
-        uint32_t* const hspec = reinterpret_cast<uint32_t*>(hf + 48 * n);
-        uint8_t* const hflags = hf + 52 * n;
-        // Snapshot q_done moves to the host in two stages, each started as soon
-        // as the one before has finished, so the calling thread keeps queueing
-        // steps meanwhile: once its pack (and count) is done, the copy starts
-        // on a side stream - of the changed ions' records when they are few,
-        // else of the whole snapshot; once the copy is done, the records are
-        // scattered over the previous snapshot and fn is called.  block = wait
-        // (spinning) until snapshot q_done is delivered.
-        bool copying = false, compact = false;
-        size_t n_rec = 0;
-        auto ready = [](hipEvent_t e, bool block) {
-            for (;;) {
-                const hipError_t r = hipEventQuery(e);
-                if (r == hipSuccess) return true;
-                if (r != hipErrorNotReady) hip_check(r, "hipEventQuery");
-                if (!block) return false;
-            }
-        };
-        auto advance = [&](bool block) {
-            while (q_done < q_issued) {
-                const int j = static_cast<int>(q_done % kSnapSlots);
-                if (!copying) {
-                    if (!ready(R.packed[j], block)) return;
-                    n_rec = R.host_count[j];
-                    if (n_rec > n) fail(ICARION_ERR_HIP, "snapshot record count exceeds the ion count");
-                    compact = q_done > 0 && n_rec * 8 <= n;
-                    if (!compact)
-                        hip_check(hipMemcpyAsync(hf, R.full[j].p, kSnapFullBytes * n, hipMemcpyDeviceToHost,
-                                                 h->side[0]),
-                                  "snapshot copy");
-                    else if (n_rec)
-                        hip_check(hipMemcpyAsync(R.host_rec, R.rec[j].p, kSnapRecordBytes * n_rec,
-                                                 hipMemcpyDeviceToHost, h->side[0]),
-                                  "snapshot record copy");
-                    hip_check(hipEventRecord(R.copied, h->side[0]), "hipEventRecord");
-                    copying = true;
-                }
-                if (!ready(R.copied, block)) return;
-                copying = false;
-                if (compact) {
-                    const uint8_t* r = static_cast<const uint8_t*>(R.host_rec);
-                    for (size_t k = 0; k < n_rec; ++k, r += kSnapRecordBytes) {
-                        double pv[6];
-                        uint32_t idx, fl;
-                        std::memcpy(pv, r, sizeof(pv));
-                        std::memcpy(&idx, r + 48, sizeof(idx));
-                        std::memcpy(&fl, r + 52, sizeof(fl));
-                        if (idx >= n) fail(ICARION_ERR_HIP, "snapshot record index out of range");
-                        std::memcpy(hpos + 3 * size_t(idx), pv, 3 * sizeof(double));
-                        std::memcpy(hvel + 3 * size_t(idx), pv + 3, 3 * sizeof(double));
-                        hflags[idx] = static_cast<uint8_t>(fl);
-                    }
-                }
-                const SnapshotAt& a = plan[q_done];
-                *snap->next = a.next;
-                ++q_done;
-                if (snap->fn(snap->user, q_done - 1, step_offset + a.k, a.t, n, hpos, hvel, hspec, hflags) != 0) {
-                    stopped = true;
-                    stop_step = step_offset + a.k;
-                    return;
-                }
-                block = false;
-            }
-        };
-        // Pack snapshot q_issued on the step stream, its count published to
-        // pinned memory behind it; first free its slot if every slot is in flight.
-        auto issue = [&]() {
-            if (q_issued - q_done == kSnapSlots) advance(true);
-            if (stopped) return;
-            const int j = static_cast<int>(q_issued % kSnapSlots);
-            uint8_t* d = R.full[j].p;
-            double* pos = reinterpret_cast<double*>(d);
-            if (q_issued == 0) hip_check(launch_zero_u32(R.count.p, h->stream), "snapshot count clear");
-            hip_check(launch_pack_snapshot(S, static_cast<uint32_t>(n), q_issued == 0 ? 1 : 0, pos, pos + 3 * n,
-                                           reinterpret_cast<uint32_t*>(d + 48 * n), d + 52 * n, R.prev.p,
-                                           R.rec[j].p, R.count.p + j, R.host_count_dev + j,
-                                           R.count.p + (j + 1) % kSnapSlots, h->stream),
-                      "snapshot pack launch");
-            hip_check(hipEventRecord(R.packed[j], h->stream), "hipEventRecord");
-            ++q_issued;
-            advance(false);
-        };
-        hip_check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
-        size_t q_next = 0;
-        uint64_t k0 = 0;
-        while (k0 < n_steps && !stopped) {
-            uint64_t k1 = std::min<uint64_t>(k0 + spl, n_steps);
-            if (q_next < plan.size()) k1 = std::min<uint64_t>(k1, plan[q_next].k + 1);
+        ms = run_launches(h, S, n_steps, step_offset, spl, plan, snap, [&](uint64_t k0, uint64_t nk) {
             P.table = h->table.p + k0;
             P.step0 = step_offset + k0;
-            P.n_steps = static_cast<uint32_t>(k1 - k0);
+            P.n_steps = static_cast<uint32_t>(nk);
             const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), launch_coll_kind(h),
-                                             h->phys.precision,
-                                             surrogate ? 1 : 0, P, S, h->block, h->stream);
+                                             h->phys.precision, surrogate ? 1 : 0, P, S, h->block, h->stream);
             if (e == hipErrorInvalidValue)
                 fail(ICARION_ERR_UNSUPPORTED, "no kernel for this integrator/field/collision/precision");
             hip_check(e, "step kernel launch");
-            if (q_next < plan.size() && plan[q_next].k + 1 == k1) {
-                issue();
-                ++q_next;
-            } else if (q_done < q_issued) {
-                advance(false);
-            }
-            k0 = k1;
-        }
-        hip_check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
-        while (q_done < q_issued && !stopped) advance(true);
-        if (copying) (void)hipStreamSynchronize(h->side[0]);
-        hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
-        hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
-        if (stopped)
-            fail(ICARION_ERR_STATE, "snapshot callback ended the run after step " + std::to_string(stop_step) +
-                                        "; steps through " + std::to_string(step_offset + k0 - 1) + " ran");
+        });
     } else {
         for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
     }
@@ -1101,7 +1122,7 @@ void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* st
 // coefficient tables, the span list, then one multi-domain launch per
 // steps_per_launch steps.
 void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
-                   icarion_step_stats* stats) {
+                   icarion_step_stats* stats, const SnapshotReq* snap) {
     if (!h->have_species) fail(ICARION_ERR_STATE, "icarion_gpu_set_species not called");
     if (!h->have_state) fail(ICARION_ERR_STATE, "icarion_gpu_upload_state not called");
     if (!(dt > 0.0) || !std::isfinite(dt)) fail(ICARION_ERR_INVALID, "dt must be finite and > 0");
@@ -1122,6 +1143,8 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
     zero_counters(h, h->stream);
     double t_end = t0;
     float ms = 0.0f;
+    if (snap && n_steps > 0 && h->n == 0)
+        deliver_empty_snapshots(h->doms[0], h->phys.integrator, t0, dt, n_steps, step_offset, *snap);
     if (n_steps > 0 && h->n > 0) {
         std::vector<StepParams> Ps(nd);
         std::vector<StepEntry> tabs;
@@ -1163,18 +1186,20 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         const IonPtrs S = ion_ptrs(h);
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
         const int integ = h->phys.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0;
-        hip_check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
-        for (uint64_t k0 = 0; k0 < n_steps; k0 += spl) {
+        std::vector<SnapshotAt> plan;
+        if (snap) {
+            // the engine times are the same in every domain's table
+            plan = plan_snapshots(std::vector<StepEntry>(tabs.begin(), tabs.begin() + n_steps), dt, n_steps, *snap);
+            ensure_snap_ring(h);
+        }
+        ms = run_launches(h, S, n_steps, step_offset, spl, plan, snap, [&](uint64_t k0, uint64_t nk) {
             M.k0 = static_cast<uint32_t>(k0);
             M.step0 = step_offset + k0;
-            M.n_steps = static_cast<uint32_t>(std::min<uint64_t>(spl, n_steps - k0));
+            M.n_steps = static_cast<uint32_t>(nk);
             const hipError_t e = launch_multi_step(integ, coll, M, S, h->block, h->stream);
             if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no multi-domain kernel for this case");
             hip_check(e, "multi-domain step launch");
-        }
-        hip_check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
-        hip_check(hipEventSynchronize(h->ev1), "hipEventSynchronize");
-        hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "hipEventElapsedTime");
+        });
     } else {
         for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
     }
@@ -1235,9 +1260,10 @@ void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
     copy_in(h->r_k1y, zeros.data(), n, s, "upload rk45 k1");
     copy_in(h->r_k1z, zeros.data(), n, s, "upload rk45 k1");
     copy_in(h->r_fsal, nofsal.data(), n, s, "upload rk45 fsal");
-    // domain index before the first step: none (set at each step start)
-    std::vector<int32_t> no_dom(n, -1);
-    copy_in(h->ion_dom, no_dom.data(), n, s, "upload domain index");
+    // domain index before the first step: 0, IonState's default (IonState.h:48);
+    // several domains set it at each step start
+    std::vector<int32_t> dom0(n, 0);
+    copy_in(h->ion_dom, dom0.data(), n, s, "upload domain index");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     h->n = n;
     h->have_state = true;

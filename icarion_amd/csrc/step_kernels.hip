@@ -691,7 +691,8 @@ hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint3
 //
 // One pass over the SoA state per snapshot:
 //   * the full snapshot in OutputManager's layout (positions and velocities
-//     interleaved [n][3] in FP64, species, flags) into a device staging slot;
+//     interleaved [n][3] in FP64, species, domain index, flags) into a
+//     device staging slot;
 //   * the records of the ions that may have changed since the previous
 //     snapshot: those active now, or whose flags changed (died or were born
 //     since).  Unborn and dead ions do not move, so the others hold the values
@@ -703,7 +704,9 @@ struct SnapRecord {
     double p[3];
     double v[3];
     uint32_t idx;
-    uint32_t flags;
+    uint8_t flags;
+    int8_t domain;
+    uint16_t pad;
 };
 static_assert(sizeof(SnapRecord) == 56, "snapshot record layout");
 
@@ -714,6 +717,7 @@ constexpr int kPackThreads = 1024;
 __global__ void __launch_bounds__(kPackThreads) pack_snapshot_kernel(const IonPtrs S, uint32_t n, int full,
                                                                      double* __restrict__ pos, double* __restrict__ vel,
                                                                      uint32_t* __restrict__ spec,
+                                                                     int32_t* __restrict__ dom,
                                                                      uint8_t* __restrict__ flags,
                                                                      uint8_t* __restrict__ prev,
                                                                      SnapRecord* __restrict__ rec,
@@ -725,14 +729,17 @@ __global__ void __launch_bounds__(kPackThreads) pack_snapshot_kernel(const IonPt
     bool sel = false;
     double x = 0, y = 0, z = 0, vx = 0, vy = 0, vz = 0;
     uint8_t f = 0;
+    int32_t d = 0;
     if (in) {
         x = S.x[i]; y = S.y[i]; z = S.z[i];
         vx = S.vx[i]; vy = S.vy[i]; vz = S.vz[i];
         f = S.flags[i];
+        d = S.domain ? S.domain[i] : 0;
         const size_t o = 3ull * i;
         pos[o] = x; pos[o + 1] = y; pos[o + 2] = z;
         vel[o] = vx; vel[o + 1] = vy; vel[o + 2] = vz;
         spec[i] = S.species[i];
+        dom[i] = d;
         flags[i] = f;
         sel = full || (f & 1) || f != prev[i];
         prev[i] = f;
@@ -761,6 +768,8 @@ __global__ void __launch_bounds__(kPackThreads) pack_snapshot_kernel(const IonPt
         r.v[0] = vx; r.v[1] = vy; r.v[2] = vz;
         r.idx = i;
         r.flags = f;
+        r.domain = static_cast<int8_t>(d);
+        r.pad = 0;
         rec[k] = r;
     }
 }
@@ -788,11 +797,11 @@ hipError_t launch_zero_u32(uint32_t* p, hipStream_t s) {
 // count must be zero on entry (launch_zero_u32 before the first pack; each
 // publish clears next_count for the pack after it).
 hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* pos, double* vel, uint32_t* spec,
-                                uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count, uint32_t* host_count,
-                                uint32_t* next_count, hipStream_t s) {
+                                int32_t* dom, uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count,
+                                uint32_t* host_count, uint32_t* next_count, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(pack_snapshot_kernel, dim3((n + kPackThreads - 1) / kPackThreads), dim3(kPackThreads), 0, s,
-                       S, n, full, pos, vel, spec, flags, prev, static_cast<SnapRecord*>(rec), count);
+                       S, n, full, pos, vel, spec, dom, flags, prev, static_cast<SnapRecord*>(rec), count);
     hipLaunchKernelGGL(publish_count_kernel, dim3(1), dim3(64), 0, s, count, host_count, next_count);
     return hipGetLastError();
 }

@@ -437,22 +437,24 @@ class Handle:
         nxt = C.c_double(write_interval if next_write is None else next_write)
         out = []
 
-        def cb(_user, index, step, t, n, pos, vel, spec, flags):
+        def cb(_user, index, step, t, n, pos, vel, spec, dom, flags):
             if n:
                 snap = {
                     "index": index, "step": step, "t": t,
                     "positions": np.ctypeslib.as_array(pos, shape=(n, 3)),
                     "velocities": np.ctypeslib.as_array(vel, shape=(n, 3)),
                     "species": np.ctypeslib.as_array(spec, shape=(n,)),
+                    "domain": np.ctypeslib.as_array(dom, shape=(n,)),
                     "flags": np.ctypeslib.as_array(flags, shape=(n,)),
                 }
                 if copy or on_snapshot is None:
-                    for k in ("positions", "velocities", "species", "flags"):
+                    for k in ("positions", "velocities", "species", "domain", "flags"):
                         snap[k] = snap[k].copy()
             else:
                 snap = {"index": index, "step": step, "t": t,
                         "positions": np.zeros((0, 3)), "velocities": np.zeros((0, 3)),
-                        "species": np.zeros(0, np.uint32), "flags": np.zeros(0, np.uint8)}
+                        "species": np.zeros(0, np.uint32), "domain": np.zeros(0, np.int32),
+                        "flags": np.zeros(0, np.uint8)}
             if on_snapshot is not None:
                 on_snapshot(snap)
             else:

@@ -13,7 +13,7 @@ the buffered snapshots to /trajectory (hdf5Writer.cpp:533-660):
 TrajectoryBuffer collects the snapshots that gpu.Handle.step_snapshots hands over.
 It writes the same datasets into an .npz file, with keys "trajectory/<name>".
 This image has no HDF5 library, so HDF5 output itself is not provided.
-A handle holds one domain, so every domain index is 0.
+domain_indices are the snapshot's domain indices, or 0 when it carries none.
 
 time_per_ion is not produced. For the fixed-step integrators the device keeps no
 per-ion time, and the reference writes that dataset only when it has one.
@@ -65,7 +65,8 @@ class TrajectoryBuffer:
             "time": np.array([s["t"] for s in snaps], dtype=np.float64),
             "positions": np.stack([s["positions"] for s in snaps]).reshape(T, n, 3),
             "velocities": np.stack([s["velocities"] for s in snaps]).reshape(T, n, 3),
-            "domain_indices": np.zeros((T, n), dtype=np.int32),
+            "domain_indices": np.stack([s["domain"] if "domain" in s else np.zeros(n, np.int32)
+                                        for s in snaps]).astype(np.int32).reshape(T, n),
             "species_id_indices": np.stack([s["species"] for s in snaps]).astype(np.uint32).reshape(T, n),
         }
 

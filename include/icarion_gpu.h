@@ -478,24 +478,28 @@ int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
  * Replaces any domain set before; n_domains = 1 is icarion_gpu_set_domain.
  * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
  * voltages, RK4 or Boris, FP64, no / HSS / Langevin / Friction collisions, one
- * species table for all domains.  Other combinations return
- * ICARION_ERR_UNSUPPORTED when stepped. */
+ * species table for all domains; icarion_gpu_step, icarion_gpu_step_batch and
+ * icarion_gpu_step_snapshots (with each ion's domain index).  Other
+ * combinations return ICARION_ERR_UNSUPPORTED when stepped. */
 int icarion_gpu_set_domains(icarion_gpu *h, const icarion_domain_desc *domains,
                             const double *end_aperture_m, uint32_t n_domains,
                             const icarion_physics_desc *physics);
 
 /* Every resident ion's domain index after the last step (the ensemble's
- * domain_index; -1 before the ion's first step).  n = the resident count. */
+ * domain_index; 0 before the ion's first step, IonState's default).  n = the
+ * resident count. */
 int icarion_gpu_download_domains(icarion_gpu *h, int32_t *domain_index);
 
 /* One trajectory snapshot, in the layout OutputManager::log_step buffers and
  * HDF5Writer writes (OutputManager.cpp:197-227, hdf5Writer.cpp:158-299):
  * positions and velocities interleaved per ion [n][3] (FP64), the ion's
- * species index and flags (bit 0 active, bit 1 born).  The arrays are valid
- * during the call only.  Return 0 to continue, nonzero to stop the run. */
+ * species index, domain index and flags (bit 0 active, bit 1 born).  The
+ * arrays are valid during the call only.  Return 0 to continue, nonzero to
+ * stop the run. */
 typedef int (*icarion_snapshot_fn)(void *user, uint64_t index, uint64_t step, double t_s, size_t n,
                                    const double *positions, const double *velocities,
-                                   const uint32_t *species, const uint8_t *flags);
+                                   const uint32_t *species, const int32_t *domain_index,
+                                   const uint8_t *flags);
 
 /* icarion_gpu_step with the trajectory sampling of SimulationEngine::run /
  * OutputManager (SimulationEngine.cpp:268-283, OutputManager.cpp:246-253):

@@ -154,6 +154,7 @@ def test_snapshots_with_deaths_and_births(precision):
             np.testing.assert_array_equal(s["flags"] & 1, e.active)
             np.testing.assert_array_equal((s["flags"] >> 1) & 1, e.born)
             np.testing.assert_array_equal(s["species"], e.species)
+            assert not s["domain"].any()   # one domain: index 0 throughout
             t, prev = st.t_end_s, s["step"] + 1
     finally:
         h.close()
