@@ -148,7 +148,8 @@ MAX_GAS_COMPONENTS = 4
 
 class OracleGasMixture(C.Structure):
     _fields_ = [("n_components", _u32), ("n_species", _u32),
-                ("components", C.POINTER(GasComponent)), ("sigma_m2", C.POINTER(_d))]
+                ("components", C.POINTER(GasComponent)), ("sigma_m2", C.POINTER(_d)),
+                ("radius_m", C.POINTER(_d))]
 
 
 class OracleEhssOffline(C.Structure):

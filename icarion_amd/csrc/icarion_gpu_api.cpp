@@ -180,6 +180,11 @@ struct icarion_gpu {
     std::vector<icarion_gas_component> mix;
     uint32_t mix_species = 0;
     DevBuf<double> mix_sigma;
+    // the same mixture as given (every component, for the EHSS handler's loop),
+    // its full cross-section table and the components' contact radii
+    std::vector<icarion_gas_component> mix_all;
+    std::vector<double> mix_radius;
+    DevBuf<double> emix_sigma;
     // offline EHSS sample sets (per species {first orientation, orientations,
     // mu per orientation, first mu}; none when off_species == 0)
     uint32_t off_species = 0;
@@ -318,6 +323,7 @@ int coll_kind(const icarion_physics_desc& p) {
 int launch_coll_kind(const icarion_gpu* h) {
     const int c = coll_kind(h->phys);
     if (c == COLL_HSS && !h->mix.empty()) return COLL_HSS_MIX;
+    if (c == COLL_EHSS && !h->mix_all.empty() && h->phys.ehss_mode == ICARION_EHSS_GEOMETRY) return COLL_EHSS_MIX;
     if (c == COLL_EHSS && (h->off_species || h->ori_species) && h->phys.ehss_mode == ICARION_EHSS_GEOMETRY)
         return COLL_EHSS_TAB;
     return c;
@@ -547,6 +553,20 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
         // VelocitySampling::sample_neutral_velocity's width (VelocitySampling.cpp:17)
         P.mix_sig[c] = on ? std::sqrt(kBoltzmannHost * h->dom.temperature_K / h->mix[c].mass_kg) : 0.0;
     }
+    // EHSS: every component; contact radius = the component's, else the
+    // domain's gas radius (EHSSCollisionHandler.cpp:348, 415)
+    P.n_emix = static_cast<int>(h->mix_all.size());
+    P.emix_sigma = h->mix_all.empty() ? nullptr : h->emix_sigma.p;
+    for (int c = 0; c < kMaxGasComponents; ++c) {
+        const bool on = c < P.n_emix;
+        const icarion_gas_component g = on ? h->mix_all[c] : icarion_gas_component{0.0, 0.0, 0, 0};
+        P.emix_part[c] = on && g.participates ? 1 : 0;
+        P.emix_n[c] = g.density_m3;
+        P.emix_m[c] = g.mass_kg;
+        P.emix_sig[c] = (on && g.mass_kg > 0.0) ? std::sqrt(kBoltzmannHost * h->dom.temperature_K / g.mass_kg) : 0.0;
+        const double r = (on && c < static_cast<int>(h->mix_radius.size())) ? h->mix_radius[c] : 0.0;
+        P.emix_r[c] = r > 0.0 ? r : h->dom.gas_radius_m;
+    }
 }
 
 // CompositeFieldProvider::eval_voltage (CompositeFieldProvider.h:150-171): the
@@ -657,6 +677,19 @@ IonPtrs ion_ptrs(icarion_gpu* h) {
     return S;
 }
 
+// EHSS over a gas mixture: the geometry method only; orientation samples would
+// need an area per component gas, which the single-gas tables do not hold;
+// every participating component needs a mass for its neutral draw.
+void check_ehss_mixture(const icarion_gpu* h, bool surrogate) {
+    if (h->mix_all.empty()) return;
+    if (surrogate) fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with a single gas");
+    if (h->ori_species)
+        fail(ICARION_ERR_UNSUPPORTED, "EHSS orientation samples hold areas for one gas, not for a mixture");
+    for (const auto& g : h->mix_all)
+        if (g.participates && !(g.mass_kg > 0.0))
+            fail(ICARION_ERR_INVALID, "every participating gas component needs a mass > 0");
+}
+
 // The RK45 engine loop: one launch pair (step kernel + engine-time advance) per
 // engine step, replayed from a captured HIP graph of up to 32 steps.
 void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
@@ -668,6 +701,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
         fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with the fixed-step FP32 path");
+    if (coll == COLL_EHSS) check_ehss_mixture(h, false);
     set_device(h);
     hipStream_t s = h->stream;
     StepParams P = h->base;
@@ -1015,6 +1049,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
         fail(ICARION_ERR_STATE, "EHSS surrogate selected but no parameters set");
     if (coll == COLL_EHSS && !surrogate && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
+    if (coll == COLL_EHSS) check_ehss_mixture(h, surrogate);
     if (h->has_waves && field == FIELD_ORBITRAP)
         fail(ICARION_ERR_UNSUPPORTED, "the Orbitrap field runs with constant voltages");
     set_device(h);
@@ -1874,7 +1909,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
     for (auto& a : h->ax64) a.release();
     for (auto& a : h->ax32) a.release();
-    h->mix_sigma.release();
+    h->mix_sigma.release(); h->emix_sigma.release();
     h->off_tab.release(); h->off_sigma.release(); h->off_mu.release();
     h->ori_tab.release(); h->ori_R.release(); h->ori_area.release();
     h->r_t.release(); h->r_dt.release(); h->r_err.release(); h->r_k1x.release(); h->r_k1y.release();
@@ -2128,8 +2163,11 @@ int icarion_gpu_set_gas_mixture(icarion_gpu* h, const icarion_gas_component* com
         if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
         if (n_components == 0) {
             h->mix.clear();
+            h->mix_all.clear();
+            h->mix_radius.clear();
             h->mix_species = 0;
             h->mix_sigma.release();
+            h->emix_sigma.release();
             return;
         }
         if (!components || !sigma_m2) fail(ICARION_ERR_INVALID, "NULL argument");
@@ -2165,9 +2203,25 @@ int icarion_gpu_set_gas_mixture(icarion_gpu* h, const icarion_gas_component* com
                                                 1, 0});
         }
         copy_in(h->mix_sigma, sig.data(), sig.size(), h->stream, "upload mixture cross sections");
+        copy_in(h->emix_sigma, sigma_m2, static_cast<size_t>(n_species) * n_components, h->stream,
+                "upload mixture cross sections");
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         h->mix = std::move(mix);
+        h->mix_all.assign(components, components + n_components);
+        h->mix_radius.assign(n_components, 0.0);
         h->mix_species = n_species;
+    });
+}
+
+int icarion_gpu_set_gas_radii(icarion_gpu* h, const double* radius_m, uint32_t n_components) {
+    return guarded([&] {
+        if (!h || !radius_m) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (n_components != h->mix_all.size())
+            fail(ICARION_ERR_INVALID, "n_components must match the gas mixture (set_gas_mixture first)");
+        for (uint32_t c = 0; c < n_components; ++c)
+            if (!(radius_m[c] >= 0.0) || !std::isfinite(radius_m[c]))
+                fail(ICARION_ERR_INVALID, "radii must be finite and >= 0");
+        h->mix_radius.assign(radius_m, radius_m + n_components);
     });
 }
 

@@ -522,6 +522,7 @@ Rk45Fn pick_rk45_coll(int coll, int precision) {
         case COLL_HSS_MIX: return rk45_step_kernel<FIELD, COLL_HSS_MIX>;
         case COLL_EHSS: return rk45_step_kernel<FIELD, COLL_EHSS>;
         case COLL_EHSS_TAB: return rk45_step_kernel<FIELD, COLL_EHSS_TAB>;
+        case COLL_EHSS_MIX: return rk45_step_kernel<FIELD, COLL_EHSS_MIX>;
         case COLL_DAMPING: return rk45_step_kernel<FIELD, COLL_DAMPING>;
         default: return nullptr;
     }

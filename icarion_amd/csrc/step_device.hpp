@@ -222,8 +222,8 @@ __device__ __forceinline__ void ortho_basis_f64(d3 e, d3& t1, d3& t2) {
 // impact parameters, specular reflection.  Rotated atoms are recomputed per
 // attempt instead of stored (same arithmetic).
 template <class Rot>
-__device__ d3 ehss_core_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, uint32_t a0, uint32_t nat,
-                            const Rot& O, double sigma_eff, int max_attempts, PhiloxStream& rng) {
+__device__ d3 ehss_core_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, double r_gas, uint32_t a0,
+                            uint32_t nat, const Rot& O, double sigma_eff, int max_attempts, PhiloxStream& rng) {
     d3 vrel = v_ion - v_n;
     double vrel_mag = sqrt(dot(vrel, vrel));
     if (vrel_mag <= kMinVelocityMag) {
@@ -253,7 +253,7 @@ __device__ d3 ehss_core_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
         const double cur = sqrt(dot(rp, rp)) + c.w;
         if (cur > b_max) b_max = cur;
     }
-    b_max += P.r_gas;
+    b_max += r_gas;
     if (sigma_eff > 0.0) {
         const double b_sigma = sqrt(sigma_eff / kPi);
         if (b_sigma > b_max) b_max = b_sigma;
@@ -271,7 +271,7 @@ __device__ d3 ehss_core_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
         for (uint32_t j = 0; j < nat; ++j) {
             const double4 c = atom(j);
             const d3 ra = mat_mul(Rm, d3{c.x, c.y, c.z});
-            const double Rsum = c.w + P.r_gas;
+            const double Rsum = c.w + r_gas;
             const double Rsum2 = Rsum * Rsum;
             const d3 rel = off - ra;
             const double sstar = dot(ax, rel);
@@ -310,8 +310,8 @@ __device__ d3 ehss_core_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
 
 // CollisionKernels::ehss_collision (CollisionKernels.cpp:13-42): a Shoemake
 // orientation (CollisionGeometry::generate_random_rotation), then the core
-// with sigma_eff = 0 and 256 attempts.
-__device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn,
+// with sigma_eff = 0 and 256 attempts; r_gas = the contact radius.
+__device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, double r_gas,
                        uint32_t a0, uint32_t nat, PhiloxStream& rng) {
     const double u1 = rng.uniform_f64();
     const double u2 = rng.uniform_f64();
@@ -326,13 +326,78 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
     O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
     O[3] = 2 * (x * y + z * w);     O[4] = 1 - 2 * (x * x + z * z); O[5] = 2 * (y * z - x * w);
     O[6] = 2 * (x * z - y * w);     O[7] = 2 * (y * z + x * w);     O[8] = 1 - 2 * (x * x + y * y);
-    return ehss_core_f64(P, v_ion, v_n, mi, mn, a0, nat, O, 0.0, kEhssMaxAttempts, rng);
+    return ehss_core_f64(P, v_ion, v_n, mi, mn, r_gas, a0, nat, O, 0.0, kEhssMaxAttempts, rng);
 }
 
 // One collision stage; returns true on a collision event.
 template <int COLL>
 __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDev& sp, uint32_t sidx,
                                             double m, double lpre, d3& v, PhiloxStream& rng, double dt) {
+    if constexpr (COLL == COLL_EHSS_MIX) {
+        // EHSSCollisionHandler::handle_collision mixture path
+        // (EHSSCollisionHandler.cpp:311-424): every component visited in order,
+        // a neutral drawn for each participating one, rates from the actual
+        // relative speeds, one test on the total rate, the partner drawn with
+        // weight k_c (the last component when the running sum falls short),
+        // then the geometry method with the partner's neutral, mass and contact
+        // radius and no second test.  Offline sets are not used with a mixture.
+        const double* sg = P.emix_sigma + static_cast<size_t>(sidx) * P.n_emix;
+        d3 vn[kMaxGasComponents];
+        double k[kMaxGasComponents];
+        double k_total = 0.0;
+#pragma unroll
+        for (int c = 0; c < kMaxGasComponents; ++c) {
+            k[c] = 0.0;
+            vn[c] = d3{0.0, 0.0, 0.0};
+            if (c < P.n_emix && P.emix_part[c]) {
+                const double sn = P.emix_sig[c];
+                double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
+                vn[c].x = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+                vn[c].y = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+                vn[c].z = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+                vn[c] = vn[c] + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+                const d3 vr = v - vn[c];
+                const double vm = sqrt(dot(vr, vr));
+                if (sg[c] > 0.0 && P.emix_n[c] > 0.0 && vm >= kMinRelativeVelocity) {
+                    k[c] = P.emix_n[c] * sg[c] * vm;
+                    k_total += k[c];
+                }
+            }
+        }
+        if (!(k_total > 0.0)) return false;
+        double Pt = 1.0;
+        if (k_total * dt <= 50.0) Pt = 1.0 - exp(-k_total * dt);
+        if (rng.uniform_f64() >= Pt) return false;
+        const double r = rng.uniform_f64() * k_total;
+        int sel = -1;
+        double cum = 0.0;
+#pragma unroll
+        for (int c = 0; c < kMaxGasComponents; ++c) {
+            if (c < P.n_emix) {
+                cum += k[c];
+                if (sel < 0 && r < cum) sel = c;
+            }
+        }
+        if (sel < 0) sel = P.n_emix - 1;
+        d3 vs{0.0, 0.0, 0.0};
+        double ms = 0.0, rs = 0.0, sig_eff = 0.0;
+#pragma unroll
+        for (int c = 0; c < kMaxGasComponents; ++c) {
+            if (c == sel) {
+                vs = vn[c];
+                ms = P.emix_m[c];
+                rs = P.emix_r[c];
+                sig_eff = sg[c];
+            }
+        }
+        if (sig_eff <= 0.0) return false;
+        const d3 vr = v - vs;
+        if (sqrt(dot(vr, vr)) < kMinRelativeVelocity) return false;
+        v = ehss_f64(P, v, vs, m, ms, rs, sp.atom_offset, sp.atom_count, rng);
+        return true;
+    }
     if constexpr (COLL == COLL_HSS_MIX) {
         {
             // HSSCollisionHandler::handle_collision mixture path
@@ -482,10 +547,10 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
             // the stored orientation, sigma_eff and the forced-hit budget
             // (EHSSCollisionHandler.cpp:286-293, CollisionKernels.cpp:110)
             const double* O = P.ehss_ori_R + 9ull * (ori.x + oi);
-            v = ehss_core_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, O, sigma, kEhssMaxAttempts * 16,
-                              rng);
+            v = ehss_core_f64(P, v, vn, m, P.m_gas, P.r_gas, sp.atom_offset, sp.atom_count, O, sigma,
+                              kEhssMaxAttempts * 16, rng);
         } else {
-            v = ehss_f64(P, v, vn, m, P.m_gas, sp.atom_offset, sp.atom_count, rng);
+            v = ehss_f64(P, v, vn, m, P.m_gas, P.r_gas, sp.atom_offset, sp.atom_count, rng);
         }
         return true;
     } else if constexpr (COLL == COLL_DAMPING) {

@@ -822,6 +822,7 @@ KernelFn pick_coll(int coll, int precision) {
         case COLL_HSS_MIX: return pick_prec<I, F, COLL_HSS_MIX>(precision);
         case COLL_EHSS: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS>;
         case COLL_EHSS_TAB: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS_TAB>;
+        case COLL_EHSS_MIX: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS_MIX>;
         case COLL_DAMPING: return pick_prec<I, F, COLL_DAMPING>(precision);
         default: return pick_prec<I, F, COLL_NONE>(precision);
     }

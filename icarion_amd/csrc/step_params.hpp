@@ -22,7 +22,8 @@ enum CollKind : int {
     COLL_EHSS = 2,
     COLL_DAMPING = 3,    // Langevin or Friction damping force (+ OU kicks)
     COLL_HSS_MIX = 4,    // HSS over a gas mixture
-    COLL_EHSS_TAB = 5    // EHSS with offline sample sets for some species
+    COLL_EHSS_TAB = 5,   // EHSS with offline sample sets for some species
+    COLL_EHSS_MIX = 6    // EHSS over a gas mixture
 };
 
 // Per-species record on the device.
@@ -169,6 +170,14 @@ struct StepParams {
     double mix_n[kMaxGasComponents], mix_m[kMaxGasComponents];
     double mix_sig[kMaxGasComponents];   // sqrt(kB T / m_c)
     const double *mix_sigma;
+    // EHSS gas mixture: every component in order (the EHSS handler's loop
+    // visits them all), its participation flag, contact radius, and the EHSS
+    // cross sections of species s at emix_sigma[s * n_emix + c]
+    int n_emix, emix_pad;
+    int emix_part[kMaxGasComponents];
+    double emix_n[kMaxGasComponents], emix_m[kMaxGasComponents];
+    double emix_sig[kMaxGasComponents], emix_r[kMaxGasComponents];
+    const double *emix_sigma;
     // offline EHSS sample sets per species (NULL: none): {first orientation,
     // orientations (0: the online geometry), mu per orientation, first mu}
     const uint4 *ehss_off;

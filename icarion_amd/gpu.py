@@ -25,7 +25,7 @@ EXPORTED = (
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
-    "icarion_gpu_set_ehss_orientations",
+    "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii",
 )
 
 
@@ -89,6 +89,7 @@ def load(path=None):
     lib.icarion_gpu_download_domains.argtypes = [C.c_void_p, P(C.c_int32)]
     lib.icarion_gpu_set_ehss_orientations.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_double),
                                                       P(C.c_double), C.c_uint32]
+    lib.icarion_gpu_set_gas_radii.argtypes = [C.c_void_p, P(C.c_double), C.c_uint32]
     lib.icarion_gpu_synchronize.argtypes = [C.c_void_p]
     lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
                                            C.c_uint64, P(_abi.StepStats)]
@@ -371,6 +372,12 @@ class Handle:
         _check(self.lib.icarion_gpu_set_ehss_offline(
             self.h, n_orient.ctypes.data_as(u32), stride.ctypes.data_as(u32),
             sg.ctypes.data_as(d), mu.ctypes.data_as(d), n), self.lib)
+
+    def set_gas_radii(self, radii):
+        """Contact radius per gas component of the mixture (0: the domain's)."""
+        r = np.ascontiguousarray(radii, dtype=np.float64)
+        _check(self.lib.icarion_gpu_set_gas_radii(self.h, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                                  len(r)), self.lib)
 
     def set_ehss_orientations(self, sets):
         """sets[s]: None (no orientation samples) or (quat_wxyz[n, 4], area_m2[n]);

@@ -358,9 +358,23 @@ int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
  * gas > the component's cross_section_m2 > the ion CCS); it must be > 0 for
  * every component that participates with n_i > 0 (the reference throws
  * otherwise).  n_species must equal that of icarion_gpu_set_species.
- * n_components = 0 returns to the single-gas path.  Affects HSS only. */
+ * n_components = 0 returns to the single-gas path.
+ * With EHSS (geometry method) the same table holds the EHSS effective cross
+ * sections per gas (compute_effective_ccs with the component's radius), and
+ * the handler's mixture path runs (EHSSCollisionHandler.cpp:311-424): every
+ * component visited in order, a neutral drawn for each participating one
+ * (also at n_i = 0), k_i = n_i sigma_i |v - v_i| where all three are > 0,
+ * one test on k_total, the partner drawn with weight k_i (the last component
+ * when the running sum falls short), then the first-contact geometry method
+ * with that neutral, mass and contact radius (icarion_gpu_set_gas_radii).
+ * Offline sample sets are not used with a mixture, as in the reference; EHSS
+ * orientation samples and the surrogate return ICARION_ERR_UNSUPPORTED with it. */
 int icarion_gpu_set_gas_mixture(icarion_gpu *h, const icarion_gas_component *components,
                                 uint32_t n_components, const double *sigma_m2, uint32_t n_species);
+
+/* Contact radius of each gas component of the mixture (GasMixtureComponent
+ * radius_m; 0 = the domain's gas_radius_m), for EHSS over a mixture. */
+int icarion_gpu_set_gas_radii(icarion_gpu *h, const double *radius_m, uint32_t n_components);
 
 /* EHSS orientation samples per species (EHSSOrientationSamples,
  * EHSSSamples.h:14-17: orientations_quat and the areas_by_gas_m2 entry of the

@@ -70,6 +70,9 @@ def load():
     lib.oracle_mt_ehss_oriented.argtypes = [C.c_uint64, P(d), P(d), d, d, d, P(d), C.c_uint32,
                                             P(d), d, P(d)]
     lib.oracle_quaternion_to_rotation.argtypes = [P(d), P(d)]
+    lib.oracle_mt_ehss_mixture.argtypes = [C.c_uint64, P(_abi.DomainDesc), P(_abi.OracleGasMixture),
+                                           C.c_uint32, P(d), C.c_uint32, d, P(d), d]
+    lib.oracle_mt_ehss_mixture.restype = C.c_int
     lib.oracle_oapa_sigma.argtypes = [P(d), C.c_uint32, d]
     lib.oracle_oapa_sigma.restype = d
     lib.oracle_push_one.argtypes = [P(_abi.DomainDesc), P(_abi.PhysicsDesc), P(_abi.SpeciesDesc),
@@ -204,18 +207,20 @@ def species_array(species):
 
 
 class GasMixture:
-    """oracle_gas_mixture: components [(density_m3, mass_kg, participates), ...]
-    and sigma[n_species, n_components] (m^2)."""
+    """oracle_gas_mixture: components [(density_m3, mass_kg, participates), ...],
+    sigma[n_species, n_components] (m^2) and optional contact radii (m)."""
 
-    def __init__(self, components, sigma):
+    def __init__(self, components, sigma, radii=None):
         self.comps = (_abi.GasComponent * max(len(components), 1))()
         for i, (n, m, part) in enumerate(components):
             self.comps[i] = _abi.GasComponent(n, m, int(part), 0)
         self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
         n_species = self.sigma.shape[0] if len(components) else 0
+        self.radii = None if radii is None else np.ascontiguousarray(radii, dtype=np.float64)
         self.c = _abi.OracleGasMixture(len(components), n_species,
                                        C.cast(self.comps, C.POINTER(_abi.GasComponent)),
-                                       _dp(self.sigma))
+                                       _dp(self.sigma),
+                                       _dp(self.radii) if self.radii is not None else C.POINTER(C.c_double)())
 
 
 class EhssOffline:
@@ -283,6 +288,16 @@ def mt_ehss_oriented(seed, vi, vn, mi, mn, rion, atoms, quat, sigma):
     load().oracle_mt_ehss_oriented(seed, vec(*vi), vec(*vn), mi, mn, rion, _dp(a), a.shape[0],
                                    vec(*quat), sigma, out)
     return list(out)
+
+
+def mt_ehss_mixture(seed, domain, mixture, species, atoms, mass, vel, dt):
+    """One EHSS collision stage over a gas mixture on PhysicsRng(seed):
+    returns (collided, velocity after)."""
+    a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
+    v = vec(*vel)
+    hit = load().oracle_mt_ehss_mixture(seed, C.byref(domain), C.byref(mixture.c), species, _dp(a),
+                                        a.shape[0], mass, v, dt)
+    return bool(hit), np.array(v[:])
 
 
 def quaternion_to_rotation(quat):

@@ -1068,6 +1068,9 @@ bool collide(const icarion_domain_desc& d, const icarion_physics_desc& ph,
     const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
     if (model == ICARION_COLLISION_HSS && tb.mix && tb.mix->n_components > 0)
         return collide_mixture(d, *tb.mix, tb.mix_row, m_ion, vel, dt, rng) >= 0;
+    if (model == ICARION_COLLISION_EHSS && tb.mix && tb.mix->n_components > 0)
+        return ehss_mixture_collide(d, *tb.mix, tb.mix_row, atoms + 4 * static_cast<size_t>(sp.atom_offset),
+                                    sp.atom_count, m_ion, vel, dt, rng, scratch);
     if (model == ICARION_COLLISION_EHSS && tb.off_n > 0)
         return ehss_offline_collide(d, tb.off_n, tb.off_stride, tb.off_sigma, tb.off_mu, m_ion, vel,
                                     dt, rng);
@@ -1108,6 +1111,54 @@ bool collide(const icarion_domain_desc& d, const icarion_physics_desc& ph,
         return false;
     }
     return false;
+}
+
+// EHSSCollisionHandler::handle_collision mixture path (EHSSCollisionHandler.cpp:
+// 311-424) with the online geometry method: every component in order, a
+// neutral for each participating one, k_i = n_i sigma_i |v - v_i| where all
+// are > 0, one test on k_total, the partner by weight (the last component
+// when the running sum falls short), then apply_collision with that neutral
+// and no second test (EHSSCollisionHandler.cpp:219-250, 294-296).
+template <class Rng>
+bool ehss_mixture_collide(const icarion_domain_desc& d, const oracle_gas_mixture& mix, const double* sig_row,
+                          const double* atoms, uint32_t nat, double m_ion, V3& vel, double dt, Rng& rng,
+                          std::vector<V3>& scratch) {
+    const uint32_t nc = mix.n_components;
+    const V3 v_gas{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    std::vector<double> k(nc, 0.0);
+    std::vector<V3> vn(nc, V3{0.0, 0.0, 0.0});
+    double k_total = 0.0;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const icarion_gas_component& comp = mix.components[c];
+        if (!comp.participates) continue;
+        vn[c] = sample_neutral_velocity(d.temperature_K, comp.mass_kg, v_gas, rng);
+        const double sigma_i = sig_row[c];
+        const double n_i = comp.density_m3;
+        const double v_rel_i = norm(vel - vn[c]);
+        if (sigma_i <= 0.0 || n_i <= 0.0 || v_rel_i < kMinRelativeVelocity) continue;
+        k[c] = n_i * sigma_i * v_rel_i;
+        k_total += k[c];
+    }
+    if (k_total <= 0.0) return false;
+    double P_total = 1.0;
+    if (k_total * dt <= 50.0) P_total = 1.0 - std::exp(-k_total * dt);
+    if (rng.uniform01() >= P_total) return false;
+    const double r = rng.uniform01() * k_total;
+    uint32_t idx = 0;
+    double cum = 0.0;
+    for (; idx < nc; ++idx) {
+        cum += k[idx];
+        if (r < cum) break;
+    }
+    if (idx >= nc) idx = nc - 1;
+    const icarion_gas_component& comp = mix.components[idx];
+    const double sigma_eff = sig_row[idx];
+    if (sigma_eff <= 0.0) return false;
+    const double radius = (mix.radius_m && mix.radius_m[idx] > 0.0) ? mix.radius_m[idx] : d.gas_radius_m;
+    const V3 v_rel = vel - vn[idx];
+    if (norm(v_rel) < kMinRelativeVelocity) return false;
+    vel = ehss_collision(vel, vn[idx], m_ion, comp.mass_kg, radius, atoms, nat, rng, scratch);
+    return true;
 }
 
 // Per-ion engine step (SimulationEngine::process_timestep,
@@ -1464,6 +1515,18 @@ int oracle_mt_ehss_offline(uint64_t seed, const icarion_domain_desc* domain, uin
     if (vn_out) {
         vn_out[0] = vn.x; vn_out[1] = vn.y; vn_out[2] = vn.z;
     }
+    return hit ? 1 : 0;
+}
+
+int oracle_mt_ehss_mixture(uint64_t seed, const icarion_domain_desc* domain, const oracle_gas_mixture* mix,
+                           uint32_t species, const double* atoms_xyzr, uint32_t n_atoms, double mass_kg,
+                           double vel[3], double dt) {
+    MtRng rng(seed);
+    std::vector<V3> scratch;
+    V3 v{vel[0], vel[1], vel[2]};
+    const double* row = mix->sigma_m2 + static_cast<size_t>(species) * mix->n_components;
+    const bool hit = ehss_mixture_collide(*domain, *mix, row, atoms_xyzr, n_atoms, mass_kg, v, dt, rng, scratch);
+    vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
     return hit ? 1 : 0;
 }
 

@@ -80,6 +80,7 @@ typedef struct oracle_gas_mixture {
     uint32_t n_species;
     const icarion_gas_component *components;
     const double *sigma_m2;
+    const double *radius_m;   /* per component, EHSS contact radius (NULL / 0: the domain's) */
 } oracle_gas_mixture;
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30)
@@ -121,6 +122,13 @@ void oracle_mt_ehss_oriented(uint64_t seed, const double v_ion[3], const double 
                              double neutral_mass, double ion_radius, const double *atoms_xyzr, uint32_t n_atoms,
                              const double quat_wxyz[4], double sigma_eff, double out[3]);
 void oracle_quaternion_to_rotation(const double quat_wxyz[4], double R[9]);
+
+/* One EHSS collision stage over a gas mixture (EHSSCollisionHandler.cpp:311-424,
+ * geometry method) for an ion of the given species, on one MT19937-64 stream;
+ * vel in/out, returns 1 on a collision. */
+int oracle_mt_ehss_mixture(uint64_t seed, const icarion_domain_desc *domain, const oracle_gas_mixture *mix,
+                           uint32_t species, const double *atoms_xyzr, uint32_t n_atoms, double mass_kg,
+                           double vel[3], double dt);
 
 enum { ORACLE_RNG_PHILOX = 0, ORACLE_RNG_MT19937_64 = 1 };
 

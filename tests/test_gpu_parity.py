@@ -39,6 +39,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
             h.upload_field_terms(w["axes"], w["terms"])
         if w.get("mixture") is not None:
             h.set_gas_mixture(*w["mixture"])
+            if w.get("mixture_radii") is not None:
+                h.set_gas_radii(w["mixture_radii"])
         if w.get("ehss_offline") is not None:
             h.set_ehss_offline(w["ehss_offline"])
         if w.get("ehss_orient") is not None:
@@ -59,7 +61,8 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
         field = oracle.FieldTerms(w["axes"], w["terms"], w.get("waveforms"))
     elif w.get("waveforms") is not None and w.get("field") is None:
         field = oracle.Waveforms(w["waveforms"])
-    mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
+    mixture = (oracle.GasMixture(*w["mixture"], radii=w.get("mixture_radii"))
+               if w.get("mixture") is not None else None)
     off = oracle.EhssOffline(w["ehss_offline"]) if w.get("ehss_offline") is not None else None
     ori = oracle.EhssOrientations(w["ehss_orient"]) if w.get("ehss_orient") is not None else None
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
@@ -267,6 +270,59 @@ def test_ehss_orientation_samples_fp64_ion_by_ion(oracle, integrator):
     gpu_run(w4, b, 600)
     for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions"):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_ehss_mixture_fp64_ion_by_ion(oracle, integrator):
+    """EHSS over a four-component gas mixture: one component does not take
+    part, one takes part at zero density (its neutral is still drawn), two
+    collide with their own cross sections and contact radii; offline sets are
+    not used with a mixture."""
+    w = ehss_workload(n=2048, seed=38)
+    w["physics"] = W.physics(integrator, _abi.COLLISION_EHSS, seed=38)
+    n = w["domain"].gas_density_m3
+    comps = [(0.5 * n, 28.0134 * W.AMU_TO_KG, 1), (0.2 * n, 39.948 * W.AMU_TO_KG, 0),
+             (0.0, 44.01 * W.AMU_TO_KG, 1), (0.5 * n, 4.0026 * W.AMU_TO_KG, 1)]
+    sig = np.array([[60e-20, 50e-20, 70e-20, 40e-20], [130e-20, 110e-20, 150e-20, 90e-20],
+                    [260e-20, 240e-20, 300e-20, 200e-20]])
+    w["mixture"] = (comps, sig)
+    w["mixture_radii"] = [1.8e-10, 1.7e-10, 2.0e-10, 0.0]
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, 1000)
+    so = oracle_run(oracle, w, o, 1000)
+    assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    # offline sets are ignored with a mixture: bit for bit the same run
+    w2 = dict(w)
+    rng = np.random.default_rng(38)
+    w2["ehss_offline"] = [(rng.uniform(30e-20, 60e-20, 16), np.sqrt(rng.uniform(0.0, 1.0, (16, 50)))),
+                          None, None]
+    a = w["ensemble"].copy()
+    gpu_run(w2, a, 1000)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(g, f), err_msg=f)
+
+
+def test_ehss_mixture_controls():
+    from icarion_amd import gpu
+    w = ehss_workload(n=64, seed=39)
+    n = w["domain"].gas_density_m3
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"], w["atoms"])
+        h.set_gas_mixture([(n, 28.0134 * W.AMU_TO_KG, 1)], np.full((3, 1), 100e-20))
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_gas_radii([1e-10, 1e-10])
+        assert ei.value.code == _abi.ERR_INVALID
+        h.set_gas_radii([1.8e-10])
+        h.set_ehss_orientations([(np.array([[1.0, 0, 0, 0]]), np.array([1e-18])), None, None])
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, w["dt"], 5)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        h.set_ehss_orientations([])
+        st = h.step(0.0, w["dt"], 5)
+        assert st.ion_steps > 0
 
 
 def test_ehss_offline_controls():

@@ -51,6 +51,8 @@ def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
             h.upload_field_terms(w["axes"], w["terms"])
         if w.get("mixture") is not None:
             h.set_gas_mixture(*w["mixture"])
+            if w.get("mixture_radii") is not None:
+                h.set_gas_radii(w["mixture_radii"])
         h.upload_state(ens)
         st = h.step(t0, dt, n_steps, step_offset)
         h.download_state(ens)
@@ -62,7 +64,8 @@ def oracle_rk45(oracle, w, ph, ens, n_steps, dt, t0=0.0, cfg=None):
     field = oracle.Field(*w["field"]) if w.get("field") is not None else None
     if w.get("terms") is not None:
         field = oracle.FieldTerms(w["axes"], w["terms"])
-    mixture = oracle.GasMixture(*w["mixture"]) if w.get("mixture") is not None else None
+    mixture = (oracle.GasMixture(*w["mixture"], radii=w.get("mixture_radii"))
+               if w.get("mixture") is not None else None)
     st, _ = oracle.run_rk45(w["domain"], ph, cfg, w["species"], ens, t0, dt, n_steps,
                             atoms=w.get("atoms"), field=field, mixture=mixture)
     return st
