@@ -98,6 +98,10 @@ SnapshotFn = C.CFUNCTYPE(C.c_int, C.c_void_p, _u64, _u64, _d, C.c_size_t, C.POIN
                          C.POINTER(C.c_uint8))
 
 
+class ArrivalStats(C.Structure):
+    _fields_ = [("count", _u64), ("min_s", _d), ("max_s", _d), ("mean_s", _d), ("std_s", _d)]
+
+
 class Rk45Desc(C.Structure):
     _fields_ = [
         ("atol", _d), ("rtol", _d), ("safety_factor", _d), ("min_step_factor", _d),

@@ -12,8 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libicarion_gpu.so")
-SOURCES = ["step_kernels.hip", "rk45_kernels.hip", "multi_kernels.hip", "ehss_tools.hip", "surrogate.hip",
-           "icarion_gpu_api.cpp"]
+SOURCES = ["step_kernels.hip", "rk45_kernels.hip", "multi_kernels.hip", "metrics_kernels.hip", "ehss_tools.hip",
+           "surrogate.hip", "icarion_gpu_api.cpp"]
 ARCH = os.environ.get("ICARION_OFFLOAD_ARCH", "gfx950")
 
 

@@ -38,6 +38,10 @@ hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* 
                                 int32_t* dom, uint8_t* flags, uint8_t* prev, void* rec, uint32_t* count,
                                 uint32_t* host_count, uint32_t* next_count, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, hipStream_t s);
+uint32_t arrival_blocks(uint32_t n);
+hipError_t launch_arrival_partials(const double* arrival, uint32_t n, void* partials, hipStream_t s);
+hipError_t launch_arrival_histogram(const double* arrival, uint32_t n, double lo, double hi, uint32_t n_bins,
+                                    unsigned long long* bins, hipStream_t s);
 hipError_t launch_multi_step(int integrator, int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
                              hipStream_t s);
 hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
@@ -2008,6 +2012,63 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         std::sort(h->dom_spans.begin(), h->dom_spans.end(),
                   [](const DomainSpan& a, const DomainSpan& b) { return a.z_min < b.z_min; });
         h->n_dom = n_domains;
+    });
+}
+
+int icarion_gpu_arrival_stats(icarion_gpu* h, icarion_arrival_stats* stats) {
+    return guarded([&] {
+        if (!h || !stats) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_state) fail(ICARION_ERR_STATE, "no resident state");
+        set_device(h);
+        // per workgroup {sum, sum of squares, min, max, count}, summed here in order
+        struct Partial {
+            double sum, sumsq, min, max;
+            unsigned long long count, pad;
+        };
+        const uint32_t n = static_cast<uint32_t>(h->n);
+        const uint32_t nb = arrival_blocks(n);
+        DevBuf<Partial> part;
+        std::vector<Partial> hp(nb);
+        if (nb) {
+            part.resize(nb);
+            hip_check(launch_arrival_partials(h->arrival.p, n, part.p, h->stream), "arrival statistics launch");
+            copy_out(hp.data(), part, nb, h->stream, "read arrival statistics");
+            hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+        }
+        double sum = 0.0, sq = 0.0, mn = INFINITY, mx = -INFINITY;
+        uint64_t cnt = 0;
+        for (const Partial& p : hp) {
+            sum += p.sum;
+            sq += p.sumsq;
+            mn = std::min(mn, p.min);
+            mx = std::max(mx, p.max);
+            cnt += p.count;
+        }
+        stats->count = cnt;
+        stats->min_s = cnt ? mn : NAN;
+        stats->max_s = cnt ? mx : NAN;
+        stats->mean_s = cnt ? sum / static_cast<double>(cnt) : NAN;
+        const double var = cnt ? sq / static_cast<double>(cnt) - stats->mean_s * stats->mean_s : NAN;
+        stats->std_s = cnt ? std::sqrt(std::max(0.0, var)) : NAN;
+    });
+}
+
+int icarion_gpu_arrival_histogram(icarion_gpu* h, double lo_s, double hi_s, uint32_t n_bins, uint64_t* counts) {
+    return guarded([&] {
+        if (!h || !counts) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_state) fail(ICARION_ERR_STATE, "no resident state");
+        if (n_bins == 0 || n_bins > ICARION_MAX_HIST_BINS)
+            fail(ICARION_ERR_INVALID, "n_bins must be in 1 .. ICARION_MAX_HIST_BINS");
+        if (!(lo_s < hi_s) || !std::isfinite(lo_s) || !std::isfinite(hi_s))
+            fail(ICARION_ERR_INVALID, "need finite lo_s < hi_s");
+        set_device(h);
+        DevBuf<unsigned long long> bins;
+        bins.resize(n_bins);
+        hip_check(launch_arrival_histogram(h->arrival.p, static_cast<uint32_t>(h->n), lo_s, hi_s, n_bins, bins.p,
+                                           h->stream),
+                  "arrival histogram launch");
+        copy_out(reinterpret_cast<unsigned long long*>(counts), bins, n_bins, h->stream, "read arrival histogram");
+        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     });
 }
 

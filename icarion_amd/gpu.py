@@ -25,7 +25,8 @@ EXPORTED = (
     "icarion_gpu_batch_autotune", "icarion_gpu_batch_download", "icarion_gpu_upload_field_terms",
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
-    "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii",
+    "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
+    "icarion_gpu_arrival_histogram",
 )
 
 
@@ -90,6 +91,9 @@ def load(path=None):
     lib.icarion_gpu_set_ehss_orientations.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_double),
                                                       P(C.c_double), C.c_uint32]
     lib.icarion_gpu_set_gas_radii.argtypes = [C.c_void_p, P(C.c_double), C.c_uint32]
+    lib.icarion_gpu_arrival_stats.argtypes = [C.c_void_p, P(_abi.ArrivalStats)]
+    lib.icarion_gpu_arrival_histogram.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint32,
+                                                  P(C.c_uint64)]
     lib.icarion_gpu_synchronize.argtypes = [C.c_void_p]
     lib.icarion_gpu_step_batch.argtypes = [C.c_void_p, P(_abi.IonSoA), C.c_double, C.c_double,
                                            C.c_uint64, P(_abi.StepStats)]
@@ -372,6 +376,20 @@ class Handle:
         _check(self.lib.icarion_gpu_set_ehss_offline(
             self.h, n_orient.ctypes.data_as(u32), stride.ctypes.data_as(u32),
             sg.ctypes.data_as(d), mu.ctypes.data_as(d), n), self.lib)
+
+    def arrival_stats(self):
+        """Count, min, max, mean and std of the resident ions' arrival times."""
+        st = _abi.ArrivalStats()
+        _check(self.lib.icarion_gpu_arrival_stats(self.h, C.byref(st)), self.lib)
+        return st
+
+    def arrival_histogram(self, lo, hi, n_bins):
+        """Arrival-time counts in n_bins equal bins over [lo, hi] (numpy's rule)."""
+        out = np.zeros(n_bins, dtype=np.uint64)
+        _check(self.lib.icarion_gpu_arrival_histogram(self.h, lo, hi, n_bins,
+                                                      out.ctypes.data_as(C.POINTER(C.c_uint64))),
+               self.lib)
+        return out
 
     def set_gas_radii(self, radii):
         """Contact radius per gas component of the mixture (0: the domain's)."""

@@ -475,6 +475,29 @@ int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
 
+/* Arrival-time statistics of the resident ions: the finite in-kernel
+ * arrival times (first crossing of the detector plane, linearly interpolated
+ * in the step; NaN = none). */
+typedef struct icarion_arrival_stats {
+    uint64_t count;
+    double min_s, max_s;
+    double mean_s, std_s;         /* population std, as numpy's values.std()  */
+} icarion_arrival_stats;
+
+#define ICARION_MAX_HIST_BINS 8192
+
+/* The arrival statistics (analysis/ims_mobility.py and arrival_time_distribution.py
+ * take the mean and std of the arrival times), reduced on the device per
+ * workgroup and summed on the host in a fixed order. */
+int icarion_gpu_arrival_stats(icarion_gpu *h, icarion_arrival_stats *stats);
+
+/* Counts of the arrival times in n_bins equal bins over [lo_s, hi_s] with
+ * numpy's bin rule (np.histogram over the edges linspace(lo, hi, n_bins + 1):
+ * each bin [e_i, e_i+1), the last one closed), as the reference's analysis
+ * bins them with edges linspace(min, max, bins + 1) (analysis/physics.py:96-104,
+ * 119-145).  1 <= n_bins <= ICARION_MAX_HIST_BINS, lo_s < hi_s. */
+int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint32_t n_bins, uint64_t *counts);
+
 #define ICARION_MAX_DOMAINS 8
 
 /* Several domains on one handle: a multi-domain instrument as DomainManager
