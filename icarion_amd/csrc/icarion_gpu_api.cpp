@@ -846,6 +846,10 @@ constexpr size_t kSnapRecordBytes = 56;   // per changed ion: 6 doubles, index, 
 
 void ensure_snap_ring(icarion_gpu* h) {
     auto& R = h->snap;
+    // a call that ended in an error may have left a pack or a copy queued on
+    // the ring's buffers: let them finish before the buffers are reused or freed
+    hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    hip_check(hipStreamSynchronize(h->side[0]), "hipStreamSynchronize");
     if (!R.copied) hip_check(hipEventCreateWithFlags(&R.copied, hipEventDisableTiming), "hipEventCreate");
     for (int j = 0; j < kSnapSlots; ++j)
         if (!R.packed[j]) hip_check(hipEventCreateWithFlags(&R.packed[j], hipEventDisableTiming), "hipEventCreate");
