@@ -858,7 +858,8 @@ def bench_domains(args):
     """The three-domain instrument (IMS region -> RF quadrupole -> IMS region,
     HSS in each domain's gas, RK4 dt 1 ns) with 1M ions in FP64: ion-steps/s of
     the multi-domain kernel (a domain lookup per ion and step, the domain's
-    field and gas per lane) and the oracle's multi-domain loop on the host."""
+    field and gas per lane) and the oracle's multi-domain loop on the host;
+    the FP32 + Kahan kernel on the same run is reported beside it."""
     from icarion_amd import _abi, gpu
     from icarion_amd import workloads as W
     steps = args.steps if args.steps is not None else 10_000
@@ -884,6 +885,18 @@ def bench_domains(args):
         "alive_by_domain": np.bincount(dom[(ens.active == 1) & (ens.born == 1)], minlength=3).tolist(),
         "avg_launch_ms": st.device_ms / launches,
     }
+    # the same run with FP32 + Kahan numerics (domain decisions in FP64)
+    ph32 = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, precision=_abi.PRECISION_FP32_KAHAN,
+                     seed=w["physics"].rng_seed)
+    e32 = w["ensemble"].copy()
+    with gpu.Handle(device=0, steps_per_launch=args.steps_per_launch) as h:
+        h.set_domains(w["domains"], ph32, w["apertures"])
+        h.set_species(w["species"])
+        st32, wall32 = timed_steps(h, e32, w["dt"], steps, args.warmup)
+    res["fp32_kahan"] = {"value": st32.ion_steps / wall32, "unit": "ion-steps/s",
+                         "ms_per_step": wall32 * 1e3 / steps, "alive_fraction": st32.alive / args.ions,
+                         "avg_launch_ms": st32.device_ms / launches}
+    log(f"domains: FP64 {value:.3e}, FP32 + Kahan {st32.ion_steps / wall32:.3e} ion-steps/s")
     if not args.no_cpu:
         from oracle import binding
         threads = len(os.sched_getaffinity(0))

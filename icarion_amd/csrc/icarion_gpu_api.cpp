@@ -42,8 +42,8 @@ uint32_t arrival_blocks(uint32_t n);
 hipError_t launch_arrival_partials(const double* arrival, uint32_t n, void* partials, hipStream_t s);
 hipError_t launch_arrival_histogram(const double* arrival, uint32_t n, double lo, double hi, uint32_t n_bins,
                                     unsigned long long* bins, hipStream_t s);
-hipError_t launch_multi_step(int integrator, int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
-                             hipStream_t s);
+hipError_t launch_multi_step(int integrator, int coll, int precision, const MultiParams& M, const IonPtrs& S,
+                             uint32_t block, hipStream_t s);
 hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint32_t* grid,
                         const uint4* blob, uint32_t n_clouds, unsigned long long* counters, hipStream_t s,
                         const hipStream_t* side, hipEvent_t fork, const hipEvent_t* join);
@@ -1172,7 +1172,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
     if (!std::isfinite(t0)) fail(ICARION_ERR_INVALID, "t0 must be finite");
     if (h->phys.integrator != ICARION_INTEGRATOR_RK4 && h->phys.integrator != ICARION_INTEGRATOR_BORIS)
         fail(ICARION_ERR_UNSUPPORTED, "several domains run with RK4 or Boris");
-    if (h->phys.precision != ICARION_PRECISION_FP64) fail(ICARION_ERR_UNSUPPORTED, "several domains run in FP64");
+
     const int coll = coll_kind(h->phys);
     if (coll == COLL_EHSS) fail(ICARION_ERR_UNSUPPORTED, "several domains: EHSS is not covered");
     if (!h->mix.empty()) fail(ICARION_ERR_UNSUPPORTED, "several domains: gas mixtures are not covered");
@@ -1198,6 +1198,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             P.dt = dt;
             P.species = h->species.p;
             P.n_ions = static_cast<uint32_t>(h->n);
+            fill_fast_const(P);
             append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps);
         }
         t_end = tabs[n_steps - 1].t + dt;
@@ -1239,7 +1240,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             M.k0 = static_cast<uint32_t>(k0);
             M.step0 = step_offset + k0;
             M.n_steps = static_cast<uint32_t>(nk);
-            const hipError_t e = launch_multi_step(integ, coll, M, S, h->block, h->stream);
+            const hipError_t e = launch_multi_step(integ, coll, h->phys.precision, M, S, h->block, h->stream);
             if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no multi-domain kernel for this case");
             hip_check(e, "multi-domain step launch");
         });

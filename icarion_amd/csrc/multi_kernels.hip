@@ -1,4 +1,5 @@
-// multi_kernels.hip - the fused ion step over several domains (FP64).
+// multi_kernels.hip - the fused ion step over several domains (FP64, and FP32
+// + Kahan with the domain logic kept in FP64).
 //
 // A multi-domain instrument: every domain keeps its own geometry, analytic
 // field and gas, read per lane from a parameter record in global memory (a
@@ -18,6 +19,7 @@
 //             (DomainManager.cpp:176-198) may move the index on; NaN kill.
 // An ion kept alive by the bridge is looked up again at the next step start.
 #include "step_device.hpp"
+#include "step_device_f32.hpp"
 
 namespace icarion {
 
@@ -218,6 +220,181 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
     S.domain[i] = cur;
 }
 
+template <int COLL>
+__device__ __forceinline__ f3 domain_accel_f32(const StepParams& D, f3 p, f3 v, double c, double ez, float qm,
+                                               float lpre) {
+    if (D.instrument == 3) return accel_f32<FIELD_ORBITRAP, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
+    return accel_f32<FIELD_LINEAR, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
+}
+
+// The FP32 + Kahan body of the single-domain fast path over several domains:
+// position and velocity as FP32 (hi, lo) pairs, forces and collisions in FP32
+// with each domain's FP32 constants; the domain lookup, bridge and handoff
+// in FP64 on hi + lo, as the FP64 kernel takes them.
+template <int INTEG, int COLL>
+__device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtrs& S, const uint32_t i,
+                                              uint32_t& pushed, uint32_t& ncoll_launch) {
+    const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
+    float hi[6], lo[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        hi[c] = (float)X[c];
+        lo[c] = (float)(X[c] - (double)hi[c]);
+    }
+    auto pos64 = [&]() {
+        return d3{(double)hi[0] + (double)lo[0], (double)hi[1] + (double)lo[1], (double)hi[2] + (double)lo[2]};
+    };
+    const double md = S.mass[i];
+    const double qd = S.charge[i];
+    const uint32_t sidx = S.species[i];
+    const SpeciesDev sp = M.species[sidx];
+    const float m = (float)md;
+    const float qm = (float)(qd / md);
+    uint8_t flags = S.flags[i];
+    bool active = flags & 1, born = (flags >> 1) & 1;
+    const double birth = S.birth ? S.birth[i] : -INFINITY;
+    double death = S.death[i];
+    double arrival = S.arrival[i];
+    uint32_t nc = S.ncoll[i];
+    int cur = S.domain[i];
+    // the ion's constants in its current domain's gas
+    int const_dom = -1;
+    float lpre = 0.0f, inv_m_plus = 0.0f, kbt2_dt_over_m = 0.0f;
+    int found = -1;
+    bool checked = false;
+    const float dt = (float)M.dt;
+    const float half_dt = 0.5f * dt;
+    const float h6 = dt * (1.0f / 6.0f);
+    for (uint32_t k = 0; k < M.n_steps; ++k) {
+        const double t = M.t_table[M.k0 + k].t;
+        if (!born) {
+            if (t >= birth) {
+                born = true;
+                active = true;
+                checked = false;
+            } else {
+                continue;
+            }
+        }
+        if (!active) break;
+        const int dom = checked ? found : find_domain(M, pos64());
+        if (dom < 0) {
+            active = false;
+            death = t;
+            break;
+        }
+        cur = dom;
+        const StepParams& D = M.dom[dom];
+        if (dom != const_dom) {
+            if constexpr (COLL == COLL_DAMPING) lpre = (float)damping_pre_f64(D, sp, md, qd);
+            inv_m_plus = (float)(1.0 / (md + D.m_gas));
+            kbt2_dt_over_m = (float)(2.0 * kBoltzmann * D.T / md * D.dt);
+            const_dom = dom;
+        }
+        const StepEntry te = D.table[M.k0 + k];
+        const float z_before = hi[2];
+        const float z_before_lo = lo[2];
+        const f3 p0{hi[0], hi[1], hi[2]};
+        const f3 v0{hi[3], hi[4], hi[5]};
+        f3 dpos, dvel;
+        if constexpr (INTEG == 0) {
+            const f3 k1a = domain_accel_f32<COLL>(D, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 k2v = v0 + k1a * half_dt;
+            const f3 k2a = domain_accel_f32<COLL>(D, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lpre);
+            const f3 k3v = v0 + k2a * half_dt;
+            const f3 k3a = domain_accel_f32<COLL>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lpre);
+            const f3 k4v = v0 + k3a * dt;
+            const f3 k4a = domain_accel_f32<COLL>(D, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lpre);
+            dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
+            dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
+        } else {
+            const f3 a = domain_accel_f32<COLL>(D, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 vm = v0 + a * half_dt;
+            f3 rot{0.0f, 0.0f, 0.0f};
+            if (D.magnetic) {
+                const f3 tv = f3{(float)D.B[0], (float)D.B[1], (float)D.B[2]} * (qm * half_dt);
+                const float s_f = 2.0f / (1.0f + dot(tv, tv));
+                const f3 vprime = vm + cross(vm, tv);
+                rot = cross(vprime, tv * s_f);
+            }
+            dpos = (vm + rot) * dt;
+            dvel = a * dt + rot;
+        }
+        kahan_add(hi[0], lo[0], dpos.x);
+        kahan_add(hi[1], lo[1], dpos.y);
+        kahan_add(hi[2], lo[2], dpos.z);
+        kahan_add(hi[3], lo[3], dvel.x);
+        kahan_add(hi[4], lo[4], dvel.y);
+        kahan_add(hi[5], lo[5], dvel.z);
+        ++pushed;
+        if constexpr (COLL != COLL_NONE) {
+            PhiloxStream rng(M.seed, i, M.step0 + k);
+            f3 v{hi[3], hi[4], hi[5]};
+            const f3 vin = v;
+            const bool hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng, dt,
+                                               D.fc.n_dt);
+            if (COLL == COLL_HSS) {
+                if (hit) {
+                    hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
+                    lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
+                    ++nc;
+                    ++ncoll_launch;
+                }
+            } else {
+                kahan_add(hi[3], lo[3], v.x - vin.x);
+                kahan_add(hi[4], lo[4], v.y - vin.y);
+                kahan_add(hi[5], lo[5], v.z - vin.z);
+            }
+        }
+        if (D.detect && isnan(arrival)) {
+            const double zb = (double)z_before + (double)z_before_lo;
+            const double za = (double)hi[2] + (double)lo[2];
+            if (zb < D.det_z && za >= D.det_z) arrival = t + M.dt * (D.det_z - zb) / (za - zb);
+        }
+        const d3 p_after = pos64();
+        found = find_domain(M, p_after);
+        if (found < 0) {
+            const int b = bridge_domain(M, cur, p_after);
+            if (b < 0) {
+                active = false;
+                death = t;
+                break;
+            }
+            cur = b;
+            checked = false;
+        } else {
+            const int hnd = handoff_domain(M, cur, found, p_after);
+            cur = hnd >= 0 ? hnd : found;
+            checked = true;
+        }
+        if (!(isfinite(hi[0]) && isfinite(hi[1]) && isfinite(hi[2]) && isfinite(hi[3]) && isfinite(hi[4]) &&
+              isfinite(hi[5]))) {
+            active = false;
+            death = t;
+            break;
+        }
+    }
+    S.x[i] = (double)hi[0] + (double)lo[0];
+    S.y[i] = (double)hi[1] + (double)lo[1];
+    S.z[i] = (double)hi[2] + (double)lo[2];
+    S.vx[i] = (double)hi[3] + (double)lo[3];
+    S.vy[i] = (double)hi[4] + (double)lo[4];
+    S.vz[i] = (double)hi[5] + (double)lo[5];
+    S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+    S.death[i] = death;
+    S.arrival[i] = arrival;
+    S.ncoll[i] = nc;
+    S.domain[i] = cur;
+}
+
+template <int INTEG, int COLL>
+__global__ void __launch_bounds__(256) step_multi_f32_kernel(const MultiParams M, const IonPtrs S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t pushed = 0, ncoll_launch = 0;
+    if (i < M.n_ions) multi_ion_f32<INTEG, COLL>(M, S, i, pushed, ncoll_launch);
+    add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
+}
+
 template <int INTEG, int COLL>
 __global__ void __launch_bounds__(256) step_multi_f64_kernel(const MultiParams M, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -226,19 +403,22 @@ __global__ void __launch_bounds__(256) step_multi_f64_kernel(const MultiParams M
     add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
 }
 
-template <int INTEG>
+template <int INTEG, bool F32>
 static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
                                      hipStream_t s) {
     const dim3 grid((M.n_ions + block - 1) / block);
     switch (coll) {
         case COLL_NONE:
-            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_NONE>), grid, dim3(block), 0, s, M, S);
+            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_NONE>), grid, dim3(block), 0, s, M, S);
+            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_NONE>), grid, dim3(block), 0, s, M, S);
             break;
         case COLL_HSS:
-            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS>), grid, dim3(block), 0, s, M, S);
+            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_HSS>), grid, dim3(block), 0, s, M, S);
+            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS>), grid, dim3(block), 0, s, M, S);
             break;
         case COLL_DAMPING:
-            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
+            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
+            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
             break;
         default:
             return hipErrorInvalidValue;
@@ -246,13 +426,16 @@ static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPt
     return hipGetLastError();
 }
 
-// integrator 0 RK4, 1 Boris; hipErrorInvalidValue for a collision kind with no
-// multi-domain kernel.
-hipError_t launch_multi_step(int integrator, int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
-                             hipStream_t s) {
+// integrator 0 RK4, 1 Boris; precision 0 FP64, 1 FP32 + Kahan;
+// hipErrorInvalidValue for a collision kind with no multi-domain kernel.
+hipError_t launch_multi_step(int integrator, int coll, int precision, const MultiParams& M, const IonPtrs& S,
+                             uint32_t block, hipStream_t s) {
     if (M.n_ions == 0) return hipSuccess;
-    if (integrator == 0) return launch_multi_integ<0>(coll, M, S, block, s);
-    if (integrator == 1) return launch_multi_integ<1>(coll, M, S, block, s);
+    const bool f32 = precision == 1;
+    if (integrator == 0) return f32 ? launch_multi_integ<0, true>(coll, M, S, block, s)
+                                    : launch_multi_integ<0, false>(coll, M, S, block, s);
+    if (integrator == 1) return f32 ? launch_multi_integ<1, true>(coll, M, S, block, s)
+                                    : launch_multi_integ<1, false>(coll, M, S, block, s);
     return hipErrorInvalidValue;
 }
 

@@ -514,7 +514,9 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
  *     (0 or NULL: the radius).
  * Replaces any domain set before; n_domains = 1 is icarion_gpu_set_domain.
  * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
- * voltages, RK4 or Boris, FP64, no / HSS / Langevin / Friction collisions, one
+ * voltages, RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
+ * handoff in FP64 on the compensated position), no / HSS / Langevin /
+ * Friction collisions, one
  * species table for all domains; icarion_gpu_step, icarion_gpu_step_batch and
  * icarion_gpu_step_snapshots (with each ion's domain index).  Other
  * combinations return ICARION_ERR_UNSUPPORTED when stepped. */

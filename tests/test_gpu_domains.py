@@ -9,9 +9,12 @@ Identical Philox streams on both sides, so ions are compared one by one:
     collision count within 0.3 %.
 The hand-derived bridge and handoff cases of the oracle's own tests
 (tests/test_oracle_domains.py) run on the device and must agree exactly.
+FP32 + Kahan runs are held to the FP64 oracle by path agreement and phase-space
+error (no gas) and by statistics (with gas).
 """
 import numpy as np
 import pytest
+from scipy import stats
 
 from icarion_amd import _abi
 from icarion_amd import workloads as W
@@ -138,7 +141,6 @@ def test_unsupported_combinations():
     w = W.multi_domain_workload(n=64, seed=55)
     p = w["physics"]
     cases = [
-        W.physics(p.integrator, p.collision_model, precision=_abi.PRECISION_FP32_KAHAN),
         W.physics(_abi.INTEGRATOR_RK45, p.collision_model),
         W.physics(p.integrator, _abi.COLLISION_EHSS),
     ]
@@ -154,6 +156,62 @@ def test_unsupported_combinations():
         with pytest.raises(gpu.IcarionError) as ei:
             h.set_domains(w["domains"] * 3, p)
         assert ei.value.code == _abi.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_fp32_kahan_three_domains_collisionless(oracle, integrator):
+    """FP32 + Kahan over three domains against the FP64 oracle, no gas.  The
+    domain lookup, bridge and handoff run in FP64 on the (hi, lo) position, so
+    nearly every ion takes the oracle's path (liveness, death step, domain
+    index); positions of ions alive on both sides agree to 1e-6 of the
+    quadrupole radius at the median and 1e-5 at the 99th percentile (all
+    masses sit at q <= 0.55, far from the stability edge)."""
+    w = W.multi_domain_workload(n=4096, seed=57, integrator=integrator)
+    ph = W.physics(integrator, _abi.COLLISION_NONE, precision=_abi.PRECISION_FP32_KAHAN, seed=57)
+    print(f"domains fp32 integ={integrator}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000, physics=ph)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    same = (g.active == o.active) & (dg == do) & (g.death_time_s == o.death_time_s)
+    assert same.mean() > 0.995, same.mean()
+    assert sg.ion_steps == pytest.approx(so.ion_steps, rel=2e-3)
+    both = g.alive & o.alive
+    assert both.sum() > 200
+    r0 = 2e-3
+    dp = np.sqrt((g.x - o.x) ** 2 + (g.y - o.y) ** 2 + (g.z - o.z) ** 2)[both] / r0
+    q50, q99 = np.quantile(dp, 0.5), np.quantile(dp, 0.99)
+    print(f"pos err/r0 median {q50:.2e} p99 {q99:.2e}")
+    assert q50 < 1e-6 and q99 < 1e-5
+
+
+@pytest.mark.parametrize("integrator,collision", [
+    (_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS),
+    (_abi.INTEGRATOR_BORIS, _abi.COLLISION_LANGEVIN),
+])
+def test_fp32_kahan_three_domains_statistics(oracle, integrator, collision):
+    """FP32 + Kahan with gas over three domains against the FP64 oracle: the
+    surviving fraction in each domain and overall within four binomial standard
+    errors (+0.5 %), the mean collision count within 0.5 %, and the axial
+    distribution of the survivors by a two-sample KS test."""
+    w = W.multi_domain_workload(n=20000, seed=58, integrator=integrator, collision=collision)
+    ph = W.physics(integrator, collision, precision=_abi.PRECISION_FP32_KAHAN, seed=58)
+    if collision == _abi.COLLISION_LANGEVIN:
+        ph.ou_thermal_kicks = 1
+        w["physics"].ou_thermal_kicks = 1
+    print(f"domains fp32 integ={integrator} coll={collision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000, physics=ph)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    n = g.n
+    for d in (None, 0, 1, 2):
+        sel_g = g.alive if d is None else g.alive & (dg == d)
+        sel_o = o.alive if d is None else o.alive & (do == d)
+        fg, fo = sel_g.mean(), sel_o.mean()
+        err = 4 * np.sqrt(max(fo * (1 - fo), 1e-4) / n) + 0.005
+        assert abs(fg - fo) < err, (d, fg, fo)
+    if collision == _abi.COLLISION_HSS:
+        assert g.n_collisions.mean() == pytest.approx(o.n_collisions.mean(), rel=5e-3)
+    assert stats.ks_2samp(g.z[g.alive], o.z[o.alive]).pvalue > 1e-3
 
 
 def test_snapshots_over_several_domains():
