@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -181,6 +182,8 @@ struct icarion_gpu {
     // voltage waveforms of the analytic fields (per slot; none when !has_waves)
     std::unique_ptr<HostWaveform> wave[ICARION_N_VOLTAGE_SLOTS];
     bool has_waves = false;
+    // several domains: each domain's own voltage waveforms (n_dom sets)
+    std::vector<std::array<std::unique_ptr<HostWaveform>, ICARION_N_VOLTAGE_SLOTS>> dom_wave;
     std::vector<icarion_gas_component> mix;
     uint32_t mix_species = 0;
     DevBuf<double> mix_sigma;
@@ -354,6 +357,22 @@ void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p
     }
 }
 
+// The Orbitrap's field constant for a DC radial voltage V
+// (AnalyticalFieldModel.cpp:201-202).
+double orbitrap_k(const icarion_domain_desc& d, double V) {
+    const double r_in = d.radius_in_m, r_out = d.radius_out_m;
+    const double rc_sq = d.radius_char_m * d.radius_char_m;
+    return 2.0 * V / (rc_sq * std::log(r_out / r_in) - 0.5 * (r_out * r_out - r_in * r_in));
+}
+
+// compute_orbitrap_field evaluates the DC radial voltage at t = 0 only
+// (AnalyticalFieldModel.cpp:198): a waveform there fixes the constant at its
+// t = 0 value, and the Orbitrap reads no other voltage slot.
+void apply_orbitrap_waveform(const icarion_gpu* h, StepParams& P) {
+    if (h->dom.instrument != ICARION_INSTRUMENT_ORBITRAP || !h->has_waves) return;
+    if (const HostWaveform* w = h->wave[ICARION_SLOT_DC_RADIAL].get()) P.orbi_k = orbitrap_k(h->dom, w->eval(0.0));
+}
+
 StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_desc& p) {
     StepParams P{};
     P.instrument = d.instrument;
@@ -382,10 +401,7 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
     if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) {
         P.ln_R_in = std::log(d.radius_in_m);
         P.ln_R_out = std::log(d.radius_out_m);
-        const double r_in = d.radius_in_m, r_out = d.radius_out_m;
-        // AnalyticalFieldModel.cpp:201-202
-        P.orbi_k = 2.0 * d.dc_radial_V /
-                   (P.R_char_sq * std::log(r_out / r_in) - 0.5 * (r_out * r_out - r_in * r_in));
+        P.orbi_k = orbitrap_k(d, d.dc_radial_V);
     }
     std::memcpy(P.g2l, d.rot_g2l, sizeof(P.g2l));
     std::memcpy(P.l2g, d.rot_l2g, sizeof(P.l2g));
@@ -700,7 +716,8 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
                   icarion_step_stats* stats) {
     const int field = launch_field_kind(h);
     const int coll = launch_coll_kind(h);
-    if (h->has_waves) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with constant voltages");
+    if (h->has_waves && field != FIELD_ORBITRAP)
+        fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with constant voltages");
     if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -720,6 +737,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     P.counters = h->counters.p;
     P.counter_stripes = kCounterStripes;
     P.rk45 = h->rk45;
+    apply_orbitrap_waveform(h, P);
     fill_fast_const(P);
     const IonPtrs S = ion_ptrs(h);
     h->r_eng.resize(1);
@@ -1058,8 +1076,6 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     if (coll == COLL_EHSS && !surrogate && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS) check_ehss_mixture(h, surrogate);
-    if (h->has_waves && field == FIELD_ORBITRAP)
-        fail(ICARION_ERR_UNSUPPORTED, "the Orbitrap field runs with constant voltages");
     set_device(h);
     StepParams P = h->base;
     if (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
@@ -1074,6 +1090,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     P.n_ions = static_cast<uint32_t>(h->n);
     P.counters = h->counters.p;
     P.counter_stripes = kCounterStripes;
+    apply_orbitrap_waveform(h, P);
     fill_fast_const(P);
     zero_counters(h, h->stream);
     double t_end = t0;
@@ -1176,7 +1193,8 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
     const int coll = coll_kind(h->phys);
     if (coll == COLL_EHSS) fail(ICARION_ERR_UNSUPPORTED, "several domains: EHSS is not covered");
     if (!h->mix.empty()) fail(ICARION_ERR_UNSUPPORTED, "several domains: gas mixtures are not covered");
-    if (h->has_waves) fail(ICARION_ERR_UNSUPPORTED, "several domains run with constant voltages");
+    if (h->has_waves)
+        fail(ICARION_ERR_UNSUPPORTED, "several domains take their waveforms per domain (icarion_gpu_set_domain_waveforms)");
     for (const auto& d : h->doms)
         if (d.use_field_array) fail(ICARION_ERR_UNSUPPORTED, "several domains run with analytic fields");
     if (h->species_used > h->n_species)
@@ -1198,8 +1216,19 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             P.dt = dt;
             P.species = h->species.p;
             P.n_ions = static_cast<uint32_t>(h->n);
+            // the domain's own waveforms, as a single domain's (AnalyticalFieldModel.cpp:130-248)
+            const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
+            bool waves = false;
+            for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+                wf[k] = h->dom_wave[d][k].get();
+                waves = waves || wf[k];
+            }
+            if (wf[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
+            if (h->doms[d].instrument == ICARION_INSTRUMENT_ORBITRAP && wf[ICARION_SLOT_DC_RADIAL])
+                P.orbi_k = orbitrap_k(h->doms[d], wf[ICARION_SLOT_DC_RADIAL]->eval(0.0));
             fill_fast_const(P);
-            append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps);
+            append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps, false,
+                         waves ? wf : nullptr);
         }
         t_end = tabs[n_steps - 1].t + dt;
         h->dom_tables.resize(tabs.size());
@@ -1992,8 +2021,10 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_ap_sq.clear();
         h->dom_spans.clear();
         h->dom_has_orbitrap = false;
+        h->dom_wave.clear();
         if (n_domains == 1) return;
         h->doms.assign(domains, domains + n_domains);
+        h->dom_wave.resize(n_domains);
         // DomainManager's constructor (DomainManager.cpp:26-63): a span per
         // cylindrical domain, z range and radius widened by 1e-12 m, with the
         // index of its domain, sorted by z_min (std::sort, as there)
@@ -2291,42 +2322,60 @@ int icarion_gpu_set_gas_radii(icarion_gpu* h, const double* radius_m, uint32_t n
     });
 }
 
+// Checks and copies the slot descriptors (NULL entries: the constant); true
+// when any slot holds a waveform.
+static bool parse_waveform_slots(const icarion_waveform_desc* const* slots, std::unique_ptr<HostWaveform>* w) {
+    bool any = false;
+    for (int k = 0; slots && k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+        const icarion_waveform_desc* d = slots[k];
+        if (!d) continue;
+        if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
+            fail(ICARION_ERR_INVALID, "unknown waveform type");
+        if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
+            fail(ICARION_ERR_INVALID, "unknown waveform source");
+        for (double v : d->p)
+            if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
+        auto hw = std::make_unique<HostWaveform>();
+        hw->d = *d;
+        if (d->type == ICARION_WAVEFORM_ARBITRARY) {
+            // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
+            if (!d->times_s || !d->values || d->n_points == 0)
+                fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
+            if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
+                fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
+            hw->ts.assign(d->times_s, d->times_s + d->n_points);
+            hw->vs.assign(d->values, d->values + d->n_points);
+            for (uint32_t i = 0; i < d->n_points; ++i)
+                if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
+                    (i && !(hw->ts[i] > hw->ts[i - 1])))
+                    fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
+        }
+        hw->d.times_s = nullptr;
+        hw->d.values = nullptr;
+        w[k] = std::move(hw);
+        any = true;
+    }
+    return any;
+}
+
 int icarion_gpu_set_waveforms(icarion_gpu* h, const icarion_waveform_desc* const* slots) {
     return guarded([&] {
         if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
         std::unique_ptr<HostWaveform> w[ICARION_N_VOLTAGE_SLOTS];
-        bool any = false;
-        for (int k = 0; slots && k < ICARION_N_VOLTAGE_SLOTS; ++k) {
-            const icarion_waveform_desc* d = slots[k];
-            if (!d) continue;
-            if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
-                fail(ICARION_ERR_INVALID, "unknown waveform type");
-            if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
-                fail(ICARION_ERR_INVALID, "unknown waveform source");
-            for (double v : d->p)
-                if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
-            auto hw = std::make_unique<HostWaveform>();
-            hw->d = *d;
-            if (d->type == ICARION_WAVEFORM_ARBITRARY) {
-                // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
-                if (!d->times_s || !d->values || d->n_points == 0)
-                    fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
-                if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
-                    fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
-                hw->ts.assign(d->times_s, d->times_s + d->n_points);
-                hw->vs.assign(d->values, d->values + d->n_points);
-                for (uint32_t i = 0; i < d->n_points; ++i)
-                    if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
-                        (i && !(hw->ts[i] > hw->ts[i - 1])))
-                        fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
-            }
-            hw->d.times_s = nullptr;
-            hw->d.values = nullptr;
-            w[k] = std::move(hw);
-            any = true;
-        }
+        const bool any = parse_waveform_slots(slots, w);
         for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) h->wave[k] = std::move(w[k]);
         h->has_waves = any;
+    });
+}
+
+int icarion_gpu_set_domain_waveforms(icarion_gpu* h, uint32_t domain, const icarion_waveform_desc* const* slots) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (h->n_dom < 2) fail(ICARION_ERR_STATE, "icarion_gpu_set_domains with two or more domains not called");
+        if (domain >= h->n_dom) fail(ICARION_ERR_INVALID, "domain index out of range");
+        std::unique_ptr<HostWaveform> w[ICARION_N_VOLTAGE_SLOTS];
+        parse_waveform_slots(slots, w);
+        for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) h->dom_wave[domain][k] = std::move(w[k]);
     });
 }
 

@@ -26,7 +26,7 @@ EXPORTED = (
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
-    "icarion_gpu_arrival_histogram",
+    "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms",
 )
 
 
@@ -354,6 +354,17 @@ class Handle:
             arr[k] = C.pointer(w)
         self._keep.append(dict(slots))
         _check(self.lib.icarion_gpu_set_waveforms(self.h, arr), self.lib)
+
+    def set_domain_waveforms(self, domain, slots):
+        """One domain's waveforms on a multi-domain handle: slots as in
+        set_waveforms; {} or None clears that domain's."""
+        arr = None
+        if slots:
+            arr = (C.POINTER(_abi.WaveformDesc) * _abi.N_VOLTAGE_SLOTS)()
+            for k, w in slots.items():
+                arr[k] = C.pointer(w)
+            self._keep.append(dict(slots))
+        _check(self.lib.icarion_gpu_set_domain_waveforms(self.h, C.c_uint32(domain), arr), self.lib)
 
     def set_ehss_offline(self, sets):
         """sets[s]: None (species s keeps the online geometry) or

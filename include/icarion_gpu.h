@@ -404,9 +404,13 @@ int icarion_gpu_set_ehss_orientations(icarion_gpu *h, const uint32_t *n_orient, 
  *     (CompositeFieldProvider.h:117-171: DC kinds V(t); RF V(t) cos(2 pi f t +
  *     phi) with f the term's own frequency, else f(t); an inline waveform
  *     counts as 0 there).  One field array is not scaled (GridFieldProvider).
+ *   - The Orbitrap: compute_orbitrap_field reads the DC radial slot at t = 0
+ *     only (AnalyticalFieldModel.cpp:188-202), so its waveform fixes the field
+ *     constant at V(0) and the other slots are not read; any integrator.
  * The host evaluates them into FP64 tables per step.  slots = NULL clears.
- * Fixed-step integrators (RK4, Boris); RK45, the Orbitrap and the batch keep
- * constant voltages (ICARION_ERR_UNSUPPORTED when combined). */
+ * Fixed-step integrators (RK4, Boris); RK45 on the analytic IMS / quadrupole
+ * fields and on field arrays, and the batch, keep constant voltages
+ * (ICARION_ERR_UNSUPPORTED when combined). */
 int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const *slots);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
@@ -514,7 +518,7 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
  *     (0 or NULL: the radius).
  * Replaces any domain set before; n_domains = 1 is icarion_gpu_set_domain.
  * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
- * voltages, RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
+ * voltages or per-domain waveforms (icarion_gpu_set_domain_waveforms), RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
  * handoff in FP64 on the compensated position), no / HSS / Langevin /
  * Friction collisions, one
  * species table for all domains; icarion_gpu_step, icarion_gpu_step_batch and
@@ -523,6 +527,17 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
 int icarion_gpu_set_domains(icarion_gpu *h, const icarion_domain_desc *domains,
                             const double *end_aperture_m, uint32_t n_domains,
                             const icarion_physics_desc *physics);
+
+/* Voltage waveforms of one domain of a multi-domain set (each domain's
+ * ValueOrWaveform fields and waveform_library, DomainConfig.h), with the
+ * slots and rules of icarion_gpu_set_waveforms for that domain's analytic
+ * field: IMS / TIMS and quadrupole voltages evaluated at every stage time, the
+ * Orbitrap's DC radial voltage at t = 0.  slots = NULL clears the domain's
+ * waveforms; icarion_gpu_set_domains clears every domain's.  Handle-level
+ * waveforms (icarion_gpu_set_waveforms) are not combined with several domains:
+ * stepping returns ICARION_ERR_UNSUPPORTED.  No reference counterpart as a
+ * call: it carries the per-domain part of FullConfig::domains. */
+int icarion_gpu_set_domain_waveforms(icarion_gpu *h, uint32_t domain, const icarion_waveform_desc *const *slots);
 
 /* Every resident ion's domain index after the last step (the ensemble's
  * domain_index; 0 before the ion's first step, IonState's default).  n = the

@@ -84,7 +84,8 @@ def load():
     lib.oracle_run.restype = C.c_int
     lib.oracle_find_domain.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(d)]
     lib.oracle_find_domain.restype = C.c_int
-    lib.oracle_run_multi.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(_abi.PhysicsDesc),
+    lib.oracle_run_multi.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(P(_abi.OracleField)),
+                                     P(_abi.PhysicsDesc),
                                      P(_abi.SpeciesDesc), C.c_uint32, P(_abi.IonSoA), P(C.c_int32),
                                      d, d, C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats)]
     lib.oracle_run_multi.restype = C.c_int
@@ -368,16 +369,27 @@ def find_domain(domains, pos, end_apertures=None):
 
 
 def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
-              end_apertures=None, threads=0):
+              end_apertures=None, threads=0, waveforms=None):
     """The engine loop over several domains on ens and domain_index (int32,
-    in/out) in place; returns StepStats."""
+    in/out) in place; returns StepStats.  waveforms: None, or per domain None
+    or {slot: WaveformDesc}."""
     lib = load()
     arr, ap, app = _domains(domains, end_apertures)
     sp = species_array(species)
     soa = ens.soa()
     st = _abi.StepStats()
     assert domain_index.dtype == np.int32 and domain_index.flags.c_contiguous
-    rc = lib.oracle_run_multi(arr, app, len(domains), C.byref(physics), sp, len(species), C.byref(soa),
+    fptr = C.POINTER(C.POINTER(_abi.OracleField))()
+    keep = []
+    if waveforms is not None:
+        fl = (C.POINTER(_abi.OracleField) * len(domains))()
+        for k, slots in enumerate(waveforms):
+            if slots:
+                wf = Waveforms(slots)
+                keep.append(wf)
+                fl[k] = C.pointer(wf.c)
+        fptr = fl
+    rc = lib.oracle_run_multi(arr, app, len(domains), fptr, C.byref(physics), sp, len(species), C.byref(soa),
                               domain_index.ctypes.data_as(C.POINTER(C.c_int32)), t0, dt, n_steps,
                               step_offset, threads, C.byref(st))
     if rc != 0:

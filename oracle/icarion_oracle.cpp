@@ -289,14 +289,15 @@ V3 quadrupole_field(const icarion_domain_desc& d, V3 pos, double t, const oracle
     return E;
 }
 
-// AnalyticalFieldModel::compute_orbitrap_field (AnalyticalFieldModel.cpp:188-217).
-V3 orbitrap_field(const icarion_domain_desc& d, V3 gpos) {
+// AnalyticalFieldModel::compute_orbitrap_field (AnalyticalFieldModel.cpp:188-217):
+// the DC radial voltage is evaluated at t = 0 (:198), whatever the step time.
+V3 orbitrap_field(const icarion_domain_desc& d, V3 gpos, const oracle_field* f) {
     const V3 p = mat_mul(d.rot_g2l, gpos - V3{d.origin_m[0], d.origin_m[1], d.origin_m[2]});
     const double r_sq = std::max(1e-18, p.x * p.x + p.y * p.y);
     const double r_char = d.radius_char_m;
     const double r_char_sq = r_char * r_char;
     const double C = 1.0 - r_char_sq / r_sq;
-    const double voltage = d.dc_radial_V;
+    const double voltage = slot_value(f, ICARION_SLOT_DC_RADIAL, d.dc_radial_V, 0.0);
     const double r_in = d.radius_in_m;
     const double r_out = d.radius_out_m;
     const double k = 2.0 * voltage / (r_char_sq * std::log(r_out / r_in) -
@@ -419,7 +420,7 @@ V3 field_E(const icarion_domain_desc& d, const oracle_field* f, V3 pos, double t
         case ICARION_INSTRUMENT_QUADRUPOLE:
             return quadrupole_field(d, pos, t, f);
         case ICARION_INSTRUMENT_ORBITRAP:
-            return orbitrap_field(d, pos);
+            return orbitrap_field(d, pos, f);
         default:
             return V3{0.0, 0.0, 0.0};
     }
@@ -1248,6 +1249,7 @@ struct Domains {
     };
     const icarion_domain_desc* d;
     uint32_t n;
+    const oracle_field* const* fields = nullptr;   // per domain (waveforms), may be NULL
     std::vector<double> ap_sq;
     std::vector<Span> spans;
     bool has_orbitrap = false;
@@ -1327,11 +1329,12 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
         }
         dom_idx = dom;
         const icarion_domain_desc& d = D.d[dom];
+        const oracle_field* f = D.fields ? D.fields[dom] : nullptr;
         const double z_before = ion.pos.z;
         if (ph.integrator == ICARION_INTEGRATOR_BORIS)
-            boris_step(d, nullptr, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+            boris_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         else
-            rk4_step(d, nullptr, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+            rk4_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         ++pushed;
         if (ph.collision_model != ICARION_COLLISION_NONE) {
             PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
@@ -1602,12 +1605,13 @@ int oracle_find_domain(const icarion_domain_desc* domains, const double* end_ape
 }
 
 int oracle_run_multi(const icarion_domain_desc* domains, const double* end_aperture_m, uint32_t n_domains,
-                     const icarion_physics_desc* physics, const icarion_species_desc* species, uint32_t n_species,
+                     const oracle_field* const* fields, const icarion_physics_desc* physics, const icarion_species_desc* species, uint32_t n_species,
                      icarion_ion_soa* ions, int32_t* domain_index, double t0, double dt, uint64_t n_steps,
                      uint64_t step_offset, int n_threads, icarion_step_stats* stats) {
     if (!domains || n_domains == 0 || !physics || !species || !ions || !domain_index || n_species == 0)
         return ICARION_ERR_INVALID;
-    const Domains D(domains, end_aperture_m, n_domains);
+    Domains D(domains, end_aperture_m, n_domains);
+    D.fields = fields;
     const size_t n = ions->n;
     std::vector<double> t_table(n_steps + 1);
     double t = t0;

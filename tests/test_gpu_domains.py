@@ -19,7 +19,7 @@ from scipy import stats
 from icarion_amd import _abi
 from icarion_amd import workloads as W
 from test_gpu_parity import compare_ions
-from test_oracle_domains import _gap_domains, _one_ion
+from test_oracle_domains import _gap_domains, _one_ion, domain_waveforms
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +34,9 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
         if own:
             h.set_domains(domains or w["domains"], physics or w["physics"],
                           apertures if apertures is not None else w.get("apertures"))
+            for k, slots in enumerate(w.get("waveforms") or []):
+                if slots:
+                    h.set_domain_waveforms(k, slots)
             h.set_species(w["species"])
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
@@ -50,7 +53,8 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
     di = np.zeros(ens.n, np.int32) if dom is None else dom
     st = oracle.run_multi(domains or w["domains"], physics or w["physics"], w["species"], ens, di,
                           t0, w["dt"], n_steps, step_offset=step_offset,
-                          end_apertures=apertures if apertures is not None else w.get("apertures"))
+                          end_apertures=apertures if apertures is not None else w.get("apertures"),
+                          waveforms=w.get("waveforms"))
     return st, di
 
 
@@ -81,6 +85,47 @@ def test_three_domains_ion_by_ion(oracle, integrator, collision):
     assert sg.alive == pytest.approx(so.alive, rel=0.02, abs=5)
     # the run crosses every domain: live ions sit in several, dead ones died in several
     assert len(set(do[o.active == 1].tolist())) >= 2 or len(set(do.tolist())) >= 3
+
+
+@pytest.mark.parametrize("integrator,collision", [
+    (_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE),
+    (_abi.INTEGRATOR_BORIS, _abi.COLLISION_HSS),
+])
+def test_per_domain_waveforms_ion_by_ion(oracle, integrator, collision):
+    """Each domain with its own waveforms (an axial ramp in the first IMS
+    region, a modulated RF amplitude and a frequency ramp in the quadrupole, a
+    pulsed axial voltage in the last region) against the oracle with the same
+    waveforms per domain: ion by ion as the constant-voltage runs are held."""
+    w = W.multi_domain_workload(n=2048, seed=61, integrator=integrator, collision=collision)
+    w["waveforms"] = domain_waveforms()
+    print(f"domain waveforms integ={integrator} coll={collision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9) & (dg == do) & (g.death_time_s == o.death_time_s)
+    if collision == _abi.COLLISION_NONE:
+        assert ok.all(), np.flatnonzero(~ok)[:10]
+        assert sg.ion_steps == so.ion_steps
+    else:
+        assert ok.mean() >= 0.99
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    # the waveforms change the outcome: the constant-voltage run differs
+    c = w["ensemble"].copy()
+    gpu_multi(dict(w, waveforms=None), c, 6000)
+    assert (c.z != g.z).mean() > 0.5
+    # clearing a domain's waveforms restores its constants
+    from icarion_amd import gpu
+    with gpu.Handle() as h:
+        h.set_domains(w["domains"], w["physics"], w["apertures"])
+        for k, slots in enumerate(w["waveforms"]):
+            h.set_domain_waveforms(k, slots)
+            h.set_domain_waveforms(k, None)
+        h.set_species(w["species"])
+        e = w["ensemble"].copy()
+        h.upload_state(e)
+        h.step(0.0, w["dt"], 6000)
+        h.download_state(e)
+    np.testing.assert_array_equal(e.z, c.z)
 
 
 def test_bridge_and_handoff_cases_match_the_oracle(oracle):
@@ -155,6 +200,18 @@ def test_unsupported_combinations():
     with gpu.Handle() as h:
         with pytest.raises(gpu.IcarionError) as ei:
             h.set_domains(w["domains"] * 3, p)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+    # handle-level waveforms with several domains; a domain index out of range
+    with gpu.Handle() as h:
+        h.set_domains(w["domains"], p, w["apertures"])
+        h.set_species(w["species"])
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_domain_waveforms(3, domain_waveforms()[0])
+        assert ei.value.code == _abi.ERR_INVALID
+        h.set_waveforms(domain_waveforms()[0])
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, w["dt"], 5)
         assert ei.value.code == _abi.ERR_UNSUPPORTED
 
 

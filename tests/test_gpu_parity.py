@@ -737,6 +737,34 @@ def test_fp32_kahan_quadrupole_transmission(oracle):
         assert abs(tg - to) < err, (lo, hi, tg, to)
 
 
+@pytest.mark.parametrize("precision", [_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN])
+def test_orbitrap_radial_waveform(oracle, precision):
+    """An Orbitrap whose DC radial voltage is a waveform (a 3.0 -> 3.5 kV ramp
+    over the first microsecond) plus waveforms on slots the Orbitrap never
+    reads: the field takes V(0) (AnalyticalFieldModel.cpp:198), so the run
+    equals the constant-3.0 kV run bit for bit; in FP64 every ion also matches
+    the oracle with the same waveforms to 1e-9."""
+    import copy
+    w = W.orbitrap_workload(n=1024, seed=32)
+    w["waveforms"] = {
+        _abi.SLOT_DC_RADIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (3000.0, 3500.0, 0.0, 1e-6)),
+        _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_CONSTANT, (900.0,)),
+        _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (300.0, 0.0, 1e6, 0.0)),
+    }
+    c = dict(w, domain=copy.copy(w["domain"]))
+    c["domain"].dc_radial_V = 3000.0
+    c.pop("waveforms")
+    g, k = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 3000, precision=precision)
+    gpu_run(c, k, 3000, precision=precision)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active"):
+        np.testing.assert_array_equal(getattr(g, f), getattr(k, f), err_msg=f)
+    if precision == _abi.PRECISION_FP64:
+        o = w["ensemble"].copy()
+        oracle_run(oracle, w, o, 3000)
+        assert compare_ions(g, o, rtol=1e-9).all()
+
+
 # ------------------------------------------------------------------ API errors
 def test_error_paths():
     from icarion_amd import gpu

@@ -53,6 +53,8 @@ def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
             h.set_gas_mixture(*w["mixture"])
             if w.get("mixture_radii") is not None:
                 h.set_gas_radii(w["mixture_radii"])
+        if w.get("waveforms") is not None:
+            h.set_waveforms(w["waveforms"])
         h.upload_state(ens)
         st = h.step(t0, dt, n_steps, step_offset)
         h.download_state(ens)
@@ -64,6 +66,8 @@ def oracle_rk45(oracle, w, ph, ens, n_steps, dt, t0=0.0, cfg=None):
     field = oracle.Field(*w["field"]) if w.get("field") is not None else None
     if w.get("terms") is not None:
         field = oracle.FieldTerms(w["axes"], w["terms"])
+    elif w.get("waveforms") is not None and field is None:
+        field = oracle.Waveforms(w["waveforms"])
     mixture = (oracle.GasMixture(*w["mixture"], radii=w.get("mixture_radii"))
                if w.get("mixture") is not None else None)
     st, _ = oracle.run_rk45(w["domain"], ph, cfg, w["species"], ens, t0, dt, n_steps,
@@ -335,3 +339,26 @@ def test_rk45_fp32_ims_hss_statistics(oracle):
     dz_g, dz_o = g.z - z0, o.z - z0
     se = np.sqrt(dz_g.var() / g.n + dz_o.var() / o.n)
     assert abs(dz_g.mean() - dz_o.mean()) < 4 * se + 0.01 * abs(dz_o.mean())
+
+
+def test_orbitrap_radial_waveform_rk45(oracle):
+    """RK45 in an Orbitrap whose DC radial voltage is a waveform: the field
+    takes V(0) (AnalyticalFieldModel.cpp:198), so the run equals the run at the
+    constant V(0) bit for bit, and the oracle with the waveform within the
+    adaptive-step bounds (first step ion by ion)."""
+    import copy
+    w = W.orbitrap_workload(n=512, seed=31)
+    w["waveforms"] = {_abi.SLOT_DC_RADIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (3000.0, 3500.0, 0.0, 1e-6))}
+    c = dict(w, domain=copy.copy(w["domain"]))
+    c["domain"].dc_radial_V = 3000.0
+    c.pop("waveforms")
+    ph = rk45_physics(w)
+    g, k, o = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_rk45(w, ph, g, 40, w["dt"])
+    gpu_rk45(c, ph, k, 40, w["dt"])
+    for f in ("x", "y", "z", "vx", "vy", "vz", "time_s", "dt_s", "active"):
+        np.testing.assert_array_equal(getattr(g, f), getattr(k, f), err_msg=f)
+    g1, o1 = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_rk45(w, ph, g1, 1, w["dt"])
+    oracle_rk45(oracle, w, ph, o1, 1, w["dt"])
+    assert match(g1, o1, rtol=1e-8).all()

@@ -16,7 +16,11 @@ restatement is pinned here by properties and by hand-derived cases:
   * the handoff moves the index on within 1e-9 m of the end face, inside the
     end aperture only;
   * one domain through the multi-domain loop is the single-domain loop, bit
-    for bit.
+    for bit;
+  * each domain's waveforms act in that domain only: while every ion stays in
+    the first domain, the multi-domain run with waveforms on all three domains
+    is the single-domain run with the first domain's waveforms, bit for bit,
+    and differs from the run without them.
 """
 import ctypes as C
 
@@ -144,3 +148,34 @@ def test_one_domain_is_the_single_domain_loop(oracle, collision):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
     alive = a.active == 1
     assert (di[alive] == 0).all()
+
+
+def domain_waveforms():
+    """Waveforms for the three-domain workload: an axial ramp in the first IMS
+    region, a modulated RF amplitude and a frequency ramp in the quadrupole,
+    pulsed axial voltage in the last region."""
+    return [
+        {_abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (20.0, 60.0, 0.0, 1e-7), 1)},
+        {_abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (200.0, 20.0, 1e5, 0.0)),
+         _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_LINEAR, (3.0e6, 3.2e6, 0.0, 6e-6), 1)},
+        {_abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_PULSED, (10.0, 30.0, 2e-6, 1e-6))},
+    ]
+
+
+def test_per_domain_waveforms_act_in_their_domain(oracle):
+    w = W.multi_domain_workload(n=256, seed=60)
+    wfs = domain_waveforms()
+    n_steps = 200   # 0.3 mm of drift: no ion reaches the first boundary at 5 mm
+    a, b, c = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+    di = np.zeros(a.n, np.int32)
+    oracle.run_multi(w["domains"], w["physics"], w["species"], a, di, 0.0, w["dt"], n_steps,
+                     end_apertures=w["apertures"], waveforms=wfs)
+    single = dict(domain=w["domains"][0], physics=w["physics"], species=w["species"], dt=w["dt"])
+    oracle.run(single["domain"], w["physics"], w["species"], b, 0.0, w["dt"], n_steps,
+               field=oracle.Waveforms(wfs[0]))
+    oracle.run(single["domain"], w["physics"], w["species"], c, 0.0, w["dt"], n_steps)
+    assert (di[a.active == 1] == 0).all()
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    live = a.active == 1
+    assert live.sum() > 100 and (a.vz[live] != c.vz[live]).all()

@@ -776,3 +776,26 @@ def test_cylinder_boundary_and_death_time(oracle):
     z_at_death = 1000.0 * t_die
     # the step [t_die, t_die + dt) crosses the end face at L + eps (1e-12 m)
     assert z_at_death <= 15e-3 + 1e-12 + 1e-15 < z_at_death + 1000.0 * dt
+
+
+def test_orbitrap_radial_waveform_fixed_at_t0(oracle):
+    """compute_orbitrap_field evaluates the DC radial voltage at t = 0
+    (AnalyticalFieldModel.cpp:198): with a waveform on that slot, the field at
+    any step time equals the constant-voltage field at V(0), and a waveform on
+    any other slot changes nothing.  A field that read V(t) fails at the later
+    times, where the ramp is 40 % higher."""
+    import copy
+    d = W.orbitrap_domain(V=3500.0)
+    ramp = _abi.waveform(_abi.WAVEFORM_LINEAR, (2500.0, 3500.0, 0.0, 1e-6))
+    wf = oracle.Waveforms({_abi.SLOT_DC_RADIAL: ramp,
+                           _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_CONSTANT, (900.0,)),
+                           _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_CONSTANT, (300.0,))})
+    d0 = copy.copy(d)
+    d0.dc_radial_V = 2500.0
+    pos = (9e-3, 1e-3, 4e-3)
+    for t in (0.0, 3e-7, 1e-6, 5e-6):
+        got = oracle.field_E(d, pos, t, wf)
+        np.testing.assert_array_equal(got, oracle.field_E(d0, pos, t))
+    # and it is the 2500 V field, not the domain's 3500 V one: E scales with V
+    np.testing.assert_allclose(oracle.field_E(d, pos, 1e-6, wf) * 3500.0 / 2500.0,
+                               oracle.field_E(d, pos, 1e-6), rtol=1e-14)
