@@ -120,15 +120,6 @@ def rk45_defaults(**kw):
     return d
 
 
-class CloudDesc(C.Structure):
-    _fields_ = [
-        ("domain", DomainDesc), ("physics", PhysicsDesc),
-        ("species", C.POINTER(SpeciesDesc)), ("n_species", _u32), ("reserved0", _u32),
-        ("ions", C.POINTER(IonSoA)), ("t0_s", _d), ("dt_s", _d), ("n_steps", _u64),
-        ("step_offset", _u64),
-    ]
-
-
 class BatchTuning(C.Structure):
     _fields_ = [("block_size", _u32), ("groups_per_cu", _u32)]
 
@@ -176,6 +167,15 @@ class WaveformDesc(C.Structure):
     _fields_ = [("type", _i32), ("interp", _i32), ("clamp", _i32), ("source", _i32),
                 ("p", _d * 8), ("times_s", C.POINTER(_d)), ("values", C.POINTER(_d)),
                 ("n_points", _u32), ("reserved1", _u32)]
+
+
+class CloudDesc(C.Structure):
+    _fields_ = [
+        ("domain", DomainDesc), ("physics", PhysicsDesc),
+        ("species", C.POINTER(SpeciesDesc)), ("n_species", _u32), ("reserved0", _u32),
+        ("ions", C.POINTER(IonSoA)), ("t0_s", _d), ("dt_s", _d), ("n_steps", _u64),
+        ("step_offset", _u64), ("waveforms", C.POINTER(C.POINTER(WaveformDesc))),
+    ]
 
 
 def waveform(type_, p=(), clamp=1, interp=0, points=(), source=WAVEFORM_LIBRARY):

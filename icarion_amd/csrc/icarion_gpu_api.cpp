@@ -589,6 +589,42 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
     }
 }
 
+// Checks and copies the slot descriptors (NULL entries: the constant); true
+// when any slot holds a waveform.
+static bool parse_waveform_slots(const icarion_waveform_desc* const* slots, std::unique_ptr<HostWaveform>* w) {
+    bool any = false;
+    for (int k = 0; slots && k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+        const icarion_waveform_desc* d = slots[k];
+        if (!d) continue;
+        if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
+            fail(ICARION_ERR_INVALID, "unknown waveform type");
+        if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
+            fail(ICARION_ERR_INVALID, "unknown waveform source");
+        for (double v : d->p)
+            if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
+        auto hw = std::make_unique<HostWaveform>();
+        hw->d = *d;
+        if (d->type == ICARION_WAVEFORM_ARBITRARY) {
+            // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
+            if (!d->times_s || !d->values || d->n_points == 0)
+                fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
+            if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
+                fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
+            hw->ts.assign(d->times_s, d->times_s + d->n_points);
+            hw->vs.assign(d->values, d->values + d->n_points);
+            for (uint32_t i = 0; i < d->n_points; ++i)
+                if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
+                    (i && !(hw->ts[i] > hw->ts[i - 1])))
+                    fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
+        }
+        hw->d.times_s = nullptr;
+        hw->d.values = nullptr;
+        w[k] = std::move(hw);
+        any = true;
+    }
+    return any;
+}
+
 // CompositeFieldProvider::eval_voltage (CompositeFieldProvider.h:150-171): the
 // domain's constant, else a library waveform at t; an inline waveform gives 0.
 double composite_voltage(const icarion_gpu* h, int slot, double constant, double t) {
@@ -1387,9 +1423,11 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     std::vector<uint32_t> species_off(n_clouds);
     std::vector<StepEntry> tables;
     std::vector<size_t> table_off(n_clouds);
-    // identical step tables are shared: (t0, dt, n, RF inputs, stage layout)
+    // identical step tables are shared: (t0, dt, n, RF inputs, axial field, stage layout)
+    // E_z rides in the table too (e0 / eh / e1), so it is part of the key;
+    // clouds with waveforms get tables of their own
     struct Key {
-        double t0, dt, U, V, f, r0;
+        double t0, dt, U, V, f, r0, Ez;
         uint64_t n;
         int rf, rk4;
         bool operator==(const Key& o) const {
@@ -1447,15 +1485,34 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
             species.push_back(sd);
         }
         StepParams P = derive_params(cd.domain, cd.physics);
-        const Key key{cd.t0_s, cd.dt_s, cd.domain.dc_quad_V, cd.domain.rf_voltage_V, cd.domain.rf_frequency_Hz,
-                      cd.domain.radius_m, cd.n_steps, P.has_rf, cd.physics.integrator == ICARION_INTEGRATOR_RK4};
+        // the cloud's voltage waveforms, as icarion_gpu_set_waveforms on a handle of its own
+        std::unique_ptr<HostWaveform> wv[ICARION_N_VOLTAGE_SLOTS];
+        const bool waves = parse_waveform_slots(cd.waveforms, wv);
+        const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
+        for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) wf[k] = wv[k].get();
+        if (wf[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
+        if (field == FIELD_ORBITRAP && wf[ICARION_SLOT_DC_RADIAL])
+            P.orbi_k = orbitrap_k(cd.domain, wf[ICARION_SLOT_DC_RADIAL]->eval(0.0));
+        Key key{};
+        key.t0 = cd.t0_s;
+        key.dt = cd.dt_s;
+        key.U = cd.domain.dc_quad_V;
+        key.V = cd.domain.rf_voltage_V;
+        key.f = cd.domain.rf_frequency_Hz;
+        key.r0 = cd.domain.radius_m;
+        key.Ez = P.Ez_const;
+        key.n = cd.n_steps;
+        key.rf = P.has_rf;
+        key.rk4 = cd.physics.integrator == ICARION_INTEGRATOR_RK4;
         size_t off = SIZE_MAX;
-        for (const auto& kv : seen)
-            if (kv.first == key) off = kv.second;
+        if (!waves)
+            for (const auto& kv : seen)
+                if (kv.first == key) off = kv.second;
         if (off == SIZE_MAX) {
             off = tables.size();
-            append_table(tables, cd.domain, P.has_rf != 0, cd.physics.integrator, cd.t0_s, cd.dt_s, cd.n_steps);
-            seen.push_back({key, off});
+            append_table(tables, cd.domain, P.has_rf != 0, cd.physics.integrator, cd.t0_s, cd.dt_s, cd.n_steps,
+                         false, waves ? wf : nullptr);
+            if (!waves) seen.push_back({key, off});
         }
         table_off[c] = off;
         bc[c].t_end = cd.n_steps ? tables[off + cd.n_steps - 1].t + cd.dt_s : cd.t0_s;
@@ -2320,42 +2377,6 @@ int icarion_gpu_set_gas_radii(icarion_gpu* h, const double* radius_m, uint32_t n
                 fail(ICARION_ERR_INVALID, "radii must be finite and >= 0");
         h->mix_radius.assign(radius_m, radius_m + n_components);
     });
-}
-
-// Checks and copies the slot descriptors (NULL entries: the constant); true
-// when any slot holds a waveform.
-static bool parse_waveform_slots(const icarion_waveform_desc* const* slots, std::unique_ptr<HostWaveform>* w) {
-    bool any = false;
-    for (int k = 0; slots && k < ICARION_N_VOLTAGE_SLOTS; ++k) {
-        const icarion_waveform_desc* d = slots[k];
-        if (!d) continue;
-        if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
-            fail(ICARION_ERR_INVALID, "unknown waveform type");
-        if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
-            fail(ICARION_ERR_INVALID, "unknown waveform source");
-        for (double v : d->p)
-            if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
-        auto hw = std::make_unique<HostWaveform>();
-        hw->d = *d;
-        if (d->type == ICARION_WAVEFORM_ARBITRARY) {
-            // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
-            if (!d->times_s || !d->values || d->n_points == 0)
-                fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
-            if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
-                fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
-            hw->ts.assign(d->times_s, d->times_s + d->n_points);
-            hw->vs.assign(d->values, d->values + d->n_points);
-            for (uint32_t i = 0; i < d->n_points; ++i)
-                if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
-                    (i && !(hw->ts[i] > hw->ts[i - 1])))
-                    fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
-        }
-        hw->d.times_s = nullptr;
-        hw->d.values = nullptr;
-        w[k] = std::move(hw);
-        any = true;
-    }
-    return any;
 }
 
 int icarion_gpu_set_waveforms(icarion_gpu* h, const icarion_waveform_desc* const* slots) {

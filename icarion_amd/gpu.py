@@ -186,7 +186,7 @@ class Handle:
     # ---- batched clouds ------------------------------------------------------
     def _cloud_array(self, clouds):
         """clouds: list of dicts with domain, physics, species (list), ensemble,
-        t0, dt, n_steps and optional step_offset."""
+        t0, dt, n_steps and optional step_offset and waveforms ({slot: desc})."""
         arr = (_abi.CloudDesc * len(clouds))()
         keep = []
         for k, c in enumerate(clouds):
@@ -202,6 +202,12 @@ class Handle:
             arr[k].dt_s = c["dt"]
             arr[k].n_steps = c["n_steps"]
             arr[k].step_offset = c.get("step_offset", 0)
+            if c.get("waveforms"):
+                wa = (C.POINTER(_abi.WaveformDesc) * _abi.N_VOLTAGE_SLOTS)()
+                for slot, w in c["waveforms"].items():
+                    wa[slot] = C.pointer(w)
+                keep += [wa, dict(c["waveforms"])]
+                arr[k].waveforms = C.cast(wa, C.POINTER(C.POINTER(_abi.WaveformDesc)))
         return arr, keep
 
     def batch_upload(self, clouds):

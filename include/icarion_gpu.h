@@ -409,8 +409,8 @@ int icarion_gpu_set_ehss_orientations(icarion_gpu *h, const uint32_t *n_orient, 
  *     constant at V(0) and the other slots are not read; any integrator.
  * The host evaluates them into FP64 tables per step.  slots = NULL clears.
  * Fixed-step integrators (RK4, Boris); RK45 on the analytic IMS / quadrupole
- * fields and on field arrays, and the batch, keep constant voltages
- * (ICARION_ERR_UNSUPPORTED when combined). */
+ * fields and on field arrays keeps constant voltages (ICARION_ERR_UNSUPPORTED
+ * when combined).  Batched clouds take theirs in icarion_cloud_desc.waveforms. */
 int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const *slots);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
@@ -655,6 +655,11 @@ typedef struct icarion_cloud_desc {
     double dt_s;
     uint64_t n_steps;             /* < 2^32                                    */
     uint64_t step_offset;         /* first RNG step number                     */
+    const icarion_waveform_desc *const *waveforms;
+                                  /* NULL, or ICARION_N_VOLTAGE_SLOTS entries
+                                     (NULL = the domain's constant): the cloud's
+                                     voltage waveforms, as icarion_gpu_set_waveforms
+                                     on a handle of its own (analytic fields)  */
 } icarion_cloud_desc;
 
 /* Launch shape of the persistent batch kernel (per precision class). */

@@ -67,6 +67,8 @@ def single_runs(clouds, params=None):
             h.set_species(c["species"])
             if c["physics"].collision_model == _abi.COLLISION_EHSS:
                 h.surrogate_set_params(params)
+            if c.get("waveforms"):
+                h.set_waveforms(c["waveforms"])
             h.upload_state(c["ensemble"])
             stats.append(h.step(c["t0"], c["dt"], c["n_steps"], c["step_offset"]))
             h.download_state(c["ensemble"])
@@ -118,6 +120,77 @@ def test_batch_equals_one_handle_per_cloud():
         assert a.collisions == pytest.approx(b.collisions, rel=5e-3, abs=2)
         assert a.t_end_s == b.t_end_s
         assert 0.0 < a.done_ms <= ms * 1.05 + 0.05
+
+
+def rtol_of(cloud):
+    """Batch against a handle of its own: 1e-9 in FP64 (contraction differs
+    between the two compiled kernels at ~1e-13); 1e-5 in FP32 + Kahan, where
+    the same differences are single-precision roundings."""
+    return 1e-9 if cloud["physics"].precision == _abi.PRECISION_FP64 else 1e-5
+
+
+def test_batch_clouds_differing_only_in_the_axial_field():
+    """Clouds with the same engine times and RF inputs but different axial
+    voltages or lengths: the step tables carry E_z, so each cloud must still
+    get its own field (each equals a handle of its own).  Collisionless, so
+    that the comparison is deterministic in both precisions."""
+    import copy
+    ref = []
+    V0 = W.ims_hss_workload(n=8)["domain"].dc_axial_V
+    for V, L in ((V0, 20e-3), (2.0 * V0, 20e-3), (V0, 28e-3)):
+        for prec in (_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN):
+            w = W.ims_hss_workload(n=800, seed=200, precision=prec)
+            d = copy.copy(w["domain"])
+            d.dc_axial_V, d.length_m = V, L
+            ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, precision=prec, seed=200)
+            ref.append(dict(kind=f"ims V={V:.0f} L={L} prec={prec}", domain=d, physics=ph,
+                            species=w["species"], ensemble=w["ensemble"].copy(), t0=0.0, dt=w["dt"],
+                            n_steps=300, step_offset=0))
+    bat = copy_clouds(ref)
+    single_runs(ref)
+    batch_runs(bat)
+    fr = [close_fraction(ca["ensemble"], cb["ensemble"], rtol_of(ca)) for ca, cb in zip(bat, ref)]
+    print("close fractions", [(c["kind"], f) for c, f in zip(bat, fr)])
+    assert min(fr) == 1.0
+    # and the fields differ (same start, same random streams): twice the
+    # voltage drifts further, the longer region less far
+    z = [c["ensemble"].z.mean() for c in bat]
+    assert z[2] > z[0] > z[4] and z[3] > z[1] > z[5]
+
+
+def waveform_clouds():
+    """An IMS cloud with an axial ramp, a quadrupole with a modulated RF
+    amplitude and a frequency ramp, an Orbitrap with a radial ramp (read at
+    t = 0), and the same IMS cloud without waveforms on the same engine times."""
+    ims = W.ims_hss_workload(n=1500, seed=301)
+    quad = W.quadrupole_workload(n=1500, U=10.0, seed=302)
+    orbi = W.orbitrap_workload(n=800, seed=303)
+    wf_ims = {_abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (10.0, 40.0, 0.0, 2e-8))}
+    wf_quad = {_abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (500.0, 50.0, 2e5, 0.0)),
+               _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_LINEAR, (1.0e6, 1.2e6, 0.0, 4e-7))}
+    wf_orbi = {_abi.SLOT_DC_RADIAL: _abi.waveform(_abi.WAVEFORM_LINEAR, (3000.0, 3500.0, 0.0, 1e-6))}
+    out = []
+    for kind, w, wf, steps in (("ims_wf", ims, wf_ims, 300), ("quad_wf", quad, wf_quad, 400),
+                               ("orbi_wf", orbi, wf_orbi, 600), ("ims_plain", ims, None, 300)):
+        out.append(dict(kind=kind, domain=w["domain"], physics=w["physics"], species=w["species"],
+                        ensemble=w["ensemble"].copy(), t0=0.0, dt=w["dt"], n_steps=steps, step_offset=0,
+                        waveforms=wf))
+    return out
+
+
+def test_batch_waveform_clouds_equal_one_handle_per_cloud():
+    ref = waveform_clouds()
+    bat = copy_clouds(ref)
+    single_runs(ref)
+    batch_runs(bat)
+    for ca, cb in zip(bat, ref):
+        stochastic = ca["physics"].collision_model != _abi.COLLISION_NONE
+        frac = close_fraction(ca["ensemble"], cb["ensemble"], rtol_of(ca))
+        print(ca["kind"], frac)
+        assert frac >= (0.995 if stochastic else 1.0), (ca["kind"], frac)
+    # the ramped IMS cloud and the plain one share engine times but not fields
+    a, b = bat[0]["ensemble"], bat[3]["ensemble"]
+    assert (a.z != b.z).mean() > 0.9
 
 
 @pytest.mark.parametrize("shape", [(64, 1), (128, 8), (256, 16)])
