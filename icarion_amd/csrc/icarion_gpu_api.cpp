@@ -184,6 +184,9 @@ struct icarion_gpu {
     bool has_waves = false;
     // several domains: each domain's own voltage waveforms (n_dom sets)
     std::vector<std::array<std::unique_ptr<HostWaveform>, ICARION_N_VOLTAGE_SLOTS>> dom_wave;
+    // RK45 evaluates the waveforms in the kernel: their descriptors and points
+    DevBuf<icarion_waveform_desc> wave_desc;
+    DevBuf<double> wave_pts;
     std::vector<icarion_gas_component> mix;
     uint32_t mix_species = 0;
     DevBuf<double> mix_sigma;
@@ -746,14 +749,51 @@ void check_ehss_mixture(const icarion_gpu* h, bool surrogate) {
             fail(ICARION_ERR_INVALID, "every participating gas component needs a mass > 0");
 }
 
+// RK45 on an analytic IMS / quadrupole field with waveforms: the kernel
+// evaluates every slot at each ion's stage times, from the descriptors and
+// ARBITRARY points copied here and each slot's constant.
+void upload_rk45_waves(icarion_gpu* h, StepParams& P) {
+    const icarion_domain_desc& d = h->dom;
+    const double cst[ICARION_N_VOLTAGE_SLOTS] = {d.dc_axial_V, d.dc_quad_V, d.dc_radial_V, d.rf_voltage_V,
+                                                 d.rf_frequency_Hz};
+    std::vector<icarion_waveform_desc> desc(ICARION_N_VOLTAGE_SLOTS);
+    std::vector<double> pts;
+    size_t off_t[ICARION_N_VOLTAGE_SLOTS] = {}, off_v[ICARION_N_VOLTAGE_SLOTS] = {};
+    uint32_t mask = 0;
+    for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+        P.wave_const[k] = cst[k];
+        const HostWaveform* w = h->wave[k].get();
+        if (!w) continue;
+        mask |= 1u << k;
+        desc[k] = w->d;
+        desc[k].n_points = static_cast<uint32_t>(w->ts.size());
+        off_t[k] = pts.size();
+        pts.insert(pts.end(), w->ts.begin(), w->ts.end());
+        off_v[k] = pts.size();
+        pts.insert(pts.end(), w->vs.begin(), w->vs.end());
+    }
+    h->wave_pts.resize(std::max<size_t>(pts.size(), 1));
+    if (!pts.empty()) copy_in(h->wave_pts, pts.data(), pts.size(), h->stream, "waveform points");
+    for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+        desc[k].times_s = (mask >> k & 1u) && desc[k].n_points ? h->wave_pts.p + off_t[k] : nullptr;
+        desc[k].values = (mask >> k & 1u) && desc[k].n_points ? h->wave_pts.p + off_v[k] : nullptr;
+    }
+    h->wave_desc.resize(ICARION_N_VOLTAGE_SLOTS);
+    copy_in(h->wave_desc, desc.data(), desc.size(), h->stream, "waveform descriptors");
+    hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    P.waves = h->wave_desc.p;
+    P.wave_mask = mask;
+    if (h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
+}
+
 // The RK45 engine loop: one launch pair (step kernel + engine-time advance) per
 // engine step, replayed from a captured HIP graph of up to 32 steps.
 void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
                   icarion_step_stats* stats) {
     const int field = launch_field_kind(h);
     const int coll = launch_coll_kind(h);
-    if (h->has_waves && field != FIELD_ORBITRAP)
-        fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with constant voltages");
+    if (h->has_waves && field != FIELD_ORBITRAP && field != FIELD_LINEAR)
+        fail(ICARION_ERR_UNSUPPORTED, "RK45 on a field array runs with constant voltages");
     if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -774,6 +814,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     P.counter_stripes = kCounterStripes;
     P.rk45 = h->rk45;
     apply_orbitrap_waveform(h, P);
+    if (h->has_waves && field == FIELD_LINEAR) upload_rk45_waves(h, P);
     fill_fast_const(P);
     const IonPtrs S = ion_ptrs(h);
     h->r_eng.resize(1);

@@ -9,6 +9,7 @@
 // the threshold to the other side.
 #pragma clang fp contract(off)
 
+#include "waveform.hpp"
 #include "step_device.hpp"
 #include "step_device_f32.hpp"
 
@@ -57,6 +58,36 @@ __device__ __forceinline__ double ord_value(unsigned long long o) {
 // U_eff(t) / r0^2 (AnalyticalFieldModel.cpp:226-241) at the ion's stage time.
 __device__ __forceinline__ double rf_coeff(const StepParams& P, double t) {
     return P.has_rf ? (P.rf_U + P.rf_V * cos(P.rf_omega * t)) / P.r0_sq : 0.0;
+}
+
+// A voltage slot at t: its waveform where one is set, else the constant.
+__device__ __forceinline__ double wave_slot(const StepParams& P, int k, double t) {
+    if (P.wave_mask & (1u << k)) {
+        const icarion_waveform_desc& w = P.waves[k];
+        return waveform_eval(w, w.times_s, w.values, w.n_points, t);
+    }
+    return P.wave_const[k];
+}
+
+// The analytic IMS / quadrupole field's inputs at t with waveforms
+// (AnalyticalFieldModel.cpp:136-161, :219-248): E_z = axial(t) / L (the
+// quadrupole's where |axial| > 1e-12), and the RF coefficient
+// (quad(t) + V(t) cos(2 pi f(t) t)) / r0^2 where |V(t)| > 1e-12 - the values
+// the fixed-step paths take from their step tables.
+__device__ __forceinline__ void wave_field_inputs(const StepParams& P, double t, double& c, double& ez) {
+    const double ax = wave_slot(P, ICARION_SLOT_DC_AXIAL, t);
+    if (P.instrument == ICARION_INSTRUMENT_QUADRUPOLE) ez = fabs(ax) > 1e-12 ? ax / P.L : 0.0;
+    else ez = ax / P.L;
+    c = 0.0;
+    if (P.has_rf) {
+        const double V = wave_slot(P, ICARION_SLOT_RF_VOLTAGE, t);
+        if (fabs(V) > 1e-12) {
+            const double f = wave_slot(P, ICARION_SLOT_RF_FREQUENCY, t);
+            const double U = wave_slot(P, ICARION_SLOT_DC_QUAD, t);
+            const double om = 2.0 * kPi * f;
+            c = (U + V * cos(om * t)) / P.r0_sq;
+        }
+    }
 }
 
 // RK45Strategy::compute_new_step (RK45Strategy.cpp:315-351)
@@ -133,8 +164,10 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                 }
                 // analytic fields take the RF coefficient, field arrays the stage time
                 auto acc = [&](d3 p, d3 v, double tt) {
-                    const double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
-                    return force_f64<FIELD, COLL>(P, p, v, c, P.Ez_const, q, m, lpre) * inv_m;
+                    double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
+                    double ez = P.Ez_const;
+                    if (FIELD == FIELD_LINEAR && P.wave_mask) wave_field_inputs(P, tt, c, ez);
+                    return force_f64<FIELD, COLL>(P, p, v, c, ez, q, m, lpre) * inv_m;
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);
@@ -318,8 +351,10 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
                     lpre = (float)l;
                 }
                 auto acc = [&](f3 p, f3 v, double tt) {
-                    const double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
-                    return accel_f32<FIELD, COLL>(P, C, p, v, c, P.Ez_const, qm, lpre);
+                    double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
+                    double ez = P.Ez_const;
+                    if (FIELD == FIELD_LINEAR && P.wave_mask) wave_field_inputs(P, tt, c, ez);
+                    return accel_f32<FIELD, COLL>(P, C, p, v, c, ez, qm, lpre);
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);

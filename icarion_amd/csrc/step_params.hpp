@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/icarion_gpu.h"
+
 namespace icarion {
 
 // Field kinds the kernels are specialised for.
@@ -207,6 +209,12 @@ struct StepParams {
     unsigned long long *counters;   // [0] ion-steps, [1] collisions
     FastConst fc;
     Rk45Cfg rk45;
+    // RK45 with voltage waveforms on the analytic IMS / TIMS and quadrupole
+    // fields: the slot descriptors (ARBITRARY points in device memory), the
+    // mask of slots that hold one, and every slot's constant
+    const icarion_waveform_desc *waves;
+    uint32_t wave_mask, wave_pad;
+    double wave_const[ICARION_N_VOLTAGE_SLOTS];
 };
 
 // Fill P.fc from the FP64 fields (after P.dt is set).

@@ -407,10 +407,11 @@ int icarion_gpu_set_ehss_orientations(icarion_gpu *h, const uint32_t *n_orient, 
  *   - The Orbitrap: compute_orbitrap_field reads the DC radial slot at t = 0
  *     only (AnalyticalFieldModel.cpp:188-202), so its waveform fixes the field
  *     constant at V(0) and the other slots are not read; any integrator.
- * The host evaluates them into FP64 tables per step.  slots = NULL clears.
- * Fixed-step integrators (RK4, Boris); RK45 on the analytic IMS / quadrupole
- * fields and on field arrays keeps constant voltages (ICARION_ERR_UNSUPPORTED
- * when combined).  Batched clouds take theirs in icarion_cloud_desc.waveforms. */
+ * The fixed-step integrators (RK4, Boris) take them from FP64 tables the host
+ * evaluates per step; RK45 evaluates them in the kernel at each ion's own
+ * stage times.  slots = NULL clears.  RK45 on a field array keeps constant
+ * voltages (ICARION_ERR_UNSUPPORTED when combined).  Batched clouds take
+ * theirs in icarion_cloud_desc.waveforms. */
 int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const *slots);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;

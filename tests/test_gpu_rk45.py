@@ -121,11 +121,37 @@ def case_workload(case):
     if case == "hss_mixture":
         w = W.ims_mixture_workload(n=2048, seed=34)
         return w, rk45_physics(w), w["dt"]
+    if case == "quadrupole_waveforms":
+        w = W.quadrupole_workload(n=2048, U=10.0, seed=6)
+        w["waveforms"] = rk45_quadrupole_waveforms()
+        return w, rk45_physics(w, collision=_abi.COLLISION_NONE, ou=0), 1e-9
+    if case == "ims_waveforms":
+        w = W.ims_hss_workload(n=2048, seed=35)
+        V0 = w["domain"].dc_axial_V
+        w["waveforms"] = {
+            _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_EXPONENTIAL, (V0, 0.3 * V0, 1e8, 0.0, 8e-9), 1),
+            _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_PWM, (0.0, 50.0, 5e8, 0.5, 0.0, 0.0, 1e-8), 1),
+        }
+        return w, rk45_physics(w), w["dt"]
     w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
     return w, rk45_physics(w), w["dt"]
 
 
-CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss", "field_terms", "hss_mixture"]
+def rk45_quadrupole_waveforms():
+    """Waveforms on the time scale of an RK45 test (0-100 ns): RF amplitude
+    modulated (sinusoidal), RF frequency ramped (linear), a DC quad pulse and
+    an axial DC curve (arbitrary, cubic)."""
+    return {
+        _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_SINUSOIDAL, (450.0, 120.0, 2e7, 0.3)),
+        _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_LINEAR, (1.0e6, 1.3e6, 20e-9, 80e-9)),
+        _abi.SLOT_DC_QUAD: _abi.waveform(_abi.WAVEFORM_PULSED, (0.0, 35.0, 30e-9, 30e-9)),
+        _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_ARBITRARY, (), 1, _abi.INTERP_CUBIC,
+                                          [(0.0, 0.0), (25e-9, 4.0), (60e-9, -3.0), (100e-9, 6.0)]),
+    }
+
+
+CASES = ["quadrupole_rf", "orbitrap", "langevin_ou", "field_array_hss", "field_terms", "hss_mixture",
+         "quadrupole_waveforms", "ims_waveforms"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -145,7 +171,7 @@ def test_rk45_constant_dt_chain_ion_by_ion(oracle, case):
     assert sg.rejected == 0 and so.rejected == 0
     assert sg.ion_steps == pytest.approx(so.ion_steps, rel=1e-3)
     assert sg.t_end_s == pytest.approx(so.t_end_s, rel=1e-12)
-    need = 0.99 if case in ("field_array_hss", "hss_mixture") else 0.999
+    need = 0.99 if case in ("field_array_hss", "hss_mixture", "ims_waveforms") else 0.999
     assert match(g, o, rtol=1e-9, dt_rtol=1e-12).mean() >= need
 
 
@@ -264,7 +290,7 @@ def fp32_physics(ph):
                      _abi.PRECISION_FP32_KAHAN, ph.rng_seed)
 
 
-@pytest.mark.parametrize("case", ["quadrupole_rf", "orbitrap"])
+@pytest.mark.parametrize("case", ["quadrupole_rf", "orbitrap", "quadrupole_waveforms"])
 def test_rk45_fp32_constant_dt_chain(oracle, case):
     """FP32 stages with compensated state against the FP64 oracle on the
     Dormand-Prince chain (every trial accepted): the FP32 fast path's bar."""
@@ -362,3 +388,33 @@ def test_orbitrap_radial_waveform_rk45(oracle):
     gpu_rk45(w, ph, g1, 1, w["dt"])
     oracle_rk45(oracle, w, ph, o1, 1, w["dt"])
     assert match(g1, o1, rtol=1e-8).all()
+
+
+def test_rk45_constant_waveforms_equal_the_constants():
+    """Every slot as a CONSTANT waveform with the domain's own value: the
+    kernel's waveform path gives the constant-voltage run bit for bit (same
+    E_z, same RF coefficient), adaptive steps included."""
+    w = W.quadrupole_workload(n=1024, U=10.0, seed=9)
+    d = w["domain"]
+    c = dict(w)
+    w = dict(w, waveforms={
+        _abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_CONSTANT, (d.dc_axial_V,)),
+        _abi.SLOT_DC_QUAD: _abi.waveform(_abi.WAVEFORM_CONSTANT, (d.dc_quad_V,)),
+        _abi.SLOT_RF_VOLTAGE: _abi.waveform(_abi.WAVEFORM_CONSTANT, (d.rf_voltage_V,)),
+        _abi.SLOT_RF_FREQUENCY: _abi.waveform(_abi.WAVEFORM_CONSTANT, (d.rf_frequency_Hz,)),
+    })
+    ph = rk45_physics(w, collision=_abi.COLLISION_NONE, ou=0)
+    g, k = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_rk45(w, ph, g, 60, 2e-8)
+    gpu_rk45(c, ph, k, 60, 2e-8)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "time_s", "dt_s", "active"):
+        np.testing.assert_array_equal(getattr(g, f), getattr(k, f), err_msg=f)
+
+
+def test_rk45_waveforms_on_a_field_array_are_unsupported():
+    from icarion_amd import gpu
+    w, ph, dt = case_workload("field_array_hss")
+    w = dict(w, waveforms={_abi.SLOT_DC_AXIAL: _abi.waveform(_abi.WAVEFORM_CONSTANT, (1.0,))})
+    with pytest.raises(gpu.IcarionError) as ei:
+        gpu_rk45(w, ph, w["ensemble"].copy(), 1, dt)
+    assert ei.value.code == _abi.ERR_UNSUPPORTED
