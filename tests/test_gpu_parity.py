@@ -615,9 +615,8 @@ def test_waveforms_constant_equals_constant_and_limits():
         h.set_species(w["species"])
         h.set_waveforms(quadrupole_waveforms())
         h.upload_state(w["ensemble"].copy())
-        with pytest.raises(gpu.IcarionError) as ei:
-            h.step(0.0, 1e-9, 5)
-        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        st = h.step(0.0, 1e-9, 5)   # RK45 evaluates the waveforms in the kernel
+        assert st.ion_steps > 0
         with pytest.raises(gpu.IcarionError) as ei:
             h.set_waveforms({_abi.SLOT_DC_AXIAL: _abi.waveform(
                 _abi.WAVEFORM_ARBITRARY, (), 1, 0, [(1e-6, 1.0), (0.5e-6, 2.0)])})
