@@ -1268,7 +1268,14 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         fail(ICARION_ERR_UNSUPPORTED, "several domains run with RK4 or Boris");
 
     const int coll = coll_kind(h->phys);
-    if (coll == COLL_EHSS) fail(ICARION_ERR_UNSUPPORTED, "several domains: EHSS is not covered");
+    if (coll == COLL_EHSS) {
+        if (h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
+            fail(ICARION_ERR_UNSUPPORTED, "several domains: the EHSS surrogate is not covered");
+        if (h->phys.precision != ICARION_PRECISION_FP64)
+            fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
+        if (h->off_species || h->ori_species)
+            fail(ICARION_ERR_UNSUPPORTED, "several domains: offline EHSS sets and orientation samples are not covered");
+    }
     if (!h->mix.empty()) fail(ICARION_ERR_UNSUPPORTED, "several domains: gas mixtures are not covered");
     if (h->has_waves)
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their waveforms per domain (icarion_gpu_set_domain_waveforms)");
@@ -1292,6 +1299,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             P = derive_params(h->doms[d], h->phys);
             P.dt = dt;
             P.species = h->species.p;
+            P.atoms = h->atoms_f32.p;
             P.n_ions = static_cast<uint32_t>(h->n);
             // the domain's own waveforms, as a single domain's (AnalyticalFieldModel.cpp:130-248)
             const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];

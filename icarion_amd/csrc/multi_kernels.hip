@@ -420,6 +420,10 @@ static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPt
             if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
             else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
             break;
+        case COLL_EHSS:   // geometry EHSS with each domain's gas, FP64 only
+            if constexpr (F32) return hipErrorInvalidValue;
+            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_EHSS>), grid, dim3(block), 0, s, M, S);
+            break;
         default:
             return hipErrorInvalidValue;
     }

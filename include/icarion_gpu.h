@@ -521,7 +521,7 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
  * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
  * voltages or per-domain waveforms (icarion_gpu_set_domain_waveforms), RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
  * handoff in FP64 on the compensated position), no / HSS / Langevin /
- * Friction collisions, one
+ * Friction collisions and geometry EHSS (FP64, online), one
  * species table for all domains; icarion_gpu_step, icarion_gpu_step_batch and
  * icarion_gpu_step_snapshots (with each ion's domain index).  Other
  * combinations return ICARION_ERR_UNSUPPORTED when stepped. */

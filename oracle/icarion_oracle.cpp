@@ -1310,7 +1310,7 @@ struct Domains {
 // domain found at the step start carries the push and the collision; after
 // the push the lookup, the bridge and the handoff decide life and the index.
 uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const icarion_species_desc& sp,
-                       IonCtx& ion, int& dom_idx, const double* t_table, double dt, uint64_t n_steps,
+                       const double* atoms, IonCtx& ion, int& dom_idx, const double* t_table, double dt, uint64_t n_steps,
                        uint32_t ion_index, uint64_t step0, uint64_t& collisions, std::vector<V3>& scratch) {
     uint64_t pushed = 0;
     const IonTables tb;
@@ -1338,7 +1338,7 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
         ++pushed;
         if (ph.collision_model != ICARION_COLLISION_NONE) {
             PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
-            if (collide(d, ph, sp, nullptr, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
+            if (collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
                 ++ion.ncoll;
                 ++collisions;
             }
@@ -1605,7 +1605,8 @@ int oracle_find_domain(const icarion_domain_desc* domains, const double* end_ape
 }
 
 int oracle_run_multi(const icarion_domain_desc* domains, const double* end_aperture_m, uint32_t n_domains,
-                     const oracle_field* const* fields, const icarion_physics_desc* physics, const icarion_species_desc* species, uint32_t n_species,
+                     const oracle_field* const* fields, const icarion_physics_desc* physics,
+                     const icarion_species_desc* species, uint32_t n_species, const double* atoms,
                      icarion_ion_soa* ions, int32_t* domain_index, double t0, double dt, uint64_t n_steps,
                      uint64_t step_offset, int n_threads, icarion_step_stats* stats) {
     if (!domains || n_domains == 0 || !physics || !species || !ions || !domain_index || n_species == 0)
@@ -1646,7 +1647,7 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
             ion.ncoll = ions->n_collisions ? ions->n_collisions[i] : 0;
             int dom = domain_index[i];
             uint64_t coll = 0;
-            const uint64_t pushed = run_ion_multi(D, *physics, sp, ion, dom, t_table.data(), dt, n_steps,
+            const uint64_t pushed = run_ion_multi(D, *physics, sp, atoms, ion, dom, t_table.data(), dt, n_steps,
                                                   static_cast<uint32_t>(i), step_offset, coll, scratch);
             ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
             ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;

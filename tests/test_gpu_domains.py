@@ -37,7 +37,7 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
             for k, slots in enumerate(w.get("waveforms") or []):
                 if slots:
                     h.set_domain_waveforms(k, slots)
-            h.set_species(w["species"])
+            h.set_species(w["species"], w.get("atoms"))
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -52,7 +52,7 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
                  apertures=None, dom=None):
     di = np.zeros(ens.n, np.int32) if dom is None else dom
     st = oracle.run_multi(domains or w["domains"], physics or w["physics"], w["species"], ens, di,
-                          t0, w["dt"], n_steps, step_offset=step_offset,
+                          t0, w["dt"], n_steps, step_offset=step_offset, atoms=w.get("atoms"),
                           end_apertures=apertures if apertures is not None else w.get("apertures"),
                           waveforms=w.get("waveforms"))
     return st, di
@@ -128,6 +128,29 @@ def test_per_domain_waveforms_ion_by_ion(oracle, integrator, collision):
     np.testing.assert_array_equal(e.z, c.z)
 
 
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_ehss_three_domains_ion_by_ion(oracle, integrator):
+    """Geometry EHSS over three domains (N2, He, N2): each ion collides with
+    its step domain's gas (mass, density, temperature, contact radius), as the
+    oracle's multi-domain loop does with the same clusters (1, 5 and 23 atoms,
+    FP32 coordinates on both sides)."""
+    from test_gpu_parity import ehss_workload
+    e = ehss_workload(n=2048, seed=62)
+    w = W.multi_domain_workload(n=2048, seed=62, integrator=integrator, collision=_abi.COLLISION_EHSS)
+    w["species"], w["atoms"] = e["species"], e["atoms"]
+    w["ensemble"].species[:] = e["ensemble"].species
+    print(f"domains EHSS integ={integrator}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9) & (dg == do) & (g.death_time_s == o.death_time_s)
+    assert ok.mean() >= 0.99, ok.mean()
+    assert so.collisions > 2 * g.n
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    # collisions happen in every gas: ions that reached the last region collided there
+    assert len(set(do[o.active == 1].tolist())) >= 2 or len(set(do.tolist())) >= 3
+
+
 def test_bridge_and_handoff_cases_match_the_oracle(oracle):
     w = {"species": [W.species()], "dt": 1e-9}
     ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE)
@@ -187,7 +210,9 @@ def test_unsupported_combinations():
     p = w["physics"]
     cases = [
         W.physics(_abi.INTEGRATOR_RK45, p.collision_model),
-        W.physics(p.integrator, _abi.COLLISION_EHSS),
+        W.physics(p.integrator, _abi.COLLISION_EHSS, precision=_abi.PRECISION_FP32_KAHAN),
+        W.physics(p.integrator, _abi.COLLISION_EHSS, ehss_mode=_abi.EHSS_SURROGATE,
+                  precision=_abi.PRECISION_FP32_KAHAN),
     ]
     for ph in cases:
         with gpu.Handle() as h:
