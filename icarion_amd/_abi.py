@@ -175,6 +175,8 @@ class CloudDesc(C.Structure):
         ("species", C.POINTER(SpeciesDesc)), ("n_species", _u32), ("reserved0", _u32),
         ("ions", C.POINTER(IonSoA)), ("t0_s", _d), ("dt_s", _d), ("n_steps", _u64),
         ("step_offset", _u64), ("waveforms", C.POINTER(C.POINTER(WaveformDesc))),
+        ("mixture", C.POINTER(GasComponent)), ("mixture_sigma_m2", C.POINTER(_d)),
+        ("n_components", _u32), ("reserved1", _u32),
     ]
 
 

@@ -261,6 +261,7 @@ struct icarion_gpu {
     DevBuf<uint8_t> b_flags;
     DevBuf<SpeciesDev> b_species;
     DevBuf<StepEntry> b_tables;
+    DevBuf<double> b_mix_sigma;
     icarion_batch_tuning b_tuned[2] = {{0, 0}, {0, 0}};
     hipStream_t side[kBatchClasses - 1] = {nullptr, nullptr};   // streams of the other classes
     hipEvent_t ev_fork = nullptr, ev_join[kBatchClasses - 1] = {nullptr, nullptr};
@@ -542,6 +543,49 @@ void apply_field(const icarion_gpu* h, StepParams& P) {
     }
 }
 
+// The components HSSCollisionHandler's loop reaches (participating, n > 0;
+// HSSCollisionHandler.cpp:128-134) and their cross sections per species
+// (sig[s * mix.size() + j]); with none, one zero-density component, which
+// gives the kernels the handler's k_total = 0.
+void build_hss_mixture(const icarion_gas_component* components, uint32_t n_components, const double* sigma_m2,
+                       uint32_t n_species, std::vector<icarion_gas_component>& mix, std::vector<double>& sig) {
+    if (n_components > static_cast<uint32_t>(kMaxGasComponents))
+        fail(ICARION_ERR_UNSUPPORTED, "at most " + std::to_string(kMaxGasComponents) + " gas components");
+    std::vector<uint32_t> keep;
+    for (uint32_t c = 0; c < n_components; ++c) {
+        const icarion_gas_component& g = components[c];
+        if (!std::isfinite(g.density_m3) || !std::isfinite(g.mass_kg))
+            fail(ICARION_ERR_INVALID, "gas component values must be finite");
+        if (!g.participates || g.density_m3 <= 0.0) continue;
+        if (!(g.mass_kg > 0.0)) fail(ICARION_ERR_INVALID, "gas component mass must be > 0");
+        for (uint32_t sp = 0; sp < n_species; ++sp)
+            if (!(sigma_m2[static_cast<size_t>(sp) * n_components + c] > 0.0))
+                fail(ICARION_ERR_INVALID, "missing cross section for species " + std::to_string(sp) +
+                                              " and gas component " + std::to_string(c));
+        keep.push_back(c);
+    }
+    sig.assign(static_cast<size_t>(n_species) * std::max<size_t>(keep.size(), 1), 0.0);
+    for (uint32_t sp = 0; sp < n_species; ++sp)
+        for (size_t j = 0; j < keep.size(); ++j)
+            sig[sp * keep.size() + j] = sigma_m2[static_cast<size_t>(sp) * n_components + keep[j]];
+    mix.clear();
+    for (uint32_t c : keep) mix.push_back(components[c]);
+    if (mix.empty())
+        mix.push_back(icarion_gas_component{0.0, components[0].mass_kg > 0.0 ? components[0].mass_kg : 1.0, 1, 0});
+}
+
+// The HSS mixture's per-component values into the launch parameters.
+void mixture_params(const std::vector<icarion_gas_component>& mix, double temperature_K, StepParams& P) {
+    P.n_mix = static_cast<int>(mix.size());
+    for (int c = 0; c < kMaxGasComponents; ++c) {
+        const bool on = c < P.n_mix;
+        P.mix_n[c] = on ? mix[c].density_m3 : 0.0;
+        P.mix_m[c] = on ? mix[c].mass_kg : 0.0;
+        // VelocitySampling::sample_neutral_velocity's width (VelocitySampling.cpp:17)
+        P.mix_sig[c] = on ? std::sqrt(kBoltzmannHost * temperature_K / mix[c].mass_kg) : 0.0;
+    }
+}
+
 // The gas mixture into the launch parameters (neutral widths at the domain's
 // temperature); an error when the species table changed size since.
 void apply_mixture(const icarion_gpu* h, StepParams& P) {
@@ -569,13 +613,7 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
         P.ehss_ori_R = h->ori_R.p;
         P.ehss_ori_area = h->ori_area.p;
     }
-    for (int c = 0; c < kMaxGasComponents; ++c) {
-        const bool on = c < P.n_mix;
-        P.mix_n[c] = on ? h->mix[c].density_m3 : 0.0;
-        P.mix_m[c] = on ? h->mix[c].mass_kg : 0.0;
-        // VelocitySampling::sample_neutral_velocity's width (VelocitySampling.cpp:17)
-        P.mix_sig[c] = on ? std::sqrt(kBoltzmannHost * h->dom.temperature_K / h->mix[c].mass_kg) : 0.0;
-    }
+    mixture_params(h->mix, h->dom.temperature_K, P);
     // EHSS: every component; contact radius = the component's, else the
     // domain's gas radius (EHSSCollisionHandler.cpp:348, 415)
     P.n_emix = static_cast<int>(h->mix_all.size());
@@ -1456,9 +1494,12 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
 
 // ---- batched clouds --------------------------------------------------------
 
+// kinds from here on are the gas-mixture ones (12-15), in their own kernel variant
+constexpr uint32_t kBatchFirstMixKind = 12;
+
 uint32_t batch_kind(const icarion_physics_desc& p, int field, int coll) {
     // the surrogate class has its own kernel: integrator + 2 * Orbitrap
-    const uint32_t c = coll == COLL_HSS ? 1u : (coll == COLL_DAMPING ? 2u : 0u);
+    const uint32_t c = coll == COLL_HSS ? 1u : (coll == COLL_DAMPING ? 2u : (coll == COLL_HSS_MIX ? 3u : 0u));
     return static_cast<uint32_t>(p.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
            2u * (field == FIELD_ORBITRAP ? 1u : 0u) + 4u * c;
 }
@@ -1472,6 +1513,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     std::vector<uint32_t> species_off(n_clouds);
     std::vector<StepEntry> tables;
     std::vector<size_t> table_off(n_clouds);
+    std::vector<double> mix_sigma;           // the clouds' mixture cross sections
+    std::vector<size_t> mix_off(n_clouds);
     // identical step tables are shared: (t0, dt, n, RF inputs, axial field, stage layout)
     // E_z rides in the table too (e0 / eh / e1), so it is part of the key;
     // clouds with waveforms get tables of their own
@@ -1499,6 +1542,10 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         if (surrogate && cd.physics.precision != ICARION_PRECISION_FP32_KAHAN)
             fail(ICARION_ERR_UNSUPPORTED, tag + "the EHSS surrogate runs in the FP32 + Kahan path");
         if (field == FIELD_ARRAY) fail(ICARION_ERR_UNSUPPORTED, tag + "field arrays are not covered by the batch");
+        const bool mixture = cd.n_components > 0;
+        if (mixture && (!cd.mixture || !cd.mixture_sigma_m2)) fail(ICARION_ERR_INVALID, tag + "mixture arrays are NULL");
+        if (mixture && coll != COLL_HSS)
+            fail(ICARION_ERR_UNSUPPORTED, tag + "a gas mixture is covered by the batch with HSS collisions");
         if (!(cd.dt_s > 0.0) || !std::isfinite(cd.dt_s) || !std::isfinite(cd.t0_s))
             fail(ICARION_ERR_INVALID, tag + "t0 must be finite and dt finite and > 0");
         if (cd.n_steps >= 0xFFFFFFFFull) fail(ICARION_ERR_INVALID, tag + "n_steps must be < 2^32");
@@ -1519,7 +1566,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         bc[c].cls = surrogate ? 2 : (cd.physics.precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0);
         bc[c].kind = surrogate ? static_cast<uint32_t>(cd.physics.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
                                      2u * (field == FIELD_ORBITRAP ? 1u : 0u)
-                               : batch_kind(cd.physics, field, coll);
+                               : batch_kind(cd.physics, field, mixture ? COLL_HSS_MIX : coll);
         bc[c].n_steps = cd.n_steps;
         bc[c].has_birth = io->birth_time_s != nullptr;
         total += n;
@@ -1534,6 +1581,15 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
             species.push_back(sd);
         }
         StepParams P = derive_params(cd.domain, cd.physics);
+        mix_off[c] = SIZE_MAX;
+        if (mixture) {
+            std::vector<icarion_gas_component> mix;
+            std::vector<double> sig;
+            build_hss_mixture(cd.mixture, cd.n_components, cd.mixture_sigma_m2, cd.n_species, mix, sig);
+            mixture_params(mix, cd.domain.temperature_K, P);
+            mix_off[c] = mix_sigma.size();
+            mix_sigma.insert(mix_sigma.end(), sig.begin(), sig.end());
+        }
         // the cloud's voltage waveforms, as icarion_gpu_set_waveforms on a handle of its own
         std::unique_ptr<HostWaveform> wv[ICARION_N_VOLTAGE_SLOTS];
         const bool waves = parse_waveform_slots(cd.waveforms, wv);
@@ -1607,6 +1663,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     copy_in(h->b_ncoll, hn.data(), total, s, "batch ncoll"); copy_in(h->b_flags, hf.data(), total, s, "batch flags");
     copy_in(h->b_species, species.data(), species.size(), s, "batch species table");
     copy_in(h->b_tables, tables.data(), std::max<size_t>(tables.size(), 1), s, "batch step tables");
+    h->b_mix_sigma.resize(std::max<size_t>(mix_sigma.size(), 1));
+    if (!mix_sigma.empty()) copy_in(h->b_mix_sigma, mix_sigma.data(), mix_sigma.size(), s, "batch mixture cross sections");
     h->b_counters.resize(4ull * n_clouds);
     h->b_done.resize(n_clouds);
     h->b_start.resize(1);
@@ -1624,6 +1682,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         StepParams& P = params[c];
         P.species = h->b_species.p + species_off[c];
         P.table = h->b_tables.p + table_off[c];
+        P.mix_sigma = mix_off[c] == SIZE_MAX ? nullptr : h->b_mix_sigma.p + mix_off[c];
         P.counters = h->b_counters.p + 4ull * c;
         P.atoms = nullptr;
         P.fa_f32 = nullptr;
@@ -1697,7 +1756,9 @@ float batch_launch(icarion_gpu* h, icarion_batch_tuning t64, icarion_batch_tunin
         B[cls].kinds = h->b_kinds_d.p;
         B[cls].tiles = h->b_tiles[cls].p;
         B[cls].n_tiles = (mask >> cls) & 1 ? h->b_n_tiles[cls] : 0;
-        B[cls].pad = 0;
+        B[cls].with_mix = 0;
+        for (const auto& c : h->b_clouds)
+            if (c.cls == cls && c.kind >= kBatchFirstMixKind) B[cls].with_mix = 1;
         B[cls].queue = h->b_queue.p + cls;
         B[cls].done = h->b_done.p;
         B[cls].start = h->b_start.p;
@@ -2374,37 +2435,12 @@ int icarion_gpu_set_gas_mixture(icarion_gpu* h, const icarion_gas_component* com
             return;
         }
         if (!components || !sigma_m2) fail(ICARION_ERR_INVALID, "NULL argument");
-        if (n_components > static_cast<uint32_t>(kMaxGasComponents))
-            fail(ICARION_ERR_UNSUPPORTED, "at most " + std::to_string(kMaxGasComponents) + " gas components");
         if (!h->have_species || n_species != h->n_species)
             fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
-        // the components the handler's loop reaches (HSSCollisionHandler.cpp:128-134)
-        std::vector<uint32_t> keep;
-        for (uint32_t c = 0; c < n_components; ++c) {
-            const icarion_gas_component& g = components[c];
-            if (!std::isfinite(g.density_m3) || !std::isfinite(g.mass_kg))
-                fail(ICARION_ERR_INVALID, "gas component values must be finite");
-            if (!g.participates || g.density_m3 <= 0.0) continue;
-            if (!(g.mass_kg > 0.0)) fail(ICARION_ERR_INVALID, "gas component mass must be > 0");
-            for (uint32_t sp = 0; sp < n_species; ++sp)
-                if (!(sigma_m2[static_cast<size_t>(sp) * n_components + c] > 0.0))
-                    fail(ICARION_ERR_INVALID, "missing cross section for species " + std::to_string(sp) +
-                                                  " and gas component " + std::to_string(c));
-            keep.push_back(c);
-        }
-        std::vector<double> sig(static_cast<size_t>(n_species) * std::max<size_t>(keep.size(), 1), 0.0);
-        for (uint32_t sp = 0; sp < n_species; ++sp)
-            for (size_t j = 0; j < keep.size(); ++j)
-                sig[sp * keep.size() + j] = sigma_m2[static_cast<size_t>(sp) * n_components + keep[j]];
-        set_device(h);
         std::vector<icarion_gas_component> mix;
-        for (uint32_t c : keep) mix.push_back(components[c]);
-        if (mix.empty()) {
-            // no component can collide: the handler finds k_total = 0 and
-            // returns; one zero-density component gives the kernels the same
-            mix.push_back(icarion_gas_component{0.0, components[0].mass_kg > 0.0 ? components[0].mass_kg : 1.0,
-                                                1, 0});
-        }
+        std::vector<double> sig;
+        build_hss_mixture(components, n_components, sigma_m2, n_species, mix, sig);
+        set_device(h);
         copy_in(h->mix_sigma, sig.data(), sig.size(), h->stream, "upload mixture cross sections");
         copy_in(h->emix_sigma, sigma_m2, static_cast<size_t>(n_species) * n_components, h->stream,
                 "upload mixture cross sections");

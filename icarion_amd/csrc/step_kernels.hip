@@ -526,7 +526,10 @@ __device__ __forceinline__ void batch_ion(const StepParams& P, const IonPtrs& S,
     }
 }
 
-template <int PREC>
+// MIX: the kernel also holds the gas-mixture kinds (12-15).  Their body needs
+// more registers, so a batch without mixture clouds runs the variant without
+// them (FP32: 90 VGPRs, 5 waves per SIMD, against 110 and 4).
+template <int PREC, bool MIX>
 __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& P, const IonPtrs& S,
                                                uint32_t i, uint32_t& pushed, uint32_t& nc) {
     switch (kind) {
@@ -542,7 +545,17 @@ __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& 
         case 9: batch_ion<PREC, 1, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
         case 10: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
         case 11: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
-        default: break;
+        default:
+            if constexpr (MIX) {
+                switch (kind) {
+                    case 12: batch_ion<PREC, 0, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 13: batch_ion<PREC, 1, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 14: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 15: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    default: break;
+                }
+            }
+            break;
     }
 }
 
@@ -583,7 +596,7 @@ __device__ __forceinline__ void batch_queue_loop(const BatchTile* __restrict__ t
 
 // The per-cloud tables come in as separate read-only, non-aliased arguments so
 // that the wave-uniform parameter loads stay scalar.
-template <int PREC>
+template <int PREC, bool MIX>
 __global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict__ params,
                                                     const IonPtrs* __restrict__ ions,
                                                     const uint32_t* __restrict__ kinds,
@@ -596,7 +609,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict
         const uint32_t kind = __builtin_amdgcn_readfirstlane(kinds[c]);
         const uint32_t i = first + threadIdx.x;
         uint32_t pushed = 0, nc = 0;
-        if (i < P.n_ions) batch_dispatch<PREC>(kind, P, S, i, pushed, nc);
+        if (i < P.n_ions) batch_dispatch<PREC, MIX>(kind, P, S, i, pushed, nc);
         add_counters(P, pushed, nc);
     });
 }
@@ -669,12 +682,18 @@ hipError_t launch_batch(const BatchParams* B, const uint32_t* block, const uint3
         hipStream_t st = k == 0 ? s : side[k - 1];
         if (k > 0 && (e = hipStreamWaitEvent(st, fork, 0)) != hipSuccess) return e;
         const BatchParams& b = B[c];
-        if (c == 0)
-            hipLaunchKernelGGL(batch_kernel<0>, dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions, b.kinds,
-                               b.tiles, b.n_tiles, b.queue, b.done, b.start);
+        if (c == 0 && b.with_mix)
+            hipLaunchKernelGGL((batch_kernel<0, true>), dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions,
+                               b.kinds, b.tiles, b.n_tiles, b.queue, b.done, b.start);
+        else if (c == 0)
+            hipLaunchKernelGGL((batch_kernel<0, false>), dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions,
+                               b.kinds, b.tiles, b.n_tiles, b.queue, b.done, b.start);
+        else if (c == 1 && b.with_mix)
+            hipLaunchKernelGGL((batch_kernel<1, true>), dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions,
+                               b.kinds, b.tiles, b.n_tiles, b.queue, b.done, b.start);
         else if (c == 1)
-            hipLaunchKernelGGL(batch_kernel<1>, dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions, b.kinds,
-                               b.tiles, b.n_tiles, b.queue, b.done, b.start);
+            hipLaunchKernelGGL((batch_kernel<1, false>), dim3(grid[c]), dim3(block[c]), 0, st, b.params, b.ions,
+                               b.kinds, b.tiles, b.n_tiles, b.queue, b.done, b.start);
         else
             hipLaunchKernelGGL(batch_surrogate_kernel, dim3(grid[c]), dim3(256), 0, st, b.params, b.ions,
                                b.kinds, b.tiles, b.n_tiles, b.queue, b.done, b.start, blob);

@@ -288,7 +288,7 @@ struct BatchParams {
     const uint32_t *kinds;          // per cloud
     const BatchTile *tiles;         // in the order the queue hands them out
     uint32_t n_tiles;
-    uint32_t pad;
+    uint32_t with_mix;              // some cloud of the class has an HSS gas mixture
     uint32_t *queue;                // next tile to hand out
     unsigned long long *done;       // per cloud: clock at the end of its last tile
     unsigned long long *start;      // clock at the first workgroup's start

@@ -186,7 +186,8 @@ class Handle:
     # ---- batched clouds ------------------------------------------------------
     def _cloud_array(self, clouds):
         """clouds: list of dicts with domain, physics, species (list), ensemble,
-        t0, dt, n_steps and optional step_offset and waveforms ({slot: desc})."""
+        t0, dt, n_steps and optional step_offset, waveforms ({slot: desc}) and
+        mixture ((components, sigma) as set_gas_mixture takes them)."""
         arr = (_abi.CloudDesc * len(clouds))()
         keep = []
         for k, c in enumerate(clouds):
@@ -208,6 +209,16 @@ class Handle:
                     wa[slot] = C.pointer(w)
                 keep += [wa, dict(c["waveforms"])]
                 arr[k].waveforms = C.cast(wa, C.POINTER(C.POINTER(_abi.WaveformDesc)))
+            if c.get("mixture"):
+                comps, sigma = c["mixture"]
+                ga = (_abi.GasComponent * len(comps))()
+                for i, (n, m, part) in enumerate(comps):
+                    ga[i] = _abi.GasComponent(n, m, int(part), 0)
+                sg = np.ascontiguousarray(sigma, dtype=np.float64)
+                keep += [ga, sg]
+                arr[k].mixture = C.cast(ga, C.POINTER(_abi.GasComponent))
+                arr[k].mixture_sigma_m2 = sg.ctypes.data_as(C.POINTER(C.c_double))
+                arr[k].n_components = len(comps)
         return arr, keep
 
     def batch_upload(self, clouds):

@@ -661,6 +661,14 @@ typedef struct icarion_cloud_desc {
                                      (NULL = the domain's constant): the cloud's
                                      voltage waveforms, as icarion_gpu_set_waveforms
                                      on a handle of its own (analytic fields)  */
+    const icarion_gas_component *mixture;
+                                  /* NULL, or n_components components: the
+                                     cloud's HSS gas mixture, as
+                                     icarion_gpu_set_gas_mixture on a handle of
+                                     its own                                   */
+    const double *mixture_sigma_m2; /* [n_species][n_components]               */
+    uint32_t n_components;
+    uint32_t reserved1;
 } icarion_cloud_desc;
 
 /* Launch shape of the persistent batch kernel (per precision class). */

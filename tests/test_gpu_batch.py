@@ -69,6 +69,8 @@ def single_runs(clouds, params=None):
                 h.surrogate_set_params(params)
             if c.get("waveforms"):
                 h.set_waveforms(c["waveforms"])
+            if c.get("mixture"):
+                h.set_gas_mixture(*c["mixture"])
             h.upload_state(c["ensemble"])
             stats.append(h.step(c["t0"], c["dt"], c["n_steps"], c["step_offset"]))
             h.download_state(c["ensemble"])
@@ -191,6 +193,42 @@ def test_batch_waveform_clouds_equal_one_handle_per_cloud():
     # the ramped IMS cloud and the plain one share engine times but not fields
     a, b = bat[0]["ensemble"], bat[3]["ensemble"]
     assert (a.z != b.z).mean() > 0.9
+
+
+def test_batch_gas_mixture_clouds_equal_one_handle_per_cloud():
+    """HSS gas-mixture clouds (IMS in a three-gas mixture, RK4 and Boris, FP64
+    and FP32 + Kahan) in one batch with plain clouds: each FP64 cloud equals a
+    handle of its own with the same mixture ion by ion (the 0.995 bar of the
+    stochastic clouds); each FP32 cloud agrees in its collision count and
+    drift, since single-precision roundings of two compiled kernels flip a few
+    of the ~500 collision decisions per ion.  The plain clouds run in the same
+    launch through the kernel variant that holds the mixture kinds."""
+    ref = []
+    for k, (integ, prec) in enumerate(((_abi.INTEGRATOR_RK4, _abi.PRECISION_FP64),
+                                       (_abi.INTEGRATOR_BORIS, _abi.PRECISION_FP64),
+                                       (_abi.INTEGRATOR_RK4, _abi.PRECISION_FP32_KAHAN))):
+        w = W.ims_mixture_workload(n=1500, seed=400 + k, precision=prec)
+        w["physics"] = W.physics(integ, _abi.COLLISION_HSS, precision=prec, seed=400 + k)
+        ref.append(dict(kind=f"mixture integ={integ} prec={prec}", domain=w["domain"], physics=w["physics"],
+                        species=w["species"], ensemble=w["ensemble"].copy(), t0=0.0, dt=w["dt"],
+                        n_steps=300, step_offset=0, mixture=w["mixture"]))
+    ref += mixed_clouds(seed=12)[:4]
+    bat = copy_clouds(ref)
+    s_ref = single_runs(ref)
+    s_bat, _, _ = batch_runs(bat)
+    for ca, cb, a, b in zip(bat, ref, s_bat, s_ref):
+        fp64 = ca["physics"].precision == _abi.PRECISION_FP64
+        stochastic = ca["physics"].collision_model != _abi.COLLISION_NONE
+        if fp64 or not stochastic:
+            frac = close_fraction(ca["ensemble"], cb["ensemble"], rtol_of(ca))
+            print(ca["kind"], frac)
+            assert frac >= (0.995 if stochastic else 1.0), (ca["kind"], frac)
+        else:
+            assert a.collisions == pytest.approx(b.collisions, rel=5e-3), ca["kind"]
+            za, zb = ca["ensemble"].z.mean(), cb["ensemble"].z.mean()
+            z0 = ca["ensemble"].z.std()
+            assert abs(za - zb) < 0.02 * z0, ca["kind"]
+    assert s_bat[0].collisions > 100 * 1500
 
 
 @pytest.mark.parametrize("shape", [(64, 1), (128, 8), (256, 16)])
