@@ -176,7 +176,7 @@ class CloudDesc(C.Structure):
         ("ions", C.POINTER(IonSoA)), ("t0_s", _d), ("dt_s", _d), ("n_steps", _u64),
         ("step_offset", _u64), ("waveforms", C.POINTER(C.POINTER(WaveformDesc))),
         ("mixture", C.POINTER(GasComponent)), ("mixture_sigma_m2", C.POINTER(_d)),
-        ("n_components", _u32), ("reserved1", _u32),
+        ("n_components", _u32), ("n_atoms", _u32), ("atoms_xyzr", C.POINTER(C.c_float)),
     ]
 
 

@@ -262,6 +262,7 @@ struct icarion_gpu {
     DevBuf<SpeciesDev> b_species;
     DevBuf<StepEntry> b_tables;
     DevBuf<double> b_mix_sigma;
+    DevBuf<float4> b_atoms;
     icarion_batch_tuning b_tuned[2] = {{0, 0}, {0, 0}};
     hipStream_t side[kBatchClasses - 1] = {nullptr, nullptr};   // streams of the other classes
     hipEvent_t ev_fork = nullptr, ev_join[kBatchClasses - 1] = {nullptr, nullptr};
@@ -1494,12 +1495,17 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
 
 // ---- batched clouds --------------------------------------------------------
 
-// kinds from here on are the gas-mixture ones (12-15), in their own kernel variant
+// kinds from here on (gas mixtures 12-15, geometry EHSS 16-19) are in their own
+// kernel variant
 constexpr uint32_t kBatchFirstMixKind = 12;
 
 uint32_t batch_kind(const icarion_physics_desc& p, int field, int coll) {
     // the surrogate class has its own kernel: integrator + 2 * Orbitrap
-    const uint32_t c = coll == COLL_HSS ? 1u : (coll == COLL_DAMPING ? 2u : (coll == COLL_HSS_MIX ? 3u : 0u));
+    const uint32_t c = coll == COLL_HSS       ? 1u
+                       : coll == COLL_DAMPING ? 2u
+                       : coll == COLL_HSS_MIX ? 3u
+                       : coll == COLL_EHSS    ? 4u
+                                              : 0u;
     return static_cast<uint32_t>(p.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
            2u * (field == FIELD_ORBITRAP ? 1u : 0u) + 4u * c;
 }
@@ -1515,6 +1521,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     std::vector<size_t> table_off(n_clouds);
     std::vector<double> mix_sigma;           // the clouds' mixture cross sections
     std::vector<size_t> mix_off(n_clouds);
+    std::vector<float4> atoms_all;           // the clouds' EHSS atom tables
+    std::vector<size_t> atom_off(n_clouds);
     // identical step tables are shared: (t0, dt, n, RF inputs, axial field, stage layout)
     // E_z rides in the table too (e0 / eh / e1), so it is part of the key;
     // clouds with waveforms get tables of their own
@@ -1537,8 +1545,10 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         const int coll = coll_kind(cd.physics);
         const int field = field_kind(cd.domain);
         const bool surrogate = coll == COLL_EHSS && cd.physics.ehss_mode == ICARION_EHSS_SURROGATE;
-        if (coll == COLL_EHSS && !surrogate)
-            fail(ICARION_ERR_UNSUPPORTED, tag + "EHSS geometry collisions are not covered by the batch");
+        const bool ehss_geo = coll == COLL_EHSS && !surrogate;
+        if (ehss_geo && cd.physics.precision != ICARION_PRECISION_FP64)
+            fail(ICARION_ERR_UNSUPPORTED, tag + "EHSS geometry collisions run in FP64 only");
+        if (cd.n_atoms && !cd.atoms_xyzr) fail(ICARION_ERR_INVALID, tag + "atoms_xyzr is NULL");
         if (surrogate && cd.physics.precision != ICARION_PRECISION_FP32_KAHAN)
             fail(ICARION_ERR_UNSUPPORTED, tag + "the EHSS surrogate runs in the FP32 + Kahan path");
         if (field == FIELD_ARRAY) fail(ICARION_ERR_UNSUPPORTED, tag + "field arrays are not covered by the batch");
@@ -1567,6 +1577,10 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         bc[c].kind = surrogate ? static_cast<uint32_t>(cd.physics.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
                                      2u * (field == FIELD_ORBITRAP ? 1u : 0u)
                                : batch_kind(cd.physics, field, mixture ? COLL_HSS_MIX : coll);
+        atom_off[c] = atoms_all.size();
+        for (uint32_t a = 0; a < cd.n_atoms; ++a)
+            atoms_all.push_back(float4{cd.atoms_xyzr[4 * a], cd.atoms_xyzr[4 * a + 1], cd.atoms_xyzr[4 * a + 2],
+                                       cd.atoms_xyzr[4 * a + 3]});
         bc[c].n_steps = cd.n_steps;
         bc[c].has_birth = io->birth_time_s != nullptr;
         total += n;
@@ -1578,6 +1592,12 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
             sd.sigma_ehss = d.sigma_ehss_m2;
             sd.K0_cm2_Vs = d.K0_cm2_Vs;
             for (int f = 0; f < 3; ++f) sd.feat[f] = d.surrogate_features[f];
+            if (ehss_geo) {
+                if (static_cast<uint64_t>(d.atom_offset) + d.atom_count > cd.n_atoms)
+                    fail(ICARION_ERR_INVALID, tag + "species " + std::to_string(k) + " atoms exceed the atom table");
+                sd.atom_offset = d.atom_offset;
+                sd.atom_count = d.atom_count;
+            }
             species.push_back(sd);
         }
         StepParams P = derive_params(cd.domain, cd.physics);
@@ -1663,6 +1683,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     copy_in(h->b_ncoll, hn.data(), total, s, "batch ncoll"); copy_in(h->b_flags, hf.data(), total, s, "batch flags");
     copy_in(h->b_species, species.data(), species.size(), s, "batch species table");
     copy_in(h->b_tables, tables.data(), std::max<size_t>(tables.size(), 1), s, "batch step tables");
+    h->b_atoms.resize(std::max<size_t>(atoms_all.size(), 1));
+    if (!atoms_all.empty()) copy_in(h->b_atoms, atoms_all.data(), atoms_all.size(), s, "batch atoms");
     h->b_mix_sigma.resize(std::max<size_t>(mix_sigma.size(), 1));
     if (!mix_sigma.empty()) copy_in(h->b_mix_sigma, mix_sigma.data(), mix_sigma.size(), s, "batch mixture cross sections");
     h->b_counters.resize(4ull * n_clouds);
@@ -1684,7 +1706,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         P.table = h->b_tables.p + table_off[c];
         P.mix_sigma = mix_off[c] == SIZE_MAX ? nullptr : h->b_mix_sigma.p + mix_off[c];
         P.counters = h->b_counters.p + 4ull * c;
-        P.atoms = nullptr;
+        P.atoms = h->b_atoms.p + atom_off[c];
         P.fa_f32 = nullptr;
         P.fa_f64 = nullptr;
         P.mlp_blob = nullptr;

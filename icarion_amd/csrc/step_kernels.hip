@@ -526,9 +526,10 @@ __device__ __forceinline__ void batch_ion(const StepParams& P, const IonPtrs& S,
     }
 }
 
-// MIX: the kernel also holds the gas-mixture kinds (12-15).  Their body needs
-// more registers, so a batch without mixture clouds runs the variant without
-// them (FP32: 90 VGPRs, 5 waves per SIMD, against 110 and 4).
+// MIX: the kernel also holds the gas-mixture kinds (12-15) and, in FP64, the
+// geometry EHSS kinds (16-19).  Their bodies need more registers, so a batch
+// without such clouds runs the variant without them (FP32: 90 VGPRs, 5 waves
+// per SIMD, against 110 and 4).
 template <int PREC, bool MIX>
 __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& P, const IonPtrs& S,
                                                uint32_t i, uint32_t& pushed, uint32_t& nc) {
@@ -552,7 +553,17 @@ __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& 
                     case 13: batch_ion<PREC, 1, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
                     case 14: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
                     case 15: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
-                    default: break;
+                    default:
+                        if constexpr (PREC == 0) {
+                            switch (kind) {
+                                case 16: batch_ion<0, 0, FIELD_LINEAR, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 17: batch_ion<0, 1, FIELD_LINEAR, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 18: batch_ion<0, 0, FIELD_ORBITRAP, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 19: batch_ion<0, 1, FIELD_ORBITRAP, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                default: break;
+                            }
+                        }
+                        break;
                 }
             }
             break;

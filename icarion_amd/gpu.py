@@ -186,8 +186,9 @@ class Handle:
     # ---- batched clouds ------------------------------------------------------
     def _cloud_array(self, clouds):
         """clouds: list of dicts with domain, physics, species (list), ensemble,
-        t0, dt, n_steps and optional step_offset, waveforms ({slot: desc}) and
-        mixture ((components, sigma) as set_gas_mixture takes them)."""
+        t0, dt, n_steps and optional step_offset, waveforms ({slot: desc}),
+        mixture ((components, sigma) as set_gas_mixture takes them) and atoms
+        ((n, 4) x, y, z, r for geometry EHSS, held in FP32)."""
         arr = (_abi.CloudDesc * len(clouds))()
         keep = []
         for k, c in enumerate(clouds):
@@ -219,6 +220,11 @@ class Handle:
                 arr[k].mixture = C.cast(ga, C.POINTER(_abi.GasComponent))
                 arr[k].mixture_sigma_m2 = sg.ctypes.data_as(C.POINTER(C.c_double))
                 arr[k].n_components = len(comps)
+            if c.get("atoms") is not None and len(c["atoms"]):
+                at = np.ascontiguousarray(c["atoms"], dtype=np.float32).reshape(-1, 4)
+                keep.append(at)
+                arr[k].atoms_xyzr = at.ctypes.data_as(C.POINTER(C.c_float))
+                arr[k].n_atoms = at.shape[0]
         return arr, keep
 
     def batch_upload(self, clouds):

@@ -668,7 +668,11 @@ typedef struct icarion_cloud_desc {
                                      its own                                   */
     const double *mixture_sigma_m2; /* [n_species][n_components]               */
     uint32_t n_components;
-    uint32_t reserved1;
+    uint32_t n_atoms;             /* entries of atoms_xyzr                     */
+    const float *atoms_xyzr;      /* NULL, or [n_atoms][4] x, y, z, r (m, FP32):
+                                     the cloud's EHSS atom table, indexed by its
+                                     species' atom_offset / atom_count (geometry
+                                     EHSS, FP64 clouds)                        */
 } icarion_cloud_desc;
 
 /* Launch shape of the persistent batch kernel (per precision class). */

@@ -64,8 +64,9 @@ def single_runs(clouds, params=None):
     for c in clouds:
         with gpu.Handle() as h:
             h.set_domain(c["domain"], c["physics"])
-            h.set_species(c["species"])
-            if c["physics"].collision_model == _abi.COLLISION_EHSS:
+            h.set_species(c["species"], c.get("atoms"))
+            if (c["physics"].collision_model == _abi.COLLISION_EHSS
+                    and c["physics"].ehss_mode == _abi.EHSS_SURROGATE):
                 h.surrogate_set_params(params)
             if c.get("waveforms"):
                 h.set_waveforms(c["waveforms"])
@@ -231,6 +232,33 @@ def test_batch_gas_mixture_clouds_equal_one_handle_per_cloud():
     assert s_bat[0].collisions > 100 * 1500
 
 
+def test_batch_ehss_geometry_clouds_equal_one_handle_per_cloud():
+    """Geometry EHSS clouds (FP64, RK4 and Boris; 1-, 5- and 23-atom clusters
+    in each cloud's own atom table) in one batch with plain clouds: each equals
+    a handle of its own with the same species and atoms, ion by ion at the
+    0.995 bar of the stochastic clouds."""
+    from test_gpu_parity import ehss_workload
+    ref = []
+    for k, integ in enumerate((_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS)):
+        w = ehss_workload(n=1200, seed=500 + k)
+        w["physics"] = W.physics(integ, _abi.COLLISION_EHSS, seed=500 + k)
+        ref.append(dict(kind=f"ehss integ={integ}", domain=w["domain"], physics=w["physics"],
+                        species=w["species"], atoms=w["atoms"], ensemble=w["ensemble"].copy(), t0=0.0,
+                        dt=w["dt"], n_steps=400, step_offset=0))
+    ref += mixed_clouds(seed=13)[:3]
+    bat = copy_clouds(ref)
+    s_ref = single_runs(ref)
+    s_bat, _, _ = batch_runs(bat)
+    for ca, cb, a, b in zip(bat, ref, s_bat, s_ref):
+        stochastic = ca["physics"].collision_model != _abi.COLLISION_NONE
+        if ca["physics"].precision == _abi.PRECISION_FP64 or not stochastic:
+            frac = close_fraction(ca["ensemble"], cb["ensemble"], rtol_of(ca))
+            print(ca["kind"], frac)
+            assert frac >= (0.995 if stochastic else 1.0), (ca["kind"], frac)
+        assert a.collisions == pytest.approx(b.collisions, rel=5e-3, abs=2), ca["kind"]
+    assert s_bat[0].collisions > 1200 * 50
+
+
 @pytest.mark.parametrize("shape", [(64, 1), (128, 8), (256, 16)])
 def test_batch_result_does_not_depend_on_the_launch_shape(shape):
     base = mixed_clouds(seed=9)
@@ -260,7 +288,8 @@ def test_batch_rejects_what_it_does_not_cover():
         with pytest.raises(gpu.IcarionError) as ei:
             h.batch_upload(bad)
         assert ei.value.code == _abi.ERR_UNSUPPORTED
-        bad[0]["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS)
+        bad[0]["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_EHSS,
+                                      precision=_abi.PRECISION_FP32_KAHAN)
         with pytest.raises(gpu.IcarionError) as ei:
             h.batch_upload(bad)
         assert ei.value.code == _abi.ERR_UNSUPPORTED
