@@ -184,6 +184,12 @@ struct icarion_gpu {
     bool has_waves = false;
     // several domains: each domain's own voltage waveforms (n_dom sets)
     std::vector<std::array<std::unique_ptr<HostWaveform>, ICARION_N_VOLTAGE_SLOTS>> dom_wave;
+    // several domains: each domain's HSS gas mixture (filtered components, the
+    // cross sections per species, the species count given)
+    std::vector<std::vector<icarion_gas_component>> dom_mix;
+    std::vector<std::vector<double>> dom_mix_sig;
+    std::vector<uint32_t> dom_mix_ns;
+    DevBuf<double> dom_mix_d;
     // RK45 evaluates the waveforms in the kernel: their descriptors and points
     DevBuf<icarion_waveform_desc> wave_desc;
     DevBuf<double> wave_pts;
@@ -1315,7 +1321,17 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         if (h->off_species || h->ori_species)
             fail(ICARION_ERR_UNSUPPORTED, "several domains: offline EHSS sets and orientation samples are not covered");
     }
-    if (!h->mix.empty()) fail(ICARION_ERR_UNSUPPORTED, "several domains: gas mixtures are not covered");
+    if (!h->mix.empty())
+        fail(ICARION_ERR_UNSUPPORTED, "several domains take their gas mixtures per domain (icarion_gpu_set_domain_gas_mixture)");
+    bool dom_mixture = false;
+    for (uint32_t d = 0; d < h->n_dom; ++d) {
+        if (h->dom_mix[d].empty()) continue;
+        dom_mixture = true;
+        if (h->dom_mix_ns[d] != h->n_species)
+            fail(ICARION_ERR_STATE, "a domain's gas mixture species count differs from the species table");
+    }
+    if (dom_mixture && coll != COLL_HSS)
+        fail(ICARION_ERR_UNSUPPORTED, "per-domain gas mixtures run with HSS collisions");
     if (h->has_waves)
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their waveforms per domain (icarion_gpu_set_domain_waveforms)");
     for (const auto& d : h->doms)
@@ -1331,6 +1347,15 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         deliver_empty_snapshots(h->doms[0], h->phys.integrator, t0, dt, n_steps, step_offset, *snap);
     if (n_steps > 0 && h->n > 0) {
         std::vector<StepParams> Ps(nd);
+        // the domains' mixture cross sections, concatenated
+        std::vector<double> msig;
+        std::vector<size_t> moff(nd, 0);
+        for (uint32_t d = 0; d < nd; ++d) {
+            moff[d] = msig.size();
+            msig.insert(msig.end(), h->dom_mix_sig[d].begin(), h->dom_mix_sig[d].end());
+        }
+        h->dom_mix_d.resize(std::max<size_t>(msig.size(), 1));
+        if (!msig.empty()) copy_in(h->dom_mix_d, msig.data(), msig.size(), h->stream, "domain mixture cross sections");
         std::vector<StepEntry> tabs;
         tabs.reserve(size_t(nd) * n_steps);
         for (uint32_t d = 0; d < nd; ++d) {
@@ -1340,6 +1365,12 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             P.species = h->species.p;
             P.atoms = h->atoms_f32.p;
             P.n_ions = static_cast<uint32_t>(h->n);
+            P.n_mix = 0;
+            P.mix_sigma = nullptr;
+            if (!h->dom_mix[d].empty()) {
+                mixture_params(h->dom_mix[d], h->doms[d].temperature_K, P);
+                P.mix_sigma = h->dom_mix_d.p + moff[d];
+            }
             // the domain's own waveforms, as a single domain's (AnalyticalFieldModel.cpp:130-248)
             const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
             bool waves = false;
@@ -1393,7 +1424,8 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             M.k0 = static_cast<uint32_t>(k0);
             M.step0 = step_offset + k0;
             M.n_steps = static_cast<uint32_t>(nk);
-            const hipError_t e = launch_multi_step(integ, coll, h->phys.precision, M, S, h->block, h->stream);
+            const hipError_t e = launch_multi_step(integ, dom_mixture ? COLL_HSS_MIX : coll, h->phys.precision, M, S,
+                                                   h->block, h->stream);
             if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no multi-domain kernel for this case");
             hip_check(e, "multi-domain step launch");
         });
@@ -2211,9 +2243,15 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_spans.clear();
         h->dom_has_orbitrap = false;
         h->dom_wave.clear();
+        h->dom_mix.clear();
+        h->dom_mix_sig.clear();
+        h->dom_mix_ns.clear();
         if (n_domains == 1) return;
         h->doms.assign(domains, domains + n_domains);
         h->dom_wave.resize(n_domains);
+        h->dom_mix.resize(n_domains);
+        h->dom_mix_sig.resize(n_domains);
+        h->dom_mix_ns.assign(n_domains, 0);
         // DomainManager's constructor (DomainManager.cpp:26-63): a span per
         // cylindrical domain, z range and radius widened by 1e-12 m, with the
         // index of its domain, sorted by z_min (std::sort, as there)
@@ -2493,6 +2531,26 @@ int icarion_gpu_set_waveforms(icarion_gpu* h, const icarion_waveform_desc* const
         const bool any = parse_waveform_slots(slots, w);
         for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) h->wave[k] = std::move(w[k]);
         h->has_waves = any;
+    });
+}
+
+int icarion_gpu_set_domain_gas_mixture(icarion_gpu* h, uint32_t domain, const icarion_gas_component* components,
+                                       uint32_t n_components, const double* sigma_m2, uint32_t n_species) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (h->n_dom < 2) fail(ICARION_ERR_STATE, "icarion_gpu_set_domains with two or more domains not called");
+        if (domain >= h->n_dom) fail(ICARION_ERR_INVALID, "domain index out of range");
+        if (n_components == 0) {
+            h->dom_mix[domain].clear();
+            h->dom_mix_sig[domain].clear();
+            h->dom_mix_ns[domain] = 0;
+            return;
+        }
+        if (!components || !sigma_m2) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (!h->have_species || n_species != h->n_species)
+            fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
+        build_hss_mixture(components, n_components, sigma_m2, n_species, h->dom_mix[domain], h->dom_mix_sig[domain]);
+        h->dom_mix_ns[domain] = n_species;
     });
 }
 

@@ -180,7 +180,13 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
         ++pushed;
         if constexpr (COLL != COLL_NONE) {
             PhiloxStream rng(M.seed, i, M.step0 + k);
-            if (collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dt)) {
+            bool hit;
+            if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
+                hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dt)
+                                  : collide_f64<COLL_HSS>(D, sp, sidx, m, lpre, vel, rng, dt);
+            else
+                hit = collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dt);
+            if (hit) {
                 ++nc;
                 ++ncoll_launch;
             }
@@ -331,9 +337,16 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
             PhiloxStream rng(M.seed, i, M.step0 + k);
             f3 v{hi[3], hi[4], hi[5]};
             const f3 vin = v;
-            const bool hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng, dt,
-                                               D.fc.n_dt);
-            if (COLL == COLL_HSS) {
+            bool hit;
+            if constexpr (COLL == COLL_HSS_MIX)
+                hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m,
+                                                              v, rng, dt, D.fc.n_dt)
+                                  : collide_f32<COLL_HSS>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v,
+                                                          rng, dt, D.fc.n_dt);
+            else
+                hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng, dt,
+                                        D.fc.n_dt);
+            if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
                 if (hit) {
                     hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
                     lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
@@ -419,6 +432,10 @@ static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPt
         case COLL_DAMPING:
             if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
             else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
+            break;
+        case COLL_HSS_MIX:   // HSS with per-domain gas mixtures
+            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_HSS_MIX>), grid, dim3(block), 0, s, M, S);
+            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS_MIX>), grid, dim3(block), 0, s, M, S);
             break;
         case COLL_EHSS:   // geometry EHSS with each domain's gas, FP64 only
             if constexpr (F32) return hipErrorInvalidValue;

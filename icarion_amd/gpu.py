@@ -26,7 +26,7 @@ EXPORTED = (
     "icarion_gpu_set_gas_mixture", "icarion_gpu_set_ehss_offline", "icarion_gpu_set_waveforms",
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
-    "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms",
+    "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
 )
 
 
@@ -377,6 +377,18 @@ class Handle:
             arr[k] = C.pointer(w)
         self._keep.append(dict(slots))
         _check(self.lib.icarion_gpu_set_waveforms(self.h, arr), self.lib)
+
+    def set_domain_gas_mixture(self, domain, components, sigma):
+        """One domain's HSS gas mixture on a multi-domain handle, arguments as
+        set_gas_mixture; an empty list clears that domain's."""
+        arr = (_abi.GasComponent * max(len(components), 1))()
+        for i, (n, m, part) in enumerate(components):
+            arr[i] = _abi.GasComponent(n, m, int(part), 0)
+        sig = np.ascontiguousarray(sigma if len(components) else np.zeros((0, 0)), dtype=np.float64)
+        n_species = sig.shape[0] if len(components) else 0
+        _check(self.lib.icarion_gpu_set_domain_gas_mixture(
+            self.h, C.c_uint32(domain), arr, len(components), sig.ctypes.data_as(C.POINTER(C.c_double)),
+            n_species), self.lib)
 
     def set_domain_waveforms(self, domain, slots):
         """One domain's waveforms on a multi-domain handle: slots as in

@@ -521,7 +521,8 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
  * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
  * voltages or per-domain waveforms (icarion_gpu_set_domain_waveforms), RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
  * handoff in FP64 on the compensated position), no / HSS / Langevin /
- * Friction collisions and geometry EHSS (FP64, online), one
+ * Friction collisions (HSS also with per-domain gas mixtures,
+ * icarion_gpu_set_domain_gas_mixture) and geometry EHSS (FP64, online), one
  * species table for all domains; icarion_gpu_step, icarion_gpu_step_batch and
  * icarion_gpu_step_snapshots (with each ion's domain index).  Other
  * combinations return ICARION_ERR_UNSUPPORTED when stepped. */
@@ -539,6 +540,17 @@ int icarion_gpu_set_domains(icarion_gpu *h, const icarion_domain_desc *domains,
  * stepping returns ICARION_ERR_UNSUPPORTED.  No reference counterpart as a
  * call: it carries the per-domain part of FullConfig::domains. */
 int icarion_gpu_set_domain_waveforms(icarion_gpu *h, uint32_t domain, const icarion_waveform_desc *const *slots);
+
+/* The HSS gas mixture of one domain of a multi-domain set (that domain's
+ * EnvironmentConfig gas_mixture), with the arguments and rules of
+ * icarion_gpu_set_gas_mixture; an ion collides with its step domain's mixture,
+ * or with its single gas where the domain has none.  n_components = 0 clears
+ * the domain's; icarion_gpu_set_domains clears every domain's.  A handle-level
+ * mixture (icarion_gpu_set_gas_mixture) is not combined with several domains.
+ * No reference counterpart as a call: it carries the per-domain part of
+ * FullConfig::domains. */
+int icarion_gpu_set_domain_gas_mixture(icarion_gpu *h, uint32_t domain, const icarion_gas_component *components,
+                                       uint32_t n_components, const double *sigma_m2, uint32_t n_species);
 
 /* Every resident ion's domain index after the last step (the ensemble's
  * domain_index; 0 before the ion's first step, IonState's default).  n = the

@@ -85,7 +85,7 @@ def load():
     lib.oracle_find_domain.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(d)]
     lib.oracle_find_domain.restype = C.c_int
     lib.oracle_run_multi.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(P(_abi.OracleField)),
-                                     P(_abi.PhysicsDesc),
+                                     P(P(_abi.OracleGasMixture)), P(_abi.PhysicsDesc),
                                      P(_abi.SpeciesDesc), C.c_uint32, P(d), P(_abi.IonSoA), P(C.c_int32),
                                      d, d, C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats)]
     lib.oracle_run_multi.restype = C.c_int
@@ -369,7 +369,7 @@ def find_domain(domains, pos, end_apertures=None):
 
 
 def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
-              end_apertures=None, threads=0, waveforms=None, atoms=None):
+              end_apertures=None, threads=0, waveforms=None, atoms=None, mixtures=None):
     """The engine loop over several domains on ens and domain_index (int32,
     in/out) in place; returns StepStats.  waveforms: None, or per domain None
     or {slot: WaveformDesc}."""
@@ -389,11 +389,20 @@ def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, ste
                 keep.append(wf)
                 fl[k] = C.pointer(wf.c)
         fptr = fl
+    mptr = C.POINTER(C.POINTER(_abi.OracleGasMixture))()
+    if mixtures is not None:
+        ml = (C.POINTER(_abi.OracleGasMixture) * len(domains))()
+        for k, mx in enumerate(mixtures):
+            if mx:
+                gm = GasMixture(*mx)
+                keep.append(gm)
+                ml[k] = C.pointer(gm.c)
+        mptr = ml
     at = None
     if atoms is not None:
         at = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1)
     aptr = at.ctypes.data_as(C.POINTER(C.c_double)) if at is not None else C.POINTER(C.c_double)()
-    rc = lib.oracle_run_multi(arr, app, len(domains), fptr, C.byref(physics), sp, len(species), aptr, C.byref(soa),
+    rc = lib.oracle_run_multi(arr, app, len(domains), fptr, mptr, C.byref(physics), sp, len(species), aptr, C.byref(soa),
                               domain_index.ctypes.data_as(C.POINTER(C.c_int32)), t0, dt, n_steps,
                               step_offset, threads, C.byref(st))
     if rc != 0:

@@ -1250,6 +1250,7 @@ struct Domains {
     const icarion_domain_desc* d;
     uint32_t n;
     const oracle_field* const* fields = nullptr;   // per domain (waveforms), may be NULL
+    const oracle_gas_mixture* const* mixtures = nullptr;   // per domain (HSS gas mixture), may be NULL
     std::vector<double> ap_sq;
     std::vector<Span> spans;
     bool has_orbitrap = false;
@@ -1310,10 +1311,9 @@ struct Domains {
 // domain found at the step start carries the push and the collision; after
 // the push the lookup, the bridge and the handoff decide life and the index.
 uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const icarion_species_desc& sp,
-                       const double* atoms, IonCtx& ion, int& dom_idx, const double* t_table, double dt, uint64_t n_steps,
+                       uint32_t sidx, const double* atoms, IonCtx& ion, int& dom_idx, const double* t_table, double dt, uint64_t n_steps,
                        uint32_t ion_index, uint64_t step0, uint64_t& collisions, std::vector<V3>& scratch) {
     uint64_t pushed = 0;
-    const IonTables tb;
     for (uint64_t k = 0; k < n_steps; ++k) {
         const double t = t_table[k];
         if (!ion.born && t >= ion.birth) {
@@ -1337,6 +1337,13 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
             rk4_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         ++pushed;
         if (ph.collision_model != ICARION_COLLISION_NONE) {
+            // the step domain's gas mixture, if it has one
+            IonTables tb;
+            const oracle_gas_mixture* gm = D.mixtures ? D.mixtures[dom] : nullptr;
+            if (gm && gm->n_components > 0) {
+                tb.mix = gm;
+                tb.mix_row = gm->sigma_m2 + static_cast<size_t>(sidx) * gm->n_components;
+            }
             PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
             if (collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
                 ++ion.ncoll;
@@ -1605,7 +1612,8 @@ int oracle_find_domain(const icarion_domain_desc* domains, const double* end_ape
 }
 
 int oracle_run_multi(const icarion_domain_desc* domains, const double* end_aperture_m, uint32_t n_domains,
-                     const oracle_field* const* fields, const icarion_physics_desc* physics,
+                     const oracle_field* const* fields, const oracle_gas_mixture* const* mixtures,
+                     const icarion_physics_desc* physics,
                      const icarion_species_desc* species, uint32_t n_species, const double* atoms,
                      icarion_ion_soa* ions, int32_t* domain_index, double t0, double dt, uint64_t n_steps,
                      uint64_t step_offset, int n_threads, icarion_step_stats* stats) {
@@ -1613,6 +1621,7 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
         return ICARION_ERR_INVALID;
     Domains D(domains, end_aperture_m, n_domains);
     D.fields = fields;
+    D.mixtures = mixtures;
     const size_t n = ions->n;
     std::vector<double> t_table(n_steps + 1);
     double t = t0;
@@ -1647,7 +1656,7 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
             ion.ncoll = ions->n_collisions ? ions->n_collisions[i] : 0;
             int dom = domain_index[i];
             uint64_t coll = 0;
-            const uint64_t pushed = run_ion_multi(D, *physics, sp, atoms, ion, dom, t_table.data(), dt, n_steps,
+            const uint64_t pushed = run_ion_multi(D, *physics, sp, s < n_species ? s : 0, atoms, ion, dom, t_table.data(), dt, n_steps,
                                                   static_cast<uint32_t>(i), step_offset, coll, scratch);
             ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
             ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;

@@ -62,9 +62,11 @@ int oracle_find_domain(const icarion_domain_desc *domains, const double *end_ape
 /* The engine loop over several domains (analytic fields, constant voltages,
  * RK4 / Boris, Philox streams); domain_index[n] in/out is each ion's domain
  * index (-1 before its first step).  fields: NULL, or per domain NULL or an
- * oracle_field carrying that domain's voltage waveforms. */
+ * oracle_field carrying that domain's voltage waveforms.  mixtures: NULL, or
+ * per domain NULL or that domain's gas mixture (HSS). */
 int oracle_run_multi(const icarion_domain_desc *domains, const double *end_aperture_m, uint32_t n_domains,
-                     const oracle_field *const *fields, const icarion_physics_desc *physics,
+                     const oracle_field *const *fields, const struct oracle_gas_mixture *const *mixtures,
+                     const icarion_physics_desc *physics,
                      const icarion_species_desc *species, uint32_t n_species, const double *atoms,
                      icarion_ion_soa *ions, int32_t *domain_index, double t0, double dt, uint64_t n_steps,
                      uint64_t step_offset, int n_threads, icarion_step_stats *stats);

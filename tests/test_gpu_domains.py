@@ -19,7 +19,7 @@ from scipy import stats
 from icarion_amd import _abi
 from icarion_amd import workloads as W
 from test_gpu_parity import compare_ions
-from test_oracle_domains import _gap_domains, _one_ion, domain_waveforms
+from test_oracle_domains import _gap_domains, _one_ion, domain_mixtures, domain_waveforms
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +38,9 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
                 if slots:
                     h.set_domain_waveforms(k, slots)
             h.set_species(w["species"], w.get("atoms"))
+            for k, mx in enumerate(w.get("mixtures") or []):
+                if mx:
+                    h.set_domain_gas_mixture(k, *mx)
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -53,6 +56,7 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
     di = np.zeros(ens.n, np.int32) if dom is None else dom
     st = oracle.run_multi(domains or w["domains"], physics or w["physics"], w["species"], ens, di,
                           t0, w["dt"], n_steps, step_offset=step_offset, atoms=w.get("atoms"),
+                          mixtures=w.get("mixtures"),
                           end_apertures=apertures if apertures is not None else w.get("apertures"),
                           waveforms=w.get("waveforms"))
     return st, di
@@ -151,6 +155,26 @@ def test_ehss_three_domains_ion_by_ion(oracle, integrator):
     assert len(set(do[o.active == 1].tolist())) >= 2 or len(set(do.tolist())) >= 3
 
 
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
+def test_per_domain_gas_mixtures_ion_by_ion(oracle, integrator):
+    """HSS with an air-like mixture in the first region, the quadrupole's
+    single gas and a two-gas mixture in the last region: each ion collides
+    with its step domain's gas, ion by ion against the oracle."""
+    w = W.multi_domain_workload(n=2048, seed=64, integrator=integrator, collision=_abi.COLLISION_HSS)
+    w["mixtures"] = domain_mixtures()
+    print(f"domain mixtures integ={integrator}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9) & (dg == do) & (g.death_time_s == o.death_time_s)
+    assert ok.mean() >= 0.99, ok.mean()
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    # the mixtures change the run: the single-gas run collides differently
+    c = w["ensemble"].copy()
+    sc, _ = gpu_multi(dict(w, mixtures=None), c, 6000)
+    assert sc.collisions != sg.collisions
+
+
 def test_bridge_and_handoff_cases_match_the_oracle(oracle):
     w = {"species": [W.species()], "dt": 1e-9}
     ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE)
@@ -237,6 +261,12 @@ def test_unsupported_combinations():
         h.set_waveforms(domain_waveforms()[0])
         with pytest.raises(gpu.IcarionError) as ei:
             h.step(0.0, w["dt"], 5)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        h.set_waveforms(None)
+        # a per-domain mixture needs HSS; a handle-level one is not combined
+        h.set_domain_gas_mixture(0, *domain_mixtures()[0])
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, w["dt"], 5)   # the workload's physics: no collisions
         assert ei.value.code == _abi.ERR_UNSUPPORTED
 
 

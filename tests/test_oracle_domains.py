@@ -20,7 +20,8 @@ restatement is pinned here by properties and by hand-derived cases:
   * each domain's waveforms act in that domain only: while every ion stays in
     the first domain, the multi-domain run with waveforms on all three domains
     is the single-domain run with the first domain's waveforms, bit for bit,
-    and differs from the run without them.
+    and differs from the run without them;
+  * each domain's gas mixture acts in that domain only, in the same way.
 """
 import ctypes as C
 
@@ -179,3 +180,30 @@ def test_per_domain_waveforms_act_in_their_domain(oracle):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
     live = a.active == 1
     assert live.sum() > 100 and (a.vz[live] != c.vz[live]).all()
+
+
+def domain_mixtures(n_species=1):
+    """Air-like mixtures at 10 Pa / 300 K for the first and last regions of the
+    three-domain workload (the quadrupole keeps its single gas)."""
+    n = 10.0 / (1.380649e-23 * 300.0)
+    amu = 1.66053906660e-27
+    comps = [(0.78 * n, 28.0 * amu, 1), (0.21 * n, 32.0 * amu, 1), (0.01 * n, 40.0 * amu, 1)]
+    sigma = np.tile([150e-20, 160e-20, 120e-20], (n_species, 1))
+    return [(comps, sigma), None, (comps[:2], sigma[:, :2].copy())]
+
+
+def test_per_domain_mixtures_act_in_their_domain(oracle):
+    w = W.multi_domain_workload(n=256, seed=63, collision=_abi.COLLISION_HSS)
+    mixes = domain_mixtures()
+    n_steps = 200   # no ion reaches the first boundary
+    a, b, c = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+    di = np.zeros(a.n, np.int32)
+    sa = oracle.run_multi(w["domains"], w["physics"], w["species"], a, di, 0.0, w["dt"], n_steps,
+                          end_apertures=w["apertures"], mixtures=mixes)
+    sb = oracle.run(w["domains"][0], w["physics"], w["species"], b, 0.0, w["dt"], n_steps,
+                    mixture=oracle.GasMixture(*mixes[0]))
+    sc = oracle.run(w["domains"][0], w["physics"], w["species"], c, 0.0, w["dt"], n_steps)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    assert sa.collisions == sb.collisions > 0
+    assert sa.collisions != sc.collisions
