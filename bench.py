@@ -921,7 +921,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); "
-              "multi-GPU sharding is not part of this build", file=sys.stderr)
+              "multi-GPU sharding is not implemented", file=sys.stderr)
         sys.exit(2)
     from icarion_amd import gpu
     if gpu.device_count() < 1:
