@@ -190,6 +190,10 @@ struct icarion_gpu {
     std::vector<std::vector<double>> dom_mix_sig;
     std::vector<uint32_t> dom_mix_ns;
     DevBuf<double> dom_mix_d;
+    // the same mixtures as given (every component, for the EHSS handler's loop)
+    std::vector<std::vector<icarion_gas_component>> dom_mix_all;
+    std::vector<std::vector<double>> dom_emix_sig;
+    DevBuf<double> dom_emix_d;
     // RK45 evaluates the waveforms in the kernel: their descriptors and points
     DevBuf<icarion_waveform_desc> wave_desc;
     DevBuf<double> wave_pts;
@@ -1330,8 +1334,8 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         if (h->dom_mix_ns[d] != h->n_species)
             fail(ICARION_ERR_STATE, "a domain's gas mixture species count differs from the species table");
     }
-    if (dom_mixture && coll != COLL_HSS)
-        fail(ICARION_ERR_UNSUPPORTED, "per-domain gas mixtures run with HSS collisions");
+    if (dom_mixture && coll != COLL_HSS && coll != COLL_EHSS)
+        fail(ICARION_ERR_UNSUPPORTED, "per-domain gas mixtures run with HSS or geometry EHSS collisions");
     if (h->has_waves)
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their waveforms per domain (icarion_gpu_set_domain_waveforms)");
     for (const auto& d : h->doms)
@@ -1356,6 +1360,14 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         }
         h->dom_mix_d.resize(std::max<size_t>(msig.size(), 1));
         if (!msig.empty()) copy_in(h->dom_mix_d, msig.data(), msig.size(), h->stream, "domain mixture cross sections");
+        std::vector<double> esig;
+        std::vector<size_t> eoff(nd, 0);
+        for (uint32_t d = 0; d < nd; ++d) {
+            eoff[d] = esig.size();
+            esig.insert(esig.end(), h->dom_emix_sig[d].begin(), h->dom_emix_sig[d].end());
+        }
+        h->dom_emix_d.resize(std::max<size_t>(esig.size(), 1));
+        if (!esig.empty()) copy_in(h->dom_emix_d, esig.data(), esig.size(), h->stream, "domain mixture cross sections");
         std::vector<StepEntry> tabs;
         tabs.reserve(size_t(nd) * n_steps);
         for (uint32_t d = 0; d < nd; ++d) {
@@ -1370,6 +1382,21 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             if (!h->dom_mix[d].empty()) {
                 mixture_params(h->dom_mix[d], h->doms[d].temperature_K, P);
                 P.mix_sigma = h->dom_mix_d.p + moff[d];
+            }
+            // EHSS: every component, contact radius the domain's gas radius
+            // (EHSSCollisionHandler.cpp:311-424)
+            const auto& all = h->dom_mix_all[d];
+            P.n_emix = static_cast<int>(all.size());
+            P.emix_sigma = all.empty() ? nullptr : h->dom_emix_d.p + eoff[d];
+            for (int c = 0; c < kMaxGasComponents; ++c) {
+                const bool on = c < P.n_emix;
+                const icarion_gas_component g = on ? all[c] : icarion_gas_component{0.0, 0.0, 0, 0};
+                P.emix_part[c] = on && g.participates ? 1 : 0;
+                P.emix_n[c] = g.density_m3;
+                P.emix_m[c] = g.mass_kg;
+                P.emix_sig[c] =
+                    (on && g.mass_kg > 0.0) ? std::sqrt(kBoltzmannHost * h->doms[d].temperature_K / g.mass_kg) : 0.0;
+                P.emix_r[c] = h->doms[d].gas_radius_m;
             }
             // the domain's own waveforms, as a single domain's (AnalyticalFieldModel.cpp:130-248)
             const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
@@ -1424,8 +1451,8 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             M.k0 = static_cast<uint32_t>(k0);
             M.step0 = step_offset + k0;
             M.n_steps = static_cast<uint32_t>(nk);
-            const hipError_t e = launch_multi_step(integ, dom_mixture ? COLL_HSS_MIX : coll, h->phys.precision, M, S,
-                                                   h->block, h->stream);
+            const int kcoll = !dom_mixture ? coll : (coll == COLL_EHSS ? COLL_EHSS_MIX : COLL_HSS_MIX);
+            const hipError_t e = launch_multi_step(integ, kcoll, h->phys.precision, M, S, h->block, h->stream);
             if (e == hipErrorInvalidValue) fail(ICARION_ERR_UNSUPPORTED, "no multi-domain kernel for this case");
             hip_check(e, "multi-domain step launch");
         });
@@ -2246,12 +2273,16 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_mix.clear();
         h->dom_mix_sig.clear();
         h->dom_mix_ns.clear();
+        h->dom_mix_all.clear();
+        h->dom_emix_sig.clear();
         if (n_domains == 1) return;
         h->doms.assign(domains, domains + n_domains);
         h->dom_wave.resize(n_domains);
         h->dom_mix.resize(n_domains);
         h->dom_mix_sig.resize(n_domains);
         h->dom_mix_ns.assign(n_domains, 0);
+        h->dom_mix_all.resize(n_domains);
+        h->dom_emix_sig.resize(n_domains);
         // DomainManager's constructor (DomainManager.cpp:26-63): a span per
         // cylindrical domain, z range and radius widened by 1e-12 m, with the
         // index of its domain, sorted by z_min (std::sort, as there)
@@ -2544,6 +2575,8 @@ int icarion_gpu_set_domain_gas_mixture(icarion_gpu* h, uint32_t domain, const ic
             h->dom_mix[domain].clear();
             h->dom_mix_sig[domain].clear();
             h->dom_mix_ns[domain] = 0;
+            h->dom_mix_all[domain].clear();
+            h->dom_emix_sig[domain].clear();
             return;
         }
         if (!components || !sigma_m2) fail(ICARION_ERR_INVALID, "NULL argument");
@@ -2551,6 +2584,8 @@ int icarion_gpu_set_domain_gas_mixture(icarion_gpu* h, uint32_t domain, const ic
             fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
         build_hss_mixture(components, n_components, sigma_m2, n_species, h->dom_mix[domain], h->dom_mix_sig[domain]);
         h->dom_mix_ns[domain] = n_species;
+        h->dom_mix_all[domain].assign(components, components + n_components);
+        h->dom_emix_sig[domain].assign(sigma_m2, sigma_m2 + static_cast<size_t>(n_species) * n_components);
     });
 }
 

@@ -184,6 +184,9 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
             if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
                 hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dt)
                                   : collide_f64<COLL_HSS>(D, sp, sidx, m, lpre, vel, rng, dt);
+            else if constexpr (COLL == COLL_EHSS_MIX)
+                hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dt)
+                                   : collide_f64<COLL_EHSS>(D, sp, sidx, m, lpre, vel, rng, dt);
             else
                 hit = collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dt);
             if (hit) {
@@ -436,6 +439,10 @@ static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPt
         case COLL_HSS_MIX:   // HSS with per-domain gas mixtures
             if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_HSS_MIX>), grid, dim3(block), 0, s, M, S);
             else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS_MIX>), grid, dim3(block), 0, s, M, S);
+            break;
+        case COLL_EHSS_MIX:   // geometry EHSS with per-domain gas mixtures, FP64 only
+            if constexpr (F32) return hipErrorInvalidValue;
+            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_EHSS_MIX>), grid, dim3(block), 0, s, M, S);
             break;
         case COLL_EHSS:   // geometry EHSS with each domain's gas, FP64 only
             if constexpr (F32) return hipErrorInvalidValue;

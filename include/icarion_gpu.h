@@ -544,7 +544,8 @@ int icarion_gpu_set_domain_waveforms(icarion_gpu *h, uint32_t domain, const icar
 /* The HSS gas mixture of one domain of a multi-domain set (that domain's
  * EnvironmentConfig gas_mixture), with the arguments and rules of
  * icarion_gpu_set_gas_mixture; an ion collides with its step domain's mixture,
- * or with its single gas where the domain has none.  n_components = 0 clears
+ * or with its single gas where the domain has none.  HSS and geometry EHSS
+ * (FP64; every component's contact radius is the domain's gas radius).  n_components = 0 clears
  * the domain's; icarion_gpu_set_domains clears every domain's.  A handle-level
  * mixture (icarion_gpu_set_gas_mixture) is not combined with several domains.
  * No reference counterpart as a call: it carries the per-domain part of

@@ -175,6 +175,27 @@ def test_per_domain_gas_mixtures_ion_by_ion(oracle, integrator):
     assert sc.collisions != sg.collisions
 
 
+def test_ehss_per_domain_gas_mixtures_ion_by_ion(oracle):
+    """Geometry EHSS with the per-domain mixtures (air-like first region, the
+    quadrupole's single He, a two-gas last region): each lane takes the EHSS
+    handler's mixture path with its step domain's components, or the
+    single-gas path in the quadrupole, ion by ion against the oracle."""
+    from test_gpu_parity import ehss_workload
+    e = ehss_workload(n=2048, seed=65)
+    w = W.multi_domain_workload(n=2048, seed=65, collision=_abi.COLLISION_EHSS)
+    w["species"], w["atoms"] = e["species"], e["atoms"]
+    w["ensemble"].species[:] = e["ensemble"].species
+    w["mixtures"] = domain_mixtures(n_species=3)
+    print("domain EHSS mixtures", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9) & (dg == do) & (g.death_time_s == o.death_time_s)
+    assert ok.mean() >= 0.99, ok.mean()
+    assert so.collisions > 2 * g.n
+    assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+
+
 def test_bridge_and_handoff_cases_match_the_oracle(oracle):
     w = {"species": [W.species()], "dt": 1e-9}
     ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE)
