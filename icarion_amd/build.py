@@ -2,7 +2,11 @@
 
 Runs on a machine without a GPU (hipcc cross-compiles for gfx950).  Every HIP
 translation unit is compiled with packed-FP32 instruction selection disabled.
+The translation units share no device code across files (the device building
+blocks are headers), so each is compiled to its own object in parallel and the
+objects are linked into one shared library.
 """
+import concurrent.futures as cf
 import os
 import shutil
 import subprocess
@@ -11,10 +15,14 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
+OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libicarion_gpu.so")
-SOURCES = ["step_kernels.hip", "rk45_kernels.hip", "multi_kernels.hip", "metrics_kernels.hip", "ehss_tools.hip",
-           "surrogate.hip", "icarion_gpu_api.cpp"]
+SOURCES = ["step_kernels.hip", "rk45_kernels.hip", "multi_kernels.hip", "metrics_kernels.hip",
+           "ehss_tools.hip", "surrogate.hip", "icarion_gpu_api.cpp"]
 ARCH = os.environ.get("ICARION_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
 
 
 def hipcc():
@@ -24,19 +32,33 @@ def hipcc():
     raise RuntimeError("hipcc not found (set HIPCC)")
 
 
-def build(verbose=False, extra_flags=()):
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-command-line-argument",
-           "-x", "hip"] + [os.path.join(CSRC, s) for s in SOURCES] + list(extra_flags) + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        raise RuntimeError(f"hipcc failed with exit code {res.returncode}: {cmd[-1]}")
+    return res
+
+
+def build(verbose=False, extra_flags=(), jobs=None):
+    os.makedirs(OBJDIR, exist_ok=True)
+    cc = hipcc()
+    sources = list(SOURCES)
+
+    def compile_one(src):
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        cmd = [cc, f"--offload-arch={ARCH}"] + HIP_FLAGS + list(extra_flags) + [
+            "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        _run(cmd, verbose)
+        return obj
+
+    jobs = jobs or min(len(sources), max(1, (os.cpu_count() or 1)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, sources))
+    tmp = LIB + ".tmp"
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

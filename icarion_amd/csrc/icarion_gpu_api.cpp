@@ -240,6 +240,10 @@ struct icarion_gpu {
 
     DevBuf<StepEntry> table;
     DevBuf<unsigned long long> counters;
+    // the folded counters {ion-steps, collisions, alive, arrived}: mapped
+    // pinned host memory and its device address
+    unsigned long long* host_ctr = nullptr;
+    unsigned long long* host_ctr_dev = nullptr;
     std::vector<StepEntry> table_host;
     // composite field-array term scales at every stage time (voltage waveforms)
     DevBuf<double> term_scales;
@@ -755,19 +759,66 @@ void zero_counters(icarion_gpu* h, hipStream_t s) {
     hip_check(hipGetLastError(), "zero counters");
 }
 
-// Ion-steps and collisions summed over the stripes (stream-ordered read).
-void read_counters(icarion_gpu* h, hipStream_t s, uint64_t* ion_steps, uint64_t* collisions) {
-    std::vector<unsigned long long> c(kCounterWords);
-    hip_check(hipMemcpyAsync(c.data(), h->counters.p, c.size() * sizeof(c[0]), hipMemcpyDeviceToHost, s),
-              "read counters");
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    uint64_t a = 0, b = 0;
-    for (int k = 0; k < kCounterStripes; ++k) {
-        a += c[size_t(k) * kCounterStride];
-        b += c[size_t(k) * kCounterStride + 1];
+// Alive (active and born) and arrived (arrival time set) ions of the resident
+// state, counted per wavefront with ballots and added to words 2 and 3 of the
+// launch counter stripes.
+__global__ void __launch_bounds__(256) count_state_kernel(const uint8_t* __restrict__ flags,
+                                                          const double* __restrict__ arrival, uint32_t n,
+                                                          unsigned long long* counters, int stripes) {
+    uint32_t alive = 0, arrived = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        alive += (flags[i] & 3u) == 3u;
+        arrived += arrival[i] == arrival[i];
     }
-    *ion_steps = a;
-    *collisions = b;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        alive += __shfl_xor(alive, off, 64);
+        arrived += __shfl_xor(arrived, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && (alive | arrived)) {
+        const uint32_t s =
+            ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % static_cast<uint32_t>(stripes)) * kCounterStride;
+        if (alive) atomicAdd(&counters[s + 2], static_cast<unsigned long long>(alive));
+        if (arrived) atomicAdd(&counters[s + 3], static_cast<unsigned long long>(arrived));
+    }
+}
+
+// Sum of the stripes: word w of every stripe into out[w] (w = 0..3), written
+// straight to mapped pinned host memory.
+__global__ void fold_counters_kernel(const unsigned long long* counters, int stripes, unsigned long long* out) {
+    const int w = threadIdx.x;
+    if (w < 4) {
+        unsigned long long a = 0;
+        for (int k = 0; k < stripes; ++k) a += counters[k * kCounterStride + w];
+        out[w] = a;
+    }
+}
+
+// Counters of the call: ion-steps and collisions of its launches, and the
+// alive / arrived counts of the state they left (stream-ordered; 32 bytes
+// cross to the host).
+struct CallCounts {
+    uint64_t ion_steps = 0, collisions = 0, alive = 0, arrived = 0;
+};
+
+CallCounts read_counters(icarion_gpu* h, hipStream_t s, bool with_state = true) {
+    if (with_state && h->n) {
+        const uint32_t n = static_cast<uint32_t>(h->n);
+        const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 1024);
+        hipLaunchKernelGGL(count_state_kernel, dim3(grid), dim3(256), 0, s, h->flags.p, h->arrival.p, n,
+                           h->counters.p, kCounterStripes);
+        hip_check(hipGetLastError(), "count state launch");
+    }
+    hipLaunchKernelGGL(fold_counters_kernel, dim3(1), dim3(64), 0, s, h->counters.p, kCounterStripes,
+                       h->host_ctr_dev);
+    hip_check(hipGetLastError(), "fold counters launch");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    CallCounts c;
+    c.ion_steps = h->host_ctr[0];
+    c.collisions = h->host_ctr[1];
+    c.alive = h->host_ctr[2];
+    c.arrived = h->host_ctr[3];
+    return c;
 }
 
 IonPtrs ion_ptrs(icarion_gpu* h) {
@@ -919,32 +970,19 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
                   "upload engine record");
     }
     Rk45Engine e1{};
-    uint64_t c[2] = {0, 0};
     hip_check(hipMemcpyAsync(&e1, h->r_eng.p, sizeof(e1), hipMemcpyDeviceToHost, s), "read engine record");
-    read_counters(h, s, &c[0], &c[1]);
+    const CallCounts c = read_counters(h, s, stats != nullptr);
     if (e1.failed)
         fail(ICARION_ERR_INVALID, "RK45: " + std::to_string(e1.failed) +
                                       " ion steps found no accepted step in 10 attempts");
     if (stats) {
-        stats->ion_steps = c[0];
-        stats->collisions = c[1];
+        stats->ion_steps = c.ion_steps;
+        stats->collisions = c.collisions;
         stats->rejected = e1.rejected;
         stats->t_end_s = e1.T;
         stats->device_ms = ms;
-        std::vector<uint8_t> fl(h->n);
-        std::vector<double> ar(h->n);
-        if (h->n) {
-            copy_out(fl.data(), h->flags, h->n, s, "read flags");
-            copy_out(ar.data(), h->arrival, h->n, s, "read arrival");
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-        }
-        uint64_t alive = 0, arrived = 0;
-        for (size_t i = 0; i < h->n; ++i) {
-            if ((fl[i] & 3) == 3) ++alive;
-            if (!std::isnan(ar[i])) ++arrived;
-        }
-        stats->alive = alive;
-        stats->arrived = arrived;
+        stats->alive = c.alive;
+        stats->arrived = c.arrived;
     }
 }
 
@@ -1168,6 +1206,8 @@ void deliver_empty_snapshots(const icarion_domain_desc& d, int integrator, doubl
     }
 }
 
+void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats);
+
 void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
              icarion_step_stats* stats, const SnapshotReq* snap = nullptr) {
     if (h->have_domain && h->n_dom >= 2) {
@@ -1247,61 +1287,25 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     } else {
         for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
     }
-    if (stats) {
-        uint64_t c[2] = {0, 0};
-        read_counters(h, h->stream, &c[0], &c[1]);
-        stats->ion_steps = c[0];
-        stats->collisions = c[1];
-        stats->rejected = 0;
-        stats->t_end_s = t_end;
-        stats->device_ms = ms;
-        // alive / arrived from the flags and arrival arrays
-        std::vector<uint8_t> fl(h->n);
-        std::vector<double> ar(h->n);
-        if (h->n) {
-            hip_check(hipMemcpy(fl.data(), h->flags.p, h->n, hipMemcpyDeviceToHost), "read flags");
-            hip_check(hipMemcpy(ar.data(), h->arrival.p, h->n * sizeof(double), hipMemcpyDeviceToHost),
-                      "read arrival");
-        }
-        uint64_t alive = 0, arrived = 0;
-        for (size_t i = 0; i < h->n; ++i) {
-            if ((fl[i] & 3) == 3) ++alive;
-            if (!std::isnan(ar[i])) ++arrived;
-        }
-        stats->alive = alive;
-        stats->arrived = arrived;
-    } else {
-        hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
-    }
+    finish_stats(h, t_end, ms, stats);
 }
 
 
-// Alive / arrived counts and the counters into stats (stream-ordered reads).
+// Alive / arrived counts and the counters into stats (stream-ordered reads;
+// the counts are reduced on the device).
 void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats) {
     if (!stats) {
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         return;
     }
-    uint64_t c[2] = {0, 0};
-    read_counters(h, h->stream, &c[0], &c[1]);
-    stats->ion_steps = c[0];
-    stats->collisions = c[1];
+    const CallCounts c = read_counters(h, h->stream);
+    stats->ion_steps = c.ion_steps;
+    stats->collisions = c.collisions;
     stats->rejected = 0;
     stats->t_end_s = t_end;
     stats->device_ms = ms;
-    std::vector<uint8_t> fl(h->n);
-    std::vector<double> ar(h->n);
-    if (h->n) {
-        hip_check(hipMemcpy(fl.data(), h->flags.p, h->n, hipMemcpyDeviceToHost), "read flags");
-        hip_check(hipMemcpy(ar.data(), h->arrival.p, h->n * sizeof(double), hipMemcpyDeviceToHost), "read arrival");
-    }
-    uint64_t alive = 0, arrived = 0;
-    for (size_t i = 0; i < h->n; ++i) {
-        if ((fl[i] & 3) == 3) ++alive;
-        if (!std::isnan(ar[i])) ++arrived;
-    }
-    stats->alive = alive;
-    stats->arrived = arrived;
+    stats->alive = c.alive;
+    stats->arrived = c.arrived;
 }
 
 // The engine loop over several domains: per-domain parameter records and
@@ -2168,6 +2172,11 @@ int icarion_gpu_create(const icarion_gpu_desc* desc, icarion_gpu** out) {
             hip_check(hipEventCreate(&h->ev0), "hipEventCreate");
             hip_check(hipEventCreate(&h->ev1), "hipEventCreate");
             h->counters.resize(kCounterWords);
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&h->host_ctr), 4 * sizeof(unsigned long long),
+                                    hipHostMallocMapped | hipHostMallocCoherent),
+                      "hipHostMalloc counters");
+            hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&h->host_ctr_dev), h->host_ctr, 0),
+                      "hipHostGetDevicePointer");
             hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
             for (int k = 0; k < kBatchClasses - 1; ++k) {
                 hip_check(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking), "hipStreamCreate");
@@ -2229,6 +2238,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     if (h->snap.host_full) (void)hipHostFree(h->snap.host_full);
     if (h->snap.host_rec) (void)hipHostFree(h->snap.host_rec);
     if (h->snap.host_count) (void)hipHostFree(h->snap.host_count);
+    if (h->host_ctr) (void)hipHostFree(h->host_ctr);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -2400,6 +2410,11 @@ int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species,
         if (!h || !species || n_species == 0) fail(ICARION_ERR_INVALID, "species table is empty");
         if (n_atoms && !atoms_xyzr) fail(ICARION_ERR_INVALID, "atoms_xyzr is NULL");
         if (atoms_dtype != 0 && atoms_dtype != 1) fail(ICARION_ERR_INVALID, "atoms_dtype must be 0 or 1");
+        // The device holds the atom table in float only; a double table is
+        // declined rather than rounded behind the caller's back.
+        if (n_atoms && atoms_dtype == 1)
+            fail(ICARION_ERR_UNSUPPORTED, "the device atom table is float (atoms_dtype 0); round the "
+                                          "coordinates to float or use the CPU path for double atoms");
         std::vector<SpeciesDev> sd(n_species);
         std::vector<uint32_t> offs(n_species), cnts(n_species);
         for (uint32_t s = 0; s < n_species; ++s) {
@@ -2420,18 +2435,8 @@ int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species,
         copy_in(h->species, sd.data(), n_species, h->stream, "upload species");
         copy_in(h->atom_offsets, offs.data(), n_species, h->stream, "upload atom offsets");
         copy_in(h->atom_counts, cnts.data(), n_species, h->stream, "upload atom counts");
-        // The device holds one FP32 atom table; a double table is rounded to
-        // float here (the staging vector lives until the copy has finished).
         h->atoms_f32.release();
-        std::vector<float4> at;
-        if (n_atoms && atoms_dtype == 1) {
-            const double* a = static_cast<const double*>(atoms_xyzr);
-            at.resize(n_atoms);
-            for (uint32_t i = 0; i < n_atoms; ++i)
-                at[i] = make_float4(static_cast<float>(a[4 * i]), static_cast<float>(a[4 * i + 1]),
-                                    static_cast<float>(a[4 * i + 2]), static_cast<float>(a[4 * i + 3]));
-            copy_in(h->atoms_f32, at.data(), n_atoms, h->stream, "upload atoms");
-        } else if (n_atoms) {
+        if (n_atoms) {
             copy_in(h->atoms_f32, static_cast<const float4*>(atoms_xyzr), n_atoms, h->stream, "upload atoms");
         }
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");

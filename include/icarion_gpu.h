@@ -340,8 +340,9 @@ int icarion_gpu_set_domain(icarion_gpu *h, const icarion_domain_desc *domain,
 
 /* Species table and EHSS atom table (x, y, z, r per atom, metres, COM-centred,
  * geometryUtils.h:35-60).  atoms_dtype 0 = float, 1 = double.  The device
- * keeps the table in float: a double table is rounded to float on upload, and
- * the EHSS geometry arithmetic then runs in double on those values.
+ * keeps the table in float and the EHSS geometry arithmetic runs in double on
+ * those values; a double table (atoms_dtype 1 with n_atoms > 0) returns
+ * ICARION_ERR_UNSUPPORTED, so that the caller rounds it or keeps the CPU path.
  * atoms_xyzr may be NULL when n_atoms is 0. */
 int icarion_gpu_set_species(icarion_gpu *h, const icarion_species_desc *species,
                             uint32_t n_species, const void *atoms_xyzr,

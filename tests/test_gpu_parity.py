@@ -193,6 +193,17 @@ def test_ehss_fp64_double_atom_table(oracle):
     assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
 
 
+def test_double_atom_table_declined():
+    """A double atom table is declined (UNSUPPORTED), not rounded silently."""
+    from icarion_amd import gpu
+    w = ehss_workload(n=64, atom_dtype=np.float64)
+    with gpu.Handle(device=0) as h:
+        h.set_domain(w["domain"], w["physics"])
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.set_species(w["species"], w["atoms"])
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+
+
 def test_ehss_fp64_ion_by_ion(oracle):
     w = ehss_workload()
     g, o = w["ensemble"].copy(), w["ensemble"].copy()
