@@ -29,6 +29,30 @@
 #include "surrogate.hpp"
 #include "waveform.hpp"
 
+#include <mutex>
+
+// Scratch memory of icarion_gpu_step_noalloc, one copy per device held until
+// the process ends: the striped launch counters, then the step table of up to
+// ICARION_NOALLOC_MAX_STEPS entries.  In host code `scratch` names a host-side
+// stand-in, not device memory: its device address is looked up per device
+// (hipGetSymbolAddress, at handle creation with the device current) and only
+// that address is handed to kernels and copies.
+namespace {
+constexpr size_t kNoAllocCounterBytes = size_t(icarion::kCounterStripes) * icarion::kCounterStride * 8;
+constexpr size_t kNoAllocTableBytes = size_t(ICARION_NOALLOC_MAX_STEPS) * sizeof(icarion::StepEntry);
+constexpr size_t kScratchBytes = kNoAllocCounterBytes + kNoAllocTableBytes;
+static_assert(kNoAllocCounterBytes % 256 == 0, "the step table must start 256-byte aligned");
+constexpr int kMaxScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxScratchDevices] = {};
+std::string g_scratch_error[kMaxScratchDevices];
+}  // namespace
+
 namespace icarion {
 hipError_t launch_step(int integrator, int field, int coll, int precision, int surrogate,
                        const StepParams& P, const IonPtrs& S, uint32_t block, hipStream_t stream);
@@ -244,6 +268,8 @@ struct icarion_gpu {
     // pinned host memory and its device address
     unsigned long long* host_ctr = nullptr;
     unsigned long long* host_ctr_dev = nullptr;
+    // device address of this device's scratch array (icarion_gpu_step_noalloc)
+    unsigned char* scratch_dev = nullptr;
     std::vector<StepEntry> table_host;
     // composite field-array term scales at every stage time (voltage waveforms)
     DevBuf<double> term_scales;
@@ -710,7 +736,10 @@ double composite_scale(const icarion_gpu* h, int j, double t) {
     }
 }
 
-void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
+// The call's step table on the host, then on the device: in the handle's table
+// (resized to the call), or at dst when the caller gives one (at least n_steps
+// entries).
+void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps, StepEntry* dst = nullptr) {
     h->table_host.clear();
     const bool stage_times = field_kind(h->dom) == FIELD_ARRAY && h->fsrc.n_terms >= 2;
     const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
@@ -741,9 +770,12 @@ void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps) {
                                  h->term_scales_host.size() * sizeof(double), hipMemcpyHostToDevice, h->stream),
                   "upload term scales");
     }
-    h->table.resize(std::max<size_t>(n_steps, 1));
-    hip_check(hipMemcpyAsync(h->table.p, h->table_host.data(), n_steps * sizeof(StepEntry),
-                             hipMemcpyHostToDevice, h->stream),
+    if (!dst) {
+        h->table.resize(std::max<size_t>(n_steps, 1));
+        dst = h->table.p;
+    }
+    hip_check(hipMemcpyAsync(dst, h->table_host.data(), n_steps * sizeof(StepEntry), hipMemcpyHostToDevice,
+                             h->stream),
               "upload step table");
 }
 
@@ -754,8 +786,8 @@ __global__ void zero_u64(unsigned long long* p, int n) {
 constexpr int kCounterWords = kCounterStripes * kCounterStride;
 
 // Clear the striped launch counters of a single cloud.
-void zero_counters(icarion_gpu* h, hipStream_t s) {
-    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, s, h->counters.p, kCounterWords);
+void zero_counters(unsigned long long* counters, hipStream_t s) {
+    hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, s, counters, kCounterWords);
     hip_check(hipGetLastError(), "zero counters");
 }
 
@@ -801,15 +833,15 @@ struct CallCounts {
     uint64_t ion_steps = 0, collisions = 0, alive = 0, arrived = 0;
 };
 
-CallCounts read_counters(icarion_gpu* h, hipStream_t s, bool with_state = true) {
+CallCounts read_counters(icarion_gpu* h, hipStream_t s, bool with_state, unsigned long long* counters) {
     if (with_state && h->n) {
         const uint32_t n = static_cast<uint32_t>(h->n);
         const uint32_t grid = std::min<uint32_t>((n + 255) / 256, 1024);
         hipLaunchKernelGGL(count_state_kernel, dim3(grid), dim3(256), 0, s, h->flags.p, h->arrival.p, n,
-                           h->counters.p, kCounterStripes);
+                           counters, kCounterStripes);
         hip_check(hipGetLastError(), "count state launch");
     }
-    hipLaunchKernelGGL(fold_counters_kernel, dim3(1), dim3(64), 0, s, h->counters.p, kCounterStripes,
+    hipLaunchKernelGGL(fold_counters_kernel, dim3(1), dim3(64), 0, s, counters, kCounterStripes,
                        h->host_ctr_dev);
     hip_check(hipGetLastError(), "fold counters launch");
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -921,7 +953,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     Rk45Ptrs R{h->r_t.p, h->r_dt.p, h->r_err.p, h->r_k1x.p, h->r_k1y.p, h->r_k1z.p, h->r_fsal.p, h->r_eng.p};
     Rk45Engine e0{t0, dt, step_offset, 0ull, ~0ull, 0ull, 0ull, 0ull};
     hip_check(hipMemcpyAsync(h->r_eng.p, &e0, sizeof(e0), hipMemcpyHostToDevice, s), "upload engine record");
-    zero_counters(h, s);
+    zero_counters(h->counters.p, s);
     float ms = 0.0f;
     if (h->n > 0) {
         hip_check(launch_rk45_init(R, static_cast<uint32_t>(h->n), t0, dt, s), "rk45 init");
@@ -971,7 +1003,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     }
     Rk45Engine e1{};
     hip_check(hipMemcpyAsync(&e1, h->r_eng.p, sizeof(e1), hipMemcpyDeviceToHost, s), "read engine record");
-    const CallCounts c = read_counters(h, s, stats != nullptr);
+    const CallCounts c = read_counters(h, s, stats != nullptr, h->counters.p);
     if (e1.failed)
         fail(ICARION_ERR_INVALID, "RK45: " + std::to_string(e1.failed) +
                                       " ion steps found no accepted step in 10 attempts");
@@ -1206,10 +1238,35 @@ void deliver_empty_snapshots(const icarion_domain_desc& d, int integrator, doubl
     }
 }
 
-void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats);
+void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats, unsigned long long* counters);
+
+// Device memory of a fixed-step call besides the ion state: its step table
+// and launch counters.  By default the handle's own buffers (the table is
+// resized to the call); a caller that may not allocate passes fixed memory
+// that holds up to max_steps table entries.
+struct Workspace {
+    StepEntry* table = nullptr;
+    unsigned long long* counters = nullptr;
+    uint64_t max_steps = 0;
+};
 
 void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
-             icarion_step_stats* stats, const SnapshotReq* snap = nullptr) {
+             icarion_step_stats* stats, const SnapshotReq* snap = nullptr, const Workspace* ws = nullptr) {
+    if (ws) {
+        // the allocation-free variant covers the single-domain fixed-step path
+        if (h->have_domain && h->n_dom >= 2)
+            fail(ICARION_ERR_UNSUPPORTED, "the allocation-free step covers single-domain handles");
+        if (h->have_domain && h->phys.integrator == ICARION_INTEGRATOR_RK45)
+            fail(ICARION_ERR_UNSUPPORTED, "the allocation-free step covers the fixed-step integrators");
+        if (snap) fail(ICARION_ERR_UNSUPPORTED, "the allocation-free step takes no snapshots");
+        if (n_steps > ws->max_steps)
+            fail(ICARION_ERR_INVALID, "n_steps " + std::to_string(n_steps) + " exceeds the allocation-free "
+                                      "step's maximum of " + std::to_string(ws->max_steps) + " per call");
+        if (h->have_domain && h->has_waves && field_kind(h->dom) == FIELD_ARRAY && h->fsrc.n_terms >= 2)
+            fail(ICARION_ERR_UNSUPPORTED, "waveform-scaled field-array terms need a per-call scale table; "
+                                          "use icarion_gpu_step");
+    }
+    unsigned long long* const counters = ws ? ws->counters : h->counters.p;
     if (h->have_domain && h->n_dom >= 2) {
         do_step_multi(h, t0, dt, n_steps, step_offset, stats, snap);
         return;
@@ -1254,17 +1311,18 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     apply_field(h, P);
     apply_mixture(h, P);
     P.n_ions = static_cast<uint32_t>(h->n);
-    P.counters = h->counters.p;
+    P.counters = counters;
     P.counter_stripes = kCounterStripes;
     apply_orbitrap_waveform(h, P);
     fill_fast_const(P);
-    zero_counters(h, h->stream);
+    zero_counters(counters, h->stream);
     double t_end = t0;
     float ms = 0.0f;
     if (snap && n_steps > 0 && h->n == 0)
         deliver_empty_snapshots(h->dom, h->phys.integrator, t0, dt, n_steps, step_offset, *snap);
     if (n_steps > 0 && h->n > 0) {
-        build_table(h, t0, dt, n_steps);
+        build_table(h, t0, dt, n_steps, ws ? ws->table : nullptr);
+        StepEntry* const table = ws ? ws->table : h->table.p;
         P.term_scales = h->use_term_scales ? h->term_scales.p : nullptr;
         t_end = h->table_host[n_steps - 1].t + dt;
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
@@ -1275,7 +1333,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
             ensure_snap_ring(h);
         }
         ms = run_launches(h, S, n_steps, step_offset, spl, plan, snap, [&](uint64_t k0, uint64_t nk) {
-            P.table = h->table.p + k0;
+            P.table = table + k0;
             P.step0 = step_offset + k0;
             P.n_steps = static_cast<uint32_t>(nk);
             const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), launch_coll_kind(h),
@@ -1287,18 +1345,18 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     } else {
         for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
     }
-    finish_stats(h, t_end, ms, stats);
+    finish_stats(h, t_end, ms, stats, counters);
 }
 
 
 // Alive / arrived counts and the counters into stats (stream-ordered reads;
 // the counts are reduced on the device).
-void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats) {
+void finish_stats(icarion_gpu* h, double t_end, float ms, icarion_step_stats* stats, unsigned long long* counters) {
     if (!stats) {
         hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
         return;
     }
-    const CallCounts c = read_counters(h, h->stream);
+    const CallCounts c = read_counters(h, h->stream, true, counters);
     stats->ion_steps = c.ion_steps;
     stats->collisions = c.collisions;
     stats->rejected = 0;
@@ -1348,7 +1406,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         fail(ICARION_ERR_STATE, "the species table no longer covers the resident ions' species indices");
     set_device(h);
     const uint32_t nd = h->n_dom;
-    zero_counters(h, h->stream);
+    zero_counters(h->counters.p, h->stream);
     double t_end = t0;
     float ms = 0.0f;
     if (snap && n_steps > 0 && h->n == 0)
@@ -1463,7 +1521,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
     } else {
         for (uint64_t k = 0; k < n_steps; ++k) t_end = t_end + dt;
     }
-    finish_stats(h, t_end, ms, stats);
+    finish_stats(h, t_end, ms, stats, h->counters.p);
 }
 
 void do_upload(icarion_gpu* h, const icarion_ion_soa* ions) {
@@ -2177,6 +2235,26 @@ int icarion_gpu_create(const icarion_gpu_desc* desc, icarion_gpu** out) {
                       "hipHostMalloc counters");
             hip_check(hipHostGetDevicePointer(reinterpret_cast<void**>(&h->host_ctr_dev), h->host_ctr, 0),
                       "hipHostGetDevicePointer");
+            if (h->device < kMaxScratchDevices) {
+                // once per device, with that device current; a failed lookup
+                // is kept and reported by icarion_gpu_step_noalloc, the one
+                // entry point that needs the address
+                std::lock_guard<std::mutex> lock(g_scratch_mu);
+                if (!g_scratch[h->device] && g_scratch_error[h->device].empty()) {
+                    void* ptr = nullptr;
+                    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+                    if (e != hipSuccess) {
+                        g_scratch_error[h->device] =
+                            std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+                        (void)hipGetLastError();
+                    } else if (!ptr) {
+                        g_scratch_error[h->device] = "hipGetSymbolAddress(scratch) returned NULL";
+                    } else {
+                        g_scratch[h->device] = ptr;
+                    }
+                }
+                h->scratch_dev = static_cast<unsigned char*>(g_scratch[h->device]);
+            }
             hip_check(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming), "hipEventCreate");
             for (int k = 0; k < kBatchClasses - 1; ++k) {
                 hip_check(hipStreamCreateWithFlags(&h->side[k], hipStreamNonBlocking), "hipStreamCreate");
@@ -2780,6 +2858,24 @@ int icarion_gpu_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uin
     return guarded([&] {
         if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
         do_step(h, t0, dt, n_steps, step_offset, stats);
+    });
+}
+
+int icarion_gpu_step_noalloc(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t step_offset,
+                             icarion_step_stats* stats) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (!h->scratch_dev) {
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            fail(ICARION_ERR_HIP, h->device < kMaxScratchDevices && !g_scratch_error[h->device].empty()
+                                      ? g_scratch_error[h->device]
+                                      : std::string("no scratch array on devices above 63"));
+        }
+        Workspace ws;
+        ws.counters = reinterpret_cast<unsigned long long*>(h->scratch_dev);
+        ws.table = reinterpret_cast<StepEntry*>(h->scratch_dev + kNoAllocCounterBytes);
+        ws.max_steps = ICARION_NOALLOC_MAX_STEPS;
+        do_step(h, t0, dt, n_steps, step_offset, stats, nullptr, &ws);
     });
 }
 

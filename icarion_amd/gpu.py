@@ -27,7 +27,11 @@ EXPORTED = (
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
     "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
+    "icarion_gpu_step_noalloc",
 )
+
+# icarion_gpu_step_noalloc's per-call step limit (ICARION_NOALLOC_MAX_STEPS)
+NOALLOC_MAX_STEPS = 16384
 
 
 class IcarionError(RuntimeError):
@@ -81,6 +85,8 @@ def load(path=None):
     lib.icarion_gpu_upload_state.argtypes = [C.c_void_p, P(_abi.IonSoA)]
     lib.icarion_gpu_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
                                      P(_abi.StepStats)]
+    lib.icarion_gpu_step_noalloc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_uint64,
+                                             C.c_uint64, P(_abi.StepStats)]
     lib.icarion_gpu_step_snapshots.argtypes = [
         C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_uint64, C.c_double, C.c_int,
         P(C.c_double), _abi.SnapshotFn, C.c_void_p, P(_abi.StepStats)]
@@ -493,6 +499,14 @@ class Handle:
         st = _abi.StepStats()
         _check(self.lib.icarion_gpu_step(self.h, t0, dt, n_steps, step_offset, C.byref(st)),
                self.lib)
+        return st
+
+    def step_noalloc(self, t0, dt, n_steps, step_offset=0):
+        """icarion_gpu_step through the allocation-free entry point (the step
+        table and counters in the library's fixed per-device scratch)."""
+        st = _abi.StepStats()
+        _check(self.lib.icarion_gpu_step_noalloc(self.h, t0, dt, n_steps, step_offset,
+                                                 C.byref(st)), self.lib)
         return st
 
     def step_snapshots(self, t0, dt, n_steps, write_interval, step_offset=0, write_first=True,

@@ -478,6 +478,23 @@ int icarion_gpu_upload_state(icarion_gpu *h, const icarion_ion_soa *ions);
 int icarion_gpu_step(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
                      uint64_t step_offset, icarion_step_stats *stats);
 
+/* icarion_gpu_step that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The step table
+ * and launch counters live in a fixed scratch array of the library (one per
+ * device, 1 MiB + 8 KiB, held until the process ends) instead of the handle's
+ * buffers, which icarion_gpu_step resizes to each call.  Limits: at most
+ * ICARION_NOALLOC_MAX_STEPS steps per call (ICARION_ERR_INVALID above);
+ * single-domain handles with RK4 or Boris (ICARION_ERR_UNSUPPORTED for
+ * several domains, RK45 and waveform-scaled field-array terms).  The ion
+ * state, species and field buffers are the handle's, set up beforehand.
+ * Results equal icarion_gpu_step's bit for bit.  Every call on one device
+ * shares that device's scratch: calls that overlap on one device (two host
+ * threads, two handles) race and must be serialised by the caller. */
+#define ICARION_NOALLOC_MAX_STEPS 16384
+int icarion_gpu_step_noalloc(icarion_gpu *h, double t0, double dt, uint64_t n_steps,
+                             uint64_t step_offset, icarion_step_stats *stats);
+
 /* Copy the resident state back into host SoA arrays. */
 int icarion_gpu_download_state(icarion_gpu *h, icarion_ion_soa *ions);
 
