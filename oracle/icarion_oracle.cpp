@@ -436,6 +436,17 @@ double orbitrap_residual(double r, double z, double R, double R_m) {
     return term1 - term2 - term3;
 }
 
+// Ions per OpenMP work chunk: 64, or fewer for small ensembles so that every
+// thread gets work (long runs of a few hundred ions).
+long long omp_chunk(size_t n) {
+#ifdef _OPENMP
+    const long long per = static_cast<long long>(n) / (4LL * omp_get_max_threads());
+#else
+    const long long per = static_cast<long long>(n);
+#endif
+    return std::max(1LL, std::min(64LL, per));
+}
+
 // OrbitrapGeometry::orbitrap_r_for_z (OrbitrapGeometry.h:181-219)
 double orbitrap_r_for_z(double z, double R, double R_m) {
     const double z_abs = std::fabs(z);
@@ -1635,10 +1646,11 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
 #else
     (void)n_threads;
 #endif
+    const long long chunk = omp_chunk(n);
     #pragma omp parallel reduction(+ : pushed_total, coll_total, alive, arrived)
     {
         std::vector<V3> scratch;
-        #pragma omp for schedule(dynamic, 64)
+        #pragma omp for schedule(dynamic, chunk)
         for (long long ii = 0; ii < static_cast<long long>(n); ++ii) {
             const size_t i = static_cast<size_t>(ii);
             const uint32_t s = ions->species[i];
@@ -1709,10 +1721,11 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
 #else
     (void)n_threads;
 #endif
+    const long long chunk = omp_chunk(n);
     #pragma omp parallel reduction(+ : pushed_total, coll_total, alive, arrived)
     {
         std::vector<V3> scratch;
-        #pragma omp for schedule(dynamic, 64)
+        #pragma omp for schedule(dynamic, chunk)
         for (long long ii = 0; ii < static_cast<long long>(n); ++ii) {
             const size_t i = static_cast<size_t>(ii);
             const uint32_t s = ions->species[i];
@@ -1819,13 +1832,14 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
     double T = t0;
     uint64_t pushed_total = 0, coll_total = 0, rejected_total = 0;
     bool failed = false;
+    const long long chunk = omp_chunk(n);
     for (uint64_t k = 0; k < n_steps; ++k) {
         double max_dt_used = 0.0;
         #pragma omp parallel reduction(+ : pushed_total, coll_total, rejected_total) \
             reduction(max : max_dt_used) reduction(|| : failed)
         {
             std::vector<V3> scratch;
-            #pragma omp for schedule(dynamic, 64)
+            #pragma omp for schedule(dynamic, chunk)
             for (long long ii = 0; ii < static_cast<long long>(n); ++ii) {
                 const size_t i = static_cast<size_t>(ii);
                 const uint32_t s = ions->species[i];

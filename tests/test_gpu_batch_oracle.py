@@ -1,0 +1,181 @@
+"""Batched clouds (one persistent launch, icarion_gpu_batch_*) against the FP64
+CPU oracle directly, cloud by cloud: the same engine loop and Philox streams
+(keyed by each cloud's seed, ion index and step), so FP64 clouds are compared
+ion by ion and FP32 + Kahan clouds by path agreement and statistics.
+
+Bars:
+  * FP64, collisionless: every ion to 1e-9 relative;
+  * FP64, stochastic: >= 99 % of ions to 1e-9 with equal collision counts (a
+    collision test within rounding of its threshold can send one ion on
+    another path);
+  * FP32 + Kahan, collisionless: >= 99.9 % of ions to 2e-5 (positions on a
+    1 mm scale, velocities on 10 m/s);
+  * FP32 + Kahan, stochastic: the FP32 collision and kick code draws its
+    normals from both Box-Muller branches, so its random numbers map to other
+    events than the FP64 code's and the paths are independent samples:
+    survival fraction, mean drift and collisions per ion within 4 standard
+    errors of the difference (plus 1 % of the value), and a KS test on the
+    final z (p > 1e-3).
+The long Orbitrap test runs the mixed workload's full 1e6 steps and states the
+FP32 + Kahan error against the FP64 oracle (position and orbital phase)."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from icarion_amd import _abi
+from icarion_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_clouds(seed=17):
+    """Sixteen clouds over the batch kinds and both precisions: IMS (HSS, RK4),
+    quadrupole (Langevin + OU, Boris), Orbitrap (collisionless RK4) and a
+    friction + OU quadrupole (RK4)."""
+    rng = np.random.default_rng(seed)
+    clouds = []
+    for k in range(16):
+        n = int(rng.integers(200, 4000))
+        prec = _abi.PRECISION_FP64 if (k // 4) % 2 == 0 else _abi.PRECISION_FP32_KAHAN
+        kind = ("ims", "quadrupole", "orbitrap", "quadrupole_rk4_friction")[k % 4]
+        s = 300 + k
+        if kind == "ims":
+            w = W.ims_hss_workload(n=n, seed=s, precision=prec)
+            steps = int(rng.integers(300, 1500))
+        elif kind == "quadrupole":
+            w = W.quadrupole_workload(n=n, U=10.0, seed=s, precision=prec)
+            steps = int(rng.integers(300, 1500))
+        elif kind == "orbitrap":
+            w = W.orbitrap_workload(n=n, seed=s, precision=prec)
+            steps = int(rng.integers(1000, 4000))
+        else:
+            w = W.quadrupole_workload(n=n, U=5.0, seed=s, precision=prec)
+            w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_FRICTION, ou=1,
+                                     precision=prec, seed=s)
+            w["species"] = [W.species(K0=2.0)]
+            steps = int(rng.integers(300, 1500))
+        clouds.append(dict(kind=kind, domain=w["domain"], physics=w["physics"],
+                           species=w["species"], ensemble=w["ensemble"].copy(),
+                           t0=1e-7 * k, dt=w["dt"], n_steps=steps, step_offset=11 * k))
+    return clouds
+
+
+def batch_run(clouds):
+    from icarion_amd import gpu
+    with gpu.Handle() as h:
+        h.batch_upload(clouds)
+        stats, _ = h.batch_run()
+        h.batch_download(clouds)
+    return stats
+
+
+def oracle_cloud(oracle, c, ens):
+    """The oracle's FP64 engine loop over one cloud's initial state."""
+    ph = c["physics"]
+    ph64 = W.physics(ph.integrator, ph.collision_model, ph.ou_thermal_kicks, _abi.PRECISION_FP64,
+                     ph.rng_seed, ph.ehss_mode)
+    return oracle.run(c["domain"], ph64, c["species"], ens, c["t0"], c["dt"], c["n_steps"],
+                      step_offset=c["step_offset"])
+
+
+def agree(g, o, rtol, pos_scale, vel_scale):
+    ok = g.active == o.active
+    for f in ("x", "y", "z"):
+        a, b = getattr(g, f), getattr(o, f)
+        ok &= np.abs(a - b) <= rtol * np.maximum(np.abs(b), pos_scale)
+    for f in ("vx", "vy", "vz"):
+        a, b = getattr(g, f), getattr(o, f)
+        ok &= np.abs(a - b) <= rtol * np.maximum(np.abs(b), vel_scale)
+    return ok
+
+
+def close_mean(a, b, what):
+    """mean(a) and mean(b) within 4 standard errors of their difference plus
+    1 % of mean(b)."""
+    ma, mb = a.mean(), b.mean()
+    se = math.hypot(a.std(ddof=1) / math.sqrt(a.size), b.std(ddof=1) / math.sqrt(b.size))
+    print(f"{what}: {ma:.5e} vs {mb:.5e} (sigma of the difference {se:.2e})", flush=True)
+    assert abs(ma - mb) <= 4 * se + 0.01 * abs(mb), what
+
+
+def test_mixed_batch_against_oracle(oracle):
+    clouds = mixed_clouds()
+    init = [c["ensemble"].copy() for c in clouds]
+    stats = batch_run(clouds)
+    for c, e0, st in zip(clouds, init, stats):
+        o = e0.copy()
+        so = oracle_cloud(oracle, c, o)
+        g = c["ensemble"]
+        fp64 = c["physics"].precision == _abi.PRECISION_FP64
+        stochastic = c["physics"].collision_model != _abi.COLLISION_NONE
+        tag = f"{c['kind']} {'fp64' if fp64 else 'fp32'} n={g.n} steps={c['n_steps']}"
+        assert st.ion_steps == pytest.approx(so.ion_steps, rel=0.01 if stochastic else 0), tag
+        if fp64:
+            ok = agree(g, o, 1e-9, 1e-6, 1.0) & (g.n_collisions == o.n_collisions)
+            print(f"{tag}: {ok.mean():.4f} of ions equal the oracle to 1e-9", flush=True)
+            assert ok.mean() >= (0.99 if stochastic else 1.0), tag
+            if not stochastic:
+                np.testing.assert_array_equal(g.death_time_s, o.death_time_s)
+        elif not stochastic:
+            ok = agree(g, o, 2e-5, 1e-3, 10.0)
+            print(f"{tag}: {ok.mean():.4f} of ions within 2e-5 of the FP64 oracle", flush=True)
+            assert ok.mean() >= 0.999, tag
+        else:
+            z0 = e0.z
+            sg, so_ = g.alive.mean(), o.alive.mean()
+            s_err = 4 * math.sqrt(2 * max(so_ * (1 - so_), 1.0 / g.n) / g.n) + 0.01
+            print(f"{tag}: survival {sg:.4f} vs {so_:.4f}", flush=True)
+            assert abs(sg - so_) <= s_err, tag
+            close_mean(((g.z - z0)[g.alive]), ((o.z - z0)[o.alive]), tag + " drift")
+            close_mean(g.n_collisions.astype(float), o.n_collisions.astype(float),
+                       tag + " collisions")
+            assert stats.ks_2samp(g.z[g.alive], o.z[o.alive]).pvalue > 1e-3, tag
+
+
+def test_batch_orbitrap_1e6_steps_against_fp64_oracle(oracle):
+    """One Orbitrap cloud of 192 ions at the batched workload's 1e6 steps
+    (1 ms, dt 1 ns), in FP64 and in FP32 + Kahan (FP64 time and RF phase) as two
+    clouds of one launch, against the FP64 oracle.  Stated errors (position
+    against 9 mm, velocity against the ion's speed, azimuth in rad): FP64
+    within 2e-9 at the maximum (measured 9.7e-10 / 9.0e-10 / 9.7e-10); FP32 +
+    Kahan within 1e-3 at the median and 1e-2 at the maximum (measured median
+    4.7e-4 / 4.4e-4 / 4.7e-4, maximum 4.0e-3 / 3.6e-3 / 3.9e-3).  The FP32
+    phase error grows linearly with the number of orbits, about 1.5e-7 of the
+    orbital phase: the rounding of the force constants to float is a
+    systematic frequency offset that compensated summation of the state does
+    not remove."""
+    w = W.orbitrap_workload(n=192, seed=41)
+    clouds = []
+    for prec in (_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN):
+        ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, precision=prec, seed=41)
+        clouds.append(dict(kind="orbitrap", domain=w["domain"], physics=ph, species=w["species"],
+                           ensemble=w["ensemble"].copy(), t0=0.0, dt=w["dt"],
+                           n_steps=1_000_000, step_offset=0))
+    print("orbitrap 192 ions x 1e6 steps: batch (FP64 + FP32 clouds), then the oracle",
+          flush=True)
+    stats = batch_run(clouds)
+    o = w["ensemble"].copy()
+    so = oracle.run(w["domain"], w["physics"], w["species"], o, 0.0, w["dt"], 1_000_000)
+    assert so.alive == 192
+    r_scale = 9e-3
+    v_scale = np.sqrt(o.vx ** 2 + o.vy ** 2 + o.vz ** 2)
+    res = {}
+    for c, st in zip(clouds, stats):
+        g = c["ensemble"]
+        assert st.ion_steps == so.ion_steps and st.alive == 192
+        dp = np.sqrt((g.x - o.x) ** 2 + (g.y - o.y) ** 2 + (g.z - o.z) ** 2) / r_scale
+        dv = np.sqrt((g.vx - o.vx) ** 2 + (g.vy - o.vy) ** 2 + (g.vz - o.vz) ** 2) / v_scale
+        dphi = np.abs(np.angle(np.exp(1j * (np.arctan2(g.y, g.x) - np.arctan2(o.y, o.x)))))
+        name = "fp64" if c["physics"].precision == _abi.PRECISION_FP64 else "fp32"
+        res[name] = (np.median(dp), dp.max(), np.median(dv), dv.max(), np.median(dphi),
+                     dphi.max())
+        print(f"orbitrap 1e6 steps {name} vs FP64 oracle: position/9mm median {res[name][0]:.2e} "
+              f"max {res[name][1]:.2e}; velocity median {res[name][2]:.2e} max "
+              f"{res[name][3]:.2e}; azimuth median {res[name][4]:.2e} max {res[name][5]:.2e} rad",
+              flush=True)
+    assert res["fp64"][1] < 2e-9 and res["fp64"][3] < 2e-9 and res["fp64"][5] < 2e-9
+    assert res["fp32"][0] < 1e-3 and res["fp32"][1] < 1e-2
+    assert res["fp32"][2] < 1e-3 and res["fp32"][3] < 1e-2
+    assert res["fp32"][4] < 1e-3 and res["fp32"][5] < 1e-2

@@ -363,6 +363,52 @@ def test_field_array_fp64(oracle):
     assert compare_ions(g, o, rtol=1e-9).mean() >= 0.99
 
 
+def test_field_array_fp32_collisionless_against_fp64_oracle(oracle):
+    """The uniform single-grid FP32 + Kahan kernel (step_f32_kernel<RK4,
+    FIELD_ARRAY, ...>, the field-array workload's fast path) without gas: ion by
+    ion against the FP64 oracle's trilinear interpolation over 3000 steps.
+    Errors as vector norms: position against 1 mm (median 1e-6, max 1e-5),
+    velocity against the ion's speed (median 1e-6, max 2e-5); the same fate for
+    99.9 % of ions.  (A CPU float32 emulation of the kernel gives medians
+    2.7e-7 / 1.1e-7 and maxima 1.3e-6 / 4.6e-6 on this workload.)"""
+    w = W.field_array_workload(n=4096, grid=(24, 20, 40), seed=2)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, seed=2)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, g, 3000, precision=_abi.PRECISION_FP32_KAHAN)
+    oracle_run(oracle, w, o, 3000)
+    assert (g.active == o.active).mean() >= 0.999
+    both = g.alive & o.alive
+    dp = np.linalg.norm(g.pos - o.pos, axis=1)[both] / 1e-3
+    speed = np.maximum(np.linalg.norm(o.vel, axis=1), 10.0)
+    dv = (np.linalg.norm(g.vel - o.vel, axis=1) / speed)[both]
+    print("field array FP32 vs FP64 oracle: position/1mm median %.2e max %.2e; velocity/speed "
+          "median %.2e max %.2e" % (np.median(dp), dp.max(), np.median(dv), dv.max()))
+    assert np.median(dp) < 1e-6 and dp.max() < 1e-5
+    assert np.median(dv) < 1e-6 and dv.max() < 2e-5
+    # the ions crossed cells: the test sees the interpolation, not one cell
+    assert np.abs(g.z - w["ensemble"].z).max() > 50e-3 / 39
+
+
+def test_field_array_fp32_hss_statistics_against_fp64_oracle(oracle):
+    """The uniform-grid FP32 + Kahan kernel with HSS in N2 (the field-array
+    workload's kernel, step_f32_kernel<RK4, FIELD_ARRAY, HSS>) against the FP64
+    oracle: survival 0.5 %, mean drift 1 %, drift spread 3 %, collisions per
+    ion 0.5 %, and a two-sample KS test on the final z (p > 1e-3)."""
+    w = W.field_array_workload(n=20000, grid=(24, 20, 40), seed=4)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    z0 = w["ensemble"].z.copy()
+    sg = gpu_run(w, g, 4000, precision=_abi.PRECISION_FP32_KAHAN)
+    so = oracle_run(oracle, w, o, 4000)
+    assert abs(g.alive.mean() - o.alive.mean()) < 0.005
+    dzg = (g.z - z0)[g.alive]
+    dzo = (o.z - z0)[o.alive]
+    assert abs(dzo.mean()) > 5 * dzo.std() / math.sqrt(dzo.size)  # a real drift to compare
+    assert dzg.mean() == pytest.approx(dzo.mean(), rel=0.01)
+    assert dzg.std() == pytest.approx(dzo.std(), rel=0.03)
+    assert stats.ks_2samp(g.z[g.alive], o.z[o.alive]).pvalue > 1e-3
+    assert sg.collisions / sg.ion_steps == pytest.approx(so.collisions / so.ion_steps, rel=0.005)
+
+
 @pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
 def test_field_terms_nonuniform_collisionless_fp64(oracle, integrator):
     """Composite field (DC + RF + constant terms) on non-uniform axes: ion by
