@@ -68,14 +68,25 @@ public:
     bool supports_batch() const { return h_ != nullptr && ready_; }
 
     // Throwing configuration calls.
+    // atoms_xyzr: the EHSS hard spheres, float x, y, z, r per atom (the device's
+    // atom table is float; icarion::config::species_table builds it).
     void configure(const icarion_domain_desc& domain, const icarion_physics_desc& physics,
                    const std::vector<icarion_species_desc>& species,
-                   const std::vector<double>& atoms_xyzr = {}) {
+                   const std::vector<float>& atoms_xyzr = {}) {
         check(icarion_gpu_set_domain(h_, &domain, &physics));
         check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()),
                                       atoms_xyzr.empty() ? nullptr : atoms_xyzr.data(),
-                                      static_cast<uint32_t>(atoms_xyzr.size() / 4), 1));
+                                      static_cast<uint32_t>(atoms_xyzr.size() / 4), 0));
         ready_ = true;
+    }
+    // A gas mixture: components, sigma per [species][component] and (EHSS) the
+    // components' contact radii (icarion_gpu_set_gas_mixture / _set_gas_radii).
+    void set_gas_mixture(const std::vector<icarion_gas_component>& components, const std::vector<double>& sigma_m2,
+                         uint32_t n_species, const std::vector<double>& radius_m = {}) {
+        check(icarion_gpu_set_gas_mixture(h_, components.data(), static_cast<uint32_t>(components.size()),
+                                          sigma_m2.data(), n_species));
+        if (!radius_m.empty())
+            check(icarion_gpu_set_gas_radii(h_, radius_m.data(), static_cast<uint32_t>(radius_m.size())));
     }
     // Several domains (icarion_gpu_set_domains): one descriptor and end aperture
     // per DomainConfig, in the config's order.
@@ -85,7 +96,7 @@ public:
             throw Error(ICARION_ERR_INVALID, "one end aperture per domain");
         check(icarion_gpu_set_domains(h_, domains.data(), end_aperture_m.empty() ? nullptr : end_aperture_m.data(),
                                       static_cast<uint32_t>(domains.size()), &physics));
-        check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()), nullptr, 0, 1));
+        check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()), nullptr, 0, 0));
         ready_ = true;
     }
     // Step control of ICARION_INTEGRATOR_RK45 (RK45Strategy(const AdaptiveConfig&)).

@@ -90,17 +90,28 @@ def test_ctypes_layout_matches_c_compiler(tmp_path):
 
 
 def test_cpp_adaptor_compiles_and_links(tmp_path):
-    """include/icarion_gpu.hpp compiles as C++17 and links against the built
-    library; without a device the never-throwing factory yields an empty
-    stepper whose step_batch() reports "not handled"."""
+    """include/icarion_gpu.hpp and include/icarion_config.hpp compile as C++17
+    and link against the built library; without a device the never-throwing
+    factory yields an empty stepper whose step_batch() reports "not handled"."""
     lib = gpu.load()
     so = lib._name
     src = tmp_path / "adaptor.cpp"
     src.write_text(
         '#include "icarion_gpu.hpp"\n'
+        '#include "icarion_config.hpp"\n'
         "#include <cstdio>\n"
         "int main() {\n"
         "    icarion::BatchStepper s = icarion::BatchStepper::try_create(icarion_gpu_desc{0, 0, 0, 0});\n"
+        "    if (s.handle()) {  // the config helpers feed the adaptor's configure calls\n"
+        "        icarion::config::Domain d;\n"
+        "        d.geometry.length_m = 0.02; d.geometry.radius_m = 0.01; d.fields.EN_Td = 10.0;\n"
+        "        d.env.derive();\n"
+        "        icarion::config::Physics p; p.collision_model = \"HSS\";\n"
+        "        icarion::config::SpeciesRecord r; r.id = \"X\"; r.CCS_A2 = 100.0;\n"
+        "        const auto t = icarion::config::species_table({r}, d, p);\n"
+        "        s.configure(icarion::config::to_desc(d), icarion::config::to_physics({}, p), t.species,\n"
+        "                    t.atoms_xyzr);\n"
+        "    }\n"
         "    icarion_ion_soa soa{};\n"
         "    const bool handled = s.step_batch(soa, 0.0, 1e-9, 0);\n"
         "    int code = -1;\n"
