@@ -145,6 +145,318 @@ __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const
     add_counters(P, pushed, ncoll_launch);
 }
 
+// ======================= FP64 geometry EHSS, wave-cooperative ===============
+//
+// The plain EHSS method of ehss_f64 / ehss_core_f64 with the work of one
+// collision spread over the whole wavefront.  Each lane pushes its own ion and
+// runs its own collision test (the handler's Maxwellian neutral and
+// P = 1 - exp(-n sigma_eff |v_rel| dt)); then the wavefront takes the ions that
+// collide one after another, and for each of them:
+//   * its random stream is handed to every lane (the stream is counter-based,
+//     so every lane draws the same numbers the ion's own lane would);
+//   * atom j of its cluster is handled by lane j mod 64: b_max is a wave max
+//     over the rotated atoms' radial extents, and for each impact parameter
+//     the first contact is a wave min over (s_hit, j) - the smallest s_hit,
+//     the lowest atom index among equal ones, which is the sequential loop's
+//     choice (CollisionKernels.cpp:283-381);
+//   * every lane forms the contact normal of the winning atom with the
+//     sequential loop's arithmetic, and the reflected velocity; the ion's lane
+//     keeps it.
+// So the wavefront spends the atoms x attempts work of the collisions it has,
+// instead of waiting on its slowest lane while the lanes without a collision
+// idle.
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t x, int lane) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), lane));
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ d3 readlane_d3(d3 v, int lane) {
+    return d3{readlane_f64(v.x, lane), readlane_f64(v.y, lane), readlane_f64(v.z, lane)};
+}
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// (s, j) <- the lexicographic minimum over the wavefront, on every lane
+__device__ __forceinline__ void wave_min_contact(double& s, uint32_t& j) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double so = __shfl_xor(s, off, 64);
+        const uint32_t jo = __shfl_xor(j, off, 64);
+        if (so < s || (so == s && jo < j)) {
+            s = so;
+            j = jo;
+        }
+    }
+}
+
+// ehss_f64 of one ion, called by every lane of the wavefront with the same
+// (wave-uniform) arguments; returns the ion's new velocity on every lane.
+__device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, double r_gas,
+                            uint32_t a0, uint32_t nat, PhiloxStream& rng, uint32_t lane) {
+    const double u1 = rng.uniform_f64();
+    const double u2 = rng.uniform_f64();
+    const double u3 = rng.uniform_f64();
+    if (nat == 0) return v_ion;
+    const double q1 = sqrt(1.0 - u1) * sin(2.0 * kPi * u2);
+    const double q2 = sqrt(1.0 - u1) * cos(2.0 * kPi * u2);
+    const double q3 = sqrt(u1) * sin(2.0 * kPi * u3);
+    const double q4 = sqrt(u1) * cos(2.0 * kPi * u3);
+    const double x = q1, y = q2, z = q3, w = q4;
+    double O[9];
+    O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
+    O[3] = 2 * (x * y + z * w);     O[4] = 1 - 2 * (x * x + z * z); O[5] = 2 * (y * z - x * w);
+    O[6] = 2 * (x * z - y * w);     O[7] = 2 * (y * z + x * w);     O[8] = 1 - 2 * (x * x + y * y);
+    d3 vrel = v_ion - v_n;
+    double vrel_mag = sqrt(dot(vrel, vrel));
+    if (vrel_mag <= kMinVelocityMag) {
+        vrel = iso_dir_f64(rng) * kMinVelocityMag;
+        vrel_mag = kMinVelocityMag;
+    }
+    const d3 ax{vrel.x / vrel_mag, vrel.y / vrel_mag, vrel.z / vrel_mag};
+    d3 t1, t2;
+    ortho_basis_f64(ax, t1, t2);
+    const double Bm[3][3] = {{t1.x, t2.x, ax.x}, {t1.y, t2.y, ax.y}, {t1.z, t2.z, ax.z}};
+    double Rm[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            Rm[3 * a + b] = Bm[a][0] * O[b] + Bm[a][1] * O[3 + b] + Bm[a][2] * O[6 + b];
+    const float4* __restrict__ atoms = P.atoms + a0;
+    double bm = 0.0;
+    for (uint32_t j = lane; j < nat; j += 64) {
+        const float4 f = atoms[j];
+        const d3 ra = mat_mul(Rm, d3{f.x, f.y, f.z});
+        const double sra = dot(ax, ra);
+        const d3 rp = ra - ax * sra;
+        const double cur = sqrt(dot(rp, rp)) + static_cast<double>(f.w);
+        bm = fmax(bm, cur);
+    }
+    double b_max = wave_max_f64(bm) + r_gas;
+    for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
+        const double uA = rng.uniform_f64();
+        const double uB = rng.uniform_f64();
+        const double b = b_max * sqrt(uA);
+        double sphi, cphi;
+        sincos(2.0 * kPi * uB, &sphi, &cphi);
+        const d3 off = t1 * (b * cphi) + t2 * (b * sphi);
+        double best_s = INFINITY;
+        uint32_t best_j = 0xFFFFFFFFu;
+        for (uint32_t j = lane; j < nat; j += 64) {
+            const float4 f = atoms[j];
+            const d3 ra = mat_mul(Rm, d3{f.x, f.y, f.z});
+            const double Rsum = static_cast<double>(f.w) + r_gas;
+            const double Rsum2 = Rsum * Rsum;
+            const d3 rel = off - ra;
+            const double sstar = dot(ax, rel);
+            const d3 dv = rel - ax * sstar;
+            const double dmin2 = dot(dv, dv);
+            if (dmin2 <= Rsum2) {
+                const double s_hit = sstar - sqrt(fmax(0.0, Rsum2 - dmin2));
+                if (s_hit < best_s) {
+                    best_s = s_hit;
+                    best_j = j;
+                }
+            }
+        }
+        wave_min_contact(best_s, best_j);
+        if (best_j != 0xFFFFFFFFu) {
+            // the contact normal of the first atom hit, as the sequential loop forms it
+            const float4 f = atoms[best_j];
+            const d3 ra = mat_mul(Rm, d3{f.x, f.y, f.z});
+            const double Rsum = static_cast<double>(f.w) + r_gas;
+            const double Rsum2 = Rsum * Rsum;
+            const d3 rel = off - ra;
+            const double sstar = dot(ax, rel);
+            const d3 dv = rel - ax * sstar;
+            const double dmin2 = dot(dv, dv);
+            const double h = sqrt(fmax(0.0, Rsum2 - dmin2));
+            const double s_hit = sstar - h;
+            const d3 ph = rel - ax * s_hit;
+            const double phr2 = dot(ph, ph);
+            d3 nb;
+            if (phr2 <= kMinContactDistSq) {
+                nb = ax * -1.0;
+            } else {
+                nb = ph * (1.0 / sqrt(phr2));
+                if (dot(ax, nb) > 0.0) nb = nb * -1.0;
+            }
+            const double vdn = dot(vrel, nb);
+            const d3 vr = vrel - nb * (2.0 * vdn);
+            const double inv_mt = 1.0 / (mi + mn);
+            const d3 Vcom = (v_ion * mi + v_n * mn) * inv_mt;
+            return Vcom + vr * (mn * inv_mt);
+        }
+        if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
+    }
+    return v_ion;
+}
+
+// step_f64_kernel<INTEG, FIELD, COLL_EHSS> with the geometry method run by
+// the wavefront (above).  Every lane stays in the step loop until no lane of
+// its wavefront has an ion left to step (lanes past the last ion, dead ions
+// and ions done are carried along), so the wave-wide parts see all 64 lanes.
+template <int INTEG, int FIELD>
+__global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, const IonPtrs S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool have = i < P.n_ions;
+    d3 pos{0.0, 0.0, 0.0}, vel{0.0, 0.0, 0.0};
+    double m = 1.0, q = 0.0, birth = -INFINITY, death = -1.0, arrival = NAN;
+    uint32_t sidx = 0, nc = 0;
+    bool active = false, born = true;
+    if (have) {
+        pos = d3{S.x[i], S.y[i], S.z[i]};
+        vel = d3{S.vx[i], S.vy[i], S.vz[i]};
+        m = S.mass[i];
+        q = S.charge[i];
+        sidx = S.species[i];
+        const uint8_t flags = S.flags[i];
+        active = flags & 1;
+        born = (flags >> 1) & 1;
+        birth = S.birth ? S.birth[i] : -INFINITY;
+        death = S.death[i];
+        arrival = S.arrival[i];
+        nc = S.ncoll[i];
+    }
+    const SpeciesDev sp = P.species[sidx];
+    const double inv_m = 1.0 / m;
+    uint32_t pushed = 0, ncoll_launch = 0;
+    bool checked = false;
+    const double dt = P.dt;
+    const double half_dt = dt * 0.5;
+    for (uint32_t k = 0; k < P.n_steps; ++k) {
+        // unborn ions wait for their birth, live ones step; none left: the
+        // whole wavefront leaves together
+        if (__ballot(have && (!born || active)) == 0ull) break;
+        const StepEntry te = P.table[k];
+        const double t = te.t;
+        bool live = false;
+        if (have) {
+            if (!born && t >= birth) {
+                born = true;
+                active = true;
+                checked = false;
+            }
+            if (born && active) {
+                if (!checked && !contains_f64(P, pos)) {
+                    active = false;
+                    death = t;
+                } else {
+                    live = true;
+                }
+            }
+        }
+        const double z_before = pos.z;
+        if (live) {
+            if constexpr (INTEG == 0) {
+                const d3 p0 = pos, v0 = vel;
+                const d3 k1v = v0;
+                const d3 k1a = force_f64<FIELD, COLL_EHSS>(P, p0, v0, te.c0, te.e0, q, m, 0.0) * inv_m;
+                const d3 k2v = v0 + k1a * half_dt;
+                const d3 k2a = force_f64<FIELD, COLL_EHSS>(P, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, 0.0) * inv_m;
+                const d3 k3v = v0 + k2a * half_dt;
+                const d3 k3a = force_f64<FIELD, COLL_EHSS>(P, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, 0.0) * inv_m;
+                const d3 k4v = v0 + k3a * dt;
+                const d3 k4a = force_f64<FIELD, COLL_EHSS>(P, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, 0.0) * inv_m;
+                const double h6 = dt / 6.0;
+                pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
+                vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
+            } else {
+                const d3 a = force_f64<FIELD, COLL_EHSS>(P, pos, vel, te.c0, te.e0, q, m, 0.0) * inv_m;
+                const d3 vm = vel + a * half_dt;
+                d3 vp = vm;
+                if (P.magnetic) {
+                    const double qm = q * inv_m;
+                    const d3 tv = d3{P.B[0], P.B[1], P.B[2]} * (qm * dt * 0.5);
+                    const double s_f = 2.0 / (1.0 + dot(tv, tv));
+                    const d3 sv = tv * s_f;
+                    const d3 vprime = vm + cross(vm, tv);
+                    vp = vm + cross(vprime, sv);
+                }
+                pos = pos + vp * dt;
+                vel = vp + a * half_dt;
+            }
+            ++pushed;
+        }
+        // the collision test of each lane (collide_f64<COLL_EHSS> up to the
+        // geometry method)
+        PhiloxStream rng(P.seed, i, P.step0 + k);
+        bool hit = false;
+        d3 vn{0.0, 0.0, 0.0};
+        if (live && sp.sigma_ehss > 0.0) {
+            double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
+            vn.x = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+            vn.y = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
+            vn.z = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
+            vn = vn + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+            const d3 vrel = vel - vn;
+            const double vrel_mag = sqrt(dot(vrel, vrel));
+            if (vrel_mag >= kMinRelativeVelocity) {
+                const double Pc = 1.0 - exp(-P.n * sp.sigma_ehss * vrel_mag * dt);
+                hit = rng.uniform_f64() < Pc;
+            }
+        }
+        // the geometry method of each colliding ion, on the whole wavefront
+        uint64_t pending = __ballot(hit);
+        while (pending) {
+            const int L = __builtin_amdgcn_readfirstlane(static_cast<int>(__builtin_ctzll(pending)));
+            pending &= pending - 1ull;
+            PhiloxStream r(P.seed, readlane_u32(i, L), P.step0 + k);
+            r.block = readlane_u32(rng.block, L);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) r.buf[c] = readlane_u32(rng.buf[c], L);
+            r.pos = static_cast<int>(readlane_u32(static_cast<uint32_t>(rng.pos), L));
+            const d3 v_new = ehss_wave_f64(P, readlane_d3(vel, L), readlane_d3(vn, L), readlane_f64(m, L),
+                                           P.m_gas, P.r_gas, readlane_u32(sp.atom_offset, L),
+                                           readlane_u32(sp.atom_count, L), r, lane);
+            if (static_cast<int>(lane) == L) {
+                vel = v_new;
+                ++nc;
+                ++ncoll_launch;
+            }
+        }
+        if (live) {
+            if (P.detect && isnan(arrival)) {
+                if (z_before < P.det_z && pos.z >= P.det_z)
+                    arrival = t + dt * (P.det_z - z_before) / (pos.z - z_before);
+            }
+            if (!contains_f64(P, pos)) {
+                active = false;
+                death = t;
+            } else {
+                checked = true;
+                if (!(isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z) && isfinite(vel.x) &&
+                      isfinite(vel.y) && isfinite(vel.z))) {
+                    active = false;
+                    death = t;
+                }
+            }
+        }
+    }
+    if (have) {
+        S.x[i] = pos.x; S.y[i] = pos.y; S.z[i] = pos.z;
+        S.vx[i] = vel.x; S.vy[i] = vel.y; S.vz[i] = vel.z;
+        S.flags[i] = static_cast<uint8_t>((active ? 1 : 0) | (born ? 2 : 0));
+        S.death[i] = death;
+        S.arrival[i] = arrival;
+        S.ncoll[i] = nc;
+    }
+    add_counters(P, pushed, ncoll_launch);
+}
+
 // ======================= FP32 + Kahan (fast path) ===========================
 
 // the FP32 helpers (fields, containment, collisions) are in step_device_f32.hpp
@@ -850,7 +1162,7 @@ KernelFn pick_coll(int coll, int precision) {
     switch (coll) {
         case COLL_HSS: return pick_prec<I, F, COLL_HSS>(precision);
         case COLL_HSS_MIX: return pick_prec<I, F, COLL_HSS_MIX>(precision);
-        case COLL_EHSS: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS>;
+        case COLL_EHSS: return precision == 1 ? nullptr : step_f64_ehss_kernel<I, F>;
         case COLL_EHSS_TAB: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS_TAB>;
         case COLL_EHSS_MIX: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS_MIX>;
         case COLL_DAMPING: return pick_prec<I, F, COLL_DAMPING>(precision);

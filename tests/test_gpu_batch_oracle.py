@@ -8,8 +8,9 @@ Bars:
   * FP64, stochastic: >= 99 % of ions to 1e-9 with equal collision counts (a
     collision test within rounding of its threshold can send one ion on
     another path);
-  * FP32 + Kahan, collisionless: >= 99.9 % of ions to 2e-5 (positions on a
-    1 mm scale, velocities on 10 m/s);
+  * FP32 + Kahan, collisionless (Orbitrap): the same fate for >= 99.9 % of
+    ions, position within 2e-5 of the 9 mm orbit and velocity within 2e-5 of
+    the speed (vector norms);
   * FP32 + Kahan, stochastic: the FP32 collision and kick code draws its
     normals from both Box-Muller branches, so its random numbers map to other
     events than the FP64 code's and the paths are independent samples:
@@ -103,8 +104,8 @@ def close_mean(a, b, what):
 def test_mixed_batch_against_oracle(oracle):
     clouds = mixed_clouds()
     init = [c["ensemble"].copy() for c in clouds]
-    stats = batch_run(clouds)
-    for c, e0, st in zip(clouds, init, stats):
+    cloud_stats = batch_run(clouds)
+    for c, e0, st in zip(clouds, init, cloud_stats):
         o = e0.copy()
         so = oracle_cloud(oracle, c, o)
         g = c["ensemble"]
@@ -119,9 +120,14 @@ def test_mixed_batch_against_oracle(oracle):
             if not stochastic:
                 np.testing.assert_array_equal(g.death_time_s, o.death_time_s)
         elif not stochastic:
-            ok = agree(g, o, 2e-5, 1e-3, 10.0)
-            print(f"{tag}: {ok.mean():.4f} of ions within 2e-5 of the FP64 oracle", flush=True)
-            assert ok.mean() >= 0.999, tag
+            # vector errors: position against the 9 mm orbit, velocity against the speed
+            both = g.alive & o.alive
+            dp = (np.linalg.norm(g.pos - o.pos, axis=1) / 9e-3)[both]
+            dv = (np.linalg.norm(g.vel - o.vel, axis=1) / np.linalg.norm(o.vel, axis=1))[both]
+            print(f"{tag}: position/9mm max {dp.max():.2e}, velocity/speed max {dv.max():.2e} "
+                  f"against the FP64 oracle", flush=True)
+            assert (g.active == o.active).mean() >= 0.999, tag
+            assert dp.max() < 2e-5 and dv.max() < 2e-5, tag
         else:
             z0 = e0.z
             sg, so_ = g.alive.mean(), o.alive.mean()
@@ -155,14 +161,14 @@ def test_batch_orbitrap_1e6_steps_against_fp64_oracle(oracle):
                            n_steps=1_000_000, step_offset=0))
     print("orbitrap 192 ions x 1e6 steps: batch (FP64 + FP32 clouds), then the oracle",
           flush=True)
-    stats = batch_run(clouds)
+    cloud_stats = batch_run(clouds)
     o = w["ensemble"].copy()
     so = oracle.run(w["domain"], w["physics"], w["species"], o, 0.0, w["dt"], 1_000_000)
     assert so.alive == 192
     r_scale = 9e-3
     v_scale = np.sqrt(o.vx ** 2 + o.vy ** 2 + o.vz ** 2)
     res = {}
-    for c, st in zip(clouds, stats):
+    for c, st in zip(clouds, cloud_stats):
         g = c["ensemble"]
         assert st.ion_steps == so.ion_steps and st.alive == 192
         dp = np.sqrt((g.x - o.x) ** 2 + (g.y - o.y) ** 2 + (g.z - o.z) ** 2) / r_scale
