@@ -199,18 +199,28 @@ __device__ __forceinline__ void wave_min_contact(double& s, uint32_t& j) {
     }
 }
 
+// Rotated atoms of the collision in progress, per wavefront: the first
+// kWaveAtoms atoms of a cluster (x, y, z in the collision frame and
+// (r_atom + r_gas)^2), each written and read by its own lane; atoms past that
+// are rotated again from the table on every impact parameter.
+constexpr uint32_t kWaveAtoms = 512;
+
 // ehss_f64 of one ion, called by every lane of the wavefront with the same
 // (wave-uniform) arguments; returns the ion's new velocity on every lane.
+// lds: this wavefront's kWaveAtoms-entry staging area.
 __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, double r_gas,
-                            uint32_t a0, uint32_t nat, PhiloxStream& rng, uint32_t lane) {
+                            uint32_t a0, uint32_t nat, PhiloxStream& rng, uint32_t lane, double4* lds) {
     const double u1 = rng.uniform_f64();
     const double u2 = rng.uniform_f64();
     const double u3 = rng.uniform_f64();
     if (nat == 0) return v_ion;
-    const double q1 = sqrt(1.0 - u1) * sin(2.0 * kPi * u2);
-    const double q2 = sqrt(1.0 - u1) * cos(2.0 * kPi * u2);
-    const double q3 = sqrt(u1) * sin(2.0 * kPi * u3);
-    const double q4 = sqrt(u1) * cos(2.0 * kPi * u3);
+    double s2, c2, s3, c3;
+    sincos(2.0 * kPi * u2, &s2, &c2);
+    sincos(2.0 * kPi * u3, &s3, &c3);
+    const double q1 = sqrt(1.0 - u1) * s2;
+    const double q2 = sqrt(1.0 - u1) * c2;
+    const double q3 = sqrt(u1) * s3;
+    const double q4 = sqrt(u1) * c3;
     const double x = q1, y = q2, z = q3, w = q4;
     double O[9];
     O[0] = 1 - 2 * (y * y + z * z); O[1] = 2 * (x * y - z * w);     O[2] = 2 * (x * z + y * w);
@@ -241,6 +251,10 @@ __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
         const d3 rp = ra - ax * sra;
         const double cur = sqrt(dot(rp, rp)) + static_cast<double>(f.w);
         bm = fmax(bm, cur);
+        if (j < kWaveAtoms) {
+            const double Rsum = static_cast<double>(f.w) + r_gas;
+            lds[j] = make_double4(ra.x, ra.y, ra.z, Rsum * Rsum);
+        }
     }
     double b_max = wave_max_f64(bm) + r_gas;
     for (int attempt = 0; attempt < kEhssMaxAttempts; ++attempt) {
@@ -253,10 +267,18 @@ __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
         double best_s = INFINITY;
         uint32_t best_j = 0xFFFFFFFFu;
         for (uint32_t j = lane; j < nat; j += 64) {
-            const float4 f = atoms[j];
-            const d3 ra = mat_mul(Rm, d3{f.x, f.y, f.z});
-            const double Rsum = static_cast<double>(f.w) + r_gas;
-            const double Rsum2 = Rsum * Rsum;
+            d3 ra;
+            double Rsum2;
+            if (j < kWaveAtoms) {
+                const double4 a = lds[j];
+                ra = d3{a.x, a.y, a.z};
+                Rsum2 = a.w;
+            } else {
+                const float4 f = atoms[j];
+                ra = mat_mul(Rm, d3{f.x, f.y, f.z});
+                const double Rsum = static_cast<double>(f.w) + r_gas;
+                Rsum2 = Rsum * Rsum;
+            }
             const d3 rel = off - ra;
             const double sstar = dot(ax, rel);
             const d3 dv = rel - ax * sstar;
@@ -269,8 +291,13 @@ __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
                 }
             }
         }
+        // no lane found a contact: the next impact parameter (wave-uniform)
+        if (__ballot(best_j != 0xFFFFFFFFu) == 0ull) {
+            if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
+            continue;
+        }
         wave_min_contact(best_s, best_j);
-        if (best_j != 0xFFFFFFFFu) {
+        {
             // the contact normal of the first atom hit, as the sequential loop forms it
             const float4 f = atoms[best_j];
             const d3 ra = mat_mul(Rm, d3{f.x, f.y, f.z});
@@ -297,7 +324,6 @@ __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
             const d3 Vcom = (v_ion * mi + v_n * mn) * inv_mt;
             return Vcom + vr * (mn * inv_mt);
         }
-        if (attempt == kBmaxExpansionAttempt) b_max *= kBmaxExpansionFactor;
     }
     return v_ion;
 }
@@ -335,6 +361,8 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
     bool checked = false;
     const double dt = P.dt;
     const double half_dt = dt * 0.5;
+    __shared__ double4 s_atoms[4][kWaveAtoms];   // 256 threads = 4 wavefronts
+    double4* const lds = s_atoms[threadIdx.x >> 6];
     for (uint32_t k = 0; k < P.n_steps; ++k) {
         // unborn ions wait for their birth, live ones step; none left: the
         // whole wavefront leaves together
@@ -421,7 +449,7 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
             r.pos = static_cast<int>(readlane_u32(static_cast<uint32_t>(rng.pos), L));
             const d3 v_new = ehss_wave_f64(P, readlane_d3(vel, L), readlane_d3(vn, L), readlane_f64(m, L),
                                            P.m_gas, P.r_gas, readlane_u32(sp.atom_offset, L),
-                                           readlane_u32(sp.atom_count, L), r, lane);
+                                           readlane_u32(sp.atom_count, L), r, lane, lds);
             if (static_cast<int>(lane) == L) {
                 vel = v_new;
                 ++nc;
