@@ -205,11 +205,36 @@ __device__ __forceinline__ void wave_min_contact(double& s, uint32_t& j) {
 // are rotated again from the table on every impact parameter.
 constexpr uint32_t kWaveAtoms = 512;
 
+// A per-(ion, step) stream whose state is wave-uniform: the draws of one ion
+// replayed by every lane, with its state and the Philox rounds held in scalar
+// registers (the same sequence as PhiloxStream).
+struct WaveStream {
+    uint32_t k0, k1, ion, s_lo, s_hi, block;
+    uint32_t buf[4];
+    int pos;
+
+    __device__ __forceinline__ void refill() {
+        const u32x4 o = philox4x32_10(__builtin_amdgcn_readfirstlane(block), __builtin_amdgcn_readfirstlane(ion),
+                                      __builtin_amdgcn_readfirstlane(s_lo), __builtin_amdgcn_readfirstlane(s_hi),
+                                      __builtin_amdgcn_readfirstlane(k0), __builtin_amdgcn_readfirstlane(k1));
+#pragma unroll
+        for (int c = 0; c < 4; ++c) buf[c] = __builtin_amdgcn_readfirstlane(o.v[c]);
+        ++block;
+        pos = 0;
+    }
+    __device__ __forceinline__ double uniform_f64() {
+        if (pos >= 4) refill();
+        const double u = uniform53(buf[pos], buf[pos + 1]);
+        pos += 2;
+        return u;
+    }
+};
+
 // ehss_f64 of one ion, called by every lane of the wavefront with the same
 // (wave-uniform) arguments; returns the ion's new velocity on every lane.
 // lds: this wavefront's kWaveAtoms-entry staging area.
 __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double mn, double r_gas,
-                            uint32_t a0, uint32_t nat, PhiloxStream& rng, uint32_t lane, double4* lds) {
+                            uint32_t a0, uint32_t nat, WaveStream& rng, uint32_t lane, double4* lds) {
     const double u1 = rng.uniform_f64();
     const double u2 = rng.uniform_f64();
     const double u3 = rng.uniform_f64();
@@ -229,7 +254,14 @@ __device__ d3 ehss_wave_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, do
     d3 vrel = v_ion - v_n;
     double vrel_mag = sqrt(dot(vrel, vrel));
     if (vrel_mag <= kMinVelocityMag) {
-        vrel = iso_dir_f64(rng) * kMinVelocityMag;
+        // iso_dir_f64 on the wave-uniform stream
+        const double w1 = rng.uniform_f64();
+        const double w2 = rng.uniform_f64();
+        const double cosT = 2.0 * w1 - 1.0;
+        const double sinT = sqrt(fmax(0.0, 1.0 - cosT * cosT));
+        double sp, cp;
+        sincos(2.0 * kPi * w2, &sp, &cp);
+        vrel = d3{sinT * cp, sinT * sp, cosT} * kMinVelocityMag;
         vrel_mag = kMinVelocityMag;
     }
     const d3 ax{vrel.x / vrel_mag, vrel.y / vrel_mag, vrel.z / vrel_mag};
@@ -363,11 +395,14 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
     const double half_dt = dt * 0.5;
     __shared__ double4 s_atoms[4][kWaveAtoms];   // 256 threads = 4 wavefronts
     double4* const lds = s_atoms[threadIdx.x >> 6];
+    // the next step's table entry is loaded one step ahead
+    StepEntry te_next = P.n_steps ? P.table[0] : StepEntry{};
     for (uint32_t k = 0; k < P.n_steps; ++k) {
         // unborn ions wait for their birth, live ones step; none left: the
         // whole wavefront leaves together
         if (__ballot(have && (!born || active)) == 0ull) break;
-        const StepEntry te = P.table[k];
+        const StepEntry te = te_next;
+        if (k + 1 < P.n_steps) te_next = P.table[k + 1];
         const double t = te.t;
         bool live = false;
         if (have) {
@@ -442,7 +477,12 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
         while (pending) {
             const int L = __builtin_amdgcn_readfirstlane(static_cast<int>(__builtin_ctzll(pending)));
             pending &= pending - 1ull;
-            PhiloxStream r(P.seed, readlane_u32(i, L), P.step0 + k);
+            WaveStream r;
+            r.k0 = rng.k0;
+            r.k1 = rng.k1;
+            r.s_lo = rng.s_lo;
+            r.s_hi = rng.s_hi;
+            r.ion = readlane_u32(i, L);
             r.block = readlane_u32(rng.block, L);
 #pragma unroll
             for (int c = 0; c < 4; ++c) r.buf[c] = readlane_u32(rng.buf[c], L);
@@ -695,8 +735,11 @@ __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, 
     const float dt = (float)P.dt;
     const float half_dt = 0.5f * dt;
     const float h6 = dt * (1.0f / 6.0f);
+    // the next step's table entry is loaded one step ahead
+    StepEntry te_next = P.n_steps ? P.table[0] : StepEntry{};
     for (uint32_t k = 0; k < P.n_steps; ++k) {
-        const StepEntry te = P.table[k];
+        const StepEntry te = te_next;
+        if (k + 1 < P.n_steps) te_next = P.table[k + 1];
         const double t = te.t;
         if (valid && !born && t >= birth) {
             born = true;

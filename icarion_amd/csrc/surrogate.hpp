@@ -16,7 +16,8 @@
 // Layer 1 uses v_mfma_f32_32x32x8_bf16 (K = 8: the six inputs, a constant 1
 // that carries the bias, a zero):
 //   A lane l: row l%32, k = 4*(l/32) + j;  B lane l: col l%32, k = 4*(l/32) + j.
-// Layers 2 and 3 use v_mfma_f32_32x32x16_bf16 (K = 16 per instruction): an
+// Layers 2 and 3 use v_mfma_f32_32x32x16_bf16 (K = 16 per instruction; layer
+// 2's bias is the accumulators' initial value, in fp32): an
 // operand slot (half h = l/32, element j = 0..7) holds the same k for A and B,
 // and the sum over k does not depend on which hidden unit a slot carries.  The
 // B operand of K-chunk c = 2*mb + q is taken straight from accumulator
@@ -45,18 +46,19 @@ constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
 constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
 
-// Packed inference blob (bytes): bf16 A fragments in lane order, then a bias.
+// Packed inference blob (bytes): bf16 A fragments in lane order, then biases.
 //   A1: [4 mb][64 lanes] x 4 bf16         ( 2048 B)  column 6 = b1
-//   A2: [4 mb][9 kc][64 lanes] x 8 bf16   (36864 B)  chunk 8 = b2 in slot (0, 0)
+//   A2: [4 mb][8 kc][64 lanes] x 8 bf16   (32768 B)
 //   A3: [8 kc][64 lanes] x 8 bf16         ( 8192 B)  rows 3..31 zero
+//   b2[128] float                         (  512 B)  layer 2's accumulators start at b2
 //   b3[32] float                          (  128 B)
-// Chunk 8 of layer 2 multiplies a constant operand that is 1 in slot (0, 0)
-// and 0 elsewhere, so the matrix cores add b2.
 constexpr int kPackA1 = 0;
 constexpr int kPackA2 = kPackA1 + 4 * 64 * 8;
-constexpr int kPackA3 = kPackA2 + 4 * 9 * 64 * 16;
-constexpr int kPackB3 = kPackA3 + 8 * 64 * 16;
+constexpr int kPackA3 = kPackA2 + 4 * 8 * 64 * 16;
+constexpr int kPackB2 = kPackA3 + 8 * 64 * 16;
+constexpr int kPackB3 = kPackB2 + kMlpH * 4;
 constexpr int kPackBytes = kPackB3 + 32 * 4;
+static_assert(kPackBytes % 16 == 0, "the blob is staged as uint4");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -99,16 +101,15 @@ inline void mlp_pack_blob(const float* p, unsigned char* blob) {
                 const float v = k < kMlpIn ? W1[row * kMlpIn + k] : (k == kMlpIn ? b1[row] : 0.f);
                 put(kPackA1 + (mb * 64 + lane) * 8, j, v);
             }
-    const float* b2 = p + kOffB2;
     for (int mb = 0; mb < 4; ++mb)
-        for (int kc = 0; kc < 9; ++kc)
+        for (int kc = 0; kc < 8; ++kc)
             for (int lane = 0; lane < 64; ++lane)
                 for (int j = 0; j < 8; ++j) {
                     const int row = 32 * mb + lane % 32;
-                    const float v = kc < 8 ? W2[row * kMlpH + mlp_chunk_unit(kc, lane / 32, j)]
-                                           : (lane / 32 == 0 && j == 0 ? b2[row] : 0.f);
-                    put(kPackA2 + ((mb * 9 + kc) * 64 + lane) * 16, j, v);
+                    put(kPackA2 + ((mb * 8 + kc) * 64 + lane) * 16, j,
+                        W2[row * kMlpH + mlp_chunk_unit(kc, lane / 32, j)]);
                 }
+    memcpy(blob + kPackB2, p + kOffB2, kMlpH * 4);
     for (int kc = 0; kc < 8; ++kc)
         for (int lane = 0; lane < 64; ++lane)
             for (int j = 0; j < 8; ++j) {
@@ -150,6 +151,7 @@ __device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const fl
     const bf16x4* A1 = reinterpret_cast<const bf16x4*>(lds + kPackA1);
     const bf16x8* A2 = reinterpret_cast<const bf16x8*>(lds + kPackA2);
     const bf16x8* A3 = reinterpret_cast<const bf16x8*>(lds + kPackA3);
+    const float* b2 = reinterpret_cast<const float*>(lds + kPackB2);
     const float* b3 = reinterpret_cast<const float*>(lds + kPackB3);
 
     // B operand of layer 1 for the two 32-ion blocks: lane l needs slots
@@ -184,16 +186,23 @@ __device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const fl
             H1[2 * mb + q] = relu_pack8(c1, 8 * q);
         }
     }
-    // the constant operand of layer 2's bias chunk: 1 in slot (0, 0)
-    const bf16x8 one = __builtin_bit_cast(bf16x8, (u32v4){half == 0 ? 0x3F80u : 0u, 0u, 0u, 0u});
     // layers 2 and 3, one 32-unit block of layer 2 at a time
     f32x16 y0 = {}, y1 = {};
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-        const bf16x8* A2m = A2 + mb * 9 * 64 + lane;
-        f32x16 c0 = {}, c1 = {};
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2m[8 * 64], one, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2m[8 * 64], one, c1, 0, 0, 0);
+        const bf16x8* A2m = A2 + mb * 8 * 64 + lane;
+        // the accumulators start at the bias of their rows:
+        // register v holds row 32 mb + 8 (v / 4) + 4 half + v % 4
+        f32x16 c0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float4 bq = *reinterpret_cast<const float4*>(b2 + 32 * mb + 8 * g + 4 * half);
+            c0[4 * g] = bq.x;
+            c0[4 * g + 1] = bq.y;
+            c0[4 * g + 2] = bq.z;
+            c0[4 * g + 3] = bq.w;
+        }
+        f32x16 c1 = c0;
 #pragma unroll
         for (int kc = 0; kc < 8; ++kc) {
             const bf16x8 a = A2m[kc * 64];
