@@ -38,8 +38,40 @@ def _run(cmd, verbose):
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}: {cmd[-1]}")
+        raise RuntimeError(f"{os.path.basename(cmd[0])} failed with exit code {res.returncode}: {cmd[-1]}")
     return res
+
+
+# The trajectory file writer (csrc/trajectory_h5.cpp, include/icarion_h5.h) is a
+# host library of its own, linked against the HDF5 C library; the GPU library
+# does not depend on it.  HDF5_DIR (default /opt/conda) holds include/hdf5.h
+# and lib/libhdf5.so.
+H5_LIB = os.path.join(LIBDIR, "libicarion_h5.so")
+
+
+def hdf5_dir():
+    d = os.environ.get("HDF5_DIR", "/opt/conda")
+    if os.path.exists(os.path.join(d, "include", "hdf5.h")) and os.path.exists(
+            os.path.join(d, "lib", "libhdf5.so")):
+        return d
+    return None
+
+
+def build_h5(verbose=False):
+    """Build libicarion_h5.so; returns its path, or None without an HDF5 library."""
+    d = hdf5_dir()
+    if d is None:
+        if verbose:
+            print("HDF5 not found: libicarion_h5.so not built", flush=True)
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = H5_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+          "-I", os.path.join(d, "include"), os.path.join(CSRC, "trajectory_h5.cpp"),
+          "-L", os.path.join(d, "lib"), "-lhdf5", f"-Wl,-rpath,{os.path.join(d, 'lib')}",
+          "-o", tmp], verbose)
+    os.replace(tmp, H5_LIB)
+    return H5_LIB
 
 
 def build(verbose=False, extra_flags=(), jobs=None):
@@ -60,6 +92,7 @@ def build(verbose=False, extra_flags=(), jobs=None):
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, LIB)
+    build_h5(verbose)
     return LIB
 
 

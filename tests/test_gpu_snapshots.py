@@ -237,3 +237,67 @@ def test_snapshots_reject_rk45():
         assert ei.value.code == _abi.ERR_UNSUPPORTED
     finally:
         h.close()
+
+
+def test_snapshots_into_the_trajectory_file(tmp_path):
+    """The device snapshot path writing the reference's trajectory file: the
+    library's C callback (icarion_h5_on_snapshot) handed straight to
+    icarion_gpu_step_snapshots, and the Python consumer, give the same
+    /trajectory datasets as the in-memory TrajectoryBuffer of the same run;
+    /ions/death_time_s takes the final death times."""
+    import ctypes as C
+    import json
+    import os
+    import shutil
+    import subprocess
+    from icarion_amd import build as B
+    from icarion_amd import trajectory as TR
+    if B.hdf5_dir() is None or not os.path.exists(TR.H5_LIB_PATH):
+        pytest.skip("no HDF5 C library / libicarion_h5.so")
+    d5 = B.hdf5_dir()
+    dump = str(tmp_path / "h5_dump")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([shutil.which("g++") or "g++", "-O1", "-std=c++17", "-I", os.path.join(d5, "include"),
+                    os.path.join(root, "tests", "cpp", "h5_dump.cpp"), "-L", os.path.join(d5, "lib"),
+                    "-lhdf5", f"-Wl,-rpath,{os.path.join(d5, 'lib')}", "-o", dump], check=True)
+    w = W.quadrupole_workload(n=700, U=30.0, seed=6)
+    dt, n_steps, interval = w["dt"], 400, 25 * w["dt"]
+    print("trajectory file: quadrupole n=700 steps=400", flush=True)
+    # (1) Python consumers: a TrajectoryBuffer and the file, fed the same snapshots
+    buf = TR.TrajectoryBuffer(buffer_max=4)
+    e1 = w["ensemble"].copy()
+    p1 = str(tmp_path / "py.h5")
+    with _handle(w) as h, TR.H5TrajectoryFile(p1, e1, batch=4) as f:
+        h.upload_state(e1)
+
+        def both(s):
+            buf(s)
+            f(s)
+        h.step_snapshots(0.0, dt, n_steps, interval, on_snapshot=both, copy=False)
+        h.download_state(e1)
+        f.death_times(e1.death_time_s)
+    # (2) the C callback of the writer passed to the library directly
+    e2 = w["ensemble"].copy()
+    p2 = str(tmp_path / "c.h5")
+    lib5 = TR.load_h5()
+    f2 = TR.H5TrajectoryFile(p2, e2, batch=4)
+    fn = C.cast(lib5.icarion_h5_on_snapshot, _abi.SnapshotFn)
+    with _handle(w) as h:
+        h.upload_state(e2)
+        nxt = C.c_double(interval)
+        st = _abi.StepStats()
+        rc = h.lib.icarion_gpu_step_snapshots(h.h, 0.0, dt, n_steps, 0, interval, 1, C.byref(nxt), fn,
+                                              f2.h, C.byref(st))
+        assert rc == 0
+        h.download_state(e2)
+    f2.death_times(e2.death_time_s)
+    f2.close()
+    ref = buf.datasets()
+    assert ref["time"].size >= 10 and (e1.active == 0).any()
+    for p in (p1, p2):
+        d = json.loads(subprocess.run([dump, p], check=True, capture_output=True, text=True).stdout)
+        for k in ("time", "positions", "velocities", "domain_indices", "species_id_indices"):
+            np.testing.assert_array_equal(np.array(d["/trajectory/" + k]["data"]).reshape(ref[k].shape),
+                                          ref[k], err_msg=f"{p} {k}")
+        np.testing.assert_array_equal(d["/ions/death_time_s"]["data"], e1.death_time_s)
+        np.testing.assert_array_equal(d["/ions/initial_pos_x"]["data"], w["ensemble"].x)
