@@ -66,6 +66,14 @@ EHSS_FLOPS_STEP = 115
 # 141.1 MB read + 70.0 MB written (state round trip + each ion's species row).
 EHSS_TRAFFIC_BYTES_PER_ION_LAUNCH = 211.1
 EHSS_FLOPS_COLLISION = 2 * (8 * 128 + 128 * 128 + 128 * 3) + 2 * 128 + 2 * 128 + 3 + 60
+# Field-array FP32 kernel (RK4 + trilinear + HSS), per ion-step: per stage the
+# cell location 9, corner weights 19, 8 corners x 3 components FMA 48, q/m 3
+# (x 4 stages = 316); RK4 stage and final combinations 78; Kahan 24; HSS 40;
+# two containment tests 18.
+FA_FLOPS_PER_ION_STEP = 476
+# HBM bytes per ion-step of that kernel in grid-brick order (rocprofv3
+# FETCH_SIZE + WRITE_SIZE, copy-calibrated, 4M ions x 2000 steps)
+FA_TRAFFIC_BYTES_PER_ION_STEP = 1.06
 
 
 def parse():
@@ -91,8 +99,8 @@ def parse():
     ap.add_argument("--params-in", default=None,
                     help="ehss: load surrogate parameters saved by --params-out instead of "
                          "training (for profiler runs)")
-    ap.add_argument("--k-ions", type=int, default=65536)
-    ap.add_argument("--k-steps", type=int, default=25_000)
+    ap.add_argument("--k-ions", type=int, default=262_144)
+    ap.add_argument("--k-steps", type=int, default=40_000)
     ap.add_argument("--no-k-check", action="store_true")
     ap.add_argument("--cpu-sample-ions", type=int, default=None)
     ap.add_argument("--cpu-sample-steps", type=int, default=None)
@@ -166,13 +174,14 @@ def train_surrogate(w, args):
     h, _, feat = ehss_handle(w, _abi.PRECISION_FP32_KAHAN, _abi.EHSS_SURROGATE, n_clusters=n_tr)
     with h:
         t0 = time.perf_counter()
+        tm = {}
         losses, X, _ = S.train_on_device(h, n_tr, samples_per_cluster=args.train_samples,
                                          features=feat, epochs=args.train_epochs, batch=8192,
-                                         lr=2e-3, seed=2)
+                                         lr=2e-3, seed=2, timings=tm)
         params = h.surrogate_get_params()
     return params, dict(clusters=n_tr, samples=int(X.shape[0]), epochs=args.train_epochs,
                         batch=8192, loss_first=float(losses[0]), loss_last=float(losses[-1]),
-                        seconds=time.perf_counter() - t0)
+                        seconds=time.perf_counter() - t0, **tm)
 
 
 def k_of(ens, z0, T, E):
@@ -206,10 +215,13 @@ def mobility_check(args, params):
         out[name] = (k_of(e, z0, T, E), st, el)
         log(f"mobility check {name}: K {out[name][0][0]:.5e} in {el:.1f} s")
     (kp, se), sp, tp = out["plain"]
-    (ks, _), ss, ts = out["surrogate"]
+    (ks, ses), ss, ts = out["surrogate"]
+    # the two runs are independent samples: the difference carries both errors
+    sd = math.hypot(se, ses) / kp
     return {"ions": args.k_ions, "steps": args.k_steps, "drift_time_s": T,
             "K_plain_m2_Vs": kp, "K_surrogate_m2_Vs": ks, "rel_error": abs(ks / kp - 1),
-            "stat_error_one_K": se / kp,
+            "stat_error_one_K": se / kp, "stat_error_difference": sd,
+            "rel_error_upper_95": abs(ks / kp - 1) + 1.96 * sd,
             "collisions_plain": sp.collisions, "collisions_surrogate": ss.collisions,
             "rate_plain": sp.ion_steps / tp, "rate_surrogate": ss.ion_steps / ts}
 
@@ -355,8 +367,8 @@ def bench_ehss(args):
         res["surrogate_over_plain"] = value / pv
     if not args.no_k_check:
         res["mobility_check"] = mc = mobility_check(args, params)
-        log(f"K rel. error {mc['rel_error']:.4%} (one-K statistical error "
-            f"{mc['stat_error_one_K']:.3%})")
+        log(f"K rel. error {mc['rel_error']:.4%} (statistical error of the difference "
+            f"{mc['stat_error_difference']:.3%}, 95 % upper bound {mc['rel_error_upper_95']:.3%})")
     if not args.no_k_check:
         res["angle_law_check"] = al = angle_law_check(w, params)
         log(f"angle law KS D: surrogate {al['ks_D_surrogate_vs_ehss']:.4f}, two EHSS samples "
@@ -648,7 +660,12 @@ def bench_field_array(args):
     spl = args.steps_per_launch or 2000
     launches = math.ceil(steps / spl)
     avg = fs.device_ms * 1e-3 / launches
-    # trilinear gathers: 4 stages x 8 corners x 16 B (float4) per ion-step
+    # the launches run in grid-brick order, so the corner gathers (4 stages x 8
+    # corners x 16 B per ion-step) are served by the caches: rocprofv3 counts
+    # 1.06 B of HBM traffic per ion-step (FETCH_SIZE + WRITE_SIZE, copy-
+    # calibrated) and 99 % L2 hits; the kernel is bound by vector issue, priced
+    # against the FP32 vector peak
+    tflops = FA_FLOPS_PER_ION_STEP * fs.ion_steps / launches / avg / 1e12
     gbytes = 4 * 8 * 16 * fs.ion_steps / launches / avg / 1e9
     res = {
         "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
@@ -658,11 +675,15 @@ def bench_field_array(args):
         "dtype": "f32 (Kahan-compensated state; f64 time)", "data": "synthetic",
         "config": {"workload": "field_array_hss_rk4", "ions": n, "grid": [256, 256, 256],
                    "extent_mm": [10, 10, 50], "gas": "N2 300 Pa 300 K", "dt_s": w["dt"]},
-        "roofline": {"bound": "hbm", "achieved": gbytes, "peak": 8000.0, "unit": "GB/s",
-                     "frac": gbytes / 8000.0, "traffic": None,
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": MI355X_PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / MI355X_PEAK_FP32_TFLOPS,
+                     "traffic": FA_TRAFFIC_BYTES_PER_ION_STEP * fs.ion_steps / launches,
                      "kernel": "step_f32_kernel<RK4, field array, HSS>",
-                     "note": "achieved = algorithmic gather bytes (4 stages x 8 corners x 16 B "
-                             "per ion-step); most gathers hit L2 / MALL"},
+                     "avg_launch_ms": avg * 1e3, "launches": launches,
+                     "flops_per_ion_step": FA_FLOPS_PER_ION_STEP,
+                     "gather_rate_GB_s": gbytes,
+                     "note": "vector FP32 (not matrix) peak; gather_rate = 512 B of corner "
+                             "gathers per ion-step, served by L1/L2 in grid-brick order"},
         "alive_fraction": fs.alive / n,
     }
     if "fp64" in out:

@@ -18,7 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libicarion_gpu.so")
 SOURCES = ["step_kernels.hip", "rk45_kernels.hip", "multi_kernels.hip", "metrics_kernels.hip",
-           "ehss_tools.hip", "surrogate.hip", "icarion_gpu_api.cpp"]
+           "ehss_tools.hip", "surrogate.hip", "order_kernels.hip", "icarion_gpu_api.cpp"]
 ARCH = os.environ.get("ICARION_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -91,6 +91,11 @@ struct TrainBuffers {
 hipError_t mlp_fwd_bwd(const TrainBuffers& t, hipStream_t s);
 hipError_t adam_step(const TrainBuffers& t, float lr, float b1, float b2, float eps, int step, hipStream_t s);
 hipError_t mlp_infer(const uint4* blob, const float* X, float* Y, uint32_t n, hipStream_t s);
+size_t brick_sort_temp_bytes(uint32_t n);
+hipError_t sort_ions_by_brick(const double* x, const double* y, const double* z, const uint8_t* flags, uint32_t n,
+                              const double origin[3], const double inv_brick[3], const uint32_t nb[3],
+                              uint32_t* keys_a, uint32_t* keys_b, uint32_t* idx, uint32_t* order, void* temp,
+                              size_t temp_bytes, hipStream_t s);
 }
 
 using namespace icarion;
@@ -201,6 +206,9 @@ struct icarion_gpu {
     } fsrc;
     DevBuf<double> ax64[3];
     DevBuf<float> ax32[3];
+    // field arrays: the ions in grid-brick order for each launch (lane -> ion)
+    DevBuf<uint32_t> ord, ord_keys_a, ord_keys_b, ord_idx;
+    DevBuf<unsigned char> ord_temp;
     // HSS gas mixture: the participating components with n > 0 (in order) and
     // their per-species cross sections, as the kernels read them
     // voltage waveforms of the analytic fields (per slot; none when !has_waves)
@@ -785,6 +793,9 @@ __global__ void zero_u64(unsigned long long* p, int n) {
 
 constexpr int kCounterWords = kCounterStripes * kCounterStride;
 
+// Field-array clouds of at least this many ions run in grid-brick order.
+constexpr size_t kOrderMinIons = 65536;
+
 // Clear the striped launch counters of a single cloud.
 void zero_counters(unsigned long long* counters, hipStream_t s) {
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(256), 0, s, counters, kCounterWords);
@@ -1332,10 +1343,42 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
             plan = plan_snapshots(h->table_host, dt, n_steps, *snap);
             ensure_snap_ring(h);
         }
+        // field arrays: each launch runs the ions in grid-brick order (bricks of
+        // 4 x 4 x 4 nodes), sorted on the device from the state the launch
+        // starts from; small clouds and the allocation-free path keep lane = ion
+        const bool order = !ws && field == FIELD_ARRAY && h->n >= kOrderMinIons;
+        double o_origin[3] = {0, 0, 0}, o_inv[3] = {0, 0, 0};
+        uint32_t o_nb[3] = {1, 1, 1};
+        if (order) {
+            const uint32_t n = static_cast<uint32_t>(h->n);
+            h->ord.resize(n);
+            h->ord_keys_a.resize(n);
+            h->ord_keys_b.resize(n);
+            h->ord_idx.resize(n);
+            const size_t tb = brick_sort_temp_bytes(n);
+            if (h->ord_temp.n < tb) h->ord_temp.resize(tb);
+            const auto& f = h->fsrc;
+            const double first[3] = {f.x0, f.y0, f.z0}, last[3] = {f.x_last, f.y_last, f.z_last};
+            const uint32_t nn[3] = {static_cast<uint32_t>(f.nx), static_cast<uint32_t>(f.ny), static_cast<uint32_t>(f.nz)};
+            for (int a = 0; a < 3; ++a) {
+                const double spacing = nn[a] > 1 ? (last[a] - first[a]) / (nn[a] - 1) : 1.0;
+                o_origin[a] = first[a];
+                o_nb[a] = std::max<uint32_t>(1, (nn[a] + 2) / 4);
+                o_inv[a] = spacing > 0.0 ? 1.0 / (4.0 * spacing) : 0.0;
+            }
+        }
         ms = run_launches(h, S, n_steps, step_offset, spl, plan, snap, [&](uint64_t k0, uint64_t nk) {
             P.table = table + k0;
             P.step0 = step_offset + k0;
             P.n_steps = static_cast<uint32_t>(nk);
+            P.order = nullptr;
+            if (order) {
+                hip_check(sort_ions_by_brick(h->x.p, h->y.p, h->z.p, h->flags.p, static_cast<uint32_t>(h->n),
+                                             o_origin, o_inv, o_nb, h->ord_keys_a.p, h->ord_keys_b.p, h->ord_idx.p,
+                                             h->ord.p, h->ord_temp.p, h->ord_temp.n, h->stream),
+                          "brick sort");
+                P.order = h->ord.p;
+            }
             const hipError_t e = launch_step(h->phys.integrator, launch_field_kind(h), launch_coll_kind(h),
                                              h->phys.precision, surrogate ? 1 : 0, P, S, h->block, h->stream);
             if (e == hipErrorInvalidValue)
@@ -2282,6 +2325,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
     for (auto& a : h->ax64) a.release();
     for (auto& a : h->ax32) a.release();
+    h->ord.release(); h->ord_keys_a.release(); h->ord_keys_b.release(); h->ord_idx.release(); h->ord_temp.release();
     h->mix_sigma.release(); h->emix_sigma.release();
     h->off_tab.release(); h->off_sigma.release(); h->off_mu.release();
     h->ori_tab.release(); h->ori_R.release(); h->ori_area.release();

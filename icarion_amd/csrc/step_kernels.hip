@@ -141,7 +141,7 @@ template <int INTEG, int FIELD, int COLL>
 __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL>(P, S, i, pushed, ncoll_launch);
+    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -687,7 +687,7 @@ template <int INTEG, int FIELD, int COLL>
 __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL>(P, S, i, pushed, ncoll_launch);
+    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 

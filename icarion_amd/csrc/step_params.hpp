@@ -206,6 +206,10 @@ struct StepParams {
     uint64_t seed;
     uint32_t n_steps;
     uint32_t n_ions;
+    // lane t of the launch takes ion order[t] (a permutation of the ions that
+    // puts neighbours in the field-array grid on neighbouring lanes); null:
+    // lane t takes ion t.  The ion keeps its index (state, random streams).
+    const uint32_t *order;
     unsigned long long *counters;   // [0] ion-steps, [1] collisions
     FastConst fc;
     Rk45Cfg rk45;

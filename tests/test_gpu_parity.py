@@ -409,6 +409,40 @@ def test_field_array_fp32_hss_statistics_against_fp64_oracle(oracle):
     assert sg.collisions / sg.ion_steps == pytest.approx(so.collisions / so.ion_steps, rel=0.005)
 
 
+@pytest.mark.parametrize("precision", [_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN])
+def test_field_array_brick_order_changes_nothing(precision):
+    """Field-array clouds of 65536 ions or more run each launch in grid-brick
+    order (lane t takes ion order[t]); the allocation-free entry point keeps
+    lane = ion.  Both give the same state bit for bit, over launches that re-sort
+    a cloud that moves, with dead ions (sorted last) and ions born during the
+    run."""
+    from icarion_amd import gpu
+    w = W.field_array_workload(n=100_000, grid=(40, 36, 90), seed=6)
+    ph = w["physics"]
+    ph = W.physics(ph.integrator, ph.collision_model, precision=precision, seed=6)
+    e0 = w["ensemble"].copy()
+    e0.active[::10] = 0
+    e0.birth_time_s = np.zeros(e0.n)
+    e0.birth_time_s[5::20] = 300 * w["dt"]
+    e0.born[5::20] = 0
+    res = []
+    for noalloc in (False, True):
+        e = e0.copy()
+        with gpu.Handle(steps_per_launch=150) as h:
+            h.set_domain(w["domain"], ph)
+            h.set_species(w["species"])
+            h.upload_field_array(*w["field"])
+            h.upload_state(e)
+            st = (h.step_noalloc if noalloc else h.step)(0.0, w["dt"], 600, 0)
+            h.download_state(e)
+        res.append((e, st))
+    (a, sa), (b, sb) = res
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    assert sa.ion_steps == sb.ion_steps and sa.collisions == sb.collisions
+    assert sa.alive == 90_000 and (a.born != 0).all()
+
+
 @pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4, _abi.INTEGRATOR_BORIS])
 def test_field_terms_nonuniform_collisionless_fp64(oracle, integrator):
     """Composite field (DC + RF + constant terms) on non-uniform axes: ion by
