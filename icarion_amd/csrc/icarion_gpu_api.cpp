@@ -86,6 +86,7 @@ struct TrainBuffers {
     float *X, *T;
     float *Z1, *H1, *Z2, *H2, *Y, *dY, *dH2, *dH1;
     float *params, *grad, *m, *v, *loss;
+    float* ws;
     int B;
 };
 hipError_t mlp_fwd_bwd(const TrainBuffers& t, hipStream_t s);
@@ -2207,7 +2208,7 @@ __global__ void zero_u32_kernel(uint32_t* p, int n) {
 }
 
 struct TrainDev {
-    DevBuf<float> Z1, H1, Z2, H2, Y, dY, dH2, dH1, params, grad, m, v, loss, Xb, Tb;
+    DevBuf<float> Z1, H1, Z2, H2, Y, dY, dH2, dH1, params, grad, m, v, loss, Xb, Tb, ws;
     void alloc(int B) {
         Z1.resize(static_cast<size_t>(B) * kMlpH); H1.resize(static_cast<size_t>(B) * kMlpH);
         Z2.resize(static_cast<size_t>(B) * kMlpH); H2.resize(static_cast<size_t>(B) * kMlpH);
@@ -2216,16 +2217,17 @@ struct TrainDev {
         params.resize(kMlpParams); grad.resize(kMlpParams); m.resize(kMlpParams); v.resize(kMlpParams);
         loss.resize(1);
         Xb.resize(static_cast<size_t>(B) * kMlpIn); Tb.resize(static_cast<size_t>(B) * kMlpOut);
+        ws.resize(kGemmWorkspaceFloats);
     }
     TrainBuffers view(float* X, float* T, int B) {
         TrainBuffers t{};
         t.X = X; t.T = T; t.Z1 = Z1.p; t.H1 = H1.p; t.Z2 = Z2.p; t.H2 = H2.p; t.Y = Y.p; t.dY = dY.p;
         t.dH2 = dH2.p; t.dH1 = dH1.p; t.params = params.p; t.grad = grad.p; t.m = m.p; t.v = v.p;
-        t.loss = loss.p; t.B = B;
+        t.loss = loss.p; t.ws = ws.p; t.B = B;
         return t;
     }
     void release() {
-        for (DevBuf<float>* b : {&Z1, &H1, &Z2, &H2, &Y, &dY, &dH2, &dH1, &params, &grad, &m, &v, &loss, &Xb, &Tb})
+        for (DevBuf<float>* b : {&Z1, &H1, &Z2, &H2, &Y, &dY, &dH2, &dH1, &params, &grad, &m, &v, &loss, &Xb, &Tb, &ws})
             b->release();
     }
 };

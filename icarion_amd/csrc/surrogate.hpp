@@ -45,6 +45,11 @@ constexpr int kOffW2 = kOffB1 + kMlpH;
 constexpr int kOffB2 = kOffW2 + kMlpH * kMlpH;
 constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpOut * kMlpH;
+// Training GEMMs over the batch (K = batch rows) are split into at most
+// kGemmSplitMax K ranges; the workspace holds their partial products
+// ([split][M][N], M, N <= kMlpH) and partial column sums ([split][M]).
+constexpr int kGemmSplitMax = 64;
+constexpr int kGemmWorkspaceFloats = kGemmSplitMax * (kMlpH * kMlpH + kMlpH);
 
 // Packed inference blob (bytes): bf16 A fragments in lane order, then biases.
 //   A1: [4 mb][64 lanes] x 4 bf16         ( 2048 B)  column 6 = b1
