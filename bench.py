@@ -487,6 +487,33 @@ def bench_quadrupole(args):
 BATCH_FLOPS = {"ims": 130, "quadrupole": 122, "orbitrap": 154}
 
 
+def cpu_baseline_batched(clouds, stats, sample_ions, sample_steps):
+    """The batch on the CPU is the sum of per-cloud oracle runs: one cloud of
+    each kind is timed on a sample (its first sample_ions ions x sample_steps
+    steps), and every cloud's ion-steps are priced at its kind's rate."""
+    from oracle import binding
+    threads = len(os.sched_getaffinity(0))
+    rate, parts = {}, []
+    for c in clouds:
+        k = c["kind"]
+        if k in rate:
+            continue
+        n = min(sample_ions, c["ensemble"].n)
+        e = c["ensemble"].subset(np.arange(n))
+        steps = min(sample_steps, c["n_steps"])
+        t0 = time.perf_counter()
+        st = binding.run(c["domain"], c["physics"], c["species"], e, 0.0, c["dt"], steps, threads=threads)
+        el = time.perf_counter() - t0
+        rate[k] = st.ion_steps / el
+        parts.append(f"{k} {n} ions x {steps} steps {el:.1f} s ({rate[k]:.3e} ion-steps/s)")
+    total = sum(st.ion_steps for st in stats)
+    cpu_s = sum(st.ion_steps / rate[c["kind"]] for c, st in zip(clouds, stats))
+    return dict(value=total / cpu_s, unit="ion-steps/s", cores=threads, kind="port",
+                sample="sum of per-cloud FP64 oracle runs, each kind timed on one cloud's sample "
+                       f"(OpenMP over {threads} threads): " + "; ".join(parts)
+                       + f"; whole batch {cpu_s:.0f} CPU-seconds")
+
+
 def bench_batched(args):
     from icarion_amd import _abi, gpu
     from icarion_amd import workloads as W
@@ -531,7 +558,7 @@ def bench_batched(args):
                                                / max(1, sum(clouds[i]["ensemble"].n for i in sel))),
                        "done_ms_max": float(done[sel].max()) if sel else 0.0,
                        "done_ms_median": float(np.median(done[sel])) if sel else 0.0}
-    return {
+    res = {
         "metric": "ion-steps/s on 1 MI355X, plain FP64 vs FP32+surrogate at <=0.5% K error",
         "value": value, "unit": "ion-steps/s", "n_gpus": 1, "steps": max(steps.values()),
         "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
@@ -550,6 +577,10 @@ def bench_batched(args):
         "completion_ms": {"min": float(done.min()), "median": float(np.median(done)),
                           "max": float(done.max())},
     }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_batched(clouds, stats, args.cpu_sample_ions or 16384,
+                                                   args.cpu_sample_steps or 2000)
+    return res
 
 
 def bench_batched_ehss(args):

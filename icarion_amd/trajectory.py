@@ -133,7 +133,9 @@ class TrajectoryBuffer:
 
     ``buffer_max`` snapshots are held before a flush (OutputManager.cpp:184-188,
     255-260).  ``sink(batch)`` receives each flushed batch as a dict of the
-    datasets above; the default sink concatenates them in memory.
+    datasets above; the default sink concatenates them in memory.  The buffer
+    keeps copies of the snapshot arrays, so it may be fed the transient views
+    of ``step_snapshots(copy=False)``.
     """
 
     def __init__(self, buffer_max=100, sink=None):
@@ -157,7 +159,7 @@ class TrajectoryBuffer:
             raise RuntimeError("TrajectoryBuffer: ion count changed during the run")
         if len(self._buf) >= self.buffer_max:
             self.flush()
-        self._buf.append(snap)
+        self._buf.append({k: np.array(v) if isinstance(v, np.ndarray) else v for k, v in snap.items()})
         self.last_time = snap["t"]
 
     def _batch(self, snaps):

@@ -260,9 +260,11 @@ def test_snapshots_into_the_trajectory_file(tmp_path):
     subprocess.run([shutil.which("g++") or "g++", "-O1", "-std=c++17", "-I", os.path.join(d5, "include"),
                     os.path.join(root, "tests", "cpp", "h5_dump.cpp"), "-L", os.path.join(d5, "lib"),
                     "-lhdf5", f"-Wl,-rpath,{os.path.join(d5, 'lib')}", "-o", dump], check=True)
-    w = W.quadrupole_workload(n=700, U=30.0, seed=6)
-    dt, n_steps, interval = w["dt"], 400, 25 * w["dt"]
-    print("trajectory file: quadrupole n=700 steps=400", flush=True)
+    # U = 200 V makes every mass unstable: ions die during the run, so the
+    # death times written at the end are not all -1
+    w = W.quadrupole_workload(n=700, U=200.0, seed=6)
+    dt, n_steps, interval = w["dt"], 2000, 100 * w["dt"]
+    print("trajectory file: quadrupole n=700 U=200 steps=2000", flush=True)
     # (1) Python consumers: a TrajectoryBuffer and the file, fed the same snapshots
     buf = TR.TrajectoryBuffer(buffer_max=4)
     e1 = w["ensemble"].copy()

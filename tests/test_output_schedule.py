@@ -120,3 +120,18 @@ def test_trajectory_buffer_npz(tmp_path):
             "time", "positions", "velocities", "domain_indices", "species_id_indices"))
         np.testing.assert_array_equal(z["trajectory/time"], [s["t"] for s in snaps])
         np.testing.assert_array_equal(z["trajectory/velocities"][4], snaps[4]["velocities"])
+
+
+def test_trajectory_buffer_keeps_copies_of_transient_snapshots():
+    """A consumer of step_snapshots(copy=False) receives views of buffers the
+    library reuses for the next snapshot; the buffer must hold its own copies."""
+    buf = TrajectoryBuffer(buffer_max=10)
+    scratch = _snap(0.0, 6, 1)
+    first = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in scratch.items()}
+    buf(scratch)
+    for k in ("positions", "velocities", "species"):
+        scratch[k][...] = 0                       # the library overwrites its buffer
+    d = buf.datasets()
+    np.testing.assert_array_equal(d["positions"][0], first["positions"])
+    np.testing.assert_array_equal(d["velocities"][0], first["velocities"])
+    np.testing.assert_array_equal(d["species_id_indices"][0], first["species"])
