@@ -91,9 +91,12 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=100,
                     help="ehss: engine steps of the timed plain FP64 run")
     ap.add_argument("--train-clusters", type=int, default=10_000)
-    ap.add_argument("--train-samples", type=int, default=1000,
+    ap.add_argument("--train-samples", type=int, default=3000,
                     help="first-contact samples per training cluster")
-    ap.add_argument("--train-epochs", type=int, default=10)
+    ap.add_argument("--train-schedule", default="30:2e-3,10:5e-4,10:1.25e-4",
+                    help="training stages epochs:learning-rate, comma separated")
+    ap.add_argument("--train-epochs", type=int, default=None,
+                    help="one constant-rate stage of this many epochs instead of --train-schedule")
     ap.add_argument("--params-out", default=None,
                     help="ehss: save the trained surrogate parameters (.npy)")
     ap.add_argument("--params-in", default=None,
@@ -167,6 +170,17 @@ def ehss_handle(w, precision, mode, spl=0, params=None, n_clusters=None):
     return h, sig, feat
 
 
+def train_schedule(args):
+    """[(epochs, learning rate), ...] from --train-epochs or --train-schedule."""
+    if args.train_epochs is not None:
+        return [(args.train_epochs, 2e-3)]
+    out = []
+    for part in args.train_schedule.split(","):
+        e, lr = part.split(":")
+        out.append((int(e), float(lr)))
+    return out
+
+
 def train_surrogate(w, args):
     from icarion_amd import _abi
     from icarion_amd import surrogate as S
@@ -175,11 +189,13 @@ def train_surrogate(w, args):
     with h:
         t0 = time.perf_counter()
         tm = {}
+        schedule = train_schedule(args)
         losses, X, _ = S.train_on_device(h, n_tr, samples_per_cluster=args.train_samples,
-                                         features=feat, epochs=args.train_epochs, batch=8192,
-                                         lr=2e-3, seed=2, timings=tm)
+                                         features=feat, schedule=schedule, batch=8192, seed=2,
+                                         timings=tm)
         params = h.surrogate_get_params()
-    return params, dict(clusters=n_tr, samples=int(X.shape[0]), epochs=args.train_epochs,
+    return params, dict(clusters=n_tr, samples=int(X.shape[0]), schedule=schedule,
+                        epochs=sum(e for e, _ in schedule),
                         batch=8192, loss_first=float(losses[0]), loss_last=float(losses[-1]),
                         seconds=time.perf_counter() - t0, **tm)
 

@@ -105,15 +105,21 @@ struct PhiloxStream {
         ++block;
         pos = 0;
     }
+    // The words are selected rather than indexed by pos, so that the block
+    // stays in registers where the compiler cannot resolve pos (draws inside
+    // data-dependent loops); an indexed buffer goes to private memory there.
+    // A stream takes FP64 draws only (pos 0 or 2) or FP32 draws only.
     __device__ __forceinline__ double uniform_f64() {
         if (pos >= 4) refill();
-        const double u = uniform53(buf[pos], buf[pos + 1]);
+        const bool second = pos >= 2;
+        const double u = uniform53(second ? buf[2] : buf[0], second ? buf[3] : buf[1]);
         pos += 2;
         return u;
     }
     __device__ __forceinline__ float uniform_f32() {
         if (pos >= 4) refill();
-        const float u = uniform24(buf[pos]);
+        const uint32_t w = pos == 0 ? buf[0] : pos == 1 ? buf[1] : pos == 2 ? buf[2] : buf[3];
+        const float u = uniform24(w);
         pos += 1;
         return u;
     }

@@ -93,6 +93,8 @@ hipError_t mlp_fwd_bwd(const TrainBuffers& t, hipStream_t s);
 hipError_t adam_step(const TrainBuffers& t, float lr, float b1, float b2, float eps, int step, hipStream_t s);
 hipError_t mlp_infer(const uint4* blob, const float* X, float* Y, uint32_t n, hipStream_t s);
 size_t brick_sort_temp_bytes(uint32_t n);
+hipError_t random_permutation(uint32_t n, uint64_t seed, uint32_t epoch, uint32_t* keys_a, uint32_t* keys_b,
+                              uint32_t* idx, uint32_t* perm, void* temp, size_t temp_bytes, hipStream_t s);
 hipError_t sort_ions_by_brick(const double* x, const double* y, const double* z, const uint8_t* flags, uint32_t n,
                               const double origin[3], const double inv_brick[3], const uint32_t nb[3],
                               uint32_t* keys_a, uint32_t* keys_b, uint32_t* idx, uint32_t* order, void* temp,
@@ -3075,11 +3077,19 @@ int icarion_gpu_surrogate_train(icarion_gpu* h, const float* X, const float* T, 
         copy_in(td.params, h->params.data(), kMlpParams, s, "upload params");
         hipLaunchKernelGGL(zero_f32_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, td.m.p, kMlpParams);
         hipLaunchKernelGGL(zero_f32_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, td.v.p, kMlpParams);
-        DevBuf<uint32_t> ctr, perm_d;
+        DevBuf<uint32_t> ctr, perm_d, keys_a, keys_b, idx;
+        DevBuf<unsigned char> sort_temp;
         ctr.resize(2);
         hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctr.p, 2);
         const size_t n_batches = n / static_cast<size_t>(B);
-        perm_d.resize(n_batches * B);
+        // the epoch's sample order: a device permutation of all n samples, of
+        // which the first n_batches * B are used
+        perm_d.resize(n);
+        keys_a.resize(n);
+        keys_b.resize(n);
+        idx.resize(n);
+        const size_t temp_bytes = brick_sort_temp_bytes(static_cast<uint32_t>(n));
+        sort_temp.resize(temp_bytes);
         const TrainBuffers tb = td.view(td.Xb.p, td.Tb.p, B);
         // one training step, captured once as a HIP graph and replayed: the batch
         // offset and the Adam step count live in device counters
@@ -3097,14 +3107,11 @@ int icarion_gpu_surrogate_train(icarion_gpu* h, const float* X, const float* T, 
         hip_check(ce, "capture training step");
         hip_check(ee, "hipStreamEndCapture");
         hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        std::mt19937_64 eng(desc->seed ^ 0x5deece66dull);
-        std::vector<uint32_t> perm(n);
-        std::iota(perm.begin(), perm.end(), 0u);
         try {
             for (uint32_t ep = 0; ep < desc->epochs; ++ep) {
-                std::shuffle(perm.begin(), perm.end(), eng);
-                hip_check(hipMemcpyAsync(perm_d.p, perm.data(), sizeof(uint32_t) * n_batches * B,
-                                         hipMemcpyHostToDevice, s), "upload permutation");
+                hip_check(random_permutation(static_cast<uint32_t>(n), desc->seed, ep, keys_a.p, keys_b.p, idx.p,
+                                             perm_d.p, sort_temp.p, temp_bytes, s),
+                          "epoch permutation");
                 hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, td.loss.p, 1);
                 hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, ctr.p, 1);  // batch index only
                 for (size_t b = 0; b < n_batches; ++b) hip_check(hipGraphLaunch(exec, s), "hipGraphLaunch");

@@ -4,10 +4,15 @@
 // cache lines (an unsorted cloud scatters every wavefront's gathers over the
 // whole grid).  Dead ions sort last.  The order only changes which lane runs an
 // ion: every ion keeps its index, so the results do not depend on it.
+//
+// Also the epoch shuffle of the surrogate training: a random permutation as
+// the sort of one Philox key per sample.
 #include <hip/hip_runtime.h>
 #include <hipcub/device/device_radix_sort.hpp>
 
 #include <stdint.h>
+
+#include "device_common.hpp"
 
 namespace icarion {
 
@@ -37,7 +42,29 @@ __global__ void __launch_bounds__(256) brick_keys_kernel(const double* __restric
     keys[i] = brick(x[i], x0, sx, nbx) + nbx * (brick(y[i], y0, sy, nby) + nby * brick(z[i], z0, sz, nbz));
 }
 
+// keys[i] = Philox4x32-10 of counter (i, epoch, 0, 0) under the key (seed), idx[i] = i
+__global__ void __launch_bounds__(256) shuffle_keys_kernel(uint32_t n, uint64_t seed, uint32_t epoch,
+                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = philox4x32_10(i, epoch, 0u, 0u, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)).v[0];
+    idx[i] = i;
+}
+
 }  // namespace
+
+// perm[0..n) <- a random permutation of 0..n-1 that depends only on (seed,
+// epoch): the indices ordered by their Philox keys (ties by index, the sort is
+// stable).  keys_a, keys_b, idx: n entries each; temp: brick_sort_temp_bytes(n).
+hipError_t random_permutation(uint32_t n, uint64_t seed, uint32_t epoch, uint32_t* keys_a, uint32_t* keys_b,
+                              uint32_t* idx, uint32_t* perm, void* temp, size_t temp_bytes, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(shuffle_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, seed, epoch, keys_a, idx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_a, keys_b, idx, perm, static_cast<int>(n), 0, 32,
+                                              s);
+}
 
 // Scratch bytes of sort_ions_by_brick for n ions (hipCUB's temporary storage).
 size_t brick_sort_temp_bytes(uint32_t n) {

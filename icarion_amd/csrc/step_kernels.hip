@@ -222,9 +222,12 @@ struct WaveStream {
         ++block;
         pos = 0;
     }
+    // pos is 0 or 2: the words are selected, not indexed, so that the block
+    // stays in (scalar) registers instead of a private-memory array
     __device__ __forceinline__ double uniform_f64() {
         if (pos >= 4) refill();
-        const double u = uniform53(buf[pos], buf[pos + 1]);
+        const bool second = pos != 0;
+        const double u = uniform53(second ? buf[2] : buf[0], second ? buf[3] : buf[1]);
         pos += 2;
         return u;
     }
