@@ -673,10 +673,34 @@ def bench_batched_ehss(args):
         res["fp64_plain"] = {"value": pv, "unit": "ion-steps/s", "steps": args.plain_steps,
                              "workload": "ehss_he_20td, %d ions" % args.ions}
         res["surrogate_batched_over_plain"] = value / pv
+    if not args.no_cpu:
+        # the same physics (plain EHSS of random clusters in He, RK4) as the CPU
+        # would run it, timed on the EHSS workload's first ions
+        w = W.ehss_surrogate_workload(n=args.ions, n_steps=args.plain_steps, seed=21)
+        w["cluster_seed"] = 21
+        res["cpu_baseline"] = cpu_baseline_ehss(w, min(args.cpu_sample_ions or 8192, args.ions),
+                                                args.cpu_sample_steps or 16000)
     return res
 
 
 # ---------------------------------------------------------------- field array (C3)
+def cpu_baseline_field_array(w, n_ions, n_steps):
+    """The oracle (FP64, trilinear interpolation on the same 256^3 grid) over the
+    first n_ions ions of the workload."""
+    from oracle import binding
+    threads = len(os.sched_getaffinity(0))
+    E, origin, spacing = w["field"]
+    field = binding.Field(E, origin, spacing)
+    e = w["ensemble"].subset(np.arange(n_ions))
+    t0 = time.perf_counter()
+    st = binding.run(w["domain"], w["physics"], w["species"], e, 0.0, w["dt"], n_steps, field=field,
+                     threads=threads)
+    el = time.perf_counter() - t0
+    return dict(value=st.ion_steps / el, unit="ion-steps/s", cores=threads, kind="port",
+                sample=f"{n_ions} ions x {n_steps} steps of the same workload "
+                       f"(FP64 oracle, OpenMP over {threads} threads, {el:.1f} s)")
+
+
 def bench_field_array(args):
     from icarion_amd import _abi, gpu
     from icarion_amd import workloads as W
@@ -743,6 +767,9 @@ def bench_field_array(args):
         res["fp32_vs_fp64"] = {
             "alive": [float(g.alive.mean()), float(o.alive.mean())],
             "mean_dz_m": [float((g.z - z0)[g.alive].mean()), float((o.z - z0)[o.alive].mean())]}
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_field_array(w, min(args.cpu_sample_ions or 65536, n),
+                                                       min(args.cpu_sample_steps or 2000, steps))
     return res
 
 
