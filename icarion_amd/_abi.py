@@ -5,7 +5,7 @@ sizes against offsets computed by the C compiler.
 """
 import ctypes as C
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NO_DEVICE = range(6)
 
@@ -177,6 +177,8 @@ class CloudDesc(C.Structure):
         ("step_offset", _u64), ("waveforms", C.POINTER(C.POINTER(WaveformDesc))),
         ("mixture", C.POINTER(GasComponent)), ("mixture_sigma_m2", C.POINTER(_d)),
         ("n_components", _u32), ("n_atoms", _u32), ("atoms_xyzr", C.POINTER(C.c_float)),
+        ("field_E", C.POINTER(C.c_float)), ("field_n", _u32 * 3), ("reserved1", _u32),
+        ("field_origin_m", _d * 3), ("field_spacing_m", _d * 3),
     ]
 
 

@@ -314,6 +314,7 @@ struct icarion_gpu {
     DevBuf<StepEntry> b_tables;
     DevBuf<double> b_mix_sigma;
     DevBuf<float4> b_atoms;
+    DevBuf<float4> b_fa32;      // the clouds' field arrays, one copy per distinct array
     icarion_batch_tuning b_tuned[2] = {{0, 0}, {0, 0}};
     hipStream_t side[kBatchClasses - 1] = {nullptr, nullptr};   // streams of the other classes
     hipEvent_t ev_fork = nullptr, ev_join[kBatchClasses - 1] = {nullptr, nullptr};
@@ -1662,19 +1663,21 @@ void do_download(icarion_gpu* h, icarion_ion_soa* ions) {
 
 // ---- batched clouds --------------------------------------------------------
 
-// kinds from here on (gas mixtures 12-15, geometry EHSS 16-19) are in their own
-// kernel variant
+// kinds from here on (gas mixtures 12-15, geometry EHSS 16-19, field arrays
+// 20-25) are in their own kernel variant
 constexpr uint32_t kBatchFirstMixKind = 12;
 
 uint32_t batch_kind(const icarion_physics_desc& p, int field, int coll) {
     // the surrogate class has its own kernel: integrator + 2 * Orbitrap
+    const uint32_t integ = p.integrator == ICARION_INTEGRATOR_BORIS ? 1u : 0u;
+    if (field == FIELD_ARRAY)
+        return 20u + integ + 2u * (coll == COLL_HSS ? 1u : coll == COLL_DAMPING ? 2u : 0u);
     const uint32_t c = coll == COLL_HSS       ? 1u
                        : coll == COLL_DAMPING ? 2u
                        : coll == COLL_HSS_MIX ? 3u
                        : coll == COLL_EHSS    ? 4u
                                               : 0u;
-    return static_cast<uint32_t>(p.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
-           2u * (field == FIELD_ORBITRAP ? 1u : 0u) + 4u * c;
+    return integ + 2u * (field == FIELD_ORBITRAP ? 1u : 0u) + 4u * c;
 }
 
 void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t n_clouds) {
@@ -1690,6 +1693,14 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     std::vector<size_t> mix_off(n_clouds);
     std::vector<float4> atoms_all;           // the clouds' EHSS atom tables
     std::vector<size_t> atom_off(n_clouds);
+    std::vector<float4> fa_all;              // the clouds' field arrays (x, y, z, 0)
+    std::vector<size_t> fa_off(n_clouds, SIZE_MAX);
+    struct FieldKey {
+        const float* E;
+        uint32_t n[3];
+        size_t off;
+    };
+    std::vector<FieldKey> fields_seen;
     // identical step tables are shared: (t0, dt, n, RF inputs, axial field, stage layout)
     // E_z rides in the table too (e0 / eh / e1), so it is part of the key;
     // clouds with waveforms get tables of their own
@@ -1718,8 +1729,18 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         if (cd.n_atoms && !cd.atoms_xyzr) fail(ICARION_ERR_INVALID, tag + "atoms_xyzr is NULL");
         if (surrogate && cd.physics.precision != ICARION_PRECISION_FP32_KAHAN)
             fail(ICARION_ERR_UNSUPPORTED, tag + "the EHSS surrogate runs in the FP32 + Kahan path");
-        if (field == FIELD_ARRAY) fail(ICARION_ERR_UNSUPPORTED, tag + "field arrays are not covered by the batch");
         const bool mixture = cd.n_components > 0;
+        if (field == FIELD_ARRAY) {
+            if (!cd.field_E) fail(ICARION_ERR_INVALID, tag + "the domain uses a field array but field_E is NULL");
+            if (cd.field_n[0] < 2 || cd.field_n[1] < 2 || cd.field_n[2] < 2)
+                fail(ICARION_ERR_INVALID, tag + "field array needs >= 2 nodes per axis");
+            for (int a = 0; a < 3; ++a)
+                if (!(cd.field_spacing_m[a] > 0.0) || !std::isfinite(cd.field_origin_m[a]))
+                    fail(ICARION_ERR_INVALID, tag + "field grid spacing must be > 0 and the origin finite");
+            if (mixture || coll == COLL_EHSS)
+                fail(ICARION_ERR_UNSUPPORTED, tag + "a field array is covered by the batch without gas "
+                                                    "mixtures and EHSS");
+        }
         if (mixture && (!cd.mixture || !cd.mixture_sigma_m2)) fail(ICARION_ERR_INVALID, tag + "mixture arrays are NULL");
         if (mixture && coll != COLL_HSS)
             fail(ICARION_ERR_UNSUPPORTED, tag + "a gas mixture is covered by the batch with HSS collisions");
@@ -1768,6 +1789,37 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
             species.push_back(sd);
         }
         StepParams P = derive_params(cd.domain, cd.physics);
+        P.order = nullptr;
+        if (field == FIELD_ARRAY) {
+            // one uniform grid, one term, as icarion_gpu_upload_field_array
+            const uint32_t* nn = cd.field_n;
+            size_t off = SIZE_MAX;
+            for (const auto& f : fields_seen)
+                if (f.E == cd.field_E && f.n[0] == nn[0] && f.n[1] == nn[1] && f.n[2] == nn[2]) off = f.off;
+            if (off == SIZE_MAX) {
+                off = fa_all.size();
+                const size_t nodes = static_cast<size_t>(nn[0]) * nn[1] * nn[2];
+                fa_all.resize(off + nodes);
+                for (size_t i = 0; i < nodes; ++i)
+                    fa_all[off + i] = make_float4(cd.field_E[3 * i], cd.field_E[3 * i + 1], cd.field_E[3 * i + 2], 0.0f);
+                fields_seen.push_back({cd.field_E, {nn[0], nn[1], nn[2]}, off});
+            }
+            fa_off[c] = off;
+            P.nx = static_cast<int>(nn[0]); P.ny = static_cast<int>(nn[1]); P.nz = static_cast<int>(nn[2]);
+            P.n_terms = 1;
+            P.x0 = cd.field_origin_m[0]; P.y0 = cd.field_origin_m[1]; P.z0 = cd.field_origin_m[2];
+            P.dx = cd.field_spacing_m[0]; P.dy = cd.field_spacing_m[1]; P.dz = cd.field_spacing_m[2];
+            P.inv_dx = 1.0 / P.dx; P.inv_dy = 1.0 / P.dy; P.inv_dz = 1.0 / P.dz;
+            P.x_last = P.x0 + (nn[0] - 1) * P.dx;
+            P.y_last = P.y0 + (nn[1] - 1) * P.dy;
+            P.z_last = P.z0 + (nn[2] - 1) * P.dz;
+            P.ax_x = P.ax_y = P.ax_z = nullptr;
+            P.axf_x = P.axf_y = P.axf_z = nullptr;
+            for (int j = 0; j < kMaxFieldTerms; ++j) {
+                P.term_rf[j] = 0;
+                P.term_a[j] = P.term_w[j] = P.term_phi[j] = 0.0;
+            }
+        }
         mix_off[c] = SIZE_MAX;
         if (mixture) {
             std::vector<icarion_gas_component> mix;
@@ -1852,6 +1904,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
     copy_in(h->b_tables, tables.data(), std::max<size_t>(tables.size(), 1), s, "batch step tables");
     h->b_atoms.resize(std::max<size_t>(atoms_all.size(), 1));
     if (!atoms_all.empty()) copy_in(h->b_atoms, atoms_all.data(), atoms_all.size(), s, "batch atoms");
+    h->b_fa32.resize(std::max<size_t>(fa_all.size(), 1));
+    if (!fa_all.empty()) copy_in(h->b_fa32, fa_all.data(), fa_all.size(), s, "batch field arrays");
     h->b_mix_sigma.resize(std::max<size_t>(mix_sigma.size(), 1));
     if (!mix_sigma.empty()) copy_in(h->b_mix_sigma, mix_sigma.data(), mix_sigma.size(), s, "batch mixture cross sections");
     h->b_counters.resize(4ull * n_clouds);
@@ -1874,8 +1928,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         P.mix_sigma = mix_off[c] == SIZE_MAX ? nullptr : h->b_mix_sigma.p + mix_off[c];
         P.counters = h->b_counters.p + 4ull * c;
         P.atoms = h->b_atoms.p + atom_off[c];
-        P.fa_f32 = nullptr;
-        P.fa_f64 = nullptr;
+        P.fa_f32 = fa_off[c] == SIZE_MAX ? nullptr : h->b_fa32.p + fa_off[c];
+        P.fa_f64 = nullptr;   // FP64 clouds read the FP32 nodes, as after a dtype 0 upload
         P.mlp_blob = nullptr;
     }
     copy_in(h->b_params_d, params.data(), n_clouds, s, "batch params");

@@ -912,10 +912,10 @@ __device__ __forceinline__ void batch_ion(const StepParams& P, const IonPtrs& S,
     }
 }
 
-// MIX: the kernel also holds the gas-mixture kinds (12-15) and, in FP64, the
-// geometry EHSS kinds (16-19).  Their bodies need more registers, so a batch
-// without such clouds runs the variant without them (FP32: 90 VGPRs, 5 waves
-// per SIMD, against 110 and 4).
+// MIX: the kernel also holds the gas-mixture kinds (12-15), in FP64 the
+// geometry EHSS kinds (16-19), and the uniform-grid field-array kinds (20-25).
+// Their bodies need more registers, so a batch without such clouds runs the
+// variant without them (FP32: 90 VGPRs, 5 waves per SIMD).
 template <int PREC, bool MIX>
 __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& P, const IonPtrs& S,
                                                uint32_t i, uint32_t& pushed, uint32_t& nc) {
@@ -939,6 +939,12 @@ __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& 
                     case 13: batch_ion<PREC, 1, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
                     case 14: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
                     case 15: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 20: batch_ion<PREC, 0, FIELD_ARRAY, COLL_NONE>(P, S, i, pushed, nc); break;
+                    case 21: batch_ion<PREC, 1, FIELD_ARRAY, COLL_NONE>(P, S, i, pushed, nc); break;
+                    case 22: batch_ion<PREC, 0, FIELD_ARRAY, COLL_HSS>(P, S, i, pushed, nc); break;
+                    case 23: batch_ion<PREC, 1, FIELD_ARRAY, COLL_HSS>(P, S, i, pushed, nc); break;
+                    case 24: batch_ion<PREC, 0, FIELD_ARRAY, COLL_DAMPING>(P, S, i, pushed, nc); break;
+                    case 25: batch_ion<PREC, 1, FIELD_ARRAY, COLL_DAMPING>(P, S, i, pushed, nc); break;
                     default:
                         if constexpr (PREC == 0) {
                             switch (kind) {

@@ -193,10 +193,12 @@ class Handle:
     def _cloud_array(self, clouds):
         """clouds: list of dicts with domain, physics, species (list), ensemble,
         t0, dt, n_steps and optional step_offset, waveforms ({slot: desc}),
-        mixture ((components, sigma) as set_gas_mixture takes them) and atoms
-        ((n, 4) x, y, z, r for geometry EHSS, held in FP32)."""
+        mixture ((components, sigma) as set_gas_mixture takes them), atoms
+        ((n, 4) x, y, z, r for geometry EHSS, held in FP32) and field ((E, origin,
+        spacing) as upload_field_array takes them, E held in FP32)."""
         arr = (_abi.CloudDesc * len(clouds))()
         keep = []
+        fields = {}   # one FP32 copy per field array, so that clouds share it
         for k, c in enumerate(clouds):
             sp = species_array(c["species"])
             soa = c["ensemble"].soa()
@@ -231,6 +233,21 @@ class Handle:
                 keep.append(at)
                 arr[k].atoms_xyzr = at.ctypes.data_as(C.POINTER(C.c_float))
                 arr[k].n_atoms = at.shape[0]
+            if c.get("field") is not None:
+                E, origin, spacing = c["field"]
+                key = id(E)
+                if key not in fields:
+                    fields[key] = np.ascontiguousarray(E, dtype=np.float32)
+                Ef = fields[key]
+                if Ef.ndim != 4 or Ef.shape[3] != 3:
+                    raise ValueError("field E must be [nz][ny][nx][3]")
+                keep.append(Ef)
+                arr[k].field_E = Ef.ctypes.data_as(C.POINTER(C.c_float))
+                nz, ny, nx = Ef.shape[:3]
+                for a, v in enumerate((nx, ny, nz)):
+                    arr[k].field_n[a] = v
+                    arr[k].field_origin_m[a] = float(origin[a])
+                    arr[k].field_spacing_m[a] = float(spacing[a])
         return arr, keep
 
     def batch_upload(self, clouds):

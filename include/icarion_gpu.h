@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define ICARION_GPU_ABI_VERSION 3
+#define ICARION_GPU_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -670,11 +670,14 @@ int icarion_gpu_synchronize(icarion_gpu *h);
  * kernel, so FP32 clouds can differ from a single-cloud run by FP32
  * rounding, which collisions amplify into statistical agreement).  Covered: RK4 / Boris; IMS, TIMS, quadrupole and Orbitrap
  * analytic fields; no collisions, HSS, Langevin / Friction (+ OU kicks);
- * FP64 or FP32 + Kahan per cloud; EHSS with the surrogate (FP32 + Kahan,
+ * FP64 or FP32 + Kahan per cloud; HSS gas mixtures; geometry EHSS in FP64
+ * clouds (the cloud's own atom table); EHSS with the surrogate (FP32 + Kahan,
  * the handle's surrogate parameters, species sigma_ehss and features; atoms
- * are not needed).  Not covered (ICARION_ERR_UNSUPPORTED): field arrays, EHSS
- * geometry collisions, RK45.  The batch lives in the handle beside (and
- * independent of) its single-cloud state. */
+ * are not needed); a uniform-grid field array per cloud (RK4 / Boris; no
+ * collisions, HSS, Langevin / Friction).  Not covered
+ * (ICARION_ERR_UNSUPPORTED): RK45, geometry EHSS in FP32 clouds, field arrays
+ * with a gas mixture or EHSS, composite / non-uniform field arrays.  The batch
+ * lives in the handle beside (and independent of) its single-cloud state. */
 typedef struct icarion_cloud_desc {
     icarion_domain_desc domain;
     icarion_physics_desc physics;
@@ -704,6 +707,17 @@ typedef struct icarion_cloud_desc {
                                      the cloud's EHSS atom table, indexed by its
                                      species' atom_offset / atom_count (geometry
                                      EHSS, FP64 clouds)                        */
+    const float *field_E;         /* NULL, or [nz][ny][nx][3] E (V/m, FP32) at
+                                     the nodes of a uniform grid: the cloud's
+                                     field array (domain.use_field_array = 1),
+                                     as icarion_gpu_upload_field_array (dtype 0)
+                                     on a handle of its own; clouds passing the
+                                     same pointer and grid share one device
+                                     copy                                      */
+    uint32_t field_n[3];          /* nodes along x, y, z (>= 2 each)           */
+    uint32_t reserved1;
+    double field_origin_m[3];     /* the first node                            */
+    double field_spacing_m[3];    /* node spacing per axis (> 0)               */
 } icarion_cloud_desc;
 
 /* Launch shape of the persistent batch kernel (per precision class). */

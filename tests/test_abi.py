@@ -59,7 +59,8 @@ STRUCTS = {
     "icarion_train_desc": (_abi.TrainDesc, ["epochs", "lr", "eps", "seed"]),
     "icarion_cloud_desc": (_abi.CloudDesc, ["domain", "physics", "species", "n_species", "ions",
                                             "t0_s", "n_steps", "step_offset", "waveforms", "mixture",
-                                            "n_components", "atoms_xyzr"]),
+                                            "n_components", "atoms_xyzr", "field_E", "field_n",
+                                            "field_origin_m", "field_spacing_m"]),
     "icarion_batch_tuning": (_abi.BatchTuning, ["block_size", "groups_per_cu"]),
     "icarion_cloud_stats": (_abi.CloudStats, ["ion_steps", "arrived", "t_end_s", "done_ms"]),
     "icarion_field_term_desc": (_abi.FieldTermDesc, ["E", "dtype", "scale_kind", "constant_scale",

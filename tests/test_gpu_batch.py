@@ -293,6 +293,19 @@ def test_batch_rejects_what_it_does_not_cover():
         with pytest.raises(gpu.IcarionError) as ei:
             h.batch_upload(bad)
         assert ei.value.code == _abi.ERR_UNSUPPORTED
+        # a field-array domain needs the cloud's array; with a gas mixture it is
+        # not covered
+        wf = W.field_array_workload(n=64, grid=(8, 8, 8), seed=1)
+        fa = dict(domain=wf["domain"], physics=wf["physics"], species=wf["species"],
+                  ensemble=wf["ensemble"], dt=wf["dt"], n_steps=10)
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_upload([fa])
+        assert ei.value.code == _abi.ERR_INVALID
+        fa["field"] = wf["field"]
+        fa["mixture"] = ([(1e22, 4.65e-26, True)], np.full((len(wf["species"]), 1), 1e-18))
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.batch_upload([fa])
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
         with pytest.raises(gpu.IcarionError) as ei:
             h.batch_run()
         assert ei.value.code == _abi.ERR_STATE

@@ -77,8 +77,9 @@ def oracle_cloud(oracle, c, ens):
     ph = c["physics"]
     ph64 = W.physics(ph.integrator, ph.collision_model, ph.ou_thermal_kicks, _abi.PRECISION_FP64,
                      ph.rng_seed, ph.ehss_mode)
+    field = oracle.Field(*c["field"]) if c.get("field") is not None else None
     return oracle.run(c["domain"], ph64, c["species"], ens, c["t0"], c["dt"], c["n_steps"],
-                      step_offset=c["step_offset"])
+                      step_offset=c["step_offset"], field=field)
 
 
 def agree(g, o, rtol, pos_scale, vel_scale):
@@ -185,3 +186,78 @@ def test_batch_orbitrap_1e6_steps_against_fp64_oracle(oracle):
     assert res["fp32"][0] < 1e-3 and res["fp32"][1] < 1e-2
     assert res["fp32"][2] < 1e-3 and res["fp32"][3] < 1e-2
     assert res["fp32"][4] < 1e-3 and res["fp32"][5] < 1e-2
+
+
+def field_array_clouds(seed=23):
+    """Uniform-grid field-array clouds (IMS in N2 at 300 Pa on a 24 x 20 x 40
+    grid, FP32 nodes) in both precisions - RK4 + HSS, Boris collisionless, RK4
+    collisionless - two of them sharing one array, next to an analytic IMS
+    cloud and a quadrupole cloud, in one launch."""
+    rng = np.random.default_rng(seed)
+    wa = W.field_array_workload(n=3000, grid=(24, 20, 40), seed=2)
+    wb = W.field_array_workload(n=2500, grid=(24, 20, 40), seed=5)
+    plan = [("fa_rk4_hss", wa, _abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, _abi.PRECISION_FP64),
+            ("fa_boris", wa, _abi.INTEGRATOR_BORIS, _abi.COLLISION_NONE, _abi.PRECISION_FP64),
+            ("fa_rk4_hss", wb, _abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, _abi.PRECISION_FP32_KAHAN),
+            ("fa_rk4", wb, _abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, _abi.PRECISION_FP32_KAHAN),
+            ("fa_rk4", wa, _abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, _abi.PRECISION_FP64)]
+    clouds = []
+    for k, (kind, w, integ, coll, prec) in enumerate(plan):
+        ph = W.physics(integ, coll, precision=prec, seed=70 + k)
+        n = int(rng.integers(1000, w["ensemble"].n))
+        clouds.append(dict(kind=kind, domain=w["domain"], physics=ph, species=w["species"],
+                           ensemble=w["ensemble"].subset(np.arange(n)), field=w["field"],
+                           t0=2e-8 * k, dt=w["dt"], n_steps=int(rng.integers(800, 2000)),
+                           step_offset=13 * k))
+    for prec in (_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN):
+        w = W.ims_hss_workload(n=900, seed=9, precision=prec)
+        clouds.append(dict(kind="ims", domain=w["domain"], physics=w["physics"], species=w["species"],
+                           ensemble=w["ensemble"].copy(), t0=0.0, dt=w["dt"], n_steps=700,
+                           step_offset=0))
+    return clouds
+
+
+def test_field_array_clouds_in_the_batch_against_oracle(oracle):
+    """Field-array clouds of the batch against the FP64 oracle's trilinear
+    interpolation on the same nodes: FP64 ion by ion (1e-9; >= 99 % with HSS),
+    FP32 + Kahan collisionless by vector errors (position against 1 mm: median
+    1e-6, max 1e-5; velocity against the speed: median 1e-6, max 2e-5; the same
+    fate for >= 99.9 % of ions), FP32 + Kahan with HSS by statistics."""
+    clouds = field_array_clouds()
+    init = [c["ensemble"].copy() for c in clouds]
+    print("batch with field-array clouds: " + ", ".join(
+        f"{c['kind']}/{c['physics'].precision} n={c['ensemble'].n} steps={c['n_steps']}" for c in clouds),
+        flush=True)
+    cloud_stats = batch_run(clouds)
+    for c, e0, st in zip(clouds, init, cloud_stats):
+        o = e0.copy()
+        so = oracle_cloud(oracle, c, o)
+        g = c["ensemble"]
+        fp64 = c["physics"].precision == _abi.PRECISION_FP64
+        stochastic = c["physics"].collision_model != _abi.COLLISION_NONE
+        tag = f"{c['kind']} {'fp64' if fp64 else 'fp32'} n={g.n} steps={c['n_steps']}"
+        assert st.ion_steps == pytest.approx(so.ion_steps, rel=0.01 if stochastic else 1e-3), tag
+        if fp64:
+            ok = agree(g, o, 1e-9, 1e-6, 1.0) & (g.n_collisions == o.n_collisions)
+            print(f"{tag}: {ok.mean():.4f} of ions equal the oracle to 1e-9", flush=True)
+            assert ok.mean() >= (0.99 if stochastic else 1.0), tag
+        elif not stochastic:
+            assert (g.active == o.active).mean() >= 0.999, tag
+            both = g.alive & o.alive
+            dp = (np.linalg.norm(g.pos - o.pos, axis=1) / 1e-3)[both]
+            speed = np.maximum(np.linalg.norm(o.vel, axis=1), 10.0)
+            dv = (np.linalg.norm(g.vel - o.vel, axis=1) / speed)[both]
+            print(f"{tag}: position/1mm median {np.median(dp):.2e} max {dp.max():.2e}; "
+                  f"velocity/speed median {np.median(dv):.2e} max {dv.max():.2e}", flush=True)
+            assert np.median(dp) < 1e-6 and dp.max() < 1e-5, tag
+            assert np.median(dv) < 1e-6 and dv.max() < 2e-5, tag
+        else:
+            z0 = e0.z
+            sg, so_ = g.alive.mean(), o.alive.mean()
+            s_err = 4 * math.sqrt(2 * max(so_ * (1 - so_), 1.0 / g.n) / g.n) + 0.01
+            print(f"{tag}: survival {sg:.4f} vs {so_:.4f}", flush=True)
+            assert abs(sg - so_) <= s_err, tag
+            close_mean(((g.z - z0)[g.alive]), ((o.z - z0)[o.alive]), tag + " drift")
+            close_mean(g.n_collisions.astype(float), o.n_collisions.astype(float),
+                       tag + " collisions")
+            assert stats.ks_2samp(g.z[g.alive], o.z[o.alive]).pvalue > 1e-3, tag
