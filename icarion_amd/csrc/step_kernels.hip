@@ -702,15 +702,18 @@ __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const
 // a colliding ion draws (u, psi), the MLP maps (u, cos psi, sin psi, features)
 // to the contact normal in the collision frame, and the relative velocity is
 // reflected about it (offline-table branch, EHSSCollisionHandler.cpp:253-277).
-// Every lane stays in the step loop (dead lanes are masked, not exited) so the
-// wave-wide matrix instructions always see the whole wavefront.
-// One ion per lane over P.n_steps steps; every lane of the wavefront must call
-// it (lanes with i >= P.n_ions are carried along masked); `lds` holds the
-// packed MLP weights.
-template <int INTEG, int FIELD>
+// mlp_eval(step, coll, in, out) evaluates the MLP for the colliding lanes:
+// mlp_group_eval (the single-cloud kernel: the workgroup's colliding ions
+// gathered into 64-ion batches through LDS, so the matrix cores evaluate about
+// as many rows as there are collisions) or mlp_wave_eval_any (the batch
+// kernel: per wavefront, no barriers).  Every lane stays in the step loop
+// (dead lanes are masked, not exited) so that barriers and the wave-wide matrix
+// instructions see every lane they need.  One ion per lane over P.n_steps
+// steps; with mlp_group_eval every lane of the workgroup calls it with the same
+// P.n_steps (lanes with i >= P.n_ions are carried along masked).
+template <int INTEG, int FIELD, class MlpEval>
 __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, uint32_t i,
-                                         const unsigned char* lds, uint32_t& pushed,
-                                         uint32_t& ncoll_launch) {
+                                         MlpEval&& mlp_eval, uint32_t& pushed, uint32_t& ncoll_launch) {
     const bool valid = i < P.n_ions;
     const uint32_t ii = valid ? i : 0;
     const FastConst& C = P.fc;
@@ -823,14 +826,8 @@ __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, 
                 }
             }
         }
-        if (__any(coll)) {
-            float out[3];
-            // a wave in its matrix-core section runs at raised priority so that
-            // its MFMA chain keeps issuing while the other wave of the SIMD
-            // fills the gaps with vector work (5-7 % per launch at 1M ions)
-            __builtin_amdgcn_s_setprio(1);
-            mlp_wave_eval(lds, in, out);
-            __builtin_amdgcn_s_setprio(0);
+        float out[3];
+        if (mlp_eval(k, coll, in, out)) {
             if (coll) {
                 const float nrm = fast_sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
                 float mu = nrm > 0.f ? -out[2] / nrm : 0.f;
@@ -885,12 +882,18 @@ __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, 
 }
 
 template <int INTEG, int FIELD>
-__global__ void __launch_bounds__(256, 2) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
+__global__ void __launch_bounds__(kSurrogateBlock, 3) step_surrogate_kernel(const StepParams P, const IonPtrs S) {
     __shared__ uint4 mlp_lds[kPackBytes / 16];
+    __shared__ MlpExchange<kSurrogateBlock / 64> xch;
+    mlp_exchange_init(xch);
     mlp_stage(P.mlp_blob, mlp_lds);
     const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
     uint32_t pushed = 0, nc = 0;
-    surr_ion<INTEG, FIELD>(P, S, blockIdx.x * blockDim.x + threadIdx.x, lds, pushed, nc);
+    surr_ion<INTEG, FIELD>(P, S, blockIdx.x * blockDim.x + threadIdx.x,
+                           [&](uint32_t k, bool coll, const float* in, float* out) {
+                               return mlp_group_eval(lds, xch, k, coll, in, out, k + blockIdx.x);
+                           },
+                           pushed, nc);
     add_counters(P, pushed, nc);
 }
 
@@ -1019,9 +1022,12 @@ __global__ void __launch_bounds__(256) batch_kernel(const StepParams* __restrict
 
 // FP32 + EHSS surrogate clouds: the packed MLP weights are staged into LDS once
 // per persistent workgroup and serve every tile it takes.  kind = integrator
-// (0 RK4, 1 Boris) + 2 * (Orbitrap field).  A wavefront whose 64 ions all lie
-// beyond the cloud's end skips the tile (the MLP is evaluated per wavefront,
-// so the other wavefronts do not need it).
+// (0 RK4, 1 Boris) + 2 * (Orbitrap field).  The MLP runs per wavefront, both
+// halves together (mlp_wave_eval_any, two waves per SIMD): in this kernel the
+// workgroup-wide batches and the one-block-at-a-time core of the single-cloud
+// kernel measured slower (2.53e10 against 2.88e10 ion-steps/s on 256 EHSS
+// clouds).  A wavefront whose 64 ions all lie beyond the cloud's end skips the
+// tile.
 __global__ void __launch_bounds__(256, 2) batch_surrogate_kernel(const StepParams* __restrict__ params,
                                                                  const IonPtrs* __restrict__ ions,
                                                                  const uint32_t* __restrict__ kinds,
@@ -1033,6 +1039,9 @@ __global__ void __launch_bounds__(256, 2) batch_surrogate_kernel(const StepParam
     __shared__ uint4 mlp_lds[kPackBytes / 16];
     mlp_stage(blob, mlp_lds);
     const unsigned char* lds = reinterpret_cast<const unsigned char*>(mlp_lds);
+    const auto eval = [&](uint32_t, bool coll, const float* in, float* out) {
+        return mlp_wave_eval_any(lds, coll, in, out);
+    };
     batch_queue_loop(tiles, n_tiles, queue, done, start, [&](uint32_t c, uint32_t first) {
         const StepParams& P = params[c];
         const IonPtrs S = ions[c];
@@ -1042,10 +1051,10 @@ __global__ void __launch_bounds__(256, 2) batch_surrogate_kernel(const StepParam
         if (wave_first < P.n_ions) {
             const uint32_t i = first + threadIdx.x;
             switch (kind) {
-                case 0: surr_ion<0, FIELD_LINEAR>(P, S, i, lds, pushed, nc); break;
-                case 1: surr_ion<1, FIELD_LINEAR>(P, S, i, lds, pushed, nc); break;
-                case 2: surr_ion<0, FIELD_ORBITRAP>(P, S, i, lds, pushed, nc); break;
-                case 3: surr_ion<1, FIELD_ORBITRAP>(P, S, i, lds, pushed, nc); break;
+                case 0: surr_ion<0, FIELD_LINEAR>(P, S, i, eval, pushed, nc); break;
+                case 1: surr_ion<1, FIELD_LINEAR>(P, S, i, eval, pushed, nc); break;
+                case 2: surr_ion<0, FIELD_ORBITRAP>(P, S, i, eval, pushed, nc); break;
+                case 3: surr_ion<1, FIELD_ORBITRAP>(P, S, i, eval, pushed, nc); break;
                 default: break;
             }
         }
@@ -1282,6 +1291,9 @@ hipError_t launch_step(int integrator, int field, int coll, int precision, int s
                              : pick_field<0>(field, coll, precision);
     }
     if (!fn) return hipErrorInvalidValue;
+    // the surrogate kernel runs its own workgroup size (the handle's block size
+    // does not change any result)
+    if (surrogate) block = kSurrogateBlock;
     const uint32_t grid = (P.n_ions + block - 1) / block;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, stream, P, S);
     return hipGetLastError();

@@ -37,6 +37,10 @@ namespace icarion {
 constexpr int kMlpIn = 6;
 constexpr int kMlpH = 128;
 constexpr int kMlpOut = 3;
+// Workgroup size of the single-cloud surrogate step kernel, which runs three
+// wavefronts per SIMD (3 workgroups of 50 KB LDS per CU).  Eight-wavefront
+// workgroups at four wavefronts per SIMD spill registers and run no faster.
+constexpr int kSurrogateBlock = 256;
 constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpOut * kMlpH + kMlpOut;
 // natural parameter layout (row-major, out x in)
 constexpr int kOffW1 = 0;
@@ -147,10 +151,21 @@ __device__ __forceinline__ void mlp_stage(const uint4* __restrict__ blob, uint4*
     __syncthreads();
 }
 
-// Evaluate the MLP for the 64 ions of this wavefront; in[0..5] are this lane's
-// inputs, out[] receives this lane's outputs.  Must be called by all 64 lanes
-// together (matrix instructions are wave-wide).
-__device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const float in[6], float out[3]) {
+// bf16 of four floats (round to nearest even), as one layer-1 operand slot quad.
+__device__ __forceinline__ bf16x4 mlp_pack4(float a, float b, float c, float d) {
+    bf16v4 r;
+    r[0] = (__bf16)a; r[1] = (__bf16)b; r[2] = (__bf16)c; r[3] = (__bf16)d;
+    return __builtin_bit_cast(bf16x4, r);
+}
+
+// The MLP for 64 ions on the matrix cores.  X0 / X1 are layer 1's B operands
+// of the two 32-ion blocks: lane l carries slots 4*(l/32) .. 4*(l/32)+3 of
+// ion 32*nb + l%32, slots = (x0..x5, 1, 0) in bf16.  On return lanes < 32 hold
+// the outputs of ion lane in r0 and of ion 32 + lane in r1 (lanes >= 32 hold
+// nothing useful).  Must be called by all 64 lanes together (matrix
+// instructions are wave-wide).
+__device__ __forceinline__ void mlp_wave_core(const unsigned char* lds, bf16x4 X0, bf16x4 X1,
+                                              float r0[3], float r1[3]) {
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5;  // 0: lanes 0-31, 1: lanes 32-63
     const bf16x4* A1 = reinterpret_cast<const bf16x4*>(lds + kPackA1);
@@ -158,23 +173,6 @@ __device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const fl
     const bf16x8* A3 = reinterpret_cast<const bf16x8*>(lds + kPackA3);
     const float* b2 = reinterpret_cast<const float*>(lds + kPackB2);
     const float* b3 = reinterpret_cast<const float*>(lds + kPackB3);
-
-    // B operand of layer 1 for the two 32-ion blocks: lane l needs slots
-    // 4*half..4*half+3 of ion 32*nb + l%32, slots = (x0..x5, 1, 0).
-    float partner[6];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) partner[j] = __shfl_xor(in[j], 32, 64);
-    auto pack_in = [](float a, float b, float c, float d) {
-        bf16v4 r;
-        r[0] = (__bf16)a; r[1] = (__bf16)b; r[2] = (__bf16)c; r[3] = (__bf16)d;
-        return __builtin_bit_cast(bf16x4, r);
-    };
-    // nb = 0: lanes < 32 own ion (slots 0-3), lanes >= 32 partner's ion (4-7);
-    // nb = 1: lanes < 32 partner's ion (0-3), lanes >= 32 own ion (4-7)
-    const bf16x4 X0 = half == 0 ? pack_in(in[0], in[1], in[2], in[3])
-                                : pack_in(partner[4], partner[5], 1.0f, 0.0f);
-    const bf16x4 X1 = half == 0 ? pack_in(partner[0], partner[1], partner[2], partner[3])
-                                : pack_in(in[4], in[5], 1.0f, 0.0f);
     // Both 32-ion blocks go through each weight fragment together, so every
     // A fragment is read from LDS once per wavefront.
     // layer 1 (bias in slot 6): chunk kc = 2*mb + q holds rows 32*mb + 16*q + ...
@@ -222,8 +220,76 @@ __device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const fl
         }
     }
     // lanes < 32 hold output rows 0..2 (v = 0..2) of ion 32*nb + lane in y{nb}
-    const float res0[3] = {y0[0] + b3[0], y0[1] + b3[1], y0[2] + b3[2]};
-    const float res1[3] = {y1[0] + b3[0], y1[1] + b3[1], y1[2] + b3[2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r0[c] = y0[c] + b3[c];
+        r1[c] = y1[c] + b3[c];
+    }
+}
+
+// The MLP for one 32-ion block: X is layer 1's B operand (lane l carries slots
+// 4*(l/32) .. 4*(l/32)+3 of ion l%32); on return lanes < 32 hold the outputs of
+// ion lane in r.  Half the live registers of mlp_wave_core (each weight
+// fragment is read from LDS once per block instead of once per two blocks);
+// every output equals mlp_wave_core's for the same operand.
+__device__ __forceinline__ void mlp_block_core(const unsigned char* lds, bf16x4 X, float r[3]) {
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    const bf16x4* A1 = reinterpret_cast<const bf16x4*>(lds + kPackA1);
+    const bf16x8* A2 = reinterpret_cast<const bf16x8*>(lds + kPackA2);
+    const bf16x8* A3 = reinterpret_cast<const bf16x8*>(lds + kPackA3);
+    const float* b2 = reinterpret_cast<const float*>(lds + kPackB2);
+    const float* b3 = reinterpret_cast<const float*>(lds + kPackB3);
+    bf16x8 H[8];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        f32x16 c = {};
+        c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A1[mb * 64 + lane], X, c, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) H[2 * mb + q] = relu_pack8(c, 8 * q);
+    }
+    f32x16 y = {};
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        const bf16x8* A2m = A2 + mb * 8 * 64 + lane;
+        f32x16 c;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float4 bq = *reinterpret_cast<const float4*>(b2 + 32 * mb + 8 * g + 4 * half);
+            c[4 * g] = bq.x;
+            c[4 * g + 1] = bq.y;
+            c[4 * g + 2] = bq.z;
+            c[4 * g + 3] = bq.w;
+        }
+#pragma unroll
+        for (int kc = 0; kc < 8; ++kc) c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A2m[kc * 64], H[kc], c, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            y = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A3[(2 * mb + q) * 64 + lane], relu_pack8(c, 8 * q), y, 0, 0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = y[c] + b3[c];
+}
+
+// Evaluate the MLP for the 64 ions of this wavefront; in[0..5] are this lane's
+// inputs, out[] receives this lane's outputs.  Must be called by all 64 lanes
+// together.
+__device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const float in[6], float out[3]) {
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5;
+    // B operand of layer 1 for the two 32-ion blocks: lane l needs slots
+    // 4*half..4*half+3 of ion 32*nb + l%32, slots = (x0..x5, 1, 0).
+    float partner[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) partner[j] = __shfl_xor(in[j], 32, 64);
+    // nb = 0: lanes < 32 own ion (slots 0-3), lanes >= 32 partner's ion (4-7);
+    // nb = 1: lanes < 32 partner's ion (0-3), lanes >= 32 own ion (4-7)
+    const bf16x4 X0 = half == 0 ? mlp_pack4(in[0], in[1], in[2], in[3])
+                                : mlp_pack4(partner[4], partner[5], 1.0f, 0.0f);
+    const bf16x4 X1 = half == 0 ? mlp_pack4(partner[0], partner[1], partner[2], partner[3])
+                                : mlp_pack4(in[4], in[5], 1.0f, 0.0f);
+    float res0[3], res1[3];
+    mlp_wave_core(lds, X0, X1, res0, res1);
     // ion `lane`: lanes < 32 read their nb = 0 result; lanes >= 32 take the
     // nb = 1 result computed in lane - 32
     float o1[3];
@@ -231,6 +297,154 @@ __device__ __forceinline__ void mlp_wave_eval(const unsigned char* lds, const fl
     for (int c = 0; c < 3; ++c) o1[c] = __shfl_xor(res1[c], 32, 64);
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = half == 0 ? res0[c] : o1[c];
+}
+
+// The MLP for the colliding lanes of this wavefront only (no barriers), both
+// 32-lane halves together through mlp_wave_core.  `coll` lanes receive their
+// outputs in out[].  Returns false when no lane of the wavefront collides.
+// Must be called by all 64 lanes together.
+__device__ __forceinline__ bool mlp_wave_eval_any(const unsigned char* lds, bool coll, const float in[6],
+                                                  float out[3]) {
+    if (!__any(coll)) return false;
+    // a wave in its matrix-core section runs at raised priority so that its
+    // MFMA chain keeps issuing while the other wave of the SIMD fills the gaps
+    // with vector work
+    __builtin_amdgcn_s_setprio(1);
+    mlp_wave_eval(lds, in, out);
+    __builtin_amdgcn_s_setprio(0);
+    return true;
+}
+
+// LDS exchange of a workgroup (up to 4 wavefronts) whose colliding ions are
+// gathered into full 64-ion batches for the MLP.  Wavefront w writes the inputs
+// of its colliding lanes, in lane order, to slots 64 w .. 64 w + count - 1, as
+// the eight bf16 operand slots of layer 1 (x0..x5, 1, 0); the counts are
+// double-buffered by step parity, since a workgroup without any colliding ion
+// passes only one barrier per step.
+template <int WAVES>
+struct MlpExchange {
+    static constexpr int kWaves = WAVES;
+    u32v4 in[64 * WAVES];
+    float out[kMlpOut][64 * WAVES];
+    uint32_t count[2][WAVES];
+};
+
+// Zero the counts (wavefronts beyond blockDim.x never write theirs); the
+// barrier that follows in mlp_stage publishes them.
+template <int WAVES>
+__device__ __forceinline__ void mlp_exchange_init(MlpExchange<WAVES>& X) {
+    if (threadIdx.x < 2 * WAVES) X.count[threadIdx.x / WAVES][threadIdx.x % WAVES] = 0u;
+}
+
+// MLP outputs for the colliding lanes of the whole workgroup.  Every lane of
+// every wavefront calls this once per step (it holds barriers); `coll` lanes
+// pass their inputs in in[] and receive their outputs in out[].  The colliding
+// ions are numbered wavefront by wavefront, lane by lane; batch b holds ions
+// 64 b .. 64 b + 63 of that numbering and is evaluated by wavefront
+// (b + rot) mod (waves), so that successive steps spread the matrix work over
+// the SIMDs.  Batches are padded with zero operands whose outputs are dropped.
+// An ion's outputs do not depend on its batch or column: each output column of
+// a matrix instruction is its own dot product, and the operands are the bf16
+// values mlp_wave_eval forms.  Returns false when no ion of the workgroup
+// collides (out[] is then untouched).
+template <int WAVES>
+__device__ __forceinline__ bool mlp_group_eval(const unsigned char* lds, MlpExchange<WAVES>& X, uint32_t step,
+                                               bool coll, const float in[kMlpIn], float out[kMlpOut],
+                                               uint32_t rot) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t nw = __builtin_amdgcn_readfirstlane(blockDim.x >> 6);
+    const unsigned long long m = __ballot(coll);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    const uint32_t slot = 64 * w + rank;
+    if (coll) {
+        // the operand bits move as integers (a float vector in between may be
+        // copied with packed-float moves, which are disabled in this build)
+        const uint64_t lo4 = __builtin_bit_cast(uint64_t, mlp_pack4(in[0], in[1], in[2], in[3]));
+        const uint64_t hi4 = __builtin_bit_cast(uint64_t, mlp_pack4(in[4], in[5], 1.0f, 0.0f));
+        X.in[slot] = u32v4{static_cast<uint32_t>(lo4), static_cast<uint32_t>(lo4 >> 32),
+                           static_cast<uint32_t>(hi4), static_cast<uint32_t>(hi4 >> 32)};
+    }
+    uint32_t* cnt = X.count[step & 1];
+    if (lane == 0) cnt[w] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    // start[v]: number of colliding ions in wavefronts before v
+    uint32_t start[WAVES + 1];
+    start[0] = 0;
+#pragma unroll
+    for (int v = 0; v < WAVES; ++v) start[v + 1] = start[v] + __builtin_amdgcn_readfirstlane(cnt[v]);
+    const uint32_t total = start[WAVES];
+    if (total == 0) return false;
+    const uint32_t nb = (total + 63) >> 6;
+    const uint32_t b = (w + nw - rot % nw) % nw;
+    if (b < nb) {
+        // slot of colliding ion g (g < total) in the exchange: its wavefront
+        // is the number of wavefront starts past 0 at or below g
+        auto slot_of = [&](uint32_t g) {
+            uint32_t ww = 0, base = 0;
+#pragma unroll
+            for (int v = 1; v < WAVES; ++v) {
+                const bool past = g >= start[v];
+                ww += past ? 1u : 0u;
+                base = past ? start[v] : base;
+            }
+            return 64 * ww + (g - base);
+        };
+        const uint32_t half = lane >> 5;
+        const uint32_t g_own = 64 * b + lane, g_par = g_own ^ 32u;
+        const bool have_own = g_own < total, have_par = g_par < total;
+        const uint32_t s_own = have_own ? slot_of(g_own) : 0u;
+        const uint32_t s_par = have_par ? slot_of(g_par) : 0u;
+        // lane l carries operand slots 4*half .. 4*half+3: dwords 2*half, 2*half+1
+        const uint32_t* xin = reinterpret_cast<const uint32_t*>(X.in);
+        uint32_t o0 = 0u, o1 = 0u, q0 = 0u, q1 = 0u;
+        if (have_own) {
+            o0 = xin[4 * s_own + 2 * half];
+            o1 = xin[4 * s_own + 2 * half + 1];
+        }
+        if (have_par) {
+            q0 = xin[4 * s_par + 2 * half];
+            q1 = xin[4 * s_par + 2 * half + 1];
+        }
+        const bf16x4 own = __builtin_bit_cast(bf16x4, static_cast<uint64_t>(o0) | (static_cast<uint64_t>(o1) << 32));
+        const bf16x4 par = __builtin_bit_cast(bf16x4, static_cast<uint64_t>(q0) | (static_cast<uint64_t>(q1) << 32));
+        // block 0 = ions 64 b + 0..31, block 1 = ions 64 b + 32..63; the
+        // blocks go through the network one at a time (fewer live registers),
+        // and a block past the last colliding ion is skipped
+        const bf16x4 X0 = half == 0 ? own : par;
+        const bf16x4 X1 = half == 0 ? par : own;
+        const uint32_t n_blocks = total > 64 * b + 32 ? 2u : 1u;
+        // a wave in its matrix-core section runs at raised priority so that
+        // its MFMA chain keeps issuing while the other wave of the SIMD fills
+        // the gaps with vector work
+        __builtin_amdgcn_s_setprio(1);
+        // lanes < 32: block 0 is ion g_own, block 1 is ion g_par = g_own + 32
+        float r[3];
+        mlp_block_core(lds, X0, r);
+        if (half == 0 && have_own) {
+#pragma unroll
+            for (int c = 0; c < kMlpOut; ++c) X.out[c][s_own] = r[c];
+        }
+        if (n_blocks > 1) {
+            // a compiler memory fence: the second block reloads the weight
+            // fragments from LDS instead of keeping the first block's 128
+            // registers of them live
+            asm volatile("" ::: "memory");
+            mlp_block_core(lds, X1, r);
+            if (half == 0 && have_par) {
+#pragma unroll
+                for (int c = 0; c < kMlpOut; ++c) X.out[c][s_par] = r[c];
+            }
+        }
+        __builtin_amdgcn_s_setprio(0);
+    }
+    __syncthreads();
+    if (coll) {
+#pragma unroll
+        for (int c = 0; c < kMlpOut; ++c) out[c] = X.out[c][slot];
+    }
+    return true;
 }
 
 }  // namespace icarion
