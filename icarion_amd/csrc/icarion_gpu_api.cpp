@@ -229,6 +229,10 @@ struct icarion_gpu {
     std::vector<std::vector<icarion_gas_component>> dom_mix_all;
     std::vector<std::vector<double>> dom_emix_sig;
     DevBuf<double> dom_emix_d;
+    // several domains: each domain's species table (empty: the handle's),
+    // its rows concatenated on the device at each step call
+    std::vector<std::vector<SpeciesDev>> dom_species;
+    DevBuf<SpeciesDev> dom_species_d;
     // RK45 evaluates the waveforms in the kernel: their descriptors and points
     DevBuf<icarion_waveform_desc> wave_desc;
     DevBuf<double> wave_pts;
@@ -1437,6 +1441,9 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
     if (!h->mix.empty())
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their gas mixtures per domain (icarion_gpu_set_domain_gas_mixture)");
     bool dom_mixture = false;
+    for (uint32_t d = 0; d < h->n_dom; ++d)
+        if (!h->dom_species[d].empty() && h->dom_species[d].size() != h->n_species)
+            fail(ICARION_ERR_STATE, "a domain's species table differs in size from the species table");
     for (uint32_t d = 0; d < h->n_dom; ++d) {
         if (h->dom_mix[d].empty()) continue;
         dom_mixture = true;
@@ -1477,13 +1484,22 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         }
         h->dom_emix_d.resize(std::max<size_t>(esig.size(), 1));
         if (!esig.empty()) copy_in(h->dom_emix_d, esig.data(), esig.size(), h->stream, "domain mixture cross sections");
+        // the domains' own species tables, concatenated
+        std::vector<SpeciesDev> dsp;
+        std::vector<size_t> soff(nd, 0);
+        for (uint32_t d = 0; d < nd; ++d) {
+            soff[d] = dsp.size();
+            dsp.insert(dsp.end(), h->dom_species[d].begin(), h->dom_species[d].end());
+        }
+        h->dom_species_d.resize(std::max<size_t>(dsp.size(), 1));
+        if (!dsp.empty()) copy_in(h->dom_species_d, dsp.data(), dsp.size(), h->stream, "domain species tables");
         std::vector<StepEntry> tabs;
         tabs.reserve(size_t(nd) * n_steps);
         for (uint32_t d = 0; d < nd; ++d) {
             StepParams& P = Ps[d];
             P = derive_params(h->doms[d], h->phys);
             P.dt = dt;
-            P.species = h->species.p;
+            P.species = h->dom_species[d].empty() ? h->species.p : h->dom_species_d.p + soff[d];
             P.atoms = h->atoms_f32.p;
             P.n_ions = static_cast<uint32_t>(h->n);
             P.n_mix = 0;
@@ -2465,6 +2481,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_mix_ns.clear();
         h->dom_mix_all.clear();
         h->dom_emix_sig.clear();
+        h->dom_species.clear();
         if (n_domains == 1) return;
         h->doms.assign(domains, domains + n_domains);
         h->dom_wave.resize(n_domains);
@@ -2473,6 +2490,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_mix_ns.assign(n_domains, 0);
         h->dom_mix_all.resize(n_domains);
         h->dom_emix_sig.resize(n_domains);
+        h->dom_species.resize(n_domains);
         // DomainManager's constructor (DomainManager.cpp:26-63): a span per
         // cylindrical domain, z range and radius widened by 1e-12 m, with the
         // index of its domain, sorted by z_min (std::sort, as there)
@@ -2584,6 +2602,29 @@ int icarion_gpu_set_rk45(icarion_gpu* h, const icarion_rk45_desc* d) {
     });
 }
 
+namespace {
+
+// Device rows of a species table; every atom range must lie in the n_atoms
+// atoms of the handle's table.
+std::vector<SpeciesDev> species_rows(const icarion_species_desc* species, uint32_t n_species, uint64_t n_atoms) {
+    std::vector<SpeciesDev> sd(n_species);
+    for (uint32_t s = 0; s < n_species; ++s) {
+        const icarion_species_desc& d = species[s];
+        if (static_cast<uint64_t>(d.atom_offset) + d.atom_count > n_atoms)
+            fail(ICARION_ERR_INVALID, "species " + std::to_string(s) + " atoms exceed the atom table");
+        sd[s].sigma_hss = d.sigma_hss_m2;
+        sd[s].sigma_ehss = d.sigma_ehss_m2;
+        sd[s].K0_cm2_Vs = d.K0_cm2_Vs;
+        sd[s].atom_offset = d.atom_offset;
+        sd[s].atom_count = d.atom_count;
+        for (int f = 0; f < 3; ++f) sd[s].feat[f] = d.surrogate_features[f];
+        sd[s].pad = 0.f;
+    }
+    return sd;
+}
+
+}  // namespace
+
 int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species, uint32_t n_species,
                             const void* atoms_xyzr, uint32_t n_atoms, int32_t atoms_dtype) {
     return guarded([&] {
@@ -2595,21 +2636,11 @@ int icarion_gpu_set_species(icarion_gpu* h, const icarion_species_desc* species,
         if (n_atoms && atoms_dtype == 1)
             fail(ICARION_ERR_UNSUPPORTED, "the device atom table is float (atoms_dtype 0); round the "
                                           "coordinates to float or use the CPU path for double atoms");
-        std::vector<SpeciesDev> sd(n_species);
+        const std::vector<SpeciesDev> sd = species_rows(species, n_species, n_atoms);
         std::vector<uint32_t> offs(n_species), cnts(n_species);
         for (uint32_t s = 0; s < n_species; ++s) {
-            const icarion_species_desc& d = species[s];
-            if (static_cast<uint64_t>(d.atom_offset) + d.atom_count > n_atoms)
-                fail(ICARION_ERR_INVALID, "species " + std::to_string(s) + " atoms exceed the atom table");
-            sd[s].sigma_hss = d.sigma_hss_m2;
-            sd[s].sigma_ehss = d.sigma_ehss_m2;
-            sd[s].K0_cm2_Vs = d.K0_cm2_Vs;
-            sd[s].atom_offset = d.atom_offset;
-            sd[s].atom_count = d.atom_count;
-            for (int f = 0; f < 3; ++f) sd[s].feat[f] = d.surrogate_features[f];
-            sd[s].pad = 0.f;
-            offs[s] = d.atom_offset;
-            cnts[s] = d.atom_count;
+            offs[s] = species[s].atom_offset;
+            cnts[s] = species[s].atom_count;
         }
         set_device(h);
         copy_in(h->species, sd.data(), n_species, h->stream, "upload species");
@@ -2771,6 +2802,27 @@ int icarion_gpu_set_domain_gas_mixture(icarion_gpu* h, uint32_t domain, const ic
         h->dom_mix_ns[domain] = n_species;
         h->dom_mix_all[domain].assign(components, components + n_components);
         h->dom_emix_sig[domain].assign(sigma_m2, sigma_m2 + static_cast<size_t>(n_species) * n_components);
+    });
+}
+
+int icarion_gpu_set_domain_species(icarion_gpu* h, uint32_t domain, const icarion_species_desc* species,
+                                   uint32_t n_species) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (h->n_dom < 2) fail(ICARION_ERR_STATE, "icarion_gpu_set_domains with two or more domains not called");
+        if (domain >= h->n_dom) fail(ICARION_ERR_INVALID, "domain index out of range");
+        if (!species || n_species == 0) {
+            h->dom_species[domain].clear();
+            return;
+        }
+        if (!h->have_species || n_species != h->n_species)
+            fail(ICARION_ERR_INVALID, "n_species must match the species table (set_species first)");
+        for (uint32_t s = 0; s < n_species; ++s)
+            if (species[s].atom_offset != h->species_host[s].atom_offset ||
+                species[s].atom_count != h->species_host[s].atom_count)
+                fail(ICARION_ERR_INVALID, "species " + std::to_string(s) +
+                                              ": a domain's row keeps the handle's atom range (one geometry per species)");
+        h->dom_species[domain] = species_rows(species, n_species, h->n_atoms);
     });
 }
 

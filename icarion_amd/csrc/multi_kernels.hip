@@ -106,7 +106,10 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
     const double m = S.mass[i];
     const double q = S.charge[i];
     const uint32_t sidx = S.species[i];
-    const SpeciesDev sp = M.species[sidx];
+    // the species row of the step domain's table (cross sections and mobility
+    // resolved for that domain's gas), loaded when the domain changes
+    SpeciesDev sp = M.species[sidx];
+    int sp_dom = -1;
     uint8_t flags = S.flags[i];
     bool active = flags & 1, born = (flags >> 1) & 1;
     const double birth = S.birth ? S.birth[i] : -INFINITY;
@@ -141,6 +144,10 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
         }
         cur = dom;
         const StepParams& D = M.dom[dom];
+        if (dom != sp_dom) {
+            sp = D.species[sidx];
+            sp_dom = dom;
+        }
         if constexpr (COLL == COLL_DAMPING) {
             if (dom != lpre_dom) {
                 lpre = damping_pre_f64(D, sp, m, q);
@@ -256,7 +263,7 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
     const double md = S.mass[i];
     const double qd = S.charge[i];
     const uint32_t sidx = S.species[i];
-    const SpeciesDev sp = M.species[sidx];
+    SpeciesDev sp = M.species[sidx];   // the step domain's row, reloaded with the domain's constants
     const float m = (float)md;
     const float qm = (float)(qd / md);
     uint8_t flags = S.flags[i];
@@ -295,6 +302,7 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
         cur = dom;
         const StepParams& D = M.dom[dom];
         if (dom != const_dom) {
+            sp = D.species[sidx];
             if constexpr (COLL == COLL_DAMPING) lpre = (float)damping_pre_f64(D, sp, md, qd);
             inv_m_plus = (float)(1.0 / (md + D.m_gas));
             kbt2_dt_over_m = (float)(2.0 * kBoltzmann * D.T / md * D.dt);

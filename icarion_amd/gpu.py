@@ -27,7 +27,7 @@ EXPORTED = (
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
     "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
-    "icarion_gpu_step_noalloc",
+    "icarion_gpu_step_noalloc", "icarion_gpu_set_domain_species",
 )
 
 # icarion_gpu_step_noalloc's per-call step limit (ICARION_NOALLOC_MAX_STEPS)
@@ -412,6 +412,17 @@ class Handle:
         _check(self.lib.icarion_gpu_set_domain_gas_mixture(
             self.h, C.c_uint32(domain), arr, len(components), sig.ctypes.data_as(C.POINTER(C.c_double)),
             n_species), self.lib)
+
+    def set_domain_species(self, domain, species_list):
+        """One domain's species table on a multi-domain handle (the handle's
+        species, in order, with that domain's cross sections, CCS and K0);
+        an empty list clears that domain's."""
+        if not species_list:
+            _check(self.lib.icarion_gpu_set_domain_species(self.h, C.c_uint32(domain), None, 0), self.lib)
+            return
+        arr = species_array(species_list)
+        _check(self.lib.icarion_gpu_set_domain_species(self.h, C.c_uint32(domain), arr, len(species_list)),
+               self.lib)
 
     def set_domain_waveforms(self, domain, slots):
         """One domain's waveforms on a multi-domain handle: slots as in

@@ -540,8 +540,9 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
  * voltages or per-domain waveforms (icarion_gpu_set_domain_waveforms), RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
  * handoff in FP64 on the compensated position), no / HSS / Langevin /
  * Friction collisions (HSS also with per-domain gas mixtures,
- * icarion_gpu_set_domain_gas_mixture) and geometry EHSS (FP64, online), one
- * species table for all domains; icarion_gpu_step, icarion_gpu_step_batch and
+ * icarion_gpu_set_domain_gas_mixture) and geometry EHSS (FP64, online), the
+ * handle's species table or a domain's own (icarion_gpu_set_domain_species);
+ * icarion_gpu_step, icarion_gpu_step_batch and
  * icarion_gpu_step_snapshots (with each ion's domain index).  Other
  * combinations return ICARION_ERR_UNSUPPORTED when stepped. */
 int icarion_gpu_set_domains(icarion_gpu *h, const icarion_domain_desc *domains,
@@ -570,6 +571,21 @@ int icarion_gpu_set_domain_waveforms(icarion_gpu *h, uint32_t domain, const icar
  * FullConfig::domains. */
 int icarion_gpu_set_domain_gas_mixture(icarion_gpu *h, uint32_t domain, const icarion_gas_component *components,
                                        uint32_t n_components, const double *sigma_m2, uint32_t n_species);
+
+/* The species table of one domain of a multi-domain set: the species of
+ * icarion_gpu_set_species in the same order (n_species equal), with the cross
+ * sections, CCS and K0 resolved for that domain's gas.  The reference resolves
+ * them per ion species and the environment of the ion's domain
+ * (HSSCollisionHandler.cpp:263-310, EHSSCollisionHandler.cpp:462-547, called
+ * with the domain's EnvironmentConfig from SimulationEngineCollisionRuntime.cpp:
+ * 36-46; DampingForce reads K0 and the domain's gas, DampingForce.cpp:105-277).
+ * An ion takes the row of its step domain.  Each row keeps the handle's atom
+ * range (one geometry per species).  species = NULL or n_species = 0 clears the
+ * domain's table (the domain then uses the handle's); icarion_gpu_set_domains
+ * clears every domain's.  No reference counterpart as a call: it carries the
+ * per-domain part of the species-database lookup. */
+int icarion_gpu_set_domain_species(icarion_gpu *h, uint32_t domain, const icarion_species_desc *species,
+                                   uint32_t n_species);
 
 /* Every resident ion's domain index after the last step (the ensemble's
  * domain_index; 0 before the ion's first step, IonState's default).  n = the

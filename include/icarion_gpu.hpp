@@ -90,13 +90,23 @@ public:
     }
     // Several domains (icarion_gpu_set_domains): one descriptor and end aperture
     // per DomainConfig, in the config's order.
+    // domain_species: empty, or per domain its species rows (the same species
+    // in the same order, cross sections / CCS / K0 resolved for that domain's
+    // gas); an empty entry leaves that domain on `species`.
     void configure_domains(const std::vector<icarion_domain_desc>& domains, const std::vector<double>& end_aperture_m,
-                           const icarion_physics_desc& physics, const std::vector<icarion_species_desc>& species) {
+                           const icarion_physics_desc& physics, const std::vector<icarion_species_desc>& species,
+                           const std::vector<std::vector<icarion_species_desc>>& domain_species = {}) {
         if (!end_aperture_m.empty() && end_aperture_m.size() != domains.size())
             throw Error(ICARION_ERR_INVALID, "one end aperture per domain");
+        if (!domain_species.empty() && domain_species.size() != domains.size())
+            throw Error(ICARION_ERR_INVALID, "one species table (or none) per domain");
         check(icarion_gpu_set_domains(h_, domains.data(), end_aperture_m.empty() ? nullptr : end_aperture_m.data(),
                                       static_cast<uint32_t>(domains.size()), &physics));
         check(icarion_gpu_set_species(h_, species.data(), static_cast<uint32_t>(species.size()), nullptr, 0, 0));
+        for (size_t k = 0; k < domain_species.size(); ++k)
+            if (!domain_species[k].empty())
+                check(icarion_gpu_set_domain_species(h_, static_cast<uint32_t>(k), domain_species[k].data(),
+                                                     static_cast<uint32_t>(domain_species[k].size())));
         ready_ = true;
     }
     // Step control of ICARION_INTEGRATOR_RK45 (RK45Strategy(const AdaptiveConfig&)).

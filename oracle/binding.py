@@ -85,7 +85,7 @@ def load():
     lib.oracle_find_domain.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(d)]
     lib.oracle_find_domain.restype = C.c_int
     lib.oracle_run_multi.argtypes = [P(_abi.DomainDesc), P(d), C.c_uint32, P(P(_abi.OracleField)),
-                                     P(P(_abi.OracleGasMixture)), P(_abi.PhysicsDesc),
+                                     P(P(_abi.OracleGasMixture)), P(P(_abi.SpeciesDesc)), P(_abi.PhysicsDesc),
                                      P(_abi.SpeciesDesc), C.c_uint32, P(d), P(_abi.IonSoA), P(C.c_int32),
                                      d, d, C.c_uint64, C.c_uint64, C.c_int, P(_abi.StepStats)]
     lib.oracle_run_multi.restype = C.c_int
@@ -369,10 +369,12 @@ def find_domain(domains, pos, end_apertures=None):
 
 
 def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
-              end_apertures=None, threads=0, waveforms=None, atoms=None, mixtures=None):
+              end_apertures=None, threads=0, waveforms=None, atoms=None, mixtures=None,
+              domain_species=None):
     """The engine loop over several domains on ens and domain_index (int32,
     in/out) in place; returns StepStats.  waveforms: None, or per domain None
-    or {slot: WaveformDesc}."""
+    or {slot: WaveformDesc}.  domain_species: None, or per domain None or a
+    species list (that domain's rows of the species table)."""
     lib = load()
     arr, ap, app = _domains(domains, end_apertures)
     sp = species_array(species)
@@ -398,11 +400,21 @@ def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, ste
                 keep.append(gm)
                 ml[k] = C.pointer(gm.c)
         mptr = ml
+    sptr = C.POINTER(C.POINTER(_abi.SpeciesDesc))()
+    if domain_species is not None:
+        sl = (C.POINTER(_abi.SpeciesDesc) * len(domains))()
+        for k, rows in enumerate(domain_species):
+            if rows:
+                assert len(rows) == len(species)
+                ra = species_array(rows)
+                keep.append(ra)
+                sl[k] = C.cast(ra, C.POINTER(_abi.SpeciesDesc))
+        sptr = sl
     at = None
     if atoms is not None:
         at = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1)
     aptr = at.ctypes.data_as(C.POINTER(C.c_double)) if at is not None else C.POINTER(C.c_double)()
-    rc = lib.oracle_run_multi(arr, app, len(domains), fptr, mptr, C.byref(physics), sp, len(species), aptr, C.byref(soa),
+    rc = lib.oracle_run_multi(arr, app, len(domains), fptr, mptr, sptr, C.byref(physics), sp, len(species), aptr, C.byref(soa),
                               domain_index.ctypes.data_as(C.POINTER(C.c_int32)), t0, dt, n_steps,
                               step_offset, threads, C.byref(st))
     if rc != 0:

@@ -1262,6 +1262,7 @@ struct Domains {
     uint32_t n;
     const oracle_field* const* fields = nullptr;   // per domain (waveforms), may be NULL
     const oracle_gas_mixture* const* mixtures = nullptr;   // per domain (HSS gas mixture), may be NULL
+    const icarion_species_desc* const* species = nullptr;  // per domain (species table), may be NULL
     std::vector<double> ap_sq;
     std::vector<Span> spans;
     bool has_orbitrap = false;
@@ -1341,11 +1342,16 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
         dom_idx = dom;
         const icarion_domain_desc& d = D.d[dom];
         const oracle_field* f = D.fields ? D.fields[dom] : nullptr;
+        // the species row resolved for the step domain's gas: the handlers
+        // and DampingForce take the ion's species with the environment of its
+        // domain (SimulationEngineCollisionRuntime.cpp:36-46,
+        // HSSCollisionHandler.cpp:263-310, EHSSCollisionHandler.cpp:462-547)
+        const icarion_species_desc& sp_d = (D.species && D.species[dom]) ? D.species[dom][sidx] : sp;
         const double z_before = ion.pos.z;
         if (ph.integrator == ICARION_INTEGRATOR_BORIS)
-            boris_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+            boris_step(d, f, ph.collision_model, sp_d, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         else
-            rk4_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+            rk4_step(d, f, ph.collision_model, sp_d, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         ++pushed;
         if (ph.collision_model != ICARION_COLLISION_NONE) {
             // the step domain's gas mixture, if it has one
@@ -1356,7 +1362,7 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
                 tb.mix_row = gm->sigma_m2 + static_cast<size_t>(sidx) * gm->n_components;
             }
             PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
-            if (collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
+            if (collide(d, ph, sp_d, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
                 ++ion.ncoll;
                 ++collisions;
             }
@@ -1624,6 +1630,7 @@ int oracle_find_domain(const icarion_domain_desc* domains, const double* end_ape
 
 int oracle_run_multi(const icarion_domain_desc* domains, const double* end_aperture_m, uint32_t n_domains,
                      const oracle_field* const* fields, const oracle_gas_mixture* const* mixtures,
+                     const icarion_species_desc* const* domain_species,
                      const icarion_physics_desc* physics,
                      const icarion_species_desc* species, uint32_t n_species, const double* atoms,
                      icarion_ion_soa* ions, int32_t* domain_index, double t0, double dt, uint64_t n_steps,
@@ -1633,6 +1640,7 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
     Domains D(domains, end_aperture_m, n_domains);
     D.fields = fields;
     D.mixtures = mixtures;
+    D.species = domain_species;
     const size_t n = ions->n;
     std::vector<double> t_table(n_steps + 1);
     double t = t0;

@@ -63,9 +63,12 @@ int oracle_find_domain(const icarion_domain_desc *domains, const double *end_ape
  * RK4 / Boris, Philox streams); domain_index[n] in/out is each ion's domain
  * index (-1 before its first step).  fields: NULL, or per domain NULL or an
  * oracle_field carrying that domain's voltage waveforms.  mixtures: NULL, or
- * per domain NULL or that domain's gas mixture (HSS). */
+ * per domain NULL or that domain's gas mixture (HSS).  domain_species: NULL,
+ * or per domain NULL (the species table) or a table of n_species rows with
+ * that domain's cross sections, CCS and K0. */
 int oracle_run_multi(const icarion_domain_desc *domains, const double *end_aperture_m, uint32_t n_domains,
                      const oracle_field *const *fields, const struct oracle_gas_mixture *const *mixtures,
+                     const icarion_species_desc *const *domain_species,
                      const icarion_physics_desc *physics,
                      const icarion_species_desc *species, uint32_t n_species, const double *atoms,
                      icarion_ion_soa *ions, int32_t *domain_index, double t0, double dt, uint64_t n_steps,

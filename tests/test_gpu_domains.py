@@ -19,7 +19,7 @@ from scipy import stats
 from icarion_amd import _abi
 from icarion_amd import workloads as W
 from test_gpu_parity import compare_ions
-from test_oracle_domains import _gap_domains, _one_ion, domain_mixtures, domain_waveforms
+from test_oracle_domains import _gap_domains, _one_ion, domain_mixtures, domain_species_tables, domain_waveforms
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +41,9 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
             for k, mx in enumerate(w.get("mixtures") or []):
                 if mx:
                     h.set_domain_gas_mixture(k, *mx)
+            for k, rows in enumerate(w.get("domain_species") or []):
+                if rows:
+                    h.set_domain_species(k, rows)
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -56,7 +59,7 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
     di = np.zeros(ens.n, np.int32) if dom is None else dom
     st = oracle.run_multi(domains or w["domains"], physics or w["physics"], w["species"], ens, di,
                           t0, w["dt"], n_steps, step_offset=step_offset, atoms=w.get("atoms"),
-                          mixtures=w.get("mixtures"),
+                          mixtures=w.get("mixtures"), domain_species=w.get("domain_species"),
                           end_apertures=apertures if apertures is not None else w.get("apertures"),
                           waveforms=w.get("waveforms"))
     return st, di
@@ -194,6 +197,67 @@ def test_ehss_per_domain_gas_mixtures_ion_by_ion(oracle):
     assert ok.mean() >= 0.99, ok.mean()
     assert so.collisions > 2 * g.n
     assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+
+
+@pytest.mark.parametrize("integrator,collision", [
+    (_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS),
+    (_abi.INTEGRATOR_BORIS, _abi.COLLISION_FRICTION),
+])
+def test_per_domain_species_ion_by_ion(oracle, integrator, collision):
+    """Each domain with its own species table (cross section, CCS, K0 for its
+    gas): an ion collides and is damped with its step domain's row."""
+    w = W.multi_domain_workload(n=2048, seed=58, integrator=integrator, collision=collision)
+    if collision == _abi.COLLISION_FRICTION:
+        w["physics"].ou_thermal_kicks = 1
+    w["domain_species"] = domain_species_tables()
+    print(f"per-domain species integ={integrator} coll={collision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9) & (dg == do) & (g.death_time_s == o.death_time_s)
+    assert ok.mean() >= 0.99
+    if collision == _abi.COLLISION_HSS:
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    assert sg.alive == pytest.approx(so.alive, rel=0.02, abs=5)
+    # the per-domain rows matter: the same run on the handle's table differs
+    w2 = dict(w)
+    w2["domain_species"] = None
+    g2 = w["ensemble"].copy()
+    gpu_multi(w2, g2, 6000)
+    assert (compare_ions(g2, o, rtol=1e-9)).mean() < 0.5
+
+
+@pytest.mark.parametrize("integrator", [_abi.INTEGRATOR_RK4])
+def test_fp32_kahan_per_domain_species_statistics(oracle, integrator):
+    w = W.multi_domain_workload(n=8192, seed=59, integrator=integrator, collision=_abi.COLLISION_HSS)
+    w["domain_species"] = domain_species_tables()
+    print("per-domain species FP32", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    ph = W.physics(integrator, _abi.COLLISION_HSS, precision=_abi.PRECISION_FP32_KAHAN, seed=59)
+    sg, dg = gpu_multi(w, g, 4000, physics=ph)
+    so, do = oracle_multi(oracle, w, o, 4000)
+    assert sg.collisions == pytest.approx(so.collisions, rel=0.02)
+    for d in range(3):
+        assert (dg == d).sum() == pytest.approx((do == d).sum(), rel=0.05, abs=30)
+    assert sg.alive == pytest.approx(so.alive, rel=0.03, abs=30)
+
+
+def test_domain_species_arguments():
+    from icarion_amd import gpu
+    w = W.multi_domain_workload(n=16, collision=_abi.COLLISION_HSS)
+    with gpu.Handle() as h:
+        h.set_domains(w["domains"], w["physics"], w["apertures"])
+        h.set_species(w["species"])
+        with pytest.raises(gpu.IcarionError):
+            h.set_domain_species(3, w["species"])
+        with pytest.raises(gpu.IcarionError):
+            h.set_domain_species(0, w["species"] * 2)
+        with pytest.raises(gpu.IcarionError):
+            h.set_domain_species(0, [W.species(sigma_hss=1e-18, atom_offset=0, atom_count=4)])
+        h.set_domain_species(1, domain_species_tables()[1])
+        h.set_domain_species(1, [])
+        h.upload_state(w["ensemble"].copy())
+        h.step(0.0, w["dt"], 10)
 
 
 def test_bridge_and_handoff_cases_match_the_oracle(oracle):

@@ -21,7 +21,10 @@ restatement is pinned here by properties and by hand-derived cases:
     the first domain, the multi-domain run with waveforms on all three domains
     is the single-domain run with the first domain's waveforms, bit for bit,
     and differs from the run without them;
-  * each domain's gas mixture acts in that domain only, in the same way.
+  * each domain's gas mixture acts in that domain only, in the same way;
+  * each domain's species table (cross sections, CCS and K0 for its gas) acts
+    in that domain only, in the same way, for HSS and for Friction damping
+    (K0 enters the damping force).
 """
 import ctypes as C
 
@@ -207,3 +210,49 @@ def test_per_domain_mixtures_act_in_their_domain(oracle):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
     assert sa.collisions == sb.collisions > 0
     assert sa.collisions != sc.collisions
+
+
+def domain_species_tables():
+    """Per-domain rows of the three-domain workload's one species: its cross
+    section and K0 as if resolved for N2 at 10 Pa, He at 1 Pa and N2 again
+    (the values are the test's own; the handlers' lookup is tested with the
+    configuration helpers)."""
+    return [[W.species(sigma_hss=240e-20, ccs=240e-20, K0=1.4)],
+            [W.species(sigma_hss=90e-20, ccs=90e-20, K0=5.5)],
+            [W.species(sigma_hss=210e-20, ccs=210e-20, K0=1.7)]]
+
+
+@pytest.mark.parametrize("collision,integrator", [(_abi.COLLISION_HSS, _abi.INTEGRATOR_RK4),
+                                                  (_abi.COLLISION_FRICTION, _abi.INTEGRATOR_BORIS)])
+def test_per_domain_species_act_in_their_domain(oracle, collision, integrator):
+    w = W.multi_domain_workload(n=256, seed=64, collision=collision, integrator=integrator)
+    if collision == _abi.COLLISION_FRICTION:
+        w["physics"].ou_thermal_kicks = 1
+    tabs = domain_species_tables()
+    n_steps = 200   # no ion reaches the first boundary
+    a, b, c = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+    di = np.zeros(a.n, np.int32)
+    oracle.run_multi(w["domains"], w["physics"], w["species"], a, di, 0.0, w["dt"], n_steps,
+                     end_apertures=w["apertures"], domain_species=tabs)
+    oracle.run(w["domains"][0], w["physics"], tabs[0], b, 0.0, w["dt"], n_steps)
+    oracle.run(w["domains"][0], w["physics"], w["species"], c, 0.0, w["dt"], n_steps)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    assert not np.array_equal(a.vz, c.vz)
+    if collision == _abi.COLLISION_HSS:
+        # 240 against 150 A^2: more collisions with the domain's row
+        assert a.n_collisions.sum() > 1.3 * c.n_collisions.sum() > 0
+
+
+def test_per_domain_species_equal_to_the_table_change_nothing(oracle):
+    w = W.multi_domain_workload(n=256, seed=65, collision=_abi.COLLISION_HSS)
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    da, db = np.zeros(a.n, np.int32), np.zeros(b.n, np.int32)
+    sa = oracle.run_multi(w["domains"], w["physics"], w["species"], a, da, 0.0, w["dt"], 4000,
+                          end_apertures=w["apertures"], domain_species=[w["species"], None, w["species"]])
+    sb = oracle.run_multi(w["domains"], w["physics"], w["species"], b, db, 0.0, w["dt"], 4000,
+                          end_apertures=w["apertures"])
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "death_time_s", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    np.testing.assert_array_equal(da, db)
+    assert sa.collisions == sb.collisions > 0
