@@ -5,7 +5,7 @@ sizes against offsets computed by the C compiler.
 """
 import ctypes as C
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NO_DEVICE = range(6)
 
@@ -49,7 +49,7 @@ class DomainDesc(C.Structure):
 class PhysicsDesc(C.Structure):
     _fields_ = [
         ("integrator", _i32), ("collision_model", _i32), ("ou_thermal_kicks", _i32),
-        ("precision", _i32), ("ehss_mode", _i32), ("reserved0", _i32), ("rng_seed", _u64),
+        ("precision", _i32), ("ehss_mode", _i32), ("collision_substeps", _i32), ("rng_seed", _u64),
     ]
 
 

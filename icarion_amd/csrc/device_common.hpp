@@ -93,11 +93,13 @@ struct PhiloxStream {
     uint32_t block;
     uint32_t buf[4];
     int pos;
+    bool n_have;       // normal_f64: the second value of the last pair is pending
+    double n_cached;
 
     __device__ __forceinline__ PhiloxStream(uint64_t seed, uint32_t ion_index, uint64_t step)
         : k0(static_cast<uint32_t>(seed)), k1(static_cast<uint32_t>(seed >> 32)), ion(ion_index),
           s_lo(static_cast<uint32_t>(step)), s_hi(static_cast<uint32_t>(step >> 32)), block(0),
-          pos(4) {}
+          pos(4), n_have(false), n_cached(0.0) {}
 
     __device__ __forceinline__ void refill() {
         const u32x4 o = philox4x32_10(block, ion, s_lo, s_hi, k0, k1);
@@ -115,6 +117,24 @@ struct PhiloxStream {
         const double u = uniform53(second ? buf[2] : buf[0], second ? buf[3] : buf[1]);
         pos += 2;
         return u;
+    }
+    // Standard normal by Box-Muller on two FP64 draws, the pair's second value
+    // kept for the next call (the oracle's stream and std::normal_distribution
+    // hand out both values of a pair), so consecutive collision substeps share
+    // pairs as the engine's one stream per ion does.
+    __device__ __forceinline__ double normal_f64() {
+        if (n_have) {
+            n_have = false;
+            return n_cached;
+        }
+        const double u1 = uniform_f64();
+        const double u2 = uniform_f64();
+        const double r = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(2.0 * 3.14159265358979323846 * u2, &sn, &cs);
+        n_cached = r * sn;
+        n_have = true;
+        return r * cs;
     }
     __device__ __forceinline__ float uniform_f32() {
         if (pos >= 4) refill();

@@ -300,6 +300,7 @@ struct icarion_gpu {
         uint64_t n_steps = 0;
         double t_end = 0.0;
         bool has_birth = false;
+        bool substeps = false;       // more than one collision stage per step
     };
     uint32_t b_n = 0;
     size_t b_total = 0;
@@ -405,6 +406,8 @@ void validate_domain(const icarion_domain_desc& d, const icarion_physics_desc& p
     if (p.collision_model < 0 || p.collision_model > 4) fail(ICARION_ERR_INVALID, "unknown collision model");
     if (p.precision != ICARION_PRECISION_FP64 && p.precision != ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_INVALID, "unknown precision");
+    if (p.collision_substeps < 0 || p.collision_substeps > ICARION_MAX_COLLISION_SUBSTEPS)
+        fail(ICARION_ERR_INVALID, "collision_substeps must be 0 .. ICARION_MAX_COLLISION_SUBSTEPS");
     if (!(d.length_m > 0.0)) fail(ICARION_ERR_INVALID, "geometry.length_m must be > 0");
     if (d.instrument == ICARION_INSTRUMENT_ORBITRAP) {
         if (!(d.radius_in_m > 0.0 && d.radius_out_m > d.radius_in_m && d.radius_char_m > d.radius_out_m))
@@ -442,6 +445,7 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
     P.magnetic = d.magnetic_enabled;
     P.damping_model = p.collision_model;
     P.ou_kicks = p.ou_thermal_kicks;
+    P.n_sub = std::max(1, static_cast<int>(p.collision_substeps));
     P.detect = std::isfinite(d.detector_z_m) ? 1 : 0;
     P.has_rf = (d.instrument != ICARION_INSTRUMENT_ORBITRAP && std::fabs(d.rf_voltage_V) > 1e-12) ? 1 : 0;
     P.rf_U = d.dc_quad_V;
@@ -1318,6 +1322,11 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     if (coll == COLL_EHSS && !surrogate && h->phys.precision == ICARION_PRECISION_FP32_KAHAN)
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS) check_ehss_mixture(h, surrogate);
+    // the wave-cooperative geometry EHSS kernel and the surrogate kernel run
+    // one collision stage per step
+    if (h->base.n_sub > 1 && (surrogate || launch_coll_kind(h) == COLL_EHSS))
+        fail(ICARION_ERR_UNSUPPORTED, "collision substeps with EHSS run with offline sets, orientation "
+                                      "samples, gas mixtures, several domains, RK45 or the batch");
     set_device(h);
     StepParams P = h->base;
     if (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
@@ -1563,6 +1572,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         M.dt = dt;
         M.seed = h->phys.rng_seed;
         M.n_ions = static_cast<uint32_t>(h->n);
+        M.n_sub = Ps[0].n_sub;
         const IonPtrs S = ion_ptrs(h);
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
         const int integ = h->phys.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0;
@@ -1745,6 +1755,8 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         if (cd.n_atoms && !cd.atoms_xyzr) fail(ICARION_ERR_INVALID, tag + "atoms_xyzr is NULL");
         if (surrogate && cd.physics.precision != ICARION_PRECISION_FP32_KAHAN)
             fail(ICARION_ERR_UNSUPPORTED, tag + "the EHSS surrogate runs in the FP32 + Kahan path");
+        if (surrogate && cd.physics.collision_substeps > 1)
+            fail(ICARION_ERR_UNSUPPORTED, tag + "the EHSS surrogate runs one collision stage per step");
         const bool mixture = cd.n_components > 0;
         if (field == FIELD_ARRAY) {
             if (!cd.field_E) fail(ICARION_ERR_INVALID, tag + "the domain uses a field array but field_E is NULL");
@@ -1778,6 +1790,7 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         bc[c].off = total;
         bc[c].precision = cd.physics.precision;
         bc[c].cls = surrogate ? 2 : (cd.physics.precision == ICARION_PRECISION_FP32_KAHAN ? 1 : 0);
+        bc[c].substeps = cd.physics.collision_substeps > 1;
         bc[c].kind = surrogate ? static_cast<uint32_t>(cd.physics.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0) +
                                      2u * (field == FIELD_ORBITRAP ? 1u : 0u)
                                : batch_kind(cd.physics, field, mixture ? COLL_HSS_MIX : coll);
@@ -2017,7 +2030,7 @@ float batch_launch(icarion_gpu* h, icarion_batch_tuning t64, icarion_batch_tunin
         B[cls].n_tiles = (mask >> cls) & 1 ? h->b_n_tiles[cls] : 0;
         B[cls].with_mix = 0;
         for (const auto& c : h->b_clouds)
-            if (c.cls == cls && c.kind >= kBatchFirstMixKind) B[cls].with_mix = 1;
+            if (c.cls == cls && (c.kind >= kBatchFirstMixKind || c.substeps)) B[cls].with_mix = 1;
         B[cls].queue = h->b_queue.p + cls;
         B[cls].done = h->b_done.p;
         B[cls].start = h->b_start.p;

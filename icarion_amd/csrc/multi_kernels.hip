@@ -98,7 +98,7 @@ __device__ __forceinline__ d3 domain_force(const StepParams& D, d3 p, d3 v, doub
     return force_f64<FIELD_LINEAR, COLL>(D, p, v, c, ez, q, m, lpre);
 }
 
-template <int INTEG, int COLL>
+template <int INTEG, int COLL, bool SUB>
 __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S, const uint32_t i, uint32_t& pushed,
                                           uint32_t& ncoll_launch) {
     d3 pos{S.x[i], S.y[i], S.z[i]};
@@ -187,18 +187,23 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
         ++pushed;
         if constexpr (COLL != COLL_NONE) {
             PhiloxStream rng(M.seed, i, M.step0 + k);
-            bool hit;
-            if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
-                hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dt)
-                                  : collide_f64<COLL_HSS>(D, sp, sidx, m, lpre, vel, rng, dt);
-            else if constexpr (COLL == COLL_EHSS_MIX)
-                hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dt)
-                                   : collide_f64<COLL_EHSS>(D, sp, sidx, m, lpre, vel, rng, dt);
-            else
-                hit = collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dt);
-            if (hit) {
-                ++nc;
-                ++ncoll_launch;
+            // collision substeps in the SUB instantiation (one stage otherwise)
+            const int ns = SUB ? D.n_sub : 1;
+            const double dtc = SUB ? dt / static_cast<double>(D.n_sub) : dt;
+            for (int s = 0; s < ns; ++s) {
+                bool hit;
+                if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
+                    hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dtc)
+                                      : collide_f64<COLL_HSS>(D, sp, sidx, m, lpre, vel, rng, dtc);
+                else if constexpr (COLL == COLL_EHSS_MIX)
+                    hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dtc)
+                                       : collide_f64<COLL_EHSS>(D, sp, sidx, m, lpre, vel, rng, dtc);
+                else
+                    hit = collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dtc);
+                if (hit) {
+                    ++nc;
+                    ++ncoll_launch;
+                }
             }
         }
         if (D.detect && isnan(arrival)) {
@@ -247,7 +252,7 @@ __device__ __forceinline__ f3 domain_accel_f32(const StepParams& D, f3 p, f3 v, 
 // position and velocity as FP32 (hi, lo) pairs, forces and collisions in FP32
 // with each domain's FP32 constants; the domain lookup, bridge and handoff
 // in FP64 on hi + lo, as the FP64 kernel takes them.
-template <int INTEG, int COLL>
+template <int INTEG, int COLL, bool SUB>
 __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtrs& S, const uint32_t i,
                                               uint32_t& pushed, uint32_t& ncoll_launch) {
     const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
@@ -348,21 +353,29 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
             PhiloxStream rng(M.seed, i, M.step0 + k);
             f3 v{hi[3], hi[4], hi[5]};
             const f3 vin = v;
-            bool hit;
-            if constexpr (COLL == COLL_HSS_MIX)
-                hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m,
-                                                              v, rng, dt, D.fc.n_dt)
-                                  : collide_f32<COLL_HSS>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v,
-                                                          rng, dt, D.fc.n_dt);
-            else
-                hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng, dt,
-                                        D.fc.n_dt);
+            // collision substeps in the SUB instantiation (one stage otherwise)
+            const int ns = SUB ? D.n_sub : 1;
+            const float sub = SUB ? 1.0f / static_cast<float>(D.n_sub) : 1.0f;
+            const float dtc = SUB ? static_cast<float>(M.dt / static_cast<double>(D.n_sub)) : dt;
+            uint32_t hits = 0;
+            for (int s = 0; s < ns; ++s) {
+                bool hit;
+                if constexpr (COLL == COLL_HSS_MIX)
+                    hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX>(D, D.fc, sp, sidx, m, inv_m_plus, lpre,
+                                                                  kbt2_dt_over_m * sub, v, rng, dtc, D.fc.n_dt * sub)
+                                      : collide_f32<COLL_HSS>(D, D.fc, sp, sidx, m, inv_m_plus, lpre,
+                                                              kbt2_dt_over_m * sub, v, rng, dtc, D.fc.n_dt * sub);
+                else
+                    hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m * sub, v, rng,
+                                            dtc, D.fc.n_dt * sub);
+                hits += hit ? 1u : 0u;
+            }
             if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
-                if (hit) {
+                if (hits) {
                     hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
                     lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
-                    ++nc;
-                    ++ncoll_launch;
+                    nc += hits;
+                    ncoll_launch += hits;
                 }
             } else {
                 kahan_add(hi[3], lo[3], v.x - vin.x);
@@ -411,50 +424,57 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
     S.domain[i] = cur;
 }
 
-template <int INTEG, int COLL>
+template <int INTEG, int COLL, bool SUB>
 __global__ void __launch_bounds__(256) step_multi_f32_kernel(const MultiParams M, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < M.n_ions) multi_ion_f32<INTEG, COLL>(M, S, i, pushed, ncoll_launch);
+    if (i < M.n_ions) multi_ion_f32<INTEG, COLL, SUB>(M, S, i, pushed, ncoll_launch);
     add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
 }
 
-template <int INTEG, int COLL>
+template <int INTEG, int COLL, bool SUB>
 __global__ void __launch_bounds__(256) step_multi_f64_kernel(const MultiParams M, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < M.n_ions) multi_ion<INTEG, COLL>(M, S, i, pushed, ncoll_launch);
+    if (i < M.n_ions) multi_ion<INTEG, COLL, SUB>(M, S, i, pushed, ncoll_launch);
     add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
 }
 
+// One multi-domain kernel; SUB: more than one collision stage per step
+template <int INTEG, bool F32, int COLL>
+static void launch_multi_kernel(bool sub, dim3 grid, uint32_t block, hipStream_t s, const MultiParams& M,
+                                const IonPtrs& S) {
+    if constexpr (F32) {
+        if (sub && COLL != COLL_NONE)
+            hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL, true>), grid, dim3(block), 0, s, M, S);
+        else
+            hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL, false>), grid, dim3(block), 0, s, M, S);
+    } else {
+        if (sub && COLL != COLL_NONE)
+            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL, true>), grid, dim3(block), 0, s, M, S);
+        else
+            hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL, false>), grid, dim3(block), 0, s, M, S);
+    }
+}
+
 template <int INTEG, bool F32>
-static hipError_t launch_multi_integ(int coll, const MultiParams& M, const IonPtrs& S, uint32_t block,
+static hipError_t launch_multi_integ(int coll, bool sub, const MultiParams& M, const IonPtrs& S, uint32_t block,
                                      hipStream_t s) {
     const dim3 grid((M.n_ions + block - 1) / block);
     switch (coll) {
-        case COLL_NONE:
-            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_NONE>), grid, dim3(block), 0, s, M, S);
-            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_NONE>), grid, dim3(block), 0, s, M, S);
-            break;
-        case COLL_HSS:
-            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_HSS>), grid, dim3(block), 0, s, M, S);
-            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS>), grid, dim3(block), 0, s, M, S);
-            break;
-        case COLL_DAMPING:
-            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
-            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_DAMPING>), grid, dim3(block), 0, s, M, S);
-            break;
+        case COLL_NONE: launch_multi_kernel<INTEG, F32, COLL_NONE>(false, grid, block, s, M, S); break;
+        case COLL_HSS: launch_multi_kernel<INTEG, F32, COLL_HSS>(sub, grid, block, s, M, S); break;
+        case COLL_DAMPING: launch_multi_kernel<INTEG, F32, COLL_DAMPING>(sub, grid, block, s, M, S); break;
         case COLL_HSS_MIX:   // HSS with per-domain gas mixtures
-            if (F32) hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL_HSS_MIX>), grid, dim3(block), 0, s, M, S);
-            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_HSS_MIX>), grid, dim3(block), 0, s, M, S);
+            launch_multi_kernel<INTEG, F32, COLL_HSS_MIX>(sub, grid, block, s, M, S);
             break;
         case COLL_EHSS_MIX:   // geometry EHSS with per-domain gas mixtures, FP64 only
             if constexpr (F32) return hipErrorInvalidValue;
-            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_EHSS_MIX>), grid, dim3(block), 0, s, M, S);
+            else launch_multi_kernel<INTEG, false, COLL_EHSS_MIX>(sub, grid, block, s, M, S);
             break;
         case COLL_EHSS:   // geometry EHSS with each domain's gas, FP64 only
             if constexpr (F32) return hipErrorInvalidValue;
-            else hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL_EHSS>), grid, dim3(block), 0, s, M, S);
+            else launch_multi_kernel<INTEG, false, COLL_EHSS>(sub, grid, block, s, M, S);
             break;
         default:
             return hipErrorInvalidValue;
@@ -468,10 +488,11 @@ hipError_t launch_multi_step(int integrator, int coll, int precision, const Mult
                              uint32_t block, hipStream_t s) {
     if (M.n_ions == 0) return hipSuccess;
     const bool f32 = precision == 1;
-    if (integrator == 0) return f32 ? launch_multi_integ<0, true>(coll, M, S, block, s)
-                                    : launch_multi_integ<0, false>(coll, M, S, block, s);
-    if (integrator == 1) return f32 ? launch_multi_integ<1, true>(coll, M, S, block, s)
-                                    : launch_multi_integ<1, false>(coll, M, S, block, s);
+    const bool sub = M.n_sub > 1;
+    if (integrator == 0) return f32 ? launch_multi_integ<0, true>(coll, sub, M, S, block, s)
+                                    : launch_multi_integ<0, false>(coll, sub, M, S, block, s);
+    if (integrator == 1) return f32 ? launch_multi_integ<1, true>(coll, sub, M, S, block, s)
+                                    : launch_multi_integ<1, false>(coll, sub, M, S, block, s);
     return hipErrorInvalidValue;
 }
 

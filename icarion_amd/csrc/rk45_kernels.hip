@@ -227,9 +227,13 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                     const double z_before = p0.z;
                     if constexpr (COLL != COLL_NONE) {
                         PhiloxStream rng(P.seed, i, kstep);
-                        if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, used)) {
-                            ++nc;
-                            ++ncoll_launch;
+                        // collision substeps of the ion's own dt (n_sub = 1: one stage)
+                        const double dtc = P.n_sub == 1 ? used : used / static_cast<double>(P.n_sub);
+                        for (int s = 0; s < P.n_sub; ++s) {
+                            if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, dtc)) {
+                                ++nc;
+                                ++ncoll_launch;
+                            }
                         }
                     }
                     if (P.detect && isnan(arrival)) {
@@ -441,15 +445,19 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
                         PhiloxStream rng(P.seed, i, kstep);
                         f3 v{hi[3], hi[4], hi[5]};
                         const f3 vin = v;
-                        const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * used);
-                        const bool hit = collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m,
-                                                           v, rng, (float)used, (float)(P.n * used));
+                        // collision substeps of the ion's own dt (n_sub = 1: one stage)
+                        const double dtc = P.n_sub == 1 ? used : used / static_cast<double>(P.n_sub);
+                        const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * dtc);
+                        uint32_t hits = 0;
+                        for (int s = 0; s < P.n_sub; ++s)
+                            hits += collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng,
+                                                      (float)dtc, (float)(P.n * dtc)) ? 1u : 0u;
                         if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
-                            if (hit) {
+                            if (hits) {
                                 hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
                                 lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
-                                ++nc;
-                                ++ncoll_launch;
+                                nc += hits;
+                                ncoll_launch += hits;
                             }
                         } else {
                             kahan_add(hi[3], lo[3], v.x - vin.x);

@@ -560,15 +560,11 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
             if (g <= 0.0 || dt <= 0.0) return false;
             const double kBT_over_m = kBoltzmann * P.T / m;
             const double s = sqrt(2.0 * g * kBT_over_m * dt);
-            double u1 = rng.uniform_f64(), u2 = rng.uniform_f64();
-            double r = sqrt(-2.0 * log(u1));
-            double sn, cs;
-            sincos(2.0 * kPi * u2, &sn, &cs);
-            v.x += s * (r * cs);
-            v.y += s * (r * sn);
-            u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
-            r = sqrt(-2.0 * log(u1));
-            v.z += s * (r * cos(2.0 * kPi * u2));
+            // three normals of the stream (a pair's second value carries over
+            // to the next collision substep)
+            v.x += s * rng.normal_f64();
+            v.y += s * rng.normal_f64();
+            v.z += s * rng.normal_f64();
         }
         return false;
     } else {

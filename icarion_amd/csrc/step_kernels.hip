@@ -19,7 +19,7 @@
 
 namespace icarion {
 
-template <int INTEG, int FIELD, int COLL>
+template <int INTEG, int FIELD, int COLL, bool SUB = false>
 __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, const uint32_t i,
         uint32_t& pushed, uint32_t& ncoll_launch) {
     d3 pos{S.x[i], S.y[i], S.z[i]};
@@ -107,9 +107,15 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
         ++pushed;
         if constexpr (COLL != COLL_NONE) {
             PhiloxStream rng(P.seed, i, P.step0 + k);
-            if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, dt)) {
-                ++nc;
-                ++ncoll_launch;
+            // collision substeps, each an event of its own, in the SUB
+            // instantiation (the loop costs the one-stage kernels registers)
+            const int ns = SUB ? P.n_sub : 1;
+            const double dtc = SUB ? dt / static_cast<double>(P.n_sub) : dt;
+            for (int s = 0; s < ns; ++s) {
+                if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, dtc)) {
+                    ++nc;
+                    ++ncoll_launch;
+                }
             }
         }
         if (P.detect && isnan(arrival)) {
@@ -137,11 +143,11 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
     S.ncoll[i] = nc;
 }
 
-template <int INTEG, int FIELD, int COLL>
+template <int INTEG, int FIELD, int COLL, bool SUB = false>
 __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
+    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL, SUB>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -532,7 +538,7 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
 
 // the FP32 helpers (fields, containment, collisions) are in step_device_f32.hpp
 
-template <int INTEG, int FIELD, int COLL>
+template <int INTEG, int FIELD, int COLL, bool SUB = false>
 __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, const uint32_t i,
         uint32_t& pushed, uint32_t& ncoll_launch) {
     const FastConst& C = P.fc;
@@ -639,15 +645,23 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
             PhiloxStream rng(P.seed, i, P.step0 + k);
             f3 v{hi[3], hi[4], hi[5]};
             const f3 vin = v;
-            const bool hit = collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng, dt,
-                                               C.n_dt);
+            // collision substeps: dt / n_sub each, one continuing stream, in the
+            // SUB instantiation (the loop costs the one-stage kernels a
+            // wavefront per SIMD)
+            const int ns = SUB ? P.n_sub : 1;
+            const float sub = SUB ? 1.0f / static_cast<float>(P.n_sub) : 1.0f;
+            const float dtc = SUB ? static_cast<float>(P.dt / static_cast<double>(P.n_sub)) : dt;
+            uint32_t hits = 0;
+            for (int s = 0; s < ns; ++s)
+                hits += collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m * sub, v, rng, dtc,
+                                          C.n_dt * sub) ? 1u : 0u;
             if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
-                if (hit) {
+                if (hits) {
                     // a collision replaces the velocity: restart its compensation
                     hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
                     lo[3] = 0.0f; lo[4] = 0.0f; lo[5] = 0.0f;
-                    ++nc;
-                    ++ncoll_launch;
+                    nc += hits;
+                    ncoll_launch += hits;
                 }
             } else {
                 kahan_add(hi[3], lo[3], v.x - vin.x);
@@ -686,11 +700,11 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
     S.ncoll[i] = nc;
 }
 
-template <int INTEG, int FIELD, int COLL>
+template <int INTEG, int FIELD, int COLL, bool SUB>
 __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
+    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL, SUB>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -905,56 +919,58 @@ __global__ void __launch_bounds__(kSurrogateBlock, 3) step_surrogate_kernel(cons
 // queue index passes the tile count.  Each tile runs the same per-ion body as
 // the single-cloud kernels, so a cloud's result does not depend on the batch.
 
-template <int PREC, int I, int F, int Cc>
+template <int PREC, bool SUB, int I, int F, int Cc>
 __device__ __forceinline__ void batch_ion(const StepParams& P, const IonPtrs& S, uint32_t i,
                                           uint32_t& pushed, uint32_t& nc) {
     if constexpr (PREC == 1) {
-        f32_ion<I, F, Cc>(P, S, i, pushed, nc);
+        f32_ion<I, F, Cc, SUB>(P, S, i, pushed, nc);
     } else {
-        f64_ion<I, F, Cc>(P, S, i, pushed, nc);
+        f64_ion<I, F, Cc, SUB>(P, S, i, pushed, nc);
     }
 }
 
 // MIX: the kernel also holds the gas-mixture kinds (12-15), in FP64 the
-// geometry EHSS kinds (16-19), and the uniform-grid field-array kinds (20-25).
+// geometry EHSS kinds (16-19), and the uniform-grid field-array kinds (20-25),
+// and its FP32 bodies run collision substeps (clouds with more than one
+// collision stage per step run in this variant).
 // Their bodies need more registers, so a batch without such clouds runs the
 // variant without them (FP32: 90 VGPRs, 5 waves per SIMD).
 template <int PREC, bool MIX>
 __device__ __forceinline__ void batch_dispatch(uint32_t kind, const StepParams& P, const IonPtrs& S,
                                                uint32_t i, uint32_t& pushed, uint32_t& nc) {
     switch (kind) {
-        case 0: batch_ion<PREC, 0, FIELD_LINEAR, COLL_NONE>(P, S, i, pushed, nc); break;
-        case 1: batch_ion<PREC, 1, FIELD_LINEAR, COLL_NONE>(P, S, i, pushed, nc); break;
-        case 2: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_NONE>(P, S, i, pushed, nc); break;
-        case 3: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_NONE>(P, S, i, pushed, nc); break;
-        case 4: batch_ion<PREC, 0, FIELD_LINEAR, COLL_HSS>(P, S, i, pushed, nc); break;
-        case 5: batch_ion<PREC, 1, FIELD_LINEAR, COLL_HSS>(P, S, i, pushed, nc); break;
-        case 6: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_HSS>(P, S, i, pushed, nc); break;
-        case 7: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_HSS>(P, S, i, pushed, nc); break;
-        case 8: batch_ion<PREC, 0, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
-        case 9: batch_ion<PREC, 1, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
-        case 10: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
-        case 11: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 0: batch_ion<PREC, MIX, 0, FIELD_LINEAR, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 1: batch_ion<PREC, MIX, 1, FIELD_LINEAR, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 2: batch_ion<PREC, MIX, 0, FIELD_ORBITRAP, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 3: batch_ion<PREC, MIX, 1, FIELD_ORBITRAP, COLL_NONE>(P, S, i, pushed, nc); break;
+        case 4: batch_ion<PREC, MIX, 0, FIELD_LINEAR, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 5: batch_ion<PREC, MIX, 1, FIELD_LINEAR, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 6: batch_ion<PREC, MIX, 0, FIELD_ORBITRAP, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 7: batch_ion<PREC, MIX, 1, FIELD_ORBITRAP, COLL_HSS>(P, S, i, pushed, nc); break;
+        case 8: batch_ion<PREC, MIX, 0, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 9: batch_ion<PREC, MIX, 1, FIELD_LINEAR, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 10: batch_ion<PREC, MIX, 0, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
+        case 11: batch_ion<PREC, MIX, 1, FIELD_ORBITRAP, COLL_DAMPING>(P, S, i, pushed, nc); break;
         default:
             if constexpr (MIX) {
                 switch (kind) {
-                    case 12: batch_ion<PREC, 0, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
-                    case 13: batch_ion<PREC, 1, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
-                    case 14: batch_ion<PREC, 0, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
-                    case 15: batch_ion<PREC, 1, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
-                    case 20: batch_ion<PREC, 0, FIELD_ARRAY, COLL_NONE>(P, S, i, pushed, nc); break;
-                    case 21: batch_ion<PREC, 1, FIELD_ARRAY, COLL_NONE>(P, S, i, pushed, nc); break;
-                    case 22: batch_ion<PREC, 0, FIELD_ARRAY, COLL_HSS>(P, S, i, pushed, nc); break;
-                    case 23: batch_ion<PREC, 1, FIELD_ARRAY, COLL_HSS>(P, S, i, pushed, nc); break;
-                    case 24: batch_ion<PREC, 0, FIELD_ARRAY, COLL_DAMPING>(P, S, i, pushed, nc); break;
-                    case 25: batch_ion<PREC, 1, FIELD_ARRAY, COLL_DAMPING>(P, S, i, pushed, nc); break;
+                    case 12: batch_ion<PREC, MIX, 0, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 13: batch_ion<PREC, MIX, 1, FIELD_LINEAR, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 14: batch_ion<PREC, MIX, 0, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 15: batch_ion<PREC, MIX, 1, FIELD_ORBITRAP, COLL_HSS_MIX>(P, S, i, pushed, nc); break;
+                    case 20: batch_ion<PREC, MIX, 0, FIELD_ARRAY, COLL_NONE>(P, S, i, pushed, nc); break;
+                    case 21: batch_ion<PREC, MIX, 1, FIELD_ARRAY, COLL_NONE>(P, S, i, pushed, nc); break;
+                    case 22: batch_ion<PREC, MIX, 0, FIELD_ARRAY, COLL_HSS>(P, S, i, pushed, nc); break;
+                    case 23: batch_ion<PREC, MIX, 1, FIELD_ARRAY, COLL_HSS>(P, S, i, pushed, nc); break;
+                    case 24: batch_ion<PREC, MIX, 0, FIELD_ARRAY, COLL_DAMPING>(P, S, i, pushed, nc); break;
+                    case 25: batch_ion<PREC, MIX, 1, FIELD_ARRAY, COLL_DAMPING>(P, S, i, pushed, nc); break;
                     default:
                         if constexpr (PREC == 0) {
                             switch (kind) {
-                                case 16: batch_ion<0, 0, FIELD_LINEAR, COLL_EHSS>(P, S, i, pushed, nc); break;
-                                case 17: batch_ion<0, 1, FIELD_LINEAR, COLL_EHSS>(P, S, i, pushed, nc); break;
-                                case 18: batch_ion<0, 0, FIELD_ORBITRAP, COLL_EHSS>(P, S, i, pushed, nc); break;
-                                case 19: batch_ion<0, 1, FIELD_ORBITRAP, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 16: batch_ion<0, MIX, 0, FIELD_LINEAR, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 17: batch_ion<0, MIX, 1, FIELD_LINEAR, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 18: batch_ion<0, MIX, 0, FIELD_ORBITRAP, COLL_EHSS>(P, S, i, pushed, nc); break;
+                                case 19: batch_ion<0, MIX, 1, FIELD_ORBITRAP, COLL_EHSS>(P, S, i, pushed, nc); break;
                                 default: break;
                             }
                         }
@@ -1241,31 +1257,41 @@ hipError_t launch_pack_snapshot(const IonPtrs& S, uint32_t n, int full, double* 
 
 using KernelFn = void (*)(const StepParams, const IonPtrs);
 
+// sub: more than one collision stage per step (the FP32 kernels have an
+// instantiation of their own for it; the FP64 bodies loop at run time)
 template <int I, int F, int Cc>
-KernelFn pick_prec(int precision) {
-    return precision == 1 ? step_f32_kernel<I, F, Cc> : step_f64_kernel<I, F, Cc>;
+KernelFn pick_prec(int precision, bool sub) {
+    if constexpr (Cc != COLL_NONE) {
+        if (sub) return precision == 1 ? step_f32_kernel<I, F, Cc, true> : step_f64_kernel<I, F, Cc, true>;
+    }
+    return precision == 1 ? step_f32_kernel<I, F, Cc, false> : step_f64_kernel<I, F, Cc, false>;
+}
+
+template <int I, int F, int Cc>
+KernelFn pick_f64(bool sub) {
+    return sub ? step_f64_kernel<I, F, Cc, true> : step_f64_kernel<I, F, Cc, false>;
 }
 
 template <int I, int F>
-KernelFn pick_coll(int coll, int precision) {
+KernelFn pick_coll(int coll, int precision, bool sub) {
     switch (coll) {
-        case COLL_HSS: return pick_prec<I, F, COLL_HSS>(precision);
-        case COLL_HSS_MIX: return pick_prec<I, F, COLL_HSS_MIX>(precision);
+        case COLL_HSS: return pick_prec<I, F, COLL_HSS>(precision, sub);
+        case COLL_HSS_MIX: return pick_prec<I, F, COLL_HSS_MIX>(precision, sub);
         case COLL_EHSS: return precision == 1 ? nullptr : step_f64_ehss_kernel<I, F>;
-        case COLL_EHSS_TAB: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS_TAB>;
-        case COLL_EHSS_MIX: return precision == 1 ? nullptr : step_f64_kernel<I, F, COLL_EHSS_MIX>;
-        case COLL_DAMPING: return pick_prec<I, F, COLL_DAMPING>(precision);
-        default: return pick_prec<I, F, COLL_NONE>(precision);
+        case COLL_EHSS_TAB: return precision == 1 ? nullptr : pick_f64<I, F, COLL_EHSS_TAB>(sub);
+        case COLL_EHSS_MIX: return precision == 1 ? nullptr : pick_f64<I, F, COLL_EHSS_MIX>(sub);
+        case COLL_DAMPING: return pick_prec<I, F, COLL_DAMPING>(precision, sub);
+        default: return pick_prec<I, F, COLL_NONE>(precision, sub);
     }
 }
 
 template <int I>
-KernelFn pick_field(int field, int coll, int precision) {
+KernelFn pick_field(int field, int coll, int precision, bool sub) {
     switch (field) {
-        case FIELD_ORBITRAP: return pick_coll<I, FIELD_ORBITRAP>(coll, precision);
-        case FIELD_ARRAY: return pick_coll<I, FIELD_ARRAY>(coll, precision);
-        case FIELD_ARRAY_X: return pick_coll<I, FIELD_ARRAY_X>(coll, precision);
-        default: return pick_coll<I, FIELD_LINEAR>(coll, precision);
+        case FIELD_ORBITRAP: return pick_coll<I, FIELD_ORBITRAP>(coll, precision, sub);
+        case FIELD_ARRAY: return pick_coll<I, FIELD_ARRAY>(coll, precision, sub);
+        case FIELD_ARRAY_X: return pick_coll<I, FIELD_ARRAY_X>(coll, precision, sub);
+        default: return pick_coll<I, FIELD_LINEAR>(coll, precision, sub);
     }
 }
 
@@ -1287,8 +1313,8 @@ hipError_t launch_step(int integrator, int field, int coll, int precision, int s
     if (surrogate) {
         fn = integrator == 1 ? pick_surrogate<1>(field) : pick_surrogate<0>(field);
     } else {
-        fn = integrator == 1 ? pick_field<1>(field, coll, precision)
-                             : pick_field<0>(field, coll, precision);
+        fn = integrator == 1 ? pick_field<1>(field, coll, precision, P.n_sub > 1)
+                             : pick_field<0>(field, coll, precision, P.n_sub > 1);
     }
     if (!fn) return hipErrorInvalidValue;
     // the surrogate kernel runs its own workgroup size (the handle's block size

@@ -168,7 +168,12 @@ struct StepParams {
     // HSS gas mixture (n_mix = 0: the single gas above): the participating
     // components with n > 0, in order; sigma of species s with component c at
     // mix_sigma[s * n_mix + c]
-    int n_mix, mix_pad;
+    int n_mix;
+    // collision substeps per engine step (>= 1): the collision stage runs
+    // n_sub times with dt / n_sub on the pushed state, drawing from one
+    // continuing stream (handle_collisions_cpu's subcycle loop,
+    // SimulationEngineCollisionRuntime.cpp:84-191)
+    int n_sub;
     double mix_n[kMaxGasComponents], mix_m[kMaxGasComponents];
     double mix_sig[kMaxGasComponents];   // sqrt(kB T / m_c)
     const double *mix_sigma;
@@ -268,7 +273,8 @@ struct MultiParams {
     double handoff_tol;             // shared_boundary_handoff_domain's tolerance
     double dt;
     uint64_t step0, seed;
-    uint32_t k0, n_steps, n_ions, pad;
+    uint32_t k0, n_steps, n_ions;
+    int n_sub;                      // collision substeps per step (every domain's n_sub)
 };
 
 // ---- batched clouds ----------------------------------------------------------

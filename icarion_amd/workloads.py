@@ -92,13 +92,15 @@ def orbitrap_k(d):
 
 
 def physics(integrator=_abi.INTEGRATOR_RK4, collision=_abi.COLLISION_NONE, ou=0,
-            precision=_abi.PRECISION_FP64, seed=123, ehss_mode=_abi.EHSS_GEOMETRY):
+            precision=_abi.PRECISION_FP64, seed=123, ehss_mode=_abi.EHSS_GEOMETRY, substeps=0):
+    """substeps: collision stages per engine step (0 or 1: one)."""
     p = _abi.PhysicsDesc()
     p.integrator = integrator
     p.collision_model = collision
     p.ou_thermal_kicks = ou
     p.precision = precision
     p.ehss_mode = ehss_mode
+    p.collision_substeps = substeps
     p.rng_seed = seed
     return p
 

@@ -322,6 +322,14 @@ inline icarion_domain_desc to_desc(const Domain& d, double detector_z_m = NAN) {
 }
 
 // precision: enum icarion_precision; ehss_mode: enum icarion_ehss_mode.
+// Collision stages per engine step of the engine's CPU path: subcycles, or in
+// multi-event mode max(subcycles, max events) (SimulationEngineCollisionRuntime.cpp:26-33).
+inline int32_t collision_substeps(const Physics& p) {
+    const int configured = std::max(1, p.collision_subcycles_per_step);
+    const int min_multi = std::max(1, p.collision_max_events_per_step);
+    return p.collision_multi_event_mode ? std::max(configured, min_multi) : configured;
+}
+
 inline icarion_physics_desc to_physics(const Simulation& s, const Physics& p,
                                        int32_t precision = ICARION_PRECISION_FP64,
                                        int32_t ehss_mode = ICARION_EHSS_GEOMETRY) {
@@ -331,6 +339,7 @@ inline icarion_physics_desc to_physics(const Simulation& s, const Physics& p,
     o.ou_thermal_kicks = p.ou_thermal_kicks ? 1 : 0;
     o.precision = precision;
     o.ehss_mode = ehss_mode;
+    o.collision_substeps = collision_substeps(p);
     o.rng_seed = s.rng_seed;
     return o;
 }
@@ -570,13 +579,19 @@ inline SpeciesTable species_table(const std::vector<SpeciesRecord>& records, con
 
 // ---- the fused step's eligibility --------------------------------------------
 // The device step runs push, collisions and boundary elimination of every ion
-// in one call.  The engine takes its batch collision path only when no
-// multi-event mode, one subcycle, no deep collision diagnostics, a constant
-// flow model in every domain and one dt for all active ions hold; otherwise it
-// collides ions on the CPU (handle_collisions_cpu), which after a fused step
-// would collide them twice.  A reference-side strategy therefore answers
-// supports_batch() / step_batch() with this predicate and leaves the whole step
-// to the CPU when it is false.
+// in one call, the collision stage as often per step as the engine's CPU path
+// would (collision_substeps: subcycles, multi-event mode).  The engine takes
+// its batch collision path only when no multi-event mode, one subcycle, no
+// deep collision diagnostics, a constant flow model in every domain and one
+// dt for all active ions hold; otherwise it collides ions on the CPU
+// (handle_collisions_cpu), which after a fused step would collide them twice.
+// So a reference-side collision handler skips its per-ion handle_collision for
+// the ions of a fused step (they have had their substeps), and the fused step
+// itself is eligible when the device covers what the CPU path would do: no
+// deep collision diagnostics (the per-event records are host-side), constant
+// flow in every domain, one dt for all active ions.  A reference-side strategy
+// answers supports_batch() / step_batch() with this predicate and leaves the
+// whole step to the CPU when it is false.
 struct EngineConditions {
     bool collision_multi_event_mode = false;
     int collision_subcycles_per_step = 1;
@@ -586,12 +601,17 @@ struct EngineConditions {
     bool uniform_dt = true;                 // every active ion has the same dt
 };
 
-inline bool fused_step_eligible(const EngineConditions& c) {
+// the engine's own batch-collision preconditions (SimulationEngineCollisionRuntime.cpp:46)
+inline bool engine_batch_collisions(const EngineConditions& c) {
     const int configured = std::max(1, c.collision_subcycles_per_step);
     const int min_multi = std::max(1, c.collision_max_events_per_step);
     const int subcycles = c.collision_multi_event_mode ? std::max(configured, min_multi) : configured;
     return !c.collision_multi_event_mode && subcycles == 1 && !c.deep_collision_diagnostics &&
            c.all_flow_models_constant && c.uniform_dt;
+}
+
+inline bool fused_step_eligible(const EngineConditions& c) {
+    return !c.deep_collision_diagnostics && c.all_flow_models_constant && c.uniform_dt;
 }
 
 inline EngineConditions engine_conditions(const Physics& p, bool deep_collision_diagnostics,

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define ICARION_GPU_ABI_VERSION 4
+#define ICARION_GPU_ABI_VERSION 5
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -191,9 +191,20 @@ typedef struct icarion_physics_desc {
                                      false, CollisionKernels.cpp:204-213)     */
     int32_t precision;            /* enum icarion_precision                   */
     int32_t ehss_mode;            /* enum icarion_ehss_mode                   */
-    int32_t reserved0;
+    int32_t collision_substeps;   /* collision stages per engine step, each
+                                     with dt / substeps on the pushed state
+                                     (handle_collisions_cpu's loop,
+                                     SimulationEngineCollisionRuntime.cpp:
+                                     84-191): collision_subcycles_per_step, or
+                                     max(it, collision_max_events_per_step)
+                                     in multi-event mode (:26-33); 0 or 1 =
+                                     one stage.  Not with the wave-cooperative
+                                     geometry EHSS of a single-domain
+                                     fixed-step handle or the EHSS surrogate */
     uint64_t rng_seed;            /* simulation.rng_seed                      */
 } icarion_physics_desc;
+
+#define ICARION_MAX_COLLISION_SUBSTEPS 1024
 
 /* Adaptive step control of ICARION_INTEGRATOR_RK45
  * (RK45Strategy::AdaptiveConfig, RK45Strategy.h:54-65; defaults in brackets,

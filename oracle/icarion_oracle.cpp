@@ -1188,6 +1188,10 @@ struct IonCtx {
 
 // rng_kind ORACLE_RNG_PHILOX: a fresh (seed, ion, step) stream per step;
 // ORACLE_RNG_MT19937_64: the ion's persistent PhysicsRng (*mt).
+// Collision stages per engine step: the substeps the engine's CPU path runs
+// (SimulationEngineCollisionRuntime.cpp:26-33, 77-82), 1 when none are set.
+int collision_substeps(const icarion_physics_desc& ph) { return std::max(1, static_cast<int>(ph.collision_substeps)); }
+
 uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
                  const icarion_species_desc& sp, const double* atoms, const oracle_field* f,
                  IonCtx& ion, const double* t_table, double dt, uint64_t n_steps,
@@ -1215,18 +1219,20 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
         else
             rk4_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         ++pushed;
-        // perform_collisions
+        // perform_collisions: handle_collisions_cpu's substep loop
+        // (SimulationEngineCollisionRuntime.cpp:84-191), collision_dt =
+        // dt / substeps, each substep an event of its own, one RNG stream
         if (ph.collision_model != ICARION_COLLISION_NONE) {
-            bool hit;
-            if (mt) {
-                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, *mt, scratch, tb);
-            } else {
-                PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
-                hit = collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb);
-            }
-            if (hit) {
-                ++ion.ncoll;
-                ++collisions;
+            const int ns = collision_substeps(ph);
+            const double collision_dt = dt / static_cast<double>(ns);
+            PhiloxStream prng(ph.rng_seed, ion_index, step0 + k);
+            for (int sub = 0; sub < ns; ++sub) {
+                const bool hit = mt ? collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, *mt, scratch, tb)
+                                    : collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, prng, scratch, tb);
+                if (hit) {
+                    ++ion.ncoll;
+                    ++collisions;
+                }
             }
         }
         // arrival: first crossing of the detector plane (linear in the step)
@@ -1362,9 +1368,13 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
                 tb.mix_row = gm->sigma_m2 + static_cast<size_t>(sidx) * gm->n_components;
             }
             PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
-            if (collide(d, ph, sp_d, atoms, ion.m, ion.q, ion.vel, dt, rng, scratch, tb)) {
-                ++ion.ncoll;
-                ++collisions;
+            const int ns = collision_substeps(ph);
+            const double collision_dt = dt / static_cast<double>(ns);
+            for (int sub = 0; sub < ns; ++sub) {
+                if (collide(d, ph, sp_d, atoms, ion.m, ion.q, ion.vel, collision_dt, rng, scratch, tb)) {
+                    ++ion.ncoll;
+                    ++collisions;
+                }
             }
         }
         if (std::isfinite(d.detector_z_m) && std::isnan(ion.arrival)) {
@@ -1890,10 +1900,15 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                 if (ph.collision_model != ICARION_COLLISION_NONE) {
                     PhiloxStream rng(ph.rng_seed, static_cast<uint32_t>(i), step_offset + k);
                     const IonTables tb = rt.for_species(s < n_species ? s : 0);
-                    if (collide(d, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, used,
-                                rng, scratch, tb)) {
-                        if (ions->n_collisions) ++ions->n_collisions[i];
-                        ++coll_total;
+                    // substeps of the ion's own dt (dt_used_per_ion / substeps)
+                    const int ns = collision_substeps(ph);
+                    const double collision_dt = used / static_cast<double>(ns);
+                    for (int sub = 0; sub < ns; ++sub) {
+                        if (collide(d, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, collision_dt,
+                                    rng, scratch, tb)) {
+                            if (ions->n_collisions) ++ions->n_collisions[i];
+                            ++coll_total;
+                        }
                     }
                 }
                 if (detect && ions->arrival_time_s && std::isnan(ions->arrival_time_s[i])) {

@@ -105,7 +105,8 @@ static void num(std::ostream& o, const char* k, double v, bool comma = true) {
 
 int main(int argc, char** argv) {
     if (argc == 2 && std::string(argv[1]) == "--gate") {
-        // fused_step_eligible over every combination of the engine conditions
+        // fused_step_eligible, the engine's own batch-collision predicate and
+        // the collision substeps over every combination of the engine conditions
         std::cout << "[";
         bool first = true;
         for (int multi = 0; multi < 2; ++multi)
@@ -118,11 +119,13 @@ int main(int argc, char** argv) {
                                 ph.collision_multi_event_mode = multi != 0;
                                 ph.collision_subcycles_per_step = sub;
                                 ph.collision_max_events_per_step = maxev;
-                                const bool ok = C::fused_step_eligible(
-                                    C::engine_conditions(ph, deep != 0, flow != 0, udt != 0));
+                                const C::EngineConditions ec =
+                                    C::engine_conditions(ph, deep != 0, flow != 0, udt != 0);
+                                const bool ok = C::fused_step_eligible(ec);
+                                const bool engine = C::engine_batch_collisions(ec);
                                 std::cout << (first ? "" : ", ") << "[" << multi << ", " << sub << ", " << maxev
                                           << ", " << deep << ", " << flow << ", " << udt << ", " << (ok ? 1 : 0)
-                                          << "]";
+                                          << ", " << (engine ? 1 : 0) << ", " << C::collision_substeps(ph) << "]";
                                 first = false;
                             }
         std::cout << "]\n";

@@ -190,19 +190,27 @@ def test_unsupported_configurations(driver):
 
 
 def test_fused_step_gate(driver):
-    """Eligible only with no multi-event mode, one collision subcycle, no deep
-    collision diagnostics, constant flow and a uniform dt (the maximum events
-    per step counts only in multi-event mode): two of the 64 combinations; each
-    condition alone makes the step ineligible."""
+    """Over the 64 combinations of the engine conditions:
+      * the engine's own batch-collision path (SimulationEngineCollisionRuntime.cpp:46)
+        needs no multi-event mode, one subcycle, no deep diagnostics, constant
+        flow and a uniform dt: two rows (the maximum events per step counts only
+        in multi-event mode), and each condition alone breaks it;
+      * the fused device step, which runs the CPU path's collision substeps
+        itself, needs no deep diagnostics, constant flow and a uniform dt: the
+        eight rows with those, whatever the subcycles and multi-event mode;
+      * its collision substeps are the subcycles, or max(subcycles, max events)
+        in multi-event mode (:26-33)."""
     rows = json.loads(subprocess.run([driver, "--gate"], check=True, capture_output=True,
                                      text=True).stdout)
     assert len(rows) == 64
-    ok = [r for r in rows if r[6] == 1]
-    assert [r[:6] for r in ok] == [[0, 1, 1, 0, 1, 1], [0, 1, 3, 0, 1, 1]]
+    engine = [r for r in rows if r[7] == 1]
+    assert [r[:6] for r in engine] == [[0, 1, 1, 0, 1, 1], [0, 1, 3, 0, 1, 1]]
     base = dict(zip(("multi", "sub", "maxev", "deep", "flow", "udt"), (0, 1, 1, 0, 1, 1)))
     for key, bad in (("multi", 1), ("sub", 2), ("deep", 1), ("flow", 0), ("udt", 0)):
         row = dict(base, **{key: bad})
         match = [r for r in rows if r[:6] == list(row.values())]
-        assert match and match[0][6] == 0, key
-    # max events per step only matters in multi-event mode
-    assert [r for r in rows if r[:6] == [0, 1, 3, 0, 1, 1]][0][6] == 1
+        assert match and match[0][7] == 0, key
+    for multi, sub, maxev, deep, flow, udt, ok, eng, nsub in rows:
+        assert ok == int(deep == 0 and flow == 1 and udt == 1)
+        assert nsub == (max(sub, maxev) if multi else sub)
+        assert eng <= ok

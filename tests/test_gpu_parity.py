@@ -28,7 +28,7 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
     phys = w["physics"]
     if precision is not None:
         phys = W.physics(phys.integrator, phys.collision_model, phys.ou_thermal_kicks, precision,
-                         phys.rng_seed, phys.ehss_mode)
+                         phys.rng_seed, phys.ehss_mode, phys.collision_substeps)
     with gpu.Handle(steps_per_launch=steps_per_launch, block_size=block) as h:
         h.set_domain(w["domain"], phys)
         h.set_species(w["species"], w.get("atoms"))
