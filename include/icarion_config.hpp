@@ -132,6 +132,11 @@ struct Environment {
     double pressure_Pa = 101325.0;
     std::string gas_species = "He";
     std::array<double, 3> gas_velocity_m_s{{0.0, 0.0, 0.0}};
+    // flow_model and its parameters (EnvironmentConfig.h:22-26, 60-62; names as
+    // parse_flow_model accepts them, EnvironmentConfig.h:81-95)
+    std::string flow_model = "constant";
+    double axial_flow_velocity_m_s = 0.0;
+    double axial_flow_max_velocity_m_s = 0.0;
     std::vector<GasComponent> gas_mixture;
     // derived
     double particle_density_m_3 = 0.0;
@@ -281,6 +286,23 @@ inline double axial_V_from_EN(double EN_Td, double pressure_Pa, double temperatu
 // (Environment::derive).  detector_z_m is the arrival plane of the in-kernel
 // first-crossing times (NaN: none; the reference has no such plane on the
 // engine path, its analysis finds arrivals in the snapshots).
+// FlowModelKind of a flow_model name: 0 Constant, 1 AxialUniform, 2 AxialParabolic
+// (EnvironmentConfig::parse_flow_model, case-insensitive).
+inline int flow_kind(const std::string& name) {
+    std::string n = name;
+    for (char& c : n) c = static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+    if (n == "constant" || n == "legacy") return 0;
+    if (n == "axial_uniform" || n == "uniform" || n == "axial_const") return 1;
+    if (n == "axial_parabolic" || n == "parabolic" || n == "poiseuille") return 2;
+    throw std::invalid_argument("unknown flow_model '" + name + "'");
+}
+
+// Whether the device step covers the domain's flow model: Constant, and
+// AxialUniform, whose gas velocity gas_velocity_at (EnvironmentConfig.h:97-129)
+// gives as the constant R_l2g (0, 0, u) everywhere.  AxialParabolic depends on
+// the ion's radius and stays on the CPU path.
+inline bool flow_covered(const Environment& e) { return flow_kind(e.flow_model) != 2; }
+
 inline icarion_domain_desc to_desc(const Domain& d, double detector_z_m = NAN) {
     icarion_domain_desc o{};
     o.instrument = instrument_code(d.instrument);
@@ -316,7 +338,18 @@ inline icarion_domain_desc to_desc(const Domain& d, double detector_z_m = NAN) {
     o.gas_mass_kg = e.gas_mass_kg;
     o.gas_radius_m = e.gas_radius_m;
     o.gas_polarizability_m3 = e.gas_polarizability_m3;
-    for (int k = 0; k < 3; ++k) o.gas_velocity_m_s[k] = e.gas_velocity_m_s[k];
+    const int flow = flow_kind(e.flow_model);
+    if (flow == 2)
+        throw Unsupported("domain '" + d.name + "': the axial parabolic flow model is not covered by the device path");
+    if (flow == 1) {
+        // gas_velocity_at's AxialUniform branch: to_global_vel((0, 0, u)), the
+        // local-to-global rotation's third column times u (the collision
+        // handlers on the engine's CPU path and DampingForce both use it)
+        const auto& R = d.geometry.rotation_local_to_global;
+        for (int k = 0; k < 3; ++k) o.gas_velocity_m_s[k] = R[3 * k + 2] * e.axial_flow_velocity_m_s;
+    } else {
+        for (int k = 0; k < 3; ++k) o.gas_velocity_m_s[k] = e.gas_velocity_m_s[k];
+    }
     o.detector_z_m = detector_z_m;
     return o;
 }
@@ -589,7 +622,7 @@ inline SpeciesTable species_table(const std::vector<SpeciesRecord>& records, con
 // the ions of a fused step (they have had their substeps), and the fused step
 // itself is eligible when the device covers what the CPU path would do: no
 // deep collision diagnostics (the per-event records are host-side), constant
-// flow in every domain, one dt for all active ions.  A reference-side strategy
+// or axial uniform flow in every domain, one dt for all active ions.  A reference-side strategy
 // answers supports_batch() / step_batch() with this predicate and leaves the
 // whole step to the CPU when it is false.
 struct EngineConditions {
@@ -598,6 +631,7 @@ struct EngineConditions {
     int collision_max_events_per_step = 1;
     bool deep_collision_diagnostics = false;
     bool all_flow_models_constant = true;   // EnvironmentConfig::flow_model == Constant everywhere
+    bool all_flow_models_covered = true;    // flow_covered() in every domain
     bool uniform_dt = true;                 // every active ion has the same dt
 };
 
@@ -611,17 +645,21 @@ inline bool engine_batch_collisions(const EngineConditions& c) {
 }
 
 inline bool fused_step_eligible(const EngineConditions& c) {
-    return !c.deep_collision_diagnostics && c.all_flow_models_constant && c.uniform_dt;
+    return !c.deep_collision_diagnostics && c.all_flow_models_covered && c.uniform_dt;
 }
 
+// all_flow_models_covered: flow_covered() in every domain (by default the
+// same as all_flow_models_constant).
 inline EngineConditions engine_conditions(const Physics& p, bool deep_collision_diagnostics,
-                                          bool all_flow_models_constant, bool uniform_dt) {
+                                          bool all_flow_models_constant, bool uniform_dt,
+                                          std::optional<bool> all_flow_models_covered = std::nullopt) {
     EngineConditions c;
     c.collision_multi_event_mode = p.collision_multi_event_mode;
     c.collision_subcycles_per_step = p.collision_subcycles_per_step;
     c.collision_max_events_per_step = p.collision_max_events_per_step;
     c.deep_collision_diagnostics = deep_collision_diagnostics;
     c.all_flow_models_constant = all_flow_models_constant;
+    c.all_flow_models_covered = all_flow_models_covered.value_or(all_flow_models_constant);
     c.uniform_dt = uniform_dt;
     return c;
 }

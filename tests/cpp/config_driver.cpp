@@ -42,6 +42,17 @@ static C::Domain load_domain(const Value& j) {
     d.env.pressure_Pa = e["pressure_Pa"].as_double(101325.0);
     d.env.gas_species = e["gas_species"].as_string("He");
     d.env.gas_velocity_m_s = vec3(e["gas_velocity_m_s"]);
+    // flow_model and flow_parameters as DomainConfigLoader::load_environment reads them
+    // (DomainConfigLoader.cpp:204-221)
+    d.env.flow_model = e["flow_model"].as_string("constant");
+    d.env.axial_flow_velocity_m_s = e["axial_flow_velocity_m_s"].as_double(0.0);
+    d.env.axial_flow_max_velocity_m_s = e["axial_flow_max_velocity_m_s"].as_double(0.0);
+    {
+        const Value& fp = e["flow_parameters"];
+        d.env.axial_flow_velocity_m_s = fp["axial_flow_velocity_m_s"].as_double(d.env.axial_flow_velocity_m_s);
+        d.env.axial_flow_max_velocity_m_s =
+            fp["axial_flow_max_velocity_m_s"].as_double(d.env.axial_flow_max_velocity_m_s);
+    }
     const Value& mix = e["gas_mixture"];
     for (size_t k = 0; k < mix.size(); ++k) {
         C::GasComponent c;
@@ -129,6 +140,53 @@ int main(int argc, char** argv) {
                                 first = false;
                             }
         std::cout << "]\n";
+        return 0;
+    }
+    if (argc == 2 && std::string(argv[1]) == "--flow") {
+        // to_desc's gas velocity for the flow models: the rotated AxialUniform case
+        // of the reference's engine test (test_simulation_engine.cpp:652-685:
+        // u = 100 m/s, local-to-global columns (0,0,-1), (0,1,0), (1,0,0)), an
+        // unrotated one, the constant model, and whether AxialParabolic is taken
+        C::Domain d;
+        d.name = "flow";
+        d.geometry.length_m = 0.02;
+        d.geometry.radius_m = 0.01;
+        d.env.gas_species = "N2";
+        d.env.derive();
+        d.fields.axial_V = 10.0;
+        auto show = [&](const char* tag) {
+            std::cout << "\"" << tag << "\": ";
+            try {
+                const icarion_domain_desc o = C::to_desc(d);
+                std::cout << "[" << o.gas_velocity_m_s[0] << ", " << o.gas_velocity_m_s[1] << ", "
+                          << o.gas_velocity_m_s[2] << "]";
+            } catch (const C::Unsupported&) {
+                std::cout << "\"unsupported\"";
+            }
+        };
+        std::cout << "{";
+        d.env.flow_model = "axial_uniform";
+        d.env.axial_flow_velocity_m_s = 100.0;
+        d.env.gas_velocity_m_s = {{0.0, 0.0, 100.0}};
+        d.geometry.rotation_local_to_global = {{0, 0, 1, 0, 1, 0, -1, 0, 0}};   // row-major
+        d.geometry.rotation_global_to_local = {{0, 0, -1, 0, 1, 0, 1, 0, 0}};
+        show("uniform_rotated");
+        std::cout << ", ";
+        d.geometry.rotation_local_to_global = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+        d.geometry.rotation_global_to_local = {{1, 0, 0, 0, 1, 0, 0, 0, 1}};
+        d.env.flow_model = "Uniform";
+        d.env.axial_flow_velocity_m_s = 134.0;
+        show("uniform");
+        std::cout << ", ";
+        d.env.flow_model = "legacy";
+        d.env.gas_velocity_m_s = {{1.0, 2.0, 3.0}};
+        show("constant");
+        std::cout << ", ";
+        d.env.flow_model = "poiseuille";
+        show("parabolic");
+        std::cout << ", \"covered\": [" << C::flow_covered(d.env);
+        d.env.flow_model = "axial_const";
+        std::cout << ", " << C::flow_covered(d.env) << "]}\n";
         return 0;
     }
     if (argc < 4) {

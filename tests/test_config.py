@@ -214,3 +214,20 @@ def test_fused_step_gate(driver):
         assert ok == int(deep == 0 and flow == 1 and udt == 1)
         assert nsub == (max(sub, maxev) if multi else sub)
         assert eng <= ok
+
+
+def test_flow_models(driver):
+    """to_desc's gas velocity for the flow models.  The rotated AxialUniform
+    case is the reference's own engine test (test_simulation_engine.cpp:652-685,
+    u = 100 m/s with local-to-global columns (0,0,-1), (0,1,0), (1,0,0)), whose
+    collision handler receives (100, 0, 0) from gas_velocity_at; unrotated, the
+    flow is (0, 0, u); the constant model passes gas_velocity_m_s through; the
+    axial parabolic model (radius-dependent) is declined, and flow_covered says
+    which models the device step takes."""
+    out = json.loads(subprocess.run([driver, "--flow"], check=True, capture_output=True,
+                                    text=True).stdout)
+    assert out["uniform_rotated"] == pytest.approx([100.0, 0.0, 0.0], abs=1e-12)
+    assert out["uniform"] == pytest.approx([0.0, 0.0, 134.0], abs=1e-12)
+    assert out["constant"] == [1.0, 2.0, 3.0]
+    assert out["parabolic"] == "unsupported"
+    assert out["covered"] == [0, 1]
