@@ -233,6 +233,16 @@ struct icarion_gpu {
     // its rows concatenated on the device at each step call
     std::vector<std::vector<SpeciesDev>> dom_species;
     DevBuf<SpeciesDev> dom_species_d;
+    // several domains: each domain's uniform field array (one unscaled term,
+    // float nodes and, for double input, double nodes)
+    struct DomField {
+        bool have = false;
+        int nx = 0, ny = 0, nz = 0;
+        double x0 = 0, y0 = 0, z0 = 0, dx = 1, dy = 1, dz = 1;
+        DevBuf<float4> fa32;
+        DevBuf<double4> fa64;
+    };
+    std::vector<DomField> dom_field;
     // RK45 evaluates the waveforms in the kernel: their descriptors and points
     DevBuf<icarion_waveform_desc> wave_desc;
     DevBuf<double> wave_pts;
@@ -1463,8 +1473,14 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         fail(ICARION_ERR_UNSUPPORTED, "per-domain gas mixtures run with HSS or geometry EHSS collisions");
     if (h->has_waves)
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their waveforms per domain (icarion_gpu_set_domain_waveforms)");
-    for (const auto& d : h->doms)
-        if (d.use_field_array) fail(ICARION_ERR_UNSUPPORTED, "several domains run with analytic fields");
+    bool dom_fields = false;
+    for (uint32_t d = 0; d < h->n_dom; ++d) {
+        if (!h->doms[d].use_field_array) continue;
+        if (!h->dom_field[d].have)
+            fail(ICARION_ERR_STATE, "domain " + std::to_string(d) +
+                                        " uses a field array but none was uploaded (icarion_gpu_upload_domain_field_array)");
+        dom_fields = true;
+    }
     if (h->species_used > h->n_species)
         fail(ICARION_ERR_STATE, "the species table no longer covers the resident ions' species indices");
     set_device(h);
@@ -1542,6 +1558,29 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             if (wf[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
             if (h->doms[d].instrument == ICARION_INSTRUMENT_ORBITRAP && wf[ICARION_SLOT_DC_RADIAL])
                 P.orbi_k = orbitrap_k(h->doms[d], wf[ICARION_SLOT_DC_RADIAL]->eval(0.0));
+            P.field_array = 0;
+            if (h->doms[d].use_field_array) {
+                // GridFieldProvider: E from the domain's own array, unscaled for
+                // one term (PhysicsSetup.cpp:168-171) apart from field_array_scale
+                const auto& F = h->dom_field[d];
+                P.field_array = 1;
+                P.nx = F.nx; P.ny = F.ny; P.nz = F.nz;
+                P.n_terms = 1;
+                P.x0 = F.x0; P.y0 = F.y0; P.z0 = F.z0;
+                P.dx = F.dx; P.dy = F.dy; P.dz = F.dz;
+                P.inv_dx = 1.0 / F.dx; P.inv_dy = 1.0 / F.dy; P.inv_dz = 1.0 / F.dz;
+                P.x_last = F.x0 + (F.nx - 1) * F.dx;
+                P.y_last = F.y0 + (F.ny - 1) * F.dy;
+                P.z_last = F.z0 + (F.nz - 1) * F.dz;
+                P.ax_x = P.ax_y = P.ax_z = nullptr;
+                P.axf_x = P.axf_y = P.axf_z = nullptr;
+                for (int j = 0; j < kMaxFieldTerms; ++j) {
+                    P.term_rf[j] = 0;
+                    P.term_a[j] = P.term_w[j] = P.term_phi[j] = 0.0;
+                }
+                P.fa_f32 = F.fa32.p;
+                P.fa_f64 = h->phys.precision == ICARION_PRECISION_FP64 ? F.fa64.p : nullptr;
+            }
             fill_fast_const(P);
             append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps, false,
                          waves ? wf : nullptr);
@@ -1573,6 +1612,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         M.seed = h->phys.rng_seed;
         M.n_ions = static_cast<uint32_t>(h->n);
         M.n_sub = Ps[0].n_sub;
+        M.has_field_array = dom_fields ? 1 : 0;
         const IonPtrs S = ion_ptrs(h);
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
         const int integ = h->phys.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0;
@@ -2410,6 +2450,11 @@ int icarion_gpu_destroy(icarion_gpu* h) {
         if (st) (void)hipStreamSynchronize(st);
     h->species.release(); h->atoms_f32.release(); h->atom_offsets.release(); h->atom_counts.release();
     h->mlp_blob.release(); h->fa32.release(); h->fa64.release();
+    for (auto& F : h->dom_field) {
+        F.fa32.release();
+        F.fa64.release();
+    }
+    h->dom_species_d.release();
     for (auto& a : h->ax64) a.release();
     for (auto& a : h->ax32) a.release();
     h->ord.release(); h->ord_keys_a.release(); h->ord_keys_b.release(); h->ord_idx.release(); h->ord_temp.release();
@@ -2435,6 +2480,7 @@ int icarion_gpu_destroy(icarion_gpu* h) {
     h->spec.release(); h->ncoll.release(); h->flags.release(); h->table.release(); h->counters.release();
     h->term_scales.release();
     h->dom_params_d.release(); h->dom_tables.release(); h->dom_spans_d.release(); h->dom_ap_d.release();
+    h->dom_mix_d.release(); h->dom_emix_d.release();
     h->ion_dom.release();
     for (int j = 0; j < 3; ++j) {
         h->snap.full[j].release();
@@ -2495,6 +2541,11 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_mix_all.clear();
         h->dom_emix_sig.clear();
         h->dom_species.clear();
+        for (auto& F : h->dom_field) {   // DevBuf frees only on release()
+            F.fa32.release();
+            F.fa64.release();
+        }
+        h->dom_field.clear();
         if (n_domains == 1) return;
         h->doms.assign(domains, domains + n_domains);
         h->dom_wave.resize(n_domains);
@@ -2504,6 +2555,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_mix_all.resize(n_domains);
         h->dom_emix_sig.resize(n_domains);
         h->dom_species.resize(n_domains);
+        h->dom_field.resize(n_domains);
         // DomainManager's constructor (DomainManager.cpp:26-63): a span per
         // cylindrical domain, z range and radius widened by 1e-12 m, with the
         // index of its domain, sorted by z_min (std::sort, as there)
@@ -2675,7 +2727,8 @@ namespace {
 
 // Node-major device copies of the terms (element node * n_terms + term);
 // double copies too when any term arrives in double.
-void upload_field_data(icarion_gpu* h, size_t nn, const icarion_field_term_desc* terms, uint32_t nt) {
+void upload_field_data_to(icarion_gpu* h, DevBuf<float4>& fa32, DevBuf<double4>& fa64, size_t nn,
+                          const icarion_field_term_desc* terms, uint32_t nt) {
     bool any_double = false;
     for (uint32_t j = 0; j < nt; ++j) any_double |= terms[j].dtype == 1;
     std::vector<float4> f(nn * nt);
@@ -2695,10 +2748,14 @@ void upload_field_data(icarion_gpu* h, size_t nn, const icarion_field_term_desc*
             }
         }
     }
-    if (any_double) copy_in(h->fa64, g.data(), g.size(), h->stream, "upload field (double)");
-    else h->fa64.release();
-    copy_in(h->fa32, f.data(), f.size(), h->stream, "upload field (float)");
+    if (any_double) copy_in(fa64, g.data(), g.size(), h->stream, "upload field (double)");
+    else fa64.release();
+    copy_in(fa32, f.data(), f.size(), h->stream, "upload field (float)");
     hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+void upload_field_data(icarion_gpu* h, size_t nn, const icarion_field_term_desc* terms, uint32_t nt) {
+    upload_field_data_to(h, h->fa32, h->fa64, nn, terms, nt);
 }
 
 // An axis whose nodes are x0 + i * d to within 1e-12 of its span.
@@ -2738,6 +2795,31 @@ int icarion_gpu_upload_field_array(icarion_gpu* h, uint32_t nx, uint32_t ny, uin
         f.z_last = z0 + (nz - 1) * dz;
         h->fsrc = f;
         h->have_field = true;
+    });
+}
+
+int icarion_gpu_upload_domain_field_array(icarion_gpu* h, uint32_t domain, uint32_t nx, uint32_t ny, uint32_t nz,
+                                          double x0, double dx, double y0, double dy, double z0, double dz,
+                                          const void* E, int32_t dtype) {
+    return guarded([&] {
+        if (!h || !E) fail(ICARION_ERR_INVALID, "NULL argument");
+        if (h->n_dom < 2) fail(ICARION_ERR_STATE, "icarion_gpu_set_domains with two or more domains not called");
+        if (domain >= h->n_dom) fail(ICARION_ERR_INVALID, "domain index out of range");
+        if (!h->doms[domain].use_field_array)
+            fail(ICARION_ERR_INVALID, "the domain's use_field_array is 0");
+        if (nx < 2 || ny < 2 || nz < 2) fail(ICARION_ERR_INVALID, "field array needs >= 2 nodes per axis");
+        if (!(dx > 0.0 && dy > 0.0 && dz > 0.0)) fail(ICARION_ERR_INVALID, "grid spacing must be > 0");
+        if (dtype != 0 && dtype != 1) fail(ICARION_ERR_INVALID, "dtype must be 0 (float) or 1 (double)");
+        set_device(h);
+        icarion_field_term_desc t{};
+        t.E = E;
+        t.dtype = dtype;
+        auto& F = h->dom_field[domain];
+        upload_field_data_to(h, F.fa32, F.fa64, static_cast<size_t>(nx) * ny * nz, &t, 1);
+        F.nx = static_cast<int>(nx); F.ny = static_cast<int>(ny); F.nz = static_cast<int>(nz);
+        F.x0 = x0; F.y0 = y0; F.z0 = z0;
+        F.dx = dx; F.dy = dy; F.dz = dz;
+        F.have = true;
     });
 }
 

@@ -91,14 +91,19 @@ __device__ __forceinline__ double damping_pre_f64(const StepParams& D, const Spe
     return lpre;
 }
 
-template <int COLL>
+// EXT: the instantiation that also covers domains with a uniform field array
+// (GridFieldProvider, one unscaled term) and collision substeps
+template <int COLL, bool EXT>
 __device__ __forceinline__ d3 domain_force(const StepParams& D, d3 p, d3 v, double c, double ez, double q, double m,
                                            double lpre) {
+    if constexpr (EXT) {
+        if (D.field_array) return force_f64<FIELD_ARRAY, COLL>(D, p, v, c, ez, q, m, lpre);
+    }
     if (D.instrument == 3) return force_f64<FIELD_ORBITRAP, COLL>(D, p, v, c, ez, q, m, lpre);
     return force_f64<FIELD_LINEAR, COLL>(D, p, v, c, ez, q, m, lpre);
 }
 
-template <int INTEG, int COLL, bool SUB>
+template <int INTEG, int COLL, bool EXT>
 __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S, const uint32_t i, uint32_t& pushed,
                                           uint32_t& ncoll_launch) {
     d3 pos{S.x[i], S.y[i], S.z[i]};
@@ -159,18 +164,18 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
         if constexpr (INTEG == 0) {
             const d3 p0 = pos, v0 = vel;
             const d3 k1v = v0;
-            const d3 k1a = domain_force<COLL>(D, p0, v0, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 k1a = domain_force<COLL, EXT>(D, p0, v0, te.c0, te.e0, q, m, lpre) * inv_m;
             const d3 k2v = v0 + k1a * half_dt;
-            const d3 k2a = domain_force<COLL>(D, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k2a = domain_force<COLL, EXT>(D, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lpre) * inv_m;
             const d3 k3v = v0 + k2a * half_dt;
-            const d3 k3a = domain_force<COLL>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k3a = domain_force<COLL, EXT>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lpre) * inv_m;
             const d3 k4v = v0 + k3a * dt;
-            const d3 k4a = domain_force<COLL>(D, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lpre) * inv_m;
+            const d3 k4a = domain_force<COLL, EXT>(D, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lpre) * inv_m;
             const double h6 = dt / 6.0;
             pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
             vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
         } else {
-            const d3 a = domain_force<COLL>(D, pos, vel, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 a = domain_force<COLL, EXT>(D, pos, vel, te.c0, te.e0, q, m, lpre) * inv_m;
             const d3 vm = vel + a * half_dt;
             d3 vp = vm;
             if (D.magnetic) {
@@ -187,9 +192,9 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
         ++pushed;
         if constexpr (COLL != COLL_NONE) {
             PhiloxStream rng(M.seed, i, M.step0 + k);
-            // collision substeps in the SUB instantiation (one stage otherwise)
-            const int ns = SUB ? D.n_sub : 1;
-            const double dtc = SUB ? dt / static_cast<double>(D.n_sub) : dt;
+            // collision substeps in the EXT instantiation (one stage otherwise)
+            const int ns = EXT ? D.n_sub : 1;
+            const double dtc = EXT ? dt / static_cast<double>(D.n_sub) : dt;
             for (int s = 0; s < ns; ++s) {
                 bool hit;
                 if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
@@ -241,9 +246,12 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
     S.domain[i] = cur;
 }
 
-template <int COLL>
+template <int COLL, bool EXT>
 __device__ __forceinline__ f3 domain_accel_f32(const StepParams& D, f3 p, f3 v, double c, double ez, float qm,
                                                float lpre) {
+    if constexpr (EXT) {
+        if (D.field_array) return accel_f32<FIELD_ARRAY, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
+    }
     if (D.instrument == 3) return accel_f32<FIELD_ORBITRAP, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
     return accel_f32<FIELD_LINEAR, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
 }
@@ -252,7 +260,7 @@ __device__ __forceinline__ f3 domain_accel_f32(const StepParams& D, f3 p, f3 v, 
 // position and velocity as FP32 (hi, lo) pairs, forces and collisions in FP32
 // with each domain's FP32 constants; the domain lookup, bridge and handoff
 // in FP64 on hi + lo, as the FP64 kernel takes them.
-template <int INTEG, int COLL, bool SUB>
+template <int INTEG, int COLL, bool EXT>
 __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtrs& S, const uint32_t i,
                                               uint32_t& pushed, uint32_t& ncoll_launch) {
     const double X[6] = {S.x[i], S.y[i], S.z[i], S.vx[i], S.vy[i], S.vz[i]};
@@ -320,17 +328,17 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
         const f3 v0{hi[3], hi[4], hi[5]};
         f3 dpos, dvel;
         if constexpr (INTEG == 0) {
-            const f3 k1a = domain_accel_f32<COLL>(D, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 k1a = domain_accel_f32<COLL, EXT>(D, p0, v0, te.c0, te.e0, qm, lpre);
             const f3 k2v = v0 + k1a * half_dt;
-            const f3 k2a = domain_accel_f32<COLL>(D, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lpre);
+            const f3 k2a = domain_accel_f32<COLL, EXT>(D, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lpre);
             const f3 k3v = v0 + k2a * half_dt;
-            const f3 k3a = domain_accel_f32<COLL>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lpre);
+            const f3 k3a = domain_accel_f32<COLL, EXT>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lpre);
             const f3 k4v = v0 + k3a * dt;
-            const f3 k4a = domain_accel_f32<COLL>(D, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lpre);
+            const f3 k4a = domain_accel_f32<COLL, EXT>(D, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lpre);
             dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
             dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
         } else {
-            const f3 a = domain_accel_f32<COLL>(D, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 a = domain_accel_f32<COLL, EXT>(D, p0, v0, te.c0, te.e0, qm, lpre);
             const f3 vm = v0 + a * half_dt;
             f3 rot{0.0f, 0.0f, 0.0f};
             if (D.magnetic) {
@@ -353,10 +361,10 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
             PhiloxStream rng(M.seed, i, M.step0 + k);
             f3 v{hi[3], hi[4], hi[5]};
             const f3 vin = v;
-            // collision substeps in the SUB instantiation (one stage otherwise)
-            const int ns = SUB ? D.n_sub : 1;
-            const float sub = SUB ? 1.0f / static_cast<float>(D.n_sub) : 1.0f;
-            const float dtc = SUB ? static_cast<float>(M.dt / static_cast<double>(D.n_sub)) : dt;
+            // collision substeps in the EXT instantiation (one stage otherwise)
+            const int ns = EXT ? D.n_sub : 1;
+            const float sub = EXT ? 1.0f / static_cast<float>(D.n_sub) : 1.0f;
+            const float dtc = EXT ? static_cast<float>(M.dt / static_cast<double>(D.n_sub)) : dt;
             uint32_t hits = 0;
             for (int s = 0; s < ns; ++s) {
                 bool hit;
@@ -424,33 +432,33 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
     S.domain[i] = cur;
 }
 
-template <int INTEG, int COLL, bool SUB>
+template <int INTEG, int COLL, bool EXT>
 __global__ void __launch_bounds__(256) step_multi_f32_kernel(const MultiParams M, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < M.n_ions) multi_ion_f32<INTEG, COLL, SUB>(M, S, i, pushed, ncoll_launch);
+    if (i < M.n_ions) multi_ion_f32<INTEG, COLL, EXT>(M, S, i, pushed, ncoll_launch);
     add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
 }
 
-template <int INTEG, int COLL, bool SUB>
+template <int INTEG, int COLL, bool EXT>
 __global__ void __launch_bounds__(256) step_multi_f64_kernel(const MultiParams M, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < M.n_ions) multi_ion<INTEG, COLL, SUB>(M, S, i, pushed, ncoll_launch);
+    if (i < M.n_ions) multi_ion<INTEG, COLL, EXT>(M, S, i, pushed, ncoll_launch);
     add_counters_to(M.counters, M.counter_stripes, pushed, ncoll_launch);
 }
 
-// One multi-domain kernel; SUB: more than one collision stage per step
+// One multi-domain kernel; ext: collision substeps or a domain with a field array
 template <int INTEG, bool F32, int COLL>
-static void launch_multi_kernel(bool sub, dim3 grid, uint32_t block, hipStream_t s, const MultiParams& M,
+static void launch_multi_kernel(bool ext, dim3 grid, uint32_t block, hipStream_t s, const MultiParams& M,
                                 const IonPtrs& S) {
     if constexpr (F32) {
-        if (sub && COLL != COLL_NONE)
+        if (ext)
             hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL, true>), grid, dim3(block), 0, s, M, S);
         else
             hipLaunchKernelGGL((step_multi_f32_kernel<INTEG, COLL, false>), grid, dim3(block), 0, s, M, S);
     } else {
-        if (sub && COLL != COLL_NONE)
+        if (ext)
             hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL, true>), grid, dim3(block), 0, s, M, S);
         else
             hipLaunchKernelGGL((step_multi_f64_kernel<INTEG, COLL, false>), grid, dim3(block), 0, s, M, S);
@@ -458,23 +466,23 @@ static void launch_multi_kernel(bool sub, dim3 grid, uint32_t block, hipStream_t
 }
 
 template <int INTEG, bool F32>
-static hipError_t launch_multi_integ(int coll, bool sub, const MultiParams& M, const IonPtrs& S, uint32_t block,
+static hipError_t launch_multi_integ(int coll, bool ext, const MultiParams& M, const IonPtrs& S, uint32_t block,
                                      hipStream_t s) {
     const dim3 grid((M.n_ions + block - 1) / block);
     switch (coll) {
-        case COLL_NONE: launch_multi_kernel<INTEG, F32, COLL_NONE>(false, grid, block, s, M, S); break;
-        case COLL_HSS: launch_multi_kernel<INTEG, F32, COLL_HSS>(sub, grid, block, s, M, S); break;
-        case COLL_DAMPING: launch_multi_kernel<INTEG, F32, COLL_DAMPING>(sub, grid, block, s, M, S); break;
+        case COLL_NONE: launch_multi_kernel<INTEG, F32, COLL_NONE>(ext, grid, block, s, M, S); break;
+        case COLL_HSS: launch_multi_kernel<INTEG, F32, COLL_HSS>(ext, grid, block, s, M, S); break;
+        case COLL_DAMPING: launch_multi_kernel<INTEG, F32, COLL_DAMPING>(ext, grid, block, s, M, S); break;
         case COLL_HSS_MIX:   // HSS with per-domain gas mixtures
-            launch_multi_kernel<INTEG, F32, COLL_HSS_MIX>(sub, grid, block, s, M, S);
+            launch_multi_kernel<INTEG, F32, COLL_HSS_MIX>(ext, grid, block, s, M, S);
             break;
         case COLL_EHSS_MIX:   // geometry EHSS with per-domain gas mixtures, FP64 only
             if constexpr (F32) return hipErrorInvalidValue;
-            else launch_multi_kernel<INTEG, false, COLL_EHSS_MIX>(sub, grid, block, s, M, S);
+            else launch_multi_kernel<INTEG, false, COLL_EHSS_MIX>(ext, grid, block, s, M, S);
             break;
         case COLL_EHSS:   // geometry EHSS with each domain's gas, FP64 only
             if constexpr (F32) return hipErrorInvalidValue;
-            else launch_multi_kernel<INTEG, false, COLL_EHSS>(sub, grid, block, s, M, S);
+            else launch_multi_kernel<INTEG, false, COLL_EHSS>(ext, grid, block, s, M, S);
             break;
         default:
             return hipErrorInvalidValue;
@@ -488,11 +496,11 @@ hipError_t launch_multi_step(int integrator, int coll, int precision, const Mult
                              uint32_t block, hipStream_t s) {
     if (M.n_ions == 0) return hipSuccess;
     const bool f32 = precision == 1;
-    const bool sub = M.n_sub > 1;
-    if (integrator == 0) return f32 ? launch_multi_integ<0, true>(coll, sub, M, S, block, s)
-                                    : launch_multi_integ<0, false>(coll, sub, M, S, block, s);
-    if (integrator == 1) return f32 ? launch_multi_integ<1, true>(coll, sub, M, S, block, s)
-                                    : launch_multi_integ<1, false>(coll, sub, M, S, block, s);
+    const bool ext = M.n_sub > 1 || M.has_field_array;
+    if (integrator == 0) return f32 ? launch_multi_integ<0, true>(coll, ext, M, S, block, s)
+                                    : launch_multi_integ<0, false>(coll, ext, M, S, block, s);
+    if (integrator == 1) return f32 ? launch_multi_integ<1, true>(coll, ext, M, S, block, s)
+                                    : launch_multi_integ<1, false>(coll, ext, M, S, block, s);
     return hipErrorInvalidValue;
 }
 

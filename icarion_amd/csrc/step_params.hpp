@@ -180,7 +180,8 @@ struct StepParams {
     // EHSS gas mixture: every component in order (the EHSS handler's loop
     // visits them all), its participation flag, contact radius, and the EHSS
     // cross sections of species s at emix_sigma[s * n_emix + c]
-    int n_emix, emix_pad;
+    int n_emix;
+    int field_array;    // several domains: this domain's E comes from its uniform field array
     int emix_part[kMaxGasComponents];
     double emix_n[kMaxGasComponents], emix_m[kMaxGasComponents];
     double emix_sig[kMaxGasComponents], emix_r[kMaxGasComponents];
@@ -275,6 +276,8 @@ struct MultiParams {
     uint64_t step0, seed;
     uint32_t k0, n_steps, n_ions;
     int n_sub;                      // collision substeps per step (every domain's n_sub)
+    int has_field_array;            // some domain takes E from a field array
+    int pad;
 };
 
 // ---- batched clouds ----------------------------------------------------------

@@ -27,7 +27,7 @@ EXPORTED = (
     "icarion_gpu_step_snapshots", "icarion_gpu_set_domains", "icarion_gpu_download_domains",
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
     "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
-    "icarion_gpu_step_noalloc", "icarion_gpu_set_domain_species",
+    "icarion_gpu_step_noalloc", "icarion_gpu_set_domain_species", "icarion_gpu_upload_domain_field_array",
 )
 
 # icarion_gpu_step_noalloc's per-call step limit (ICARION_NOALLOC_MAX_STEPS)
@@ -375,6 +375,22 @@ class Handle:
         rc = self.lib.icarion_gpu_upload_field_array(
             self.h, nx, ny, nz, origin[0], spacing[0], origin[1], spacing[1], origin[2],
             spacing[2], E.ctypes.data_as(C.c_void_p), dtype)
+        _check(rc, self.lib)
+
+    def upload_domain_field_array(self, domain, E, origin, spacing):
+        """One domain's field array on a multi-domain handle (its descriptor has
+        use_field_array = 1); E, origin, spacing as upload_field_array."""
+        E = np.ascontiguousarray(E)
+        nz, ny, nx = E.shape[:3]
+        dtype = 0 if E.dtype == np.float32 else 1
+        if dtype == 1:
+            E = E.astype(np.float64, copy=False)
+        self._keep.append(E)
+        d = C.c_double
+        rc = self.lib.icarion_gpu_upload_domain_field_array(
+            self.h, C.c_uint32(domain), C.c_uint32(nx), C.c_uint32(ny), C.c_uint32(nz), d(origin[0]),
+            d(spacing[0]), d(origin[1]), d(spacing[1]), d(origin[2]), d(spacing[2]),
+            E.ctypes.data_as(C.c_void_p), C.c_int32(dtype))
         _check(rc, self.lib)
 
     def set_gas_mixture(self, components, sigma):

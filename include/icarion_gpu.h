@@ -548,7 +548,8 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
  *     (0 or NULL: the radius).
  * Replaces any domain set before; n_domains = 1 is icarion_gpu_set_domain.
  * Covered: analytic fields (IMS / TIMS, quadrupole, Orbitrap) with constant
- * voltages or per-domain waveforms (icarion_gpu_set_domain_waveforms), RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
+ * voltages or per-domain waveforms (icarion_gpu_set_domain_waveforms), uniform
+ * field arrays per domain (icarion_gpu_upload_domain_field_array), RK4 or Boris, FP64 or FP32 + Kahan (domain lookup, bridge and
  * handoff in FP64 on the compensated position), no / HSS / Langevin /
  * Friction collisions (HSS also with per-domain gas mixtures,
  * icarion_gpu_set_domain_gas_mixture) and geometry EHSS (FP64, online), the
@@ -559,6 +560,18 @@ int icarion_gpu_arrival_histogram(icarion_gpu *h, double lo_s, double hi_s, uint
 int icarion_gpu_set_domains(icarion_gpu *h, const icarion_domain_desc *domains,
                             const double *end_aperture_m, uint32_t n_domains,
                             const icarion_physics_desc *physics);
+
+/* The field array of one domain of a multi-domain set whose descriptor has
+ * use_field_array = 1: that domain's GridFieldProvider, a uniform grid with
+ * one unscaled term (times the domain's field_array_scale), with the
+ * arguments and rules of icarion_gpu_upload_field_array.  An ion in that
+ * domain moves in its array (clamped trilinear interpolation), elsewhere in
+ * its domain's analytic field.  icarion_gpu_set_domains clears every domain's
+ * array.  No reference counterpart as a call: it carries the field_array_terms
+ * of one entry of FullConfig::domains (PhysicsSetup.cpp:96-249). */
+int icarion_gpu_upload_domain_field_array(icarion_gpu *h, uint32_t domain, uint32_t nx, uint32_t ny, uint32_t nz,
+                                          double x0, double dx, double y0, double dy, double z0, double dz,
+                                          const void *E_xyz, int32_t dtype);
 
 /* Voltage waveforms of one domain of a multi-domain set (each domain's
  * ValueOrWaveform fields and waveform_library, DomainConfig.h), with the

@@ -370,11 +370,13 @@ def find_domain(domains, pos, end_apertures=None):
 
 def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
               end_apertures=None, threads=0, waveforms=None, atoms=None, mixtures=None,
-              domain_species=None):
+              domain_species=None, fields=None):
     """The engine loop over several domains on ens and domain_index (int32,
     in/out) in place; returns StepStats.  waveforms: None, or per domain None
     or {slot: WaveformDesc}.  domain_species: None, or per domain None or a
-    species list (that domain's rows of the species table)."""
+    species list (that domain's rows of the species table).  fields: None, or
+    per domain None or a Field (the field array of a domain with
+    use_field_array = 1)."""
     lib = load()
     arr, ap, app = _domains(domains, end_apertures)
     sp = species_array(species)
@@ -383,10 +385,16 @@ def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, ste
     assert domain_index.dtype == np.int32 and domain_index.flags.c_contiguous
     fptr = C.POINTER(C.POINTER(_abi.OracleField))()
     keep = []
-    if waveforms is not None:
+    if waveforms is not None or fields is not None:
         fl = (C.POINTER(_abi.OracleField) * len(domains))()
-        for k, slots in enumerate(waveforms):
-            if slots:
+        for k in range(len(domains)):
+            slots = waveforms[k] if waveforms is not None else None
+            fld = fields[k] if fields is not None else None
+            assert not (slots and fld is not None), "a domain takes a field array or waveforms"
+            if fld is not None:
+                keep.append(fld)
+                fl[k] = C.pointer(fld.c)
+            elif slots:
                 wf = Waveforms(slots)
                 keep.append(wf)
                 fl[k] = C.pointer(wf.c)

@@ -19,7 +19,8 @@ from scipy import stats
 from icarion_amd import _abi
 from icarion_amd import workloads as W
 from test_gpu_parity import compare_ions
-from test_oracle_domains import _gap_domains, _one_ion, domain_mixtures, domain_species_tables, domain_waveforms
+from test_oracle_domains import (_gap_domains, _one_ion, domain_mixtures, domain_species_tables, domain_waveforms,
+                                 field_array_domain_workload)
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +45,9 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
             for k, rows in enumerate(w.get("domain_species") or []):
                 if rows:
                     h.set_domain_species(k, rows)
+            for k, fld in enumerate(w.get("fields") or []):
+                if fld is not None:
+                    h.upload_domain_field_array(k, *fld)
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -60,6 +64,8 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
     st = oracle.run_multi(domains or w["domains"], physics or w["physics"], w["species"], ens, di,
                           t0, w["dt"], n_steps, step_offset=step_offset, atoms=w.get("atoms"),
                           mixtures=w.get("mixtures"), domain_species=w.get("domain_species"),
+                          fields=[None if f is None else oracle.Field(*f) for f in w["fields"]]
+                          if w.get("fields") else None,
                           end_apertures=apertures if apertures is not None else w.get("apertures"),
                           waveforms=w.get("waveforms"))
     return st, di
@@ -240,6 +246,95 @@ def test_fp32_kahan_per_domain_species_statistics(oracle, integrator):
     for d in range(3):
         assert (dg == d).sum() == pytest.approx((do == d).sum(), rel=0.05, abs=30)
     assert sg.alive == pytest.approx(so.alive, rel=0.03, abs=30)
+
+
+@pytest.mark.parametrize("integrator,collision", [
+    (_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS),
+    (_abi.INTEGRATOR_BORIS, _abi.COLLISION_NONE),
+])
+def test_per_domain_field_array_ion_by_ion(oracle, integrator, collision):
+    """The middle domain's field from a field array, the others analytic."""
+    w, fields = field_array_domain_workload(n=2048, seed=67, collision=collision, integrator=integrator)
+    w["fields"] = fields
+    print(f"per-domain field array integ={integrator} coll={collision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 6000)
+    so, do = oracle_multi(oracle, w, o, 6000)
+    ok = compare_ions(g, o, rtol=1e-9) & (dg == do) & (g.death_time_s == o.death_time_s)
+    print(f"  match {ok.mean():.4f} alive {sg.alive}/{so.alive}", flush=True)
+    if collision == _abi.COLLISION_NONE:
+        assert ok.all(), np.flatnonzero(~ok)[:10]
+        assert sg.ion_steps == so.ion_steps
+    else:
+        assert ok.mean() >= 0.99
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    # many ions have reached the array's domain
+    assert (do == 1).sum() > 100
+
+
+def test_fp32_kahan_per_domain_field_array_collisionless(oracle):
+    """FP32 + Kahan through the field-array domain against the FP64 oracle, no
+    gas: nearly every ion takes the oracle's path, positions agree to 1e-6 of
+    the array domain's radius at the median."""
+    w, fields = field_array_domain_workload(n=4096, seed=69, collision=_abi.COLLISION_NONE)
+    w["fields"] = fields
+    ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_NONE, precision=_abi.PRECISION_FP32_KAHAN, seed=69)
+    print("per-domain field array FP32 collisionless", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    # 4000 steps: the leading ions are a few mm into the array domain
+    sg, dg = gpu_multi(w, g, 4000, physics=ph)
+    so, do = oracle_multi(oracle, w, o, 4000)
+    same = (g.active == o.active) & (dg == do) & (g.death_time_s == o.death_time_s)
+    both = g.alive & o.alive & (do == 1)
+    dp = np.sqrt((g.x - o.x) ** 2 + (g.y - o.y) ** 2 + (g.z - o.z) ** 2)[both] / 2e-3
+    print(f"  same path {same.mean():.4f}; in the array domain {both.sum()}", flush=True)
+    assert same.mean() > 0.995
+    assert both.sum() > 100
+    print(f"  pos err/R median {np.median(dp):.2e} p99 {np.quantile(dp, 0.99):.2e}", flush=True)
+    assert np.median(dp) < 1e-6 and np.quantile(dp, 0.99) < 1e-5
+
+
+def test_fp32_kahan_per_domain_field_array_statistics(oracle):
+    """With gas: collision count within 2 %, the ions per domain index and the
+    survivors within four standard errors of the difference of two binomial
+    samples."""
+    w, fields = field_array_domain_workload(n=8192, seed=68, collision=_abi.COLLISION_HSS)
+    w["fields"] = fields
+    print("per-domain field array FP32", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    ph = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, precision=_abi.PRECISION_FP32_KAHAN, seed=68)
+    sg, dg = gpu_multi(w, g, 5000, physics=ph)
+    so, do = oracle_multi(oracle, w, o, 5000)
+    assert sg.collisions == pytest.approx(so.collisions, rel=0.02)
+    n = g.n
+
+    def within(a, b, what):
+        p = b / n
+        tol = 4.0 * np.sqrt(2.0 * n * p * (1.0 - p)) + 5
+        print(f"  {what}: {a} vs {b} (tolerance {tol:.0f})", flush=True)
+        assert abs(a - b) <= tol, what
+
+    for d in range(3):
+        within(int((dg == d).sum()), int((do == d).sum()), f"domain {d}")
+    within(int(sg.alive), int(so.alive), "alive")
+
+
+def test_domain_field_array_arguments():
+    from icarion_amd import gpu
+    w, fields = field_array_domain_workload(n=16)
+    E, origin, spacing = fields[1]
+    with gpu.Handle() as h:
+        h.set_domains(w["domains"], w["physics"], w["apertures"])
+        h.set_species(w["species"])
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError):   # the domain needs its array
+            h.step(0.0, w["dt"], 4)
+        with pytest.raises(gpu.IcarionError):   # domain 0 has use_field_array = 0
+            h.upload_domain_field_array(0, E, origin, spacing)
+        with pytest.raises(gpu.IcarionError):
+            h.upload_domain_field_array(5, E, origin, spacing)
+        h.upload_domain_field_array(1, E.astype(np.float32), origin, spacing)
+        h.step(0.0, w["dt"], 4)
 
 
 def test_domain_species_arguments():

@@ -256,3 +256,53 @@ def test_per_domain_species_equal_to_the_table_change_nothing(oracle):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
     np.testing.assert_array_equal(da, db)
     assert sa.collisions == sb.collisions > 0
+
+
+def field_array_domain_workload(n=256, seed=66, collision=_abi.COLLISION_HSS, integrator=_abi.INTEGRATOR_RK4):
+    """The three-domain workload with its middle region (5-15 mm) a drift
+    region of radius 2 mm in He at 1 Pa whose field comes from a field array:
+    E = (-g x, -g y, E0 (1 + 0.3 sin(2 pi (z - 5 mm) / 10 mm))) on a uniform
+    24 x 24 x 41 grid.  Returns the workload and the per-domain fields
+    [None, (E, origin, spacing), None]."""
+    w = W.multi_domain_workload(n=n, seed=seed, collision=collision, integrator=integrator)
+    d1 = W.ims_domain(origin_z=5e-3, length=10e-3, radius=2e-3, axial_V=15.0, gas="He", pressure_Pa=1.0)
+    d1.use_field_array = 1
+    nx, ny, nz = 24, 24, 41
+    origin = (-2.3e-3, -2.3e-3, 5e-3)
+    spacing = (4.6e-3 / (nx - 1), 4.6e-3 / (ny - 1), 10e-3 / (nz - 1))
+    x = origin[0] + spacing[0] * np.arange(nx)
+    y = origin[1] + spacing[1] * np.arange(ny)
+    z = origin[2] + spacing[2] * np.arange(nz)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    g, E0 = 2e5, 1500.0
+    E = np.stack([-g * X, -g * Y, E0 * (1.0 + 0.3 * np.sin(2 * np.pi * (Z - 5e-3) / 10e-3))], axis=-1)
+    w["domains"][1] = d1
+    w["apertures"] = [1.5e-3, 0.0, 0.0]
+    return w, [None, (E, origin, spacing), None]
+
+
+def test_per_domain_field_array_acts_in_its_domain(oracle):
+    """A domain's field array through the multi-domain loop is the
+    single-domain loop on that array, bit for bit, while every ion stays in it."""
+    w, fields = field_array_domain_workload(n=128)
+    E, origin, spacing = fields[1]
+    e = w["ensemble"]
+    e.z[:] = 9e-3 + 0.2e-3 * np.sin(np.arange(e.n))   # every ion in the middle domain
+    e.x[:] *= 0.5
+    e.y[:] *= 0.5
+    a, b = e.copy(), e.copy()
+    di = np.ones(a.n, np.int32)
+    n_steps = 150   # 1.5 km/s x 150 ns: no ion reaches a boundary
+    sa = oracle.run_multi(w["domains"], w["physics"], w["species"], a, di, 0.0, w["dt"], n_steps,
+                          end_apertures=w["apertures"], fields=[None, oracle.Field(E, origin, spacing), None])
+    sb = oracle.run(w["domains"][1], w["physics"], w["species"], b, 0.0, w["dt"], n_steps,
+                    field=oracle.Field(E, origin, spacing))
+    for f in ("x", "y", "z", "vx", "vy", "vz", "active", "n_collisions"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    assert (di == 1).all()
+    assert sa.ion_steps == sb.ion_steps == a.n * n_steps
+    # the array's focusing pulls |x| in: the analytic drift field has none
+    c = e.copy()
+    oracle.run(W.ims_domain(origin_z=5e-3, length=10e-3, radius=2e-3, axial_V=15.0, gas="He",
+                            pressure_Pa=1.0), w["physics"], w["species"], c, 0.0, w["dt"], n_steps)
+    assert not np.array_equal(a.vx, c.vx)
