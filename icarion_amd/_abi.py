@@ -213,6 +213,7 @@ class OracleField(C.Structure):
         ("Ex", C.POINTER(_d)), ("Ey", C.POINTER(_d)), ("Ez", C.POINTER(_d)),
         ("n_terms", _u32), ("reserved0", _u32), ("terms", C.POINTER(OracleFieldTerm)),
         ("waveforms", C.POINTER(WaveformDesc) * 5),
+        ("pressure", C.POINTER(WaveformDesc)),
     ]
 
 

@@ -219,6 +219,10 @@ struct icarion_gpu {
     bool has_waves = false;
     // several domains: each domain's own voltage waveforms (n_dom sets)
     std::vector<std::array<std::unique_ptr<HostWaveform>, ICARION_N_VOLTAGE_SLOTS>> dom_wave;
+    // the gas pressure's waveform (EnvironmentConfig::pressure_Pa_waveform;
+    // null: the domain's constant pressure), and each domain's on several
+    std::unique_ptr<HostWaveform> pressure;
+    std::vector<std::unique_ptr<HostWaveform>> dom_pressure;
     // several domains: each domain's HSS gas mixture (filtered components, the
     // cross sections per species, the species count given)
     std::vector<std::vector<icarion_gas_component>> dom_mix;
@@ -507,9 +511,14 @@ StepParams derive_params(const icarion_domain_desc& d, const icarion_physics_des
 // t + dt/2, t + dt; Boris: t).  Appends n_steps entries to out.
 // stage_times: a field array of several time-scaled terms, whose kernels take
 // the stage times t, t + dt/2, t + dt in place of the RF coefficients.
+// pressure: the domain's pressure waveform (may be null).  The engine evaluates
+// it at the engine time of each step start (update_dynamic_environments,
+// SimulationEngine.cpp:380, SimulationEngineLifecycle.cpp:34-47) and derives
+// n = P(t)/(kB T) (EnvironmentConfig.h:132-152); the entry keeps n over the
+// domain's constant density, by which the kernels scale every rate.
 void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, bool rf, int integrator,
                   double t0, double dt, uint64_t n_steps, bool stage_times = false,
-                  const HostWaveform* const* wf = nullptr) {
+                  const HostWaveform* const* wf = nullptr, const HostWaveform* pressure = nullptr) {
     const size_t base = out.size();
     out.resize(base + n_steps);
     const double r0_sq = d.radius_m * d.radius_m;
@@ -538,11 +547,20 @@ void append_table(std::vector<StepEntry>& out, const icarion_domain_desc& d, boo
         return (U + V * std::cos(om * t)) / r0_sq;
     };
     auto ez_wf = [&](double t) { return ez_of(slot(ICARION_SLOT_DC_AXIAL, d.dc_axial_V, t)); };
+    if (pressure && !(d.gas_density_m3 > 0.0 && d.temperature_K > 0.0))
+        fail(ICARION_ERR_INVALID, "a pressure waveform needs the domain's gas density and temperature > 0");
     double t = t0;
     for (uint64_t k = 0; k < n_steps; ++k) {
         StepEntry& e = out[base + k];
         e.t = t;
-        e.pad = 0.0;
+        e.nrel = 1.0;
+        if (pressure) {
+            const double p = pressure->eval(t);
+            if (!(p > 0.0))   // EnvironmentConfig::update_time_dependent's check (EnvironmentConfig.h:137-139)
+                fail(ICARION_ERR_INVALID, "pressure waveform evaluated to a non-positive value at t = " +
+                                              std::to_string(t));
+            e.nrel = (p / (kBoltzmannHost * d.temperature_K)) / d.gas_density_m3;
+        }
         e.e0 = e.eh = e.e1 = ez_const;
         if (stage_times) {
             e.c0 = t;
@@ -701,37 +719,41 @@ void apply_mixture(const icarion_gpu* h, StepParams& P) {
     }
 }
 
-// Checks and copies the slot descriptors (NULL entries: the constant); true
-// when any slot holds a waveform.
+// Checks and copies one waveform descriptor.
+static std::unique_ptr<HostWaveform> parse_waveform(const icarion_waveform_desc* d) {
+    if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
+        fail(ICARION_ERR_INVALID, "unknown waveform type");
+    if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
+        fail(ICARION_ERR_INVALID, "unknown waveform source");
+    for (double v : d->p)
+        if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
+    auto hw = std::make_unique<HostWaveform>();
+    hw->d = *d;
+    if (d->type == ICARION_WAVEFORM_ARBITRARY) {
+        // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
+        if (!d->times_s || !d->values || d->n_points == 0)
+            fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
+        if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
+            fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
+        hw->ts.assign(d->times_s, d->times_s + d->n_points);
+        hw->vs.assign(d->values, d->values + d->n_points);
+        for (uint32_t i = 0; i < d->n_points; ++i)
+            if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
+                (i && !(hw->ts[i] > hw->ts[i - 1])))
+                fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
+    }
+    hw->d.times_s = nullptr;
+    hw->d.values = nullptr;
+    return hw;
+}
+
+// The voltage slots (NULL entries: the constant); true when any slot holds a
+// waveform.
 static bool parse_waveform_slots(const icarion_waveform_desc* const* slots, std::unique_ptr<HostWaveform>* w) {
     bool any = false;
     for (int k = 0; slots && k < ICARION_N_VOLTAGE_SLOTS; ++k) {
-        const icarion_waveform_desc* d = slots[k];
-        if (!d) continue;
-        if (d->type < ICARION_WAVEFORM_CONSTANT || d->type > ICARION_WAVEFORM_ARBITRARY)
-            fail(ICARION_ERR_INVALID, "unknown waveform type");
-        if (d->source != ICARION_WAVEFORM_LIBRARY && d->source != ICARION_WAVEFORM_INLINE)
-            fail(ICARION_ERR_INVALID, "unknown waveform source");
-        for (double v : d->p)
-            if (!std::isfinite(v)) fail(ICARION_ERR_INVALID, "waveform parameters must be finite");
-        auto hw = std::make_unique<HostWaveform>();
-        hw->d = *d;
-        if (d->type == ICARION_WAVEFORM_ARBITRARY) {
-            // ArbitraryWaveform::evaluate's own checks (WaveformConfig.cpp:11-17)
-            if (!d->times_s || !d->values || d->n_points == 0)
-                fail(ICARION_ERR_INVALID, "arbitrary waveform: empty time/value arrays");
-            if (d->interp < ICARION_INTERP_LINEAR || d->interp > ICARION_INTERP_CUBIC)
-                fail(ICARION_ERR_INVALID, "arbitrary waveform: unknown interpolation");
-            hw->ts.assign(d->times_s, d->times_s + d->n_points);
-            hw->vs.assign(d->values, d->values + d->n_points);
-            for (uint32_t i = 0; i < d->n_points; ++i)
-                if (!std::isfinite(hw->ts[i]) || !std::isfinite(hw->vs[i]) ||
-                    (i && !(hw->ts[i] > hw->ts[i - 1])))
-                    fail(ICARION_ERR_INVALID, "arbitrary waveform: times must increase, values be finite");
-        }
-        hw->d.times_s = nullptr;
-        hw->d.values = nullptr;
-        w[k] = std::move(hw);
+        if (!slots[k]) continue;
+        w[k] = parse_waveform(slots[k]);
         any = true;
     }
     return any;
@@ -777,7 +799,7 @@ void build_table(icarion_gpu* h, double t0, double dt, uint64_t n_steps, StepEnt
     const bool rf = h->base.has_rf != 0 || (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE] &&
                                             h->dom.instrument != ICARION_INSTRUMENT_ORBITRAP);
     append_table(h->table_host, h->dom, rf, h->phys.integrator, t0, dt, n_steps, stage_times,
-                 h->has_waves ? wf : nullptr);
+                 h->has_waves ? wf : nullptr, h->pressure.get());
     // with waveforms, the composite's scales at every stage time go into a
     // table of their own, and the stage values become its row numbers
     h->use_term_scales = stage_times && h->has_waves;
@@ -959,6 +981,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     const int coll = launch_coll_kind(h);
     if (h->has_waves && field != FIELD_ORBITRAP && field != FIELD_LINEAR)
         fail(ICARION_ERR_UNSUPPORTED, "RK45 on a field array runs with constant voltages");
+    if (h->pressure) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with a constant gas pressure");
     if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -1337,6 +1360,7 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     if (h->base.n_sub > 1 && (surrogate || launch_coll_kind(h) == COLL_EHSS))
         fail(ICARION_ERR_UNSUPPORTED, "collision substeps with EHSS run with offline sets, orientation "
                                       "samples, gas mixtures, several domains, RK45 or the batch");
+    if (h->pressure && surrogate) fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with a constant gas pressure");
     set_device(h);
     StepParams P = h->base;
     if (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
@@ -1473,6 +1497,9 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         fail(ICARION_ERR_UNSUPPORTED, "per-domain gas mixtures run with HSS or geometry EHSS collisions");
     if (h->has_waves)
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their waveforms per domain (icarion_gpu_set_domain_waveforms)");
+    if (h->pressure)
+        fail(ICARION_ERR_UNSUPPORTED, "several domains take their pressure waveforms per domain "
+                                      "(icarion_gpu_set_domain_pressure_waveform)");
     bool dom_fields = false;
     for (uint32_t d = 0; d < h->n_dom; ++d) {
         if (!h->doms[d].use_field_array) continue;
@@ -1583,7 +1610,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
             }
             fill_fast_const(P);
             append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps, false,
-                         waves ? wf : nullptr);
+                         waves ? wf : nullptr, h->dom_pressure[d].get());
         }
         t_end = tabs[n_steps - 1].t + dt;
         h->dom_tables.resize(tabs.size());
@@ -2535,6 +2562,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_spans.clear();
         h->dom_has_orbitrap = false;
         h->dom_wave.clear();
+        h->dom_pressure.clear();
         h->dom_mix.clear();
         h->dom_mix_sig.clear();
         h->dom_mix_ns.clear();
@@ -2549,6 +2577,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         if (n_domains == 1) return;
         h->doms.assign(domains, domains + n_domains);
         h->dom_wave.resize(n_domains);
+        h->dom_pressure.resize(n_domains);
         h->dom_mix.resize(n_domains);
         h->dom_mix_sig.resize(n_domains);
         h->dom_mix_ns.assign(n_domains, 0);
@@ -2929,6 +2958,22 @@ int icarion_gpu_set_domain_waveforms(icarion_gpu* h, uint32_t domain, const icar
         std::unique_ptr<HostWaveform> w[ICARION_N_VOLTAGE_SLOTS];
         parse_waveform_slots(slots, w);
         for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) h->dom_wave[domain][k] = std::move(w[k]);
+    });
+}
+
+int icarion_gpu_set_pressure_waveform(icarion_gpu* h, const icarion_waveform_desc* pressure) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        h->pressure = pressure ? parse_waveform(pressure) : nullptr;
+    });
+}
+
+int icarion_gpu_set_domain_pressure_waveform(icarion_gpu* h, uint32_t domain, const icarion_waveform_desc* pressure) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (h->n_dom < 2) fail(ICARION_ERR_STATE, "icarion_gpu_set_domains with two or more domains not called");
+        if (domain >= h->n_dom) fail(ICARION_ERR_INVALID, "domain index out of range");
+        h->dom_pressure[domain] = pressure ? parse_waveform(pressure) : nullptr;
     });
 }
 

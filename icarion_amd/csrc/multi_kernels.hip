@@ -161,21 +161,23 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
         }
         const StepEntry te = D.table[M.k0 + k];
         const double z_before = pos.z;
+        // the domain's gas density at the step start over its constant one
+        const double lp = lpre * te.nrel;
         if constexpr (INTEG == 0) {
             const d3 p0 = pos, v0 = vel;
             const d3 k1v = v0;
-            const d3 k1a = domain_force<COLL, EXT>(D, p0, v0, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 k1a = domain_force<COLL, EXT>(D, p0, v0, te.c0, te.e0, q, m, lp) * inv_m;
             const d3 k2v = v0 + k1a * half_dt;
-            const d3 k2a = domain_force<COLL, EXT>(D, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k2a = domain_force<COLL, EXT>(D, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lp) * inv_m;
             const d3 k3v = v0 + k2a * half_dt;
-            const d3 k3a = domain_force<COLL, EXT>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k3a = domain_force<COLL, EXT>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lp) * inv_m;
             const d3 k4v = v0 + k3a * dt;
-            const d3 k4a = domain_force<COLL, EXT>(D, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lpre) * inv_m;
+            const d3 k4a = domain_force<COLL, EXT>(D, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lp) * inv_m;
             const double h6 = dt / 6.0;
             pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
             vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
         } else {
-            const d3 a = domain_force<COLL, EXT>(D, pos, vel, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 a = domain_force<COLL, EXT>(D, pos, vel, te.c0, te.e0, q, m, lp) * inv_m;
             const d3 vm = vel + a * half_dt;
             d3 vp = vm;
             if (D.magnetic) {
@@ -195,16 +197,18 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
             // collision substeps in the EXT instantiation (one stage otherwise)
             const int ns = EXT ? D.n_sub : 1;
             const double dtc = EXT ? dt / static_cast<double>(D.n_sub) : dt;
+            // rates take the density ratio through dt, the OU kick through gamma
+            const double dtn = COLL == COLL_DAMPING ? dtc : dtc * te.nrel;
             for (int s = 0; s < ns; ++s) {
                 bool hit;
                 if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
-                    hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dtc)
-                                      : collide_f64<COLL_HSS>(D, sp, sidx, m, lpre, vel, rng, dtc);
+                    hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lp, vel, rng, dtn)
+                                      : collide_f64<COLL_HSS>(D, sp, sidx, m, lp, vel, rng, dtn);
                 else if constexpr (COLL == COLL_EHSS_MIX)
-                    hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX>(D, sp, sidx, m, lpre, vel, rng, dtc)
-                                       : collide_f64<COLL_EHSS>(D, sp, sidx, m, lpre, vel, rng, dtc);
+                    hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX>(D, sp, sidx, m, lp, vel, rng, dtn)
+                                       : collide_f64<COLL_EHSS>(D, sp, sidx, m, lp, vel, rng, dtn);
                 else
-                    hit = collide_f64<COLL>(D, sp, sidx, m, lpre, vel, rng, dtc);
+                    hit = collide_f64<COLL>(D, sp, sidx, m, lp, vel, rng, dtn);
                 if (hit) {
                     ++nc;
                     ++ncoll_launch;
@@ -324,21 +328,23 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
         const StepEntry te = D.table[M.k0 + k];
         const float z_before = hi[2];
         const float z_before_lo = lo[2];
+        const float nrel = (float)te.nrel;
+        const float lp = lpre * nrel;
         const f3 p0{hi[0], hi[1], hi[2]};
         const f3 v0{hi[3], hi[4], hi[5]};
         f3 dpos, dvel;
         if constexpr (INTEG == 0) {
-            const f3 k1a = domain_accel_f32<COLL, EXT>(D, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 k1a = domain_accel_f32<COLL, EXT>(D, p0, v0, te.c0, te.e0, qm, lp);
             const f3 k2v = v0 + k1a * half_dt;
-            const f3 k2a = domain_accel_f32<COLL, EXT>(D, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lpre);
+            const f3 k2a = domain_accel_f32<COLL, EXT>(D, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lp);
             const f3 k3v = v0 + k2a * half_dt;
-            const f3 k3a = domain_accel_f32<COLL, EXT>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lpre);
+            const f3 k3a = domain_accel_f32<COLL, EXT>(D, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lp);
             const f3 k4v = v0 + k3a * dt;
-            const f3 k4a = domain_accel_f32<COLL, EXT>(D, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lpre);
+            const f3 k4a = domain_accel_f32<COLL, EXT>(D, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lp);
             dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
             dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
         } else {
-            const f3 a = domain_accel_f32<COLL, EXT>(D, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 a = domain_accel_f32<COLL, EXT>(D, p0, v0, te.c0, te.e0, qm, lp);
             const f3 vm = v0 + a * half_dt;
             f3 rot{0.0f, 0.0f, 0.0f};
             if (D.magnetic) {
@@ -369,13 +375,15 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
             for (int s = 0; s < ns; ++s) {
                 bool hit;
                 if constexpr (COLL == COLL_HSS_MIX)
-                    hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX>(D, D.fc, sp, sidx, m, inv_m_plus, lpre,
-                                                                  kbt2_dt_over_m * sub, v, rng, dtc, D.fc.n_dt * sub)
-                                      : collide_f32<COLL_HSS>(D, D.fc, sp, sidx, m, inv_m_plus, lpre,
-                                                              kbt2_dt_over_m * sub, v, rng, dtc, D.fc.n_dt * sub);
+                    hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX>(D, D.fc, sp, sidx, m, inv_m_plus, lp,
+                                                                  kbt2_dt_over_m * sub, v, rng, dtc * nrel,
+                                                                  D.fc.n_dt * sub * nrel)
+                                      : collide_f32<COLL_HSS>(D, D.fc, sp, sidx, m, inv_m_plus, lp,
+                                                              kbt2_dt_over_m * sub, v, rng, dtc * nrel,
+                                                              D.fc.n_dt * sub * nrel);
                 else
-                    hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m * sub, v, rng,
-                                            dtc, D.fc.n_dt * sub);
+                    hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lp, kbt2_dt_over_m * sub, v, rng,
+                                            dtc * nrel, D.fc.n_dt * sub * nrel);
                 hits += hit ? 1u : 0u;
             }
             if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {

@@ -74,23 +74,27 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
             break;
         }
         const double z_before = pos.z;
+        // the step's gas density (update_dynamic_environments at the step
+        // start, SimulationEngine.cpp:380): the damping coefficient and the
+        // collision rates scale with it
+        const double lp = lpre * te.nrel;
         if constexpr (INTEG == 0) {
             // RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361)
             const d3 p0 = pos, v0 = vel;
             const d3 k1v = v0;
-            const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, te.e0, q, m, lp) * inv_m;
             const d3 k2v = v0 + k1a * half_dt;
-            const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lp) * inv_m;
             const d3 k3v = v0 + k2a * half_dt;
-            const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lpre) * inv_m;
+            const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lp) * inv_m;
             const d3 k4v = v0 + k3a * dt;
-            const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lpre) * inv_m;
+            const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lp) * inv_m;
             const double h6 = dt / 6.0;
             pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
             vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
         } else {
             // BorisStrategy::step (BorisStrategy.cpp:18-98)
-            const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, te.e0, q, m, lpre) * inv_m;
+            const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, te.e0, q, m, lp) * inv_m;
             const d3 vm = vel + a * half_dt;
             d3 vp = vm;
             if (P.magnetic) {
@@ -111,8 +115,11 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
             // instantiation (the loop costs the one-stage kernels registers)
             const int ns = SUB ? P.n_sub : 1;
             const double dtc = SUB ? dt / static_cast<double>(P.n_sub) : dt;
+            // a rate n sigma |v| dt takes the density ratio through dt; the OU
+            // kick's variance 2 gamma kT dt / m through gamma alone
+            const double dtn = COLL == COLL_DAMPING ? dtc : dtc * te.nrel;
             for (int s = 0; s < ns; ++s) {
-                if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, dtc)) {
+                if (collide_f64<COLL>(P, sp, sidx, m, lp, vel, rng, dtn)) {
                     ++nc;
                     ++ncoll_launch;
                 }
@@ -477,7 +484,7 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
             const d3 vrel = vel - vn;
             const double vrel_mag = sqrt(dot(vrel, vrel));
             if (vrel_mag >= kMinRelativeVelocity) {
-                const double Pc = 1.0 - exp(-P.n * sp.sigma_ehss * vrel_mag * dt);
+                const double Pc = 1.0 - exp(-P.n * sp.sigma_ehss * vrel_mag * (dt * te.nrel));
                 hit = rng.uniform_f64() < Pc;
             }
         }
@@ -607,21 +614,24 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
         }
         const float z_before = hi[2];
         const float z_before_lo = lo[2];
+        // the step's gas density over the domain's (scales gamma and the rates)
+        const float nrel = (float)te.nrel;
+        const float lp = lpre * nrel;
         const f3 p0{hi[0], hi[1], hi[2]};
         const f3 v0{hi[3], hi[4], hi[5]};
         f3 dpos, dvel;
         if constexpr (INTEG == 0) {
-            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lp);
             const f3 k2v = v0 + k1a * half_dt;
-            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lpre);
+            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lp);
             const f3 k3v = v0 + k2a * half_dt;
-            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lpre);
+            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lp);
             const f3 k4v = v0 + k3a * dt;
-            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lpre);
+            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lp);
             dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
             dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
         } else {
-            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lpre);
+            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lp);
             const f3 vm = v0 + a * half_dt;
             f3 rot{0.0f, 0.0f, 0.0f};
             if (P.magnetic) {
@@ -653,8 +663,8 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
             const float dtc = SUB ? static_cast<float>(P.dt / static_cast<double>(P.n_sub)) : dt;
             uint32_t hits = 0;
             for (int s = 0; s < ns; ++s)
-                hits += collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m * sub, v, rng, dtc,
-                                          C.n_dt * sub) ? 1u : 0u;
+                hits += collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lp, kbt2_dt_over_m * sub, v, rng,
+                                          dtc * nrel, C.n_dt * sub * nrel) ? 1u : 0u;
             if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
                 if (hits) {
                     // a collision replaces the velocity: restart its compensation

@@ -54,7 +54,10 @@ constexpr int kMaxGasComponents = 4;
 // Per-step uniform values at the step start, mid point and end: for analytic
 // fields the RF coefficient c = U_eff(t)/r0^2 and the axial field E_z(t); for a
 // field array of several time-scaled terms c is the stage time itself (the
-// kernel evaluates the scales).
+// kernel evaluates the scales).  nrel: the gas number density of the step over
+// the domain's constant one (1 unless the pressure follows a waveform); every
+// density-proportional rate of the step (collision rates of each gas
+// component, the damping coefficient) is scaled by it.
 struct StepEntry {
     double t;
     double c0;
@@ -63,7 +66,7 @@ struct StepEntry {
     double e0;
     double eh;
     double e1;
-    double pad;
+    double nrel;
 };
 
 struct IonPtrs {

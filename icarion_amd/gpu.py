@@ -28,6 +28,7 @@ EXPORTED = (
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
     "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
     "icarion_gpu_step_noalloc", "icarion_gpu_set_domain_species", "icarion_gpu_upload_domain_field_array",
+    "icarion_gpu_set_pressure_waveform", "icarion_gpu_set_domain_pressure_waveform",
 )
 
 # icarion_gpu_step_noalloc's per-call step limit (ICARION_NOALLOC_MAX_STEPS)
@@ -416,6 +417,20 @@ class Handle:
             arr[k] = C.pointer(w)
         self._keep.append(dict(slots))
         _check(self.lib.icarion_gpu_set_waveforms(self.h, arr), self.lib)
+
+    def set_pressure_waveform(self, w):
+        """The gas pressure as a waveform (_abi.waveform(...)), evaluated at
+        every step start; None clears."""
+        if w is not None:
+            self._keep.append(w)
+        _check(self.lib.icarion_gpu_set_pressure_waveform(self.h, None if w is None else C.pointer(w)), self.lib)
+
+    def set_domain_pressure_waveform(self, domain, w):
+        """One domain's pressure waveform on a multi-domain handle; None clears."""
+        if w is not None:
+            self._keep.append(w)
+        _check(self.lib.icarion_gpu_set_domain_pressure_waveform(
+            self.h, C.c_uint32(domain), None if w is None else C.pointer(w)), self.lib)
 
     def set_domain_gas_mixture(self, domain, components, sigma):
         """One domain's HSS gas mixture on a multi-domain handle, arguments as

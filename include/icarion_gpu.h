@@ -426,6 +426,21 @@ int icarion_gpu_set_ehss_orientations(icarion_gpu *h, const uint32_t *n_orient, 
  * theirs in icarion_cloud_desc.waveforms. */
 int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const *slots);
 
+/* The gas pressure of the domain as a waveform (EnvironmentConfig::
+ * pressure_Pa_waveform, EnvironmentConfig.h:58; any waveform there counts as
+ * time-varying, a CONSTANT one included).  Replaces
+ * SimulationEngine::update_dynamic_environments (SimulationEngineLifecycle.cpp:
+ * 34-47), which sets P(t) at the engine time t of each step start and derives
+ * n = P(t)/(kB T) and a mixture's n_i = x_i n (EnvironmentConfig.h:132-175):
+ * every rate of that step (each gas component's collision rate, the Langevin
+ * and Friction damping coefficients and with them the OU kicks) scales by
+ * n / gas_density_m3 of the domain descriptor, which must be > 0 (it holds
+ * the constants the rest derive from: the reference's P(0)).  A step whose
+ * P(t) <= 0 returns ICARION_ERR_INVALID, as the reference throws.  Fixed-step
+ * integrators, FP64 and FP32; RK45 and the EHSS surrogate return
+ * ICARION_ERR_UNSUPPORTED with one set.  pressure = NULL clears. */
+int icarion_gpu_set_pressure_waveform(icarion_gpu *h, const icarion_waveform_desc *pressure);
+
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
  * what tools/ehss_offline_precompute.cpp and icarion_gpu_ehss_precompute
  * produce).  Species s gets n_orient[s] orientations (0: it keeps the online
@@ -583,6 +598,14 @@ int icarion_gpu_upload_domain_field_array(icarion_gpu *h, uint32_t domain, uint3
  * stepping returns ICARION_ERR_UNSUPPORTED.  No reference counterpart as a
  * call: it carries the per-domain part of FullConfig::domains. */
 int icarion_gpu_set_domain_waveforms(icarion_gpu *h, uint32_t domain, const icarion_waveform_desc *const *slots);
+
+/* The pressure waveform of one domain of a multi-domain set (each domain's
+ * environment, updated at every step start by update_dynamic_environments,
+ * SimulationEngineLifecycle.cpp:34-47), with the rules of
+ * icarion_gpu_set_pressure_waveform for that domain's gas; NULL clears.
+ * icarion_gpu_set_domains clears every domain's; a handle-level pressure
+ * waveform is not combined with several domains (ICARION_ERR_UNSUPPORTED). */
+int icarion_gpu_set_domain_pressure_waveform(icarion_gpu *h, uint32_t domain, const icarion_waveform_desc *pressure);
 
 /* The HSS gas mixture of one domain of a multi-domain set (that domain's
  * EnvironmentConfig gas_mixture), with the arguments and rules of

@@ -334,10 +334,32 @@ def mt_hss_mixture(seed, domain, mixture, sigma_row, mass, vel, dt):
     return sel, np.array(v[:]), np.array(vn[:])
 
 
+def _with_pressure(field, pressure):
+    """The oracle_field to pass: field (or an empty one) carrying the pressure
+    waveform; field itself when pressure is None."""
+    if pressure is None:
+        return field
+    f = copy_field(field) if field is not None else Waveforms({})
+    f.c.pressure = C.pointer(pressure)
+    f._pressure = pressure
+    return f
+
+
+def copy_field(field):
+    """A shallow copy of a Field / FieldTerms / Waveforms whose oracle_field can
+    be changed (the arrays stay shared)."""
+    import copy
+    g = copy.copy(field)
+    g.c = _abi.OracleField.from_buffer_copy(field.c)
+    return g
+
+
 def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=None, field=None,
-        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None, ehss_orient=None):
-    """Run the oracle engine loop on ens in place; returns StepStats."""
+        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None, ehss_orient=None, pressure=None):
+    """Run the oracle engine loop on ens in place; returns StepStats.
+    pressure: None, or the gas pressure's WaveformDesc."""
     lib = load()
+    field = _with_pressure(field, pressure)
     sp = species_array(species)
     if atoms is not None and len(atoms):
         a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)
@@ -370,13 +392,14 @@ def find_domain(domains, pos, end_apertures=None):
 
 def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
               end_apertures=None, threads=0, waveforms=None, atoms=None, mixtures=None,
-              domain_species=None, fields=None):
+              domain_species=None, fields=None, pressures=None):
     """The engine loop over several domains on ens and domain_index (int32,
     in/out) in place; returns StepStats.  waveforms: None, or per domain None
     or {slot: WaveformDesc}.  domain_species: None, or per domain None or a
     species list (that domain's rows of the species table).  fields: None, or
     per domain None or a Field (the field array of a domain with
-    use_field_array = 1)."""
+    use_field_array = 1).  pressures: None, or per domain None or the
+    domain's pressure WaveformDesc."""
     lib = load()
     arr, ap, app = _domains(domains, end_apertures)
     sp = species_array(species)
@@ -385,19 +408,19 @@ def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, ste
     assert domain_index.dtype == np.int32 and domain_index.flags.c_contiguous
     fptr = C.POINTER(C.POINTER(_abi.OracleField))()
     keep = []
-    if waveforms is not None or fields is not None:
+    if waveforms is not None or fields is not None or pressures is not None:
         fl = (C.POINTER(_abi.OracleField) * len(domains))()
         for k in range(len(domains)):
             slots = waveforms[k] if waveforms is not None else None
             fld = fields[k] if fields is not None else None
+            pw = pressures[k] if pressures is not None else None
             assert not (slots and fld is not None), "a domain takes a field array or waveforms"
+            if fld is None and (slots or pw is not None):
+                fld = Waveforms(slots or {})
             if fld is not None:
+                fld = _with_pressure(fld, pw)
                 keep.append(fld)
                 fl[k] = C.pointer(fld.c)
-            elif slots:
-                wf = Waveforms(slots)
-                keep.append(wf)
-                fl[k] = C.pointer(wf.c)
         fptr = fl
     mptr = C.POINTER(C.POINTER(_abi.OracleGasMixture))()
     if mixtures is not None:

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <random>
 #include <vector>
 
@@ -1178,6 +1179,47 @@ bool ehss_mixture_collide(const icarion_domain_desc& d, const oracle_gas_mixture
 // SimulationEngineLifecycle.cpp:72-195).  The reference runs each phase over
 // all ions before the next; ions do not interact, so running all phases of
 // one ion back to back gives the same per-ion result.
+// update_dynamic_environments (SimulationEngineLifecycle.cpp:34-47) through
+// EnvironmentConfig::update_time_dependent and compute_derived_properties
+// (EnvironmentConfig.h:132-175): at the engine time of each step start the
+// domain's pressure takes its waveform's value, n = P / (kB T), and a gas
+// mixture's components n_i = x_i n, with x_i the fractions the constant state
+// holds (n_i / n of the descriptor).  ok = false: a step's P(t) <= 0, which the
+// reference rejects.
+struct EnvSteps {
+    std::vector<icarion_domain_desc> dom;
+    std::vector<oracle_gas_mixture> mix;
+    std::vector<icarion_gas_component> comps;
+    bool ok = true;
+
+    EnvSteps(const icarion_domain_desc& d, const icarion_waveform_desc& pressure, const oracle_gas_mixture* m,
+             const double* t_table, uint64_t n_steps) {
+        dom.assign(n_steps, d);
+        const uint32_t nc = m ? m->n_components : 0;
+        if (nc) {
+            mix.assign(n_steps, *m);
+            comps.resize(n_steps * nc);
+        }
+        for (uint64_t k = 0; k < n_steps; ++k) {
+            const double P = waveform_value(pressure, t_table[k]);
+            if (!(P > 0.0)) {
+                ok = false;
+                return;
+            }
+            const double n = P / (kBoltzmann * d.temperature_K);
+            dom[k].pressure_Pa = P;
+            dom[k].gas_density_m3 = n;
+            for (uint32_t c = 0; c < nc; ++c) {
+                icarion_gas_component g = m->components[c];
+                const double x = d.gas_density_m3 > 0.0 ? g.density_m3 / d.gas_density_m3 : 0.0;
+                g.density_m3 = x * n;
+                comps[k * nc + c] = g;
+            }
+            if (nc) mix[k].components = &comps[k * nc];
+        }
+    }
+};
+
 struct IonCtx {
     V3 pos, vel;
     double m, q;
@@ -1196,7 +1238,7 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
                  const icarion_species_desc& sp, const double* atoms, const oracle_field* f,
                  IonCtx& ion, const double* t_table, double dt, uint64_t n_steps,
                  uint32_t ion_index, uint64_t step0, MtRng* mt, uint64_t& collisions,
-                 std::vector<V3>& scratch, const IonTables& tb) {
+                 std::vector<V3>& scratch, const IonTables& tb, const EnvSteps* env) {
     uint64_t pushed = 0;
     const bool detect = std::isfinite(d.detector_z_m);
     for (uint64_t k = 0; k < n_steps; ++k) {
@@ -1212,12 +1254,16 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
             ion.death = t;
             continue;
         }
+        // the step's environment (a pressure waveform: its density)
+        const icarion_domain_desc& de = env ? env->dom[k] : d;
+        IonTables tbe = tb;
+        if (env && tb.mix) tbe.mix = &env->mix[k];
         // perform_integration
         const double z_before = ion.pos.z;
         if (ph.integrator == ICARION_INTEGRATOR_BORIS)
-            boris_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+            boris_step(de, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         else
-            rk4_step(d, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
+            rk4_step(de, f, ph.collision_model, sp, ion.m, ion.q, ion.pos, ion.vel, t, dt);
         ++pushed;
         // perform_collisions: handle_collisions_cpu's substep loop
         // (SimulationEngineCollisionRuntime.cpp:84-191), collision_dt =
@@ -1227,8 +1273,8 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
             const double collision_dt = dt / static_cast<double>(ns);
             PhiloxStream prng(ph.rng_seed, ion_index, step0 + k);
             for (int sub = 0; sub < ns; ++sub) {
-                const bool hit = mt ? collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, *mt, scratch, tb)
-                                    : collide(d, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, prng, scratch, tb);
+                const bool hit = mt ? collide(de, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, *mt, scratch, tbe)
+                                    : collide(de, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, prng, scratch, tbe);
                 if (hit) {
                     ++ion.ncoll;
                     ++collisions;
@@ -1269,6 +1315,7 @@ struct Domains {
     const oracle_field* const* fields = nullptr;   // per domain (waveforms), may be NULL
     const oracle_gas_mixture* const* mixtures = nullptr;   // per domain (HSS gas mixture), may be NULL
     const icarion_species_desc* const* species = nullptr;  // per domain (species table), may be NULL
+    std::vector<std::unique_ptr<EnvSteps>> env;            // per domain with a pressure waveform
     std::vector<double> ap_sq;
     std::vector<Span> spans;
     bool has_orbitrap = false;
@@ -1346,7 +1393,9 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
             continue;
         }
         dom_idx = dom;
-        const icarion_domain_desc& d = D.d[dom];
+        // the domain's environment of this step (with a pressure waveform: its density)
+        const EnvSteps* env = D.env.empty() ? nullptr : D.env[dom].get();
+        const icarion_domain_desc& d = env ? env->dom[k] : D.d[dom];
         const oracle_field* f = D.fields ? D.fields[dom] : nullptr;
         // the species row resolved for the step domain's gas: the handlers
         // and DampingForce take the ion's species with the environment of its
@@ -1363,6 +1412,7 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
             // the step domain's gas mixture, if it has one
             IonTables tb;
             const oracle_gas_mixture* gm = D.mixtures ? D.mixtures[dom] : nullptr;
+            if (env && gm && gm->n_components > 0) gm = &env->mix[k];
             if (gm && gm->n_components > 0) {
                 tb.mix = gm;
                 tb.mix_row = gm->sigma_m2 + static_cast<size_t>(sidx) * gm->n_components;
@@ -1658,6 +1708,13 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
         t_table[k] = t;
         t = t + dt;
     }
+    for (uint32_t k = 0; fields && k < n_domains; ++k) {
+        if (!fields[k] || !fields[k]->pressure) continue;
+        D.env.resize(n_domains);
+        D.env[k] = std::make_unique<EnvSteps>(domains[k], *fields[k]->pressure, mixtures ? mixtures[k] : nullptr,
+                                              t_table.data(), n_steps);
+        if (!D.env[k]->ok) return ICARION_ERR_INVALID;
+    }
     uint64_t pushed_total = 0, coll_total = 0, alive = 0, arrived = 0;
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
@@ -1733,6 +1790,12 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
         t_table[k] = t;
         t = t + dt;
     }
+    std::unique_ptr<EnvSteps> env;
+    if (field && field->pressure) {
+        env = std::make_unique<EnvSteps>(*domain, *field->pressure, tables ? tables->mixture : nullptr,
+                                         t_table.data(), n_steps);
+        if (!env->ok) return ICARION_ERR_INVALID;
+    }
     uint64_t pushed_total = 0, coll_total = 0, alive = 0, arrived = 0;
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
@@ -1766,11 +1829,11 @@ int oracle_run(const icarion_domain_desc* domain, const icarion_physics_desc* ph
                 MtRng rng(physics->rng_seed + static_cast<uint64_t>(i));
                 pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
                                  n_steps, static_cast<uint32_t>(i), step_offset, &rng, coll,
-                                 scratch, tb);
+                                 scratch, tb, env.get());
             } else {
                 pushed = run_ion(*domain, *physics, sp, atoms, field, ion, t_table.data(), dt,
                                  n_steps, static_cast<uint32_t>(i), step_offset, nullptr, coll,
-                                 scratch, tb);
+                                 scratch, tb, env.get());
             }
             ions->x[i] = ion.pos.x; ions->y[i] = ion.pos.y; ions->z[i] = ion.pos.z;
             ions->vx[i] = ion.vel.x; ions->vy[i] = ion.vel.y; ions->vz[i] = ion.vel.z;
@@ -1832,6 +1895,7 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
     (void)n_atoms;
     if (!domain || !physics || !cfg || !species || !ions || !rk45 || n_species == 0)
         return ICARION_ERR_INVALID;
+    if (field && field->pressure) return ICARION_ERR_UNSUPPORTED;   // constant pressure only
     const RunTables rt(tables);
     if (!rt.ok(n_species)) return ICARION_ERR_INVALID;
     const icarion_domain_desc& d = *domain;

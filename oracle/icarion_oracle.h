@@ -48,6 +48,10 @@ typedef struct oracle_field {
      * the domain's constant); the grid members are unused when the domain has
      * no field array */
     const icarion_waveform_desc *waveforms[ICARION_N_VOLTAGE_SLOTS];
+    /* the domain's gas pressure as a waveform (EnvironmentConfig::
+     * pressure_Pa_waveform), NULL: the descriptor's constant pressure; the
+     * fixed-step engine loops only */
+    const icarion_waveform_desc *pressure;
 } oracle_field;
 
 /* ValueOrWaveform::evaluate of one waveform (WaveformConfig.h:33-200,

@@ -48,6 +48,9 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
             for k, fld in enumerate(w.get("fields") or []):
                 if fld is not None:
                     h.upload_domain_field_array(k, *fld)
+            for k, pw in enumerate(w.get("pressures") or []):
+                if pw is not None:
+                    h.set_domain_pressure_waveform(k, pw)
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -67,7 +70,7 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
                           fields=[None if f is None else oracle.Field(*f) for f in w["fields"]]
                           if w.get("fields") else None,
                           end_apertures=apertures if apertures is not None else w.get("apertures"),
-                          waveforms=w.get("waveforms"))
+                          waveforms=w.get("waveforms"), pressures=w.get("pressures"))
     return st, di
 
 

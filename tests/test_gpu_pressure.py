@@ -1,0 +1,217 @@
+"""Time-varying gas pressure on the device against the oracle: the step table
+carries each step's density over the domain's constant one (P(t_k) / (kB T) /
+n at the engine time of the step start, update_dynamic_environments,
+SimulationEngineLifecycle.cpp:34-47), and the kernels scale every rate and
+damping coefficient of that step by it.  The oracle's loop is pinned on the
+CPU (tests/test_oracle_pressure.py).
+
+FP64 runs are compared ion by ion (>= 99 % of ions to 1e-9: a collision test
+within rounding of its threshold can flip), with the collision count within
+0.3 %; FP32 + Kahan by statistics; a CONSTANT waveform at the descriptor's
+pressure equals the constant run bit for bit."""
+import math
+
+import numpy as np
+import pytest
+
+from icarion_amd import _abi
+from icarion_amd import workloads as W
+from test_gpu_domains import gpu_multi, oracle_multi
+from test_gpu_parity import compare_ions, gpu_run, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+
+def pulse_and_ramp(P0, dt):
+    """A pressure that rises linearly to 3 P0 over steps 200..600 and drops
+    back to P0 / 2 at step 1200 (a PWM, a LINEAR and a SINUSOIDAL form are
+    covered by the waveform tests of the voltages; the pressure takes the
+    same evaluator)."""
+    return _abi.waveform(_abi.WAVEFORM_ARBITRARY, interp=_abi.INTERP_LINEAR,
+                         points=[(0.0, P0), (200 * dt, P0), (600 * dt, 3 * P0), (1199.5 * dt, 3 * P0),
+                                 (1200.5 * dt, 0.5 * P0), (1e-3, 0.5 * P0)])
+
+
+def falling(P0, dt):
+    """P0 for 200 steps, then a linear fall to 0.3 P0 by step 600.  Langevin
+    damping (a drag of constant magnitude) under the explicit pushes at
+    dt = 1e-10 s turns unstable above about 2 atm: there the oracle's own runs
+    part ion by ion under a 1e-14 change of the pressure, so the Langevin case
+    compares ion by ion at falling pressure."""
+    return _abi.waveform(_abi.WAVEFORM_ARBITRARY, interp=_abi.INTERP_LINEAR,
+                         points=[(0.0, P0), (200 * dt, P0), (600 * dt, 0.3 * P0), (1e-3, 0.3 * P0)])
+
+
+@pytest.mark.parametrize("case", ["hss", "friction_ou", "langevin_boris", "mixture", "ehss"])
+def test_pressure_waveform_fp64_ion_by_ion(oracle, case):
+    if case == "mixture":
+        w = W.ims_mixture_workload(n=2048, seed=91)
+    elif case == "ehss":
+        from test_gpu_parity import ehss_workload
+        w = ehss_workload(n=1024, seed=92)
+    else:
+        w = W.ims_hss_workload(n=2048, seed=93)
+        if case == "friction_ou":
+            w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_FRICTION, ou=1, seed=93)
+            w["species"] = [W.species(K0=2.1, ccs=100e-20)]
+        elif case == "langevin_boris":
+            w["physics"] = W.physics(_abi.INTEGRATOR_BORIS, _abi.COLLISION_LANGEVIN, ou=1, seed=93)
+    steps = 1600 if case != "ehss" else 800
+    wave = falling if case == "langevin_boris" else pulse_and_ramp
+    w["pressure"] = wave(w["domain"].pressure_Pa, w["dt"])
+    print(f"pressure FP64 {case}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, steps)
+    so = oracle_run(oracle, w, o, steps)
+    ok = compare_ions(g, o, rtol=1e-9)
+    print(f"  match {ok.mean():.4f} collisions {sg.collisions}/{so.collisions}", flush=True)
+    assert ok.mean() >= 0.99
+    assert sg.ion_steps == so.ion_steps
+    w0 = dict(w)
+    w0["pressure"] = None
+    g0 = w["ensemble"].copy()
+    s0 = gpu_run(w0, g0, steps)
+    if case in ("hss", "mixture", "ehss"):
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+        # the constant pressure's count differs
+        assert abs(sg.collisions - s0.collisions) > 0.02 * s0.collisions
+    else:
+        # the damped drift follows the density
+        assert abs(g.vz.mean() - g0.vz.mean()) > 0.2 * abs(g0.vz.mean())
+
+
+@pytest.mark.parametrize("precision", [_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN])
+def test_constant_waveform_at_the_descriptor_pressure_is_the_constant_run(precision):
+    w = W.ims_hss_workload(n=4096, seed=94, precision=precision)
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    sa = gpu_run(w, a, 500, precision=precision)
+    w["pressure"] = _abi.waveform(_abi.WAVEFORM_CONSTANT, (w["domain"].pressure_Pa,))
+    sb = gpu_run(w, b, 500, precision=precision)
+    for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions", "active"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    assert sa.collisions == sb.collisions
+
+
+def test_pressure_waveform_fp32_statistics(oracle):
+    w = W.ims_hss_workload(n=16384, seed=95)
+    w["pressure"] = pulse_and_ramp(w["domain"].pressure_Pa, w["dt"])
+    print("pressure FP32 ims_hss", flush=True)
+    steps = 1600
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_run(w, g, steps, precision=_abi.PRECISION_FP32_KAHAN)
+    so = oracle_run(oracle, w, o, steps)
+    assert sg.collisions == pytest.approx(so.collisions, rel=0.01)
+    dz_g, dz_o = g.z - w["ensemble"].z, o.z - w["ensemble"].z
+    se = math.hypot(dz_g.std() / math.sqrt(g.n), dz_o.std() / math.sqrt(o.n))
+    print(f"  drift {dz_g.mean():.4e} vs {dz_o.mean():.4e} (se {se:.1e})", flush=True)
+    assert abs(dz_g.mean() - dz_o.mean()) < 4 * se + 0.01 * abs(dz_o.mean())
+
+
+def test_friction_drift_follows_the_pressure_fp32():
+    """FP32 + Kahan: the Friction drift under a ramp to 2 P0 ends at half the
+    constant pressure's (K proportional to 1 / n)."""
+    K0, T, P0 = 2.0, 300.0, 101325.0
+    w = W.ims_hss_workload(n=64, seed=96, precision=_abi.PRECISION_FP32_KAHAN)
+    w["domain"] = W.ims_domain(axial_V=100.0, gas="N2", pressure_Pa=P0, temperature_K=T)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_FRICTION,
+                             precision=_abi.PRECISION_FP32_KAHAN)
+    w["species"] = [W.species(K0=K0)]
+    w["ensemble"] = W.gaussian_cloud(64, 100.0, 1e-6, temperature_K=0.0, seed=1)
+    w["dt"] = 1e-10
+    a, b = w["ensemble"].copy(), w["ensemble"].copy()
+    gpu_run(w, a, 2000)
+    w["pressure"] = _abi.waveform(_abi.WAVEFORM_LINEAR, (P0, 2 * P0, 20e-9, 100e-9), clamp=1)
+    gpu_run(w, b, 2000)
+    np.testing.assert_allclose(b.vz, 0.5 * a.vz, rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", [_abi.PRECISION_FP64, _abi.PRECISION_FP32_KAHAN])
+def test_several_domains_with_their_own_pressure(oracle, precision):
+    w = W.multi_domain_workload(n=2048 if precision == _abi.PRECISION_FP64 else 16384, seed=97,
+                                collision=_abi.COLLISION_HSS)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, _abi.COLLISION_HSS, precision=precision, seed=97)
+    doms = w["domains"]
+    w["pressures"] = [pulse_and_ramp(doms[0].pressure_Pa, w["dt"]), None,
+                      _abi.waveform(_abi.WAVEFORM_CONSTANT, (0.25 * doms[2].pressure_Pa,))]
+    print(f"pressure domains precision={precision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 3000)
+    so, do = oracle_multi(oracle, w, o, 3000)
+    if precision == _abi.PRECISION_FP64:
+        ok = compare_ions(g, o, rtol=1e-9) & (dg == do)
+        assert ok.mean() >= 0.99
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    else:
+        # FP32 + Kahan against the FP64 oracle: the surviving fraction in each
+        # domain within four binomial standard errors (+0.5 %), the collisions
+        # within 1 %
+        assert sg.collisions == pytest.approx(so.collisions, rel=0.01)
+        for d in (None, 0, 1, 2):
+            sel_g = g.alive if d is None else g.alive & (dg == d)
+            sel_o = o.alive if d is None else o.alive & (do == d)
+            fg, fo = sel_g.mean(), sel_o.mean()
+            err = 4 * np.sqrt(max(fo * (1 - fo), 1e-4) / g.n) + 0.005
+            assert abs(fg - fo) < err, (d, fg, fo)
+
+
+def test_pressure_arguments_and_unsupported():
+    from icarion_amd import gpu
+    w = W.ims_hss_workload(n=64, seed=98)
+    dt = w["dt"]
+    # P(t) <= 0 at a step start: rejected, as the reference throws
+    ramp = _abi.waveform(_abi.WAVEFORM_LINEAR, (1000.0, -1000.0, 0.0, 100 * dt), clamp=1)
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], w["physics"])
+        h.set_species(w["species"])
+        h.set_pressure_waveform(ramp)
+        h.upload_state(w["ensemble"].copy())
+        h.step(0.0, dt, 40)
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(40 * dt, dt, 40, 40)
+        assert ei.value.code == _abi.ERR_INVALID
+        h.set_pressure_waveform(None)
+        h.step(40 * dt, dt, 40, 40)
+    # RK45 keeps a constant pressure
+    with gpu.Handle() as h:
+        h.set_domain(w["domain"], W.physics(_abi.INTEGRATOR_RK45, _abi.COLLISION_HSS, seed=98))
+        h.set_species(w["species"])
+        h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (2e5,)))
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, dt, 4)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+    # a descriptor without a gas density cannot be scaled
+    d0 = type(w["domain"]).from_buffer_copy(w["domain"])
+    d0.pressure_Pa = 0.0
+    d0.gas_density_m3 = 0.0
+    with gpu.Handle() as h:
+        h.set_domain(d0, w["physics"])
+        h.set_species(w["species"])
+        h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (2e5,)))
+        h.upload_state(w["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, dt, 4)
+        assert ei.value.code == _abi.ERR_INVALID
+    # the surrogate kernel runs at the constant pressure
+    we = W.ehss_surrogate_workload(n=256, seed=99)
+    with gpu.Handle() as h:
+        h.set_domain(we["domain"], we["physics"])
+        h.synth_clusters(we["counts"], seed=99)
+        h.surrogate_set_params(np.zeros(_abi.SURROGATE_NPARAMS, np.float32))
+        h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (2e5,)))
+        h.upload_state(we["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, we["dt"], 4)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+    # several domains take theirs per domain
+    wm = W.multi_domain_workload(n=64, seed=100)
+    with gpu.Handle() as h:
+        h.set_domains(wm["domains"], wm["physics"], wm.get("apertures"))
+        h.set_species(wm["species"])
+        h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (2e5,)))
+        h.upload_state(wm["ensemble"].copy())
+        with pytest.raises(gpu.IcarionError) as ei:
+            h.step(0.0, wm["dt"], 4)
+        assert ei.value.code == _abi.ERR_UNSUPPORTED
+        with pytest.raises(gpu.IcarionError):
+            h.set_domain_pressure_waveform(3, None)
