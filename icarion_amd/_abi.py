@@ -20,6 +20,7 @@ MAX_FIELD_TERMS = 4
 (WAVEFORM_CONSTANT, WAVEFORM_LINEAR, WAVEFORM_QUADRATIC, WAVEFORM_EXPONENTIAL, WAVEFORM_SINUSOIDAL,
  WAVEFORM_PWM, WAVEFORM_PULSED, WAVEFORM_ARBITRARY) = range(8)
 INTERP_LINEAR, INTERP_STEP, INTERP_CUBIC = range(3)
+FLOW_CONSTANT, FLOW_AXIAL_PARABOLIC = range(2)
 WAVEFORM_LIBRARY, WAVEFORM_INLINE = range(2)
 SLOT_DC_AXIAL, SLOT_DC_QUAD, SLOT_DC_RADIAL, SLOT_RF_VOLTAGE, SLOT_RF_FREQUENCY = range(5)
 N_VOLTAGE_SLOTS = 5
@@ -214,6 +215,7 @@ class OracleField(C.Structure):
         ("n_terms", _u32), ("reserved0", _u32), ("terms", C.POINTER(OracleFieldTerm)),
         ("waveforms", C.POINTER(WaveformDesc) * 5),
         ("pressure", C.POINTER(WaveformDesc)),
+        ("flow_model", _i32), ("reserved1", _i32), ("flow_max_velocity_m_s", _d),
     ]
 
 

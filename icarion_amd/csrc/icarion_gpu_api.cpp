@@ -223,6 +223,9 @@ struct icarion_gpu {
     // null: the domain's constant pressure), and each domain's on several
     std::unique_ptr<HostWaveform> pressure;
     std::vector<std::unique_ptr<HostWaveform>> dom_pressure;
+    // the gas flow profile (enum icarion_flow_model) and its centre-line speed
+    int flow_model = ICARION_FLOW_CONSTANT;
+    double flow_umax = 0.0;
     // several domains: each domain's HSS gas mixture (filtered components, the
     // cross sections per species, the species count given)
     std::vector<std::vector<icarion_gas_component>> dom_mix;
@@ -982,6 +985,7 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     if (h->has_waves && field != FIELD_ORBITRAP && field != FIELD_LINEAR)
         fail(ICARION_ERR_UNSUPPORTED, "RK45 on a field array runs with constant voltages");
     if (h->pressure) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with a constant gas pressure");
+    if (h->flow_model != ICARION_FLOW_CONSTANT) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with a constant gas flow");
     if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -1361,8 +1365,17 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
         fail(ICARION_ERR_UNSUPPORTED, "collision substeps with EHSS run with offline sets, orientation "
                                       "samples, gas mixtures, several domains, RK45 or the batch");
     if (h->pressure && surrogate) fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with a constant gas pressure");
+    const bool parabolic = h->flow_model == ICARION_FLOW_AXIAL_PARABOLIC;
+    if (parabolic && (surrogate || launch_coll_kind(h) == COLL_EHSS))
+        fail(ICARION_ERR_UNSUPPORTED, "the parabolic gas flow runs with HSS, damping, or EHSS with offline sets, "
+                                      "orientation samples or a gas mixture");
     set_device(h);
     StepParams P = h->base;
+    if (parabolic) {
+        P.flow_parabolic = 1;
+        P.flow_umax = h->flow_umax;
+        P.flow_R2 = h->dom.radius_m > 0.0 ? h->dom.radius_m * h->dom.radius_m : 0.0;
+    }
     if (h->has_waves && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
     P.dt = dt;
     P.species = h->species.p;
@@ -1500,6 +1513,8 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
     if (h->pressure)
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their pressure waveforms per domain "
                                       "(icarion_gpu_set_domain_pressure_waveform)");
+    if (h->flow_model != ICARION_FLOW_CONSTANT)
+        fail(ICARION_ERR_UNSUPPORTED, "several domains run with constant gas flows");
     bool dom_fields = false;
     for (uint32_t d = 0; d < h->n_dom; ++d) {
         if (!h->doms[d].use_field_array) continue;
@@ -2965,6 +2980,17 @@ int icarion_gpu_set_pressure_waveform(icarion_gpu* h, const icarion_waveform_des
     return guarded([&] {
         if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
         h->pressure = pressure ? parse_waveform(pressure) : nullptr;
+    });
+}
+
+int icarion_gpu_set_gas_flow(icarion_gpu* h, int32_t flow_model, double axial_max_velocity_m_s) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (flow_model != ICARION_FLOW_CONSTANT && flow_model != ICARION_FLOW_AXIAL_PARABOLIC)
+            fail(ICARION_ERR_INVALID, "unknown flow model");
+        if (!std::isfinite(axial_max_velocity_m_s)) fail(ICARION_ERR_INVALID, "the flow velocity must be finite");
+        h->flow_model = flow_model;
+        h->flow_umax = flow_model == ICARION_FLOW_CONSTANT ? 0.0 : axial_max_velocity_m_s;
     });
 }
 

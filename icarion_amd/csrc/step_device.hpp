@@ -180,14 +180,32 @@ __device__ __forceinline__ double gamma_f64(const StepParams& P, double lpre, d3
     return lpre;
 }
 
-template <int FIELD, int COLL>
+// EnvironmentConfig::gas_velocity_at (EnvironmentConfig.h:96-129) at the
+// global position p: the constant gas velocity, or with the AxialParabolic
+// flow l2g (0, 0, u_max max(0, 1 - r^2 / R^2)), r the radial distance of
+// g2l (p - origin) (u_max everywhere when the domain has no radius).
+__device__ __forceinline__ d3 gas_velocity_f64(const StepParams& P, d3 p) {
+    if (!P.flow_parabolic) return d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+    double uz = P.flow_umax;
+    if (P.flow_R2 > 0.0) {
+        const double dx = p.x - P.origin[0], dy = p.y - P.origin[1], dz = p.z - P.origin[2];
+        const double lx = P.g2l[0] * dx + P.g2l[1] * dy + P.g2l[2] * dz;
+        const double ly = P.g2l[3] * dx + P.g2l[4] * dy + P.g2l[5] * dz;
+        uz = P.flow_umax * fmax(0.0, 1.0 - (lx * lx + ly * ly) / P.flow_R2);
+    }
+    return d3{P.l2g[2] * uz, P.l2g[5] * uz, P.l2g[8] * uz};
+}
+
+// FLOW: the gas velocity at p (gas_velocity_f64), else the constant one.
+template <int FIELD, int COLL, bool FLOW = false>
 __device__ __forceinline__ d3 force_f64(const StepParams& P, d3 p, d3 v, double c_rf, double ez,
                                         double q, double m, double lpre) {
     d3 F = field_f64<FIELD>(P, p, c_rf, ez) * q;
     if constexpr (COLL == COLL_DAMPING) {
         const double g = gamma_f64(P, lpre, v);
         if (g > 0.0) {
-            const d3 rel{v.x - P.u_gas[0], v.y - P.u_gas[1], v.z - P.u_gas[2]};
+            const d3 u = FLOW ? gas_velocity_f64(P, p) : d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+            const d3 rel{v.x - u.x, v.y - u.y, v.z - u.z};
             F = F + rel * (-g * m);
         }
     }
@@ -329,10 +347,14 @@ __device__ d3 ehss_f64(const StepParams& P, d3 v_ion, d3 v_n, double mi, double 
     return ehss_core_f64(P, v_ion, v_n, mi, mn, r_gas, a0, nat, O, 0.0, kEhssMaxAttempts, rng);
 }
 
-// One collision stage; returns true on a collision event.
-template <int COLL>
+// One collision stage; returns true on a collision event.  FLOW: the neutrals
+// move with ug (the gas velocity at the ion's position), else with the
+// constant gas velocity.
+template <int COLL, bool FLOW = false>
 __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDev& sp, uint32_t sidx,
-                                            double m, double lpre, d3& v, PhiloxStream& rng, double dt) {
+                                            double m, double lpre, d3& v, PhiloxStream& rng, double dt,
+                                            d3 ug = d3{0.0, 0.0, 0.0}) {
+    const d3 u_gas = FLOW ? ug : d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
     if constexpr (COLL == COLL_EHSS_MIX) {
         // EHSSCollisionHandler::handle_collision mixture path
         // (EHSSCollisionHandler.cpp:311-424): every component visited in order,
@@ -357,7 +379,7 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
                 vn[c].y = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
                 u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
                 vn[c].z = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-                vn[c] = vn[c] + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+                vn[c] = vn[c] + u_gas;
                 const d3 vr = v - vn[c];
                 const double vm = sqrt(dot(vr, vr));
                 if (sg[c] > 0.0 && P.emix_n[c] > 0.0 && vm >= kMinRelativeVelocity) {
@@ -420,7 +442,7 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
                     vn[c].y = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
                     u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
                     vn[c].z = sn * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-                    vn[c] = vn[c] + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+                    vn[c] = vn[c] + u_gas;
                     const d3 vr = v - vn[c];
                     const double vm = sqrt(dot(vr, vr));
                     if (vm >= kMinRelativeVelocity) {
@@ -507,7 +529,7 @@ __device__ __forceinline__ bool collide_f64(const StepParams& P, const SpeciesDe
             vn.y = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
             u1 = rng.uniform_f64(); u2 = rng.uniform_f64();
             vn.z = P.sigma_neutral * sqrt(-2.0 * log(u1)) * cos(2.0 * kPi * u2);
-            vn = vn + d3{P.u_gas[0], P.u_gas[1], P.u_gas[2]};
+            vn = vn + u_gas;
         }
         const d3 vrel = v - vn;
         const double vrel_mag = sqrt(dot(vrel, vrel));

@@ -168,11 +168,15 @@ __device__ __forceinline__ f3 iso_dir_f32(PhiloxStream& rng) {
     return f3{sinT * fast_cos_rev(u2), sinT * fast_sin_rev(u2), cosT};
 }
 
-template <int COLL>
+// FLOW: the neutrals move with ug (the gas velocity at the ion's position),
+// else with the constant gas velocity.
+template <int COLL, bool FLOW = false>
 __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst& C,
                                             const SpeciesDev& sp, uint32_t sidx, float m,
                                             float inv_m_plus, float lpre, float kbt2_dt_over_m, f3& v,
-                                            PhiloxStream& rng, float dt, float n_dt) {
+                                            PhiloxStream& rng, float dt, float n_dt,
+                                            f3 ug = f3{0.0f, 0.0f, 0.0f}) {
+    const f3 u_gas = FLOW ? ug : f3{C.u_gas[0], C.u_gas[1], C.u_gas[2]};
     if constexpr (COLL == COLL_HSS_MIX) {
         {
             // the mixture path of collide_f64 in FP32 (rates, test and partner
@@ -191,8 +195,8 @@ __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst
                     const float u3 = rng.uniform_f32(), u4 = rng.uniform_f32();
                     const float r1 = sn * fast_sqrt(-2.0f * fast_ln(u1));
                     const float r2 = sn * fast_sqrt(-2.0f * fast_ln(u3));
-                    vn[c] = f3{r1 * fast_cos_rev(u2) + C.u_gas[0], r1 * fast_sin_rev(u2) + C.u_gas[1],
-                               r2 * fast_cos_rev(u4) + C.u_gas[2]};
+                    vn[c] = f3{r1 * fast_cos_rev(u2) + u_gas.x, r1 * fast_sin_rev(u2) + u_gas.y,
+                               r2 * fast_cos_rev(u4) + u_gas.z};
                     const f3 vr = v - vn[c];
                     const float vm = fast_sqrt(dot(vr, vr));
                     if (vm >= (float)kMinRelativeVelocity) {
@@ -243,8 +247,8 @@ __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst
         const float u3 = rng.uniform_f32(), u4 = rng.uniform_f32();
         const float r1 = C.sigma_neutral * fast_sqrt(-2.0f * fast_ln(u1));
         const float r2 = C.sigma_neutral * fast_sqrt(-2.0f * fast_ln(u3));
-        const f3 vn{r1 * fast_cos_rev(u2) + C.u_gas[0], r1 * fast_sin_rev(u2) + C.u_gas[1],
-                    r2 * fast_cos_rev(u4) + C.u_gas[2]};
+        const f3 vn{r1 * fast_cos_rev(u2) + u_gas.x, r1 * fast_sin_rev(u2) + u_gas.y,
+                    r2 * fast_cos_rev(u4) + u_gas.z};
         const f3 vrel = v - vn;
         const float vm2 = dot(vrel, vrel);
         const float vm = fast_sqrt(vm2);
@@ -277,13 +281,31 @@ __device__ __forceinline__ bool collide_f32(const StepParams& P, const FastConst
     }
 }
 
-template <int FIELD, int COLL>
+// gas_velocity_f64 in FP32: the constant gas velocity, or the AxialParabolic
+// profile at p.
+__device__ __forceinline__ f3 gas_velocity_f32(const StepParams& P, const FastConst& C, f3 p) {
+    if (!P.flow_parabolic) return f3{C.u_gas[0], C.u_gas[1], C.u_gas[2]};
+    float uz = C.flow_umax;
+    if (C.flow_R2 > 0.0f) {
+        const float dx = p.x - C.ox, dy = p.y - C.oy, dz = p.z - C.oz;
+        const float lx = C.g2l[0] * dx + C.g2l[1] * dy + C.g2l[2] * dz;
+        const float ly = C.g2l[3] * dx + C.g2l[4] * dy + C.g2l[5] * dz;
+        uz = C.flow_umax * fmaxf(0.0f, 1.0f - (lx * lx + ly * ly) / C.flow_R2);
+    }
+    return f3{C.l2g[2] * uz, C.l2g[5] * uz, C.l2g[8] * uz};
+}
+
+// FLOW: the gas velocity at p (gas_velocity_f32), else the constant one.
+template <int FIELD, int COLL, bool FLOW = false>
 __device__ __forceinline__ f3 accel_f32(const StepParams& P, const FastConst& C, f3 p, f3 v,
                                         double c, double ez, float qm, float lpre) {
     f3 a = field_f32<FIELD>(P, C, p, c, ez) * qm;
     if constexpr (COLL == COLL_DAMPING) {
         const float g = (P.damping_model == 3) ? lpre / fmaxf(1e-6f, fast_sqrt(dot(v, v))) : lpre;
-        if (g > 0.0f) a = a - f3{v.x - C.u_gas[0], v.y - C.u_gas[1], v.z - C.u_gas[2]} * g;
+        if (g > 0.0f) {
+            const f3 u = FLOW ? gas_velocity_f32(P, C, p) : f3{C.u_gas[0], C.u_gas[1], C.u_gas[2]};
+            a = a - f3{v.x - u.x, v.y - u.y, v.z - u.z} * g;
+        }
     }
     return a;
 }

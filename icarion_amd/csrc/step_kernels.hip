@@ -19,7 +19,10 @@
 
 namespace icarion {
 
-template <int INTEG, int FIELD, int COLL, bool SUB = false>
+// SUB: collision substeps; FLOW: the AxialParabolic gas flow (both read from P
+// at run time, instantiated apart because they cost the plain kernels
+// registers).
+template <int INTEG, int FIELD, int COLL, bool SUB = false, bool FLOW = false>
 __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, const uint32_t i,
         uint32_t& pushed, uint32_t& ncoll_launch) {
     d3 pos{S.x[i], S.y[i], S.z[i]};
@@ -82,19 +85,19 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
             // RK4Strategy::step_batch_with_stage_refresh (RK4Strategy.cpp:181-361)
             const d3 p0 = pos, v0 = vel;
             const d3 k1v = v0;
-            const d3 k1a = force_f64<FIELD, COLL>(P, p0, v0, te.c0, te.e0, q, m, lp) * inv_m;
+            const d3 k1a = force_f64<FIELD, COLL, FLOW>(P, p0, v0, te.c0, te.e0, q, m, lp) * inv_m;
             const d3 k2v = v0 + k1a * half_dt;
-            const d3 k2a = force_f64<FIELD, COLL>(P, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lp) * inv_m;
+            const d3 k2a = force_f64<FIELD, COLL, FLOW>(P, p0 + k1v * half_dt, k2v, te.ch, te.eh, q, m, lp) * inv_m;
             const d3 k3v = v0 + k2a * half_dt;
-            const d3 k3a = force_f64<FIELD, COLL>(P, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lp) * inv_m;
+            const d3 k3a = force_f64<FIELD, COLL, FLOW>(P, p0 + k2v * half_dt, k3v, te.ch, te.eh, q, m, lp) * inv_m;
             const d3 k4v = v0 + k3a * dt;
-            const d3 k4a = force_f64<FIELD, COLL>(P, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lp) * inv_m;
+            const d3 k4a = force_f64<FIELD, COLL, FLOW>(P, p0 + k3v * dt, k4v, te.c1, te.e1, q, m, lp) * inv_m;
             const double h6 = dt / 6.0;
             pos = p0 + (k1v + k2v * 2.0 + k3v * 2.0 + k4v) * h6;
             vel = v0 + (k1a + k2a * 2.0 + k3a * 2.0 + k4a) * h6;
         } else {
             // BorisStrategy::step (BorisStrategy.cpp:18-98)
-            const d3 a = force_f64<FIELD, COLL>(P, pos, vel, te.c0, te.e0, q, m, lp) * inv_m;
+            const d3 a = force_f64<FIELD, COLL, FLOW>(P, pos, vel, te.c0, te.e0, q, m, lp) * inv_m;
             const d3 vm = vel + a * half_dt;
             d3 vp = vm;
             if (P.magnetic) {
@@ -118,8 +121,11 @@ __device__ __forceinline__ void f64_ion(const StepParams& P, const IonPtrs& S, c
             // a rate n sigma |v| dt takes the density ratio through dt; the OU
             // kick's variance 2 gamma kT dt / m through gamma alone
             const double dtn = COLL == COLL_DAMPING ? dtc : dtc * te.nrel;
+            // the gas velocity at the pushed position, for every substep
+            // (SimulationEngineCollisionRuntime.cpp:122-130)
+            const d3 ug = FLOW ? gas_velocity_f64(P, pos) : d3{0.0, 0.0, 0.0};
             for (int s = 0; s < ns; ++s) {
-                if (collide_f64<COLL>(P, sp, sidx, m, lp, vel, rng, dtn)) {
+                if (collide_f64<COLL, FLOW>(P, sp, sidx, m, lp, vel, rng, dtn, ug)) {
                     ++nc;
                     ++ncoll_launch;
                 }
@@ -154,7 +160,7 @@ template <int INTEG, int FIELD, int COLL, bool SUB = false>
 __global__ void __launch_bounds__(256) step_f64_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL, SUB>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
+    if (i < P.n_ions) f64_ion<INTEG, FIELD, COLL, SUB, SUB>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -545,7 +551,7 @@ __global__ void __launch_bounds__(256) step_f64_ehss_kernel(const StepParams P, 
 
 // the FP32 helpers (fields, containment, collisions) are in step_device_f32.hpp
 
-template <int INTEG, int FIELD, int COLL, bool SUB = false>
+template <int INTEG, int FIELD, int COLL, bool SUB = false, bool FLOW = false>
 __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, const uint32_t i,
         uint32_t& pushed, uint32_t& ncoll_launch) {
     const FastConst& C = P.fc;
@@ -621,17 +627,17 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
         const f3 v0{hi[3], hi[4], hi[5]};
         f3 dpos, dvel;
         if constexpr (INTEG == 0) {
-            const f3 k1a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lp);
+            const f3 k1a = accel_f32<FIELD, COLL, FLOW>(P, C, p0, v0, te.c0, te.e0, qm, lp);
             const f3 k2v = v0 + k1a * half_dt;
-            const f3 k2a = accel_f32<FIELD, COLL>(P, C, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lp);
+            const f3 k2a = accel_f32<FIELD, COLL, FLOW>(P, C, p0 + v0 * half_dt, k2v, te.ch, te.eh, qm, lp);
             const f3 k3v = v0 + k2a * half_dt;
-            const f3 k3a = accel_f32<FIELD, COLL>(P, C, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lp);
+            const f3 k3a = accel_f32<FIELD, COLL, FLOW>(P, C, p0 + k2v * half_dt, k3v, te.ch, te.eh, qm, lp);
             const f3 k4v = v0 + k3a * dt;
-            const f3 k4a = accel_f32<FIELD, COLL>(P, C, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lp);
+            const f3 k4a = accel_f32<FIELD, COLL, FLOW>(P, C, p0 + k3v * dt, k4v, te.c1, te.e1, qm, lp);
             dpos = (v0 + k2v * 2.0f + k3v * 2.0f + k4v) * h6;
             dvel = (k1a + k2a * 2.0f + k3a * 2.0f + k4a) * h6;
         } else {
-            const f3 a = accel_f32<FIELD, COLL>(P, C, p0, v0, te.c0, te.e0, qm, lp);
+            const f3 a = accel_f32<FIELD, COLL, FLOW>(P, C, p0, v0, te.c0, te.e0, qm, lp);
             const f3 vm = v0 + a * half_dt;
             f3 rot{0.0f, 0.0f, 0.0f};
             if (P.magnetic) {
@@ -662,9 +668,10 @@ __device__ __forceinline__ void f32_ion(const StepParams& P, const IonPtrs& S, c
             const float sub = SUB ? 1.0f / static_cast<float>(P.n_sub) : 1.0f;
             const float dtc = SUB ? static_cast<float>(P.dt / static_cast<double>(P.n_sub)) : dt;
             uint32_t hits = 0;
+            const f3 ug = FLOW ? gas_velocity_f32(P, C, f3{hi[0], hi[1], hi[2]}) : f3{0.0f, 0.0f, 0.0f};
             for (int s = 0; s < ns; ++s)
-                hits += collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lp, kbt2_dt_over_m * sub, v, rng,
-                                          dtc * nrel, C.n_dt * sub * nrel) ? 1u : 0u;
+                hits += collide_f32<COLL, FLOW>(P, C, sp, sidx, m, inv_m_plus, lp, kbt2_dt_over_m * sub, v, rng,
+                                                dtc * nrel, C.n_dt * sub * nrel, ug) ? 1u : 0u;
             if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
                 if (hits) {
                     // a collision replaces the velocity: restart its compensation
@@ -714,7 +721,7 @@ template <int INTEG, int FIELD, int COLL, bool SUB>
 __global__ void __launch_bounds__(256) step_f32_kernel(const StepParams P, const IonPtrs S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pushed = 0, ncoll_launch = 0;
-    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL, SUB>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
+    if (i < P.n_ions) f32_ion<INTEG, FIELD, COLL, SUB, SUB>(P, S, P.order ? P.order[i] : i, pushed, ncoll_launch);
     add_counters(P, pushed, ncoll_launch);
 }
 
@@ -1323,8 +1330,9 @@ hipError_t launch_step(int integrator, int field, int coll, int precision, int s
     if (surrogate) {
         fn = integrator == 1 ? pick_surrogate<1>(field) : pick_surrogate<0>(field);
     } else {
-        fn = integrator == 1 ? pick_field<1>(field, coll, precision, P.n_sub > 1)
-                             : pick_field<0>(field, coll, precision, P.n_sub > 1);
+        // the extended instantiation: collision substeps or a parabolic flow
+        const bool ext = P.n_sub > 1 || P.flow_parabolic;
+        fn = integrator == 1 ? pick_field<1>(field, coll, precision, ext) : pick_field<0>(field, coll, precision, ext);
     }
     if (!fn) return hipErrorInvalidValue;
     // the surrogate kernel runs its own workgroup size (the handle's block size

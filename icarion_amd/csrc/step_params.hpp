@@ -113,6 +113,7 @@ struct FastConst {
     float Ez, orbi_hk, orbi_k, R_char_sq;
     float ox, oy, oz;
     float u_gas[3];
+    float flow_umax, flow_R2;   // AxialParabolic flow (StepParams.flow_*)
     float sigma_neutral;
     float n_dt;          // n * dt
     float m_gas;
@@ -151,6 +152,12 @@ struct StepParams {
     double r_gas;
     double alpha_gas;
     double u_gas[3];
+    // EnvironmentConfig's AxialParabolic flow (flow_parabolic = 1): the gas
+    // velocity at p is l2g (0, 0, flow_umax max(0, 1 - r^2 / flow_R2)), r the
+    // local radial distance; flow_R2 = 0: flow_umax everywhere.  Read by the
+    // extended (SUB) instantiations only.
+    int flow_parabolic;
+    double flow_umax, flow_R2;
     double sigma_neutral;    // sqrt(kB T / m_gas)
     double v_th;             // sqrt(8 kB T / (pi m_gas))
     double det_z;
@@ -239,6 +246,8 @@ inline void fill_fast_const(StepParams& P) {
     C.R_char_sq = (float)P.R_char_sq;
     C.ox = (float)P.origin[0]; C.oy = (float)P.origin[1]; C.oz = (float)P.origin[2];
     C.u_gas[0] = (float)P.u_gas[0]; C.u_gas[1] = (float)P.u_gas[1]; C.u_gas[2] = (float)P.u_gas[2];
+    C.flow_umax = (float)P.flow_umax;
+    C.flow_R2 = (float)P.flow_R2;
     C.sigma_neutral = (float)P.sigma_neutral;
     C.n_dt = (float)(P.n * P.dt);
     C.m_gas = (float)P.m_gas;

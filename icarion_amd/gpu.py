@@ -28,7 +28,7 @@ EXPORTED = (
     "icarion_gpu_set_ehss_orientations", "icarion_gpu_set_gas_radii", "icarion_gpu_arrival_stats",
     "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
     "icarion_gpu_step_noalloc", "icarion_gpu_set_domain_species", "icarion_gpu_upload_domain_field_array",
-    "icarion_gpu_set_pressure_waveform", "icarion_gpu_set_domain_pressure_waveform",
+    "icarion_gpu_set_pressure_waveform", "icarion_gpu_set_domain_pressure_waveform", "icarion_gpu_set_gas_flow",
 )
 
 # icarion_gpu_step_noalloc's per-call step limit (ICARION_NOALLOC_MAX_STEPS)
@@ -424,6 +424,12 @@ class Handle:
         if w is not None:
             self._keep.append(w)
         _check(self.lib.icarion_gpu_set_pressure_waveform(self.h, None if w is None else C.pointer(w)), self.lib)
+
+    def set_gas_flow(self, model, axial_max_velocity_m_s=0.0):
+        """The gas flow profile (_abi.FLOW_CONSTANT clears, _abi.FLOW_AXIAL_PARABOLIC
+        with its centre-line speed)."""
+        _check(self.lib.icarion_gpu_set_gas_flow(self.h, C.c_int32(model), C.c_double(axial_max_velocity_m_s)),
+               self.lib)
 
     def set_domain_pressure_waveform(self, domain, w):
         """One domain's pressure waveform on a multi-domain handle; None clears."""

@@ -299,11 +299,35 @@ inline int flow_kind(const std::string& name) {
 
 // Whether the device step covers the domain's flow model: Constant, and
 // AxialUniform, whose gas velocity gas_velocity_at (EnvironmentConfig.h:97-129)
-// gives as the constant R_l2g (0, 0, u) everywhere.  AxialParabolic depends on
-// the ion's radius and stays on the CPU path.
-inline bool flow_covered(const Environment& e) { return flow_kind(e.flow_model) != 2; }
+// gives as the constant R_l2g (0, 0, u) everywhere; AxialParabolic on a
+// single-domain handle (icarion_gpu_set_gas_flow with gas_flow(e)), while
+// several domains keep it on the CPU path.
+inline bool flow_covered(const Environment& e, bool single_domain = true) {
+    return flow_kind(e.flow_model) != 2 || single_domain;
+}
 
-inline icarion_domain_desc to_desc(const Domain& d, double detector_z_m = NAN) {
+// The flow profile icarion_gpu_set_gas_flow takes for the environment:
+// AxialParabolic with axial_flow_max_velocity_m_s, else the constant model
+// (AxialUniform goes into the descriptor's gas velocity, see to_desc).
+struct GasFlow {
+    int32_t model = ICARION_FLOW_CONSTANT;
+    double axial_max_velocity_m_s = 0.0;
+};
+
+inline GasFlow gas_flow(const Environment& e) {
+    GasFlow f;
+    if (flow_kind(e.flow_model) == 2) {
+        f.model = ICARION_FLOW_AXIAL_PARABOLIC;
+        f.axial_max_velocity_m_s = e.axial_flow_max_velocity_m_s;
+    }
+    return f;
+}
+
+// gas_flow_applied: the caller passes gas_flow(d.env) to
+// icarion_gpu_set_gas_flow, so an AxialParabolic environment is taken (the
+// descriptor's gas velocity is then not read and holds 0); without it such an
+// environment throws Unsupported.
+inline icarion_domain_desc to_desc(const Domain& d, double detector_z_m = NAN, bool gas_flow_applied = false) {
     icarion_domain_desc o{};
     o.instrument = instrument_code(d.instrument);
     o.use_field_array = d.fields.field_array ? 1 : 0;
@@ -339,9 +363,12 @@ inline icarion_domain_desc to_desc(const Domain& d, double detector_z_m = NAN) {
     o.gas_radius_m = e.gas_radius_m;
     o.gas_polarizability_m3 = e.gas_polarizability_m3;
     const int flow = flow_kind(e.flow_model);
-    if (flow == 2)
-        throw Unsupported("domain '" + d.name + "': the axial parabolic flow model is not covered by the device path");
-    if (flow == 1) {
+    if (flow == 2 && !gas_flow_applied)
+        throw Unsupported("domain '" + d.name + "': the axial parabolic flow model needs icarion_gpu_set_gas_flow "
+                          "(to_desc with gas_flow_applied)");
+    if (flow == 2) {
+        for (int k = 0; k < 3; ++k) o.gas_velocity_m_s[k] = 0.0;
+    } else if (flow == 1) {
         // gas_velocity_at's AxialUniform branch: to_global_vel((0, 0, u)), the
         // local-to-global rotation's third column times u (the collision
         // handlers on the engine's CPU path and DampingForce both use it)

@@ -441,6 +441,30 @@ int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const
  * ICARION_ERR_UNSUPPORTED with one set.  pressure = NULL clears. */
 int icarion_gpu_set_pressure_waveform(icarion_gpu *h, const icarion_waveform_desc *pressure);
 
+/* Gas-flow profiles of EnvironmentConfig (flow_model, EnvironmentConfig.h:22-26,
+ * 60-62).  CONSTANT and AxialUniform are a constant velocity, which the domain
+ * descriptor's gas_velocity_m_s carries (AxialUniform as R_l2g (0, 0, u)). */
+enum icarion_flow_model {
+    ICARION_FLOW_CONSTANT = 0,          /* the descriptor's gas_velocity_m_s    */
+    ICARION_FLOW_AXIAL_PARABOLIC = 1    /* Poiseuille profile, below            */
+};
+
+/* The domain's gas flow as EnvironmentConfig::gas_velocity_at gives it
+ * (EnvironmentConfig.h:96-129), evaluated at each ion's own position: for
+ * ICARION_FLOW_AXIAL_PARABOLIC, u = R_l2g (0, 0, u_max max(0, 1 - r^2 / R^2))
+ * with r the radial distance in the domain's local frame (R_g2l (p - origin))
+ * and R the descriptor's radius_m (radius_m <= 0: u_max everywhere).  The
+ * engine reads it in the damping force at every force evaluation
+ * (DampingForce.cpp:83-88, stage positions) and for the neutral velocity of
+ * every collision substep at the ion's pushed position
+ * (SimulationEngineCollisionRuntime.cpp:122-130).  Fixed-step integrators,
+ * FP64 and FP32, HSS (single gas and mixture), EHSS with offline sets,
+ * orientation samples or a mixture, Langevin / Friction; RK45, the
+ * wave-cooperative geometry EHSS kernel, the EHSS surrogate and several
+ * domains return ICARION_ERR_UNSUPPORTED with it.  ICARION_FLOW_CONSTANT
+ * clears. */
+int icarion_gpu_set_gas_flow(icarion_gpu *h, int32_t flow_model, double axial_max_velocity_m_s);
+
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
  * what tools/ehss_offline_precompute.cpp and icarion_gpu_ehss_precompute
  * produce).  Species s gets n_orient[s] orientations (0: it keeps the online

@@ -88,6 +88,11 @@ public:
         if (!radius_m.empty())
             check(icarion_gpu_set_gas_radii(h_, radius_m.data(), static_cast<uint32_t>(radius_m.size())));
     }
+    // The gas flow profile (icarion_gpu_set_gas_flow): ICARION_FLOW_CONSTANT
+    // clears, ICARION_FLOW_AXIAL_PARABOLIC with its centre-line speed.
+    void set_gas_flow(int32_t flow_model, double axial_max_velocity_m_s) {
+        check(icarion_gpu_set_gas_flow(h_, flow_model, axial_max_velocity_m_s));
+    }
     // Several domains (icarion_gpu_set_domains): one descriptor and end aperture
     // per DomainConfig, in the config's order.
     // domain_species: empty, or per domain its species rows (the same species

@@ -57,6 +57,7 @@ def load():
     lib.oracle_interpolate_field.argtypes = [P(_abi.OracleField), P(d), P(d)]
     lib.oracle_waveform_value.argtypes = [P(_abi.WaveformDesc), d]
     lib.oracle_waveform_value.restype = d
+    lib.oracle_gas_velocity.argtypes = [P(_abi.DomainDesc), P(_abi.OracleField), P(d), P(d)]
     lib.oracle_field_term_scale.argtypes = [P(_abi.DomainDesc), P(_abi.OracleFieldTerm),
                                             P(_abi.OracleField), d]
     lib.oracle_field_term_scale.restype = d
@@ -188,6 +189,14 @@ class Waveforms:
 
 def waveform_value(w, t):
     return load().oracle_waveform_value(C.byref(w), t)
+
+
+def gas_velocity(domain, pos, flow=None):
+    """gas_velocity_at of the domain at pos; flow: None or (model, max velocity)."""
+    f = _with_flow(None, flow)
+    out = (C.c_double * 3)()
+    load().oracle_gas_velocity(C.byref(domain), C.byref(f.c) if f is not None else None, vec(*pos), out)
+    return list(out)
 
 
 def field_term_scale(domain, term, t, field=None):
@@ -354,12 +363,24 @@ def copy_field(field):
     return g
 
 
+def _with_flow(field, flow):
+    """field (or an empty one) carrying the gas flow (model, max velocity);
+    field itself when flow is None."""
+    if flow is None:
+        return field
+    f = copy_field(field) if field is not None else Waveforms({})
+    f.c.flow_model, f.c.flow_max_velocity_m_s = int(flow[0]), float(flow[1])
+    return f
+
+
 def run(domain, physics, species, ens, t0, dt, n_steps, step_offset=0, atoms=None, field=None,
-        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None, ehss_orient=None, pressure=None):
+        rng_kind=RNG_PHILOX, threads=0, mixture=None, ehss_offline=None, ehss_orient=None, pressure=None,
+        flow=None):
     """Run the oracle engine loop on ens in place; returns StepStats.
-    pressure: None, or the gas pressure's WaveformDesc."""
+    pressure: None, or the gas pressure's WaveformDesc.  flow: None, or
+    (icarion_flow_model, axial max velocity [m/s])."""
     lib = load()
-    field = _with_pressure(field, pressure)
+    field = _with_flow(_with_pressure(field, pressure), flow)
     sp = species_array(species)
     if atoms is not None and len(atoms):
         a = np.ascontiguousarray(atoms, dtype=np.float64).reshape(-1, 4)

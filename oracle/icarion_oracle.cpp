@@ -536,15 +536,41 @@ double damping_gamma(const icarion_domain_desc& d, int model, const icarion_spec
     return 0.0;
 }
 
+// EnvironmentConfig::gas_velocity_at (EnvironmentConfig.h:96-129) with the
+// domain's geometry and rotations, as the engine passes them
+// (SimulationEngineCollisionRuntime.cpp:93-101, 122-130; DampingForce.cpp:
+// 70-88): Constant (and AxialUniform, given as a constant) the descriptor's
+// gas velocity; AxialParabolic R_l2g (0, 0, u_max max(0, 1 - r^2/R^2)) with r
+// from R_g2l (pos - origin), u_max everywhere when radius_m <= 0.
+V3 gas_velocity_at(const icarion_domain_desc& d, const oracle_field* f, V3 pos) {
+    if (!f || f->flow_model != ICARION_FLOW_AXIAL_PARABOLIC)
+        return V3{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+    const double u_max = f->flow_max_velocity_m_s;
+    double uz = u_max;
+    if (d.radius_m > 0.0) {
+        const V3 rel{pos.x - d.origin_m[0], pos.y - d.origin_m[1], pos.z - d.origin_m[2]};
+        const double* G = d.rot_g2l;
+        const V3 local{G[0] * rel.x + G[1] * rel.y + G[2] * rel.z, G[3] * rel.x + G[4] * rel.y + G[5] * rel.z,
+                       G[6] * rel.x + G[7] * rel.y + G[8] * rel.z};
+        const double r2 = local.x * local.x + local.y * local.y;
+        const double R2 = d.radius_m * d.radius_m;
+        const double rr = R2 > 0.0 ? r2 / R2 : 0.0;
+        uz = u_max * std::max(0.0, 1.0 - rr);
+    }
+    const double* L = d.rot_l2g;
+    return V3{L[0] * 0.0 + L[1] * 0.0 + L[2] * uz, L[3] * 0.0 + L[4] * 0.0 + L[5] * uz,
+              L[6] * 0.0 + L[7] * 0.0 + L[8] * uz};
+}
+
 // ForceRegistry::compute_total_force_soa (ForceRegistry.cpp:42-88):
-// ElectricFieldForce (F = E q) + DampingForce (F = (v - u) * (-gamma m)).
+// ElectricFieldForce (F = E q) + DampingForce (F = (v - u(pos)) * (-gamma m)).
 V3 total_force(const icarion_domain_desc& d, const oracle_field* f, int coll,
                const icarion_species_desc& sp, double m, double q, V3 pos, V3 vel, double t) {
     V3 F = field_E(d, f, pos, t) * q;
     if (coll == ICARION_COLLISION_LANGEVIN || coll == ICARION_COLLISION_FRICTION) {
         const double gamma = damping_gamma(d, coll, sp, m, q, vel);
         if (gamma > 0.0) {
-            const V3 u{d.gas_velocity_m_s[0], d.gas_velocity_m_s[1], d.gas_velocity_m_s[2]};
+            const V3 u = gas_velocity_at(d, f, pos);
             F = F + (vel - u) * (-gamma * m);
         }
     }
@@ -1272,9 +1298,21 @@ uint64_t run_ion(const icarion_domain_desc& d, const icarion_physics_desc& ph,
             const int ns = collision_substeps(ph);
             const double collision_dt = dt / static_cast<double>(ns);
             PhiloxStream prng(ph.rng_seed, ion_index, step0 + k);
+            // a non-constant flow: the environment takes the gas velocity at the
+            // ion's pushed position (SimulationEngineCollisionRuntime.cpp:122-130)
+            icarion_domain_desc dflow;
+            const bool flow = f && f->flow_model != ICARION_FLOW_CONSTANT;
+            if (flow) {
+                dflow = de;
+                const V3 u = gas_velocity_at(de, f, ion.pos);
+                dflow.gas_velocity_m_s[0] = u.x;
+                dflow.gas_velocity_m_s[1] = u.y;
+                dflow.gas_velocity_m_s[2] = u.z;
+            }
+            const icarion_domain_desc& dc = flow ? dflow : de;
             for (int sub = 0; sub < ns; ++sub) {
-                const bool hit = mt ? collide(de, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, *mt, scratch, tbe)
-                                    : collide(de, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, prng, scratch, tbe);
+                const bool hit = mt ? collide(dc, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, *mt, scratch, tbe)
+                                    : collide(dc, ph, sp, atoms, ion.m, ion.q, ion.vel, collision_dt, prng, scratch, tbe);
                 if (hit) {
                     ++ion.ncoll;
                     ++collisions;
@@ -1539,6 +1577,14 @@ double oracle_orbitrap_r_for_z(double z, double R, double R_m) { return orbitrap
 
 double oracle_waveform_value(const icarion_waveform_desc* w, double t) { return waveform_value(*w, t); }
 
+void oracle_gas_velocity(const icarion_domain_desc* domain, const oracle_field* field, const double pos[3],
+                         double out[3]) {
+    const V3 u = gas_velocity_at(*domain, field, V3{pos[0], pos[1], pos[2]});
+    out[0] = u.x;
+    out[1] = u.y;
+    out[2] = u.z;
+}
+
 double oracle_field_term_scale(const icarion_domain_desc* domain, const oracle_field_term* term,
                                const oracle_field* field, double t) {
     return term_scale(*domain, *term, t, field);
@@ -1708,6 +1754,8 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
         t_table[k] = t;
         t = t + dt;
     }
+    for (uint32_t k = 0; fields && k < n_domains; ++k)
+        if (fields[k] && fields[k]->flow_model != ICARION_FLOW_CONSTANT) return ICARION_ERR_UNSUPPORTED;
     for (uint32_t k = 0; fields && k < n_domains; ++k) {
         if (!fields[k] || !fields[k]->pressure) continue;
         D.env.resize(n_domains);
@@ -1895,7 +1943,8 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
     (void)n_atoms;
     if (!domain || !physics || !cfg || !species || !ions || !rk45 || n_species == 0)
         return ICARION_ERR_INVALID;
-    if (field && field->pressure) return ICARION_ERR_UNSUPPORTED;   // constant pressure only
+    // constant pressure and flow only
+    if (field && (field->pressure || field->flow_model != ICARION_FLOW_CONSTANT)) return ICARION_ERR_UNSUPPORTED;
     const RunTables rt(tables);
     if (!rt.ok(n_species)) return ICARION_ERR_INVALID;
     const icarion_domain_desc& d = *domain;

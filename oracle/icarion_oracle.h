@@ -52,11 +52,21 @@ typedef struct oracle_field {
      * pressure_Pa_waveform), NULL: the descriptor's constant pressure; the
      * fixed-step engine loops only */
     const icarion_waveform_desc *pressure;
+    /* the domain's gas flow (enum icarion_flow_model) and the AxialParabolic
+     * centre-line speed; the fixed-step single-domain loop only */
+    int32_t flow_model;
+    int32_t reserved1;
+    double flow_max_velocity_m_s;
 } oracle_field;
 
 /* ValueOrWaveform::evaluate of one waveform (WaveformConfig.h:33-200,
  * WaveformConfig.cpp:10-91). */
 double oracle_waveform_value(const icarion_waveform_desc *w, double t);
+
+/* EnvironmentConfig::gas_velocity_at (EnvironmentConfig.h:96-129) of the
+ * domain at the global position pos, with field's flow (NULL: constant). */
+void oracle_gas_velocity(const icarion_domain_desc *domain, const oracle_field *field, const double pos[3],
+                         double out[3]);
 
 /* DomainManager::find_domain_index (DomainManager.cpp:66-109) over n_domains
  * domains with their end apertures (NULL: the radii). */

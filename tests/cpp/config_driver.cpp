@@ -183,10 +183,18 @@ int main(int argc, char** argv) {
         show("constant");
         std::cout << ", ";
         d.env.flow_model = "poiseuille";
+        d.env.axial_flow_max_velocity_m_s = 250.0;
         show("parabolic");
-        std::cout << ", \"covered\": [" << C::flow_covered(d.env);
+        const icarion_domain_desc pa = C::to_desc(d, NAN, true);
+        const C::GasFlow gf = C::gas_flow(d.env);
+        std::cout << ", \"parabolic_applied\": [" << pa.gas_velocity_m_s[0] << ", " << pa.gas_velocity_m_s[1]
+                  << ", " << pa.gas_velocity_m_s[2] << "], \"parabolic_flow\": [" << gf.model << ", "
+                  << gf.axial_max_velocity_m_s << "]";
+        std::cout << ", \"covered\": [" << C::flow_covered(d.env) << ", " << C::flow_covered(d.env, false);
         d.env.flow_model = "axial_const";
-        std::cout << ", " << C::flow_covered(d.env) << "]}\n";
+        const C::GasFlow gu = C::gas_flow(d.env);
+        std::cout << ", " << C::flow_covered(d.env) << ", " << C::flow_covered(d.env, false) << "], \"uniform_flow\": ["
+                  << gu.model << ", " << gu.axial_max_velocity_m_s << "]}\n";
         return 0;
     }
     if (argc < 4) {

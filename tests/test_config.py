@@ -222,12 +222,16 @@ def test_flow_models(driver):
     u = 100 m/s with local-to-global columns (0,0,-1), (0,1,0), (1,0,0)), whose
     collision handler receives (100, 0, 0) from gas_velocity_at; unrotated, the
     flow is (0, 0, u); the constant model passes gas_velocity_m_s through; the
-    axial parabolic model (radius-dependent) is declined, and flow_covered says
-    which models the device step takes."""
+    axial parabolic model (radius-dependent) is declined unless the caller
+    applies gas_flow() through icarion_gpu_set_gas_flow, and flow_covered says
+    which models the device step takes (the parabolic one on a single domain)."""
     out = json.loads(subprocess.run([driver, "--flow"], check=True, capture_output=True,
                                     text=True).stdout)
     assert out["uniform_rotated"] == pytest.approx([100.0, 0.0, 0.0], abs=1e-12)
     assert out["uniform"] == pytest.approx([0.0, 0.0, 134.0], abs=1e-12)
     assert out["constant"] == [1.0, 2.0, 3.0]
     assert out["parabolic"] == "unsupported"
-    assert out["covered"] == [0, 1]
+    assert out["parabolic_applied"] == [0.0, 0.0, 0.0]
+    assert out["parabolic_flow"] == [1, 250.0]
+    assert out["covered"] == [1, 0, 1, 1]
+    assert out["uniform_flow"] == [0, 0.0]

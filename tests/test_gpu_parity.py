@@ -49,6 +49,8 @@ def gpu_run(w, ens, n_steps, t0=0.0, step_offset=0, precision=None, steps_per_la
             h.set_waveforms(w["waveforms"])
         if w.get("pressure") is not None:
             h.set_pressure_waveform(w["pressure"])
+        if w.get("flow") is not None:
+            h.set_gas_flow(*w["flow"])
         h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -69,7 +71,7 @@ def oracle_run(oracle, w, ens, n_steps, t0=0.0, step_offset=0):
     ori = oracle.EhssOrientations(w["ehss_orient"]) if w.get("ehss_orient") is not None else None
     return oracle.run(w["domain"], w["physics"], w["species"], ens, t0, w["dt"], n_steps,
                       step_offset=step_offset, atoms=w.get("atoms"), field=field, mixture=mixture,
-                      ehss_offline=off, ehss_orient=ori, pressure=w.get("pressure"))
+                      ehss_offline=off, ehss_orient=ori, pressure=w.get("pressure"), flow=w.get("flow"))
 
 
 def rel_close(a, b, rtol, scale):
