@@ -223,6 +223,7 @@ struct icarion_gpu {
     // null: the domain's constant pressure), and each domain's on several
     std::unique_ptr<HostWaveform> pressure;
     std::vector<std::unique_ptr<HostWaveform>> dom_pressure;
+    std::vector<std::pair<int, double>> dom_flow;   // several domains: (flow model, u_max) each
     // the gas flow profile (enum icarion_flow_model) and its centre-line speed
     int flow_model = ICARION_FLOW_CONSTANT;
     double flow_umax = 0.0;
@@ -1514,8 +1515,9 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         fail(ICARION_ERR_UNSUPPORTED, "several domains take their pressure waveforms per domain "
                                       "(icarion_gpu_set_domain_pressure_waveform)");
     if (h->flow_model != ICARION_FLOW_CONSTANT)
-        fail(ICARION_ERR_UNSUPPORTED, "several domains run with constant gas flows");
+        fail(ICARION_ERR_UNSUPPORTED, "several domains take their gas flows per domain (icarion_gpu_set_domain_gas_flow)");
     bool dom_fields = false;
+    bool any_flow = false;
     for (uint32_t d = 0; d < h->n_dom; ++d) {
         if (!h->doms[d].use_field_array) continue;
         if (!h->dom_field[d].have)
@@ -1623,6 +1625,13 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
                 P.fa_f32 = F.fa32.p;
                 P.fa_f64 = h->phys.precision == ICARION_PRECISION_FP64 ? F.fa64.p : nullptr;
             }
+            // the domain's AxialParabolic gas flow (EnvironmentConfig::gas_velocity_at)
+            if (h->dom_flow[d].first == ICARION_FLOW_AXIAL_PARABOLIC) {
+                P.flow_parabolic = 1;
+                P.flow_umax = h->dom_flow[d].second;
+                P.flow_R2 = h->doms[d].radius_m > 0.0 ? h->doms[d].radius_m * h->doms[d].radius_m : 0.0;
+                any_flow = true;
+            }
             fill_fast_const(P);
             append_table(tabs, h->doms[d], P.has_rf != 0, h->phys.integrator, t0, dt, n_steps, false,
                          waves ? wf : nullptr, h->dom_pressure[d].get());
@@ -1655,6 +1664,7 @@ void do_step_multi(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint6
         M.n_ions = static_cast<uint32_t>(h->n);
         M.n_sub = Ps[0].n_sub;
         M.has_field_array = dom_fields ? 1 : 0;
+        M.has_flow = any_flow ? 1 : 0;
         const IonPtrs S = ion_ptrs(h);
         const uint64_t spl = h->steps_per_launch ? h->steps_per_launch : 2000;
         const int integ = h->phys.integrator == ICARION_INTEGRATOR_BORIS ? 1 : 0;
@@ -2578,6 +2588,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->dom_has_orbitrap = false;
         h->dom_wave.clear();
         h->dom_pressure.clear();
+        h->dom_flow.clear();
         h->dom_mix.clear();
         h->dom_mix_sig.clear();
         h->dom_mix_ns.clear();
@@ -2593,6 +2604,7 @@ int icarion_gpu_set_domains(icarion_gpu* h, const icarion_domain_desc* domains, 
         h->doms.assign(domains, domains + n_domains);
         h->dom_wave.resize(n_domains);
         h->dom_pressure.resize(n_domains);
+        h->dom_flow.assign(n_domains, {ICARION_FLOW_CONSTANT, 0.0});
         h->dom_mix.resize(n_domains);
         h->dom_mix_sig.resize(n_domains);
         h->dom_mix_ns.assign(n_domains, 0);
@@ -2991,6 +3003,19 @@ int icarion_gpu_set_gas_flow(icarion_gpu* h, int32_t flow_model, double axial_ma
         if (!std::isfinite(axial_max_velocity_m_s)) fail(ICARION_ERR_INVALID, "the flow velocity must be finite");
         h->flow_model = flow_model;
         h->flow_umax = flow_model == ICARION_FLOW_CONSTANT ? 0.0 : axial_max_velocity_m_s;
+    });
+}
+
+int icarion_gpu_set_domain_gas_flow(icarion_gpu* h, uint32_t domain, int32_t flow_model,
+                                    double axial_max_velocity_m_s) {
+    return guarded([&] {
+        if (!h) fail(ICARION_ERR_INVALID, "NULL handle");
+        if (h->n_dom < 2) fail(ICARION_ERR_STATE, "icarion_gpu_set_domains with two or more domains not called");
+        if (domain >= h->n_dom) fail(ICARION_ERR_INVALID, "domain index out of range");
+        if (flow_model != ICARION_FLOW_CONSTANT && flow_model != ICARION_FLOW_AXIAL_PARABOLIC)
+            fail(ICARION_ERR_INVALID, "unknown flow model");
+        if (!std::isfinite(axial_max_velocity_m_s)) fail(ICARION_ERR_INVALID, "the flow velocity must be finite");
+        h->dom_flow[domain] = {flow_model, flow_model == ICARION_FLOW_CONSTANT ? 0.0 : axial_max_velocity_m_s};
     });
 }
 

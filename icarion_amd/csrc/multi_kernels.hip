@@ -92,15 +92,16 @@ __device__ __forceinline__ double damping_pre_f64(const StepParams& D, const Spe
 }
 
 // EXT: the instantiation that also covers domains with a uniform field array
-// (GridFieldProvider, one unscaled term) and collision substeps
+// (GridFieldProvider, one unscaled term), collision substeps and the
+// AxialParabolic gas flow (each domain's own, D.flow_parabolic)
 template <int COLL, bool EXT>
 __device__ __forceinline__ d3 domain_force(const StepParams& D, d3 p, d3 v, double c, double ez, double q, double m,
                                            double lpre) {
     if constexpr (EXT) {
-        if (D.field_array) return force_f64<FIELD_ARRAY, COLL>(D, p, v, c, ez, q, m, lpre);
+        if (D.field_array) return force_f64<FIELD_ARRAY, COLL, EXT>(D, p, v, c, ez, q, m, lpre);
     }
-    if (D.instrument == 3) return force_f64<FIELD_ORBITRAP, COLL>(D, p, v, c, ez, q, m, lpre);
-    return force_f64<FIELD_LINEAR, COLL>(D, p, v, c, ez, q, m, lpre);
+    if (D.instrument == 3) return force_f64<FIELD_ORBITRAP, COLL, EXT>(D, p, v, c, ez, q, m, lpre);
+    return force_f64<FIELD_LINEAR, COLL, EXT>(D, p, v, c, ez, q, m, lpre);
 }
 
 template <int INTEG, int COLL, bool EXT>
@@ -199,16 +200,18 @@ __device__ __forceinline__ void multi_ion(const MultiParams& M, const IonPtrs& S
             const double dtc = EXT ? dt / static_cast<double>(D.n_sub) : dt;
             // rates take the density ratio through dt, the OU kick through gamma
             const double dtn = COLL == COLL_DAMPING ? dtc : dtc * te.nrel;
+            // the step domain's gas velocity at the pushed position
+            const d3 ug = EXT ? gas_velocity_f64(D, pos) : d3{0.0, 0.0, 0.0};
             for (int s = 0; s < ns; ++s) {
                 bool hit;
                 if constexpr (COLL == COLL_HSS_MIX)   // per domain: its mixture, else its single gas
-                    hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX>(D, sp, sidx, m, lp, vel, rng, dtn)
-                                      : collide_f64<COLL_HSS>(D, sp, sidx, m, lp, vel, rng, dtn);
+                    hit = D.n_mix > 0 ? collide_f64<COLL_HSS_MIX, EXT>(D, sp, sidx, m, lp, vel, rng, dtn, ug)
+                                      : collide_f64<COLL_HSS, EXT>(D, sp, sidx, m, lp, vel, rng, dtn, ug);
                 else if constexpr (COLL == COLL_EHSS_MIX)
-                    hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX>(D, sp, sidx, m, lp, vel, rng, dtn)
-                                       : collide_f64<COLL_EHSS>(D, sp, sidx, m, lp, vel, rng, dtn);
+                    hit = D.n_emix > 0 ? collide_f64<COLL_EHSS_MIX, EXT>(D, sp, sidx, m, lp, vel, rng, dtn, ug)
+                                       : collide_f64<COLL_EHSS, EXT>(D, sp, sidx, m, lp, vel, rng, dtn, ug);
                 else
-                    hit = collide_f64<COLL>(D, sp, sidx, m, lp, vel, rng, dtn);
+                    hit = collide_f64<COLL, EXT>(D, sp, sidx, m, lp, vel, rng, dtn, ug);
                 if (hit) {
                     ++nc;
                     ++ncoll_launch;
@@ -254,10 +257,10 @@ template <int COLL, bool EXT>
 __device__ __forceinline__ f3 domain_accel_f32(const StepParams& D, f3 p, f3 v, double c, double ez, float qm,
                                                float lpre) {
     if constexpr (EXT) {
-        if (D.field_array) return accel_f32<FIELD_ARRAY, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
+        if (D.field_array) return accel_f32<FIELD_ARRAY, COLL, EXT>(D, D.fc, p, v, c, ez, qm, lpre);
     }
-    if (D.instrument == 3) return accel_f32<FIELD_ORBITRAP, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
-    return accel_f32<FIELD_LINEAR, COLL>(D, D.fc, p, v, c, ez, qm, lpre);
+    if (D.instrument == 3) return accel_f32<FIELD_ORBITRAP, COLL, EXT>(D, D.fc, p, v, c, ez, qm, lpre);
+    return accel_f32<FIELD_LINEAR, COLL, EXT>(D, D.fc, p, v, c, ez, qm, lpre);
 }
 
 // The FP32 + Kahan body of the single-domain fast path over several domains:
@@ -372,18 +375,19 @@ __device__ __forceinline__ void multi_ion_f32(const MultiParams& M, const IonPtr
             const float sub = EXT ? 1.0f / static_cast<float>(D.n_sub) : 1.0f;
             const float dtc = EXT ? static_cast<float>(M.dt / static_cast<double>(D.n_sub)) : dt;
             uint32_t hits = 0;
+            const f3 ug = EXT ? gas_velocity_f32(D, D.fc, f3{hi[0], hi[1], hi[2]}) : f3{0.0f, 0.0f, 0.0f};
             for (int s = 0; s < ns; ++s) {
                 bool hit;
                 if constexpr (COLL == COLL_HSS_MIX)
-                    hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX>(D, D.fc, sp, sidx, m, inv_m_plus, lp,
-                                                                  kbt2_dt_over_m * sub, v, rng, dtc * nrel,
-                                                                  D.fc.n_dt * sub * nrel)
-                                      : collide_f32<COLL_HSS>(D, D.fc, sp, sidx, m, inv_m_plus, lp,
-                                                              kbt2_dt_over_m * sub, v, rng, dtc * nrel,
-                                                              D.fc.n_dt * sub * nrel);
+                    hit = D.n_mix > 0 ? collide_f32<COLL_HSS_MIX, EXT>(D, D.fc, sp, sidx, m, inv_m_plus, lp,
+                                                                       kbt2_dt_over_m * sub, v, rng, dtc * nrel,
+                                                                       D.fc.n_dt * sub * nrel, ug)
+                                      : collide_f32<COLL_HSS, EXT>(D, D.fc, sp, sidx, m, inv_m_plus, lp,
+                                                                   kbt2_dt_over_m * sub, v, rng, dtc * nrel,
+                                                                   D.fc.n_dt * sub * nrel, ug);
                 else
-                    hit = collide_f32<COLL>(D, D.fc, sp, sidx, m, inv_m_plus, lp, kbt2_dt_over_m * sub, v, rng,
-                                            dtc * nrel, D.fc.n_dt * sub * nrel);
+                    hit = collide_f32<COLL, EXT>(D, D.fc, sp, sidx, m, inv_m_plus, lp, kbt2_dt_over_m * sub, v,
+                                                 rng, dtc * nrel, D.fc.n_dt * sub * nrel, ug);
                 hits += hit ? 1u : 0u;
             }
             if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
@@ -504,7 +508,7 @@ hipError_t launch_multi_step(int integrator, int coll, int precision, const Mult
                              uint32_t block, hipStream_t s) {
     if (M.n_ions == 0) return hipSuccess;
     const bool f32 = precision == 1;
-    const bool ext = M.n_sub > 1 || M.has_field_array;
+    const bool ext = M.n_sub > 1 || M.has_field_array || M.has_flow;
     if (integrator == 0) return f32 ? launch_multi_integ<0, true>(coll, ext, M, S, block, s)
                                     : launch_multi_integ<0, false>(coll, ext, M, S, block, s);
     if (integrator == 1) return f32 ? launch_multi_integ<1, true>(coll, ext, M, S, block, s)

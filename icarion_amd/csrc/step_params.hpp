@@ -289,7 +289,7 @@ struct MultiParams {
     uint32_t k0, n_steps, n_ions;
     int n_sub;                      // collision substeps per step (every domain's n_sub)
     int has_field_array;            // some domain takes E from a field array
-    int pad;
+    int has_flow;                   // some domain has the AxialParabolic gas flow
 };
 
 // ---- batched clouds ----------------------------------------------------------

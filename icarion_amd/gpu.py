@@ -29,6 +29,7 @@ EXPORTED = (
     "icarion_gpu_arrival_histogram", "icarion_gpu_set_domain_waveforms", "icarion_gpu_set_domain_gas_mixture",
     "icarion_gpu_step_noalloc", "icarion_gpu_set_domain_species", "icarion_gpu_upload_domain_field_array",
     "icarion_gpu_set_pressure_waveform", "icarion_gpu_set_domain_pressure_waveform", "icarion_gpu_set_gas_flow",
+    "icarion_gpu_set_domain_gas_flow",
 )
 
 # icarion_gpu_step_noalloc's per-call step limit (ICARION_NOALLOC_MAX_STEPS)
@@ -430,6 +431,11 @@ class Handle:
         with its centre-line speed)."""
         _check(self.lib.icarion_gpu_set_gas_flow(self.h, C.c_int32(model), C.c_double(axial_max_velocity_m_s)),
                self.lib)
+
+    def set_domain_gas_flow(self, domain, model, axial_max_velocity_m_s=0.0):
+        """One domain's gas flow on a multi-domain handle (as set_gas_flow)."""
+        _check(self.lib.icarion_gpu_set_domain_gas_flow(self.h, C.c_uint32(domain), C.c_int32(model),
+                                                        C.c_double(axial_max_velocity_m_s)), self.lib)
 
     def set_domain_pressure_waveform(self, domain, w):
         """One domain's pressure waveform on a multi-domain handle; None clears."""

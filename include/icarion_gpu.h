@@ -460,9 +460,9 @@ enum icarion_flow_model {
  * (SimulationEngineCollisionRuntime.cpp:122-130).  Fixed-step integrators,
  * FP64 and FP32, HSS (single gas and mixture), EHSS with offline sets,
  * orientation samples or a mixture, Langevin / Friction; RK45, the
- * wave-cooperative geometry EHSS kernel, the EHSS surrogate and several
- * domains return ICARION_ERR_UNSUPPORTED with it.  ICARION_FLOW_CONSTANT
- * clears. */
+ * wave-cooperative geometry EHSS kernel and the EHSS surrogate return
+ * ICARION_ERR_UNSUPPORTED with it (several domains take theirs with
+ * icarion_gpu_set_domain_gas_flow).  ICARION_FLOW_CONSTANT clears. */
 int icarion_gpu_set_gas_flow(icarion_gpu *h, int32_t flow_model, double axial_max_velocity_m_s);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;
@@ -630,6 +630,15 @@ int icarion_gpu_set_domain_waveforms(icarion_gpu *h, uint32_t domain, const icar
  * icarion_gpu_set_domains clears every domain's; a handle-level pressure
  * waveform is not combined with several domains (ICARION_ERR_UNSUPPORTED). */
 int icarion_gpu_set_domain_pressure_waveform(icarion_gpu *h, uint32_t domain, const icarion_waveform_desc *pressure);
+
+/* The gas flow of one domain of a multi-domain set (its EnvironmentConfig's
+ * flow_model, read with that domain's geometry and rotations as the engine
+ * passes them, SimulationEngineCollisionRuntime.cpp:93-101), with the rules of
+ * icarion_gpu_set_gas_flow: an ion takes its step domain's flow.
+ * icarion_gpu_set_domains clears every domain's; a handle-level flow is not
+ * combined with several domains (ICARION_ERR_UNSUPPORTED). */
+int icarion_gpu_set_domain_gas_flow(icarion_gpu *h, uint32_t domain, int32_t flow_model,
+                                    double axial_max_velocity_m_s);
 
 /* The HSS gas mixture of one domain of a multi-domain set (that domain's
  * EnvironmentConfig gas_mixture), with the arguments and rules of

@@ -413,14 +413,15 @@ def find_domain(domains, pos, end_apertures=None):
 
 def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, step_offset=0,
               end_apertures=None, threads=0, waveforms=None, atoms=None, mixtures=None,
-              domain_species=None, fields=None, pressures=None):
+              domain_species=None, fields=None, pressures=None, flows=None):
     """The engine loop over several domains on ens and domain_index (int32,
     in/out) in place; returns StepStats.  waveforms: None, or per domain None
     or {slot: WaveformDesc}.  domain_species: None, or per domain None or a
     species list (that domain's rows of the species table).  fields: None, or
     per domain None or a Field (the field array of a domain with
     use_field_array = 1).  pressures: None, or per domain None or the
-    domain's pressure WaveformDesc."""
+    domain's pressure WaveformDesc.  flows: None, or per domain None or
+    (icarion_flow_model, axial max velocity)."""
     lib = load()
     arr, ap, app = _domains(domains, end_apertures)
     sp = species_array(species)
@@ -429,17 +430,18 @@ def run_multi(domains, physics, species, ens, domain_index, t0, dt, n_steps, ste
     assert domain_index.dtype == np.int32 and domain_index.flags.c_contiguous
     fptr = C.POINTER(C.POINTER(_abi.OracleField))()
     keep = []
-    if waveforms is not None or fields is not None or pressures is not None:
+    if waveforms is not None or fields is not None or pressures is not None or flows is not None:
         fl = (C.POINTER(_abi.OracleField) * len(domains))()
         for k in range(len(domains)):
             slots = waveforms[k] if waveforms is not None else None
             fld = fields[k] if fields is not None else None
             pw = pressures[k] if pressures is not None else None
+            fw = flows[k] if flows is not None else None
             assert not (slots and fld is not None), "a domain takes a field array or waveforms"
-            if fld is None and (slots or pw is not None):
+            if fld is None and (slots or pw is not None or fw is not None):
                 fld = Waveforms(slots or {})
             if fld is not None:
-                fld = _with_pressure(fld, pw)
+                fld = _with_flow(_with_pressure(fld, pw), fw)
                 keep.append(fld)
                 fl[k] = C.pointer(fld.c)
         fptr = fl

@@ -1458,8 +1458,20 @@ uint64_t run_ion_multi(const Domains& D, const icarion_physics_desc& ph, const i
             PhiloxStream rng(ph.rng_seed, ion_index, step0 + k);
             const int ns = collision_substeps(ph);
             const double collision_dt = dt / static_cast<double>(ns);
+            // the step domain's gas velocity at the pushed position
+            // (SimulationEngineCollisionRuntime.cpp:93-101, 122-130)
+            icarion_domain_desc dflow;
+            const bool flow = f && f->flow_model != ICARION_FLOW_CONSTANT;
+            if (flow) {
+                dflow = d;
+                const V3 u = gas_velocity_at(d, f, ion.pos);
+                dflow.gas_velocity_m_s[0] = u.x;
+                dflow.gas_velocity_m_s[1] = u.y;
+                dflow.gas_velocity_m_s[2] = u.z;
+            }
+            const icarion_domain_desc& dc = flow ? dflow : d;
             for (int sub = 0; sub < ns; ++sub) {
-                if (collide(d, ph, sp_d, atoms, ion.m, ion.q, ion.vel, collision_dt, rng, scratch, tb)) {
+                if (collide(dc, ph, sp_d, atoms, ion.m, ion.q, ion.vel, collision_dt, rng, scratch, tb)) {
                     ++ion.ncoll;
                     ++collisions;
                 }
@@ -1754,8 +1766,6 @@ int oracle_run_multi(const icarion_domain_desc* domains, const double* end_apert
         t_table[k] = t;
         t = t + dt;
     }
-    for (uint32_t k = 0; fields && k < n_domains; ++k)
-        if (fields[k] && fields[k]->flow_model != ICARION_FLOW_CONSTANT) return ICARION_ERR_UNSUPPORTED;
     for (uint32_t k = 0; fields && k < n_domains; ++k) {
         if (!fields[k] || !fields[k]->pressure) continue;
         D.env.resize(n_domains);

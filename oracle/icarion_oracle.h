@@ -53,7 +53,7 @@ typedef struct oracle_field {
      * fixed-step engine loops only */
     const icarion_waveform_desc *pressure;
     /* the domain's gas flow (enum icarion_flow_model) and the AxialParabolic
-     * centre-line speed; the fixed-step single-domain loop only */
+     * centre-line speed; the fixed-step loops only */
     int32_t flow_model;
     int32_t reserved1;
     double flow_max_velocity_m_s;

@@ -51,6 +51,9 @@ def gpu_multi(w, ens, n_steps, t0=0.0, step_offset=0, physics=None, domains=None
             for k, pw in enumerate(w.get("pressures") or []):
                 if pw is not None:
                     h.set_domain_pressure_waveform(k, pw)
+            for k, fw in enumerate(w.get("flows") or []):
+                if fw is not None:
+                    h.set_domain_gas_flow(k, *fw)
             h.upload_state(ens)
         st = h.step(t0, w["dt"], n_steps, step_offset)
         h.download_state(ens)
@@ -70,7 +73,7 @@ def oracle_multi(oracle, w, ens, n_steps, t0=0.0, step_offset=0, physics=None, d
                           fields=[None if f is None else oracle.Field(*f) for f in w["fields"]]
                           if w.get("fields") else None,
                           end_apertures=apertures if apertures is not None else w.get("apertures"),
-                          waveforms=w.get("waveforms"), pressures=w.get("pressures"))
+                          waveforms=w.get("waveforms"), pressures=w.get("pressures"), flows=w.get("flows"))
     return st, di
 
 

@@ -16,6 +16,7 @@ import pytest
 
 from icarion_amd import _abi
 from icarion_amd import workloads as W
+from test_gpu_domains import gpu_multi, oracle_multi
 from test_gpu_parity import compare_ions, gpu_run, oracle_run
 from test_oracle_flow import radial_cloud, rotated
 
@@ -97,6 +98,43 @@ def test_parabolic_flow_fp32_statistics(oracle):
         se = math.hypot(g.vx[sel].std() / math.sqrt(per), o.vx[sel].std() / math.sqrt(per))
         print(f"  radius {k}: {g.vx[sel].mean():.2f} vs {o.vx[sel].mean():.2f} (se {se:.2f})", flush=True)
         assert abs(g.vx[sel].mean() - o.vx[sel].mean()) < 4 * se
+
+
+@pytest.mark.parametrize("precision,collision", [(_abi.PRECISION_FP64, _abi.COLLISION_LANGEVIN),
+                                                 (_abi.PRECISION_FP64, _abi.COLLISION_HSS),
+                                                 (_abi.PRECISION_FP32_KAHAN, _abi.COLLISION_LANGEVIN)])
+def test_several_domains_with_their_own_flow(oracle, precision, collision):
+    """Flows on the two IMS regions of the three-domain instrument (none in the
+    quadrupole): FP64 ion by ion with domain indices, FP32 by per-domain
+    survival and mean axial velocity."""
+    n = 2048 if precision == _abi.PRECISION_FP64 else 16384
+    w = W.multi_domain_workload(n=n, seed=117, collision=collision)
+    w["physics"] = W.physics(_abi.INTEGRATOR_RK4, collision, ou=1 if collision == _abi.COLLISION_LANGEVIN else 0,
+                             precision=precision, seed=117)
+    w["flows"] = [(_abi.FLOW_AXIAL_PARABOLIC, 800.0), None, (_abi.FLOW_AXIAL_PARABOLIC, -300.0)]
+    print(f"flow domains precision={precision} coll={collision}", flush=True)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg, dg = gpu_multi(w, g, 4000)
+    so, do = oracle_multi(oracle, w, o, 4000)
+    if precision == _abi.PRECISION_FP64:
+        ok = compare_ions(g, o, rtol=1e-9) & (dg == do)
+        print(f"  match {ok.mean():.4f}", flush=True)
+        assert ok.mean() >= 0.99
+        if collision == _abi.COLLISION_HSS:
+            assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    else:
+        for d in (None, 0, 1, 2):
+            sel_g = g.alive if d is None else g.alive & (dg == d)
+            sel_o = o.alive if d is None else o.alive & (do == d)
+            fg, fo = sel_g.mean(), sel_o.mean()
+            err = 4 * np.sqrt(max(fo * (1 - fo), 1e-4) / g.n) + 0.005
+            assert abs(fg - fo) < err, (d, fg, fo)
+    # the flows change the run
+    w0 = dict(w)
+    w0["flows"] = None
+    g0 = w["ensemble"].copy()
+    gpu_multi(w0, g0, 4000)
+    assert np.abs(g.vz - g0.vz).max() > 10.0
 
 
 def test_flow_arguments_and_unsupported():

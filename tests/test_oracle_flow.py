@@ -125,9 +125,29 @@ def test_hss_ions_take_the_local_flow(oracle, substeps):
         assert abs(e.vz[sel].mean()) < 4 * se
 
 
-def test_flow_is_declined_where_not_restated(oracle):
-    w = W.multi_domain_workload(n=16, seed=5, collision=_abi.COLLISION_HSS)
-    f = oracle._with_flow(None, PARABOLIC)
+@pytest.mark.parametrize("collision", [_abi.COLLISION_HSS, _abi.COLLISION_FRICTION])
+def test_one_domain_set_equals_the_single_domain_loop(oracle, collision):
+    """The multi-domain loop with a flow on its one domain is the single-domain
+    loop with that flow, bit for bit (the same engine steps for ions that stay
+    in the domain)."""
+    d = rotated(W.ims_domain(length=20e-3, radius=10e-3, axial_V=50.0, gas="N2", pressure_Pa=101325.0))
+    R = d.radius_m
+    e, _ = radial_cloud(d, (0.0, 0.4 * R, 0.8 * R), 256, 40.0, seed=6)
+    ph = W.physics(_abi.INTEGRATOR_RK4, collision, ou=1, seed=6, substeps=2 if collision == _abi.COLLISION_HSS else 0)
+    sp = [W.species(sigma_hss=60e-20, ccs=60e-20, K0=2.5)]
+    flow = (_abi.FLOW_AXIAL_PARABOLIC, 300.0)
+    a, b = e.copy(), e.copy()
+    sa = oracle.run(d, ph, sp, a, 0.0, 1e-10, 800, flow=flow)
+    di = np.zeros(b.n, np.int32)
+    sb = oracle.run_multi([d], ph, sp, b, di, 0.0, 1e-10, 800, flows=[flow])
+    for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions", "active"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+    assert sa.collisions == sb.collisions
+
+
+def test_rk45_keeps_a_constant_flow(oracle):
+    w = W.ims_hss_workload(n=8, seed=5)
+    ph = W.physics(_abi.INTEGRATOR_RK45, _abi.COLLISION_HSS, seed=5)
     with pytest.raises(RuntimeError):
-        oracle.run_multi(w["domains"], w["physics"], w["species"], w["ensemble"].copy(),
-                         np.zeros(16, np.int32), 0.0, w["dt"], 4, fields=[f, None, None])
+        oracle.run_rk45(w["domain"], ph, _abi.rk45_defaults(), w["species"], w["ensemble"].copy(), 0.0, w["dt"], 2,
+                        field=oracle._with_flow(None, PARABOLIC))
