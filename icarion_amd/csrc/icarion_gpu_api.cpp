@@ -940,20 +940,24 @@ void check_ehss_mixture(const icarion_gpu* h, bool surrogate) {
             fail(ICARION_ERR_INVALID, "every participating gas component needs a mass > 0");
 }
 
-// RK45 on an analytic IMS / quadrupole field with waveforms: the kernel
-// evaluates every slot at each ion's stage times, from the descriptors and
-// ARBITRARY points copied here and each slot's constant.
-void upload_rk45_waves(icarion_gpu* h, StepParams& P) {
+// RK45 on an analytic IMS / quadrupole field with waveforms (voltages): the
+// kernel evaluates every slot at each ion's stage times, from the descriptors
+// and ARBITRARY points copied here and each slot's constant.  The pressure
+// waveform goes into the descriptor after the voltage slots; the kernel
+// evaluates it at each engine step's time.
+void upload_rk45_waves(icarion_gpu* h, StepParams& P, bool voltages) {
     const icarion_domain_desc& d = h->dom;
     const double cst[ICARION_N_VOLTAGE_SLOTS] = {d.dc_axial_V, d.dc_quad_V, d.dc_radial_V, d.rf_voltage_V,
                                                  d.rf_frequency_Hz};
-    std::vector<icarion_waveform_desc> desc(ICARION_N_VOLTAGE_SLOTS);
+    constexpr int kSlots = ICARION_N_VOLTAGE_SLOTS + 1;
+    std::vector<icarion_waveform_desc> desc(kSlots);
     std::vector<double> pts;
-    size_t off_t[ICARION_N_VOLTAGE_SLOTS] = {}, off_v[ICARION_N_VOLTAGE_SLOTS] = {};
+    size_t off_t[kSlots] = {}, off_v[kSlots] = {};
     uint32_t mask = 0;
-    for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) {
-        P.wave_const[k] = cst[k];
-        const HostWaveform* w = h->wave[k].get();
+    for (int k = 0; k < kSlots; ++k) {
+        if (k < ICARION_N_VOLTAGE_SLOTS) P.wave_const[k] = cst[k];
+        const HostWaveform* w = k < ICARION_N_VOLTAGE_SLOTS ? (voltages ? h->wave[k].get() : nullptr)
+                                                            : h->pressure.get();
         if (!w) continue;
         mask |= 1u << k;
         desc[k] = w->d;
@@ -965,16 +969,22 @@ void upload_rk45_waves(icarion_gpu* h, StepParams& P) {
     }
     h->wave_pts.resize(std::max<size_t>(pts.size(), 1));
     if (!pts.empty()) copy_in(h->wave_pts, pts.data(), pts.size(), h->stream, "waveform points");
-    for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) {
+    for (int k = 0; k < kSlots; ++k) {
         desc[k].times_s = (mask >> k & 1u) && desc[k].n_points ? h->wave_pts.p + off_t[k] : nullptr;
         desc[k].values = (mask >> k & 1u) && desc[k].n_points ? h->wave_pts.p + off_v[k] : nullptr;
     }
-    h->wave_desc.resize(ICARION_N_VOLTAGE_SLOTS);
+    h->wave_desc.resize(kSlots);
     copy_in(h->wave_desc, desc.data(), desc.size(), h->stream, "waveform descriptors");
     hip_check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     P.waves = h->wave_desc.p;
-    P.wave_mask = mask;
-    if (h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
+    P.wave_mask = mask & ((1u << ICARION_N_VOLTAGE_SLOTS) - 1u);
+    if (voltages && h->wave[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
+    if (h->pressure) {
+        if (!(d.gas_density_m3 > 0.0 && d.temperature_K > 0.0))
+            fail(ICARION_ERR_INVALID, "a pressure waveform needs the domain's gas density and temperature > 0");
+        P.pressure_wave = 1;
+        P.pressure_scale = 1.0 / (kBoltzmannHost * d.temperature_K * d.gas_density_m3);
+    }
 }
 
 // The RK45 engine loop: one launch pair (step kernel + engine-time advance) per
@@ -985,8 +995,6 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     const int coll = launch_coll_kind(h);
     if (h->has_waves && field != FIELD_ORBITRAP && field != FIELD_LINEAR)
         fail(ICARION_ERR_UNSUPPORTED, "RK45 on a field array runs with constant voltages");
-    if (h->pressure) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with a constant gas pressure");
-    if (h->flow_model != ICARION_FLOW_CONSTANT) fail(ICARION_ERR_UNSUPPORTED, "RK45 runs with a constant gas flow");
     if (h->phys.precision == ICARION_PRECISION_FP32_KAHAN && (coll == COLL_EHSS || coll == COLL_EHSS_TAB))
         fail(ICARION_ERR_UNSUPPORTED, "EHSS geometry collisions run in FP64 only");
     if (coll == COLL_EHSS && h->phys.ehss_mode == ICARION_EHSS_SURROGATE)
@@ -1007,7 +1015,13 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     P.counter_stripes = kCounterStripes;
     P.rk45 = h->rk45;
     apply_orbitrap_waveform(h, P);
-    if (h->has_waves && field == FIELD_LINEAR) upload_rk45_waves(h, P);
+    if ((h->has_waves && field == FIELD_LINEAR) || h->pressure)
+        upload_rk45_waves(h, P, h->has_waves && field == FIELD_LINEAR);
+    if (h->flow_model == ICARION_FLOW_AXIAL_PARABOLIC) {
+        P.flow_parabolic = 1;
+        P.flow_umax = h->flow_umax;
+        P.flow_R2 = h->dom.radius_m > 0.0 ? h->dom.radius_m * h->dom.radius_m : 0.0;
+    }
     fill_fast_const(P);
     const IonPtrs S = ion_ptrs(h);
     h->r_eng.resize(1);
@@ -1065,6 +1079,8 @@ void do_step_rk45(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64
     Rk45Engine e1{};
     hip_check(hipMemcpyAsync(&e1, h->r_eng.p, sizeof(e1), hipMemcpyDeviceToHost, s), "read engine record");
     const CallCounts c = read_counters(h, s, stats != nullptr, h->counters.p);
+    if (e1.bad_pressure)   // the steps before it ran, as the reference's engine does before it throws
+        fail(ICARION_ERR_INVALID, "pressure waveform evaluated to a non-positive value at t = " + std::to_string(e1.T));
     if (e1.failed)
         fail(ICARION_ERR_INVALID, "RK45: " + std::to_string(e1.failed) +
                                       " ion steps found no accepted step in 10 attempts");

@@ -60,6 +60,16 @@ __device__ __forceinline__ double rf_coeff(const StepParams& P, double t) {
     return P.has_rf ? (P.rf_U + P.rf_V * cos(P.rf_omega * t)) / P.r0_sq : 0.0;
 }
 
+// The engine step's gas density over the domain's (update_dynamic_environments
+// at the engine time T, SimulationEngineLifecycle.cpp:34-47): 1 without a
+// pressure waveform; a pressure <= 0 gives 0 (the reference throws there).
+__device__ __forceinline__ double rk45_density_ratio(const StepParams& P, double T) {
+    if (!P.pressure_wave) return 1.0;
+    const icarion_waveform_desc& w = P.waves[ICARION_N_VOLTAGE_SLOTS];
+    const double p = waveform_eval(w, w.times_s, w.values, w.n_points, T);
+    return p > 0.0 ? p * P.pressure_scale : 0.0;
+}
+
 // A voltage slot at t: its waveform where one is set, else the constant.
 __device__ __forceinline__ double wave_slot(const StepParams& P, int k, double t) {
     if (P.wave_mask & (1u << k)) {
@@ -118,6 +128,11 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
     Rk45Engine* E = R.eng;
     const double T = E->T;
     const unsigned long long kstep = E->step;
+    const double nrel = rk45_density_ratio(P, T);
+    if (!(nrel > 0.0)) {   // no step: the host reports the engine time
+        if (i == 0) E->bad_pressure = 1ull;
+        return;
+    }
     uint32_t pushed = 0, ncoll_launch = 0, rejected = 0, failed = 0;
     double tmax = -INFINITY, bmin = INFINITY, dtmax = 0.0;
     if (i < P.n_ions) {
@@ -161,13 +176,15 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                             lpre = q / ((K0t * (kLoschmidt / P.n)) * m);
                         }
                     }
+                    lpre *= nrel;
                 }
-                // analytic fields take the RF coefficient, field arrays the stage time
+                // analytic fields take the RF coefficient, field arrays the stage time;
+                // the damping force relative to the gas velocity at the stage position
                 auto acc = [&](d3 p, d3 v, double tt) {
                     double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
                     double ez = P.Ez_const;
                     if (FIELD == FIELD_LINEAR && P.wave_mask) wave_field_inputs(P, tt, c, ez);
-                    return force_f64<FIELD, COLL>(P, p, v, c, ez, q, m, lpre) * inv_m;
+                    return force_f64<FIELD, COLL, true>(P, p, v, c, ez, q, m, lpre) * inv_m;
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);
@@ -229,8 +246,10 @@ __global__ void __launch_bounds__(256) rk45_step_kernel(const StepParams P, cons
                         PhiloxStream rng(P.seed, i, kstep);
                         // collision substeps of the ion's own dt (n_sub = 1: one stage)
                         const double dtc = P.n_sub == 1 ? used : used / static_cast<double>(P.n_sub);
+                        const double dtn = COLL == COLL_DAMPING ? dtc : dtc * nrel;
+                        const d3 ug = gas_velocity_f64(P, pos);
                         for (int s = 0; s < P.n_sub; ++s) {
-                            if (collide_f64<COLL>(P, sp, sidx, m, lpre, vel, rng, dtc)) {
+                            if (collide_f64<COLL, true>(P, sp, sidx, m, lpre, vel, rng, dtn, ug)) {
                                 ++nc;
                                 ++ncoll_launch;
                             }
@@ -302,6 +321,11 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
     Rk45Engine* E = R.eng;
     const double T = E->T;
     const unsigned long long kstep = E->step;
+    const double nrel = rk45_density_ratio(P, T);
+    if (!(nrel > 0.0)) {   // no step: the host reports the engine time
+        if (i == 0) E->bad_pressure = 1ull;
+        return;
+    }
     uint32_t pushed = 0, ncoll_launch = 0, rejected = 0, failed = 0;
     double tmax = -INFINITY, bmin = INFINITY, dtmax = 0.0;
     if (i < P.n_ions) {
@@ -352,13 +376,13 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
                             l = qd / ((K0t * (kLoschmidt / P.n)) * md);
                         }
                     }
-                    lpre = (float)l;
+                    lpre = (float)(l * nrel);
                 }
                 auto acc = [&](f3 p, f3 v, double tt) {
                     double c = FIELD >= FIELD_ARRAY ? tt : rf_coeff(P, tt);
                     double ez = P.Ez_const;
                     if (FIELD == FIELD_LINEAR && P.wave_mask) wave_field_inputs(P, tt, c, ez);
-                    return accel_f32<FIELD, COLL>(P, C, p, v, c, ez, qm, lpre);
+                    return accel_f32<FIELD, COLL, true>(P, C, p, v, c, ez, qm, lpre);
                 };
                 const Rk45Cfg& c = P.rk45;
                 const double dt_min = fmax(dt_i * c.min_factor, c.abs_min);
@@ -449,9 +473,11 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
                         const double dtc = P.n_sub == 1 ? used : used / static_cast<double>(P.n_sub);
                         const float kbt2_dt_over_m = (float)(2.0 * kBoltzmann * P.T / md * dtc);
                         uint32_t hits = 0;
+                        const f3 ug = gas_velocity_f32(P, C, f3{hi[0], hi[1], hi[2]});
                         for (int s = 0; s < P.n_sub; ++s)
-                            hits += collide_f32<COLL>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v, rng,
-                                                      (float)dtc, (float)(P.n * dtc)) ? 1u : 0u;
+                            hits += collide_f32<COLL, true>(P, C, sp, sidx, m, inv_m_plus, lpre, kbt2_dt_over_m, v,
+                                                            rng, (float)(dtc * nrel), (float)(P.n * nrel * dtc), ug)
+                                        ? 1u : 0u;
                         if (COLL == COLL_HSS || COLL == COLL_HSS_MIX) {
                             if (hits) {
                                 hi[3] = v.x; hi[4] = v.y; hi[5] = v.z;
@@ -521,6 +547,7 @@ __global__ void __launch_bounds__(256) rk45_step_f32_kernel(const StepParams P, 
 // unbounded; then clears the reductions for the next step.
 __global__ void rk45_advance_kernel(Rk45Engine* E) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (E->bad_pressure) return;   // the step did not run: the engine time stays
     const double T = E->T;
     double Tn = T;
     if (E->tmax != 0ull) Tn = fmax(T, ord_value(E->tmax));

@@ -99,6 +99,7 @@ struct Rk45Engine {
     unsigned long long dtmax;       // ordered bits: max accepted dt
     unsigned long long rejected;    // rejected trial steps
     unsigned long long failed;      // ions with no accepted trial
+    unsigned long long bad_pressure;    // 1: the pressure waveform was <= 0 at T (T then stays)
 };
 
 struct Rk45Ptrs {
@@ -233,7 +234,11 @@ struct StepParams {
     // fields: the slot descriptors (ARBITRARY points in device memory), the
     // mask of slots that hold one, and every slot's constant
     const icarion_waveform_desc *waves;
-    uint32_t wave_mask, wave_pad;
+    uint32_t wave_mask;
+    // RK45 with a pressure waveform (descriptor waves[ICARION_N_VOLTAGE_SLOTS]):
+    // the engine step's density ratio is P(T) * pressure_scale
+    uint32_t pressure_wave;
+    double pressure_scale;
     double wave_const[ICARION_N_VOLTAGE_SLOTS];
 };
 

@@ -436,8 +436,9 @@ int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const
  * and Friction damping coefficients and with them the OU kicks) scales by
  * n / gas_density_m3 of the domain descriptor, which must be > 0 (it holds
  * the constants the rest derive from: the reference's P(0)).  A step whose
- * P(t) <= 0 returns ICARION_ERR_INVALID, as the reference throws.  Fixed-step
- * integrators, FP64 and FP32; RK45 and the EHSS surrogate return
+ * P(t) <= 0 returns ICARION_ERR_INVALID, as the reference throws (the steps
+ * before it have run).  RK4, Boris and RK45 (evaluated in the kernel at each
+ * engine step's time), FP64 and FP32; the EHSS surrogate returns
  * ICARION_ERR_UNSUPPORTED with one set.  pressure = NULL clears. */
 int icarion_gpu_set_pressure_waveform(icarion_gpu *h, const icarion_waveform_desc *pressure);
 
@@ -459,10 +460,11 @@ enum icarion_flow_model {
  * every collision substep at the ion's pushed position
  * (SimulationEngineCollisionRuntime.cpp:122-130).  Fixed-step integrators,
  * FP64 and FP32, HSS (single gas and mixture), EHSS with offline sets,
- * orientation samples or a mixture, Langevin / Friction; RK45, the
- * wave-cooperative geometry EHSS kernel and the EHSS surrogate return
- * ICARION_ERR_UNSUPPORTED with it (several domains take theirs with
- * icarion_gpu_set_domain_gas_flow).  ICARION_FLOW_CONSTANT clears. */
+ * orientation samples or a mixture, Langevin / Friction, and RK45 (its
+ * damping force at each stage position); the wave-cooperative geometry EHSS
+ * kernel and the EHSS surrogate return ICARION_ERR_UNSUPPORTED with it
+ * (several domains take theirs with icarion_gpu_set_domain_gas_flow).
+ * ICARION_FLOW_CONSTANT clears. */
 int icarion_gpu_set_gas_flow(icarion_gpu *h, int32_t flow_model, double axial_max_velocity_m_s);
 
 /* Offline EHSS sample sets (EHSSOfflineSampleSet, EHSSOfflineSampleSet.h:15-30;

@@ -1953,8 +1953,6 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
     (void)n_atoms;
     if (!domain || !physics || !cfg || !species || !ions || !rk45 || n_species == 0)
         return ICARION_ERR_INVALID;
-    // constant pressure and flow only
-    if (field && (field->pressure || field->flow_model != ICARION_FLOW_CONSTANT)) return ICARION_ERR_UNSUPPORTED;
     const RunTables rt(tables);
     if (!rt.ok(n_species)) return ICARION_ERR_INVALID;
     const icarion_domain_desc& d = *domain;
@@ -1976,6 +1974,15 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
     const long long chunk = omp_chunk(n);
     for (uint64_t k = 0; k < n_steps; ++k) {
         double max_dt_used = 0.0;
+        // the step's environment: with a pressure waveform, its density at the
+        // engine time T (update_dynamic_environments)
+        std::unique_ptr<EnvSteps> env;
+        if (field && field->pressure) {
+            env = std::make_unique<EnvSteps>(d, *field->pressure, tables ? tables->mixture : nullptr, &T, 1);
+            if (!env->ok) return ICARION_ERR_INVALID;
+        }
+        const icarion_domain_desc& de = env ? env->dom[0] : d;
+        const bool flow = field && field->flow_model != ICARION_FLOW_CONSTANT;
         #pragma omp parallel reduction(+ : pushed_total, coll_total, rejected_total) \
             reduction(max : max_dt_used) reduction(|| : failed)
         {
@@ -2009,7 +2016,7 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                 const double z_before = pos.z;
                 double dt_next = 0.0;
                 int rej = 0;
-                const double used = rk45_step(d, field, ph.collision_model, *cfg, sp,
+                const double used = rk45_step(de, field, ph.collision_model, *cfg, sp,
                                               ions->mass_kg[i], ions->charge_C[i], pos, vel, T,
                                               st.dt, st, dt_next, rej);
                 rejected_total += static_cast<uint64_t>(rej);
@@ -2022,12 +2029,23 @@ int oracle_run_rk45(const icarion_domain_desc* domain, const icarion_physics_des
                 // perform_collisions with the accepted dt
                 if (ph.collision_model != ICARION_COLLISION_NONE) {
                     PhiloxStream rng(ph.rng_seed, static_cast<uint32_t>(i), step_offset + k);
-                    const IonTables tb = rt.for_species(s < n_species ? s : 0);
+                    IonTables tb = rt.for_species(s < n_species ? s : 0);
+                    if (env && tb.mix) tb.mix = &env->mix[0];
+                    // the gas velocity at the pushed position
+                    icarion_domain_desc dflow;
+                    if (flow) {
+                        dflow = de;
+                        const V3 u = gas_velocity_at(de, field, pos);
+                        dflow.gas_velocity_m_s[0] = u.x;
+                        dflow.gas_velocity_m_s[1] = u.y;
+                        dflow.gas_velocity_m_s[2] = u.z;
+                    }
+                    const icarion_domain_desc& dc = flow ? dflow : de;
                     // substeps of the ion's own dt (dt_used_per_ion / substeps)
                     const int ns = collision_substeps(ph);
                     const double collision_dt = used / static_cast<double>(ns);
                     for (int sub = 0; sub < ns; ++sub) {
-                        if (collide(d, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, collision_dt,
+                        if (collide(dc, ph, sp, atoms, ions->mass_kg[i], ions->charge_C[i], vel, collision_dt,
                                     rng, scratch, tb)) {
                             if (ions->n_collisions) ++ions->n_collisions[i];
                             ++coll_total;

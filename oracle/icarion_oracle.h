@@ -49,11 +49,10 @@ typedef struct oracle_field {
      * no field array */
     const icarion_waveform_desc *waveforms[ICARION_N_VOLTAGE_SLOTS];
     /* the domain's gas pressure as a waveform (EnvironmentConfig::
-     * pressure_Pa_waveform), NULL: the descriptor's constant pressure; the
-     * fixed-step engine loops only */
+     * pressure_Pa_waveform), NULL: the descriptor's constant pressure */
     const icarion_waveform_desc *pressure;
     /* the domain's gas flow (enum icarion_flow_model) and the AxialParabolic
-     * centre-line speed; the fixed-step loops only */
+     * centre-line speed (none: 0) */
     int32_t flow_model;
     int32_t reserved1;
     double flow_max_velocity_m_s;

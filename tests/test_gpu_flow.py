@@ -137,6 +137,39 @@ def test_several_domains_with_their_own_flow(oracle, precision, collision):
     assert np.abs(g.vz - g0.vz).max() > 10.0
 
 
+@pytest.mark.parametrize("collision", [_abi.COLLISION_HSS, _abi.COLLISION_FRICTION])
+def test_rk45_parabolic_flow(oracle, collision):
+    """RK45: the damping force at each stage position and the neutrals at the
+    pushed position take the local flow, in the rotated drift tube.  HSS is
+    compared ion by ion with the oracle's RK45 loop; Friction + OU at 1 atm
+    holds the adaptive step at the stiff damping's stability edge, where
+    rounding flips accept / reject decisions, so it is held to the oracle's
+    mean velocity along the flow and its spread."""
+    from test_gpu_rk45 import gpu_rk45, match, oracle_rk45
+    d = rotated(W.ims_domain(length=20e-3, radius=10e-3, axial_V=50.0, gas="N2", pressure_Pa=101325.0))
+    R = d.radius_m
+    e, _ = radial_cloud(d, (0.0, 0.3 * R, 0.6 * R), 512, 40.0, seed=8)
+    ph = W.physics(_abi.INTEGRATOR_RK45, collision, ou=1, seed=8)
+    w = dict(domain=d, species=[W.species(sigma_hss=60e-20, ccs=60e-20, K0=2.5)], dt=1e-10, flow=FLOW)
+    print(f"flow RK45 coll={collision}", flush=True)
+    g, o = e.copy(), e.copy()
+    sg = gpu_rk45(w, ph, g, 80, w["dt"])
+    so = oracle_rk45(oracle, w, ph, o, 80, w["dt"])
+    if collision == _abi.COLLISION_HSS:
+        assert match(g, o, rtol=1e-9, dt_rtol=1e-9).mean() >= 0.99
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    else:
+        se = math.hypot(g.vx.std() / math.sqrt(g.n), o.vx.std() / math.sqrt(o.n))
+        print(f"  vx {g.vx.mean():.3f} vs {o.vx.mean():.3f} (se {se:.3f})", flush=True)
+        assert abs(g.vx.mean() - o.vx.mean()) < 4 * se
+        assert g.vx.std() == pytest.approx(o.vx.std(), rel=0.1)
+    g0 = e.copy()
+    w0 = dict(w)
+    w0["flow"] = None
+    gpu_rk45(w0, ph, g0, 80, w["dt"])
+    assert abs(g.vx.mean() - g0.vx.mean()) > 20.0
+
+
 def test_flow_arguments_and_unsupported():
     from icarion_amd import gpu
     w = W.ims_hss_workload(n=64, seed=113)
@@ -159,9 +192,8 @@ def test_flow_arguments_and_unsupported():
         h.download_state(b)
     for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions"):
         np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
-    # RK45, the geometry EHSS kernel and the surrogate keep a constant flow
+    # the geometry EHSS kernel and the surrogate keep a constant flow
     cases = []
-    cases.append((w, W.physics(_abi.INTEGRATOR_RK45, _abi.COLLISION_HSS, seed=113), None))
     from test_gpu_parity import ehss_workload
     we = ehss_workload(n=64, seed=114)
     cases.append((we, we["physics"], None))

@@ -154,6 +154,44 @@ def test_several_domains_with_their_own_pressure(oracle, precision):
             assert abs(fg - fo) < err, (d, fg, fo)
 
 
+@pytest.mark.parametrize("collision,precision", [(_abi.COLLISION_HSS, _abi.PRECISION_FP64),
+                                                 (_abi.COLLISION_FRICTION, _abi.PRECISION_FP64),
+                                                 (_abi.COLLISION_HSS, _abi.PRECISION_FP32_KAHAN)])
+def test_rk45_pressure_waveform(oracle, collision, precision):
+    """RK45: the density of each engine step at its engine time; FP64 HSS ion
+    by ion against the oracle's RK45 loop, FP32 + Kahan HSS by the collision
+    count.  Friction + OU at 1-3 atm holds the adaptive step at the stiff
+    damping's stability edge, where the oracle's own runs part ion by ion
+    under a one-ulp change of the start (accept / reject flips), so that case
+    is held to the oracle's mean drift velocity and its spread."""
+    from test_gpu_rk45 import gpu_rk45, match, oracle_rk45
+    w = W.ims_hss_workload(n=2048, seed=101)
+    ph = W.physics(_abi.INTEGRATOR_RK45, collision, ou=1, precision=precision, seed=101)
+    if collision == _abi.COLLISION_FRICTION:
+        w["species"] = [W.species(K0=2.1, ccs=100e-20)]
+    w["pressure"] = pulse_and_ramp(w["domain"].pressure_Pa, w["dt"])
+    print(f"pressure RK45 coll={collision} precision={precision}", flush=True)
+    # FP32: on the Dormand-Prince chain (every trial accepted, dt constant),
+    # so that both runs cover the same engine times
+    cfg = (_abi.rk45_defaults(atol=1.0, rtol=1.0, max_step_increase=1.0, max_step_decrease=1.0)
+           if precision == _abi.PRECISION_FP32_KAHAN else None)
+    g, o = w["ensemble"].copy(), w["ensemble"].copy()
+    sg = gpu_rk45(w, ph, g, 60, w["dt"], cfg=cfg)
+    so = oracle_rk45(oracle, w, W.physics(_abi.INTEGRATOR_RK45, collision, ou=1, seed=101), o, 60, w["dt"],
+                     cfg=cfg)
+    if collision == _abi.COLLISION_FRICTION:
+        se = math.hypot(g.vz.std() / math.sqrt(g.n), o.vz.std() / math.sqrt(o.n))
+        print(f"  vz {g.vz.mean():.3f} vs {o.vz.mean():.3f} (se {se:.3f})", flush=True)
+        assert abs(g.vz.mean() - o.vz.mean()) < 4 * se
+        assert g.vz.std() == pytest.approx(o.vz.std(), rel=0.1)
+    elif precision == _abi.PRECISION_FP64:
+        assert match(g, o, rtol=1e-9, dt_rtol=1e-9).mean() >= 0.99
+        assert sg.collisions == pytest.approx(so.collisions, rel=3e-3)
+    else:
+        assert sg.t_end_s == pytest.approx(so.t_end_s, rel=1e-12)
+        assert sg.collisions == pytest.approx(so.collisions, rel=0.02)
+
+
 def test_pressure_arguments_and_unsupported():
     from icarion_amd import gpu
     w = W.ims_hss_workload(n=64, seed=98)
@@ -171,15 +209,15 @@ def test_pressure_arguments_and_unsupported():
         assert ei.value.code == _abi.ERR_INVALID
         h.set_pressure_waveform(None)
         h.step(40 * dt, dt, 40, 40)
-    # RK45 keeps a constant pressure
+    # RK45: the same rejection at the engine step whose time reaches P <= 0
     with gpu.Handle() as h:
         h.set_domain(w["domain"], W.physics(_abi.INTEGRATOR_RK45, _abi.COLLISION_HSS, seed=98))
         h.set_species(w["species"])
-        h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (2e5,)))
+        h.set_pressure_waveform(ramp)
         h.upload_state(w["ensemble"].copy())
         with pytest.raises(gpu.IcarionError) as ei:
-            h.step(0.0, dt, 4)
-        assert ei.value.code == _abi.ERR_UNSUPPORTED
+            h.step(0.0, dt, 200)
+        assert ei.value.code == _abi.ERR_INVALID
     # a descriptor without a gas density cannot be scaled
     d0 = type(w["domain"]).from_buffer_copy(w["domain"])
     d0.pressure_Pa = 0.0

@@ -49,6 +49,10 @@ def gpu_rk45(w, ph, ens, n_steps, dt, t0=0.0, step_offset=0, cfg=None):
             h.upload_field_array(E, origin, spacing)
         if w.get("terms") is not None:
             h.upload_field_terms(w["axes"], w["terms"])
+        if w.get("pressure") is not None:
+            h.set_pressure_waveform(w["pressure"])
+        if w.get("flow") is not None:
+            h.set_gas_flow(*w["flow"])
         if w.get("mixture") is not None:
             h.set_gas_mixture(*w["mixture"])
             if w.get("mixture_radii") is not None:
@@ -70,6 +74,7 @@ def oracle_rk45(oracle, w, ph, ens, n_steps, dt, t0=0.0, cfg=None):
         field = oracle.Waveforms(w["waveforms"])
     mixture = (oracle.GasMixture(*w["mixture"], radii=w.get("mixture_radii"))
                if w.get("mixture") is not None else None)
+    field = oracle._with_flow(oracle._with_pressure(field, w.get("pressure")), w.get("flow"))
     st, _ = oracle.run_rk45(w["domain"], ph, cfg, w["species"], ens, t0, dt, n_steps,
                             atoms=w.get("atoms"), field=field, mixture=mixture)
     return st

@@ -145,9 +145,16 @@ def test_one_domain_set_equals_the_single_domain_loop(oracle, collision):
     assert sa.collisions == sb.collisions
 
 
-def test_rk45_keeps_a_constant_flow(oracle):
-    w = W.ims_hss_workload(n=8, seed=5)
-    ph = W.physics(_abi.INTEGRATOR_RK45, _abi.COLLISION_HSS, seed=5)
-    with pytest.raises(RuntimeError):
-        oracle.run_rk45(w["domain"], ph, _abi.rk45_defaults(), w["species"], w["ensemble"].copy(), 0.0, w["dt"], 2,
-                        field=oracle._with_flow(None, PARABOLIC))
+def test_rk45_friction_relaxes_to_the_local_flow(oracle):
+    """RK45 (the damping force at each stage position): the ions settle at
+    v = u(r), to within the step controller's tolerance (the stiff damping
+    holds the adaptive step at its stability edge, where a residual of
+    relative size ~rtol remains)."""
+    d = rotated(W.ims_domain(length=20e-3, radius=10e-3, axial_V=0.0, gas="N2", pressure_Pa=101325.0))
+    R, u_max = d.radius_m, 500.0
+    e, r = radial_cloud(d, (0.0, 0.3 * R, 0.6 * R, 0.9 * R), 4, 100.0)
+    ph = W.physics(_abi.INTEGRATOR_RK45, _abi.COLLISION_FRICTION)
+    oracle.run_rk45(d, ph, _abi.rk45_defaults(), [W.species(K0=2.0)], e, 0.0, 1e-10, 400,
+                    field=oracle._with_flow(None, (_abi.FLOW_AXIAL_PARABOLIC, u_max)))
+    np.testing.assert_allclose(e.vx, u_max * (1.0 - (r / R) ** 2), rtol=1e-5)
+    np.testing.assert_allclose(e.vz, 0.0, atol=1e-9)

@@ -177,3 +177,32 @@ def test_several_domains_take_their_own_pressure(oracle):
     sc = oracle.run_multi(doms, w["physics"], w["species"], c, dc, 0.0, w["dt"], 3000,
                           end_apertures=w.get("apertures"))
     assert sa.collisions > 1.2 * sc.collisions
+
+
+def test_rk45_takes_the_pressure_at_each_engine_step(oracle):
+    """RK45: a CONSTANT waveform at P0 is the constant run bit for bit, one at
+    3 P0 the run of the tripled density (HSS and Friction + OU), and a
+    pressure that falls through zero is rejected."""
+    w = W.ims_hss_workload(n=256, seed=31)
+    cfg = _abi.rk45_defaults()
+    for coll in (_abi.COLLISION_HSS, _abi.COLLISION_FRICTION):
+        ph = W.physics(_abi.INTEGRATOR_RK45, coll, ou=1, seed=31)
+        sp = [W.species(sigma_hss=100e-20, ccs=100e-20, K0=2.1)]
+        d = w["domain"]
+        a, b, c = w["ensemble"].copy(), w["ensemble"].copy(), w["ensemble"].copy()
+        sa, _ = oracle.run_rk45(d, ph, cfg, sp, a, 0.0, w["dt"], 40)
+        sb, _ = oracle.run_rk45(d, ph, cfg, sp, b, 0.0, w["dt"], 40,
+                                field=oracle._with_pressure(None, const_wave(d.pressure_Pa)))
+        assert_same(a, b)
+        sc, _ = oracle.run_rk45(d, ph, cfg, sp, c, 0.0, w["dt"], 40,
+                                field=oracle._with_pressure(None, const_wave(3 * d.pressure_Pa)))
+        e3 = w["ensemble"].copy()
+        s3, _ = oracle.run_rk45(scaled_domain(d, 3.0), ph, cfg, sp, e3, 0.0, w["dt"], 40)
+        assert close_fraction(c, e3) >= 0.99
+        if coll == _abi.COLLISION_HSS:
+            assert sc.collisions == pytest.approx(s3.collisions, rel=3e-3)
+            assert sc.collisions != sa.collisions
+    ramp = _abi.waveform(_abi.WAVEFORM_LINEAR, (1000.0, -1000.0, 0.0, 100 * w["dt"]), clamp=1)
+    with pytest.raises(RuntimeError):
+        oracle.run_rk45(w["domain"], w["physics"], cfg, w["species"], w["ensemble"].copy(), 0.0, w["dt"], 80,
+                        field=oracle._with_pressure(None, ramp))
