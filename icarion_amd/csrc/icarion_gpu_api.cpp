@@ -1381,7 +1381,6 @@ void do_step(icarion_gpu* h, double t0, double dt, uint64_t n_steps, uint64_t st
     if (h->base.n_sub > 1 && (surrogate || launch_coll_kind(h) == COLL_EHSS))
         fail(ICARION_ERR_UNSUPPORTED, "collision substeps with EHSS run with offline sets, orientation "
                                       "samples, gas mixtures, several domains, RK45 or the batch");
-    if (h->pressure && surrogate) fail(ICARION_ERR_UNSUPPORTED, "the EHSS surrogate runs with a constant gas pressure");
     const bool parabolic = h->flow_model == ICARION_FLOW_AXIAL_PARABOLIC;
     if (parabolic && (surrogate || launch_coll_kind(h) == COLL_EHSS))
         fail(ICARION_ERR_UNSUPPORTED, "the parabolic gas flow runs with HSS, damping, or EHSS with offline sets, "

@@ -840,7 +840,8 @@ __device__ __forceinline__ void surr_ion(const StepParams& P, const IonPtrs& S, 
                 vrel = v - vn;
                 vm = fast_sqrt(dot(vrel, vrel));
                 if (vm >= (float)kMinRelativeVelocity) {
-                    const float x = C.n_dt * sigma * vm;
+                    // n dt of the step (the density ratio of a pressure waveform)
+                    const float x = C.n_dt * (float)te.nrel * sigma * vm;
                     const float Pc = x < 1e-3f ? x * (1.0f - 0.5f * x * (1.0f - x * (1.0f / 3.0f)))
                                                : 1.0f - __builtin_amdgcn_exp2f(-x * 1.44269504089f);
                     if (rng.uniform_f32() < Pc) {

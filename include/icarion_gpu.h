@@ -438,8 +438,8 @@ int icarion_gpu_set_waveforms(icarion_gpu *h, const icarion_waveform_desc *const
  * the constants the rest derive from: the reference's P(0)).  A step whose
  * P(t) <= 0 returns ICARION_ERR_INVALID, as the reference throws (the steps
  * before it have run).  RK4, Boris and RK45 (evaluated in the kernel at each
- * engine step's time), FP64 and FP32; the EHSS surrogate returns
- * ICARION_ERR_UNSUPPORTED with one set.  pressure = NULL clears. */
+ * engine step's time), FP64 and FP32, the EHSS surrogate included.
+ * pressure = NULL clears. */
 int icarion_gpu_set_pressure_waveform(icarion_gpu *h, const icarion_waveform_desc *pressure);
 
 /* Gas-flow profiles of EnvironmentConfig (flow_model, EnvironmentConfig.h:22-26,

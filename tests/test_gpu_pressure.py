@@ -192,6 +192,38 @@ def test_rk45_pressure_waveform(oracle, collision, precision):
         assert sg.collisions == pytest.approx(so.collisions, rel=0.02)
 
 
+def test_surrogate_kernel_takes_the_pressure():
+    """The EHSS surrogate kernel scales its rate n dt by the step's density
+    ratio: a CONSTANT waveform at P0 is the constant run, and one at 2 P0 the
+    run of a doubled density, bit for bit (the ratio is exactly 2 and the
+    doubling of n dt is exact)."""
+    from icarion_amd import gpu
+    params = np.load("icarion_amd/data/surrogate_he_20td_clusters.npy")
+    we = W.ehss_surrogate_workload(n=20000, n_steps=300, seed=102)
+    d = we["domain"]
+    d2 = type(d).from_buffer_copy(d)
+    d2.pressure_Pa = 2 * d.pressure_Pa
+    d2.gas_density_m3 = 2 * d.gas_density_m3
+    runs = []
+    for dom, pw in ((d, None), (d, d.pressure_Pa), (d, 2 * d.pressure_Pa), (d2, None)):
+        e = we["ensemble"].copy()
+        with gpu.Handle() as h:
+            h.set_domain(dom, we["physics"])
+            h.synth_clusters(we["counts"], seed=102)
+            h.surrogate_set_params(params)
+            if pw is not None:
+                h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (pw,)))
+            h.upload_state(e)
+            st = h.step(0.0, we["dt"], 300)
+            h.download_state(e)
+        runs.append((e, st.collisions))
+    for (a, ca), (b, cb) in ((runs[0], runs[1]), (runs[2], runs[3])):
+        for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions"):
+            np.testing.assert_array_equal(getattr(a, f), getattr(b, f), err_msg=f)
+        assert ca == cb
+    assert runs[2][1] > 1.2 * runs[0][1]
+
+
 def test_pressure_arguments_and_unsupported():
     from icarion_amd import gpu
     w = W.ims_hss_workload(n=64, seed=98)
@@ -230,17 +262,6 @@ def test_pressure_arguments_and_unsupported():
         with pytest.raises(gpu.IcarionError) as ei:
             h.step(0.0, dt, 4)
         assert ei.value.code == _abi.ERR_INVALID
-    # the surrogate kernel runs at the constant pressure
-    we = W.ehss_surrogate_workload(n=256, seed=99)
-    with gpu.Handle() as h:
-        h.set_domain(we["domain"], we["physics"])
-        h.synth_clusters(we["counts"], seed=99)
-        h.surrogate_set_params(np.zeros(_abi.SURROGATE_NPARAMS, np.float32))
-        h.set_pressure_waveform(_abi.waveform(_abi.WAVEFORM_CONSTANT, (2e5,)))
-        h.upload_state(we["ensemble"].copy())
-        with pytest.raises(gpu.IcarionError) as ei:
-            h.step(0.0, we["dt"], 4)
-        assert ei.value.code == _abi.ERR_UNSUPPORTED
     # several domains take theirs per domain
     wm = W.multi_domain_workload(n=64, seed=100)
     with gpu.Handle() as h:
