@@ -5,7 +5,7 @@ sizes against offsets computed by the C compiler.
 """
 import ctypes as C
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_STATE, ERR_NO_DEVICE = range(6)
 
@@ -180,6 +180,7 @@ class CloudDesc(C.Structure):
         ("n_components", _u32), ("n_atoms", _u32), ("atoms_xyzr", C.POINTER(C.c_float)),
         ("field_E", C.POINTER(C.c_float)), ("field_n", _u32 * 3), ("reserved1", _u32),
         ("field_origin_m", _d * 3), ("field_spacing_m", _d * 3),
+        ("pressure", C.POINTER(WaveformDesc)),
     ]
 
 

@@ -1968,6 +1968,9 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         // the cloud's voltage waveforms, as icarion_gpu_set_waveforms on a handle of its own
         std::unique_ptr<HostWaveform> wv[ICARION_N_VOLTAGE_SLOTS];
         const bool waves = parse_waveform_slots(cd.waveforms, wv);
+        // the cloud's pressure waveform (its step table of its own, as with voltages)
+        const std::unique_ptr<HostWaveform> pw = cd.pressure ? parse_waveform(cd.pressure) : nullptr;
+        const bool own_table = waves || pw;
         const HostWaveform* wf[ICARION_N_VOLTAGE_SLOTS];
         for (int k = 0; k < ICARION_N_VOLTAGE_SLOTS; ++k) wf[k] = wv[k].get();
         if (wf[ICARION_SLOT_RF_VOLTAGE]) P.has_rf = 1;
@@ -1985,14 +1988,14 @@ void do_batch_upload(icarion_gpu* h, const icarion_cloud_desc* clouds, uint32_t 
         key.rf = P.has_rf;
         key.rk4 = cd.physics.integrator == ICARION_INTEGRATOR_RK4;
         size_t off = SIZE_MAX;
-        if (!waves)
+        if (!own_table)
             for (const auto& kv : seen)
                 if (kv.first == key) off = kv.second;
         if (off == SIZE_MAX) {
             off = tables.size();
             append_table(tables, cd.domain, P.has_rf != 0, cd.physics.integrator, cd.t0_s, cd.dt_s, cd.n_steps,
-                         false, waves ? wf : nullptr);
-            if (!waves) seen.push_back({key, off});
+                         false, waves ? wf : nullptr, pw.get());
+            if (!own_table) seen.push_back({key, off});
         }
         table_off[c] = off;
         bc[c].t_end = cd.n_steps ? tables[off + cd.n_steps - 1].t + cd.dt_s : cd.t0_s;

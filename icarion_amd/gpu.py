@@ -197,7 +197,8 @@ class Handle:
         t0, dt, n_steps and optional step_offset, waveforms ({slot: desc}),
         mixture ((components, sigma) as set_gas_mixture takes them), atoms
         ((n, 4) x, y, z, r for geometry EHSS, held in FP32) and field ((E, origin,
-        spacing) as upload_field_array takes them, E held in FP32)."""
+        spacing) as upload_field_array takes them, E held in FP32) and pressure
+        (a WaveformDesc, as set_pressure_waveform)."""
         arr = (_abi.CloudDesc * len(clouds))()
         keep = []
         fields = {}   # one FP32 copy per field array, so that clouds share it
@@ -214,6 +215,9 @@ class Handle:
             arr[k].dt_s = c["dt"]
             arr[k].n_steps = c["n_steps"]
             arr[k].step_offset = c.get("step_offset", 0)
+            if c.get("pressure") is not None:
+                arr[k].pressure = C.pointer(c["pressure"])
+                keep.append(c["pressure"])
             if c.get("waveforms"):
                 wa = (C.POINTER(_abi.WaveformDesc) * _abi.N_VOLTAGE_SLOTS)()
                 for slot, w in c["waveforms"].items():

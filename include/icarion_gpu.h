@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define ICARION_GPU_ABI_VERSION 5
+#define ICARION_GPU_ABI_VERSION 6
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -816,6 +816,10 @@ typedef struct icarion_cloud_desc {
     uint32_t reserved1;
     double field_origin_m[3];     /* the first node                            */
     double field_spacing_m[3];    /* node spacing per axis (> 0)               */
+    const icarion_waveform_desc *pressure;   /* NULL, or the cloud's gas
+                                     pressure waveform, as
+                                     icarion_gpu_set_pressure_waveform on a
+                                     handle of its own (ABI 6)                 */
 } icarion_cloud_desc;
 
 /* Launch shape of the persistent batch kernel (per precision class). */

@@ -60,7 +60,7 @@ STRUCTS = {
     "icarion_cloud_desc": (_abi.CloudDesc, ["domain", "physics", "species", "n_species", "ions",
                                             "t0_s", "n_steps", "step_offset", "waveforms", "mixture",
                                             "n_components", "atoms_xyzr", "field_E", "field_n",
-                                            "field_origin_m", "field_spacing_m"]),
+                                            "field_origin_m", "field_spacing_m", "pressure"]),
     "icarion_batch_tuning": (_abi.BatchTuning, ["block_size", "groups_per_cu"]),
     "icarion_cloud_stats": (_abi.CloudStats, ["ion_steps", "arrived", "t_end_s", "done_ms"]),
     "icarion_field_term_desc": (_abi.FieldTermDesc, ["E", "dtype", "scale_kind", "constant_scale",

@@ -274,3 +274,49 @@ def test_pressure_arguments_and_unsupported():
         assert ei.value.code == _abi.ERR_UNSUPPORTED
         with pytest.raises(gpu.IcarionError):
             h.set_domain_pressure_waveform(3, None)
+
+
+def test_batch_clouds_with_pressure_equal_single_handles():
+    """Clouds carrying a pressure waveform (icarion_cloud_desc.pressure) equal
+    handles of their own with the same waveform: FP64 bit for bit; FP32 +
+    Kahan with the same collisions and survival and the state within 1e-6
+    (in this mix of clouds the batch kernel's FP32 state differs from the
+    single-cloud kernel's at the float rounding level, with or without a
+    waveform: 2e-7 to 5e-7 relative)."""
+    from icarion_amd import gpu
+    clouds = []
+    for k, (prec, coll) in enumerate(((_abi.PRECISION_FP64, _abi.COLLISION_HSS),
+                                      (_abi.PRECISION_FP32_KAHAN, _abi.COLLISION_HSS),
+                                      (_abi.PRECISION_FP64, _abi.COLLISION_FRICTION),
+                                      (_abi.PRECISION_FP32_KAHAN, _abi.COLLISION_HSS))):
+        w = W.ims_hss_workload(n=1200 + 200 * k, seed=120 + k, precision=prec)
+        ph = W.physics(_abi.INTEGRATOR_RK4, coll, ou=1, precision=prec, seed=120 + k)
+        sp = [W.species(sigma_hss=100e-20, ccs=100e-20, K0=2.1)]
+        pw = None if k == 3 else pulse_and_ramp(w["domain"].pressure_Pa, w["dt"])
+        clouds.append(dict(domain=w["domain"], physics=ph, species=sp, ensemble=w["ensemble"].copy(), t0=0.0,
+                           dt=w["dt"], n_steps=900 + 100 * k, step_offset=2 * k, pressure=pw))
+    singles = []
+    for c in clouds:
+        e = c["ensemble"].copy()
+        with gpu.Handle() as h:
+            h.set_domain(c["domain"], c["physics"])
+            h.set_species(c["species"])
+            if c["pressure"] is not None:
+                h.set_pressure_waveform(c["pressure"])
+            h.upload_state(e)
+            h.step(c["t0"], c["dt"], c["n_steps"], c["step_offset"])
+            h.download_state(e)
+        singles.append(e)
+    with gpu.Handle() as h:
+        h.batch_upload(clouds)
+        h.batch_run()
+        h.batch_download(clouds)
+    for c, e in zip(clouds, singles):
+        b = c["ensemble"]
+        exact = c["physics"].precision == _abi.PRECISION_FP64
+        for f in ("x", "y", "z", "vx", "vy", "vz", "n_collisions", "active"):
+            if exact or f in ("n_collisions", "active"):
+                np.testing.assert_array_equal(getattr(b, f), getattr(e, f), err_msg=f)
+            else:
+                scale = 1e-3 if f in ("x", "y", "z") else 1e3
+                np.testing.assert_allclose(getattr(b, f), getattr(e, f), rtol=1e-6, atol=1e-6 * scale, err_msg=f)
